@@ -1,0 +1,517 @@
+// cell.h -- the per-(omega,k) generalized reflection/transmission solve.
+//
+// One call handles one wavenumber cell of one frequency.  Layer constants of
+// the frequency come in as LayerW records; everything per cell lives in
+// registers and the layers are streamed twice (half-space -> source, then
+// free surface -> source), so no per-layer state is ever written to memory.
+//
+// Reference algorithm: cusrc/popm/func/green.cu set_pars (164-195), get_emat
+// (196-233), interface R/T (235-262 + the cuBLAS 4x4 inverse/GEMM at
+// 1813-1816 + 339-387), get_grt (424-613), get_src_* (1271-1413) and
+// get_ypars_a_* (615-1267).  The 4x4 interface solve is replaced by a
+// closed-form 2x2 block factorisation (see interface_rt below).
+#pragma once
+#include "cx.h"
+
+namespace grtm {
+
+// Constants of one layer at one complex frequency w.
+template <typename T>
+struct LayerW {
+    cx<T> sa, sb;  // w/alpha, w/beta (complex slownesses times w)
+    cx<T> ga, gb;  // (w/alpha)^2, (w/beta)^2
+    cx<T> ca, cb;  // alpha/w, beta/w (normalisation of the E columns)
+    cx<T> mu;      // shear modulus
+    cx<T> a, b;    // alpha, beta
+    T h;           // thickness
+    T pad;
+};
+
+// Geometry of one (model, source) pair.
+struct Geo {
+    int L;       // layers including the half-space
+    int s;       // source layer
+    int rl;      // receiver layer
+    int branch;  // 1: rl<s, 2: rl==s & zr<=zs, 3: rl==s & zr>zs, 4: rl>s
+    int up;      // 1: receiver-above coupling, 0: receiver-below coupling
+    int pad;
+    double zs, zr;
+};
+
+// Layer quantities at wavenumber k.
+template <typename T>
+struct LQ {
+    cx<T> r, v, x, gb, mu, sa, sb, ca, cb;
+    T h;
+};
+
+template <typename T>
+GRTM_HD LQ<T> layer_q(const LayerW<T> &L, T k)
+{
+    LQ<T> q;
+    // vertical wavenumbers sqrt((k - w/V)(k + w/V)) with Re >= 0
+    // (green.cu:184-194); the factored form avoids cancellation near k = w/V
+    q.r = csqrt((C<T>(k, 0) - L.sa) * (C<T>(k, 0) + L.sa));
+    q.v = csqrt((C<T>(k, 0) - L.sb) * (C<T>(k, 0) + L.sb));
+    q.x = C<T>(T(2) * k * k, 0) - L.gb;  // k^2 + v^2
+    q.gb = L.gb;
+    q.mu = L.mu;
+    q.sa = L.sa;
+    q.sb = L.sb;
+    q.ca = L.ca;
+    q.cb = L.cb;
+    q.h = L.h;
+    return q;
+}
+
+template <typename T>
+struct RT {
+    m2<T> td, rud, rdu, tu;  // P-SV: T_d, R_ud, R_du, T_u
+    cx<T> td0, rud0, rdu0, tu0;  // SH
+};
+
+// Interface between layer u (above) and d (below).
+// The E matrix (green.cu:196-214) has columns [D | Sigma D] with
+// Sigma = diag(1,-1,-1,1).  Reordering rows to (0,3 | 1,2) gives
+// E = [[P, P], [N, -N]] and E^-1 = 1/2 [[P^-1, N^-1], [P^-1, -N^-1]].
+// With X = Pu^-1 Pd, Y = Nu^-1 Nd and S = X + Y the solution of
+// E1 M = E2 (green.cu:235-262, 1813-1816) is
+//   T_d = 2 S^-1, R_ud = 2 S^-1 Y - I, R_du = I - 2 Y S^-1, T_u = 2 X S^-1 Y.
+// P = Pt diag(a/w, b/w), Pt = [[k, v], [-mu x, -2 mu k v]],
+// N = Nt diag(a/w, b/w), Nt = [[r, k], [-2 mu k r, -mu x]],
+// det Pt = -mu v w^2/b^2, det Nt = mu r w^2/b^2 (closed forms, no cancellation);
+// the products adj(Pt_u) Pt_d etc. are written in terms of dmu = mu_d - mu_u so
+// identical layers give X = Y = I exactly.
+template <typename T>
+GRTM_HD RT<T> interface_rt(const LQ<T> &u, const LQ<T> &d, T k)
+{
+    RT<T> o;
+    const cx<T> dmu = d.mu - u.mu;
+    const T k2 = k * k;
+    const cx<T> k2dmu2 = (T(2) * k2) * dmu;
+    const cx<T> mgu = u.mu * u.gb, mgd = d.mu * d.gb;
+    // X' = adj(Pt_u) Pt_d
+    m2<T> X, Y;
+    X.a = u.v * (k2dmu2 - mgd);
+    X.b = (T(2) * k) * (u.v * d.v) * dmu;
+    X.c = k * (mgd - mgu - k2dmu2);
+    X.d = d.v * (-mgu - k2dmu2);
+    // Y' = adj(Nt_u) Nt_d
+    Y.a = d.r * (k2dmu2 + mgu);
+    Y.b = k * (k2dmu2 - mgd + mgu);
+    Y.c = -((T(2) * k) * (u.r * d.r) * dmu);
+    Y.d = u.r * (mgd - k2dmu2);
+    // divide by the determinants of Pt_u, Nt_u and apply the column scalings
+    const cx<T> ipu = recip(-(u.mu * u.v * u.gb));
+    const cx<T> inu = recip(u.mu * u.r * u.gb);
+    X = dmd(ipu * u.sa, ipu * u.sb, X, d.ca, d.cb);
+    Y = dmd(inu * u.sa, inu * u.sb, Y, d.ca, d.cb);
+    const m2<T> Si = minv(X + Y);
+    const m2<T> SiY2 = mscale(T(2), Si * Y);
+    o.td = mscale(T(2), Si);
+    o.rud = SiY2 - ident<T>();
+    o.rdu = ident<T>() - mscale(T(2), Y * Si);
+    o.tu = X * SiY2;
+    // SH (green.cu:339-353)
+    const cx<T> wa = u.v * u.mu, wb = d.v * d.mu;
+    const cx<T> is = recip(wa + wb);
+    o.td0 = T(2) * wa * is;
+    o.rdu0 = (wa - wb) * is;
+    o.rud0 = -o.rdu0;
+    o.tu0 = T(2) * wb * is;
+    return o;
+}
+
+// Free-surface reflection R_ud^0 = -(E21)^-1 E22 (green.cu:449-468) in closed
+// form: E21 = -mu F diag(a/w, b/w), F = [[2kr, x], [x, 2kv]], E22 = diag(-1,1) E21.
+template <typename T>
+GRTM_HD m2<T> free_surface(const LQ<T> &q, T k)
+{
+    const T k2 = k * k;
+    const cx<T> rv4 = (T(4) * k2) * (q.r * q.v);
+    const cx<T> x2 = q.x * q.x;
+    const cx<T> nr = recip(rv4 - x2);  // 1/det F
+    const cx<T> dd = (x2 + rv4) * nr;
+    m2<T> R;
+    R.a = dd;
+    R.b = (T(4) * k) * (q.v * q.x) * nr * (q.cb * q.sa);
+    R.c = -((T(4) * k) * (q.r * q.x) * nr * (q.ca * q.sb));
+    R.d = -dd;
+    return R;
+}
+
+// Downward step across interface j (green.cu:471-520): Rg = R_ud at the top
+// of layer j, e = Lambda_j.  Returns the generalized T_u of layer j+1.
+template <typename T>
+GRTM_HD void down_step(m2<T> &Rg, cx<T> &rg0, const RT<T> &t, cx<T> e0, cx<T> e1, m2<T> &Tu,
+                       cx<T> &tu0)
+{
+    const cx<T> tem1 = rg0 * e1;
+    tu0 = t.tu0 * recip(C<T>(1, 0) - tem1 * t.rdu0 * e1);
+    rg0 = t.rud0 + t.td0 * e1 * tem1 * tu0;
+    const m2<T> LRL = dmd(e0, e1, Rg, e0, e1);
+    const m2<T> A = ident<T>() - t.rdu * LRL;
+    const m2<T> B = t.td * LRL;
+    Tu = minv(A) * t.tu;
+    Rg = t.rud + B * Tu;
+}
+
+// Upward step across interface j (green.cu:532-594): Rg = R_du at the bottom
+// of layer j+1, e = Lambda_{j+1}.  Returns the generalized T_d of layer j+1.
+template <typename T>
+GRTM_HD void up_step(m2<T> &Rg, cx<T> &rg0, const RT<T> &t, cx<T> e0, cx<T> e1, m2<T> &Td,
+                     cx<T> &td0)
+{
+    const cx<T> tem2 = rg0 * e1;
+    td0 = t.td0 * recip(C<T>(1, 0) - t.rud0 * e1 * tem2);
+    rg0 = t.rdu0 + t.tu0 * e1 * tem2 * td0;
+    const m2<T> LRL = dmd(e0, e1, Rg, e0, e1);
+    const m2<T> A = ident<T>() - t.rud * LRL;
+    const m2<T> B = t.tu * LRL;
+    Td = minv(A) * t.td;
+    Rg = t.rdu + B * Td;
+}
+
+template <typename T>
+GRTM_HD void lam(const LQ<T> &q, T h, cx<T> &e0, cx<T> &e1)
+{
+    e0 = cexp(-(q.r * h));
+    e1 = cexp(-(q.v * h));
+}
+
+// Result of the receiver-side solve for one cell.
+template <typename T>
+struct CellOut {
+    m2<T> Y, Yt;        // P-SV receiver response (incl. source-layer reverberation)
+    cx<T> y0, y0t;      // SH receiver response
+    m2<T> ERd, ERu;     // sexd R_du sexd, sexu R_ud sexu at the source
+    cx<T> erd0, eru0;   // SH analogues
+    LQ<T> sq;           // source-layer quantities
+    int status;         // 0 ok
+};
+
+// compat: reproduce the reference's index choices (rexut from the first cell,
+// the extra v in u0t, yu0t/yd0t products, the shifted r>s chain).
+// NEED_T: also form the d/dz responses.  par0_r/par0_v: vertical wavenumbers
+// of layer rl at the first cell (k = 0, w_0), used only in compat mode.
+template <typename T, bool COMPAT, bool NEED_T>
+GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, T k, cx<T> par0_r, cx<T> par0_v,
+                        CellOut<T> &out)
+{
+    const int L = g.L, s = g.s, rl = g.rl;
+    const T zs = T(g.zs), zr = T(g.zr);
+    out.status = 0;
+
+    // ---------------- upward sweep: half-space -> bottom of source layer
+    m2<T> Rdu_s1, Rdu_r1, Md;  // R_du at bottom of s, at bottom of rl, T_d chain
+    cx<T> rdu0_s1, rdu0_r1, md0;
+    Rdu_r1 = ident<T>();
+    rdu0_r1 = C<T>(0, 0);
+    Md = ident<T>();
+    md0 = C<T>(1, 0);
+    {
+        LQ<T> qd = layer_q(lw[L - 1], k);  // layer below the interface
+        LQ<T> qu = layer_q(lw[L - 2], k);
+        RT<T> t = interface_rt(qu, qd, k);
+        m2<T> Rg = t.rdu, Td = t.td;
+        cx<T> rg0 = t.rdu0, td0 = t.td0;
+        // grt[L-1] = (Rg, Td)
+        int j = L - 1;  // index of the grt entry just formed
+        // lj0, lj1 = Lambda_jj (layer jj, just below the interface formed)
+        auto visit = [&](int jj, const m2<T> &RgJ, cx<T> rg0J, const m2<T> &TdJ, cx<T> td0J,
+                         cx<T> lj0, cx<T> lj1) {
+            // jj: grt index formed (R_du at interface (jj-1, jj), T_d into jj)
+            if (jj == s + 1) {
+                Rdu_s1 = RgJ;
+                rdu0_s1 = rg0J;
+            }
+            if (g.branch == 4) {
+                if (jj == rl + 1) {
+                    Rdu_r1 = RgJ;
+                    rdu0_r1 = rg0J;
+                }
+                if (!COMPAT) {
+                    // chain T_d[rl] L_{rl-1} ... L_{s+1} T_d[s+1]
+                    if (jj == rl) {
+                        Md = TdJ;
+                        md0 = td0J;
+                    } else if (jj < rl && jj >= s + 1) {
+                        Md = Md * dmd(lj0, lj1, TdJ, C<T>(1, 0), C<T>(1, 0));
+                        md0 = md0 * lj1 * td0J;
+                    }
+                } else {
+                    // reference chain T_d[rl+1] L_{rl-2} T_d[rl] ... L_s T_d[s+2]
+                    if (jj == rl + 1) {
+                        Md = TdJ;
+                        md0 = td0J;
+                    } else if (jj <= rl && jj >= s + 2) {
+                        LQ<T> q2 = layer_q(lw[jj - 2], k);
+                        cx<T> e0, e1;
+                        lam(q2, lw[jj - 2].h, e0, e1);
+                        Md = Md * dmd(e0, e1, TdJ, C<T>(1, 0), C<T>(1, 0));
+                        md0 = md0 * e1 * td0J;
+                    }
+                }
+            }
+        };
+        visit(j, Rg, rg0, Td, td0, C<T>(0, 0), C<T>(0, 0));  // Lambda of the half-space is never used
+        for (int lay = L - 3; lay >= s; lay--) {
+            // interface lay: between layer lay (above) and lay+1 (below, = qu)
+            cx<T> e0, e1;
+            lam(qu, lw[lay + 1].h, e0, e1);
+            qd = qu;
+            qu = layer_q(lw[lay], k);
+            t = interface_rt(qu, qd, k);
+            up_step(Rg, rg0, t, e0, e1, Td, td0);
+            visit(lay + 1, Rg, rg0, Td, td0, e0, e1);
+        }
+    }
+
+    // ---------------- downward sweep: free surface -> top of source layer
+    m2<T> Rud_s, Rud_r, Mu;
+    cx<T> rud0_s, rud0_r, mu0;
+    Mu = ident<T>();
+    mu0 = C<T>(1, 0);
+    LQ<T> qr;  // receiver layer quantities
+    {
+        LQ<T> qu = layer_q(lw[0], k);
+        m2<T> Rg = free_surface(qu, k);
+        cx<T> rg0 = C<T>(1, 0);
+        if (rl == 0) qr = qu;
+        Rud_r = Rg;
+        rud0_r = rg0;
+        for (int lay = 0; lay < s; lay++) {
+            cx<T> e0, e1;
+            lam(qu, lw[lay].h, e0, e1);  // Lambda_lay
+            LQ<T> qd = layer_q(lw[lay + 1], k);
+            RT<T> t = interface_rt(qu, qd, k);
+            m2<T> Tu;
+            cx<T> tu0;
+            down_step(Rg, rg0, t, e0, e1, Tu, tu0);
+            const int jn = lay + 1;  // grt index formed
+            if (g.branch == 1) {
+                if (jn == rl) {
+                    Rud_r = Rg;
+                    rud0_r = rg0;
+                }
+                if (jn == rl + 1) {
+                    Mu = Tu;
+                    mu0 = tu0;
+                } else if (jn > rl + 1) {
+                    Mu = Mu * dmd(e0, e1, Tu, C<T>(1, 0), C<T>(1, 0));
+                    mu0 = mu0 * e1 * tu0;
+                }
+            }
+            if (jn == rl) qr = qd;
+            qu = qd;
+        }
+        Rud_s = Rg;
+        rud0_s = rg0;
+        out.sq = qu;  // layer s
+    }
+    if (rl > s) qr = layer_q(lw[rl], k);
+
+    // ---------------- source layer reverberation (green.cu:654-702)
+    const LQ<T> &sq = out.sq;
+    const T hs_ = lw[s].h;
+    cx<T> sexd0 = cexp(-(sq.r * (hs_ - zs))), sexd1 = cexp(-(sq.v * (hs_ - zs)));
+    cx<T> sexu0 = cexp(-(sq.r * zs)), sexu1 = cexp(-(sq.v * zs));
+    out.ERd = dmd(sexd0, sexd1, Rdu_s1, sexd0, sexd1);
+    out.ERu = dmd(sexu0, sexu1, Rud_s, sexu0, sexu1);
+    out.erd0 = sexd1 * rdu0_s1 * sexd1;
+    out.eru0 = sexu1 * rud0_s * sexu1;
+    const m2<T> yu_psv = minv(ident<T>() - out.ERd * out.ERu);
+    const m2<T> yd_psv = minv(ident<T>() - out.ERu * out.ERd);
+    const cx<T> y_sh = recip(C<T>(1, 0) - out.erd0 * out.eru0);
+
+    // receiver-layer E rows 0,1 (displacements): down cols [[ca k, cb v], [ca r, cb k]],
+    // up cols [[ca k, cb v], [-ca r, -cb k]]
+    const m2<T> Ed{qr.ca * k, qr.cb * qr.v, qr.ca * qr.r, qr.cb * k};
+    const m2<T> Eu{qr.ca * k, qr.cb * qr.v, -(qr.ca * qr.r), -(qr.cb * k)};
+    const T hr_ = lw[rl].h;
+    const bool rl_half = (rl == L - 1);
+
+    if (g.branch == 2) {  // green.cu:709-743
+        cx<T> rd0 = cexp(-(qr.r * zr)), rd1 = cexp(-(qr.v * zr));
+        cx<T> ru0 = cexp(-(qr.r * (zs - zr))), ru1 = cexp(-(qr.v * (zs - zr)));
+        const m2<T> RS = dmd(rd0, rd1, Rud_s, sexu0, sexu1);
+        const m2<T> u = Ed * RS + md(Eu, ru0, ru1);
+        const cx<T> u0 = rd1 * rud0_s * sexu1 + ru1;
+        out.Y = u * yu_psv;
+        out.y0 = u0 * y_sh;
+        if (NEED_T) {
+            const m2<T> RSt = dmd(-(qr.r * rd0), -(qr.v * rd1), Rud_s, sexu0, sexu1);
+            const m2<T> ut = Ed * RSt + md(Eu, qr.r * ru0, qr.v * ru1);
+            cx<T> u0t = -(qr.v * rd1) * rud0_s * sexu1 + qr.v * ru1 * (COMPAT ? qr.v : C<T>(1, 0));
+            out.Yt = ut * yu_psv;
+            out.y0t = u0t * y_sh;
+        }
+    } else if (g.branch == 3) {  // green.cu:744-774
+        cx<T> rd0 = cexp(-(qr.r * (zr - zs))), rd1 = cexp(-(qr.v * (zr - zs)));
+        cx<T> ru0 = cexp(-(qr.r * (hr_ - zr))), ru1 = cexp(-(qr.v * (hr_ - zr)));
+        const m2<T> RS = dmd(ru0, ru1, Rdu_s1, sexd0, sexd1);
+        const m2<T> d = md(Ed, rd0, rd1) + Eu * RS;
+        const cx<T> d0 = rd1 + ru1 * rdu0_s1 * sexd1;
+        out.Y = d * yd_psv;
+        out.y0 = d0 * y_sh;
+        if (NEED_T) {
+            const m2<T> RSt = dmd(qr.r * ru0, qr.v * ru1, Rdu_s1, sexd0, sexd1);
+            const m2<T> dt = md(Ed, -(qr.r * rd0), -(qr.v * rd1)) + Eu * RSt;
+            const cx<T> d0t = -(qr.v * rd1) + qr.v * ru1 * rdu0_s1 * sexd1;
+            out.Yt = dt * yd_psv;
+            out.y0t = d0t * y_sh;
+        }
+    } else if (g.branch == 1) {  // green.cu:775-857
+        cx<T> rd0 = cexp(-(qr.r * zr)), rd1 = cexp(-(qr.v * zr));
+        cx<T> ru0 = cexp(-(qr.r * (hr_ - zr))), ru1 = cexp(-(qr.v * (hr_ - zr)));
+        cx<T> ex0, ex1;
+        lam(qr, hr_, ex0, ex1);
+        const m2<T> RS = dmd(rd0, rd1, Rud_r, ex0, ex1);
+        const m2<T> u = Ed * RS + md(Eu, ru0, ru1);
+        const cx<T> u0 = rd1 * rud0_r * ex1 + ru1;
+        const m2<T> chain = Mu * dmd(sexu0, sexu1, yu_psv, C<T>(1, 0), C<T>(1, 0));
+        const cx<T> chain0 = mu0 * sexu1 * y_sh;
+        out.Y = u * chain;
+        out.y0 = u0 * chain0;
+        if (NEED_T) {
+            const cx<T> pr = COMPAT ? par0_r : qr.r, pv = COMPAT ? par0_v : qr.v;
+            const m2<T> RSt = dmd(-(qr.r * rd0), -(qr.v * rd1), Rud_r, ex0, ex1);
+            const m2<T> ut = Ed * RSt + md(Eu, ru0 * pr, ru1 * pv);
+            const cx<T> u0t = -(qr.v * rd1) * rud0_r * ex1 + ru1 * pv;
+            out.Yt = ut * chain;
+            out.y0t = COMPAT ? u0t * out.y0 : u0t * chain0;
+        }
+    } else {  // branch 4, rl > s: green.cu:858-915
+        cx<T> rd0 = cexp(-(qr.r * zr)), rd1 = cexp(-(qr.v * zr));
+        cx<T> ru0 = C<T>(0, 0), ru1 = C<T>(0, 0);
+        if (!rl_half) {
+            ru0 = cexp(-(qr.r * (hr_ - zr)));
+            ru1 = cexp(-(qr.v * (hr_ - zr)));
+        }
+        cx<T> ex0, ex1;
+        if (COMPAT) {
+            LQ<T> qp = layer_q(lw[rl - 1], k);
+            lam(qp, lw[rl - 1].h, ex0, ex1);
+        } else {
+            lam(qr, hr_, ex0, ex1);
+        }
+        const m2<T> RS = dmd(ru0, ru1, Rdu_r1, ex0, ex1);
+        const m2<T> d = md(Ed, rd0, rd1) + Eu * RS;
+        const cx<T> d0 = ru1 * rdu0_r1 * ex1 + rd1;
+        const m2<T> &ypsv = COMPAT ? yu_psv : yd_psv;
+        const m2<T> chain = Md * dmd(sexd0, sexd1, ypsv, C<T>(1, 0), C<T>(1, 0));
+        const cx<T> chain0 = md0 * sexd1 * y_sh;
+        out.Y = d * chain;
+        out.y0 = d0 * chain0;
+        if (NEED_T) {
+            const cx<T> pr = COMPAT ? par0_r : qr.r, pv = COMPAT ? par0_v : qr.v;
+            const m2<T> RSt = dmd(ru0 * pr, ru1 * pv, Rdu_r1, ex0, ex1);
+            const m2<T> dt = md(Ed, -(qr.r * rd0), -(qr.v * rd1)) + Eu * RSt;
+            const cx<T> d0t = ru1 * pv * rdu0_r1 * ex1 - qr.v * rd1;
+            out.Yt = dt * chain;
+            out.y0t = COMPAT ? d0 * out.y0 : d0t * chain0;
+        }
+    }
+}
+
+// Kernel coefficients (popm_parameter a0,a1,a2,b1,b2 and their d/dz
+// versions) from a solved cell.
+template <typename T>
+struct Coef {
+    cx<T> a0[2], a1[2], a2[2], b1, b2;
+    cx<T> a0t[2], a1t[2], a2t[2], b1t, b2t;
+};
+
+// Source vectors and coupling.  src_type 0: single force (green.cu:1271-1332,
+// 918-938); otherwise moment tensor (green.cu:1334-1413, 1239-1266).
+template <typename T, bool NEED_T>
+GRTM_HD void coef_from_cell(const CellOut<T> &c, int src_type, int up, T k, cx<T> w, Coef<T> &o)
+{
+    const LQ<T> &q = c.sq;
+    const cx<T> b = w * recip(q.sb);  // beta
+    const m2<T> I2 = ident<T>();
+    // wr?0 and wr?1 for the selected direction
+    const m2<T> W0 = up ? (c.ERd - I2) : (I2 - c.ERu);
+    const m2<T> W1 = up ? (c.ERd + I2) : (I2 + c.ERu);
+    const cx<T> ssh = recip(T(2) * q.mu * q.v);
+    if (src_type == 0) {
+        // f0 = b/(2 mu a w r v) * (b v r, -a k r); f1 = (k b v, -a r v) * same
+        const cx<T> tmp = b * recip(T(2) * q.mu * w * w * q.r * q.v) * q.sa;  // b/(2 mu a w r v)
+        const cx<T> f00 = b * q.v * q.r * tmp, f01 = -(k * (w * recip(q.sa)) * q.r * tmp);
+        const cx<T> f10 = k * b * q.v * tmp, f11 = -((w * recip(q.sa)) * q.r * q.v * tmp);
+        cx<T> g00, g01, g10, g11;
+        mv(W0, f00, f01, g00, g01);
+        mv(W1, f10, f11, g10, g11);
+        const cx<T> gsh = (up ? (C<T>(1, 0) + c.erd0) : (c.eru0 + C<T>(1, 0))) * ssh;
+        o.b1 = c.y0 * gsh;
+        mv(c.Y, g00, g01, o.a0[0], o.a0[1]);
+        mv(c.Y, g10, g11, o.a1[0], o.a1[1]);
+        o.a2[0] = o.a2[1] = o.b2 = C<T>(0, 0);
+        if (NEED_T) {
+            o.b1t = c.y0t * gsh;
+            mv(c.Yt, g00, g01, o.a0t[0], o.a0t[1]);
+            mv(c.Yt, g10, g11, o.a1t[0], o.a1t[1]);
+            o.a2t[0] = o.a2t[1] = o.b2t = C<T>(0, 0);
+        }
+    } else {
+        const cx<T> av = w * recip(q.sa);
+        const cx<T> tmp = b * recip(T(2) * q.mu * av * w);  // b/(2 mu a w)
+        const cx<T> s20 = tmp * k * recip(q.r) * k * b, s21 = -(k * tmp * av);
+        const cx<T> s10 = T(-2) * k * b * tmp, s11 = tmp * recip(q.v) * q.x * av;
+        const cx<T> s00 = tmp * q.r * b, s01 = -(tmp * av * k);
+        cx<T> u00, u01, u10, u11, u20, u21;
+        mv(W1, s00, s01, u00, u01);
+        mv(W0, s10, s11, u10, u11);
+        mv(W1, s20, s21, u20, u21);
+        const cx<T> shf = up ? (C<T>(1, 0) + c.erd0) : (c.eru0 - C<T>(1, 0));
+        const cx<T> sh1 = shf * q.v * ssh, sh2 = shf * k * ssh;
+        o.b1 = c.y0 * sh1;
+        o.b2 = c.y0 * sh2;
+        mv(c.Y, u00, u01, o.a0[0], o.a0[1]);
+        mv(c.Y, u10, u11, o.a1[0], o.a1[1]);
+        mv(c.Y, u20, u21, o.a2[0], o.a2[1]);
+        if (NEED_T) {
+            o.b1t = c.y0t * sh1;
+            o.b2t = c.y0t * sh2;
+            mv(c.Yt, u00, u01, o.a0t[0], o.a0t[1]);
+            mv(c.Yt, u10, u11, o.a1t[0], o.a1t[1]);
+            mv(c.Yt, u20, u21, o.a2t[0], o.a2t[1]);
+        }
+    }
+}
+
+// Z/R k-kernels of the n = 0 sources used by the trace path:
+//  explosion      = s0 - s2 tensor source (only the P part survives:
+//                   s0 - s2 = (-b (w/a)^2 tmp / r, 0), tmp = b/(2 mu a w)),
+//                   Z = (a0[1]-a2[1]) k, R = -(a0[0]-a2[0]) k  [J0' = -J1]
+//  vertical force = single-force f0 source, Z = a0[1] k, R = -a0[0] k.
+// compat keeps the reference's index slip in the R column (a.[1], Q9).
+template <typename T, bool COMPAT>
+GRTM_HD void zr_from_cell(const CellOut<T> &c, int up, T k, cx<T> w, cx<T> &kz_ex, cx<T> &kr_ex,
+                          cx<T> &kz_vf, cx<T> &kr_vf)
+{
+    const LQ<T> &q = c.sq;
+    const m2<T> I2 = ident<T>();
+    const m2<T> W0 = up ? (c.ERd - I2) : (I2 - c.ERu);
+    const m2<T> W1 = up ? (c.ERd + I2) : (I2 + c.ERu);
+    const cx<T> b = w * recip(q.sb);
+    const cx<T> av = w * recip(q.sa);
+    const cx<T> tmp = b * recip(T(2) * q.mu * av * w);
+    // explosion
+    const cx<T> sx = -(b * q.sa * q.sa * tmp * recip(q.r));
+    const cx<T> e0 = W1.a * sx, e1 = W1.c * sx;
+    cx<T> a0, a1;
+    mv(c.Y, e0, e1, a0, a1);
+    kz_ex = a1 * k;
+    kr_ex = COMPAT ? -(a1 * k) : -(a0 * k);
+    // vertical force: f0 = (b^2/(2 mu a w), -b k/(2 mu w v))
+    const cx<T> f00 = b * tmp;
+    const cx<T> f01 = -(b * k * recip(T(2) * q.mu * w * q.v));
+    cx<T> g0, g1, v0, v1;
+    mv(W0, f00, f01, g0, g1);
+    mv(c.Y, g0, g1, v0, v1);
+    kz_vf = v1 * k;
+    kr_vf = -(v0 * k);
+}
+
+}  // namespace grtm

@@ -1,0 +1,771 @@
+// host.hip -- host driver and C ABI of libgrtm.so.
+//
+// Input handling follows cusrc/popm/func/green.cu:1989-2101 (input.txt
+// parsing, source-layer insertion, thickness conversion, source/receiver
+// layer search), the grid set-up follows green.cu:2063-2194 (incl. cal_kmax,
+// green.cu:108-162), and the outputs follow green.cu:2242-2258.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/grtm.h"
+#include "bessel.h"
+#include "cell.h"
+#include "kernels.h"
+
+using grtm::Geo;
+using grtm::LayerW;
+using cd = std::complex<double>;
+
+namespace {
+
+thread_local int g_last_status = GRTM_OK;
+
+#define HIPCHK(x)                                      \
+    do {                                               \
+        hipError_t e_ = (x);                           \
+        if (e_ != hipSuccess) return GRTM_E_DEVICE;    \
+    } while (0)
+
+// ------------------------------------------------------------------ model
+struct Stack {
+    int L = 0;
+    double h[grtm::kMaxLayers], a0[grtm::kMaxLayers], b0[grtm::kMaxLayers], rho[grtm::kMaxLayers];
+    double qp[grtm::kMaxLayers], qs[grtm::kMaxLayers];
+    double hs = 0, hr = 0;
+    int s = -1, rl = -1;
+    double zs = 0, zr = 0;
+};
+
+struct RunParams {
+    double dt = 0, dist = 0, twin = 0, t0 = 0, taper = 0;
+    int nf1 = 0, nf2 = 0, log2n = 0, n_ptam = 0, src_type = 0;
+};
+
+// Layer stack from layer tops (green.cu:2001-2049) and source / receiver
+// location (green.cu:2081-2101).  compat: the receiver search keys on the
+// source depth as the reference does, and a source that needs no split is
+// rejected (the reference would leave its last layer uninitialised).
+int prepare_stack(int compat, int n, const double *top, const double *vp, const double *vs, const double *rho,
+                  const double *qp, const double *qs, double hs, double hr, Stack &st)
+{
+    if (n < 2 || n + 1 > grtm::kMaxLayers) return GRTM_E_INPUT;
+    for (int i = 0; i < n; i++) {
+        if (!(std::isfinite(top[i]) && vp[i] > 0 && vs[i] > 0 && rho[i] > 0 && qp[i] > 0 && qs[i] > 0))
+            return GRTM_E_INPUT;
+        if (i > 0 && !(top[i] > top[i - 1])) return GRTM_E_INPUT;
+    }
+    if (!(std::isfinite(hs) && std::isfinite(hr) && hs >= 0 && hr >= 0)) return GRTM_E_INPUT;
+    int c = 0;
+    bool split = false;
+    auto push = [&](double t, int i) {
+        st.h[c] = t;
+        st.a0[c] = vp[i];
+        st.b0[c] = vs[i];
+        st.rho[c] = rho[i];
+        st.qp[c] = qp[i];
+        st.qs[c] = qs[i];
+        c++;
+    };
+    push(top[0], 0);
+    for (int i = 1; i < n; i++) {
+        if (top[i - 1] < hs && top[i] > hs) {
+            push(hs, i - 1);
+            split = true;
+        }
+        push(top[i], i);
+    }
+    if (!split) {
+        if (compat) return GRTM_E_UNSUPPORTED;
+        if (hs > top[n - 1]) {  // source inside the half-space: split it
+            push(hs, n - 1);
+            split = true;
+        }
+    }
+    st.L = c;
+    for (int i = 0; i < c - 1; i++) st.h[i] = st.h[i + 1] - st.h[i];
+    st.hs = hs;
+    st.hr = hr;
+    double zs = hs, zr = hr;
+    st.s = st.rl = -1;
+    for (int i = 0; i < c; i++) {
+        if (zs <= st.h[i] || i == c - 1) {
+            st.s = i;
+            st.zs = zs;
+            break;
+        }
+        zs -= st.h[i];
+    }
+    for (int i = 0; i < c; i++) {
+        const double key = compat ? zs : zr;
+        if (key <= st.h[i] || (!compat && i == c - 1)) {
+            st.rl = i;
+            st.zr = zr;
+            break;
+        }
+        zr -= st.h[i];
+    }
+    if (st.s < 0 || st.rl < 0) return GRTM_E_INPUT;
+    if (st.s >= c - 1) return GRTM_E_UNSUPPORTED;
+    return GRTM_OK;
+}
+
+int make_geo(const Stack &st, int compat, Geo &g)
+{
+    g.L = st.L;
+    g.s = st.s;
+    g.rl = st.rl;
+    g.zs = st.zs;
+    g.zr = st.zr;
+    g.pad = 0;
+    if (st.rl < st.s) g.branch = 1;
+    else if (st.rl == st.s) g.branch = (st.zr <= st.zs) ? 2 : 3;
+    else g.branch = 4;
+    // coupling direction (green.cu:918-938)
+    if ((st.rl == st.s && st.zr < st.zs) || st.rl < st.s) g.up = 1;
+    else if ((st.rl == st.s && st.zr > st.zs) || st.rl > st.s) g.up = 0;
+    else {
+        if (compat) return GRTM_E_UNSUPPORTED;  // the reference writes no coefficients
+        g.up = 1;
+    }
+    if (compat && g.branch == 4 && st.rl >= st.L - 1) return GRTM_E_UNSUPPORTED;
+    return GRTM_OK;
+}
+
+int parse_input(const char *path, int compat, Stack &st, RunParams &rp)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) return GRTM_E_IO;
+    double hs, hr;
+    int n;
+    std::vector<double> top, vp, vs, rho, qp, qs;
+    int status = GRTM_OK;
+    if (fscanf(f, "%lf", &hs) != 1 || fscanf(f, "%lf", &hr) != 1 || fscanf(f, "%d", &n) != 1) {
+        status = GRTM_E_PARSE;
+    } else if (n < 2 || n + 1 > grtm::kMaxLayers) {
+        status = GRTM_E_INPUT;
+    } else {
+        top.resize(n), vp.resize(n), vs.resize(n), rho.resize(n), qp.resize(n), qs.resize(n);
+        for (int i = 0; i < n && status == GRTM_OK; i++)
+            if (fscanf(f, " %lf , %lf , %lf , %lf , %lf , %lf", &top[i], &vp[i], &vs[i], &rho[i], &qp[i], &qs[i]) !=
+                6)
+                status = GRTM_E_PARSE;
+        if (status == GRTM_OK &&
+            (fscanf(f, "%lf", &rp.dt) != 1 || fscanf(f, "%lf", &rp.dist) != 1 || fscanf(f, "%d", &rp.nf1) != 1 ||
+             fscanf(f, "%d", &rp.nf2) != 1 || fscanf(f, "%lf", &rp.twin) != 1 || fscanf(f, "%d", &rp.log2n) != 1 ||
+             fscanf(f, "%lf", &rp.t0) != 1 || fscanf(f, "%lf", &rp.taper) != 1 ||
+             fscanf(f, "%d", &rp.n_ptam) != 1 || fscanf(f, "%d", &rp.src_type) != 1))
+            status = GRTM_E_PARSE;
+    }
+    fclose(f);
+    if (status) return status;
+    if (rp.log2n < 3 || rp.log2n > 14 || !(rp.dt > 0) || !(rp.dist > 0) || !std::isfinite(rp.t0) ||
+        !(rp.taper >= 0 && rp.taper <= 1))
+        return GRTM_E_INPUT;
+    return prepare_stack(compat, n, top.data(), vp.data(), vs.data(), rho.data(), qp.data(), qs.data(), hs, hr, st);
+}
+
+// Dispersed velocity v0 (1 + ln(w/2pi)/(pi Q) + i/(2Q)) (green.cu:177-180).
+cd disp_vel(cd w, double v0, double q)
+{
+    const cd lw = std::log(w / (2.0 * M_PI));
+    return v0 * (1.0 + lw / (M_PI * q) + cd(0, 1) / (2.0 * q));
+}
+
+// Layer constants at one complex frequency.
+void layer_w(const Stack &st, int compat, cd w, LayerW<double> *out)
+{
+    for (int j = 0; j < st.L; j++) {
+        const cd a = disp_vel(w, st.a0[j], st.qp[j]);
+        const cd b = disp_vel(w, st.b0[j], st.qs[j]);
+        // the reference keeps mu = beta0^2 rho real (green.cu:2047-2049)
+        const cd mu = compat ? cd(st.b0[j] * st.b0[j] * st.rho[j], 0) : st.rho[j] * b * b;
+        const cd sa = w / a, sb = w / b;
+        auto C = [](cd z) { return grtm::cx<double>{z.real(), z.imag()}; };
+        LayerW<double> &q = out[j];
+        q.sa = C(sa);
+        q.sb = C(sb);
+        q.ga = C(sa * sa);
+        q.gb = C(sb * sb);
+        q.ca = C(a / w);
+        q.cb = C(b / w);
+        q.mu = C(mu);
+        q.a = C(a);
+        q.b = C(b);
+        q.h = st.h[j];
+        q.pad = 0;
+    }
+}
+
+// cal_kmax (green.cu:108-162); b: S velocities dispersed at the grid's
+// current frequency, w: the frequency at which the estimate is evaluated.
+double cal_kmax(cd w, const cd *b, const Stack &st, int nf)
+{
+    const int s = st.s, rl = st.rl;
+    const double *h = st.h;
+    double pmin, dzmax, kmax, t;
+    auto MX = [](double x, double y) { return x > y ? x : y; };
+    auto MN = [](double x, double y) { return x < y ? x : y; };
+    auto AB = [](double x) { return x > 0 ? x : -x; };
+    if (rl == s - 1) {
+        pmin = MX(std::abs(w / b[s]), std::abs(w / b[s - 1]));
+        dzmax = MN(st.zs, h[rl] - st.zr);
+    } else if (rl < s - 1) {
+        pmin = MX(std::abs(w / b[s]), std::abs(w / b[s - 1]));
+        dzmax = MN(st.zs, st.hs - st.hr);
+        for (int lay = rl; lay <= s - 2; lay++) {
+            t = std::abs(w / b[lay + 1]);
+            if (pmin < t) pmin = t;
+            if (dzmax > h[lay]) dzmax = h[lay];
+        }
+    } else if (rl == s) {
+        pmin = std::abs(w / b[s]);
+        dzmax = AB(st.hr - st.hs);
+    } else if (rl == s + 1) {
+        pmin = MX(std::abs(w / b[s]), std::abs(w / b[s + 1]));
+        dzmax = MN(h[s] - st.zs, st.zr);
+    } else {
+        pmin = MX(std::abs(w / b[s]), std::abs(w / b[s + 1]));
+        dzmax = MN(h[s] - st.zs, AB(st.hr - st.hs));
+        for (int lay = s + 1; lay < rl - 1; lay++) {
+            t = std::abs(w / b[lay]);
+            if (pmin < t) pmin = t;
+            if (dzmax > h[lay]) dzmax = h[lay];
+        }
+    }
+    if (dzmax < 0.2) kmax = (nf < 10) ? MN(pmin + 2.0, 20. * pmin) : MN(pmin + 2.0, 2.0 * pmin);
+    else kmax = std::sqrt((3.0 / dzmax) * (3.0 / dzmax) + pmin * pmin);
+    if ((kmax - pmin) < 0.5) kmax = pmin + 0.5;
+    return kmax;
+}
+
+// Device buffer that frees itself.
+struct DBuf {
+    void *p = nullptr;
+    size_t n = 0;
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int alloc(size_t bytes)
+    {
+        release();
+        if (bytes == 0) bytes = 16;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            p = nullptr;
+            return GRTM_E_NOMEM;
+        }
+        n = bytes;
+        return GRTM_OK;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+int device_ready()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return GRTM_E_DEVICE;
+    return GRTM_OK;
+}
+
+// ------------------------------------------------------------------ green()
+// The whole drop-in run; spec/traces in the layout of grtm_green_mem.
+int green_core(const char *iname, int compat, std::vector<double> *spec, std::vector<double> *traces, int *nw_out)
+{
+    Stack st;
+    RunParams rp;
+    int status = parse_input(iname, compat, st, rp);
+    if (status) return status;
+    Geo g;
+    status = make_geo(st, compat, g);
+    if (status) return status;
+    if ((status = device_ready())) return status;
+
+    // grid constants (green.cu:2063-2080)
+    const int nw = 1 << (rp.log2n - 1), N = 2 * nw;
+    const double twin = N * rp.dt, df = 1.0 / (N * rp.dt), omega_i = 2.0 / twin;
+    double vmax = 0, vmin = 100000;
+    for (int i = 0; i < st.L; i++) {
+        if (st.a0[i] > vmax) vmax = st.a0[i];
+        if (st.b0[i] < vmin) vmin = st.b0[i];
+    }
+    const double vavg = (vmax + vmin) * 0.5;
+    const double Lw = 2.0 * (vmax * twin + vmax / vavg * std::sqrt((st.hs - st.hr) * (st.hs - st.hr)) + 100);
+    const double dk = 2 * M_PI / Lw;
+    if (nw_out) *nw_out = nw;
+    if (!spec && !traces) return GRTM_OK;
+
+    // per-frequency layer tables and ragged k counts (green.cu:2145-2194)
+    std::vector<LayerW<double>> lw((size_t)nw * st.L);
+    std::vector<int> nk_w(nw);
+    int nk_max = 0;
+    std::vector<cd> bvel(st.L);
+    for (int iw = 0; iw < nw; iw++) {
+        const cd w(2 * M_PI * df * iw, -omega_i);
+        layer_w(st, compat, w, &lw[(size_t)iw * st.L]);
+        for (int j = 0; j < st.L; j++) bvel[j] = disp_vel(w, st.b0[j], st.qs[j]);
+        double km = (iw < 3) ? cal_kmax(cd(2 * M_PI * df * 3.0, -omega_i), bvel.data(), st, iw)
+                             : cal_kmax(w, bvel.data(), st, iw);
+        km = 1.1 * km;
+        const double nkd = km / dk + 2.0;
+        if (!(nkd < 2.0e9)) return GRTM_E_INPUT;
+        nk_w[iw] = (int)(km / dk) + 2;
+        if (nk_w[iw] > nk_max) nk_max = nk_w[iw];
+    }
+    // vertical wavenumbers of the receiver layer in the first cell (k = 0, w_0),
+    // used by the reference's d/dz columns (green.cu:674-675)
+    double p0[4];
+    {
+        const cd w0(0.0, -omega_i);
+        const cd a = disp_vel(w0, st.a0[st.rl], st.qp[st.rl]), b = disp_vel(w0, st.b0[st.rl], st.qs[st.rl]);
+        cd r = std::sqrt(-(w0 / a) * (w0 / a)), v = std::sqrt(-(w0 / b) * (w0 / b));
+        if (r.real() < 0) r = -r;
+        if (v.real() < 0) v = -v;
+        p0[0] = r.real(), p0[1] = r.imag(), p0[2] = v.real(), p0[3] = v.imag();
+    }
+    const int nblk = (nk_max + 255) / 256;
+    DBuf d_lw, d_nk, d_part, d_raw, d_tr, d_sp;
+    if ((status = d_lw.alloc(lw.size() * sizeof(LayerW<double>)))) return status;
+    if ((status = d_nk.alloc(nk_w.size() * sizeof(int)))) return status;
+    if ((status = d_part.alloc((size_t)nw * nblk * 60 * sizeof(double)))) return status;
+    if ((status = d_raw.alloc((size_t)30 * nw * 2 * sizeof(double)))) return status;
+    if ((status = d_tr.alloc((size_t)30 * N * sizeof(double)))) return status;
+    if ((status = d_sp.alloc((size_t)30 * nw * 2 * sizeof(double)))) return status;
+    hipStream_t s;
+    HIPCHK(hipStreamCreate(&s));
+    struct SG {
+        hipStream_t s;
+        ~SG() { (void)hipStreamDestroy(s); }
+    } sg{s};
+    HIPCHK(hipMemcpyAsync(d_lw.p, lw.data(), lw.size() * sizeof(LayerW<double>), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_nk.p, nk_w.data(), nk_w.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(grtm::launch_cells_full(compat, rp.src_type ? 1 : 0, d_lw.p, d_nk.as<int>(), st.L, g, nw, nblk, dk,
+                                   rp.dist, df, omega_i, p0, d_part.as<double>(), s));
+    HIPCHK(grtm::launch_green_finish(nw, nblk, d_part.as<double>(), d_raw.as<double>(), s));
+    grtm::PostParams pp;
+    pp.nw = nw;
+    pp.wc = (int)((double)nw * (1 - rp.taper));
+    pp.df = df;
+    pp.dt = rp.dt;
+    pp.t0 = rp.t0;
+    pp.omega_i = omega_i;
+    HIPCHK(grtm::launch_post_ifft(0, pp, 0, 30, d_raw.p, nullptr, nullptr, 0, 0, d_tr.p, d_sp.p, s));
+    if (spec) {
+        spec->resize((size_t)30 * nw * 2);
+        HIPCHK(hipMemcpyAsync(spec->data(), d_sp.p, spec->size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    if (traces) {
+        traces->resize((size_t)30 * N);
+        HIPCHK(hipMemcpyAsync(traces->data(), d_tr.p, traces->size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return GRTM_OK;
+}
+
+int write_outputs(const char *oname_w, const char *oname_t, int nw, const std::vector<double> &spec,
+                  const std::vector<double> &tr)
+{
+    const int N = 2 * nw;
+    FILE *ft = fopen(oname_t, "w");
+    if (!ft) return GRTM_E_IO;
+    for (int j = 0; j < N; j++) {
+        for (int c = 0; c < 30; c++) fprintf(ft, "%.18e,", tr[(size_t)c * N + j]);
+        fprintf(ft, "\n");
+    }
+    if (fclose(ft) != 0) return GRTM_E_IO;
+    FILE *fw = fopen(oname_w, "w");
+    if (!fw) return GRTM_E_IO;
+    for (int j = 0; j < N; j++) {
+        for (int c = 0; c < 30; c++) {
+            double re, im;
+            if (j < nw) {
+                re = spec[((size_t)c * nw + j) * 2];
+                im = spec[((size_t)c * nw + j) * 2 + 1];
+            } else if (j == nw) {
+                re = im = 0.0;
+            } else {  // conjugate mirror (green.cu:1925-1938)
+                re = spec[((size_t)c * nw + (N - j)) * 2];
+                im = -spec[((size_t)c * nw + (N - j)) * 2 + 1];
+            }
+            fprintf(fw, "%.18e,%.18e,", re, im);
+        }
+        fprintf(fw, "\n");
+    }
+    if (fclose(fw) != 0) return GRTM_E_IO;
+    return GRTM_OK;
+}
+
+}  // namespace
+
+// ==================================================================== C ABI
+extern "C" {
+
+const char *grtm_version(void) { return "grtm-mi355x 0.1 (gfx950)"; }
+
+const char *grtm_status_str(int s)
+{
+    switch (s) {
+    case GRTM_OK: return "ok";
+    case GRTM_E_IO: return "file i/o error";
+    case GRTM_E_PARSE: return "malformed input";
+    case GRTM_E_INPUT: return "invalid input values";
+    case GRTM_E_UNSUPPORTED: return "configuration not supported in this mode";
+    case GRTM_E_DEVICE: return "HIP device error";
+    case GRTM_E_NOMEM: return "device allocation failed";
+    default: return "unknown status";
+    }
+}
+
+int grtm_last_status(void) { return g_last_status; }
+
+int grtm_green_mem(const char *iname, int compat, double *spec, double *traces, int *nw_out)
+{
+    if (!iname) return GRTM_E_INPUT;
+    std::vector<double> sp, tr;
+    int nw = 0;
+    int st = green_core(iname, compat, spec ? &sp : nullptr, traces ? &tr : nullptr, &nw);
+    if (nw_out) *nw_out = nw;
+    if (st) return st;
+    if (spec) memcpy(spec, sp.data(), sp.size() * sizeof(double));
+    if (traces) memcpy(traces, tr.data(), tr.size() * sizeof(double));
+    return GRTM_OK;
+}
+
+int grtm_green(const char *iname, const char *oname_w, const char *oname_t, int compat)
+{
+    if (!iname || !oname_w || !oname_t) return GRTM_E_INPUT;
+    std::vector<double> sp, tr;
+    int nw = 0;
+    int st = green_core(iname, compat, &sp, &tr, &nw);
+    if (st) return st;
+    return write_outputs(oname_w, oname_t, nw, sp, tr);
+}
+
+void green(char *iname, char *oname_w, char *oname_t) { g_last_status = grtm_green(iname, oname_w, oname_t, 1); }
+
+}  // extern "C"
+
+// ============================================================ trace plans
+struct grtm_ctx {
+    int device = 0;
+};
+
+struct grtm_plan {
+    grtm_ctx *ctx = nullptr;
+    int prec = 0, compat = 0;
+    int nw = 0, N = 0, nk = 0, nr = 0, L = 0;
+    double dk = 0, df = 0, omega_i = 0;
+    Geo g{};
+    grtm::PostParams pp{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    DBuf lw, dists, A0, A1, B0, B1, C0, C1, tr;
+    double bessel_ms = 0;
+    ~grtm_plan()
+    {
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+size_t esize(int prec) { return prec == 0 ? sizeof(double) : sizeof(float); }
+
+int run_cells(grtm_plan *p, hipStream_t s)
+{
+    HIPCHK(grtm::launch_cells_zr(p->prec, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->nw,
+                                 p->A0.p, p->A1.p, p->nk, s));
+    return GRTM_OK;
+}
+int run_gemm(grtm_plan *p, hipStream_t s)
+{
+    HIPCHK(grtm::launch_gemm(p->prec, 4 * p->nw, p->nr, p->nk, p->A0.p, p->nk, p->B0.p, p->nr, p->C0.p, p->nr, s));
+    HIPCHK(grtm::launch_gemm(p->prec, 4 * p->nw, p->nr, p->nk, p->A1.p, p->nk, p->B1.p, p->nr, p->C1.p, p->nr, s));
+    return GRTM_OK;
+}
+int run_ifft(grtm_plan *p, hipStream_t s)
+{
+    HIPCHK(grtm::launch_post_ifft(p->prec, p->pp, 1, 4 * p->nr, nullptr, p->C0.p, p->C1.p, p->nr, p->nr, p->tr.p,
+                                  nullptr, s));
+    return GRTM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int grtm_create(grtm_ctx **ctx, int device)
+{
+    if (!ctx) return GRTM_E_INPUT;
+    *ctx = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return GRTM_E_DEVICE;
+    if (device < 0 || device >= n) return GRTM_E_INPUT;
+    HIPCHK(hipSetDevice(device));
+    grtm_ctx *c = new grtm_ctx;
+    c->device = device;
+    *ctx = c;
+    return GRTM_OK;
+}
+
+void grtm_destroy(grtm_ctx *ctx) { delete ctx; }
+
+int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, int precision, int compat,
+                     grtm_plan **plan_out)
+{
+    if (!ctx || !job || !grid || !plan_out) return GRTM_E_INPUT;
+    *plan_out = nullptr;
+    if (precision != GRTM_PREC_FP64 && precision != GRTM_PREC_FP32) return GRTM_E_INPUT;
+    if (job->n_layers < 2 || job->n_layers + 1 > grtm::kMaxLayers || !job->layers || job->nr < 1 || !job->dists)
+        return GRTM_E_INPUT;
+    if (grid->nw < 4 || (grid->nw & (grid->nw - 1)) || grid->nw > 8192 || grid->nk < 1 || !(grid->kmax > 0) ||
+        !(grid->dt > 0) || !(grid->taper >= 0 && grid->taper <= 1))
+        return GRTM_E_INPUT;
+    if (precision == GRTM_PREC_FP64 && grid->nw > 4096) return GRTM_E_UNSUPPORTED;  // IFFT LDS limit
+    for (int i = 0; i < job->nr; i++)
+        if (!(job->dists[i] > 0)) return GRTM_E_INPUT;
+    HIPCHK(hipSetDevice(ctx->device));
+    const int n = job->n_layers;
+    std::vector<double> top(n), vp(n), vs(n), rho(n), qp(n), qs(n);
+    double z = 0;
+    for (int i = 0; i < n; i++) {
+        const double *r = job->layers + 6 * i;
+        top[i] = z;
+        if (i < n - 1 && !(r[0] > 0)) return GRTM_E_INPUT;
+        z += r[0];
+        vp[i] = r[1], vs[i] = r[2], rho[i] = r[3], qp[i] = r[4], qs[i] = r[5];
+    }
+    Stack st;
+    int status = prepare_stack(compat, n, top.data(), vp.data(), vs.data(), rho.data(), qp.data(), qs.data(),
+                               job->src_depth, job->rcv_depth, st);
+    if (status) return status;
+    grtm_plan *p = new grtm_plan;
+    struct Guard {
+        grtm_plan *&p;
+        ~Guard() { delete p; }
+    } guard{p};
+    if ((status = make_geo(st, compat, p->g))) return status;
+    p->ctx = ctx;
+    p->prec = precision;
+    p->compat = compat;
+    p->nw = grid->nw;
+    p->N = 2 * grid->nw;
+    p->nk = grid->nk;
+    p->nr = job->nr;
+    p->L = st.L;
+    p->dk = grid->kmax / grid->nk;
+    const double T = p->N * grid->dt;
+    p->df = 1.0 / T;
+    p->omega_i = grid->omega_i > 0 ? grid->omega_i : M_PI / T;
+    p->pp.nw = p->nw;
+    p->pp.wc = (int)((double)p->nw * (1 - grid->taper));
+    p->pp.df = p->df;
+    p->pp.dt = grid->dt;
+    p->pp.t0 = grid->t0;
+    p->pp.omega_i = p->omega_i;
+    HIPCHK(hipStreamCreate(&p->stream));
+    p->own_stream = true;
+
+    // layer tables
+    std::vector<LayerW<double>> lw((size_t)p->nw * st.L);
+    for (int iw = 0; iw < p->nw; iw++) layer_w(st, compat, cd(2 * M_PI * p->df * iw, -p->omega_i), &lw[(size_t)iw * st.L]);
+    const size_t es = esize(precision);
+    if (precision == 0) {
+        if ((status = p->lw.alloc(lw.size() * sizeof(LayerW<double>)))) return status;
+        HIPCHK(hipMemcpy(p->lw.p, lw.data(), lw.size() * sizeof(LayerW<double>), hipMemcpyHostToDevice));
+    } else {
+        std::vector<LayerW<float>> lf(lw.size());
+        auto F = [](grtm::cx<double> z) { return grtm::cx<float>{float(z.x), float(z.y)}; };
+        for (size_t i = 0; i < lw.size(); i++) {
+            const LayerW<double> &d = lw[i];
+            lf[i] = LayerW<float>{F(d.sa), F(d.sb), F(d.ga), F(d.gb), F(d.ca), F(d.cb), F(d.mu), F(d.a), F(d.b),
+                                  float(d.h), 0.f};
+        }
+        if ((status = p->lw.alloc(lf.size() * sizeof(LayerW<float>)))) return status;
+        HIPCHK(hipMemcpy(p->lw.p, lf.data(), lf.size() * sizeof(LayerW<float>), hipMemcpyHostToDevice));
+    }
+    const size_t arows = (size_t)4 * p->nw;
+    if ((status = p->dists.alloc(sizeof(double) * p->nr))) return status;
+    if ((status = p->A0.alloc(arows * p->nk * es))) return status;
+    if ((status = p->A1.alloc(arows * p->nk * es))) return status;
+    if ((status = p->B0.alloc((size_t)p->nk * p->nr * es))) return status;
+    if ((status = p->B1.alloc((size_t)p->nk * p->nr * es))) return status;
+    if ((status = p->C0.alloc(arows * p->nr * es))) return status;
+    if ((status = p->C1.alloc(arows * p->nr * es))) return status;
+    if ((status = p->tr.alloc((size_t)4 * p->nr * p->N * es))) return status;
+    HIPCHK(hipMemcpy(p->dists.p, job->dists, sizeof(double) * p->nr, hipMemcpyHostToDevice));
+
+    // Bessel tables (planning work: they depend on the grid and distances only)
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, p->stream));
+    HIPCHK(grtm::launch_bessel_tab(precision, compat ? 0 : 1, p->nk, p->dk, p->dists.as<double>(), p->nr, p->B0.p,
+                                   p->B1.p, p->nr, p->stream));
+    HIPCHK(hipEventRecord(e1, p->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    p->bessel_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *plan_out = p;
+    p = nullptr;  // released from the guard
+    return GRTM_OK;
+}
+
+int grtm_plan_execute(grtm_plan *p, void *stream)
+{
+    if (!p) return GRTM_E_INPUT;
+    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    int st;
+    if ((st = run_cells(p, s))) return st;
+    if ((st = run_gemm(p, s))) return st;
+    return run_ifft(p, s);
+}
+
+int grtm_plan_sync(grtm_plan *p)
+{
+    if (!p) return GRTM_E_INPUT;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    HIPCHK(hipDeviceSynchronize());
+    return GRTM_OK;
+}
+
+int grtm_plan_traces_out(grtm_plan *p, double *traces)
+{
+    if (!p || !traces) return GRTM_E_INPUT;
+    const size_t n = (size_t)4 * p->nr * p->N;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (p->prec == 0) {
+        HIPCHK(hipMemcpy(traces, p->tr.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> f(n);
+        HIPCHK(hipMemcpy(f.data(), p->tr.p, n * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) traces[i] = f[i];
+    }
+    return GRTM_OK;
+}
+
+int grtm_plan_kernels_out(grtm_plan *p, int q, int w_lo, int w_hi, double *kern)
+{
+    if (!p || !kern || q < 0 || q > 1 || w_lo < 0 || w_hi > p->nw || w_lo >= w_hi) return GRTM_E_INPUT;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    const size_t es = esize(p->prec);
+    std::vector<unsigned char> buf((size_t)p->nk * es);
+    for (int iw = w_lo; iw < w_hi; iw++) {
+        for (int part = 0; part < 2; part++) {        // 0: Z (A0), 1: R (A1)
+            for (int ri = 0; ri < 2; ri++) {           // re, im
+                const DBuf &A = part ? p->A1 : p->A0;
+                const size_t row = (size_t)4 * iw + 2 * q + ri;
+                HIPCHK(hipMemcpy(buf.data(), (const unsigned char *)A.p + row * p->nk * es, p->nk * es,
+                                 hipMemcpyDeviceToHost));
+                for (int ik = 0; ik < p->nk; ik++) {
+                    double v = p->prec == 0 ? ((const double *)buf.data())[ik] : ((const float *)buf.data())[ik];
+                    const double wt = ik == 0 ? 0.5 * p->dk : p->dk;
+                    kern[((size_t)(iw - w_lo) * p->nk + ik) * 4 + part * 2 + ri] = v / wt;
+                }
+            }
+        }
+    }
+    return GRTM_OK;
+}
+
+int grtm_plan_timing(grtm_plan *p, int reps, double *ms)
+{
+    if (!p || !ms || reps < 1) return GRTM_E_INPUT;
+    hipEvent_t ev[4];
+    for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&ev[i]));
+    double acc[4] = {0, 0, 0, 0};
+    int st = GRTM_OK;
+    for (int r = 0; r < reps && !st; r++) {
+        HIPCHK(hipEventRecord(ev[0], p->stream));
+        if ((st = run_cells(p, p->stream))) break;
+        HIPCHK(hipEventRecord(ev[1], p->stream));
+        if ((st = run_gemm(p, p->stream))) break;
+        HIPCHK(hipEventRecord(ev[2], p->stream));
+        if ((st = run_ifft(p, p->stream))) break;
+        HIPCHK(hipEventRecord(ev[3], p->stream));
+        HIPCHK(hipEventSynchronize(ev[3]));
+        float a, b, c, d;
+        HIPCHK(hipEventElapsedTime(&a, ev[0], ev[3]));
+        HIPCHK(hipEventElapsedTime(&b, ev[0], ev[1]));
+        HIPCHK(hipEventElapsedTime(&c, ev[1], ev[2]));
+        HIPCHK(hipEventElapsedTime(&d, ev[2], ev[3]));
+        acc[0] += a, acc[1] += b, acc[2] += c, acc[3] += d;
+    }
+    for (int i = 0; i < 4; i++) (void)hipEventDestroy(ev[i]);
+    if (st) return st;
+    for (int i = 0; i < 4; i++) ms[i] = acc[i] / reps;
+    ms[4] = p->bessel_ms;
+    return GRTM_OK;
+}
+
+int grtm_plan_work(grtm_plan *p, double *w)
+{
+    if (!p || !w) return GRTM_E_INPUT;
+    const double cells = (double)p->nw * p->nk;
+    w[0] = cells;
+    w[1] = cells * p->L;
+    w[2] = 2.0 * 2.0 * (4.0 * p->nw) * p->nk * p->nr;  // two contractions, 2 flop per MAC
+    w[3] = 4.0 * p->nr * (8.0 * p->nw + 4.0 * p->N);   // complex64 in + fp32 out per trace component
+    w[4] = 2.0 * p->nr;                                 // (model, source, distance) traces, Z+R each
+    return GRTM_OK;
+}
+
+void grtm_plan_destroy(grtm_plan *p) { delete p; }
+
+// ------------------------------------------------------------ self-tests
+int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi, int n_layer, const double *lay,
+                       int s_layer, double zs, int r_layer, double zr, const double *par0_rv, double *out)
+{
+    if (n_layer < 2 || n_layer > grtm::kMaxLayers || !lay || !out) return GRTM_E_INPUT;
+    const cd w(wr, wi);
+    std::vector<LayerW<double>> lw(n_layer);
+    for (int j = 0; j < n_layer; j++) {
+        const double *l = lay + 7 * j;
+        const cd a(l[1], l[2]), b(l[3], l[4]), mu(l[5], l[6]);
+        auto C = [](cd z) { return grtm::cx<double>{z.real(), z.imag()}; };
+        lw[j].sa = C(w / a);
+        lw[j].sb = C(w / b);
+        lw[j].ga = C((w / a) * (w / a));
+        lw[j].gb = C((w / b) * (w / b));
+        lw[j].ca = C(a / w);
+        lw[j].cb = C(b / w);
+        lw[j].mu = C(mu);
+        lw[j].a = C(a);
+        lw[j].b = C(b);
+        lw[j].h = l[0];
+        lw[j].pad = 0;
+    }
+    Stack st;
+    st.L = n_layer;
+    st.s = s_layer;
+    st.rl = r_layer;
+    st.zs = zs;
+    st.zr = zr;
+    Geo g;
+    int status = make_geo(st, compat, g);
+    if (status) return status;
+    grtm::cx<double> p0r{0, 0}, p0v{0, 0};
+    if (par0_rv) p0r = {par0_rv[0], par0_rv[1]}, p0v = {par0_rv[2], par0_rv[3]};
+    grtm::CellOut<double> c;
+    grtm::Coef<double> p;
+    if (compat) grtm::cell_solve<double, true, true>(lw.data(), g, k, p0r, p0v, c);
+    else grtm::cell_solve<double, false, true>(lw.data(), g, k, p0r, p0v, c);
+    grtm::coef_from_cell<double, true>(c, src_type, g.up, k, grtm::cx<double>{wr, wi}, p);
+    const grtm::cx<double> v[18] = {p.a0[0], p.a0[1], p.a1[0], p.a1[1], p.a2[0], p.a2[1], p.b1, p.b2, p.a0t[0],
+                                    p.a0t[1], p.a1t[0], p.a1t[1], p.a2t[0], p.a2t[1], p.b1t, p.b2t, {0, 0}, {0, 0}};
+    for (int i = 0; i < 18; i++) out[2 * i] = v[i].x, out[2 * i + 1] = v[i].y;
+    return GRTM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,38 @@
+// kernels.h -- launch interface between the host driver and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "cell.h"
+
+namespace grtm {
+
+constexpr int kMaxLayers = 64;
+
+// Spectral post-processing (cusrc/popm/func/green.cu:1905-1921) and time
+// scaling (green.cu:2236-2237) parameters.
+struct PostParams {
+    int nw;          // number of frequencies (N = 2 nw)
+    int wc;          // taper start index (int)(nw (1 - taper))
+    double df;       // frequency step
+    double dt;       // time step
+    double t0;       // reduction time
+    double omega_i;  // imaginary frequency
+};
+
+// prec: 0 = FP64, 1 = FP32
+hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
+                           double df, double omega_i, int w0, int nwr, void *A0, void *A1, long lda,
+                           hipStream_t st);
+hipError_t launch_cells_full(int compat, int src, const void *lw, const int *nk_w, int L, const Geo &g,
+                             int nw, int nblk, double dk, double dist, double df, double omega_i,
+                             const double p0[4], double *partial, hipStream_t st);
+hipError_t launch_green_finish(int nw, int nblk, const double *partial, double *raw, hipStream_t st);
+hipError_t launch_bessel_tab(int prec, int mode, int nk, double dk, const double *dists, int nr, void *B0,
+                             void *B1, long ldb, hipStream_t st);
+hipError_t launch_gemm(int prec, int M, int N, int K, const void *A, long lda, const void *B, long ldb, void *Cm,
+                       long ldc, hipStream_t st);
+size_t post_ifft_lds_bytes(int prec, int nw);
+hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr, const void *raw, const void *C0,
+                            const void *C1, long ldc, int nr, void *traces, void *spec_post, hipStream_t st);
+
+}  // namespace grtm

@@ -1,0 +1,360 @@
+// kernels.hip -- HIP kernels of the Green's-function path for gfx950.
+//
+//   k_cells_zr     per-(omega,k) recursion -> Z/R k-kernels of the explosion
+//                  and vertical-force sources, weighted by the k-quadrature
+//                  and written as the A operands of the k-sum contraction
+//   k_cells_full   per-(omega,k) recursion -> all 30 integrand columns of the
+//                  drop-in output at one distance, reduced over k per block
+//   k_green_finish ordered sum of the per-block partials
+//   k_bessel_tab   J0(k r), J1(k r) tables (B operands of the contraction)
+//   k_gemm         C = A B on VALU (reference-precision path)
+//   k_post_ifft    phase / taper / scale + Hermitian C2R inverse FFT
+//                  (Stockham, in LDS) + exp(omega_I t) scaling
+#include <hip/hip_runtime.h>
+
+#include "cell.h"
+#include "integrand.h"
+#include "kernels.h"
+
+namespace grtm {
+
+constexpr int kCellThreads = 256;
+
+// ---------------------------------------------------------------- recursion
+template <typename T, bool COMPAT>
+__global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__restrict__ lw_all, int L,
+                                                           Geo g, int nk, double dk, double df,
+                                                           double omega_i, int w0, T *__restrict__ A0,
+                                                           T *__restrict__ A1, long lda)
+{
+    __shared__ LayerW<T> lw[kMaxLayers];
+    const int iw = blockIdx.y;  // row within this launch
+    const int iwg = w0 + iw;    // global frequency index
+    for (int j = threadIdx.x; j < L; j += blockDim.x) lw[j] = lw_all[(size_t)iw * L + j];
+    __syncthreads();
+    const int ik = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ik >= nk) return;
+    const T k = T(dk * ik);
+    const cx<T> w = C<T>(T(2.0 * M_PI * df * iwg), T(-omega_i));
+    CellOut<T> c;
+    cell_solve<T, COMPAT, false>(lw, g, k, C<T>(0, 0), C<T>(0, 0), c);
+    cx<T> kze, kre, kzv, krv;
+    zr_from_cell<T, COMPAT>(c, g.up, k, w, kze, kre, kzv, krv);
+    const T wt = T(ik == 0 ? 0.5 * dk : dk);
+    const size_t r0 = (size_t)(4 * iw) * lda + ik;
+    A0[r0] = wt * kze.x;
+    A0[r0 + lda] = wt * kze.y;
+    A0[r0 + 2 * lda] = wt * kzv.x;
+    A0[r0 + 3 * lda] = wt * kzv.y;
+    A1[r0] = wt * kre.x;
+    A1[r0 + lda] = wt * kre.y;
+    A1[r0 + 2 * lda] = wt * krv.x;
+    A1[r0 + 3 * lda] = wt * krv.y;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename T, bool COMPAT, int SRC>
+__global__ __launch_bounds__(kCellThreads) void k_cells_full(const LayerW<T> *__restrict__ lw_all,
+                                                             const int *__restrict__ nk_w, int L, Geo g,
+                                                             double dk, double dist, double df,
+                                                             double omega_i, double p0r_x, double p0r_y,
+                                                             double p0v_x, double p0v_y, int nblk,
+                                                             double *__restrict__ partial)
+{
+    __shared__ LayerW<T> lw[kMaxLayers];
+    __shared__ double red[kCellThreads / 64][60];
+    const int iw = blockIdx.y;
+    for (int j = threadIdx.x; j < L; j += blockDim.x) lw[j] = lw_all[(size_t)iw * L + j];
+    __syncthreads();
+    const int ik = blockIdx.x * blockDim.x + threadIdx.x;
+    cx<double> col[30];
+    for (int i = 0; i < 30; i++) col[i] = cx<double>{0, 0};
+    if (ik < nk_w[iw]) {
+        const T k = T(dk * ik);
+        const cx<T> w = C<T>(T(2.0 * M_PI * df * iw), T(-omega_i));
+        CellOut<T> c;
+        cell_solve<T, COMPAT, true>(lw, g, k, C<T>(T(p0r_x), T(p0r_y)), C<T>(T(p0v_x), T(p0v_y)), c);
+        Coef<T> p;
+        coef_from_cell<T, true>(c, SRC, g.up, k, w, p);
+        BesselSet bs;
+        bessel_set(COMPAT ? 0 : 1, dist * double(k), bs);
+        integrand_cols<T>(COMPAT ? 1 : 0, SRC, p, double(k), dist, bs, col);
+        const double wt = (ik == 0) ? dk / 2.0 : dk;
+        for (int i = 0; i < 30; i++) col[i] = col[i] * wt;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = 0; i < 30; i++) {
+        double sx = wave_sum(col[i].x), sy = wave_sum(col[i].y);
+        if (lane == 0) {
+            red[wv][2 * i] = sx;
+            red[wv][2 * i + 1] = sy;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 60) {
+        double s = 0;
+        for (int q = 0; q < kCellThreads / 64; q++) s += red[q][threadIdx.x];
+        partial[((size_t)iw * nblk + blockIdx.x) * 60 + threadIdx.x] = s;
+    }
+}
+
+__global__ void k_green_finish(int nw, int nblk, const double *__restrict__ partial, double *__restrict__ raw)
+{
+    // raw[(col * nw + iw) * 2 + {re, im}]
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nw * 60) return;
+    const int iw = t / 60, q = t % 60;
+    double s = 0;
+    for (int b = 0; b < nblk; b++) s += partial[((size_t)iw * nblk + b) * 60 + q];
+    const int col = q >> 1, ri = q & 1;
+    raw[((size_t)col * nw + iw) * 2 + ri] = s;
+}
+
+// ---------------------------------------------------------------- Bessel tables
+template <typename T>
+__global__ void k_bessel_tab(int mode, int nk, double dk, const double *__restrict__ dists, int nr,
+                             T *__restrict__ B0, T *__restrict__ B1, long ldb)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nk * nr) return;
+    const int ik = int(t / nr), ir = int(t % nr);
+    const double x = dk * ik * dists[ir];  // phase formed in FP64
+    double j0v, j1v;
+    if (mode == 0) {
+        j0v = as_j0(x);
+        j1v = as_j1(x);
+    } else {
+        j0v = j0(x);
+        j1v = j1(x);
+    }
+    B0[(size_t)ik * ldb + ir] = T(j0v);
+    B1[(size_t)ik * ldb + ir] = T(j1v);
+}
+
+// ---------------------------------------------------------------- VALU GEMM
+// C[M x N] = A[M x K] B[K x N], row-major, 64x64 tiles, 256 threads x 4x4.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T *__restrict__ A, long lda,
+                                              const T *__restrict__ B, long ldb, T *__restrict__ Cm,
+                                              long ldc)
+{
+    constexpr int BM = 64, BN = 64, BK = 16;
+    __shared__ T As[BK][BM + 1];
+    __shared__ T Bs[BK][BN + 1];
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    T acc[4][4];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) acc[i][j] = T(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        for (int e = threadIdx.x; e < BM * BK; e += 256) {
+            int mm = e / BK, kk = e % BK;
+            int gm = m0 + mm, gk = k0 + kk;
+            As[kk][mm] = (gm < M && gk < K) ? A[(size_t)gm * lda + gk] : T(0);
+        }
+        for (int e = threadIdx.x; e < BN * BK; e += 256) {
+            int kk = e / BN, nn = e % BN;
+            int gk = k0 + kk, gn = n0 + nn;
+            Bs[kk][nn] = (gk < K && gn < N) ? B[(size_t)gk * ldb + gn] : T(0);
+        }
+        __syncthreads();
+        for (int kk = 0; kk < BK; kk++) {
+            T a[4], b[4];
+            for (int i = 0; i < 4; i++) a[i] = As[kk][ty * 4 + i];
+            for (int j = 0; j < 4; j++) b[j] = Bs[kk][tx * 4 + j];
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) acc[i][j] += a[i] * b[j];
+        }
+        __syncthreads();
+    }
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            int gm = m0 + ty * 4 + i, gn = n0 + tx * 4 + j;
+            if (gm < M && gn < N) Cm[(size_t)gm * ldc + gn] = acc[i][j];
+        }
+}
+
+// ---------------------------------------------------------------- post + IFFT
+template <typename T>
+__device__ __forceinline__ cx<T> post_val(cx<T> v, int iw, const PostParams &pp)
+{
+    const double phi = 2.0 * M_PI * pp.df * iw * pp.t0;
+    double s, c;
+    sincos(phi, &s, &c);
+    double st0 = 1.0;
+    if (iw > pp.wc) st0 = 0.5 * (1. + cos((iw - pp.wc + 1) * M_PI / (pp.nw - pp.wc + 1)));
+    const double coef = pp.df * pp.nw / M_PI * st0;
+    const cx<T> ph = C<T>(T(c * coef), T(s * coef));
+    return v * ph;
+}
+
+// One workgroup per trace component.  layout 0: raw[(t*nw + iw)*2 + {re,im}];
+// layout 1: rows of the contraction outputs: component c = (t / nr) % 2 reads
+// C_c[(4 iw + 2 q + {0,1}) * ldc + ir] with q = t / (2 nr), ir = t % nr.
+template <typename T>
+__global__ __launch_bounds__(256) void k_post_ifft(PostParams pp, int layout, const T *__restrict__ raw,
+                                                   const T *__restrict__ C0, const T *__restrict__ C1,
+                                                   long ldc, int nr, T *__restrict__ traces,
+                                                   T *__restrict__ spec_post)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *bufA = reinterpret_cast<cx<T> *>(smem_raw);
+    const int M = pp.nw, N = 2 * pp.nw;
+    cx<T> *bufB = bufA + M;
+    const int t = blockIdx.x;
+    auto load = [&](int iw) -> cx<T> {
+        if (layout == 0) {
+            const T *p = raw + ((size_t)t * M + iw) * 2;
+            return C<T>(p[0], p[1]);
+        }
+        const int q = t / (2 * nr), c = (t / nr) % 2, ir = t % nr;
+        const T *Cc = c ? C1 : C0;
+        const size_t row = (size_t)(4 * iw + 2 * q);
+        return C<T>(Cc[row * ldc + ir], Cc[(row + 1) * ldc + ir]);
+    };
+    for (int m = threadIdx.x; m < M; m += blockDim.x) {
+        cx<T> Xm = post_val(load(m), m, pp);
+        if (spec_post) {
+            spec_post[((size_t)t * M + m) * 2] = Xm.x;
+            spec_post[((size_t)t * M + m) * 2 + 1] = Xm.y;
+        }
+        cx<T> Xc = C<T>(0, 0);  // conj(X[M-m]); X[M] (Nyquist) = 0
+        if (m > 0) Xc = conj(post_val(load(M - m), M - m, pp));
+        if (m == 0) Xm.y = T(0);  // Im X[0] only reaches Im of the output
+        if (M - m == 0) Xc.y = T(0);
+        const cx<T> Xe = Xm + Xc;
+        double sn, cs;
+        sincospi(double(m) / double(M), &sn, &cs);  // e^{+2 pi i m / N}
+        const cx<T> Xo = (Xm - Xc) * C<T>(T(cs), T(sn));
+        bufA[m] = C<T>(Xe.x - Xo.y, Xe.y + Xo.x);  // Xe + i Xo
+    }
+    __syncthreads();
+    // Stockham radix-2 (decimation in frequency), inverse sign
+    cx<T> *x = bufA, *y = bufB;
+    int s = 1;
+    for (int n = M; n > 1; n >>= 1) {
+        const int m = n >> 1;
+        for (int idx = threadIdx.x; idx < M / 2; idx += blockDim.x) {
+            const int p = idx / s, q = idx - p * s;
+            double sn, cs;
+            sincospi(2.0 * double(p) / double(n), &sn, &cs);
+            const cx<T> a = x[q + s * p], b = x[q + s * (p + m)];
+            y[q + s * (2 * p)] = a + b;
+            y[q + s * (2 * p + 1)] = (a - b) * C<T>(T(cs), T(sn));
+        }
+        __syncthreads();
+        cx<T> *tmp = x;
+        x = y;
+        y = tmp;
+        s <<= 1;
+    }
+    const double invN = 1.0 / double(N);
+    for (int nn = threadIdx.x; nn < M; nn += blockDim.x) {
+        const cx<T> z = x[nn];
+        const int j0 = 2 * nn, j1 = 2 * nn + 1;
+        traces[(size_t)t * N + j0] = T(double(z.x) * invN * exp(pp.omega_i * (j0 * pp.dt + pp.t0)));
+        traces[(size_t)t * N + j1] = T(double(z.y) * invN * exp(pp.omega_i * (j1 * pp.dt + pp.t0)));
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+template <typename T, bool COMPAT>
+static hipError_t launch_cells_zr_t(const void *lw, int L, const Geo &g, int nk, double dk, double df,
+                                    double omega_i, int w0, int nwr, void *A0, void *A1, long lda,
+                                    hipStream_t st)
+{
+    dim3 grid((nk + kCellThreads - 1) / kCellThreads, nwr);
+    hipLaunchKernelGGL((k_cells_zr<T, COMPAT>), grid, dim3(kCellThreads), 0, st,
+                       (const LayerW<T> *)lw, L, g, nk, dk, df, omega_i, w0, (T *)A0, (T *)A1, lda);
+    return hipGetLastError();
+}
+
+hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
+                           double df, double omega_i, int w0, int nwr, void *A0, void *A1, long lda,
+                           hipStream_t st)
+{
+    if (prec == 0) {
+        return compat ? launch_cells_zr_t<double, true>(lw, L, g, nk, dk, df, omega_i, w0, nwr, A0, A1, lda, st)
+                      : launch_cells_zr_t<double, false>(lw, L, g, nk, dk, df, omega_i, w0, nwr, A0, A1, lda, st);
+    }
+    return compat ? launch_cells_zr_t<float, true>(lw, L, g, nk, dk, df, omega_i, w0, nwr, A0, A1, lda, st)
+                  : launch_cells_zr_t<float, false>(lw, L, g, nk, dk, df, omega_i, w0, nwr, A0, A1, lda, st);
+}
+
+hipError_t launch_cells_full(int compat, int src, const void *lw, const int *nk_w, int L, const Geo &g,
+                             int nw, int nblk, double dk, double dist, double df, double omega_i,
+                             const double p0[4], double *partial, hipStream_t st)
+{
+    dim3 grid(nblk, nw);
+#define GRTM_FULL(CP, SR)                                                                              \
+    hipLaunchKernelGGL((k_cells_full<double, CP, SR>), grid, dim3(kCellThreads), 0, st,              \
+                       (const LayerW<double> *)lw, nk_w, L, g, dk, dist, df, omega_i, p0[0], p0[1],  \
+                       p0[2], p0[3], nblk, partial)
+    if (compat) {
+        if (src == 0) GRTM_FULL(true, 0);
+        else GRTM_FULL(true, 1);
+    } else {
+        if (src == 0) GRTM_FULL(false, 0);
+        else GRTM_FULL(false, 1);
+    }
+#undef GRTM_FULL
+    return hipGetLastError();
+}
+
+hipError_t launch_green_finish(int nw, int nblk, const double *partial, double *raw, hipStream_t st)
+{
+    const int n = nw * 60;
+    hipLaunchKernelGGL(k_green_finish, dim3((n + 255) / 256), dim3(256), 0, st, nw, nblk, partial, raw);
+    return hipGetLastError();
+}
+
+hipError_t launch_bessel_tab(int prec, int mode, int nk, double dk, const double *dists, int nr, void *B0,
+                             void *B1, long ldb, hipStream_t st)
+{
+    const size_t n = (size_t)nk * nr;
+    dim3 grid((unsigned)((n + 255) / 256));
+    if (prec == 0)
+        hipLaunchKernelGGL(k_bessel_tab<double>, grid, dim3(256), 0, st, mode, nk, dk, dists, nr, (double *)B0,
+                           (double *)B1, ldb);
+    else
+        hipLaunchKernelGGL(k_bessel_tab<float>, grid, dim3(256), 0, st, mode, nk, dk, dists, nr, (float *)B0,
+                           (float *)B1, ldb);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm(int prec, int M, int N, int K, const void *A, long lda, const void *B, long ldb, void *Cm,
+                       long ldc, hipStream_t st)
+{
+    dim3 grid((N + 63) / 64, (M + 63) / 64);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_gemm<double>, grid, dim3(256), 0, st, M, N, K, (const double *)A, lda,
+                           (const double *)B, ldb, (double *)Cm, ldc);
+    else
+        hipLaunchKernelGGL(k_gemm<float>, grid, dim3(256), 0, st, M, N, K, (const float *)A, lda,
+                           (const float *)B, ldb, (float *)Cm, ldc);
+    return hipGetLastError();
+}
+
+size_t post_ifft_lds_bytes(int prec, int nw) { return (size_t)2 * nw * (prec == 0 ? 16 : 8); }
+
+hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr, const void *raw, const void *C0,
+                            const void *C1, long ldc, int nr, void *traces, void *spec_post, hipStream_t st)
+{
+    const size_t lds = post_ifft_lds_bytes(prec, pp.nw);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_post_ifft<double>, dim3(ntr), dim3(256), lds, st, pp, layout, (const double *)raw,
+                           (const double *)C0, (const double *)C1, ldc, nr, (double *)traces,
+                           (double *)spec_post);
+    else
+        hipLaunchKernelGGL(k_post_ifft<float>, dim3(ntr), dim3(256), lds, st, pp, layout, (const float *)raw,
+                           (const float *)C0, (const float *)C1, ldc, nr, (float *)traces, (float *)spec_post);
+    return hipGetLastError();
+}
+
+}  // namespace grtm

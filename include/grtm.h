@@ -1,0 +1,119 @@
+/*
+ * grtm.h -- C ABI of libgrtm.so, the MI355X (gfx950) implementation of the
+ * layered-medium Green's-function path of cangyeone/grtm_cuda.
+ *
+ * Plain C types only: host pointers, sizes, and an opaque `void *stream`
+ * (a hipStream_t, or NULL for the library's own stream).
+ */
+#ifndef GRTM_H_
+#define GRTM_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define GRTM_OK 0
+#define GRTM_E_IO -1          /* cannot open / write a file */
+#define GRTM_E_PARSE -2       /* malformed input.txt */
+#define GRTM_E_INPUT -3       /* invalid values (sizes, NaN, depths, ...) */
+#define GRTM_E_UNSUPPORTED -4 /* a configuration the selected mode cannot run */
+#define GRTM_E_DEVICE -5      /* HIP runtime error or no GPU */
+#define GRTM_E_NOMEM -6       /* device allocation failed */
+
+#define GRTM_PREC_FP64 0 /* FP64 recursion, FP64 contraction */
+#define GRTM_PREC_FP32 1 /* FP32 recursion, FP32 contraction */
+
+/* ---------------------------------------------------------- drop-in entry */
+/* Replaces the reference's only exported symbol:
+ *   void green(char *iname, char *oname_w, char *oname_t);
+ * declared at cusrc/popm/popm.h:159-161, defined at
+ * cusrc/popm/func/green.cu:1975-2281.  Same input.txt format, same output
+ * files (oname_w: N rows of 30 "re,im," pairs; oname_t: N rows of 30 reals),
+ * same column meaning, computed on the GPU.  Reproduces the reference's
+ * behaviour including its quirks (compat mode).  Prints nothing; errors are
+ * reported through grtm_last_status(). */
+void green(char *iname, char *oname_w, char *oname_t);
+
+/* Status-returning form of green(); compat = 1 reproduces the reference,
+ * compat = 0 runs the corrected physics (receiver layer located from the
+ * receiver depth, attenuated mu, fixed d/dr columns, ...). */
+int grtm_green(const char *iname, const char *oname_w, const char *oname_t, int compat);
+
+/* In-memory form used by bindings and tests.  spec: 30 x nw complex
+ * (interleaved re,im; the post-processed spectrum rows 0..nw-1 of oname_w),
+ * traces: 30 x N reals (column c of oname_t at [c*N + j]).  Either may be
+ * NULL.  *nw_out receives nw (call first with both NULL to size buffers). */
+int grtm_green_mem(const char *iname, int compat, double *spec, double *traces, int *nw_out);
+
+/* Status of the last green() call on this thread. */
+int grtm_last_status(void);
+const char *grtm_status_str(int status);
+const char *grtm_version(void);
+
+/* -------------------------------------------------- multi-receiver traces */
+/* One layered model, one receiver depth, many epicentral distances, two
+ * source kinds (explosion = isotropic moment tensor, vertical force) from
+ * one recursion.  Replaces the reference's per-distance green() call loop
+ * (the reference handles one distance per call, green.cu:2052) and its host
+ * k-sum (green.cu:1878-1903) with a k-sum contraction on the GPU. */
+typedef struct grtm_ctx grtm_ctx;
+typedef struct grtm_plan grtm_plan;
+
+typedef struct {
+    int nw;          /* frequencies; N = 2 nw time samples */
+    double dt;       /* sample interval (s) */
+    int nk;          /* uniform wavenumbers k_i = i dk, i < nk */
+    double kmax;     /* dk = kmax / nk */
+    double omega_i;  /* imaginary frequency; <= 0 selects pi / T */
+    double t0;       /* reduction time (s) */
+    double taper;    /* fraction of the band cosine-tapered at the top */
+} grtm_grid;
+
+typedef struct {
+    int n_layers;          /* rows including the half-space */
+    const double *layers;  /* n_layers x 6: thickness km, vp, vs, rho, qp, qs
+                              (thickness of the last row is ignored) */
+    double src_depth;      /* km */
+    double rcv_depth;      /* km */
+    int nr;                /* number of distances */
+    const double *dists;   /* km */
+} grtm_job;
+
+int grtm_create(grtm_ctx **ctx, int device);
+void grtm_destroy(grtm_ctx *ctx);
+
+/* Prepare a job: layer tables, Bessel tables and all device buffers. */
+int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, int precision, int compat,
+                     grtm_plan **plan);
+/* Run recursion + contraction + IFFT on resident data (asynchronous). */
+int grtm_plan_execute(grtm_plan *plan, void *stream);
+int grtm_plan_sync(grtm_plan *plan);
+/* Copy the traces out: traces[((q*2 + c)*nr + ir)*N + j], q: 0 explosion,
+ * 1 vertical force; c: 0 Z, 1 R (radial). */
+int grtm_plan_traces_out(grtm_plan *plan, double *traces);
+/* Raw k-kernels (tests): kern[((iw*nk + ik)*4 + {zr,zi,rr,ri})] for source q,
+ * unweighted, for frequency rows [w_lo, w_hi). */
+int grtm_plan_kernels_out(grtm_plan *plan, int q, int w_lo, int w_hi, double *kern);
+/* Timing on the plan's stream with HIP events: reps executions;
+ * ms[0] total per execution, ms[1] recursion, ms[2] contraction,
+ * ms[3] IFFT, ms[4] Bessel tables (one-off, part of planning). */
+int grtm_plan_timing(grtm_plan *plan, int reps, double *ms);
+/* Work of one execution: [0] cells, [1] cell-layers, [2] contraction flops,
+ * [3] IFFT algorithmic bytes, [4] traces. */
+int grtm_plan_work(grtm_plan *plan, double *work);
+void grtm_plan_destroy(grtm_plan *plan);
+
+/* ------------------------------------------------------------ self-tests */
+/* Host-compiled copies of the device arithmetic, for unit checks without a
+ * GPU.  Not used by any entry point above. */
+/* lay: n_layer rows of (h, alpha re, im, beta re, im, mu re, im); out as
+ * the oracle's cell: a0[2],a1[2],a2[2],b1,b2 and d/dz versions, 18 complex. */
+int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi, int n_layer,
+                       const double *lay, int s_layer, double zs, int r_layer, double zr,
+                       const double *par0_rv, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
