@@ -1,0 +1,133 @@
+"""Benchmark: Green's traces/s of the GPU path on the production workload.
+
+Workload (one GPU): 8 layers over a half-space from a recorded seed,
+explosion + vertical force at 12 km, 256 receivers 5-500 km, dt 0.05 s,
+Nw = 2048 (N = 4096), Nk = 16384 uniform to 1.5 w_max / Vs_min.  One step =
+one pass of recursion + k-sum contraction + inverse FFT producing
+2 sources x 256 distances = 512 traces (Z and R each), inputs resident in HBM.
+
+Prints one JSON line (rank 0).  The CPU baseline is the FP64 oracle on a
+bounded sample of the same workload (a block of frequency rows), run on the
+host cores and scaled to the full job.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+
+def cpu_baseline(cfg, rows, threads):
+    """FP64 oracle on frequency rows [lo, lo+rows) of the workload, both sources."""
+    from oracle import oracle
+    nw = cfg["grid"]["nw"]
+    lo = nw // 2
+    t0 = time.perf_counter()
+    for kind in (0, 1):
+        oracle.traces(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], kind, cfg["grid"],
+                      w_range=(lo, lo + rows), nthreads=threads)
+    dt = time.perf_counter() - t0
+    full = dt * nw / rows
+    traces = 2 * len(cfg["dists"])
+    return dict(value=traces / full, unit="traces/s", cores=threads, kind="port",
+                sample=f"FP64 oracle, {rows} of {nw} frequency rows x {cfg['grid']['nk']} k x "
+                       f"{len(cfg['dists'])} receivers, both sources, {dt:.1f} s on {threads} threads, "
+                       f"scaled x{nw / rows:.0f}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--cpu-rows", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.exit("bench.py: this path runs on one GPU; multi-GPU runs are not part of this build (use --gpus 1)")
+
+    import grtm_cuda_amd as g
+    from grtm_cuda_amd import models
+
+    cfg = models.config(args.workload)
+    prec = g.PREC_FP64 if args.precision == "fp64" else g.PREC_FP32
+    print(f"bench: planning {args.workload} {args.precision}", file=sys.stderr, flush=True)
+    plan = g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"], precision=prec)
+    work = plan.work()
+    for _ in range(args.warmup):
+        plan.execute()
+    plan.sync()
+    print("bench: timing", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.execute()
+    plan.sync()
+    elapsed = time.perf_counter() - t0
+    ms_step = 1e3 * elapsed / args.steps
+    traces_per_step = work["traces"]
+    value = traces_per_step * args.steps / elapsed
+
+    # per-kernel durations with HIP events on the plan's stream
+    tm = plan.timing(reps=3)
+    kernels = {"recursion": tm["cells"], "ksum": tm["ksum"], "ifft": tm["ifft"]}
+    dom = max(kernels, key=kernels.get)
+    peak = FP64_PEAK_TFLOPS if prec == g.PREC_FP64 else FP32_PEAK_TFLOPS
+    if dom == "recursion":
+        flops = work["cells"] * work["flops_per_cell"]
+        achieved = flops / (tm["cells"] * 1e-3) / 1e12
+        roof = dict(bound="valu", achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak, traffic=None,
+                    kernel="k_cells_zr", flops_per_cell=work["flops_per_cell"],
+                    transcendentals_per_cell=work["trans_per_cell"])
+    elif dom == "ksum":
+        achieved = work["ksum_flop"] / (tm["ksum"] * 1e-3) / 1e12
+        roof = dict(bound="mfma" if prec != g.PREC_FP64 else "valu", achieved=achieved, peak=peak, unit="TFLOP/s",
+                    frac=achieved / peak, traffic=None, kernel="k_gemm")
+    else:
+        achieved = work["ifft_bytes"] / (tm["ifft"] * 1e-3) / 1e9
+        roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                    traffic=None, kernel="k_post_ifft")
+
+    cpu = None
+    if not args.no_cpu:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(cfg, args.cpu_rows, threads)
+
+    line = {
+        "metric": "Green's traces/s at <=1e-4 rel-L2 vs FP64; learned-adaptive vs plain dense-k",
+        "value": value,
+        "unit": "traces/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == g.PREC_FP64 else "f32",
+        "data": "synthetic (seeded layered model)",
+        "config": {"workload": f"{args.workload}: plain dense-k, {len(cfg['layers']) - 1} layers + half-space, "
+                               f"explosion + vertical force, {len(cfg['dists'])} receivers, "
+                               f"Nw={cfg['grid']['nw']}, Nk={cfg['grid']['nk']}",
+                   "path": args.precision, "traces_per_step": traces_per_step},
+        "kernel_ms": {k: round(v, 4) for k, v in kernels.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

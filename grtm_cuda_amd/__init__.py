@@ -1,0 +1,138 @@
+"""grtm_cuda_amd -- MI355X (gfx950) Green's functions for layered media.
+
+Host-side mirror of the reference's interface (cangyeone/grtm_cuda):
+
+* ``green(iname, oname_w, oname_t)`` -- the reference's only entry point
+  (cusrc/popm/popm.h:159-161): reads input.txt, writes the spectrum and trace
+  files, computed on the GPU.
+* ``green_mem(iname)`` -- the same run returning arrays.
+* ``TracePlan`` -- one model, many receiver distances, explosion and
+  vertical-force Z/R traces from one recursion (the production path).
+
+All compute runs in libgrtm.so (HIP).  There is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import inputfile, models  # noqa: F401
+from .ffi import (GRTM_OK, PREC_FP32, PREC_FP64, EXPORTS, GrtmError, Grid, Job, D, P,  # noqa: F401
+                   _lib, check)
+
+__all__ = ["green", "green_mem", "TracePlan", "GrtmError", "PREC_FP64", "PREC_FP32", "models", "inputfile"]
+
+
+def green(iname, oname_w, oname_t, compat=True):
+    """Run the drop-in path: read ``iname`` (input.txt format), write
+    ``oname_w`` (N rows x 30 complex) and ``oname_t`` (N rows x 30 reals).
+    Raises GrtmError on failure (the reference returns void and ignores
+    errors; the library's ``green`` symbol keeps that signature)."""
+    check(_lib().grtm_green(iname.encode(), oname_w.encode(), oname_t.encode(), int(bool(compat))), "green")
+
+
+def green_mem(iname, compat=True):
+    """Returns (spec, traces): spec[30, nw] complex post-processed spectrum
+    (rows 0..nw-1 of oname_w), traces[30, N] (columns of oname_t)."""
+    lib = _lib()
+    nw = ctypes.c_int(0)
+    check(lib.grtm_green_mem(iname.encode(), int(bool(compat)), None, None, ctypes.byref(nw)), "green_mem")
+    n = nw.value
+    spec = np.zeros((30, n, 2))
+    tr = np.zeros((30, 2 * n))
+    check(lib.grtm_green_mem(iname.encode(), int(bool(compat)), spec.ctypes.data_as(P), tr.ctypes.data_as(P),
+                             ctypes.byref(nw)), "green_mem")
+    return spec[..., 0] + 1j * spec[..., 1], tr
+
+
+class _Ctx:
+    def __init__(self, device=0):
+        self.ptr = ctypes.c_void_p()
+        check(_lib().grtm_create(ctypes.byref(self.ptr), int(device)), "grtm_create")
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            _lib().grtm_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+
+_CTX = {}
+
+
+def _ctx(device=0):
+    if device not in _CTX:
+        _CTX[device] = _Ctx(device)
+    return _CTX[device]
+
+
+class TracePlan:
+    """Explosion and vertical-force Z/R traces of one layered model at many
+    distances.  ``layers`` rows: (thickness km, vp, vs, rho, qp, qs), last row
+    the half-space.  ``grid``: dict with nw, dt, nk, kmax, omega_i, t0, taper."""
+
+    def __init__(self, layers, src_depth, rcv_depth, dists, grid, precision=PREC_FP64, compat=False, device=0):
+        self._ctx = _ctx(device)
+        self.layers = np.ascontiguousarray(layers, dtype=np.float64)
+        self.dists = np.ascontiguousarray(dists, dtype=np.float64)
+        self.grid = dict(grid)
+        self.nw = int(grid["nw"])
+        self.N = 2 * self.nw
+        self.nk = int(grid["nk"])
+        self.nr = len(self.dists)
+        self.precision = precision
+        g = Grid(self.nw, float(grid["dt"]), self.nk, float(grid["kmax"]), float(grid.get("omega_i", 0.0)),
+                 float(grid.get("t0", 0.0)), float(grid.get("taper", 0.0)))
+        j = Job(self.layers.shape[0], self.layers.ctypes.data_as(P), float(src_depth), float(rcv_depth), self.nr,
+                self.dists.ctypes.data_as(P))
+        self.ptr = ctypes.c_void_p()
+        check(_lib().grtm_plan_traces(self._ctx.ptr, ctypes.byref(j), ctypes.byref(g), int(precision),
+                                      int(bool(compat)), ctypes.byref(self.ptr)), "grtm_plan_traces")
+
+    def execute(self):
+        check(_lib().grtm_plan_execute(self.ptr, None), "grtm_plan_execute")
+
+    def sync(self):
+        check(_lib().grtm_plan_sync(self.ptr), "grtm_plan_sync")
+
+    def traces(self):
+        """Run and return traces[2 (explosion, vertical force), 2 (Z, R), nr, N]."""
+        self.execute()
+        out = np.zeros((2, 2, self.nr, self.N))
+        check(_lib().grtm_plan_traces_out(self.ptr, out.ctypes.data_as(P)), "grtm_plan_traces_out")
+        return out
+
+    def kernels(self, q, w_lo, w_hi):
+        """Unweighted k-kernels of source q for frequency rows [w_lo, w_hi):
+        array [w_hi - w_lo, nk, 2 (Z, R)] complex (after an execute())."""
+        out = np.zeros((w_hi - w_lo, self.nk, 4))
+        check(_lib().grtm_plan_kernels_out(self.ptr, int(q), int(w_lo), int(w_hi), out.ctypes.data_as(P)),
+              "grtm_plan_kernels_out")
+        return out[..., 0::2] + 1j * out[..., 1::2]
+
+    def timing(self, reps=5):
+        """ms per execution: total, recursion, contraction, IFFT, Bessel tables."""
+        ms = np.zeros(5)
+        check(_lib().grtm_plan_timing(self.ptr, int(reps), ms.ctypes.data_as(P)), "grtm_plan_timing")
+        return dict(total=ms[0], cells=ms[1], ksum=ms[2], ifft=ms[3], bessel=ms[4])
+
+    def work(self):
+        w = np.zeros(7)
+        check(_lib().grtm_plan_work(self.ptr, w.ctypes.data_as(P)), "grtm_plan_work")
+        return dict(cells=w[0], cell_layers=w[1], ksum_flop=w[2], ifft_bytes=w[3], traces=w[4],
+                    flops_per_cell=w[5], trans_per_cell=w[6])
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            _lib().grtm_plan_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+
+def selftest_cell(compat, src_type, k, w, lay, s_layer, zs, r_layer, zr, par0_rv=None):
+    """Host-compiled cell solve (test hook): returns 18 complex coefficients."""
+    lay = np.ascontiguousarray(lay, dtype=np.float64)
+    out = np.zeros(36)
+    p0 = None if par0_rv is None else np.ascontiguousarray(par0_rv, dtype=np.float64)
+    check(_lib().grtm_selftest_cell(int(compat), int(src_type), float(k), float(w.real), float(w.imag),
+                                    lay.shape[0], lay.ctypes.data_as(P), int(s_layer), float(zs), int(r_layer),
+                                    float(zr), None if p0 is None else p0.ctypes.data_as(P),
+                                    out.ctypes.data_as(P)), "grtm_selftest_cell")
+    return out[0::2] + 1j * out[1::2]

@@ -718,6 +718,10 @@ int grtm_plan_work(grtm_plan *p, double *w)
     w[2] = 2.0 * 2.0 * (4.0 * p->nw) * p->nk * p->nr;  // two contractions, 2 flop per MAC
     w[3] = 4.0 * p->nr * (8.0 * p->nw + 4.0 * p->N);   // complex64 in + fp32 out per trace component
     w[4] = 2.0 * p->nr;                                 // (model, source, distance) traces, Z+R each
+    double oc[2] = {0, 0};
+    if (grtm_opcount_cell(p->L, p->g.s, p->g.rl, p->g.branch, p->g.up, oc)) return GRTM_E_INPUT;
+    w[5] = oc[0];  // flops per cell
+    w[6] = oc[1];  // transcendental calls per cell
     return GRTM_OK;
 }
 

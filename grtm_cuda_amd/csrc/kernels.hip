@@ -347,6 +347,13 @@ hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr,
                             const void *C1, long ldc, int nr, void *traces, void *spec_post, hipStream_t st)
 {
     const size_t lds = post_ifft_lds_bytes(prec, pp.nw);
+    if (lds > 65536) {  // above the default dynamic-LDS limit of a launch
+        hipError_t e = prec == 0 ? hipFuncSetAttribute((const void *)k_post_ifft<double>,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)
+                                 : hipFuncSetAttribute((const void *)k_post_ifft<float>,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
     if (prec == 0)
         hipLaunchKernelGGL(k_post_ifft<double>, dim3(ntr), dim3(256), lds, st, pp, layout, (const double *)raw,
                            (const double *)C0, (const double *)C1, ldc, nr, (double *)traces,

@@ -1,0 +1,96 @@
+// opcount.hip -- exact operation count of one recursion cell.
+//
+// Instantiates the device cell solver on a counting scalar type on the host:
+// every add / sub / mul / div / sqrt executed by cell_solve + zr_from_cell
+// for a given layer stack and geometry is tallied, transcendental calls
+// (exp, sin/cos pair, log, atan2, hypot) separately.  This is the algorithmic
+// flop count the benchmark's roofline uses for the recursion kernel.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace grtm_count {
+struct Tally {
+    double flops = 0, trans = 0;
+};
+static thread_local Tally g_t;
+
+struct Cnt {
+    double v;
+    Cnt() : v(0) {}
+    Cnt(double x) : v(x) {}
+};
+inline Cnt operator+(Cnt a, Cnt b) { g_t.flops += 1; return Cnt(a.v + b.v); }
+inline Cnt operator-(Cnt a, Cnt b) { g_t.flops += 1; return Cnt(a.v - b.v); }
+inline Cnt operator*(Cnt a, Cnt b) { g_t.flops += 1; return Cnt(a.v * b.v); }
+inline Cnt operator/(Cnt a, Cnt b) { g_t.flops += 1; return Cnt(a.v / b.v); }
+inline Cnt operator-(Cnt a) { return Cnt(-a.v); }
+inline bool operator>=(Cnt a, Cnt b) { return a.v >= b.v; }
+inline bool operator==(Cnt a, Cnt b) { return a.v == b.v; }
+inline Cnt fabs(Cnt a) { return Cnt(std::fabs(a.v)); }
+inline Cnt copysign(Cnt a, Cnt b) { return Cnt(std::copysign(a.v, b.v)); }
+inline Cnt sqrt(Cnt a) { g_t.flops += 1; return Cnt(std::sqrt(a.v)); }
+inline Cnt hypot(Cnt a, Cnt b) { g_t.trans += 1; return Cnt(std::hypot(a.v, b.v)); }
+inline Cnt exp(Cnt a) { g_t.trans += 1; return Cnt(std::exp(a.v)); }
+inline Cnt log(Cnt a) { g_t.trans += 1; return Cnt(std::log(a.v)); }
+inline Cnt atan2(Cnt a, Cnt b) { g_t.trans += 1; return Cnt(std::atan2(a.v, b.v)); }
+inline void sincos_t(Cnt a, Cnt *s, Cnt *c)
+{
+    g_t.trans += 1;
+    s->v = std::sin(a.v);
+    c->v = std::cos(a.v);
+}
+}  // namespace grtm_count
+
+namespace grtm {
+using grtm_count::copysign;
+using grtm_count::exp;
+using grtm_count::fabs;
+using grtm_count::hypot;
+using grtm_count::sincos_t;
+using grtm_count::sqrt;
+}  // namespace grtm
+
+#include "cell.h"
+
+extern "C" int grtm_opcount_cell(int L, int s, int rl, int branch, int up, double *out)
+{
+    using grtm::cx;
+    using grtm::LayerW;
+    using grtm_count::Cnt;
+    if (L < 2 || L > 64 || s < 0 || s >= L - 1 || rl < 0 || rl >= L || !out) return -3;
+    LayerW<Cnt> lw[64];
+    const double w_re = 3.0, w_im = -0.05;
+    for (int j = 0; j < L; j++) {
+        const double a = 5.0 + 0.3 * j, b = 3.0 + 0.15 * j;
+        LayerW<Cnt> &q = lw[j];
+        q.sa = cx<Cnt>{Cnt(w_re / a), Cnt(w_im / a)};
+        q.sb = cx<Cnt>{Cnt(w_re / b), Cnt(w_im / b)};
+        q.ga = cx<Cnt>{Cnt((w_re * w_re - w_im * w_im) / (a * a)), Cnt(2 * w_re * w_im / (a * a))};
+        q.gb = cx<Cnt>{Cnt((w_re * w_re - w_im * w_im) / (b * b)), Cnt(2 * w_re * w_im / (b * b))};
+        q.ca = cx<Cnt>{Cnt(a / w_re), Cnt(0)};
+        q.cb = cx<Cnt>{Cnt(b / w_re), Cnt(0)};
+        q.mu = cx<Cnt>{Cnt(2.7 * b * b), Cnt(0)};
+        q.a = cx<Cnt>{Cnt(a), Cnt(0)};
+        q.b = cx<Cnt>{Cnt(b), Cnt(0)};
+        q.h = Cnt(2.0 + j);
+        q.pad = Cnt(0);
+    }
+    grtm::Geo g;
+    g.L = L;
+    g.s = s;
+    g.rl = rl;
+    g.branch = branch;
+    g.up = up;
+    g.pad = 0;
+    g.zs = 0.5 * (2.0 + s);
+    g.zr = (rl == s) ? (branch == 2 ? 0.25 * (2.0 + s) : 0.75 * (2.0 + s)) : 0.3;
+    grtm_count::g_t = grtm_count::Tally();
+    grtm::CellOut<Cnt> c;
+    grtm::cell_solve<Cnt, false, false>(lw, g, Cnt(0.7), cx<Cnt>{Cnt(0), Cnt(0)}, cx<Cnt>{Cnt(0), Cnt(0)}, c);
+    cx<Cnt> a, b, d, e;
+    grtm::zr_from_cell<Cnt, false>(c, up, Cnt(0.7), cx<Cnt>{Cnt(w_re), Cnt(w_im)}, a, b, d, e);
+    out[0] = grtm_count::g_t.flops;
+    out[1] = grtm_count::g_t.trans;
+    return 0;
+}

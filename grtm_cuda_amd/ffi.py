@@ -1,0 +1,88 @@
+"""ctypes binding of libgrtm.so (the HIP implementation).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C grtm_cuda_amd/csrc``).  Nothing here falls back to a CPU path: if the
+library is missing, loading raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgrtm.so")
+
+GRTM_OK = 0
+GRTM_E_IO = -1
+GRTM_E_PARSE = -2
+GRTM_E_INPUT = -3
+GRTM_E_UNSUPPORTED = -4
+GRTM_E_DEVICE = -5
+GRTM_E_NOMEM = -6
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+
+P = ctypes.c_void_p
+D = ctypes.c_double
+I = ctypes.c_int
+
+
+class GrtmError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        msg = _lib().grtm_status_str(status).decode() if _LIB is not None else str(status)
+        super().__init__(f"{what}: {msg} (status {status})" if what else f"{msg} (status {status})")
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nw", I), ("dt", D), ("nk", I), ("kmax", D), ("omega_i", D), ("t0", D), ("taper", D)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("n_layers", I), ("layers", P), ("src_depth", D), ("rcv_depth", D), ("nr", I), ("dists", P)]
+
+
+_LIB = None
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libgrtm.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.grtm_status_str.restype = ctypes.c_char_p
+        lib.grtm_version.restype = ctypes.c_char_p
+        lib.green.restype = None
+        lib.green.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
+        lib.grtm_green.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, I]
+        lib.grtm_green_mem.argtypes = [ctypes.c_char_p, I, P, P, P]
+        lib.grtm_create.argtypes = [P, I]
+        lib.grtm_destroy.argtypes = [P]
+        lib.grtm_destroy.restype = None
+        lib.grtm_plan_traces.argtypes = [P, P, P, I, I, P]
+        lib.grtm_plan_execute.argtypes = [P, P]
+        lib.grtm_plan_sync.argtypes = [P]
+        lib.grtm_plan_traces_out.argtypes = [P, P]
+        lib.grtm_plan_kernels_out.argtypes = [P, I, I, I, P]
+        lib.grtm_plan_timing.argtypes = [P, I, P]
+        lib.grtm_plan_work.argtypes = [P, P]
+        lib.grtm_plan_destroy.argtypes = [P]
+        lib.grtm_plan_destroy.restype = None
+        lib.grtm_selftest_cell.argtypes = [I, I, D, D, D, I, P, I, D, I, D, P, P]
+        lib.grtm_opcount_cell.argtypes = [I, I, I, I, I, P]
+        _LIB = lib
+    return _LIB
+
+
+# every symbol include/grtm.h declares
+EXPORTS = [
+    "green", "grtm_green", "grtm_green_mem", "grtm_last_status", "grtm_status_str", "grtm_version",
+    "grtm_create", "grtm_destroy", "grtm_plan_traces", "grtm_plan_execute", "grtm_plan_sync",
+    "grtm_plan_traces_out", "grtm_plan_kernels_out", "grtm_plan_timing", "grtm_plan_work",
+    "grtm_plan_destroy", "grtm_selftest_cell", "grtm_opcount_cell",
+]
+
+
+def check(status, what=""):
+    if status != GRTM_OK:
+        raise GrtmError(status, what)

@@ -100,7 +100,8 @@ int grtm_plan_kernels_out(grtm_plan *plan, int q, int w_lo, int w_hi, double *ke
  * ms[3] IFFT, ms[4] Bessel tables (one-off, part of planning). */
 int grtm_plan_timing(grtm_plan *plan, int reps, double *ms);
 /* Work of one execution: [0] cells, [1] cell-layers, [2] contraction flops,
- * [3] IFFT algorithmic bytes, [4] traces. */
+ * [3] IFFT algorithmic bytes, [4] traces, [5] recursion flops per cell,
+ * [6] transcendental calls per cell (7 doubles). */
 int grtm_plan_work(grtm_plan *plan, double *work);
 void grtm_plan_destroy(grtm_plan *plan);
 
@@ -112,6 +113,14 @@ void grtm_plan_destroy(grtm_plan *plan);
 int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi, int n_layer,
                        const double *lay, int s_layer, double zs, int r_layer, double zr,
                        const double *par0_rv, double *out);
+
+/* Exact operation count of one recursion cell of the trace path (the
+ * device template run on a counting scalar type on the host):
+ * out[0] = add/sub/mul/div/sqrt, out[1] = transcendental calls
+ * (exp, sincos, log, atan2, hypot).  branch/up as the geometry selects
+ * (1: receiver layer above the source layer, 2/3: same layer above/below,
+ * 4: below). */
+int grtm_opcount_cell(int n_layers, int s_layer, int r_layer, int branch, int up, double *out);
 
 #ifdef __cplusplus
 }

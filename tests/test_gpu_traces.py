@@ -1,0 +1,70 @@
+"""GPU parity of the multi-receiver trace path against the FP64 oracle."""
+import numpy as np
+import pytest
+
+import grtm_cuda_amd as g
+from grtm_cuda_amd import models
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return np.linalg.norm(a - b, axis=-1) / np.linalg.norm(b, axis=-1)
+
+
+@pytest.fixture(scope="module")
+def c1_oracle(orc):
+    cfg = models.config("c1")
+    out = {}
+    for compat in (False, True):
+        for kind in (0, 1):
+            tr, sp = orc.traces(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], kind, cfg["grid"],
+                                compat=compat)
+            out[(compat, kind)] = tr
+    return cfg, out
+
+
+@pytest.mark.parametrize("compat", [False, True])
+def test_c1_fp64_traces_match_oracle(c1_oracle, compat):
+    cfg, ref = c1_oracle
+    print(f"gpu: c1 fp64 compat={compat}", flush=True)
+    plan = g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
+                       precision=g.PREC_FP64, compat=compat)
+    tr = plan.traces()  # [q, c, r, N]
+    for q in (0, 1):
+        o = ref[(compat, q)]  # [r, c, N]
+        rl = rel_l2(tr[q].transpose(1, 0, 2), o)
+        assert rl.max() < 1e-9, (q, rl)
+
+
+def test_c1_kernels_match_oracle(c1_oracle, orc):
+    cfg, _ = c1_oracle
+    plan = g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
+                       precision=g.PREC_FP64)
+    plan.execute()
+    plan.sync()
+    for q in (0, 1):
+        kg = plan.kernels(q, 100, 104)
+        ko = orc.kernels(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], q, cfg["grid"], 100, 104)
+        assert np.abs(kg - ko).max() <= 1e-11 * np.abs(ko).max()
+
+
+def test_c1_fp32_traces_error(c1_oracle):
+    cfg, ref = c1_oracle
+    print("gpu: c1 fp32", flush=True)
+    plan = g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
+                       precision=g.PREC_FP32)
+    tr = plan.traces()
+    worst = 0.0
+    for q in (0, 1):
+        rl = rel_l2(tr[q].transpose(1, 0, 2), ref[(False, q)])
+        worst = max(worst, rl.max())
+    print("fp32 worst rel-L2", worst)
+    assert worst < 1e-2
+
+
+def test_plan_rejects_bad_grid():
+    cfg = models.config("c1")
+    bad = dict(cfg["grid"], nw=500)
+    with pytest.raises(g.GrtmError):
+        g.TracePlan(cfg["layers"], 10.0, 0.0, cfg["dists"], bad)
