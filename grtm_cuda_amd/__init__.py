@@ -115,10 +115,10 @@ class TracePlan:
         return dict(total=ms[0], cells=ms[1], ksum=ms[2], ifft=ms[3], bessel=ms[4])
 
     def work(self):
-        w = np.zeros(7)
+        w = np.zeros(9)
         check(_lib().grtm_plan_work(self.ptr, w.ctypes.data_as(P)), "grtm_plan_work")
         return dict(cells=w[0], cell_layers=w[1], ksum_flop=w[2], ifft_bytes=w[3], traces=w[4],
-                    flops_per_cell=w[5], trans_per_cell=w[6])
+                    flops_per_cell=w[5], trans_per_cell=w[6], rows_fp64=int(w[7]), cells_refixed=int(w[8]))
 
     def __del__(self):
         if getattr(self, "ptr", None) is not None and self.ptr.value:

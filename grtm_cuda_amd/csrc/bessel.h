@@ -113,16 +113,31 @@ GRTM_HD double as_ddjn(double x, int n)
     return (double)n * as_jn(x, n - 1) / x - as_jn(x, n) * (n + n * n + x * x) / x / x;
 }
 
+// J_m(x) for |x| below 1e-3 from the first two series terms
+// (x/2)^m / m! (1 - (x/2)^2 / (m+1)); exact at x = 0, where a forward
+// recurrence in m would divide by x.
+GRTM_HD double small_jn(int m, double x)
+{
+    const double h = 0.5 * x;
+    double t = 1.0;
+    for (int i = 1; i <= m; i++) t *= h / i;
+    return t * (1.0 - h * h / (m + 1));
+}
+
 // Accurate integer-order J_n from the C / device math library.
 GRTM_HD double lib_jn(int n, double x)
 {
     int m = n < 0 ? -n : n;
     double v;
+    if (fabs(x) < 1e-3 && m >= 2) {
+        v = small_jn(m, x);
+    } else {
 #ifdef __HIP_DEVICE_COMPILE__
-    v = (m == 0) ? ::j0(x) : (m == 1) ? ::j1(x) : ::jn(m, x);
+        v = (m == 0) ? ::j0(x) : (m == 1) ? ::j1(x) : ::jn(m, x);
 #else
-    v = ::jn(m, x);
+        v = ::jn(m, x);
 #endif
+    }
     return (n < 0 && (m & 1)) ? -v : v;
 }
 

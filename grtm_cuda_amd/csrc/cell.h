@@ -15,16 +15,25 @@
 
 namespace grtm {
 
+// Precision of the vertical wavenumbers and exponent arguments: FP64 for
+// both FP32 and FP64 cells.
+template <typename T>
+struct WideOf {
+    using type = double;
+};
+
 // Constants of one layer at one complex frequency w.
 template <typename T>
 struct LayerW {
+    using W = typename WideOf<T>::type;
+    cx<W> sad, sbd;  // w/alpha, w/beta in FP64 (vertical wavenumbers are formed in FP64)
     cx<T> sa, sb;  // w/alpha, w/beta (complex slownesses times w)
     cx<T> ga, gb;  // (w/alpha)^2, (w/beta)^2
     cx<T> ca, cb;  // alpha/w, beta/w (normalisation of the E columns)
     cx<T> mu;      // shear modulus
     cx<T> a, b;    // alpha, beta
-    T h;           // thickness
-    T pad;
+    W h;           // thickness
+    W pad;
 };
 
 // Geometry of one (model, source) pair.
@@ -41,18 +50,47 @@ struct Geo {
 // Layer quantities at wavenumber k.
 template <typename T>
 struct LQ {
+    using W = typename WideOf<T>::type;
+    cx<W> rd, vd;  // vertical wavenumbers in FP64 (exponent arguments)
     cx<T> r, v, x, gb, mu, sa, sb, ca, cb;
-    T h;
+    W h;
 };
+
+template <typename T, typename W>
+GRTM_HD cx<T> narrow(cx<W> z)
+{
+    return C<T>(T(z.x), T(z.y));
+}
+
+// e^{-nu z} with the argument formed in FP64.  For FP32 cells the phase is
+// reduced modulo 2 pi in FP64 first: Im(nu z) reaches hundreds of radians, and
+// rounding it to FP32 would cost ~1e-5 of relative accuracy per factor.
+template <typename T, typename W>
+GRTM_HD cx<T> cexp_nz(cx<W> nu, W z)
+{
+    const W ax = -(nu.x * z), ay = -(nu.y * z);
+    if constexpr (sizeof(T) == sizeof(W)) {
+        return narrow<T>(cexp(C<W>(ax, ay)));
+    } else {
+        const double tp = 6.283185307179586476925286766559;
+        const double ry = ay - tp * rint(ay * (1.0 / tp));
+        return cexp(C<T>(T(ax), T(ry)));
+    }
+}
 
 template <typename T>
 GRTM_HD LQ<T> layer_q(const LayerW<T> &L, T k)
 {
     LQ<T> q;
     // vertical wavenumbers sqrt((k - w/V)(k + w/V)) with Re >= 0
-    // (green.cu:184-194); the factored form avoids cancellation near k = w/V
-    q.r = csqrt((C<T>(k, 0) - L.sa) * (C<T>(k, 0) + L.sa));
-    q.v = csqrt((C<T>(k, 0) - L.sb) * (C<T>(k, 0) + L.sb));
+    // (green.cu:184-194), in FP64; the factored form avoids cancellation
+    // near k = w/V
+    using W = typename WideOf<T>::type;
+    const W kd = W(k);
+    q.rd = csqrt((C<W>(kd, W(0)) - L.sad) * (C<W>(kd, W(0)) + L.sad));
+    q.vd = csqrt((C<W>(kd, W(0)) - L.sbd) * (C<W>(kd, W(0)) + L.sbd));
+    q.r = narrow<T>(q.rd);
+    q.v = narrow<T>(q.vd);
     q.x = C<T>(T(2) * k * k, 0) - L.gb;  // k^2 + v^2
     q.gb = L.gb;
     q.mu = L.mu;
@@ -172,11 +210,11 @@ GRTM_HD void up_step(m2<T> &Rg, cx<T> &rg0, const RT<T> &t, cx<T> e0, cx<T> e1, 
     Rg = t.rdu + B * Td;
 }
 
-template <typename T>
-GRTM_HD void lam(const LQ<T> &q, T h, cx<T> &e0, cx<T> &e1)
+template <typename T, typename W>
+GRTM_HD void lam(const LQ<T> &q, W h, cx<T> &e0, cx<T> &e1)
 {
-    e0 = cexp(-(q.r * h));
-    e1 = cexp(-(q.v * h));
+    e0 = cexp_nz<T>(q.rd, h);
+    e1 = cexp_nz<T>(q.vd, h);
 }
 
 // Result of the receiver-side solve for one cell.
@@ -199,7 +237,8 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, T k, cx<T> par0_r, cx
                         CellOut<T> &out)
 {
     const int L = g.L, s = g.s, rl = g.rl;
-    const T zs = T(g.zs), zr = T(g.zr);
+    using W = typename WideOf<T>::type;
+    const W zs = W(g.zs), zr = W(g.zr);
     out.status = 0;
 
     // ---------------- upward sweep: half-space -> bottom of source layer
@@ -313,9 +352,9 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, T k, cx<T> par0_r, cx
 
     // ---------------- source layer reverberation (green.cu:654-702)
     const LQ<T> &sq = out.sq;
-    const T hs_ = lw[s].h;
-    cx<T> sexd0 = cexp(-(sq.r * (hs_ - zs))), sexd1 = cexp(-(sq.v * (hs_ - zs)));
-    cx<T> sexu0 = cexp(-(sq.r * zs)), sexu1 = cexp(-(sq.v * zs));
+    const W hs_ = lw[s].h;
+    cx<T> sexd0 = cexp_nz<T>(sq.rd, hs_ - zs), sexd1 = cexp_nz<T>(sq.vd, hs_ - zs);
+    cx<T> sexu0 = cexp_nz<T>(sq.rd, zs), sexu1 = cexp_nz<T>(sq.vd, zs);
     out.ERd = dmd(sexd0, sexd1, Rdu_s1, sexd0, sexd1);
     out.ERu = dmd(sexu0, sexu1, Rud_s, sexu0, sexu1);
     out.erd0 = sexd1 * rdu0_s1 * sexd1;
@@ -328,12 +367,12 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, T k, cx<T> par0_r, cx
     // up cols [[ca k, cb v], [-ca r, -cb k]]
     const m2<T> Ed{qr.ca * k, qr.cb * qr.v, qr.ca * qr.r, qr.cb * k};
     const m2<T> Eu{qr.ca * k, qr.cb * qr.v, -(qr.ca * qr.r), -(qr.cb * k)};
-    const T hr_ = lw[rl].h;
+    const W hr_ = lw[rl].h;
     const bool rl_half = (rl == L - 1);
 
     if (g.branch == 2) {  // green.cu:709-743
-        cx<T> rd0 = cexp(-(qr.r * zr)), rd1 = cexp(-(qr.v * zr));
-        cx<T> ru0 = cexp(-(qr.r * (zs - zr))), ru1 = cexp(-(qr.v * (zs - zr)));
+        cx<T> rd0 = cexp_nz<T>(qr.rd, zr), rd1 = cexp_nz<T>(qr.vd, zr);
+        cx<T> ru0 = cexp_nz<T>(qr.rd, zs - zr), ru1 = cexp_nz<T>(qr.vd, zs - zr);
         const m2<T> RS = dmd(rd0, rd1, Rud_s, sexu0, sexu1);
         const m2<T> u = Ed * RS + md(Eu, ru0, ru1);
         const cx<T> u0 = rd1 * rud0_s * sexu1 + ru1;
@@ -347,8 +386,8 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, T k, cx<T> par0_r, cx
             out.y0t = u0t * y_sh;
         }
     } else if (g.branch == 3) {  // green.cu:744-774
-        cx<T> rd0 = cexp(-(qr.r * (zr - zs))), rd1 = cexp(-(qr.v * (zr - zs)));
-        cx<T> ru0 = cexp(-(qr.r * (hr_ - zr))), ru1 = cexp(-(qr.v * (hr_ - zr)));
+        cx<T> rd0 = cexp_nz<T>(qr.rd, zr - zs), rd1 = cexp_nz<T>(qr.vd, zr - zs);
+        cx<T> ru0 = cexp_nz<T>(qr.rd, hr_ - zr), ru1 = cexp_nz<T>(qr.vd, hr_ - zr);
         const m2<T> RS = dmd(ru0, ru1, Rdu_s1, sexd0, sexd1);
         const m2<T> d = md(Ed, rd0, rd1) + Eu * RS;
         const cx<T> d0 = rd1 + ru1 * rdu0_s1 * sexd1;
@@ -362,8 +401,8 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, T k, cx<T> par0_r, cx
             out.y0t = d0t * y_sh;
         }
     } else if (g.branch == 1) {  // green.cu:775-857
-        cx<T> rd0 = cexp(-(qr.r * zr)), rd1 = cexp(-(qr.v * zr));
-        cx<T> ru0 = cexp(-(qr.r * (hr_ - zr))), ru1 = cexp(-(qr.v * (hr_ - zr)));
+        cx<T> rd0 = cexp_nz<T>(qr.rd, zr), rd1 = cexp_nz<T>(qr.vd, zr);
+        cx<T> ru0 = cexp_nz<T>(qr.rd, hr_ - zr), ru1 = cexp_nz<T>(qr.vd, hr_ - zr);
         cx<T> ex0, ex1;
         lam(qr, hr_, ex0, ex1);
         const m2<T> RS = dmd(rd0, rd1, Rud_r, ex0, ex1);
@@ -382,11 +421,11 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, T k, cx<T> par0_r, cx
             out.y0t = COMPAT ? u0t * out.y0 : u0t * chain0;
         }
     } else {  // branch 4, rl > s: green.cu:858-915
-        cx<T> rd0 = cexp(-(qr.r * zr)), rd1 = cexp(-(qr.v * zr));
+        cx<T> rd0 = cexp_nz<T>(qr.rd, zr), rd1 = cexp_nz<T>(qr.vd, zr);
         cx<T> ru0 = C<T>(0, 0), ru1 = C<T>(0, 0);
         if (!rl_half) {
-            ru0 = cexp(-(qr.r * (hr_ - zr)));
-            ru1 = cexp(-(qr.v * (hr_ - zr)));
+            ru0 = cexp_nz<T>(qr.rd, hr_ - zr);
+            ru1 = cexp_nz<T>(qr.vd, hr_ - zr);
         }
         cx<T> ex0, ex1;
         if (COMPAT) {

@@ -6,6 +6,7 @@
 // green.cu:108-162), and the outputs follow green.cu:2242-2258.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -188,6 +189,8 @@ void layer_w(const Stack &st, int compat, cd w, LayerW<double> *out)
         const cd sa = w / a, sb = w / b;
         auto C = [](cd z) { return grtm::cx<double>{z.real(), z.imag()}; };
         LayerW<double> &q = out[j];
+        q.sad = C(sa);
+        q.sbd = C(sb);
         q.sa = C(sa);
         q.sb = C(sb);
         q.ga = C(sa * sa);
@@ -467,7 +470,8 @@ struct grtm_plan {
     grtm::PostParams pp{};
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    DBuf lw, dists, A0, A1, B0, B1, C0, C1, tr;
+    int rows64 = 0;  // FP32 plans: frequency rows [0, rows64) are solved in FP64
+    DBuf lw, lw64, flags, dists, A0, A1, B0, B1, C0, C1, tr;
     double bessel_ms = 0;
     ~grtm_plan()
     {
@@ -481,8 +485,19 @@ size_t esize(int prec) { return prec == 0 ? sizeof(double) : sizeof(float); }
 
 int run_cells(grtm_plan *p, hipStream_t s)
 {
-    HIPCHK(grtm::launch_cells_zr(p->prec, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->nw,
-                                 p->A0.p, p->A1.p, p->nk, s));
+    if (p->prec == GRTM_PREC_FP64) {
+        HIPCHK(grtm::launch_cells_zr(0, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->nw,
+                                     p->A0.p, p->A1.p, p->nk, nullptr, nullptr, s));
+        return GRTM_OK;
+    }
+    unsigned *cnt = p->flags.as<unsigned>(), *list = cnt + 1;
+    HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
+    HIPCHK(grtm::launch_cells_zr(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->rows64,
+                                 p->A0.p, p->A1.p, p->nk, nullptr, nullptr, s));
+    HIPCHK(grtm::launch_cells_zr(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->rows64,
+                                 p->nw - p->rows64, p->A0.p, p->A1.p, p->nk, cnt, list, s));
+    HIPCHK(grtm::launch_cells_fix(p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->A0.p,
+                                  p->A1.p, p->nk, cnt, list, s));
     return GRTM_OK;
 }
 int run_gemm(grtm_plan *p, hipStream_t s)
@@ -582,12 +597,26 @@ int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, 
         if ((status = p->lw.alloc(lw.size() * sizeof(LayerW<double>)))) return status;
         HIPCHK(hipMemcpy(p->lw.p, lw.data(), lw.size() * sizeof(LayerW<double>), hipMemcpyHostToDevice));
     } else {
+        // Low frequencies are solved in FP64: as w -> 0 the P and S columns of
+        // the eigenvector basis become parallel and the recursion loses
+        // ~(k beta / |w|)^2 of relative accuracy, with the integrand significant
+        // up to k ~ |w| / beta + O(1 / dz).  Rows with |w| < 2.5 beta_min / dz.
+        double bmin = 1e30;
+        for (int j = 0; j < st.L; j++) bmin = std::min(bmin, st.b0[j]);
+        const double dz = std::max(std::fabs(job->src_depth - job->rcv_depth), 1e-3);
+        const double wc = 2.5 * bmin / dz;
+        int r64 = 0;
+        while (r64 < p->nw && std::hypot(2 * M_PI * p->df * r64, p->omega_i) < wc) r64++;
+        p->rows64 = std::min(p->nw, std::max(r64, 4));
+        if ((status = p->lw64.alloc(lw.size() * sizeof(LayerW<double>)))) return status;
+        HIPCHK(hipMemcpy(p->lw64.p, lw.data(), lw.size() * sizeof(LayerW<double>), hipMemcpyHostToDevice));
+        if ((status = p->flags.alloc(sizeof(unsigned) * ((size_t)p->nw * p->nk + 1)))) return status;
         std::vector<LayerW<float>> lf(lw.size());
         auto F = [](grtm::cx<double> z) { return grtm::cx<float>{float(z.x), float(z.y)}; };
         for (size_t i = 0; i < lw.size(); i++) {
             const LayerW<double> &d = lw[i];
-            lf[i] = LayerW<float>{F(d.sa), F(d.sb), F(d.ga), F(d.gb), F(d.ca), F(d.cb), F(d.mu), F(d.a), F(d.b),
-                                  float(d.h), 0.f};
+            lf[i] = LayerW<float>{d.sad, d.sbd, F(d.sa), F(d.sb), F(d.ga), F(d.gb), F(d.ca), F(d.cb), F(d.mu),
+                                  F(d.a), F(d.b), d.h, 0.0};
         }
         if ((status = p->lw.alloc(lf.size() * sizeof(LayerW<float>)))) return status;
         HIPCHK(hipMemcpy(p->lw.p, lf.data(), lf.size() * sizeof(LayerW<float>), hipMemcpyHostToDevice));
@@ -722,6 +751,14 @@ int grtm_plan_work(grtm_plan *p, double *w)
     if (grtm_opcount_cell(p->L, p->g.s, p->g.rl, p->g.branch, p->g.up, oc)) return GRTM_E_INPUT;
     w[5] = oc[0];  // flops per cell
     w[6] = oc[1];  // transcendental calls per cell
+    w[7] = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;  // rows solved in FP64
+    w[8] = 0;  // cells re-solved in FP64 after a non-finite FP32 result (last execution)
+    if (p->prec != GRTM_PREC_FP64) {
+        unsigned c = 0;
+        HIPCHK(hipStreamSynchronize(p->stream));
+        HIPCHK(hipMemcpy(&c, p->flags.p, sizeof(unsigned), hipMemcpyDeviceToHost));
+        w[8] = c;
+    }
     return GRTM_OK;
 }
 
@@ -738,6 +775,8 @@ int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi,
         const double *l = lay + 7 * j;
         const cd a(l[1], l[2]), b(l[3], l[4]), mu(l[5], l[6]);
         auto C = [](cd z) { return grtm::cx<double>{z.real(), z.imag()}; };
+        lw[j].sad = C(w / a);
+        lw[j].sbd = C(w / b);
         lw[j].sa = C(w / a);
         lw[j].sb = C(w / b);
         lw[j].ga = C((w / a) * (w / a));

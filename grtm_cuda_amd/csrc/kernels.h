@@ -19,10 +19,15 @@ struct PostParams {
     double omega_i;  // imaginary frequency
 };
 
-// prec: 0 = FP64, 1 = FP32
+// prec: 0 = FP64 cells / FP64 operands, 1 = FP32 cells / FP32 operands
+// (non-finite cells appended to flag_list), 2 = FP64 cells / FP32 operands
 hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
                            double df, double omega_i, int w0, int nwr, void *A0, void *A1, long lda,
-                           hipStream_t st);
+                           unsigned *flag_count, unsigned *flag_list, hipStream_t st);
+hipError_t launch_cells_fix(int compat, const void *lw64, int L, const Geo &g, int nk, double dk, double df,
+                            double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,
+                            const unsigned *flag_list, hipStream_t st);
+hipError_t launch_zero_u32(unsigned *p, int n, hipStream_t st);
 hipError_t launch_cells_full(int compat, int src, const void *lw, const int *nk_w, int L, const Geo &g,
                              int nw, int nblk, double dk, double dist, double df, double omega_i,
                              const double p0[4], double *partial, hipStream_t st);

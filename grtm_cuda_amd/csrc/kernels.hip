@@ -21,19 +21,13 @@ namespace grtm {
 constexpr int kCellThreads = 256;
 
 // ---------------------------------------------------------------- recursion
-template <typename T, bool COMPAT>
-__global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__restrict__ lw_all, int L,
-                                                           Geo g, int nk, double dk, double df,
-                                                           double omega_i, int w0, T *__restrict__ A0,
-                                                           T *__restrict__ A1, long lda)
+// One thread per (omega, k) cell; T = arithmetic of the recursion, TO = type
+// of the A operands.  FP32 cells whose result is not finite are appended to
+// a list (flag_count / flag_list) and re-solved in FP64 by k_cells_fix.
+template <typename T, typename TO, bool COMPAT>
+__device__ __forceinline__ bool cell_zr(const LayerW<T> *lw, const Geo &g, int iwg, int ik, double dk, double df,
+                                        double omega_i, TO v[8])
 {
-    __shared__ LayerW<T> lw[kMaxLayers];
-    const int iw = blockIdx.y;  // row within this launch
-    const int iwg = w0 + iw;    // global frequency index
-    for (int j = threadIdx.x; j < L; j += blockDim.x) lw[j] = lw_all[(size_t)iw * L + j];
-    __syncthreads();
-    const int ik = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ik >= nk) return;
     const T k = T(dk * ik);
     const cx<T> w = C<T>(T(2.0 * M_PI * df * iwg), T(-omega_i));
     CellOut<T> c;
@@ -41,15 +35,70 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__re
     cx<T> kze, kre, kzv, krv;
     zr_from_cell<T, COMPAT>(c, g.up, k, w, kze, kre, kzv, krv);
     const T wt = T(ik == 0 ? 0.5 * dk : dk);
+    const T r[8] = {wt * kze.x, wt * kze.y, wt * kzv.x, wt * kzv.y, wt * kre.x, wt * kre.y, wt * krv.x, wt * krv.y};
+    bool ok = true;
+    for (int i = 0; i < 8; i++) {
+        v[i] = TO(r[i]);
+        ok = ok && isfinite(float(v[i]));
+    }
+    return ok;
+}
+
+template <typename TO>
+__device__ __forceinline__ void store_zr(const TO v[8], int iw, int ik, TO *A0, TO *A1, long lda)
+{
     const size_t r0 = (size_t)(4 * iw) * lda + ik;
-    A0[r0] = wt * kze.x;
-    A0[r0 + lda] = wt * kze.y;
-    A0[r0 + 2 * lda] = wt * kzv.x;
-    A0[r0 + 3 * lda] = wt * kzv.y;
-    A1[r0] = wt * kre.x;
-    A1[r0 + lda] = wt * kre.y;
-    A1[r0 + 2 * lda] = wt * krv.x;
-    A1[r0 + 3 * lda] = wt * krv.y;
+    for (int i = 0; i < 4; i++) {
+        A0[r0 + i * lda] = v[i];
+        A1[r0 + i * lda] = v[4 + i];
+    }
+}
+
+template <typename T, typename TO, bool COMPAT>
+__global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__restrict__ lw_all, int L,
+                                                           Geo g, int nk, double dk, double df,
+                                                           double omega_i, int w0, TO *__restrict__ A0,
+                                                           TO *__restrict__ A1, long lda,
+                                                           unsigned *__restrict__ flag_count,
+                                                           unsigned *__restrict__ flag_list)
+{
+    __shared__ LayerW<T> lw[kMaxLayers];
+    const int iw = w0 + blockIdx.y;  // frequency row
+    for (int j = threadIdx.x; j < L; j += blockDim.x) lw[j] = lw_all[(size_t)iw * L + j];
+    __syncthreads();
+    const int ik = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ik >= nk) return;
+    TO v[8];
+    const bool ok = cell_zr<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
+    store_zr(v, iw, ik, A0, A1, lda);
+    if (!ok && flag_count) {
+        const unsigned slot = atomicAdd(flag_count, 1u);
+        flag_list[slot] = unsigned(iw) * unsigned(nk) + unsigned(ik);
+    }
+}
+
+// FP64 re-solve of the flagged cells (grid-stride over the list).
+template <typename TO, bool COMPAT>
+__global__ __launch_bounds__(kCellThreads) void k_cells_fix(const LayerW<double> *__restrict__ lw_all, int L,
+                                                            Geo g, int nk, double dk, double df, double omega_i,
+                                                            TO *__restrict__ A0, TO *__restrict__ A1, long lda,
+                                                            const unsigned *__restrict__ flag_count,
+                                                            const unsigned *__restrict__ flag_list)
+{
+    const unsigned n = *flag_count;
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const unsigned cell = flag_list[t];
+        const int iw = int(cell / unsigned(nk)), ik = int(cell % unsigned(nk));
+        TO v[8];
+        cell_zr<double, TO, COMPAT>(lw_all + (size_t)iw * L, g, iw, ik, dk, df, omega_i, v);
+        store_zr(v, iw, ik, A0, A1, lda);
+    }
+}
+
+__global__ void k_zero_u32(unsigned *p, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0u;
 }
 
 template <typename T>
@@ -264,27 +313,52 @@ __global__ __launch_bounds__(256) void k_post_ifft(PostParams pp, int layout, co
 }
 
 // ---------------------------------------------------------------- launchers
-template <typename T, bool COMPAT>
-static hipError_t launch_cells_zr_t(const void *lw, int L, const Geo &g, int nk, double dk, double df,
-                                    double omega_i, int w0, int nwr, void *A0, void *A1, long lda,
-                                    hipStream_t st)
+// Recursion over frequency rows [w0, w0 + nwr).  prec: 0 FP64 cells with FP64
+// A operands, 1 FP32 cells with FP32 A operands (flagged cells appended to the
+// list), 2 FP64 cells with FP32 A operands.  lw points at row 0.
+hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
+                           double df, double omega_i, int w0, int nwr, void *A0, void *A1, long lda,
+                           unsigned *flag_count, unsigned *flag_list, hipStream_t st)
 {
+    if (nwr <= 0) return hipSuccess;
     dim3 grid((nk + kCellThreads - 1) / kCellThreads, nwr);
-    hipLaunchKernelGGL((k_cells_zr<T, COMPAT>), grid, dim3(kCellThreads), 0, st,
-                       (const LayerW<T> *)lw, L, g, nk, dk, df, omega_i, w0, (T *)A0, (T *)A1, lda);
+#define GRTM_ZR(T, TO, CP, FC, FL)                                                                     \
+    hipLaunchKernelGGL((k_cells_zr<T, TO, CP>), grid, dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, L, g, \
+                       nk, dk, df, omega_i, w0, (TO *)A0, (TO *)A1, lda, FC, FL)
+    if (prec == 0) {
+        if (compat) GRTM_ZR(double, double, true, nullptr, nullptr);
+        else GRTM_ZR(double, double, false, nullptr, nullptr);
+    } else if (prec == 1) {
+        if (compat) GRTM_ZR(float, float, true, flag_count, flag_list);
+        else GRTM_ZR(float, float, false, flag_count, flag_list);
+    } else {
+        if (compat) GRTM_ZR(double, float, true, nullptr, nullptr);
+        else GRTM_ZR(double, float, false, nullptr, nullptr);
+    }
+#undef GRTM_ZR
     return hipGetLastError();
 }
 
-hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
-                           double df, double omega_i, int w0, int nwr, void *A0, void *A1, long lda,
-                           hipStream_t st)
+hipError_t launch_cells_fix(int compat, const void *lw64, int L, const Geo &g, int nk, double dk, double df,
+                            double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,
+                            const unsigned *flag_list, hipStream_t st)
 {
-    if (prec == 0) {
-        return compat ? launch_cells_zr_t<double, true>(lw, L, g, nk, dk, df, omega_i, w0, nwr, A0, A1, lda, st)
-                      : launch_cells_zr_t<double, false>(lw, L, g, nk, dk, df, omega_i, w0, nwr, A0, A1, lda, st);
-    }
-    return compat ? launch_cells_zr_t<float, true>(lw, L, g, nk, dk, df, omega_i, w0, nwr, A0, A1, lda, st)
-                  : launch_cells_zr_t<float, false>(lw, L, g, nk, dk, df, omega_i, w0, nwr, A0, A1, lda, st);
+    dim3 grid(1024);
+    if (compat)
+        hipLaunchKernelGGL((k_cells_fix<float, true>), grid, dim3(kCellThreads), 0, st,
+                           (const LayerW<double> *)lw64, L, g, nk, dk, df, omega_i, (float *)A0, (float *)A1, lda,
+                           flag_count, flag_list);
+    else
+        hipLaunchKernelGGL((k_cells_fix<float, false>), grid, dim3(kCellThreads), 0, st,
+                           (const LayerW<double> *)lw64, L, g, nk, dk, df, omega_i, (float *)A0, (float *)A1, lda,
+                           flag_count, flag_list);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(unsigned *p, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_cells_full(int compat, int src, const void *lw, const int *nk_w, int L, const Geo &g,

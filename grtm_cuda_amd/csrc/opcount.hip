@@ -53,6 +53,12 @@ using grtm_count::sqrt;
 
 #include "cell.h"
 
+// count the FP64 vertical-wavenumber work as well
+template <>
+struct grtm::WideOf<grtm_count::Cnt> {
+    using type = grtm_count::Cnt;
+};
+
 extern "C" int grtm_opcount_cell(int L, int s, int rl, int branch, int up, double *out)
 {
     using grtm::cx;
@@ -64,6 +70,8 @@ extern "C" int grtm_opcount_cell(int L, int s, int rl, int branch, int up, doubl
     for (int j = 0; j < L; j++) {
         const double a = 5.0 + 0.3 * j, b = 3.0 + 0.15 * j;
         LayerW<Cnt> &q = lw[j];
+        q.sad = cx<Cnt>{Cnt(w_re / a), Cnt(w_im / a)};
+        q.sbd = cx<Cnt>{Cnt(w_re / b), Cnt(w_im / b)};
         q.sa = cx<Cnt>{Cnt(w_re / a), Cnt(w_im / a)};
         q.sb = cx<Cnt>{Cnt(w_re / b), Cnt(w_im / b)};
         q.ga = cx<Cnt>{Cnt((w_re * w_re - w_im * w_im) / (a * a)), Cnt(2 * w_re * w_im / (a * a))};

@@ -22,7 +22,9 @@ extern "C" {
 #define GRTM_E_NOMEM -6       /* device allocation failed */
 
 #define GRTM_PREC_FP64 0 /* FP64 recursion, FP64 contraction */
-#define GRTM_PREC_FP32 1 /* FP32 recursion, FP32 contraction */
+#define GRTM_PREC_FP32 1 /* mixed: FP32 recursion with FP64 vertical wavenumbers,
+                            FP64 low-frequency rows and FP64 re-solve of
+                            non-finite cells; FP32 contraction */
 
 /* ---------------------------------------------------------- drop-in entry */
 /* Replaces the reference's only exported symbol:
@@ -101,7 +103,8 @@ int grtm_plan_kernels_out(grtm_plan *plan, int q, int w_lo, int w_hi, double *ke
 int grtm_plan_timing(grtm_plan *plan, int reps, double *ms);
 /* Work of one execution: [0] cells, [1] cell-layers, [2] contraction flops,
  * [3] IFFT algorithmic bytes, [4] traces, [5] recursion flops per cell,
- * [6] transcendental calls per cell (7 doubles). */
+ * [6] transcendental calls per cell, [7] frequency rows solved in FP64,
+ * [8] FP32 cells re-solved in FP64 in the last execution (9 doubles). */
 int grtm_plan_work(grtm_plan *plan, double *work);
 void grtm_plan_destroy(grtm_plan *plan);
 

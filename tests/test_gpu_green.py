@@ -92,7 +92,18 @@ def test_branches_match_oracle(tmp_path, orc, compat, src_type, geom):
     spec, tr = g.green_mem(p, compat=compat)
     rows = np.arange(2, spec.shape[1], 5)
     o = orc.green_rows(p, rows, compat=compat)
+    # FP64 noise floor of the formulation: the oracle's two solves of the
+    # interface systems (direct LU vs inverse-then-product)
+    try:
+        orc.set_inverse_mode(1)
+        o2 = orc.green_rows(p, rows, compat=compat)
+    finally:
+        orc.set_inverse_mode(0)
     s = spec[:, rows].T
     live = np.abs(o).max(axis=0) > 0
     rel = np.linalg.norm(s[:, live] - o[:, live]) / np.linalg.norm(o[:, live])
-    assert rel < 1e-9
+    noise = np.linalg.norm(o2[:, live] - o[:, live]) / np.linalg.norm(o[:, live])
+    # compat "r_below" puts the receiver 20 km below the bottom of its layer
+    # (the reference's receiver search keys on the source depth): e^{+20 nu}
+    # growth makes that output rounding noise in any FP64 implementation
+    assert rel < max(1e-9, 5 * noise), (rel, noise)
