@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def cpu_baseline(cfg, rows, threads):
@@ -51,7 +52,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
     ap.add_argument("--cpu-rows", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -62,7 +63,7 @@ def main():
     from grtm_cuda_amd import models
 
     cfg = models.config(args.workload)
-    prec = g.PREC_FP64 if args.precision == "fp64" else g.PREC_FP32
+    prec = {"fp64": g.PREC_FP64, "fp32": g.PREC_FP32, "bf16x3": g.PREC_BF16X3}[args.precision]
     print(f"bench: planning {args.workload} {args.precision}", file=sys.stderr, flush=True)
     plan = g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"], precision=prec)
     work = plan.work()
@@ -92,8 +93,9 @@ def main():
                     transcendentals_per_cell=work["trans_per_cell"])
     elif dom == "ksum":
         achieved = work["ksum_flop"] / (tm["ksum"] * 1e-3) / 1e12
-        roof = dict(bound="mfma" if prec != g.PREC_FP64 else "valu", achieved=achieved, peak=peak, unit="TFLOP/s",
-                    frac=achieved / peak, traffic=None, kernel="k_gemm")
+        kpeak = {g.PREC_FP64: FP64_PEAK_TFLOPS, g.PREC_FP32: FP32_PEAK_TFLOPS, g.PREC_BF16X3: BF16_PEAK_TFLOPS / 3}[prec]
+        roof = dict(bound="mfma", achieved=achieved, peak=kpeak, unit="TFLOP/s", frac=achieved / kpeak,
+                    traffic=None, kernel="k_gemm_mfma")
     else:
         achieved = work["ifft_bytes"] / (tm["ifft"] * 1e-3) / 1e9
         roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
@@ -116,7 +118,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == g.PREC_FP64 else "f32",
+        "dtype": {g.PREC_FP64: "f64", g.PREC_FP32: "f32", g.PREC_BF16X3: "f32+bf16x3"}[prec],
         "data": "synthetic (seeded layered model)",
         "config": {"workload": f"{args.workload}: plain dense-k, {len(cfg['layers']) - 1} layers + half-space, "
                                f"explosion + vertical force, {len(cfg['dists'])} receivers, "

@@ -16,10 +16,11 @@ import ctypes
 import numpy as np
 
 from . import inputfile, models  # noqa: F401
-from .ffi import (GRTM_OK, PREC_FP32, PREC_FP64, EXPORTS, GrtmError, Grid, Job, D, P,  # noqa: F401
+from .ffi import (GRTM_OK, PREC_BF16X3, PREC_FP32, PREC_FP64, EXPORTS, GrtmError, Grid, Job, D, P,  # noqa: F401
                    _lib, check)
 
-__all__ = ["green", "green_mem", "TracePlan", "GrtmError", "PREC_FP64", "PREC_FP32", "models", "inputfile"]
+__all__ = ["green", "green_mem", "TracePlan", "GrtmError", "PREC_FP64", "PREC_FP32", "PREC_BF16X3", "models",
+           "inputfile"]
 
 
 def green(iname, oname_w, oname_t, compat=True):
@@ -136,3 +137,16 @@ def selftest_cell(compat, src_type, k, w, lay, s_layer, zs, r_layer, zr, par0_rv
                                     float(zr), None if p0 is None else p0.ctypes.data_as(P),
                                     out.ctypes.data_as(P)), "grtm_selftest_cell")
     return out[0::2] + 1j * out[1::2]
+
+
+def selftest_gemm(mode, A, B):
+    """C = A B on the GPU with a contraction kernel (test hook): mode -1 VALU
+    FP64, 0 FP64 MFMA, 1 FP32 MFMA, 2 BF16x3 MFMA."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    M, K = A.shape
+    N = B.shape[1]
+    C = np.zeros((M, N))
+    check(_lib().grtm_selftest_gemm(int(mode), M, N, K, A.ctypes.data_as(P), B.ctypes.data_as(P),
+                                    C.ctypes.data_as(P)), "grtm_selftest_gemm")
+    return C

@@ -1,0 +1,272 @@
+// gemm.hip -- the k-sum contraction G = A B on the matrix cores (gfx950).
+//
+// A [M x K] row-major: rows (4 iw + {Re explosion, Im explosion, Re force,
+// Im force}) of the weighted k-kernels; B [K x N] row-major: J_n(k_i r_j);
+// C [M x N].  Split-K over grid.z: every slice writes its partial product to
+// P[z][M][N] and k_gemm_reduce sums the slices in FP64 in a fixed order.
+// kend (optional, one entry per BM-row tile) limits the K range of a tile:
+// columns at and beyond it are zero in A (wavenumbers past the truncation).
+//
+//   k_gemm_f64    v_mfma_f64_16x16x4f64, 64x64 block tile, 4 waves of 32x32
+//   k_gemm_f32    v_mfma_f32_32x32x2f32, 128x128 block tile, 4 waves of 64x64
+//   k_gemm_bf16x3 v_mfma_f32_32x32x16_bf16 on split operands
+//                 a = a_hi + a_lo, b = b_hi + b_lo (bf16 each):
+//                 a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi, FP32 accumulation
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace grtm {
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// ------------------------------------------------------------------ FP64
+__global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, int kslice, const double *__restrict__ A,
+                                                  long lda, const double *__restrict__ B, long ldb,
+                                                  double *__restrict__ P, const int *__restrict__ kend)
+{
+    constexpr int BM = 64, BN = 64, BK = 16;
+    __shared__ double As[BK][BM + 1];
+    __shared__ double Bs[BK][BN + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv >> 1, wn = wv & 1;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int kt_end = min(K, kend ? kend[blockIdx.y] : K);
+    const int kb = blockIdx.z * kslice, ke = min(kt_end, kb + kslice);
+    v4d acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = v4d{0, 0, 0, 0};
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+        for (int e = tid; e < BM * BK; e += 256) {
+            const int mm = e / BK, kk = e % BK, gm = m0 + mm, gk = k0 + kk;
+            As[kk][mm] = (gm < M && gk < ke) ? A[(size_t)gm * lda + gk] : 0.0;
+        }
+        for (int e = tid; e < BN * BK; e += 256) {
+            const int kk = e / BN, nn = e % BN, gk = k0 + kk, gn = n0 + nn;
+            Bs[kk][nn] = (gk < ke && gn < N) ? B[(size_t)gk * ldb + gn] : 0.0;
+        }
+        __syncthreads();
+        const int i = lane & 15, kq = lane >> 4;
+        for (int kk = 0; kk < BK; kk += 4) {
+            double a[2], b[2];
+            for (int t = 0; t < 2; t++) a[t] = As[kk + kq][wm * 32 + t * 16 + i];
+            for (int t = 0; t < 2; t++) b[t] = Bs[kk + kq][wn * 32 + t * 16 + i];
+            for (int tm = 0; tm < 2; tm++)
+                for (int tn = 0; tn < 2; tn++)
+                    acc[tm][tn] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[tm], b[tn], acc[tm][tn], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    double *Pz = P + (size_t)blockIdx.z * M * N;
+    for (int tm = 0; tm < 2; tm++)
+        for (int tn = 0; tn < 2; tn++)
+            for (int r = 0; r < 4; r++) {
+                // accumulator r of lane l holds row (l / 16) + 4 r, column l % 16
+                const int row = m0 + wm * 32 + tm * 16 + (lane >> 4) + 4 * r;
+                const int col = n0 + wn * 32 + tn * 16 + (lane & 15);
+                if (row < M && col < N) Pz[(size_t)row * N + col] = acc[tm][tn][r];
+            }
+}
+
+// ------------------------------------------------------------------ FP32
+__global__ __launch_bounds__(256) void k_gemm_f32(int M, int N, int K, int kslice, const float *__restrict__ A,
+                                                  long lda, const float *__restrict__ B, long ldb,
+                                                  float *__restrict__ P, const int *__restrict__ kend)
+{
+    constexpr int BM = 128, BN = 128, BK = 16;
+    __shared__ float As[BK][BM + 4];
+    __shared__ float Bs[BK][BN + 4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv >> 1, wn = wv & 1;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int kt_end = min(K, kend ? kend[blockIdx.y] : K);
+    const int kb = blockIdx.z * kslice, ke = min(kt_end, kb + kslice);
+    v16f acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int r = 0; r < 16; r++) acc[a][b][r] = 0.f;
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+        {  // A tile: 128 rows x 16 k, 8 per thread
+            const int mm = tid >> 1, kk0 = (tid & 1) * 8, gm = m0 + mm;
+            for (int q = 0; q < 8; q++) {
+                const int gk = k0 + kk0 + q;
+                As[kk0 + q][mm] = (gm < M && gk < ke) ? A[(size_t)gm * lda + gk] : 0.f;
+            }
+        }
+        {  // B tile: 16 k x 128 cols, 8 per thread
+            const int kk = tid >> 4, nn0 = (tid & 15) * 8, gk = k0 + kk;
+            for (int q = 0; q < 8; q++) {
+                const int gn = n0 + nn0 + q;
+                Bs[kk][nn0 + q] = (gk < ke && gn < N) ? B[(size_t)gk * ldb + gn] : 0.f;
+            }
+        }
+        __syncthreads();
+        const int i = lane & 31, kq = lane >> 5;
+        for (int kk = 0; kk < BK; kk += 2) {
+            float a[2], b[2];
+            for (int t = 0; t < 2; t++) a[t] = As[kk + kq][wm * 64 + t * 32 + i];
+            for (int t = 0; t < 2; t++) b[t] = Bs[kk + kq][wn * 64 + t * 32 + i];
+            for (int tm = 0; tm < 2; tm++)
+                for (int tn = 0; tn < 2; tn++)
+                    acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[tm], b[tn], acc[tm][tn], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    float *Pz = P + (size_t)blockIdx.z * M * N;
+    for (int tm = 0; tm < 2; tm++)
+        for (int tn = 0; tn < 2; tn++)
+            for (int r = 0; r < 16; r++) {
+                const int row = m0 + wm * 64 + tm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int col = n0 + wn * 64 + tn * 32 + (lane & 31);
+                if (row < M && col < N) Pz[(size_t)row * N + col] = acc[tm][tn][r];
+            }
+}
+
+// ------------------------------------------------------------------ BF16x3
+__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo)
+{
+    hi = __bf16(x);
+    lo = __bf16(x - float(hi));
+}
+
+__global__ __launch_bounds__(256) void k_gemm_bf16x3(int M, int N, int K, int kslice, const float *__restrict__ A,
+                                                     long lda, const float *__restrict__ B, long ldb,
+                                                     float *__restrict__ P, const int *__restrict__ kend)
+{
+    constexpr int BM = 128, BN = 128, BK = 32, PADK = BK + 8;
+    // k-contiguous rows so one lane's 8 consecutive k are one 16-byte read
+    __shared__ __attribute__((aligned(16))) __bf16 Ah[BM][PADK], Al[BM][PADK], Bh[BN][PADK], Bl[BN][PADK];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv >> 1, wn = wv & 1;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int kt_end = min(K, kend ? kend[blockIdx.y] : K);
+    const int kb = blockIdx.z * kslice, ke = min(kt_end, kb + kslice);
+    v16f acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int r = 0; r < 16; r++) acc[a][b][r] = 0.f;
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+        {  // A tile 128 x 32: 16 per thread
+            const int mm = tid >> 1, kk0 = (tid & 1) * 16, gm = m0 + mm;
+            for (int q = 0; q < 16; q++) {
+                const int gk = k0 + kk0 + q;
+                const float x = (gm < M && gk < ke) ? A[(size_t)gm * lda + gk] : 0.f;
+                split_bf16(x, Ah[mm][kk0 + q], Al[mm][kk0 + q]);
+            }
+        }
+        {  // B tile 32 x 128 -> stored [n][k]: 16 per thread
+            const int kk = tid >> 3, nn0 = (tid & 7) * 16, gk = k0 + kk;
+            for (int q = 0; q < 16; q++) {
+                const int gn = n0 + nn0 + q;
+                const float x = (gk < ke && gn < N) ? B[(size_t)gk * ldb + gn] : 0.f;
+                split_bf16(x, Bh[nn0 + q][kk], Bl[nn0 + q][kk]);
+            }
+        }
+        __syncthreads();
+        const int i = lane & 31, kq = (lane >> 5) * 8;
+        for (int kk = 0; kk < BK; kk += 16) {
+            v8bf ah[2], al[2], bh[2], bl[2];
+            for (int t = 0; t < 2; t++) {
+                const int r = wm * 64 + t * 32 + i, c = wn * 64 + t * 32 + i;
+                ah[t] = *reinterpret_cast<const v8bf *>(&Ah[r][kk + kq]);
+                al[t] = *reinterpret_cast<const v8bf *>(&Al[r][kk + kq]);
+                bh[t] = *reinterpret_cast<const v8bf *>(&Bh[c][kk + kq]);
+                bl[t] = *reinterpret_cast<const v8bf *>(&Bl[c][kk + kq]);
+            }
+            for (int tm = 0; tm < 2; tm++)
+                for (int tn = 0; tn < 2; tn++) {
+                    acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[tm], bh[tn], acc[tm][tn], 0, 0, 0);
+                    acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[tm], bl[tn], acc[tm][tn], 0, 0, 0);
+                    acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[tm], bh[tn], acc[tm][tn], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    float *Pz = P + (size_t)blockIdx.z * M * N;
+    for (int tm = 0; tm < 2; tm++)
+        for (int tn = 0; tn < 2; tn++)
+            for (int r = 0; r < 16; r++) {
+                const int row = m0 + wm * 64 + tm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int col = n0 + wn * 64 + tn * 32 + (lane & 31);
+                if (row < M && col < N) Pz[(size_t)row * N + col] = acc[tm][tn][r];
+            }
+}
+
+// ------------------------------------------------------------------ layout probe
+// One wave: lane l0 holds a = 1 (all other lanes 0), lane l holds b = 1000 + l;
+// out[l*4 + r] = accumulator r of lane l after one v_mfma_f64_16x16x4f64.
+__global__ void k_probe_f64(int l0, double *out)
+{
+    const int l = threadIdx.x;
+    const double a = (l == l0) ? 1.0 : 0.0, b = 1000.0 + l;
+    v4d acc = v4d{0, 0, 0, 0};
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    for (int r = 0; r < 4; r++) out[l * 4 + r] = acc[r];
+}
+
+hipError_t launch_probe_f64(int l0, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_probe_f64, dim3(1), dim3(64), 0, st, l0, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ split-K sum
+template <typename T>
+__global__ void k_gemm_reduce(int MN, int S, const T *__restrict__ P, T *__restrict__ Cm)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= MN) return;
+    double s = 0;
+    for (int z = 0; z < S; z++) s += double(P[(size_t)z * MN + t]);
+    Cm[t] = T(s);
+}
+
+// Split-K launch plan: slice length a multiple of the K step, enough slices
+// to give every compute unit a few blocks.
+static void split_plan(int M, int N, int K, int BM, int BN, int BK, int &S, int &kslice)
+{
+    const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+    S = 1;
+    while (tiles * S < 1024 && S < 64 && (K / (S * 2)) >= 4 * BK) S *= 2;
+    kslice = ((K + S - 1) / S + BK - 1) / BK * BK;
+}
+
+size_t gemm_workspace_bytes(int mode, int M, int N, int K)
+{
+    int S, ks;
+    if (mode == 0) split_plan(M, N, K, 64, 64, 16, S, ks);
+    else split_plan(M, N, K, 128, 128, mode == 1 ? 16 : 32, S, ks);
+    return (size_t)S * M * N * (mode == 0 ? 8 : 4);
+}
+
+// mode 0: FP64 MFMA, 1: FP32 MFMA, 2: BF16x3 MFMA.  P: workspace of
+// gemm_workspace_bytes(mode, M, N, K).
+hipError_t launch_gemm_mfma(int mode, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
+                            void *Cm, void *P, const int *kend, hipStream_t st)
+{
+    int S, ks;
+    if (mode == 0) {
+        split_plan(M, N, K, 64, 64, 16, S, ks);
+        dim3 grid((N + 63) / 64, (M + 63) / 64, S);
+        hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, st, M, N, K, ks, (const double *)A, lda,
+                           (const double *)B, ldb, (double *)P, kend);
+        hipLaunchKernelGGL(k_gemm_reduce<double>, dim3((M * N + 255) / 256), dim3(256), 0, st, M * N, S,
+                           (const double *)P, (double *)Cm);
+    } else {
+        split_plan(M, N, K, 128, 128, mode == 1 ? 16 : 32, S, ks);
+        dim3 grid((N + 127) / 128, (M + 127) / 128, S);
+        if (mode == 1)
+            hipLaunchKernelGGL(k_gemm_f32, grid, dim3(256), 0, st, M, N, K, ks, (const float *)A, lda,
+                               (const float *)B, ldb, (float *)P, kend);
+        else
+            hipLaunchKernelGGL(k_gemm_bf16x3, grid, dim3(256), 0, st, M, N, K, ks, (const float *)A, lda,
+                               (const float *)B, ldb, (float *)P, kend);
+        hipLaunchKernelGGL(k_gemm_reduce<float>, dim3((M * N + 255) / 256), dim3(256), 0, st, M * N, S,
+                           (const float *)P, (float *)Cm);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace grtm

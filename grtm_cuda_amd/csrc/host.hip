@@ -471,7 +471,8 @@ struct grtm_plan {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int rows64 = 0;  // FP32 plans: frequency rows [0, rows64) are solved in FP64
-    DBuf lw, lw64, flags, dists, A0, A1, B0, B1, C0, C1, tr;
+    int gemm_mode = 0;  // 0 FP64 MFMA, 1 FP32 MFMA, 2 BF16x3 MFMA
+    DBuf lw, lw64, flags, dists, A0, A1, B0, B1, C0, C1, tr, ws;
     double bessel_ms = 0;
     ~grtm_plan()
     {
@@ -481,7 +482,7 @@ struct grtm_plan {
 
 namespace {
 
-size_t esize(int prec) { return prec == 0 ? sizeof(double) : sizeof(float); }
+size_t esize(int prec) { return prec == GRTM_PREC_FP64 ? sizeof(double) : sizeof(float); }
 
 int run_cells(grtm_plan *p, hipStream_t s)
 {
@@ -502,14 +503,17 @@ int run_cells(grtm_plan *p, hipStream_t s)
 }
 int run_gemm(grtm_plan *p, hipStream_t s)
 {
-    HIPCHK(grtm::launch_gemm(p->prec, 4 * p->nw, p->nr, p->nk, p->A0.p, p->nk, p->B0.p, p->nr, p->C0.p, p->nr, s));
-    HIPCHK(grtm::launch_gemm(p->prec, 4 * p->nw, p->nr, p->nk, p->A1.p, p->nk, p->B1.p, p->nr, p->C1.p, p->nr, s));
+    const int M = 4 * p->nw;
+    HIPCHK(grtm::launch_gemm_mfma(p->gemm_mode, M, p->nr, p->nk, p->A0.p, p->nk, p->B0.p, p->nr, p->C0.p, p->ws.p,
+                                  nullptr, s));
+    HIPCHK(grtm::launch_gemm_mfma(p->gemm_mode, M, p->nr, p->nk, p->A1.p, p->nk, p->B1.p, p->nr, p->C1.p, p->ws.p,
+                                  nullptr, s));
     return GRTM_OK;
 }
 int run_ifft(grtm_plan *p, hipStream_t s)
 {
-    HIPCHK(grtm::launch_post_ifft(p->prec, p->pp, 1, 4 * p->nr, nullptr, p->C0.p, p->C1.p, p->nr, p->nr, p->tr.p,
-                                  nullptr, s));
+    HIPCHK(grtm::launch_post_ifft(p->prec == GRTM_PREC_FP64 ? 0 : 1, p->pp, 1, 4 * p->nr, nullptr, p->C0.p, p->C1.p,
+                                  p->nr, p->nr, p->tr.p, nullptr, s));
     return GRTM_OK;
 }
 
@@ -538,7 +542,8 @@ int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, 
 {
     if (!ctx || !job || !grid || !plan_out) return GRTM_E_INPUT;
     *plan_out = nullptr;
-    if (precision != GRTM_PREC_FP64 && precision != GRTM_PREC_FP32) return GRTM_E_INPUT;
+    if (precision != GRTM_PREC_FP64 && precision != GRTM_PREC_FP32 && precision != GRTM_PREC_BF16X3)
+        return GRTM_E_INPUT;
     if (job->n_layers < 2 || job->n_layers + 1 > grtm::kMaxLayers || !job->layers || job->nr < 1 || !job->dists)
         return GRTM_E_INPUT;
     if (grid->nw < 4 || (grid->nw & (grid->nw - 1)) || grid->nw > 8192 || grid->nk < 1 || !(grid->kmax > 0) ||
@@ -570,6 +575,7 @@ int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, 
     if ((status = make_geo(st, compat, p->g))) return status;
     p->ctx = ctx;
     p->prec = precision;
+    p->gemm_mode = precision == GRTM_PREC_FP64 ? 0 : precision == GRTM_PREC_FP32 ? 1 : 2;
     p->compat = compat;
     p->nw = grid->nw;
     p->N = 2 * grid->nw;
@@ -630,6 +636,7 @@ int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, 
     if ((status = p->C0.alloc(arows * p->nr * es))) return status;
     if ((status = p->C1.alloc(arows * p->nr * es))) return status;
     if ((status = p->tr.alloc((size_t)4 * p->nr * p->N * es))) return status;
+    if ((status = p->ws.alloc(grtm::gemm_workspace_bytes(p->gemm_mode, (int)arows, p->nr, p->nk)))) return status;
     HIPCHK(hipMemcpy(p->dists.p, job->dists, sizeof(double) * p->nr, hipMemcpyHostToDevice));
 
     // Bessel tables (planning work: they depend on the grid and distances only)
@@ -637,7 +644,8 @@ int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, 
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, p->stream));
-    HIPCHK(grtm::launch_bessel_tab(precision, compat ? 0 : 1, p->nk, p->dk, p->dists.as<double>(), p->nr, p->B0.p,
+    HIPCHK(grtm::launch_bessel_tab(precision == GRTM_PREC_FP64 ? 0 : 1, compat ? 0 : 1, p->nk, p->dk,
+                                   p->dists.as<double>(), p->nr, p->B0.p,
                                    p->B1.p, p->nr, p->stream));
     HIPCHK(hipEventRecord(e1, p->stream));
     HIPCHK(hipEventSynchronize(e1));
@@ -674,7 +682,7 @@ int grtm_plan_traces_out(grtm_plan *p, double *traces)
     if (!p || !traces) return GRTM_E_INPUT;
     const size_t n = (size_t)4 * p->nr * p->N;
     HIPCHK(hipStreamSynchronize(p->stream));
-    if (p->prec == 0) {
+    if (p->prec == GRTM_PREC_FP64) {
         HIPCHK(hipMemcpy(traces, p->tr.p, n * sizeof(double), hipMemcpyDeviceToHost));
     } else {
         std::vector<float> f(n);
@@ -698,7 +706,8 @@ int grtm_plan_kernels_out(grtm_plan *p, int q, int w_lo, int w_hi, double *kern)
                 HIPCHK(hipMemcpy(buf.data(), (const unsigned char *)A.p + row * p->nk * es, p->nk * es,
                                  hipMemcpyDeviceToHost));
                 for (int ik = 0; ik < p->nk; ik++) {
-                    double v = p->prec == 0 ? ((const double *)buf.data())[ik] : ((const float *)buf.data())[ik];
+                    double v = p->prec == GRTM_PREC_FP64 ? ((const double *)buf.data())[ik]
+                                                         : ((const float *)buf.data())[ik];
                     const double wt = ik == 0 ? 0.5 * p->dk : p->dk;
                     kern[((size_t)(iw - w_lo) * p->nk + ik) * 4 + part * 2 + ri] = v / wt;
                 }
@@ -765,6 +774,52 @@ int grtm_plan_work(grtm_plan *p, double *w)
 void grtm_plan_destroy(grtm_plan *p) { delete p; }
 
 // ------------------------------------------------------------ self-tests
+int grtm_selftest_gemm(int mode, int M, int N, int K, const double *A, const double *B, double *C)
+{
+    if (mode < -1 || mode > 2 || M < 1 || N < 1 || K < 1 || !A || !B || !C) return GRTM_E_INPUT;
+    int st;
+    if ((st = device_ready())) return st;
+    const size_t es = (mode == -1 || mode == 0) ? 8 : 4;
+    DBuf dA, dB, dC, dP;
+    if ((st = dA.alloc((size_t)M * K * es)) || (st = dB.alloc((size_t)K * N * es)) || (st = dC.alloc((size_t)M * N * es)))
+        return st;
+    if (mode >= 0 && (st = dP.alloc(grtm::gemm_workspace_bytes(mode, M, N, K)))) return st;
+    auto up = [&](DBuf &d, const double *h, size_t n) -> int {
+        if (es == 8) {
+            HIPCHK(hipMemcpy(d.p, h, n * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(h, h + n);
+            HIPCHK(hipMemcpy(d.p, f.data(), n * 4, hipMemcpyHostToDevice));
+        }
+        return GRTM_OK;
+    };
+    if ((st = up(dA, A, (size_t)M * K)) || (st = up(dB, B, (size_t)K * N))) return st;
+    if (mode == -1) HIPCHK(grtm::launch_gemm(0, M, N, K, dA.p, K, dB.p, N, dC.p, N, nullptr));
+    else HIPCHK(grtm::launch_gemm_mfma(mode, M, N, K, dA.p, K, dB.p, N, dC.p, dP.p, nullptr, nullptr));
+    HIPCHK(hipDeviceSynchronize());
+    if (es == 8) {
+        HIPCHK(hipMemcpy(C, dC.p, (size_t)M * N * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> f((size_t)M * N);
+        HIPCHK(hipMemcpy(f.data(), dC.p, f.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < f.size(); i++) C[i] = f[i];
+    }
+    return GRTM_OK;
+}
+
+int grtm_selftest_mfma_probe(int l0, double *out)
+{
+    if (l0 < 0 || l0 > 63 || !out) return GRTM_E_INPUT;
+    int st;
+    if ((st = device_ready())) return st;
+    DBuf d;
+    if ((st = d.alloc(64 * 4 * sizeof(double)))) return st;
+    HIPCHK(grtm::launch_probe_f64(l0, d.as<double>(), nullptr));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(out, d.p, 64 * 4 * sizeof(double), hipMemcpyDeviceToHost));
+    return GRTM_OK;
+}
+
 int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi, int n_layer, const double *lay,
                        int s_layer, double zs, int r_layer, double zr, const double *par0_rv, double *out)
 {

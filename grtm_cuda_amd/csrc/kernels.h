@@ -37,6 +37,11 @@ hipError_t launch_bessel_tab(int prec, int mode, int nk, double dk, const double
 hipError_t launch_gemm(int prec, int M, int N, int K, const void *A, long lda, const void *B, long ldb, void *Cm,
                        long ldc, hipStream_t st);
 size_t post_ifft_lds_bytes(int prec, int nw);
+// k-sum contraction on the matrix cores; mode 0 FP64, 1 FP32, 2 BF16x3
+size_t gemm_workspace_bytes(int mode, int M, int N, int K);
+hipError_t launch_probe_f64(int l0, double *out, hipStream_t st);
+hipError_t launch_gemm_mfma(int mode, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
+                            void *Cm, void *P, const int *kend, hipStream_t st);
 hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr, const void *raw, const void *C0,
                             const void *C1, long ldc, int nr, void *traces, void *spec_post, hipStream_t st);
 

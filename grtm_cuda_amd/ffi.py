@@ -20,6 +20,7 @@ GRTM_E_NOMEM = -6
 
 PREC_FP64 = 0
 PREC_FP32 = 1
+PREC_BF16X3 = 2
 
 P = ctypes.c_void_p
 D = ctypes.c_double
@@ -70,6 +71,8 @@ def _lib():
         lib.grtm_plan_destroy.restype = None
         lib.grtm_selftest_cell.argtypes = [I, I, D, D, D, I, P, I, D, I, D, P, P]
         lib.grtm_opcount_cell.argtypes = [I, I, I, I, I, P]
+        lib.grtm_selftest_gemm.argtypes = [I, I, I, I, P, P, P]
+        lib.grtm_selftest_mfma_probe.argtypes = [I, P]
         _LIB = lib
     return _LIB
 
@@ -79,7 +82,8 @@ EXPORTS = [
     "green", "grtm_green", "grtm_green_mem", "grtm_last_status", "grtm_status_str", "grtm_version",
     "grtm_create", "grtm_destroy", "grtm_plan_traces", "grtm_plan_execute", "grtm_plan_sync",
     "grtm_plan_traces_out", "grtm_plan_kernels_out", "grtm_plan_timing", "grtm_plan_work",
-    "grtm_plan_destroy", "grtm_selftest_cell", "grtm_opcount_cell",
+    "grtm_plan_destroy", "grtm_selftest_cell", "grtm_opcount_cell", "grtm_selftest_gemm",
+    "grtm_selftest_mfma_probe",
 ]
 
 

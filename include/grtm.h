@@ -25,6 +25,8 @@ extern "C" {
 #define GRTM_PREC_FP32 1 /* mixed: FP32 recursion with FP64 vertical wavenumbers,
                             FP64 low-frequency rows and FP64 re-solve of
                             non-finite cells; FP32 contraction */
+#define GRTM_PREC_BF16X3 2 /* mixed recursion as GRTM_PREC_FP32; contraction on
+                              split-BF16x3 matrix-core products, FP32 accumulation */
 
 /* ---------------------------------------------------------- drop-in entry */
 /* Replaces the reference's only exported symbol:
@@ -113,6 +115,15 @@ void grtm_plan_destroy(grtm_plan *plan);
  * GPU.  Not used by any entry point above. */
 /* lay: n_layer rows of (h, alpha re, im, beta re, im, mu re, im); out as
  * the oracle's cell: a0[2],a1[2],a2[2],b1,b2 and d/dz versions, 18 complex. */
+/* C[M x N] = A[M x K] B[K x N] (row-major host arrays) on the GPU with the
+ * contraction kernels: mode -1 VALU FP64, 0 FP64 MFMA, 1 FP32 MFMA,
+ * 2 BF16x3 MFMA (operands rounded to FP32 first for modes 1 and 2). */
+int grtm_selftest_gemm(int mode, int M, int N, int K, const double *A, const double *B, double *C);
+
+/* Lane-layout probe of the FP64 matrix-core instruction: lane l0 feeds a = 1,
+ * lane l feeds b = 1000 + l; out[64*4] receives every lane's accumulators. */
+int grtm_selftest_mfma_probe(int l0, double *out);
+
 int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi, int n_layer,
                        const double *lay, int s_layer, double zs, int r_layer, double zr,
                        const double *par0_rv, double *out);

@@ -51,14 +51,15 @@ def test_c1_kernels_match_oracle(c1_oracle, orc):
         assert np.abs(kg - ko).max() <= 1e-11 * np.abs(ko).max()
 
 
-def test_c1_mixed_precision_traces(c1_oracle):
+@pytest.mark.parametrize("prec", [g.PREC_FP32, g.PREC_BF16X3])
+def test_c1_mixed_precision_traces(c1_oracle, prec):
     """FP32 recursion (FP64 vertical wavenumbers, FP64 low-frequency rows,
     FP64 re-solve of non-finite cells) + FP32 contraction: every trace within
     1e-4 rel-L2 of the FP64 oracle."""
     cfg, ref = c1_oracle
-    print("gpu: c1 mixed precision", flush=True)
+    print(f"gpu: c1 mixed precision {prec}", flush=True)
     plan = g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
-                       precision=g.PREC_FP32)
+                       precision=prec)
     tr = plan.traces()
     w = plan.work()
     worst = 0.0
