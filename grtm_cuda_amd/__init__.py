@@ -121,6 +121,28 @@ class TracePlan:
         return dict(cells=w[0], cell_layers=w[1], ksum_flop=w[2], ifft_bytes=w[3], traces=w[4],
                     flops_per_cell=w[5], trans_per_cell=w[6], rows_fp64=int(w[7]), cells_refixed=int(w[8]))
 
+    def set_adaptive(self, kcut, window=None, stride=8, tol=1e-5):
+        """Use a truncated wavenumber grid: kcut[iw] (km^-1) per frequency row,
+        residual check every `stride`-th wavenumber over `window` km^-1 beyond
+        the cut (default: 8 / |z_s - z_r| bounded below by 0.5), fallback to the
+        dense row when the checked tail exceeds tol x the row peak.
+        kcut=None restores the dense grid."""
+        if kcut is None:
+            check(_lib().grtm_plan_set_adaptive(self.ptr, None, 0.0, 1, 1.0), "grtm_plan_set_adaptive")
+            return
+        kc = np.ascontiguousarray(kcut, dtype=np.float64)
+        if kc.shape != (self.nw,):
+            raise ValueError("kcut must have one entry per frequency row")
+        if window is None:
+            window = 0.5
+        check(_lib().grtm_plan_set_adaptive(self.ptr, kc.ctypes.data_as(P), float(window), int(stride), float(tol)),
+              "grtm_plan_set_adaptive")
+
+    def adaptive_stats(self):
+        o = np.zeros(4)
+        check(_lib().grtm_plan_adaptive_stats(self.ptr, o.ctypes.data_as(P)), "grtm_plan_adaptive_stats")
+        return dict(cells_pred=o[0], cells_check=o[1], cells_final=o[2], rows_fallback=int(o[3]))
+
     def __del__(self):
         if getattr(self, "ptr", None) is not None and self.ptr.value:
             _lib().grtm_plan_destroy(self.ptr)

@@ -4,8 +4,10 @@
 // Im force}) of the weighted k-kernels; B [K x N] row-major: J_n(k_i r_j);
 // C [M x N].  Split-K over grid.z: every slice writes its partial product to
 // P[z][M][N] and k_gemm_reduce sums the slices in FP64 in a fixed order.
-// kend (optional, one entry per BM-row tile) limits the K range of a tile:
-// columns at and beyond it are zero in A (wavenumbers past the truncation).
+// kend (optional, one entry per BM-row tile) limits the K range of a tile;
+// nk_w (optional) masks A per row: row r is taken as zero at and beyond
+// column nk_w[r / 4] (four operand rows per frequency), so entries past a
+// frequency's wavenumber cut are never read.
 //
 //   k_gemm_f64    v_mfma_f64_16x16x4f64, 64x64 block tile, 4 waves of 32x32
 //   k_gemm_f32    v_mfma_f32_32x32x2f32, 128x128 block tile, 4 waves of 64x64
@@ -25,7 +27,8 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 // ------------------------------------------------------------------ FP64
 __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, int kslice, const double *__restrict__ A,
                                                   long lda, const double *__restrict__ B, long ldb,
-                                                  double *__restrict__ P, const int *__restrict__ kend)
+                                                  double *__restrict__ P, const int *__restrict__ kend,
+                                                  const int *__restrict__ nk_w)
 {
     constexpr int BM = 64, BN = 64, BK = 16;
     __shared__ double As[BK][BM + 1];
@@ -41,7 +44,8 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, int kslic
     for (int k0 = kb; k0 < ke; k0 += BK) {
         for (int e = tid; e < BM * BK; e += 256) {
             const int mm = e / BK, kk = e % BK, gm = m0 + mm, gk = k0 + kk;
-            As[kk][mm] = (gm < M && gk < ke) ? A[(size_t)gm * lda + gk] : 0.0;
+            const int kr = (gm < M && nk_w) ? min(ke, nk_w[gm >> 2]) : ke;
+            As[kk][mm] = (gm < M && gk < kr) ? A[(size_t)gm * lda + gk] : 0.0;
         }
         for (int e = tid; e < BN * BK; e += 256) {
             const int kk = e / BN, nn = e % BN, gk = k0 + kk, gn = n0 + nn;
@@ -73,7 +77,8 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, int kslic
 // ------------------------------------------------------------------ FP32
 __global__ __launch_bounds__(256) void k_gemm_f32(int M, int N, int K, int kslice, const float *__restrict__ A,
                                                   long lda, const float *__restrict__ B, long ldb,
-                                                  float *__restrict__ P, const int *__restrict__ kend)
+                                                  float *__restrict__ P, const int *__restrict__ kend,
+                                                  const int *__restrict__ nk_w)
 {
     constexpr int BM = 128, BN = 128, BK = 16;
     __shared__ float As[BK][BM + 4];
@@ -90,9 +95,10 @@ __global__ __launch_bounds__(256) void k_gemm_f32(int M, int N, int K, int kslic
     for (int k0 = kb; k0 < ke; k0 += BK) {
         {  // A tile: 128 rows x 16 k, 8 per thread
             const int mm = tid >> 1, kk0 = (tid & 1) * 8, gm = m0 + mm;
+            const int kr = (gm < M && nk_w) ? min(ke, nk_w[gm >> 2]) : ke;
             for (int q = 0; q < 8; q++) {
                 const int gk = k0 + kk0 + q;
-                As[kk0 + q][mm] = (gm < M && gk < ke) ? A[(size_t)gm * lda + gk] : 0.f;
+                As[kk0 + q][mm] = (gm < M && gk < kr) ? A[(size_t)gm * lda + gk] : 0.f;
             }
         }
         {  // B tile: 16 k x 128 cols, 8 per thread
@@ -133,7 +139,8 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo)
 
 __global__ __launch_bounds__(256) void k_gemm_bf16x3(int M, int N, int K, int kslice, const float *__restrict__ A,
                                                      long lda, const float *__restrict__ B, long ldb,
-                                                     float *__restrict__ P, const int *__restrict__ kend)
+                                                     float *__restrict__ P, const int *__restrict__ kend,
+                                                  const int *__restrict__ nk_w)
 {
     constexpr int BM = 128, BN = 128, BK = 32, PADK = BK + 8;
     // k-contiguous rows so one lane's 8 consecutive k are one 16-byte read
@@ -150,9 +157,10 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(int M, int N, int K, int ks
     for (int k0 = kb; k0 < ke; k0 += BK) {
         {  // A tile 128 x 32: 16 per thread
             const int mm = tid >> 1, kk0 = (tid & 1) * 16, gm = m0 + mm;
+            const int kr = (gm < M && nk_w) ? min(ke, nk_w[gm >> 2]) : ke;
             for (int q = 0; q < 16; q++) {
                 const int gk = k0 + kk0 + q;
-                const float x = (gm < M && gk < ke) ? A[(size_t)gm * lda + gk] : 0.f;
+                const float x = (gm < M && gk < kr) ? A[(size_t)gm * lda + gk] : 0.f;
                 split_bf16(x, Ah[mm][kk0 + q], Al[mm][kk0 + q]);
             }
         }
@@ -244,14 +252,14 @@ size_t gemm_workspace_bytes(int mode, int M, int N, int K)
 // mode 0: FP64 MFMA, 1: FP32 MFMA, 2: BF16x3 MFMA.  P: workspace of
 // gemm_workspace_bytes(mode, M, N, K).
 hipError_t launch_gemm_mfma(int mode, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
-                            void *Cm, void *P, const int *kend, hipStream_t st)
+                            void *Cm, void *P, const int *kend, const int *nk_w, hipStream_t st)
 {
     int S, ks;
     if (mode == 0) {
         split_plan(M, N, K, 64, 64, 16, S, ks);
         dim3 grid((N + 63) / 64, (M + 63) / 64, S);
         hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, st, M, N, K, ks, (const double *)A, lda,
-                           (const double *)B, ldb, (double *)P, kend);
+                           (const double *)B, ldb, (double *)P, kend, nk_w);
         hipLaunchKernelGGL(k_gemm_reduce<double>, dim3((M * N + 255) / 256), dim3(256), 0, st, M * N, S,
                            (const double *)P, (double *)Cm);
     } else {
@@ -259,10 +267,10 @@ hipError_t launch_gemm_mfma(int mode, int M, int N, int K, const void *A, long l
         dim3 grid((N + 127) / 128, (M + 127) / 128, S);
         if (mode == 1)
             hipLaunchKernelGGL(k_gemm_f32, grid, dim3(256), 0, st, M, N, K, ks, (const float *)A, lda,
-                               (const float *)B, ldb, (float *)P, kend);
+                               (const float *)B, ldb, (float *)P, kend, nk_w);
         else
             hipLaunchKernelGGL(k_gemm_bf16x3, grid, dim3(256), 0, st, M, N, K, ks, (const float *)A, lda,
-                               (const float *)B, ldb, (float *)P, kend);
+                               (const float *)B, ldb, (float *)P, kend, nk_w);
         hipLaunchKernelGGL(k_gemm_reduce<float>, dim3((M * N + 255) / 256), dim3(256), 0, st, M * N, S,
                            (const float *)P, (float *)Cm);
     }

@@ -473,6 +473,12 @@ struct grtm_plan {
     int rows64 = 0;  // FP32 plans: frequency rows [0, rows64) are solved in FP64
     int gemm_mode = 0;  // 0 FP64 MFMA, 1 FP32 MFMA, 2 BF16x3 MFMA
     DBuf lw, lw64, flags, dists, A0, A1, B0, B1, C0, C1, tr, ws;
+    // adaptive wavenumber grid
+    bool adaptive = false;
+    int chk_stride = 8, max_chk = 0;
+    double chk_tol = 1e-5;
+    std::vector<int> nk_pred_h, nchk_h;
+    DBuf nk_pred, nk_w, nchk, counters, fb_list, kend;  // counters: [row peak nw][tail max nw][fb count]
     double bessel_ms = 0;
     ~grtm_plan()
     {
@@ -486,28 +492,51 @@ size_t esize(int prec) { return prec == GRTM_PREC_FP64 ? sizeof(double) : sizeof
 
 int run_cells(grtm_plan *p, hipStream_t s)
 {
+    const bool ad = p->adaptive;
+    int *nkw = ad ? p->nk_w.as<int>() : nullptr;
+    unsigned *rowpk = ad ? p->counters.as<unsigned>() : nullptr;
+    unsigned *tailmx = ad ? rowpk + p->nw : nullptr;
+    unsigned *fbcnt = ad ? rowpk + 2 * p->nw : nullptr;
+    if (ad) {
+        HIPCHK(grtm::launch_copy_i32(p->nk_pred.as<int>(), nkw, p->nw, s));
+        HIPCHK(grtm::launch_zero_u32(rowpk, 2 * p->nw + 1, s));
+    }
+    const int r64 = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;
     if (p->prec == GRTM_PREC_FP64) {
         HIPCHK(grtm::launch_cells_zr(0, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->nw,
-                                     p->A0.p, p->A1.p, p->nk, nullptr, nullptr, s));
-        return GRTM_OK;
+                                     p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk, s));
+    } else {
+        unsigned *cnt = p->flags.as<unsigned>(), *list = cnt + 1;
+        HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
+        HIPCHK(grtm::launch_cells_zr(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0,
+                                     p->rows64, p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk, s));
+        HIPCHK(grtm::launch_cells_zr(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->rows64,
+                                     p->nw - p->rows64, p->A0.p, p->A1.p, p->nk, cnt, list, nkw, rowpk, s));
+        HIPCHK(grtm::launch_cells_fix(p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->A0.p,
+                                      p->A1.p, p->nk, cnt, list, s));
     }
-    unsigned *cnt = p->flags.as<unsigned>(), *list = cnt + 1;
-    HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
-    HIPCHK(grtm::launch_cells_zr(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->rows64,
-                                 p->A0.p, p->A1.p, p->nk, nullptr, nullptr, s));
-    HIPCHK(grtm::launch_cells_zr(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->rows64,
-                                 p->nw - p->rows64, p->A0.p, p->A1.p, p->nk, cnt, list, s));
-    HIPCHK(grtm::launch_cells_fix(p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->A0.p,
-                                  p->A1.p, p->nk, cnt, list, s));
+    if (!ad) return GRTM_OK;
+    // residual check beyond the cut (FP64 on the FP64 rows), dense fallback, tile extents
+    const void *lwf64 = p->prec == GRTM_PREC_FP64 ? p->lw.p : p->lw64.p;
+    HIPCHK(grtm::launch_check_tail(0, p->compat, lwf64, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, r64,
+                                   p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s));
+    HIPCHK(grtm::launch_check_tail(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, r64,
+                                   p->nw - r64, p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s));
+    HIPCHK(grtm::launch_fallback(p->prec == GRTM_PREC_FP64, p->compat, lwf64, p->L, p->g, p->nk, p->nw, p->dk, p->df,
+                                 p->omega_i, p->chk_tol, rowpk, tailmx, nkw, fbcnt, p->fb_list.as<unsigned>(), p->A0.p,
+                                 p->A1.p, p->nk, s));
+    HIPCHK(grtm::launch_kend_tiles(p->nw, p->gemm_mode == 0 ? 16 : 32, p->nk, nkw, p->kend.as<int>(), s));
     return GRTM_OK;
 }
 int run_gemm(grtm_plan *p, hipStream_t s)
 {
     const int M = 4 * p->nw;
+    const int *kend = p->adaptive ? p->kend.as<int>() : nullptr;
+    const int *nkw = p->adaptive ? p->nk_w.as<int>() : nullptr;
     HIPCHK(grtm::launch_gemm_mfma(p->gemm_mode, M, p->nr, p->nk, p->A0.p, p->nk, p->B0.p, p->nr, p->C0.p, p->ws.p,
-                                  nullptr, s));
+                                  kend, nkw, s));
     HIPCHK(grtm::launch_gemm_mfma(p->gemm_mode, M, p->nr, p->nk, p->A1.p, p->nk, p->B1.p, p->nr, p->C1.p, p->ws.p,
-                                  nullptr, s));
+                                  kend, nkw, s));
     return GRTM_OK;
 }
 int run_ifft(grtm_plan *p, hipStream_t s)
@@ -747,6 +776,67 @@ int grtm_plan_timing(grtm_plan *p, int reps, double *ms)
     return GRTM_OK;
 }
 
+int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int stride, double tol)
+{
+    if (!p) return GRTM_E_INPUT;
+    if (!kcut) {
+        p->adaptive = false;
+        return GRTM_OK;
+    }
+    if (!(window >= 0) || stride < 1 || !(tol > 0)) return GRTM_E_INPUT;
+    p->nk_pred_h.assign(p->nw, 0);
+    p->nchk_h.assign(p->nw, 0);
+    int mx = 0;
+    for (int iw = 0; iw < p->nw; iw++) {
+        if (!(kcut[iw] >= 0)) return GRTM_E_INPUT;
+        const double nkd = std::ceil(kcut[iw] / p->dk);
+        const int n = (int)std::min<double>(p->nk, std::max(1.0, nkd));
+        p->nk_pred_h[iw] = n;
+        const int room = p->nk - n;
+        const int nc = std::min((room + stride - 1) / stride, (int)std::ceil(window / (p->dk * stride)));
+        p->nchk_h[iw] = std::max(0, nc);
+        mx = std::max(mx, p->nchk_h[iw]);
+    }
+    int st;
+    const int ntile = (p->nw + 15) / 16;
+    if (!p->nk_pred.p) {
+        if ((st = p->nk_pred.alloc(sizeof(int) * p->nw)) || (st = p->nk_w.alloc(sizeof(int) * p->nw)) ||
+            (st = p->nchk.alloc(sizeof(int) * p->nw)) || (st = p->counters.alloc(sizeof(unsigned) * (2 * p->nw + 1))) ||
+            (st = p->fb_list.alloc(sizeof(unsigned) * 2 * p->nw)) || (st = p->kend.alloc(sizeof(int) * ntile)))
+            return st;
+    }
+    HIPCHK(hipMemcpy(p->nk_pred.p, p->nk_pred_h.data(), sizeof(int) * p->nw, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p->nchk.p, p->nchk_h.data(), sizeof(int) * p->nw, hipMemcpyHostToDevice));
+    p->max_chk = mx;
+    p->chk_stride = stride;
+    p->chk_tol = tol;
+    p->adaptive = true;
+    return GRTM_OK;
+}
+
+int grtm_plan_adaptive_stats(grtm_plan *p, double *out)
+{
+    if (!p || !out) return GRTM_E_INPUT;
+    out[0] = out[1] = out[2] = out[3] = 0;
+    if (!p->adaptive) return GRTM_OK;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    unsigned fb = 0;
+    HIPCHK(hipMemcpy(&fb, p->counters.as<unsigned>() + 2 * p->nw, sizeof(unsigned), hipMemcpyDeviceToHost));
+    std::vector<int> nkw(p->nw);
+    HIPCHK(hipMemcpy(nkw.data(), p->nk_w.p, sizeof(int) * p->nw, hipMemcpyDeviceToHost));
+    double pred = 0, fin = 0, chk = 0;
+    for (int iw = 0; iw < p->nw; iw++) {
+        pred += p->nk_pred_h[iw];
+        fin += nkw[iw];
+        chk += p->nchk_h[iw];
+    }
+    out[0] = pred;  // cells solved by the predicted grid
+    out[1] = chk;   // residual-check cells
+    out[2] = fin;   // cells after dense fallback
+    out[3] = fb;    // rows that fell back
+    return GRTM_OK;
+}
+
 int grtm_plan_work(grtm_plan *p, double *w)
 {
     if (!p || !w) return GRTM_E_INPUT;
@@ -795,7 +885,7 @@ int grtm_selftest_gemm(int mode, int M, int N, int K, const double *A, const dou
     };
     if ((st = up(dA, A, (size_t)M * K)) || (st = up(dB, B, (size_t)K * N))) return st;
     if (mode == -1) HIPCHK(grtm::launch_gemm(0, M, N, K, dA.p, K, dB.p, N, dC.p, N, nullptr));
-    else HIPCHK(grtm::launch_gemm_mfma(mode, M, N, K, dA.p, K, dB.p, N, dC.p, dP.p, nullptr, nullptr));
+    else HIPCHK(grtm::launch_gemm_mfma(mode, M, N, K, dA.p, K, dB.p, N, dC.p, dP.p, nullptr, nullptr, nullptr));
     HIPCHK(hipDeviceSynchronize());
     if (es == 8) {
         HIPCHK(hipMemcpy(C, dC.p, (size_t)M * N * 8, hipMemcpyDeviceToHost));

@@ -23,7 +23,18 @@ struct PostParams {
 // (non-finite cells appended to flag_list), 2 = FP64 cells / FP32 operands
 hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
                            double df, double omega_i, int w0, int nwr, void *A0, void *A1, long lda,
-                           unsigned *flag_count, unsigned *flag_list, hipStream_t st);
+                           unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
+                           hipStream_t st);
+// adaptive grid: residual check beyond the cut, dense fallback, tile extents
+hipError_t launch_check_tail(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk, double df,
+                             double omega_i, int w0, int nwr, int max_chk, const int *nk_w, const int *nchk,
+                             int stride, unsigned *tail_max, hipStream_t st);
+hipError_t launch_fallback(int out_f64, int compat, const void *lw64, int L, const Geo &g, int nk, int nw, double dk,
+                           double df, double omega_i, double tol, const unsigned *row_peak, const unsigned *tail_max,
+                           int *nk_w, unsigned *fb_count, unsigned *fb_list, void *A0, void *A1, long lda,
+                           hipStream_t st);
+hipError_t launch_kend_tiles(int nw, int rows_per_tile, int nk, const int *nk_w, int *kend, hipStream_t st);
+hipError_t launch_copy_i32(const int *a, int *b, int n, hipStream_t st);
 hipError_t launch_cells_fix(int compat, const void *lw64, int L, const Geo &g, int nk, double dk, double df,
                             double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,
                             const unsigned *flag_list, hipStream_t st);
@@ -40,8 +51,9 @@ size_t post_ifft_lds_bytes(int prec, int nw);
 // k-sum contraction on the matrix cores; mode 0 FP64, 1 FP32, 2 BF16x3
 size_t gemm_workspace_bytes(int mode, int M, int N, int K);
 hipError_t launch_probe_f64(int l0, double *out, hipStream_t st);
+// nk_w (optional): A row r is zero at and beyond column nk_w[r / 4]
 hipError_t launch_gemm_mfma(int mode, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
-                            void *Cm, void *P, const int *kend, hipStream_t st);
+                            void *Cm, void *P, const int *kend, const int *nk_w, hipStream_t st);
 hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr, const void *raw, const void *C0,
                             const void *C1, long ldc, int nr, void *traces, void *spec_post, hipStream_t st);
 

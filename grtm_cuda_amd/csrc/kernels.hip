@@ -54,27 +54,130 @@ __device__ __forceinline__ void store_zr(const TO v[8], int iw, int ik, TO *A0, 
     }
 }
 
+// Largest |value| of a cell, as the bit pattern of a non-negative float
+// (monotone in the value, so atomicMax on the bits is a float max).
+template <typename TO>
+__device__ __forceinline__ unsigned cell_mag_bits(const TO v[8])
+{
+    float m = 0.f;
+    for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(float(v[i])));
+    return isfinite(m) ? __float_as_uint(m) : 0u;
+}
+
+// nk_w (optional): per-row number of wavenumbers solved (adaptive grid);
+// row_peak (optional): per-row max |A| as float bits.
 template <typename T, typename TO, bool COMPAT>
 __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__restrict__ lw_all, int L,
                                                            Geo g, int nk, double dk, double df,
                                                            double omega_i, int w0, TO *__restrict__ A0,
                                                            TO *__restrict__ A1, long lda,
                                                            unsigned *__restrict__ flag_count,
-                                                           unsigned *__restrict__ flag_list)
+                                                           unsigned *__restrict__ flag_list,
+                                                           const int *__restrict__ nk_w,
+                                                           unsigned *__restrict__ row_peak)
 {
     __shared__ LayerW<T> lw[kMaxLayers];
     const int iw = w0 + blockIdx.y;  // frequency row
+    const int nkr = nk_w ? nk_w[iw] : nk;
+    if ((int)(blockIdx.x * blockDim.x) >= nkr) return;  // whole block past this row's cut
     for (int j = threadIdx.x; j < L; j += blockDim.x) lw[j] = lw_all[(size_t)iw * L + j];
     __syncthreads();
     const int ik = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ik >= nk) return;
+    if (ik >= nkr) return;
     TO v[8];
     const bool ok = cell_zr<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
     store_zr(v, iw, ik, A0, A1, lda);
+    if (row_peak) atomicMax(row_peak + iw, cell_mag_bits(v));
     if (!ok && flag_count) {
         const unsigned slot = atomicAdd(flag_count, 1u);
         flag_list[slot] = unsigned(iw) * unsigned(nk) + unsigned(ik);
     }
+}
+
+// Residual check of the truncated grid: wavenumbers nk_w[iw] + stride j + stride/2,
+// j < nchk, are solved (values are not stored); their largest |value| goes to
+// tail_max[iw].  Rows [w0, w0 + gridDim.y).
+template <typename T, bool COMPAT>
+__global__ __launch_bounds__(kCellThreads) void k_check_tail(const LayerW<T> *__restrict__ lw_all, int L, Geo g,
+                                                             int nk, double dk, double df, double omega_i, int w0,
+                                                             const int *__restrict__ nk_w, const int *__restrict__ nchk,
+                                                             int stride, unsigned *__restrict__ tail_max)
+{
+    __shared__ LayerW<T> lw[kMaxLayers];
+    const int iw = w0 + blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nc = nchk[iw];
+    if ((int)(blockIdx.x * blockDim.x) >= nc) return;
+    for (int q = threadIdx.x; q < L; q += blockDim.x) lw[q] = lw_all[(size_t)iw * L + q];
+    __syncthreads();
+    const int ik = nk_w[iw] + stride * j + stride / 2;
+    if (j >= nc || ik >= nk) return;
+    float v[8];
+    cell_zr<T, float, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
+    float m = 0.f;
+    for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(v[i]));
+    if (!isfinite(m)) m = 3.0e38f;  // a non-finite check value fails the row
+    atomicMax(tail_max + iw, __float_as_uint(m));
+}
+
+// Rows whose checked tail exceeds tol * row peak fall back to the dense grid.
+__global__ void k_fallback_select(int nw, int nk, double tol, const unsigned *__restrict__ row_peak,
+                                  const unsigned *__restrict__ tail_max, int *__restrict__ nk_w,
+                                  unsigned *__restrict__ fb_count, unsigned *__restrict__ fb_list)
+{
+    const int iw = blockIdx.x * blockDim.x + threadIdx.x;
+    if (iw >= nw) return;
+    const float pk = __uint_as_float(row_peak[iw]), tm = __uint_as_float(tail_max[iw]);
+    if (nk_w[iw] < nk && !(tm <= float(tol) * pk)) {
+        const unsigned slot = atomicAdd(fb_count, 1u);
+        fb_list[slot] = unsigned(iw);
+        fb_list[nw + slot] = unsigned(nk_w[iw]);  // first wavenumber still to solve
+        nk_w[iw] = nk;
+    }
+}
+
+// Dense completion of fallback rows in FP64: wavenumbers [start, nk) of each
+// listed row.  Grid (k blocks, row slots); slots stride over the list.
+template <typename TO, bool COMPAT>
+__global__ __launch_bounds__(kCellThreads) void k_fallback_cells(const LayerW<double> *__restrict__ lw_all, int L,
+                                                                 Geo g, int nk, int nw, double dk, double df,
+                                                                 double omega_i, TO *__restrict__ A0,
+                                                                 TO *__restrict__ A1, long lda,
+                                                                 const unsigned *__restrict__ fb_count,
+                                                                 const unsigned *__restrict__ fb_list)
+{
+    __shared__ LayerW<double> lw[kMaxLayers];
+    const unsigned n = *fb_count;
+    for (unsigned slot = blockIdx.y; slot < n; slot += gridDim.y) {
+        const int iw = int(fb_list[slot]);
+        const int start = int(fb_list[nw + slot]);
+        __syncthreads();
+        for (int q = threadIdx.x; q < L; q += blockDim.x) lw[q] = lw_all[(size_t)iw * L + q];
+        __syncthreads();
+        for (int ik = start + blockIdx.x * blockDim.x + threadIdx.x; ik < nk; ik += gridDim.x * blockDim.x) {
+            TO v[8];
+            cell_zr<double, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
+            store_zr(v, iw, ik, A0, A1, lda);
+        }
+    }
+}
+
+// K extent of every contraction tile (rows_per_tile frequency rows each),
+// rounded up to a multiple of 32.
+__global__ void k_kend_tiles(int nw, int rows_per_tile, int nk, const int *__restrict__ nk_w, int *__restrict__ kend)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ntile = (nw + rows_per_tile - 1) / rows_per_tile;
+    if (t >= ntile) return;
+    int m = 0;
+    for (int r = t * rows_per_tile; r < min(nw, (t + 1) * rows_per_tile); r++) m = max(m, nk_w[r]);
+    kend[t] = min(nk, (m + 31) / 32 * 32);
+}
+
+__global__ void k_copy_i32(const int *__restrict__ a, int *__restrict__ b, int n)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = a[t];
 }
 
 // FP64 re-solve of the flagged cells (grid-stride over the list).
@@ -318,13 +421,14 @@ __global__ __launch_bounds__(256) void k_post_ifft(PostParams pp, int layout, co
 // list), 2 FP64 cells with FP32 A operands.  lw points at row 0.
 hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
                            double df, double omega_i, int w0, int nwr, void *A0, void *A1, long lda,
-                           unsigned *flag_count, unsigned *flag_list, hipStream_t st)
+                           unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
+                           hipStream_t st)
 {
     if (nwr <= 0) return hipSuccess;
     dim3 grid((nk + kCellThreads - 1) / kCellThreads, nwr);
 #define GRTM_ZR(T, TO, CP, FC, FL)                                                                     \
     hipLaunchKernelGGL((k_cells_zr<T, TO, CP>), grid, dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, L, g, \
-                       nk, dk, df, omega_i, w0, (TO *)A0, (TO *)A1, lda, FC, FL)
+                       nk, dk, df, omega_i, w0, (TO *)A0, (TO *)A1, lda, FC, FL, nk_w, row_peak)
     if (prec == 0) {
         if (compat) GRTM_ZR(double, double, true, nullptr, nullptr);
         else GRTM_ZR(double, double, false, nullptr, nullptr);
@@ -336,6 +440,61 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
         else GRTM_ZR(double, float, false, nullptr, nullptr);
     }
 #undef GRTM_ZR
+    return hipGetLastError();
+}
+
+hipError_t launch_check_tail(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk, double df,
+                             double omega_i, int w0, int nwr, int max_chk, const int *nk_w, const int *nchk,
+                             int stride, unsigned *tail_max, hipStream_t st)
+{
+    if (nwr <= 0 || max_chk <= 0) return hipSuccess;
+    dim3 grid((max_chk + kCellThreads - 1) / kCellThreads, nwr);
+#define GRTM_CHK(T, CP)                                                                                 \
+    hipLaunchKernelGGL((k_check_tail<T, CP>), grid, dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, L, g, nk, \
+                       dk, df, omega_i, w0, nk_w, nchk, stride, tail_max)
+    if (prec == 0) {
+        if (compat) GRTM_CHK(double, true);
+        else GRTM_CHK(double, false);
+    } else {
+        if (compat) GRTM_CHK(float, true);
+        else GRTM_CHK(float, false);
+    }
+#undef GRTM_CHK
+    return hipGetLastError();
+}
+
+hipError_t launch_fallback(int out_f64, int compat, const void *lw64, int L, const Geo &g, int nk, int nw, double dk,
+                           double df, double omega_i, double tol, const unsigned *row_peak, const unsigned *tail_max,
+                           int *nk_w, unsigned *fb_count, unsigned *fb_list, void *A0, void *A1, long lda,
+                           hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fallback_select, dim3((nw + 255) / 256), dim3(256), 0, st, nw, nk, tol, row_peak, tail_max,
+                       nk_w, fb_count, fb_list);
+    dim3 grid((nk + kCellThreads - 1) / kCellThreads, 16);
+#define GRTM_FB(TO, CP)                                                                                 \
+    hipLaunchKernelGGL((k_fallback_cells<TO, CP>), grid, dim3(kCellThreads), 0, st, (const LayerW<double> *)lw64, \
+                       L, g, nk, nw, dk, df, omega_i, (TO *)A0, (TO *)A1, lda, fb_count, fb_list)
+    if (out_f64) {
+        if (compat) GRTM_FB(double, true);
+        else GRTM_FB(double, false);
+    } else {
+        if (compat) GRTM_FB(float, true);
+        else GRTM_FB(float, false);
+    }
+#undef GRTM_FB
+    return hipGetLastError();
+}
+
+hipError_t launch_kend_tiles(int nw, int rows_per_tile, int nk, const int *nk_w, int *kend, hipStream_t st)
+{
+    const int ntile = (nw + rows_per_tile - 1) / rows_per_tile;
+    hipLaunchKernelGGL(k_kend_tiles, dim3((ntile + 255) / 256), dim3(256), 0, st, nw, rows_per_tile, nk, nk_w, kend);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_i32(const int *a, int *b, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_copy_i32, dim3((n + 255) / 256), dim3(256), 0, st, a, b, n);
     return hipGetLastError();
 }
 

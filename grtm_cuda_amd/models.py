@@ -83,3 +83,21 @@ def config(name):
                     dists=np.linspace(5.0, 500.0, 256),
                     grid=dense_grid(2048, 0.05, 16384, layers[:, 2].min()))
     raise KeyError(name)
+
+
+def analytic_kcut(layers, src_depth, rcv_depth, grid, tau=1e-7):
+    """Truncation wavenumber per frequency row without the learned model:
+    surface-wave poles lie below 1.15 w / Vs_min, and beyond them the integrand
+    decays at least like exp(-(k - k_pole) |z_s - z_r|); keep ln(1/tau) decay
+    lengths past the last pole."""
+    nw, dt = grid["nw"], grid["dt"]
+    w = 2 * math.pi * np.arange(nw) / (2 * nw * dt)
+    vs_min = float(np.min(layers[:, 2]))
+    dz = max(abs(src_depth - rcv_depth), 0.5)
+    return np.minimum(1.15 * w / vs_min + math.log(1.0 / tau) / dz, grid["kmax"])
+
+
+def check_window(src_depth, rcv_depth):
+    """Width (km^-1) of the residual-check window beyond the cut: four decay
+    lengths of the direct wave between source and receiver."""
+    return 4.0 / max(abs(src_depth - rcv_depth), 0.5)

@@ -110,6 +110,19 @@ int grtm_plan_timing(grtm_plan *plan, int reps, double *ms);
 int grtm_plan_work(grtm_plan *plan, double *work);
 void grtm_plan_destroy(grtm_plan *plan);
 
+/* Adaptive wavenumber grid (no counterpart in the reference, whose ragged
+ * grid uses the cal_kmax heuristic, green.cu:108-162).  kcut[iw] (km^-1) is
+ * the truncation wavenumber of each frequency row (from the learned
+ * predictor or any other estimate): the recursion solves k_i < kcut only.  A
+ * residual check solves every `stride`-th wavenumber in a window of `window`
+ * km^-1 beyond the cut; a row whose checked tail exceeds tol times the row's
+ * peak is completed on the dense grid in FP64 (fallback).  kcut = NULL
+ * returns the plan to the dense grid. */
+int grtm_plan_set_adaptive(grtm_plan *plan, const double *kcut, double window, int stride, double tol);
+/* After an execution: out[0] cells of the predicted grid, out[1] residual-check
+ * cells, out[2] cells after fallback, out[3] rows that fell back. */
+int grtm_plan_adaptive_stats(grtm_plan *plan, double *out);
+
 /* ------------------------------------------------------------ self-tests */
 /* Host-compiled copies of the device arithmetic, for unit checks without a
  * GPU.  Not used by any entry point above. */
