@@ -1,10 +1,17 @@
-"""Benchmark: Green's traces/s of the GPU path on the production workload.
+"""Benchmark: Green's traces/s of the GPU path on the production workload,
+learned adaptive-k mixed precision against the plain FP64 dense-k path.
 
 Workload (one GPU): 8 layers over a half-space from a recorded seed,
 explosion + vertical force at 12 km, 256 receivers 5-500 km, dt 0.05 s,
 Nw = 2048 (N = 4096), Nk = 16384 uniform to 1.5 w_max / Vs_min.  One step =
 one pass of recursion + k-sum contraction + inverse FFT producing
 2 sources x 256 distances = 512 traces (Z and R each), inputs resident in HBM.
+
+The default measures the learned adaptive path (MLP-predicted per-frequency
+wavenumber cut, residual check and dense fallback, FP32 recursion with FP64
+vertical wavenumbers and low-frequency rows, FP32 MFMA contraction); the
+plain FP64 dense path runs in the same process as the denominator and as the
+accuracy reference (max trace rel-L2 reported).
 
 Prints one JSON line (rank 0).  The CPU baseline is the FP64 oracle on a
 bounded sample of the same workload (a block of frequency rows), run on the
@@ -46,13 +53,48 @@ def cpu_baseline(cfg, rows, threads):
                        f"scaled x{nw / rows:.0f}")
 
 
+def time_plan(plan, steps, warmup):
+    for _ in range(warmup):
+        plan.execute()
+    plan.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.execute()
+    plan.sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def roofline(g, plan, prec, tm, work):
+    kernels = {"recursion": tm["cells"], "ksum": tm["ksum"], "ifft": tm["ifft"]}
+    dom = max(kernels, key=kernels.get)
+    peak = FP64_PEAK_TFLOPS if prec == g.PREC_FP64 else FP32_PEAK_TFLOPS
+    if dom == "recursion":
+        st = plan.adaptive_stats()
+        cells = st["cells_final"] + st["cells_check"] if st["cells_final"] else work["cells"]
+        achieved = cells * work["flops_per_cell"] / (tm["cells"] * 1e-3) / 1e12
+        return dict(bound="valu", achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak, traffic=None,
+                    kernel="k_cells_zr", flops_per_cell=work["flops_per_cell"],
+                    transcendentals_per_cell=work["trans_per_cell"], cells=cells), kernels
+    if dom == "ksum":
+        achieved = work["ksum_flop"] / (tm["ksum"] * 1e-3) / 1e12
+        kpeak = {g.PREC_FP64: FP64_PEAK_TFLOPS, g.PREC_FP32: FP32_PEAK_TFLOPS, g.PREC_BF16X3: BF16_PEAK_TFLOPS / 3}[prec]
+        return dict(bound="mfma", achieved=achieved, peak=kpeak, unit="TFLOP/s", frac=achieved / kpeak,
+                    traffic=None, kernel="k_gemm_mfma"), kernels
+    achieved = work["ifft_bytes"] / (tm["ifft"] * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                traffic=None, kernel="k_post_ifft"), kernels
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3"])
+    ap.add_argument("--path", default="adaptive", choices=["adaptive", "dense"],
+                    help="adaptive: learned truncated grid, mixed precision (the headline); dense: plain grid")
+    ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16x3"],
+                    help="arithmetic of the measured path (default fp32 adaptive / fp64 dense)")
     ap.add_argument("--cpu-rows", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -63,43 +105,44 @@ def main():
     from grtm_cuda_amd import models
 
     cfg = models.config(args.workload)
-    prec = {"fp64": g.PREC_FP64, "fp32": g.PREC_FP32, "bf16x3": g.PREC_BF16X3}[args.precision]
-    print(f"bench: planning {args.workload} {args.precision}", file=sys.stderr, flush=True)
-    plan = g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"], precision=prec)
-    work = plan.work()
-    for _ in range(args.warmup):
-        plan.execute()
-    plan.sync()
-    print("bench: timing", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.execute()
-    plan.sync()
-    elapsed = time.perf_counter() - t0
-    ms_step = 1e3 * elapsed / args.steps
-    traces_per_step = work["traces"]
-    value = traces_per_step * args.steps / elapsed
+    pname = args.precision or ("fp32" if args.path == "adaptive" else "fp64")
+    prec = {"fp64": g.PREC_FP64, "fp32": g.PREC_FP32, "bf16x3": g.PREC_BF16X3}[pname]
+    mk = lambda p: g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
+                               precision=p)
 
-    # per-kernel durations with HIP events on the plan's stream
-    tm = plan.timing(reps=3)
-    kernels = {"recursion": tm["cells"], "ksum": tm["ksum"], "ifft": tm["ifft"]}
-    dom = max(kernels, key=kernels.get)
-    peak = FP64_PEAK_TFLOPS if prec == g.PREC_FP64 else FP32_PEAK_TFLOPS
-    if dom == "recursion":
-        flops = work["cells"] * work["flops_per_cell"]
-        achieved = flops / (tm["cells"] * 1e-3) / 1e12
-        roof = dict(bound="valu", achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak, traffic=None,
-                    kernel="k_cells_zr", flops_per_cell=work["flops_per_cell"],
-                    transcendentals_per_cell=work["trans_per_cell"])
-    elif dom == "ksum":
-        achieved = work["ksum_flop"] / (tm["ksum"] * 1e-3) / 1e12
-        kpeak = {g.PREC_FP64: FP64_PEAK_TFLOPS, g.PREC_FP32: FP32_PEAK_TFLOPS, g.PREC_BF16X3: BF16_PEAK_TFLOPS / 3}[prec]
-        roof = dict(bound="mfma", achieved=achieved, peak=kpeak, unit="TFLOP/s", frac=achieved / kpeak,
-                    traffic=None, kernel="k_gemm_mfma")
+    # the plain FP64 dense-k path: the denominator and the accuracy reference
+    print(f"bench: {args.workload} plain FP64 dense", file=sys.stderr, flush=True)
+    ref_plan = mk(g.PREC_FP64)
+    t_ref = time_plan(ref_plan, max(3, args.steps // 2), 1)
+    ref = ref_plan.traces()
+    work = ref_plan.work()
+    traces_per_step = work["traces"]
+
+    extra = {}
+    if args.path == "dense" and prec == g.PREC_FP64:
+        plan, t_step, out = ref_plan, t_ref, ref
     else:
-        achieved = work["ifft_bytes"] / (tm["ifft"] * 1e-3) / 1e9
-        roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=None, kernel="k_post_ifft")
+        plan = mk(prec)
+        if args.path == "adaptive":
+            from grtm_cuda_amd import learned
+            t0 = time.perf_counter()
+            pred = learned.Predictor()
+            kcut = pred.kcut(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["grid"])
+            extra["predictor_ms"] = 1e3 * (time.perf_counter() - t0)
+            plan.set_adaptive(kcut, window=models.check_window(cfg["src_depth"], cfg["rcv_depth"]), stride=8,
+                              tol=3e-6)
+        print(f"bench: {args.path} {pname}", file=sys.stderr, flush=True)
+        t_step = time_plan(plan, args.steps, args.warmup)
+        out = plan.traces()
+        if args.path == "adaptive":
+            st = plan.adaptive_stats()
+            dense_cells = cfg["grid"]["nw"] * cfg["grid"]["nk"]
+            extra.update(cells_frac=st["cells_final"] / dense_cells, check_frac=st["cells_check"] / dense_cells,
+                         rows_fallback=st["rows_fallback"])
+    err = np.linalg.norm(out - ref, axis=-1) / np.linalg.norm(ref, axis=-1)
+    value = traces_per_step / t_step
+    tm = plan.timing(reps=3)
+    roof, kernels = roofline(g, plan, prec, tm, work)
 
     cpu = None
     if not args.no_cpu:
@@ -114,16 +157,20 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": 1e3 * t_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {g.PREC_FP64: "f64", g.PREC_FP32: "f32", g.PREC_BF16X3: "f32+bf16x3"}[prec],
         "data": "synthetic (seeded layered model)",
-        "config": {"workload": f"{args.workload}: plain dense-k, {len(cfg['layers']) - 1} layers + half-space, "
-                               f"explosion + vertical force, {len(cfg['dists'])} receivers, "
+        "config": {"workload": f"{args.workload}: {len(cfg['layers']) - 1} layers + half-space, explosion + "
+                               f"vertical force at {cfg['src_depth']} km, {len(cfg['dists'])} receivers, "
                                f"Nw={cfg['grid']['nw']}, Nk={cfg['grid']['nk']}",
-                   "path": args.precision, "traces_per_step": traces_per_step},
+                   "path": f"{args.path} {pname}", "traces_per_step": traces_per_step},
+        "plain_fp64_dense": {"value": traces_per_step / t_ref, "ms_per_step": 1e3 * t_ref},
+        "speedup_vs_plain_fp64": t_ref / t_step,
+        "max_rel_l2_vs_fp64": float(err.max()),
+        "adaptive": extra or None,
         "kernel_ms": {k: round(v, 4) for k, v in kernels.items()},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -138,6 +138,13 @@ class TracePlan:
         check(_lib().grtm_plan_set_adaptive(self.ptr, kc.ctypes.data_as(P), float(window), int(stride), float(tol)),
               "grtm_plan_set_adaptive")
 
+    def row_labels(self, tau=1e-7):
+        """Per-row training labels after a dense execute(): array [nw, 26]
+        (peak, cut index, 8 x (peak index, half width, relative height))."""
+        out = np.zeros((self.nw, 26), dtype=np.float32)
+        check(_lib().grtm_plan_row_labels(self.ptr, float(tau), out.ctypes.data_as(P)), "grtm_plan_row_labels")
+        return out
+
     def adaptive_stats(self):
         o = np.zeros(4)
         check(_lib().grtm_plan_adaptive_stats(self.ptr, o.ctypes.data_as(P)), "grtm_plan_adaptive_stats")

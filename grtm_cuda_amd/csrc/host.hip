@@ -89,11 +89,28 @@ int prepare_stack(int compat, int n, const double *top, const double *vp, const 
         }
     }
     st.L = c;
+    double tops[grtm::kMaxLayers];
+    for (int i = 0; i < c; i++) tops[i] = st.h[i];
     for (int i = 0; i < c - 1; i++) st.h[i] = st.h[i + 1] - st.h[i];
     st.hs = hs;
     st.hr = hr;
-    double zs = hs, zr = hr;
     st.s = st.rl = -1;
+    if (!compat) {
+        // physics mode: locate from the layer tops (the source sits at the
+        // bottom of the layer above its interface, zs = h_s exactly)
+        int s = 0, r = 0;
+        for (int i = 0; i < c; i++) {
+            if (tops[i] < hs) s = i;
+            if (tops[i] < hr) r = i;
+        }
+        st.s = s;
+        st.zs = hs - tops[s];
+        st.rl = r;
+        st.zr = hr - tops[r];
+        if (st.s >= c - 1) return GRTM_E_UNSUPPORTED;
+        return GRTM_OK;
+    }
+    double zs = hs, zr = hr;
     for (int i = 0; i < c; i++) {
         if (zs <= st.h[i] || i == c - 1) {
             st.s = i;
@@ -811,6 +828,20 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
     p->chk_stride = stride;
     p->chk_tol = tol;
     p->adaptive = true;
+    return GRTM_OK;
+}
+
+int grtm_plan_row_labels(grtm_plan *p, double tau, float *out)
+{
+    if (!p || !out || !(tau > 0) || p->adaptive) return GRTM_E_INPUT;
+    if ((size_t)p->nk * sizeof(float) > 160 * 1024) return GRTM_E_UNSUPPORTED;
+    DBuf d;
+    int st;
+    if ((st = d.alloc(sizeof(float) * 26 * (size_t)p->nw))) return st;
+    HIPCHK(grtm::launch_row_labels(p->prec == GRTM_PREC_FP64 ? 0 : 1, p->nw, p->nk, p->nk, p->A0.p, p->A1.p, tau,
+                                   d.as<float>(), p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    HIPCHK(hipMemcpy(out, d.p, sizeof(float) * 26 * (size_t)p->nw, hipMemcpyDeviceToHost));
     return GRTM_OK;
 }
 

@@ -35,6 +35,9 @@ hipError_t launch_fallback(int out_f64, int compat, const void *lw64, int L, con
                            hipStream_t st);
 hipError_t launch_kend_tiles(int nw, int rows_per_tile, int nk, const int *nk_w, int *kend, hipStream_t st);
 hipError_t launch_copy_i32(const int *a, int *b, int n, hipStream_t st);
+// per-row training labels (peak, truncation index, 8 local maxima); out: nw x 26 floats
+hipError_t launch_row_labels(int prec, int nw, int nk, long lda, const void *A0, const void *A1, double tau,
+                             float *out, hipStream_t st);
 hipError_t launch_cells_fix(int compat, const void *lw64, int L, const Geo &g, int nk, double dk, double df,
                             double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,
                             const unsigned *flag_list, hipStream_t st);

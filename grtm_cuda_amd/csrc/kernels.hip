@@ -174,6 +174,123 @@ __global__ void k_kend_tiles(int nw, int rows_per_tile, int nk, const int *__res
     kend[t] = min(nk, (m + 31) / 32 * 32);
 }
 
+// Training labels of one frequency row from its solved k-kernels (dense grid):
+// m(k) = max |A| over the 8 operand entries of wavenumber k;
+// out[0] = peak of m, out[1] = 1 + last k with m(k) > tau peak,
+// out[2 + 3i .. 4 + 3i] = (index, half width at half maximum, height / peak)
+// of the i-th largest local maximum above 1e-3 peak (index -1 when absent).
+template <typename T>
+__global__ __launch_bounds__(256) void k_row_labels(int nk, long lda, const T *__restrict__ A0,
+                                                    const T *__restrict__ A1, double tau, float *__restrict__ out)
+{
+    extern __shared__ float m[];
+    __shared__ float redv[256];
+    __shared__ int redi[256];
+    const int iw = blockIdx.x, tid = threadIdx.x;
+    for (int k = tid; k < nk; k += 256) {
+        float v = 0.f;
+        for (int q = 0; q < 4; q++) {
+            v = fmaxf(v, fabsf(float(A0[(size_t)(4 * iw + q) * lda + k])));
+            v = fmaxf(v, fabsf(float(A1[(size_t)(4 * iw + q) * lda + k])));
+        }
+        m[k] = isfinite(v) ? v : 0.f;
+    }
+    __syncthreads();
+    // peak
+    float pk = 0.f;
+    for (int k = tid; k < nk; k += 256) pk = fmaxf(pk, m[k]);
+    redv[tid] = pk;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) redv[tid] = fmaxf(redv[tid], redv[tid + o]);
+        __syncthreads();
+    }
+    pk = redv[0];
+    __syncthreads();
+    // truncation index
+    int last = 0;
+    for (int k = tid; k < nk; k += 256)
+        if (m[k] > float(tau) * pk) last = max(last, k + 1);
+    redi[tid] = last;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) redi[tid] = max(redi[tid], redi[tid + o]);
+        __syncthreads();
+    }
+    float *o = out + (size_t)iw * 26;
+    if (tid == 0) {
+        o[0] = pk;
+        o[1] = float(redi[0]);
+    }
+    __syncthreads();
+    // top-8 local maxima; a taken peak suppresses +-2 half widths around it
+    for (int round = 0; round < 8; round++) {
+        float best = 0.f;
+        int bi = -1;
+        for (int k = tid + 1; k < nk - 1; k += 256) {
+            const float v = m[k];
+            if (v > m[k - 1] && v >= m[k + 1] && v > 1e-3f * pk && v > best) {
+                best = v;
+                bi = k;
+            }
+        }
+        redv[tid] = best;
+        redi[tid] = bi;
+        __syncthreads();
+        for (int s2 = 128; s2 > 0; s2 >>= 1) {
+            if (tid < s2 && redv[tid + s2] > redv[tid]) {
+                redv[tid] = redv[tid + s2];
+                redi[tid] = redi[tid + s2];
+            }
+            __syncthreads();
+        }
+        const int c = redi[0];
+        const float h = redv[0];
+        __syncthreads();
+        if (c < 0) {
+            if (tid == 0) o[2 + 3 * round] = -1.f, o[3 + 3 * round] = 0.f, o[4 + 3 * round] = 0.f;
+            continue;
+        }
+        // half width at half maximum (scan both sides, bounded)
+        int hw = 0;
+        if (tid == 0) {
+            int l = c, r = c;
+            while (l > 0 && c - l < 2048 && m[l] > 0.5f * h) l--;
+            while (r < nk - 1 && r - c < 2048 && m[r] > 0.5f * h) r++;
+            hw = max(1, (r - l) / 2);
+            o[2 + 3 * round] = float(c);
+            o[3 + 3 * round] = float(hw);
+            o[4 + 3 * round] = h / pk;
+            redi[0] = hw;
+        }
+        __syncthreads();
+        hw = redi[0];
+        __syncthreads();
+        for (int k = max(0, c - 2 * hw) + tid; k <= min(nk - 1, c + 2 * hw); k += 256) m[k] = 0.f;
+        __syncthreads();
+    }
+}
+
+hipError_t launch_row_labels(int prec, int nw, int nk, long lda, const void *A0, const void *A1, double tau,
+                             float *out, hipStream_t st)
+{
+    const size_t lds = sizeof(float) * (size_t)nk;
+    if (lds > 65536) {
+        hipError_t e = prec == 0 ? hipFuncSetAttribute((const void *)k_row_labels<double>,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)
+                                 : hipFuncSetAttribute((const void *)k_row_labels<float>,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    if (prec == 0)
+        hipLaunchKernelGGL(k_row_labels<double>, dim3(nw), dim3(256), lds, st, nk, lda, (const double *)A0,
+                           (const double *)A1, tau, out);
+    else
+        hipLaunchKernelGGL(k_row_labels<float>, dim3(nw), dim3(256), lds, st, nk, lda, (const float *)A0,
+                           (const float *)A1, tau, out);
+    return hipGetLastError();
+}
+
 __global__ void k_copy_i32(const int *__restrict__ a, int *__restrict__ b, int n)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;

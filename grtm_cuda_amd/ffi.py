@@ -75,6 +75,19 @@ def _lib():
         lib.grtm_selftest_mfma_probe.argtypes = [I, P]
         lib.grtm_plan_set_adaptive.argtypes = [P, P, D, I, D]
         lib.grtm_plan_adaptive_stats.argtypes = [P, P]
+        lib.grtm_plan_row_labels.argtypes = [P, D, P]
+        lib.grtm_mlp_create.argtypes = [I, I, I, I, I, P]
+        lib.grtm_mlp_destroy.argtypes = [P]
+        lib.grtm_mlp_destroy.restype = None
+        lib.grtm_mlp_nparams.argtypes = [P]
+        lib.grtm_mlp_nparams.restype = ctypes.c_long
+        lib.grtm_mlp_set_params.argtypes = [P, P]
+        lib.grtm_mlp_get_params.argtypes = [P, P]
+        lib.grtm_mlp_set_data.argtypes = [P, P, P, P, I]
+        lib.grtm_mlp_train.argtypes = [P, I, I, ctypes.c_float, ctypes.c_ulonglong, P]
+        lib.grtm_mlp_grads.argtypes = [P, P, P, P, I, P, P]
+        lib.grtm_mlp_predict.argtypes = [P, P, I, P]
+        lib.grtm_mlp_set_loss.argtypes = [P, I, ctypes.c_float]
         _LIB = lib
     return _LIB
 
@@ -86,6 +99,9 @@ EXPORTS = [
     "grtm_plan_traces_out", "grtm_plan_kernels_out", "grtm_plan_timing", "grtm_plan_work",
     "grtm_plan_destroy", "grtm_selftest_cell", "grtm_opcount_cell", "grtm_selftest_gemm",
     "grtm_selftest_mfma_probe", "grtm_plan_set_adaptive", "grtm_plan_adaptive_stats",
+    "grtm_plan_row_labels", "grtm_mlp_create", "grtm_mlp_destroy", "grtm_mlp_nparams", "grtm_mlp_set_params",
+    "grtm_mlp_get_params", "grtm_mlp_set_data", "grtm_mlp_train", "grtm_mlp_grads", "grtm_mlp_predict",
+    "grtm_mlp_set_loss",
 ]
 
 

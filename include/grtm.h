@@ -119,6 +119,11 @@ void grtm_plan_destroy(grtm_plan *plan);
  * peak is completed on the dense grid in FP64 (fallback).  kcut = NULL
  * returns the plan to the dense grid. */
 int grtm_plan_set_adaptive(grtm_plan *plan, const double *kcut, double window, int stride, double tol);
+/* Training labels of every frequency row after a dense execution: out[26*iw
+ * + 0] peak |kernel|, [1] 1 + last wavenumber index whose |kernel| exceeds
+ * tau x peak, [2+3i, 3+3i, 4+3i] index, half width (samples) and relative
+ * height of the i-th largest local maximum (index -1 when absent), i < 8. */
+int grtm_plan_row_labels(grtm_plan *plan, double tau, float *out);
 /* After an execution: out[0] cells of the predicted grid, out[1] residual-check
  * cells, out[2] cells after fallback, out[3] rows that fell back. */
 int grtm_plan_adaptive_stats(grtm_plan *plan, double *out);
@@ -148,6 +153,31 @@ int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi,
  * (1: receiver layer above the source layer, 2/3: same layer above/below,
  * 4: below). */
 int grtm_opcount_cell(int n_layers, int s_layer, int r_layer, int branch, int up, double *out);
+
+/* ------------------------------------------- learned wavenumber predictor */
+/* Multilayer perceptron n_in -> depth x width (ReLU) -> n_out, trained and
+ * evaluated on the GPU: BF16 weights and activations, FP32 accumulation,
+ * FP32 master weights and Adam.  No counterpart in the reference: it predicts
+ * the per-frequency truncation wavenumber and pole clusters that drive the
+ * adaptive grid (grtm_plan_set_adaptive).  Parameters are laid out layer by
+ * layer as W_l [fan_in x fan_out] row-major followed by b_l. */
+typedef struct grtm_mlp grtm_mlp;
+int grtm_mlp_create(int n_in, int width, int depth, int n_out, int max_batch, grtm_mlp **mlp);
+void grtm_mlp_destroy(grtm_mlp *mlp);
+long grtm_mlp_nparams(grtm_mlp *mlp);
+int grtm_mlp_set_params(grtm_mlp *mlp, const float *params);
+int grtm_mlp_get_params(grtm_mlp *mlp, float *params);
+/* Loss: the first n_quantile outputs use the pinball loss of quantile q
+ * (a per-row conservative bound), the others the masked squared error. */
+int grtm_mlp_set_loss(grtm_mlp *mlp, int n_quantile, float q);
+/* Resident training set: X [n x n_in], targets T and mask M [n x n_out]. */
+int grtm_mlp_set_data(grtm_mlp *mlp, const float *X, const float *T, const float *M, int n);
+/* `steps` Adam steps on random batches (rows from a splitmix64 stream seeded
+ * by `seed`); loss of the masked mean squared error per step in losses. */
+int grtm_mlp_train(grtm_mlp *mlp, int steps, int batch, float lr, unsigned long long seed, float *losses);
+/* Gradient of the loss on n <= max_batch samples (params layout). */
+int grtm_mlp_grads(grtm_mlp *mlp, const float *X, const float *T, const float *M, int n, float *grads, float *loss);
+int grtm_mlp_predict(grtm_mlp *mlp, const float *X, int n, float *Y);
 
 #ifdef __cplusplus
 }

@@ -873,12 +873,26 @@ static int prepare_stack(int compat, int n_in, const double *top, const double *
 #undef PUSH
     run->n_layer = cnt;
     int L = cnt;
+    double tops[OR_MAX_LAYERS];
+    for (int i = 0; i < L; i++) tops[i] = run->h[i];
     for (int i = 0; i < L - 1; i++) run->h[i] = run->h[i + 1] - run->h[i];
     run->hs = hs;
     run->hr = hr;
-    double zs = hs, zr = hr;
     run->s_layer = -1;
     run->r_layer = -1;
+    if (!compat) { /* physics mode: locate from the layer tops */
+        int s = 0, r = 0;
+        for (int i = 0; i < L; i++) {
+            if (tops[i] < hs) s = i;
+            if (tops[i] < hr) r = i;
+        }
+        run->s_layer = s;
+        run->zs = hs - tops[s];
+        run->r_layer = r;
+        run->zr = hr - tops[r];
+        return s >= L - 1 ? OR_E_UNSUPPORTED : OR_OK;
+    }
+    double zs = hs, zr = hr;
     for (int i = 0; i < L; i++) {
         if (zs <= run->h[i] || i == L - 1) {
             run->s_layer = i;
