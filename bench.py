@@ -97,6 +97,8 @@ def main():
                     help="arithmetic of the measured path (default fp32 adaptive / fp64 dense)")
     ap.add_argument("--cpu-rows", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run one plain 256 MiB device copy (calibrates hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py: this path runs on one GPU; multi-GPU runs are not part of this build (use --gpus 1)")
@@ -104,6 +106,9 @@ def main():
     import grtm_cuda_amd as g
     from grtm_cuda_amd import models
 
+    if args.copy_probe:
+        from grtm_cuda_amd.ffi import _lib, check
+        check(_lib().grtm_selftest_copy(64 << 20), "grtm_selftest_copy")
     cfg = models.config(args.workload)
     pname = args.precision or ("fp32" if args.path == "adaptive" else "fp64")
     prec = {"fp64": g.PREC_FP64, "fp32": g.PREC_FP32, "bf16x3": g.PREC_BF16X3}[pname]

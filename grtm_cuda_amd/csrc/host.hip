@@ -496,6 +496,8 @@ struct grtm_plan {
     double chk_tol = 1e-5;
     std::vector<int> nk_pred_h, nchk_h;
     DBuf nk_pred, nk_w, nchk, counters, fb_list, kend;  // counters: [row peak nw][tail max nw][fb count]
+    DBuf tiles;             // recursion work items of the predicted grid, ordered by row
+    int ntiles = 0, ntiles64 = 0;  // all items; items of rows [0, rows64)
     double bessel_ms = 0;
     ~grtm_plan()
     {
@@ -519,16 +521,23 @@ int run_cells(grtm_plan *p, hipStream_t s)
         HIPCHK(grtm::launch_zero_u32(rowpk, 2 * p->nw + 1, s));
     }
     const int r64 = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;
+    // work items: the predicted grid's list (rows [0, r64) first), or every tile
+    const unsigned *t_lo = ad ? p->tiles.as<unsigned>() : nullptr;
+    const unsigned *t_hi = ad ? t_lo + p->ntiles64 : nullptr;
+    const int n_lo = ad ? p->ntiles64 : 0, n_hi = ad ? p->ntiles - p->ntiles64 : 0;
     if (p->prec == GRTM_PREC_FP64) {
         HIPCHK(grtm::launch_cells_zr(0, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->nw,
-                                     p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk, s));
+                                     t_lo, n_lo, p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk,
+                                     s));
     } else {
         unsigned *cnt = p->flags.as<unsigned>(), *list = cnt + 1;
         HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
         HIPCHK(grtm::launch_cells_zr(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0,
-                                     p->rows64, p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk, s));
+                                     p->rows64, t_lo, n_lo, p->A0.p, p->A1.p, p->nk, nullptr, nullptr,
+                                     nkw, rowpk, s));
         HIPCHK(grtm::launch_cells_zr(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->rows64,
-                                     p->nw - p->rows64, p->A0.p, p->A1.p, p->nk, cnt, list, nkw, rowpk, s));
+                                     p->nw - p->rows64, t_hi, n_hi, p->A0.p, p->A1.p, p->nk, cnt, list,
+                                     nkw, rowpk, s));
         HIPCHK(grtm::launch_cells_fix(p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->A0.p,
                                       p->A1.p, p->nk, cnt, list, s));
     }
@@ -596,6 +605,7 @@ int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, 
         !(grid->dt > 0) || !(grid->taper >= 0 && grid->taper <= 1))
         return GRTM_E_INPUT;
     if (precision == GRTM_PREC_FP64 && grid->nw > 4096) return GRTM_E_UNSUPPORTED;  // IFFT LDS limit
+    if (!grtm::cells_tile_limit(grid->nw, grid->nk)) return GRTM_E_UNSUPPORTED;
     for (int i = 0; i < job->nr; i++)
         if (!(job->dists[i] > 0)) return GRTM_E_INPUT;
     HIPCHK(hipSetDevice(ctx->device));
@@ -824,6 +834,20 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
     }
     HIPCHK(hipMemcpy(p->nk_pred.p, p->nk_pred_h.data(), sizeof(int) * p->nw, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(p->nchk.p, p->nchk_h.data(), sizeof(int) * p->nw, hipMemcpyHostToDevice));
+    // recursion work items of the predicted grid, rows solved in FP64 first
+    const int r64 = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;
+    std::vector<unsigned> items;
+    int n64 = 0;
+    for (int iw = 0; iw < p->nw; iw++) {
+        const int nt = grtm::cells_tiles_per_row(p->nk_pred_h[iw]);
+        for (int kb = 0; kb < nt; kb++) items.push_back(grtm::cells_tile(iw, kb));
+        if (iw + 1 == r64) n64 = (int)items.size();
+    }
+    if ((st = p->tiles.alloc(sizeof(unsigned) * std::max<size_t>(1, items.size())))) return st;
+    if (!items.empty())
+        HIPCHK(hipMemcpy(p->tiles.p, items.data(), sizeof(unsigned) * items.size(), hipMemcpyHostToDevice));
+    p->ntiles = (int)items.size();
+    p->ntiles64 = n64;
     p->max_chk = mx;
     p->chk_stride = stride;
     p->chk_tol = tol;
@@ -938,6 +962,19 @@ int grtm_selftest_mfma_probe(int l0, double *out)
     HIPCHK(grtm::launch_probe_f64(l0, d.as<double>(), nullptr));
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(out, d.p, 64 * 4 * sizeof(double), hipMemcpyDeviceToHost));
+    return GRTM_OK;
+}
+
+int grtm_selftest_copy(int n_int32)
+{
+    if (n_int32 < 1) return GRTM_E_INPUT;
+    int st;
+    if ((st = device_ready())) return st;
+    DBuf a, b;
+    if ((st = a.alloc(sizeof(int) * (size_t)n_int32)) || (st = b.alloc(sizeof(int) * (size_t)n_int32))) return st;
+    HIPCHK(grtm::launch_zero_u32(a.as<unsigned>(), n_int32, nullptr));
+    HIPCHK(grtm::launch_copy_i32(a.as<int>(), b.as<int>(), n_int32, nullptr));
+    HIPCHK(hipDeviceSynchronize());
     return GRTM_OK;
 }
 

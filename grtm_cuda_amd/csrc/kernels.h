@@ -20,11 +20,17 @@ struct PostParams {
 };
 
 // prec: 0 = FP64 cells / FP64 operands, 1 = FP32 cells / FP32 operands
-// (non-finite cells appended to flag_list), 2 = FP64 cells / FP32 operands
+// (non-finite cells appended to flag_list), 2 = FP64 cells / FP32 operands.
+// Work items are (row, tile of 256 wavenumbers), one block each: the list
+// `tiles` (ntiles entries cells_tile(row, tile)), or, when tiles is null,
+// every tile of rows [w0, w0 + nwr).
+int cells_tiles_per_row(int nk);
+unsigned cells_tile(int iw, int kb);
+int cells_tile_limit(int nw, int nk);  // 1 if (nw, nk) fit the item encoding
 hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
-                           double df, double omega_i, int w0, int nwr, void *A0, void *A1, long lda,
-                           unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
-                           hipStream_t st);
+                           double df, double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles, void *A0,
+                           void *A1, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
+                           unsigned *row_peak, hipStream_t st);
 // adaptive grid: residual check beyond the cut, dense fallback, tile extents
 hipError_t launch_check_tail(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk, double df,
                              double omega_i, int w0, int nwr, int max_chk, const int *nk_w, const int *nchk,

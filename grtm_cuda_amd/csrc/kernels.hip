@@ -12,6 +12,8 @@
 //                  (Stockham, in LDS) + exp(omega_I t) scaling
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "cell.h"
 #include "integrand.h"
 #include "kernels.h"
@@ -19,6 +21,7 @@
 namespace grtm {
 
 constexpr int kCellThreads = 256;
+constexpr int kTileBits = 12;  // work item = row << kTileBits | k tile
 
 // ---------------------------------------------------------------- recursion
 // One thread per (omega, k) cell; T = arithmetic of the recursion, TO = type
@@ -64,33 +67,62 @@ __device__ __forceinline__ unsigned cell_mag_bits(const TO v[8])
     return isfinite(m) ? __float_as_uint(m) : 0u;
 }
 
+// Largest value over the 64 lanes of a wave.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, unsigned(__shfl_xor(int(v), o, 64)));
+    return v;
+}
+
+// Recursion over (frequency row, k tile) work items, one block per item.
+// Items come from `tiles` (row << kTileBits | k tile, ordered by row) when
+// given, otherwise they are all tiles of rows [w0, w0 + gridDim.x / tpr).  A
+// truncated grid launches only the tiles below each row's cut, so its cost
+// follows the cells it solves (launching empty blocks for the dropped tiles
+// makes workgroup dispatch the bound).
 // nk_w (optional): per-row number of wavenumbers solved (adaptive grid);
 // row_peak (optional): per-row max |A| as float bits.
 template <typename T, typename TO, bool COMPAT>
 __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__restrict__ lw_all, int L,
                                                            Geo g, int nk, double dk, double df,
-                                                           double omega_i, int w0, TO *__restrict__ A0,
-                                                           TO *__restrict__ A1, long lda,
+                                                           double omega_i, int w0, int tpr,
+                                                           const unsigned *__restrict__ tiles,
+                                                           TO *__restrict__ A0, TO *__restrict__ A1, long lda,
                                                            unsigned *__restrict__ flag_count,
                                                            unsigned *__restrict__ flag_list,
                                                            const int *__restrict__ nk_w,
                                                            unsigned *__restrict__ row_peak)
 {
     __shared__ LayerW<T> lw[kMaxLayers];
-    const int iw = w0 + blockIdx.y;  // frequency row
+    int iw, kb;
+    if (tiles) {
+        const unsigned e = tiles[blockIdx.x];
+        iw = int(e >> kTileBits);
+        kb = int(e & ((1u << kTileBits) - 1));
+    } else {
+        const int q = int(blockIdx.x) / tpr;
+        iw = w0 + q;
+        kb = int(blockIdx.x) - q * tpr;
+    }
     const int nkr = nk_w ? nk_w[iw] : nk;
-    if ((int)(blockIdx.x * blockDim.x) >= nkr) return;  // whole block past this row's cut
+    if (kb * kCellThreads >= nkr) return;  // uniform over the block
     for (int j = threadIdx.x; j < L; j += blockDim.x) lw[j] = lw_all[(size_t)iw * L + j];
     __syncthreads();
-    const int ik = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ik >= nkr) return;
-    TO v[8];
-    const bool ok = cell_zr<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
-    store_zr(v, iw, ik, A0, A1, lda);
-    if (row_peak) atomicMax(row_peak + iw, cell_mag_bits(v));
-    if (!ok && flag_count) {
-        const unsigned slot = atomicAdd(flag_count, 1u);
-        flag_list[slot] = unsigned(iw) * unsigned(nk) + unsigned(ik);
+    const int ik = kb * kCellThreads + threadIdx.x;
+    unsigned mag = 0;
+    if (ik < nkr) {
+        TO v[8];
+        const bool ok = cell_zr<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
+        store_zr(v, iw, ik, A0, A1, lda);
+        mag = cell_mag_bits(v);
+        if (!ok && flag_count) {
+            const unsigned slot = atomicAdd(flag_count, 1u);
+            flag_list[slot] = unsigned(iw) * unsigned(nk) + unsigned(ik);
+        }
+    }
+    if (row_peak) {
+        mag = wave_max_u32(mag);
+        if ((threadIdx.x & 63) == 0 && mag) atomicMax(row_peak + iw, mag);
     }
 }
 
@@ -111,13 +143,17 @@ __global__ __launch_bounds__(kCellThreads) void k_check_tail(const LayerW<T> *__
     for (int q = threadIdx.x; q < L; q += blockDim.x) lw[q] = lw_all[(size_t)iw * L + q];
     __syncthreads();
     const int ik = nk_w[iw] + stride * j + stride / 2;
-    if (j >= nc || ik >= nk) return;
-    float v[8];
-    cell_zr<T, float, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
-    float m = 0.f;
-    for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(v[i]));
-    if (!isfinite(m)) m = 3.0e38f;  // a non-finite check value fails the row
-    atomicMax(tail_max + iw, __float_as_uint(m));
+    unsigned mb = 0;
+    if (j < nc && ik < nk) {
+        float v[8];
+        cell_zr<T, float, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
+        float m = 0.f;
+        for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(v[i]));
+        if (!isfinite(m)) m = 3.0e38f;  // a non-finite check value fails the row
+        mb = __float_as_uint(m);
+    }
+    mb = wave_max_u32(mb);
+    if ((threadIdx.x & 63) == 0 && mb) atomicMax(tail_max + iw, mb);
 }
 
 // Rows whose checked tail exceeds tol * row peak fall back to the dense grid.
@@ -536,16 +572,26 @@ __global__ __launch_bounds__(256) void k_post_ifft(PostParams pp, int layout, co
 // Recursion over frequency rows [w0, w0 + nwr).  prec: 0 FP64 cells with FP64
 // A operands, 1 FP32 cells with FP32 A operands (flagged cells appended to the
 // list), 2 FP64 cells with FP32 A operands.  lw points at row 0.
-hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
-                           double df, double omega_i, int w0, int nwr, void *A0, void *A1, long lda,
-                           unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
-                           hipStream_t st)
+int cells_tiles_per_row(int nk) { return (nk + kCellThreads - 1) / kCellThreads; }
+
+unsigned cells_tile(int iw, int kb) { return (unsigned(iw) << kTileBits) | unsigned(kb); }
+
+int cells_tile_limit(int nw, int nk)
 {
-    if (nwr <= 0) return hipSuccess;
-    dim3 grid((nk + kCellThreads - 1) / kCellThreads, nwr);
-#define GRTM_ZR(T, TO, CP, FC, FL)                                                                     \
-    hipLaunchKernelGGL((k_cells_zr<T, TO, CP>), grid, dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, L, g, \
-                       nk, dk, df, omega_i, w0, (TO *)A0, (TO *)A1, lda, FC, FL, nk_w, row_peak)
+    return (nw < (1 << (32 - kTileBits)) && cells_tiles_per_row(nk) <= (1 << kTileBits)) ? 1 : 0;
+}
+
+hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
+                           double df, double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles, void *A0,
+                           void *A1, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
+                           unsigned *row_peak, hipStream_t st)
+{
+    const int tpr = cells_tiles_per_row(nk);
+    if (!tiles) ntiles = nwr * tpr;
+    if (ntiles <= 0) return hipSuccess;
+#define GRTM_ZR(T, TO, CP, FC, FL)                                                                           \
+    hipLaunchKernelGGL((k_cells_zr<T, TO, CP>), dim3(ntiles), dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, \
+                       L, g, nk, dk, df, omega_i, w0, tpr, tiles, (TO *)A0, (TO *)A1, lda, FC, FL, nk_w, row_peak)
     if (prec == 0) {
         if (compat) GRTM_ZR(double, double, true, nullptr, nullptr);
         else GRTM_ZR(double, double, false, nullptr, nullptr);

@@ -4,7 +4,7 @@ Training data come from the plain method itself: dense FP64 runs of seeded
 layered models (3-29 layers over a half-space, Vs rising 2.8 -> 4.6 km/s,
 source depth U(0.5, 40) km, surface receivers, dt 0.05 s, Nw = 1024,
 Nk = 8192), labelled per frequency row on the GPU (truncation index at
-1e-7 of the row peak, pole clusters).  The network is trained with its own
+1e-6 of the row peak, pole clusters).  The network is trained with its own
 GPU forward/backward kernels and saved to predictor_weights.npz.
 
   python -m grtm_cuda_amd.train_predictor --models 500 --rows 100

@@ -141,6 +141,10 @@ int grtm_selftest_gemm(int mode, int M, int N, int K, const double *A, const dou
 /* Lane-layout probe of the FP64 matrix-core instruction: lane l0 feeds a = 1,
  * lane l feeds b = 1000 + l; out[64*4] receives every lane's accumulators. */
 int grtm_selftest_mfma_probe(int l0, double *out);
+/* One plain device copy of n_int32 4-byte words (k_copy_i32): the known-size
+ * kernel that calibrates hardware byte counters against algorithmic bytes.
+ * No reference counterpart (measurement hook). */
+int grtm_selftest_copy(int n_int32);
 
 int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi, int n_layer,
                        const double *lay, int s_layer, double zs, int r_layer, double zr,
