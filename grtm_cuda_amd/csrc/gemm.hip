@@ -16,6 +16,9 @@
 //                 a b ~ a_hi b_hi + a_hi b_lo + a_lo b_hi, FP32 accumulation
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdint>
+
 #include "kernels.h"
 
 namespace grtm {
@@ -130,6 +133,149 @@ __global__ __launch_bounds__(256) void k_gemm_f32(int M, int N, int K, int kslic
             }
 }
 
+// ------------------------------------------------------------------ FP32, streamed
+// Two contractions C_p = A_p B_p (p = blockIdx.z / S) in one launch.  Block
+// tile 128 x 128, 4 waves of 64 x 64, K slab 32; the next slab's 16-byte
+// global loads are in flight while the matrix cores work on the current one
+// (two LDS buffers).  Within a slab, lanes 0-31 of every MFMA take k in
+// [0, 16) and lanes 32-63 take k in [16, 32), so a lane's A operands for four
+// steps are one 16-byte LDS read.  Split-K: slice z covers [z kc, (z+1) kc);
+// a slice at or past its row tile's extent kend returns without writing
+// (k_gemm_reduce_ragged sums only the live slices).  Requires lda, ldb
+// multiples of 4 and 16-byte aligned A, B.
+constexpr int kSBM = 128, kSBN = 128, kSBK = 32, kSPA = kSBK + 4, kSPB = kSBN + 4;
+
+struct GemmPair {
+    const float *A[2];
+    const float *B[2];
+    float *P[2];
+};
+
+__global__ __launch_bounds__(256, 2) void k_gemm_f32s(int M, int N, int K, int kc, int S, GemmPair gp, long lda,
+                                                      long ldb, const int *__restrict__ kend,
+                                                      const int *__restrict__ nk_w)
+{
+    __shared__ __attribute__((aligned(16))) float As[2][kSBM][kSPA];
+    __shared__ __attribute__((aligned(16))) float Bs[2][kSBK][kSPB];
+    const int pi = blockIdx.z / S, z = blockIdx.z - pi * S;
+    const float *__restrict__ A = gp.A[pi];
+    const float *__restrict__ B = gp.B[pi];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv >> 1, wn = wv & 1;
+    const int m0 = blockIdx.y * kSBM, n0 = blockIdx.x * kSBN;
+    const int kt = min(K, kend ? kend[blockIdx.y] : K);
+    const int kb = z * kc;
+    if (kb >= kt) return;  // uniform: this slice is past the tile's extent
+    const int ke = min(kt, kb + kc);
+
+    // staging: A items idx = tid + 256 e -> row idx >> 3, k offset (idx & 7) * 4;
+    // B items -> k row idx >> 5, column offset (idx & 31) * 4
+    int a_lim[4];
+    for (int e = 0; e < 4; e++) {
+        const int gm = m0 + ((tid + 256 * e) >> 3);
+        a_lim[e] = gm < M ? (nk_w ? min(ke, nk_w[gm >> 2]) : ke) : 0;
+    }
+    float4 ra[4], rb[4];
+    auto load = [&](int k0) {
+        for (int e = 0; e < 4; e++) {
+            const int idx = tid + 256 * e, gm = m0 + (idx >> 3), gk = k0 + (idx & 7) * 4;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (gk + 3 < a_lim[e]) {
+                v = *reinterpret_cast<const float4 *>(A + (size_t)gm * lda + gk);
+            } else if (gk < a_lim[e]) {
+                const float *r = A + (size_t)gm * lda;
+                v.x = r[gk];
+                if (gk + 1 < a_lim[e]) v.y = r[gk + 1];
+                if (gk + 2 < a_lim[e]) v.z = r[gk + 2];
+            }
+            ra[e] = v;
+        }
+        for (int e = 0; e < 4; e++) {
+            const int idx = tid + 256 * e, gk = k0 + (idx >> 5), gn = n0 + (idx & 31) * 4;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (gk < ke) {
+                const float *r = B + (size_t)gk * ldb;
+                if (gn + 3 < N) {
+                    v = *reinterpret_cast<const float4 *>(r + gn);
+                } else {
+                    if (gn < N) v.x = r[gn];
+                    if (gn + 1 < N) v.y = r[gn + 1];
+                    if (gn + 2 < N) v.z = r[gn + 2];
+                }
+            }
+            rb[e] = v;
+        }
+    };
+    auto store = [&](int buf) {
+        for (int e = 0; e < 4; e++) {
+            const int idx = tid + 256 * e;
+            *reinterpret_cast<float4 *>(&As[buf][idx >> 3][(idx & 7) * 4]) = ra[e];
+            *reinterpret_cast<float4 *>(&Bs[buf][idx >> 5][(idx & 31) * 4]) = rb[e];
+        }
+    };
+
+    v16f acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int r = 0; r < 16; r++) acc[a][b][r] = 0.f;
+    load(kb);
+    store(0);
+    __syncthreads();
+    const int i = lane & 31, kq = lane >> 5;
+    int buf = 0;
+    for (int k0 = kb; k0 < ke; k0 += kSBK) {
+        const bool more = k0 + kSBK < ke;
+        if (more) load(k0 + kSBK);
+        for (int q = 0; q < 4; q++) {
+            float4 a4[2];
+            for (int t = 0; t < 2; t++)
+                a4[t] = *reinterpret_cast<const float4 *>(&As[buf][wm * 64 + t * 32 + i][16 * kq + 4 * q]);
+            for (int j = 0; j < 4; j++) {
+                const int kk = 16 * kq + 4 * q + j;
+                float b[2];
+                for (int t = 0; t < 2; t++) b[t] = Bs[buf][kk][wn * 64 + t * 32 + i];
+                for (int tm = 0; tm < 2; tm++) {
+                    const float a = j == 0 ? a4[tm].x : j == 1 ? a4[tm].y : j == 2 ? a4[tm].z : a4[tm].w;
+                    for (int tn = 0; tn < 2; tn++)
+                        acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b[tn], acc[tm][tn], 0, 0, 0);
+                }
+            }
+        }
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    float *Pz = gp.P[pi] + (size_t)z * M * N;
+    for (int tm = 0; tm < 2; tm++)
+        for (int tn = 0; tn < 2; tn++)
+            for (int r = 0; r < 16; r++) {
+                const int row = m0 + wm * 64 + tm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int col = n0 + wn * 64 + tn * 32 + (lane & 31);
+                if (row < M && col < N) Pz[(size_t)row * N + col] = acc[tm][tn][r];
+            }
+}
+
+// Sum of the live split-K slices of the two contractions, in FP64, in slice
+// order: C_p[row] = sum_{z < ceil(kend[row / bm] / kc)} P_p[z][row].
+template <typename T>
+__global__ void k_gemm_reduce_ragged(int M, int N, int K, int kc, int bm, const int *__restrict__ kend,
+                                     const T *__restrict__ P0, const T *__restrict__ P1, T *__restrict__ C0,
+                                     T *__restrict__ C1)
+{
+    const long MN = (long)M * N;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * MN) return;
+    const int pi = t >= MN;
+    const long e = t - pi * MN;
+    const int row = int(e / N);
+    const int kt = min(K, kend ? kend[row / bm] : K);
+    const int nz = (kt + kc - 1) / kc;
+    const T *P = pi ? P1 : P0;
+    double s = 0;
+    for (int zz = 0; zz < nz; zz++) s += double(P[(size_t)zz * MN + e]);
+    (pi ? C1 : C0)[e] = T(s);
+}
+
 // ------------------------------------------------------------------ BF16x3
 __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo)
 {
@@ -239,6 +385,47 @@ static void split_plan(int M, int N, int K, int BM, int BN, int BK, int &S, int 
     S = 1;
     while (tiles * S < 1024 && S < 64 && (K / (S * 2)) >= 4 * BK) S *= 2;
     kslice = ((K + S - 1) / S + BK - 1) / BK * BK;
+}
+
+// Streamed FP32 pair: slice length from the mean K extent of the row tiles
+// (k_avg; the full K for a dense grid) so that about three blocks per compute
+// unit stay live; a power of two, at least 256 and at most K rounded up.
+int gemm_pair_slice(int M, int N, int K, double k_avg)
+{
+    const double tiles = 2.0 * ((M + kSBM - 1) / kSBM) * ((N + kSBN - 1) / kSBN);
+    const double want = tiles * std::max(1.0, k_avg) / 768.0;
+    int kc = 256;
+    while (kc * 2 <= want && kc < K) kc *= 2;
+    return kc;
+}
+
+size_t gemm_pair_workspace_bytes(int M, int N, int K, int kc)
+{
+    const size_t S = (size_t)(K + kc - 1) / kc;
+    return 2 * S * (size_t)M * N * sizeof(float);
+}
+
+bool gemm_pair_ok(long lda, long ldb, const void *A0, const void *A1, const void *B0, const void *B1)
+{
+    auto al = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
+    return lda % 4 == 0 && ldb % 4 == 0 && al(A0) && al(A1) && al(B0) && al(B1);
+}
+
+hipError_t launch_gemm_pair_f32(int M, int N, int K, int kc, const float *A0, const float *A1, long lda,
+                                const float *B0, const float *B1, long ldb, float *C0, float *C1, float *P,
+                                const int *kend, const int *nk_w, hipStream_t st)
+{
+    if (kc < kSBK || kc % kSBK) return hipErrorInvalidValue;
+    const int S = (K + kc - 1) / kc;
+    GemmPair gp;
+    gp.A[0] = A0, gp.A[1] = A1, gp.B[0] = B0, gp.B[1] = B1;
+    gp.P[0] = P, gp.P[1] = P + (size_t)S * M * N;
+    dim3 grid((N + kSBN - 1) / kSBN, (M + kSBM - 1) / kSBM, 2 * S);
+    hipLaunchKernelGGL(k_gemm_f32s, grid, dim3(256), 0, st, M, N, K, kc, S, gp, lda, ldb, kend, nk_w);
+    const long n2 = 2L * M * N;
+    hipLaunchKernelGGL(k_gemm_reduce_ragged<float>, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, M, N, K,
+                       kc, kSBM, kend, (const float *)gp.P[0], (const float *)gp.P[1], C0, C1);
+    return hipGetLastError();
 }
 
 size_t gemm_workspace_bytes(int mode, int M, int N, int K)

@@ -489,6 +489,8 @@ struct grtm_plan {
     bool own_stream = false;
     int rows64 = 0;  // FP32 plans: frequency rows [0, rows64) are solved in FP64
     int gemm_mode = 0;  // 0 FP64 MFMA, 1 FP32 MFMA, 2 BF16x3 MFMA
+    bool pair = false;  // FP32: both contractions in one streamed launch
+    int kc = 0;         // its split-K slice length
     DBuf lw, lw64, flags, dists, A0, A1, B0, B1, C0, C1, tr, ws;
     // adaptive wavenumber grid
     bool adaptive = false;
@@ -559,6 +561,12 @@ int run_gemm(grtm_plan *p, hipStream_t s)
     const int M = 4 * p->nw;
     const int *kend = p->adaptive ? p->kend.as<int>() : nullptr;
     const int *nkw = p->adaptive ? p->nk_w.as<int>() : nullptr;
+    if (p->pair) {
+        HIPCHK(grtm::launch_gemm_pair_f32(M, p->nr, p->nk, p->kc, p->A0.as<float>(), p->A1.as<float>(), p->nk,
+                                          p->B0.as<float>(), p->B1.as<float>(), p->nr, p->C0.as<float>(),
+                                          p->C1.as<float>(), p->ws.as<float>(), kend, nkw, s));
+        return GRTM_OK;
+    }
     HIPCHK(grtm::launch_gemm_mfma(p->gemm_mode, M, p->nr, p->nk, p->A0.p, p->nk, p->B0.p, p->nr, p->C0.p, p->ws.p,
                                   kend, nkw, s));
     HIPCHK(grtm::launch_gemm_mfma(p->gemm_mode, M, p->nr, p->nk, p->A1.p, p->nk, p->B1.p, p->nr, p->C1.p, p->ws.p,
@@ -692,7 +700,13 @@ int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, 
     if ((status = p->C0.alloc(arows * p->nr * es))) return status;
     if ((status = p->C1.alloc(arows * p->nr * es))) return status;
     if ((status = p->tr.alloc((size_t)4 * p->nr * p->N * es))) return status;
-    if ((status = p->ws.alloc(grtm::gemm_workspace_bytes(p->gemm_mode, (int)arows, p->nr, p->nk)))) return status;
+    size_t wsb = grtm::gemm_workspace_bytes(p->gemm_mode, (int)arows, p->nr, p->nk);
+    if (p->gemm_mode == 1 && grtm::gemm_pair_ok(p->nk, p->nr, p->A0.p, p->A1.p, p->B0.p, p->B1.p)) {
+        p->pair = true;
+        p->kc = grtm::gemm_pair_slice((int)arows, p->nr, p->nk, p->nk);
+        wsb = std::max(wsb, grtm::gemm_pair_workspace_bytes((int)arows, p->nr, p->nk, p->kc));
+    }
+    if ((status = p->ws.alloc(wsb))) return status;
     HIPCHK(hipMemcpy(p->dists.p, job->dists, sizeof(double) * p->nr, hipMemcpyHostToDevice));
 
     // Bessel tables (planning work: they depend on the grid and distances only)
@@ -808,6 +822,7 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
     if (!p) return GRTM_E_INPUT;
     if (!kcut) {
         p->adaptive = false;
+        if (p->pair) p->kc = grtm::gemm_pair_slice(4 * p->nw, p->nr, p->nk, p->nk);
         return GRTM_OK;
     }
     if (!(window >= 0) || stride < 1 || !(tol > 0)) return GRTM_E_INPUT;
@@ -848,6 +863,20 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
         HIPCHK(hipMemcpy(p->tiles.p, items.data(), sizeof(unsigned) * items.size(), hipMemcpyHostToDevice));
     p->ntiles = (int)items.size();
     p->ntiles64 = n64;
+    if (p->pair) {
+        // contraction slices from the mean K extent of the 32-row tiles
+        double ksum = 0;
+        int nt = 0;
+        for (int r0 = 0; r0 < p->nw; r0 += 32, nt++) {
+            int m = 0;
+            for (int r = r0; r < std::min(p->nw, r0 + 32); r++) m = std::max(m, p->nk_pred_h[r]);
+            ksum += std::min(p->nk, (m + 31) / 32 * 32);
+        }
+        const int kc = grtm::gemm_pair_slice(4 * p->nw, p->nr, p->nk, ksum / std::max(1, nt));
+        const size_t need = grtm::gemm_pair_workspace_bytes(4 * p->nw, p->nr, p->nk, kc);
+        if (need > p->ws.n && (st = p->ws.alloc(need))) return st;
+        p->kc = kc;
+    }
     p->max_chk = mx;
     p->chk_stride = stride;
     p->chk_tol = tol;
@@ -921,9 +950,36 @@ void grtm_plan_destroy(grtm_plan *p) { delete p; }
 // ------------------------------------------------------------ self-tests
 int grtm_selftest_gemm(int mode, int M, int N, int K, const double *A, const double *B, double *C)
 {
-    if (mode < -1 || mode > 2 || M < 1 || N < 1 || K < 1 || !A || !B || !C) return GRTM_E_INPUT;
+    if (mode < -1 || mode > 3 || M < 1 || N < 1 || K < 1 || !A || !B || !C) return GRTM_E_INPUT;
     int st;
     if ((st = device_ready())) return st;
+    if (mode == 3) {
+        // streamed FP32 pair kernel on (A, B) twice; padded leading dimensions
+        const long lda = (K + 3) / 4 * 4, ldb = (N + 3) / 4 * 4;
+        std::vector<float> a((size_t)M * lda, 0.f), b((size_t)K * ldb, 0.f);
+        for (int i = 0; i < M; i++)
+            for (int k = 0; k < K; k++) a[(size_t)i * lda + k] = (float)A[(size_t)i * K + k];
+        for (int k = 0; k < K; k++)
+            for (int j = 0; j < N; j++) b[(size_t)k * ldb + j] = (float)B[(size_t)k * N + j];
+        const int kc = grtm::gemm_pair_slice(M, N, K, K);
+        DBuf dA, dB, dC, dP;
+        if ((st = dA.alloc(a.size() * 4)) || (st = dB.alloc(b.size() * 4)) || (st = dC.alloc((size_t)2 * M * N * 4)) ||
+            (st = dP.alloc(grtm::gemm_pair_workspace_bytes(M, N, K, kc))))
+            return st;
+        HIPCHK(hipMemcpy(dA.p, a.data(), a.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dB.p, b.data(), b.size() * 4, hipMemcpyHostToDevice));
+        float *c0 = dC.as<float>(), *c1 = c0 + (size_t)M * N;
+        HIPCHK(grtm::launch_gemm_pair_f32(M, N, K, kc, dA.as<float>(), dA.as<float>(), lda, dB.as<float>(),
+                                          dB.as<float>(), ldb, c0, c1, dP.as<float>(), nullptr, nullptr, nullptr));
+        HIPCHK(hipDeviceSynchronize());
+        std::vector<float> f((size_t)2 * M * N);
+        HIPCHK(hipMemcpy(f.data(), dC.p, f.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < (size_t)M * N; i++) {
+            if (f[i] != f[(size_t)M * N + i]) return GRTM_E_DEVICE;  // the two halves must agree bit for bit
+            C[i] = f[i];
+        }
+        return GRTM_OK;
+    }
     const size_t es = (mode == -1 || mode == 0) ? 8 : 4;
     DBuf dA, dB, dC, dP;
     if ((st = dA.alloc((size_t)M * K * es)) || (st = dB.alloc((size_t)K * N * es)) || (st = dC.alloc((size_t)M * N * es)))

@@ -63,6 +63,15 @@ hipError_t launch_probe_f64(int l0, double *out, hipStream_t st);
 // nk_w (optional): A row r is zero at and beyond column nk_w[r / 4]
 hipError_t launch_gemm_mfma(int mode, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
                             void *Cm, void *P, const int *kend, const int *nk_w, hipStream_t st);
+// Both contractions C_p = A_p B_p (p = 0, 1) in FP32 on the matrix cores,
+// streamed kernel: K slices of kc (a multiple of 32; gemm_pair_slice picks
+// one), workspace P of gemm_pair_workspace_bytes, kend per 128-row tile.
+int gemm_pair_slice(int M, int N, int K, double k_avg);
+size_t gemm_pair_workspace_bytes(int M, int N, int K, int kc);
+bool gemm_pair_ok(long lda, long ldb, const void *A0, const void *A1, const void *B0, const void *B1);
+hipError_t launch_gemm_pair_f32(int M, int N, int K, int kc, const float *A0, const float *A1, long lda,
+                                const float *B0, const float *B1, long ldb, float *C0, float *C1, float *P,
+                                const int *kend, const int *nk_w, hipStream_t st);
 hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr, const void *raw, const void *C0,
                             const void *C1, long ldc, int nr, void *traces, void *spec_post, hipStream_t st);
 
