@@ -121,6 +121,10 @@ class TracePlan:
         return dict(cells=w[0], cell_layers=w[1], ksum_flop=w[2], ifft_bytes=w[3], traces=w[4],
                     flops_per_cell=w[5], trans_per_cell=w[6], rows_fp64=int(w[7]), cells_refixed=int(w[8]))
 
+    def set_rows64(self, rows64):
+        """Mixed-precision plans: solve frequency rows [0, rows64) in FP64."""
+        check(_lib().grtm_plan_set_rows64(self.ptr, int(rows64)), "grtm_plan_set_rows64")
+
     def set_adaptive(self, kcut, window=None, stride=8, tol=1e-5):
         """Use a truncated wavenumber grid: kcut[iw] (km^-1) per frequency row,
         residual check every `stride`-th wavenumber over `window` km^-1 beyond

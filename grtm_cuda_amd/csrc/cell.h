@@ -79,19 +79,23 @@ GRTM_HD cx<T> cexp_nz(cx<W> nu, W z)
 }
 
 template <typename T>
-GRTM_HD LQ<T> layer_q(const LayerW<T> &L, T k)
+GRTM_HD LQ<T> layer_q(const LayerW<T> &L, typename WideOf<T>::type kd)
 {
     LQ<T> q;
     // vertical wavenumbers sqrt((k - w/V)(k + w/V)) with Re >= 0
-    // (green.cu:184-194), in FP64; the factored form avoids cancellation
-    // near k = w/V
+    // (green.cu:184-194), in FP64 at the FP64 wavenumber (rounding k to FP32
+    // first would shift the abscissa by up to 6e-8 k, to which the phase of
+    // the kernel is sensitive near k = w/V); the factored form avoids
+    // cancellation near k = w/V
     using W = typename WideOf<T>::type;
-    const W kd = W(k);
+    const T k = T(kd);
     q.rd = csqrt((C<W>(kd, W(0)) - L.sad) * (C<W>(kd, W(0)) + L.sad));
     q.vd = csqrt((C<W>(kd, W(0)) - L.sbd) * (C<W>(kd, W(0)) + L.sbd));
     q.r = narrow<T>(q.rd);
     q.v = narrow<T>(q.vd);
-    q.x = C<T>(T(2) * k * k, 0) - L.gb;  // k^2 + v^2
+    // k^2 + v^2 = 2 k^2 - (w/beta)^2, formed in FP64: it cancels near
+    // k = w / (beta sqrt 2)
+    q.x = narrow<T>(C<W>(W(2) * kd * kd, W(0)) - L.sbd * L.sbd);
     q.gb = L.gb;
     q.mu = L.mu;
     q.sa = L.sa;
@@ -233,9 +237,10 @@ struct CellOut {
 // NEED_T: also form the d/dz responses.  par0_r/par0_v: vertical wavenumbers
 // of layer rl at the first cell (k = 0, w_0), used only in compat mode.
 template <typename T, bool COMPAT, bool NEED_T>
-GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, T k, cx<T> par0_r, cx<T> par0_v,
-                        CellOut<T> &out)
+GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::type kd, cx<T> par0_r,
+                        cx<T> par0_v, CellOut<T> &out)
 {
+    const T k = T(kd);
     const int L = g.L, s = g.s, rl = g.rl;
     using W = typename WideOf<T>::type;
     const W zs = W(g.zs), zr = W(g.zr);
@@ -249,8 +254,8 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, T k, cx<T> par0_r, cx
     Md = ident<T>();
     md0 = C<T>(1, 0);
     {
-        LQ<T> qd = layer_q(lw[L - 1], k);  // layer below the interface
-        LQ<T> qu = layer_q(lw[L - 2], k);
+        LQ<T> qd = layer_q(lw[L - 1], kd);  // layer below the interface
+        LQ<T> qu = layer_q(lw[L - 2], kd);
         RT<T> t = interface_rt(qu, qd, k);
         m2<T> Rg = t.rdu, Td = t.td;
         cx<T> rg0 = t.rdu0, td0 = t.td0;
@@ -284,7 +289,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, T k, cx<T> par0_r, cx
                         Md = TdJ;
                         md0 = td0J;
                     } else if (jj <= rl && jj >= s + 2) {
-                        LQ<T> q2 = layer_q(lw[jj - 2], k);
+                        LQ<T> q2 = layer_q(lw[jj - 2], kd);
                         cx<T> e0, e1;
                         lam(q2, lw[jj - 2].h, e0, e1);
                         Md = Md * dmd(e0, e1, TdJ, C<T>(1, 0), C<T>(1, 0));
@@ -299,7 +304,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, T k, cx<T> par0_r, cx
             cx<T> e0, e1;
             lam(qu, lw[lay + 1].h, e0, e1);
             qd = qu;
-            qu = layer_q(lw[lay], k);
+            qu = layer_q(lw[lay], kd);
             t = interface_rt(qu, qd, k);
             up_step(Rg, rg0, t, e0, e1, Td, td0);
             visit(lay + 1, Rg, rg0, Td, td0, e0, e1);
@@ -313,7 +318,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, T k, cx<T> par0_r, cx
     mu0 = C<T>(1, 0);
     LQ<T> qr;  // receiver layer quantities
     {
-        LQ<T> qu = layer_q(lw[0], k);
+        LQ<T> qu = layer_q(lw[0], kd);
         m2<T> Rg = free_surface(qu, k);
         cx<T> rg0 = C<T>(1, 0);
         if (rl == 0) qr = qu;
@@ -322,7 +327,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, T k, cx<T> par0_r, cx
         for (int lay = 0; lay < s; lay++) {
             cx<T> e0, e1;
             lam(qu, lw[lay].h, e0, e1);  // Lambda_lay
-            LQ<T> qd = layer_q(lw[lay + 1], k);
+            LQ<T> qd = layer_q(lw[lay + 1], kd);
             RT<T> t = interface_rt(qu, qd, k);
             m2<T> Tu;
             cx<T> tu0;
@@ -348,7 +353,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, T k, cx<T> par0_r, cx
         rud0_s = rg0;
         out.sq = qu;  // layer s
     }
-    if (rl > s) qr = layer_q(lw[rl], k);
+    if (rl > s) qr = layer_q(lw[rl], kd);
 
     // ---------------- source layer reverberation (green.cu:654-702)
     const LQ<T> &sq = out.sq;
@@ -429,7 +434,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, T k, cx<T> par0_r, cx
         }
         cx<T> ex0, ex1;
         if (COMPAT) {
-            LQ<T> qp = layer_q(lw[rl - 1], k);
+            LQ<T> qp = layer_q(lw[rl - 1], kd);
             lam(qp, lw[rl - 1].h, ex0, ex1);
         } else {
             lam(qr, hr_, ex0, ex1);

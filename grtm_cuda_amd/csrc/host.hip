@@ -817,6 +817,33 @@ int grtm_plan_timing(grtm_plan *p, int reps, double *ms)
     return GRTM_OK;
 }
 
+// Recursion work items of the predicted grid, rows solved in FP64 first.
+static int build_items(grtm_plan *p)
+{
+    const int r64 = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;
+    std::vector<unsigned> items;
+    int n64 = 0, st;
+    for (int iw = 0; iw < p->nw; iw++) {
+        if (iw == r64) n64 = (int)items.size();
+        const int nt = grtm::cells_tiles_per_row(p->nk_pred_h[iw]);
+        for (int kb = 0; kb < nt; kb++) items.push_back(grtm::cells_tile(iw, kb));
+    }
+    if (r64 >= p->nw) n64 = (int)items.size();
+    if ((st = p->tiles.alloc(sizeof(unsigned) * std::max<size_t>(1, items.size())))) return st;
+    if (!items.empty())
+        HIPCHK(hipMemcpy(p->tiles.p, items.data(), sizeof(unsigned) * items.size(), hipMemcpyHostToDevice));
+    p->ntiles = (int)items.size();
+    p->ntiles64 = n64;
+    return GRTM_OK;
+}
+
+int grtm_plan_set_rows64(grtm_plan *p, int rows64)
+{
+    if (!p || p->prec == GRTM_PREC_FP64 || rows64 < 0 || rows64 > p->nw) return GRTM_E_INPUT;
+    p->rows64 = rows64;
+    return p->adaptive ? build_items(p) : GRTM_OK;
+}
+
 int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int stride, double tol)
 {
     if (!p) return GRTM_E_INPUT;
@@ -849,20 +876,7 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
     }
     HIPCHK(hipMemcpy(p->nk_pred.p, p->nk_pred_h.data(), sizeof(int) * p->nw, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(p->nchk.p, p->nchk_h.data(), sizeof(int) * p->nw, hipMemcpyHostToDevice));
-    // recursion work items of the predicted grid, rows solved in FP64 first
-    const int r64 = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;
-    std::vector<unsigned> items;
-    int n64 = 0;
-    for (int iw = 0; iw < p->nw; iw++) {
-        const int nt = grtm::cells_tiles_per_row(p->nk_pred_h[iw]);
-        for (int kb = 0; kb < nt; kb++) items.push_back(grtm::cells_tile(iw, kb));
-        if (iw + 1 == r64) n64 = (int)items.size();
-    }
-    if ((st = p->tiles.alloc(sizeof(unsigned) * std::max<size_t>(1, items.size())))) return st;
-    if (!items.empty())
-        HIPCHK(hipMemcpy(p->tiles.p, items.data(), sizeof(unsigned) * items.size(), hipMemcpyHostToDevice));
-    p->ntiles = (int)items.size();
-    p->ntiles64 = n64;
+    if ((st = build_items(p))) return st;
     if (p->pair) {
         // contraction slices from the mean K extent of the 32-row tiles
         double ksum = 0;

@@ -31,10 +31,11 @@ template <typename T, typename TO, bool COMPAT>
 __device__ __forceinline__ bool cell_zr(const LayerW<T> *lw, const Geo &g, int iwg, int ik, double dk, double df,
                                         double omega_i, TO v[8])
 {
-    const T k = T(dk * ik);
+    const double kd = dk * ik;
+    const T k = T(kd);
     const cx<T> w = C<T>(T(2.0 * M_PI * df * iwg), T(-omega_i));
     CellOut<T> c;
-    cell_solve<T, COMPAT, false>(lw, g, k, C<T>(0, 0), C<T>(0, 0), c);
+    cell_solve<T, COMPAT, false>(lw, g, kd, C<T>(0, 0), C<T>(0, 0), c);
     cx<T> kze, kre, kzv, krv;
     zr_from_cell<T, COMPAT>(c, g.up, k, w, kze, kre, kzv, krv);
     const T wt = T(ik == 0 ? 0.5 * dk : dk);

@@ -75,6 +75,7 @@ def _lib():
         lib.grtm_selftest_mfma_probe.argtypes = [I, P]
         lib.grtm_selftest_copy.argtypes = [I]
         lib.grtm_plan_set_adaptive.argtypes = [P, P, D, I, D]
+        lib.grtm_plan_set_rows64.argtypes = [P, I]
         lib.grtm_plan_adaptive_stats.argtypes = [P, P]
         lib.grtm_plan_row_labels.argtypes = [P, D, P]
         lib.grtm_mlp_create.argtypes = [I, I, I, I, I, P]
@@ -99,7 +100,7 @@ EXPORTS = [
     "grtm_create", "grtm_destroy", "grtm_plan_traces", "grtm_plan_execute", "grtm_plan_sync",
     "grtm_plan_traces_out", "grtm_plan_kernels_out", "grtm_plan_timing", "grtm_plan_work",
     "grtm_plan_destroy", "grtm_selftest_cell", "grtm_opcount_cell", "grtm_selftest_gemm",
-    "grtm_selftest_mfma_probe", "grtm_selftest_copy", "grtm_plan_set_adaptive", "grtm_plan_adaptive_stats",
+    "grtm_selftest_mfma_probe", "grtm_selftest_copy", "grtm_plan_set_adaptive", "grtm_plan_set_rows64", "grtm_plan_adaptive_stats",
     "grtm_plan_row_labels", "grtm_mlp_create", "grtm_mlp_destroy", "grtm_mlp_nparams", "grtm_mlp_set_params",
     "grtm_mlp_get_params", "grtm_mlp_set_data", "grtm_mlp_train", "grtm_mlp_grads", "grtm_mlp_predict",
     "grtm_mlp_set_loss",

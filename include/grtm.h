@@ -118,6 +118,13 @@ void grtm_plan_destroy(grtm_plan *plan);
  * km^-1 beyond the cut; a row whose checked tail exceeds tol times the row's
  * peak is completed on the dense grid in FP64 (fallback).  kcut = NULL
  * returns the plan to the dense grid. */
+/* Mixed-precision plans: solve frequency rows [0, rows64) of the recursion in
+ * FP64 (the rest in FP32).  The plan's default is the rows with
+ * |w| < 2.5 Vs_min / |z_s - z_r| (at least 4); rows64 = nw solves every row
+ * in FP64 and keeps FP32 operands for the contraction.  No reference
+ * counterpart (precision control of the new path). */
+int grtm_plan_set_rows64(grtm_plan *plan, int rows64);
+
 int grtm_plan_set_adaptive(grtm_plan *plan, const double *kcut, double window, int stride, double tol);
 /* Training labels of every frequency row after a dense execution: out[26*iw
  * + 0] peak |kernel|, [1] 1 + last wavenumber index whose |kernel| exceeds

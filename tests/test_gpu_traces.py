@@ -71,6 +71,31 @@ def test_c1_mixed_precision_traces(c1_oracle, prec):
     assert worst < 1e-4
 
 
+def test_c1_rows64_all_fp64_recursion(c1_oracle):
+    """set_rows64(nw): every row's recursion in FP64, FP32 contraction; the
+    error is then that of the FP32 contraction and FFT alone, and smaller than
+    the mixed recursion's.  Out-of-range row counts are rejected."""
+    cfg, ref = c1_oracle
+    nw = cfg["grid"]["nw"]
+    print("gpu: c1 rows64 = nw", flush=True)
+    mk = lambda: g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
+                             precision=g.PREC_FP32)
+    mixed, full = mk(), mk()
+    full.set_rows64(nw)
+    assert full.work()["rows_fp64"] == nw
+    e = []
+    for plan in (mixed, full):
+        tr = plan.traces()
+        e.append(max(rel_l2(tr[q].transpose(1, 0, 2), ref[(False, q)]).max() for q in (0, 1)))
+    print("rel-L2 mixed", e[0], "FP64 recursion", e[1])
+    assert e[1] < 1e-5 and e[1] <= e[0]
+    for bad in (-1, nw + 1):
+        with pytest.raises(g.GrtmError):
+            full.set_rows64(bad)
+    with pytest.raises(g.GrtmError):
+        g.TracePlan(cfg["layers"], cfg["src_depth"], 0.0, cfg["dists"], cfg["grid"]).set_rows64(4)
+
+
 def test_plan_rejects_bad_grid():
     cfg = models.config("c1")
     bad = dict(cfg["grid"], nw=500)
