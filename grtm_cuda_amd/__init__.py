@@ -8,6 +8,8 @@ Host-side mirror of the reference's interface (cangyeone/grtm_cuda):
 * ``green_mem(iname)`` -- the same run returning arrays.
 * ``TracePlan`` -- one model, many receiver distances, explosion and
   vertical-force Z/R traces from one recursion (the production path).
+* ``BatchPlan`` -- many models (uneven layer counts, source depths) on one
+  grid, replayed from one captured HIP graph.
 
 All compute runs in libgrtm.so (HIP).  There is no CPU fallback.
 """
@@ -16,10 +18,10 @@ import ctypes
 import numpy as np
 
 from . import inputfile, models  # noqa: F401
-from .ffi import (GRTM_OK, PREC_BF16X3, PREC_FP32, PREC_FP64, EXPORTS, GrtmError, Grid, Job, D, P,  # noqa: F401
+from .ffi import (GRTM_OK, PREC_BF16X3, PREC_FP32, PREC_FP64, EXPORTS, GrtmError, Grid, Job, Model, D, P,  # noqa: F401
                    _lib, check)
 
-__all__ = ["green", "green_mem", "TracePlan", "GrtmError", "PREC_FP64", "PREC_FP32", "PREC_BF16X3", "models",
+__all__ = ["green", "green_mem", "TracePlan", "BatchPlan", "GrtmError", "PREC_FP64", "PREC_FP32", "PREC_BF16X3", "models",
            "inputfile"]
 
 
@@ -170,6 +172,69 @@ def selftest_cell(compat, src_type, k, w, lay, s_layer, zs, r_layer, zr, par0_rv
                                     float(zr), None if p0 is None else p0.ctypes.data_as(P),
                                     out.ctypes.data_as(P)), "grtm_selftest_cell")
     return out[0::2] + 1j * out[1::2]
+
+
+class BatchPlan:
+    """Explosion and vertical-force Z/R traces of many layered models on one
+    grid and one set of surface-receiver distances.  ``models``: list of
+    (layers, src_depth) with layers rows (thickness km, vp, vs, rho, qp, qs).
+    The launch sequence is captured once in a HIP graph; ``autotune`` times the
+    recursion tile and contraction slice candidates on the first group."""
+
+    def __init__(self, models, rcv_depth, dists, grid, precision=PREC_FP32, compat=False, autotune=True, device=0):
+        self._ctx = _ctx(device)
+        self._layers = [np.ascontiguousarray(l, dtype=np.float64) for l, _ in models]
+        ms = (Model * len(models))()
+        for i, (l, (_, zs)) in enumerate(zip(self._layers, models)):
+            ms[i] = Model(l.shape[0], l.ctypes.data_as(P), float(zs), float(rcv_depth))
+        self.dists = np.ascontiguousarray(dists, dtype=np.float64)
+        self.nm, self.nr = len(models), len(self.dists)
+        self.nw = int(grid["nw"])
+        self.N = 2 * self.nw
+        g = Grid(self.nw, float(grid["dt"]), int(grid["nk"]), float(grid["kmax"]), float(grid.get("omega_i", 0.0)),
+                 float(grid.get("t0", 0.0)), float(grid.get("taper", 0.0)))
+        self.ptr = ctypes.c_void_p()
+        check(_lib().grtm_batch_create(self._ctx.ptr, ms, self.nm, self.dists.ctypes.data_as(P), self.nr,
+                                       ctypes.byref(g), int(precision), int(bool(compat)), 1 if autotune else 0,
+                                       ctypes.byref(self.ptr)), "grtm_batch_create")
+
+    def execute(self):
+        check(_lib().grtm_batch_execute(self.ptr, None), "grtm_batch_execute")
+
+    def execute_eager(self):
+        check(_lib().grtm_batch_execute_eager(self.ptr), "grtm_batch_execute_eager")
+
+    def sync(self):
+        check(_lib().grtm_batch_sync(self.ptr), "grtm_batch_sync")
+
+    def traces(self, m0=0, m1=None):
+        """traces[m1 - m0, 2 (explosion, vertical force), 2 (Z, R), nr, N] of
+        the last execution."""
+        m1 = self.nm if m1 is None else m1
+        out = np.zeros((m1 - m0, 2, 2, self.nr, self.N))
+        check(_lib().grtm_batch_traces_out(self.ptr, int(m0), int(m1), out.ctypes.data_as(P)), "grtm_batch_traces_out")
+        return out
+
+    def timing(self, reps=3):
+        ms = np.zeros(1)
+        check(_lib().grtm_batch_timing(self.ptr, int(reps), ms.ctypes.data_as(P)), "grtm_batch_timing")
+        return float(ms[0])
+
+    def config(self):
+        o = np.zeros(7)
+        check(_lib().grtm_batch_config(self.ptr, o.ctypes.data_as(P)), "grtm_batch_config")
+        return dict(models_per_group=int(o[0]), tile=int(o[1]), k_slice=int(o[2]), groups=int(o[3]),
+                    graph_nodes=int(o[4]), tuned=int(o[5]), tune_ms=float(o[6]))
+
+    def work(self):
+        o = np.zeros(5)
+        check(_lib().grtm_batch_work(self.ptr, o.ctypes.data_as(P)), "grtm_batch_work")
+        return dict(cells=o[0], cell_layers=o[1], ksum_flop=o[2], traces=o[3], rec_flop=o[4])
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            _lib().grtm_batch_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
 
 
 def selftest_gemm(mode, A, B):

@@ -88,7 +88,6 @@ GRTM_HD LQ<T> layer_q(const LayerW<T> &L, typename WideOf<T>::type kd)
     // the kernel is sensitive near k = w/V); the factored form avoids
     // cancellation near k = w/V
     using W = typename WideOf<T>::type;
-    const T k = T(kd);
     q.rd = csqrt((C<W>(kd, W(0)) - L.sad) * (C<W>(kd, W(0)) + L.sad));
     q.vd = csqrt((C<W>(kd, W(0)) - L.sbd) * (C<W>(kd, W(0)) + L.sbd));
     q.r = narrow<T>(q.rd);

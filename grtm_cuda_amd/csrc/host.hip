@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -473,6 +474,88 @@ void green(char *iname, char *oname_w, char *oname_t) { g_last_status = grtm_gre
 
 }  // extern "C"
 
+namespace {
+
+// ------------------------------------------------------------ model set-up
+// One model of a plan or batch: layer stack, geometry, per-frequency layer
+// tables [nw x L] and (mixed precision) the rows solved in FP64.
+struct ModelTables {
+    Stack st;
+    Geo g{};
+    int rows64 = 0;
+    std::vector<LayerW<double>> lw;
+};
+
+int model_tables(const double *layers, int n, double src_depth, double rcv_depth, int compat, int nw, double df,
+                 double omega_i, bool mixed, ModelTables &mt)
+{
+    if (n < 2 || n + 1 > grtm::kMaxLayers || !layers) return GRTM_E_INPUT;
+    std::vector<double> top(n), vp(n), vs(n), rho(n), qp(n), qs(n);
+    double z = 0;
+    for (int i = 0; i < n; i++) {
+        const double *r = layers + 6 * i;
+        top[i] = z;
+        if (i < n - 1 && !(r[0] > 0)) return GRTM_E_INPUT;
+        z += r[0];
+        vp[i] = r[1], vs[i] = r[2], rho[i] = r[3], qp[i] = r[4], qs[i] = r[5];
+    }
+    int status = prepare_stack(compat, n, top.data(), vp.data(), vs.data(), rho.data(), qp.data(), qs.data(),
+                               src_depth, rcv_depth, mt.st);
+    if (status) return status;
+    if ((status = make_geo(mt.st, compat, mt.g))) return status;
+    const int L = mt.st.L;
+    mt.lw.resize((size_t)nw * L);
+    for (int iw = 0; iw < nw; iw++) layer_w(mt.st, compat, cd(2 * M_PI * df * iw, -omega_i), &mt.lw[(size_t)iw * L]);
+    if (mixed) {
+        // Low frequencies are solved in FP64: as w -> 0 the P and S columns of
+        // the eigenvector basis become parallel and the recursion loses
+        // ~(k beta / |w|)^2 of relative accuracy, with the integrand significant
+        // up to k ~ |w| / beta + O(1 / dz).  Rows with |w| < 2.5 beta_min / dz.
+        double bmin = 1e30;
+        for (int j = 0; j < L; j++) bmin = std::min(bmin, mt.st.b0[j]);
+        const double dz = std::max(std::fabs(src_depth - rcv_depth), 1e-3);
+        const double wc = 2.5 * bmin / dz;
+        int r64 = 0;
+        while (r64 < nw && std::hypot(2 * M_PI * df * r64, omega_i) < wc) r64++;
+        mt.rows64 = std::min(nw, std::max(r64, 4));
+    }
+    return GRTM_OK;
+}
+
+std::vector<LayerW<float>> narrow_tables(const std::vector<LayerW<double>> &lw)
+{
+    std::vector<LayerW<float>> lf(lw.size());
+    auto F = [](grtm::cx<double> z) { return grtm::cx<float>{float(z.x), float(z.y)}; };
+    for (size_t i = 0; i < lw.size(); i++) {
+        const LayerW<double> &d = lw[i];
+        lf[i] = LayerW<float>{d.sad, d.sbd, F(d.sa), F(d.sb), F(d.ga), F(d.gb), F(d.ca), F(d.cb), F(d.mu),
+                              F(d.a), F(d.b), d.h, 0.0};
+    }
+    return lf;
+}
+
+grtm::PostParams post_params(const grtm_grid *grid, int nw, double df, double omega_i)
+{
+    grtm::PostParams pp;
+    pp.nw = nw;
+    pp.wc = (int)((double)nw * (1 - grid->taper));
+    pp.df = df;
+    pp.dt = grid->dt;
+    pp.t0 = grid->t0;
+    pp.omega_i = omega_i;
+    return pp;
+}
+
+bool grid_ok(const grtm_grid *grid, int precision)
+{
+    if (grid->nw < 4 || (grid->nw & (grid->nw - 1)) || grid->nw > 8192 || grid->nk < 1 || !(grid->kmax > 0) ||
+        !(grid->dt > 0) || !(grid->taper >= 0 && grid->taper <= 1))
+        return false;
+    return true;
+}
+
+}  // namespace
+
 // ============================================================ trace plans
 struct grtm_ctx {
     int device = 0;
@@ -617,77 +700,47 @@ int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, 
     for (int i = 0; i < job->nr; i++)
         if (!(job->dists[i] > 0)) return GRTM_E_INPUT;
     HIPCHK(hipSetDevice(ctx->device));
-    const int n = job->n_layers;
-    std::vector<double> top(n), vp(n), vs(n), rho(n), qp(n), qs(n);
-    double z = 0;
-    for (int i = 0; i < n; i++) {
-        const double *r = job->layers + 6 * i;
-        top[i] = z;
-        if (i < n - 1 && !(r[0] > 0)) return GRTM_E_INPUT;
-        z += r[0];
-        vp[i] = r[1], vs[i] = r[2], rho[i] = r[3], qp[i] = r[4], qs[i] = r[5];
-    }
-    Stack st;
-    int status = prepare_stack(compat, n, top.data(), vp.data(), vs.data(), rho.data(), qp.data(), qs.data(),
-                               job->src_depth, job->rcv_depth, st);
+    const int nw = grid->nw;
+    const double T = 2.0 * nw * grid->dt;
+    const double df = 1.0 / T, omega_i = grid->omega_i > 0 ? grid->omega_i : M_PI / T;
+    ModelTables mt;
+    int status = model_tables(job->layers, job->n_layers, job->src_depth, job->rcv_depth, compat, nw, df, omega_i,
+                              precision != GRTM_PREC_FP64, mt);
     if (status) return status;
     grtm_plan *p = new grtm_plan;
     struct Guard {
         grtm_plan *&p;
         ~Guard() { delete p; }
     } guard{p};
-    if ((status = make_geo(st, compat, p->g))) return status;
+    p->g = mt.g;
     p->ctx = ctx;
     p->prec = precision;
     p->gemm_mode = precision == GRTM_PREC_FP64 ? 0 : precision == GRTM_PREC_FP32 ? 1 : 2;
     p->compat = compat;
-    p->nw = grid->nw;
-    p->N = 2 * grid->nw;
+    p->nw = nw;
+    p->N = 2 * nw;
     p->nk = grid->nk;
     p->nr = job->nr;
-    p->L = st.L;
+    p->L = mt.st.L;
     p->dk = grid->kmax / grid->nk;
-    const double T = p->N * grid->dt;
-    p->df = 1.0 / T;
-    p->omega_i = grid->omega_i > 0 ? grid->omega_i : M_PI / T;
-    p->pp.nw = p->nw;
-    p->pp.wc = (int)((double)p->nw * (1 - grid->taper));
-    p->pp.df = p->df;
-    p->pp.dt = grid->dt;
-    p->pp.t0 = grid->t0;
-    p->pp.omega_i = p->omega_i;
+    p->df = df;
+    p->omega_i = omega_i;
+    p->pp = post_params(grid, nw, df, omega_i);
     HIPCHK(hipStreamCreate(&p->stream));
     p->own_stream = true;
 
     // layer tables
-    std::vector<LayerW<double>> lw((size_t)p->nw * st.L);
-    for (int iw = 0; iw < p->nw; iw++) layer_w(st, compat, cd(2 * M_PI * p->df * iw, -p->omega_i), &lw[(size_t)iw * st.L]);
+    const std::vector<LayerW<double>> &lw = mt.lw;
     const size_t es = esize(precision);
     if (precision == 0) {
         if ((status = p->lw.alloc(lw.size() * sizeof(LayerW<double>)))) return status;
         HIPCHK(hipMemcpy(p->lw.p, lw.data(), lw.size() * sizeof(LayerW<double>), hipMemcpyHostToDevice));
     } else {
-        // Low frequencies are solved in FP64: as w -> 0 the P and S columns of
-        // the eigenvector basis become parallel and the recursion loses
-        // ~(k beta / |w|)^2 of relative accuracy, with the integrand significant
-        // up to k ~ |w| / beta + O(1 / dz).  Rows with |w| < 2.5 beta_min / dz.
-        double bmin = 1e30;
-        for (int j = 0; j < st.L; j++) bmin = std::min(bmin, st.b0[j]);
-        const double dz = std::max(std::fabs(job->src_depth - job->rcv_depth), 1e-3);
-        const double wc = 2.5 * bmin / dz;
-        int r64 = 0;
-        while (r64 < p->nw && std::hypot(2 * M_PI * p->df * r64, p->omega_i) < wc) r64++;
-        p->rows64 = std::min(p->nw, std::max(r64, 4));
+        p->rows64 = mt.rows64;
         if ((status = p->lw64.alloc(lw.size() * sizeof(LayerW<double>)))) return status;
         HIPCHK(hipMemcpy(p->lw64.p, lw.data(), lw.size() * sizeof(LayerW<double>), hipMemcpyHostToDevice));
         if ((status = p->flags.alloc(sizeof(unsigned) * ((size_t)p->nw * p->nk + 1)))) return status;
-        std::vector<LayerW<float>> lf(lw.size());
-        auto F = [](grtm::cx<double> z) { return grtm::cx<float>{float(z.x), float(z.y)}; };
-        for (size_t i = 0; i < lw.size(); i++) {
-            const LayerW<double> &d = lw[i];
-            lf[i] = LayerW<float>{d.sad, d.sbd, F(d.sa), F(d.sb), F(d.ga), F(d.gb), F(d.ca), F(d.cb), F(d.mu),
-                                  F(d.a), F(d.b), d.h, 0.0};
-        }
+        const std::vector<LayerW<float>> lf = narrow_tables(lw);
         if ((status = p->lw.alloc(lf.size() * sizeof(LayerW<float>)))) return status;
         HIPCHK(hipMemcpy(p->lw.p, lf.data(), lf.size() * sizeof(LayerW<float>), hipMemcpyHostToDevice));
     }
@@ -1093,5 +1146,402 @@ int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi,
     for (int i = 0; i < 18; i++) out[2 * i] = v[i].x, out[2 * i + 1] = v[i].y;
     return GRTM_OK;
 }
+
+}  // extern "C"
+
+// ============================================================ batches
+struct grtm_batch {
+    grtm_ctx *ctx = nullptr;
+    int prec = 0, compat = 0, gemm_mode = 0;
+    int nm = 0, nw = 0, N = 0, nk = 0, nr = 0;
+    double dk = 0, df = 0, omega_i = 0;
+    grtm::PostParams pp{};
+    hipStream_t stream = nullptr;
+    std::vector<int> L, rows64;  // per model
+    double flops = 0;            // recursion flops of one execution (operation count)
+    std::vector<grtm::ModelDesc> md_h;
+    DBuf md, lw64, lw32, dists, B0, B1, A0, A1, C0, C1, ws, flags, tr, items;
+    int nb = 0;       // models per group
+    int bs = 256;     // recursion tile: wavenumbers per block
+    int kc = 0;       // streamed FP32 contraction: K slice
+    bool pair = false;
+    int tuned = 0;    // autotune candidates timed
+    double tune_ms = 0;
+    struct Group {
+        int m0, n;
+        size_t off64, n64, off32, n32;  // item ranges: FP64 rows, FP32 rows
+    };
+    std::vector<Group> groups;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    size_t nodes = 0;
+    ~grtm_batch()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+bool batch_mixed(const grtm_batch *b) { return b->prec != GRTM_PREC_FP64; }
+
+// Work items of every group for recursion tiles of bs wavenumbers; within a
+// group the models are taken in decreasing layer count.  Only groups
+// [g0, g1) are built (the autotuner times group 0 alone).
+int batch_items(grtm_batch *b, int bs, int g0, int g1)
+{
+    const int tpr = (b->nk + bs - 1) / bs;
+    if (tpr > 4096) return GRTM_E_UNSUPPORTED;
+    std::vector<unsigned long long> it;
+    for (int gi = g0; gi < g1; gi++) {
+        grtm_batch::Group &g = b->groups[gi];
+        std::vector<int> order(g.n);
+        for (int j = 0; j < g.n; j++) order[j] = j;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return b->L[g.m0 + x] > b->L[g.m0 + y]; });
+        for (int pass = 0; pass < 2; pass++) {
+            const size_t start = it.size();
+            for (int j : order) {
+                const int r64 = batch_mixed(b) ? b->rows64[g.m0 + j] : b->nw;
+                const int lo = pass == 0 ? 0 : r64, hi = pass == 0 ? r64 : b->nw;
+                for (int iw = lo; iw < hi; iw++)
+                    for (int kb = 0; kb < tpr; kb++)
+                        it.push_back(((unsigned long long)j << 32) | grtm::cells_tile(iw, kb));
+            }
+            if (pass == 0) g.off64 = start, g.n64 = it.size() - start;
+            else g.off32 = start, g.n32 = it.size() - start;
+        }
+    }
+    int st;
+    if ((st = b->items.alloc(sizeof(unsigned long long) * std::max<size_t>(1, it.size())))) return st;
+    if (!it.empty()) HIPCHK(hipMemcpy(b->items.p, it.data(), sizeof(unsigned long long) * it.size(), hipMemcpyHostToDevice));
+    b->bs = bs;
+    return GRTM_OK;
+}
+
+int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
+{
+    const unsigned long long *it = b->items.as<unsigned long long>();
+    const grtm::ModelDesc *md = b->md.as<grtm::ModelDesc>();
+    if (!batch_mixed(b)) {
+        HIPCHK(grtm::launch_cells_zr_batch(0, b->compat, b->bs, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df,
+                                           b->omega_i, it + g.off64, (int)g.n64, b->A0.p, b->A1.p, b->nk, nullptr,
+                                           nullptr, s));
+        return GRTM_OK;
+    }
+    unsigned *cnt = b->flags.as<unsigned>(), *list = cnt + 1;
+    HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
+    HIPCHK(grtm::launch_cells_zr_batch(2, b->compat, b->bs, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df,
+                                       b->omega_i, it + g.off64, (int)g.n64, b->A0.p, b->A1.p, b->nk, nullptr,
+                                       nullptr, s));
+    HIPCHK(grtm::launch_cells_zr_batch(1, b->compat, b->bs, md, g.m0, b->lw32.p, b->nw, b->nk, b->dk, b->df,
+                                       b->omega_i, it + g.off32, (int)g.n32, b->A0.p, b->A1.p, b->nk, cnt, list, s));
+    HIPCHK(grtm::launch_cells_fix_batch(b->compat, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df, b->omega_i,
+                                        b->A0.p, b->A1.p, b->nk, cnt, list, s));
+    return GRTM_OK;
+}
+
+int batch_gemm(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
+{
+    const int M = 4 * b->nw * g.n;
+    if (b->pair) {
+        HIPCHK(grtm::launch_gemm_pair_f32(M, b->nr, b->nk, b->kc, b->A0.as<float>(), b->A1.as<float>(), b->nk,
+                                          b->B0.as<float>(), b->B1.as<float>(), b->nr, b->C0.as<float>(),
+                                          b->C1.as<float>(), b->ws.as<float>(), nullptr, nullptr, s));
+        return GRTM_OK;
+    }
+    HIPCHK(grtm::launch_gemm_mfma(b->gemm_mode, M, b->nr, b->nk, b->A0.p, b->nk, b->B0.p, b->nr, b->C0.p, b->ws.p,
+                                  nullptr, nullptr, s));
+    HIPCHK(grtm::launch_gemm_mfma(b->gemm_mode, M, b->nr, b->nk, b->A1.p, b->nk, b->B1.p, b->nr, b->C1.p, b->ws.p,
+                                  nullptr, nullptr, s));
+    return GRTM_OK;
+}
+
+int batch_group(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
+{
+    int st;
+    if ((st = batch_cells(b, g, s)) || (st = batch_gemm(b, g, s))) return st;
+    const size_t es = esize(b->prec);
+    unsigned char *out = (unsigned char *)b->tr.p + (size_t)g.m0 * 4 * b->nr * b->N * es;
+    HIPCHK(grtm::launch_post_ifft(batch_mixed(b) ? 1 : 0, b->pp, 1, 4 * b->nr * g.n, nullptr, b->C0.p, b->C1.p,
+                                  b->nr, b->nr, out, nullptr, s, (long)4 * b->nw * b->nr));
+    return GRTM_OK;
+}
+
+int batch_all(grtm_batch *b, hipStream_t s)
+{
+    int st;
+    for (const grtm_batch::Group &g : b->groups)
+        if ((st = batch_group(b, g, s))) return st;
+    return GRTM_OK;
+}
+
+// mean ms of fn() over reps launches on the batch's stream (one warm-up)
+template <typename F>
+int time_ms(grtm_batch *b, int reps, F fn, double &ms)
+{
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int st = fn();
+    if (!st) {
+        HIPCHK(hipEventRecord(e0, b->stream));
+        for (int r = 0; r < reps && !st; r++) st = fn();
+        HIPCHK(hipEventRecord(e1, b->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float f = 0;
+        HIPCHK(hipEventElapsedTime(&f, e0, e1));
+        ms = f / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+}
+
+// Autotuner: recursion tile and contraction slice, timed on group 0.
+int batch_autotune(grtm_batch *b)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    int st, best_bs = 256;
+    double best = 1e30;
+    for (int bs : {64, 128, 256}) {
+        if ((st = batch_items(b, bs, 0, 1))) return st;
+        double ms = 0;
+        if ((st = time_ms(b, 2, [&] { return batch_cells(b, b->groups[0], b->stream); }, ms))) return st;
+        b->tuned++;
+        if (ms < best) best = ms, best_bs = bs;
+    }
+    b->bs = best_bs;
+    if (b->pair) {
+        const int M = 4 * b->nw * b->groups[0].n;
+        const int k0 = grtm::gemm_pair_slice(M, b->nr, b->nk, b->nk);
+        int best_kc = k0;
+        best = 1e30;
+        for (int kc : {k0 / 2, k0, 2 * k0}) {
+            if (kc < 256 || kc % 32 || (kc > b->nk && kc / 2 >= b->nk)) continue;
+            if (grtm::gemm_pair_workspace_bytes(M, b->nr, b->nk, kc) > b->ws.n) continue;
+            b->kc = kc;
+            double ms = 0;
+            if ((st = time_ms(b, 2, [&] { return batch_gemm(b, b->groups[0], b->stream); }, ms))) return st;
+            b->tuned++;
+            if (ms < best) best = ms, best_kc = kc;
+        }
+        b->kc = best_kc;
+    }
+    b->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return GRTM_OK;
+}
+
+int batch_capture(grtm_batch *b)
+{
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int st = batch_all(b, b->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    HIPCHK(e);
+    b->graph = g;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    size_t n = 0;
+    HIPCHK(hipGraphGetNodes(b->graph, nullptr, &n));
+    b->nodes = n;
+    return GRTM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, const double *dists, int nr,
+                      const grtm_grid *grid, int precision, int compat, int flags, grtm_batch **out)
+{
+    if (!ctx || !models || n_models < 1 || !dists || nr < 1 || !grid || !out) return GRTM_E_INPUT;
+    *out = nullptr;
+    if (precision != GRTM_PREC_FP64 && precision != GRTM_PREC_FP32 && precision != GRTM_PREC_BF16X3)
+        return GRTM_E_INPUT;
+    if (!grid_ok(grid, precision)) return GRTM_E_INPUT;
+    if (precision == GRTM_PREC_FP64 && grid->nw > 4096) return GRTM_E_UNSUPPORTED;
+    if (!grtm::cells_tile_limit(grid->nw, grid->nk) || (grid->nk + 63) / 64 > 4096) return GRTM_E_UNSUPPORTED;
+    for (int i = 0; i < nr; i++)
+        if (!(dists[i] > 0)) return GRTM_E_INPUT;
+    HIPCHK(hipSetDevice(ctx->device));
+    grtm_batch *b = new grtm_batch;
+    struct Guard {
+        grtm_batch *&b;
+        ~Guard() { delete b; }
+    } guard{b};
+    b->ctx = ctx;
+    b->prec = precision;
+    b->compat = compat;
+    b->gemm_mode = precision == GRTM_PREC_FP64 ? 0 : precision == GRTM_PREC_FP32 ? 1 : 2;
+    b->nm = n_models;
+    b->nw = grid->nw;
+    b->N = 2 * grid->nw;
+    b->nk = grid->nk;
+    b->nr = nr;
+    b->dk = grid->kmax / grid->nk;
+    const double T = b->N * grid->dt;
+    b->df = 1.0 / T;
+    b->omega_i = grid->omega_i > 0 ? grid->omega_i : M_PI / T;
+    b->pp = post_params(grid, b->nw, b->df, b->omega_i);
+    HIPCHK(hipStreamCreate(&b->stream));
+
+    // models: descriptors and layer tables, concatenated
+    int st;
+    std::vector<LayerW<double>> all64;
+    b->L.resize(n_models);
+    b->rows64.resize(n_models);
+    b->md_h.resize(n_models);
+    for (int m = 0; m < n_models; m++) {
+        ModelTables mt;
+        const grtm_model &gm = models[m];
+        if ((st = model_tables(gm.layers, gm.n_layers, gm.src_depth, gm.rcv_depth, compat, b->nw, b->df, b->omega_i,
+                               precision != GRTM_PREC_FP64, mt)))
+            return st;
+        b->L[m] = mt.st.L;
+        b->rows64[m] = mt.rows64;
+        b->md_h[m].g = mt.g;
+        b->md_h[m].lw_off = (long)all64.size();
+        b->md_h[m].pad = 0;
+        all64.insert(all64.end(), mt.lw.begin(), mt.lw.end());
+        double oc[2] = {0, 0};
+        if (grtm_opcount_cell(mt.st.L, mt.g.s, mt.g.rl, mt.g.branch, mt.g.up, oc)) return GRTM_E_INPUT;
+        b->flops += oc[0] * b->nw * b->nk;
+    }
+    if ((st = b->md.alloc(sizeof(grtm::ModelDesc) * n_models)) ||
+        (st = b->lw64.alloc(sizeof(LayerW<double>) * all64.size())))
+        return st;
+    HIPCHK(hipMemcpy(b->md.p, b->md_h.data(), sizeof(grtm::ModelDesc) * n_models, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->lw64.p, all64.data(), sizeof(LayerW<double>) * all64.size(), hipMemcpyHostToDevice));
+    if (precision != GRTM_PREC_FP64) {
+        const std::vector<LayerW<float>> lf = narrow_tables(all64);
+        if ((st = b->lw32.alloc(sizeof(LayerW<float>) * lf.size()))) return st;
+        HIPCHK(hipMemcpy(b->lw32.p, lf.data(), sizeof(LayerW<float>) * lf.size(), hipMemcpyHostToDevice));
+    }
+    std::vector<LayerW<double>>().swap(all64);
+
+    // group size: device bytes per model of the group buffers within a budget
+    const size_t es = esize(precision);
+    const size_t per_model = 2 * (size_t)4 * b->nw * ((size_t)b->nk + b->nr) * es +
+                             (precision != GRTM_PREC_FP64 ? (size_t)b->nw * b->nk * 4 : 0);
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    const size_t budget = std::min<size_t>((size_t)24 << 30, free_b / 4);
+    size_t nb = std::max<size_t>(1, budget / per_model);
+    nb = std::min<size_t>(nb, 64);
+    nb = std::min<size_t>(nb, (size_t)n_models);
+    // flag entries (j nw + iw) nk + ik must fit 32 bits
+    while (nb > 1 && (double)nb * b->nw * b->nk >= 4294967295.0) nb--;
+    b->nb = (int)nb;
+    for (int m0 = 0; m0 < n_models; m0 += b->nb)
+        b->groups.push_back({m0, std::min(b->nb, n_models - m0), 0, 0, 0, 0});
+
+    const size_t arows = (size_t)4 * b->nw * b->nb;
+    if ((st = b->dists.alloc(sizeof(double) * nr)) || (st = b->B0.alloc((size_t)b->nk * nr * es)) ||
+        (st = b->B1.alloc((size_t)b->nk * nr * es)) || (st = b->A0.alloc(arows * b->nk * es)) ||
+        (st = b->A1.alloc(arows * b->nk * es)) || (st = b->C0.alloc(arows * nr * es)) ||
+        (st = b->C1.alloc(arows * nr * es)) || (st = b->tr.alloc((size_t)n_models * 4 * nr * b->N * es)))
+        return st;
+    if (precision != GRTM_PREC_FP64 && (st = b->flags.alloc(sizeof(unsigned) * ((size_t)b->nb * b->nw * b->nk + 1))))
+        return st;
+    size_t wsb = grtm::gemm_workspace_bytes(b->gemm_mode, (int)arows, nr, b->nk);
+    if (b->gemm_mode == 1 && grtm::gemm_pair_ok(b->nk, nr, b->A0.p, b->A1.p, b->B0.p, b->B1.p)) {
+        b->pair = true;
+        b->kc = grtm::gemm_pair_slice((int)arows, nr, b->nk, b->nk);
+        const int kmin = std::max(256, b->kc / 2);
+        wsb = std::max(wsb, grtm::gemm_pair_workspace_bytes((int)arows, nr, b->nk, kmin));
+    }
+    if ((st = b->ws.alloc(wsb))) return st;
+    HIPCHK(hipMemcpy(b->dists.p, dists, sizeof(double) * nr, hipMemcpyHostToDevice));
+    HIPCHK(grtm::launch_bessel_tab(precision == GRTM_PREC_FP64 ? 0 : 1, compat ? 0 : 1, b->nk, b->dk,
+                                   b->dists.as<double>(), nr, b->B0.p, b->B1.p, nr, b->stream));
+
+    if (flags & GRTM_BATCH_AUTOTUNE) {
+        if ((st = batch_autotune(b))) return st;
+    }
+    if ((st = batch_items(b, b->bs, 0, (int)b->groups.size()))) return st;
+    // one eager pass of the first group (kernel attributes are set outside the capture)
+    if ((st = batch_group(b, b->groups[0], b->stream))) return st;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if ((st = batch_capture(b))) return st;
+    *out = b;
+    b = nullptr;
+    return GRTM_OK;
+}
+
+int grtm_batch_execute(grtm_batch *b, void *stream)
+{
+    if (!b || !b->exec) return GRTM_E_INPUT;
+    HIPCHK(hipGraphLaunch(b->exec, stream ? (hipStream_t)stream : b->stream));
+    return GRTM_OK;
+}
+
+int grtm_batch_execute_eager(grtm_batch *b)
+{
+    if (!b) return GRTM_E_INPUT;
+    return batch_all(b, b->stream);
+}
+
+int grtm_batch_sync(grtm_batch *b)
+{
+    if (!b) return GRTM_E_INPUT;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return GRTM_OK;
+}
+
+int grtm_batch_traces_out(grtm_batch *b, int m0, int m1, double *traces)
+{
+    if (!b || !traces || m0 < 0 || m1 > b->nm || m0 >= m1) return GRTM_E_INPUT;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const size_t per = (size_t)4 * b->nr * b->N, n = per * (m1 - m0);
+    if (b->prec == GRTM_PREC_FP64) {
+        HIPCHK(hipMemcpy(traces, b->tr.as<double>() + per * m0, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> f(n);
+        HIPCHK(hipMemcpy(f.data(), b->tr.as<float>() + per * m0, n * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) traces[i] = f[i];
+    }
+    return GRTM_OK;
+}
+
+int grtm_batch_timing(grtm_batch *b, int reps, double *ms)
+{
+    if (!b || !ms || reps < 1) return GRTM_E_INPUT;
+    return time_ms(b, reps, [&] { return grtm_batch_execute(b, nullptr); }, ms[0]);
+}
+
+int grtm_batch_config(grtm_batch *b, double *out)
+{
+    if (!b || !out) return GRTM_E_INPUT;
+    out[0] = b->nb;
+    out[1] = b->bs;
+    out[2] = b->pair ? b->kc : 0;
+    out[3] = (double)b->groups.size();
+    out[4] = (double)b->nodes;
+    out[5] = b->tuned;
+    out[6] = b->tune_ms;
+    return GRTM_OK;
+}
+
+int grtm_batch_work(grtm_batch *b, double *out)
+{
+    if (!b || !out) return GRTM_E_INPUT;
+    double cl = 0;
+    for (int m = 0; m < b->nm; m++) cl += b->L[m];
+    const double cells = (double)b->nw * b->nk;
+    out[0] = cells * b->nm;
+    out[1] = cells * cl;
+    out[2] = 2.0 * 2.0 * (4.0 * b->nw) * b->nk * b->nr * b->nm;
+    out[3] = 2.0 * b->nr * b->nm;
+    out[4] = b->flops;
+    return GRTM_OK;
+}
+
+void grtm_batch_destroy(grtm_batch *b) { delete b; }
 
 }  // extern "C"

@@ -72,7 +72,27 @@ bool gemm_pair_ok(long lda, long ldb, const void *A0, const void *A1, const void
 hipError_t launch_gemm_pair_f32(int M, int N, int K, int kc, const float *A0, const float *A1, long lda,
                                 const float *B0, const float *B1, long ldb, float *C0, float *C1, float *P,
                                 const int *kend, const int *nk_w, hipStream_t st);
+// layout 1 with several models: trace t reads model t / (4 nr), whose
+// contraction outputs start cms elements further per model
 hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr, const void *raw, const void *C0,
-                            const void *C1, long ldc, int nr, void *traces, void *spec_post, hipStream_t st);
+                            const void *C1, long ldc, int nr, void *traces, void *spec_post, hipStream_t st,
+                            long cms = 0);
+
+// One model of a batch on the device: geometry and the offset (in LayerW
+// entries) of its [nw x L] layer table in the batch table.
+struct ModelDesc {
+    Geo g;
+    long lw_off;
+    long pad;
+};
+// Recursion of a group of models; items (j << 32) | cells_tile(row, tile)
+// with tiles of bs in {64, 128, 256} wavenumbers; prec as launch_cells_zr.
+hipError_t launch_cells_zr_batch(int prec, int compat, int bs, const ModelDesc *md, int m0, const void *lw_all,
+                                 int nw, int nk, double dk, double df, double omega_i,
+                                 const unsigned long long *items, int nitems, void *A0, void *A1, long lda,
+                                 unsigned *flag_count, unsigned *flag_list, hipStream_t st);
+hipError_t launch_cells_fix_batch(int compat, const ModelDesc *md, int m0, const void *lw64, int nw, int nk, double dk,
+                                  double df, double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,
+                                  const unsigned *flag_list, hipStream_t st);
 
 }  // namespace grtm

@@ -173,6 +173,64 @@ __global__ void k_fallback_select(int nw, int nk, double tol, const unsigned *__
     }
 }
 
+// ---------------------------------------------------------------- batches
+// Recursion of a group of models in one launch: one block of BS threads per
+// work item (model j of the group, row, k tile of BS wavenumbers), items
+// ordered heaviest model first so the launch ends on light items.  Model
+// m0 + j has its geometry and the offset of its layer table in md; its A
+// operand rows are group rows j nw + iw.  Non-finite FP32 cells go to the
+// flag list as (j nw + iw) nk + ik.
+template <typename T, typename TO, bool COMPAT, int BS>
+__global__ __launch_bounds__(BS) void k_cells_zr_batch(const ModelDesc *__restrict__ md, int m0,
+                                                       const LayerW<T> *__restrict__ lw_all, int nw, int nk,
+                                                       double dk, double df, double omega_i,
+                                                       const unsigned long long *__restrict__ items,
+                                                       TO *__restrict__ A0, TO *__restrict__ A1, long lda,
+                                                       unsigned *__restrict__ flag_count,
+                                                       unsigned *__restrict__ flag_list)
+{
+    __shared__ LayerW<T> lw[kMaxLayers];
+    const unsigned long long e = items[blockIdx.x];
+    const int j = int(e >> 32);
+    const unsigned lo = unsigned(e & 0xffffffffull);
+    const int iw = int(lo >> kTileBits), kb = int(lo & ((1u << kTileBits) - 1));
+    const Geo g = md[m0 + j].g;
+    const size_t off = md[m0 + j].lw_off + (size_t)iw * g.L;
+    for (int q = threadIdx.x; q < g.L; q += BS) lw[q] = lw_all[off + q];
+    __syncthreads();
+    const int ik = kb * BS + threadIdx.x;
+    if (ik >= nk) return;
+    TO v[8];
+    const bool ok = cell_zr<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
+    store_zr(v, j * nw + iw, ik, A0, A1, lda);
+    if (!ok && flag_count) {
+        const unsigned slot = atomicAdd(flag_count, 1u);
+        flag_list[slot] = unsigned(j * nw + iw) * unsigned(nk) + unsigned(ik);
+    }
+}
+
+// FP64 re-solve of the flagged cells of a group.
+template <typename TO, bool COMPAT>
+__global__ __launch_bounds__(kCellThreads) void k_cells_fix_batch(const ModelDesc *__restrict__ md, int m0,
+                                                                  const LayerW<double> *__restrict__ lw_all,
+                                                                  int nw, int nk, double dk, double df,
+                                                                  double omega_i, TO *__restrict__ A0,
+                                                                  TO *__restrict__ A1, long lda,
+                                                                  const unsigned *__restrict__ flag_count,
+                                                                  const unsigned *__restrict__ flag_list)
+{
+    const unsigned n = *flag_count;
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const unsigned cell = flag_list[t];
+        const int grow = int(cell / unsigned(nk)), ik = int(cell % unsigned(nk));
+        const int j = grow / nw, iw = grow - j * nw;
+        const Geo g = md[m0 + j].g;
+        TO v[8];
+        cell_zr<double, TO, COMPAT>(lw_all + md[m0 + j].lw_off + (size_t)iw * g.L, g, iw, ik, dk, df, omega_i, v);
+        store_zr(v, grow, ik, A0, A1, lda);
+    }
+}
+
 // Dense completion of fallback rows in FP64: wavenumbers [start, nk) of each
 // listed row.  Grid (k blocks, row slots); slots stride over the list.
 template <typename TO, bool COMPAT>
@@ -503,10 +561,12 @@ __device__ __forceinline__ cx<T> post_val(cx<T> v, int iw, const PostParams &pp)
 // One workgroup per trace component.  layout 0: raw[(t*nw + iw)*2 + {re,im}];
 // layout 1: rows of the contraction outputs: component c = (t / nr) % 2 reads
 // C_c[(4 iw + 2 q + {0,1}) * ldc + ir] with q = t / (2 nr), ir = t % nr.
+// Several models (layout 1): trace t belongs to model t / (4 nr), whose
+// contraction outputs start cms elements further per model.
 template <typename T>
 __global__ __launch_bounds__(256) void k_post_ifft(PostParams pp, int layout, const T *__restrict__ raw,
                                                    const T *__restrict__ C0, const T *__restrict__ C1,
-                                                   long ldc, int nr, T *__restrict__ traces,
+                                                   long ldc, int nr, long cms, T *__restrict__ traces,
                                                    T *__restrict__ spec_post)
 {
     extern __shared__ unsigned char smem_raw[];
@@ -519,8 +579,9 @@ __global__ __launch_bounds__(256) void k_post_ifft(PostParams pp, int layout, co
             const T *p = raw + ((size_t)t * M + iw) * 2;
             return C<T>(p[0], p[1]);
         }
-        const int q = t / (2 * nr), c = (t / nr) % 2, ir = t % nr;
-        const T *Cc = c ? C1 : C0;
+        const int mj = t / (4 * nr), tl = t - mj * 4 * nr;
+        const int q = tl / (2 * nr), c = (tl / nr) % 2, ir = tl % nr;
+        const T *Cc = (c ? C1 : C0) + (size_t)mj * cms;
         const size_t row = (size_t)(4 * iw + 2 * q);
         return C<T>(Cc[row * ldc + ir], Cc[(row + 1) * ldc + ir]);
     };
@@ -741,7 +802,8 @@ hipError_t launch_gemm(int prec, int M, int N, int K, const void *A, long lda, c
 size_t post_ifft_lds_bytes(int prec, int nw) { return (size_t)2 * nw * (prec == 0 ? 16 : 8); }
 
 hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr, const void *raw, const void *C0,
-                            const void *C1, long ldc, int nr, void *traces, void *spec_post, hipStream_t st)
+                            const void *C1, long ldc, int nr, void *traces, void *spec_post, hipStream_t st,
+                            long cms)
 {
     const size_t lds = post_ifft_lds_bytes(prec, pp.nw);
     if (lds > 65536) {  // above the default dynamic-LDS limit of a launch
@@ -753,11 +815,59 @@ hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr,
     }
     if (prec == 0)
         hipLaunchKernelGGL(k_post_ifft<double>, dim3(ntr), dim3(256), lds, st, pp, layout, (const double *)raw,
-                           (const double *)C0, (const double *)C1, ldc, nr, (double *)traces,
+                           (const double *)C0, (const double *)C1, ldc, nr, cms, (double *)traces,
                            (double *)spec_post);
     else
         hipLaunchKernelGGL(k_post_ifft<float>, dim3(ntr), dim3(256), lds, st, pp, layout, (const float *)raw,
-                           (const float *)C0, (const float *)C1, ldc, nr, (float *)traces, (float *)spec_post);
+                           (const float *)C0, (const float *)C1, ldc, nr, cms, (float *)traces,
+                           (float *)spec_post);
+    return hipGetLastError();
+}
+
+hipError_t launch_cells_zr_batch(int prec, int compat, int bs, const ModelDesc *md, int m0, const void *lw_all,
+                                 int nw, int nk, double dk, double df, double omega_i,
+                                 const unsigned long long *items, int nitems, void *A0, void *A1, long lda,
+                                 unsigned *flag_count, unsigned *flag_list, hipStream_t st)
+{
+    if (nitems <= 0) return hipSuccess;
+#define GRTM_ZB(T, TO, CP, BSZ, FC, FL)                                                                          \
+    hipLaunchKernelGGL((k_cells_zr_batch<T, TO, CP, BSZ>), dim3(nitems), dim3(BSZ), 0, st, md, m0,              \
+                       (const LayerW<T> *)lw_all, nw, nk, dk, df, omega_i, items, (TO *)A0, (TO *)A1, lda, FC, FL)
+#define GRTM_ZB_BS(T, TO, CP, FC, FL)                                \
+    do {                                                             \
+        if (bs == 64) GRTM_ZB(T, TO, CP, 64, FC, FL);                \
+        else if (bs == 128) GRTM_ZB(T, TO, CP, 128, FC, FL);         \
+        else GRTM_ZB(T, TO, CP, 256, FC, FL);                        \
+    } while (0)
+    if (bs != 64 && bs != 128 && bs != 256) return hipErrorInvalidValue;
+    if (prec == 0) {
+        if (compat) GRTM_ZB_BS(double, double, true, nullptr, nullptr);
+        else GRTM_ZB_BS(double, double, false, nullptr, nullptr);
+    } else if (prec == 1) {
+        if (compat) GRTM_ZB_BS(float, float, true, flag_count, flag_list);
+        else GRTM_ZB_BS(float, float, false, flag_count, flag_list);
+    } else {
+        if (compat) GRTM_ZB_BS(double, float, true, nullptr, nullptr);
+        else GRTM_ZB_BS(double, float, false, nullptr, nullptr);
+    }
+#undef GRTM_ZB_BS
+#undef GRTM_ZB
+    return hipGetLastError();
+}
+
+hipError_t launch_cells_fix_batch(int compat, const ModelDesc *md, int m0, const void *lw64, int nw, int nk, double dk,
+                                  double df, double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,
+                                  const unsigned *flag_list, hipStream_t st)
+{
+    dim3 grid(1024);
+    if (compat)
+        hipLaunchKernelGGL((k_cells_fix_batch<float, true>), grid, dim3(kCellThreads), 0, st, md, m0,
+                           (const LayerW<double> *)lw64, nw, nk, dk, df, omega_i, (float *)A0, (float *)A1, lda,
+                           flag_count, flag_list);
+    else
+        hipLaunchKernelGGL((k_cells_fix_batch<float, false>), grid, dim3(kCellThreads), 0, st, md, m0,
+                           (const LayerW<double> *)lw64, nw, nk, dk, df, omega_i, (float *)A0, (float *)A1, lda,
+                           flag_count, flag_list);
     return hipGetLastError();
 }
 

@@ -42,6 +42,10 @@ class Job(ctypes.Structure):
     _fields_ = [("n_layers", I), ("layers", P), ("src_depth", D), ("rcv_depth", D), ("nr", I), ("dists", P)]
 
 
+class Model(ctypes.Structure):
+    _fields_ = [("n_layers", I), ("layers", P), ("src_depth", D), ("rcv_depth", D)]
+
+
 _LIB = None
 
 
@@ -90,6 +94,16 @@ def _lib():
         lib.grtm_mlp_grads.argtypes = [P, P, P, P, I, P, P]
         lib.grtm_mlp_predict.argtypes = [P, P, I, P]
         lib.grtm_mlp_set_loss.argtypes = [P, I, ctypes.c_float]
+        lib.grtm_batch_create.argtypes = [P, P, I, P, I, P, I, I, I, P]
+        lib.grtm_batch_execute.argtypes = [P, P]
+        lib.grtm_batch_execute_eager.argtypes = [P]
+        lib.grtm_batch_sync.argtypes = [P]
+        lib.grtm_batch_traces_out.argtypes = [P, I, I, P]
+        lib.grtm_batch_timing.argtypes = [P, I, P]
+        lib.grtm_batch_config.argtypes = [P, P]
+        lib.grtm_batch_work.argtypes = [P, P]
+        lib.grtm_batch_destroy.argtypes = [P]
+        lib.grtm_batch_destroy.restype = None
         _LIB = lib
     return _LIB
 
@@ -100,10 +114,12 @@ EXPORTS = [
     "grtm_create", "grtm_destroy", "grtm_plan_traces", "grtm_plan_execute", "grtm_plan_sync",
     "grtm_plan_traces_out", "grtm_plan_kernels_out", "grtm_plan_timing", "grtm_plan_work",
     "grtm_plan_destroy", "grtm_selftest_cell", "grtm_opcount_cell", "grtm_selftest_gemm",
-    "grtm_selftest_mfma_probe", "grtm_selftest_copy", "grtm_plan_set_adaptive", "grtm_plan_set_rows64", "grtm_plan_adaptive_stats",
+    "grtm_selftest_mfma_probe", "grtm_selftest_copy", "grtm_plan_set_adaptive", "grtm_plan_set_rows64",
+    "grtm_plan_adaptive_stats",
     "grtm_plan_row_labels", "grtm_mlp_create", "grtm_mlp_destroy", "grtm_mlp_nparams", "grtm_mlp_set_params",
     "grtm_mlp_get_params", "grtm_mlp_set_data", "grtm_mlp_train", "grtm_mlp_grads", "grtm_mlp_predict",
-    "grtm_mlp_set_loss",
+    "grtm_mlp_set_loss", "grtm_batch_create", "grtm_batch_execute", "grtm_batch_execute_eager", "grtm_batch_sync",
+    "grtm_batch_traces_out", "grtm_batch_timing", "grtm_batch_config", "grtm_batch_work", "grtm_batch_destroy",
 ]
 
 

@@ -61,6 +61,20 @@ C1_LAYERS = np.array([
 ])
 
 C2_SEED = 20261203
+C4_SEED = 20261204
+
+
+def batch_models(seed, n_models, n_min=4, n_max=30, src_lo=0.0, src_hi=40.0):
+    """n_models layered models from one seed: layer count U{n_min..n_max}
+    (finite layers plus the half-space) and source depth U(src_lo, src_hi) km,
+    each drawn with the layered_model generator.  Returns [(layers, src_depth)]."""
+    rng = SplitMix64(seed)
+    out = []
+    for m in range(n_models):
+        n_total = rng.randint(n_min, n_max)
+        src = rng.uniform(src_lo, src_hi)
+        out.append((layered_model(seed * 1000003 + m, n_total - 1), src))
+    return out
 
 
 def dense_grid(nw, dt, nk, vs_min, t0=0.0, taper=0.3):
@@ -70,8 +84,11 @@ def dense_grid(nw, dt, nk, vs_min, t0=0.0, taper=0.3):
                 taper=taper)
 
 
-def config(name):
-    """Workloads: c1 (3 layers, 8 receivers), c2 (8 layers, 256 receivers)."""
+def config(name, n_models=1024):
+    """Workloads: c1 (3 layers, 8 receivers), c2 (8 layers, 256 receivers),
+    c4 (n_models models of 4-30 layers, sources at 0-40 km, 64 receivers
+    5-300 km, Nw 1024, Nk 8192, dt 0.05 s; one grid for the batch with k_max
+    from the slowest shear velocity of all models)."""
     if name == "c1":
         layers = C1_LAYERS.copy()
         return dict(name="c1", layers=layers, src_depth=10.0, rcv_depth=0.0,
@@ -82,6 +99,11 @@ def config(name):
         return dict(name="c2", layers=layers, src_depth=12.0, rcv_depth=0.0,
                     dists=np.linspace(5.0, 500.0, 256),
                     grid=dense_grid(2048, 0.05, 16384, layers[:, 2].min()))
+    if name == "c4":
+        ms = batch_models(C4_SEED, n_models)
+        vs_min = min(float(l[:, 2].min()) for l, _ in ms)
+        return dict(name="c4", models=ms, rcv_depth=0.0, dists=np.linspace(5.0, 300.0, 64),
+                    grid=dense_grid(1024, 0.05, 8192, vs_min))
     raise KeyError(name)
 
 

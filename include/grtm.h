@@ -110,6 +110,13 @@ int grtm_plan_timing(grtm_plan *plan, int reps, double *ms);
 int grtm_plan_work(grtm_plan *plan, double *work);
 void grtm_plan_destroy(grtm_plan *plan);
 
+/* Mixed-precision plans: solve frequency rows [0, rows64) of the recursion in
+ * FP64 (the rest in FP32).  The plan's default is the rows with
+ * |w| < 2.5 Vs_min / |z_s - z_r| (at least 4); rows64 = nw solves every row
+ * in FP64 and keeps FP32 operands for the contraction.  No reference
+ * counterpart (precision control of the new path). */
+int grtm_plan_set_rows64(grtm_plan *plan, int rows64);
+
 /* Adaptive wavenumber grid (no counterpart in the reference, whose ragged
  * grid uses the cal_kmax heuristic, green.cu:108-162).  kcut[iw] (km^-1) is
  * the truncation wavenumber of each frequency row (from the learned
@@ -118,13 +125,6 @@ void grtm_plan_destroy(grtm_plan *plan);
  * km^-1 beyond the cut; a row whose checked tail exceeds tol times the row's
  * peak is completed on the dense grid in FP64 (fallback).  kcut = NULL
  * returns the plan to the dense grid. */
-/* Mixed-precision plans: solve frequency rows [0, rows64) of the recursion in
- * FP64 (the rest in FP32).  The plan's default is the rows with
- * |w| < 2.5 Vs_min / |z_s - z_r| (at least 4); rows64 = nw solves every row
- * in FP64 and keeps FP32 operands for the contraction.  No reference
- * counterpart (precision control of the new path). */
-int grtm_plan_set_rows64(grtm_plan *plan, int rows64);
-
 int grtm_plan_set_adaptive(grtm_plan *plan, const double *kcut, double window, int stride, double tol);
 /* Training labels of every frequency row after a dense execution: out[26*iw
  * + 0] peak |kernel|, [1] 1 + last wavenumber index whose |kernel| exceeds
@@ -134,6 +134,46 @@ int grtm_plan_row_labels(grtm_plan *plan, double tau, float *out);
 /* After an execution: out[0] cells of the predicted grid, out[1] residual-check
  * cells, out[2] cells after fallback, out[3] rows that fell back. */
 int grtm_plan_adaptive_stats(grtm_plan *plan, double *out);
+
+/* ------------------------------------------------------------ batches
+ * Many layered models (uneven layer counts and source depths) on one grid
+ * and one set of surface-receiver distances: the production workload of
+ * Green's-function databases.  No counterpart in the reference, which runs
+ * one model and one distance per green() call.  Models are processed in
+ * groups sharing device buffers: one recursion launch per group over
+ * (model, frequency row, k tile) work items, ordered heaviest model first,
+ * one contraction over the group's stacked k-kernels, one inverse FFT; the
+ * whole sequence is captured once in a HIP graph and replayed by
+ * grtm_batch_execute.  With GRTM_BATCH_AUTOTUNE the recursion tile (64, 128
+ * or 256 wavenumbers per block) and the contraction's K slice are chosen by
+ * timing the candidates on the first group. */
+typedef struct grtm_batch grtm_batch;
+typedef struct {
+    int n_layers;          /* rows including the half-space */
+    const double *layers;  /* n_layers x 6, as in grtm_job */
+    double src_depth;      /* km */
+    double rcv_depth;      /* km */
+} grtm_model;
+#define GRTM_BATCH_AUTOTUNE 1
+int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, const double *dists, int nr,
+                      const grtm_grid *grid, int precision, int compat, int flags, grtm_batch **batch);
+/* Replay the captured graph on `stream` (NULL: the batch's stream). */
+int grtm_batch_execute(grtm_batch *batch, void *stream);
+/* The same launches issued one by one on the batch's stream, no graph. */
+int grtm_batch_execute_eager(grtm_batch *batch);
+int grtm_batch_sync(grtm_batch *batch);
+/* Traces of models [m0, m1): traces[(((m - m0)*2 + q)*2 + c)*nr + ir)*N + j]. */
+int grtm_batch_traces_out(grtm_batch *batch, int m0, int m1, double *traces);
+/* ms[0]: mean of reps graph replays (HIP events on the batch's stream). */
+int grtm_batch_timing(grtm_batch *batch, int reps, double *ms);
+/* out[0] models per group, [1] recursion tile (wavenumbers per block),
+ * [2] contraction K slice (0: split-K plan of the FP64 / BF16x3 kernels),
+ * [3] groups, [4] graph nodes, [5] autotune candidates timed, [6] autotune ms. */
+int grtm_batch_config(grtm_batch *batch, double *out);
+/* out[0] cells, [1] cell-layers, [2] contraction flops, [3] traces,
+ * [4] recursion flops (operation count of the cell arithmetic). */
+int grtm_batch_work(grtm_batch *batch, double *out);
+void grtm_batch_destroy(grtm_batch *batch);
 
 /* ------------------------------------------------------------ self-tests */
 /* Host-compiled copies of the device arithmetic, for unit checks without a
