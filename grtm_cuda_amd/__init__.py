@@ -216,9 +216,11 @@ class BatchPlan:
         return out
 
     def timing(self, reps=3):
-        ms = np.zeros(1)
+        """ms: graph replay (mean of reps) and the recursion / contraction /
+        IFFT split of one eager pass."""
+        ms = np.zeros(4)
         check(_lib().grtm_batch_timing(self.ptr, int(reps), ms.ctypes.data_as(P)), "grtm_batch_timing")
-        return float(ms[0])
+        return dict(total=float(ms[0]), cells=float(ms[1]), ksum=float(ms[2]), ifft=float(ms[3]))
 
     def config(self):
         o = np.zeros(7)

@@ -165,13 +165,20 @@ GRTM_HD RT<T> interface_rt(const LQ<T> &u, const LQ<T> &d, T k)
 
 // Free-surface reflection R_ud^0 = -(E21)^-1 E22 (green.cu:449-468) in closed
 // form: E21 = -mu F diag(a/w, b/w), F = [[2kr, x], [x, 2kv]], E22 = diag(-1,1) E21.
+// det F = 4 k^2 r v - x^2 (the Rayleigh function of the top layer) cancels
+// near the layer's Rayleigh slowness, where a shallow source excites the
+// surface waves strongly: it is formed in FP64 from the FP64 wavenumber,
+// vertical wavenumbers and w / beta (sbd).
 template <typename T>
-GRTM_HD m2<T> free_surface(const LQ<T> &q, T k)
+GRTM_HD m2<T> free_surface(const LQ<T> &q, typename WideOf<T>::type kd, cx<typename WideOf<T>::type> sbd)
 {
+    using W = typename WideOf<T>::type;
+    const T k = T(kd);
     const T k2 = k * k;
     const cx<T> rv4 = (T(4) * k2) * (q.r * q.v);
     const cx<T> x2 = q.x * q.x;
-    const cx<T> nr = recip(rv4 - x2);  // 1/det F
+    const cx<W> xd = C<W>(W(2) * kd * kd, W(0)) - sbd * sbd;
+    const cx<T> nr = narrow<T>(recip(C<W>(W(4) * kd * kd, W(0)) * (q.rd * q.vd) - xd * xd));  // 1/det F
     const cx<T> dd = (x2 + rv4) * nr;
     m2<T> R;
     R.a = dd;
@@ -318,7 +325,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
     LQ<T> qr;  // receiver layer quantities
     {
         LQ<T> qu = layer_q(lw[0], kd);
-        m2<T> Rg = free_surface(qu, k);
+        m2<T> Rg = free_surface(qu, kd, lw[0].sbd);
         cx<T> rg0 = C<T>(1, 0);
         if (rl == 0) qr = qu;
         Rud_r = Rg;

@@ -1512,7 +1512,38 @@ int grtm_batch_traces_out(grtm_batch *b, int m0, int m1, double *traces)
 int grtm_batch_timing(grtm_batch *b, int reps, double *ms)
 {
     if (!b || !ms || reps < 1) return GRTM_E_INPUT;
-    return time_ms(b, reps, [&] { return grtm_batch_execute(b, nullptr); }, ms[0]);
+    int st = time_ms(b, reps, [&] { return grtm_batch_execute(b, nullptr); }, ms[0]);
+    if (st) return st;
+    // stage split of one eager pass: events around every group's stages
+    const size_t ng = b->groups.size();
+    std::vector<hipEvent_t> ev(4 * ng, nullptr);
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    for (size_t i = 0; i < ng && !st; i++) {
+        const grtm_batch::Group &g = b->groups[i];
+        HIPCHK(hipEventRecord(ev[4 * i], b->stream));
+        if ((st = batch_cells(b, g, b->stream))) break;
+        HIPCHK(hipEventRecord(ev[4 * i + 1], b->stream));
+        if ((st = batch_gemm(b, g, b->stream))) break;
+        HIPCHK(hipEventRecord(ev[4 * i + 2], b->stream));
+        const size_t es = esize(b->prec);
+        unsigned char *out = (unsigned char *)b->tr.p + (size_t)g.m0 * 4 * b->nr * b->N * es;
+        HIPCHK(grtm::launch_post_ifft(batch_mixed(b) ? 1 : 0, b->pp, 1, 4 * b->nr * g.n, nullptr, b->C0.p, b->C1.p,
+                                      b->nr, b->nr, out, nullptr, b->stream, (long)4 * b->nw * b->nr));
+        HIPCHK(hipEventRecord(ev[4 * i + 3], b->stream));
+    }
+    if (!st) {
+        HIPCHK(hipStreamSynchronize(b->stream));
+        ms[1] = ms[2] = ms[3] = 0;
+        for (size_t i = 0; i < ng; i++) {
+            float a = 0, c = 0, d = 0;
+            HIPCHK(hipEventElapsedTime(&a, ev[4 * i], ev[4 * i + 1]));
+            HIPCHK(hipEventElapsedTime(&c, ev[4 * i + 1], ev[4 * i + 2]));
+            HIPCHK(hipEventElapsedTime(&d, ev[4 * i + 2], ev[4 * i + 3]));
+            ms[1] += a, ms[2] += c, ms[3] += d;
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return st;
 }
 
 int grtm_batch_config(grtm_batch *b, double *out)

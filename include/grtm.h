@@ -164,7 +164,9 @@ int grtm_batch_execute_eager(grtm_batch *batch);
 int grtm_batch_sync(grtm_batch *batch);
 /* Traces of models [m0, m1): traces[(((m - m0)*2 + q)*2 + c)*nr + ir)*N + j]. */
 int grtm_batch_traces_out(grtm_batch *batch, int m0, int m1, double *traces);
-/* ms[0]: mean of reps graph replays (HIP events on the batch's stream). */
+/* ms[0]: mean of reps graph replays (HIP events on the batch's stream);
+ * ms[1..3]: recursion, contraction and inverse FFT of one eager pass (sums
+ * over the groups). */
 int grtm_batch_timing(grtm_batch *batch, int reps, double *ms);
 /* out[0] models per group, [1] recursion tile (wavenumbers per block),
  * [2] contraction K slice (0: split-K plan of the FP64 / BF16x3 kernels),
