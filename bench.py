@@ -132,7 +132,25 @@ def bench_batch(args, g, models):
     }
 
 
-def roofline(g, plan, prec, tm, work):
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "c2_adaptive_fp32_pmc_bytes.md")
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, each scaled
+    by its reading for a plain copy of known size) from the committed counter
+    summary of the default workload, or None."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "TRAFFIC_MB" and parts[1] == kernel:
+                    return float(parts[2]) * 1e6
+    except OSError:
+        pass
+    return None
+
+
+def roofline(g, plan, prec, tm, work, counted=False):
     kernels = {"recursion": tm["cells"], "ksum": tm["ksum"], "ifft": tm["ifft"]}
     dom = max(kernels, key=kernels.get)
     peak = FP64_PEAK_TFLOPS if prec == g.PREC_FP64 else FP32_PEAK_TFLOPS
@@ -140,7 +158,9 @@ def roofline(g, plan, prec, tm, work):
         st = plan.adaptive_stats()
         cells = st["cells_final"] + st["cells_check"] if st["cells_final"] else work["cells"]
         achieved = cells * work["flops_per_cell"] / (tm["cells"] * 1e-3) / 1e12
-        return dict(bound="valu", achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak, traffic=None,
+        traffic = pmc_traffic("k_cells_zr<float,float,false>") if counted else None
+        return dict(bound="valu", achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak, traffic=traffic,
+                    traffic_source=os.path.relpath(PMC_SUMMARY, ROOT) if traffic else None,
                     kernel="k_cells_zr", flops_per_cell=work["flops_per_cell"],
                     transcendentals_per_cell=work["trans_per_cell"], cells=cells), kernels
     if dom == "ksum":
@@ -220,7 +240,9 @@ def main():
     err = np.linalg.norm(out - ref, axis=-1) / np.linalg.norm(ref, axis=-1)
     value = traces_per_step / t_step
     tm = plan.timing(reps=3)
-    roof, kernels = roofline(g, plan, prec, tm, work)
+    # the committed counter summary was taken on the default run (c2, learned adaptive, fp32)
+    counted = args.workload == "c2" and args.path == "adaptive" and prec == g.PREC_FP32
+    roof, kernels = roofline(g, plan, prec, tm, work, counted)
 
     cpu = None
     if not args.no_cpu:
