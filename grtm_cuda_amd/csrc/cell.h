@@ -88,8 +88,16 @@ GRTM_HD LQ<T> layer_q(const LayerW<T> &L, typename WideOf<T>::type kd)
     // the kernel is sensitive near k = w/V); the factored form avoids
     // cancellation near k = w/V
     using W = typename WideOf<T>::type;
-    q.rd = csqrt((C<W>(kd, W(0)) - L.sad) * (C<W>(kd, W(0)) + L.sad));
-    q.vd = csqrt((C<W>(kd, W(0)) - L.sbd) * (C<W>(kd, W(0)) + L.sbd));
+    const cx<W> zr = (C<W>(kd, W(0)) - L.sad) * (C<W>(kd, W(0)) + L.sad);
+    const cx<W> zv = (C<W>(kd, W(0)) - L.sbd) * (C<W>(kd, W(0)) + L.sbd);
+    if constexpr (sizeof(T) == 4) {
+        // FP32 cells: FP32 estimate + FP64 Newton step (about 1e-13 relative)
+        q.rd = csqrt_nr(zr);
+        q.vd = csqrt_nr(zv);
+    } else {
+        q.rd = csqrt(zr);
+        q.vd = csqrt(zv);
+    }
     q.r = narrow<T>(q.rd);
     q.v = narrow<T>(q.vd);
     // k^2 + v^2 = 2 k^2 - (w/beta)^2, formed in FP64: it cancels near

@@ -26,6 +26,33 @@ template <typename T> GRTM_HD cx<T> operator*(cx<T> a, T s) { return {s * a.x, s
 template <typename T> GRTM_HD cx<T> conj(cx<T> a) { return {a.x, -a.y}; }
 template <typename T> GRTM_HD T norm2(cx<T> a) { return a.x * a.x + a.y * a.y; }
 
+// Hardware reciprocals on the device (v_rcp_f32: 1 ulp; v_rcp_f64: about
+// 2^-26, for a Newton start); exact division in host builds.
+GRTM_HD float frcp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+GRTM_HD double drcp_est(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcp(x);
+#else
+    return 1.0 / x;
+#endif
+}
+GRTM_HD float fsqrt_est(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
+
 // Reciprocal with Smith-style scaling (robust against over/underflow of |z|^2).
 template <typename T> GRTM_HD cx<T> recip(cx<T> b)
 {
@@ -38,6 +65,16 @@ template <typename T> GRTM_HD cx<T> recip(cx<T> b)
         T d = T(1) / (b.x * r + b.y);
         return {r * d, -d};
     }
+}
+// FP32: the same scaling with the hardware reciprocal (1 ulp) in place of
+// two IEEE divides, and no branch.
+template <> GRTM_HD cx<float> recip(cx<float> b)
+{
+    const bool xb = fabsf(b.x) >= fabsf(b.y);
+    const float p = xb ? b.x : b.y, q = xb ? b.y : b.x;
+    const float r = q * frcp(p);
+    const float d = frcp(p + q * r);
+    return xb ? cx<float>{d, -r * d} : cx<float>{r * d, -d};
 }
 template <typename T> GRTM_HD cx<T> operator/(cx<T> a, cx<T> b) { return a * recip(b); }
 
@@ -53,6 +90,28 @@ template <typename T> GRTM_HD cx<T> csqrt(cx<T> z)
     }
     T t = sqrt(T(0.5) * (m - a));
     return {fabs(b) / (T(2) * t), copysign(t, b)};
+}
+
+// Principal square root of an FP64 value to about 1e-13 relative without FP64
+// divides or square roots: an FP32 estimate (hardware square root and
+// reciprocal, about 2^-21) refined by one Newton step s = (s0 + z / s0) / 2 in
+// FP64, z / s0 = z conj(s0) / |s0|^2 with a Newton-refined reciprocal.  The
+// step keeps the estimate's branch (Re >= 0).  |z| must lie well inside the
+// FP32 range (the vertical wavenumbers: 1e-12 < |z| < 1e18).
+GRTM_HD cx<double> csqrt_nr(cx<double> z)
+{
+    const float a = float(z.x), b = float(z.y);
+    const float m = fsqrt_est(a * a + b * b);
+    const float t = fsqrt_est(0.5f * (m + fabsf(a)));
+    const float u = 0.5f * b * frcp(t);
+    const double s0x = a >= 0.f ? double(t) : double(fabsf(u));
+    const double s0y = a >= 0.f ? double(u) : double(copysignf(t, b));
+    const double n = s0x * s0x + s0y * s0y;
+    double inv = drcp_est(n);
+    inv = inv * (2.0 - n * inv);
+    inv = inv * (2.0 - n * inv);
+    const double qx = (z.x * s0x + z.y * s0y) * inv, qy = (z.y * s0x - z.x * s0y) * inv;
+    return {0.5 * (s0x + qx), 0.5 * (s0y + qy)};
 }
 
 GRTM_HD void sincos_t(double a, double *s, double *c) { sincos(a, s, c); }
