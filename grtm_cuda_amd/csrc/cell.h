@@ -63,8 +63,10 @@ GRTM_HD cx<T> narrow(cx<W> z)
 }
 
 // e^{-nu z} with the argument formed in FP64.  For FP32 cells the phase is
-// reduced modulo 2 pi in FP64 first: Im(nu z) reaches hundreds of radians, and
-// rounding it to FP32 would cost ~1e-5 of relative accuracy per factor.
+// reduced to revolutions in [-1/2, 1/2] in FP64 first: Im(nu z) reaches
+// hundreds of radians, and rounding it to FP32 would cost ~1e-5 of relative
+// accuracy per factor.  The modulus is 2^(Re(-nu z) log2 e) with the product
+// in FP64; exp, sin and cos are then the hardware FP32 instructions.
 template <typename T, typename W>
 GRTM_HD cx<T> cexp_nz(cx<W> nu, W z)
 {
@@ -72,9 +74,10 @@ GRTM_HD cx<T> cexp_nz(cx<W> nu, W z)
     if constexpr (sizeof(T) == sizeof(W)) {
         return narrow<T>(cexp(C<W>(ax, ay)));
     } else {
-        const double tp = 6.283185307179586476925286766559;
-        const double ry = ay - tp * rint(ay * (1.0 / tp));
-        return cexp(C<T>(T(ax), T(ry)));
+        const double rev = ay * 0.15915494309189533576888376337251;  // 1 / (2 pi)
+        const float f = float(rev - rint(rev));
+        const float e = fexp2(float(ax * 1.4426950408889634073599246810019));
+        return C<T>(e * fcos_rev(f), e * fsin_rev(f));
     }
 }
 
