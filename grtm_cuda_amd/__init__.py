@@ -242,7 +242,7 @@ class BatchPlan:
 def selftest_gemm(mode, A, B):
     """C = A B on the GPU with a contraction kernel (test hook): mode -1 VALU
     FP64, 0 FP64 MFMA, 1 FP32 MFMA, 2 BF16x3 MFMA, 3 streamed FP32 MFMA pair
-    kernel."""
+    kernel, 4 streamed FP64 MFMA pair kernel."""
     A = np.ascontiguousarray(A, dtype=np.float64)
     B = np.ascontiguousarray(B, dtype=np.float64)
     M, K = A.shape

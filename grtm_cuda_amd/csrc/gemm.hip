@@ -276,6 +276,107 @@ __global__ void k_gemm_reduce_ragged(int M, int N, int K, int kc, int bm, const 
     (pi ? C1 : C0)[e] = T(s);
 }
 
+// ------------------------------------------------------------------ FP64, streamed
+// Two contractions in FP64 on v_mfma_f64_16x16x4f64, as k_gemm_f32s: block
+// tile 128 x 128 (A is read N / 128 times instead of N / 64), 4 waves of
+// 64 x 64 (4 x 4 MFMA tiles), K slab 16 with the next slab's 16-byte loads in
+// flight (two LDS buffers).  Lane l of an MFMA takes A[m0 + l % 16][k + l / 16]
+// and B[k + l / 16][n0 + l % 16]; accumulator r of lane l is row
+// (l >> 4) + 4 r, column l % 16.  Requires lda, ldb even and 16-byte aligned
+// A, B.
+constexpr int kDBM = 128, kDBN = 128, kDBK = 16, kDPA = kDBK + 2, kDPB = kDBN + 2;
+
+struct GemmPair64 {
+    const double *A[2];
+    const double *B[2];
+    double *P[2];
+};
+
+__global__ __launch_bounds__(256) void k_gemm_f64s(int M, int N, int K, int kc, int S, GemmPair64 gp, long lda,
+                                                   long ldb, const int *__restrict__ kend,
+                                                   const int *__restrict__ nk_w)
+{
+    __shared__ __attribute__((aligned(16))) double As[2][kDBM][kDPA];
+    __shared__ __attribute__((aligned(16))) double Bs[2][kDBK][kDPB];
+    const int pi = blockIdx.z / S, z = blockIdx.z - pi * S;
+    const double *__restrict__ A = gp.A[pi];
+    const double *__restrict__ B = gp.B[pi];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv >> 1, wn = wv & 1;
+    const int m0 = blockIdx.y * kDBM, n0 = blockIdx.x * kDBN;
+    const int kt = min(K, kend ? kend[blockIdx.y] : K);
+    const int kb = z * kc;
+    if (kb >= kt) return;  // uniform: this slice is past the tile's extent
+    const int ke = min(kt, kb + kc);
+
+    // staging: A items idx = tid + 256 e (e < 4) -> row idx >> 3, k offset (idx & 7) * 2;
+    // B items -> k row idx >> 6, column offset (idx & 63) * 2
+    int a_lim[4];
+    for (int e = 0; e < 4; e++) {
+        const int gm = m0 + ((tid + 256 * e) >> 3);
+        a_lim[e] = gm < M ? (nk_w ? min(ke, nk_w[gm >> 2]) : ke) : 0;
+    }
+    double2 ra[4], rb[4];
+    auto load = [&](int k0) {
+        for (int e = 0; e < 4; e++) {
+            const int idx = tid + 256 * e, gm = m0 + (idx >> 3), gk = k0 + (idx & 7) * 2;
+            double2 v = make_double2(0.0, 0.0);
+            if (gk + 1 < a_lim[e]) v = *reinterpret_cast<const double2 *>(A + (size_t)gm * lda + gk);
+            else if (gk < a_lim[e]) v.x = A[(size_t)gm * lda + gk];
+            ra[e] = v;
+        }
+        for (int e = 0; e < 4; e++) {
+            const int idx = tid + 256 * e, gk = k0 + (idx >> 6), gn = n0 + (idx & 63) * 2;
+            double2 v = make_double2(0.0, 0.0);
+            if (gk < ke) {
+                const double *r = B + (size_t)gk * ldb;
+                if (gn + 1 < N) v = *reinterpret_cast<const double2 *>(r + gn);
+                else if (gn < N) v.x = r[gn];
+            }
+            rb[e] = v;
+        }
+    };
+    auto store = [&](int buf) {
+        for (int e = 0; e < 4; e++) {
+            const int idx = tid + 256 * e;
+            *reinterpret_cast<double2 *>(&As[buf][idx >> 3][(idx & 7) * 2]) = ra[e];
+            *reinterpret_cast<double2 *>(&Bs[buf][idx >> 6][(idx & 63) * 2]) = rb[e];
+        }
+    };
+
+    v4d acc[4][4];
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) acc[a][b] = v4d{0, 0, 0, 0};
+    load(kb);
+    store(0);
+    __syncthreads();
+    const int i = lane & 15, kq = lane >> 4;
+    int buf = 0;
+    for (int k0 = kb; k0 < ke; k0 += kDBK) {
+        const bool more = k0 + kDBK < ke;
+        if (more) load(k0 + kDBK);
+        for (int kk = 0; kk < kDBK; kk += 4) {
+            double a[4], b[4];
+            for (int t = 0; t < 4; t++) a[t] = As[buf][wm * 64 + t * 16 + i][kk + kq];
+            for (int t = 0; t < 4; t++) b[t] = Bs[buf][kk + kq][wn * 64 + t * 16 + i];
+            for (int tm = 0; tm < 4; tm++)
+                for (int tn = 0; tn < 4; tn++)
+                    acc[tm][tn] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[tm], b[tn], acc[tm][tn], 0, 0, 0);
+        }
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    double *Pz = gp.P[pi] + (size_t)z * M * N;
+    for (int tm = 0; tm < 4; tm++)
+        for (int tn = 0; tn < 4; tn++)
+            for (int r = 0; r < 4; r++) {
+                const int row = m0 + wm * 64 + tm * 16 + (lane >> 4) + 4 * r;
+                const int col = n0 + wn * 64 + tn * 16 + (lane & 15);
+                if (row < M && col < N) Pz[(size_t)row * N + col] = acc[tm][tn][r];
+            }
+}
+
 // ------------------------------------------------------------------ BF16x3
 __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo)
 {
@@ -403,6 +504,23 @@ size_t gemm_pair_workspace_bytes(int M, int N, int K, int kc)
 {
     const size_t S = (size_t)(K + kc - 1) / kc;
     return 2 * S * (size_t)M * N * sizeof(float);
+}
+
+hipError_t launch_gemm_pair_f64(int M, int N, int K, int kc, const double *A0, const double *A1, long lda,
+                                const double *B0, const double *B1, long ldb, double *C0, double *C1, double *P,
+                                const int *kend, const int *nk_w, hipStream_t st)
+{
+    if (kc < kDBK || kc % 32 || lda % 2 || ldb % 2) return hipErrorInvalidValue;
+    const int S = (K + kc - 1) / kc;
+    GemmPair64 gp;
+    gp.A[0] = A0, gp.A[1] = A1, gp.B[0] = B0, gp.B[1] = B1;
+    gp.P[0] = P, gp.P[1] = P + (size_t)S * M * N;
+    dim3 grid((N + kDBN - 1) / kDBN, (M + kDBM - 1) / kDBM, 2 * S);
+    hipLaunchKernelGGL(k_gemm_f64s, grid, dim3(256), 0, st, M, N, K, kc, S, gp, lda, ldb, kend, nk_w);
+    const long n2 = 2L * M * N;
+    hipLaunchKernelGGL(k_gemm_reduce_ragged<double>, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, M, N, K,
+                       kc, kDBM, kend, (const double *)gp.P[0], (const double *)gp.P[1], C0, C1);
+    return hipGetLastError();
 }
 
 bool gemm_pair_ok(long lda, long ldb, const void *A0, const void *A1, const void *B0, const void *B1)

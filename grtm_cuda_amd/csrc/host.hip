@@ -636,7 +636,8 @@ int run_cells(grtm_plan *p, hipStream_t s)
     HIPCHK(grtm::launch_fallback(p->prec == GRTM_PREC_FP64, p->compat, lwf64, p->L, p->g, p->nk, p->nw, p->dk, p->df,
                                  p->omega_i, p->chk_tol, rowpk, tailmx, nkw, fbcnt, p->fb_list.as<unsigned>(), p->A0.p,
                                  p->A1.p, p->nk, s));
-    HIPCHK(grtm::launch_kend_tiles(p->nw, p->gemm_mode == 0 ? 16 : 32, p->nk, nkw, p->kend.as<int>(), s));
+    HIPCHK(grtm::launch_kend_tiles(p->nw, (p->gemm_mode == 0 && !p->pair) ? 16 : 32, p->nk, nkw,
+                                   p->kend.as<int>(), s));
     return GRTM_OK;
 }
 int run_gemm(grtm_plan *p, hipStream_t s)
@@ -644,10 +645,16 @@ int run_gemm(grtm_plan *p, hipStream_t s)
     const int M = 4 * p->nw;
     const int *kend = p->adaptive ? p->kend.as<int>() : nullptr;
     const int *nkw = p->adaptive ? p->nk_w.as<int>() : nullptr;
-    if (p->pair) {
+    if (p->pair && p->gemm_mode == 1) {
         HIPCHK(grtm::launch_gemm_pair_f32(M, p->nr, p->nk, p->kc, p->A0.as<float>(), p->A1.as<float>(), p->nk,
                                           p->B0.as<float>(), p->B1.as<float>(), p->nr, p->C0.as<float>(),
                                           p->C1.as<float>(), p->ws.as<float>(), kend, nkw, s));
+        return GRTM_OK;
+    }
+    if (p->pair && p->gemm_mode == 0) {
+        HIPCHK(grtm::launch_gemm_pair_f64(M, p->nr, p->nk, p->kc, p->A0.as<double>(), p->A1.as<double>(), p->nk,
+                                          p->B0.as<double>(), p->B1.as<double>(), p->nr, p->C0.as<double>(),
+                                          p->C1.as<double>(), p->ws.as<double>(), kend, nkw, s));
         return GRTM_OK;
     }
     HIPCHK(grtm::launch_gemm_mfma(p->gemm_mode, M, p->nr, p->nk, p->A0.p, p->nk, p->B0.p, p->nr, p->C0.p, p->ws.p,
@@ -754,10 +761,10 @@ int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, 
     if ((status = p->C1.alloc(arows * p->nr * es))) return status;
     if ((status = p->tr.alloc((size_t)4 * p->nr * p->N * es))) return status;
     size_t wsb = grtm::gemm_workspace_bytes(p->gemm_mode, (int)arows, p->nr, p->nk);
-    if (p->gemm_mode == 1 && grtm::gemm_pair_ok(p->nk, p->nr, p->A0.p, p->A1.p, p->B0.p, p->B1.p)) {
+    if (p->gemm_mode != 2 && grtm::gemm_pair_ok(p->nk, p->nr, p->A0.p, p->A1.p, p->B0.p, p->B1.p)) {
         p->pair = true;
         p->kc = grtm::gemm_pair_slice((int)arows, p->nr, p->nk, p->nk);
-        wsb = std::max(wsb, grtm::gemm_pair_workspace_bytes((int)arows, p->nr, p->nk, p->kc));
+        wsb = std::max(wsb, grtm::gemm_pair_workspace_bytes((int)arows, p->nr, p->nk, p->kc) * es / 4);
     }
     if ((status = p->ws.alloc(wsb))) return status;
     HIPCHK(hipMemcpy(p->dists.p, job->dists, sizeof(double) * p->nr, hipMemcpyHostToDevice));
@@ -940,7 +947,7 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
             ksum += std::min(p->nk, (m + 31) / 32 * 32);
         }
         const int kc = grtm::gemm_pair_slice(4 * p->nw, p->nr, p->nk, ksum / std::max(1, nt));
-        const size_t need = grtm::gemm_pair_workspace_bytes(4 * p->nw, p->nr, p->nk, kc);
+        const size_t need = grtm::gemm_pair_workspace_bytes(4 * p->nw, p->nr, p->nk, kc) * esize(p->prec) / 4;
         if (need > p->ws.n && (st = p->ws.alloc(need))) return st;
         p->kc = kc;
     }
@@ -1017,7 +1024,7 @@ void grtm_plan_destroy(grtm_plan *p) { delete p; }
 // ------------------------------------------------------------ self-tests
 int grtm_selftest_gemm(int mode, int M, int N, int K, const double *A, const double *B, double *C)
 {
-    if (mode < -1 || mode > 3 || M < 1 || N < 1 || K < 1 || !A || !B || !C) return GRTM_E_INPUT;
+    if (mode < -1 || mode > 4 || M < 1 || N < 1 || K < 1 || !A || !B || !C) return GRTM_E_INPUT;
     int st;
     if ((st = device_ready())) return st;
     if (mode == 3) {
@@ -1043,6 +1050,33 @@ int grtm_selftest_gemm(int mode, int M, int N, int K, const double *A, const dou
         HIPCHK(hipMemcpy(f.data(), dC.p, f.size() * 4, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < (size_t)M * N; i++) {
             if (f[i] != f[(size_t)M * N + i]) return GRTM_E_DEVICE;  // the two halves must agree bit for bit
+            C[i] = f[i];
+        }
+        return GRTM_OK;
+    }
+    if (mode == 4) {
+        // streamed FP64 pair kernel on (A, B) twice; padded leading dimensions
+        const long lda = (K + 3) / 4 * 4, ldb = (N + 3) / 4 * 4;
+        std::vector<double> a((size_t)M * lda, 0.0), b((size_t)K * ldb, 0.0);
+        for (int i = 0; i < M; i++)
+            for (int k = 0; k < K; k++) a[(size_t)i * lda + k] = A[(size_t)i * K + k];
+        for (int k = 0; k < K; k++)
+            for (int j = 0; j < N; j++) b[(size_t)k * ldb + j] = B[(size_t)k * N + j];
+        const int kc = grtm::gemm_pair_slice(M, N, K, K);
+        DBuf dA, dB, dC, dP;
+        if ((st = dA.alloc(a.size() * 8)) || (st = dB.alloc(b.size() * 8)) || (st = dC.alloc((size_t)2 * M * N * 8)) ||
+            (st = dP.alloc(2 * grtm::gemm_pair_workspace_bytes(M, N, K, kc))))
+            return st;
+        HIPCHK(hipMemcpy(dA.p, a.data(), a.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dB.p, b.data(), b.size() * 8, hipMemcpyHostToDevice));
+        double *c0 = dC.as<double>(), *c1 = c0 + (size_t)M * N;
+        HIPCHK(grtm::launch_gemm_pair_f64(M, N, K, kc, dA.as<double>(), dA.as<double>(), lda, dB.as<double>(),
+                                          dB.as<double>(), ldb, c0, c1, dP.as<double>(), nullptr, nullptr, nullptr));
+        HIPCHK(hipDeviceSynchronize());
+        std::vector<double> f((size_t)2 * M * N);
+        HIPCHK(hipMemcpy(f.data(), dC.p, f.size() * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < (size_t)M * N; i++) {
+            if (f[i] != f[(size_t)M * N + i]) return GRTM_E_DEVICE;
             C[i] = f[i];
         }
         return GRTM_OK;
@@ -1246,10 +1280,16 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
 int batch_gemm(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
 {
     const int M = 4 * b->nw * g.n;
-    if (b->pair) {
+    if (b->pair && b->gemm_mode == 1) {
         HIPCHK(grtm::launch_gemm_pair_f32(M, b->nr, b->nk, b->kc, b->A0.as<float>(), b->A1.as<float>(), b->nk,
                                           b->B0.as<float>(), b->B1.as<float>(), b->nr, b->C0.as<float>(),
                                           b->C1.as<float>(), b->ws.as<float>(), nullptr, nullptr, s));
+        return GRTM_OK;
+    }
+    if (b->pair && b->gemm_mode == 0) {
+        HIPCHK(grtm::launch_gemm_pair_f64(M, b->nr, b->nk, b->kc, b->A0.as<double>(), b->A1.as<double>(), b->nk,
+                                          b->B0.as<double>(), b->B1.as<double>(), b->nr, b->C0.as<double>(),
+                                          b->C1.as<double>(), b->ws.as<double>(), nullptr, nullptr, s));
         return GRTM_OK;
     }
     HIPCHK(grtm::launch_gemm_mfma(b->gemm_mode, M, b->nr, b->nk, b->A0.p, b->nk, b->B0.p, b->nr, b->C0.p, b->ws.p,
@@ -1321,7 +1361,7 @@ int batch_autotune(grtm_batch *b)
         best = 1e30;
         for (int kc : {k0 / 2, k0, 2 * k0}) {
             if (kc < 256 || kc % 32 || (kc > b->nk && kc / 2 >= b->nk)) continue;
-            if (grtm::gemm_pair_workspace_bytes(M, b->nr, b->nk, kc) > b->ws.n) continue;
+            if (grtm::gemm_pair_workspace_bytes(M, b->nr, b->nk, kc) * esize(b->prec) / 4 > b->ws.n) continue;
             b->kc = kc;
             double ms = 0;
             if ((st = time_ms(b, 2, [&] { return batch_gemm(b, b->groups[0], b->stream); }, ms))) return st;
@@ -1450,11 +1490,11 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
     if (precision != GRTM_PREC_FP64 && (st = b->flags.alloc(sizeof(unsigned) * ((size_t)b->nb * b->nw * b->nk + 1))))
         return st;
     size_t wsb = grtm::gemm_workspace_bytes(b->gemm_mode, (int)arows, nr, b->nk);
-    if (b->gemm_mode == 1 && grtm::gemm_pair_ok(b->nk, nr, b->A0.p, b->A1.p, b->B0.p, b->B1.p)) {
+    if (b->gemm_mode != 2 && grtm::gemm_pair_ok(b->nk, nr, b->A0.p, b->A1.p, b->B0.p, b->B1.p)) {
         b->pair = true;
         b->kc = grtm::gemm_pair_slice((int)arows, nr, b->nk, b->nk);
         const int kmin = std::max(256, b->kc / 2);
-        wsb = std::max(wsb, grtm::gemm_pair_workspace_bytes((int)arows, nr, b->nk, kmin));
+        wsb = std::max(wsb, grtm::gemm_pair_workspace_bytes((int)arows, nr, b->nk, kmin) * es / 4);
     }
     if ((st = b->ws.alloc(wsb))) return st;
     HIPCHK(hipMemcpy(b->dists.p, dists, sizeof(double) * nr, hipMemcpyHostToDevice));

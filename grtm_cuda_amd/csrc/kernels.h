@@ -72,6 +72,11 @@ bool gemm_pair_ok(long lda, long ldb, const void *A0, const void *A1, const void
 hipError_t launch_gemm_pair_f32(int M, int N, int K, int kc, const float *A0, const float *A1, long lda,
                                 const float *B0, const float *B1, long ldb, float *C0, float *C1, float *P,
                                 const int *kend, const int *nk_w, hipStream_t st);
+// The same in FP64 (128 x 128 tiles, kend per 128-row tile; workspace
+// 2 x gemm_pair_workspace_bytes(M, N, K, kc)).
+hipError_t launch_gemm_pair_f64(int M, int N, int K, int kc, const double *A0, const double *A1, long lda,
+                                const double *B0, const double *B1, long ldb, double *C0, double *C1, double *P,
+                                const int *kend, const int *nk_w, hipStream_t st);
 // layout 1 with several models: trace t reads model t / (4 nr), whose
 // contraction outputs start cms elements further per model
 hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr, const void *raw, const void *C0,

@@ -84,7 +84,7 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned v)
 // nk_w (optional): per-row number of wavenumbers solved (adaptive grid);
 // row_peak (optional): per-row max |A| as float bits.
 template <typename T, typename TO, bool COMPAT>
-__global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__restrict__ lw_all, int L,
+__global__ __launch_bounds__(kCellThreads, sizeof(T) == 4 ? 4 : 1) void k_cells_zr(const LayerW<T> *__restrict__ lw_all, int L,
                                                            Geo g, int nk, double dk, double df,
                                                            double omega_i, int w0, int tpr,
                                                            const unsigned *__restrict__ tiles,
@@ -181,7 +181,7 @@ __global__ void k_fallback_select(int nw, int nk, double tol, const unsigned *__
 // operand rows are group rows j nw + iw.  Non-finite FP32 cells go to the
 // flag list as (j nw + iw) nk + ik.
 template <typename T, typename TO, bool COMPAT, int BS>
-__global__ __launch_bounds__(BS) void k_cells_zr_batch(const ModelDesc *__restrict__ md, int m0,
+__global__ __launch_bounds__(BS, sizeof(T) == 4 ? 4 : 1) void k_cells_zr_batch(const ModelDesc *__restrict__ md, int m0,
                                                        const LayerW<T> *__restrict__ lw_all, int nw, int nk,
                                                        double dk, double df, double omega_i,
                                                        const unsigned long long *__restrict__ items,

@@ -184,9 +184,10 @@ void grtm_batch_destroy(grtm_batch *batch);
  * the oracle's cell: a0[2],a1[2],a2[2],b1,b2 and d/dz versions, 18 complex. */
 /* C[M x N] = A[M x K] B[K x N] (row-major host arrays) on the GPU with the
  * contraction kernels: mode -1 VALU FP64, 0 FP64 MFMA, 1 FP32 MFMA,
- * 2 BF16x3 MFMA, 3 streamed FP32 MFMA pair kernel (both of its problems
- * solved on (A, B) and required to agree bit for bit); operands rounded to
- * FP32 first for modes 1-3. */
+ * 2 BF16x3 MFMA, 3 streamed FP32 MFMA pair kernel, 4 streamed FP64 MFMA
+ * pair kernel (modes 3, 4: both of the pair's problems solved on (A, B) and
+ * required to agree bit for bit); operands rounded to FP32 first for modes
+ * 1-3. */
 int grtm_selftest_gemm(int mode, int M, int N, int K, const double *A, const double *B, double *C);
 
 /* Lane-layout probe of the FP64 matrix-core instruction: lane l0 feeds a = 1,

@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("shape", [(256, 256, 512), (200, 40, 300), (8192, 8, 64), (64, 256, 4096),
                                    (130, 37, 1001), (1024, 256, 8192)])
-@pytest.mark.parametrize("mode", [-1, 0, 1, 2, 3])
+@pytest.mark.parametrize("mode", [-1, 0, 1, 2, 3, 4])
 def test_gemm_matches_numpy(mode, shape):
     M, N, K = shape
     rng = np.random.default_rng(M + N + K)
@@ -24,7 +24,7 @@ def test_gemm_matches_numpy(mode, shape):
     else:
         ref = A @ B
     err = np.abs(C - ref).max() / (np.abs(A).max() * np.abs(B).max() * np.sqrt(K))
-    tol = {-1: 1e-14, 0: 1e-14, 1: 1e-6, 2: 2e-5, 3: 1e-6}[mode]
+    tol = {-1: 1e-14, 0: 1e-14, 1: 1e-6, 2: 2e-5, 3: 1e-6, 4: 1e-14}[mode]
     assert err < tol, err
 
 
