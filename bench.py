@@ -88,11 +88,25 @@ def bench_batch(args, g, models):
     ref = mk(g.PREC_FP64)
     build_ref = time.perf_counter() - t_build
     t_ref = time_batch(ref, max(1, args.steps // 2), 1)
-    print(f"bench: c4 {pname} batch", file=sys.stderr, flush=True)
+    print(f"bench: c4 {pname} batch, {args.path} grids", file=sys.stderr, flush=True)
     t_build = time.perf_counter()
     plan = mk(prec)
     build = time.perf_counter() - t_build
+    extra = {}
+    if args.path == "adaptive":
+        from grtm_cuda_amd import learned
+        t0 = time.perf_counter()
+        pred = learned.Predictor()
+        kc = np.stack([pred.kcut(l, zs, cfg["rcv_depth"], cfg["grid"]) for l, zs in cfg["models"]])
+        win = np.array([models.check_window(zs, cfg["rcv_depth"]) for _, zs in cfg["models"]])
+        extra["predictor_s"] = round(time.perf_counter() - t0, 2)
+        plan.set_adaptive(kc, win, stride=8, tol=3e-6)
     t_step = time_batch(plan, args.steps, args.warmup)
+    if args.path == "adaptive":
+        st = plan.adaptive_stats()
+        dense_cells = nm * cfg["grid"]["nw"] * cfg["grid"]["nk"]
+        extra.update(cells_frac=st["cells_final"] / dense_cells, check_frac=st["cells_check"] / dense_cells,
+                     rows_fallback_frac=st["rows_fallback"] / (nm * cfg["grid"]["nw"]))
     worst = 0.0
     for m0 in range(0, nm, 64):
         a, b = plan.traces(m0, min(nm, m0 + 64)), ref.traces(m0, min(nm, m0 + 64))
@@ -101,7 +115,9 @@ def bench_batch(args, g, models):
     work = plan.work()
     tm = plan.timing(reps=1)
     peak = FP64_PEAK_TFLOPS if prec == g.PREC_FP64 else FP32_PEAK_TFLOPS
-    achieved = work["rec_flop"] / (tm["cells"] * 1e-3) / 1e12
+    # counted recursion flops of the cells actually solved (adaptive: the kept fraction)
+    frac = extra.get("cells_frac", 1.0) + extra.get("check_frac", 0.0)
+    achieved = frac * work["rec_flop"] / (tm["cells"] * 1e-3) / 1e12
     layers = [len(l) for l, _ in cfg["models"]]
     return {
         "metric": "Green's traces/s at <=1e-4 rel-L2 vs FP64; learned-adaptive vs plain dense-k",
@@ -119,15 +135,17 @@ def bench_batch(args, g, models):
         "config": {"workload": f"c4: {nm} models of {min(layers)}-{max(layers)} layers (mean {np.mean(layers):.1f}), "
                                f"sources 0-40 km, 64 receivers 5-300 km, Nw={cfg['grid']['nw']}, "
                                f"Nk={cfg['grid']['nk']}, dt={cfg['grid']['dt']}",
-                   "path": f"batched dense {pname}, one captured HIP graph", "traces_per_step": work["traces"]},
+                   "path": f"batched {'learned adaptive' if args.path == 'adaptive' else 'dense'} {pname}, "
+                           f"one captured HIP graph", "traces_per_step": work["traces"]},
         "plain_fp64_batch": {"value": work["traces"] / t_ref, "ms_per_step": 1e3 * t_ref},
         "speedup_vs_plain_fp64": t_ref / t_step,
         "max_rel_l2_vs_fp64": worst,
         "batch": dict(plan.config(), build_s=round(build, 2), build_fp64_s=round(build_ref, 2)),
+        "adaptive": extra or None,
         "kernel_ms": {"recursion": round(tm["cells"], 3), "ksum": round(tm["ksum"], 3), "ifft": round(tm["ifft"], 3)},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None, "kernel": "k_cells_zr_batch", "flops": work["rec_flop"],
-                     "cells": work["cells"]},
+                     "traffic": None, "kernel": "k_cells_zr_batch", "flops": frac * work["rec_flop"],
+                     "cells": frac * work["cells"]},
         "cpu_baseline": None,
     }
 

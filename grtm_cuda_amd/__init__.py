@@ -207,6 +207,25 @@ class BatchPlan:
     def sync(self):
         check(_lib().grtm_batch_sync(self.ptr), "grtm_batch_sync")
 
+    def set_adaptive(self, kcuts, windows, stride=8, tol=3e-6):
+        """Adaptive wavenumber grids (as TracePlan.set_adaptive) for every model:
+        kcuts [n_models, nw] km^-1, windows [n_models] km^-1; kcuts=None
+        returns to the dense grids.  Recaptures the graph."""
+        if kcuts is None:
+            check(_lib().grtm_batch_set_adaptive(self.ptr, None, None, 1, 1.0), "grtm_batch_set_adaptive")
+            return
+        kc = np.ascontiguousarray(kcuts, dtype=np.float64)
+        wn = np.ascontiguousarray(windows, dtype=np.float64)
+        if kc.shape != (self.nm, self.nw) or wn.shape != (self.nm,):
+            raise ValueError("kcuts must be [n_models, nw] and windows [n_models]")
+        check(_lib().grtm_batch_set_adaptive(self.ptr, kc.ctypes.data_as(P), wn.ctypes.data_as(P), int(stride),
+                                             float(tol)), "grtm_batch_set_adaptive")
+
+    def adaptive_stats(self):
+        o = np.zeros(4)
+        check(_lib().grtm_batch_adaptive_stats(self.ptr, o.ctypes.data_as(P)), "grtm_batch_adaptive_stats")
+        return dict(cells_pred=o[0], cells_check=o[1], cells_final=o[2], rows_fallback=int(o[3]))
+
     def traces(self, m0=0, m1=None):
         """traces[m1 - m0, 2 (explosion, vertical force), 2 (Z, R), nr, N] of
         the last execution."""

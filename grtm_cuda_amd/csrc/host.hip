@@ -1209,6 +1209,12 @@ struct grtm_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     size_t nodes = 0;
+    // adaptive wavenumber grids, one per model ([model][row] arrays)
+    bool adaptive = false;
+    int chk_stride = 8;
+    double chk_tol = 3e-6;
+    std::vector<int> nk_pred_h, nchk_h, max_chk;
+    DBuf nk_pred, nk_w, nchk, row_peak, tail_max, fb_count, fb_list, kend;
     ~grtm_batch()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -1221,6 +1227,15 @@ namespace {
 
 bool batch_mixed(const grtm_batch *b) { return b->prec != GRTM_PREC_FP64; }
 
+// Frequency rows per K-extent tile of the contraction, and the offset of a
+// group's tile extents (each group's tiles start at its own first row).
+int batch_rpt(const grtm_batch *b) { return (b->gemm_mode == 0 && !b->pair) ? 16 : 32; }
+size_t batch_kend_per_group(const grtm_batch *b) { return ((size_t)b->nb * b->nw + 15) / 16; }
+size_t batch_kend_off(const grtm_batch *b, const grtm_batch::Group &g)
+{
+    return (size_t)(g.m0 / b->nb) * batch_kend_per_group(b);
+}
+
 // Work items of every group for recursion tiles of bs wavenumbers; within a
 // group the models are taken in decreasing layer count.  Only groups
 // [g0, g1) are built (the autotuner times group 0 alone).
@@ -1228,6 +1243,8 @@ int batch_items(grtm_batch *b, int bs, int g0, int g1)
 {
     const int tpr = (b->nk + bs - 1) / bs;
     if (tpr > 4096) return GRTM_E_UNSUPPORTED;
+    // tiles of a row: all, or those below the row's predicted cut
+    auto ntile = [&](int m, int iw) { return b->adaptive ? (b->nk_pred_h[(size_t)m * b->nw + iw] + bs - 1) / bs : tpr; };
     std::vector<unsigned long long> it;
     for (int gi = g0; gi < g1; gi++) {
         grtm_batch::Group &g = b->groups[gi];
@@ -1241,7 +1258,7 @@ int batch_items(grtm_batch *b, int bs, int g0, int g1)
                 const int r64 = batch_mixed(b) ? b->rows64[g.m0 + j] : b->nw;
                 const int lo = pass == 0 ? 0 : r64, hi = pass == 0 ? r64 : b->nw;
                 for (int iw = lo; iw < hi; iw++)
-                    for (int kb = 0; kb < tpr; kb++)
+                    for (int kb = 0, nt = ntile(g.m0 + j, iw); kb < nt; kb++)
                         it.push_back(((unsigned long long)j << 32) | grtm::cells_tile(iw, kb));
             }
             if (pass == 0) g.off64 = start, g.n64 = it.size() - start;
@@ -1259,43 +1276,83 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
 {
     const unsigned long long *it = b->items.as<unsigned long long>();
     const grtm::ModelDesc *md = b->md.as<grtm::ModelDesc>();
+    const size_t r0 = (size_t)g.m0 * b->nw, nrow = (size_t)g.n * b->nw;
+    int *nkw = b->adaptive ? b->nk_w.as<int>() + r0 : nullptr;
+    unsigned *rowpk = b->adaptive ? b->row_peak.as<unsigned>() + r0 : nullptr;
+    if (b->adaptive) {
+        HIPCHK(grtm::launch_copy_i32(b->nk_pred.as<int>() + r0, nkw, (int)nrow, s));
+        HIPCHK(grtm::launch_zero_u32(rowpk, (int)nrow, s));
+        HIPCHK(grtm::launch_zero_u32(b->tail_max.as<unsigned>() + r0, (int)nrow, s));
+        HIPCHK(grtm::launch_zero_u32(b->fb_count.as<unsigned>() + g.m0, g.n, s));
+    }
     if (!batch_mixed(b)) {
         HIPCHK(grtm::launch_cells_zr_batch(0, b->compat, b->bs, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df,
                                            b->omega_i, it + g.off64, (int)g.n64, b->A0.p, b->A1.p, b->nk, nullptr,
-                                           nullptr, s));
-        return GRTM_OK;
+                                           nullptr, nkw, rowpk, s));
+    } else {
+        unsigned *cnt = b->flags.as<unsigned>(), *list = cnt + 1;
+        HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
+        HIPCHK(grtm::launch_cells_zr_batch(2, b->compat, b->bs, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df,
+                                           b->omega_i, it + g.off64, (int)g.n64, b->A0.p, b->A1.p, b->nk, nullptr,
+                                           nullptr, nkw, rowpk, s));
+        HIPCHK(grtm::launch_cells_zr_batch(1, b->compat, b->bs, md, g.m0, b->lw32.p, b->nw, b->nk, b->dk, b->df,
+                                           b->omega_i, it + g.off32, (int)g.n32, b->A0.p, b->A1.p, b->nk, cnt, list,
+                                           nkw, rowpk, s));
+        HIPCHK(grtm::launch_cells_fix_batch(b->compat, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df, b->omega_i,
+                                            b->A0.p, b->A1.p, b->nk, cnt, list, s));
     }
-    unsigned *cnt = b->flags.as<unsigned>(), *list = cnt + 1;
-    HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
-    HIPCHK(grtm::launch_cells_zr_batch(2, b->compat, b->bs, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df,
-                                       b->omega_i, it + g.off64, (int)g.n64, b->A0.p, b->A1.p, b->nk, nullptr,
-                                       nullptr, s));
-    HIPCHK(grtm::launch_cells_zr_batch(1, b->compat, b->bs, md, g.m0, b->lw32.p, b->nw, b->nk, b->dk, b->df,
-                                       b->omega_i, it + g.off32, (int)g.n32, b->A0.p, b->A1.p, b->nk, cnt, list, s));
-    HIPCHK(grtm::launch_cells_fix_batch(b->compat, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df, b->omega_i,
-                                        b->A0.p, b->A1.p, b->nk, cnt, list, s));
+    if (!b->adaptive) return GRTM_OK;
+    // per model: residual check beyond the cut (FP64 on the FP64 rows), dense
+    // fallback of the rows that fail it, then the group's tile extents
+    const size_t es = esize(b->prec);
+    for (int j = 0; j < g.n; j++) {
+        const int m = g.m0 + j;
+        const grtm::ModelDesc &d = b->md_h[m];
+        const int L = d.g.L;
+        const size_t mr = (size_t)m * b->nw;
+        const int r64 = batch_mixed(b) ? b->rows64[m] : b->nw;
+        const void *l64 = b->lw64.as<LayerW<double>>() + d.lw_off;
+        int *nkw_m = b->nk_w.as<int>() + mr;
+        unsigned *tail_m = b->tail_max.as<unsigned>() + mr;
+        HIPCHK(grtm::launch_check_tail(0, b->compat, l64, L, d.g, b->nk, b->dk, b->df, b->omega_i, 0, r64,
+                                       b->max_chk[m], nkw_m, b->nchk.as<int>() + mr, b->chk_stride, tail_m, s));
+        if (batch_mixed(b))
+            HIPCHK(grtm::launch_check_tail(1, b->compat, b->lw32.as<LayerW<float>>() + d.lw_off, L, d.g, b->nk, b->dk,
+                                           b->df, b->omega_i, r64, b->nw - r64, b->max_chk[m], nkw_m,
+                                           b->nchk.as<int>() + mr, b->chk_stride, tail_m, s));
+        unsigned char *a0 = (unsigned char *)b->A0.p + (size_t)j * 4 * b->nw * b->nk * es;
+        unsigned char *a1 = (unsigned char *)b->A1.p + (size_t)j * 4 * b->nw * b->nk * es;
+        HIPCHK(grtm::launch_fallback(b->prec == GRTM_PREC_FP64, b->compat, l64, L, d.g, b->nk, b->nw, b->dk, b->df,
+                                     b->omega_i, b->chk_tol, b->row_peak.as<unsigned>() + mr, tail_m, nkw_m,
+                                     b->fb_count.as<unsigned>() + m, b->fb_list.as<unsigned>() + 2 * mr, a0, a1,
+                                     b->nk, s));
+    }
+    HIPCHK(grtm::launch_kend_tiles((int)nrow, batch_rpt(b), b->nk, nkw, b->kend.as<int>() + batch_kend_off(b, g), s));
     return GRTM_OK;
 }
 
 int batch_gemm(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
 {
     const int M = 4 * b->nw * g.n;
+    const size_t r0 = (size_t)g.m0 * b->nw;
+    const int *kend = b->adaptive ? b->kend.as<int>() + batch_kend_off(b, g) : nullptr;
+    const int *nkw = b->adaptive ? b->nk_w.as<int>() + r0 : nullptr;
     if (b->pair && b->gemm_mode == 1) {
         HIPCHK(grtm::launch_gemm_pair_f32(M, b->nr, b->nk, b->kc, b->A0.as<float>(), b->A1.as<float>(), b->nk,
                                           b->B0.as<float>(), b->B1.as<float>(), b->nr, b->C0.as<float>(),
-                                          b->C1.as<float>(), b->ws.as<float>(), nullptr, nullptr, s));
+                                          b->C1.as<float>(), b->ws.as<float>(), kend, nkw, s));
         return GRTM_OK;
     }
     if (b->pair && b->gemm_mode == 0) {
         HIPCHK(grtm::launch_gemm_pair_f64(M, b->nr, b->nk, b->kc, b->A0.as<double>(), b->A1.as<double>(), b->nk,
                                           b->B0.as<double>(), b->B1.as<double>(), b->nr, b->C0.as<double>(),
-                                          b->C1.as<double>(), b->ws.as<double>(), nullptr, nullptr, s));
+                                          b->C1.as<double>(), b->ws.as<double>(), kend, nkw, s));
         return GRTM_OK;
     }
     HIPCHK(grtm::launch_gemm_mfma(b->gemm_mode, M, b->nr, b->nk, b->A0.p, b->nk, b->B0.p, b->nr, b->C0.p, b->ws.p,
-                                  nullptr, nullptr, s));
+                                  kend, nkw, s));
     HIPCHK(grtm::launch_gemm_mfma(b->gemm_mode, M, b->nr, b->nk, b->A1.p, b->nk, b->B1.p, b->nr, b->C1.p, b->ws.p,
-                                  nullptr, nullptr, s));
+                                  kend, nkw, s));
     return GRTM_OK;
 }
 
@@ -1511,6 +1568,71 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
     if ((st = batch_capture(b))) return st;
     *out = b;
     b = nullptr;
+    return GRTM_OK;
+}
+
+int grtm_batch_set_adaptive(grtm_batch *b, const double *kcut, const double *window, int stride, double tol)
+{
+    if (!b) return GRTM_E_INPUT;
+    int st;
+    if (!kcut) {
+        b->adaptive = false;
+    } else {
+        if (!window || stride < 1 || !(tol > 0)) return GRTM_E_INPUT;
+        const size_t n = (size_t)b->nm * b->nw;
+        b->nk_pred_h.assign(n, 0);
+        b->nchk_h.assign(n, 0);
+        b->max_chk.assign(b->nm, 0);
+        for (int m = 0; m < b->nm; m++) {
+            if (!(window[m] >= 0)) return GRTM_E_INPUT;
+            for (int iw = 0; iw < b->nw; iw++) {
+                const size_t i = (size_t)m * b->nw + iw;
+                if (!(kcut[i] >= 0)) return GRTM_E_INPUT;
+                const int nkp = (int)std::min<double>(b->nk, std::max(1.0, std::ceil(kcut[i] / b->dk)));
+                const int room = b->nk - nkp;
+                const int nc = std::min((room + stride - 1) / stride, (int)std::ceil(window[m] / (b->dk * stride)));
+                b->nk_pred_h[i] = nkp;
+                b->nchk_h[i] = std::max(0, nc);
+                b->max_chk[m] = std::max(b->max_chk[m], b->nchk_h[i]);
+            }
+        }
+        if (!b->nk_pred.p) {
+            if ((st = b->nk_pred.alloc(sizeof(int) * n)) || (st = b->nk_w.alloc(sizeof(int) * n)) ||
+                (st = b->nchk.alloc(sizeof(int) * n)) || (st = b->row_peak.alloc(sizeof(unsigned) * n)) ||
+                (st = b->tail_max.alloc(sizeof(unsigned) * n)) || (st = b->fb_count.alloc(sizeof(unsigned) * b->nm)) ||
+                (st = b->fb_list.alloc(sizeof(unsigned) * 2 * n)) ||
+                (st = b->kend.alloc(sizeof(int) * b->groups.size() * batch_kend_per_group(b))))
+                return st;
+        }
+        HIPCHK(hipMemcpy(b->nk_pred.p, b->nk_pred_h.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b->nchk.p, b->nchk_h.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+        b->chk_stride = stride;
+        b->chk_tol = tol;
+        b->adaptive = true;
+    }
+    if ((st = batch_items(b, b->bs, 0, (int)b->groups.size()))) return st;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    if ((st = batch_group(b, b->groups[0], b->stream))) return st;  // eager pass outside the capture
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return batch_capture(b);
+}
+
+int grtm_batch_adaptive_stats(grtm_batch *b, double *out)
+{
+    if (!b || !out) return GRTM_E_INPUT;
+    out[0] = out[1] = out[2] = out[3] = 0;
+    if (!b->adaptive) return GRTM_OK;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const size_t n = (size_t)b->nm * b->nw;
+    std::vector<int> nkw(n);
+    std::vector<unsigned> fb(b->nm);
+    HIPCHK(hipMemcpy(nkw.data(), b->nk_w.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(fb.data(), b->fb_count.p, sizeof(unsigned) * b->nm, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) out[0] += b->nk_pred_h[i], out[1] += b->nchk_h[i], out[2] += nkw[i];
+    for (int m = 0; m < b->nm; m++) out[3] += fb[m];
     return GRTM_OK;
 }
 

@@ -95,7 +95,8 @@ struct ModelDesc {
 hipError_t launch_cells_zr_batch(int prec, int compat, int bs, const ModelDesc *md, int m0, const void *lw_all,
                                  int nw, int nk, double dk, double df, double omega_i,
                                  const unsigned long long *items, int nitems, void *A0, void *A1, long lda,
-                                 unsigned *flag_count, unsigned *flag_list, hipStream_t st);
+                                 unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
+                                 hipStream_t st);
 hipError_t launch_cells_fix_batch(int compat, const ModelDesc *md, int m0, const void *lw64, int nw, int nk, double dk,
                                   double df, double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,
                                   const unsigned *flag_list, hipStream_t st);

@@ -179,7 +179,8 @@ __global__ void k_fallback_select(int nw, int nk, double tol, const unsigned *__
 // ordered heaviest model first so the launch ends on light items.  Model
 // m0 + j has its geometry and the offset of its layer table in md; its A
 // operand rows are group rows j nw + iw.  Non-finite FP32 cells go to the
-// flag list as (j nw + iw) nk + ik.
+// flag list as (j nw + iw) nk + ik.  Adaptive grids: nk_w[group row] limits
+// the wavenumbers solved and row_peak[group row] collects the row maxima.
 template <typename T, typename TO, bool COMPAT, int BS>
 __global__ __launch_bounds__(BS, sizeof(T) == 4 ? 4 : 1) void k_cells_zr_batch(const ModelDesc *__restrict__ md, int m0,
                                                        const LayerW<T> *__restrict__ lw_all, int nw, int nk,
@@ -187,25 +188,37 @@ __global__ __launch_bounds__(BS, sizeof(T) == 4 ? 4 : 1) void k_cells_zr_batch(c
                                                        const unsigned long long *__restrict__ items,
                                                        TO *__restrict__ A0, TO *__restrict__ A1, long lda,
                                                        unsigned *__restrict__ flag_count,
-                                                       unsigned *__restrict__ flag_list)
+                                                       unsigned *__restrict__ flag_list,
+                                                       const int *__restrict__ nk_w,
+                                                       unsigned *__restrict__ row_peak)
 {
     __shared__ LayerW<T> lw[kMaxLayers];
     const unsigned long long e = items[blockIdx.x];
     const int j = int(e >> 32);
     const unsigned lo = unsigned(e & 0xffffffffull);
     const int iw = int(lo >> kTileBits), kb = int(lo & ((1u << kTileBits) - 1));
+    const int grow = j * nw + iw;  // row of the group
+    const int nkr = nk_w ? nk_w[grow] : nk;
+    if (kb * BS >= nkr) return;  // uniform over the block
     const Geo g = md[m0 + j].g;
     const size_t off = md[m0 + j].lw_off + (size_t)iw * g.L;
     for (int q = threadIdx.x; q < g.L; q += BS) lw[q] = lw_all[off + q];
     __syncthreads();
     const int ik = kb * BS + threadIdx.x;
-    if (ik >= nk) return;
-    TO v[8];
-    const bool ok = cell_zr<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
-    store_zr(v, j * nw + iw, ik, A0, A1, lda);
-    if (!ok && flag_count) {
-        const unsigned slot = atomicAdd(flag_count, 1u);
-        flag_list[slot] = unsigned(j * nw + iw) * unsigned(nk) + unsigned(ik);
+    unsigned mag = 0;
+    if (ik < nkr) {
+        TO v[8];
+        const bool ok = cell_zr<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
+        store_zr(v, grow, ik, A0, A1, lda);
+        mag = cell_mag_bits(v);
+        if (!ok && flag_count) {
+            const unsigned slot = atomicAdd(flag_count, 1u);
+            flag_list[slot] = unsigned(grow) * unsigned(nk) + unsigned(ik);
+        }
+    }
+    if (row_peak) {
+        mag = wave_max_u32(mag);
+        if ((threadIdx.x & 63) == 0 && mag) atomicMax(row_peak + grow, mag);
     }
 }
 
@@ -827,12 +840,14 @@ hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr,
 hipError_t launch_cells_zr_batch(int prec, int compat, int bs, const ModelDesc *md, int m0, const void *lw_all,
                                  int nw, int nk, double dk, double df, double omega_i,
                                  const unsigned long long *items, int nitems, void *A0, void *A1, long lda,
-                                 unsigned *flag_count, unsigned *flag_list, hipStream_t st)
+                                 unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
+                                 hipStream_t st)
 {
     if (nitems <= 0) return hipSuccess;
 #define GRTM_ZB(T, TO, CP, BSZ, FC, FL)                                                                          \
     hipLaunchKernelGGL((k_cells_zr_batch<T, TO, CP, BSZ>), dim3(nitems), dim3(BSZ), 0, st, md, m0,              \
-                       (const LayerW<T> *)lw_all, nw, nk, dk, df, omega_i, items, (TO *)A0, (TO *)A1, lda, FC, FL)
+                       (const LayerW<T> *)lw_all, nw, nk, dk, df, omega_i, items, (TO *)A0, (TO *)A1, lda, FC, FL, \
+                       nk_w, row_peak)
 #define GRTM_ZB_BS(T, TO, CP, FC, FL)                                \
     do {                                                             \
         if (bs == 64) GRTM_ZB(T, TO, CP, 64, FC, FL);                \

@@ -102,6 +102,8 @@ def _lib():
         lib.grtm_batch_timing.argtypes = [P, I, P]
         lib.grtm_batch_config.argtypes = [P, P]
         lib.grtm_batch_work.argtypes = [P, P]
+        lib.grtm_batch_set_adaptive.argtypes = [P, P, P, I, D]
+        lib.grtm_batch_adaptive_stats.argtypes = [P, P]
         lib.grtm_batch_destroy.argtypes = [P]
         lib.grtm_batch_destroy.restype = None
         _LIB = lib
@@ -120,6 +122,7 @@ EXPORTS = [
     "grtm_mlp_get_params", "grtm_mlp_set_data", "grtm_mlp_train", "grtm_mlp_grads", "grtm_mlp_predict",
     "grtm_mlp_set_loss", "grtm_batch_create", "grtm_batch_execute", "grtm_batch_execute_eager", "grtm_batch_sync",
     "grtm_batch_traces_out", "grtm_batch_timing", "grtm_batch_config", "grtm_batch_work", "grtm_batch_destroy",
+    "grtm_batch_set_adaptive", "grtm_batch_adaptive_stats",
 ]
 
 

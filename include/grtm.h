@@ -157,6 +157,14 @@ typedef struct {
 #define GRTM_BATCH_AUTOTUNE 1
 int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, const double *dists, int nr,
                       const grtm_grid *grid, int precision, int compat, int flags, grtm_batch **batch);
+/* Adaptive wavenumber grids for every model (as grtm_plan_set_adaptive):
+ * kcut[m * nw + iw] km^-1, window[m] km^-1, one stride and tolerance for
+ * all; kcut = NULL returns to the dense grid.  The graph is captured again. */
+int grtm_batch_set_adaptive(grtm_batch *batch, const double *kcut, const double *window, int stride, double tol);
+/* Summed over the models after an execution: out[0] cells of the predicted
+ * grids, out[1] residual-check cells, out[2] cells after fallback, out[3]
+ * rows that fell back. */
+int grtm_batch_adaptive_stats(grtm_batch *batch, double *out);
 /* Replay the captured graph on `stream` (NULL: the batch's stream). */
 int grtm_batch_execute(grtm_batch *batch, void *stream);
 /* The same launches issued one by one on the batch's stream, no graph. */

@@ -105,3 +105,74 @@ def test_batch_rejects_bad_input():
         g.BatchPlan(ms, 0.0, [-1.0], grid)
     with pytest.raises(g.GrtmError):
         g.BatchPlan(ms, 0.0, dists, dict(grid, nw=100))
+
+
+def adaptive_inputs(ms, grid, scale=1.0):
+    from grtm_cuda_amd import learned
+    pred = learned.Predictor()
+    kc = np.stack([pred.kcut(l, zs, 0.0, grid) * scale for l, zs in ms])
+    win = np.array([models.check_window(zs, 0.0) for _, zs in ms])
+    return kc, win
+
+
+@pytest.mark.parametrize("prec,tol", [(g.PREC_FP64, 1e-11), (g.PREC_FP32, 2e-5)])
+def test_batch_adaptive_matches_per_model_plans(prec, tol):
+    """Learned cuts in a batch: the same cells, fallbacks and traces as one
+    adaptive TracePlan per model.  In FP32 the batch's contraction splits its
+    K range at other points than a single-model plan does, so the FP32
+    partial sums round differently (about 1e-5 on far receivers)."""
+    ms = models.batch_models(83, 6, n_min=4, n_max=14, src_lo=2.0, src_hi=30.0)
+    vs_min = min(float(l[:, 2].min()) for l, _ in ms)
+    grid = models.dense_grid(256, 0.05, 2048, vs_min)
+    dists = np.linspace(5.0, 150.0, 8)
+    kc, win = adaptive_inputs(ms, grid)
+    print(f"gpu: adaptive batch of {len(ms)} models, precision {prec}", flush=True)
+    bp = g.BatchPlan(ms, 0.0, dists, grid, precision=prec)
+    bp.set_adaptive(kc, win, stride=8, tol=3e-6)
+    bp.execute()
+    tr = bp.traces()
+    st = bp.adaptive_stats()
+    ref, cells, fb = [], 0.0, 0
+    for i, (l, zs) in enumerate(ms):
+        p = g.TracePlan(l, zs, 0.0, dists, grid, precision=prec)
+        p.set_adaptive(kc[i], window=win[i], stride=8, tol=3e-6)
+        ref.append(p.traces())
+        s = p.adaptive_stats()
+        cells += s["cells_final"]
+        fb += s["rows_fallback"]
+    err = rel_l2(tr, np.stack(ref)).max()
+    print("batch stats", st, "per-model cells", cells, "fallback rows", fb, "max rel-L2", err)
+    assert st["cells_final"] == cells and st["rows_fallback"] == fb
+    assert st["cells_final"] < 0.8 * len(ms) * grid["nw"] * grid["nk"]
+    assert err < tol
+
+
+def test_batch_adaptive_graph_replays():
+    """Two groups with adaptive grids (one model's cut far too short, so rows
+    fall back): graph replays equal the eager result bit for bit, and the
+    traces stay within 1e-4 of the dense FP64 batch."""
+    ms = models.batch_models(92, 70, n_min=4, n_max=9, src_lo=3.0, src_hi=20.0)
+    vs_min = min(float(l[:, 2].min()) for l, _ in ms)
+    grid = models.dense_grid(64, 0.1, 512, vs_min)
+    dists = np.linspace(5.0, 60.0, 4)
+    kc, win = adaptive_inputs(ms, grid)
+    kc[65] *= 0.3
+    print("gpu: adaptive batch of 70 models, graph replays", flush=True)
+    bp = g.BatchPlan(ms, 0.0, dists, grid, precision=g.PREC_FP32)
+    bp.set_adaptive(kc, win, stride=8, tol=3e-6)
+    bp.execute_eager()
+    eager = bp.traces()
+    st = bp.adaptive_stats()
+    assert st["rows_fallback"] > 0
+    for rep in range(3):
+        bp.execute()
+        bp.sync()
+        assert np.array_equal(bp.traces(), eager), rep
+        bp.execute_eager()
+        bp.sync()
+    ref = g.BatchPlan(ms, 0.0, dists, grid, precision=g.PREC_FP64)
+    ref.execute()
+    assert rel_l2(eager, ref.traces()).max() < 1e-4
+    bp.set_adaptive(None, None)
+    bp.execute()
+    assert rel_l2(bp.traces(), ref.traces()).max() < 1e-4
