@@ -269,6 +269,9 @@ double cal_kmax(cd w, const cd *b, const Stack &st, int nf)
 struct DBuf {
     void *p = nullptr;
     size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;  // owns device memory: no copies
+    DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { release(); }
     void release()
     {
@@ -623,21 +626,28 @@ int run_cells(grtm_plan *p, hipStream_t s)
         HIPCHK(grtm::launch_cells_zr(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->rows64,
                                      p->nw - p->rows64, t_hi, n_hi, p->A0.p, p->A1.p, p->nk, cnt, list,
                                      nkw, rowpk, s));
+    }
+    const bool mixed = p->prec != GRTM_PREC_FP64;
+    unsigned *cnt = mixed ? p->flags.as<unsigned>() : nullptr, *list = mixed ? cnt + 1 : nullptr;
+    if (ad) {
+        // residual check beyond the cut (FP64 on the FP64 rows), dense fallback
+        // of the rows that fail it (each row in its own precision)
+        const void *lwf64 = mixed ? p->lw64.p : p->lw.p;
+        HIPCHK(grtm::launch_check_tail(0, p->compat, lwf64, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, r64,
+                                       p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s));
+        HIPCHK(grtm::launch_check_tail(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, r64,
+                                       p->nw - r64, p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s));
+        HIPCHK(grtm::launch_fallback(!mixed, p->compat, lwf64, mixed ? p->lw.p : nullptr, r64, p->L, p->g, p->nk,
+                                     p->nw, p->dk, p->df, p->omega_i, p->chk_tol, rowpk, tailmx, nkw, fbcnt,
+                                     p->fb_list.as<unsigned>(), p->A0.p, p->A1.p, p->nk, cnt, list, 0, s));
+    }
+    // FP64 re-solve of the non-finite FP32 cells (main grid and fallback rows)
+    if (mixed)
         HIPCHK(grtm::launch_cells_fix(p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->A0.p,
                                       p->A1.p, p->nk, cnt, list, s));
-    }
-    if (!ad) return GRTM_OK;
-    // residual check beyond the cut (FP64 on the FP64 rows), dense fallback, tile extents
-    const void *lwf64 = p->prec == GRTM_PREC_FP64 ? p->lw.p : p->lw64.p;
-    HIPCHK(grtm::launch_check_tail(0, p->compat, lwf64, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, r64,
-                                   p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s));
-    HIPCHK(grtm::launch_check_tail(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, r64,
-                                   p->nw - r64, p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s));
-    HIPCHK(grtm::launch_fallback(p->prec == GRTM_PREC_FP64, p->compat, lwf64, p->L, p->g, p->nk, p->nw, p->dk, p->df,
-                                 p->omega_i, p->chk_tol, rowpk, tailmx, nkw, fbcnt, p->fb_list.as<unsigned>(), p->A0.p,
-                                 p->A1.p, p->nk, s));
-    HIPCHK(grtm::launch_kend_tiles(p->nw, (p->gemm_mode == 0 && !p->pair) ? 16 : 32, p->nk, nkw,
-                                   p->kend.as<int>(), s));
+    if (ad)
+        HIPCHK(grtm::launch_kend_tiles(p->nw, (p->gemm_mode == 0 && !p->pair) ? 16 : 32, p->nk, nkw,
+                                       p->kend.as<int>(), s));
     return GRTM_OK;
 }
 int run_gemm(grtm_plan *p, hipStream_t s)
@@ -1298,10 +1308,14 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
         HIPCHK(grtm::launch_cells_zr_batch(1, b->compat, b->bs, md, g.m0, b->lw32.p, b->nw, b->nk, b->dk, b->df,
                                            b->omega_i, it + g.off32, (int)g.n32, b->A0.p, b->A1.p, b->nk, cnt, list,
                                            nkw, rowpk, s));
-        HIPCHK(grtm::launch_cells_fix_batch(b->compat, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df, b->omega_i,
-                                            b->A0.p, b->A1.p, b->nk, cnt, list, s));
     }
-    if (!b->adaptive) return GRTM_OK;
+    unsigned *cnt = batch_mixed(b) ? b->flags.as<unsigned>() : nullptr, *list = cnt ? cnt + 1 : nullptr;
+    if (!b->adaptive) {
+        if (cnt)
+            HIPCHK(grtm::launch_cells_fix_batch(b->compat, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df,
+                                                b->omega_i, b->A0.p, b->A1.p, b->nk, cnt, list, s));
+        return GRTM_OK;
+    }
     // per model: residual check beyond the cut (FP64 on the FP64 rows), dense
     // fallback of the rows that fail it, then the group's tile extents
     const size_t es = esize(b->prec);
@@ -1322,11 +1336,15 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
                                            b->nchk.as<int>() + mr, b->chk_stride, tail_m, s));
         unsigned char *a0 = (unsigned char *)b->A0.p + (size_t)j * 4 * b->nw * b->nk * es;
         unsigned char *a1 = (unsigned char *)b->A1.p + (size_t)j * 4 * b->nw * b->nk * es;
-        HIPCHK(grtm::launch_fallback(b->prec == GRTM_PREC_FP64, b->compat, l64, L, d.g, b->nk, b->nw, b->dk, b->df,
-                                     b->omega_i, b->chk_tol, b->row_peak.as<unsigned>() + mr, tail_m, nkw_m,
-                                     b->fb_count.as<unsigned>() + m, b->fb_list.as<unsigned>() + 2 * mr, a0, a1,
-                                     b->nk, s));
+        HIPCHK(grtm::launch_fallback(!batch_mixed(b), b->compat, l64,
+                                     batch_mixed(b) ? (const void *)(b->lw32.as<LayerW<float>>() + d.lw_off) : nullptr,
+                                     r64, L, d.g, b->nk, b->nw, b->dk, b->df, b->omega_i, b->chk_tol,
+                                     b->row_peak.as<unsigned>() + mr, tail_m, nkw_m, b->fb_count.as<unsigned>() + m,
+                                     b->fb_list.as<unsigned>() + 2 * mr, a0, a1, b->nk, cnt, list, j * b->nw, s));
     }
+    if (cnt)
+        HIPCHK(grtm::launch_cells_fix_batch(b->compat, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df, b->omega_i,
+                                            b->A0.p, b->A1.p, b->nk, cnt, list, s));
     HIPCHK(grtm::launch_kend_tiles((int)nrow, batch_rpt(b), b->nk, nkw, b->kend.as<int>() + batch_kend_off(b, g), s));
     return GRTM_OK;
 }

@@ -35,10 +35,15 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
 hipError_t launch_check_tail(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk, double df,
                              double omega_i, int w0, int nwr, int max_chk, const int *nk_w, const int *nchk,
                              int stride, unsigned *tail_max, hipStream_t st);
-hipError_t launch_fallback(int out_f64, int compat, const void *lw64, int L, const Geo &g, int nk, int nw, double dk,
-                           double df, double omega_i, double tol, const unsigned *row_peak, const unsigned *tail_max,
-                           int *nk_w, unsigned *fb_count, unsigned *fb_list, void *A0, void *A1, long lda,
-                           hipStream_t st);
+// Rows whose checked tail exceeds tol x row peak are completed on the dense
+// grid: FP64 plans in FP64; mixed plans in FP64 below row r64 and in FP32
+// (lw32) above it, non-finite FP32 cells appended to the flag list as
+// (row_base + row) nk + k.
+hipError_t launch_fallback(int out_f64, int compat, const void *lw64, const void *lw32, int r64, int L, const Geo &g,
+                           int nk, int nw, double dk, double df, double omega_i, double tol,
+                           const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *fb_count,
+                           unsigned *fb_list, void *A0, void *A1, long lda, unsigned *flag_count,
+                           unsigned *flag_list, int row_base, hipStream_t st);
 hipError_t launch_kend_tiles(int nw, int rows_per_tile, int nk, const int *nk_w, int *kend, hipStream_t st);
 hipError_t launch_copy_i32(const int *a, int *b, int n, hipStream_t st);
 // per-row training labels (peak, truncation index, 8 local maxima); out: nw x 26 floats

@@ -244,28 +244,34 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_fix_batch(const ModelDes
     }
 }
 
-// Dense completion of fallback rows in FP64: wavenumbers [start, nk) of each
-// listed row.  Grid (k blocks, row slots); slots stride over the list.
-template <typename TO, bool COMPAT>
-__global__ __launch_bounds__(kCellThreads) void k_fallback_cells(const LayerW<double> *__restrict__ lw_all, int L,
-                                                                 Geo g, int nk, int nw, double dk, double df,
-                                                                 double omega_i, TO *__restrict__ A0,
-                                                                 TO *__restrict__ A1, long lda,
-                                                                 const unsigned *__restrict__ fb_count,
-                                                                 const unsigned *__restrict__ fb_list)
+// Dense completion of fallback rows: wavenumbers [start, nk) of each listed
+// row whose index lies in [r_lo, r_hi), in the arithmetic T of that row's
+// recursion.  Grid (k blocks, row slots); slots stride over the list.  FP32
+// cells that are not finite go to the flag list as (row_base + iw) nk + ik.
+template <typename T, typename TO, bool COMPAT>
+__global__ __launch_bounds__(kCellThreads, sizeof(T) == 4 ? 4 : 1) void k_fallback_cells(
+    const LayerW<T> *__restrict__ lw_all, int L, Geo g, int nk, int nw, double dk, double df, double omega_i,
+    int r_lo, int r_hi, TO *__restrict__ A0, TO *__restrict__ A1, long lda, const unsigned *__restrict__ fb_count,
+    const unsigned *__restrict__ fb_list, unsigned *__restrict__ flag_count, unsigned *__restrict__ flag_list,
+    int row_base)
 {
-    __shared__ LayerW<double> lw[kMaxLayers];
+    __shared__ LayerW<T> lw[kMaxLayers];
     const unsigned n = *fb_count;
     for (unsigned slot = blockIdx.y; slot < n; slot += gridDim.y) {
         const int iw = int(fb_list[slot]);
+        if (iw < r_lo || iw >= r_hi) continue;  // uniform over the block
         const int start = int(fb_list[nw + slot]);
         __syncthreads();
         for (int q = threadIdx.x; q < L; q += blockDim.x) lw[q] = lw_all[(size_t)iw * L + q];
         __syncthreads();
         for (int ik = start + blockIdx.x * blockDim.x + threadIdx.x; ik < nk; ik += gridDim.x * blockDim.x) {
             TO v[8];
-            cell_zr<double, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
+            const bool ok = cell_zr<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
             store_zr(v, iw, ik, A0, A1, lda);
+            if (!ok && flag_count) {
+                const unsigned slot2 = atomicAdd(flag_count, 1u);
+                flag_list[slot2] = unsigned(row_base + iw) * unsigned(nk) + unsigned(ik);
+            }
         }
     }
 }
@@ -701,23 +707,30 @@ hipError_t launch_check_tail(int prec, int compat, const void *lw, int L, const 
     return hipGetLastError();
 }
 
-hipError_t launch_fallback(int out_f64, int compat, const void *lw64, int L, const Geo &g, int nk, int nw, double dk,
-                           double df, double omega_i, double tol, const unsigned *row_peak, const unsigned *tail_max,
-                           int *nk_w, unsigned *fb_count, unsigned *fb_list, void *A0, void *A1, long lda,
-                           hipStream_t st)
+hipError_t launch_fallback(int out_f64, int compat, const void *lw64, const void *lw32, int r64, int L, const Geo &g,
+                           int nk, int nw, double dk, double df, double omega_i, double tol,
+                           const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *fb_count,
+                           unsigned *fb_list, void *A0, void *A1, long lda, unsigned *flag_count,
+                           unsigned *flag_list, int row_base, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fallback_select, dim3((nw + 255) / 256), dim3(256), 0, st, nw, nk, tol, row_peak, tail_max,
                        nk_w, fb_count, fb_list);
     dim3 grid((nk + kCellThreads - 1) / kCellThreads, 16);
-#define GRTM_FB(TO, CP)                                                                                 \
-    hipLaunchKernelGGL((k_fallback_cells<TO, CP>), grid, dim3(kCellThreads), 0, st, (const LayerW<double> *)lw64, \
-                       L, g, nk, nw, dk, df, omega_i, (TO *)A0, (TO *)A1, lda, fb_count, fb_list)
+#define GRTM_FB(T, TO, CP, LWP, LO, HI, FC, FL)                                                                  \
+    hipLaunchKernelGGL((k_fallback_cells<T, TO, CP>), grid, dim3(kCellThreads), 0, st, (const LayerW<T> *)LWP, L, g, \
+                       nk, nw, dk, df, omega_i, LO, HI, (TO *)A0, (TO *)A1, lda, fb_count, fb_list, FC, FL, row_base)
     if (out_f64) {
-        if (compat) GRTM_FB(double, true);
-        else GRTM_FB(double, false);
+        if (compat) GRTM_FB(double, double, true, lw64, 0, nw, nullptr, nullptr);
+        else GRTM_FB(double, double, false, lw64, 0, nw, nullptr, nullptr);
     } else {
-        if (compat) GRTM_FB(float, true);
-        else GRTM_FB(float, false);
+        // rows below r64 complete in FP64, the others in FP32 like the rest of their row
+        if (compat) {
+            GRTM_FB(double, float, true, lw64, 0, r64, nullptr, nullptr);
+            GRTM_FB(float, float, true, lw32, r64, nw, flag_count, flag_list);
+        } else {
+            GRTM_FB(double, float, false, lw64, 0, r64, nullptr, nullptr);
+            GRTM_FB(float, float, false, lw32, r64, nw, flag_count, flag_list);
+        }
     }
 #undef GRTM_FB
     return hipGetLastError();
