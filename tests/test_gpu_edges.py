@@ -2,7 +2,8 @@
 (physics mode), with per-trace rel-L2 and peak error: one layer over a
 half-space, a source in the half-space, buried receivers above and below the
 source and in the source layer, and small or odd grids (one receiver, 4
-frequencies, K not a multiple of the contraction tiles)."""
+frequencies, K not a multiple of the contraction tiles); and source-receiver
+reciprocity of the vertical-force response on the GPU path."""
 import numpy as np
 import pytest
 
@@ -48,3 +49,19 @@ def test_edge_geometry_matches_oracle(orc, case):
         rl, pk = errors(tr, ref)
         print(f"  precision {prec}: max rel-L2 {rl.max():.2e}, max peak error {pk.max():.2e}")
         assert rl.max() < tol and pk.max() < tol, (name, prec)
+
+
+@pytest.mark.parametrize("prec,tol", [(g.PREC_FP64, 1e-11), (g.PREC_FP32, 1e-5)])
+def test_vertical_force_reciprocity(prec, tol):
+    """Vertical displacement at depth b from a vertical force at depth a equals
+    that at a from the force at b, on the GPU path (every receiver branch
+    against its mirror)."""
+    grid = models.dense_grid(128, 0.1, 1024, float(models.C1_LAYERS[:, 2].min()))
+    dists = [15.0, 40.0]
+    print(f"gpu: reciprocity, precision {prec}", flush=True)
+    for a, b in ((10.0, 0.0), (20.0, 5.0), (5.0, 25.0), (3.0, 10.0)):
+        z1 = g.TracePlan(models.C1_LAYERS, a, b, dists, grid, precision=prec).traces()[1, 0]
+        z2 = g.TracePlan(models.C1_LAYERS, b, a, dists, grid, precision=prec).traces()[1, 0]
+        err = (np.linalg.norm(z1 - z2, axis=-1) / np.linalg.norm(z1, axis=-1)).max()
+        print(f"  force {a} km, receiver {b} km: {err:.2e}")
+        assert err < tol, (a, b)

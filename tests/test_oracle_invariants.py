@@ -118,3 +118,21 @@ def test_rayleigh_pole_location(orc):
     w = 2 * np.pi * iw / T
     kpk = ks[np.argmax(np.abs(kern))]
     assert abs(kpk - w / cr) < 2e-3
+
+
+# (force depth, receiver depth) pairs covering every receiver branch and its mirror:
+# receiver above the source in another layer, below it, in the source layer, at the surface
+RECIPROCAL_PAIRS = ((10.0, 0.0), (20.0, 5.0), (5.0, 25.0), (3.0, 10.0))
+
+
+def test_vertical_force_reciprocity(orc):
+    """Elastodynamic reciprocity: the vertical displacement at depth b from a
+    vertical force at depth a equals that at a from the force at b (physics
+    mode; surface receivers and surface forces included).  A wrong sign or
+    index in one receiver branch breaks the equality with its mirror."""
+    grid = models.dense_grid(128, 0.1, 1024, float(models.C1_LAYERS[:, 2].min()))
+    dists = [15.0, 40.0]
+    for a, b in RECIPROCAL_PAIRS:
+        z1 = orc.traces(models.C1_LAYERS, a, b, dists, 1, grid)[0][:, 0]
+        z2 = orc.traces(models.C1_LAYERS, b, a, dists, 1, grid)[0][:, 0]
+        assert (np.linalg.norm(z1 - z2, axis=-1) / np.linalg.norm(z1, axis=-1)).max() < 1e-11, (a, b)
