@@ -152,6 +152,51 @@ def bench_batch(args, g, models):
     }
 
 
+def bench_precision(args, g, models):
+    """Precision study: the 3-layer model and geometry, every trace of each
+    mixed precision against the plain FP64 path (itself within 1e-7 of the
+    FP64 oracle in the tests); and the contraction kernels at the production
+    shape (K = 16384 wavenumbers, 256 receivers, M = 4 x 2048 operand rows,
+    i.e. four [2048 x 16384] x [16384 x 256] products)."""
+    cfg = models.config("c1")
+    ref = g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
+                      precision=g.PREC_FP64).traces()
+    errors = {}
+    for pname, prec in (("fp32", g.PREC_FP32), ("bf16x3", g.PREC_BF16X3)):
+        tr = g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
+                         precision=prec).traces()
+        d = tr - ref
+        rl = np.linalg.norm(d, axis=-1) / np.linalg.norm(ref, axis=-1)
+        pk = np.abs(d).max(axis=-1) / np.abs(ref).max(axis=-1)
+        errors[pname] = dict(max_rel_l2=float(rl.max()), median_rel_l2=float(np.median(rl)),
+                             max_peak=float(pk.max()), traces=int(rl.size))
+    c2 = models.config("c2")
+    gemm = {}
+    for pname, prec, peak in (("fp64", g.PREC_FP64, FP64_PEAK_TFLOPS), ("fp32", g.PREC_FP32, FP32_PEAK_TFLOPS),
+                              ("bf16x3", g.PREC_BF16X3, BF16_PEAK_TFLOPS / 3)):
+        plan = g.TracePlan(c2["layers"], c2["src_depth"], c2["rcv_depth"], c2["dists"], c2["grid"], precision=prec)
+        plan.execute()
+        tm = plan.timing(reps=max(2, args.steps // 3))
+        flop = plan.work()["ksum_flop"]
+        gemm[pname] = dict(ms=tm["ksum"], tflops=flop / (tm["ksum"] * 1e-3) / 1e12,
+                           frac=flop / (tm["ksum"] * 1e-3) / 1e12 / peak, peak=peak)
+    fp32 = gemm["fp32"]
+    return {
+        "metric": "Green's traces/s at <=1e-4 rel-L2 vs FP64; learned-adaptive vs plain dense-k",
+        "value": None, "unit": "traces/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 / f32+bf16x3", "data": "synthetic (3-layer model; seeded 8-layer model for the contraction)",
+        "config": {"workload": "c3: 3 layers over a half-space, explosion at 10 km, 8 receivers 10-80 km, "
+                               "Nw=512, Nk=4096, dt=0.1 (errors); contraction at Nw=2048, Nk=16384, 256 receivers",
+                   "path": "precision study"},
+        "errors_vs_fp64": errors,
+        "contraction": gemm,
+        "roofline": {"bound": "mfma", "achieved": fp32["tflops"], "peak": fp32["peak"], "unit": "TFLOP/s",
+                     "frac": fp32["frac"], "traffic": None, "kernel": "k_gemm_f32s"},
+        "cpu_baseline": None,
+    }
+
+
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "c2_adaptive_fp32_pmc_bytes.md")
 
 
@@ -198,8 +243,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c4"],
-                    help="c2: one model, 256 receivers (the headline); c4: batch of --models models")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"],
+                    help="c2: one model, 256 receivers (the headline); c3: precision study; "
+                         "c4: batch of --models models")
     ap.add_argument("--models", type=int, default=1024, help="c4: models in the batch")
     ap.add_argument("--path", default="adaptive", choices=["adaptive", "dense"],
                     help="adaptive: learned truncated grid, mixed precision (the headline); dense: plain grid")
@@ -218,6 +264,9 @@ def main():
 
     if args.workload == "c4":
         print(json.dumps(bench_batch(args, g, models)))
+        return
+    if args.workload == "c3":
+        print(json.dumps(bench_precision(args, g, models)))
         return
     if args.copy_probe:
         from grtm_cuda_amd.ffi import _lib, check
