@@ -84,7 +84,7 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned v)
 // nk_w (optional): per-row number of wavenumbers solved (adaptive grid);
 // row_peak (optional): per-row max |A| as float bits.
 template <typename T, typename TO, bool COMPAT>
-__global__ __launch_bounds__(kCellThreads, sizeof(T) == 4 ? 4 : 1) void k_cells_zr(const LayerW<T> *__restrict__ lw_all, int L,
+__global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__restrict__ lw_all, int L,
                                                            Geo g, int nk, double dk, double df,
                                                            double omega_i, int w0, int tpr,
                                                            const unsigned *__restrict__ tiles,
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_fix_batch(const ModelDes
 // recursion.  Grid (k blocks, row slots); slots stride over the list.  FP32
 // cells that are not finite go to the flag list as (row_base + iw) nk + ik.
 template <typename T, typename TO, bool COMPAT>
-__global__ __launch_bounds__(kCellThreads, sizeof(T) == 4 ? 4 : 1) void k_fallback_cells(
+__global__ __launch_bounds__(kCellThreads) void k_fallback_cells(
     const LayerW<T> *__restrict__ lw_all, int L, Geo g, int nk, int nw, double dk, double df, double omega_i,
     int r_lo, int r_hi, TO *__restrict__ A0, TO *__restrict__ A1, long lda, const unsigned *__restrict__ fb_count,
     const unsigned *__restrict__ fb_list, unsigned *__restrict__ flag_count, unsigned *__restrict__ flag_list,
