@@ -72,6 +72,8 @@ class TracePlan:
     distances.  ``layers`` rows: (thickness km, vp, vs, rho, qp, qs), last row
     the half-space.  ``grid``: dict with nw, dt, nk, kmax, omega_i, t0, taper."""
 
+    _CREATE = "grtm_plan_traces"
+
     def __init__(self, layers, src_depth, rcv_depth, dists, grid, precision=PREC_FP64, compat=False, device=0):
         self._ctx = _ctx(device)
         self.layers = np.ascontiguousarray(layers, dtype=np.float64)
@@ -87,8 +89,8 @@ class TracePlan:
         j = Job(self.layers.shape[0], self.layers.ctypes.data_as(P), float(src_depth), float(rcv_depth), self.nr,
                 self.dists.ctypes.data_as(P))
         self.ptr = ctypes.c_void_p()
-        check(_lib().grtm_plan_traces(self._ctx.ptr, ctypes.byref(j), ctypes.byref(g), int(precision),
-                                      int(bool(compat)), ctypes.byref(self.ptr)), "grtm_plan_traces")
+        check(getattr(_lib(), self._CREATE)(self._ctx.ptr, ctypes.byref(j), ctypes.byref(g), int(precision),
+                                            int(bool(compat)), ctypes.byref(self.ptr)), self._CREATE)
 
     def execute(self):
         check(_lib().grtm_plan_execute(self.ptr, None), "grtm_plan_execute")
@@ -173,6 +175,29 @@ def selftest_cell(compat, src_type, k, w, lay, s_layer, zs, r_layer, zr, par0_rv
                                     out.ctypes.data_as(P)), "grtm_selftest_cell")
     return out[0::2] + 1j * out[1::2]
 
+
+
+# Displacement columns of a ColumnPlan (the reference's integrand columns,
+# cusrc/popm/func/green.cu:1582-1586 single force, 1651-1662 tensor).
+COLUMNS = ("tensor R n2", "tensor R n1", "tensor R n0 s2", "tensor R n0 s0", "tensor T n2", "tensor T n1",
+           "tensor Z n2", "tensor Z n1", "tensor Z n0 s0", "tensor Z n0 s2",
+           "hforce R", "vforce R", "hforce T", "hforce Z", "vforce Z")
+N_COLUMNS = len(COLUMNS)
+
+
+class ColumnPlan(TracePlan):
+    """All displacement columns of both sources (tensor columns 0-9 and
+    single-force columns 0-4 of green()'s output) of one layered model at many
+    distances from one recursion, on a dense wavenumber grid.  Same arguments
+    as TracePlan; traces() returns [15 (COLUMNS), nr, N]."""
+
+    _CREATE = "grtm_plan_columns"
+
+    def traces(self):
+        self.execute()
+        out = np.zeros((N_COLUMNS, self.nr, self.N))
+        check(_lib().grtm_plan_traces_out(self.ptr, out.ctypes.data_as(P)), "grtm_plan_traces_out")
+        return out
 
 class BatchPlan:
     """Explosion and vertical-force Z/R traces of many layered models on one

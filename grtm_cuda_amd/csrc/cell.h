@@ -575,4 +575,55 @@ GRTM_HD void zr_from_cell(const CellOut<T> &c, int up, T k, cx<T> w, cx<T> &kz_e
     kr_vf = -(v0 * k);
 }
 
+// Displacement columns of both sources for many receivers: tensor columns
+// 0-9 (green.cu:1651-1662) and single-force columns 0-4 (green.cu:1582-1586).
+// With k J_n'(k r) = n J_n(k r) / r - k J_{n+1}(k r), the reference's
+// d_besselj form (green.cu:1530-1541), every column is a sum of terms
+// coef(w, k) J_m(k r) r^p, so the columns are assembled after the k-sum from
+// four contractions against J0..J3 tables, the r^p factors applied per
+// receiver.  Complex rows per Bessel order (t: tensor, f: single force):
+//   J0: t.a0[1] k, t.a2[1] k, f.a0[1] k
+//   J1: t.a2[q] k, t.a0[q] k, t.a1[1] k, t.b1 + t.a1[0], f.a0[0] k, f.a1[1] k,
+//       f.b1 + f.a1[0], and in compat t.b2 + t.a1[0]
+//   J2: t.b2 + t.a2[0], t.a1[0] k, t.b5 k, t.a2[1] k, f.a1[0] k, f.b1 k
+//   J3: t.a2[0] k, t.b2 k
+// with q = 0 (compat: 1) and b5 = b1 (compat: b2), the index slips of the
+// reference's columns 2, 3 and 5 (green.cu:1653-1657).  put(set, row, value)
+// receives them in this order.
+constexpr int kMtSets = 4;
+GRTM_HD constexpr int mt_set_rows(int compat, int set)
+{
+    return set == 0 ? 3 : set == 1 ? (compat ? 8 : 7) : set == 2 ? 6 : 2;
+}
+
+template <typename T, bool COMPAT, typename Put>
+GRTM_HD void mt_rows_from_cell(const CellOut<T> &c, int up, T k, cx<T> w, Put &&put)
+{
+    {
+        Coef<T> t;
+        coef_from_cell<T, false>(c, 1, up, k, w, t);
+        put(0, 0, t.a0[1] * k);
+        put(0, 1, t.a2[1] * k);
+        put(1, 0, (COMPAT ? t.a2[1] : t.a2[0]) * k);
+        put(1, 1, (COMPAT ? t.a0[1] : t.a0[0]) * k);
+        put(1, 2, t.a1[1] * k);
+        put(1, 3, t.b1 + t.a1[0]);
+        if (COMPAT) put(1, 7, t.b2 + t.a1[0]);
+        put(2, 0, t.b2 + t.a2[0]);
+        put(2, 1, t.a1[0] * k);
+        put(2, 2, (COMPAT ? t.b2 : t.b1) * k);
+        put(2, 3, t.a2[1] * k);
+        put(3, 0, t.a2[0] * k);
+        put(3, 1, t.b2 * k);
+    }
+    Coef<T> f;
+    coef_from_cell<T, false>(c, 0, up, k, w, f);
+    put(0, 2, f.a0[1] * k);
+    put(1, 4, f.a0[0] * k);
+    put(1, 5, f.a1[1] * k);
+    put(1, 6, f.b1 + f.a1[0]);
+    put(2, 4, f.a1[0] * k);
+    put(2, 5, f.b1 * k);
+}
+
 }  // namespace grtm

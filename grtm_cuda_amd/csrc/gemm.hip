@@ -148,24 +148,37 @@ __global__ __launch_bounds__(256) void k_gemm_f32(int M, int N, int K, int kslic
 // multiples of 4 and 16-byte aligned A, B.
 constexpr int kSBM = 128, kSBN = 128, kSBK = 32, kSPA = kSBK + 4, kSPB = kSBN + 4;
 
-struct GemmPair {
-    const float *A[2];
-    const float *B[2];
-    float *P[2];
+// Up to four contractions C_p = A_p B_p with a shared N, K and strides and
+// their own row counts, in one launch: grid z = p S + slice (contraction
+// outermost, then slice), grid y = row tiles of the largest M (blocks past a
+// smaller contraction's rows return at once).
+template <typename T>
+struct GemmSetT {
+    const T *A[4];
+    const T *B[4];
+    T *P[4];             // split-K partials of contraction p: [S][M_p][N]
+    T *C[4];
+    const int *kend[4];  // per row tile K extent (null: K)
+    int M[4];            // rows of contraction p
+    int rpw[4];          // rows per frequency row of nk_w
 };
+using GemmSet = GemmSetT<float>;
 
-__global__ __launch_bounds__(256, 2) void k_gemm_f32s(int M, int N, int K, int kc, int S, GemmPair gp, long lda,
-                                                      long ldb, const int *__restrict__ kend,
+__global__ __launch_bounds__(256, 2) void k_gemm_f32s(int N, int K, int kc, int S, GemmSet gp, long lda, long ldb,
                                                       const int *__restrict__ nk_w)
 {
     __shared__ __attribute__((aligned(16))) float As[2][kSBM][kSPA];
     __shared__ __attribute__((aligned(16))) float Bs[2][kSBK][kSPB];
     const int pi = blockIdx.z / S, z = blockIdx.z - pi * S;
+    const int M = gp.M[pi];
+    const int m0 = blockIdx.y * kSBM, n0 = blockIdx.x * kSBN;
+    if (m0 >= M) return;  // uniform: a contraction with fewer row tiles
     const float *__restrict__ A = gp.A[pi];
     const float *__restrict__ B = gp.B[pi];
+    const int *__restrict__ kend = gp.kend[pi];
+    const int rpw = gp.rpw[pi];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv >> 1, wn = wv & 1;
-    const int m0 = blockIdx.y * kSBM, n0 = blockIdx.x * kSBN;
     const int kt = min(K, kend ? kend[blockIdx.y] : K);
     const int kb = z * kc;
     if (kb >= kt) return;  // uniform: this slice is past the tile's extent
@@ -176,7 +189,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32s(int M, int N, int K, int k
     int a_lim[4];
     for (int e = 0; e < 4; e++) {
         const int gm = m0 + ((tid + 256 * e) >> 3);
-        a_lim[e] = gm < M ? (nk_w ? min(ke, nk_w[gm >> 2]) : ke) : 0;
+        a_lim[e] = gm < M ? (nk_w ? min(ke, nk_w[gm / rpw]) : ke) : 0;
     }
     float4 ra[4], rb[4];
     auto load = [&](int k0) {
@@ -258,25 +271,29 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f32s(int M, int N, int K, int k
             }
 }
 
-// Sum of the live split-K slices of the two contractions, in FP64, in slice
-// order: C_p[row] = sum_{z < ceil(kend[row / bm] / kc)} P_p[z][row].
+// Sum of the live split-K slices of every contraction of a set, in FP64, in
+// slice order: C_p[row] = sum_{z < ceil(kend_p[row / bm] / kc)} P_p[z][row].
+// Thread t covers element t of the concatenated outputs.
 template <typename T>
-__global__ void k_gemm_reduce_ragged(int M, int N, int K, int kc, int bm, const int *__restrict__ kend,
-                                     const T *__restrict__ P0, const T *__restrict__ P1, T *__restrict__ C0,
-                                     T *__restrict__ C1)
+__global__ void k_gemm_reduce_set(int n, int N, int K, int kc, int bm, GemmSetT<T> gs, long total)
 {
-    const long MN = (long)M * N;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * MN) return;
-    const int pi = t >= MN;
-    const long e = t - pi * MN;
+    if (t >= total) return;
+    int pi = 0;
+    long e = t;
+    while (pi + 1 < n && e >= (long)gs.M[pi] * N) {
+        e -= (long)gs.M[pi] * N;
+        pi++;
+    }
+    const long MN = (long)gs.M[pi] * N;
     const int row = int(e / N);
+    const int *kend = gs.kend[pi];
     const int kt = min(K, kend ? kend[row / bm] : K);
     const int nz = (kt + kc - 1) / kc;
-    const T *P = pi ? P1 : P0;
+    const T *P = gs.P[pi];
     double s = 0;
     for (int zz = 0; zz < nz; zz++) s += double(P[(size_t)zz * MN + e]);
-    (pi ? C1 : C0)[e] = T(s);
+    gs.C[pi][e] = T(s);
 }
 
 // ------------------------------------------------------------------ FP64, streamed
@@ -289,24 +306,23 @@ __global__ void k_gemm_reduce_ragged(int M, int N, int K, int kc, int bm, const 
 // A, B.
 constexpr int kDBM = 128, kDBN = 128, kDBK = 16, kDPA = kDBK + 2, kDPB = kDBN + 2;
 
-struct GemmPair64 {
-    const double *A[2];
-    const double *B[2];
-    double *P[2];
-};
+using GemmSet64 = GemmSetT<double>;
 
-__global__ __launch_bounds__(256) void k_gemm_f64s(int M, int N, int K, int kc, int S, GemmPair64 gp, long lda,
-                                                   long ldb, const int *__restrict__ kend,
+__global__ __launch_bounds__(256) void k_gemm_f64s(int N, int K, int kc, int S, GemmSet64 gp, long lda, long ldb,
                                                    const int *__restrict__ nk_w)
 {
     __shared__ __attribute__((aligned(16))) double As[2][kDBM][kDPA];
     __shared__ __attribute__((aligned(16))) double Bs[2][kDBK][kDPB];
     const int pi = blockIdx.z / S, z = blockIdx.z - pi * S;
+    const int M = gp.M[pi];
+    const int m0 = blockIdx.y * kDBM, n0 = blockIdx.x * kDBN;
+    if (m0 >= M) return;  // uniform: a contraction with fewer row tiles
     const double *__restrict__ A = gp.A[pi];
     const double *__restrict__ B = gp.B[pi];
+    const int *__restrict__ kend = gp.kend[pi];
+    const int rpw = gp.rpw[pi];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv >> 1, wn = wv & 1;
-    const int m0 = blockIdx.y * kDBM, n0 = blockIdx.x * kDBN;
     const int kt = min(K, kend ? kend[blockIdx.y] : K);
     const int kb = z * kc;
     if (kb >= kt) return;  // uniform: this slice is past the tile's extent
@@ -317,7 +333,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64s(int M, int N, int K, int kc, 
     int a_lim[4];
     for (int e = 0; e < 4; e++) {
         const int gm = m0 + ((tid + 256 * e) >> 3);
-        a_lim[e] = gm < M ? (nk_w ? min(ke, nk_w[gm >> 2]) : ke) : 0;
+        a_lim[e] = gm < M ? (nk_w ? min(ke, nk_w[gm / rpw]) : ke) : 0;
     }
     double2 ra[4], rb[4];
     auto load = [&](int k0) {
@@ -509,21 +525,59 @@ size_t gemm_pair_workspace_bytes(int M, int N, int K, int kc)
     return 2 * S * (size_t)M * N * sizeof(float);
 }
 
+// Fills the partial pointers of a set from one workspace and launches the
+// streamed kernel and the reduce.
+template <typename T>
+static hipError_t launch_gemm_set_t(int n, GemmSetT<T> gs, int N, int K, int kc, long lda, long ldb, T *P,
+                                    const int *nk_w, int bm, hipStream_t st)
+{
+    if (n < 1 || n > 4) return hipErrorInvalidValue;
+    const int S = (K + kc - 1) / kc;
+    int mt = 0;
+    long total = 0;
+    for (int q = 0; q < n; q++) {
+        if (gs.M[q] < 1 || gs.rpw[q] < 1) return hipErrorInvalidValue;
+        gs.P[q] = P + (size_t)S * total;
+        total += (long)gs.M[q] * N;
+        mt = std::max(mt, (gs.M[q] + bm - 1) / bm);
+    }
+    if (mt > 65535 || (long)n * S > 65535) return hipErrorInvalidValue;
+    dim3 grid((N + kSBN - 1) / kSBN, mt, n * S);
+    if constexpr (sizeof(T) == 4)
+        hipLaunchKernelGGL(k_gemm_f32s, grid, dim3(256), 0, st, N, K, kc, S, gs, lda, ldb, nk_w);
+    else
+        hipLaunchKernelGGL(k_gemm_f64s, grid, dim3(256), 0, st, N, K, kc, S, gs, lda, ldb, nk_w);
+    hipLaunchKernelGGL(k_gemm_reduce_set<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, n, N, K, kc,
+                       bm, gs, total);
+    return hipGetLastError();
+}
+
+template <typename T>
+static GemmSetT<T> pair_set(int M, const T *A0, const T *A1, const T *B0, const T *B1, T *C0, T *C1,
+                            const int *kend)
+{
+    GemmSetT<T> gs{};
+    gs.A[0] = A0, gs.A[1] = A1, gs.B[0] = B0, gs.B[1] = B1, gs.C[0] = C0, gs.C[1] = C1;
+    gs.kend[0] = gs.kend[1] = kend;
+    gs.M[0] = gs.M[1] = M;
+    gs.rpw[0] = gs.rpw[1] = 4;
+    return gs;
+}
+
+size_t gemm_set_workspace_bytes(int n, const int *M, int N, int K, int kc, size_t es)
+{
+    size_t rows = 0;
+    for (int q = 0; q < n; q++) rows += (size_t)M[q];
+    return (size_t)((K + kc - 1) / kc) * rows * N * es;
+}
+
 hipError_t launch_gemm_pair_f64(int M, int N, int K, int kc, const double *A0, const double *A1, long lda,
                                 const double *B0, const double *B1, long ldb, double *C0, double *C1, double *P,
                                 const int *kend, const int *nk_w, hipStream_t st)
 {
-    if (kc < kDBK || kc % 32 || lda % 2 || ldb % 2 || (M + kDBM - 1) / kDBM > 65535) return hipErrorInvalidValue;
-    const int S = (K + kc - 1) / kc;
-    GemmPair64 gp;
-    gp.A[0] = A0, gp.A[1] = A1, gp.B[0] = B0, gp.B[1] = B1;
-    gp.P[0] = P, gp.P[1] = P + (size_t)S * M * N;
-    dim3 grid((N + kDBN - 1) / kDBN, (M + kDBM - 1) / kDBM, 2 * S);
-    hipLaunchKernelGGL(k_gemm_f64s, grid, dim3(256), 0, st, M, N, K, kc, S, gp, lda, ldb, kend, nk_w);
-    const long n2 = 2L * M * N;
-    hipLaunchKernelGGL(k_gemm_reduce_ragged<double>, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, M, N, K,
-                       kc, kDBM, kend, (const double *)gp.P[0], (const double *)gp.P[1], C0, C1);
-    return hipGetLastError();
+    if (kc < kDBK || kc % 32 || lda % 2 || ldb % 2) return hipErrorInvalidValue;
+    return launch_gemm_set_t<double>(2, pair_set(M, A0, A1, B0, B1, C0, C1, kend), N, K, kc, lda, ldb, P, nk_w,
+                                     kDBM, st);
 }
 
 bool gemm_pair_ok(long lda, long ldb, const void *A0, const void *A1, const void *B0, const void *B1)
@@ -536,17 +590,32 @@ hipError_t launch_gemm_pair_f32(int M, int N, int K, int kc, const float *A0, co
                                 const float *B0, const float *B1, long ldb, float *C0, float *C1, float *P,
                                 const int *kend, const int *nk_w, hipStream_t st)
 {
-    if (kc < kSBK || kc % kSBK || (M + kSBM - 1) / kSBM > 65535) return hipErrorInvalidValue;
-    const int S = (K + kc - 1) / kc;
-    GemmPair gp;
-    gp.A[0] = A0, gp.A[1] = A1, gp.B[0] = B0, gp.B[1] = B1;
-    gp.P[0] = P, gp.P[1] = P + (size_t)S * M * N;
-    dim3 grid((N + kSBN - 1) / kSBN, (M + kSBM - 1) / kSBM, 2 * S);
-    hipLaunchKernelGGL(k_gemm_f32s, grid, dim3(256), 0, st, M, N, K, kc, S, gp, lda, ldb, kend, nk_w);
-    const long n2 = 2L * M * N;
-    hipLaunchKernelGGL(k_gemm_reduce_ragged<float>, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, st, M, N, K,
-                       kc, kSBM, kend, (const float *)gp.P[0], (const float *)gp.P[1], C0, C1);
-    return hipGetLastError();
+    if (kc < kSBK || kc % kSBK) return hipErrorInvalidValue;
+    return launch_gemm_set_t<float>(2, pair_set(M, A0, A1, B0, B1, C0, C1, kend), N, K, kc, lda, ldb, P, nk_w,
+                                    kSBM, st);
+}
+
+hipError_t launch_gemm_set(int prec, int n, const int *M, const int *rpw, int N, int K, int kc, const void *const *A,
+                           long lda, const void *const *B, long ldb, void *const *C, void *P,
+                           const int *const *kend, const int *nk_w, hipStream_t st)
+{
+    if (n < 1 || n > 4) return hipErrorInvalidValue;
+    if (prec == 0) {
+        if (kc < kDBK || kc % 32 || lda % 2 || ldb % 2) return hipErrorInvalidValue;
+        GemmSet64 gs{};
+        for (int q = 0; q < n; q++) {
+            gs.A[q] = (const double *)A[q], gs.B[q] = (const double *)B[q], gs.C[q] = (double *)C[q];
+            gs.kend[q] = kend ? kend[q] : nullptr, gs.M[q] = M[q], gs.rpw[q] = rpw[q];
+        }
+        return launch_gemm_set_t<double>(n, gs, N, K, kc, lda, ldb, (double *)P, nk_w, kDBM, st);
+    }
+    if (kc < kSBK || kc % kSBK) return hipErrorInvalidValue;
+    GemmSet gs{};
+    for (int q = 0; q < n; q++) {
+        gs.A[q] = (const float *)A[q], gs.B[q] = (const float *)B[q], gs.C[q] = (float *)C[q];
+        gs.kend[q] = kend ? kend[q] : nullptr, gs.M[q] = M[q], gs.rpw[q] = rpw[q];
+    }
+    return launch_gemm_set_t<float>(n, gs, N, K, kc, lda, ldb, (float *)P, nk_w, kSBM, st);
 }
 
 size_t gemm_workspace_bytes(int mode, int M, int N, int K)

@@ -587,6 +587,14 @@ struct grtm_plan {
     DBuf tiles;             // recursion work items of the predicted grid, ordered by row
     int ntiles = 0, ntiles64 = 0;  // all items; items of rows [0, rows64)
     double bessel_ms = 0;
+    // kind 0: explosion and vertical-force Z / R traces; kind 1: the
+    // displacement columns, whose operands, tables and outputs are the four
+    // contractions against J0..J3 laid end to end in A0, B0 and C0
+    int kind = 0;
+    grtm::MtOps mto{};
+    grtm::MtCols mtc{};
+    int ntr() const { return kind ? grtm::kMtCols * nr : 4 * nr; }
+    int mt_M(int q) const { return 2 * mto.rows[q] * nw; }
     ~grtm_plan()
     {
         if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -597,8 +605,27 @@ namespace {
 
 size_t esize(int prec) { return prec == GRTM_PREC_FP64 ? sizeof(double) : sizeof(float); }
 
+int run_cells_mt(grtm_plan *p, hipStream_t s)
+{
+    if (p->prec == GRTM_PREC_FP64) {
+        HIPCHK(grtm::launch_cells_mt(0, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->nw,
+                                     p->mto, p->nk, nullptr, nullptr, s));
+        return GRTM_OK;
+    }
+    unsigned *cnt = p->flags.as<unsigned>(), *list = cnt + 1;
+    HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
+    HIPCHK(grtm::launch_cells_mt(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->rows64,
+                                 p->mto, p->nk, nullptr, nullptr, s));
+    HIPCHK(grtm::launch_cells_mt(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->rows64,
+                                 p->nw - p->rows64, p->mto, p->nk, cnt, list, s));
+    HIPCHK(grtm::launch_cells_mt_fix(p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->mto,
+                                     p->nk, cnt, list, s));
+    return GRTM_OK;
+}
+
 int run_cells(grtm_plan *p, hipStream_t s)
 {
+    if (p->kind == 1) return run_cells_mt(p, s);
     const bool ad = p->adaptive;
     int *nkw = ad ? p->nk_w.as<int>() : nullptr;
     unsigned *rowpk = ad ? p->counters.as<unsigned>() : nullptr;
@@ -650,8 +677,33 @@ int run_cells(grtm_plan *p, hipStream_t s)
                                        p->kend.as<int>(), s));
     return GRTM_OK;
 }
+int run_gemm_mt(grtm_plan *p, hipStream_t s)
+{
+    const size_t es = esize(p->prec);
+    int M[grtm::kMtSets], rpw[grtm::kMtSets];
+    const void *A[grtm::kMtSets], *B[grtm::kMtSets];
+    void *C[grtm::kMtSets];
+    for (int q = 0; q < grtm::kMtSets; q++) {
+        M[q] = p->mt_M(q);
+        rpw[q] = 2 * p->mto.rows[q];
+        A[q] = p->mto.A[q];
+        B[q] = (const unsigned char *)p->B0.p + (size_t)q * p->nk * p->nr * es;
+        C[q] = const_cast<void *>(p->mtc.C[q]);
+    }
+    if (p->pair) {
+        HIPCHK(grtm::launch_gemm_set(p->gemm_mode, grtm::kMtSets, M, rpw, p->nr, p->nk, p->kc, A, p->nk, B, p->nr, C,
+                                     p->ws.p, nullptr, nullptr, s));
+        return GRTM_OK;
+    }
+    for (int q = 0; q < grtm::kMtSets; q++)
+        HIPCHK(grtm::launch_gemm_mfma(p->gemm_mode, M[q], p->nr, p->nk, A[q], p->nk, B[q], p->nr, C[q], p->ws.p,
+                                      nullptr, nullptr, s));
+    return GRTM_OK;
+}
+
 int run_gemm(grtm_plan *p, hipStream_t s)
 {
+    if (p->kind == 1) return run_gemm_mt(p, s);
     const int M = 4 * p->nw;
     const int *kend = p->adaptive ? p->kend.as<int>() : nullptr;
     const int *nkw = p->adaptive ? p->nk_w.as<int>() : nullptr;
@@ -675,9 +727,44 @@ int run_gemm(grtm_plan *p, hipStream_t s)
 }
 int run_ifft(grtm_plan *p, hipStream_t s)
 {
+    if (p->kind == 1) {
+        HIPCHK(grtm::launch_post_ifft_cols(p->prec == GRTM_PREC_FP64 ? 0 : 1, p->pp, p->mtc, p->nr, p->nr,
+                                           p->dists.as<double>(), p->tr.p, s));
+        return GRTM_OK;
+    }
     HIPCHK(grtm::launch_post_ifft(p->prec == GRTM_PREC_FP64 ? 0 : 1, p->pp, 1, 4 * p->nr, nullptr, p->C0.p, p->C1.p,
                                   p->nr, p->nr, p->tr.p, nullptr, s));
     return GRTM_OK;
+}
+
+// Terms of each displacement column over the contraction outputs (rows of
+// mt_rows_from_cell): tensor columns 0-9, then single-force columns 0-4.
+// k J_n' = n J_n / r - k J_{n+1} turns, e.g., tensor column 0,
+// b2 2 J2 / r + a2[0] k J2', into (2 / r) (b2 + a2[0]) J2 - a2[0] k J3.
+grtm::MtCols mt_columns(int compat)
+{
+    grtm::MtCols mc{};
+    auto term = [&](int c, int set, int row, double scale, int rinv) {
+        const int u = mc.nterm[c]++;
+        mc.term[c][u] = grtm::MtTerm{set, row, rinv, 0, scale};
+    };
+    term(0, 2, 0, 2.0, 1), term(0, 3, 0, -1.0, 0);                  // R, n = 2
+    term(1, 1, 3, 1.0, 1), term(1, 2, 1, -1.0, 0);                  // R, n = 1
+    term(2, 1, 0, -1.0, 0);                                         // R, n = 0 (s2)
+    term(3, 1, 1, -1.0, 0);                                         // R, n = 0 (s0)
+    term(4, 2, 0, 2.0, 1), term(4, 3, 1, -1.0, 0);                  // T, n = 2
+    term(5, 1, compat ? 7 : 3, 1.0, 1), term(5, 2, 2, -1.0, 0);     // T, n = 1
+    term(6, 2, 3, 1.0, 0);                                          // Z, n = 2
+    term(7, 1, 2, 1.0, 0);                                          // Z, n = 1
+    term(8, 0, 0, 1.0, 0);                                          // Z, n = 0 (s0)
+    term(9, 0, 1, 1.0, 0);                                          // Z, n = 0 (s2)
+    term(10, 1, 6, 1.0, 1), term(10, 2, 4, -1.0, 0);                // horizontal force R
+    term(11, 1, 4, -1.0, 0);                                        // vertical force R
+    term(12, 1, 6, 1.0, 1), term(12, 2, 5, -1.0, 0);                // horizontal force T
+    term(13, 1, 5, 1.0, 0);                                         // horizontal force Z
+    term(14, 0, 2, 1.0, 0);                                         // vertical force Z
+    for (int q = 0; q < grtm::kMtSets; q++) mc.rows[q] = grtm::mt_set_rows(compat, q);
+    return mc;
 }
 
 }  // namespace
@@ -700,8 +787,8 @@ int grtm_create(grtm_ctx **ctx, int device)
 
 void grtm_destroy(grtm_ctx *ctx) { delete ctx; }
 
-int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, int precision, int compat,
-                     grtm_plan **plan_out)
+static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, int precision, int compat,
+                       int kind, grtm_plan **plan_out)
 {
     if (!ctx || !job || !grid || !plan_out) return GRTM_E_INPUT;
     *plan_out = nullptr;
@@ -743,6 +830,7 @@ int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, 
     p->df = df;
     p->omega_i = omega_i;
     p->pp = post_params(grid, nw, df, omega_i);
+    p->kind = kind;
     HIPCHK(hipStreamCreate(&p->stream));
     p->own_stream = true;
 
@@ -761,22 +849,54 @@ int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, 
         if ((status = p->lw.alloc(lf.size() * sizeof(LayerW<float>)))) return status;
         HIPCHK(hipMemcpy(p->lw.p, lf.data(), lf.size() * sizeof(LayerW<float>), hipMemcpyHostToDevice));
     }
-    const size_t arows = (size_t)4 * p->nw;
     if ((status = p->dists.alloc(sizeof(double) * p->nr))) return status;
-    if ((status = p->A0.alloc(arows * p->nk * es))) return status;
-    if ((status = p->A1.alloc(arows * p->nk * es))) return status;
-    if ((status = p->B0.alloc((size_t)p->nk * p->nr * es))) return status;
-    if ((status = p->B1.alloc((size_t)p->nk * p->nr * es))) return status;
-    if ((status = p->C0.alloc(arows * p->nr * es))) return status;
-    if ((status = p->C1.alloc(arows * p->nr * es))) return status;
-    if ((status = p->tr.alloc((size_t)4 * p->nr * p->N * es))) return status;
-    size_t wsb = grtm::gemm_workspace_bytes(p->gemm_mode, (int)arows, p->nr, p->nk);
-    if (p->gemm_mode != 2 && grtm::gemm_pair_ok(p->nk, p->nr, p->A0.p, p->A1.p, p->B0.p, p->B1.p)) {
-        p->pair = true;
-        p->kc = grtm::gemm_pair_slice((int)arows, p->nr, p->nk, p->nk);
-        wsb = std::max(wsb, grtm::gemm_pair_workspace_bytes((int)arows, p->nr, p->nk, p->kc) * es / 4);
+    if ((status = p->tr.alloc((size_t)p->ntr() * p->N * es))) return status;
+    if (kind == 1) {
+        // four contractions, operands / tables / outputs end to end
+        p->mtc = mt_columns(compat);
+        size_t rows = 0, off[grtm::kMtSets];
+        int M[grtm::kMtSets];
+        for (int q = 0; q < grtm::kMtSets; q++) {
+            p->mto.rows[q] = p->mtc.rows[q];
+            M[q] = p->mt_M(q);
+            off[q] = rows;
+            rows += (size_t)M[q];
+        }
+        if ((status = p->A0.alloc(rows * p->nk * es))) return status;
+        if ((status = p->B0.alloc((size_t)grtm::kMtSets * p->nk * p->nr * es))) return status;
+        if ((status = p->C0.alloc(rows * p->nr * es))) return status;
+        bool aligned = p->gemm_mode != 2 && p->nk % 4 == 0 && p->nr % 4 == 0;
+        for (int q = 0; q < grtm::kMtSets; q++) {
+            p->mto.A[q] = (unsigned char *)p->A0.p + off[q] * p->nk * es;
+            p->mtc.C[q] = (unsigned char *)p->C0.p + off[q] * p->nr * es;
+            const void *Bq = (unsigned char *)p->B0.p + (size_t)q * p->nk * p->nr * es;
+            aligned = aligned && grtm::gemm_pair_ok(p->nk, p->nr, p->mto.A[q], p->mto.A[q], Bq, Bq);
+        }
+        size_t wsb = 0;
+        for (int q = 0; q < grtm::kMtSets; q++)
+            wsb = std::max(wsb, grtm::gemm_workspace_bytes(p->gemm_mode, M[q], p->nr, p->nk));
+        if (aligned) {
+            p->pair = true;
+            p->kc = grtm::gemm_pair_slice((int)(rows / 2), p->nr, p->nk, p->nk);
+            wsb = std::max(wsb, grtm::gemm_set_workspace_bytes(grtm::kMtSets, M, p->nr, p->nk, p->kc, es));
+        }
+        if ((status = p->ws.alloc(wsb))) return status;
+    } else {
+        const size_t arows = (size_t)4 * p->nw;
+        if ((status = p->A0.alloc(arows * p->nk * es))) return status;
+        if ((status = p->A1.alloc(arows * p->nk * es))) return status;
+        if ((status = p->B0.alloc((size_t)p->nk * p->nr * es))) return status;
+        if ((status = p->B1.alloc((size_t)p->nk * p->nr * es))) return status;
+        if ((status = p->C0.alloc(arows * p->nr * es))) return status;
+        if ((status = p->C1.alloc(arows * p->nr * es))) return status;
+        size_t wsb = grtm::gemm_workspace_bytes(p->gemm_mode, (int)arows, p->nr, p->nk);
+        if (p->gemm_mode != 2 && grtm::gemm_pair_ok(p->nk, p->nr, p->A0.p, p->A1.p, p->B0.p, p->B1.p)) {
+            p->pair = true;
+            p->kc = grtm::gemm_pair_slice((int)arows, p->nr, p->nk, p->nk);
+            wsb = std::max(wsb, grtm::gemm_pair_workspace_bytes((int)arows, p->nr, p->nk, p->kc) * es / 4);
+        }
+        if ((status = p->ws.alloc(wsb))) return status;
     }
-    if ((status = p->ws.alloc(wsb))) return status;
     HIPCHK(hipMemcpy(p->dists.p, job->dists, sizeof(double) * p->nr, hipMemcpyHostToDevice));
 
     // Bessel tables (planning work: they depend on the grid and distances only)
@@ -784,9 +904,17 @@ int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, 
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, p->stream));
-    HIPCHK(grtm::launch_bessel_tab(precision == GRTM_PREC_FP64 ? 0 : 1, compat ? 0 : 1, p->nk, p->dk,
-                                   p->dists.as<double>(), p->nr, p->B0.p,
-                                   p->B1.p, p->nr, p->stream));
+    if (kind == 1) {
+        const size_t tab = (size_t)p->nk * p->nr * es;
+        unsigned char *b = (unsigned char *)p->B0.p;
+        for (int n0 = 0; n0 < grtm::kMtSets; n0 += 2)
+            HIPCHK(grtm::launch_bessel_tab(precision == GRTM_PREC_FP64 ? 0 : 1, compat ? 0 : 1, p->nk, p->dk,
+                                           p->dists.as<double>(), p->nr, b + n0 * tab, b + (n0 + 1) * tab, p->nr,
+                                           p->stream, n0));
+    } else {
+        HIPCHK(grtm::launch_bessel_tab(precision == GRTM_PREC_FP64 ? 0 : 1, compat ? 0 : 1, p->nk, p->dk,
+                                       p->dists.as<double>(), p->nr, p->B0.p, p->B1.p, p->nr, p->stream));
+    }
     HIPCHK(hipEventRecord(e1, p->stream));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
@@ -797,6 +925,18 @@ int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, 
     *plan_out = p;
     p = nullptr;  // released from the guard
     return GRTM_OK;
+}
+
+int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, int precision, int compat,
+                     grtm_plan **plan_out)
+{
+    return plan_create(ctx, job, grid, precision, compat, 0, plan_out);
+}
+
+int grtm_plan_columns(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, int precision, int compat,
+                      grtm_plan **plan_out)
+{
+    return plan_create(ctx, job, grid, precision, compat, 1, plan_out);
 }
 
 int grtm_plan_execute(grtm_plan *p, void *stream)
@@ -820,7 +960,7 @@ int grtm_plan_sync(grtm_plan *p)
 int grtm_plan_traces_out(grtm_plan *p, double *traces)
 {
     if (!p || !traces) return GRTM_E_INPUT;
-    const size_t n = (size_t)4 * p->nr * p->N;
+    const size_t n = (size_t)p->ntr() * p->N;
     HIPCHK(hipStreamSynchronize(p->stream));
     if (p->prec == GRTM_PREC_FP64) {
         HIPCHK(hipMemcpy(traces, p->tr.p, n * sizeof(double), hipMemcpyDeviceToHost));
@@ -835,6 +975,7 @@ int grtm_plan_traces_out(grtm_plan *p, double *traces)
 int grtm_plan_kernels_out(grtm_plan *p, int q, int w_lo, int w_hi, double *kern)
 {
     if (!p || !kern || q < 0 || q > 1 || w_lo < 0 || w_hi > p->nw || w_lo >= w_hi) return GRTM_E_INPUT;
+    if (p->kind != 0) return GRTM_E_UNSUPPORTED;
     HIPCHK(hipStreamSynchronize(p->stream));
     const size_t es = esize(p->prec);
     std::vector<unsigned char> buf((size_t)p->nk * es);
@@ -917,6 +1058,7 @@ int grtm_plan_set_rows64(grtm_plan *p, int rows64)
 int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int stride, double tol)
 {
     if (!p) return GRTM_E_INPUT;
+    if (p->kind != 0) return GRTM_E_UNSUPPORTED;
     if (!kcut) {
         p->adaptive = false;
         if (p->pair) p->kc = grtm::gemm_pair_slice(4 * p->nw, p->nr, p->nk, p->nk);
@@ -1011,11 +1153,21 @@ int grtm_plan_work(grtm_plan *p, double *w)
     const double cells = (double)p->nw * p->nk;
     w[0] = cells;
     w[1] = cells * p->L;
-    w[2] = 2.0 * 2.0 * (4.0 * p->nw) * p->nk * p->nr;  // two contractions, 2 flop per MAC
-    w[3] = 4.0 * p->nr * (8.0 * p->nw + 4.0 * p->N);   // complex64 in + fp32 out per trace component
-    w[4] = 2.0 * p->nr;                                 // (model, source, distance) traces, Z+R each
     double oc[2] = {0, 0};
-    if (grtm_opcount_cell(p->L, p->g.s, p->g.rl, p->g.branch, p->g.up, oc)) return GRTM_E_INPUT;
+    if (p->kind == 0) {
+        w[2] = 2.0 * 2.0 * (4.0 * p->nw) * p->nk * p->nr;  // two contractions, 2 flop per MAC
+        w[3] = 4.0 * p->nr * (8.0 * p->nw + 4.0 * p->N);   // complex64 in + fp32 out per trace component
+        w[4] = 2.0 * p->nr;                                 // (model, source, distance) traces, Z+R each
+        if (grtm_opcount_cell(p->L, p->g.s, p->g.rl, p->g.branch, p->g.up, oc)) return GRTM_E_INPUT;
+    } else {
+        double rows = 0, terms = 0;
+        for (int q = 0; q < grtm::kMtSets; q++) rows += p->mt_M(q);
+        for (int c = 0; c < grtm::kMtCols; c++) terms += p->mtc.nterm[c];
+        w[2] = 2.0 * rows * p->nk * p->nr;                         // four contractions
+        w[3] = p->nr * (terms * 8.0 * p->nw + grtm::kMtCols * 4.0 * p->N);  // complex64 terms in, fp32 out
+        w[4] = (double)grtm::kMtCols * p->nr;                      // column traces
+        if (grtm_opcount_columns_cell(p->L, p->g.s, p->g.rl, p->g.branch, p->g.up, oc)) return GRTM_E_INPUT;
+    }
     w[5] = oc[0];  // flops per cell
     w[6] = oc[1];  // transcendental calls per cell
     w[7] = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;  // rows solved in FP64

@@ -57,8 +57,9 @@ hipError_t launch_cells_full(int compat, int src, const void *lw, const int *nk_
                              int nw, int nblk, double dk, double dist, double df, double omega_i,
                              const double p0[4], double *partial, hipStream_t st);
 hipError_t launch_green_finish(int nw, int nblk, const double *partial, double *raw, hipStream_t st);
+// B0 = J_n0(k r), B1 = J_n0+1(k r)
 hipError_t launch_bessel_tab(int prec, int mode, int nk, double dk, const double *dists, int nr, void *B0,
-                             void *B1, long ldb, hipStream_t st);
+                             void *B1, long ldb, hipStream_t st, int n0 = 0);
 hipError_t launch_gemm(int prec, int M, int N, int K, const void *A, long lda, const void *B, long ldb, void *Cm,
                        long ldc, hipStream_t st);
 size_t post_ifft_lds_bytes(int prec, int nw);
@@ -82,11 +83,49 @@ hipError_t launch_gemm_pair_f32(int M, int N, int K, int kc, const float *A0, co
 hipError_t launch_gemm_pair_f64(int M, int N, int K, int kc, const double *A0, const double *A1, long lda,
                                 const double *B0, const double *B1, long ldb, double *C0, double *C1, double *P,
                                 const int *kend, const int *nk_w, hipStream_t st);
+// n <= 4 contractions C_q = A_q B_q (M[q] rows, rpw[q] rows per frequency
+// row of nk_w; shared N, K, strides) in one streamed launch; prec 0 FP64,
+// 1 FP32; the same alignment rules as the pairs; workspace P of
+// gemm_set_workspace_bytes; kend (optional) per contraction and 128-row tile.
+size_t gemm_set_workspace_bytes(int n, const int *M, int N, int K, int kc, size_t es);
+hipError_t launch_gemm_set(int prec, int n, const int *M, const int *rpw, int N, int K, int kc, const void *const *A,
+                           long lda, const void *const *B, long ldb, void *const *C, void *P,
+                           const int *const *kend, const int *nk_w, hipStream_t st);
 // layout 1 with several models: trace t reads model t / (4 nr), whose
 // contraction outputs start cms elements further per model
 hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr, const void *raw, const void *C0,
                             const void *C1, long ldc, int nr, void *traces, void *spec_post, hipStream_t st,
                             long cms = 0);
+
+// Displacement columns (tensor columns 0-9, single-force columns 0-4):
+// operands of the four contractions against J0..J3 (rows of
+// mt_rows_from_cell), and each column's terms over their outputs.
+constexpr int kMtCols = 15;
+struct MtOps {
+    void *A[kMtSets];     // A_s: [2 rows[s] nw][lda], FP64 or FP32 as the launch
+    int rows[kMtSets];    // complex rows per frequency
+};
+struct MtTerm {
+    int set, row;   // contraction and complex row within a frequency
+    int rinv, pad;  // 1: divided by the receiver distance
+    double scale;
+};
+struct MtCols {
+    const void *C[kMtSets];  // contraction outputs [2 rows[s] nw][ldc]
+    int rows[kMtSets];
+    int nterm[kMtCols];
+    MtTerm term[kMtCols][2];
+};
+// prec as launch_cells_zr; rows [w0, w0 + nwr), every wavenumber
+hipError_t launch_cells_mt(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk, double df,
+                           double omega_i, int w0, int nwr, const MtOps &o, long lda, unsigned *flag_count,
+                           unsigned *flag_list, hipStream_t st);
+hipError_t launch_cells_mt_fix(int compat, const void *lw64, int L, const Geo &g, int nk, double dk, double df,
+                               double omega_i, const MtOps &o, long lda, const unsigned *flag_count,
+                               const unsigned *flag_list, hipStream_t st);
+// traces [kMtCols][nr][2 nw]
+hipError_t launch_post_ifft_cols(int prec, const PostParams &pp, const MtCols &mc, long ldc, int nr,
+                                 const double *dists, void *traces, hipStream_t st);
 
 // One model of a batch on the device: geometry and the offset (in LayerW
 // entries) of its [nw x L] layer table in the batch table.

@@ -500,8 +500,11 @@ __global__ void k_green_finish(int nw, int nblk, const double *__restrict__ part
 }
 
 // ---------------------------------------------------------------- Bessel tables
+// J_n0(k r) into B0 and J_n0+1(k r) into B1, [nk][ldb]; mode 0: the
+// reference's forms (A&S polynomials, Miller / upward recurrence), mode 1:
+// the math library.
 template <typename T>
-__global__ void k_bessel_tab(int mode, int nk, double dk, const double *__restrict__ dists, int nr,
+__global__ void k_bessel_tab(int mode, int n0, int nk, double dk, const double *__restrict__ dists, int nr,
                              T *__restrict__ B0, T *__restrict__ B1, long ldb)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -510,11 +513,14 @@ __global__ void k_bessel_tab(int mode, int nk, double dk, const double *__restri
     const double x = dk * ik * dists[ir];  // phase formed in FP64
     double j0v, j1v;
     if (mode == 0) {
-        j0v = as_j0(x);
-        j1v = as_j1(x);
-    } else {
+        j0v = n0 == 0 ? as_j0(x) : as_jn(x, n0);
+        j1v = n0 == 0 ? as_j1(x) : as_jn(x, n0 + 1);
+    } else if (n0 == 0) {
         j0v = j0(x);
         j1v = j1(x);
+    } else {
+        j0v = lib_jn(n0, x);
+        j1v = lib_jn(n0 + 1, x);
     }
     B0[(size_t)ik * ldb + ir] = T(j0v);
     B1[(size_t)ik * ldb + ir] = T(j1v);
@@ -563,6 +569,71 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T *__re
         }
 }
 
+// ---------------------------------------------------------------- moment tensor
+// Recursion of the displacement-column operands: one thread per (omega, k)
+// cell, the rows of mt_rows_from_cell weighted by the k-quadrature and
+// stored to A_s[2 (rows_s iw + j) + {0 re, 1 im}][ik].  FP32 cells whose
+// result is not finite go to the flag list (k_cells_mt_fix re-solves them).
+template <typename T, typename TO, bool COMPAT>
+__device__ __forceinline__ bool cell_mt(const LayerW<T> *lw, const Geo &g, int iw, int ik, double dk, double df,
+                                        double omega_i, const MtOps &o, long lda)
+{
+    const double kd = dk * ik;
+    const T k = T(kd);
+    const cx<T> w = C<T>(T(2.0 * M_PI * df * iw), T(-omega_i));
+    CellOut<T> c;
+    cell_solve<T, COMPAT, false>(lw, g, kd, C<T>(0, 0), C<T>(0, 0), c);
+    const T wt = T(ik == 0 ? 0.5 * dk : dk);
+    bool ok = true;
+    mt_rows_from_cell<T, COMPAT>(c, g.up, k, w, [&](int set, int j, cx<T> v) {
+        const TO re = TO(wt * v.x), im = TO(wt * v.y);
+        TO *A = reinterpret_cast<TO *>(o.A[set]);
+        const size_t row = 2 * ((size_t)o.rows[set] * iw + j);
+        A[row * lda + ik] = re;
+        A[(row + 1) * lda + ik] = im;
+        ok = ok && isfinite(float(re)) && isfinite(float(im));
+    });
+    return ok;
+}
+
+// Rows [w0, w0 + gridDim.x / tpr), one block per (row, 256-wavenumber tile).
+template <typename T, typename TO, bool COMPAT>
+__global__ __launch_bounds__(kCellThreads) void k_cells_mt(const LayerW<T> *__restrict__ lw_all, int L, Geo g,
+                                                           int nk, double dk, double df, double omega_i, int w0,
+                                                           int tpr, MtOps o, long lda,
+                                                           unsigned *__restrict__ flag_count,
+                                                           unsigned *__restrict__ flag_list)
+{
+    __shared__ LayerW<T> lw[kMaxLayers];
+    const int q = int(blockIdx.x) / tpr;
+    const int iw = w0 + q, kb = int(blockIdx.x) - q * tpr;
+    for (int j = threadIdx.x; j < L; j += blockDim.x) lw[j] = lw_all[(size_t)iw * L + j];
+    __syncthreads();
+    const int ik = kb * kCellThreads + threadIdx.x;
+    if (ik >= nk) return;
+    const bool ok = cell_mt<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, o, lda);
+    if (!ok && flag_count) {
+        const unsigned slot = atomicAdd(flag_count, 1u);
+        flag_list[slot] = unsigned(iw) * unsigned(nk) + unsigned(ik);
+    }
+}
+
+// FP64 re-solve of the flagged cells (grid-stride over the list).
+template <typename TO, bool COMPAT>
+__global__ __launch_bounds__(kCellThreads) void k_cells_mt_fix(const LayerW<double> *__restrict__ lw_all, int L,
+                                                               Geo g, int nk, double dk, double df, double omega_i,
+                                                               MtOps o, long lda,
+                                                               const unsigned *__restrict__ flag_count,
+                                                               const unsigned *__restrict__ flag_list)
+{
+    const unsigned n = *flag_count;
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const unsigned cell = flag_list[t];
+        const int iw = int(cell / unsigned(nk)), ik = int(cell % unsigned(nk));
+        cell_mt<double, TO, COMPAT>(lw_all + (size_t)iw * L, g, iw, ik, dk, df, omega_i, o, lda);
+    }
+}
+
 // ---------------------------------------------------------------- post + IFFT
 template <typename T>
 __device__ __forceinline__ cx<T> post_val(cx<T> v, int iw, const PostParams &pp)
@@ -577,33 +648,16 @@ __device__ __forceinline__ cx<T> post_val(cx<T> v, int iw, const PostParams &pp)
     return v * ph;
 }
 
-// One workgroup per trace component.  layout 0: raw[(t*nw + iw)*2 + {re,im}];
-// layout 1: rows of the contraction outputs: component c = (t / nr) % 2 reads
-// C_c[(4 iw + 2 q + {0,1}) * ldc + ir] with q = t / (2 nr), ir = t % nr.
-// Several models (layout 1): trace t belongs to model t / (4 nr), whose
-// contraction outputs start cms elements further per model.
-template <typename T>
-__global__ __launch_bounds__(256) void k_post_ifft(PostParams pp, int layout, const T *__restrict__ raw,
-                                                   const T *__restrict__ C0, const T *__restrict__ C1,
-                                                   long ldc, int nr, long cms, T *__restrict__ traces,
-                                                   T *__restrict__ spec_post)
+// Phase / taper / scale, Hermitian C2R inverse FFT (Stockham radix 2 in LDS)
+// and time scaling of trace t, whose raw spectrum row iw is load(iw).
+template <typename T, typename Load>
+__device__ __forceinline__ void post_ifft_trace(const PostParams &pp, Load &&load, int t, T *__restrict__ traces,
+                                                T *__restrict__ spec_post)
 {
     extern __shared__ unsigned char smem_raw[];
     cx<T> *bufA = reinterpret_cast<cx<T> *>(smem_raw);
     const int M = pp.nw, N = 2 * pp.nw;
     cx<T> *bufB = bufA + M;
-    const int t = blockIdx.x;
-    auto load = [&](int iw) -> cx<T> {
-        if (layout == 0) {
-            const T *p = raw + ((size_t)t * M + iw) * 2;
-            return C<T>(p[0], p[1]);
-        }
-        const int mj = t / (4 * nr), tl = t - mj * 4 * nr;
-        const int q = tl / (2 * nr), c = (tl / nr) % 2, ir = tl % nr;
-        const T *Cc = (c ? C1 : C0) + (size_t)mj * cms;
-        const size_t row = (size_t)(4 * iw + 2 * q);
-        return C<T>(Cc[row * ldc + ir], Cc[(row + 1) * ldc + ir]);
-    };
     for (int m = threadIdx.x; m < M; m += blockDim.x) {
         cx<T> Xm = post_val(load(m), m, pp);
         if (spec_post) {
@@ -647,6 +701,58 @@ __global__ __launch_bounds__(256) void k_post_ifft(PostParams pp, int layout, co
         traces[(size_t)t * N + j0] = T(double(z.x) * invN * exp(pp.omega_i * (j0 * pp.dt + pp.t0)));
         traces[(size_t)t * N + j1] = T(double(z.y) * invN * exp(pp.omega_i * (j1 * pp.dt + pp.t0)));
     }
+}
+
+// One workgroup per trace component.  layout 0: raw[(t*nw + iw)*2 + {re,im}];
+// layout 1: rows of the contraction outputs: component c = (t / nr) % 2 reads
+// C_c[(4 iw + 2 q + {0,1}) * ldc + ir] with q = t / (2 nr), ir = t % nr.
+// Several models (layout 1): trace t belongs to model t / (4 nr), whose
+// contraction outputs start cms elements further per model.
+template <typename T>
+__global__ __launch_bounds__(256) void k_post_ifft(PostParams pp, int layout, const T *__restrict__ raw,
+                                                   const T *__restrict__ C0, const T *__restrict__ C1,
+                                                   long ldc, int nr, long cms, T *__restrict__ traces,
+                                                   T *__restrict__ spec_post)
+{
+    const int t = blockIdx.x;
+    const int M = pp.nw;
+    auto load = [&](int iw) -> cx<T> {
+        if (layout == 0) {
+            const T *p = raw + ((size_t)t * M + iw) * 2;
+            return C<T>(p[0], p[1]);
+        }
+        const int mj = t / (4 * nr), tl = t - mj * 4 * nr;
+        const int q = tl / (2 * nr), c = (tl / nr) % 2, ir = tl % nr;
+        const T *Cc = (c ? C1 : C0) + (size_t)mj * cms;
+        const size_t row = (size_t)(4 * iw + 2 * q);
+        return C<T>(Cc[row * ldc + ir], Cc[(row + 1) * ldc + ir]);
+    };
+    post_ifft_trace<T>(pp, load, t, traces, spec_post);
+}
+
+// Displacement columns: trace t = column t / nr at receiver t % nr; its raw
+// spectrum is the sum of the column's terms scale (1 / r if rinv) C_set[row]
+// over the contraction outputs (rows 2 (rows_set iw + row) + {0 re, 1 im}).
+template <typename T>
+__global__ __launch_bounds__(256) void k_post_ifft_cols(PostParams pp, MtCols mc, long ldc, int nr,
+                                                        const double *__restrict__ dists, T *__restrict__ traces)
+{
+    const int t = blockIdx.x;
+    const int col = t / nr, ir = t - col * nr;
+    const double rinv = 1.0 / dists[ir];
+    auto load = [&](int iw) -> cx<T> {
+        double re = 0, im = 0;
+        for (int u = 0; u < mc.nterm[col]; u++) {
+            const MtTerm &tm = mc.term[col][u];
+            const T *Cs = reinterpret_cast<const T *>(mc.C[tm.set]);
+            const size_t row = 2 * ((size_t)mc.rows[tm.set] * iw + tm.row);
+            const double f = tm.scale * (tm.rinv ? rinv : 1.0);
+            re += f * double(Cs[row * ldc + ir]);
+            im += f * double(Cs[(row + 1) * ldc + ir]);
+        }
+        return C<T>(T(re), T(im));
+    };
+    post_ifft_trace<T>(pp, load, t, traces, (T *)nullptr);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -799,15 +905,15 @@ hipError_t launch_green_finish(int nw, int nblk, const double *partial, double *
 }
 
 hipError_t launch_bessel_tab(int prec, int mode, int nk, double dk, const double *dists, int nr, void *B0,
-                             void *B1, long ldb, hipStream_t st)
+                             void *B1, long ldb, hipStream_t st, int n0)
 {
     const size_t n = (size_t)nk * nr;
     dim3 grid((unsigned)((n + 255) / 256));
     if (prec == 0)
-        hipLaunchKernelGGL(k_bessel_tab<double>, grid, dim3(256), 0, st, mode, nk, dk, dists, nr, (double *)B0,
+        hipLaunchKernelGGL(k_bessel_tab<double>, grid, dim3(256), 0, st, mode, n0, nk, dk, dists, nr, (double *)B0,
                            (double *)B1, ldb);
     else
-        hipLaunchKernelGGL(k_bessel_tab<float>, grid, dim3(256), 0, st, mode, nk, dk, dists, nr, (float *)B0,
+        hipLaunchKernelGGL(k_bessel_tab<float>, grid, dim3(256), 0, st, mode, n0, nk, dk, dists, nr, (float *)B0,
                            (float *)B1, ldb);
     return hipGetLastError();
 }
@@ -822,6 +928,64 @@ hipError_t launch_gemm(int prec, int M, int N, int K, const void *A, long lda, c
     else
         hipLaunchKernelGGL(k_gemm<float>, grid, dim3(256), 0, st, M, N, K, (const float *)A, lda,
                            (const float *)B, ldb, (float *)Cm, ldc);
+    return hipGetLastError();
+}
+
+hipError_t launch_cells_mt(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk, double df,
+                           double omega_i, int w0, int nwr, const MtOps &o, long lda, unsigned *flag_count,
+                           unsigned *flag_list, hipStream_t st)
+{
+    const int tpr = cells_tiles_per_row(nk);
+    if (nwr <= 0) return hipSuccess;
+#define GRTM_MT(T, TO, CP, FC, FL)                                                                             \
+    hipLaunchKernelGGL((k_cells_mt<T, TO, CP>), dim3(nwr * tpr), dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, \
+                       L, g, nk, dk, df, omega_i, w0, tpr, o, lda, FC, FL)
+    if (prec == 0) {
+        if (compat) GRTM_MT(double, double, true, nullptr, nullptr);
+        else GRTM_MT(double, double, false, nullptr, nullptr);
+    } else if (prec == 1) {
+        if (compat) GRTM_MT(float, float, true, flag_count, flag_list);
+        else GRTM_MT(float, float, false, flag_count, flag_list);
+    } else {
+        if (compat) GRTM_MT(double, float, true, nullptr, nullptr);
+        else GRTM_MT(double, float, false, nullptr, nullptr);
+    }
+#undef GRTM_MT
+    return hipGetLastError();
+}
+
+hipError_t launch_cells_mt_fix(int compat, const void *lw64, int L, const Geo &g, int nk, double dk, double df,
+                               double omega_i, const MtOps &o, long lda, const unsigned *flag_count,
+                               const unsigned *flag_list, hipStream_t st)
+{
+    dim3 grid(1024);
+    if (compat)
+        hipLaunchKernelGGL((k_cells_mt_fix<float, true>), grid, dim3(kCellThreads), 0, st,
+                           (const LayerW<double> *)lw64, L, g, nk, dk, df, omega_i, o, lda, flag_count, flag_list);
+    else
+        hipLaunchKernelGGL((k_cells_mt_fix<float, false>), grid, dim3(kCellThreads), 0, st,
+                           (const LayerW<double> *)lw64, L, g, nk, dk, df, omega_i, o, lda, flag_count, flag_list);
+    return hipGetLastError();
+}
+
+hipError_t launch_post_ifft_cols(int prec, const PostParams &pp, const MtCols &mc, long ldc, int nr,
+                                 const double *dists, void *traces, hipStream_t st)
+{
+    const size_t lds = post_ifft_lds_bytes(prec, pp.nw);
+    if (lds > 65536) {
+        hipError_t e = prec == 0 ? hipFuncSetAttribute((const void *)k_post_ifft_cols<double>,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)
+                                 : hipFuncSetAttribute((const void *)k_post_ifft_cols<float>,
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const dim3 grid(kMtCols * nr);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_post_ifft_cols<double>, grid, dim3(256), lds, st, pp, mc, ldc, nr, dists,
+                           (double *)traces);
+    else
+        hipLaunchKernelGGL(k_post_ifft_cols<float>, grid, dim3(256), lds, st, pp, mc, ldc, nr, dists,
+                           (float *)traces);
     return hipGetLastError();
 }
 

@@ -2,7 +2,8 @@
 //
 // Instantiates the device cell solver on a counting scalar type on the host:
 // every add / sub / mul / div / sqrt executed by cell_solve + zr_from_cell
-// for a given layer stack and geometry is tallied, transcendental calls
+// (Z / R plans) or + mt_rows_from_cell (displacement-column plans) for a
+// given layer stack and geometry is tallied, transcendental calls
 // (exp, sin/cos pair, log, atan2, hypot) separately.  This is the algorithmic
 // flop count the benchmark's roofline uses for the recursion kernel.
 #include <hip/hip_runtime.h>
@@ -59,7 +60,7 @@ struct grtm::WideOf<grtm_count::Cnt> {
     using type = grtm_count::Cnt;
 };
 
-extern "C" int grtm_opcount_cell(int L, int s, int rl, int branch, int up, double *out)
+static int count_cell(int kind, int L, int s, int rl, int branch, int up, double *out)
 {
     using grtm::cx;
     using grtm::LayerW;
@@ -96,9 +97,24 @@ extern "C" int grtm_opcount_cell(int L, int s, int rl, int branch, int up, doubl
     grtm_count::g_t = grtm_count::Tally();
     grtm::CellOut<Cnt> c;
     grtm::cell_solve<Cnt, false, false>(lw, g, Cnt(0.7), cx<Cnt>{Cnt(0), Cnt(0)}, cx<Cnt>{Cnt(0), Cnt(0)}, c);
-    cx<Cnt> a, b, d, e;
-    grtm::zr_from_cell<Cnt, false>(c, up, Cnt(0.7), cx<Cnt>{Cnt(w_re), Cnt(w_im)}, a, b, d, e);
+    const cx<Cnt> w{Cnt(w_re), Cnt(w_im)};
+    if (kind == 0) {
+        cx<Cnt> a, b, d, e;
+        grtm::zr_from_cell<Cnt, false>(c, up, Cnt(0.7), w, a, b, d, e);
+    } else {
+        grtm::mt_rows_from_cell<Cnt, false>(c, up, Cnt(0.7), w, [](int, int, cx<Cnt>) {});
+    }
     out[0] = grtm_count::g_t.flops;
     out[1] = grtm_count::g_t.trans;
     return 0;
+}
+
+extern "C" int grtm_opcount_cell(int L, int s, int rl, int branch, int up, double *out)
+{
+    return count_cell(0, L, s, rl, branch, up, out);
+}
+
+extern "C" int grtm_opcount_columns_cell(int L, int s, int rl, int branch, int up, double *out)
+{
+    return count_cell(1, L, s, rl, branch, up, out);
 }

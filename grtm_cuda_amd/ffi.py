@@ -65,6 +65,7 @@ def _lib():
         lib.grtm_destroy.argtypes = [P]
         lib.grtm_destroy.restype = None
         lib.grtm_plan_traces.argtypes = [P, P, P, I, I, P]
+        lib.grtm_plan_columns.argtypes = [P, P, P, I, I, P]
         lib.grtm_plan_execute.argtypes = [P, P]
         lib.grtm_plan_sync.argtypes = [P]
         lib.grtm_plan_traces_out.argtypes = [P, P]
@@ -75,6 +76,7 @@ def _lib():
         lib.grtm_plan_destroy.restype = None
         lib.grtm_selftest_cell.argtypes = [I, I, D, D, D, I, P, I, D, I, D, P, P]
         lib.grtm_opcount_cell.argtypes = [I, I, I, I, I, P]
+        lib.grtm_opcount_columns_cell.argtypes = [I, I, I, I, I, P]
         lib.grtm_selftest_gemm.argtypes = [I, I, I, I, P, P, P]
         lib.grtm_selftest_mfma_probe.argtypes = [I, P]
         lib.grtm_selftest_copy.argtypes = [I]
@@ -116,6 +118,7 @@ EXPORTS = [
     "grtm_create", "grtm_destroy", "grtm_plan_traces", "grtm_plan_execute", "grtm_plan_sync",
     "grtm_plan_traces_out", "grtm_plan_kernels_out", "grtm_plan_timing", "grtm_plan_work",
     "grtm_plan_destroy", "grtm_selftest_cell", "grtm_opcount_cell", "grtm_selftest_gemm",
+    "grtm_plan_columns", "grtm_opcount_columns_cell",
     "grtm_selftest_mfma_probe", "grtm_selftest_copy", "grtm_plan_set_adaptive", "grtm_plan_set_rows64",
     "grtm_plan_adaptive_stats",
     "grtm_plan_row_labels", "grtm_mlp_create", "grtm_mlp_destroy", "grtm_mlp_nparams", "grtm_mlp_set_params",

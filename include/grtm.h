@@ -90,11 +90,28 @@ void grtm_destroy(grtm_ctx *ctx);
 /* Prepare a job: layer tables, Bessel tables and all device buffers. */
 int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, int precision, int compat,
                      grtm_plan **plan);
+/* Displacement columns for many receivers from one recursion: replaces the
+ * per-distance integrand columns of green() (cusrc/popm/func/green.cu
+ * 1558-1621 single force, 1626-1716 tensor; one distance per call at
+ * green.cu:2052) on a dense uniform wavenumber grid.  grtm_plan_traces_out
+ * then gives GRTM_N_COLUMNS traces per receiver, traces[(c*nr + ir)*N + j]:
+ * c = 0..9 the tensor columns 0-9 (R n=2, R n=1, R n=0 from s2, R n=0 from
+ * s0, T n=2, T n=1, Z n=2, Z n=1, Z n=0 from s0, Z n=0 from s2) and
+ * c = 10..14 the single-force columns 0-4 (horizontal force R, vertical
+ * force R, horizontal force T, horizontal force Z, vertical force Z).
+ * compat = 1 keeps the reference's column contents (index slips in tensor
+ * columns 2, 3, 5; reference Bessel forms).  Dense grid only:
+ * grtm_plan_set_adaptive and grtm_plan_kernels_out return
+ * GRTM_E_UNSUPPORTED on these plans. */
+#define GRTM_N_COLUMNS 15
+int grtm_plan_columns(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, int precision, int compat,
+                      grtm_plan **plan);
 /* Run recursion + contraction + IFFT on resident data (asynchronous). */
 int grtm_plan_execute(grtm_plan *plan, void *stream);
 int grtm_plan_sync(grtm_plan *plan);
 /* Copy the traces out: traces[((q*2 + c)*nr + ir)*N + j], q: 0 explosion,
- * 1 vertical force; c: 0 Z, 1 R (radial). */
+ * 1 vertical force; c: 0 Z, 1 R (radial).  Column plans: the layout above
+ * (GRTM_N_COLUMNS * nr * N values). */
 int grtm_plan_traces_out(grtm_plan *plan, double *traces);
 /* Raw k-kernels (tests): kern[((iw*nk + ik)*4 + {zr,zi,rr,ri})] for source q,
  * unweighted, for frequency rows [w_lo, w_hi). */
@@ -104,7 +121,8 @@ int grtm_plan_kernels_out(grtm_plan *plan, int q, int w_lo, int w_hi, double *ke
  * ms[3] IFFT, ms[4] Bessel tables (one-off, part of planning). */
 int grtm_plan_timing(grtm_plan *plan, int reps, double *ms);
 /* Work of one execution: [0] cells, [1] cell-layers, [2] contraction flops,
- * [3] IFFT algorithmic bytes, [4] traces, [5] recursion flops per cell,
+ * [3] IFFT algorithmic bytes, [4] traces (column plans: column traces),
+ * [5] recursion flops per cell,
  * [6] transcendental calls per cell, [7] frequency rows solved in FP64,
  * [8] FP32 cells re-solved in FP64 in the last execution (9 doubles). */
 int grtm_plan_work(grtm_plan *plan, double *work);
@@ -217,6 +235,9 @@ int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi,
  * (1: receiver layer above the source layer, 2/3: same layer above/below,
  * 4: below). */
 int grtm_opcount_cell(int n_layers, int s_layer, int r_layer, int branch, int up, double *out);
+/* The same for a cell of the displacement-column plans (recursion + the
+ * coefficient rows of both sources). */
+int grtm_opcount_columns_cell(int n_layers, int s_layer, int r_layer, int branch, int up, double *out);
 
 /* ------------------------------------------- learned wavenumber predictor */
 /* Multilayer perceptron n_in -> depth x width (ReLU) -> n_out, trained and

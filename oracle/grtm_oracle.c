@@ -1439,3 +1439,72 @@ int or_traces(int compat, const double *lay, int n_in, double src_depth, double 
     free(kern);
     return OR_OK;
 }
+
+/* ------------------------------------------------------------------ */
+/* dense-grid multi-receiver displacement columns                      */
+/* ------------------------------------------------------------------ */
+/* The displacement columns of both sources on the dense grid: tensor
+ * columns 0-9 (green.cu:1651-1662) then single-force columns 0-4
+ * (green.cu:1582-1586), each summed over k with the trapezoid weight
+ * (green.cu:1887-1901) from the reference's integrand formulas with direct
+ * Bessel values (J_n, d_besselj, J_n / r), then phase / taper / scale and
+ * the inverse FFT (green.cu:1908-1938, 2208-2239).
+ * traces: [15][nr][2 nw]. */
+int or_columns(int compat, const double *lay, int n_in, double src_depth, double rcv_depth,
+               const double *dists, int nr, int nw, double dt, int nk, double kmax, double omega_i,
+               double t0, double taper, double *traces, int nthreads)
+{
+    if (nr < 1 || nw < 1 || nk < 1) return OR_E_INPUT;
+    or_run run;
+    int st = trace_run(compat, lay, n_in, src_depth, rcv_depth, &run);
+    if (st) return st;
+    double df = 1.0 / (2 * nw * dt), dk = kmax / nk;
+    grid_consts g = {nw, df, omega_i, dk};
+    cd r0, v0;
+    par0_rv(&run, compat, &g, &r0, &v0);
+    const int ncol = OR_N_COLUMNS;
+    cd *raw = (cd *)calloc((size_t)ncol * nr * nw, sizeof(cd));
+    int err = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
+#endif
+    for (int iw = 0; iw < nw; iw++) {
+        OPar l0[OR_MAX_LAYERS];
+        cd w = CMPLX(2 * OR_PI * df * iw, -omega_i);
+        layer_disp(&run, compat, w, l0);
+        for (int ik = 0; ik < nk && !err; ik++) {
+            double k = dk * ik, wgt = ik == 0 ? dk / 2.0 : dk;
+            OPpar pt, pf;
+            if (cell_ppar(compat, 1, k, w, run.n_layer, l0, run.s_layer, run.zs, run.r_layer, run.zr, r0, v0,
+                          &pt) ||
+                cell_ppar(compat, 0, k, w, run.n_layer, l0, run.s_layer, run.zs, run.r_layer, run.zr, r0, v0,
+                          &pf)) {
+                err |= 1;
+                break;
+            }
+            for (int ir = 0; ir < nr; ir++) {
+                cd ct[30], cf[30];
+                integrand(compat, 1, &pt, k, dists[ir], ct);
+                integrand(compat, 0, &pf, k, dists[ir], cf);
+                for (int c = 0; c < 10; c++) raw[((size_t)c * nr + ir) * nw + iw] += ct[c] * wgt;
+                for (int c = 0; c < 5; c++) raw[((size_t)(10 + c) * nr + ir) * nw + iw] += cf[c] * wgt;
+            }
+        }
+    }
+    if (err) {
+        free(raw);
+        return OR_E_UNSUPPORTED;
+    }
+    double *rw = (double *)malloc(sizeof(double) * 2 * nw);
+    for (size_t t = 0; t < (size_t)ncol * nr; t++) {
+        for (int iw = 0; iw < nw; iw++) {
+            rw[2 * iw] = creal(raw[t * nw + iw]);
+            rw[2 * iw + 1] = cimag(raw[t * nw + iw]);
+        }
+        or_post_ifft(nw, rw, df, omega_i, dt, t0, taper, NULL, traces + t * 2 * nw);
+    }
+    free(rw);
+    free(raw);
+    return OR_OK;
+}

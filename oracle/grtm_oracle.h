@@ -99,6 +99,16 @@ int or_traces(int compat, const double *lay, int n_in, double src_depth,
               double t0, double taper, int w_lo, int w_hi, double *traces,
               double *spec, int nthreads);
 
+/* Displacement columns of both sources on the dense grid (all frequency
+ * rows): tensor columns 0-9, then single-force columns 0-4, of the
+ * reference's integrand, each summed over k and post-processed like
+ * or_traces.  traces[(c*nr + ir)*N + j], c < OR_N_COLUMNS. */
+#define OR_N_COLUMNS 15
+int or_columns(int compat, const double *lay, int n_in, double src_depth,
+               double rcv_depth, const double *dists, int nr, int nw,
+               double dt, int nk, double kmax, double omega_i, double t0,
+               double taper, double *traces, int nthreads);
+
 /* The same dense-grid integration but returning the raw k-kernels
  * Kz, Kr (complex) for every (iw, ik) in [w_lo,w_hi) x [0,nk):
  * kern[((iw-w_lo)*nk + ik)*4 + {0,1,2,3}] = Kz re, im, Kr re, im, where the

@@ -125,6 +125,25 @@ def traces(layers, src_depth, rcv_depth, dists, src_kind, grid, compat=False, w_
     return tr, sp[..., 0] + 1j * sp[..., 1]
 
 
+N_COLUMNS = 15
+
+
+def columns(layers, src_depth, rcv_depth, dists, grid, compat=False, nthreads=0):
+    """Displacement-column traces [15, nr, N]: tensor columns 0-9, then
+    single-force columns 0-4, on the dense grid."""
+    layers = np.ascontiguousarray(layers, dtype=np.float64)
+    dists = np.ascontiguousarray(dists, dtype=np.float64)
+    nw = int(grid["nw"])
+    tr = np.zeros((N_COLUMNS, len(dists), 2 * nw))
+    st = lib().or_columns(int(compat), layers.ctypes.data_as(P), layers.shape[0], D(src_depth), D(rcv_depth),
+                          dists.ctypes.data_as(P), len(dists), nw, D(grid["dt"]), int(grid["nk"]), D(grid["kmax"]),
+                          D(grid["omega_i"]), D(grid.get("t0", 0.0)), D(grid.get("taper", 0.0)),
+                          tr.ctypes.data_as(P), int(nthreads))
+    if st:
+        raise RuntimeError(f"oracle columns status {st}")
+    return tr
+
+
 def kernels(layers, src_depth, rcv_depth, src_kind, grid, w_lo, w_hi, compat=False, nthreads=0):
     """Unweighted Z/R k-kernels [w_hi-w_lo, nk, 2] complex."""
     layers = np.ascontiguousarray(layers, dtype=np.float64)

@@ -136,3 +136,23 @@ def test_vertical_force_reciprocity(orc):
         z1 = orc.traces(models.C1_LAYERS, a, b, dists, 1, grid)[0][:, 0]
         z2 = orc.traces(models.C1_LAYERS, b, a, dists, 1, grid)[0][:, 0]
         assert (np.linalg.norm(z1 - z2, axis=-1) / np.linalg.norm(z1, axis=-1)).max() < 1e-11, (a, b)
+
+
+@pytest.mark.parametrize("compat", [False, True])
+def test_columns_reduce_to_zr_traces(orc, compat):
+    """The oracle's displacement columns (the reference's integrand formulas
+    with direct Bessel values) against its Z/R trace path (k-kernels times
+    J0 / J1 tables): explosion Z = column 8 - column 9, R = column 3 -
+    column 2; vertical force Z = single-force column 4, R = column 1.  Both
+    modes, so compat's index slips in columns 2 and 3 cancel as they should."""
+    grid = models.dense_grid(32, 0.2, 256, float(models.C1_LAYERS[:, 2].min()))
+    dists = [15.0, 40.0]
+    cols = orc.columns(models.C1_LAYERS, 10.0, 0.0, dists, grid, compat=compat)
+    assert cols.shape == (15, 2, 64) and np.all(np.isfinite(cols))
+    ex = orc.traces(models.C1_LAYERS, 10.0, 0.0, dists, 0, grid, compat=compat)[0]
+    vf = orc.traces(models.C1_LAYERS, 10.0, 0.0, dists, 1, grid, compat=compat)[0]
+    pairs = [(cols[8] - cols[9], ex[:, 0]), (cols[3] - cols[2], ex[:, 1]),
+             (cols[14], vf[:, 0]), (cols[11], vf[:, 1])]
+    for got, want in pairs:
+        scale = np.abs(want).max(axis=-1, keepdims=True)
+        assert np.abs(got - want).max() < 1e-11 * scale.max()
