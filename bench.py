@@ -197,6 +197,82 @@ def bench_precision(args, g, models):
     }
 
 
+def bench_columns(args, g, models):
+    """Displacement columns: the 8-layer / 256-receiver workload with every
+    tensor (0-9) and single-force (0-4) column of green()'s output, i.e.
+    15 x 256 column traces per step; the learned adaptive grid (or --path
+    dense) in mixed precision against the plain FP64 dense column plan
+    (per-column rel-L2 and peak error)."""
+    cfg = models.config("c2")
+    mk = lambda p: g.ColumnPlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
+                                precision=p)
+    print("bench: columns plain FP64 dense", file=sys.stderr, flush=True)
+    ref_plan = mk(g.PREC_FP64)
+    t_ref = time_plan(ref_plan, max(3, args.steps // 2), 1)
+    ref = ref_plan.traces()
+    tm_ref = ref_plan.timing(reps=2)
+    del ref_plan
+    pname = args.precision or "fp32"
+    prec = {"fp64": g.PREC_FP64, "fp32": g.PREC_FP32, "bf16x3": g.PREC_BF16X3}[pname]
+    print(f"bench: columns {args.path} {pname}", file=sys.stderr, flush=True)
+    plan = mk(prec)
+    extra = None
+    if args.path == "adaptive":
+        from grtm_cuda_amd import learned
+        kcut = learned.Predictor().kcut(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["grid"])
+        plan.set_adaptive(kcut, window=models.check_window(cfg["src_depth"], cfg["rcv_depth"]), stride=8, tol=3e-6)
+    t_step = time_plan(plan, args.steps, args.warmup)
+    if args.path == "adaptive":
+        st = plan.adaptive_stats()
+        dense_cells = cfg["grid"]["nw"] * cfg["grid"]["nk"]
+        extra = dict(cells_frac=st["cells_final"] / dense_cells, check_frac=st["cells_check"] / dense_cells,
+                     rows_fallback=st["rows_fallback"])
+    out = plan.traces()
+    work = plan.work()
+    tm = plan.timing(reps=3)
+    d = out - ref
+    nrm = np.linalg.norm(ref, axis=-1)
+    rl = np.linalg.norm(d, axis=-1) / np.maximum(nrm, 1e-6 * nrm.max(axis=-1, keepdims=True))
+    pk = np.abs(d).max(axis=-1) / np.abs(ref).max(axis=-1)
+    kernels = {"recursion": tm["cells"], "ksum": tm["ksum"], "ifft": tm["ifft"]}
+    peak = FP64_PEAK_TFLOPS if prec == g.PREC_FP64 else FP32_PEAK_TFLOPS
+    frac = extra["cells_frac"] if extra else 1.0  # kept share of the cells and of the contraction's K
+    cells = frac * work["cells"] + (extra["check_frac"] * work["cells"] if extra else 0.0)
+    rec = cells * work["flops_per_cell"] / (tm["cells"] * 1e-3) / 1e12
+    con = frac * work["ksum_flop"] / (tm["ksum"] * 1e-3) / 1e12
+    if tm["cells"] >= tm["ksum"]:
+        roof = dict(bound="valu", achieved=rec, peak=peak, unit="TFLOP/s", frac=rec / peak, traffic=None,
+                    kernel="k_cells_mt", flops_per_cell=work["flops_per_cell"], cells=cells)
+    else:
+        roof = dict(bound="mfma", achieved=con, peak=peak, unit="TFLOP/s", frac=con / peak, traffic=None,
+                    kernel="k_gemm_f32s" if prec == g.PREC_FP32 else "k_gemm_f64s")
+    ntr = work["traces"]
+    return {
+        "metric": "displacement-column traces/s at <=1e-4 rel-L2 vs FP64 (plain dense-k FP64 alongside)",
+        "value": ntr / t_step, "unit": "column traces/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {g.PREC_FP64: "f64", g.PREC_FP32: "f32", g.PREC_BF16X3: "f32+bf16x3"}[prec],
+        "data": "synthetic (seeded layered model)",
+        "config": {"workload": f"mt: {len(cfg['layers']) - 1} layers + half-space, tensor and single-force sources "
+                               f"at {cfg['src_depth']} km, 15 columns x {len(cfg['dists'])} receivers, "
+                               f"Nw={cfg['grid']['nw']}, Nk={cfg['grid']['nk']}",
+                   "path": f"{args.path} {pname}", "traces_per_step": ntr},
+        "plain_fp64_dense": {"value": ntr / t_ref, "ms_per_step": 1e3 * t_ref,
+                             "kernel_ms": {"recursion": round(tm_ref["cells"], 4), "ksum": round(tm_ref["ksum"], 4),
+                                           "ifft": round(tm_ref["ifft"], 4)}},
+        "speedup_vs_plain_fp64": t_ref / t_step,
+        "max_rel_l2_vs_fp64": float(rl.max()),
+        "max_peak_err_vs_fp64": float(pk.max()),
+        "max_rel_l2_per_column": [float(v) for v in rl.max(axis=1)],
+        "adaptive": extra,
+        "kernel_ms": {k: round(v, 4) for k, v in kernels.items()},
+        "contraction_tflops": con,
+        "recursion_tflops": rec,
+        "roofline": roof,
+        "cpu_baseline": None,
+    }
+
+
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "c2_adaptive_fp32_pmc_bytes.md")
 
 
@@ -243,9 +319,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "mt"],
                     help="c2: one model, 256 receivers (the headline); c3: precision study; "
-                         "c4: batch of --models models")
+                         "c4: batch of --models models; mt: all displacement columns of the c2 model")
     ap.add_argument("--models", type=int, default=1024, help="c4: models in the batch")
     ap.add_argument("--path", default="adaptive", choices=["adaptive", "dense"],
                     help="adaptive: learned truncated grid, mixed precision (the headline); dense: plain grid")
@@ -267,6 +343,9 @@ def main():
         return
     if args.workload == "c3":
         print(json.dumps(bench_precision(args, g, models)))
+        return
+    if args.workload == "mt":
+        print(json.dumps(bench_columns(args, g, models)))
         return
     if args.copy_probe:
         from grtm_cuda_amd.ffi import _lib, check

@@ -564,6 +564,26 @@ static GemmSetT<T> pair_set(int M, const T *A0, const T *A1, const T *B0, const 
     return gs;
 }
 
+// Slice length of a dense set: K / S for the power of two S <= 8 (slices of
+// at least 256) whose blocks, at two resident blocks per compute unit, fill
+// their last dispatch round best; ties go to the smaller S.
+int gemm_set_slice(int n, const int *M, int N, int K)
+{
+    long tiles = 0;
+    for (int q = 0; q < n; q++) tiles += (long)((M[q] + kSBM - 1) / kSBM) * ((N + kSBN - 1) / kSBN);
+    const long slots = 2L * 256;
+    int best = K;
+    double best_eff = -1;
+    for (int S = 1; S <= 8; S *= 2) {
+        const int kc = ((K + S - 1) / S + kSBK - 1) / kSBK * kSBK;
+        if (S > 1 && kc < 256) break;
+        const long blocks = tiles * ((K + kc - 1) / kc);
+        const double eff = double(blocks) / double((blocks + slots - 1) / slots * slots);
+        if (eff > best_eff + 1e-9) best_eff = eff, best = kc;
+    }
+    return best;
+}
+
 size_t gemm_set_workspace_bytes(int n, const int *M, int N, int K, int kc, size_t es)
 {
     size_t rows = 0;

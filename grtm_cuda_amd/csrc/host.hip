@@ -605,21 +605,54 @@ namespace {
 
 size_t esize(int prec) { return prec == GRTM_PREC_FP64 ? sizeof(double) : sizeof(float); }
 
+// Displacement-column plans: the same sequence as the Z / R plans (FP64 rows,
+// FP32 rows, residual check, dense fallback, FP64 re-solve of non-finite
+// cells, tile extents), on the column rows.
 int run_cells_mt(grtm_plan *p, hipStream_t s)
 {
-    if (p->prec == GRTM_PREC_FP64) {
-        HIPCHK(grtm::launch_cells_mt(0, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->nw,
-                                     p->mto, p->nk, nullptr, nullptr, s));
-        return GRTM_OK;
+    const bool ad = p->adaptive;
+    int *nkw = ad ? p->nk_w.as<int>() : nullptr;
+    unsigned *rowpk = ad ? p->counters.as<unsigned>() : nullptr;
+    unsigned *tailmx = ad ? rowpk + p->nw : nullptr;
+    unsigned *fbcnt = ad ? rowpk + 2 * p->nw : nullptr;
+    if (ad) {
+        HIPCHK(grtm::launch_copy_i32(p->nk_pred.as<int>(), nkw, p->nw, s));
+        HIPCHK(grtm::launch_zero_u32(rowpk, 2 * p->nw + 1, s));
     }
-    unsigned *cnt = p->flags.as<unsigned>(), *list = cnt + 1;
-    HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
-    HIPCHK(grtm::launch_cells_mt(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->rows64,
-                                 p->mto, p->nk, nullptr, nullptr, s));
-    HIPCHK(grtm::launch_cells_mt(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->rows64,
-                                 p->nw - p->rows64, p->mto, p->nk, cnt, list, s));
-    HIPCHK(grtm::launch_cells_mt_fix(p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->mto,
-                                     p->nk, cnt, list, s));
+    const bool mixed = p->prec != GRTM_PREC_FP64;
+    const int r64 = mixed ? p->rows64 : p->nw;
+    const unsigned *t_lo = ad ? p->tiles.as<unsigned>() : nullptr;
+    const unsigned *t_hi = ad ? t_lo + p->ntiles64 : nullptr;
+    const int n_lo = ad ? p->ntiles64 : 0, n_hi = ad ? p->ntiles - p->ntiles64 : 0;
+    unsigned *cnt = mixed ? p->flags.as<unsigned>() : nullptr, *list = mixed ? cnt + 1 : nullptr;
+    if (!mixed) {
+        HIPCHK(grtm::launch_cells_mt(0, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->nw,
+                                     t_lo, n_lo, p->mto, p->nk, nullptr, nullptr, nkw, rowpk, s));
+    } else {
+        HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
+        HIPCHK(grtm::launch_cells_mt(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0,
+                                     p->rows64, t_lo, n_lo, p->mto, p->nk, nullptr, nullptr, nkw, rowpk, s));
+        HIPCHK(grtm::launch_cells_mt(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->rows64,
+                                     p->nw - p->rows64, t_hi, n_hi, p->mto, p->nk, cnt, list, nkw, rowpk, s));
+    }
+    if (ad) {
+        const void *lwf64 = mixed ? p->lw64.p : p->lw.p;
+        HIPCHK(grtm::launch_check_tail(0, p->compat, lwf64, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, r64,
+                                       p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s, 1));
+        HIPCHK(grtm::launch_check_tail(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, r64,
+                                       p->nw - r64, p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s, 1));
+        HIPCHK(grtm::launch_fallback(!mixed, p->compat, lwf64, mixed ? p->lw.p : nullptr, r64, p->L, p->g, p->nk,
+                                     p->nw, p->dk, p->df, p->omega_i, p->chk_tol, rowpk, tailmx, nkw, fbcnt,
+                                     p->fb_list.as<unsigned>(), nullptr, nullptr, p->nk, cnt, list, 0, s, &p->mto));
+    }
+    if (mixed)
+        HIPCHK(grtm::launch_cells_mt_fix(p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->mto,
+                                         p->nk, cnt, list, s));
+    if (ad && p->pair) {
+        int rpw[grtm::kMtSets];
+        for (int q = 0; q < grtm::kMtSets; q++) rpw[q] = 2 * p->mto.rows[q];
+        HIPCHK(grtm::launch_kend_sets(p->nw, p->nk, grtm::kMtSets, rpw, 128, nkw, p->kend.as<int>(), s));
+    }
     return GRTM_OK;
 }
 
@@ -690,11 +723,21 @@ int run_gemm_mt(grtm_plan *p, hipStream_t s)
         B[q] = (const unsigned char *)p->B0.p + (size_t)q * p->nk * p->nr * es;
         C[q] = const_cast<void *>(p->mtc.C[q]);
     }
+    const int *nkw = p->adaptive ? p->nk_w.as<int>() : nullptr;
     if (p->pair) {
+        // adaptive: per-contraction tile extents laid end to end in kend
+        const int *kend[grtm::kMtSets];
+        int t0 = 0;
+        for (int q = 0; q < grtm::kMtSets; q++) {
+            kend[q] = p->adaptive ? p->kend.as<int>() + t0 : nullptr;
+            t0 += (M[q] + 127) / 128;
+        }
         HIPCHK(grtm::launch_gemm_set(p->gemm_mode, grtm::kMtSets, M, rpw, p->nr, p->nk, p->kc, A, p->nk, B, p->nr, C,
-                                     p->ws.p, nullptr, nullptr, s));
+                                     p->ws.p, kend, nkw, s));
         return GRTM_OK;
     }
+    // unaligned shapes and BF16x3: one contraction per set, dense grid only
+    // (grtm_plan_set_adaptive refuses these plans)
     for (int q = 0; q < grtm::kMtSets; q++)
         HIPCHK(grtm::launch_gemm_mfma(p->gemm_mode, M[q], p->nr, p->nk, A[q], p->nk, B[q], p->nr, C[q], p->ws.p,
                                       nullptr, nullptr, s));
@@ -877,7 +920,7 @@ static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid
             wsb = std::max(wsb, grtm::gemm_workspace_bytes(p->gemm_mode, M[q], p->nr, p->nk));
         if (aligned) {
             p->pair = true;
-            p->kc = grtm::gemm_pair_slice((int)(rows / 2), p->nr, p->nk, p->nk);
+            p->kc = grtm::gemm_set_slice(grtm::kMtSets, M, p->nr, p->nk);
             wsb = std::max(wsb, grtm::gemm_set_workspace_bytes(grtm::kMtSets, M, p->nr, p->nk, p->kc, es));
         }
         if ((status = p->ws.alloc(wsb))) return status;
@@ -1058,10 +1101,21 @@ int grtm_plan_set_rows64(grtm_plan *p, int rows64)
 int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int stride, double tol)
 {
     if (!p) return GRTM_E_INPUT;
-    if (p->kind != 0) return GRTM_E_UNSUPPORTED;
+    // column plans: only the streamed contraction set skips the rows' tails
+    if (p->kind == 1 && !p->pair) return GRTM_E_UNSUPPORTED;
+    int Mq[grtm::kMtSets], ntile = (p->nw + 15) / 16;
+    if (p->kind == 1) {
+        ntile = 0;
+        for (int q = 0; q < grtm::kMtSets; q++) {
+            Mq[q] = p->mt_M(q);
+            ntile += (Mq[q] + 127) / 128;
+        }
+    }
     if (!kcut) {
         p->adaptive = false;
-        if (p->pair) p->kc = grtm::gemm_pair_slice(4 * p->nw, p->nr, p->nk, p->nk);
+        if (p->pair)
+            p->kc = p->kind == 1 ? grtm::gemm_set_slice(grtm::kMtSets, Mq, p->nr, p->nk)
+                                 : grtm::gemm_pair_slice(4 * p->nw, p->nr, p->nk, p->nk);
         return GRTM_OK;
     }
     if (!(window >= 0) || stride < 1 || !(tol > 0)) return GRTM_E_INPUT;
@@ -1079,7 +1133,6 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
         mx = std::max(mx, p->nchk_h[iw]);
     }
     int st;
-    const int ntile = (p->nw + 15) / 16;
     if (!p->nk_pred.p) {
         if ((st = p->nk_pred.alloc(sizeof(int) * p->nw)) || (st = p->nk_w.alloc(sizeof(int) * p->nw)) ||
             (st = p->nchk.alloc(sizeof(int) * p->nw)) || (st = p->counters.alloc(sizeof(unsigned) * (2 * p->nw + 1))) ||
@@ -1089,7 +1142,16 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
     HIPCHK(hipMemcpy(p->nk_pred.p, p->nk_pred_h.data(), sizeof(int) * p->nw, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(p->nchk.p, p->nchk_h.data(), sizeof(int) * p->nw, hipMemcpyHostToDevice));
     if ((st = build_items(p))) return st;
-    if (p->pair) {
+    if (p->kind == 1) {
+        // slices from the mean K extent of the frequency rows
+        double ksum = 0, rows = 0;
+        for (int iw = 0; iw < p->nw; iw++) ksum += std::min(p->nk, (p->nk_pred_h[iw] + 31) / 32 * 32);
+        for (int q = 0; q < grtm::kMtSets; q++) rows += Mq[q];
+        const int kc = grtm::gemm_pair_slice((int)(rows / 2), p->nr, p->nk, ksum / p->nw);
+        const size_t need = grtm::gemm_set_workspace_bytes(grtm::kMtSets, Mq, p->nr, p->nk, kc, esize(p->prec));
+        if (need > p->ws.n && (st = p->ws.alloc(need))) return st;
+        p->kc = kc;
+    } else if (p->pair) {
         // contraction slices from the mean K extent of the 32-row tiles
         double ksum = 0;
         int nt = 0;
@@ -1113,6 +1175,7 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
 int grtm_plan_row_labels(grtm_plan *p, double tau, float *out)
 {
     if (!p || !out || !(tau > 0) || p->adaptive) return GRTM_E_INPUT;
+    if (p->kind != 0) return GRTM_E_UNSUPPORTED;
     if ((size_t)p->nk * sizeof(float) > 160 * 1024) return GRTM_E_UNSUPPORTED;
     DBuf d;
     int st;

@@ -19,6 +19,25 @@ struct PostParams {
     double omega_i;  // imaginary frequency
 };
 
+// Displacement columns (tensor columns 0-9, single-force columns 0-4):
+// operands of the four contractions against J0..J3 (rows of
+// mt_rows_from_cell), and each column's terms over their outputs.
+constexpr int kMtCols = 15;
+struct MtOps {
+    void *A[kMtSets];     // A_s: [2 rows[s] nw][lda], FP64 or FP32 as the launch
+    int rows[kMtSets];    // complex rows per frequency
+};
+struct MtTerm {
+    int set, row;   // contraction and complex row within a frequency
+    int rinv, pad;  // 1: divided by the receiver distance
+    double scale;
+};
+struct MtCols {
+    const void *C[kMtSets];  // contraction outputs [2 rows[s] nw][ldc]
+    int rows[kMtSets];
+    int nterm[kMtCols];
+    MtTerm term[kMtCols][2];
+};
 // prec: 0 = FP64 cells / FP64 operands, 1 = FP32 cells / FP32 operands
 // (non-finite cells appended to flag_list), 2 = FP64 cells / FP32 operands.
 // Work items are (row, tile of 256 wavenumbers), one block each: the list
@@ -32,9 +51,10 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
                            void *A1, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
                            unsigned *row_peak, hipStream_t st);
 // adaptive grid: residual check beyond the cut, dense fallback, tile extents
+// mt: check the displacement-column rows instead of the Z / R kernels
 hipError_t launch_check_tail(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk, double df,
                              double omega_i, int w0, int nwr, int max_chk, const int *nk_w, const int *nchk,
-                             int stride, unsigned *tail_max, hipStream_t st);
+                             int stride, unsigned *tail_max, hipStream_t st, int mt = 0);
 // Rows whose checked tail exceeds tol x row peak are completed on the dense
 // grid: FP64 plans in FP64; mixed plans in FP64 below row r64 and in FP32
 // (lw32) above it, non-finite FP32 cells appended to the flag list as
@@ -43,8 +63,11 @@ hipError_t launch_fallback(int out_f64, int compat, const void *lw64, const void
                            int nk, int nw, double dk, double df, double omega_i, double tol,
                            const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *fb_count,
                            unsigned *fb_list, void *A0, void *A1, long lda, unsigned *flag_count,
-                           unsigned *flag_list, int row_base, hipStream_t st);
+                           unsigned *flag_list, int row_base, hipStream_t st, const MtOps *mto = nullptr);
 hipError_t launch_kend_tiles(int nw, int rows_per_tile, int nk, const int *nk_w, int *kend, hipStream_t st);
+// extents of the bm-row tiles of n contractions with rpw[q] rows per frequency
+// row, laid end to end in kend
+hipError_t launch_kend_sets(int nw, int nk, int n, const int *rpw, int bm, const int *nk_w, int *kend, hipStream_t st);
 hipError_t launch_copy_i32(const int *a, int *b, int n, hipStream_t st);
 // per-row training labels (peak, truncation index, 8 local maxima); out: nw x 26 floats
 hipError_t launch_row_labels(int prec, int nw, int nk, long lda, const void *A0, const void *A1, double tau,
@@ -88,6 +111,7 @@ hipError_t launch_gemm_pair_f64(int M, int N, int K, int kc, const double *A0, c
 // 1 FP32; the same alignment rules as the pairs; workspace P of
 // gemm_set_workspace_bytes; kend (optional) per contraction and 128-row tile.
 size_t gemm_set_workspace_bytes(int n, const int *M, int N, int K, int kc, size_t es);
+int gemm_set_slice(int n, const int *M, int N, int K);
 hipError_t launch_gemm_set(int prec, int n, const int *M, const int *rpw, int N, int K, int kc, const void *const *A,
                            long lda, const void *const *B, long ldb, void *const *C, void *P,
                            const int *const *kend, const int *nk_w, hipStream_t st);
@@ -97,29 +121,11 @@ hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr,
                             const void *C1, long ldc, int nr, void *traces, void *spec_post, hipStream_t st,
                             long cms = 0);
 
-// Displacement columns (tensor columns 0-9, single-force columns 0-4):
-// operands of the four contractions against J0..J3 (rows of
-// mt_rows_from_cell), and each column's terms over their outputs.
-constexpr int kMtCols = 15;
-struct MtOps {
-    void *A[kMtSets];     // A_s: [2 rows[s] nw][lda], FP64 or FP32 as the launch
-    int rows[kMtSets];    // complex rows per frequency
-};
-struct MtTerm {
-    int set, row;   // contraction and complex row within a frequency
-    int rinv, pad;  // 1: divided by the receiver distance
-    double scale;
-};
-struct MtCols {
-    const void *C[kMtSets];  // contraction outputs [2 rows[s] nw][ldc]
-    int rows[kMtSets];
-    int nterm[kMtCols];
-    MtTerm term[kMtCols][2];
-};
-// prec as launch_cells_zr; rows [w0, w0 + nwr), every wavenumber
+// prec and work items as launch_cells_zr
 hipError_t launch_cells_mt(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk, double df,
-                           double omega_i, int w0, int nwr, const MtOps &o, long lda, unsigned *flag_count,
-                           unsigned *flag_list, hipStream_t st);
+                           double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles, const MtOps &o,
+                           long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
+                           hipStream_t st);
 hipError_t launch_cells_mt_fix(int compat, const void *lw64, int L, const Geo &g, int nk, double dk, double df,
                                double omega_i, const MtOps &o, long lda, const unsigned *flag_count,
                                const unsigned *flag_list, hipStream_t st);

@@ -130,7 +130,8 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__re
 // Residual check of the truncated grid: wavenumbers nk_w[iw] + stride j + stride/2,
 // j < nchk, are solved (values are not stored); their largest |value| goes to
 // tail_max[iw].  Rows [w0, w0 + gridDim.y).
-template <typename T, bool COMPAT>
+// MT: the displacement-column rows of the cells instead of the Z / R kernels.
+template <typename T, bool COMPAT, bool MT>
 __global__ __launch_bounds__(kCellThreads) void k_check_tail(const LayerW<T> *__restrict__ lw_all, int L, Geo g,
                                                              int nk, double dk, double df, double omega_i, int w0,
                                                              const int *__restrict__ nk_w, const int *__restrict__ nchk,
@@ -146,10 +147,16 @@ __global__ __launch_bounds__(kCellThreads) void k_check_tail(const LayerW<T> *__
     const int ik = nk_w[iw] + stride * j + stride / 2;
     unsigned mb = 0;
     if (j < nc && ik < nk) {
-        float v[8];
-        cell_zr<T, float, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
         float m = 0.f;
-        for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(v[i]));
+        if constexpr (MT) {
+            cell_mt_rows<T, COMPAT>(lw, g, iw, ik, dk, df, omega_i, [&](int, int, T vr, T vi) {
+                m = fmaxf(m, fmaxf(fabsf(float(vr)), fabsf(float(vi))));
+            });
+        } else {
+            float v[8];
+            cell_zr<T, float, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
+            for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(v[i]));
+        }
         if (!isfinite(m)) m = 3.0e38f;  // a non-finite check value fails the row
         mb = __float_as_uint(m);
     }
@@ -248,12 +255,13 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_fix_batch(const ModelDes
 // row whose index lies in [r_lo, r_hi), in the arithmetic T of that row's
 // recursion.  Grid (k blocks, row slots); slots stride over the list.  FP32
 // cells that are not finite go to the flag list as (row_base + iw) nk + ik.
-template <typename T, typename TO, bool COMPAT>
+// MT: displacement-column rows, stored through o (A0, A1 unused).
+template <typename T, typename TO, bool COMPAT, bool MT>
 __global__ __launch_bounds__(kCellThreads) void k_fallback_cells(
     const LayerW<T> *__restrict__ lw_all, int L, Geo g, int nk, int nw, double dk, double df, double omega_i,
     int r_lo, int r_hi, TO *__restrict__ A0, TO *__restrict__ A1, long lda, const unsigned *__restrict__ fb_count,
     const unsigned *__restrict__ fb_list, unsigned *__restrict__ flag_count, unsigned *__restrict__ flag_list,
-    int row_base)
+    int row_base, MtOps o)
 {
     __shared__ LayerW<T> lw[kMaxLayers];
     const unsigned n = *fb_count;
@@ -265,9 +273,15 @@ __global__ __launch_bounds__(kCellThreads) void k_fallback_cells(
         for (int q = threadIdx.x; q < L; q += blockDim.x) lw[q] = lw_all[(size_t)iw * L + q];
         __syncthreads();
         for (int ik = start + blockIdx.x * blockDim.x + threadIdx.x; ik < nk; ik += gridDim.x * blockDim.x) {
-            TO v[8];
-            const bool ok = cell_zr<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
-            store_zr(v, iw, ik, A0, A1, lda);
+            bool ok;
+            if constexpr (MT) {
+                unsigned mag;
+                ok = cell_mt<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, o, lda, mag);
+            } else {
+                TO v[8];
+                ok = cell_zr<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
+                store_zr(v, iw, ik, A0, A1, lda);
+            }
             if (!ok && flag_count) {
                 const unsigned slot2 = atomicAdd(flag_count, 1u);
                 flag_list[slot2] = unsigned(row_base + iw) * unsigned(nk) + unsigned(ik);
@@ -285,6 +299,27 @@ __global__ void k_kend_tiles(int nw, int rows_per_tile, int nk, const int *__res
     if (t >= ntile) return;
     int m = 0;
     for (int r = t * rows_per_tile; r < min(nw, (t + 1) * rows_per_tile); r++) m = max(m, nk_w[r]);
+    kend[t] = min(nk, (m + 31) / 32 * 32);
+}
+
+// K extents of the row tiles of up to four contractions laid end to end in
+// kend: contraction q has M_q = rpw_q nw rows (rpw_q per frequency row) and
+// ceil(M_q / bm) tiles; each extent is rounded up to a multiple of 32.
+struct KendSets {
+    int n, bm;
+    int rpw[4];
+    int tile0[5];  // first tile of each contraction; tile0[n] = all tiles
+};
+__global__ void k_kend_sets(int nw, int nk, KendSets ks, const int *__restrict__ nk_w, int *__restrict__ kend)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ks.tile0[ks.n]) return;
+    int q = 0;
+    while (q + 1 < ks.n && t >= ks.tile0[q + 1]) q++;
+    const int tl = t - ks.tile0[q], rpw = ks.rpw[q];
+    const int r0 = tl * ks.bm / rpw, r1 = min(nw - 1, ((tl + 1) * ks.bm - 1) / rpw);
+    int m = 0;
+    for (int r = r0; r <= r1; r++) m = max(m, nk_w[r]);
     kend[t] = min(nk, (m + 31) / 32 * 32);
 }
 
@@ -571,12 +606,11 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T *__re
 
 // ---------------------------------------------------------------- moment tensor
 // Recursion of the displacement-column operands: one thread per (omega, k)
-// cell, the rows of mt_rows_from_cell weighted by the k-quadrature and
-// stored to A_s[2 (rows_s iw + j) + {0 re, 1 im}][ik].  FP32 cells whose
-// result is not finite go to the flag list (k_cells_mt_fix re-solves them).
-template <typename T, typename TO, bool COMPAT>
-__device__ __forceinline__ bool cell_mt(const LayerW<T> *lw, const Geo &g, int iw, int ik, double dk, double df,
-                                        double omega_i, const MtOps &o, long lda)
+// cell, the rows of mt_rows_from_cell weighted by the k-quadrature;
+// put(set, row, re, im) receives them.
+template <typename T, bool COMPAT, typename Put>
+__device__ __forceinline__ void cell_mt_rows(const LayerW<T> *lw, const Geo &g, int iw, int ik, double dk, double df,
+                                             double omega_i, Put &&put)
 {
     const double kd = dk * ik;
     const T k = T(kd);
@@ -584,37 +618,70 @@ __device__ __forceinline__ bool cell_mt(const LayerW<T> *lw, const Geo &g, int i
     CellOut<T> c;
     cell_solve<T, COMPAT, false>(lw, g, kd, C<T>(0, 0), C<T>(0, 0), c);
     const T wt = T(ik == 0 ? 0.5 * dk : dk);
+    mt_rows_from_cell<T, COMPAT>(c, g.up, k, w, [&](int set, int j, cx<T> v) { put(set, j, wt * v.x, wt * v.y); });
+}
+
+// Stores a cell's rows to A_s[2 (rows_s iw + j) + {0 re, 1 im}][ik]; returns
+// whether all are finite, and their largest |value| as float bits in mag.
+template <typename T, typename TO, bool COMPAT>
+__device__ __forceinline__ bool cell_mt(const LayerW<T> *lw, const Geo &g, int iw, int ik, double dk, double df,
+                                        double omega_i, const MtOps &o, long lda, unsigned &mag)
+{
     bool ok = true;
-    mt_rows_from_cell<T, COMPAT>(c, g.up, k, w, [&](int set, int j, cx<T> v) {
-        const TO re = TO(wt * v.x), im = TO(wt * v.y);
+    float m = 0.f;
+    cell_mt_rows<T, COMPAT>(lw, g, iw, ik, dk, df, omega_i, [&](int set, int j, T vr, T vi) {
+        const TO re = TO(vr), im = TO(vi);
         TO *A = reinterpret_cast<TO *>(o.A[set]);
         const size_t row = 2 * ((size_t)o.rows[set] * iw + j);
         A[row * lda + ik] = re;
         A[(row + 1) * lda + ik] = im;
-        ok = ok && isfinite(float(re)) && isfinite(float(im));
+        const float fr = fabsf(float(re)), fi = fabsf(float(im));
+        ok = ok && isfinite(fr) && isfinite(fi);
+        m = fmaxf(m, fmaxf(fr, fi));
     });
+    mag = isfinite(m) ? __float_as_uint(m) : 0u;
     return ok;
 }
 
-// Rows [w0, w0 + gridDim.x / tpr), one block per (row, 256-wavenumber tile).
+// Work items as k_cells_zr: the list `tiles` (row << kTileBits | k tile) or
+// every tile of rows [w0, w0 + gridDim.x / tpr); nk_w (optional) limits the
+// wavenumbers of each row, row_peak (optional) collects the row maxima.
 template <typename T, typename TO, bool COMPAT>
 __global__ __launch_bounds__(kCellThreads) void k_cells_mt(const LayerW<T> *__restrict__ lw_all, int L, Geo g,
                                                            int nk, double dk, double df, double omega_i, int w0,
-                                                           int tpr, MtOps o, long lda,
-                                                           unsigned *__restrict__ flag_count,
-                                                           unsigned *__restrict__ flag_list)
+                                                           int tpr, const unsigned *__restrict__ tiles, MtOps o,
+                                                           long lda, unsigned *__restrict__ flag_count,
+                                                           unsigned *__restrict__ flag_list,
+                                                           const int *__restrict__ nk_w,
+                                                           unsigned *__restrict__ row_peak)
 {
     __shared__ LayerW<T> lw[kMaxLayers];
-    const int q = int(blockIdx.x) / tpr;
-    const int iw = w0 + q, kb = int(blockIdx.x) - q * tpr;
+    int iw, kb;
+    if (tiles) {
+        const unsigned e = tiles[blockIdx.x];
+        iw = int(e >> kTileBits);
+        kb = int(e & ((1u << kTileBits) - 1));
+    } else {
+        const int q = int(blockIdx.x) / tpr;
+        iw = w0 + q;
+        kb = int(blockIdx.x) - q * tpr;
+    }
+    const int nkr = nk_w ? nk_w[iw] : nk;
+    if (kb * kCellThreads >= nkr) return;  // uniform over the block
     for (int j = threadIdx.x; j < L; j += blockDim.x) lw[j] = lw_all[(size_t)iw * L + j];
     __syncthreads();
     const int ik = kb * kCellThreads + threadIdx.x;
-    if (ik >= nk) return;
-    const bool ok = cell_mt<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, o, lda);
-    if (!ok && flag_count) {
-        const unsigned slot = atomicAdd(flag_count, 1u);
-        flag_list[slot] = unsigned(iw) * unsigned(nk) + unsigned(ik);
+    unsigned mag = 0;
+    if (ik < nkr) {
+        const bool ok = cell_mt<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, o, lda, mag);
+        if (!ok && flag_count) {
+            const unsigned slot = atomicAdd(flag_count, 1u);
+            flag_list[slot] = unsigned(iw) * unsigned(nk) + unsigned(ik);
+        }
+    }
+    if (row_peak) {
+        mag = wave_max_u32(mag);
+        if ((threadIdx.x & 63) == 0 && mag) atomicMax(row_peak + iw, mag);
     }
 }
 
@@ -630,7 +697,8 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_mt_fix(const LayerW<doub
     for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         const unsigned cell = flag_list[t];
         const int iw = int(cell / unsigned(nk)), ik = int(cell % unsigned(nk));
-        cell_mt<double, TO, COMPAT>(lw_all + (size_t)iw * L, g, iw, ik, dk, df, omega_i, o, lda);
+        unsigned mag;
+        cell_mt<double, TO, COMPAT>(lw_all + (size_t)iw * L, g, iw, ik, dk, df, omega_i, o, lda, mag);
     }
 }
 
@@ -795,19 +863,27 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
 
 hipError_t launch_check_tail(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk, double df,
                              double omega_i, int w0, int nwr, int max_chk, const int *nk_w, const int *nchk,
-                             int stride, unsigned *tail_max, hipStream_t st)
+                             int stride, unsigned *tail_max, hipStream_t st, int mt)
 {
     if (nwr <= 0 || max_chk <= 0) return hipSuccess;
     dim3 grid((max_chk + kCellThreads - 1) / kCellThreads, nwr);
-#define GRTM_CHK(T, CP)                                                                                 \
-    hipLaunchKernelGGL((k_check_tail<T, CP>), grid, dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, L, g, nk, \
+#define GRTM_CHK(T, CP, MT)                                                                                     \
+    hipLaunchKernelGGL((k_check_tail<T, CP, MT>), grid, dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, L, g, nk, \
                        dk, df, omega_i, w0, nk_w, nchk, stride, tail_max)
-    if (prec == 0) {
-        if (compat) GRTM_CHK(double, true);
-        else GRTM_CHK(double, false);
+    if (mt) {
+        if (prec == 0) {
+            if (compat) GRTM_CHK(double, true, true);
+            else GRTM_CHK(double, false, true);
+        } else {
+            if (compat) GRTM_CHK(float, true, true);
+            else GRTM_CHK(float, false, true);
+        }
+    } else if (prec == 0) {
+        if (compat) GRTM_CHK(double, true, false);
+        else GRTM_CHK(double, false, false);
     } else {
-        if (compat) GRTM_CHK(float, true);
-        else GRTM_CHK(float, false);
+        if (compat) GRTM_CHK(float, true, false);
+        else GRTM_CHK(float, false, false);
     }
 #undef GRTM_CHK
     return hipGetLastError();
@@ -817,28 +893,50 @@ hipError_t launch_fallback(int out_f64, int compat, const void *lw64, const void
                            int nk, int nw, double dk, double df, double omega_i, double tol,
                            const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *fb_count,
                            unsigned *fb_list, void *A0, void *A1, long lda, unsigned *flag_count,
-                           unsigned *flag_list, int row_base, hipStream_t st)
+                           unsigned *flag_list, int row_base, hipStream_t st, const MtOps *mto)
 {
     hipLaunchKernelGGL(k_fallback_select, dim3((nw + 255) / 256), dim3(256), 0, st, nw, nk, tol, row_peak, tail_max,
                        nk_w, fb_count, fb_list);
     dim3 grid((nk + kCellThreads - 1) / kCellThreads, 16);
-#define GRTM_FB(T, TO, CP, LWP, LO, HI, FC, FL)                                                                  \
-    hipLaunchKernelGGL((k_fallback_cells<T, TO, CP>), grid, dim3(kCellThreads), 0, st, (const LayerW<T> *)LWP, L, g, \
-                       nk, nw, dk, df, omega_i, LO, HI, (TO *)A0, (TO *)A1, lda, fb_count, fb_list, FC, FL, row_base)
-    if (out_f64) {
-        if (compat) GRTM_FB(double, double, true, lw64, 0, nw, nullptr, nullptr);
-        else GRTM_FB(double, double, false, lw64, 0, nw, nullptr, nullptr);
-    } else {
-        // rows below r64 complete in FP64, the others in FP32 like the rest of their row
-        if (compat) {
-            GRTM_FB(double, float, true, lw64, 0, r64, nullptr, nullptr);
-            GRTM_FB(float, float, true, lw32, r64, nw, flag_count, flag_list);
-        } else {
-            GRTM_FB(double, float, false, lw64, 0, r64, nullptr, nullptr);
-            GRTM_FB(float, float, false, lw32, r64, nw, flag_count, flag_list);
-        }
+    const MtOps o = mto ? *mto : MtOps{};
+#define GRTM_FB(T, TO, CP, MT, LWP, LO, HI, FC, FL)                                                             \
+    hipLaunchKernelGGL((k_fallback_cells<T, TO, CP, MT>), grid, dim3(kCellThreads), 0, st, (const LayerW<T> *)LWP, \
+                       L, g, nk, nw, dk, df, omega_i, LO, HI, (TO *)A0, (TO *)A1, lda, fb_count, fb_list, FC, FL,  \
+                       row_base, o)
+#define GRTM_FB_ALL(MT)                                                                              \
+    if (out_f64) {                                                                                   \
+        if (compat) GRTM_FB(double, double, true, MT, lw64, 0, nw, nullptr, nullptr);                \
+        else GRTM_FB(double, double, false, MT, lw64, 0, nw, nullptr, nullptr);                      \
+    } else if (compat) {                                                                             \
+        GRTM_FB(double, float, true, MT, lw64, 0, r64, nullptr, nullptr);                            \
+        GRTM_FB(float, float, true, MT, lw32, r64, nw, flag_count, flag_list);                       \
+    } else {                                                                                         \
+        GRTM_FB(double, float, false, MT, lw64, 0, r64, nullptr, nullptr);                           \
+        GRTM_FB(float, float, false, MT, lw32, r64, nw, flag_count, flag_list);                      \
     }
+    // rows below r64 complete in FP64, the others in FP32 like the rest of their row
+    if (mto) {
+        GRTM_FB_ALL(true)
+    } else {
+        GRTM_FB_ALL(false)
+    }
+#undef GRTM_FB_ALL
 #undef GRTM_FB
+    return hipGetLastError();
+}
+
+hipError_t launch_kend_sets(int nw, int nk, int n, const int *rpw, int bm, const int *nk_w, int *kend, hipStream_t st)
+{
+    if (n < 1 || n > 4) return hipErrorInvalidValue;
+    KendSets ks{};
+    ks.n = n;
+    ks.bm = bm;
+    ks.tile0[0] = 0;
+    for (int q = 0; q < n; q++) {
+        ks.rpw[q] = rpw[q];
+        ks.tile0[q + 1] = ks.tile0[q] + (rpw[q] * nw + bm - 1) / bm;
+    }
+    hipLaunchKernelGGL(k_kend_sets, dim3((ks.tile0[n] + 255) / 256), dim3(256), 0, st, nw, nk, ks, nk_w, kend);
     return hipGetLastError();
 }
 
@@ -932,14 +1030,16 @@ hipError_t launch_gemm(int prec, int M, int N, int K, const void *A, long lda, c
 }
 
 hipError_t launch_cells_mt(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk, double df,
-                           double omega_i, int w0, int nwr, const MtOps &o, long lda, unsigned *flag_count,
-                           unsigned *flag_list, hipStream_t st)
+                           double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles, const MtOps &o,
+                           long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
+                           hipStream_t st)
 {
     const int tpr = cells_tiles_per_row(nk);
-    if (nwr <= 0) return hipSuccess;
-#define GRTM_MT(T, TO, CP, FC, FL)                                                                             \
-    hipLaunchKernelGGL((k_cells_mt<T, TO, CP>), dim3(nwr * tpr), dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, \
-                       L, g, nk, dk, df, omega_i, w0, tpr, o, lda, FC, FL)
+    if (!tiles) ntiles = nwr * tpr;
+    if (ntiles <= 0) return hipSuccess;
+#define GRTM_MT(T, TO, CP, FC, FL)                                                                                \
+    hipLaunchKernelGGL((k_cells_mt<T, TO, CP>), dim3(ntiles), dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, L, \
+                       g, nk, dk, df, omega_i, w0, tpr, tiles, o, lda, FC, FL, nk_w, row_peak)
     if (prec == 0) {
         if (compat) GRTM_MT(double, double, true, nullptr, nullptr);
         else GRTM_MT(double, double, false, nullptr, nullptr);

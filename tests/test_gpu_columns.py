@@ -98,3 +98,29 @@ def test_columns_at_production_size():
     worst = rl.max(axis=1)
     print("  FP32 vs FP64 per column (max rel-L2): " + " ".join(f"{v:.1e}" for v in worst))
     assert rl.max() < 1e-4 and pk.max() < 1e-4
+
+
+@pytest.mark.parametrize("prec,tol", [(g.PREC_FP64, 1e-5), (g.PREC_FP32, 1e-4)])
+def test_adaptive_columns_match_dense(prec, tol):
+    """The learned adaptive grid on a column plan (work items below the cut,
+    tail check of the column rows, dense fallback, ragged contractions)
+    against the plain FP64 dense column plan, 1024 x 8192, 64 receivers."""
+    from grtm_cuda_amd import learned
+    layers, zs = models.layered_model(77, 10), 7.5
+    vs_min = float(layers[:, 2].min())
+    grid = models.dense_grid(1024, 0.05, 8192, vs_min)
+    dists = np.linspace(5.0, 300.0, 64)
+    print(f"gpu: adaptive columns, precision {prec}", flush=True)
+    ref = g.ColumnPlan(layers, zs, 0.0, dists, grid, precision=g.PREC_FP64).traces()
+    plan = g.ColumnPlan(layers, zs, 0.0, dists, grid, precision=prec)
+    kcut = learned.Predictor().kcut(layers, zs, 0.0, grid)
+    plan.set_adaptive(kcut, window=models.check_window(zs, 0.0), stride=8, tol=3e-6)
+    tr = plan.traces()
+    st = plan.adaptive_stats()
+    rl, pk = rel_errors(tr, ref)
+    print(f"  cells kept {st['cells_final'] / (1024 * 8192):.3f}, rows falling back {st['rows_fallback']}, "
+          f"max rel-L2 {rl.max():.2e}, max peak error {pk.max():.2e}")
+    assert st["cells_final"] < 0.6 * 1024 * 8192
+    assert rl.max() < tol and pk.max() < tol
+    # a second execution gives the same traces (counters and extents are reset)
+    assert np.array_equal(plan.traces(), tr)
