@@ -485,7 +485,13 @@ struct Coef {
 
 // Source vectors and coupling.  src_type 0: single force (green.cu:1271-1332,
 // 918-938); otherwise moment tensor (green.cu:1334-1413, 1239-1266).
-template <typename T, bool NEED_T>
+// The tensor's SH terms: the reference (green.cu:1408-1411) gives both orders
+// the factor 1 + R above the source and R - 1 below.  Source-receiver
+// reciprocity with the single force's strain fixes them: n = 1 is a jump in
+// displacement (antisymmetric, like the P-SV s1: 1 - R above, R - 1 below)
+// and n = 2 a jump in traction (symmetric, like s0 and s2: 1 + R on both
+// sides).  COMPAT keeps the reference's factors.
+template <typename T, bool NEED_T, bool COMPAT>
 GRTM_HD void coef_from_cell(const CellOut<T> &c, int src_type, int up, T k, cx<T> w, Coef<T> &o)
 {
     const LQ<T> &q = c.sq;
@@ -524,8 +530,10 @@ GRTM_HD void coef_from_cell(const CellOut<T> &c, int src_type, int up, T k, cx<T
         mv(W1, s00, s01, u00, u01);
         mv(W0, s10, s11, u10, u11);
         mv(W1, s20, s21, u20, u21);
-        const cx<T> shf = up ? (C<T>(1, 0) + c.erd0) : (c.eru0 - C<T>(1, 0));
-        const cx<T> sh1 = shf * q.v * ssh, sh2 = shf * k * ssh;
+        const cx<T> one = C<T>(1, 0);
+        const cx<T> sh1f = up ? (COMPAT ? one + c.erd0 : one - c.erd0) : (c.eru0 - one);
+        const cx<T> sh2f = up ? (one + c.erd0) : (COMPAT ? c.eru0 - one : one + c.eru0);
+        const cx<T> sh1 = sh1f * q.v * ssh, sh2 = sh2f * k * ssh;
         o.b1 = c.y0 * sh1;
         o.b2 = c.y0 * sh2;
         mv(c.Y, u00, u01, o.a0[0], o.a0[1]);
@@ -601,7 +609,7 @@ GRTM_HD void mt_rows_from_cell(const CellOut<T> &c, int up, T k, cx<T> w, Put &&
 {
     {
         Coef<T> t;
-        coef_from_cell<T, false>(c, 1, up, k, w, t);
+        coef_from_cell<T, false, COMPAT>(c, 1, up, k, w, t);
         put(0, 0, t.a0[1] * k);
         put(0, 1, t.a2[1] * k);
         put(1, 0, (COMPAT ? t.a2[1] : t.a2[0]) * k);
@@ -617,7 +625,7 @@ GRTM_HD void mt_rows_from_cell(const CellOut<T> &c, int up, T k, cx<T> w, Put &&
         put(3, 1, t.b2 * k);
     }
     Coef<T> f;
-    coef_from_cell<T, false>(c, 0, up, k, w, f);
+    coef_from_cell<T, false, COMPAT>(c, 0, up, k, w, f);
     put(0, 2, f.a0[1] * k);
     put(1, 4, f.a0[0] * k);
     put(1, 5, f.a1[1] * k);

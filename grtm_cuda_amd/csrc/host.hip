@@ -1399,7 +1399,8 @@ int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi,
     grtm::Coef<double> p;
     if (compat) grtm::cell_solve<double, true, true>(lw.data(), g, k, p0r, p0v, c);
     else grtm::cell_solve<double, false, true>(lw.data(), g, k, p0r, p0v, c);
-    grtm::coef_from_cell<double, true>(c, src_type, g.up, k, grtm::cx<double>{wr, wi}, p);
+    if (compat) grtm::coef_from_cell<double, true, true>(c, src_type, g.up, k, grtm::cx<double>{wr, wi}, p);
+    else grtm::coef_from_cell<double, true, false>(c, src_type, g.up, k, grtm::cx<double>{wr, wi}, p);
     const grtm::cx<double> v[18] = {p.a0[0], p.a0[1], p.a1[0], p.a1[1], p.a2[0], p.a2[1], p.b1, p.b2, p.a0t[0],
                                     p.a0t[1], p.a1t[0], p.a1t[1], p.a2t[0], p.a2t[1], p.b1t, p.b2t, {0, 0}, {0, 0}};
     for (int i = 0; i < 18; i++) out[2 * i] = v[i].x, out[2 * i + 1] = v[i].y;

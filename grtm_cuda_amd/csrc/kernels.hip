@@ -499,7 +499,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_full(const LayerW<T> *__
         CellOut<T> c;
         cell_solve<T, COMPAT, true>(lw, g, k, C<T>(T(p0r_x), T(p0r_y)), C<T>(T(p0v_x), T(p0v_y)), c);
         Coef<T> p;
-        coef_from_cell<T, true>(c, SRC, g.up, k, w, p);
+        coef_from_cell<T, true, COMPAT>(c, SRC, g.up, k, w, p);
         BesselSet bs;
         bessel_set(COMPAT ? 0 : 1, dist * double(k), bs);
         integrand_cols<T>(COMPAT ? 1 : 0, SRC, p, double(k), dist, bs, col);

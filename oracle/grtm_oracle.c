@@ -151,6 +151,11 @@ static void solve4(cd A[4][4], cd B[4][4], cd X[4][4])
 static int g_inverse_mode = 0;
 void or_set_inverse_mode(int m) { g_inverse_mode = m; }
 
+/* Test switch: the reference's SH factors of the tensor source in physics
+ * mode too (a planted mistake for the reciprocity tests). */
+static int g_tensor_sh_reference = 0;
+void or_set_tensor_sh_reference(int on) { g_tensor_sh_reference = on; }
+
 /* Interface between layer j (above) and j+1 (below):
  * SH (green.cu:339-353) and P-SV (green.cu:235-262 + 1813-1817 + 359-387). */
 static void interface_rt(const OPar *pu, const OPar *pd, double k, cd w, OMrt *m)
@@ -317,7 +322,7 @@ static void grt_sweeps(const OPar *par, const OMrt *mrt, int L, int s, double k,
 }
 
 /* Source terms.  Single force: green.cu:1271-1332; tensor: green.cu:1334-1413. */
-static void source_terms(int src_type, const OPar *sp, const OGrt *grt, int s, double zs,
+static void source_terms(int compat, int src_type, const OPar *sp, const OGrt *grt, int s, double zs,
                          double k, cd w, OSrc *src)
 {
     cd sexd[2], sexu[2], wru0[2][2], wrd0[2][2], wru1[2][2], wrd1[2][2];
@@ -367,10 +372,17 @@ static void source_terms(int src_type, const OPar *sp, const OGrt *grt, int s, d
             src->su2[i] = wru1[i][0] * s2[0] + wru1[i][1] * s2[1];
             src->sd2[i] = wrd1[i][0] * s2[0] + wrd1[i][1] * s2[1];
         }
-        src->shu1 = (1. + sexd[1] * grt[s + 1].rdu0 * sexd[1]) * sp->v * ssh;
-        src->shd1 = (sexu[1] * grt[s].rud0 * sexu[1] - 1.) * sp->v * ssh;
-        src->shu2 = (1. + sexd[1] * grt[s + 1].rdu0 * sexd[1]) * k * ssh;
-        src->shd2 = (sexu[1] * grt[s].rud0 * sexu[1] - 1.) * k * ssh;
+        /* SH: the reference (green.cu:1408-1411) gives both orders the
+         * symmetric factor 1 + R above the source and the antisymmetric
+         * R - 1 below.  Physics mode: n = 1 (a jump in displacement, like
+         * the P-SV s1) is antisymmetric on both sides, 1 - R above; n = 2
+         * (a jump in traction, like s0 and s2) symmetric, 1 + R below. */
+        cd erd0 = sexd[1] * grt[s + 1].rdu0 * sexd[1], eru0 = sexu[1] * grt[s].rud0 * sexu[1];
+        int ref_sh = compat || g_tensor_sh_reference;
+        src->shu1 = (ref_sh ? 1. + erd0 : 1. - erd0) * sp->v * ssh;
+        src->shd1 = (eru0 - 1.) * sp->v * ssh;
+        src->shu2 = (1. + erd0) * k * ssh;
+        src->shd2 = (ref_sh ? eru0 - 1. : 1. + eru0) * k * ssh;
     }
 }
 
@@ -619,7 +631,7 @@ static int cell_ppar(int compat, int src_type, double k, cd w, int L, const OPar
     }
     for (int j = 0; j < L - 1; j++) interface_rt(&par[j], &par[j + 1], k, w, &mrt[j]);
     grt_sweeps(par, mrt, L, s, k, w, grt);
-    source_terms(src_type, &par[s], grt, s, zs, k, w, &src);
+    source_terms(compat, src_type, &par[s], grt, s, zs, k, w, &src);
     return ypars(compat, src_type, par, grt, &src, L, s, zs, rl, zr, k, w, par0_r, par0_v, pp);
 }
 
@@ -1454,6 +1466,14 @@ int or_columns(int compat, const double *lay, int n_in, double src_depth, double
                const double *dists, int nr, int nw, double dt, int nk, double kmax, double omega_i,
                double t0, double taper, double *traces, int nthreads)
 {
+    return or_columns_all(compat, 0, lay, n_in, src_depth, rcv_depth, dists, nr, nw, dt, nk, kmax, omega_i, t0,
+                          taper, traces, nthreads);
+}
+
+int or_columns_all(int compat, int all30, const double *lay, int n_in, double src_depth, double rcv_depth,
+                   const double *dists, int nr, int nw, double dt, int nk, double kmax, double omega_i,
+                   double t0, double taper, double *traces, int nthreads)
+{
     if (nr < 1 || nw < 1 || nk < 1) return OR_E_INPUT;
     or_run run;
     int st = trace_run(compat, lay, n_in, src_depth, rcv_depth, &run);
@@ -1462,7 +1482,7 @@ int or_columns(int compat, const double *lay, int n_in, double src_depth, double
     grid_consts g = {nw, df, omega_i, dk};
     cd r0, v0;
     par0_rv(&run, compat, &g, &r0, &v0);
-    const int ncol = OR_N_COLUMNS;
+    const int ncol = all30 ? 60 : OR_N_COLUMNS;
     cd *raw = (cd *)calloc((size_t)ncol * nr * nw, sizeof(cd));
     int err = 0;
 #ifdef _OPENMP
@@ -1487,8 +1507,9 @@ int or_columns(int compat, const double *lay, int n_in, double src_depth, double
                 cd ct[30], cf[30];
                 integrand(compat, 1, &pt, k, dists[ir], ct);
                 integrand(compat, 0, &pf, k, dists[ir], cf);
-                for (int c = 0; c < 10; c++) raw[((size_t)c * nr + ir) * nw + iw] += ct[c] * wgt;
-                for (int c = 0; c < 5; c++) raw[((size_t)(10 + c) * nr + ir) * nw + iw] += cf[c] * wgt;
+                const int nt = all30 ? 30 : 10, nf = all30 ? 30 : 5;
+                for (int c = 0; c < nt; c++) raw[((size_t)c * nr + ir) * nw + iw] += ct[c] * wgt;
+                for (int c = 0; c < nf; c++) raw[((size_t)(nt + c) * nr + ir) * nw + iw] += cf[c] * wgt;
             }
         }
     }

@@ -82,6 +82,10 @@ int or_cell(int compat, int src_type, double k, double wr, double wi,
             int n_layer, const double *lay, int s_layer, double zs,
             int r_layer, double zr, const double *par0_rv, double *out);
 
+/* Test switch: 1 = the reference's SH factors of the tensor source
+ * (green.cu:1408-1411) in physics mode as well. */
+void or_set_tensor_sh_reference(int on);
+
 /* Bessel family as used by the integrand: mode 0 = reference formulas
  * (green.cu:1418-1554), mode 1 = libm jn.  which: 0 J_n, 1 J_n', 2 "J_n''". */
 double or_bessel(int mode, int which, int n, double x);
@@ -108,6 +112,14 @@ int or_columns(int compat, const double *lay, int n_in, double src_depth,
                double rcv_depth, const double *dists, int nr, int nw,
                double dt, int nk, double kmax, double omega_i, double t0,
                double taper, double *traces, int nthreads);
+/* all30 = 1: all 30 integrand columns of each source instead,
+ * traces[((src*30 + c)*nr + ir)*N + j] with src 0 tensor, 1 single force
+ * (physics mode: d/dz in columns 5-9 / 10-19, d/dr in 10-14 / 20-29). */
+int or_columns_all(int compat, int all30, const double *lay, int n_in,
+                   double src_depth, double rcv_depth, const double *dists,
+                   int nr, int nw, double dt, int nk, double kmax,
+                   double omega_i, double t0, double taper, double *traces,
+                   int nthreads);
 
 /* The same dense-grid integration but returning the raw k-kernels
  * Kz, Kr (complex) for every (iw, ik) in [w_lo,w_hi) x [0,nk):

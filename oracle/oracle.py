@@ -73,6 +73,11 @@ def set_inverse_mode(m):
     lib().or_set_inverse_mode(int(m))
 
 
+def set_tensor_sh_reference(on):
+    """Test switch: the reference's SH factors of the tensor source in physics mode too."""
+    lib().or_set_tensor_sh_reference(int(on))
+
+
 def dispersion(w, v0, q):
     o = np.zeros(2)
     lib().or_dispersion(D(w.real), D(w.imag), D(v0), D(q), o.ctypes.data_as(P))
@@ -128,20 +133,21 @@ def traces(layers, src_depth, rcv_depth, dists, src_kind, grid, compat=False, w_
 N_COLUMNS = 15
 
 
-def columns(layers, src_depth, rcv_depth, dists, grid, compat=False, nthreads=0):
+def columns(layers, src_depth, rcv_depth, dists, grid, compat=False, nthreads=0, all30=False):
     """Displacement-column traces [15, nr, N]: tensor columns 0-9, then
-    single-force columns 0-4, on the dense grid."""
+    single-force columns 0-4, on the dense grid.  all30: every integrand
+    column of both sources instead, [2 (tensor, single force), 30, nr, N]."""
     layers = np.ascontiguousarray(layers, dtype=np.float64)
     dists = np.ascontiguousarray(dists, dtype=np.float64)
     nw = int(grid["nw"])
-    tr = np.zeros((N_COLUMNS, len(dists), 2 * nw))
-    st = lib().or_columns(int(compat), layers.ctypes.data_as(P), layers.shape[0], D(src_depth), D(rcv_depth),
-                          dists.ctypes.data_as(P), len(dists), nw, D(grid["dt"]), int(grid["nk"]), D(grid["kmax"]),
-                          D(grid["omega_i"]), D(grid.get("t0", 0.0)), D(grid.get("taper", 0.0)),
-                          tr.ctypes.data_as(P), int(nthreads))
+    tr = np.zeros((60 if all30 else N_COLUMNS, len(dists), 2 * nw))
+    st = lib().or_columns_all(int(compat), int(all30), layers.ctypes.data_as(P), layers.shape[0], D(src_depth),
+                              D(rcv_depth), dists.ctypes.data_as(P), len(dists), nw, D(grid["dt"]), int(grid["nk"]),
+                              D(grid["kmax"]), D(grid["omega_i"]), D(grid.get("t0", 0.0)), D(grid.get("taper", 0.0)),
+                              tr.ctypes.data_as(P), int(nthreads))
     if st:
         raise RuntimeError(f"oracle columns status {st}")
-    return tr
+    return tr.reshape(2, 30, len(dists), 2 * nw) if all30 else tr
 
 
 def kernels(layers, src_depth, rcv_depth, src_kind, grid, w_lo, w_hi, compat=False, nthreads=0):

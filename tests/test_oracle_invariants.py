@@ -156,3 +156,64 @@ def test_columns_reduce_to_zr_traces(orc, compat):
     for got, want in pairs:
         scale = np.abs(want).max(axis=-1, keepdims=True)
         assert np.abs(got - want).max() < 1e-11 * scale.max()
+
+
+# Source-receiver reciprocity: the displacement at receiver x from a moment
+# tensor at x' is the strain at x' of the displacement from a unit force at x.
+# Tensor column c (source at depth a, receiver at depth b) against the
+# single-force columns with the depths swapped (physics mode: columns 0-4
+# displacements R_h, R_v, T_h, Z_h, Z_v; 5-9 their d/dz; 10-14 their d/dr;
+# h: horizontal force, v: vertical force), r the distance.  The n = 0 terms
+# are the vertical dipole (column 8: d/dz of Z_v) and the horizontal
+# dilatation (column 9: d/dr R_v + R_v / r); column 6 carries the horizontal
+# shear of the vertical force, the n = 1 and n = 2 terms those of the
+# horizontal force.
+RECIPROCITY = {
+    0: [(-1, 10, 0), (1, 0, -1), (-1, 2, -1)],
+    1: [(-1, 5, 0), (1, 13, 0)],
+    2: [(1, 10, 0), (1, 0, -1), (-1, 2, -1)],
+    3: [(1, 8, 0)],
+    4: [(-1, 12, 0), (-1, 0, -1), (1, 2, -1)],
+    5: [(-1, 7, 0), (1, 3, -1)],
+    6: [(-1, 11, 0), (1, 1, -1)],
+    7: [(-1, 6, 0), (1, 14, 0)],
+    8: [(1, 9, 0)],
+    9: [(1, 11, 0), (1, 1, -1)],
+}  # column: [(sign, single-force column, power of r)]
+
+
+def _reciprocity_residuals(orc, a, b):
+    grid = models.dense_grid(32, 0.2, 512, float(models.C1_LAYERS[:, 2].min()))
+    dists = np.array([12.0, 30.0])
+    T = orc.columns(models.C1_LAYERS, a, b, dists, grid, all30=True)[0]
+    F = orc.columns(models.C1_LAYERS, b, a, dists, grid, all30=True)[1]
+    r = dists[:, None]
+    out = {}
+    for c, terms in RECIPROCITY.items():
+        rhs = sum(s * F[fc] * r ** p for s, fc, p in terms)
+        out[c] = np.linalg.norm(T[c] - rhs) / np.linalg.norm(T[c])
+    return out
+
+
+@pytest.mark.parametrize("a,b", [(10.0, 3.0), (5.0, 25.0), (10.0, 14.0), (10.0, 4.0), (3.0, 10.0)])
+def test_tensor_columns_reciprocity(orc, a, b):
+    """All ten tensor columns against the swapped single-force strains, every
+    receiver branch of the tensor solve (physics mode)."""
+    res = _reciprocity_residuals(orc, a, b)
+    assert max(res.values()) < 1e-10, res
+
+
+def test_reference_tensor_sh_factors_break_reciprocity(orc):
+    """The reference's SH factors of the tensor source (1 + R above, R - 1
+    below, for both orders; green.cu:1408-1411; compat mode keeps them),
+    planted in the oracle's physics mode, break the n = 1 relations above the
+    source and the n = 2 relations below it, and nothing else."""
+    try:
+        orc.set_tensor_sh_reference(1)
+        above = _reciprocity_residuals(orc, 10.0, 3.0)
+        below = _reciprocity_residuals(orc, 5.0, 25.0)
+    finally:
+        orc.set_tensor_sh_reference(0)
+    assert above[1] > 1e-3 and above[5] > 1e-3 and below[0] > 1e-3 and below[4] > 1e-3
+    assert max(v for c, v in above.items() if c not in (1, 5)) < 1e-10
+    assert max(v for c, v in below.items() if c not in (0, 4)) < 1e-10
