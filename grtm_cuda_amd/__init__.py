@@ -90,7 +90,11 @@ class TracePlan:
                 self.dists.ctypes.data_as(P))
         self.ptr = ctypes.c_void_p()
         check(getattr(_lib(), self._CREATE)(self._ctx.ptr, ctypes.byref(j), ctypes.byref(g), int(precision),
-                                            int(bool(compat)), ctypes.byref(self.ptr)), self._CREATE)
+                                            int(bool(compat)), *self._extra_args(), ctypes.byref(self.ptr)),
+              self._CREATE)
+
+    def _extra_args(self):
+        return ()
 
     def execute(self):
         check(_lib().grtm_plan_execute(self.ptr, None), "grtm_plan_execute")
@@ -179,23 +183,37 @@ def selftest_cell(compat, src_type, k, w, lay, s_layer, zs, r_layer, zr, par0_rv
 
 # Displacement columns of a ColumnPlan (the reference's integrand columns,
 # cusrc/popm/func/green.cu:1582-1586 single force, 1651-1662 tensor).
-COLUMNS = ("tensor R n2", "tensor R n1", "tensor R n0 s2", "tensor R n0 s0", "tensor T n2", "tensor T n1",
-           "tensor Z n2", "tensor Z n1", "tensor Z n0 s0", "tensor Z n0 s2",
-           "hforce R", "vforce R", "hforce T", "hforce Z", "vforce Z")
+_T = ("tensor R n2", "tensor R n1", "tensor R n0 s2", "tensor R n0 s0", "tensor T n2", "tensor T n1",
+      "tensor Z n2", "tensor Z n1", "tensor Z n0 s0", "tensor Z n0 s2")
+_F = ("hforce R", "vforce R", "hforce T", "hforce Z", "vforce Z")
+COLUMNS = _T + _F
 N_COLUMNS = len(COLUMNS)
+# With derivatives: the reference's physics-mode layout of both sources,
+# tensor 0-29 then single force 0-14 (cusrc/popm/func/green.cu:1589-1600, 1665-1691).
+STRAIN_COLUMNS = (_T + tuple("d/dz " + c for c in _T) + tuple("d/dr " + c for c in _T) + _F +
+                  tuple("d/dz " + c for c in _F) + tuple("d/dr " + c for c in _F))
 
 
 class ColumnPlan(TracePlan):
     """All displacement columns of both sources (tensor columns 0-9 and
     single-force columns 0-4 of green()'s output) of one layered model at many
-    distances from one recursion, on a dense wavenumber grid.  Same arguments
-    as TracePlan; traces() returns [15 (COLUMNS), nr, N]."""
+    distances from one recursion; derivatives=True (physics mode) adds their
+    d/dz and d/dr (STRAIN_COLUMNS).  Other arguments as TracePlan; traces()
+    returns [len(columns), nr, N]."""
 
     _CREATE = "grtm_plan_columns"
 
+    def __init__(self, *args, derivatives=False, **kw):
+        self.derivatives = bool(derivatives)
+        self.columns = STRAIN_COLUMNS if self.derivatives else COLUMNS
+        super().__init__(*args, **kw)
+
+    def _extra_args(self):
+        return (int(self.derivatives),)
+
     def traces(self):
         self.execute()
-        out = np.zeros((N_COLUMNS, self.nr, self.N))
+        out = np.zeros((len(self.columns), self.nr, self.N))
         check(_lib().grtm_plan_traces_out(self.ptr, out.ctypes.data_as(P)), "grtm_plan_traces_out")
         return out
 

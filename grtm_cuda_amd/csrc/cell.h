@@ -238,6 +238,22 @@ GRTM_HD void lam(const LQ<T> &q, W h, cx<T> &e0, cx<T> &e1)
     e1 = cexp_nz<T>(q.vd, h);
 }
 
+// d/dz of the SH receiver response above the source: the upgoing wave
+// ru1 = e^{-v (d - zr)} (d the depth of the layer's bottom or of the source
+// below zr, e_top = e^{-v d}) minus its reflection rud0 e_top e^{-v zr} at
+// the top, times v, written as v ((ru1 - e_top rd1) + (1 - rud0) e_top rd1)
+// with rd1 = e^{-v zr}.  At a free surface (zr = 0: ru1 and e_top are formed
+// from the same argument, rd1 = 1, rud0 = 1; the traction-free condition)
+// both brackets are exactly zero, where the direct difference of the two
+// products leaves an FP32 rounding residue that is large beside the small
+// true d/dz of T.  Every factor stays bounded.
+template <typename T>
+GRTM_HD cx<T> sh_dz_above(const LQ<T> &q, cx<T> rud0, cx<T> rd1, cx<T> e_top, cx<T> ru1)
+{
+    const cx<T> down = e_top * rd1;
+    return q.v * ((ru1 - down) + (C<T>(1, 0) - rud0) * down);
+}
+
 // Result of the receiver-side solve for one cell.
 template <typename T>
 struct CellOut {
@@ -403,7 +419,8 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
         if (NEED_T) {
             const m2<T> RSt = dmd(-(qr.r * rd0), -(qr.v * rd1), Rud_s, sexu0, sexu1);
             const m2<T> ut = Ed * RSt + md(Eu, qr.r * ru0, qr.v * ru1);
-            cx<T> u0t = -(qr.v * rd1) * rud0_s * sexu1 + qr.v * ru1 * (COMPAT ? qr.v : C<T>(1, 0));
+            cx<T> u0t = COMPAT ? -(qr.v * rd1) * rud0_s * sexu1 + qr.v * ru1 * qr.v
+                               : sh_dz_above(qr, rud0_s, rd1, sexu1, ru1);
             out.Yt = ut * yu_psv;
             out.y0t = u0t * y_sh;
         }
@@ -438,7 +455,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             const cx<T> pr = COMPAT ? par0_r : qr.r, pv = COMPAT ? par0_v : qr.v;
             const m2<T> RSt = dmd(-(qr.r * rd0), -(qr.v * rd1), Rud_r, ex0, ex1);
             const m2<T> ut = Ed * RSt + md(Eu, ru0 * pr, ru1 * pv);
-            const cx<T> u0t = -(qr.v * rd1) * rud0_r * ex1 + ru1 * pv;
+            const cx<T> u0t = COMPAT ? -(qr.v * rd1) * rud0_r * ex1 + ru1 * pv : sh_dz_above(qr, rud0_r, rd1, ex1, ru1);
             out.Yt = ut * chain;
             out.y0t = COMPAT ? u0t * out.y0 : u0t * chain0;
         }
@@ -472,6 +489,17 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             out.Yt = dt * chain;
             out.y0t = COMPAT ? d0 * out.y0 : d0t * chain0;
         }
+    }
+    if (!COMPAT && NEED_T && rl == 0 && zr == W(0)) {
+        // Receiver on the free surface: the traction-free conditions give the
+        // d/dz response exactly, d/dz (u_h, u_z) = (k u_z, -k (1 - 2 b^2/a^2) u_h)
+        // and zero for SH; the general formula forms it as a difference of
+        // up- and downgoing terms that cancels to O(k) at small k (in FP32
+        // that cost the d/dz of T 1e-3).
+        const cx<T> ba = qr.sa * recip(qr.sb);  // beta / alpha
+        const cx<T> kg = k * (C<T>(1, 0) - T(2) * ba * ba);
+        out.Yt = m2<T>{k * out.Y.c, k * out.Y.d, -(kg * out.Y.a), -(kg * out.Y.b)};
+        out.y0t = C<T>(0, 0);
     }
 }
 
@@ -588,50 +616,83 @@ GRTM_HD void zr_from_cell(const CellOut<T> &c, int up, T k, cx<T> w, cx<T> &kz_e
 // With k J_n'(k r) = n J_n(k r) / r - k J_{n+1}(k r), the reference's
 // d_besselj form (green.cu:1530-1541), every column is a sum of terms
 // coef(w, k) J_m(k r) r^p, so the columns are assembled after the k-sum from
-// four contractions against J0..J3 tables, the r^p factors applied per
-// receiver.  Complex rows per Bessel order (t: tensor, f: single force):
+// contractions against J_m tables, the r^p factors applied per receiver.
+// Displacement rows per Bessel order (complex; t: tensor, f: single force):
 //   J0: t.a0[1] k, t.a2[1] k, f.a0[1] k
 //   J1: t.a2[q] k, t.a0[q] k, t.a1[1] k, t.b1 + t.a1[0], f.a0[0] k, f.a1[1] k,
 //       f.b1 + f.a1[0], and in compat t.b2 + t.a1[0]
 //   J2: t.b2 + t.a2[0], t.a1[0] k, t.b5 k, t.a2[1] k, f.a1[0] k, f.b1 k
 //   J3: t.a2[0] k, t.b2 k
 // with q = 0 (compat: 1) and b5 = b1 (compat: b2), the index slips of the
-// reference's columns 2, 3 and 5 (green.cu:1653-1657).  put(set, row, value)
-// receives them in this order.
-constexpr int kMtSets = 4;
-GRTM_HD constexpr int mt_set_rows(int compat, int set)
+// reference's columns 2, 3 and 5 (green.cu:1653-1657).
+// Strain columns (physics mode): the d/dz columns are the same rows formed
+// from the receiver-side d/dz coefficients (a.t, b.t; green.cu:1665-1691
+// before the reference overwrites them), and d/dr of a term s r^-q S_m with
+// S_m = sum_k A J_m(k r) is s (m - q) r^-(q+1) S_m - s r^-q sum_k (k A) J_{m+1}:
+// every displacement row also enters the next order multiplied by k.  Set m
+// then holds [displacement rows][their d/dz][k x displacement rows of set m-1]
+// over J0..J4.
+constexpr int kMtSetsMax = 5;
+GRTM_HD constexpr int mt_disp_rows(int compat, int set)
 {
-    return set == 0 ? 3 : set == 1 ? (compat ? 8 : 7) : set == 2 ? 6 : 2;
+    return set == 0 ? 3 : set == 1 ? (compat ? 8 : 7) : set == 2 ? 6 : set == 3 ? 2 : 0;
+}
+GRTM_HD constexpr int mt_sets(int deriv) { return deriv ? 5 : 4; }
+GRTM_HD constexpr int mt_set_rows(int compat, int deriv, int set)
+{
+    return deriv ? 2 * mt_disp_rows(0, set) + (set > 0 ? mt_disp_rows(0, set - 1) : 0) : mt_disp_rows(compat, set);
 }
 
 template <typename T, bool COMPAT, typename Put>
+GRTM_HD void mt_tensor_rows(const cx<T> *a0, const cx<T> *a1, const cx<T> *a2, cx<T> b1, cx<T> b2, T k, Put &&put)
+{
+    put(0, 0, a0[1] * k);
+    put(0, 1, a2[1] * k);
+    put(1, 0, (COMPAT ? a2[1] : a2[0]) * k);
+    put(1, 1, (COMPAT ? a0[1] : a0[0]) * k);
+    put(1, 2, a1[1] * k);
+    put(1, 3, b1 + a1[0]);
+    if (COMPAT) put(1, 7, b2 + a1[0]);
+    put(2, 0, b2 + a2[0]);
+    put(2, 1, a1[0] * k);
+    put(2, 2, (COMPAT ? b2 : b1) * k);
+    put(2, 3, a2[1] * k);
+    put(3, 0, a2[0] * k);
+    put(3, 1, b2 * k);
+}
+
+template <typename T, typename Put>
+GRTM_HD void mt_force_rows(const cx<T> *a0, const cx<T> *a1, cx<T> b1, T k, Put &&put)
+{
+    put(0, 2, a0[1] * k);
+    put(1, 4, a0[0] * k);
+    put(1, 5, a1[1] * k);
+    put(1, 6, b1 + a1[0]);
+    put(2, 4, a1[0] * k);
+    put(2, 5, b1 * k);
+}
+
+// put(set, row, value) receives a cell's rows; DERIV (physics mode only)
+// adds the strain rows.  The cell must have been solved with NEED_T = DERIV.
+template <typename T, bool COMPAT, bool DERIV, typename Put>
 GRTM_HD void mt_rows_from_cell(const CellOut<T> &c, int up, T k, cx<T> w, Put &&put)
 {
+    static_assert(!(COMPAT && DERIV), "strain columns are physics-mode only");
+    auto disp = [&](int set, int j, cx<T> v) {
+        put(set, j, v);
+        if (DERIV) put(set + 1, 2 * mt_disp_rows(0, set + 1) + j, k * v);
+    };
+    auto dz = [&](int set, int j, cx<T> v) { put(set, mt_disp_rows(0, set) + j, v); };
     {
         Coef<T> t;
-        coef_from_cell<T, false, COMPAT>(c, 1, up, k, w, t);
-        put(0, 0, t.a0[1] * k);
-        put(0, 1, t.a2[1] * k);
-        put(1, 0, (COMPAT ? t.a2[1] : t.a2[0]) * k);
-        put(1, 1, (COMPAT ? t.a0[1] : t.a0[0]) * k);
-        put(1, 2, t.a1[1] * k);
-        put(1, 3, t.b1 + t.a1[0]);
-        if (COMPAT) put(1, 7, t.b2 + t.a1[0]);
-        put(2, 0, t.b2 + t.a2[0]);
-        put(2, 1, t.a1[0] * k);
-        put(2, 2, (COMPAT ? t.b2 : t.b1) * k);
-        put(2, 3, t.a2[1] * k);
-        put(3, 0, t.a2[0] * k);
-        put(3, 1, t.b2 * k);
+        coef_from_cell<T, DERIV, COMPAT>(c, 1, up, k, w, t);
+        mt_tensor_rows<T, COMPAT>(t.a0, t.a1, t.a2, t.b1, t.b2, k, disp);
+        if constexpr (DERIV) mt_tensor_rows<T, COMPAT>(t.a0t, t.a1t, t.a2t, t.b1t, t.b2t, k, dz);
     }
     Coef<T> f;
-    coef_from_cell<T, false, COMPAT>(c, 0, up, k, w, f);
-    put(0, 2, f.a0[1] * k);
-    put(1, 4, f.a0[0] * k);
-    put(1, 5, f.a1[1] * k);
-    put(1, 6, f.b1 + f.a1[0]);
-    put(2, 4, f.a1[0] * k);
-    put(2, 5, f.b1 * k);
+    coef_from_cell<T, DERIV, COMPAT>(c, 0, up, k, w, f);
+    mt_force_rows<T>(f.a0, f.a1, f.b1, k, disp);
+    if constexpr (DERIV) mt_force_rows<T>(f.a0t, f.a1t, f.b1t, k, dz);
 }
 
 }  // namespace grtm

@@ -148,19 +148,20 @@ __global__ __launch_bounds__(256) void k_gemm_f32(int M, int N, int K, int kslic
 // multiples of 4 and 16-byte aligned A, B.
 constexpr int kSBM = 128, kSBN = 128, kSBK = 32, kSPA = kSBK + 4, kSPB = kSBN + 4;
 
-// Up to four contractions C_p = A_p B_p with a shared N, K and strides and
+// Up to five contractions C_p = A_p B_p with a shared N, K and strides and
 // their own row counts, in one launch: grid z = p S + slice (contraction
 // outermost, then slice), grid y = row tiles of the largest M (blocks past a
 // smaller contraction's rows return at once).
+constexpr int kGemmSetMax = 5;
 template <typename T>
 struct GemmSetT {
-    const T *A[4];
-    const T *B[4];
-    T *P[4];             // split-K partials of contraction p: [S][M_p][N]
-    T *C[4];
-    const int *kend[4];  // per row tile K extent (null: K)
-    int M[4];            // rows of contraction p
-    int rpw[4];          // rows per frequency row of nk_w
+    const T *A[kGemmSetMax];
+    const T *B[kGemmSetMax];
+    T *P[kGemmSetMax];             // split-K partials of contraction p: [S][M_p][N]
+    T *C[kGemmSetMax];
+    const int *kend[kGemmSetMax];  // per row tile K extent (null: K)
+    int M[kGemmSetMax];            // rows of contraction p
+    int rpw[kGemmSetMax];          // rows per frequency row of nk_w
 };
 using GemmSet = GemmSetT<float>;
 
@@ -531,7 +532,7 @@ template <typename T>
 static hipError_t launch_gemm_set_t(int n, GemmSetT<T> gs, int N, int K, int kc, long lda, long ldb, T *P,
                                     const int *nk_w, int bm, hipStream_t st)
 {
-    if (n < 1 || n > 4) return hipErrorInvalidValue;
+    if (n < 1 || n > kGemmSetMax) return hipErrorInvalidValue;
     const int S = (K + kc - 1) / kc;
     int mt = 0;
     long total = 0;
@@ -619,7 +620,7 @@ hipError_t launch_gemm_set(int prec, int n, const int *M, const int *rpw, int N,
                            long lda, const void *const *B, long ldb, void *const *C, void *P,
                            const int *const *kend, const int *nk_w, hipStream_t st)
 {
-    if (n < 1 || n > 4) return hipErrorInvalidValue;
+    if (n < 1 || n > kGemmSetMax) return hipErrorInvalidValue;
     if (prec == 0) {
         if (kc < kDBK || kc % 32 || lda % 2 || ldb % 2) return hipErrorInvalidValue;
         GemmSet64 gs{};

@@ -591,9 +591,13 @@ struct grtm_plan {
     // displacement columns, whose operands, tables and outputs are the four
     // contractions against J0..J3 laid end to end in A0, B0 and C0
     int kind = 0;
+    int deriv = 0;   // column plans: with the d/dz and d/dr columns (physics mode)
+    int nsets = 0;   // contractions of a column plan (J0..J3, or J0..J4 with deriv)
+    int ncols = 0;   // columns per receiver (15, or 45 with deriv)
     grtm::MtOps mto{};
-    grtm::MtCols mtc{};
-    int ntr() const { return kind ? grtm::kMtCols * nr : 4 * nr; }
+    grtm::MtCols mtc{};  // host copy; the kernels read mtc_d
+    DBuf mtc_d;
+    int ntr() const { return kind ? ncols * nr : 4 * nr; }
     int mt_M(int q) const { return 2 * mto.rows[q] * nw; }
     ~grtm_plan()
     {
@@ -626,32 +630,36 @@ int run_cells_mt(grtm_plan *p, hipStream_t s)
     const int n_lo = ad ? p->ntiles64 : 0, n_hi = ad ? p->ntiles - p->ntiles64 : 0;
     unsigned *cnt = mixed ? p->flags.as<unsigned>() : nullptr, *list = mixed ? cnt + 1 : nullptr;
     if (!mixed) {
-        HIPCHK(grtm::launch_cells_mt(0, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->nw,
-                                     t_lo, n_lo, p->mto, p->nk, nullptr, nullptr, nkw, rowpk, s));
+        HIPCHK(grtm::launch_cells_mt(0, p->compat, p->deriv, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0,
+                                     p->nw, t_lo, n_lo, p->mto, p->nk, nullptr, nullptr, nkw, rowpk, s));
     } else {
         HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
-        HIPCHK(grtm::launch_cells_mt(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0,
-                                     p->rows64, t_lo, n_lo, p->mto, p->nk, nullptr, nullptr, nkw, rowpk, s));
-        HIPCHK(grtm::launch_cells_mt(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->rows64,
-                                     p->nw - p->rows64, t_hi, n_hi, p->mto, p->nk, cnt, list, nkw, rowpk, s));
+        HIPCHK(grtm::launch_cells_mt(2, p->compat, p->deriv, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i,
+                                     0, p->rows64, t_lo, n_lo, p->mto, p->nk, nullptr, nullptr, nkw, rowpk, s));
+        HIPCHK(grtm::launch_cells_mt(1, p->compat, p->deriv, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i,
+                                     p->rows64, p->nw - p->rows64, t_hi, n_hi, p->mto, p->nk, cnt, list, nkw, rowpk,
+                                     s));
     }
     if (ad) {
         const void *lwf64 = mixed ? p->lw64.p : p->lw.p;
         HIPCHK(grtm::launch_check_tail(0, p->compat, lwf64, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, r64,
-                                       p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s, 1));
+                                       p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s,
+                                       p->deriv ? 2 : 1));
         HIPCHK(grtm::launch_check_tail(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, r64,
-                                       p->nw - r64, p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s, 1));
+                                       p->nw - r64, p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s,
+                                       p->deriv ? 2 : 1));
         HIPCHK(grtm::launch_fallback(!mixed, p->compat, lwf64, mixed ? p->lw.p : nullptr, r64, p->L, p->g, p->nk,
                                      p->nw, p->dk, p->df, p->omega_i, p->chk_tol, rowpk, tailmx, nkw, fbcnt,
-                                     p->fb_list.as<unsigned>(), nullptr, nullptr, p->nk, cnt, list, 0, s, &p->mto));
+                                     p->fb_list.as<unsigned>(), nullptr, nullptr, p->nk, cnt, list, 0, s, &p->mto,
+                                     p->deriv));
     }
     if (mixed)
-        HIPCHK(grtm::launch_cells_mt_fix(p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->mto,
-                                         p->nk, cnt, list, s));
+        HIPCHK(grtm::launch_cells_mt_fix(p->compat, p->deriv, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df,
+                                         p->omega_i, p->mto, p->nk, cnt, list, s));
     if (ad && p->pair) {
-        int rpw[grtm::kMtSets];
-        for (int q = 0; q < grtm::kMtSets; q++) rpw[q] = 2 * p->mto.rows[q];
-        HIPCHK(grtm::launch_kend_sets(p->nw, p->nk, grtm::kMtSets, rpw, 128, nkw, p->kend.as<int>(), s));
+        int rpw[grtm::kMtSetsMax];
+        for (int q = 0; q < p->nsets; q++) rpw[q] = 2 * p->mto.rows[q];
+        HIPCHK(grtm::launch_kend_sets(p->nw, p->nk, p->nsets, rpw, 128, nkw, p->kend.as<int>(), s));
     }
     return GRTM_OK;
 }
@@ -713,10 +721,10 @@ int run_cells(grtm_plan *p, hipStream_t s)
 int run_gemm_mt(grtm_plan *p, hipStream_t s)
 {
     const size_t es = esize(p->prec);
-    int M[grtm::kMtSets], rpw[grtm::kMtSets];
-    const void *A[grtm::kMtSets], *B[grtm::kMtSets];
-    void *C[grtm::kMtSets];
-    for (int q = 0; q < grtm::kMtSets; q++) {
+    int M[grtm::kMtSetsMax], rpw[grtm::kMtSetsMax];
+    const void *A[grtm::kMtSetsMax], *B[grtm::kMtSetsMax];
+    void *C[grtm::kMtSetsMax];
+    for (int q = 0; q < p->nsets; q++) {
         M[q] = p->mt_M(q);
         rpw[q] = 2 * p->mto.rows[q];
         A[q] = p->mto.A[q];
@@ -726,19 +734,19 @@ int run_gemm_mt(grtm_plan *p, hipStream_t s)
     const int *nkw = p->adaptive ? p->nk_w.as<int>() : nullptr;
     if (p->pair) {
         // adaptive: per-contraction tile extents laid end to end in kend
-        const int *kend[grtm::kMtSets];
+        const int *kend[grtm::kMtSetsMax];
         int t0 = 0;
-        for (int q = 0; q < grtm::kMtSets; q++) {
+        for (int q = 0; q < p->nsets; q++) {
             kend[q] = p->adaptive ? p->kend.as<int>() + t0 : nullptr;
             t0 += (M[q] + 127) / 128;
         }
-        HIPCHK(grtm::launch_gemm_set(p->gemm_mode, grtm::kMtSets, M, rpw, p->nr, p->nk, p->kc, A, p->nk, B, p->nr, C,
+        HIPCHK(grtm::launch_gemm_set(p->gemm_mode, p->nsets, M, rpw, p->nr, p->nk, p->kc, A, p->nk, B, p->nr, C,
                                      p->ws.p, kend, nkw, s));
         return GRTM_OK;
     }
     // unaligned shapes and BF16x3: one contraction per set, dense grid only
     // (grtm_plan_set_adaptive refuses these plans)
-    for (int q = 0; q < grtm::kMtSets; q++)
+    for (int q = 0; q < p->nsets; q++)
         HIPCHK(grtm::launch_gemm_mfma(p->gemm_mode, M[q], p->nr, p->nk, A[q], p->nk, B[q], p->nr, C[q], p->ws.p,
                                       nullptr, nullptr, s));
     return GRTM_OK;
@@ -771,8 +779,8 @@ int run_gemm(grtm_plan *p, hipStream_t s)
 int run_ifft(grtm_plan *p, hipStream_t s)
 {
     if (p->kind == 1) {
-        HIPCHK(grtm::launch_post_ifft_cols(p->prec == GRTM_PREC_FP64 ? 0 : 1, p->pp, p->mtc, p->nr, p->nr,
-                                           p->dists.as<double>(), p->tr.p, s));
+        HIPCHK(grtm::launch_post_ifft_cols(p->prec == GRTM_PREC_FP64 ? 0 : 1, p->pp, p->mtc_d.as<grtm::MtCols>(),
+                                           p->ncols, p->nr, p->nr, p->dists.as<double>(), p->tr.p, s));
         return GRTM_OK;
     }
     HIPCHK(grtm::launch_post_ifft(p->prec == GRTM_PREC_FP64 ? 0 : 1, p->pp, 1, 4 * p->nr, nullptr, p->C0.p, p->C1.p,
@@ -780,17 +788,23 @@ int run_ifft(grtm_plan *p, hipStream_t s)
     return GRTM_OK;
 }
 
-// Terms of each displacement column over the contraction outputs (rows of
-// mt_rows_from_cell): tensor columns 0-9, then single-force columns 0-4.
-// k J_n' = n J_n / r - k J_{n+1} turns, e.g., tensor column 0,
+// Terms of each column over the contraction outputs (rows of
+// mt_rows_from_cell).  Displacement columns: tensor 0-9, then single force
+// 0-4; k J_n' = n J_n / r - k J_{n+1} turns, e.g., tensor column 0,
 // b2 2 J2 / r + a2[0] k J2', into (2 / r) (b2 + a2[0]) J2 - a2[0] k J3.
-grtm::MtCols mt_columns(int compat)
+// With deriv (physics mode) each source's d/dz columns follow its
+// displacement columns (the same terms on the d/dz rows), then its d/dr
+// columns: a term s r^-q S_m gives s (m - q) r^-(q+1) S_m - s r^-q S'_{m+1}
+// with S'_{m+1} the k x row in the next order; output order tensor 0-29
+// (as the reference's physics-mode columns), single force 30-44 (its 0-14).
+grtm::MtCols mt_columns(int compat, int deriv)
 {
-    grtm::MtCols mc{};
-    auto term = [&](int c, int set, int row, double scale, int rinv) {
-        const int u = mc.nterm[c]++;
-        mc.term[c][u] = grtm::MtTerm{set, row, rinv, 0, scale};
+    struct Tm {
+        int set, row, rpow;
+        double scale;
     };
+    std::vector<std::vector<Tm>> base(15);
+    auto term = [&](int c, int set, int row, double scale, int rpow) { base[c].push_back({set, row, rpow, scale}); };
     term(0, 2, 0, 2.0, 1), term(0, 3, 0, -1.0, 0);                  // R, n = 2
     term(1, 1, 3, 1.0, 1), term(1, 2, 1, -1.0, 0);                  // R, n = 1
     term(2, 1, 0, -1.0, 0);                                         // R, n = 0 (s2)
@@ -806,7 +820,36 @@ grtm::MtCols mt_columns(int compat)
     term(12, 1, 6, 1.0, 1), term(12, 2, 5, -1.0, 0);                // horizontal force T
     term(13, 1, 5, 1.0, 0);                                         // horizontal force Z
     term(14, 0, 2, 1.0, 0);                                         // vertical force Z
-    for (int q = 0; q < grtm::kMtSets; q++) mc.rows[q] = grtm::mt_set_rows(compat, q);
+    std::vector<std::vector<Tm>> cols;
+    if (!deriv) {
+        cols = base;
+    } else {
+        auto dz = [&](const std::vector<Tm> &c) {
+            std::vector<Tm> o = c;
+            for (Tm &t : o) t.row += grtm::mt_disp_rows(0, t.set);
+            return o;
+        };
+        auto dr = [&](const std::vector<Tm> &c) {
+            std::vector<Tm> o;
+            for (const Tm &t : c) {
+                if (t.set != t.rpow) o.push_back({t.set, t.row, t.rpow + 1, t.scale * (t.set - t.rpow)});
+                o.push_back({t.set + 1, 2 * grtm::mt_disp_rows(0, t.set + 1) + t.row, t.rpow, -t.scale});
+            }
+            return o;
+        };
+        for (int blk = 0; blk < 3; blk++)
+            for (int c = 0; c < 10; c++) cols.push_back(blk == 0 ? base[c] : blk == 1 ? dz(base[c]) : dr(base[c]));
+        for (int blk = 0; blk < 3; blk++)
+            for (int c = 10; c < 15; c++)
+                cols.push_back(blk == 0 ? base[c] : blk == 1 ? dz(base[c]) : dr(base[c]));
+    }
+    grtm::MtCols mc{};
+    for (size_t c = 0; c < cols.size(); c++) {
+        mc.nterm[c] = (int)cols[c].size();
+        for (size_t u = 0; u < cols[c].size(); u++)
+            mc.term[c][u] = grtm::MtTerm{cols[c][u].set, cols[c][u].row, cols[c][u].rpow, 0, cols[c][u].scale};
+    }
+    for (int q = 0; q < grtm::mt_sets(deriv); q++) mc.rows[q] = grtm::mt_set_rows(compat, deriv, q);
     return mc;
 }
 
@@ -831,7 +874,7 @@ int grtm_create(grtm_ctx **ctx, int device)
 void grtm_destroy(grtm_ctx *ctx) { delete ctx; }
 
 static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, int precision, int compat,
-                       int kind, grtm_plan **plan_out)
+                       int kind, int deriv, grtm_plan **plan_out)
 {
     if (!ctx || !job || !grid || !plan_out) return GRTM_E_INPUT;
     *plan_out = nullptr;
@@ -874,6 +917,11 @@ static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid
     p->omega_i = omega_i;
     p->pp = post_params(grid, nw, df, omega_i);
     p->kind = kind;
+    if (kind == 1) {
+        p->deriv = deriv ? 1 : 0;
+        p->nsets = grtm::mt_sets(p->deriv);
+        p->ncols = p->deriv ? grtm::kMtColsMax : 15;
+    }
     HIPCHK(hipStreamCreate(&p->stream));
     p->own_stream = true;
 
@@ -896,34 +944,36 @@ static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid
     if ((status = p->tr.alloc((size_t)p->ntr() * p->N * es))) return status;
     if (kind == 1) {
         // four contractions, operands / tables / outputs end to end
-        p->mtc = mt_columns(compat);
-        size_t rows = 0, off[grtm::kMtSets];
-        int M[grtm::kMtSets];
-        for (int q = 0; q < grtm::kMtSets; q++) {
+        p->mtc = mt_columns(compat, p->deriv);
+        size_t rows = 0, off[grtm::kMtSetsMax];
+        int M[grtm::kMtSetsMax];
+        for (int q = 0; q < p->nsets; q++) {
             p->mto.rows[q] = p->mtc.rows[q];
             M[q] = p->mt_M(q);
             off[q] = rows;
             rows += (size_t)M[q];
         }
         if ((status = p->A0.alloc(rows * p->nk * es))) return status;
-        if ((status = p->B0.alloc((size_t)grtm::kMtSets * p->nk * p->nr * es))) return status;
+        if ((status = p->B0.alloc((size_t)p->nsets * p->nk * p->nr * es))) return status;
         if ((status = p->C0.alloc(rows * p->nr * es))) return status;
         bool aligned = p->gemm_mode != 2 && p->nk % 4 == 0 && p->nr % 4 == 0;
-        for (int q = 0; q < grtm::kMtSets; q++) {
+        for (int q = 0; q < p->nsets; q++) {
             p->mto.A[q] = (unsigned char *)p->A0.p + off[q] * p->nk * es;
             p->mtc.C[q] = (unsigned char *)p->C0.p + off[q] * p->nr * es;
             const void *Bq = (unsigned char *)p->B0.p + (size_t)q * p->nk * p->nr * es;
             aligned = aligned && grtm::gemm_pair_ok(p->nk, p->nr, p->mto.A[q], p->mto.A[q], Bq, Bq);
         }
         size_t wsb = 0;
-        for (int q = 0; q < grtm::kMtSets; q++)
+        for (int q = 0; q < p->nsets; q++)
             wsb = std::max(wsb, grtm::gemm_workspace_bytes(p->gemm_mode, M[q], p->nr, p->nk));
         if (aligned) {
             p->pair = true;
-            p->kc = grtm::gemm_set_slice(grtm::kMtSets, M, p->nr, p->nk);
-            wsb = std::max(wsb, grtm::gemm_set_workspace_bytes(grtm::kMtSets, M, p->nr, p->nk, p->kc, es));
+            p->kc = grtm::gemm_set_slice(p->nsets, M, p->nr, p->nk);
+            wsb = std::max(wsb, grtm::gemm_set_workspace_bytes(p->nsets, M, p->nr, p->nk, p->kc, es));
         }
         if ((status = p->ws.alloc(wsb))) return status;
+        if ((status = p->mtc_d.alloc(sizeof(grtm::MtCols)))) return status;
+        HIPCHK(hipMemcpy(p->mtc_d.p, &p->mtc, sizeof(grtm::MtCols), hipMemcpyHostToDevice));
     } else {
         const size_t arows = (size_t)4 * p->nw;
         if ((status = p->A0.alloc(arows * p->nk * es))) return status;
@@ -950,10 +1000,10 @@ static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid
     if (kind == 1) {
         const size_t tab = (size_t)p->nk * p->nr * es;
         unsigned char *b = (unsigned char *)p->B0.p;
-        for (int n0 = 0; n0 < grtm::kMtSets; n0 += 2)
+        for (int n0 = 0; n0 < p->nsets; n0 += 2)
             HIPCHK(grtm::launch_bessel_tab(precision == GRTM_PREC_FP64 ? 0 : 1, compat ? 0 : 1, p->nk, p->dk,
-                                           p->dists.as<double>(), p->nr, b + n0 * tab, b + (n0 + 1) * tab, p->nr,
-                                           p->stream, n0));
+                                           p->dists.as<double>(), p->nr, b + n0 * tab,
+                                           n0 + 1 < p->nsets ? b + (n0 + 1) * tab : nullptr, p->nr, p->stream, n0));
     } else {
         HIPCHK(grtm::launch_bessel_tab(precision == GRTM_PREC_FP64 ? 0 : 1, compat ? 0 : 1, p->nk, p->dk,
                                        p->dists.as<double>(), p->nr, p->B0.p, p->B1.p, p->nr, p->stream));
@@ -973,13 +1023,15 @@ static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid
 int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, int precision, int compat,
                      grtm_plan **plan_out)
 {
-    return plan_create(ctx, job, grid, precision, compat, 0, plan_out);
+    return plan_create(ctx, job, grid, precision, compat, 0, 0, plan_out);
 }
 
 int grtm_plan_columns(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, int precision, int compat,
-                      grtm_plan **plan_out)
+                      int derivatives, grtm_plan **plan_out)
 {
-    return plan_create(ctx, job, grid, precision, compat, 1, plan_out);
+    if (plan_out) *plan_out = nullptr;
+    if (derivatives && compat) return GRTM_E_UNSUPPORTED;  // the reference's d/dr columns are not J'' (Q7, Q8)
+    return plan_create(ctx, job, grid, precision, compat, 1, derivatives, plan_out);
 }
 
 int grtm_plan_execute(grtm_plan *p, void *stream)
@@ -1103,10 +1155,10 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
     if (!p) return GRTM_E_INPUT;
     // column plans: only the streamed contraction set skips the rows' tails
     if (p->kind == 1 && !p->pair) return GRTM_E_UNSUPPORTED;
-    int Mq[grtm::kMtSets], ntile = (p->nw + 15) / 16;
+    int Mq[grtm::kMtSetsMax], ntile = (p->nw + 15) / 16;
     if (p->kind == 1) {
         ntile = 0;
-        for (int q = 0; q < grtm::kMtSets; q++) {
+        for (int q = 0; q < p->nsets; q++) {
             Mq[q] = p->mt_M(q);
             ntile += (Mq[q] + 127) / 128;
         }
@@ -1114,7 +1166,7 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
     if (!kcut) {
         p->adaptive = false;
         if (p->pair)
-            p->kc = p->kind == 1 ? grtm::gemm_set_slice(grtm::kMtSets, Mq, p->nr, p->nk)
+            p->kc = p->kind == 1 ? grtm::gemm_set_slice(p->nsets, Mq, p->nr, p->nk)
                                  : grtm::gemm_pair_slice(4 * p->nw, p->nr, p->nk, p->nk);
         return GRTM_OK;
     }
@@ -1146,9 +1198,9 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
         // slices from the mean K extent of the frequency rows
         double ksum = 0, rows = 0;
         for (int iw = 0; iw < p->nw; iw++) ksum += std::min(p->nk, (p->nk_pred_h[iw] + 31) / 32 * 32);
-        for (int q = 0; q < grtm::kMtSets; q++) rows += Mq[q];
+        for (int q = 0; q < p->nsets; q++) rows += Mq[q];
         const int kc = grtm::gemm_pair_slice((int)(rows / 2), p->nr, p->nk, ksum / p->nw);
-        const size_t need = grtm::gemm_set_workspace_bytes(grtm::kMtSets, Mq, p->nr, p->nk, kc, esize(p->prec));
+        const size_t need = grtm::gemm_set_workspace_bytes(p->nsets, Mq, p->nr, p->nk, kc, esize(p->prec));
         if (need > p->ws.n && (st = p->ws.alloc(need))) return st;
         p->kc = kc;
     } else if (p->pair) {
@@ -1224,12 +1276,13 @@ int grtm_plan_work(grtm_plan *p, double *w)
         if (grtm_opcount_cell(p->L, p->g.s, p->g.rl, p->g.branch, p->g.up, oc)) return GRTM_E_INPUT;
     } else {
         double rows = 0, terms = 0;
-        for (int q = 0; q < grtm::kMtSets; q++) rows += p->mt_M(q);
-        for (int c = 0; c < grtm::kMtCols; c++) terms += p->mtc.nterm[c];
-        w[2] = 2.0 * rows * p->nk * p->nr;                         // four contractions
-        w[3] = p->nr * (terms * 8.0 * p->nw + grtm::kMtCols * 4.0 * p->N);  // complex64 terms in, fp32 out
-        w[4] = (double)grtm::kMtCols * p->nr;                      // column traces
-        if (grtm_opcount_columns_cell(p->L, p->g.s, p->g.rl, p->g.branch, p->g.up, oc)) return GRTM_E_INPUT;
+        for (int q = 0; q < p->nsets; q++) rows += p->mt_M(q);
+        for (int c = 0; c < p->ncols; c++) terms += p->mtc.nterm[c];
+        w[2] = 2.0 * rows * p->nk * p->nr;                         // the contractions of the set
+        w[3] = p->nr * (terms * 8.0 * p->nw + p->ncols * 4.0 * p->N);  // complex64 terms in, fp32 out
+        w[4] = (double)p->ncols * p->nr;                           // column traces
+        if (grtm_opcount_columns_cell(p->L, p->g.s, p->g.rl, p->g.branch, p->g.up, p->deriv, oc))
+            return GRTM_E_INPUT;
     }
     w[5] = oc[0];  // flops per cell
     w[6] = oc[1];  // transcendental calls per cell

@@ -19,24 +19,27 @@ struct PostParams {
     double omega_i;  // imaginary frequency
 };
 
-// Displacement columns (tensor columns 0-9, single-force columns 0-4):
-// operands of the four contractions against J0..J3 (rows of
-// mt_rows_from_cell), and each column's terms over their outputs.
-constexpr int kMtCols = 15;
+// Displacement columns (tensor columns 0-9, single-force columns 0-4) and,
+// in physics mode, their d/dz and d/dr (tensor 10-29, single force 5-14):
+// operands of the contractions against J0..J4 (rows of mt_rows_from_cell),
+// and each column's terms over their outputs.
+constexpr int kMtColsMax = 45;
+constexpr int kMtTermsMax = 4;
 struct MtOps {
-    void *A[kMtSets];     // A_s: [2 rows[s] nw][lda], FP64 or FP32 as the launch
-    int rows[kMtSets];    // complex rows per frequency
+    void *A[kMtSetsMax];  // A_s: [2 rows[s] nw][lda], FP64 or FP32 as the launch
+    int rows[kMtSetsMax]; // complex rows per frequency
 };
 struct MtTerm {
     int set, row;   // contraction and complex row within a frequency
-    int rinv, pad;  // 1: divided by the receiver distance
+    int rpow, pad;  // divided by the receiver distance to this power
     double scale;
 };
+// Lives in device memory (larger than a kernel's argument block).
 struct MtCols {
-    const void *C[kMtSets];  // contraction outputs [2 rows[s] nw][ldc]
-    int rows[kMtSets];
-    int nterm[kMtCols];
-    MtTerm term[kMtCols][2];
+    const void *C[kMtSetsMax];  // contraction outputs [2 rows[s] nw][ldc]
+    int rows[kMtSetsMax];
+    int nterm[kMtColsMax];
+    MtTerm term[kMtColsMax][kMtTermsMax];
 };
 // prec: 0 = FP64 cells / FP64 operands, 1 = FP32 cells / FP32 operands
 // (non-finite cells appended to flag_list), 2 = FP64 cells / FP32 operands.
@@ -51,7 +54,8 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
                            void *A1, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
                            unsigned *row_peak, hipStream_t st);
 // adaptive grid: residual check beyond the cut, dense fallback, tile extents
-// mt: check the displacement-column rows instead of the Z / R kernels
+// mt: check the column rows instead of the Z / R kernels (1: displacement
+// columns, 2: with the strain rows)
 hipError_t launch_check_tail(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk, double df,
                              double omega_i, int w0, int nwr, int max_chk, const int *nk_w, const int *nchk,
                              int stride, unsigned *tail_max, hipStream_t st, int mt = 0);
@@ -63,7 +67,8 @@ hipError_t launch_fallback(int out_f64, int compat, const void *lw64, const void
                            int nk, int nw, double dk, double df, double omega_i, double tol,
                            const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *fb_count,
                            unsigned *fb_list, void *A0, void *A1, long lda, unsigned *flag_count,
-                           unsigned *flag_list, int row_base, hipStream_t st, const MtOps *mto = nullptr);
+                           unsigned *flag_list, int row_base, hipStream_t st, const MtOps *mto = nullptr,
+                           int deriv = 0);
 hipError_t launch_kend_tiles(int nw, int rows_per_tile, int nk, const int *nk_w, int *kend, hipStream_t st);
 // extents of the bm-row tiles of n contractions with rpw[q] rows per frequency
 // row, laid end to end in kend
@@ -106,7 +111,7 @@ hipError_t launch_gemm_pair_f32(int M, int N, int K, int kc, const float *A0, co
 hipError_t launch_gemm_pair_f64(int M, int N, int K, int kc, const double *A0, const double *A1, long lda,
                                 const double *B0, const double *B1, long ldb, double *C0, double *C1, double *P,
                                 const int *kend, const int *nk_w, hipStream_t st);
-// n <= 4 contractions C_q = A_q B_q (M[q] rows, rpw[q] rows per frequency
+// n <= 5 contractions C_q = A_q B_q (M[q] rows, rpw[q] rows per frequency
 // row of nk_w; shared N, K, strides) in one streamed launch; prec 0 FP64,
 // 1 FP32; the same alignment rules as the pairs; workspace P of
 // gemm_set_workspace_bytes; kend (optional) per contraction and 128-row tile.
@@ -121,16 +126,17 @@ hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr,
                             const void *C1, long ldc, int nr, void *traces, void *spec_post, hipStream_t st,
                             long cms = 0);
 
-// prec and work items as launch_cells_zr
-hipError_t launch_cells_mt(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk, double df,
-                           double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles, const MtOps &o,
-                           long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
-                           hipStream_t st);
-hipError_t launch_cells_mt_fix(int compat, const void *lw64, int L, const Geo &g, int nk, double dk, double df,
-                               double omega_i, const MtOps &o, long lda, const unsigned *flag_count,
+// prec and work items as launch_cells_zr; deriv: the strain rows as well
+// (physics mode only)
+hipError_t launch_cells_mt(int prec, int compat, int deriv, const void *lw, int L, const Geo &g, int nk, double dk,
+                           double df, double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles,
+                           const MtOps &o, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
+                           unsigned *row_peak, hipStream_t st);
+hipError_t launch_cells_mt_fix(int compat, int deriv, const void *lw64, int L, const Geo &g, int nk, double dk,
+                               double df, double omega_i, const MtOps &o, long lda, const unsigned *flag_count,
                                const unsigned *flag_list, hipStream_t st);
-// traces [kMtCols][nr][2 nw]
-hipError_t launch_post_ifft_cols(int prec, const PostParams &pp, const MtCols &mc, long ldc, int nr,
+// traces [ncol][nr][2 nw]; mc in device memory
+hipError_t launch_post_ifft_cols(int prec, const PostParams &pp, const MtCols *mc, int ncol, long ldc, int nr,
                                  const double *dists, void *traces, hipStream_t st);
 
 // One model of a batch on the device: geometry and the offset (in LayerW

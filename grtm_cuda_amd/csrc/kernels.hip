@@ -130,8 +130,9 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__re
 // Residual check of the truncated grid: wavenumbers nk_w[iw] + stride j + stride/2,
 // j < nchk, are solved (values are not stored); their largest |value| goes to
 // tail_max[iw].  Rows [w0, w0 + gridDim.y).
-// MT: the displacement-column rows of the cells instead of the Z / R kernels.
-template <typename T, bool COMPAT, bool MT>
+// MT: the column rows of the cells instead of the Z / R kernels (1:
+// displacement columns, 2: with the strain rows).
+template <typename T, bool COMPAT, int MT>
 __global__ __launch_bounds__(kCellThreads) void k_check_tail(const LayerW<T> *__restrict__ lw_all, int L, Geo g,
                                                              int nk, double dk, double df, double omega_i, int w0,
                                                              const int *__restrict__ nk_w, const int *__restrict__ nchk,
@@ -148,8 +149,8 @@ __global__ __launch_bounds__(kCellThreads) void k_check_tail(const LayerW<T> *__
     unsigned mb = 0;
     if (j < nc && ik < nk) {
         float m = 0.f;
-        if constexpr (MT) {
-            cell_mt_rows<T, COMPAT>(lw, g, iw, ik, dk, df, omega_i, [&](int, int, T vr, T vi) {
+        if constexpr (MT != 0) {
+            cell_mt_rows<T, COMPAT, MT == 2>(lw, g, iw, ik, dk, df, omega_i, [&](int, int, T vr, T vi) {
                 m = fmaxf(m, fmaxf(fabsf(float(vr)), fabsf(float(vi))));
             });
         } else {
@@ -255,8 +256,8 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_fix_batch(const ModelDes
 // row whose index lies in [r_lo, r_hi), in the arithmetic T of that row's
 // recursion.  Grid (k blocks, row slots); slots stride over the list.  FP32
 // cells that are not finite go to the flag list as (row_base + iw) nk + ik.
-// MT: displacement-column rows, stored through o (A0, A1 unused).
-template <typename T, typename TO, bool COMPAT, bool MT>
+// MT: column rows (as k_check_tail), stored through o (A0, A1 unused).
+template <typename T, typename TO, bool COMPAT, int MT>
 __global__ __launch_bounds__(kCellThreads) void k_fallback_cells(
     const LayerW<T> *__restrict__ lw_all, int L, Geo g, int nk, int nw, double dk, double df, double omega_i,
     int r_lo, int r_hi, TO *__restrict__ A0, TO *__restrict__ A1, long lda, const unsigned *__restrict__ fb_count,
@@ -274,9 +275,9 @@ __global__ __launch_bounds__(kCellThreads) void k_fallback_cells(
         __syncthreads();
         for (int ik = start + blockIdx.x * blockDim.x + threadIdx.x; ik < nk; ik += gridDim.x * blockDim.x) {
             bool ok;
-            if constexpr (MT) {
+            if constexpr (MT != 0) {
                 unsigned mag;
-                ok = cell_mt<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, o, lda, mag);
+                ok = cell_mt<T, TO, COMPAT, MT == 2>(lw, g, iw, ik, dk, df, omega_i, o, lda, mag);
             } else {
                 TO v[8];
                 ok = cell_zr<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
@@ -302,13 +303,13 @@ __global__ void k_kend_tiles(int nw, int rows_per_tile, int nk, const int *__res
     kend[t] = min(nk, (m + 31) / 32 * 32);
 }
 
-// K extents of the row tiles of up to four contractions laid end to end in
+// K extents of the row tiles of up to five contractions laid end to end in
 // kend: contraction q has M_q = rpw_q nw rows (rpw_q per frequency row) and
 // ceil(M_q / bm) tiles; each extent is rounded up to a multiple of 32.
 struct KendSets {
     int n, bm;
-    int rpw[4];
-    int tile0[5];  // first tile of each contraction; tile0[n] = all tiles
+    int rpw[kMtSetsMax];
+    int tile0[kMtSetsMax + 1];  // first tile of each contraction; tile0[n] = all tiles
 };
 __global__ void k_kend_sets(int nw, int nk, KendSets ks, const int *__restrict__ nk_w, int *__restrict__ kend)
 {
@@ -535,7 +536,7 @@ __global__ void k_green_finish(int nw, int nblk, const double *__restrict__ part
 }
 
 // ---------------------------------------------------------------- Bessel tables
-// J_n0(k r) into B0 and J_n0+1(k r) into B1, [nk][ldb]; mode 0: the
+// J_n0(k r) into B0 and J_n0+1(k r) into B1 (if given), [nk][ldb]; mode 0: the
 // reference's forms (A&S polynomials, Miller / upward recurrence), mode 1:
 // the math library.
 template <typename T>
@@ -558,7 +559,7 @@ __global__ void k_bessel_tab(int mode, int n0, int nk, double dk, const double *
         j1v = lib_jn(n0 + 1, x);
     }
     B0[(size_t)ik * ldb + ir] = T(j0v);
-    B1[(size_t)ik * ldb + ir] = T(j1v);
+    if (B1) B1[(size_t)ik * ldb + ir] = T(j1v);
 }
 
 // ---------------------------------------------------------------- VALU GEMM
@@ -608,7 +609,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T *__re
 // Recursion of the displacement-column operands: one thread per (omega, k)
 // cell, the rows of mt_rows_from_cell weighted by the k-quadrature;
 // put(set, row, re, im) receives them.
-template <typename T, bool COMPAT, typename Put>
+template <typename T, bool COMPAT, bool DERIV, typename Put>
 __device__ __forceinline__ void cell_mt_rows(const LayerW<T> *lw, const Geo &g, int iw, int ik, double dk, double df,
                                              double omega_i, Put &&put)
 {
@@ -616,20 +617,21 @@ __device__ __forceinline__ void cell_mt_rows(const LayerW<T> *lw, const Geo &g, 
     const T k = T(kd);
     const cx<T> w = C<T>(T(2.0 * M_PI * df * iw), T(-omega_i));
     CellOut<T> c;
-    cell_solve<T, COMPAT, false>(lw, g, kd, C<T>(0, 0), C<T>(0, 0), c);
+    cell_solve<T, COMPAT, DERIV>(lw, g, kd, C<T>(0, 0), C<T>(0, 0), c);
     const T wt = T(ik == 0 ? 0.5 * dk : dk);
-    mt_rows_from_cell<T, COMPAT>(c, g.up, k, w, [&](int set, int j, cx<T> v) { put(set, j, wt * v.x, wt * v.y); });
+    mt_rows_from_cell<T, COMPAT, DERIV>(c, g.up, k, w,
+                                        [&](int set, int j, cx<T> v) { put(set, j, wt * v.x, wt * v.y); });
 }
 
 // Stores a cell's rows to A_s[2 (rows_s iw + j) + {0 re, 1 im}][ik]; returns
 // whether all are finite, and their largest |value| as float bits in mag.
-template <typename T, typename TO, bool COMPAT>
+template <typename T, typename TO, bool COMPAT, bool DERIV>
 __device__ __forceinline__ bool cell_mt(const LayerW<T> *lw, const Geo &g, int iw, int ik, double dk, double df,
                                         double omega_i, const MtOps &o, long lda, unsigned &mag)
 {
     bool ok = true;
     float m = 0.f;
-    cell_mt_rows<T, COMPAT>(lw, g, iw, ik, dk, df, omega_i, [&](int set, int j, T vr, T vi) {
+    cell_mt_rows<T, COMPAT, DERIV>(lw, g, iw, ik, dk, df, omega_i, [&](int set, int j, T vr, T vi) {
         const TO re = TO(vr), im = TO(vi);
         TO *A = reinterpret_cast<TO *>(o.A[set]);
         const size_t row = 2 * ((size_t)o.rows[set] * iw + j);
@@ -646,7 +648,7 @@ __device__ __forceinline__ bool cell_mt(const LayerW<T> *lw, const Geo &g, int i
 // Work items as k_cells_zr: the list `tiles` (row << kTileBits | k tile) or
 // every tile of rows [w0, w0 + gridDim.x / tpr); nk_w (optional) limits the
 // wavenumbers of each row, row_peak (optional) collects the row maxima.
-template <typename T, typename TO, bool COMPAT>
+template <typename T, typename TO, bool COMPAT, bool DERIV>
 __global__ __launch_bounds__(kCellThreads) void k_cells_mt(const LayerW<T> *__restrict__ lw_all, int L, Geo g,
                                                            int nk, double dk, double df, double omega_i, int w0,
                                                            int tpr, const unsigned *__restrict__ tiles, MtOps o,
@@ -673,7 +675,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_mt(const LayerW<T> *__re
     const int ik = kb * kCellThreads + threadIdx.x;
     unsigned mag = 0;
     if (ik < nkr) {
-        const bool ok = cell_mt<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, o, lda, mag);
+        const bool ok = cell_mt<T, TO, COMPAT, DERIV>(lw, g, iw, ik, dk, df, omega_i, o, lda, mag);
         if (!ok && flag_count) {
             const unsigned slot = atomicAdd(flag_count, 1u);
             flag_list[slot] = unsigned(iw) * unsigned(nk) + unsigned(ik);
@@ -686,7 +688,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_mt(const LayerW<T> *__re
 }
 
 // FP64 re-solve of the flagged cells (grid-stride over the list).
-template <typename TO, bool COMPAT>
+template <typename TO, bool COMPAT, bool DERIV>
 __global__ __launch_bounds__(kCellThreads) void k_cells_mt_fix(const LayerW<double> *__restrict__ lw_all, int L,
                                                                Geo g, int nk, double dk, double df, double omega_i,
                                                                MtOps o, long lda,
@@ -698,7 +700,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_mt_fix(const LayerW<doub
         const unsigned cell = flag_list[t];
         const int iw = int(cell / unsigned(nk)), ik = int(cell % unsigned(nk));
         unsigned mag;
-        cell_mt<double, TO, COMPAT>(lw_all + (size_t)iw * L, g, iw, ik, dk, df, omega_i, o, lda, mag);
+        cell_mt<double, TO, COMPAT, DERIV>(lw_all + (size_t)iw * L, g, iw, ik, dk, df, omega_i, o, lda, mag);
     }
 }
 
@@ -798,13 +800,15 @@ __global__ __launch_bounds__(256) void k_post_ifft(PostParams pp, int layout, co
     post_ifft_trace<T>(pp, load, t, traces, spec_post);
 }
 
-// Displacement columns: trace t = column t / nr at receiver t % nr; its raw
-// spectrum is the sum of the column's terms scale (1 / r if rinv) C_set[row]
-// over the contraction outputs (rows 2 (rows_set iw + row) + {0 re, 1 im}).
+// Column traces: trace t = column t / nr at receiver t % nr; its raw
+// spectrum is the sum of the column's terms scale r^-rpow C_set[row] over
+// the contraction outputs (rows 2 (rows_set iw + row) + {0 re, 1 im}).
 template <typename T>
-__global__ __launch_bounds__(256) void k_post_ifft_cols(PostParams pp, MtCols mc, long ldc, int nr,
-                                                        const double *__restrict__ dists, T *__restrict__ traces)
+__global__ __launch_bounds__(256) void k_post_ifft_cols(PostParams pp, const MtCols *__restrict__ mcp, long ldc,
+                                                        int nr, const double *__restrict__ dists,
+                                                        T *__restrict__ traces)
 {
+    const MtCols &mc = *mcp;
     const int t = blockIdx.x;
     const int col = t / nr, ir = t - col * nr;
     const double rinv = 1.0 / dists[ir];
@@ -814,7 +818,8 @@ __global__ __launch_bounds__(256) void k_post_ifft_cols(PostParams pp, MtCols mc
             const MtTerm &tm = mc.term[col][u];
             const T *Cs = reinterpret_cast<const T *>(mc.C[tm.set]);
             const size_t row = 2 * ((size_t)mc.rows[tm.set] * iw + tm.row);
-            const double f = tm.scale * (tm.rinv ? rinv : 1.0);
+            double f = tm.scale;
+            for (int q = 0; q < tm.rpow; q++) f *= rinv;
             re += f * double(Cs[row * ldc + ir]);
             im += f * double(Cs[(row + 1) * ldc + ir]);
         }
@@ -870,20 +875,24 @@ hipError_t launch_check_tail(int prec, int compat, const void *lw, int L, const 
 #define GRTM_CHK(T, CP, MT)                                                                                     \
     hipLaunchKernelGGL((k_check_tail<T, CP, MT>), grid, dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, L, g, nk, \
                        dk, df, omega_i, w0, nk_w, nchk, stride, tail_max)
-    if (mt) {
+    if (mt == 2) {
+        if (compat) return hipErrorInvalidValue;
+        if (prec == 0) GRTM_CHK(double, false, 2);
+        else GRTM_CHK(float, false, 2);
+    } else if (mt == 1) {
         if (prec == 0) {
-            if (compat) GRTM_CHK(double, true, true);
-            else GRTM_CHK(double, false, true);
+            if (compat) GRTM_CHK(double, true, 1);
+            else GRTM_CHK(double, false, 1);
         } else {
-            if (compat) GRTM_CHK(float, true, true);
-            else GRTM_CHK(float, false, true);
+            if (compat) GRTM_CHK(float, true, 1);
+            else GRTM_CHK(float, false, 1);
         }
     } else if (prec == 0) {
-        if (compat) GRTM_CHK(double, true, false);
-        else GRTM_CHK(double, false, false);
+        if (compat) GRTM_CHK(double, true, 0);
+        else GRTM_CHK(double, false, 0);
     } else {
-        if (compat) GRTM_CHK(float, true, false);
-        else GRTM_CHK(float, false, false);
+        if (compat) GRTM_CHK(float, true, 0);
+        else GRTM_CHK(float, false, 0);
     }
 #undef GRTM_CHK
     return hipGetLastError();
@@ -893,8 +902,9 @@ hipError_t launch_fallback(int out_f64, int compat, const void *lw64, const void
                            int nk, int nw, double dk, double df, double omega_i, double tol,
                            const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *fb_count,
                            unsigned *fb_list, void *A0, void *A1, long lda, unsigned *flag_count,
-                           unsigned *flag_list, int row_base, hipStream_t st, const MtOps *mto)
+                           unsigned *flag_list, int row_base, hipStream_t st, const MtOps *mto, int deriv)
 {
+    if (deriv && (compat || !mto)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_fallback_select, dim3((nw + 255) / 256), dim3(256), 0, st, nw, nk, tol, row_peak, tail_max,
                        nk_w, fb_count, fb_list);
     dim3 grid((nk + kCellThreads - 1) / kCellThreads, 16);
@@ -915,10 +925,17 @@ hipError_t launch_fallback(int out_f64, int compat, const void *lw64, const void
         GRTM_FB(float, float, false, MT, lw32, r64, nw, flag_count, flag_list);                      \
     }
     // rows below r64 complete in FP64, the others in FP32 like the rest of their row
-    if (mto) {
-        GRTM_FB_ALL(true)
+    if (mto && deriv) {
+        if (out_f64) {
+            GRTM_FB(double, double, false, 2, lw64, 0, nw, nullptr, nullptr);
+        } else {
+            GRTM_FB(double, float, false, 2, lw64, 0, r64, nullptr, nullptr);
+            GRTM_FB(float, float, false, 2, lw32, r64, nw, flag_count, flag_list);
+        }
+    } else if (mto) {
+        GRTM_FB_ALL(1)
     } else {
-        GRTM_FB_ALL(false)
+        GRTM_FB_ALL(0)
     }
 #undef GRTM_FB_ALL
 #undef GRTM_FB
@@ -927,7 +944,7 @@ hipError_t launch_fallback(int out_f64, int compat, const void *lw64, const void
 
 hipError_t launch_kend_sets(int nw, int nk, int n, const int *rpw, int bm, const int *nk_w, int *kend, hipStream_t st)
 {
-    if (n < 1 || n > 4) return hipErrorInvalidValue;
+    if (n < 1 || n > kMtSetsMax) return hipErrorInvalidValue;
     KendSets ks{};
     ks.n = n;
     ks.bm = bm;
@@ -1029,46 +1046,52 @@ hipError_t launch_gemm(int prec, int M, int N, int K, const void *A, long lda, c
     return hipGetLastError();
 }
 
-hipError_t launch_cells_mt(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk, double df,
-                           double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles, const MtOps &o,
-                           long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
-                           hipStream_t st)
+hipError_t launch_cells_mt(int prec, int compat, int deriv, const void *lw, int L, const Geo &g, int nk, double dk,
+                           double df, double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles,
+                           const MtOps &o, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
+                           unsigned *row_peak, hipStream_t st)
 {
     const int tpr = cells_tiles_per_row(nk);
     if (!tiles) ntiles = nwr * tpr;
     if (ntiles <= 0) return hipSuccess;
-#define GRTM_MT(T, TO, CP, FC, FL)                                                                                \
-    hipLaunchKernelGGL((k_cells_mt<T, TO, CP>), dim3(ntiles), dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, L, \
-                       g, nk, dk, df, omega_i, w0, tpr, tiles, o, lda, FC, FL, nk_w, row_peak)
+    if (deriv && compat) return hipErrorInvalidValue;
+#define GRTM_MT(T, TO, CP, DV, FC, FL)                                                                         \
+    hipLaunchKernelGGL((k_cells_mt<T, TO, CP, DV>), dim3(ntiles), dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, \
+                       L, g, nk, dk, df, omega_i, w0, tpr, tiles, o, lda, FC, FL, nk_w, row_peak)
     if (prec == 0) {
-        if (compat) GRTM_MT(double, double, true, nullptr, nullptr);
-        else GRTM_MT(double, double, false, nullptr, nullptr);
+        if (deriv) GRTM_MT(double, double, false, true, nullptr, nullptr);
+        else if (compat) GRTM_MT(double, double, true, false, nullptr, nullptr);
+        else GRTM_MT(double, double, false, false, nullptr, nullptr);
     } else if (prec == 1) {
-        if (compat) GRTM_MT(float, float, true, flag_count, flag_list);
-        else GRTM_MT(float, float, false, flag_count, flag_list);
+        if (deriv) GRTM_MT(float, float, false, true, flag_count, flag_list);
+        else if (compat) GRTM_MT(float, float, true, false, flag_count, flag_list);
+        else GRTM_MT(float, float, false, false, flag_count, flag_list);
     } else {
-        if (compat) GRTM_MT(double, float, true, nullptr, nullptr);
-        else GRTM_MT(double, float, false, nullptr, nullptr);
+        if (deriv) GRTM_MT(double, float, false, true, nullptr, nullptr);
+        else if (compat) GRTM_MT(double, float, true, false, nullptr, nullptr);
+        else GRTM_MT(double, float, false, false, nullptr, nullptr);
     }
 #undef GRTM_MT
     return hipGetLastError();
 }
 
-hipError_t launch_cells_mt_fix(int compat, const void *lw64, int L, const Geo &g, int nk, double dk, double df,
-                               double omega_i, const MtOps &o, long lda, const unsigned *flag_count,
+hipError_t launch_cells_mt_fix(int compat, int deriv, const void *lw64, int L, const Geo &g, int nk, double dk,
+                               double df, double omega_i, const MtOps &o, long lda, const unsigned *flag_count,
                                const unsigned *flag_list, hipStream_t st)
 {
     dim3 grid(1024);
-    if (compat)
-        hipLaunchKernelGGL((k_cells_mt_fix<float, true>), grid, dim3(kCellThreads), 0, st,
-                           (const LayerW<double> *)lw64, L, g, nk, dk, df, omega_i, o, lda, flag_count, flag_list);
-    else
-        hipLaunchKernelGGL((k_cells_mt_fix<float, false>), grid, dim3(kCellThreads), 0, st,
-                           (const LayerW<double> *)lw64, L, g, nk, dk, df, omega_i, o, lda, flag_count, flag_list);
+    if (deriv && compat) return hipErrorInvalidValue;
+#define GRTM_FIX(CP, DV)                                                                                 \
+    hipLaunchKernelGGL((k_cells_mt_fix<float, CP, DV>), grid, dim3(kCellThreads), 0, st,                 \
+                       (const LayerW<double> *)lw64, L, g, nk, dk, df, omega_i, o, lda, flag_count, flag_list)
+    if (deriv) GRTM_FIX(false, true);
+    else if (compat) GRTM_FIX(true, false);
+    else GRTM_FIX(false, false);
+#undef GRTM_FIX
     return hipGetLastError();
 }
 
-hipError_t launch_post_ifft_cols(int prec, const PostParams &pp, const MtCols &mc, long ldc, int nr,
+hipError_t launch_post_ifft_cols(int prec, const PostParams &pp, const MtCols *mc, int ncol, long ldc, int nr,
                                  const double *dists, void *traces, hipStream_t st)
 {
     const size_t lds = post_ifft_lds_bytes(prec, pp.nw);
@@ -1079,7 +1102,8 @@ hipError_t launch_post_ifft_cols(int prec, const PostParams &pp, const MtCols &m
                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    const dim3 grid(kMtCols * nr);
+    if (ncol < 1 || ncol > kMtColsMax) return hipErrorInvalidValue;
+    const dim3 grid(ncol * nr);
     if (prec == 0)
         hipLaunchKernelGGL(k_post_ifft_cols<double>, grid, dim3(256), lds, st, pp, mc, ldc, nr, dists,
                            (double *)traces);

@@ -96,13 +96,17 @@ static int count_cell(int kind, int L, int s, int rl, int branch, int up, double
     g.zr = (rl == s) ? (branch == 2 ? 0.25 * (2.0 + s) : 0.75 * (2.0 + s)) : 0.3;
     grtm_count::g_t = grtm_count::Tally();
     grtm::CellOut<Cnt> c;
-    grtm::cell_solve<Cnt, false, false>(lw, g, Cnt(0.7), cx<Cnt>{Cnt(0), Cnt(0)}, cx<Cnt>{Cnt(0), Cnt(0)}, c);
+    const cx<Cnt> z0{Cnt(0), Cnt(0)};
+    if (kind == 2) grtm::cell_solve<Cnt, false, true>(lw, g, Cnt(0.7), z0, z0, c);
+    else grtm::cell_solve<Cnt, false, false>(lw, g, Cnt(0.7), z0, z0, c);
     const cx<Cnt> w{Cnt(w_re), Cnt(w_im)};
     if (kind == 0) {
         cx<Cnt> a, b, d, e;
         grtm::zr_from_cell<Cnt, false>(c, up, Cnt(0.7), w, a, b, d, e);
+    } else if (kind == 1) {
+        grtm::mt_rows_from_cell<Cnt, false, false>(c, up, Cnt(0.7), w, [](int, int, cx<Cnt>) {});
     } else {
-        grtm::mt_rows_from_cell<Cnt, false>(c, up, Cnt(0.7), w, [](int, int, cx<Cnt>) {});
+        grtm::mt_rows_from_cell<Cnt, false, true>(c, up, Cnt(0.7), w, [](int, int, cx<Cnt>) {});
     }
     out[0] = grtm_count::g_t.flops;
     out[1] = grtm_count::g_t.trans;
@@ -114,7 +118,7 @@ extern "C" int grtm_opcount_cell(int L, int s, int rl, int branch, int up, doubl
     return count_cell(0, L, s, rl, branch, up, out);
 }
 
-extern "C" int grtm_opcount_columns_cell(int L, int s, int rl, int branch, int up, double *out)
+extern "C" int grtm_opcount_columns_cell(int L, int s, int rl, int branch, int up, int derivatives, double *out)
 {
-    return count_cell(1, L, s, rl, branch, up, out);
+    return count_cell(derivatives ? 2 : 1, L, s, rl, branch, up, out);
 }

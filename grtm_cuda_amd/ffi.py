@@ -65,7 +65,7 @@ def _lib():
         lib.grtm_destroy.argtypes = [P]
         lib.grtm_destroy.restype = None
         lib.grtm_plan_traces.argtypes = [P, P, P, I, I, P]
-        lib.grtm_plan_columns.argtypes = [P, P, P, I, I, P]
+        lib.grtm_plan_columns.argtypes = [P, P, P, I, I, I, P]
         lib.grtm_plan_execute.argtypes = [P, P]
         lib.grtm_plan_sync.argtypes = [P]
         lib.grtm_plan_traces_out.argtypes = [P, P]
@@ -76,7 +76,7 @@ def _lib():
         lib.grtm_plan_destroy.restype = None
         lib.grtm_selftest_cell.argtypes = [I, I, D, D, D, I, P, I, D, I, D, P, P]
         lib.grtm_opcount_cell.argtypes = [I, I, I, I, I, P]
-        lib.grtm_opcount_columns_cell.argtypes = [I, I, I, I, I, P]
+        lib.grtm_opcount_columns_cell.argtypes = [I, I, I, I, I, I, P]
         lib.grtm_selftest_gemm.argtypes = [I, I, I, I, P, P, P]
         lib.grtm_selftest_mfma_probe.argtypes = [I, P]
         lib.grtm_selftest_copy.argtypes = [I]

@@ -100,18 +100,24 @@ int grtm_plan_traces(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, 
  * c = 10..14 the single-force columns 0-4 (horizontal force R, vertical
  * force R, horizontal force T, horizontal force Z, vertical force Z).
  * compat = 1 keeps the reference's column contents (index slips in tensor
- * columns 2, 3, 5; reference Bessel forms).  Dense grid only:
- * grtm_plan_set_adaptive and grtm_plan_kernels_out return
- * GRTM_E_UNSUPPORTED on these plans. */
+ * columns 2, 3, 5; its tensor SH factors; reference Bessel forms).
+ * derivatives = 1 (physics mode only; compat returns GRTM_E_UNSUPPORTED)
+ * adds the strain columns of the reference's physics layout
+ * (green.cu:1665-1691, 1589-1600): GRTM_N_STRAIN_COLUMNS traces per receiver,
+ * tensor columns 0-29 (0-9 displacement, 10-19 d/dz, 20-29 d/dr) then
+ * single-force columns 0-14 (0-4 displacement, 5-9 d/dz, 10-14 d/dr).
+ * grtm_plan_set_adaptive works when nk and nr are multiples of 4 (FP64,
+ * FP32); grtm_plan_kernels_out returns GRTM_E_UNSUPPORTED. */
 #define GRTM_N_COLUMNS 15
+#define GRTM_N_STRAIN_COLUMNS 45
 int grtm_plan_columns(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid, int precision, int compat,
-                      grtm_plan **plan);
+                      int derivatives, grtm_plan **plan);
 /* Run recursion + contraction + IFFT on resident data (asynchronous). */
 int grtm_plan_execute(grtm_plan *plan, void *stream);
 int grtm_plan_sync(grtm_plan *plan);
 /* Copy the traces out: traces[((q*2 + c)*nr + ir)*N + j], q: 0 explosion,
  * 1 vertical force; c: 0 Z, 1 R (radial).  Column plans: the layout above
- * (GRTM_N_COLUMNS * nr * N values). */
+ * (GRTM_N_COLUMNS or GRTM_N_STRAIN_COLUMNS, times nr * N values). */
 int grtm_plan_traces_out(grtm_plan *plan, double *traces);
 /* Raw k-kernels (tests): kern[((iw*nk + ik)*4 + {zr,zi,rr,ri})] for source q,
  * unweighted, for frequency rows [w_lo, w_hi). */
@@ -235,9 +241,10 @@ int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi,
  * (1: receiver layer above the source layer, 2/3: same layer above/below,
  * 4: below). */
 int grtm_opcount_cell(int n_layers, int s_layer, int r_layer, int branch, int up, double *out);
-/* The same for a cell of the displacement-column plans (recursion + the
- * coefficient rows of both sources). */
-int grtm_opcount_columns_cell(int n_layers, int s_layer, int r_layer, int branch, int up, double *out);
+/* The same for a cell of the column plans (recursion + the coefficient rows
+ * of both sources; derivatives: with the strain rows). */
+int grtm_opcount_columns_cell(int n_layers, int s_layer, int r_layer, int branch, int up, int derivatives,
+                              double *out);
 
 /* ------------------------------------------- learned wavenumber predictor */
 /* Multilayer perceptron n_in -> depth x width (ReLU) -> n_out, trained and

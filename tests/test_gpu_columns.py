@@ -124,3 +124,52 @@ def test_adaptive_columns_match_dense(prec, tol):
     assert rl.max() < tol and pk.max() < tol
     # a second execution gives the same traces (counters and extents are reset)
     assert np.array_equal(plan.traces(), tr)
+
+
+@pytest.mark.parametrize("geom", GEOMS, ids=[x[0] for x in GEOMS])
+def test_strain_columns_match_oracle(orc, geom):
+    """Physics mode with derivatives: the 45 columns (tensor displacement,
+    d/dz, d/dr; then single force) against the oracle's integrand columns
+    (the reference's formulas: t-coefficients for d/dz, exact J'' for d/dr)."""
+    name, zs, zr = geom
+    grid = models.dense_grid(64, 0.1, 512, float(models.C1_LAYERS[:, 2].min()))
+    dists = [12.0, 35.0]
+    print(f"gpu: strain columns '{name}'", flush=True)
+    ref = orc.columns(models.C1_LAYERS, zs, zr, dists, grid, all30=True)  # [2, 30, nr, N]
+    ref = np.concatenate([ref[0], ref[1, :15]])                            # [45, nr, N]
+    for prec, tol in ((g.PREC_FP64, 1e-8), (g.PREC_FP32, 1e-4)):
+        plan = g.ColumnPlan(models.C1_LAYERS, zs, zr, dists, grid, precision=prec, derivatives=True)
+        tr = plan.traces()
+        assert tr.shape == ref.shape == (len(g.STRAIN_COLUMNS), 2, 128)
+        rl, pk = rel_errors(tr, ref)
+        print(f"  precision {prec}: max rel-L2 {rl.max():.2e} (column {rl.max(axis=1).argmax()}), "
+              f"max peak error {pk.max():.2e}")
+        assert rl.max() < tol and pk.max() < tol, (name, prec)
+
+
+def test_strain_columns_refuse_compat():
+    grid = models.dense_grid(16, 0.2, 128, float(models.C1_LAYERS[:, 2].min()))
+    with pytest.raises(g.GrtmError):
+        g.ColumnPlan(models.C1_LAYERS, 10.0, 0.0, [10.0], grid, compat=True, derivatives=True)
+
+
+def test_adaptive_strain_columns_match_dense():
+    """Learned adaptive grid with the strain rows (their tail check includes
+    the k-weighted rows of the d/dr columns) against the plain FP64 dense
+    strain plan, FP32, 1024 x 8192, 64 receivers."""
+    from grtm_cuda_amd import learned
+    layers, zs = models.layered_model(77, 10), 7.5
+    grid = models.dense_grid(1024, 0.05, 8192, float(layers[:, 2].min()))
+    dists = np.linspace(5.0, 300.0, 64)
+    print("gpu: adaptive strain columns", flush=True)
+    ref = g.ColumnPlan(layers, zs, 0.0, dists, grid, precision=g.PREC_FP64, derivatives=True).traces()
+    plan = g.ColumnPlan(layers, zs, 0.0, dists, grid, precision=g.PREC_FP32, derivatives=True)
+    plan.set_adaptive(learned.Predictor().kcut(layers, zs, 0.0, grid), window=models.check_window(zs, 0.0),
+                      stride=8, tol=3e-6)
+    tr = plan.traces()
+    st = plan.adaptive_stats()
+    rl, pk = rel_errors(tr, ref)
+    print(f"  cells kept {st['cells_final'] / (1024 * 8192):.3f}, rows falling back {st['rows_fallback']}, "
+          f"max rel-L2 {rl.max():.2e} (column {rl.max(axis=1).argmax()}), max peak error {pk.max():.2e}")
+    assert st["cells_final"] < 0.7 * 1024 * 8192
+    assert rl.max() < 1e-4 and pk.max() < 1e-4

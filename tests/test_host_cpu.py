@@ -95,3 +95,19 @@ def test_cell_solver_matches_oracle(orc, compat, src):
         assert np.abs(a - b).max() <= 1e-10 * np.abs(a).max()
         seen.add(1 if rl < s else (2 if rl == s and zr <= zs else (3 if rl == s else 4)))
     assert seen == {1, 2, 3, 4}
+
+
+def test_cell_solver_free_surface_receiver(orc):
+    """Receiver on the free surface in physics mode, where the device template
+    forms the d/dz response from the traction-free conditions: against the
+    oracle's general formula, both sources."""
+    rng = np.random.default_rng(7)
+    p0 = np.zeros(4)
+    for _ in range(20):
+        L, w, lay, s, zs, rl, zr, k = _random_case(rng, orc, 0)
+        if s == 0:
+            continue
+        for src in (0, 1):
+            a = orc.cell(0, src, k, w, lay, s, zs, 0, 0.0, p0)
+            b = g.selftest_cell(0, src, k, w, lay, s, zs, 0, 0.0, p0)
+            assert np.abs(a - b).max() <= 1e-10 * np.abs(a).max()

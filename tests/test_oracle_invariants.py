@@ -217,3 +217,29 @@ def test_reference_tensor_sh_factors_break_reciprocity(orc):
     assert above[1] > 1e-3 and above[5] > 1e-3 and below[0] > 1e-3 and below[4] > 1e-3
     assert max(v for c, v in above.items() if c not in (1, 5)) < 1e-10
     assert max(v for c, v in below.items() if c not in (0, 4)) < 1e-10
+
+
+def test_free_surface_traction(orc):
+    """Receiver on the free surface (physics mode): zero traction fixes the
+    d/dz coefficients of both sources, a.t[0] = k a[1] (horizontal, from
+    tau_rz = 0), a.t[1] = -k (1 - 2 b^2/a^2) a[0] (vertical, from tau_zz = 0)
+    and b.t = 0 (SH, from tau_zphi = 0), for every order."""
+    w = 2 * np.pi * 0.7 - 0.05j
+    rows = [(2.0, 5.0, 2.9, 2.5, 400, 200), (6.0, 6.1, 3.5, 2.75, 600, 300), (20.0, 6.6, 3.8, 2.9, 800, 400)]
+    lay = np.zeros((3, 7))
+    for j, (h, vp, vs, rho, qp, qs) in enumerate(rows):
+        a = orc.dispersion(w, vp, qp)
+        b = orc.dispersion(w, vs, qs)
+        mu = rho * b * b
+        lay[j] = [h, a.real, a.imag, b.real, b.imag, mu.real, mu.imag]
+    ratio = complex(lay[0, 3], lay[0, 4]) / complex(lay[0, 1], lay[0, 2])
+    kg = 1 - 2 * ratio ** 2
+    for src in (0, 1):
+        for k in (0.05, 0.4, 1.5, 4.0):
+            c = orc.cell(0, src, k, w, lay, 1, 3.0, 0, 0.0)
+            for i in range(0, 6 if src else 4, 2):  # a0, a1 (, a2)
+                a, at = c[i:i + 2], c[8 + i:10 + i]
+                scale = abs(a).max() * k
+                assert abs(at[0] - k * a[1]) < 1e-10 * scale
+                assert abs(at[1] + k * kg * a[0]) < 1e-10 * scale
+            assert c[14] == 0 and c[15] == 0
