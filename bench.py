@@ -204,8 +204,9 @@ def bench_columns(args, g, models):
     dense) in mixed precision against the plain FP64 dense column plan
     (per-column rel-L2 and peak error)."""
     cfg = models.config("c2")
+    dv = bool(args.derivatives)
     mk = lambda p: g.ColumnPlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
-                                precision=p)
+                                precision=p, derivatives=dv)
     print("bench: columns plain FP64 dense", file=sys.stderr, flush=True)
     ref_plan = mk(g.PREC_FP64)
     t_ref = time_plan(ref_plan, max(3, args.steps // 2), 1)
@@ -248,13 +249,15 @@ def bench_columns(args, g, models):
                     kernel="k_gemm_f32s" if prec == g.PREC_FP32 else "k_gemm_f64s")
     ntr = work["traces"]
     return {
-        "metric": "displacement-column traces/s at <=1e-4 rel-L2 vs FP64 (plain dense-k FP64 alongside)",
+        "metric": ("displacement + strain" if dv else "displacement") +
+                  "-column traces/s at <=1e-4 rel-L2 vs FP64 (plain dense-k FP64 alongside)",
         "value": ntr / t_step, "unit": "column traces/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * t_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": {g.PREC_FP64: "f64", g.PREC_FP32: "f32", g.PREC_BF16X3: "f32+bf16x3"}[prec],
         "data": "synthetic (seeded layered model)",
         "config": {"workload": f"mt: {len(cfg['layers']) - 1} layers + half-space, tensor and single-force sources "
-                               f"at {cfg['src_depth']} km, 15 columns x {len(cfg['dists'])} receivers, "
+                               f"at {cfg['src_depth']} km, {len(plan.columns)} columns"
+                               f"{' (displacement, d/dz, d/dr)' if dv else ''} x {len(cfg['dists'])} receivers, "
                                f"Nw={cfg['grid']['nw']}, Nk={cfg['grid']['nk']}",
                    "path": f"{args.path} {pname}", "traces_per_step": ntr},
         "plain_fp64_dense": {"value": ntr / t_ref, "ms_per_step": 1e3 * t_ref,
@@ -327,6 +330,7 @@ def main():
                     help="adaptive: learned truncated grid, mixed precision (the headline); dense: plain grid")
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16x3"],
                     help="arithmetic of the measured path (default fp32 adaptive / fp64 dense)")
+    ap.add_argument("--derivatives", action="store_true", help="mt: the d/dz and d/dr columns as well")
     ap.add_argument("--cpu-rows", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
