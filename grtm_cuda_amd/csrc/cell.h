@@ -267,14 +267,18 @@ struct CellOut {
 
 // compat: reproduce the reference's index choices (rexut from the first cell,
 // the extra v in u0t, yu0t/yd0t products, the shifted r>s chain).
+// BR: the receiver branch fixed at compile time (1-4), or 0 to read g.branch;
+// a fixed branch lets the compiler drop the other branches' state (the
+// surface-receiver solve needs 134 VGPRs instead of 155).
 // NEED_T: also form the d/dz responses.  par0_r/par0_v: vertical wavenumbers
 // of layer rl at the first cell (k = 0, w_0), used only in compat mode.
-template <typename T, bool COMPAT, bool NEED_T>
+template <typename T, bool COMPAT, bool NEED_T, int BR = 0>
 GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::type kd, cx<T> par0_r,
                         cx<T> par0_v, CellOut<T> &out)
 {
     const T k = T(kd);
     const int L = g.L, s = g.s, rl = g.rl;
+    const int branch = BR ? BR : g.branch;
     using W = typename WideOf<T>::type;
     const W zs = W(g.zs), zr = W(g.zr);
     out.status = 0;
@@ -302,7 +306,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
                 Rdu_s1 = RgJ;
                 rdu0_s1 = rg0J;
             }
-            if (g.branch == 4) {
+            if (branch == 4) {
                 if (jj == rl + 1) {
                     Rdu_r1 = RgJ;
                     rdu0_r1 = rg0J;
@@ -366,7 +370,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             cx<T> tu0;
             down_step(Rg, rg0, t, e0, e1, Tu, tu0);
             const int jn = lay + 1;  // grt index formed
-            if (g.branch == 1) {
+            if (branch == 1) {
                 if (jn == rl) {
                     Rud_r = Rg;
                     rud0_r = rg0;
@@ -408,7 +412,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
     const W hr_ = lw[rl].h;
     const bool rl_half = (rl == L - 1);
 
-    if (g.branch == 2) {  // green.cu:709-743
+    if (branch == 2) {  // green.cu:709-743
         cx<T> rd0 = cexp_nz<T>(qr.rd, zr), rd1 = cexp_nz<T>(qr.vd, zr);
         cx<T> ru0 = cexp_nz<T>(qr.rd, zs - zr), ru1 = cexp_nz<T>(qr.vd, zs - zr);
         const m2<T> RS = dmd(rd0, rd1, Rud_s, sexu0, sexu1);
@@ -424,7 +428,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             out.Yt = ut * yu_psv;
             out.y0t = u0t * y_sh;
         }
-    } else if (g.branch == 3) {  // green.cu:744-774
+    } else if (branch == 3) {  // green.cu:744-774
         cx<T> rd0 = cexp_nz<T>(qr.rd, zr - zs), rd1 = cexp_nz<T>(qr.vd, zr - zs);
         cx<T> ru0 = cexp_nz<T>(qr.rd, hr_ - zr), ru1 = cexp_nz<T>(qr.vd, hr_ - zr);
         const m2<T> RS = dmd(ru0, ru1, Rdu_s1, sexd0, sexd1);
@@ -439,7 +443,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             out.Yt = dt * yd_psv;
             out.y0t = d0t * y_sh;
         }
-    } else if (g.branch == 1) {  // green.cu:775-857
+    } else if (branch == 1) {  // green.cu:775-857
         cx<T> rd0 = cexp_nz<T>(qr.rd, zr), rd1 = cexp_nz<T>(qr.vd, zr);
         cx<T> ru0 = cexp_nz<T>(qr.rd, hr_ - zr), ru1 = cexp_nz<T>(qr.vd, hr_ - zr);
         cx<T> ex0, ex1;

@@ -27,7 +27,7 @@ constexpr int kTileBits = 12;  // work item = row << kTileBits | k tile
 // One thread per (omega, k) cell; T = arithmetic of the recursion, TO = type
 // of the A operands.  FP32 cells whose result is not finite are appended to
 // a list (flag_count / flag_list) and re-solved in FP64 by k_cells_fix.
-template <typename T, typename TO, bool COMPAT>
+template <typename T, typename TO, bool COMPAT, int BR = 0>
 __device__ __forceinline__ bool cell_zr(const LayerW<T> *lw, const Geo &g, int iwg, int ik, double dk, double df,
                                         double omega_i, TO v[8])
 {
@@ -35,7 +35,7 @@ __device__ __forceinline__ bool cell_zr(const LayerW<T> *lw, const Geo &g, int i
     const T k = T(kd);
     const cx<T> w = C<T>(T(2.0 * M_PI * df * iwg), T(-omega_i));
     CellOut<T> c;
-    cell_solve<T, COMPAT, false>(lw, g, kd, C<T>(0, 0), C<T>(0, 0), c);
+    cell_solve<T, COMPAT, false, BR>(lw, g, kd, C<T>(0, 0), C<T>(0, 0), c);
     cx<T> kze, kre, kzv, krv;
     zr_from_cell<T, COMPAT>(c, g.up, k, w, kze, kre, kzv, krv);
     const T wt = T(ik == 0 ? 0.5 * dk : dk);
@@ -83,7 +83,8 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned v)
 // makes workgroup dispatch the bound).
 // nk_w (optional): per-row number of wavenumbers solved (adaptive grid);
 // row_peak (optional): per-row max |A| as float bits.
-template <typename T, typename TO, bool COMPAT>
+// BR: the plan's receiver branch, fixed at compile time.
+template <typename T, typename TO, bool COMPAT, int BR>
 __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__restrict__ lw_all, int L,
                                                            Geo g, int nk, double dk, double df,
                                                            double omega_i, int w0, int tpr,
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__re
     unsigned mag = 0;
     if (ik < nkr) {
         TO v[8];
-        const bool ok = cell_zr<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
+        const bool ok = cell_zr<T, TO, COMPAT, BR>(lw, g, iw, ik, dk, df, omega_i, v);
         store_zr(v, iw, ik, A0, A1, lda);
         mag = cell_mag_bits(v);
         if (!ok && flag_count) {
@@ -609,7 +610,7 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T *__re
 // Recursion of the displacement-column operands: one thread per (omega, k)
 // cell, the rows of mt_rows_from_cell weighted by the k-quadrature;
 // put(set, row, re, im) receives them.
-template <typename T, bool COMPAT, bool DERIV, typename Put>
+template <typename T, bool COMPAT, bool DERIV, typename Put, int BR = 0>
 __device__ __forceinline__ void cell_mt_rows(const LayerW<T> *lw, const Geo &g, int iw, int ik, double dk, double df,
                                              double omega_i, Put &&put)
 {
@@ -617,7 +618,7 @@ __device__ __forceinline__ void cell_mt_rows(const LayerW<T> *lw, const Geo &g, 
     const T k = T(kd);
     const cx<T> w = C<T>(T(2.0 * M_PI * df * iw), T(-omega_i));
     CellOut<T> c;
-    cell_solve<T, COMPAT, DERIV>(lw, g, kd, C<T>(0, 0), C<T>(0, 0), c);
+    cell_solve<T, COMPAT, DERIV, BR>(lw, g, kd, C<T>(0, 0), C<T>(0, 0), c);
     const T wt = T(ik == 0 ? 0.5 * dk : dk);
     mt_rows_from_cell<T, COMPAT, DERIV>(c, g.up, k, w,
                                         [&](int set, int j, cx<T> v) { put(set, j, wt * v.x, wt * v.y); });
@@ -625,13 +626,13 @@ __device__ __forceinline__ void cell_mt_rows(const LayerW<T> *lw, const Geo &g, 
 
 // Stores a cell's rows to A_s[2 (rows_s iw + j) + {0 re, 1 im}][ik]; returns
 // whether all are finite, and their largest |value| as float bits in mag.
-template <typename T, typename TO, bool COMPAT, bool DERIV>
+template <typename T, typename TO, bool COMPAT, bool DERIV, int BR = 0>
 __device__ __forceinline__ bool cell_mt(const LayerW<T> *lw, const Geo &g, int iw, int ik, double dk, double df,
                                         double omega_i, const MtOps &o, long lda, unsigned &mag)
 {
     bool ok = true;
     float m = 0.f;
-    cell_mt_rows<T, COMPAT, DERIV>(lw, g, iw, ik, dk, df, omega_i, [&](int set, int j, T vr, T vi) {
+    auto put = [&](int set, int j, T vr, T vi) {
         const TO re = TO(vr), im = TO(vi);
         TO *A = reinterpret_cast<TO *>(o.A[set]);
         const size_t row = 2 * ((size_t)o.rows[set] * iw + j);
@@ -640,15 +641,17 @@ __device__ __forceinline__ bool cell_mt(const LayerW<T> *lw, const Geo &g, int i
         const float fr = fabsf(float(re)), fi = fabsf(float(im));
         ok = ok && isfinite(fr) && isfinite(fi);
         m = fmaxf(m, fmaxf(fr, fi));
-    });
+    };
+    cell_mt_rows<T, COMPAT, DERIV, decltype(put) &, BR>(lw, g, iw, ik, dk, df, omega_i, put);
     mag = isfinite(m) ? __float_as_uint(m) : 0u;
     return ok;
 }
 
 // Work items as k_cells_zr: the list `tiles` (row << kTileBits | k tile) or
 // every tile of rows [w0, w0 + gridDim.x / tpr); nk_w (optional) limits the
-// wavenumbers of each row, row_peak (optional) collects the row maxima.
-template <typename T, typename TO, bool COMPAT, bool DERIV>
+// wavenumbers of each row, row_peak (optional) collects the row maxima.  BR:
+// the plan's receiver branch, fixed at compile time.
+template <typename T, typename TO, bool COMPAT, bool DERIV, int BR>
 __global__ __launch_bounds__(kCellThreads) void k_cells_mt(const LayerW<T> *__restrict__ lw_all, int L, Geo g,
                                                            int nk, double dk, double df, double omega_i, int w0,
                                                            int tpr, const unsigned *__restrict__ tiles, MtOps o,
@@ -675,7 +678,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_mt(const LayerW<T> *__re
     const int ik = kb * kCellThreads + threadIdx.x;
     unsigned mag = 0;
     if (ik < nkr) {
-        const bool ok = cell_mt<T, TO, COMPAT, DERIV>(lw, g, iw, ik, dk, df, omega_i, o, lda, mag);
+        const bool ok = cell_mt<T, TO, COMPAT, DERIV, BR>(lw, g, iw, ik, dk, df, omega_i, o, lda, mag);
         if (!ok && flag_count) {
             const unsigned slot = atomicAdd(flag_count, 1u);
             flag_list[slot] = unsigned(iw) * unsigned(nk) + unsigned(ik);
@@ -849,9 +852,19 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
     const int tpr = cells_tiles_per_row(nk);
     if (!tiles) ntiles = nwr * tpr;
     if (ntiles <= 0) return hipSuccess;
-#define GRTM_ZR(T, TO, CP, FC, FL)                                                                           \
-    hipLaunchKernelGGL((k_cells_zr<T, TO, CP>), dim3(ntiles), dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, \
-                       L, g, nk, dk, df, omega_i, w0, tpr, tiles, (TO *)A0, (TO *)A1, lda, FC, FL, nk_w, row_peak)
+#define GRTM_ZR_BR(T, TO, CP, BR, FC, FL)                                                                       \
+    hipLaunchKernelGGL((k_cells_zr<T, TO, CP, BR>), dim3(ntiles), dim3(kCellThreads), 0, st,                     \
+                       (const LayerW<T> *)lw, L, g, nk, dk, df, omega_i, w0, tpr, tiles, (TO *)A0, (TO *)A1, lda, FC, \
+                       FL, nk_w, row_peak)
+#define GRTM_ZR(T, TO, CP, FC, FL)                                                 \
+    do {                                                                           \
+        switch (g.branch) {                                                        \
+        case 1: GRTM_ZR_BR(T, TO, CP, 1, FC, FL); break;                           \
+        case 2: GRTM_ZR_BR(T, TO, CP, 2, FC, FL); break;                           \
+        case 3: GRTM_ZR_BR(T, TO, CP, 3, FC, FL); break;                           \
+        default: GRTM_ZR_BR(T, TO, CP, 4, FC, FL); break;                          \
+        }                                                                          \
+    } while (0)
     if (prec == 0) {
         if (compat) GRTM_ZR(double, double, true, nullptr, nullptr);
         else GRTM_ZR(double, double, false, nullptr, nullptr);
@@ -863,6 +876,7 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
         else GRTM_ZR(double, float, false, nullptr, nullptr);
     }
 #undef GRTM_ZR
+#undef GRTM_ZR_BR
     return hipGetLastError();
 }
 
@@ -1055,9 +1069,19 @@ hipError_t launch_cells_mt(int prec, int compat, int deriv, const void *lw, int 
     if (!tiles) ntiles = nwr * tpr;
     if (ntiles <= 0) return hipSuccess;
     if (deriv && compat) return hipErrorInvalidValue;
-#define GRTM_MT(T, TO, CP, DV, FC, FL)                                                                         \
-    hipLaunchKernelGGL((k_cells_mt<T, TO, CP, DV>), dim3(ntiles), dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, \
-                       L, g, nk, dk, df, omega_i, w0, tpr, tiles, o, lda, FC, FL, nk_w, row_peak)
+#define GRTM_MT_BR(T, TO, CP, DV, BR, FC, FL)                                                              \
+    hipLaunchKernelGGL((k_cells_mt<T, TO, CP, DV, BR>), dim3(ntiles), dim3(kCellThreads), 0, st,            \
+                       (const LayerW<T> *)lw, L, g, nk, dk, df, omega_i, w0, tpr, tiles, o, lda, FC, FL, nk_w, \
+                       row_peak)
+#define GRTM_MT(T, TO, CP, DV, FC, FL)                                                   \
+    do {                                                                                 \
+        switch (g.branch) {                                                              \
+        case 1: GRTM_MT_BR(T, TO, CP, DV, 1, FC, FL); break;                             \
+        case 2: GRTM_MT_BR(T, TO, CP, DV, 2, FC, FL); break;                             \
+        case 3: GRTM_MT_BR(T, TO, CP, DV, 3, FC, FL); break;                             \
+        default: GRTM_MT_BR(T, TO, CP, DV, 4, FC, FL); break;                            \
+        }                                                                                \
+    } while (0)
     if (prec == 0) {
         if (deriv) GRTM_MT(double, double, false, true, nullptr, nullptr);
         else if (compat) GRTM_MT(double, double, true, false, nullptr, nullptr);
@@ -1072,6 +1096,7 @@ hipError_t launch_cells_mt(int prec, int compat, int deriv, const void *lw, int 
         else GRTM_MT(double, float, false, false, nullptr, nullptr);
     }
 #undef GRTM_MT
+#undef GRTM_MT_BR
     return hipGetLastError();
 }
 
