@@ -1482,7 +1482,9 @@ struct grtm_batch {
     double tune_ms = 0;
     struct Group {
         int m0, n;
-        size_t off64, n64, off32, n32;  // item ranges: FP64 rows, FP32 rows
+        // item ranges of the FP64 rows (pass 0) and FP32 rows (pass 1), one per
+        // receiver branch 1-4 (the recursion is compiled per branch)
+        size_t off[2][4], cnt[2][4];
     };
     std::vector<Group> groups;
     hipGraph_t graph = nullptr;
@@ -1531,18 +1533,20 @@ int batch_items(grtm_batch *b, int bs, int g0, int g1)
         for (int j = 0; j < g.n; j++) order[j] = j;
         std::stable_sort(order.begin(), order.end(),
                          [&](int x, int y) { return b->L[g.m0 + x] > b->L[g.m0 + y]; });
-        for (int pass = 0; pass < 2; pass++) {
-            const size_t start = it.size();
-            for (int j : order) {
-                const int r64 = batch_mixed(b) ? b->rows64[g.m0 + j] : b->nw;
-                const int lo = pass == 0 ? 0 : r64, hi = pass == 0 ? r64 : b->nw;
-                for (int iw = lo; iw < hi; iw++)
-                    for (int kb = 0, nt = ntile(g.m0 + j, iw); kb < nt; kb++)
-                        it.push_back(((unsigned long long)j << 32) | grtm::cells_tile(iw, kb));
+        for (int pass = 0; pass < 2; pass++)
+            for (int br = 1; br <= 4; br++) {
+                const size_t start = it.size();
+                for (int j : order) {
+                    if (b->md_h[g.m0 + j].g.branch != br) continue;
+                    const int r64 = batch_mixed(b) ? b->rows64[g.m0 + j] : b->nw;
+                    const int lo = pass == 0 ? 0 : r64, hi = pass == 0 ? r64 : b->nw;
+                    for (int iw = lo; iw < hi; iw++)
+                        for (int kb = 0, nt = ntile(g.m0 + j, iw); kb < nt; kb++)
+                            it.push_back(((unsigned long long)j << 32) | grtm::cells_tile(iw, kb));
+                }
+                g.off[pass][br - 1] = start;
+                g.cnt[pass][br - 1] = it.size() - start;
             }
-            if (pass == 0) g.off64 = start, g.n64 = it.size() - start;
-            else g.off32 = start, g.n32 = it.size() - start;
-        }
     }
     int st;
     if ((st = b->items.alloc(sizeof(unsigned long long) * std::max<size_t>(1, it.size())))) return st;
@@ -1565,18 +1569,24 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
         HIPCHK(grtm::launch_zero_u32(b->fb_count.as<unsigned>() + g.m0, g.n, s));
     }
     if (!batch_mixed(b)) {
-        HIPCHK(grtm::launch_cells_zr_batch(0, b->compat, b->bs, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df,
-                                           b->omega_i, it + g.off64, (int)g.n64, b->A0.p, b->A1.p, b->nk, nullptr,
-                                           nullptr, nkw, rowpk, s));
+        for (int br = 1; br <= 4; br++)
+            if (g.cnt[0][br - 1])
+                HIPCHK(grtm::launch_cells_zr_batch(0, b->compat, b->bs, br, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk,
+                                                   b->df, b->omega_i, it + g.off[0][br - 1], (int)g.cnt[0][br - 1],
+                                                   b->A0.p, b->A1.p, b->nk, nullptr, nullptr, nkw, rowpk, s));
     } else {
         unsigned *cnt = b->flags.as<unsigned>(), *list = cnt + 1;
         HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
-        HIPCHK(grtm::launch_cells_zr_batch(2, b->compat, b->bs, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df,
-                                           b->omega_i, it + g.off64, (int)g.n64, b->A0.p, b->A1.p, b->nk, nullptr,
-                                           nullptr, nkw, rowpk, s));
-        HIPCHK(grtm::launch_cells_zr_batch(1, b->compat, b->bs, md, g.m0, b->lw32.p, b->nw, b->nk, b->dk, b->df,
-                                           b->omega_i, it + g.off32, (int)g.n32, b->A0.p, b->A1.p, b->nk, cnt, list,
-                                           nkw, rowpk, s));
+        for (int br = 1; br <= 4; br++)
+            if (g.cnt[0][br - 1])
+                HIPCHK(grtm::launch_cells_zr_batch(2, b->compat, b->bs, br, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk,
+                                                   b->df, b->omega_i, it + g.off[0][br - 1], (int)g.cnt[0][br - 1],
+                                                   b->A0.p, b->A1.p, b->nk, nullptr, nullptr, nkw, rowpk, s));
+        for (int br = 1; br <= 4; br++)
+            if (g.cnt[1][br - 1])
+                HIPCHK(grtm::launch_cells_zr_batch(1, b->compat, b->bs, br, md, g.m0, b->lw32.p, b->nw, b->nk, b->dk,
+                                                   b->df, b->omega_i, it + g.off[1][br - 1], (int)g.cnt[1][br - 1],
+                                                   b->A0.p, b->A1.p, b->nk, cnt, list, nkw, rowpk, s));
     }
     unsigned *cnt = batch_mixed(b) ? b->flags.as<unsigned>() : nullptr, *list = cnt ? cnt + 1 : nullptr;
     if (!b->adaptive) {

@@ -147,8 +147,9 @@ struct ModelDesc {
     long pad;
 };
 // Recursion of a group of models; items (j << 32) | cells_tile(row, tile)
-// with tiles of bs in {64, 128, 256} wavenumbers; prec as launch_cells_zr.
-hipError_t launch_cells_zr_batch(int prec, int compat, int bs, const ModelDesc *md, int m0, const void *lw_all,
+// with tiles of bs in {64, 128, 256} wavenumbers; prec as launch_cells_zr;
+// every item's model has receiver branch br (1-4).
+hipError_t launch_cells_zr_batch(int prec, int compat, int bs, int br, const ModelDesc *md, int m0, const void *lw_all,
                                  int nw, int nk, double dk, double df, double omega_i,
                                  const unsigned long long *items, int nitems, void *A0, void *A1, long lda,
                                  unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,

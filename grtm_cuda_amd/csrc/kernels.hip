@@ -6,74 +6,25 @@
 //   k_cells_full   per-(omega,k) recursion -> all 30 integrand columns of the
 //                  drop-in output at one distance, reduced over k per block
 //   k_green_finish ordered sum of the per-block partials
-//   k_bessel_tab   J0(k r), J1(k r) tables (B operands of the contraction)
+//   k_check_tail, k_fallback_cells  residual check and dense completion of
+//                  the learned adaptive grid
+//   k_bessel_tab   J_n(k r) tables (B operands of the contraction)
 //   k_gemm         C = A B on VALU (reference-precision path)
 //   k_post_ifft    phase / taper / scale + Hermitian C2R inverse FFT
-//                  (Stockham, in LDS) + exp(omega_I t) scaling
+//                  (Stockham, in LDS) + exp(omega_I t) scaling; k_post_ifft_cols
+//                  the same for column traces assembled from several outputs
+// The batch recursion is in batch.hip, the column recursion in columns.hip,
+// the cell helpers they share in cellops.h.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 
 #include "cell.h"
+#include "cellops.h"
 #include "integrand.h"
 #include "kernels.h"
 
 namespace grtm {
-
-constexpr int kCellThreads = 256;
-constexpr int kTileBits = 12;  // work item = row << kTileBits | k tile
-
-// ---------------------------------------------------------------- recursion
-// One thread per (omega, k) cell; T = arithmetic of the recursion, TO = type
-// of the A operands.  FP32 cells whose result is not finite are appended to
-// a list (flag_count / flag_list) and re-solved in FP64 by k_cells_fix.
-template <typename T, typename TO, bool COMPAT, int BR = 0>
-__device__ __forceinline__ bool cell_zr(const LayerW<T> *lw, const Geo &g, int iwg, int ik, double dk, double df,
-                                        double omega_i, TO v[8])
-{
-    const double kd = dk * ik;
-    const T k = T(kd);
-    const cx<T> w = C<T>(T(2.0 * M_PI * df * iwg), T(-omega_i));
-    CellOut<T> c;
-    cell_solve<T, COMPAT, false, BR>(lw, g, kd, C<T>(0, 0), C<T>(0, 0), c);
-    cx<T> kze, kre, kzv, krv;
-    zr_from_cell<T, COMPAT>(c, g.up, k, w, kze, kre, kzv, krv);
-    const T wt = T(ik == 0 ? 0.5 * dk : dk);
-    const T r[8] = {wt * kze.x, wt * kze.y, wt * kzv.x, wt * kzv.y, wt * kre.x, wt * kre.y, wt * krv.x, wt * krv.y};
-    bool ok = true;
-    for (int i = 0; i < 8; i++) {
-        v[i] = TO(r[i]);
-        ok = ok && isfinite(float(v[i]));
-    }
-    return ok;
-}
-
-template <typename TO>
-__device__ __forceinline__ void store_zr(const TO v[8], int iw, int ik, TO *A0, TO *A1, long lda)
-{
-    const size_t r0 = (size_t)(4 * iw) * lda + ik;
-    for (int i = 0; i < 4; i++) {
-        A0[r0 + i * lda] = v[i];
-        A1[r0 + i * lda] = v[4 + i];
-    }
-}
-
-// Largest |value| of a cell, as the bit pattern of a non-negative float
-// (monotone in the value, so atomicMax on the bits is a float max).
-template <typename TO>
-__device__ __forceinline__ unsigned cell_mag_bits(const TO v[8])
-{
-    float m = 0.f;
-    for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(float(v[i])));
-    return isfinite(m) ? __float_as_uint(m) : 0u;
-}
-
-// Largest value over the 64 lanes of a wave.
-__device__ __forceinline__ unsigned wave_max_u32(unsigned v)
-{
-    for (int o = 32; o > 0; o >>= 1) v = max(v, unsigned(__shfl_xor(int(v), o, 64)));
-    return v;
-}
 
 // Recursion over (frequency row, k tile) work items, one block per item.
 // Items come from `tiles` (row << kTileBits | k tile, ordered by row) when
@@ -179,77 +130,6 @@ __global__ void k_fallback_select(int nw, int nk, double tol, const unsigned *__
         fb_list[slot] = unsigned(iw);
         fb_list[nw + slot] = unsigned(nk_w[iw]);  // first wavenumber still to solve
         nk_w[iw] = nk;
-    }
-}
-
-// ---------------------------------------------------------------- batches
-// Recursion of a group of models in one launch: one block of BS threads per
-// work item (model j of the group, row, k tile of BS wavenumbers), items
-// ordered heaviest model first so the launch ends on light items.  Model
-// m0 + j has its geometry and the offset of its layer table in md; its A
-// operand rows are group rows j nw + iw.  Non-finite FP32 cells go to the
-// flag list as (j nw + iw) nk + ik.  Adaptive grids: nk_w[group row] limits
-// the wavenumbers solved and row_peak[group row] collects the row maxima.
-template <typename T, typename TO, bool COMPAT, int BS>
-__global__ __launch_bounds__(BS, sizeof(T) == 4 ? 4 : 1) void k_cells_zr_batch(const ModelDesc *__restrict__ md, int m0,
-                                                       const LayerW<T> *__restrict__ lw_all, int nw, int nk,
-                                                       double dk, double df, double omega_i,
-                                                       const unsigned long long *__restrict__ items,
-                                                       TO *__restrict__ A0, TO *__restrict__ A1, long lda,
-                                                       unsigned *__restrict__ flag_count,
-                                                       unsigned *__restrict__ flag_list,
-                                                       const int *__restrict__ nk_w,
-                                                       unsigned *__restrict__ row_peak)
-{
-    __shared__ LayerW<T> lw[kMaxLayers];
-    const unsigned long long e = items[blockIdx.x];
-    const int j = int(e >> 32);
-    const unsigned lo = unsigned(e & 0xffffffffull);
-    const int iw = int(lo >> kTileBits), kb = int(lo & ((1u << kTileBits) - 1));
-    const int grow = j * nw + iw;  // row of the group
-    const int nkr = nk_w ? nk_w[grow] : nk;
-    if (kb * BS >= nkr) return;  // uniform over the block
-    const Geo g = md[m0 + j].g;
-    const size_t off = md[m0 + j].lw_off + (size_t)iw * g.L;
-    for (int q = threadIdx.x; q < g.L; q += BS) lw[q] = lw_all[off + q];
-    __syncthreads();
-    const int ik = kb * BS + threadIdx.x;
-    unsigned mag = 0;
-    if (ik < nkr) {
-        TO v[8];
-        const bool ok = cell_zr<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
-        store_zr(v, grow, ik, A0, A1, lda);
-        mag = cell_mag_bits(v);
-        if (!ok && flag_count) {
-            const unsigned slot = atomicAdd(flag_count, 1u);
-            flag_list[slot] = unsigned(grow) * unsigned(nk) + unsigned(ik);
-        }
-    }
-    if (row_peak) {
-        mag = wave_max_u32(mag);
-        if ((threadIdx.x & 63) == 0 && mag) atomicMax(row_peak + grow, mag);
-    }
-}
-
-// FP64 re-solve of the flagged cells of a group.
-template <typename TO, bool COMPAT>
-__global__ __launch_bounds__(kCellThreads) void k_cells_fix_batch(const ModelDesc *__restrict__ md, int m0,
-                                                                  const LayerW<double> *__restrict__ lw_all,
-                                                                  int nw, int nk, double dk, double df,
-                                                                  double omega_i, TO *__restrict__ A0,
-                                                                  TO *__restrict__ A1, long lda,
-                                                                  const unsigned *__restrict__ flag_count,
-                                                                  const unsigned *__restrict__ flag_list)
-{
-    const unsigned n = *flag_count;
-    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-        const unsigned cell = flag_list[t];
-        const int grow = int(cell / unsigned(nk)), ik = int(cell % unsigned(nk));
-        const int j = grow / nw, iw = grow - j * nw;
-        const Geo g = md[m0 + j].g;
-        TO v[8];
-        cell_zr<double, TO, COMPAT>(lw_all + md[m0 + j].lw_off + (size_t)iw * g.L, g, iw, ik, dk, df, omega_i, v);
-        store_zr(v, grow, ik, A0, A1, lda);
     }
 }
 
@@ -606,107 +486,6 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T *__re
         }
 }
 
-// ---------------------------------------------------------------- moment tensor
-// Recursion of the displacement-column operands: one thread per (omega, k)
-// cell, the rows of mt_rows_from_cell weighted by the k-quadrature;
-// put(set, row, re, im) receives them.
-template <typename T, bool COMPAT, bool DERIV, typename Put, int BR = 0>
-__device__ __forceinline__ void cell_mt_rows(const LayerW<T> *lw, const Geo &g, int iw, int ik, double dk, double df,
-                                             double omega_i, Put &&put)
-{
-    const double kd = dk * ik;
-    const T k = T(kd);
-    const cx<T> w = C<T>(T(2.0 * M_PI * df * iw), T(-omega_i));
-    CellOut<T> c;
-    cell_solve<T, COMPAT, DERIV, BR>(lw, g, kd, C<T>(0, 0), C<T>(0, 0), c);
-    const T wt = T(ik == 0 ? 0.5 * dk : dk);
-    mt_rows_from_cell<T, COMPAT, DERIV>(c, g.up, k, w,
-                                        [&](int set, int j, cx<T> v) { put(set, j, wt * v.x, wt * v.y); });
-}
-
-// Stores a cell's rows to A_s[2 (rows_s iw + j) + {0 re, 1 im}][ik]; returns
-// whether all are finite, and their largest |value| as float bits in mag.
-template <typename T, typename TO, bool COMPAT, bool DERIV, int BR = 0>
-__device__ __forceinline__ bool cell_mt(const LayerW<T> *lw, const Geo &g, int iw, int ik, double dk, double df,
-                                        double omega_i, const MtOps &o, long lda, unsigned &mag)
-{
-    bool ok = true;
-    float m = 0.f;
-    auto put = [&](int set, int j, T vr, T vi) {
-        const TO re = TO(vr), im = TO(vi);
-        TO *A = reinterpret_cast<TO *>(o.A[set]);
-        const size_t row = 2 * ((size_t)o.rows[set] * iw + j);
-        A[row * lda + ik] = re;
-        A[(row + 1) * lda + ik] = im;
-        const float fr = fabsf(float(re)), fi = fabsf(float(im));
-        ok = ok && isfinite(fr) && isfinite(fi);
-        m = fmaxf(m, fmaxf(fr, fi));
-    };
-    cell_mt_rows<T, COMPAT, DERIV, decltype(put) &, BR>(lw, g, iw, ik, dk, df, omega_i, put);
-    mag = isfinite(m) ? __float_as_uint(m) : 0u;
-    return ok;
-}
-
-// Work items as k_cells_zr: the list `tiles` (row << kTileBits | k tile) or
-// every tile of rows [w0, w0 + gridDim.x / tpr); nk_w (optional) limits the
-// wavenumbers of each row, row_peak (optional) collects the row maxima.  BR:
-// the plan's receiver branch, fixed at compile time.
-template <typename T, typename TO, bool COMPAT, bool DERIV, int BR>
-__global__ __launch_bounds__(kCellThreads) void k_cells_mt(const LayerW<T> *__restrict__ lw_all, int L, Geo g,
-                                                           int nk, double dk, double df, double omega_i, int w0,
-                                                           int tpr, const unsigned *__restrict__ tiles, MtOps o,
-                                                           long lda, unsigned *__restrict__ flag_count,
-                                                           unsigned *__restrict__ flag_list,
-                                                           const int *__restrict__ nk_w,
-                                                           unsigned *__restrict__ row_peak)
-{
-    __shared__ LayerW<T> lw[kMaxLayers];
-    int iw, kb;
-    if (tiles) {
-        const unsigned e = tiles[blockIdx.x];
-        iw = int(e >> kTileBits);
-        kb = int(e & ((1u << kTileBits) - 1));
-    } else {
-        const int q = int(blockIdx.x) / tpr;
-        iw = w0 + q;
-        kb = int(blockIdx.x) - q * tpr;
-    }
-    const int nkr = nk_w ? nk_w[iw] : nk;
-    if (kb * kCellThreads >= nkr) return;  // uniform over the block
-    for (int j = threadIdx.x; j < L; j += blockDim.x) lw[j] = lw_all[(size_t)iw * L + j];
-    __syncthreads();
-    const int ik = kb * kCellThreads + threadIdx.x;
-    unsigned mag = 0;
-    if (ik < nkr) {
-        const bool ok = cell_mt<T, TO, COMPAT, DERIV, BR>(lw, g, iw, ik, dk, df, omega_i, o, lda, mag);
-        if (!ok && flag_count) {
-            const unsigned slot = atomicAdd(flag_count, 1u);
-            flag_list[slot] = unsigned(iw) * unsigned(nk) + unsigned(ik);
-        }
-    }
-    if (row_peak) {
-        mag = wave_max_u32(mag);
-        if ((threadIdx.x & 63) == 0 && mag) atomicMax(row_peak + iw, mag);
-    }
-}
-
-// FP64 re-solve of the flagged cells (grid-stride over the list).
-template <typename TO, bool COMPAT, bool DERIV>
-__global__ __launch_bounds__(kCellThreads) void k_cells_mt_fix(const LayerW<double> *__restrict__ lw_all, int L,
-                                                               Geo g, int nk, double dk, double df, double omega_i,
-                                                               MtOps o, long lda,
-                                                               const unsigned *__restrict__ flag_count,
-                                                               const unsigned *__restrict__ flag_list)
-{
-    const unsigned n = *flag_count;
-    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-        const unsigned cell = flag_list[t];
-        const int iw = int(cell / unsigned(nk)), ik = int(cell % unsigned(nk));
-        unsigned mag;
-        cell_mt<double, TO, COMPAT, DERIV>(lw_all + (size_t)iw * L, g, iw, ik, dk, df, omega_i, o, lda, mag);
-    }
-}
-
 // ---------------------------------------------------------------- post + IFFT
 template <typename T>
 __device__ __forceinline__ cx<T> post_val(cx<T> v, int iw, const PostParams &pp)
@@ -1060,62 +839,6 @@ hipError_t launch_gemm(int prec, int M, int N, int K, const void *A, long lda, c
     return hipGetLastError();
 }
 
-hipError_t launch_cells_mt(int prec, int compat, int deriv, const void *lw, int L, const Geo &g, int nk, double dk,
-                           double df, double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles,
-                           const MtOps &o, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
-                           unsigned *row_peak, hipStream_t st)
-{
-    const int tpr = cells_tiles_per_row(nk);
-    if (!tiles) ntiles = nwr * tpr;
-    if (ntiles <= 0) return hipSuccess;
-    if (deriv && compat) return hipErrorInvalidValue;
-#define GRTM_MT_BR(T, TO, CP, DV, BR, FC, FL)                                                              \
-    hipLaunchKernelGGL((k_cells_mt<T, TO, CP, DV, BR>), dim3(ntiles), dim3(kCellThreads), 0, st,            \
-                       (const LayerW<T> *)lw, L, g, nk, dk, df, omega_i, w0, tpr, tiles, o, lda, FC, FL, nk_w, \
-                       row_peak)
-#define GRTM_MT(T, TO, CP, DV, FC, FL)                                                   \
-    do {                                                                                 \
-        switch (g.branch) {                                                              \
-        case 1: GRTM_MT_BR(T, TO, CP, DV, 1, FC, FL); break;                             \
-        case 2: GRTM_MT_BR(T, TO, CP, DV, 2, FC, FL); break;                             \
-        case 3: GRTM_MT_BR(T, TO, CP, DV, 3, FC, FL); break;                             \
-        default: GRTM_MT_BR(T, TO, CP, DV, 4, FC, FL); break;                            \
-        }                                                                                \
-    } while (0)
-    if (prec == 0) {
-        if (deriv) GRTM_MT(double, double, false, true, nullptr, nullptr);
-        else if (compat) GRTM_MT(double, double, true, false, nullptr, nullptr);
-        else GRTM_MT(double, double, false, false, nullptr, nullptr);
-    } else if (prec == 1) {
-        if (deriv) GRTM_MT(float, float, false, true, flag_count, flag_list);
-        else if (compat) GRTM_MT(float, float, true, false, flag_count, flag_list);
-        else GRTM_MT(float, float, false, false, flag_count, flag_list);
-    } else {
-        if (deriv) GRTM_MT(double, float, false, true, nullptr, nullptr);
-        else if (compat) GRTM_MT(double, float, true, false, nullptr, nullptr);
-        else GRTM_MT(double, float, false, false, nullptr, nullptr);
-    }
-#undef GRTM_MT
-#undef GRTM_MT_BR
-    return hipGetLastError();
-}
-
-hipError_t launch_cells_mt_fix(int compat, int deriv, const void *lw64, int L, const Geo &g, int nk, double dk,
-                               double df, double omega_i, const MtOps &o, long lda, const unsigned *flag_count,
-                               const unsigned *flag_list, hipStream_t st)
-{
-    dim3 grid(1024);
-    if (deriv && compat) return hipErrorInvalidValue;
-#define GRTM_FIX(CP, DV)                                                                                 \
-    hipLaunchKernelGGL((k_cells_mt_fix<float, CP, DV>), grid, dim3(kCellThreads), 0, st,                 \
-                       (const LayerW<double> *)lw64, L, g, nk, dk, df, omega_i, o, lda, flag_count, flag_list)
-    if (deriv) GRTM_FIX(false, true);
-    else if (compat) GRTM_FIX(true, false);
-    else GRTM_FIX(false, false);
-#undef GRTM_FIX
-    return hipGetLastError();
-}
-
 hipError_t launch_post_ifft_cols(int prec, const PostParams &pp, const MtCols *mc, int ncol, long ldc, int nr,
                                  const double *dists, void *traces, hipStream_t st)
 {
@@ -1160,55 +883,6 @@ hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr,
         hipLaunchKernelGGL(k_post_ifft<float>, dim3(ntr), dim3(256), lds, st, pp, layout, (const float *)raw,
                            (const float *)C0, (const float *)C1, ldc, nr, cms, (float *)traces,
                            (float *)spec_post);
-    return hipGetLastError();
-}
-
-hipError_t launch_cells_zr_batch(int prec, int compat, int bs, const ModelDesc *md, int m0, const void *lw_all,
-                                 int nw, int nk, double dk, double df, double omega_i,
-                                 const unsigned long long *items, int nitems, void *A0, void *A1, long lda,
-                                 unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
-                                 hipStream_t st)
-{
-    if (nitems <= 0) return hipSuccess;
-#define GRTM_ZB(T, TO, CP, BSZ, FC, FL)                                                                          \
-    hipLaunchKernelGGL((k_cells_zr_batch<T, TO, CP, BSZ>), dim3(nitems), dim3(BSZ), 0, st, md, m0,              \
-                       (const LayerW<T> *)lw_all, nw, nk, dk, df, omega_i, items, (TO *)A0, (TO *)A1, lda, FC, FL, \
-                       nk_w, row_peak)
-#define GRTM_ZB_BS(T, TO, CP, FC, FL)                                \
-    do {                                                             \
-        if (bs == 64) GRTM_ZB(T, TO, CP, 64, FC, FL);                \
-        else if (bs == 128) GRTM_ZB(T, TO, CP, 128, FC, FL);         \
-        else GRTM_ZB(T, TO, CP, 256, FC, FL);                        \
-    } while (0)
-    if (bs != 64 && bs != 128 && bs != 256) return hipErrorInvalidValue;
-    if (prec == 0) {
-        if (compat) GRTM_ZB_BS(double, double, true, nullptr, nullptr);
-        else GRTM_ZB_BS(double, double, false, nullptr, nullptr);
-    } else if (prec == 1) {
-        if (compat) GRTM_ZB_BS(float, float, true, flag_count, flag_list);
-        else GRTM_ZB_BS(float, float, false, flag_count, flag_list);
-    } else {
-        if (compat) GRTM_ZB_BS(double, float, true, nullptr, nullptr);
-        else GRTM_ZB_BS(double, float, false, nullptr, nullptr);
-    }
-#undef GRTM_ZB_BS
-#undef GRTM_ZB
-    return hipGetLastError();
-}
-
-hipError_t launch_cells_fix_batch(int compat, const ModelDesc *md, int m0, const void *lw64, int nw, int nk, double dk,
-                                  double df, double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,
-                                  const unsigned *flag_list, hipStream_t st)
-{
-    dim3 grid(1024);
-    if (compat)
-        hipLaunchKernelGGL((k_cells_fix_batch<float, true>), grid, dim3(kCellThreads), 0, st, md, m0,
-                           (const LayerW<double> *)lw64, nw, nk, dk, df, omega_i, (float *)A0, (float *)A1, lda,
-                           flag_count, flag_list);
-    else
-        hipLaunchKernelGGL((k_cells_fix_batch<float, false>), grid, dim3(kCellThreads), 0, st, md, m0,
-                           (const LayerW<double> *)lw64, nw, nk, dk, df, omega_i, (float *)A0, (float *)A1, lda,
-                           flag_count, flag_list);
     return hipGetLastError();
 }
 
