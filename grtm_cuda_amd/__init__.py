@@ -118,16 +118,18 @@ class TracePlan:
         return out[..., 0::2] + 1j * out[..., 1::2]
 
     def timing(self, reps=5):
-        """ms per execution: total, recursion, contraction, IFFT, Bessel tables."""
-        ms = np.zeros(5)
+        """ms per execution: total, recursion stage, contraction, IFFT, Bessel
+        tables, and the main recursion launch alone (``main``)."""
+        ms = np.zeros(6)
         check(_lib().grtm_plan_timing(self.ptr, int(reps), ms.ctypes.data_as(P)), "grtm_plan_timing")
-        return dict(total=ms[0], cells=ms[1], ksum=ms[2], ifft=ms[3], bessel=ms[4])
+        return dict(total=ms[0], cells=ms[1], ksum=ms[2], ifft=ms[3], bessel=ms[4], main=ms[5])
 
     def work(self):
-        w = np.zeros(9)
+        w = np.zeros(10)
         check(_lib().grtm_plan_work(self.ptr, w.ctypes.data_as(P)), "grtm_plan_work")
         return dict(cells=w[0], cell_layers=w[1], ksum_flop=w[2], ifft_bytes=w[3], traces=w[4],
-                    flops_per_cell=w[5], trans_per_cell=w[6], rows_fp64=int(w[7]), cells_refixed=int(w[8]))
+                    flops_per_cell=w[5], trans_per_cell=w[6], rows_fp64=int(w[7]), cells_refixed=int(w[8]),
+                    main_cells=w[9])
 
     def set_rows64(self, rows64):
         """Mixed-precision plans: solve frequency rows [0, rows64) in FP64."""

@@ -598,6 +598,9 @@ struct grtm_plan {
     grtm::MtCols mtc{};  // host copy; the kernels read mtc_d
     DBuf mtc_d;
     int ntr() const { return kind ? ncols * nr : 4 * nr; }
+    // timing: events around the main recursion launch (the FP32 rows of a
+    // mixed plan, all rows of an FP64 plan), recorded when set
+    hipEvent_t ev_main0 = nullptr, ev_main1 = nullptr;
     int mt_M(int q) const { return 2 * mto.rows[q] * nw; }
     ~grtm_plan()
     {
@@ -630,15 +633,19 @@ int run_cells_mt(grtm_plan *p, hipStream_t s)
     const int n_lo = ad ? p->ntiles64 : 0, n_hi = ad ? p->ntiles - p->ntiles64 : 0;
     unsigned *cnt = mixed ? p->flags.as<unsigned>() : nullptr, *list = mixed ? cnt + 1 : nullptr;
     if (!mixed) {
+        if (p->ev_main0) HIPCHK(hipEventRecord(p->ev_main0, s));
         HIPCHK(grtm::launch_cells_mt(0, p->compat, p->deriv, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0,
                                      p->nw, t_lo, n_lo, p->mto, p->nk, nullptr, nullptr, nkw, rowpk, s));
+        if (p->ev_main1) HIPCHK(hipEventRecord(p->ev_main1, s));
     } else {
         HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
         HIPCHK(grtm::launch_cells_mt(2, p->compat, p->deriv, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i,
                                      0, p->rows64, t_lo, n_lo, p->mto, p->nk, nullptr, nullptr, nkw, rowpk, s));
+        if (p->ev_main0) HIPCHK(hipEventRecord(p->ev_main0, s));
         HIPCHK(grtm::launch_cells_mt(1, p->compat, p->deriv, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i,
                                      p->rows64, p->nw - p->rows64, t_hi, n_hi, p->mto, p->nk, cnt, list, nkw, rowpk,
                                      s));
+        if (p->ev_main1) HIPCHK(hipEventRecord(p->ev_main1, s));
     }
     if (ad) {
         const void *lwf64 = mixed ? p->lw64.p : p->lw.p;
@@ -682,18 +689,22 @@ int run_cells(grtm_plan *p, hipStream_t s)
     const unsigned *t_hi = ad ? t_lo + p->ntiles64 : nullptr;
     const int n_lo = ad ? p->ntiles64 : 0, n_hi = ad ? p->ntiles - p->ntiles64 : 0;
     if (p->prec == GRTM_PREC_FP64) {
+        if (p->ev_main0) HIPCHK(hipEventRecord(p->ev_main0, s));
         HIPCHK(grtm::launch_cells_zr(0, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->nw,
                                      t_lo, n_lo, p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk,
                                      s));
+        if (p->ev_main1) HIPCHK(hipEventRecord(p->ev_main1, s));
     } else {
         unsigned *cnt = p->flags.as<unsigned>(), *list = cnt + 1;
         HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
         HIPCHK(grtm::launch_cells_zr(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0,
                                      p->rows64, t_lo, n_lo, p->A0.p, p->A1.p, p->nk, nullptr, nullptr,
                                      nkw, rowpk, s));
+        if (p->ev_main0) HIPCHK(hipEventRecord(p->ev_main0, s));
         HIPCHK(grtm::launch_cells_zr(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->rows64,
                                      p->nw - p->rows64, t_hi, n_hi, p->A0.p, p->A1.p, p->nk, cnt, list,
                                      nkw, rowpk, s));
+        if (p->ev_main1) HIPCHK(hipEventRecord(p->ev_main1, s));
     }
     const bool mixed = p->prec != GRTM_PREC_FP64;
     unsigned *cnt = mixed ? p->flags.as<unsigned>() : nullptr, *list = mixed ? cnt + 1 : nullptr;
@@ -1096,30 +1107,38 @@ int grtm_plan_kernels_out(grtm_plan *p, int q, int w_lo, int w_hi, double *kern)
 int grtm_plan_timing(grtm_plan *p, int reps, double *ms)
 {
     if (!p || !ms || reps < 1) return GRTM_E_INPUT;
-    hipEvent_t ev[4];
-    for (int i = 0; i < 4; i++) HIPCHK(hipEventCreate(&ev[i]));
-    double acc[4] = {0, 0, 0, 0};
+    hipEvent_t ev[6];
+    for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&ev[i]));
+    double acc[5] = {0, 0, 0, 0, 0};
     int st = GRTM_OK;
+    p->ev_main0 = ev[4];
+    p->ev_main1 = ev[5];
     for (int r = 0; r < reps && !st; r++) {
-        HIPCHK(hipEventRecord(ev[0], p->stream));
+        if (hipEventRecord(ev[0], p->stream) != hipSuccess) { st = GRTM_E_DEVICE; break; }
         if ((st = run_cells(p, p->stream))) break;
-        HIPCHK(hipEventRecord(ev[1], p->stream));
+        if (hipEventRecord(ev[1], p->stream) != hipSuccess) { st = GRTM_E_DEVICE; break; }
         if ((st = run_gemm(p, p->stream))) break;
-        HIPCHK(hipEventRecord(ev[2], p->stream));
+        if (hipEventRecord(ev[2], p->stream) != hipSuccess) { st = GRTM_E_DEVICE; break; }
         if ((st = run_ifft(p, p->stream))) break;
-        HIPCHK(hipEventRecord(ev[3], p->stream));
-        HIPCHK(hipEventSynchronize(ev[3]));
-        float a, b, c, d;
-        HIPCHK(hipEventElapsedTime(&a, ev[0], ev[3]));
-        HIPCHK(hipEventElapsedTime(&b, ev[0], ev[1]));
-        HIPCHK(hipEventElapsedTime(&c, ev[1], ev[2]));
-        HIPCHK(hipEventElapsedTime(&d, ev[2], ev[3]));
-        acc[0] += a, acc[1] += b, acc[2] += c, acc[3] += d;
+        if (hipEventRecord(ev[3], p->stream) != hipSuccess || hipEventSynchronize(ev[3]) != hipSuccess) {
+            st = GRTM_E_DEVICE;
+            break;
+        }
+        float a, b, c, d, e;
+        if (hipEventElapsedTime(&a, ev[0], ev[3]) != hipSuccess || hipEventElapsedTime(&b, ev[0], ev[1]) != hipSuccess ||
+            hipEventElapsedTime(&c, ev[1], ev[2]) != hipSuccess || hipEventElapsedTime(&d, ev[2], ev[3]) != hipSuccess ||
+            hipEventElapsedTime(&e, ev[4], ev[5]) != hipSuccess) {
+            st = GRTM_E_DEVICE;
+            break;
+        }
+        acc[0] += a, acc[1] += b, acc[2] += c, acc[3] += d, acc[4] += e;
     }
-    for (int i = 0; i < 4; i++) (void)hipEventDestroy(ev[i]);
+    p->ev_main0 = p->ev_main1 = nullptr;
+    for (int i = 0; i < 6; i++) (void)hipEventDestroy(ev[i]);
     if (st) return st;
     for (int i = 0; i < 4; i++) ms[i] = acc[i] / reps;
     ms[4] = p->bessel_ms;
+    ms[5] = acc[4] / reps;
     return GRTM_OK;
 }
 
@@ -1287,6 +1306,13 @@ int grtm_plan_work(grtm_plan *p, double *w)
     w[5] = oc[0];  // flops per cell
     w[6] = oc[1];  // transcendental calls per cell
     w[7] = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;  // rows solved in FP64
+    {
+        // cells of the main recursion launch: rows [r0, nw) of the predicted or dense grid
+        const int r0 = p->prec == GRTM_PREC_FP64 ? 0 : p->rows64;
+        double mc = 0;
+        for (int iw = r0; iw < p->nw; iw++) mc += p->adaptive ? p->nk_pred_h[iw] : p->nk;
+        w[9] = mc;
+    }
     w[8] = 0;  // cells re-solved in FP64 after a non-finite FP32 result (last execution)
     if (p->prec != GRTM_PREC_FP64) {
         unsigned c = 0;

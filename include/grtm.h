@@ -124,13 +124,16 @@ int grtm_plan_traces_out(grtm_plan *plan, double *traces);
 int grtm_plan_kernels_out(grtm_plan *plan, int q, int w_lo, int w_hi, double *kern);
 /* Timing on the plan's stream with HIP events: reps executions;
  * ms[0] total per execution, ms[1] recursion, ms[2] contraction,
- * ms[3] IFFT, ms[4] Bessel tables (one-off, part of planning). */
+ * ms[3] IFFT, ms[4] Bessel tables (one-off, part of planning), ms[5] the
+ * main recursion launch alone (FP32 rows of a mixed plan, all rows of an
+ * FP64 plan; its cells are work[9]).  6 doubles. */
 int grtm_plan_timing(grtm_plan *plan, int reps, double *ms);
 /* Work of one execution: [0] cells, [1] cell-layers, [2] contraction flops,
  * [3] IFFT algorithmic bytes, [4] traces (column plans: column traces),
  * [5] recursion flops per cell,
  * [6] transcendental calls per cell, [7] frequency rows solved in FP64,
- * [8] FP32 cells re-solved in FP64 in the last execution (9 doubles). */
+ * [8] FP32 cells re-solved in FP64 in the last execution, [9] cells of the
+ * main recursion launch (10 doubles). */
 int grtm_plan_work(grtm_plan *plan, double *work);
 void grtm_plan_destroy(grtm_plan *plan);
 

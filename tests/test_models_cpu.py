@@ -99,7 +99,7 @@ def test_mlp_reference_gradients(monkeypatch):
 def test_bench_reads_committed_counter_summary():
     sys.path.insert(0, ROOT)
     import bench
-    t = bench.pmc_traffic("k_cells_zr<float,float,false>")
+    t = bench.pmc_traffic("k_cells_zr<float,float,false,1>")
     assert t is not None and 1e8 < t < 1e9
     assert bench.pmc_traffic("no_such_kernel") is None
     assert os.path.exists(bench.PMC_SUMMARY)
