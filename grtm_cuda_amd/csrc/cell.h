@@ -9,7 +9,7 @@
 // (196-233), interface R/T (235-262 + the cuBLAS 4x4 inverse/GEMM at
 // 1813-1816 + 339-387), get_grt (424-613), get_src_* (1271-1413) and
 // get_ypars_a_* (615-1267).  The 4x4 interface solve is replaced by a
-// closed-form 2x2 block factorisation (see interface_rt below).
+// closed-form 2x2 block factorisation (see interface_xy below).
 #pragma once
 #include "cx.h"
 
@@ -116,70 +116,90 @@ GRTM_HD LQ<T> layer_q(const LayerW<T> &L, typename WideOf<T>::type kd)
     return q;
 }
 
+// Interface between layer a, the side of the incident wave, and layer b.
+// With E = [[P, P], [N, -N]] (E's rows reordered to (0,3 | 1,2); columns
+// down | up; E^-1 = 1/2 [[P^-1, N^-1], [P^-1, -N^-1]]), continuity of
+// displacement and traction for a wave incident from a, b's side reduced to
+// its generalized reflection Q referenced at the interface, gives with
+// X = P_a^-1 P_b, Y = N_a^-1 N_b and G = X (I + Q) + Y (I - Q)
+//   generalized reflection  R = (X (I + Q) - Y (I - Q)) G^-1,
+//   generalized transmission T = 2 G^-1.
+// These are the quantities green.cu:471-594 forms from the interface
+// coefficients (235-262, 339-387 and the cuBLAS 4x4 solve at 1813-1816):
+// downward incidence (up sweep, a above b) gives R_du and T_d, upward
+// incidence (down sweep, a below b) R_ud and T_u; with Q = 0 they are the
+// interface coefficients themselves.
+// The E columns are taken without their a/w, b/w normalisation: P = Pt D,
+// N = Nt D with D = diag(a/w, b/w), Pt = [[k, v], [-mu x, -2 mu k v]],
+// Nt = [[r, k], [-2 mu k r, -mu x]].  Every generalized R and T of the
+// recursion is then the similarity transform D R D^-1 of the normalised one,
+// so the normalisation returns once, on the source vector (src_scaled).
+// det Pt = -mu v w^2/b^2 and det Nt = mu r w^2/b^2 in closed form; adj(Pt_a)
+// Pt_b and adj(Nt_a) Nt_b are written through dmu = mu_b - mu_a, so identical
+// layers give X = Y = I exactly (R = Q, T = I).
 template <typename T>
-struct RT {
-    m2<T> td, rud, rdu, tu;  // P-SV: T_d, R_ud, R_du, T_u
-    cx<T> td0, rud0, rdu0, tu0;  // SH
-};
-
-// Interface between layer u (above) and d (below).
-// The E matrix (green.cu:196-214) has columns [D | Sigma D] with
-// Sigma = diag(1,-1,-1,1).  Reordering rows to (0,3 | 1,2) gives
-// E = [[P, P], [N, -N]] and E^-1 = 1/2 [[P^-1, N^-1], [P^-1, -N^-1]].
-// With X = Pu^-1 Pd, Y = Nu^-1 Nd and S = X + Y the solution of
-// E1 M = E2 (green.cu:235-262, 1813-1816) is
-//   T_d = 2 S^-1, R_ud = 2 S^-1 Y - I, R_du = I - 2 Y S^-1, T_u = 2 X S^-1 Y.
-// P = Pt diag(a/w, b/w), Pt = [[k, v], [-mu x, -2 mu k v]],
-// N = Nt diag(a/w, b/w), Nt = [[r, k], [-2 mu k r, -mu x]],
-// det Pt = -mu v w^2/b^2, det Nt = mu r w^2/b^2 (closed forms, no cancellation);
-// the products adj(Pt_u) Pt_d etc. are written in terms of dmu = mu_d - mu_u so
-// identical layers give X = Y = I exactly.
-template <typename T>
-GRTM_HD RT<T> interface_rt(const LQ<T> &u, const LQ<T> &d, T k)
+GRTM_HD void interface_xy(const LQ<T> &a, const LQ<T> &b, T k, m2<T> &X, m2<T> &Y)
 {
-    RT<T> o;
-    const cx<T> dmu = d.mu - u.mu;
+    const cx<T> dmu = b.mu - a.mu;
     const T k2 = k * k;
     const cx<T> k2dmu2 = (T(2) * k2) * dmu;
-    const cx<T> mgu = u.mu * u.gb, mgd = d.mu * d.gb;
-    // X' = adj(Pt_u) Pt_d
-    m2<T> X, Y;
-    X.a = u.v * (k2dmu2 - mgd);
-    X.b = (T(2) * k) * (u.v * d.v) * dmu;
-    X.c = k * (mgd - mgu - k2dmu2);
-    X.d = d.v * (-mgu - k2dmu2);
-    // Y' = adj(Nt_u) Nt_d
-    Y.a = d.r * (k2dmu2 + mgu);
-    Y.b = k * (k2dmu2 - mgd + mgu);
-    Y.c = -((T(2) * k) * (u.r * d.r) * dmu);
-    Y.d = u.r * (mgd - k2dmu2);
-    // divide by the determinants of Pt_u, Nt_u and apply the column scalings
-    const cx<T> ipu = recip(-(u.mu * u.v * u.gb));
-    const cx<T> inu = recip(u.mu * u.r * u.gb);
-    X = dmd(ipu * u.sa, ipu * u.sb, X, d.ca, d.cb);
-    Y = dmd(inu * u.sa, inu * u.sb, Y, d.ca, d.cb);
-    const m2<T> Si = minv(X + Y);
-    const m2<T> SiY2 = mscale(T(2), Si * Y);
-    o.td = mscale(T(2), Si);
-    o.rud = SiY2 - ident<T>();
-    o.rdu = ident<T>() - mscale(T(2), Y * Si);
-    o.tu = X * SiY2;
-    // SH (green.cu:339-353)
-    const cx<T> wa = u.v * u.mu, wb = d.v * d.mu;
-    const cx<T> is = recip(wa + wb);
-    o.td0 = T(2) * wa * is;
-    o.rdu0 = (wa - wb) * is;
-    o.rud0 = -o.rdu0;
-    o.tu0 = T(2) * wb * is;
-    return o;
+    const cx<T> mga = a.mu * a.gb, mgb = b.mu * b.gb;
+    const cx<T> ipa = recip(-(a.mu * a.v * a.gb));  // 1 / det Pt_a (up to w^2 scaling shared by Pt_b)
+    const cx<T> ina = recip(a.mu * a.r * a.gb);     // 1 / det Nt_a
+    X.a = ipa * (a.v * (k2dmu2 - mgb));
+    X.b = ipa * ((T(2) * k) * (a.v * b.v) * dmu);
+    X.c = ipa * (k * (mgb - mga - k2dmu2));
+    X.d = ipa * (b.v * (-mga - k2dmu2));
+    Y.a = ina * (b.r * (k2dmu2 + mga));
+    Y.b = ina * (k * (k2dmu2 - mgb + mga));
+    Y.c = ina * (-((T(2) * k) * (a.r * b.r) * dmu));
+    Y.d = ina * (a.r * (mgb - k2dmu2));
+}
+
+template <typename T> GRTM_HD m2<T> i_plus(const m2<T> &Q)
+{
+    return {C<T>(T(1) + Q.a.x, Q.a.y), Q.b, Q.c, C<T>(T(1) + Q.d.x, Q.d.y)};
+}
+template <typename T> GRTM_HD m2<T> i_minus(const m2<T> &Q)
+{
+    return {C<T>(T(1) - Q.a.x, -Q.a.y), -Q.b, -Q.c, C<T>(T(1) - Q.d.x, -Q.d.y)};
+}
+
+// One step of the recursion: R, T across an interface (X, Y of
+// interface_xy) with b's side reflecting Q.
+template <typename T>
+GRTM_HD void grt_step(const m2<T> &X, const m2<T> &Y, const m2<T> &Q, m2<T> &R, m2<T> &Tr)
+{
+    const m2<T> F = X * i_plus(Q), H = Y * i_minus(Q);
+    const m2<T> Gi = minv(F + H);
+    R = (F - H) * Gi;
+    Tr = mscale(T(2), Gi);
+}
+// The same with nothing reflecting on b's side (b the half-space).
+template <typename T>
+GRTM_HD void grt_step0(const m2<T> &X, const m2<T> &Y, m2<T> &R, m2<T> &Tr)
+{
+    const m2<T> Gi = minv(X + Y);
+    R = (X - Y) * Gi;
+    Tr = mscale(T(2), Gi);
+}
+// SH (E_SH = [[1, 1], [-mu v, mu v]]): impedances wa = mu_a v_a, wb = mu_b v_b,
+// q the generalized SH reflection of b's side.
+template <typename T>
+GRTM_HD void grt_step_sh(cx<T> wa, cx<T> wb, cx<T> q, cx<T> &r, cx<T> &t)
+{
+    const cx<T> p = wa * (C<T>(1, 0) + q), m = wb * (C<T>(1, 0) - q);
+    const cx<T> gi = recip(p + m);
+    r = (p - m) * gi;
+    t = T(2) * wa * gi;
 }
 
 // Free-surface reflection R_ud^0 = -(E21)^-1 E22 (green.cu:449-468) in closed
-// form: E21 = -mu F diag(a/w, b/w), F = [[2kr, x], [x, 2kv]], E22 = diag(-1,1) E21.
-// det F = 4 k^2 r v - x^2 (the Rayleigh function of the top layer) cancels
-// near the layer's Rayleigh slowness, where a shallow source excites the
-// surface waves strongly: it is formed in FP64 from the FP64 wavenumber,
-// vertical wavenumbers and w / beta (sbd).
+// form, without the E column normalisation: E21 = -mu F, F = [[2kr, x],
+// [x, 2kv]], E22 = diag(-1, 1) E21.  det F = 4 k^2 r v - x^2 (the Rayleigh
+// function of the top layer) cancels near the layer's Rayleigh slowness,
+// where a shallow source excites the surface waves strongly: it is formed in
+// FP64 from the FP64 wavenumber, vertical wavenumbers and w / beta (sbd).
 template <typename T>
 GRTM_HD m2<T> free_surface(const LQ<T> &q, typename WideOf<T>::type kd, cx<typename WideOf<T>::type> sbd)
 {
@@ -191,44 +211,17 @@ GRTM_HD m2<T> free_surface(const LQ<T> &q, typename WideOf<T>::type kd, cx<typen
     const cx<W> xd = C<W>(W(2) * kd * kd, W(0)) - sbd * sbd;
     const cx<T> nr = narrow<T>(recip(C<W>(W(4) * kd * kd, W(0)) * (q.rd * q.vd) - xd * xd));  // 1/det F
     const cx<T> dd = (x2 + rv4) * nr;
-    m2<T> R;
-    R.a = dd;
-    R.b = (T(4) * k) * (q.v * q.x) * nr * (q.cb * q.sa);
-    R.c = -((T(4) * k) * (q.r * q.x) * nr * (q.ca * q.sb));
-    R.d = -dd;
-    return R;
+    const cx<T> k4x = (T(4) * k) * q.x * nr;
+    return m2<T>{dd, k4x * q.v, -(k4x * q.r), -dd};
 }
 
-// Downward step across interface j (green.cu:471-520): Rg = R_ud at the top
-// of layer j, e = Lambda_j.  Returns the generalized T_u of layer j+1.
+// The P-SV source vector in the normalisation-free basis: its components
+// times a/w, b/w of the source layer.
 template <typename T>
-GRTM_HD void down_step(m2<T> &Rg, cx<T> &rg0, const RT<T> &t, cx<T> e0, cx<T> e1, m2<T> &Tu,
-                       cx<T> &tu0)
+GRTM_HD void src_scaled(const LQ<T> &q, cx<T> &v0, cx<T> &v1)
 {
-    const cx<T> tem1 = rg0 * e1;
-    tu0 = t.tu0 * recip(C<T>(1, 0) - tem1 * t.rdu0 * e1);
-    rg0 = t.rud0 + t.td0 * e1 * tem1 * tu0;
-    const m2<T> LRL = dmd(e0, e1, Rg, e0, e1);
-    const m2<T> A = ident<T>() - t.rdu * LRL;
-    const m2<T> B = t.td * LRL;
-    Tu = minv(A) * t.tu;
-    Rg = t.rud + B * Tu;
-}
-
-// Upward step across interface j (green.cu:532-594): Rg = R_du at the bottom
-// of layer j+1, e = Lambda_{j+1}.  Returns the generalized T_d of layer j+1.
-template <typename T>
-GRTM_HD void up_step(m2<T> &Rg, cx<T> &rg0, const RT<T> &t, cx<T> e0, cx<T> e1, m2<T> &Td,
-                     cx<T> &td0)
-{
-    const cx<T> tem2 = rg0 * e1;
-    td0 = t.td0 * recip(C<T>(1, 0) - t.rud0 * e1 * tem2);
-    rg0 = t.rdu0 + t.tu0 * e1 * tem2 * td0;
-    const m2<T> LRL = dmd(e0, e1, Rg, e0, e1);
-    const m2<T> A = ident<T>() - t.rud * LRL;
-    const m2<T> B = t.tu * LRL;
-    Td = minv(A) * t.td;
-    Rg = t.rdu + B * Td;
+    v0 = v0 * q.ca;
+    v1 = v1 * q.cb;
 }
 
 template <typename T, typename W>
@@ -293,9 +286,11 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
     {
         LQ<T> qd = layer_q(lw[L - 1], kd);  // layer below the interface
         LQ<T> qu = layer_q(lw[L - 2], kd);
-        RT<T> t = interface_rt(qu, qd, k);
-        m2<T> Rg = t.rdu, Td = t.td;
-        cx<T> rg0 = t.rdu0, td0 = t.td0;
+        m2<T> X, Y, Rg, Td;
+        cx<T> rg0, td0;
+        interface_xy(qu, qd, k, X, Y);
+        grt_step0(X, Y, Rg, Td);
+        grt_step_sh(qu.v * qu.mu, qd.v * qd.mu, C<T>(0, 0), rg0, td0);
         // grt[L-1] = (Rg, Td)
         int j = L - 1;  // index of the grt entry just formed
         // lj0, lj1 = Lambda_jj (layer jj, just below the interface formed)
@@ -342,8 +337,11 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             lam(qu, lw[lay + 1].h, e0, e1);
             qd = qu;
             qu = layer_q(lw[lay], kd);
-            t = interface_rt(qu, qd, k);
-            up_step(Rg, rg0, t, e0, e1, Td, td0);
+            // green.cu:532-594: generalized R_du, T_d across interface lay with
+            // the stack below reflecting Lambda R Lambda
+            interface_xy(qu, qd, k, X, Y);
+            grt_step(X, Y, dmd(e0, e1, Rg, e0, e1), Rg, Td);
+            grt_step_sh(qu.v * qu.mu, qd.v * qd.mu, e1 * rg0 * e1, rg0, td0);
             visit(lay + 1, Rg, rg0, Td, td0, e0, e1);
         }
     }
@@ -365,10 +363,14 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             cx<T> e0, e1;
             lam(qu, lw[lay].h, e0, e1);  // Lambda_lay
             LQ<T> qd = layer_q(lw[lay + 1], kd);
-            RT<T> t = interface_rt(qu, qd, k);
-            m2<T> Tu;
+            // green.cu:471-520: generalized R_ud, T_u across interface lay
+            // (incidence from below) with the stack above reflecting
+            // Lambda R Lambda
+            m2<T> X, Y, Tu;
             cx<T> tu0;
-            down_step(Rg, rg0, t, e0, e1, Tu, tu0);
+            interface_xy(qd, qu, k, X, Y);
+            grt_step(X, Y, dmd(e0, e1, Rg, e0, e1), Rg, Tu);
+            grt_step_sh(qd.v * qd.mu, qu.v * qu.mu, e1 * rg0 * e1, rg0, tu0);
             const int jn = lay + 1;  // grt index formed
             if (branch == 1) {
                 if (jn == rl) {
@@ -405,10 +407,10 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
     const m2<T> yd_psv = minv(ident<T>() - out.ERu * out.ERd);
     const cx<T> y_sh = recip(C<T>(1, 0) - out.erd0 * out.eru0);
 
-    // receiver-layer E rows 0,1 (displacements): down cols [[ca k, cb v], [ca r, cb k]],
-    // up cols [[ca k, cb v], [-ca r, -cb k]]
-    const m2<T> Ed{qr.ca * k, qr.cb * qr.v, qr.ca * qr.r, qr.cb * k};
-    const m2<T> Eu{qr.ca * k, qr.cb * qr.v, -(qr.ca * qr.r), -(qr.cb * k)};
+    // receiver-layer E rows 0,1 (displacements) without the column
+    // normalisation: down cols [[k, v], [r, k]], up cols [[k, v], [-r, -k]]
+    const m2<T> Ed{C<T>(k, 0), qr.v, qr.r, C<T>(k, 0)};
+    const m2<T> Eu{C<T>(k, 0), qr.v, -qr.r, C<T>(-k, 0)};
     const W hr_ = lw[rl].h;
     const bool rl_half = (rl == L - 1);
 
@@ -481,7 +483,15 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
         const m2<T> d = md(Ed, rd0, rd1) + Eu * RS;
         const cx<T> d0 = ru1 * rdu0_r1 * ex1 + rd1;
         const m2<T> &ypsv = COMPAT ? yu_psv : yd_psv;
-        const m2<T> chain = Md * dmd(sexd0, sexd1, ypsv, C<T>(1, 0), C<T>(1, 0));
+        m2<T> Mc = Md;
+        if (COMPAT) {
+            // the reference's chain T_d[rl+1] .. T_d[s+2] (Q6): without the E
+            // column normalisation its ends leave D_rl D_{rl+1}^-1 on the left
+            // and D_{s+1} D_s^-1 on the right (D = diag(a/w, b/w) per layer)
+            const LayerW<T> &lr = lw[rl], &lr1 = lw[rl + 1], &ls1 = lw[s + 1], &ls = lw[s];
+            Mc = dmd(lr.ca * recip(lr1.ca), lr.cb * recip(lr1.cb), Md, ls1.ca * recip(ls.ca), ls1.cb * recip(ls.cb));
+        }
+        const m2<T> chain = Mc * dmd(sexd0, sexd1, ypsv, C<T>(1, 0), C<T>(1, 0));
         const cx<T> chain0 = md0 * sexd1 * y_sh;
         out.Y = d * chain;
         out.y0 = d0 * chain0;
@@ -536,8 +546,10 @@ GRTM_HD void coef_from_cell(const CellOut<T> &c, int src_type, int up, T k, cx<T
     if (src_type == 0) {
         // f0 = b/(2 mu a w r v) * (b v r, -a k r); f1 = (k b v, -a r v) * same
         const cx<T> tmp = b * recip(T(2) * q.mu * w * w * q.r * q.v) * q.sa;  // b/(2 mu a w r v)
-        const cx<T> f00 = b * q.v * q.r * tmp, f01 = -(k * (w * recip(q.sa)) * q.r * tmp);
-        const cx<T> f10 = k * b * q.v * tmp, f11 = -((w * recip(q.sa)) * q.r * q.v * tmp);
+        cx<T> f00 = b * q.v * q.r * tmp, f01 = -(k * (w * recip(q.sa)) * q.r * tmp);
+        cx<T> f10 = k * b * q.v * tmp, f11 = -((w * recip(q.sa)) * q.r * q.v * tmp);
+        src_scaled(q, f00, f01);
+        src_scaled(q, f10, f11);
         cx<T> g00, g01, g10, g11;
         mv(W0, f00, f01, g00, g01);
         mv(W1, f10, f11, g10, g11);
@@ -555,9 +567,12 @@ GRTM_HD void coef_from_cell(const CellOut<T> &c, int src_type, int up, T k, cx<T
     } else {
         const cx<T> av = w * recip(q.sa);
         const cx<T> tmp = b * recip(T(2) * q.mu * av * w);  // b/(2 mu a w)
-        const cx<T> s20 = tmp * k * recip(q.r) * k * b, s21 = -(k * tmp * av);
-        const cx<T> s10 = T(-2) * k * b * tmp, s11 = tmp * recip(q.v) * q.x * av;
-        const cx<T> s00 = tmp * q.r * b, s01 = -(tmp * av * k);
+        cx<T> s20 = tmp * k * recip(q.r) * k * b, s21 = -(k * tmp * av);
+        cx<T> s10 = T(-2) * k * b * tmp, s11 = tmp * recip(q.v) * q.x * av;
+        cx<T> s00 = tmp * q.r * b, s01 = -(tmp * av * k);
+        src_scaled(q, s00, s01);
+        src_scaled(q, s10, s11);
+        src_scaled(q, s20, s21);
         cx<T> u00, u01, u10, u11, u20, u21;
         mv(W1, s00, s01, u00, u01);
         mv(W0, s10, s11, u10, u11);
@@ -598,16 +613,18 @@ GRTM_HD void zr_from_cell(const CellOut<T> &c, int up, T k, cx<T> w, cx<T> &kz_e
     const cx<T> b = w * recip(q.sb);
     const cx<T> av = w * recip(q.sa);
     const cx<T> tmp = b * recip(T(2) * q.mu * av * w);
-    // explosion
-    const cx<T> sx = -(b * q.sa * q.sa * tmp * recip(q.r));
+    // explosion; the source vector carries the a/w normalisation of the
+    // source layer (q.sa q.ca = 1)
+    const cx<T> sx = -(b * q.sa * tmp * recip(q.r));
     const cx<T> e0 = W1.a * sx, e1 = W1.c * sx;
     cx<T> a0, a1;
     mv(c.Y, e0, e1, a0, a1);
     kz_ex = a1 * k;
     kr_ex = COMPAT ? -(a1 * k) : -(a0 * k);
     // vertical force: f0 = (b^2/(2 mu a w), -b k/(2 mu w v))
-    const cx<T> f00 = b * tmp;
-    const cx<T> f01 = -(b * k * recip(T(2) * q.mu * w * q.v));
+    cx<T> f00 = b * tmp;
+    cx<T> f01 = -(b * k * recip(T(2) * q.mu * w * q.v));
+    src_scaled(q, f00, f01);
     cx<T> g0, g1, v0, v1;
     mv(W0, f00, f01, g0, g1);
     mv(c.Y, g0, g1, v0, v1);

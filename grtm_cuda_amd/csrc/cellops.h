@@ -26,7 +26,8 @@ __device__ __forceinline__ bool cell_zr(const LayerW<T> *lw, const Geo &g, int i
     CellOut<T> c;
     cell_solve<T, COMPAT, false, BR>(lw, g, kd, C<T>(0, 0), C<T>(0, 0), c);
     cx<T> kze, kre, kzv, krv;
-    zr_from_cell<T, COMPAT>(c, g.up, k, w, kze, kre, kzv, krv);
+    // the coupling direction follows from the branch (above the source: 1, 2)
+    zr_from_cell<T, COMPAT>(c, BR ? int(BR <= 2) : g.up, k, w, kze, kre, kzv, krv);
     const T wt = T(ik == 0 ? 0.5 * dk : dk);
     const T r[8] = {wt * kze.x, wt * kze.y, wt * kzv.x, wt * kzv.y, wt * kre.x, wt * kre.y, wt * krv.x, wt * krv.y};
     bool ok = true;
@@ -78,7 +79,7 @@ __device__ __forceinline__ void cell_mt_rows(const LayerW<T> *lw, const Geo &g, 
     CellOut<T> c;
     cell_solve<T, COMPAT, DERIV, BR>(lw, g, kd, C<T>(0, 0), C<T>(0, 0), c);
     const T wt = T(ik == 0 ? 0.5 * dk : dk);
-    mt_rows_from_cell<T, COMPAT, DERIV>(c, g.up, k, w,
+    mt_rows_from_cell<T, COMPAT, DERIV>(c, BR ? int(BR <= 2) : g.up, k, w,
                                         [&](int set, int j, cx<T> v) { put(set, j, wt * v.x, wt * v.y); });
 }
 
