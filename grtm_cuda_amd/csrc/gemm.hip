@@ -134,20 +134,6 @@ __global__ __launch_bounds__(256) void k_gemm_f32(int M, int N, int K, int kslic
 }
 
 // ------------------------------------------------------------------ FP32, streamed
-// Two contractions C_p = A_p B_p (p = blockIdx.z / S) in one launch.  Block
-// tile 128 x 128, 4 waves of 64 x 64, K slab 32; the next slab's 16-byte
-// global loads are in flight while the matrix cores work on the current one
-// (two LDS buffers).  Within a slab, lanes 0-31 of every MFMA take k in
-// [0, 16) and lanes 32-63 take k in [16, 32), so a lane's A operands for four
-// steps are one 16-byte LDS read.  Split-K: slice z covers [z kc, (z+1) kc);
-// a slice at or past its row tile's extent kend returns without writing
-// (k_gemm_reduce_ragged sums only the live slices).  The slice index is the
-// outermost grid dimension, so blocks running together read the same K range
-// of B (dispatching a tile's slices together, heaviest tile first, measured
-// slower: 1.6 ms against 0.85 ms on the learned grid).  Requires lda, ldb
-// multiples of 4 and 16-byte aligned A, B.
-constexpr int kSBM = 128, kSBN = 128, kSBK = 32, kSPA = kSBK + 4, kSPB = kSBN + 4;
-
 // Up to five contractions C_p = A_p B_p with a shared N, K and strides and
 // their own row counts, in one launch: grid z = p S + slice (contraction
 // outermost, then slice), grid y = row tiles of the largest M (blocks past a
@@ -159,122 +145,184 @@ struct GemmSetT {
     const T *B[kGemmSetMax];
     T *P[kGemmSetMax];             // split-K partials of contraction p: [S][M_p][N]
     T *C[kGemmSetMax];
-    const int *kend[kGemmSetMax];  // per row tile K extent (null: K)
+    const int *kend[kGemmSetMax];  // K extent per kKendRows operand rows (null: K)
     int M[kGemmSetMax];            // rows of contraction p
     int rpw[kGemmSetMax];          // rows per frequency row of nk_w
 };
 using GemmSet = GemmSetT<float>;
 
-__global__ __launch_bounds__(256, 2) void k_gemm_f32s(int N, int K, int kc, int S, GemmSet gp, long lda, long ldb,
-                                                      const int *__restrict__ nk_w)
+// Rows covered by one kend entry (k_kend_tiles over 32 frequency rows of four
+// operand rows, k_kend_sets over 128 rows).
+constexpr int kKendRows = 128;
+
+// K extent of the operand rows [m0, m0 + bm) of a contraction with M rows:
+// the largest extent of the kend entries they touch; K without kend.
+__device__ __forceinline__ int tile_extent(const int *__restrict__ kend, int M, int K, int m0, int bm)
 {
-    __shared__ __attribute__((aligned(16))) float As[2][kSBM][kSPA];
-    __shared__ __attribute__((aligned(16))) float Bs[2][kSBK][kSPB];
+    if (!kend) return K;
+    const int j1 = (min(m0 + bm, M) - 1) / kKendRows;
+    int kt = 0;
+    for (int j = m0 / kKendRows; j <= j1; j++) kt = max(kt, kend[j]);
+    return min(K, kt);
+}
+
+// Block tile BM x BN of 8 waves, wave tile WM x WN (WM/32 x WN/32 MFMA
+// 32x32x2 tiles), K slab 32.  The next slab's 16-byte global loads are in
+// flight while the matrix cores work on the current one (two LDS buffers),
+// and within a slab the operands of the next four k steps are read from LDS
+// while the current four are multiplied.  Lanes 0-31 of every MFMA take k in
+// [0, 16) of the slab and lanes 32-63 k in [16, 32), so a lane's A operands
+// for four steps are one 16-byte LDS read.  Split-K: slice z covers
+// [z kc, (z+1) kc); a slice at or past the block's K extent returns without
+// writing (k_gemm_reduce_set sums only the live slices).  The slice index is
+// the outermost grid dimension, so blocks running together read the same K
+// range of B.  A row r is read up to its frequency row's cut nk_w[r / rpw]
+// (16-byte loads, the entries past the cut replaced by zeros).  Requires lda
+// and ldb multiples of 4, N <= ldb, 16-byte aligned A and B.
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f32t(int N, int K, int kc, int S, GemmSet gp,
+                                                                       long lda, long ldb,
+                                                                       const int *__restrict__ nk_w)
+{
+    constexpr int NT = (BM / WM) * (BN / WN) * 64, BK = 32, PA = BK + 4, PB = BN + 4;
+    constexpr int TM = WM / 32, TN = WN / 32, AE = BM * BK / 4 / NT, BE = BK * BN / 4 / NT, BNV = BN / 4;
+    static_assert(AE * NT * 4 == BM * BK && BE * NT * 4 == BK * BN, "staging must cover the slab");
+    __shared__ __attribute__((aligned(16))) float As[2][BM][PA];
+    __shared__ __attribute__((aligned(16))) float Bs[2][BK][PB];
     const int pi = blockIdx.z / S, z = blockIdx.z - pi * S;
     const int M = gp.M[pi];
-    const int m0 = blockIdx.y * kSBM, n0 = blockIdx.x * kSBN;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
     if (m0 >= M) return;  // uniform: a contraction with fewer row tiles
+    const int kt = tile_extent(gp.kend[pi], M, K, m0, BM);
+    const int kb = z * kc;
+    if (kb >= kt) return;  // uniform: this slice is past the block's extent
+    const int ke = min(kt, kb + kc);
     const float *__restrict__ A = gp.A[pi];
     const float *__restrict__ B = gp.B[pi];
-    const int *__restrict__ kend = gp.kend[pi];
     const int rpw = gp.rpw[pi];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int wm = wv >> 1, wn = wv & 1;
-    const int kt = min(K, kend ? kend[blockIdx.y] : K);
-    const int kb = z * kc;
-    if (kb >= kt) return;  // uniform: this slice is past the tile's extent
-    const int ke = min(kt, kb + kc);
+    const int wm = wv / (BN / WN), wn = wv % (BN / WN);
 
-    // staging: A items idx = tid + 256 e -> row idx >> 3, k offset (idx & 7) * 4;
-    // B items -> k row idx >> 5, column offset (idx & 31) * 4
-    int a_lim[4];
-    for (int e = 0; e < 4; e++) {
-        const int gm = m0 + ((tid + 256 * e) >> 3);
+    // staging: A item idx = tid + NT e -> row idx >> 3, k offset (idx & 7) * 4;
+    // B item -> k row idx / BNV, column offset (idx % BNV) * 4
+    int a_lim[AE];
+    #pragma unroll
+    for (int e = 0; e < AE; e++) {
+        const int gm = m0 + ((tid + NT * e) >> 3);
         a_lim[e] = gm < M ? (nk_w ? min(ke, nk_w[gm / rpw]) : ke) : 0;
     }
-    float4 ra[4], rb[4];
+    float4 ra[AE], rb[BE];
     auto load = [&](int k0) {
-        for (int e = 0; e < 4; e++) {
-            const int idx = tid + 256 * e, gm = m0 + (idx >> 3), gk = k0 + (idx & 7) * 4;
+        #pragma unroll
+        for (int e = 0; e < AE; e++) {
+            const int idx = tid + NT * e, gk = k0 + (idx & 7) * 4;
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gk + 3 < a_lim[e]) {
-                v = *reinterpret_cast<const float4 *>(A + (size_t)gm * lda + gk);
-            } else if (gk < a_lim[e]) {
-                const float *r = A + (size_t)gm * lda;
-                v.x = r[gk];
-                if (gk + 1 < a_lim[e]) v.y = r[gk + 1];
-                if (gk + 2 < a_lim[e]) v.z = r[gk + 2];
+            if (gk < a_lim[e]) {
+                // gk + 3 < lda: gk and lda are multiples of 4 and gk < K <= lda
+                v = *reinterpret_cast<const float4 *>(A + (size_t)(m0 + (idx >> 3)) * lda + gk);
+                const int lim = a_lim[e] - gk;
+                if (lim < 4) {
+                    v.w = 0.f;
+                    if (lim < 3) v.z = 0.f;
+                    if (lim < 2) v.y = 0.f;
+                }
             }
             ra[e] = v;
         }
-        for (int e = 0; e < 4; e++) {
-            const int idx = tid + 256 * e, gk = k0 + (idx >> 5), gn = n0 + (idx & 31) * 4;
+        #pragma unroll
+        for (int e = 0; e < BE; e++) {
+            const int idx = tid + NT * e, gk = k0 + idx / BNV, gn = n0 + (idx % BNV) * 4;
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gk < ke) {
-                const float *r = B + (size_t)gk * ldb;
-                if (gn + 3 < N) {
-                    v = *reinterpret_cast<const float4 *>(r + gn);
-                } else {
-                    if (gn < N) v.x = r[gn];
-                    if (gn + 1 < N) v.y = r[gn + 1];
-                    if (gn + 2 < N) v.z = r[gn + 2];
-                }
-            }
+            // gn + 3 < ldb: gn and ldb are multiples of 4 and gn < N <= ldb;
+            // columns at or past N only reach accumulators that are not written
+            if (gk < ke && gn < N) v = *reinterpret_cast<const float4 *>(B + (size_t)gk * ldb + gn);
             rb[e] = v;
         }
     };
     auto store = [&](int buf) {
-        for (int e = 0; e < 4; e++) {
-            const int idx = tid + 256 * e;
+        #pragma unroll
+        for (int e = 0; e < AE; e++) {
+            const int idx = tid + NT * e;
             *reinterpret_cast<float4 *>(&As[buf][idx >> 3][(idx & 7) * 4]) = ra[e];
-            *reinterpret_cast<float4 *>(&Bs[buf][idx >> 5][(idx & 31) * 4]) = rb[e];
+        }
+        #pragma unroll
+        for (int e = 0; e < BE; e++) {
+            const int idx = tid + NT * e;
+            *reinterpret_cast<float4 *>(&Bs[buf][idx / BNV][(idx % BNV) * 4]) = rb[e];
         }
     };
 
-    v16f acc[2][2];
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++)
+    v16f acc[TM][TN];
+    #pragma unroll
+    for (int a = 0; a < TM; a++)
+        #pragma unroll
+        for (int b = 0; b < TN; b++)
+            #pragma unroll
             for (int r = 0; r < 16; r++) acc[a][b][r] = 0.f;
     load(kb);
     store(0);
     __syncthreads();
     const int i = lane & 31, kq = lane >> 5;
     int buf = 0;
-    for (int k0 = kb; k0 < ke; k0 += kSBK) {
-        const bool more = k0 + kSBK < ke;
-        if (more) load(k0 + kSBK);
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+        const bool more = k0 + BK < ke;
+        if (more) load(k0 + BK);
+        float4 fa[2][TM];
+        float fb[2][4][TN];
+        auto frag = [&](int q, int s) {
+            #pragma unroll
+            for (int t = 0; t < TM; t++)
+                fa[s][t] = *reinterpret_cast<const float4 *>(&As[buf][wm * WM + t * 32 + i][16 * kq + 4 * q]);
+            #pragma unroll
+            for (int j = 0; j < 4; j++)
+                #pragma unroll
+                for (int t = 0; t < TN; t++) fb[s][j][t] = Bs[buf][16 * kq + 4 * q + j][wn * WN + t * 32 + i];
+        };
+        frag(0, 0);
+        #pragma unroll
         for (int q = 0; q < 4; q++) {
-            float4 a4[2];
-            for (int t = 0; t < 2; t++)
-                a4[t] = *reinterpret_cast<const float4 *>(&As[buf][wm * 64 + t * 32 + i][16 * kq + 4 * q]);
-            for (int j = 0; j < 4; j++) {
-                const int kk = 16 * kq + 4 * q + j;
-                float b[2];
-                for (int t = 0; t < 2; t++) b[t] = Bs[buf][kk][wn * 64 + t * 32 + i];
-                for (int tm = 0; tm < 2; tm++) {
-                    const float a = j == 0 ? a4[tm].x : j == 1 ? a4[tm].y : j == 2 ? a4[tm].z : a4[tm].w;
-                    for (int tn = 0; tn < 2; tn++)
-                        acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b[tn], acc[tm][tn], 0, 0, 0);
+            const int s = q & 1;
+            if (q < 3) frag(q + 1, s ^ 1);
+            #pragma unroll
+            for (int j = 0; j < 4; j++)
+                #pragma unroll
+                for (int tm = 0; tm < TM; tm++) {
+                    const float a = j == 0 ? fa[s][tm].x : j == 1 ? fa[s][tm].y : j == 2 ? fa[s][tm].z : fa[s][tm].w;
+                    #pragma unroll
+                    for (int tn = 0; tn < TN; tn++)
+                        acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, fb[s][j][tn], acc[tm][tn], 0, 0, 0);
                 }
-            }
         }
         if (more) store(buf ^ 1);
         __syncthreads();
         buf ^= 1;
     }
     float *Pz = gp.P[pi] + (size_t)z * M * N;
-    for (int tm = 0; tm < 2; tm++)
-        for (int tn = 0; tn < 2; tn++)
+    #pragma unroll
+    for (int tm = 0; tm < TM; tm++)
+        #pragma unroll
+        for (int tn = 0; tn < TN; tn++)
+            #pragma unroll
             for (int r = 0; r < 16; r++) {
-                const int row = m0 + wm * 64 + tm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                const int col = n0 + wn * 64 + tn * 32 + (lane & 31);
+                const int row = m0 + wm * WM + tm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int col = n0 + wn * WN + tn * 32 + (lane & 31);
                 if (row < M && col < N) Pz[(size_t)row * N + col] = acc[tm][tn][r];
             }
 }
 
+// FP32 block tile for N columns: 128 rows by 128 columns (waves of 32 x 64),
+// or by 64 columns (waves of 32 x 32) for N <= 64; two resident blocks of 8
+// waves per compute unit.  On the 256-receiver contraction pair the 128 x 128
+// tile and a 256 x 256 tile run the dense grid within 2% of each other, and
+// the 128 x 128 tile runs the learned (ragged) grid 1.4x faster: its finer
+// row tiles follow the per-tile K extents more closely.
+constexpr int kTBM = 128, kTBK = 32;
+static int f32_bn(int N) { return N > 64 ? 128 : 64; }
+
 // Sum of the live split-K slices of every contraction of a set, in FP64, in
-// slice order: C_p[row] = sum_{z < ceil(kend_p[row / bm] / kc)} P_p[z][row].
-// Thread t covers element t of the concatenated outputs.
+// slice order: C_p[row] = sum over z < ceil(kt / kc) of P_p[z][row], kt the K
+// extent of the row's bm-row block tile.  Thread t covers element t of the
+// concatenated outputs.
 template <typename T>
 __global__ void k_gemm_reduce_set(int n, int N, int K, int kc, int bm, GemmSetT<T> gs, long total)
 {
@@ -288,8 +336,7 @@ __global__ void k_gemm_reduce_set(int n, int N, int K, int kc, int bm, GemmSetT<
     }
     const long MN = (long)gs.M[pi] * N;
     const int row = int(e / N);
-    const int *kend = gs.kend[pi];
-    const int kt = min(K, kend ? kend[row / bm] : K);
+    const int kt = tile_extent(gs.kend[pi], gs.M[pi], K, row / bm * bm, bm);
     const int nz = (kt + kc - 1) / kc;
     const T *P = gs.P[pi];
     double s = 0;
@@ -298,7 +345,7 @@ __global__ void k_gemm_reduce_set(int n, int N, int K, int kc, int bm, GemmSetT<
 }
 
 // ------------------------------------------------------------------ FP64, streamed
-// Two contractions in FP64 on v_mfma_f64_16x16x4f64, as k_gemm_f32s: block
+// Two contractions in FP64 on v_mfma_f64_16x16x4f64, as k_gemm_f32t: block
 // tile 128 x 128 (A is read N / 128 times instead of N / 64), 4 waves of
 // 64 x 64 (4 x 4 MFMA tiles), K slab 16 with the next slab's 16-byte loads in
 // flight (two LDS buffers).  Lane l of an MFMA takes A[m0 + l % 16][k + l / 16]
@@ -508,13 +555,56 @@ static void split_plan(int M, int N, int K, int BM, int BN, int BK, int &S, int 
     kslice = ((K + S - 1) / S + BK - 1) / BK * BK;
 }
 
-// Streamed FP32 pair: slice length from the mean K extent of the row tiles
-// (k_avg; the full K for a dense grid) so that about three blocks per compute
-// unit stay live; a power of two, at least 256 and at most K rounded up.
-int gemm_pair_slice(int M, int N, int K, double k_avg)
+// Block tiles of n contractions of M[q] rows for the streamed kernels: mode 1
+// the FP32 tile (kTBM x f32_bn(N)), mode 0 the FP64 tile (128 x 128); two
+// resident blocks per compute unit either way.
+static long set_tiles(int mode, int n, const int *M, int N)
 {
-    const double tiles = 2.0 * ((M + kSBM - 1) / kSBM) * ((N + kSBN - 1) / kSBN);
-    const double want = tiles * std::max(1.0, k_avg) / 768.0;
+    const int bm = mode == 1 ? kTBM : kDBM, bn = mode == 1 ? f32_bn(N) : kDBN;
+    long tiles = 0;
+    for (int q = 0; q < n; q++) tiles += (long)((M[q] + bm - 1) / bm) * ((N + bn - 1) / bn);
+    return tiles;
+}
+static long set_slots(int mode) { return 2L * 256; }
+
+// Slice length of a dense grid: K / S for the power of two S <= smax (slices of
+// at least 256) whose blocks fill their last dispatch round best; ties go to
+// the smaller S.  min_rounds > 1 first restricts S to splits of at least that
+// many dispatch rounds where one exists.
+static int dense_slice(long tiles, long slots, int K, int smax, int min_rounds = 1)
+{
+    int best = K;
+    double best_eff = -1;
+    for (int pass = 0; pass < 2 && best_eff < 0; pass++)
+        for (int S = 1; S <= smax; S *= 2) {
+            const int kc = ((K + S - 1) / S + kTBK - 1) / kTBK * kTBK;
+            if (S > 1 && kc < 256) break;
+            const long blocks = tiles * ((K + kc - 1) / kc);
+            if (pass == 0 && blocks < min_rounds * slots) continue;
+            const double eff = double(blocks) / double((blocks + slots - 1) / slots * slots);
+            if (eff > best_eff + 1e-9) best_eff = eff, best = kc;
+        }
+    return best;
+}
+
+// Slice length of a pair (two contractions of M rows) from the mean K extent
+// of the row tiles (k_avg; K for a dense grid).  FP32: the dense grid fills
+// its dispatch rounds (dense_slice); a ragged grid takes the power of two
+// (at least 256) nearest to kRaggedRounds rounds of equal slices.  FP64: about
+// three blocks per compute unit live, a power of two of at least 256.
+constexpr double kRaggedRounds = 4.0;
+int gemm_pair_slice(int mode, int M, int N, int K, double k_avg)
+{
+    const int Ms[2] = {M, M};
+    const long tiles = set_tiles(mode, 2, Ms, N);
+    if (mode == 1) {
+        if (k_avg >= K) return dense_slice(tiles, set_slots(mode), K, 16, 2);
+        const double want = double(tiles) * std::max(1.0, k_avg) / (kRaggedRounds * set_slots(mode));
+        int kc = 256;
+        while (kc * 2 <= want * 1.4142 && kc < K) kc *= 2;
+        return kc;
+    }
+    const double want = double(tiles) * std::max(1.0, k_avg) / 768.0;
     int kc = 256;
     while (kc * 2 <= want && kc < K) kc *= 2;
     return kc;
@@ -530,9 +620,10 @@ size_t gemm_pair_workspace_bytes(int M, int N, int K, int kc)
 // streamed kernel and the reduce.
 template <typename T>
 static hipError_t launch_gemm_set_t(int n, GemmSetT<T> gs, int N, int K, int kc, long lda, long ldb, T *P,
-                                    const int *nk_w, int bm, hipStream_t st)
+                                    const int *nk_w, hipStream_t st)
 {
     if (n < 1 || n > kGemmSetMax) return hipErrorInvalidValue;
+    const int bm = sizeof(T) == 4 ? kTBM : kDBM, bn = sizeof(T) == 4 ? f32_bn(N) : kDBN;
     const int S = (K + kc - 1) / kc;
     int mt = 0;
     long total = 0;
@@ -543,11 +634,17 @@ static hipError_t launch_gemm_set_t(int n, GemmSetT<T> gs, int N, int K, int kc,
         mt = std::max(mt, (gs.M[q] + bm - 1) / bm);
     }
     if (mt > 65535 || (long)n * S > 65535) return hipErrorInvalidValue;
-    dim3 grid((N + kSBN - 1) / kSBN, mt, n * S);
-    if constexpr (sizeof(T) == 4)
-        hipLaunchKernelGGL(k_gemm_f32s, grid, dim3(256), 0, st, N, K, kc, S, gs, lda, ldb, nk_w);
-    else
+    dim3 grid((N + bn - 1) / bn, mt, n * S);
+    if constexpr (sizeof(T) == 4) {
+        if (bn == 128)
+            hipLaunchKernelGGL((k_gemm_f32t<kTBM, 128, 32, 64>), grid, dim3(512), 0, st, N, K, kc, S, gs, lda, ldb,
+                               nk_w);
+        else
+            hipLaunchKernelGGL((k_gemm_f32t<kTBM, 64, 32, 32>), grid, dim3(512), 0, st, N, K, kc, S, gs, lda, ldb,
+                               nk_w);
+    } else {
         hipLaunchKernelGGL(k_gemm_f64s, grid, dim3(256), 0, st, N, K, kc, S, gs, lda, ldb, nk_w);
+    }
     hipLaunchKernelGGL(k_gemm_reduce_set<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, n, N, K, kc,
                        bm, gs, total);
     return hipGetLastError();
@@ -565,24 +662,10 @@ static GemmSetT<T> pair_set(int M, const T *A0, const T *A1, const T *B0, const 
     return gs;
 }
 
-// Slice length of a dense set: K / S for the power of two S <= 8 (slices of
-// at least 256) whose blocks, at two resident blocks per compute unit, fill
-// their last dispatch round best; ties go to the smaller S.
-int gemm_set_slice(int n, const int *M, int N, int K)
+// Slice length of a dense set (dense_slice over the set's block tiles).
+int gemm_set_slice(int mode, int n, const int *M, int N, int K)
 {
-    long tiles = 0;
-    for (int q = 0; q < n; q++) tiles += (long)((M[q] + kSBM - 1) / kSBM) * ((N + kSBN - 1) / kSBN);
-    const long slots = 2L * 256;
-    int best = K;
-    double best_eff = -1;
-    for (int S = 1; S <= 8; S *= 2) {
-        const int kc = ((K + S - 1) / S + kSBK - 1) / kSBK * kSBK;
-        if (S > 1 && kc < 256) break;
-        const long blocks = tiles * ((K + kc - 1) / kc);
-        const double eff = double(blocks) / double((blocks + slots - 1) / slots * slots);
-        if (eff > best_eff + 1e-9) best_eff = eff, best = kc;
-    }
-    return best;
+    return dense_slice(set_tiles(mode, n, M, N), set_slots(mode), K, mode == 1 ? 16 : 8, mode == 1 ? 2 : 1);
 }
 
 size_t gemm_set_workspace_bytes(int n, const int *M, int N, int K, int kc, size_t es)
@@ -598,7 +681,7 @@ hipError_t launch_gemm_pair_f64(int M, int N, int K, int kc, const double *A0, c
 {
     if (kc < kDBK || kc % 32 || lda % 2 || ldb % 2) return hipErrorInvalidValue;
     return launch_gemm_set_t<double>(2, pair_set(M, A0, A1, B0, B1, C0, C1, kend), N, K, kc, lda, ldb, P, nk_w,
-                                     kDBM, st);
+                                     st);
 }
 
 bool gemm_pair_ok(long lda, long ldb, const void *A0, const void *A1, const void *B0, const void *B1)
@@ -611,9 +694,8 @@ hipError_t launch_gemm_pair_f32(int M, int N, int K, int kc, const float *A0, co
                                 const float *B0, const float *B1, long ldb, float *C0, float *C1, float *P,
                                 const int *kend, const int *nk_w, hipStream_t st)
 {
-    if (kc < kSBK || kc % kSBK) return hipErrorInvalidValue;
-    return launch_gemm_set_t<float>(2, pair_set(M, A0, A1, B0, B1, C0, C1, kend), N, K, kc, lda, ldb, P, nk_w,
-                                    kSBM, st);
+    if (kc < kTBK || kc % kTBK) return hipErrorInvalidValue;
+    return launch_gemm_set_t<float>(2, pair_set(M, A0, A1, B0, B1, C0, C1, kend), N, K, kc, lda, ldb, P, nk_w, st);
 }
 
 hipError_t launch_gemm_set(int prec, int n, const int *M, const int *rpw, int N, int K, int kc, const void *const *A,
@@ -628,15 +710,15 @@ hipError_t launch_gemm_set(int prec, int n, const int *M, const int *rpw, int N,
             gs.A[q] = (const double *)A[q], gs.B[q] = (const double *)B[q], gs.C[q] = (double *)C[q];
             gs.kend[q] = kend ? kend[q] : nullptr, gs.M[q] = M[q], gs.rpw[q] = rpw[q];
         }
-        return launch_gemm_set_t<double>(n, gs, N, K, kc, lda, ldb, (double *)P, nk_w, kDBM, st);
+        return launch_gemm_set_t<double>(n, gs, N, K, kc, lda, ldb, (double *)P, nk_w, st);
     }
-    if (kc < kSBK || kc % kSBK) return hipErrorInvalidValue;
+    if (kc < kTBK || kc % kTBK) return hipErrorInvalidValue;
     GemmSet gs{};
     for (int q = 0; q < n; q++) {
         gs.A[q] = (const float *)A[q], gs.B[q] = (const float *)B[q], gs.C[q] = (float *)C[q];
         gs.kend[q] = kend ? kend[q] : nullptr, gs.M[q] = M[q], gs.rpw[q] = rpw[q];
     }
-    return launch_gemm_set_t<float>(n, gs, N, K, kc, lda, ldb, (float *)P, nk_w, kSBM, st);
+    return launch_gemm_set_t<float>(n, gs, N, K, kc, lda, ldb, (float *)P, nk_w, st);
 }
 
 size_t gemm_workspace_bytes(int mode, int M, int N, int K)

@@ -979,7 +979,7 @@ static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid
             wsb = std::max(wsb, grtm::gemm_workspace_bytes(p->gemm_mode, M[q], p->nr, p->nk));
         if (aligned) {
             p->pair = true;
-            p->kc = grtm::gemm_set_slice(p->nsets, M, p->nr, p->nk);
+            p->kc = grtm::gemm_set_slice(p->gemm_mode, p->nsets, M, p->nr, p->nk);
             wsb = std::max(wsb, grtm::gemm_set_workspace_bytes(p->nsets, M, p->nr, p->nk, p->kc, es));
         }
         if ((status = p->ws.alloc(wsb))) return status;
@@ -996,7 +996,7 @@ static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid
         size_t wsb = grtm::gemm_workspace_bytes(p->gemm_mode, (int)arows, p->nr, p->nk);
         if (p->gemm_mode != 2 && grtm::gemm_pair_ok(p->nk, p->nr, p->A0.p, p->A1.p, p->B0.p, p->B1.p)) {
             p->pair = true;
-            p->kc = grtm::gemm_pair_slice((int)arows, p->nr, p->nk, p->nk);
+            p->kc = grtm::gemm_pair_slice(p->gemm_mode, (int)arows, p->nr, p->nk, p->nk);
             wsb = std::max(wsb, grtm::gemm_pair_workspace_bytes((int)arows, p->nr, p->nk, p->kc) * es / 4);
         }
         if ((status = p->ws.alloc(wsb))) return status;
@@ -1185,8 +1185,8 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
     if (!kcut) {
         p->adaptive = false;
         if (p->pair)
-            p->kc = p->kind == 1 ? grtm::gemm_set_slice(p->nsets, Mq, p->nr, p->nk)
-                                 : grtm::gemm_pair_slice(4 * p->nw, p->nr, p->nk, p->nk);
+            p->kc = p->kind == 1 ? grtm::gemm_set_slice(p->gemm_mode, p->nsets, Mq, p->nr, p->nk)
+                                 : grtm::gemm_pair_slice(p->gemm_mode, 4 * p->nw, p->nr, p->nk, p->nk);
         return GRTM_OK;
     }
     if (!(window >= 0) || stride < 1 || !(tol > 0)) return GRTM_E_INPUT;
@@ -1218,7 +1218,7 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
         double ksum = 0, rows = 0;
         for (int iw = 0; iw < p->nw; iw++) ksum += std::min(p->nk, (p->nk_pred_h[iw] + 31) / 32 * 32);
         for (int q = 0; q < p->nsets; q++) rows += Mq[q];
-        const int kc = grtm::gemm_pair_slice((int)(rows / 2), p->nr, p->nk, ksum / p->nw);
+        const int kc = grtm::gemm_pair_slice(p->gemm_mode, (int)(rows / 2), p->nr, p->nk, ksum / p->nw);
         const size_t need = grtm::gemm_set_workspace_bytes(p->nsets, Mq, p->nr, p->nk, kc, esize(p->prec));
         if (need > p->ws.n && (st = p->ws.alloc(need))) return st;
         p->kc = kc;
@@ -1231,7 +1231,7 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
             for (int r = r0; r < std::min(p->nw, r0 + 32); r++) m = std::max(m, p->nk_pred_h[r]);
             ksum += std::min(p->nk, (m + 31) / 32 * 32);
         }
-        const int kc = grtm::gemm_pair_slice(4 * p->nw, p->nr, p->nk, ksum / std::max(1, nt));
+        const int kc = grtm::gemm_pair_slice(p->gemm_mode, 4 * p->nw, p->nr, p->nk, ksum / std::max(1, nt));
         const size_t need = grtm::gemm_pair_workspace_bytes(4 * p->nw, p->nr, p->nk, kc) * esize(p->prec) / 4;
         if (need > p->ws.n && (st = p->ws.alloc(need))) return st;
         p->kc = kc;
@@ -1339,7 +1339,7 @@ int grtm_selftest_gemm(int mode, int M, int N, int K, const double *A, const dou
             for (int k = 0; k < K; k++) a[(size_t)i * lda + k] = (float)A[(size_t)i * K + k];
         for (int k = 0; k < K; k++)
             for (int j = 0; j < N; j++) b[(size_t)k * ldb + j] = (float)B[(size_t)k * N + j];
-        const int kc = grtm::gemm_pair_slice(M, N, K, K);
+        const int kc = grtm::gemm_pair_slice(1, M, N, K, K);
         DBuf dA, dB, dC, dP;
         if ((st = dA.alloc(a.size() * 4)) || (st = dB.alloc(b.size() * 4)) || (st = dC.alloc((size_t)2 * M * N * 4)) ||
             (st = dP.alloc(grtm::gemm_pair_workspace_bytes(M, N, K, kc))))
@@ -1366,7 +1366,7 @@ int grtm_selftest_gemm(int mode, int M, int N, int K, const double *A, const dou
             for (int k = 0; k < K; k++) a[(size_t)i * lda + k] = A[(size_t)i * K + k];
         for (int k = 0; k < K; k++)
             for (int j = 0; j < N; j++) b[(size_t)k * ldb + j] = B[(size_t)k * N + j];
-        const int kc = grtm::gemm_pair_slice(M, N, K, K);
+        const int kc = grtm::gemm_pair_slice(0, M, N, K, K);
         DBuf dA, dB, dC, dP;
         if ((st = dA.alloc(a.size() * 8)) || (st = dB.alloc(b.size() * 8)) || (st = dC.alloc((size_t)2 * M * N * 8)) ||
             (st = dP.alloc(2 * grtm::gemm_pair_workspace_bytes(M, N, K, kc))))
@@ -1736,7 +1736,7 @@ int batch_autotune(grtm_batch *b)
     b->bs = best_bs;
     if (b->pair) {
         const int M = 4 * b->nw * b->groups[0].n;
-        const int k0 = grtm::gemm_pair_slice(M, b->nr, b->nk, b->nk);
+        const int k0 = grtm::gemm_pair_slice(b->gemm_mode, M, b->nr, b->nk, b->nk);
         int best_kc = k0;
         best = 1e30;
         for (int kc : {k0 / 2, k0, 2 * k0}) {
@@ -1872,7 +1872,7 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
     size_t wsb = grtm::gemm_workspace_bytes(b->gemm_mode, (int)arows, nr, b->nk);
     if (b->gemm_mode != 2 && grtm::gemm_pair_ok(b->nk, nr, b->A0.p, b->A1.p, b->B0.p, b->B1.p)) {
         b->pair = true;
-        b->kc = grtm::gemm_pair_slice((int)arows, nr, b->nk, b->nk);
+        b->kc = grtm::gemm_pair_slice(b->gemm_mode, (int)arows, nr, b->nk, b->nk);
         const int kmin = std::max(256, b->kc / 2);
         wsb = std::max(wsb, grtm::gemm_pair_workspace_bytes((int)arows, nr, b->nk, kmin) * es / 4);
     }

@@ -98,9 +98,10 @@ hipError_t launch_probe_f64(int l0, double *out, hipStream_t st);
 hipError_t launch_gemm_mfma(int mode, int M, int N, int K, const void *A, long lda, const void *B, long ldb,
                             void *Cm, void *P, const int *kend, const int *nk_w, hipStream_t st);
 // Both contractions C_p = A_p B_p (p = 0, 1) in FP32 on the matrix cores,
-// streamed kernel: K slices of kc (a multiple of 32; gemm_pair_slice picks
-// one), workspace P of gemm_pair_workspace_bytes, kend per 128-row tile.
-int gemm_pair_slice(int M, int N, int K, double k_avg);
+// streamed kernel: K slices of kc (a multiple of 32; gemm_pair_slice(1, ...)
+// picks one), workspace P of gemm_pair_workspace_bytes, kend per 128 operand
+// rows.  mode: 1 FP32, 0 FP64.
+int gemm_pair_slice(int mode, int M, int N, int K, double k_avg);
 size_t gemm_pair_workspace_bytes(int M, int N, int K, int kc);
 bool gemm_pair_ok(long lda, long ldb, const void *A0, const void *A1, const void *B0, const void *B1);
 hipError_t launch_gemm_pair_f32(int M, int N, int K, int kc, const float *A0, const float *A1, long lda,
@@ -116,7 +117,7 @@ hipError_t launch_gemm_pair_f64(int M, int N, int K, int kc, const double *A0, c
 // 1 FP32; the same alignment rules as the pairs; workspace P of
 // gemm_set_workspace_bytes; kend (optional) per contraction and 128-row tile.
 size_t gemm_set_workspace_bytes(int n, const int *M, int N, int K, int kc, size_t es);
-int gemm_set_slice(int n, const int *M, int N, int K);
+int gemm_set_slice(int mode, int n, const int *M, int N, int K);
 hipError_t launch_gemm_set(int prec, int n, const int *M, const int *rpw, int N, int K, int kc, const void *const *A,
                            long lda, const void *const *B, long ldb, void *const *C, void *P,
                            const int *const *kend, const int *nk_w, hipStream_t st);
