@@ -28,7 +28,7 @@ struct LayerW {
     using W = typename WideOf<T>::type;
     cx<W> sad, sbd;  // w/alpha, w/beta in FP64 (vertical wavenumbers are formed in FP64)
     cx<T> sa, sb;  // w/alpha, w/beta (complex slownesses times w)
-    cx<T> ga, gb;  // (w/alpha)^2, (w/beta)^2
+    cx<T> mg;      // mu (w/beta)^2
     cx<T> ca, cb;  // alpha/w, beta/w (normalisation of the E columns)
     cx<T> mu;      // shear modulus
     cx<T> a, b;    // alpha, beta
@@ -52,7 +52,7 @@ template <typename T>
 struct LQ {
     using W = typename WideOf<T>::type;
     cx<W> rd, vd;  // vertical wavenumbers in FP64 (exponent arguments)
-    cx<T> r, v, x, gb, mu, sa, sb, ca, cb;
+    cx<T> r, v, x, mg, mu, sa, sb, ca, cb;
     W h;
 };
 
@@ -106,7 +106,7 @@ GRTM_HD LQ<T> layer_q(const LayerW<T> &L, typename WideOf<T>::type kd)
     // k^2 + v^2 = 2 k^2 - (w/beta)^2, formed in FP64: it cancels near
     // k = w / (beta sqrt 2)
     q.x = narrow<T>(C<W>(W(2) * kd * kd, W(0)) - L.sbd * L.sbd);
-    q.gb = L.gb;
+    q.mg = L.mg;
     q.mu = L.mu;
     q.sa = L.sa;
     q.sb = L.sb;
@@ -124,6 +124,10 @@ GRTM_HD LQ<T> layer_q(const LayerW<T> &L, typename WideOf<T>::type kd)
 // X = P_a^-1 P_b, Y = N_a^-1 N_b and G = X (I + Q) + Y (I - Q)
 //   generalized reflection  R = (X (I + Q) - Y (I - Q)) G^-1,
 //   generalized transmission T = 2 G^-1.
+// X and Y are formed times D = mu_a (w/beta_a)^2 r_a v_a, which clears both
+// determinants (D / det Pt_a = -r_a, D / det Nt_a = v_a, up to the w^2 factor
+// Pt_b and Nt_b share): R is unchanged by the common factor, and
+// T = 2 D (D G)^-1.
 // These are the quantities green.cu:471-594 forms from the interface
 // coefficients (235-262, 339-387 and the cuBLAS 4x4 solve at 1813-1816):
 // downward incidence (up sweep, a above b) gives R_du and T_d, upward
@@ -134,26 +138,27 @@ GRTM_HD LQ<T> layer_q(const LayerW<T> &L, typename WideOf<T>::type kd)
 // Nt = [[r, k], [-2 mu k r, -mu x]].  Every generalized R and T of the
 // recursion is then the similarity transform D R D^-1 of the normalised one,
 // so the normalisation returns once, on the source vector (src_scaled).
-// det Pt = -mu v w^2/b^2 and det Nt = mu r w^2/b^2 in closed form; adj(Pt_a)
-// Pt_b and adj(Nt_a) Nt_b are written through dmu = mu_b - mu_a, so identical
-// layers give X = Y = I exactly (R = Q, T = I).
+// adj(Pt_a) Pt_b and adj(Nt_a) Nt_b are written through dmu = mu_b - mu_a
+// (mg = mu (w/beta)^2 from the frequency tables), so identical layers give
+// off-diagonal zeros exactly and D X = D Y = D I to rounding (R = Q, T = I).
+// Returns D X, D Y and 2 D.
 template <typename T>
-GRTM_HD void interface_xy(const LQ<T> &a, const LQ<T> &b, T k, m2<T> &X, m2<T> &Y)
+GRTM_HD void interface_xy(const LQ<T> &a, const LQ<T> &b, T k, m2<T> &X, m2<T> &Y, cx<T> &d2)
 {
     const cx<T> dmu = b.mu - a.mu;
-    const T k2 = k * k;
-    const cx<T> k2dmu2 = (T(2) * k2) * dmu;
-    const cx<T> mga = a.mu * a.gb, mgb = b.mu * b.gb;
-    const cx<T> ipa = recip(-(a.mu * a.v * a.gb));  // 1 / det Pt_a (up to w^2 scaling shared by Pt_b)
-    const cx<T> ina = recip(a.mu * a.r * a.gb);     // 1 / det Nt_a
-    X.a = ipa * (a.v * (k2dmu2 - mgb));
-    X.b = ipa * ((T(2) * k) * (a.v * b.v) * dmu);
-    X.c = ipa * (k * (mgb - mga - k2dmu2));
-    X.d = ipa * (b.v * (-mga - k2dmu2));
-    Y.a = ina * (b.r * (k2dmu2 + mga));
-    Y.b = ina * (k * (k2dmu2 - mgb + mga));
-    Y.c = ina * (-((T(2) * k) * (a.r * b.r) * dmu));
-    Y.d = ina * (a.r * (mgb - k2dmu2));
+    const cx<T> k2dmu2 = (T(2) * k * k) * dmu;
+    const cx<T> rv = a.r * a.v;
+    const cx<T> t1 = a.mg + k2dmu2, t2 = b.mg - k2dmu2, t3 = t1 - b.mg;
+    const cx<T> w = ((T(-2) * k) * dmu) * rv;
+    X.a = rv * t2;
+    X.b = w * b.v;
+    X.c = k * (a.r * t3);
+    X.d = (a.r * b.v) * t1;
+    Y.a = (a.v * b.r) * t1;
+    Y.b = k * (a.v * t3);
+    Y.c = w * b.r;
+    Y.d = X.a;
+    d2 = T(2) * (a.mg * rv);
 }
 
 template <typename T> GRTM_HD m2<T> i_plus(const m2<T> &Q)
@@ -165,23 +170,37 @@ template <typename T> GRTM_HD m2<T> i_minus(const m2<T> &Q)
     return {C<T>(T(1) - Q.a.x, -Q.a.y), -Q.b, -Q.c, C<T>(T(1) - Q.d.x, -Q.d.y)};
 }
 
-// One step of the recursion: R, T across an interface (X, Y of
+// R and T = d2 G^-1 from F = X (I + Q), H = Y (I - Q), G = F + H.
+template <typename T>
+GRTM_HD void grt_rt(const m2<T> &F, const m2<T> &H, cx<T> d2, m2<T> &R, m2<T> &Tr)
+{
+    const m2<T> G = F + H;
+    const cx<T> di = recip(G.a * G.d - G.b * G.c);
+    const m2<T> Gi = {G.d * di, -(G.b * di), -(G.c * di), G.a * di};
+    R = (F - H) * Gi;
+    const cx<T> dt = d2 * di;
+    Tr = {G.d * dt, -(G.b * dt), -(G.c * dt), G.a * dt};
+}
+// One step of the recursion: R, T across an interface (X, Y, d2 of
 // interface_xy) with b's side reflecting Q.
 template <typename T>
-GRTM_HD void grt_step(const m2<T> &X, const m2<T> &Y, const m2<T> &Q, m2<T> &R, m2<T> &Tr)
+GRTM_HD void grt_step(const m2<T> &X, const m2<T> &Y, cx<T> d2, const m2<T> &Q, m2<T> &R, m2<T> &Tr)
 {
-    const m2<T> F = X * i_plus(Q), H = Y * i_minus(Q);
-    const m2<T> Gi = minv(F + H);
-    R = (F - H) * Gi;
-    Tr = mscale(T(2), Gi);
+    grt_rt(X * i_plus(Q), Y * i_minus(Q), d2, R, Tr);
 }
 // The same with nothing reflecting on b's side (b the half-space).
 template <typename T>
-GRTM_HD void grt_step0(const m2<T> &X, const m2<T> &Y, m2<T> &R, m2<T> &Tr)
+GRTM_HD void grt_step0(const m2<T> &X, const m2<T> &Y, cx<T> d2, m2<T> &R, m2<T> &Tr)
 {
-    const m2<T> Gi = minv(X + Y);
-    R = (X - Y) * Gi;
-    Tr = mscale(T(2), Gi);
+    grt_rt(X, Y, d2, R, Tr);
+}
+// Lambda R Lambda with Lambda = diag(e0, e1); e11 = e1^2 for the SH term.
+template <typename T>
+GRTM_HD m2<T> lrl(cx<T> e0, cx<T> e1, const m2<T> &R, cx<T> &e11)
+{
+    const cx<T> e00 = e0 * e0, e01 = e0 * e1;
+    e11 = e1 * e1;
+    return {e00 * R.a, e01 * R.b, e01 * R.c, e11 * R.d};
 }
 // SH (E_SH = [[1, 1], [-mu v, mu v]]): impedances wa = mu_a v_a, wb = mu_b v_b,
 // q the generalized SH reflection of b's side.
@@ -287,9 +306,9 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
         LQ<T> qd = layer_q(lw[L - 1], kd);  // layer below the interface
         LQ<T> qu = layer_q(lw[L - 2], kd);
         m2<T> X, Y, Rg, Td;
-        cx<T> rg0, td0;
-        interface_xy(qu, qd, k, X, Y);
-        grt_step0(X, Y, Rg, Td);
+        cx<T> rg0, td0, d2;
+        interface_xy(qu, qd, k, X, Y, d2);
+        grt_step0(X, Y, d2, Rg, Td);
         grt_step_sh(qu.v * qu.mu, qd.v * qd.mu, C<T>(0, 0), rg0, td0);
         // grt[L-1] = (Rg, Td)
         int j = L - 1;  // index of the grt entry just formed
@@ -339,9 +358,11 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             qu = layer_q(lw[lay], kd);
             // green.cu:532-594: generalized R_du, T_d across interface lay with
             // the stack below reflecting Lambda R Lambda
-            interface_xy(qu, qd, k, X, Y);
-            grt_step(X, Y, dmd(e0, e1, Rg, e0, e1), Rg, Td);
-            grt_step_sh(qu.v * qu.mu, qd.v * qd.mu, e1 * rg0 * e1, rg0, td0);
+            cx<T> e11;
+            const m2<T> Q = lrl(e0, e1, Rg, e11);
+            interface_xy(qu, qd, k, X, Y, d2);
+            grt_step(X, Y, d2, Q, Rg, Td);
+            grt_step_sh(qu.v * qu.mu, qd.v * qd.mu, e11 * rg0, rg0, td0);
             visit(lay + 1, Rg, rg0, Td, td0, e0, e1);
         }
     }
@@ -367,10 +388,11 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             // (incidence from below) with the stack above reflecting
             // Lambda R Lambda
             m2<T> X, Y, Tu;
-            cx<T> tu0;
-            interface_xy(qd, qu, k, X, Y);
-            grt_step(X, Y, dmd(e0, e1, Rg, e0, e1), Rg, Tu);
-            grt_step_sh(qd.v * qd.mu, qu.v * qu.mu, e1 * rg0 * e1, rg0, tu0);
+            cx<T> tu0, d2, e11;
+            const m2<T> Q = lrl(e0, e1, Rg, e11);
+            interface_xy(qd, qu, k, X, Y, d2);
+            grt_step(X, Y, d2, Q, Rg, Tu);
+            grt_step_sh(qd.v * qd.mu, qu.v * qu.mu, e11 * rg0, rg0, tu0);
             const int jn = lay + 1;  // grt index formed
             if (branch == 1) {
                 if (jn == rl) {

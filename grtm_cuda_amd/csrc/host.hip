@@ -211,8 +211,7 @@ void layer_w(const Stack &st, int compat, cd w, LayerW<double> *out)
         q.sbd = C(sb);
         q.sa = C(sa);
         q.sb = C(sb);
-        q.ga = C(sa * sa);
-        q.gb = C(sb * sb);
+        q.mg = C(mu * sb * sb);
         q.ca = C(a / w);
         q.cb = C(b / w);
         q.mu = C(mu);
@@ -531,7 +530,7 @@ std::vector<LayerW<float>> narrow_tables(const std::vector<LayerW<double>> &lw)
     auto F = [](grtm::cx<double> z) { return grtm::cx<float>{float(z.x), float(z.y)}; };
     for (size_t i = 0; i < lw.size(); i++) {
         const LayerW<double> &d = lw[i];
-        lf[i] = LayerW<float>{d.sad, d.sbd, F(d.sa), F(d.sb), F(d.ga), F(d.gb), F(d.ca), F(d.cb), F(d.mu),
+        lf[i] = LayerW<float>{d.sad, d.sbd, F(d.sa), F(d.sb), F(d.mg), F(d.ca), F(d.cb), F(d.mu),
                               F(d.a), F(d.b), d.h, 0.0};
     }
     return lf;
@@ -1453,8 +1452,7 @@ int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi,
         lw[j].sbd = C(w / b);
         lw[j].sa = C(w / a);
         lw[j].sb = C(w / b);
-        lw[j].ga = C((w / a) * (w / a));
-        lw[j].gb = C((w / b) * (w / b));
+        lw[j].mg = C(mu * (w / b) * (w / b));
         lw[j].ca = C(a / w);
         lw[j].cb = C(b / w);
         lw[j].mu = C(mu);

@@ -75,8 +75,7 @@ static int count_cell(int kind, int L, int s, int rl, int branch, int up, double
         q.sbd = cx<Cnt>{Cnt(w_re / b), Cnt(w_im / b)};
         q.sa = cx<Cnt>{Cnt(w_re / a), Cnt(w_im / a)};
         q.sb = cx<Cnt>{Cnt(w_re / b), Cnt(w_im / b)};
-        q.ga = cx<Cnt>{Cnt((w_re * w_re - w_im * w_im) / (a * a)), Cnt(2 * w_re * w_im / (a * a))};
-        q.gb = cx<Cnt>{Cnt((w_re * w_re - w_im * w_im) / (b * b)), Cnt(2 * w_re * w_im / (b * b))};
+        q.mg = cx<Cnt>{Cnt(2.7 * (w_re * w_re - w_im * w_im)), Cnt(2.7 * 2 * w_re * w_im)};
         q.ca = cx<Cnt>{Cnt(a / w_re), Cnt(0)};
         q.cb = cx<Cnt>{Cnt(b / w_re), Cnt(0)};
         q.mu = cx<Cnt>{Cnt(2.7 * b * b), Cnt(0)};
