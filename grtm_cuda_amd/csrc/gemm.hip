@@ -180,15 +180,158 @@ __device__ __forceinline__ int tile_extent(const int *__restrict__ kend, int M, 
 // (16-byte loads, the entries past the cut replaced by zeros).  Requires lda
 // and ldb multiples of 4, N <= ldb, 16-byte aligned A and B.
 template <int BM, int BN, int WM, int WN>
+struct F32Tile {
+    static constexpr int NT = (BM / WM) * (BN / WN) * 64, BK = 32, PA = BK + 4, PB = BN + 4;
+    static constexpr int TM = WM / 32, TN = WN / 32, AE = BM * BK / 4 / NT, BE = BK * BN / 4 / NT, BNV = BN / 4;
+    static_assert(AE * NT * 4 == BM * BK && BE * NT * 4 == BK * BN, "staging must cover the slab");
+    struct Smem {
+        float As[2][BM][PA];
+        float Bs[2][BK][PB];
+    };
+
+    // acc = A[m0 : m0 + BM, kb : ke] B[kb : ke, n0 : n0 + BN], A row r read up
+    // to nk_w[r / rpw] (all of [kb, ke) without nk_w), rows at or past M zero.
+    // Ends with a barrier, so the shared buffers may be refilled at once.
+    static __device__ __forceinline__ void mma(const float *__restrict__ A, const float *__restrict__ B, long lda,
+                                               long ldb, int M, int N, int m0, int n0, int kb, int ke, int rpw,
+                                               const int *__restrict__ nk_w, Smem &sm, v16f (&acc)[TM][TN])
+    {
+        const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+        const int wm = wv / (BN / WN), wn = wv % (BN / WN);
+        // staging: A item idx = tid + NT e -> row idx >> 3, k offset (idx & 7) * 4;
+        // B item -> k row idx / BNV, column offset (idx % BNV) * 4
+        int a_lim[AE];
+#pragma unroll
+        for (int e = 0; e < AE; e++) {
+            const int gm = m0 + ((tid + NT * e) >> 3);
+            a_lim[e] = gm < M ? (nk_w ? min(ke, nk_w[gm / rpw]) : ke) : 0;
+        }
+        float4 ra[AE], rb[BE];
+        auto load = [&](int k0) {
+#pragma unroll
+            for (int e = 0; e < AE; e++) {
+                const int idx = tid + NT * e, gk = k0 + (idx & 7) * 4;
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (gk < a_lim[e]) {
+                    // gk + 3 < lda: gk and lda are multiples of 4 and gk < K <= lda
+                    v = *reinterpret_cast<const float4 *>(A + (size_t)(m0 + (idx >> 3)) * lda + gk);
+                    const int lim = a_lim[e] - gk;
+                    if (lim < 4) {
+                        v.w = 0.f;
+                        if (lim < 3) v.z = 0.f;
+                        if (lim < 2) v.y = 0.f;
+                    }
+                }
+                ra[e] = v;
+            }
+#pragma unroll
+            for (int e = 0; e < BE; e++) {
+                const int idx = tid + NT * e, gk = k0 + idx / BNV, gn = n0 + (idx % BNV) * 4;
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                // gn + 3 < ldb: gn and ldb are multiples of 4 and gn < N <= ldb;
+                // columns at or past N only reach accumulators that are not written
+                if (gk < ke && gn < N) v = *reinterpret_cast<const float4 *>(B + (size_t)gk * ldb + gn);
+                rb[e] = v;
+            }
+        };
+        auto store = [&](int buf) {
+#pragma unroll
+            for (int e = 0; e < AE; e++) {
+                const int idx = tid + NT * e;
+                *reinterpret_cast<float4 *>(&sm.As[buf][idx >> 3][(idx & 7) * 4]) = ra[e];
+            }
+#pragma unroll
+            for (int e = 0; e < BE; e++) {
+                const int idx = tid + NT * e;
+                *reinterpret_cast<float4 *>(&sm.Bs[buf][idx / BNV][(idx % BNV) * 4]) = rb[e];
+            }
+        };
+#pragma unroll
+        for (int a = 0; a < TM; a++)
+#pragma unroll
+            for (int b = 0; b < TN; b++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[a][b][r] = 0.f;
+        load(kb);
+        store(0);
+        __syncthreads();
+        const int i = lane & 31, kq = lane >> 5;
+        int buf = 0;
+        for (int k0 = kb; k0 < ke; k0 += BK) {
+            const bool more = k0 + BK < ke;
+            if (more) load(k0 + BK);
+            float4 fa[2][TM];
+            float fb[2][4][TN];
+            auto frag = [&](int q, int s) {
+#pragma unroll
+                for (int t = 0; t < TM; t++)
+                    fa[s][t] = *reinterpret_cast<const float4 *>(&sm.As[buf][wm * WM + t * 32 + i][16 * kq + 4 * q]);
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+#pragma unroll
+                    for (int t = 0; t < TN; t++) fb[s][j][t] = sm.Bs[buf][16 * kq + 4 * q + j][wn * WN + t * 32 + i];
+            };
+            frag(0, 0);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int s = q & 1;
+                if (q < 3) frag(q + 1, s ^ 1);
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+#pragma unroll
+                    for (int tm = 0; tm < TM; tm++) {
+                        const float a = j == 0 ? fa[s][tm].x : j == 1 ? fa[s][tm].y : j == 2 ? fa[s][tm].z : fa[s][tm].w;
+#pragma unroll
+                        for (int tn = 0; tn < TN; tn++)
+                            acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, fb[s][j][tn], acc[tm][tn], 0, 0, 0);
+                    }
+            }
+            if (more) store(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+
+    // dst[r][c] = acc for the tile's rows r < nrow and columns c < ncol
+    // (tile-local indices), row stride ld.
+    static __device__ __forceinline__ void put(const v16f (&acc)[TM][TN], float *__restrict__ dst, long ld, int nrow,
+                                               int ncol)
+    {
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int wm = wv / (BN / WN), wn = wv % (BN / WN);
+#pragma unroll
+        for (int tm = 0; tm < TM; tm++)
+#pragma unroll
+            for (int tn = 0; tn < TN; tn++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int row = wm * WM + tm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    const int col = wn * WN + tn * 32 + (lane & 31);
+                    if (row < nrow && col < ncol) dst[(size_t)row * ld + col] = acc[tm][tn][r];
+                }
+    }
+};
+
+// Block tile BM x BN of 8 waves, wave tile WM x WN (WM/32 x WN/32 MFMA
+// 32x32x2 tiles), K slab 32.  The next slab's 16-byte global loads are in
+// flight while the matrix cores work on the current one (two LDS buffers),
+// and within a slab the operands of the next four k steps are read from LDS
+// while the current four are multiplied.  Lanes 0-31 of every MFMA take k in
+// [0, 16) of the slab and lanes 32-63 k in [16, 32), so a lane's A operands
+// for four steps are one 16-byte LDS read.  Split-K: slice z covers
+// [z kc, (z+1) kc); a slice at or past the block's K extent returns without
+// writing (k_gemm_reduce_set sums only the live slices).  The slice index is
+// the outermost grid dimension, so blocks running together read the same K
+// range of B.  A row r is read up to its frequency row's cut nk_w[r / rpw]
+// (16-byte loads, the entries past the cut replaced by zeros).  Requires lda
+// and ldb multiples of 4, N <= ldb, 16-byte aligned A and B.
+template <int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f32t(int N, int K, int kc, int S, GemmSet gp,
                                                                        long lda, long ldb,
                                                                        const int *__restrict__ nk_w)
 {
-    constexpr int NT = (BM / WM) * (BN / WN) * 64, BK = 32, PA = BK + 4, PB = BN + 4;
-    constexpr int TM = WM / 32, TN = WN / 32, AE = BM * BK / 4 / NT, BE = BK * BN / 4 / NT, BNV = BN / 4;
-    static_assert(AE * NT * 4 == BM * BK && BE * NT * 4 == BK * BN, "staging must cover the slab");
-    __shared__ __attribute__((aligned(16))) float As[2][BM][PA];
-    __shared__ __attribute__((aligned(16))) float Bs[2][BK][PB];
+    using Tl = F32Tile<BM, BN, WM, WN>;
+    __shared__ __attribute__((aligned(16))) typename Tl::Smem sm;
     const int pi = blockIdx.z / S, z = blockIdx.z - pi * S;
     const int M = gp.M[pi];
     const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
@@ -196,118 +339,133 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f32t(int N,
     const int kt = tile_extent(gp.kend[pi], M, K, m0, BM);
     const int kb = z * kc;
     if (kb >= kt) return;  // uniform: this slice is past the block's extent
-    const int ke = min(kt, kb + kc);
-    const float *__restrict__ A = gp.A[pi];
-    const float *__restrict__ B = gp.B[pi];
-    const int rpw = gp.rpw[pi];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int wm = wv / (BN / WN), wn = wv % (BN / WN);
+    v16f acc[Tl::TM][Tl::TN];
+    Tl::mma(gp.A[pi], gp.B[pi], lda, ldb, M, N, m0, n0, kb, min(kt, kb + kc), gp.rpw[pi], nk_w, sm, acc);
+    Tl::put(acc, gp.P[pi] + (size_t)z * M * N + (size_t)m0 * N + n0, N, M - m0, N - n0);
+}
 
-    // staging: A item idx = tid + NT e -> row idx >> 3, k offset (idx & 7) * 4;
-    // B item -> k row idx / BNV, column offset (idx % BNV) * 4
-    int a_lim[AE];
-    #pragma unroll
-    for (int e = 0; e < AE; e++) {
-        const int gm = m0 + ((tid + NT * e) >> 3);
-        a_lim[e] = gm < M ? (nk_w ? min(ke, nk_w[gm / rpw]) : ke) : 0;
+// ------------------------------------------------------------------ FP32, stream-K
+// The ragged (learned-grid) contraction set with the K work balanced over a
+// fixed number of persistent blocks.  The block tiles of all contractions,
+// in order (contraction, row tile, column tile), have n_t = ceil(kt / 32)
+// K slabs each (kt the tile's extent); off[t] is their exclusive prefix sum,
+// G = off[T] the total.  Block b of nb' = min(nb, G) takes the slabs
+// [b G / nb', (b + 1) G / nb') (at least one each): a tile it covers whole is
+// written to C directly, a tile it covers in part leaves a partial in its
+// slot 0 (its first tile) or slot 1 (its last tile), which k_streamk_reduce
+// sums in block order.
+struct StreamK {
+    int cum[kGemmSetMax + 1];  // first tile of each contraction; cum[n] = T
+    int ncol;                  // column tiles per row tile
+    int nb;                    // persistent blocks
+};
+
+__device__ __forceinline__ int sk_block_of(long g, long G, int nb) { return int(((g + 1) * nb - 1) / G); }
+
+// off[0 .. T]: exclusive prefix sum of the tiles' slab counts; one block.
+template <int BM>
+__global__ __launch_bounds__(1024) void k_streamk_plan(int n, int K, GemmSet gp, StreamK sk, int *__restrict__ off)
+{
+    __shared__ int part[1024];
+    const int T = sk.cum[n], tid = threadIdx.x;
+    const int per = (T + 1023) / 1024, t0 = tid * per;
+    int sum = 0;
+    for (int t = t0; t < min(T, t0 + per); t++) {
+        int p = 0;
+        while (p + 1 < n && t >= sk.cum[p + 1]) p++;
+        const int m0 = (t - sk.cum[p]) / sk.ncol * BM;
+        sum += (tile_extent(gp.kend[p], gp.M[p], K, m0, BM) + 31) / 32;
     }
-    float4 ra[AE], rb[BE];
-    auto load = [&](int k0) {
-        #pragma unroll
-        for (int e = 0; e < AE; e++) {
-            const int idx = tid + NT * e, gk = k0 + (idx & 7) * 4;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (gk < a_lim[e]) {
-                // gk + 3 < lda: gk and lda are multiples of 4 and gk < K <= lda
-                v = *reinterpret_cast<const float4 *>(A + (size_t)(m0 + (idx >> 3)) * lda + gk);
-                const int lim = a_lim[e] - gk;
-                if (lim < 4) {
-                    v.w = 0.f;
-                    if (lim < 3) v.z = 0.f;
-                    if (lim < 2) v.y = 0.f;
-                }
-            }
-            ra[e] = v;
-        }
-        #pragma unroll
-        for (int e = 0; e < BE; e++) {
-            const int idx = tid + NT * e, gk = k0 + idx / BNV, gn = n0 + (idx % BNV) * 4;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            // gn + 3 < ldb: gn and ldb are multiples of 4 and gn < N <= ldb;
-            // columns at or past N only reach accumulators that are not written
-            if (gk < ke && gn < N) v = *reinterpret_cast<const float4 *>(B + (size_t)gk * ldb + gn);
-            rb[e] = v;
-        }
-    };
-    auto store = [&](int buf) {
-        #pragma unroll
-        for (int e = 0; e < AE; e++) {
-            const int idx = tid + NT * e;
-            *reinterpret_cast<float4 *>(&As[buf][idx >> 3][(idx & 7) * 4]) = ra[e];
-        }
-        #pragma unroll
-        for (int e = 0; e < BE; e++) {
-            const int idx = tid + NT * e;
-            *reinterpret_cast<float4 *>(&Bs[buf][idx / BNV][(idx % BNV) * 4]) = rb[e];
-        }
-    };
-
-    v16f acc[TM][TN];
-    #pragma unroll
-    for (int a = 0; a < TM; a++)
-        #pragma unroll
-        for (int b = 0; b < TN; b++)
-            #pragma unroll
-            for (int r = 0; r < 16; r++) acc[a][b][r] = 0.f;
-    load(kb);
-    store(0);
+    part[tid] = sum;
     __syncthreads();
-    const int i = lane & 31, kq = lane >> 5;
-    int buf = 0;
-    for (int k0 = kb; k0 < ke; k0 += BK) {
-        const bool more = k0 + BK < ke;
-        if (more) load(k0 + BK);
-        float4 fa[2][TM];
-        float fb[2][4][TN];
-        auto frag = [&](int q, int s) {
-            #pragma unroll
-            for (int t = 0; t < TM; t++)
-                fa[s][t] = *reinterpret_cast<const float4 *>(&As[buf][wm * WM + t * 32 + i][16 * kq + 4 * q]);
-            #pragma unroll
-            for (int j = 0; j < 4; j++)
-                #pragma unroll
-                for (int t = 0; t < TN; t++) fb[s][j][t] = Bs[buf][16 * kq + 4 * q + j][wn * WN + t * 32 + i];
-        };
-        frag(0, 0);
-        #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int s = q & 1;
-            if (q < 3) frag(q + 1, s ^ 1);
-            #pragma unroll
-            for (int j = 0; j < 4; j++)
-                #pragma unroll
-                for (int tm = 0; tm < TM; tm++) {
-                    const float a = j == 0 ? fa[s][tm].x : j == 1 ? fa[s][tm].y : j == 2 ? fa[s][tm].z : fa[s][tm].w;
-                    #pragma unroll
-                    for (int tn = 0; tn < TN; tn++)
-                        acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, fb[s][j][tn], acc[tm][tn], 0, 0, 0);
-                }
-        }
-        if (more) store(buf ^ 1);
+    for (int d = 1; d < 1024; d *= 2) {
+        const int v = tid >= d ? part[tid - d] : 0;
         __syncthreads();
-        buf ^= 1;
+        part[tid] += v;
+        __syncthreads();
     }
-    float *Pz = gp.P[pi] + (size_t)z * M * N;
-    #pragma unroll
-    for (int tm = 0; tm < TM; tm++)
-        #pragma unroll
-        for (int tn = 0; tn < TN; tn++)
-            #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const int row = m0 + wm * WM + tm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                const int col = n0 + wn * WN + tn * 32 + (lane & 31);
-                if (row < M && col < N) Pz[(size_t)row * N + col] = acc[tm][tn][r];
-            }
+    int run = tid ? part[tid - 1] : 0;
+    for (int t = t0; t < min(T, t0 + per); t++) {
+        off[t] = run;
+        int p = 0;
+        while (p + 1 < n && t >= sk.cum[p + 1]) p++;
+        const int m0 = (t - sk.cum[p]) / sk.ncol * BM;
+        run += (tile_extent(gp.kend[p], gp.M[p], K, m0, BM) + 31) / 32;
+    }
+    if (tid == 1023) off[T] = part[1023];
+}
+
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f32sk(int n, int N, int K, GemmSet gp, StreamK sk,
+                                                                        long lda, long ldb,
+                                                                        const int *__restrict__ nk_w,
+                                                                        const int *__restrict__ off,
+                                                                        float *__restrict__ part)
+{
+    using Tl = F32Tile<BM, BN, WM, WN>;
+    __shared__ __attribute__((aligned(16))) typename Tl::Smem sm;
+    const int T = sk.cum[n];
+    const long G = off[T];
+    const int b = blockIdx.x, nbe = int(min((long)sk.nb, G));
+    if (b >= nbe) return;  // uniform
+    long lo = G * b / nbe;
+    const long hi = G * (b + 1) / nbe, lo_b = lo;
+    // the tile holding slab lo: the last t with off[t] <= lo
+    int t = 0;
+    for (int step = 1 << 30; step; step >>= 1)
+        if (t + step <= T && off[t + step] <= lo) t += step;
+    while (lo < hi) {
+        while (off[t + 1] <= lo) t++;  // tiles without slabs
+        const long t0 = off[t], t1 = off[t + 1];
+        const long seg = min(hi, t1);
+        int p = 0;
+        while (p + 1 < n && t >= sk.cum[p + 1]) p++;
+        const int M = gp.M[p];
+        const int m0 = (t - sk.cum[p]) / sk.ncol * BM, n0 = (t - sk.cum[p]) % sk.ncol * BN;
+        const int kt = tile_extent(gp.kend[p], M, K, m0, BM);
+        v16f acc[Tl::TM][Tl::TN];
+        Tl::mma(gp.A[p], gp.B[p], lda, ldb, M, N, m0, n0, int(lo - t0) * 32, min(kt, int(seg - t0) * 32), gp.rpw[p],
+                nk_w, sm, acc);
+        const bool whole = lo == t0 && seg == t1;
+        float *dst = whole ? gp.C[p] + (size_t)m0 * N + n0 : part + ((size_t)b * 2 + (lo == lo_b ? 0 : 1)) * BM * BN;
+        Tl::put(acc, dst, whole ? N : BN, whole ? M - m0 : BM, whole ? N - n0 : BN);
+        lo = seg;
+        t++;
+    }
+}
+
+// C of every tile split between blocks: the sum, in FP64 and block order, of
+// its partials; C of a tile without slabs: zero.  Thread e covers element e
+// of the concatenated outputs; tiles written whole are left alone.
+template <int BM, int BN>
+__global__ void k_streamk_reduce(int n, int N, GemmSet gs, StreamK sk, const int *__restrict__ off,
+                                 const float *__restrict__ part, long total)
+{
+    const long e0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e0 >= total) return;
+    int p = 0;
+    long e = e0;
+    while (p + 1 < n && e >= (long)gs.M[p] * N) {
+        e -= (long)gs.M[p] * N;
+        p++;
+    }
+    const int row = int(e / N), col = int(e % N);
+    const int t = sk.cum[p] + row / BM * sk.ncol + col / BN;
+    const long G = off[sk.cum[n]], t0 = off[t], t1 = off[t + 1];
+    if (t1 == t0) {
+        gs.C[p][e] = 0.f;
+        return;
+    }
+    const int nbe = int(min((long)sk.nb, G));
+    const int b0 = sk_block_of(t0, G, nbe), b1 = sk_block_of(t1 - 1, G, nbe);
+    if (b0 == b1) return;  // written whole by block b0
+    const size_t loc = (size_t)(row % BM) * BN + col % BN;
+    double s = 0;
+    for (int b = b0; b <= b1; b++) {
+        const int slot = (b == b0 && G * b / nbe < t0) ? 1 : 0;
+        s += double(part[((size_t)b * 2 + slot) * BM * BN + loc]);
+    }
+    gs.C[p][e] = float(s);
 }
 
 // FP32 block tile for N columns: 128 rows by 128 columns (waves of 32 x 64),
@@ -616,13 +774,66 @@ size_t gemm_pair_workspace_bytes(int M, int N, int K, int kc)
     return 2 * S * (size_t)M * N * sizeof(float);
 }
 
+// Stream-K blocks: two resident per compute unit, at most one per K slab.
+constexpr int kStreamKBlocks = 2 * 256;
+static int sk_blocks(long tiles, int K) { return (int)std::min<long>(kStreamKBlocks, tiles * ((K + kTBK - 1) / kTBK)); }
+
+size_t gemm_streamk_workspace_bytes(int n, const int *M, int N, int K)
+{
+    const long tiles = set_tiles(1, n, M, N);
+    return (size_t)sk_blocks(tiles, K) * 2 * kTBM * f32_bn(N) * sizeof(float) + (size_t)(tiles + 1) * sizeof(int);
+}
+
+// FP32 set with per-tile K extents: stream-K (k_streamk_plan, k_gemm_f32sk,
+// k_streamk_reduce) in a workspace of gemm_streamk_workspace_bytes.
+static hipError_t launch_streamk(int n, const GemmSet &gs, int N, int K, long lda, long ldb, float *P,
+                                 const int *nk_w, hipStream_t st)
+{
+    const int bn = f32_bn(N);
+    StreamK sk{};
+    int t = 0;
+    long total = 0;
+    for (int q = 0; q < kGemmSetMax; q++) {
+        sk.cum[q] = t;
+        if (q < n) t += (gs.M[q] + kTBM - 1) / kTBM * ((N + bn - 1) / bn), total += (long)gs.M[q] * N;
+    }
+    sk.cum[kGemmSetMax] = t;
+    sk.ncol = (N + bn - 1) / bn;
+    sk.nb = sk_blocks(t, K);
+    float *part = P;
+    int *off = reinterpret_cast<int *>(P + (size_t)sk.nb * 2 * kTBM * bn);
+    hipLaunchKernelGGL(k_streamk_plan<kTBM>, dim3(1), dim3(1024), 0, st, n, K, gs, sk, off);
+    if (bn == 128) {
+        hipLaunchKernelGGL((k_gemm_f32sk<kTBM, 128, 32, 64>), dim3(sk.nb), dim3(512), 0, st, n, N, K, gs, sk, lda,
+                           ldb, nk_w, off, part);
+        hipLaunchKernelGGL((k_streamk_reduce<kTBM, 128>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, n,
+                           N, gs, sk, off, part, total);
+    } else {
+        hipLaunchKernelGGL((k_gemm_f32sk<kTBM, 64, 32, 32>), dim3(sk.nb), dim3(512), 0, st, n, N, K, gs, sk, lda, ldb,
+                           nk_w, off, part);
+        hipLaunchKernelGGL((k_streamk_reduce<kTBM, 64>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, n, N,
+                           gs, sk, off, part, total);
+    }
+    return hipGetLastError();
+}
+
 // Fills the partial pointers of a set from one workspace and launches the
-// streamed kernel and the reduce.
+// streamed kernel and the reduce; an FP32 set with per-tile K extents runs
+// stream-K instead (kc unused).
 template <typename T>
 static hipError_t launch_gemm_set_t(int n, GemmSetT<T> gs, int N, int K, int kc, long lda, long ldb, T *P,
                                     const int *nk_w, hipStream_t st)
 {
     if (n < 1 || n > kGemmSetMax) return hipErrorInvalidValue;
+    if constexpr (sizeof(T) == 4) {
+        bool ragged = false;
+        for (int q = 0; q < n; q++) ragged = ragged || gs.kend[q];
+        if (ragged) {
+            for (int q = 0; q < n; q++)
+                if (gs.M[q] < 1 || gs.rpw[q] < 1) return hipErrorInvalidValue;
+            return launch_streamk(n, gs, N, K, lda, ldb, P, nk_w, st);
+        }
+    }
     const int bm = sizeof(T) == 4 ? kTBM : kDBM, bn = sizeof(T) == 4 ? f32_bn(N) : kDBN;
     const int S = (K + kc - 1) / kc;
     int mt = 0;

@@ -1218,7 +1218,8 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
         for (int iw = 0; iw < p->nw; iw++) ksum += std::min(p->nk, (p->nk_pred_h[iw] + 31) / 32 * 32);
         for (int q = 0; q < p->nsets; q++) rows += Mq[q];
         const int kc = grtm::gemm_pair_slice(p->gemm_mode, (int)(rows / 2), p->nr, p->nk, ksum / p->nw);
-        const size_t need = grtm::gemm_set_workspace_bytes(p->nsets, Mq, p->nr, p->nk, kc, esize(p->prec));
+        size_t need = grtm::gemm_set_workspace_bytes(p->nsets, Mq, p->nr, p->nk, kc, esize(p->prec));
+        if (p->gemm_mode == 1) need = std::max(need, grtm::gemm_streamk_workspace_bytes(p->nsets, Mq, p->nr, p->nk));
         if (need > p->ws.n && (st = p->ws.alloc(need))) return st;
         p->kc = kc;
     } else if (p->pair) {
@@ -1231,7 +1232,11 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
             ksum += std::min(p->nk, (m + 31) / 32 * 32);
         }
         const int kc = grtm::gemm_pair_slice(p->gemm_mode, 4 * p->nw, p->nr, p->nk, ksum / std::max(1, nt));
-        const size_t need = grtm::gemm_pair_workspace_bytes(4 * p->nw, p->nr, p->nk, kc) * esize(p->prec) / 4;
+        size_t need = grtm::gemm_pair_workspace_bytes(4 * p->nw, p->nr, p->nk, kc) * esize(p->prec) / 4;
+        if (p->gemm_mode == 1) {
+            const int M2[2] = {4 * p->nw, 4 * p->nw};
+            need = std::max(need, grtm::gemm_streamk_workspace_bytes(2, M2, p->nr, p->nk));
+        }
         if (need > p->ws.n && (st = p->ws.alloc(need))) return st;
         p->kc = kc;
     }
@@ -1927,6 +1932,14 @@ int grtm_batch_set_adaptive(grtm_batch *b, const double *kcut, const double *win
         }
         HIPCHK(hipMemcpy(b->nk_pred.p, b->nk_pred_h.data(), sizeof(int) * n, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(b->nchk.p, b->nchk_h.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+        if (b->pair && b->gemm_mode == 1) {
+            // the FP32 contraction of a learned grid runs stream-K
+            int gmax = 0;
+            for (const auto &gr : b->groups) gmax = std::max(gmax, gr.n);
+            const int M2[2] = {4 * b->nw * gmax, 4 * b->nw * gmax};
+            const size_t need = grtm::gemm_streamk_workspace_bytes(2, M2, b->nr, b->nk);
+            if (need > b->ws.n && (st = b->ws.alloc(need))) return st;
+        }
         b->chk_stride = stride;
         b->chk_tol = tol;
         b->adaptive = true;

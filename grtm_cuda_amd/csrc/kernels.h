@@ -117,6 +117,9 @@ hipError_t launch_gemm_pair_f64(int M, int N, int K, int kc, const double *A0, c
 // 1 FP32; the same alignment rules as the pairs; workspace P of
 // gemm_set_workspace_bytes; kend (optional) per contraction and 128-row tile.
 size_t gemm_set_workspace_bytes(int n, const int *M, int N, int K, int kc, size_t es);
+// FP32 sets and pairs with kend run stream-K: workspace of at least this size
+// (the pair: n = 2, M = {M, M}).
+size_t gemm_streamk_workspace_bytes(int n, const int *M, int N, int K);
 int gemm_set_slice(int mode, int n, const int *M, int N, int K);
 hipError_t launch_gemm_set(int prec, int n, const int *M, const int *rpw, int N, int K, int kc, const void *const *A,
                            long lda, const void *const *B, long ldb, void *const *C, void *P,
