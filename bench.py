@@ -116,9 +116,10 @@ def bench_batch(args, g, models):
     work = plan.work()
     tm = plan.timing(reps=1)
     peak = FP64_PEAK_TFLOPS if prec == g.PREC_FP64 else FP32_PEAK_TFLOPS
-    # counted recursion flops of the cells actually solved (adaptive: the kept fraction)
-    frac = extra.get("cells_frac", 1.0) + extra.get("check_frac", 0.0)
-    achieved = frac * work["rec_flop"] / (tm["cells"] * 1e-3) / 1e12
+    # counted recursion flops of the cells actually solved, each with its upward
+    # sweep as cut by depth (adaptive: the predicted cells, plus the check cells)
+    frac = 1.0 + extra.get("check_frac", 0.0) / extra["cells_frac"] if extra else 1.0
+    achieved = frac * work["rec_flop_solved"] / (tm["cells"] * 1e-3) / 1e12
     layers = [len(l) for l, _ in cfg["models"]]
     return {
         "metric": "Green's traces/s at <=1e-4 rel-L2 vs FP64; learned-adaptive vs plain dense-k",
@@ -146,8 +147,9 @@ def bench_batch(args, g, models):
         "adaptive": extra or None,
         "kernel_ms": {"recursion": round(tm["cells"], 3), "ksum": round(tm["ksum"], 3), "ifft": round(tm["ifft"], 3)},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None, "kernel": "k_cells_zr_batch", "flops": frac * work["rec_flop"],
-                     "cells": frac * work["cells"]},
+                     "traffic": None, "kernel": "k_cells_zr_batch", "flops": frac * work["rec_flop_solved"],
+                     "cells": (extra["cells_frac"] + extra.get("check_frac", 0.0)) * work["cells"] if extra
+                     else work["cells"]},
         "cpu_baseline": None,
     }
 
@@ -241,14 +243,15 @@ def bench_columns(args, g, models):
     peak = FP64_PEAK_TFLOPS if prec == g.PREC_FP64 else FP32_PEAK_TFLOPS
     frac = extra["cells_frac"] if extra else 1.0  # kept share of the cells and of the contraction's K
     cells = work["main_cells"]  # the main recursion launch, timed alone
-    rec = cells * work["flops_per_cell"] / (tm["main"] * 1e-3) / 1e12
+    rec = cells * work["main_flops_per_cell"] / (tm["main"] * 1e-3) / 1e12
     con = frac * work["ksum_flop"] / (tm["ksum"] * 1e-3) / 1e12
     if tm["cells"] >= tm["ksum"]:
         roof = dict(bound="valu", achieved=rec, peak=peak, unit="TFLOP/s", frac=rec / peak, traffic=None,
-                    kernel="k_cells_mt", launch_ms=tm["main"], flops_per_cell=work["flops_per_cell"], cells=cells)
+                    kernel="k_cells_mt", launch_ms=tm["main"], flops_per_cell=work["main_flops_per_cell"],
+                    flops_per_cell_full_sweep=work["flops_per_cell"], cells=cells)
     else:
         roof = dict(bound="mfma", achieved=con, peak=peak, unit="TFLOP/s", frac=con / peak, traffic=None,
-                    kernel="k_gemm_f32s" if prec == g.PREC_FP32 else "k_gemm_f64s")
+                    kernel=("k_gemm_f32sk" if extra else "k_gemm_f32t") if prec == g.PREC_FP32 else "k_gemm_f64s")
     ntr = work["traces"]
     return {
         "metric": ("displacement + strain" if dv else "displacement") +
@@ -304,11 +307,13 @@ def roofline(g, plan, prec, tm, work, counted=False):
         # the main recursion launch (FP32 rows of a mixed plan, all rows of an
         # FP64 plan) timed alone with HIP events on the plan's stream
         cells = work["main_cells"]
-        achieved = cells * work["flops_per_cell"] / (tm["main"] * 1e-3) / 1e12
+        # executed flops: each cell's count with its upward sweep as cut by depth
+        achieved = cells * work["main_flops_per_cell"] / (tm["main"] * 1e-3) / 1e12
         traffic = pmc_traffic("k_cells_zr<float,float,false,1>") if counted else None
         return dict(bound="valu", achieved=achieved, peak=peak, unit="TFLOP/s", frac=achieved / peak, traffic=traffic,
                     traffic_source=os.path.relpath(PMC_SUMMARY, ROOT) if traffic else None,
-                    kernel="k_cells_zr", launch_ms=tm["main"], flops_per_cell=work["flops_per_cell"],
+                    kernel="k_cells_zr", launch_ms=tm["main"], flops_per_cell=work["main_flops_per_cell"],
+                    flops_per_cell_full_sweep=work["flops_per_cell"],
                     transcendentals_per_cell=work["trans_per_cell"], cells=cells), kernels
     if dom == "ksum":
         achieved = work["ksum_flop"] / (tm["ksum"] * 1e-3) / 1e12

@@ -125,11 +125,11 @@ class TracePlan:
         return dict(total=ms[0], cells=ms[1], ksum=ms[2], ifft=ms[3], bessel=ms[4], main=ms[5])
 
     def work(self):
-        w = np.zeros(10)
+        w = np.zeros(11)
         check(_lib().grtm_plan_work(self.ptr, w.ctypes.data_as(P)), "grtm_plan_work")
         return dict(cells=w[0], cell_layers=w[1], ksum_flop=w[2], ifft_bytes=w[3], traces=w[4],
                     flops_per_cell=w[5], trans_per_cell=w[6], rows_fp64=int(w[7]), cells_refixed=int(w[8]),
-                    main_cells=w[9])
+                    main_cells=w[9], main_flops_per_cell=w[10])
 
     def set_rows64(self, rows64):
         """Mixed-precision plans: solve frequency rows [0, rows64) in FP64."""
@@ -293,9 +293,9 @@ class BatchPlan:
                     graph_nodes=int(o[4]), tuned=int(o[5]), tune_ms=float(o[6]))
 
     def work(self):
-        o = np.zeros(5)
+        o = np.zeros(6)
         check(_lib().grtm_batch_work(self.ptr, o.ctypes.data_as(P)), "grtm_batch_work")
-        return dict(cells=o[0], cell_layers=o[1], ksum_flop=o[2], traces=o[3], rec_flop=o[4])
+        return dict(cells=o[0], cell_layers=o[1], ksum_flop=o[2], traces=o[3], rec_flop=o[4], rec_flop_solved=o[5])
 
     def __del__(self):
         if getattr(self, "ptr", None) is not None and self.ptr.value:

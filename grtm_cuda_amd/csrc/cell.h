@@ -277,6 +277,32 @@ struct CellOut {
     int status;         // 0 ok
 };
 
+// Depth truncation of the upward sweep.  With the receiver in or above the
+// source layer the sweep only feeds R_du at the bottom of the source layer,
+// where the stack below layer j enters through the two-way decay
+// e^{-2 x_j} relative to the direct wave, x_j = sum of Re(nu_beta) h from the
+// deeper of source and receiver (z0, depth in layer s) down to the bottom of
+// layer j (S decays least; P and converted paths decay faster).  Once
+// x_j > kDecayCut (e^{-36} = 2e-16 two-way, below FP64 rounding) layer j is
+// taken as the half-space; j = s drops the sweep (R_du = 0).  Returns that
+// layer, or L - 1.  The criterion runs in the cell's precision.
+constexpr double kDecayCut = 18.0;
+template <typename T>
+GRTM_HD int sweep_bottom(const LayerW<T> *lw, int L, int s, typename WideOf<T>::type z0,
+                         typename WideOf<T>::type kd)
+{
+    const T k = T(kd), cut = T(kDecayCut);
+    T x = T(0);
+    for (int j = s; j < L - 1; j++) {
+        const cx<T> sb = narrow<T>(lw[j].sbd);
+        const cx<T> z = (C<T>(k, T(0)) - sb) * (C<T>(k, T(0)) + sb);
+        const T re = sqrt(T(0.5) * (sqrt(z.x * z.x + z.y * z.y) + z.x));  // Re sqrt(z)
+        x = x + re * T(j == s ? lw[j].h - z0 : lw[j].h);
+        if (x >= cut) return j;
+    }
+    return L - 1;
+}
+
 // compat: reproduce the reference's index choices (rexut from the first cell,
 // the extra v in u0t, yu0t/yd0t products, the shifted r>s chain).
 // BR: the receiver branch fixed at compile time (1-4), or 0 to read g.branch;
@@ -284,9 +310,11 @@ struct CellOut {
 // surface-receiver solve needs 134 VGPRs instead of 155).
 // NEED_T: also form the d/dz responses.  par0_r/par0_v: vertical wavenumbers
 // of layer rl at the first cell (k = 0, w_0), used only in compat mode.
+// bot_in: the upward sweep's bottom layer (operation counts); -1: from
+// sweep_bottom when the receiver is at or above the source layer, else L - 1.
 template <typename T, bool COMPAT, bool NEED_T, int BR = 0>
 GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::type kd, cx<T> par0_r,
-                        cx<T> par0_v, CellOut<T> &out)
+                        cx<T> par0_v, CellOut<T> &out, int bot_in = -1)
 {
     const T k = T(kd);
     const int L = g.L, s = g.s, rl = g.rl;
@@ -302,16 +330,20 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
     rdu0_r1 = C<T>(0, 0);
     Md = ident<T>();
     md0 = C<T>(1, 0);
-    {
-        LQ<T> qd = layer_q(lw[L - 1], kd);  // layer below the interface
-        LQ<T> qu = layer_q(lw[L - 2], kd);
+    const int bot = bot_in >= 0 ? bot_in : branch != 4 ? sweep_bottom(lw, L, s, branch == 3 ? zr : zs, kd) : L - 1;
+    if (bot == s) {
+        Rdu_s1 = m2<T>{C<T>(0, 0), C<T>(0, 0), C<T>(0, 0), C<T>(0, 0)};
+        rdu0_s1 = C<T>(0, 0);
+    } else {
+        LQ<T> qd = layer_q(lw[bot], kd);  // layer below the interface
+        LQ<T> qu = layer_q(lw[bot - 1], kd);
         m2<T> X, Y, Rg, Td;
         cx<T> rg0, td0, d2;
         interface_xy(qu, qd, k, X, Y, d2);
         grt_step0(X, Y, d2, Rg, Td);
         grt_step_sh(qu.v * qu.mu, qd.v * qd.mu, C<T>(0, 0), rg0, td0);
-        // grt[L-1] = (Rg, Td)
-        int j = L - 1;  // index of the grt entry just formed
+        // grt[bot] = (Rg, Td)
+        int j = bot;  // index of the grt entry just formed
         // lj0, lj1 = Lambda_jj (layer jj, just below the interface formed)
         auto visit = [&](int jj, const m2<T> &RgJ, cx<T> rg0J, const m2<T> &TdJ, cx<T> td0J,
                          cx<T> lj0, cx<T> lj1) {
@@ -350,7 +382,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             }
         };
         visit(j, Rg, rg0, Td, td0, C<T>(0, 0), C<T>(0, 0));  // Lambda of the half-space is never used
-        for (int lay = L - 3; lay >= s; lay--) {
+        for (int lay = bot - 2; lay >= s; lay--) {
             // interface lay: between layer lay (above) and lay+1 (below, = qu)
             cx<T> e0, e1;
             lam(qu, lw[lay + 1].h, e0, e1);

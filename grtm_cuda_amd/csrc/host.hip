@@ -1317,6 +1317,37 @@ int grtm_plan_work(grtm_plan *p, double *w)
         for (int iw = r0; iw < p->nw; iw++) mc += p->adaptive ? p->nk_pred_h[iw] : p->nk;
         w[9] = mc;
     }
+    {
+        // mean flops per cell of the main launch with the depth truncation:
+        // the upward sweep's bottom layer of every cell (sweep_bottom on the
+        // FP64 tables), each bottom counted on the counting type
+        const int r0 = p->prec == GRTM_PREC_FP64 ? 0 : p->rows64;
+        const bool cut = p->g.branch != 4;
+        std::vector<double> nbot(p->L, 0.0);
+        if (cut) {
+            std::vector<LayerW<double>> lw((size_t)p->nw * p->L);
+            const DBuf &t64 = p->prec == GRTM_PREC_FP64 ? p->lw : p->lw64;
+            HIPCHK(hipMemcpy(lw.data(), t64.p, lw.size() * sizeof(LayerW<double>), hipMemcpyDeviceToHost));
+            for (int iw = r0; iw < p->nw; iw++) {
+                const int nk = p->adaptive ? p->nk_pred_h[iw] : p->nk;
+                for (int ik = 0; ik < nk; ik++)
+                    nbot[grtm::sweep_bottom(lw.data() + (size_t)iw * p->L, p->L, p->g.s,
+                                            p->g.branch == 3 ? p->g.zr : p->g.zs, p->dk * ik)] += 1;
+            }
+        } else {
+            nbot[p->L - 1] = w[9];
+        }
+        double fl = 0;
+        for (int b = 0; b < p->L; b++) {
+            if (nbot[b] == 0) continue;
+            double ob[2];
+            if (grtm::opcount_cell_bot(p->kind == 0 ? 0 : (p->deriv ? 2 : 1), p->L, p->g.s, p->g.rl, p->g.branch,
+                                       p->g.up, b, ob))
+                return GRTM_E_INPUT;
+            fl += nbot[b] * ob[0];
+        }
+        w[10] = w[9] > 0 ? fl / w[9] : oc[0];
+    }
     w[8] = 0;  // cells re-solved in FP64 after a non-finite FP32 result (last execution)
     if (p->prec != GRTM_PREC_FP64) {
         unsigned c = 0;
@@ -2066,6 +2097,32 @@ int grtm_batch_work(grtm_batch *b, double *out)
     out[2] = 2.0 * 2.0 * (4.0 * b->nw) * b->nk * b->nr * b->nm;
     out[3] = 2.0 * b->nr * b->nm;
     out[4] = b->flops;
+    // recursion flops of the cells the current grid solves (dense, or below
+    // the predicted cuts), each cell counted with its upward sweep cut by
+    // depth: estimated on every 8th row and every 16th wavenumber
+    double fl = 0;
+    std::vector<LayerW<double>> lw;
+    for (int m = 0; m < b->nm; m++) {
+        const grtm::Geo &g = b->md_h[m].g;
+        lw.resize((size_t)b->nw * g.L);
+        HIPCHK(hipMemcpy(lw.data(), b->lw64.as<LayerW<double>>() + b->md_h[m].lw_off, lw.size() * sizeof(LayerW<double>),
+                         hipMemcpyDeviceToHost));
+        std::vector<double> nbot(g.L, 0.0);
+        for (int iw = 0; iw < b->nw; iw += 8) {
+            const int nk = b->adaptive ? b->nk_pred_h[(size_t)m * b->nw + iw] : b->nk;
+            for (int ik = 0; ik < nk; ik += 16)
+                nbot[g.branch != 4 ? grtm::sweep_bottom(lw.data() + (size_t)iw * g.L, g.L, g.s,
+                                                        g.branch == 3 ? g.zr : g.zs, b->dk * ik)
+                                   : g.L - 1] += 8.0 * 16.0;
+        }
+        for (int j = 0; j < g.L; j++) {
+            if (nbot[j] == 0) continue;
+            double oc[2];
+            if (grtm::opcount_cell_bot(0, g.L, g.s, g.rl, g.branch, g.up, j, oc)) return GRTM_E_INPUT;
+            fl += nbot[j] * oc[0];
+        }
+    }
+    out[5] = fl;
     return GRTM_OK;
 }
 

@@ -6,6 +6,11 @@
 
 namespace grtm {
 
+// Operation count of one cell whose upward sweep ends at layer bot (the depth
+// truncation's choice; L - 1: none), as grtm_opcount_cell (kind 0) and
+// grtm_opcount_columns_cell (kind 1; 2 with the strain rows).
+int opcount_cell_bot(int kind, int L, int s, int rl, int branch, int up, int bot, double *out);
+
 constexpr int kMaxLayers = 64;
 
 // Spectral post-processing (cusrc/popm/func/green.cu:1905-1921) and time

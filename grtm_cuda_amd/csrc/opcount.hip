@@ -60,12 +60,14 @@ struct grtm::WideOf<grtm_count::Cnt> {
     using type = grtm_count::Cnt;
 };
 
-static int count_cell(int kind, int L, int s, int rl, int branch, int up, double *out)
+static int count_cell(int kind, int L, int s, int rl, int branch, int up, int bot, double *out)
 {
     using grtm::cx;
     using grtm::LayerW;
     using grtm_count::Cnt;
     if (L < 2 || L > 64 || s < 0 || s >= L - 1 || rl < 0 || rl >= L || !out) return -3;
+    if (bot < 0) bot = L - 1;
+    if (bot < s || bot > L - 1 || (branch == 4 && bot != L - 1)) return -3;
     LayerW<Cnt> lw[64];
     const double w_re = 3.0, w_im = -0.05;
     for (int j = 0; j < L; j++) {
@@ -93,11 +95,20 @@ static int count_cell(int kind, int L, int s, int rl, int branch, int up, double
     g.pad = 0;
     g.zs = 0.5 * (2.0 + s);
     g.zr = (rl == s) ? (branch == 2 ? 0.25 * (2.0 + s) : 0.75 * (2.0 + s)) : 0.3;
+    // the depth-truncation criterion: one iteration per layer from the source
+    // down to bot (all L - 1 - s when nothing is cut)
+    double crit = 0;
+    if (branch != 4) {
+        grtm_count::g_t = grtm_count::Tally();
+        (void)grtm::sweep_bottom(lw, L, s, Cnt(branch == 3 ? g.zr : g.zs), Cnt(0.7));  // k = 0.7: no layer is cut
+        const int iters = bot < L - 1 ? bot - s + 1 : L - 1 - s;
+        crit = grtm_count::g_t.flops / (L - 1 - s) * iters;
+    }
     grtm_count::g_t = grtm_count::Tally();
     grtm::CellOut<Cnt> c;
     const cx<Cnt> z0{Cnt(0), Cnt(0)};
-    if (kind == 2) grtm::cell_solve<Cnt, false, true>(lw, g, Cnt(0.7), z0, z0, c);
-    else grtm::cell_solve<Cnt, false, false>(lw, g, Cnt(0.7), z0, z0, c);
+    if (kind == 2) grtm::cell_solve<Cnt, false, true>(lw, g, Cnt(0.7), z0, z0, c, bot);
+    else grtm::cell_solve<Cnt, false, false>(lw, g, Cnt(0.7), z0, z0, c, bot);
     const cx<Cnt> w{Cnt(w_re), Cnt(w_im)};
     if (kind == 0) {
         cx<Cnt> a, b, d, e;
@@ -107,17 +118,24 @@ static int count_cell(int kind, int L, int s, int rl, int branch, int up, double
     } else {
         grtm::mt_rows_from_cell<Cnt, false, true>(c, up, Cnt(0.7), w, [](int, int, cx<Cnt>) {});
     }
-    out[0] = grtm_count::g_t.flops;
+    out[0] = grtm_count::g_t.flops + crit;
     out[1] = grtm_count::g_t.trans;
     return 0;
 }
 
+namespace grtm {
+int opcount_cell_bot(int kind, int L, int s, int rl, int branch, int up, int bot, double *out)
+{
+    return count_cell(kind, L, s, rl, branch, up, bot, out);
+}
+}  // namespace grtm
+
 extern "C" int grtm_opcount_cell(int L, int s, int rl, int branch, int up, double *out)
 {
-    return count_cell(0, L, s, rl, branch, up, out);
+    return count_cell(0, L, s, rl, branch, up, -1, out);
 }
 
 extern "C" int grtm_opcount_columns_cell(int L, int s, int rl, int branch, int up, int derivatives, double *out)
 {
-    return count_cell(derivatives ? 2 : 1, L, s, rl, branch, up, out);
+    return count_cell(derivatives ? 2 : 1, L, s, rl, branch, up, -1, out);
 }

@@ -130,10 +130,11 @@ int grtm_plan_kernels_out(grtm_plan *plan, int q, int w_lo, int w_hi, double *ke
 int grtm_plan_timing(grtm_plan *plan, int reps, double *ms);
 /* Work of one execution: [0] cells, [1] cell-layers, [2] contraction flops,
  * [3] IFFT algorithmic bytes, [4] traces (column plans: column traces),
- * [5] recursion flops per cell,
+ * [5] recursion flops per cell with the full upward sweep,
  * [6] transcendental calls per cell, [7] frequency rows solved in FP64,
  * [8] FP32 cells re-solved in FP64 in the last execution, [9] cells of the
- * main recursion launch (10 doubles). */
+ * main recursion launch, [10] their mean flops per cell with the upward
+ * sweep cut where the stack below no longer reaches the source (11 doubles). */
 int grtm_plan_work(grtm_plan *plan, double *work);
 void grtm_plan_destroy(grtm_plan *plan);
 
@@ -208,7 +209,10 @@ int grtm_batch_timing(grtm_batch *batch, int reps, double *ms);
  * [3] groups, [4] graph nodes, [5] autotune candidates timed, [6] autotune ms. */
 int grtm_batch_config(grtm_batch *batch, double *out);
 /* out[0] cells, [1] cell-layers, [2] contraction flops, [3] traces,
- * [4] recursion flops (operation count of the cell arithmetic). */
+ * [4] recursion flops of the dense grid with full upward sweeps (operation
+ * count of the cell arithmetic), [5] recursion flops of the cells the
+ * current grid solves with the sweeps cut by depth (estimated on every 8th
+ * frequency row and 16th wavenumber) (6 doubles). */
 int grtm_batch_work(grtm_batch *batch, double *out);
 void grtm_batch_destroy(grtm_batch *batch);
 
@@ -237,8 +241,9 @@ int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi,
                        const double *lay, int s_layer, double zs, int r_layer, double zr,
                        const double *par0_rv, double *out);
 
-/* Exact operation count of one recursion cell of the trace path (the
- * device template run on a counting scalar type on the host):
+/* Exact operation count of one recursion cell of the trace path with the
+ * full upward sweep (the device template run on a counting scalar type on
+ * the host):
  * out[0] = add/sub/mul/div/sqrt, out[1] = transcendental calls
  * (exp, sincos, log, atan2, hypot).  branch/up as the geometry selects
  * (1: receiver layer above the source layer, 2/3: same layer above/below,

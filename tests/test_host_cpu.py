@@ -97,6 +97,37 @@ def test_cell_solver_matches_oracle(orc, compat, src):
     assert seen == {1, 2, 3, 4}
 
 
+def test_cell_solver_depth_truncation(orc):
+    """Thick layers and strongly evanescent wavenumbers, where the device
+    template cuts its upward sweep below the deeper of source and receiver
+    (or drops it): against the oracle's full recursion, receiver above the
+    source layer and above / below the source in its layer, both modes and
+    sources.  With the cut measured from the source instead of a deeper
+    receiver, the branch-3 cases miss by about e^-18."""
+    L, h, zs, s = 8, 6.0, 2.0, 2
+    w = 2 * np.pi * 1.0 - 0.05j
+    vs = np.linspace(3.0, 4.4, L)
+    lay = np.zeros((L, 7))
+    for compat in (1, 0):
+        for j in range(L):
+            a = orc.dispersion(w, 1.75 * vs[j], 100 * vs[j] ** 2)
+            b = orc.dispersion(w, vs[j], 50 * vs[j] ** 2)
+            rho = 1.74 * (1.75 * vs[j]) ** 0.25
+            mu = vs[j] ** 2 * rho if compat else rho * b * b
+            lay[j] = [h, a.real, a.imag, b.real, b.imag, np.real(mu), np.imag(mu)]
+        fired = 0
+        for rl, zr in ((0, 0.0), (s, 1.0), (s, 4.5)):
+            z0 = max(zs, zr) if rl == s else zs
+            for k in (3.0, 6.3, 12.0):
+                nu = np.sqrt(k ** 2 - (w / (lay[s, 3] + 1j * lay[s, 4])) ** 2)
+                fired += nu.real * (h - z0) + nu.real * h >= 18  # the sweep ends at layer s or s + 1
+                for src in (0, 1):
+                    a = orc.cell(compat, src, k, w, lay, s, zs, rl, zr, np.zeros(4))
+                    b = g.selftest_cell(compat, src, k, w, lay, s, zs, rl, zr, np.zeros(4))
+                    assert np.abs(a - b).max() <= 1e-10 * np.abs(a).max(), (compat, rl, zr, k, src)
+        assert fired >= 6
+
+
 def test_cell_solver_free_surface_receiver(orc):
     """Receiver on the free surface in physics mode, where the device template
     forms the d/dz response from the traction-free conditions: against the
