@@ -555,6 +555,19 @@ __device__ __forceinline__ void post_ifft_trace(const PostParams &pp, Load &&loa
     }
 }
 
+// Workgroups are dispatched to the 8 XCDs in turn, and each XCD has its own
+// L2.  Traces t and t + 1 read the same 128-byte lines of the row-major
+// spectra (adjacent receivers), so block b takes trace
+// (b % 8) ceil(ntr / 8) + b / 8: each XCD works through a contiguous range
+// of traces and fetches each line once.  The grid has 8 ceil(ntr / 8) blocks.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_trace(int ntr)
+{
+    const int per = (ntr + kXcds - 1) / kXcds;
+    return int(blockIdx.x % kXcds) * per + int(blockIdx.x / kXcds);
+}
+static unsigned xcd_grid(int ntr) { return (unsigned)((ntr + kXcds - 1) / kXcds * kXcds); }
+
 // One workgroup per trace component.  layout 0: raw[(t*nw + iw)*2 + {re,im}];
 // layout 1: rows of the contraction outputs: component c = (t / nr) % 2 reads
 // C_c[(4 iw + 2 q + {0,1}) * ldc + ir] with q = t / (2 nr), ir = t % nr.
@@ -563,10 +576,11 @@ __device__ __forceinline__ void post_ifft_trace(const PostParams &pp, Load &&loa
 template <typename T>
 __global__ __launch_bounds__(256) void k_post_ifft(PostParams pp, int layout, const T *__restrict__ raw,
                                                    const T *__restrict__ C0, const T *__restrict__ C1,
-                                                   long ldc, int nr, long cms, T *__restrict__ traces,
+                                                   long ldc, int nr, long cms, int ntr, T *__restrict__ traces,
                                                    T *__restrict__ spec_post)
 {
-    const int t = blockIdx.x;
+    const int t = xcd_trace(ntr);
+    if (t >= ntr) return;
     const int M = pp.nw;
     auto load = [&](int iw) -> cx<T> {
         if (layout == 0) {
@@ -587,11 +601,12 @@ __global__ __launch_bounds__(256) void k_post_ifft(PostParams pp, int layout, co
 // the contraction outputs (rows 2 (rows_set iw + row) + {0 re, 1 im}).
 template <typename T>
 __global__ __launch_bounds__(256) void k_post_ifft_cols(PostParams pp, const MtCols *__restrict__ mcp, long ldc,
-                                                        int nr, const double *__restrict__ dists,
+                                                        int nr, int ntr, const double *__restrict__ dists,
                                                         T *__restrict__ traces)
 {
     const MtCols &mc = *mcp;
-    const int t = blockIdx.x;
+    const int t = xcd_trace(ntr);
+    if (t >= ntr) return;
     const int col = t / nr, ir = t - col * nr;
     const double rinv = 1.0 / dists[ir];
     auto load = [&](int iw) -> cx<T> {
@@ -851,12 +866,13 @@ hipError_t launch_post_ifft_cols(int prec, const PostParams &pp, const MtCols *m
         if (e != hipSuccess) return e;
     }
     if (ncol < 1 || ncol > kMtColsMax) return hipErrorInvalidValue;
-    const dim3 grid(ncol * nr);
+    const int ntr = ncol * nr;
+    const dim3 grid(xcd_grid(ntr));
     if (prec == 0)
-        hipLaunchKernelGGL(k_post_ifft_cols<double>, grid, dim3(256), lds, st, pp, mc, ldc, nr, dists,
+        hipLaunchKernelGGL(k_post_ifft_cols<double>, grid, dim3(256), lds, st, pp, mc, ldc, nr, ntr, dists,
                            (double *)traces);
     else
-        hipLaunchKernelGGL(k_post_ifft_cols<float>, grid, dim3(256), lds, st, pp, mc, ldc, nr, dists,
+        hipLaunchKernelGGL(k_post_ifft_cols<float>, grid, dim3(256), lds, st, pp, mc, ldc, nr, ntr, dists,
                            (float *)traces);
     return hipGetLastError();
 }
@@ -876,13 +892,13 @@ hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr,
         if (e != hipSuccess) return e;
     }
     if (prec == 0)
-        hipLaunchKernelGGL(k_post_ifft<double>, dim3(ntr), dim3(256), lds, st, pp, layout, (const double *)raw,
-                           (const double *)C0, (const double *)C1, ldc, nr, cms, (double *)traces,
-                           (double *)spec_post);
+        hipLaunchKernelGGL(k_post_ifft<double>, dim3(xcd_grid(ntr)), dim3(256), lds, st, pp, layout,
+                           (const double *)raw, (const double *)C0, (const double *)C1, ldc, nr, cms, ntr,
+                           (double *)traces, (double *)spec_post);
     else
-        hipLaunchKernelGGL(k_post_ifft<float>, dim3(ntr), dim3(256), lds, st, pp, layout, (const float *)raw,
-                           (const float *)C0, (const float *)C1, ldc, nr, cms, (float *)traces,
-                           (float *)spec_post);
+        hipLaunchKernelGGL(k_post_ifft<float>, dim3(xcd_grid(ntr)), dim3(256), lds, st, pp, layout,
+                           (const float *)raw, (const float *)C0, (const float *)C1, ldc, nr, cms, ntr,
+                           (float *)traces, (float *)spec_post);
     return hipGetLastError();
 }
 
