@@ -299,6 +299,39 @@ int device_ready()
     return GRTM_OK;
 }
 
+// The post-processing tables of pp (the PostParams fields pv, tw, ts), computed
+// in FP64 and stored in buf in the trace precision (prec 0 FP64, else FP32).
+int post_tables(grtm::PostParams &pp, int prec, DBuf &buf)
+{
+    const int M = pp.nw, N = 2 * M;
+    std::vector<double> v((size_t)4 * M + N);
+    double *pv = v.data(), *tw = pv + 2 * M, *ts = tw + 2 * M;
+    for (int iw = 0; iw < M; iw++) {
+        const double phi = 2.0 * M_PI * pp.df * iw * pp.t0;
+        const double st0 = iw > pp.wc ? 0.5 * (1. + std::cos((iw - pp.wc + 1) * M_PI / (pp.nw - pp.wc + 1))) : 1.0;
+        const double coef = pp.df * pp.nw / M_PI * st0;
+        pv[2 * iw] = std::cos(phi) * coef;
+        pv[2 * iw + 1] = std::sin(phi) * coef;
+        tw[2 * iw] = std::cos(M_PI * iw / M);
+        tw[2 * iw + 1] = std::sin(M_PI * iw / M);
+    }
+    for (int j = 0; j < N; j++) ts[j] = std::exp(pp.omega_i * (j * pp.dt + pp.t0)) / N;
+    const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
+    int st;
+    if ((st = buf.alloc(v.size() * es))) return st;
+    if (prec == 0) {
+        HIPCHK(hipMemcpy(buf.p, v.data(), v.size() * es, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(v.begin(), v.end());
+        HIPCHK(hipMemcpy(buf.p, f.data(), f.size() * es, hipMemcpyHostToDevice));
+    }
+    const char *b = static_cast<const char *>(buf.p);
+    pp.pv = b;
+    pp.tw = b + (size_t)2 * M * es;
+    pp.ts = b + (size_t)4 * M * es;
+    return GRTM_OK;
+}
+
 // ------------------------------------------------------------------ green()
 // The whole drop-in run; spec/traces in the layout of grtm_green_mem.
 int green_core(const char *iname, int compat, std::vector<double> *spec, std::vector<double> *traces, int *nw_out)
@@ -373,13 +406,15 @@ int green_core(const char *iname, int compat, std::vector<double> *spec, std::ve
     HIPCHK(grtm::launch_cells_full(compat, rp.src_type ? 1 : 0, d_lw.p, d_nk.as<int>(), st.L, g, nw, nblk, dk,
                                    rp.dist, df, omega_i, p0, d_part.as<double>(), s));
     HIPCHK(grtm::launch_green_finish(nw, nblk, d_part.as<double>(), d_raw.as<double>(), s));
-    grtm::PostParams pp;
+    grtm::PostParams pp{};
     pp.nw = nw;
     pp.wc = (int)((double)nw * (1 - rp.taper));
     pp.df = df;
     pp.dt = rp.dt;
     pp.t0 = rp.t0;
     pp.omega_i = omega_i;
+    DBuf d_pt;
+    if ((status = post_tables(pp, 0, d_pt))) return status;
     HIPCHK(grtm::launch_post_ifft(0, pp, 0, 30, d_raw.p, nullptr, nullptr, 0, 0, d_tr.p, d_sp.p, s));
     if (spec) {
         spec->resize((size_t)30 * nw * 2);
@@ -538,7 +573,7 @@ std::vector<LayerW<float>> narrow_tables(const std::vector<LayerW<double>> &lw)
 
 grtm::PostParams post_params(const grtm_grid *grid, int nw, double df, double omega_i)
 {
-    grtm::PostParams pp;
+    grtm::PostParams pp{};
     pp.nw = nw;
     pp.wc = (int)((double)nw * (1 - grid->taper));
     pp.df = df;
@@ -577,6 +612,7 @@ struct grtm_plan {
     bool pair = false;  // FP32: both contractions in one streamed launch
     int kc = 0;         // its split-K slice length
     DBuf lw, lw64, flags, dists, A0, A1, B0, B1, C0, C1, tr, ws;
+    DBuf ptab;  // post-processing tables of pp
     // adaptive wavenumber grid
     bool adaptive = false;
     int chk_stride = 8, max_chk = 0;
@@ -926,6 +962,7 @@ static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid
     p->df = df;
     p->omega_i = omega_i;
     p->pp = post_params(grid, nw, df, omega_i);
+    if ((status = post_tables(p->pp, precision == GRTM_PREC_FP64 ? 0 : 1, p->ptab))) return status;
     p->kind = kind;
     if (kind == 1) {
         p->deriv = deriv ? 1 : 0;
@@ -1534,6 +1571,7 @@ struct grtm_batch {
     double flops = 0;            // recursion flops of one execution (operation count)
     std::vector<grtm::ModelDesc> md_h;
     DBuf md, lw64, lw32, dists, B0, B1, A0, A1, C0, C1, ws, flags, tr, items;
+    DBuf ptab;  // post-processing tables of pp
     int nb = 0;       // models per group
     int bs = 256;     // recursion tile: wavenumbers per block
     int kc = 0;       // streamed FP32 contraction: K slice
@@ -1843,6 +1881,7 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
     b->df = 1.0 / T;
     b->omega_i = grid->omega_i > 0 ? grid->omega_i : M_PI / T;
     b->pp = post_params(grid, b->nw, b->df, b->omega_i);
+    if (int e = post_tables(b->pp, precision == GRTM_PREC_FP64 ? 0 : 1, b->ptab)) return e;
     HIPCHK(hipStreamCreate(&b->stream));
 
     // models: descriptors and layer tables, concatenated

@@ -22,6 +22,10 @@ struct PostParams {
     double dt;       // time step
     double t0;       // reduction time
     double omega_i;  // imaginary frequency
+    // device tables in the trace precision (FP64 or FP32), computed in FP64:
+    // pv[iw] = e^{2 pi i df iw t0} taper(iw) df nw / pi (complex, nw),
+    // tw[m] = e^{2 pi i m / N} (complex, nw), ts[j] = e^{w_I (j dt + t0)} / N (N)
+    const void *pv, *tw, *ts;
 };
 
 // Displacement columns (tensor columns 0-9, single-force columns 0-4) and,

@@ -490,14 +490,8 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const T *__re
 template <typename T>
 __device__ __forceinline__ cx<T> post_val(cx<T> v, int iw, const PostParams &pp)
 {
-    const double phi = 2.0 * M_PI * pp.df * iw * pp.t0;
-    double s, c;
-    sincos(phi, &s, &c);
-    double st0 = 1.0;
-    if (iw > pp.wc) st0 = 0.5 * (1. + cos((iw - pp.wc + 1) * M_PI / (pp.nw - pp.wc + 1)));
-    const double coef = pp.df * pp.nw / M_PI * st0;
-    const cx<T> ph = C<T>(T(c * coef), T(s * coef));
-    return v * ph;
+    const T *pv = static_cast<const T *>(pp.pv);
+    return v * C<T>(pv[2 * iw], pv[2 * iw + 1]);
 }
 
 // Phase / taper / scale, Hermitian C2R inverse FFT (Stockham radix 2 in LDS)
@@ -510,6 +504,7 @@ __device__ __forceinline__ void post_ifft_trace(const PostParams &pp, Load &&loa
     cx<T> *bufA = reinterpret_cast<cx<T> *>(smem_raw);
     const int M = pp.nw, N = 2 * pp.nw;
     cx<T> *bufB = bufA + M;
+    const T *tw = static_cast<const T *>(pp.tw), *ts = static_cast<const T *>(pp.ts);
     for (int m = threadIdx.x; m < M; m += blockDim.x) {
         cx<T> Xm = post_val(load(m), m, pp);
         if (spec_post) {
@@ -521,9 +516,7 @@ __device__ __forceinline__ void post_ifft_trace(const PostParams &pp, Load &&loa
         if (m == 0) Xm.y = T(0);  // Im X[0] only reaches Im of the output
         if (M - m == 0) Xc.y = T(0);
         const cx<T> Xe = Xm + Xc;
-        double sn, cs;
-        sincospi(double(m) / double(M), &sn, &cs);  // e^{+2 pi i m / N}
-        const cx<T> Xo = (Xm - Xc) * C<T>(T(cs), T(sn));
+        const cx<T> Xo = (Xm - Xc) * C<T>(tw[2 * m], tw[2 * m + 1]);  // e^{+2 pi i m / N}
         bufA[m] = C<T>(Xe.x - Xo.y, Xe.y + Xo.x);  // Xe + i Xo
     }
     __syncthreads();
@@ -534,11 +527,10 @@ __device__ __forceinline__ void post_ifft_trace(const PostParams &pp, Load &&loa
         const int m = n >> 1;
         for (int idx = threadIdx.x; idx < M / 2; idx += blockDim.x) {
             const int p = idx / s, q = idx - p * s;
-            double sn, cs;
-            sincospi(2.0 * double(p) / double(n), &sn, &cs);
+            const int tj = 2 * p * (N / n);  // e^{2 pi i p / n} = tw[p N / n]
             const cx<T> a = x[q + s * p], b = x[q + s * (p + m)];
             y[q + s * (2 * p)] = a + b;
-            y[q + s * (2 * p + 1)] = (a - b) * C<T>(T(cs), T(sn));
+            y[q + s * (2 * p + 1)] = (a - b) * C<T>(tw[tj], tw[tj + 1]);
         }
         __syncthreads();
         cx<T> *tmp = x;
@@ -546,12 +538,10 @@ __device__ __forceinline__ void post_ifft_trace(const PostParams &pp, Load &&loa
         y = tmp;
         s <<= 1;
     }
-    const double invN = 1.0 / double(N);
     for (int nn = threadIdx.x; nn < M; nn += blockDim.x) {
         const cx<T> z = x[nn];
-        const int j0 = 2 * nn, j1 = 2 * nn + 1;
-        traces[(size_t)t * N + j0] = T(double(z.x) * invN * exp(pp.omega_i * (j0 * pp.dt + pp.t0)));
-        traces[(size_t)t * N + j1] = T(double(z.y) * invN * exp(pp.omega_i * (j1 * pp.dt + pp.t0)));
+        traces[(size_t)t * N + 2 * nn] = z.x * ts[2 * nn];
+        traces[(size_t)t * N + 2 * nn + 1] = z.y * ts[2 * nn + 1];
     }
 }
 
