@@ -303,6 +303,22 @@ class BatchPlan:
             self.ptr = ctypes.c_void_p()
 
 
+def selftest_gemm_cut(prec, A, B, nk_w):
+    """The learned-grid contraction (test hook): C = A B with row r of A read up
+    to column nk_w[r // 4]; prec 1 FP32 stream-K, 0 FP64 split-K."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    nk_w = np.ascontiguousarray(nk_w, dtype=np.int32)
+    M, K = A.shape
+    N = B.shape[1]
+    if M % 4 or len(nk_w) != M // 4:
+        raise ValueError("A needs four rows per entry of nk_w")
+    C = np.zeros((M, N))
+    check(_lib().grtm_selftest_gemm_cut(int(prec), M, N, K, A.ctypes.data_as(P), B.ctypes.data_as(P),
+                                        nk_w.ctypes.data_as(P), C.ctypes.data_as(P)), "grtm_selftest_gemm_cut")
+    return C
+
+
 def selftest_gemm(mode, A, B):
     """C = A B on the GPU with a contraction kernel (test hook): mode -1 VALU
     FP64, 0 FP64 MFMA, 1 FP32 MFMA, 2 BF16x3 MFMA, 3 streamed FP32 MFMA pair

@@ -1398,6 +1398,70 @@ int grtm_plan_work(grtm_plan *p, double *w)
 void grtm_plan_destroy(grtm_plan *p) { delete p; }
 
 // ------------------------------------------------------------ self-tests
+int grtm_selftest_gemm_cut(int prec, int M, int N, int K, const double *A, const double *B, const int *nk_w, double *C)
+{
+    if ((prec != 0 && prec != 1) || M < 4 || M % 4 || N < 1 || K < 1 || !A || !B || !nk_w || !C) return GRTM_E_INPUT;
+    const int nw = M / 4;
+    double ksum = 0;
+    for (int i = 0; i < nw; i++) {
+        if (nk_w[i] < 0 || nk_w[i] > K) return GRTM_E_INPUT;
+        ksum += nk_w[i];
+    }
+    int st;
+    if ((st = device_ready())) return st;
+    const long lda = (K + 3) / 4 * 4, ldb = (N + 3) / 4 * 4;
+    const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
+    std::vector<double> a((size_t)M * lda, 0.0), b((size_t)K * ldb, 0.0);
+    for (int i = 0; i < M; i++)
+        for (int k = 0; k < K; k++) a[(size_t)i * lda + k] = A[(size_t)i * K + k];
+    for (int k = 0; k < K; k++)
+        for (int j = 0; j < N; j++) b[(size_t)k * ldb + j] = B[(size_t)k * N + j];
+    const int kc = grtm::gemm_pair_slice(prec, M, N, K, std::max(1.0, ksum / nw));
+    const int M2[2] = {M, M};
+    size_t wsb = grtm::gemm_pair_workspace_bytes(M, N, K, kc) * es / 4;
+    if (prec == 1) wsb = std::max(wsb, grtm::gemm_streamk_workspace_bytes(2, M2, N, K));
+    DBuf dA, dB, dC, dP, dn, dk;
+    if ((st = dA.alloc(a.size() * es)) || (st = dB.alloc(b.size() * es)) || (st = dC.alloc((size_t)2 * M * N * es)) ||
+        (st = dP.alloc(wsb)) || (st = dn.alloc(sizeof(int) * nw)) || (st = dk.alloc(sizeof(int) * ((nw + 31) / 32))))
+        return st;
+    if (prec == 0) {
+        HIPCHK(hipMemcpy(dA.p, a.data(), a.size() * es, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dB.p, b.data(), b.size() * es, hipMemcpyHostToDevice));
+    } else {
+        const std::vector<float> af(a.begin(), a.end()), bf(b.begin(), b.end());
+        HIPCHK(hipMemcpy(dA.p, af.data(), af.size() * es, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dB.p, bf.data(), bf.size() * es, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(dn.p, nk_w, sizeof(int) * nw, hipMemcpyHostToDevice));
+    // tile extents as the plans form them: 32 frequency rows (128 operand rows) per tile
+    HIPCHK(grtm::launch_kend_tiles(nw, 32, K, dn.as<int>(), dk.as<int>(), nullptr));
+    if (prec == 0) {
+        double *c0 = dC.as<double>(), *c1 = c0 + (size_t)M * N;
+        HIPCHK(grtm::launch_gemm_pair_f64(M, N, K, kc, dA.as<double>(), dA.as<double>(), lda, dB.as<double>(),
+                                          dB.as<double>(), ldb, c0, c1, dP.as<double>(), dk.as<int>(), dn.as<int>(),
+                                          nullptr));
+    } else {
+        float *c0 = dC.as<float>(), *c1 = c0 + (size_t)M * N;
+        HIPCHK(grtm::launch_gemm_pair_f32(M, N, K, kc, dA.as<float>(), dA.as<float>(), lda, dB.as<float>(),
+                                          dB.as<float>(), ldb, c0, c1, dP.as<float>(), dk.as<int>(), dn.as<int>(),
+                                          nullptr));
+    }
+    HIPCHK(hipDeviceSynchronize());
+    std::vector<double> f((size_t)2 * M * N);
+    if (prec == 0) {
+        HIPCHK(hipMemcpy(f.data(), dC.p, f.size() * es, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> g32(f.size());
+        HIPCHK(hipMemcpy(g32.data(), dC.p, g32.size() * es, hipMemcpyDeviceToHost));
+        std::copy(g32.begin(), g32.end(), f.begin());
+    }
+    for (size_t i = 0; i < (size_t)M * N; i++) {
+        if (f[i] != f[(size_t)M * N + i]) return GRTM_E_DEVICE;  // the two halves must agree bit for bit
+        C[i] = f[i];
+    }
+    return GRTM_OK;
+}
+
 int grtm_selftest_gemm(int mode, int M, int N, int K, const double *A, const double *B, double *C)
 {
     if (mode < -1 || mode > 4 || M < 1 || N < 1 || K < 1 || !A || !B || !C) return GRTM_E_INPUT;

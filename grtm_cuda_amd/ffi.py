@@ -78,6 +78,7 @@ def _lib():
         lib.grtm_opcount_cell.argtypes = [I, I, I, I, I, P]
         lib.grtm_opcount_columns_cell.argtypes = [I, I, I, I, I, I, P]
         lib.grtm_selftest_gemm.argtypes = [I, I, I, I, P, P, P]
+        lib.grtm_selftest_gemm_cut.argtypes = [I, I, I, I, P, P, P, P]
         lib.grtm_selftest_mfma_probe.argtypes = [I, P]
         lib.grtm_selftest_copy.argtypes = [I]
         lib.grtm_plan_set_adaptive.argtypes = [P, P, D, I, D]
@@ -117,7 +118,7 @@ EXPORTS = [
     "green", "grtm_green", "grtm_green_mem", "grtm_last_status", "grtm_status_str", "grtm_version",
     "grtm_create", "grtm_destroy", "grtm_plan_traces", "grtm_plan_execute", "grtm_plan_sync",
     "grtm_plan_traces_out", "grtm_plan_kernels_out", "grtm_plan_timing", "grtm_plan_work",
-    "grtm_plan_destroy", "grtm_selftest_cell", "grtm_opcount_cell", "grtm_selftest_gemm",
+    "grtm_plan_destroy", "grtm_selftest_cell", "grtm_opcount_cell", "grtm_selftest_gemm", "grtm_selftest_gemm_cut",
     "grtm_plan_columns", "grtm_opcount_columns_cell",
     "grtm_selftest_mfma_probe", "grtm_selftest_copy", "grtm_plan_set_adaptive", "grtm_plan_set_rows64",
     "grtm_plan_adaptive_stats",

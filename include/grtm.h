@@ -228,6 +228,12 @@ void grtm_batch_destroy(grtm_batch *batch);
  * required to agree bit for bit); operands rounded to FP32 first for modes
  * 1-3. */
 int grtm_selftest_gemm(int mode, int M, int N, int K, const double *A, const double *B, double *C);
+/* The learned-grid contraction pair (prec 1: FP32 stream-K, 0: FP64 split-K)
+ * on (A, B) twice: row r of A is read up to column nk_w[r / 4] (M / 4
+ * entries, four operand rows per frequency row), the tile extents formed on
+ * the GPU as the plans form them; the halves must agree bit for bit. */
+int grtm_selftest_gemm_cut(int prec, int M, int N, int K, const double *A, const double *B, const int *nk_w,
+                           double *C);
 
 /* Lane-layout probe of the FP64 matrix-core instruction: lane l0 feeds a = 1,
  * lane l feeds b = 1000 + l; out[64*4] receives every lane's accumulators. */
