@@ -723,34 +723,32 @@ int run_cells(grtm_plan *p, hipStream_t s)
     const unsigned *t_lo = ad ? p->tiles.as<unsigned>() : nullptr;
     const unsigned *t_hi = ad ? t_lo + p->ntiles64 : nullptr;
     const int n_lo = ad ? p->ntiles64 : 0, n_hi = ad ? p->ntiles - p->ntiles64 : 0;
+    // adaptive: the item lists carry the residual check beyond the cut (FP64 on
+    // the FP64 rows) after each launch's tiles
+    const int *nchk = ad ? p->nchk.as<int>() : nullptr;
     if (p->prec == GRTM_PREC_FP64) {
         if (p->ev_main0) HIPCHK(hipEventRecord(p->ev_main0, s));
         HIPCHK(grtm::launch_cells_zr(0, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->nw,
-                                     t_lo, n_lo, p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk,
-                                     s));
+                                     t_lo, n_lo, p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk, s, nchk,
+                                     p->chk_stride, tailmx));
         if (p->ev_main1) HIPCHK(hipEventRecord(p->ev_main1, s));
     } else {
         unsigned *cnt = p->flags.as<unsigned>(), *list = cnt + 1;
         HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
         HIPCHK(grtm::launch_cells_zr(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0,
-                                     p->rows64, t_lo, n_lo, p->A0.p, p->A1.p, p->nk, nullptr, nullptr,
-                                     nkw, rowpk, s));
+                                     p->rows64, t_lo, n_lo, p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk, s,
+                                     nchk, p->chk_stride, tailmx));
         if (p->ev_main0) HIPCHK(hipEventRecord(p->ev_main0, s));
         HIPCHK(grtm::launch_cells_zr(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->rows64,
-                                     p->nw - p->rows64, t_hi, n_hi, p->A0.p, p->A1.p, p->nk, cnt, list,
-                                     nkw, rowpk, s));
+                                     p->nw - p->rows64, t_hi, n_hi, p->A0.p, p->A1.p, p->nk, cnt, list, nkw, rowpk, s,
+                                     nchk, p->chk_stride, tailmx));
         if (p->ev_main1) HIPCHK(hipEventRecord(p->ev_main1, s));
     }
     const bool mixed = p->prec != GRTM_PREC_FP64;
     unsigned *cnt = mixed ? p->flags.as<unsigned>() : nullptr, *list = mixed ? cnt + 1 : nullptr;
     if (ad) {
-        // residual check beyond the cut (FP64 on the FP64 rows), dense fallback
-        // of the rows that fail it (each row in its own precision)
+        // dense fallback of the rows whose check failed (each row in its own precision)
         const void *lwf64 = mixed ? p->lw64.p : p->lw.p;
-        HIPCHK(grtm::launch_check_tail(0, p->compat, lwf64, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, r64,
-                                       p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s));
-        HIPCHK(grtm::launch_check_tail(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, r64,
-                                       p->nw - r64, p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s));
         HIPCHK(grtm::launch_fallback(!mixed, p->compat, lwf64, mixed ? p->lw.p : nullptr, r64, p->L, p->g, p->nk,
                                      p->nw, p->dk, p->df, p->omega_i, p->chk_tol, rowpk, tailmx, nkw, fbcnt,
                                      p->fb_list.as<unsigned>(), p->A0.p, p->A1.p, p->nk, cnt, list, 0, s));
@@ -1184,12 +1182,28 @@ static int build_items(grtm_plan *p)
     const int r64 = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;
     std::vector<unsigned> items;
     int n64 = 0, st;
+    // Z / R plans: each launch's residual-check items follow its tiles, so
+    // the check cells run in the launch's tail
+    auto checks = [&](int w_lo, int w_hi) {
+        if (p->kind != 0) return;
+        for (int iw = w_lo; iw < w_hi; iw++)
+            for (int c = 0; c < grtm::cells_tiles_per_row(p->nchk_h[iw]); c++)
+                items.push_back(grtm::cells_tile(iw, c) | grtm::kCheckItem);
+    };
     for (int iw = 0; iw < p->nw; iw++) {
-        if (iw == r64) n64 = (int)items.size();
+        if (iw == r64) {
+            checks(0, r64);
+            n64 = (int)items.size();
+        }
         const int nt = grtm::cells_tiles_per_row(p->nk_pred_h[iw]);
         for (int kb = 0; kb < nt; kb++) items.push_back(grtm::cells_tile(iw, kb));
     }
-    if (r64 >= p->nw) n64 = (int)items.size();
+    if (r64 >= p->nw) {
+        checks(0, p->nw);
+        n64 = (int)items.size();
+    } else {
+        checks(r64, p->nw);
+    }
     if ((st = p->tiles.alloc(sizeof(unsigned) * std::max<size_t>(1, items.size())))) return st;
     if (!items.empty())
         HIPCHK(hipMemcpy(p->tiles.p, items.data(), sizeof(unsigned) * items.size(), hipMemcpyHostToDevice));
@@ -1347,11 +1361,19 @@ int grtm_plan_work(grtm_plan *p, double *w)
     w[5] = oc[0];  // flops per cell
     w[6] = oc[1];  // transcendental calls per cell
     w[7] = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;  // rows solved in FP64
+    // the main launch's residual-check cells (Z / R plans on a learned grid)
+    const bool chk_main = p->adaptive && p->kind == 0;
+    auto nchk_row = [&](int iw) {
+        int n = 0;
+        for (int j = 0; j < p->nchk_h[iw]; j++) n += p->nk_pred_h[iw] + p->chk_stride * j + p->chk_stride / 2 < p->nk;
+        return n;
+    };
     {
-        // cells of the main recursion launch: rows [r0, nw) of the predicted or dense grid
+        // cells of the main recursion launch: rows [r0, nw) of the predicted or
+        // dense grid (and their check cells)
         const int r0 = p->prec == GRTM_PREC_FP64 ? 0 : p->rows64;
         double mc = 0;
-        for (int iw = r0; iw < p->nw; iw++) mc += p->adaptive ? p->nk_pred_h[iw] : p->nk;
+        for (int iw = r0; iw < p->nw; iw++) mc += (p->adaptive ? p->nk_pred_h[iw] : p->nk) + (chk_main ? nchk_row(iw) : 0);
         w[9] = mc;
     }
     {
@@ -1365,11 +1387,16 @@ int grtm_plan_work(grtm_plan *p, double *w)
             std::vector<LayerW<double>> lw((size_t)p->nw * p->L);
             const DBuf &t64 = p->prec == GRTM_PREC_FP64 ? p->lw : p->lw64;
             HIPCHK(hipMemcpy(lw.data(), t64.p, lw.size() * sizeof(LayerW<double>), hipMemcpyDeviceToHost));
+            const double z0 = p->g.branch == 3 ? p->g.zr : p->g.zs;
             for (int iw = r0; iw < p->nw; iw++) {
+                const LayerW<double> *lr = lw.data() + (size_t)iw * p->L;
                 const int nk = p->adaptive ? p->nk_pred_h[iw] : p->nk;
-                for (int ik = 0; ik < nk; ik++)
-                    nbot[grtm::sweep_bottom(lw.data() + (size_t)iw * p->L, p->L, p->g.s,
-                                            p->g.branch == 3 ? p->g.zr : p->g.zs, p->dk * ik)] += 1;
+                for (int ik = 0; ik < nk; ik++) nbot[grtm::sweep_bottom(lr, p->L, p->g.s, z0, p->dk * ik)] += 1;
+                if (chk_main)
+                    for (int j = 0; j < p->nchk_h[iw]; j++) {
+                        const int ik = nk + p->chk_stride * j + p->chk_stride / 2;
+                        if (ik < p->nk) nbot[grtm::sweep_bottom(lr, p->L, p->g.s, z0, p->dk * ik)] += 1;
+                    }
             }
         } else {
             nbot[p->L - 1] = w[9];

@@ -54,14 +54,19 @@ struct MtCols {
 // (non-finite cells appended to flag_list), 2 = FP64 cells / FP32 operands.
 // Work items are (row, tile of 256 wavenumbers), one block each: the list
 // `tiles` (ntiles entries cells_tile(row, tile)), or, when tiles is null,
-// every tile of rows [w0, w0 + nwr).
+// every tile of rows [w0, w0 + nwr).  An item cells_tile(row, c) |
+// kCheckItem solves the residual-check cells 256 c .. 256 c + 255 of the row
+// (as k_check_tail: wavenumbers nk_w[row] + stride j + stride / 2, j <
+// nchk[row]; largest |value| to tail_max[row]) instead of storing operands.
 int cells_tiles_per_row(int nk);
 unsigned cells_tile(int iw, int kb);
+constexpr unsigned kCheckItem = 1u << 31;
 int cells_tile_limit(int nw, int nk);  // 1 if (nw, nk) fit the item encoding
 hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
                            double df, double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles, void *A0,
                            void *A1, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
-                           unsigned *row_peak, hipStream_t st);
+                           unsigned *row_peak, hipStream_t st, const int *nchk = nullptr, int stride = 0,
+                           unsigned *tail_max = nullptr);
 // adaptive grid: residual check beyond the cut, dense fallback, tile extents
 // mt: check the column rows instead of the Z / R kernels (1: displacement
 // columns, 2: with the strain rows)

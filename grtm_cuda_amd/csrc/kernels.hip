@@ -44,13 +44,17 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__re
                                                            unsigned *__restrict__ flag_count,
                                                            unsigned *__restrict__ flag_list,
                                                            const int *__restrict__ nk_w,
-                                                           unsigned *__restrict__ row_peak)
+                                                           unsigned *__restrict__ row_peak,
+                                                           const int *__restrict__ nchk, int stride,
+                                                           unsigned *__restrict__ tail_max)
 {
     __shared__ LayerW<T> lw[kMaxLayers];
     int iw, kb;
+    bool chk = false;
     if (tiles) {
         const unsigned e = tiles[blockIdx.x];
-        iw = int(e >> kTileBits);
+        chk = (e & kCheckItem) != 0;
+        iw = int((e & ~kCheckItem) >> kTileBits);
         kb = int(e & ((1u << kTileBits) - 1));
     } else {
         const int q = int(blockIdx.x) / tpr;
@@ -58,24 +62,33 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__re
         kb = int(blockIdx.x) - q * tpr;
     }
     const int nkr = nk_w ? nk_w[iw] : nk;
-    if (kb * kCellThreads >= nkr) return;  // uniform over the block
+    const int lim = chk ? nchk[iw] : nkr;   // cells (check cells) of the row
+    if (kb * kCellThreads >= lim) return;  // uniform over the block
     for (int j = threadIdx.x; j < L; j += blockDim.x) lw[j] = lw_all[(size_t)iw * L + j];
     __syncthreads();
-    const int ik = kb * kCellThreads + threadIdx.x;
+    const int j = kb * kCellThreads + threadIdx.x;
+    const int ik = chk ? nkr + stride * j + stride / 2 : j;
     unsigned mag = 0;
-    if (ik < nkr) {
+    if (j < lim && ik < nk) {
         TO v[8];
         const bool ok = cell_zr<T, TO, COMPAT, BR>(lw, g, iw, ik, dk, df, omega_i, v);
-        store_zr(v, iw, ik, A0, A1, lda);
-        mag = cell_mag_bits(v);
-        if (!ok && flag_count) {
-            const unsigned slot = atomicAdd(flag_count, 1u);
-            flag_list[slot] = unsigned(iw) * unsigned(nk) + unsigned(ik);
+        if (chk) {
+            float m = 0.f;
+            for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(float(v[i])));
+            mag = __float_as_uint(isfinite(m) ? m : 3.0e38f);  // a non-finite check value fails the row
+        } else {
+            store_zr(v, iw, ik, A0, A1, lda);
+            mag = cell_mag_bits(v);
+            if (!ok && flag_count) {
+                const unsigned slot = atomicAdd(flag_count, 1u);
+                flag_list[slot] = unsigned(iw) * unsigned(nk) + unsigned(ik);
+            }
         }
     }
-    if (row_peak) {
+    unsigned *dst = chk ? tail_max : row_peak;
+    if (dst) {
         mag = wave_max_u32(mag);
-        if ((threadIdx.x & 63) == 0 && mag) atomicMax(row_peak + iw, mag);
+        if ((threadIdx.x & 63) == 0 && mag) atomicMax(dst + iw, mag);
     }
 }
 
@@ -631,7 +644,7 @@ int cells_tile_limit(int nw, int nk)
 hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
                            double df, double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles, void *A0,
                            void *A1, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
-                           unsigned *row_peak, hipStream_t st)
+                           unsigned *row_peak, hipStream_t st, const int *nchk, int stride, unsigned *tail_max)
 {
     const int tpr = cells_tiles_per_row(nk);
     if (!tiles) ntiles = nwr * tpr;
@@ -639,7 +652,7 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
 #define GRTM_ZR_BR(T, TO, CP, BR, FC, FL)                                                                       \
     hipLaunchKernelGGL((k_cells_zr<T, TO, CP, BR>), dim3(ntiles), dim3(kCellThreads), 0, st,                     \
                        (const LayerW<T> *)lw, L, g, nk, dk, df, omega_i, w0, tpr, tiles, (TO *)A0, (TO *)A1, lda, FC, \
-                       FL, nk_w, row_peak)
+                       FL, nk_w, row_peak, nchk, stride, tail_max)
 #define GRTM_ZR(T, TO, CP, FC, FL)                                                 \
     do {                                                                           \
         switch (g.branch) {                                                        \
