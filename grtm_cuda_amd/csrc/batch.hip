@@ -13,7 +13,10 @@ namespace grtm {
 // m0 + j has its geometry and the offset of its layer table in md; its A
 // operand rows are group rows j nw + iw.  Non-finite FP32 cells go to the
 // flag list as (j nw + iw) nk + ik.  Adaptive grids: nk_w[group row] limits
-// the wavenumbers solved and row_peak[group row] collects the row maxima.
+// the wavenumbers solved and row_peak[group row] collects the row maxima; an
+// item whose row / tile word carries kCheckItem solves the row's residual-check
+// cells BS c .. BS c + BS - 1 (wavenumbers nk_w + stride j + stride / 2, j <
+// nchk[group row]) and takes their largest |value| to tail_max[group row].
 // BR: the receiver branch shared by every model of the launch's items.
 template <typename T, typename TO, bool COMPAT, int BS, int BR>
 __global__ __launch_bounds__(BS, sizeof(T) == 4 ? 4 : 1) void k_cells_zr_batch(const ModelDesc *__restrict__ md, int m0,
@@ -24,35 +27,105 @@ __global__ __launch_bounds__(BS, sizeof(T) == 4 ? 4 : 1) void k_cells_zr_batch(c
                                                        unsigned *__restrict__ flag_count,
                                                        unsigned *__restrict__ flag_list,
                                                        const int *__restrict__ nk_w,
-                                                       unsigned *__restrict__ row_peak)
+                                                       unsigned *__restrict__ row_peak,
+                                                       const int *__restrict__ nchk, int stride,
+                                                       unsigned *__restrict__ tail_max)
 {
     __shared__ LayerW<T> lw[kMaxLayers];
     const unsigned long long e = items[blockIdx.x];
     const int j = int(e >> 32);
     const unsigned lo = unsigned(e & 0xffffffffull);
-    const int iw = int(lo >> kTileBits), kb = int(lo & ((1u << kTileBits) - 1));
+    const bool chk = (lo & kCheckItem) != 0;
+    const int iw = int((lo & ~kCheckItem) >> kTileBits), kb = int(lo & ((1u << kTileBits) - 1));
     const int grow = j * nw + iw;  // row of the group
     const int nkr = nk_w ? nk_w[grow] : nk;
-    if (kb * BS >= nkr) return;  // uniform over the block
+    const int lim = chk ? nchk[grow] : nkr;  // cells (check cells) of the row
+    if (kb * BS >= lim) return;  // uniform over the block
     const Geo g = md[m0 + j].g;
     const size_t off = md[m0 + j].lw_off + (size_t)iw * g.L;
     for (int q = threadIdx.x; q < g.L; q += BS) lw[q] = lw_all[off + q];
     __syncthreads();
-    const int ik = kb * BS + threadIdx.x;
+    const int jj = kb * BS + threadIdx.x;
+    const int ik = chk ? nkr + stride * jj + stride / 2 : jj;
     unsigned mag = 0;
-    if (ik < nkr) {
+    if (jj < lim && ik < nk) {
         TO v[8];
         const bool ok = cell_zr<T, TO, COMPAT, BR>(lw, g, iw, ik, dk, df, omega_i, v);
-        store_zr(v, grow, ik, A0, A1, lda);
-        mag = cell_mag_bits(v);
-        if (!ok && flag_count) {
-            const unsigned slot = atomicAdd(flag_count, 1u);
-            flag_list[slot] = unsigned(grow) * unsigned(nk) + unsigned(ik);
+        if (chk) {
+            float m = 0.f;
+            for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(float(v[i])));
+            mag = __float_as_uint(isfinite(m) ? m : 3.0e38f);  // a non-finite check value fails the row
+        } else {
+            store_zr(v, grow, ik, A0, A1, lda);
+            mag = cell_mag_bits(v);
+            if (!ok && flag_count) {
+                const unsigned slot = atomicAdd(flag_count, 1u);
+                flag_list[slot] = unsigned(grow) * unsigned(nk) + unsigned(ik);
+            }
         }
     }
-    if (row_peak) {
+    unsigned *dst = chk ? tail_max : row_peak;
+    if (dst) {
         mag = wave_max_u32(mag);
-        if ((threadIdx.x & 63) == 0 && mag) atomicMax(row_peak + grow, mag);
+        if ((threadIdx.x & 63) == 0 && mag) atomicMax(dst + grow, mag);
+    }
+}
+
+// Rows of a group whose checked tail exceeds tol x row peak fall back to the
+// dense grid: appended to the group's list (group row, then the first
+// wavenumber still to solve at nrow + slot) and counted per model.
+__global__ void k_fallback_select_batch(int nrow, int nw, int nk, double tol, const unsigned *__restrict__ row_peak,
+                                        const unsigned *__restrict__ tail_max, int *__restrict__ nk_w,
+                                        unsigned *__restrict__ count, unsigned *__restrict__ list,
+                                        unsigned *__restrict__ model_count)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    const float pk = __uint_as_float(row_peak[r]), tm = __uint_as_float(tail_max[r]);
+    if (nk_w[r] < nk && !(tm <= float(tol) * pk)) {
+        const unsigned slot = atomicAdd(count, 1u);
+        list[slot] = unsigned(r);
+        list[nrow + slot] = unsigned(nk_w[r]);
+        nk_w[r] = nk;
+        atomicAdd(model_count + r / nw, 1u);
+    }
+}
+
+// Dense completion of a group's fallback rows: wavenumbers [start, nk) of each
+// listed row in the arithmetic T of that row (FP64 below its model's r64:
+// F64ROWS true; FP32 above: F64ROWS false).  Grid (k blocks, row slots);
+// slots stride over the list.  FP32 cells that are not finite go to the flag
+// list as group row x nk + ik.
+template <typename T, typename TO, bool COMPAT, bool F64ROWS>
+__global__ __launch_bounds__(kCellThreads) void k_fallback_batch(const ModelDesc *__restrict__ md, int m0,
+                                                                 const LayerW<T> *__restrict__ lw_all, int nw, int nk,
+                                                                 double dk, double df, double omega_i,
+                                                                 TO *__restrict__ A0, TO *__restrict__ A1, long lda,
+                                                                 const unsigned *__restrict__ count,
+                                                                 const unsigned *__restrict__ list, int nrow,
+                                                                 unsigned *__restrict__ flag_count,
+                                                                 unsigned *__restrict__ flag_list)
+{
+    __shared__ LayerW<T> lw[kMaxLayers];
+    const unsigned n = *count;
+    for (unsigned slot = blockIdx.y; slot < n; slot += gridDim.y) {
+        const int grow = int(list[slot]);
+        const int j = grow / nw, iw = grow - j * nw;
+        const ModelDesc d = md[m0 + j];
+        if ((iw < d.r64) != F64ROWS) continue;  // uniform over the block
+        const int start = int(list[nrow + slot]);
+        __syncthreads();
+        for (int q = threadIdx.x; q < d.g.L; q += blockDim.x) lw[q] = lw_all[d.lw_off + (size_t)iw * d.g.L + q];
+        __syncthreads();
+        for (int ik = start + blockIdx.x * blockDim.x + threadIdx.x; ik < nk; ik += gridDim.x * blockDim.x) {
+            TO v[8];
+            const bool ok = cell_zr<T, TO, COMPAT>(lw, d.g, iw, ik, dk, df, omega_i, v);
+            store_zr(v, grow, ik, A0, A1, lda);
+            if (!ok && flag_count) {
+                const unsigned s2 = atomicAdd(flag_count, 1u);
+                flag_list[s2] = unsigned(grow) * unsigned(nk) + unsigned(ik);
+            }
+        }
     }
 }
 
@@ -83,13 +156,13 @@ hipError_t launch_cells_zr_batch(int prec, int compat, int bs, int br, const Mod
                                  int nw, int nk, double dk, double df, double omega_i,
                                  const unsigned long long *items, int nitems, void *A0, void *A1, long lda,
                                  unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
-                                 hipStream_t st)
+                                 hipStream_t st, const int *nchk, int stride, unsigned *tail_max)
 {
     if (nitems <= 0) return hipSuccess;
 #define GRTM_ZB(T, TO, CP, BSZ, BR, FC, FL)                                                                      \
     hipLaunchKernelGGL((k_cells_zr_batch<T, TO, CP, BSZ, BR>), dim3(nitems), dim3(BSZ), 0, st, md, m0,          \
                        (const LayerW<T> *)lw_all, nw, nk, dk, df, omega_i, items, (TO *)A0, (TO *)A1, lda, FC, FL, \
-                       nk_w, row_peak)
+                       nk_w, row_peak, nchk, stride, tail_max)
 #define GRTM_ZB_BR(T, TO, CP, BSZ, FC, FL)                           \
     do {                                                             \
         if (br == 1) GRTM_ZB(T, TO, CP, BSZ, 1, FC, FL);             \
@@ -118,6 +191,33 @@ hipError_t launch_cells_zr_batch(int prec, int compat, int bs, int br, const Mod
 #undef GRTM_ZB_BS
 #undef GRTM_ZB_BR
 #undef GRTM_ZB
+    return hipGetLastError();
+}
+
+hipError_t launch_fallback_batch(int prec, int compat, const ModelDesc *md, int m0, const void *lw64, const void *lw32,
+                                 int nrow, int nw, int nk, double dk, double df, double omega_i, double tol,
+                                 const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *count,
+                                 unsigned *list, unsigned *model_count, void *A0, void *A1, long lda,
+                                 unsigned *flag_count, unsigned *flag_list, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fallback_select_batch, dim3((nrow + 255) / 256), dim3(256), 0, st, nrow, nw, nk, tol, row_peak,
+                       tail_max, nk_w, count, list, model_count);
+    const dim3 grid((nk + kCellThreads - 1) / kCellThreads, 16);
+#define GRTM_FBB(T, TO, CP, F64, LWP, FC, FL)                                                                    \
+    hipLaunchKernelGGL((k_fallback_batch<T, TO, CP, F64>), grid, dim3(kCellThreads), 0, st, md, m0,              \
+                       (const LayerW<T> *)LWP, nw, nk, dk, df, omega_i, (TO *)A0, (TO *)A1, lda, count, list, nrow, \
+                       FC, FL)
+    if (prec == 0) {
+        if (compat) GRTM_FBB(double, double, true, true, lw64, nullptr, nullptr);
+        else GRTM_FBB(double, double, false, true, lw64, nullptr, nullptr);
+    } else if (compat) {
+        GRTM_FBB(double, float, true, true, lw64, nullptr, nullptr);
+        GRTM_FBB(float, float, true, false, lw32, flag_count, flag_list);
+    } else {
+        GRTM_FBB(double, float, false, true, lw64, nullptr, nullptr);
+        GRTM_FBB(float, float, false, false, lw32, flag_count, flag_list);
+    }
+#undef GRTM_FBB
     return hipGetLastError();
 }
 

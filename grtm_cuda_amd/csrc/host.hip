@@ -1685,6 +1685,7 @@ struct grtm_batch {
     double chk_tol = 3e-6;
     std::vector<int> nk_pred_h, nchk_h, max_chk;
     DBuf nk_pred, nk_w, nchk, row_peak, tail_max, fb_count, fb_list, kend;
+    DBuf fb_gcount;  // fallback rows listed per group (one counter per group)
     ~grtm_batch()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -1733,6 +1734,19 @@ int batch_items(grtm_batch *b, int bs, int g0, int g1)
                         for (int kb = 0, nt = ntile(g.m0 + j, iw); kb < nt; kb++)
                             it.push_back(((unsigned long long)j << 32) | grtm::cells_tile(iw, kb));
                 }
+                // learned grids: the residual-check items of these rows follow
+                // their tiles, so the check cells run in the launch's tail
+                if (b->adaptive)
+                    for (int j : order) {
+                        if (b->md_h[g.m0 + j].g.branch != br) continue;
+                        const int r64 = batch_mixed(b) ? b->rows64[g.m0 + j] : b->nw;
+                        const int lo = pass == 0 ? 0 : r64, hi = pass == 0 ? r64 : b->nw;
+                        for (int iw = lo; iw < hi; iw++) {
+                            const int nc = b->nchk_h[(size_t)(g.m0 + j) * b->nw + iw];
+                            for (int c = 0; c * bs < nc; c++)
+                                it.push_back(((unsigned long long)j << 32) | grtm::cells_tile(iw, c) | grtm::kCheckItem);
+                        }
+                    }
                 g.off[pass][br - 1] = start;
                 g.cnt[pass][br - 1] = it.size() - start;
             }
@@ -1757,12 +1771,16 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
         HIPCHK(grtm::launch_zero_u32(b->tail_max.as<unsigned>() + r0, (int)nrow, s));
         HIPCHK(grtm::launch_zero_u32(b->fb_count.as<unsigned>() + g.m0, g.n, s));
     }
+    // learned grids: the item lists carry the residual-check cells (FP64 on the FP64 rows)
+    const int *nchk = b->adaptive ? b->nchk.as<int>() + r0 : nullptr;
+    unsigned *tailmx = b->adaptive ? b->tail_max.as<unsigned>() + r0 : nullptr;
     if (!batch_mixed(b)) {
         for (int br = 1; br <= 4; br++)
             if (g.cnt[0][br - 1])
                 HIPCHK(grtm::launch_cells_zr_batch(0, b->compat, b->bs, br, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk,
                                                    b->df, b->omega_i, it + g.off[0][br - 1], (int)g.cnt[0][br - 1],
-                                                   b->A0.p, b->A1.p, b->nk, nullptr, nullptr, nkw, rowpk, s));
+                                                   b->A0.p, b->A1.p, b->nk, nullptr, nullptr, nkw, rowpk, s, nchk,
+                                                   b->chk_stride, tailmx));
     } else {
         unsigned *cnt = b->flags.as<unsigned>(), *list = cnt + 1;
         HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
@@ -1770,12 +1788,14 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
             if (g.cnt[0][br - 1])
                 HIPCHK(grtm::launch_cells_zr_batch(2, b->compat, b->bs, br, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk,
                                                    b->df, b->omega_i, it + g.off[0][br - 1], (int)g.cnt[0][br - 1],
-                                                   b->A0.p, b->A1.p, b->nk, nullptr, nullptr, nkw, rowpk, s));
+                                                   b->A0.p, b->A1.p, b->nk, nullptr, nullptr, nkw, rowpk, s, nchk,
+                                                   b->chk_stride, tailmx));
         for (int br = 1; br <= 4; br++)
             if (g.cnt[1][br - 1])
                 HIPCHK(grtm::launch_cells_zr_batch(1, b->compat, b->bs, br, md, g.m0, b->lw32.p, b->nw, b->nk, b->dk,
                                                    b->df, b->omega_i, it + g.off[1][br - 1], (int)g.cnt[1][br - 1],
-                                                   b->A0.p, b->A1.p, b->nk, cnt, list, nkw, rowpk, s));
+                                                   b->A0.p, b->A1.p, b->nk, cnt, list, nkw, rowpk, s, nchk,
+                                                   b->chk_stride, tailmx));
     }
     unsigned *cnt = batch_mixed(b) ? b->flags.as<unsigned>() : nullptr, *list = cnt ? cnt + 1 : nullptr;
     if (!b->adaptive) {
@@ -1784,32 +1804,15 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
                                                 b->omega_i, b->A0.p, b->A1.p, b->nk, cnt, list, s));
         return GRTM_OK;
     }
-    // per model: residual check beyond the cut (FP64 on the FP64 rows), dense
-    // fallback of the rows that fail it, then the group's tile extents
-    const size_t es = esize(b->prec);
-    for (int j = 0; j < g.n; j++) {
-        const int m = g.m0 + j;
-        const grtm::ModelDesc &d = b->md_h[m];
-        const int L = d.g.L;
-        const size_t mr = (size_t)m * b->nw;
-        const int r64 = batch_mixed(b) ? b->rows64[m] : b->nw;
-        const void *l64 = b->lw64.as<LayerW<double>>() + d.lw_off;
-        int *nkw_m = b->nk_w.as<int>() + mr;
-        unsigned *tail_m = b->tail_max.as<unsigned>() + mr;
-        HIPCHK(grtm::launch_check_tail(0, b->compat, l64, L, d.g, b->nk, b->dk, b->df, b->omega_i, 0, r64,
-                                       b->max_chk[m], nkw_m, b->nchk.as<int>() + mr, b->chk_stride, tail_m, s));
-        if (batch_mixed(b))
-            HIPCHK(grtm::launch_check_tail(1, b->compat, b->lw32.as<LayerW<float>>() + d.lw_off, L, d.g, b->nk, b->dk,
-                                           b->df, b->omega_i, r64, b->nw - r64, b->max_chk[m], nkw_m,
-                                           b->nchk.as<int>() + mr, b->chk_stride, tail_m, s));
-        unsigned char *a0 = (unsigned char *)b->A0.p + (size_t)j * 4 * b->nw * b->nk * es;
-        unsigned char *a1 = (unsigned char *)b->A1.p + (size_t)j * 4 * b->nw * b->nk * es;
-        HIPCHK(grtm::launch_fallback(!batch_mixed(b), b->compat, l64,
-                                     batch_mixed(b) ? (const void *)(b->lw32.as<LayerW<float>>() + d.lw_off) : nullptr,
-                                     r64, L, d.g, b->nk, b->nw, b->dk, b->df, b->omega_i, b->chk_tol,
-                                     b->row_peak.as<unsigned>() + mr, tail_m, nkw_m, b->fb_count.as<unsigned>() + m,
-                                     b->fb_list.as<unsigned>() + 2 * mr, a0, a1, b->nk, cnt, list, j * b->nw, s));
-    }
+    // dense fallback of the group's rows whose check failed (each row in its
+    // own precision), then the group's tile extents
+    const int gi = int(&g - b->groups.data());
+    unsigned *gcount = b->fb_gcount.as<unsigned>() + gi;
+    HIPCHK(grtm::launch_zero_u32(gcount, 1, s));
+    HIPCHK(grtm::launch_fallback_batch(batch_mixed(b) ? 1 : 0, b->compat, md, g.m0, b->lw64.p, b->lw32.p, (int)nrow,
+                                       b->nw, b->nk, b->dk, b->df, b->omega_i, b->chk_tol, rowpk, tailmx, nkw, gcount,
+                                       b->fb_list.as<unsigned>() + 2 * r0, b->fb_count.as<unsigned>() + g.m0, b->A0.p,
+                                       b->A1.p, b->nk, cnt, list, s));
     if (cnt)
         HIPCHK(grtm::launch_cells_fix_batch(b->compat, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df, b->omega_i,
                                             b->A0.p, b->A1.p, b->nk, cnt, list, s));
@@ -1991,7 +1994,7 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
         b->rows64[m] = mt.rows64;
         b->md_h[m].g = mt.g;
         b->md_h[m].lw_off = (long)all64.size();
-        b->md_h[m].pad = 0;
+        b->md_h[m].r64 = precision != GRTM_PREC_FP64 ? mt.rows64 : b->nw;
         all64.insert(all64.end(), mt.lw.begin(), mt.lw.end());
         double oc[2] = {0, 0};
         if (grtm_opcount_cell(mt.st.L, mt.g.s, mt.g.rl, mt.g.branch, mt.g.up, oc)) return GRTM_E_INPUT;
@@ -2088,6 +2091,7 @@ int grtm_batch_set_adaptive(grtm_batch *b, const double *kcut, const double *win
                 (st = b->nchk.alloc(sizeof(int) * n)) || (st = b->row_peak.alloc(sizeof(unsigned) * n)) ||
                 (st = b->tail_max.alloc(sizeof(unsigned) * n)) || (st = b->fb_count.alloc(sizeof(unsigned) * b->nm)) ||
                 (st = b->fb_list.alloc(sizeof(unsigned) * 2 * n)) ||
+                (st = b->fb_gcount.alloc(sizeof(unsigned) * b->groups.size())) ||
                 (st = b->kend.alloc(sizeof(int) * b->groups.size() * batch_kend_per_group(b))))
                 return st;
         }

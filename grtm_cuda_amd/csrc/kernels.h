@@ -162,7 +162,7 @@ hipError_t launch_post_ifft_cols(int prec, const PostParams &pp, const MtCols *m
 struct ModelDesc {
     Geo g;
     long lw_off;
-    long pad;
+    long r64;  // frequency rows [0, r64) solved in FP64 (all rows of an FP64 batch)
 };
 // Recursion of a group of models; items (j << 32) | cells_tile(row, tile)
 // with tiles of bs in {64, 128, 256} wavenumbers; prec as launch_cells_zr;
@@ -171,7 +171,18 @@ hipError_t launch_cells_zr_batch(int prec, int compat, int bs, int br, const Mod
                                  int nw, int nk, double dk, double df, double omega_i,
                                  const unsigned long long *items, int nitems, void *A0, void *A1, long lda,
                                  unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
-                                 hipStream_t st);
+                                 hipStream_t st, const int *nchk = nullptr, int stride = 0,
+                                 unsigned *tail_max = nullptr);
+// Dense fallback of a group's rows whose residual check failed (learned
+// grids): one selection over the group's nrow rows (list: group rows, then
+// their first wavenumbers; count: the group's counter; model_count: per
+// model), then the completion of the listed rows, FP64 below each model's r64
+// and in prec (0 FP64, else FP32) above.
+hipError_t launch_fallback_batch(int prec, int compat, const ModelDesc *md, int m0, const void *lw64, const void *lw32,
+                                 int nrow, int nw, int nk, double dk, double df, double omega_i, double tol,
+                                 const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *count,
+                                 unsigned *list, unsigned *model_count, void *A0, void *A1, long lda,
+                                 unsigned *flag_count, unsigned *flag_list, hipStream_t st);
 hipError_t launch_cells_fix_batch(int compat, const ModelDesc *md, int m0, const void *lw64, int nw, int nk, double dk,
                                   double df, double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,
                                   const unsigned *flag_list, hipStream_t st);
