@@ -667,29 +667,28 @@ int run_cells_mt(grtm_plan *p, hipStream_t s)
     const unsigned *t_hi = ad ? t_lo + p->ntiles64 : nullptr;
     const int n_lo = ad ? p->ntiles64 : 0, n_hi = ad ? p->ntiles - p->ntiles64 : 0;
     unsigned *cnt = mixed ? p->flags.as<unsigned>() : nullptr, *list = mixed ? cnt + 1 : nullptr;
+    // learned grid: the item lists carry the residual check beyond the cut
+    // (FP64 on the FP64 rows) after each launch's tiles
+    const int *nchk = ad ? p->nchk.as<int>() : nullptr;
     if (!mixed) {
         if (p->ev_main0) HIPCHK(hipEventRecord(p->ev_main0, s));
         HIPCHK(grtm::launch_cells_mt(0, p->compat, p->deriv, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0,
-                                     p->nw, t_lo, n_lo, p->mto, p->nk, nullptr, nullptr, nkw, rowpk, s));
+                                     p->nw, t_lo, n_lo, p->mto, p->nk, nullptr, nullptr, nkw, rowpk, s, nchk,
+                                     p->chk_stride, tailmx));
         if (p->ev_main1) HIPCHK(hipEventRecord(p->ev_main1, s));
     } else {
         HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
         HIPCHK(grtm::launch_cells_mt(2, p->compat, p->deriv, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i,
-                                     0, p->rows64, t_lo, n_lo, p->mto, p->nk, nullptr, nullptr, nkw, rowpk, s));
+                                     0, p->rows64, t_lo, n_lo, p->mto, p->nk, nullptr, nullptr, nkw, rowpk, s, nchk,
+                                     p->chk_stride, tailmx));
         if (p->ev_main0) HIPCHK(hipEventRecord(p->ev_main0, s));
         HIPCHK(grtm::launch_cells_mt(1, p->compat, p->deriv, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i,
                                      p->rows64, p->nw - p->rows64, t_hi, n_hi, p->mto, p->nk, cnt, list, nkw, rowpk,
-                                     s));
+                                     s, nchk, p->chk_stride, tailmx));
         if (p->ev_main1) HIPCHK(hipEventRecord(p->ev_main1, s));
     }
     if (ad) {
         const void *lwf64 = mixed ? p->lw64.p : p->lw.p;
-        HIPCHK(grtm::launch_check_tail(0, p->compat, lwf64, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, r64,
-                                       p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s,
-                                       p->deriv ? 2 : 1));
-        HIPCHK(grtm::launch_check_tail(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, r64,
-                                       p->nw - r64, p->max_chk, nkw, p->nchk.as<int>(), p->chk_stride, tailmx, s,
-                                       p->deriv ? 2 : 1));
         HIPCHK(grtm::launch_fallback(!mixed, p->compat, lwf64, mixed ? p->lw.p : nullptr, r64, p->L, p->g, p->nk,
                                      p->nw, p->dk, p->df, p->omega_i, p->chk_tol, rowpk, tailmx, nkw, fbcnt,
                                      p->fb_list.as<unsigned>(), nullptr, nullptr, p->nk, cnt, list, 0, s, &p->mto,
@@ -1182,10 +1181,9 @@ static int build_items(grtm_plan *p)
     const int r64 = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;
     std::vector<unsigned> items;
     int n64 = 0, st;
-    // Z / R plans: each launch's residual-check items follow its tiles, so
-    // the check cells run in the launch's tail
+    // each launch's residual-check items follow its tiles, so the check cells
+    // run in the launch's tail
     auto checks = [&](int w_lo, int w_hi) {
-        if (p->kind != 0) return;
         for (int iw = w_lo; iw < w_hi; iw++)
             for (int c = 0; c < grtm::cells_tiles_per_row(p->nchk_h[iw]); c++)
                 items.push_back(grtm::cells_tile(iw, c) | grtm::kCheckItem);
@@ -1361,8 +1359,8 @@ int grtm_plan_work(grtm_plan *p, double *w)
     w[5] = oc[0];  // flops per cell
     w[6] = oc[1];  // transcendental calls per cell
     w[7] = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;  // rows solved in FP64
-    // the main launch's residual-check cells (Z / R plans on a learned grid)
-    const bool chk_main = p->adaptive && p->kind == 0;
+    // the main launch's residual-check cells (learned grid)
+    const bool chk_main = p->adaptive;
     auto nchk_row = [&](int iw) {
         int n = 0;
         for (int j = 0; j < p->nchk_h[iw]; j++) n += p->nk_pred_h[iw] + p->chk_stride * j + p->chk_stride / 2 < p->nk;

@@ -56,8 +56,8 @@ struct MtCols {
 // `tiles` (ntiles entries cells_tile(row, tile)), or, when tiles is null,
 // every tile of rows [w0, w0 + nwr).  An item cells_tile(row, c) |
 // kCheckItem solves the residual-check cells 256 c .. 256 c + 255 of the row
-// (as k_check_tail: wavenumbers nk_w[row] + stride j + stride / 2, j <
-// nchk[row]; largest |value| to tail_max[row]) instead of storing operands.
+// (wavenumbers nk_w[row] + stride j + stride / 2, j < nchk[row]; largest
+// |value| to tail_max[row], 3e38 if not finite) instead of storing operands.
 int cells_tiles_per_row(int nk);
 unsigned cells_tile(int iw, int kb);
 constexpr unsigned kCheckItem = 1u << 31;
@@ -67,12 +67,7 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
                            void *A1, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
                            unsigned *row_peak, hipStream_t st, const int *nchk = nullptr, int stride = 0,
                            unsigned *tail_max = nullptr);
-// adaptive grid: residual check beyond the cut, dense fallback, tile extents
-// mt: check the column rows instead of the Z / R kernels (1: displacement
-// columns, 2: with the strain rows)
-hipError_t launch_check_tail(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk, double df,
-                             double omega_i, int w0, int nwr, int max_chk, const int *nk_w, const int *nchk,
-                             int stride, unsigned *tail_max, hipStream_t st, int mt = 0);
+// adaptive grid: dense fallback, tile extents
 // Rows whose checked tail exceeds tol x row peak are completed on the dense
 // grid: FP64 plans in FP64; mixed plans in FP64 below row r64 and in FP32
 // (lw32) above it, non-finite FP32 cells appended to the flag list as
@@ -149,7 +144,8 @@ hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr,
 hipError_t launch_cells_mt(int prec, int compat, int deriv, const void *lw, int L, const Geo &g, int nk, double dk,
                            double df, double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles,
                            const MtOps &o, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
-                           unsigned *row_peak, hipStream_t st);
+                           unsigned *row_peak, hipStream_t st, const int *nchk = nullptr, int stride = 0,
+                           unsigned *tail_max = nullptr);
 hipError_t launch_cells_mt_fix(int compat, int deriv, const void *lw64, int L, const Geo &g, int nk, double dk,
                                double df, double omega_i, const MtOps &o, long lda, const unsigned *flag_count,
                                const unsigned *flag_list, hipStream_t st);

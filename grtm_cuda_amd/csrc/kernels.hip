@@ -6,8 +6,9 @@
 //   k_cells_full   per-(omega,k) recursion -> all 30 integrand columns of the
 //                  drop-in output at one distance, reduced over k per block
 //   k_green_finish ordered sum of the per-block partials
-//   k_check_tail, k_fallback_cells  residual check and dense completion of
-//                  the learned adaptive grid
+//   k_fallback_select, k_fallback_cells  row selection and dense completion
+//                  of the learned adaptive grid (the residual check runs as
+//                  flagged items of the recursion launches)
 //   k_bessel_tab   J_n(k r) tables (B operands of the contraction)
 //   k_gemm         C = A B on VALU (reference-precision path)
 //   k_post_ifft    phase / taper / scale + Hermitian C2R inverse FFT
@@ -92,44 +93,6 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__re
     }
 }
 
-// Residual check of the truncated grid: wavenumbers nk_w[iw] + stride j + stride/2,
-// j < nchk, are solved (values are not stored); their largest |value| goes to
-// tail_max[iw].  Rows [w0, w0 + gridDim.y).
-// MT: the column rows of the cells instead of the Z / R kernels (1:
-// displacement columns, 2: with the strain rows).
-template <typename T, bool COMPAT, int MT>
-__global__ __launch_bounds__(kCellThreads) void k_check_tail(const LayerW<T> *__restrict__ lw_all, int L, Geo g,
-                                                             int nk, double dk, double df, double omega_i, int w0,
-                                                             const int *__restrict__ nk_w, const int *__restrict__ nchk,
-                                                             int stride, unsigned *__restrict__ tail_max)
-{
-    __shared__ LayerW<T> lw[kMaxLayers];
-    const int iw = w0 + blockIdx.y;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nc = nchk[iw];
-    if ((int)(blockIdx.x * blockDim.x) >= nc) return;
-    for (int q = threadIdx.x; q < L; q += blockDim.x) lw[q] = lw_all[(size_t)iw * L + q];
-    __syncthreads();
-    const int ik = nk_w[iw] + stride * j + stride / 2;
-    unsigned mb = 0;
-    if (j < nc && ik < nk) {
-        float m = 0.f;
-        if constexpr (MT != 0) {
-            cell_mt_rows<T, COMPAT, MT == 2>(lw, g, iw, ik, dk, df, omega_i, [&](int, int, T vr, T vi) {
-                m = fmaxf(m, fmaxf(fabsf(float(vr)), fabsf(float(vi))));
-            });
-        } else {
-            float v[8];
-            cell_zr<T, float, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
-            for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(v[i]));
-        }
-        if (!isfinite(m)) m = 3.0e38f;  // a non-finite check value fails the row
-        mb = __float_as_uint(m);
-    }
-    mb = wave_max_u32(mb);
-    if ((threadIdx.x & 63) == 0 && mb) atomicMax(tail_max + iw, mb);
-}
-
 // Rows whose checked tail exceeds tol * row peak fall back to the dense grid.
 __global__ void k_fallback_select(int nw, int nk, double tol, const unsigned *__restrict__ row_peak,
                                   const unsigned *__restrict__ tail_max, int *__restrict__ nk_w,
@@ -150,7 +113,8 @@ __global__ void k_fallback_select(int nw, int nk, double tol, const unsigned *__
 // row whose index lies in [r_lo, r_hi), in the arithmetic T of that row's
 // recursion.  Grid (k blocks, row slots); slots stride over the list.  FP32
 // cells that are not finite go to the flag list as (row_base + iw) nk + ik.
-// MT: column rows (as k_check_tail), stored through o (A0, A1 unused).
+// MT: column rows (1: displacement columns, 2: with the strain rows), stored
+// through o (A0, A1 unused).
 template <typename T, typename TO, bool COMPAT, int MT>
 __global__ __launch_bounds__(kCellThreads) void k_fallback_cells(
     const LayerW<T> *__restrict__ lw_all, int L, Geo g, int nk, int nw, double dk, double df, double omega_i,
@@ -674,38 +638,6 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
     }
 #undef GRTM_ZR
 #undef GRTM_ZR_BR
-    return hipGetLastError();
-}
-
-hipError_t launch_check_tail(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk, double df,
-                             double omega_i, int w0, int nwr, int max_chk, const int *nk_w, const int *nchk,
-                             int stride, unsigned *tail_max, hipStream_t st, int mt)
-{
-    if (nwr <= 0 || max_chk <= 0) return hipSuccess;
-    dim3 grid((max_chk + kCellThreads - 1) / kCellThreads, nwr);
-#define GRTM_CHK(T, CP, MT)                                                                                     \
-    hipLaunchKernelGGL((k_check_tail<T, CP, MT>), grid, dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, L, g, nk, \
-                       dk, df, omega_i, w0, nk_w, nchk, stride, tail_max)
-    if (mt == 2) {
-        if (compat) return hipErrorInvalidValue;
-        if (prec == 0) GRTM_CHK(double, false, 2);
-        else GRTM_CHK(float, false, 2);
-    } else if (mt == 1) {
-        if (prec == 0) {
-            if (compat) GRTM_CHK(double, true, 1);
-            else GRTM_CHK(double, false, 1);
-        } else {
-            if (compat) GRTM_CHK(float, true, 1);
-            else GRTM_CHK(float, false, 1);
-        }
-    } else if (prec == 0) {
-        if (compat) GRTM_CHK(double, true, 0);
-        else GRTM_CHK(double, false, 0);
-    } else {
-        if (compat) GRTM_CHK(float, true, 0);
-        else GRTM_CHK(float, false, 0);
-    }
-#undef GRTM_CHK
     return hipGetLastError();
 }
 
