@@ -12,7 +12,7 @@
 //   k_bessel_tab   J_n(k r) tables (B operands of the contraction)
 //   k_gemm         C = A B on VALU (reference-precision path)
 //   k_post_ifft    phase / taper / scale + Hermitian C2R inverse FFT
-//                  (Stockham, in LDS) + exp(omega_I t) scaling; k_post_ifft_cols
+//                  (Stockham radix 4, in LDS) + exp(omega_I t) scaling; k_post_ifft_cols
 //                  the same for column traces assembled from several outputs
 // The batch recursion is in batch.hip, the column recursion in columns.hip,
 // the cell helpers they share in cellops.h.
@@ -471,8 +471,10 @@ __device__ __forceinline__ cx<T> post_val(cx<T> v, int iw, const PostParams &pp)
     return v * C<T>(pv[2 * iw], pv[2 * iw + 1]);
 }
 
-// Phase / taper / scale, Hermitian C2R inverse FFT (Stockham radix 2 in LDS)
-// and time scaling of trace t, whose raw spectrum row iw is load(iw).
+// Phase / taper / scale, Hermitian C2R inverse FFT (Stockham in LDS: radix-4
+// stages, one radix-2 stage when log2 of the half length is odd) and time
+// scaling of trace t, whose raw spectrum row iw is load(iw).  256 threads.
+constexpr int kIfftThreads = 256;
 template <typename T, typename Load>
 __device__ __forceinline__ void post_ifft_trace(const PostParams &pp, Load &&load, int t, T *__restrict__ traces,
                                                 T *__restrict__ spec_post)
@@ -482,40 +484,72 @@ __device__ __forceinline__ void post_ifft_trace(const PostParams &pp, Load &&loa
     const int M = pp.nw, N = 2 * pp.nw;
     cx<T> *bufB = bufA + M;
     const T *tw = static_cast<const T *>(pp.tw), *ts = static_cast<const T *>(pp.ts);
-    for (int m = threadIdx.x; m < M; m += blockDim.x) {
-        cx<T> Xm = post_val(load(m), m, pp);
+    // each spectrum value is loaded once, into bufB (free until the first
+    // stage); the loop is unrolled so that several loads are in flight
+#pragma unroll 4
+    for (int m = threadIdx.x; m < M; m += kIfftThreads) {
+        const cx<T> Xm = post_val(load(m), m, pp);
         if (spec_post) {
             spec_post[((size_t)t * M + m) * 2] = Xm.x;
             spec_post[((size_t)t * M + m) * 2 + 1] = Xm.y;
         }
+        bufB[m] = Xm;
+    }
+    __syncthreads();
+    // Hermitian fold to a half-length complex sequence
+#pragma unroll 4
+    for (int m = threadIdx.x; m < M; m += kIfftThreads) {
+        cx<T> Xm = bufB[m];
         cx<T> Xc = C<T>(0, 0);  // conj(X[M-m]); X[M] (Nyquist) = 0
-        if (m > 0) Xc = conj(post_val(load(M - m), M - m, pp));
+        if (m > 0) Xc = conj(bufB[M - m]);
         if (m == 0) Xm.y = T(0);  // Im X[0] only reaches Im of the output
-        if (M - m == 0) Xc.y = T(0);
         const cx<T> Xe = Xm + Xc;
         const cx<T> Xo = (Xm - Xc) * C<T>(tw[2 * m], tw[2 * m + 1]);  // e^{+2 pi i m / N}
         bufA[m] = C<T>(Xe.x - Xo.y, Xe.y + Xo.x);  // Xe + i Xo
     }
     __syncthreads();
-    // Stockham radix-2 (decimation in frequency), inverse sign
+    // Stockham decimation in frequency, inverse sign: a length-n sub-transform
+    // at stride s takes x[q + s (p + l n/4)], l = 0..3, to y[q + s (4 p + r)],
+    // r = 0..3, twiddled by w^r, w = e^{2 pi i p / n} = tw[p N / n]
     cx<T> *x = bufA, *y = bufB;
-    int s = 1;
-    for (int n = M; n > 1; n >>= 1) {
-        const int m = n >> 1;
-        for (int idx = threadIdx.x; idx < M / 2; idx += blockDim.x) {
-            const int p = idx / s, q = idx - p * s;
-            const int tj = 2 * p * (N / n);  // e^{2 pi i p / n} = tw[p N / n]
-            const cx<T> a = x[q + s * p], b = x[q + s * (p + m)];
-            y[q + s * (2 * p)] = a + b;
-            y[q + s * (2 * p + 1)] = (a - b) * C<T>(tw[tj], tw[tj + 1]);
+    int s = 1, ls = 0, n = M;
+    for (; n >= 4; n >>= 2) {
+        const int m4 = n >> 2, st = N / n;
+#pragma unroll 2
+        for (int idx = threadIdx.x; idx < M / 4; idx += kIfftThreads) {
+            const int p = idx >> ls, q = idx & (s - 1);
+            const cx<T> a0 = x[q + s * p], a1 = x[q + s * (p + m4)];
+            const cx<T> a2 = x[q + s * (p + 2 * m4)], a3 = x[q + s * (p + 3 * m4)];
+            const cx<T> w1 = C<T>(tw[2 * p * st], tw[2 * p * st + 1]);
+            const cx<T> w2 = C<T>(tw[4 * p * st], tw[4 * p * st + 1]);  // 2 p N / n < N / 2
+            const cx<T> w3 = w1 * w2;
+            const cx<T> b0 = a0 + a2, b1 = a0 - a2, b2 = a1 + a3, d = a1 - a3;
+            const cx<T> b3 = C<T>(-d.y, d.x);  // i (a1 - a3)
+            cx<T> *o = y + q + s * (4 * p);
+            o[0] = b0 + b2;
+            o[s] = (b1 + b3) * w1;
+            o[2 * s] = (b0 - b2) * w2;
+            o[3 * s] = (b1 - b3) * w3;
         }
         __syncthreads();
         cx<T> *tmp = x;
         x = y;
         y = tmp;
-        s <<= 1;
+        s <<= 2;
+        ls += 2;
     }
-    for (int nn = threadIdx.x; nn < M; nn += blockDim.x) {
+    if (n == 2) {  // last radix-2 stage: p = 0, unit twiddle
+#pragma unroll 4
+        for (int q = threadIdx.x; q < s; q += kIfftThreads) {
+            const cx<T> a = x[q], b = x[q + s];
+            y[q] = a + b;
+            y[q + s] = a - b;
+        }
+        __syncthreads();
+        x = y;
+    }
+#pragma unroll 4
+    for (int nn = threadIdx.x; nn < M; nn += kIfftThreads) {
         const cx<T> z = x[nn];
         traces[(size_t)t * N + 2 * nn] = z.x * ts[2 * nn];
         traces[(size_t)t * N + 2 * nn + 1] = z.y * ts[2 * nn + 1];
