@@ -99,7 +99,7 @@ hipError_t launch_bessel_tab(int prec, int mode, int nk, double dk, const double
                              void *B1, long ldb, hipStream_t st, int n0 = 0);
 hipError_t launch_gemm(int prec, int M, int N, int K, const void *A, long lda, const void *B, long ldb, void *Cm,
                        long ldc, hipStream_t st);
-size_t post_ifft_lds_bytes(int prec, int nw);
+size_t post_ifft_lds_bytes(int prec, int nw, int R);
 // k-sum contraction on the matrix cores; mode 0 FP64, 1 FP32, 2 BF16x3
 size_t gemm_workspace_bytes(int mode, int M, int N, int K);
 hipError_t launch_probe_f64(int l0, double *out, hipStream_t st);

@@ -473,157 +473,183 @@ __device__ __forceinline__ cx<T> post_val(cx<T> v, int iw, const PostParams &pp)
 
 // Phase / taper / scale, Hermitian C2R inverse FFT (Stockham in LDS: radix-4
 // stages, one radix-2 stage when log2 of the half length is odd) and time
-// scaling of trace t, whose raw spectrum row iw is load(iw).  256 threads.
+// scaling of R traces t0 .. t0 + R - 1, whose raw spectrum rows iw are
+// load(r, iw).  The R traces are adjacent receivers, which share every
+// 128-byte line of the row-major contraction outputs: R consecutive lanes
+// load R consecutive values, and all R spectra are staged in LDS before they
+// are transformed one after the other.
+// 256 threads; LDS post_ifft_lds_bytes.
 constexpr int kIfftThreads = 256;
-template <typename T, typename Load>
-__device__ __forceinline__ void post_ifft_trace(const PostParams &pp, Load &&load, int t, T *__restrict__ traces,
+template <typename T, int R, typename Load>
+__device__ __forceinline__ void post_ifft_group(const PostParams &pp, Load &&load, int t0, T *__restrict__ traces,
                                                 T *__restrict__ spec_post)
 {
     extern __shared__ unsigned char smem_raw[];
-    cx<T> *bufA = reinterpret_cast<cx<T> *>(smem_raw);
-    const int M = pp.nw, N = 2 * pp.nw;
-    cx<T> *bufB = bufA + M;
+    const int M = pp.nw, N = 2 * pp.nw, Mp = M;
+    cx<T> *stage = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *bufA = stage + R * Mp;
     const T *tw = static_cast<const T *>(pp.tw), *ts = static_cast<const T *>(pp.ts);
-    // each spectrum value is loaded once, into bufB (free until the first
-    // stage); the loop is unrolled so that several loads are in flight
+    // unrolled so that several loads are in flight
 #pragma unroll 4
-    for (int m = threadIdx.x; m < M; m += kIfftThreads) {
-        const cx<T> Xm = post_val(load(m), m, pp);
+    for (int i = threadIdx.x; i < R * M; i += kIfftThreads) {
+        const int r = R == 1 ? 0 : i % R, m = R == 1 ? i : i / R;
+        const cx<T> Xm = post_val(load(r, m), m, pp);
         if (spec_post) {
-            spec_post[((size_t)t * M + m) * 2] = Xm.x;
-            spec_post[((size_t)t * M + m) * 2 + 1] = Xm.y;
+            spec_post[((size_t)(t0 + r) * M + m) * 2] = Xm.x;
+            spec_post[((size_t)(t0 + r) * M + m) * 2 + 1] = Xm.y;
         }
-        bufB[m] = Xm;
+        stage[r * Mp + m] = Xm;
     }
     __syncthreads();
-    // Hermitian fold to a half-length complex sequence
+    for (int r = 0; r < R; r++) {
+        cx<T> *sr = stage + r * Mp;
+        // Hermitian fold to a half-length complex sequence
 #pragma unroll 4
-    for (int m = threadIdx.x; m < M; m += kIfftThreads) {
-        cx<T> Xm = bufB[m];
-        cx<T> Xc = C<T>(0, 0);  // conj(X[M-m]); X[M] (Nyquist) = 0
-        if (m > 0) Xc = conj(bufB[M - m]);
-        if (m == 0) Xm.y = T(0);  // Im X[0] only reaches Im of the output
-        const cx<T> Xe = Xm + Xc;
-        const cx<T> Xo = (Xm - Xc) * C<T>(tw[2 * m], tw[2 * m + 1]);  // e^{+2 pi i m / N}
-        bufA[m] = C<T>(Xe.x - Xo.y, Xe.y + Xo.x);  // Xe + i Xo
-    }
-    __syncthreads();
-    // Stockham decimation in frequency, inverse sign: a length-n sub-transform
-    // at stride s takes x[q + s (p + l n/4)], l = 0..3, to y[q + s (4 p + r)],
-    // r = 0..3, twiddled by w^r, w = e^{2 pi i p / n} = tw[p N / n]
-    cx<T> *x = bufA, *y = bufB;
-    int s = 1, ls = 0, n = M;
-    for (; n >= 4; n >>= 2) {
-        const int m4 = n >> 2, st = N / n;
+        for (int m = threadIdx.x; m < M; m += kIfftThreads) {
+            cx<T> Xm = sr[m];
+            cx<T> Xc = C<T>(0, 0);  // conj(X[M-m]); X[M] (Nyquist) = 0
+            if (m > 0) Xc = conj(sr[M - m]);
+            if (m == 0) Xm.y = T(0);  // Im X[0] only reaches Im of the output
+            const cx<T> Xe = Xm + Xc;
+            const cx<T> Xo = (Xm - Xc) * C<T>(tw[2 * m], tw[2 * m + 1]);  // e^{+2 pi i m / N}
+            bufA[m] = C<T>(Xe.x - Xo.y, Xe.y + Xo.x);                   // Xe + i Xo
+        }
+        __syncthreads();
+        // Stockham decimation in frequency, inverse sign: a length-n
+        // sub-transform at stride s takes x[q + s (p + l n/4)], l = 0..3, to
+        // y[q + s (4 p + u)], u = 0..3, twiddled by w^u, w = e^{2 pi i p / n}
+        // = tw[p N / n]; the staged spectrum's row is the second buffer
+        cx<T> *x = bufA, *y = sr;
+        int s = 1, ls = 0, n = M;
+        for (; n >= 4; n >>= 2) {
+            const int m4 = n >> 2, st = N / n;
 #pragma unroll 2
-        for (int idx = threadIdx.x; idx < M / 4; idx += kIfftThreads) {
-            const int p = idx >> ls, q = idx & (s - 1);
-            const cx<T> a0 = x[q + s * p], a1 = x[q + s * (p + m4)];
-            const cx<T> a2 = x[q + s * (p + 2 * m4)], a3 = x[q + s * (p + 3 * m4)];
-            const cx<T> w1 = C<T>(tw[2 * p * st], tw[2 * p * st + 1]);
-            const cx<T> w2 = C<T>(tw[4 * p * st], tw[4 * p * st + 1]);  // 2 p N / n < N / 2
-            const cx<T> w3 = w1 * w2;
-            const cx<T> b0 = a0 + a2, b1 = a0 - a2, b2 = a1 + a3, d = a1 - a3;
-            const cx<T> b3 = C<T>(-d.y, d.x);  // i (a1 - a3)
-            cx<T> *o = y + q + s * (4 * p);
-            o[0] = b0 + b2;
-            o[s] = (b1 + b3) * w1;
-            o[2 * s] = (b0 - b2) * w2;
-            o[3 * s] = (b1 - b3) * w3;
+            for (int idx = threadIdx.x; idx < M / 4; idx += kIfftThreads) {
+                const int p = idx >> ls, q = idx & (s - 1);
+                const cx<T> a0 = x[q + s * p], a1 = x[q + s * (p + m4)];
+                const cx<T> a2 = x[q + s * (p + 2 * m4)], a3 = x[q + s * (p + 3 * m4)];
+                const cx<T> w1 = C<T>(tw[2 * p * st], tw[2 * p * st + 1]);
+                const cx<T> w2 = C<T>(tw[4 * p * st], tw[4 * p * st + 1]);  // 2 p N / n < N / 2
+                const cx<T> w3 = w1 * w2;
+                const cx<T> b0 = a0 + a2, b1 = a0 - a2, b2 = a1 + a3, d = a1 - a3;
+                const cx<T> b3 = C<T>(-d.y, d.x);  // i (a1 - a3)
+                cx<T> *o = y + q + s * (4 * p);
+                o[0] = b0 + b2;
+                o[s] = (b1 + b3) * w1;
+                o[2 * s] = (b0 - b2) * w2;
+                o[3 * s] = (b1 - b3) * w3;
+            }
+            __syncthreads();
+            cx<T> *tmp = x;
+            x = y;
+            y = tmp;
+            s <<= 2;
+            ls += 2;
         }
-        __syncthreads();
-        cx<T> *tmp = x;
-        x = y;
-        y = tmp;
-        s <<= 2;
-        ls += 2;
-    }
-    if (n == 2) {  // last radix-2 stage: p = 0, unit twiddle
+        if (n == 2) {  // last radix-2 stage: p = 0, unit twiddle
 #pragma unroll 4
-        for (int q = threadIdx.x; q < s; q += kIfftThreads) {
-            const cx<T> a = x[q], b = x[q + s];
-            y[q] = a + b;
-            y[q + s] = a - b;
+            for (int q = threadIdx.x; q < s; q += kIfftThreads) {
+                const cx<T> a = x[q], b = x[q + s];
+                y[q] = a + b;
+                y[q + s] = a - b;
+            }
+            __syncthreads();
+            x = y;
         }
-        __syncthreads();
-        x = y;
-    }
+        T *out = traces + (size_t)(t0 + r) * N;
 #pragma unroll 4
-    for (int nn = threadIdx.x; nn < M; nn += kIfftThreads) {
-        const cx<T> z = x[nn];
-        traces[(size_t)t * N + 2 * nn] = z.x * ts[2 * nn];
-        traces[(size_t)t * N + 2 * nn + 1] = z.y * ts[2 * nn + 1];
+        for (int nn = threadIdx.x; nn < M; nn += kIfftThreads) {
+            const cx<T> z = x[nn];
+            out[2 * nn] = z.x * ts[2 * nn];
+            out[2 * nn + 1] = z.y * ts[2 * nn + 1];
+        }
+        if (r + 1 < R) __syncthreads();  // the next fold rewrites bufA
     }
 }
 
 // Workgroups are dispatched to the 8 XCDs in turn, and each XCD has its own
 // L2.  Traces t and t + 1 read the same 128-byte lines of the row-major
-// spectra (adjacent receivers), so block b takes trace
-// (b % 8) ceil(ntr / 8) + b / 8: each XCD works through a contiguous range
-// of traces and fetches each line once.  The grid has 8 ceil(ntr / 8) blocks.
+// spectra (adjacent receivers), so block b takes trace group
+// (b % 8) ceil(ng / 8) + b / 8: each XCD works through a contiguous range
+// of traces and fetches each line once.  The grid has 8 ceil(ng / 8) blocks.
 constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_trace(int ntr)
+__device__ __forceinline__ int xcd_trace(int ng)
 {
-    const int per = (ntr + kXcds - 1) / kXcds;
+    const int per = (ng + kXcds - 1) / kXcds;
     return int(blockIdx.x % kXcds) * per + int(blockIdx.x / kXcds);
 }
-static unsigned xcd_grid(int ntr) { return (unsigned)((ntr + kXcds - 1) / kXcds * kXcds); }
+static unsigned xcd_grid(int ng) { return (unsigned)((ng + kXcds - 1) / kXcds * kXcds); }
 
-// One workgroup per trace component.  layout 0: raw[(t*nw + iw)*2 + {re,im}];
-// layout 1: rows of the contraction outputs: component c = (t / nr) % 2 reads
-// C_c[(4 iw + 2 q + {0,1}) * ldc + ir] with q = t / (2 nr), ir = t % nr.
-// Several models (layout 1): trace t belongs to model t / (4 nr), whose
-// contraction outputs start cms elements further per model.
-template <typename T>
+// One workgroup per group of R trace components (R divides nr).  layout 0:
+// raw[(t*nw + iw)*2 + {re,im}] (R = 1); layout 1: rows of the contraction
+// outputs: component c = (t / nr) % 2 reads C_c[(4 iw + 2 q + {0,1}) * ldc + ir]
+// with q = t / (2 nr), ir = t % nr.  Several models (layout 1): trace t
+// belongs to model t / (4 nr), whose contraction outputs start cms elements
+// further per model.
+template <typename T, int R>
 __global__ __launch_bounds__(256) void k_post_ifft(PostParams pp, int layout, const T *__restrict__ raw,
                                                    const T *__restrict__ C0, const T *__restrict__ C1,
                                                    long ldc, int nr, long cms, int ntr, T *__restrict__ traces,
                                                    T *__restrict__ spec_post)
 {
-    const int t = xcd_trace(ntr);
-    if (t >= ntr) return;
-    const int M = pp.nw;
-    auto load = [&](int iw) -> cx<T> {
+    const int g = xcd_trace(ntr / R);
+    if (g >= ntr / R) return;
+    const int t0 = g * R, M = pp.nw;
+    const T *Cc = nullptr;
+    if (layout == 1) {
+        const int mj = t0 / (4 * nr), tl = t0 - mj * 4 * nr;
+        const int q = tl / (2 * nr), c = (tl / nr) % 2, ir0 = tl % nr;
+        Cc = (c ? C1 : C0) + (size_t)mj * cms + ir0 + (size_t)(2 * q) * ldc;
+    }
+    auto load = [&](int r, int iw) -> cx<T> {
         if (layout == 0) {
-            const T *p = raw + ((size_t)t * M + iw) * 2;
+            const T *p = raw + ((size_t)(t0 + r) * M + iw) * 2;
             return C<T>(p[0], p[1]);
         }
-        const int mj = t / (4 * nr), tl = t - mj * 4 * nr;
-        const int q = tl / (2 * nr), c = (tl / nr) % 2, ir = tl % nr;
-        const T *Cc = (c ? C1 : C0) + (size_t)mj * cms;
-        const size_t row = (size_t)(4 * iw + 2 * q);
-        return C<T>(Cc[row * ldc + ir], Cc[(row + 1) * ldc + ir]);
+        const T *p = Cc + (size_t)(4 * iw) * ldc + r;
+        return C<T>(p[0], p[ldc]);
     };
-    post_ifft_trace<T>(pp, load, t, traces, spec_post);
+    post_ifft_group<T, R>(pp, load, t0, traces, spec_post);
 }
 
 // Column traces: trace t = column t / nr at receiver t % nr; its raw
 // spectrum is the sum of the column's terms scale r^-rpow C_set[row] over
 // the contraction outputs (rows 2 (rows_set iw + row) + {0 re, 1 im}).
-template <typename T>
+// Groups of R receivers of one column (R divides nr).
+template <typename T, int R>
 __global__ __launch_bounds__(256) void k_post_ifft_cols(PostParams pp, const MtCols *__restrict__ mcp, long ldc,
                                                         int nr, int ntr, const double *__restrict__ dists,
                                                         T *__restrict__ traces)
 {
+    __shared__ double fac[kMtTermsMax][R];  // scale r^-rpow per term and receiver
     const MtCols &mc = *mcp;
-    const int t = xcd_trace(ntr);
-    if (t >= ntr) return;
-    const int col = t / nr, ir = t - col * nr;
-    const double rinv = 1.0 / dists[ir];
-    auto load = [&](int iw) -> cx<T> {
+    const int g = xcd_trace(ntr / R);
+    if (g >= ntr / R) return;
+    const int t0 = g * R;
+    const int col = t0 / nr, ir0 = t0 - col * nr;
+    const int nt = mc.nterm[col];
+    if ((int)threadIdx.x < nt * R) {
+        const int u = threadIdx.x / R, r = threadIdx.x % R;
+        const MtTerm &tm = mc.term[col][u];
+        const double rinv = 1.0 / dists[ir0 + r];
+        double f = tm.scale;
+        for (int e = 0; e < tm.rpow; e++) f *= rinv;
+        fac[u][r] = f;
+    }
+    __syncthreads();
+    auto load = [&](int r, int iw) -> cx<T> {
         double re = 0, im = 0;
-        for (int u = 0; u < mc.nterm[col]; u++) {
+        for (int u = 0; u < nt; u++) {
             const MtTerm &tm = mc.term[col][u];
-            const T *Cs = reinterpret_cast<const T *>(mc.C[tm.set]);
+            const T *Cs = reinterpret_cast<const T *>(mc.C[tm.set]) + ir0 + r;
             const size_t row = 2 * ((size_t)mc.rows[tm.set] * iw + tm.row);
-            double f = tm.scale;
-            for (int q = 0; q < tm.rpow; q++) f *= rinv;
-            re += f * double(Cs[row * ldc + ir]);
-            im += f * double(Cs[(row + 1) * ldc + ir]);
+            const double f = fac[u][r];
+            re += f * double(Cs[row * ldc]);
+            im += f * double(Cs[(row + 1) * ldc]);
         }
         return C<T>(T(re), T(im));
     };
-    post_ifft_trace<T>(pp, load, t, traces, (T *)nullptr);
+    post_ifft_group<T, R>(pp, load, t0, traces, (T *)nullptr);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -823,51 +849,78 @@ hipError_t launch_gemm(int prec, int M, int N, int K, const void *A, long lda, c
     return hipGetLastError();
 }
 
+// LDS of an IFFT workgroup of R traces: R staged spectra and one transform
+// buffer, nw complex values each.
+size_t post_ifft_lds_bytes(int prec, int nw, int R)
+{
+    return (size_t)(R + 1) * nw * (prec == 0 ? 16 : 8);
+}
+
+// Traces per IFFT workgroup: 2 adjacent receivers when nr is even and their
+// LDS (with `extra` bytes of static LDS) fits a workgroup (160 KiB), else 1.
+// Measured on the 256-receiver workloads: 2 per workgroup beat 1 (column
+// traces 0.18 -> 0.15 ms) and 4 (0.23 ms: one workgroup per compute unit,
+// four transforms in turn).
+static int ifft_group(int prec, int nw, int nr, size_t extra)
+{
+    return (nr > 0 && nr % 2 == 0 && post_ifft_lds_bytes(prec, nw, 2) + extra <= 163840) ? 2 : 1;
+}
+
+template <typename K>
+static hipError_t ifft_lds_attr(K kernel, size_t lds)
+{
+    if (lds <= 65536) return hipSuccess;  // within the default dynamic-LDS limit
+    return hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
 hipError_t launch_post_ifft_cols(int prec, const PostParams &pp, const MtCols *mc, int ncol, long ldc, int nr,
                                  const double *dists, void *traces, hipStream_t st)
 {
-    const size_t lds = post_ifft_lds_bytes(prec, pp.nw);
-    if (lds > 65536) {
-        hipError_t e = prec == 0 ? hipFuncSetAttribute((const void *)k_post_ifft_cols<double>,
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)
-                                 : hipFuncSetAttribute((const void *)k_post_ifft_cols<float>,
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
     if (ncol < 1 || ncol > kMtColsMax) return hipErrorInvalidValue;
     const int ntr = ncol * nr;
-    const dim3 grid(xcd_grid(ntr));
-    if (prec == 0)
-        hipLaunchKernelGGL(k_post_ifft_cols<double>, grid, dim3(256), lds, st, pp, mc, ldc, nr, ntr, dists,
-                           (double *)traces);
-    else
-        hipLaunchKernelGGL(k_post_ifft_cols<float>, grid, dim3(256), lds, st, pp, mc, ldc, nr, ntr, dists,
-                           (float *)traces);
+    const int R = ifft_group(prec, pp.nw, nr, sizeof(double) * kMtTermsMax * 2);
+    const size_t lds = post_ifft_lds_bytes(prec, pp.nw, R);
+    const dim3 grid(xcd_grid(ntr / R));
+#define GRTM_COLS(T, RR)                                                                                     \
+    do {                                                                                                     \
+        hipError_t e = ifft_lds_attr(k_post_ifft_cols<T, RR>, lds);                                         \
+        if (e != hipSuccess) return e;                                                                       \
+        hipLaunchKernelGGL((k_post_ifft_cols<T, RR>), grid, dim3(kIfftThreads), lds, st, pp, mc, ldc, nr, ntr, \
+                           dists, (T *)traces);                                                              \
+    } while (0)
+    if (prec == 0) {
+        if (R == 2) GRTM_COLS(double, 2);
+        else GRTM_COLS(double, 1);
+    } else {
+        if (R == 2) GRTM_COLS(float, 2);
+        else GRTM_COLS(float, 1);
+    }
+#undef GRTM_COLS
     return hipGetLastError();
 }
-
-size_t post_ifft_lds_bytes(int prec, int nw) { return (size_t)2 * nw * (prec == 0 ? 16 : 8); }
 
 hipError_t launch_post_ifft(int prec, const PostParams &pp, int layout, int ntr, const void *raw, const void *C0,
                             const void *C1, long ldc, int nr, void *traces, void *spec_post, hipStream_t st,
                             long cms)
 {
-    const size_t lds = post_ifft_lds_bytes(prec, pp.nw);
-    if (lds > 65536) {  // above the default dynamic-LDS limit of a launch
-        hipError_t e = prec == 0 ? hipFuncSetAttribute((const void *)k_post_ifft<double>,
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)
-                                 : hipFuncSetAttribute((const void *)k_post_ifft<float>,
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
+    const int R = layout == 1 ? ifft_group(prec, pp.nw, nr, 0) : 1;
+    const size_t lds = post_ifft_lds_bytes(prec, pp.nw, R);
+    const dim3 grid(xcd_grid(ntr / R));
+#define GRTM_POST(T, RR)                                                                                          \
+    do {                                                                                                          \
+        hipError_t e = ifft_lds_attr(k_post_ifft<T, RR>, lds);                                                   \
+        if (e != hipSuccess) return e;                                                                            \
+        hipLaunchKernelGGL((k_post_ifft<T, RR>), grid, dim3(kIfftThreads), lds, st, pp, layout, (const T *)raw,  \
+                           (const T *)C0, (const T *)C1, ldc, nr, cms, ntr, (T *)traces, (T *)spec_post);         \
+    } while (0)
+    if (prec == 0) {
+        if (R == 2) GRTM_POST(double, 2);
+        else GRTM_POST(double, 1);
+    } else {
+        if (R == 2) GRTM_POST(float, 2);
+        else GRTM_POST(float, 1);
     }
-    if (prec == 0)
-        hipLaunchKernelGGL(k_post_ifft<double>, dim3(xcd_grid(ntr)), dim3(256), lds, st, pp, layout,
-                           (const double *)raw, (const double *)C0, (const double *)C1, ldc, nr, cms, ntr,
-                           (double *)traces, (double *)spec_post);
-    else
-        hipLaunchKernelGGL(k_post_ifft<float>, dim3(xcd_grid(ntr)), dim3(256), lds, st, pp, layout,
-                           (const float *)raw, (const float *)C0, (const float *)C1, ldc, nr, cms, ntr,
-                           (float *)traces, (float *)spec_post);
+#undef GRTM_POST
     return hipGetLastError();
 }
 

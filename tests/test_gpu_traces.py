@@ -101,3 +101,23 @@ def test_plan_rejects_bad_grid():
     bad = dict(cfg["grid"], nw=500)
     with pytest.raises(g.GrtmError):
         g.TracePlan(cfg["layers"], 10.0, 0.0, cfg["dists"], bad)
+
+
+@pytest.mark.parametrize("nr", [5, 8])
+def test_ifft_receiver_groups(nr):
+    """The inverse FFT stages and transforms the spectra of 2 adjacent
+    receivers per workgroup when the receiver count is even (8), one
+    otherwise (5); every trace equals the one of a one-receiver plan, for the
+    Z/R traces and for the displacement and strain columns."""
+    grid = models.dense_grid(128, 0.1, 1024, float(models.C1_LAYERS[:, 2].min()))
+    dists = np.linspace(10.0, 80.0, nr)
+    print(f"gpu: ifft receiver groups, {nr} receivers", flush=True)
+    for make in (lambda d: g.TracePlan(models.C1_LAYERS, 10.0, 0.0, d, grid, precision=g.PREC_FP64),
+                 lambda d: g.ColumnPlan(models.C1_LAYERS, 10.0, 0.0, d, grid, precision=g.PREC_FP64,
+                                        derivatives=True)):
+        tr = make(dists).traces()  # receivers on the next-to-last axis
+        for i, d in enumerate(dists):
+            one = make([d]).traces()
+            a, b = tr[..., i, :], one[..., 0, :]
+            err = np.linalg.norm(a - b) / np.linalg.norm(b)
+            assert err < 1e-12, (nr, i, err)
