@@ -161,21 +161,12 @@ GRTM_HD void interface_xy(const LQ<T> &a, const LQ<T> &b, T k, m2<T> &X, m2<T> &
     d2 = T(2) * (a.mg * rv);
 }
 
-template <typename T> GRTM_HD m2<T> i_plus(const m2<T> &Q)
-{
-    return {C<T>(T(1) + Q.a.x, Q.a.y), Q.b, Q.c, C<T>(T(1) + Q.d.x, Q.d.y)};
-}
-template <typename T> GRTM_HD m2<T> i_minus(const m2<T> &Q)
-{
-    return {C<T>(T(1) - Q.a.x, -Q.a.y), -Q.b, -Q.c, C<T>(T(1) - Q.d.x, -Q.d.y)};
-}
-
 // R and T = d2 G^-1 from F = X (I + Q), H = Y (I - Q), G = F + H.
 template <typename T>
 GRTM_HD void grt_rt(const m2<T> &F, const m2<T> &H, cx<T> d2, m2<T> &R, m2<T> &Tr)
 {
     const m2<T> G = F + H;
-    const cx<T> di = recip(G.a * G.d - G.b * G.c);
+    const cx<T> di = recip(cfms(G.b, G.c, G.a * G.d));
     const m2<T> Gi = {G.d * di, -(G.b * di), -(G.c * di), G.a * di};
     R = (F - H) * Gi;
     const cx<T> dt = d2 * di;
@@ -183,10 +174,16 @@ GRTM_HD void grt_rt(const m2<T> &F, const m2<T> &H, cx<T> d2, m2<T> &R, m2<T> &T
 }
 // One step of the recursion: R, T across an interface (X, Y, d2 of
 // interface_xy) with b's side reflecting Q.
+// X (I + Q) = X + X Q and Y (I - Q) = Y - Y Q entry by entry in fused
+// multiply-adds.
 template <typename T>
 GRTM_HD void grt_step(const m2<T> &X, const m2<T> &Y, cx<T> d2, const m2<T> &Q, m2<T> &R, m2<T> &Tr)
 {
-    grt_rt(X * i_plus(Q), Y * i_minus(Q), d2, R, Tr);
+    const m2<T> F = {cfma(X.b, Q.c, cfma(X.a, Q.a, X.a)), cfma(X.a, Q.b, cfma(X.b, Q.d, X.b)),
+                     cfma(X.d, Q.c, cfma(X.c, Q.a, X.c)), cfma(X.c, Q.b, cfma(X.d, Q.d, X.d))};
+    const m2<T> H = {cfms(Y.b, Q.c, cfms(Y.a, Q.a, Y.a)), cfms(Y.a, Q.b, cfms(Y.b, Q.d, Y.b)),
+                     cfms(Y.d, Q.c, cfms(Y.c, Q.a, Y.c)), cfms(Y.c, Q.b, cfms(Y.d, Q.d, Y.d))};
+    grt_rt(F, H, d2, R, Tr);
 }
 // The same with nothing reflecting on b's side (b the half-space).
 template <typename T>
@@ -207,7 +204,7 @@ GRTM_HD m2<T> lrl(cx<T> e0, cx<T> e1, const m2<T> &R, cx<T> &e11)
 template <typename T>
 GRTM_HD void grt_step_sh(cx<T> wa, cx<T> wb, cx<T> q, cx<T> &r, cx<T> &t)
 {
-    const cx<T> p = wa * (C<T>(1, 0) + q), m = wb * (C<T>(1, 0) - q);
+    const cx<T> p = cfma(wa, q, wa), m = cfms(wb, q, wb);  // wa (1 + q), wb (1 - q)
     const cx<T> gi = recip(p + m);
     r = (p - m) * gi;
     t = T(2) * wa * gi;
@@ -457,8 +454,8 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
     out.ERu = dmd(sexu0, sexu1, Rud_s, sexu0, sexu1);
     out.erd0 = sexd1 * rdu0_s1 * sexd1;
     out.eru0 = sexu1 * rud0_s * sexu1;
-    const m2<T> yu_psv = minv(ident<T>() - out.ERd * out.ERu);
-    const m2<T> yd_psv = minv(ident<T>() - out.ERu * out.ERd);
+    const m2<T> yu_psv = minv(mmsub(out.ERd, out.ERu, ident<T>()));
+    const m2<T> yd_psv = minv(mmsub(out.ERu, out.ERd, ident<T>()));
     const cx<T> y_sh = recip(C<T>(1, 0) - out.erd0 * out.eru0);
 
     // receiver-layer E rows 0,1 (displacements) without the column
@@ -472,13 +469,13 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
         cx<T> rd0 = cexp_nz<T>(qr.rd, zr), rd1 = cexp_nz<T>(qr.vd, zr);
         cx<T> ru0 = cexp_nz<T>(qr.rd, zs - zr), ru1 = cexp_nz<T>(qr.vd, zs - zr);
         const m2<T> RS = dmd(rd0, rd1, Rud_s, sexu0, sexu1);
-        const m2<T> u = Ed * RS + md(Eu, ru0, ru1);
+        const m2<T> u = mmadd(Ed, RS, md(Eu, ru0, ru1));
         const cx<T> u0 = rd1 * rud0_s * sexu1 + ru1;
         out.Y = u * yu_psv;
         out.y0 = u0 * y_sh;
         if (NEED_T) {
             const m2<T> RSt = dmd(-(qr.r * rd0), -(qr.v * rd1), Rud_s, sexu0, sexu1);
-            const m2<T> ut = Ed * RSt + md(Eu, qr.r * ru0, qr.v * ru1);
+            const m2<T> ut = mmadd(Ed, RSt, md(Eu, qr.r * ru0, qr.v * ru1));
             cx<T> u0t = COMPAT ? -(qr.v * rd1) * rud0_s * sexu1 + qr.v * ru1 * qr.v
                                : sh_dz_above(qr, rud0_s, rd1, sexu1, ru1);
             out.Yt = ut * yu_psv;
@@ -488,13 +485,13 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
         cx<T> rd0 = cexp_nz<T>(qr.rd, zr - zs), rd1 = cexp_nz<T>(qr.vd, zr - zs);
         cx<T> ru0 = cexp_nz<T>(qr.rd, hr_ - zr), ru1 = cexp_nz<T>(qr.vd, hr_ - zr);
         const m2<T> RS = dmd(ru0, ru1, Rdu_s1, sexd0, sexd1);
-        const m2<T> d = md(Ed, rd0, rd1) + Eu * RS;
+        const m2<T> d = mmadd(Eu, RS, md(Ed, rd0, rd1));
         const cx<T> d0 = rd1 + ru1 * rdu0_s1 * sexd1;
         out.Y = d * yd_psv;
         out.y0 = d0 * y_sh;
         if (NEED_T) {
             const m2<T> RSt = dmd(qr.r * ru0, qr.v * ru1, Rdu_s1, sexd0, sexd1);
-            const m2<T> dt = md(Ed, -(qr.r * rd0), -(qr.v * rd1)) + Eu * RSt;
+            const m2<T> dt = mmadd(Eu, RSt, md(Ed, -(qr.r * rd0), -(qr.v * rd1)));
             const cx<T> d0t = -(qr.v * rd1) + qr.v * ru1 * rdu0_s1 * sexd1;
             out.Yt = dt * yd_psv;
             out.y0t = d0t * y_sh;
@@ -505,7 +502,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
         cx<T> ex0, ex1;
         lam(qr, hr_, ex0, ex1);
         const m2<T> RS = dmd(rd0, rd1, Rud_r, ex0, ex1);
-        const m2<T> u = Ed * RS + md(Eu, ru0, ru1);
+        const m2<T> u = mmadd(Ed, RS, md(Eu, ru0, ru1));
         const cx<T> u0 = rd1 * rud0_r * ex1 + ru1;
         const m2<T> chain = Mu * dmd(sexu0, sexu1, yu_psv, C<T>(1, 0), C<T>(1, 0));
         const cx<T> chain0 = mu0 * sexu1 * y_sh;
@@ -514,7 +511,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
         if (NEED_T) {
             const cx<T> pr = COMPAT ? par0_r : qr.r, pv = COMPAT ? par0_v : qr.v;
             const m2<T> RSt = dmd(-(qr.r * rd0), -(qr.v * rd1), Rud_r, ex0, ex1);
-            const m2<T> ut = Ed * RSt + md(Eu, ru0 * pr, ru1 * pv);
+            const m2<T> ut = mmadd(Ed, RSt, md(Eu, ru0 * pr, ru1 * pv));
             const cx<T> u0t = COMPAT ? -(qr.v * rd1) * rud0_r * ex1 + ru1 * pv : sh_dz_above(qr, rud0_r, rd1, ex1, ru1);
             out.Yt = ut * chain;
             out.y0t = COMPAT ? u0t * out.y0 : u0t * chain0;
@@ -534,7 +531,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             lam(qr, hr_, ex0, ex1);
         }
         const m2<T> RS = dmd(ru0, ru1, Rdu_r1, ex0, ex1);
-        const m2<T> d = md(Ed, rd0, rd1) + Eu * RS;
+        const m2<T> d = mmadd(Eu, RS, md(Ed, rd0, rd1));
         const cx<T> d0 = ru1 * rdu0_r1 * ex1 + rd1;
         const m2<T> &ypsv = COMPAT ? yu_psv : yd_psv;
         m2<T> Mc = Md;
@@ -552,7 +549,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
         if (NEED_T) {
             const cx<T> pr = COMPAT ? par0_r : qr.r, pv = COMPAT ? par0_v : qr.v;
             const m2<T> RSt = dmd(ru0 * pr, ru1 * pv, Rdu_r1, ex0, ex1);
-            const m2<T> dt = md(Ed, -(qr.r * rd0), -(qr.v * rd1)) + Eu * RSt;
+            const m2<T> dt = mmadd(Eu, RSt, md(Ed, -(qr.r * rd0), -(qr.v * rd1)));
             const cx<T> d0t = ru1 * pv * rdu0_r1 * ex1 - qr.v * rd1;
             out.Yt = dt * chain;
             out.y0t = COMPAT ? d0 * out.y0 : d0t * chain0;

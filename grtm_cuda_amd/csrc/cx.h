@@ -24,6 +24,16 @@ template <typename T> GRTM_HD cx<T> operator*(cx<T> a, cx<T> b)
 template <typename T> GRTM_HD cx<T> operator*(T s, cx<T> a) { return {s * a.x, s * a.y}; }
 template <typename T> GRTM_HD cx<T> operator*(cx<T> a, T s) { return {s * a.x, s * a.y}; }
 template <typename T> GRTM_HD cx<T> conj(cx<T> a) { return {a.x, -a.y}; }
+// c + a b and c - a b as four fused multiply-adds each (a b + d e costs one
+// complex product and one cfma: 8 instructions instead of 10)
+template <typename T> GRTM_HD cx<T> cfma(cx<T> a, cx<T> b, cx<T> c)
+{
+    return {fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y))};
+}
+template <typename T> GRTM_HD cx<T> cfms(cx<T> a, cx<T> b, cx<T> c)
+{
+    return {fma(-a.x, b.x, fma(a.y, b.y, c.x)), fma(-a.x, b.y, fma(-a.y, b.x, c.y))};
+}
 template <typename T> GRTM_HD T norm2(cx<T> a) { return a.x * a.x + a.y * a.y; }
 
 // Hardware reciprocals on the device (v_rcp_f32: 1 ulp; v_rcp_f64: about
@@ -163,7 +173,18 @@ struct m2 {
 
 template <typename T> GRTM_HD m2<T> operator*(const m2<T> &p, const m2<T> &q)
 {
-    return {p.a * q.a + p.b * q.c, p.a * q.b + p.b * q.d, p.c * q.a + p.d * q.c, p.c * q.b + p.d * q.d};
+    return {cfma(p.b, q.c, p.a * q.a), cfma(p.b, q.d, p.a * q.b), cfma(p.d, q.c, p.c * q.a), cfma(p.d, q.d, p.c * q.b)};
+}
+// r + p q and r - p q in fused multiply-adds
+template <typename T> GRTM_HD m2<T> mmadd(const m2<T> &p, const m2<T> &q, const m2<T> &r)
+{
+    return {cfma(p.b, q.c, cfma(p.a, q.a, r.a)), cfma(p.b, q.d, cfma(p.a, q.b, r.b)),
+            cfma(p.d, q.c, cfma(p.c, q.a, r.c)), cfma(p.d, q.d, cfma(p.c, q.b, r.d))};
+}
+template <typename T> GRTM_HD m2<T> mmsub(const m2<T> &p, const m2<T> &q, const m2<T> &r)
+{
+    return {cfms(p.b, q.c, cfms(p.a, q.a, r.a)), cfms(p.b, q.d, cfms(p.a, q.b, r.b)),
+            cfms(p.d, q.c, cfms(p.c, q.a, r.c)), cfms(p.d, q.d, cfms(p.c, q.b, r.d))};
 }
 template <typename T> GRTM_HD m2<T> operator+(const m2<T> &p, const m2<T> &q)
 {
@@ -191,15 +212,15 @@ template <typename T> GRTM_HD m2<T> md(const m2<T> &p, cx<T> r0, cx<T> r1)
 // Closed-form inverse adj(p)/det(p).
 template <typename T> GRTM_HD m2<T> minv(const m2<T> &p)
 {
-    cx<T> di = recip(p.a * p.d - p.b * p.c);
+    cx<T> di = recip(cfms(p.b, p.c, p.a * p.d));
     return {p.d * di, -(p.b * di), -(p.c * di), p.a * di};
 }
-template <typename T> GRTM_HD cx<T> mdet(const m2<T> &p) { return p.a * p.d - p.b * p.c; }
+template <typename T> GRTM_HD cx<T> mdet(const m2<T> &p) { return cfms(p.b, p.c, p.a * p.d); }
 // p * v for a 2-vector
 template <typename T> GRTM_HD void mv(const m2<T> &p, cx<T> v0, cx<T> v1, cx<T> &o0, cx<T> &o1)
 {
-    o0 = p.a * v0 + p.b * v1;
-    o1 = p.c * v0 + p.d * v1;
+    o0 = cfma(p.b, v1, p.a * v0);
+    o1 = cfma(p.d, v1, p.c * v0);
 }
 
 }  // namespace grtm
