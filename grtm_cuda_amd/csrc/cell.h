@@ -26,7 +26,7 @@ struct WideOf {
 template <typename T>
 struct LayerW {
     using W = typename WideOf<T>::type;
-    cx<W> sad, sbd;  // w/alpha, w/beta in FP64 (vertical wavenumbers are formed in FP64)
+    cx<W> sa2, sb2;  // (w/alpha)^2, (w/beta)^2 in FP64 (vertical wavenumbers are formed in FP64)
     cx<T> sa, sb;  // w/alpha, w/beta (complex slownesses times w)
     cx<T> mg;      // mu (w/beta)^2
     cx<T> ca, cb;  // alpha/w, beta/w (normalisation of the E columns)
@@ -85,14 +85,18 @@ template <typename T>
 GRTM_HD LQ<T> layer_q(const LayerW<T> &L, typename WideOf<T>::type kd)
 {
     LQ<T> q;
-    // vertical wavenumbers sqrt((k - w/V)(k + w/V)) with Re >= 0
-    // (green.cu:184-194), in FP64 at the FP64 wavenumber (rounding k to FP32
-    // first would shift the abscissa by up to 6e-8 k, to which the phase of
-    // the kernel is sensitive near k = w/V); the factored form avoids
-    // cancellation near k = w/V
+    // vertical wavenumbers sqrt(k^2 - (w/V)^2) with Re >= 0 (green.cu:184-194),
+    // in FP64 at the FP64 wavenumber (rounding k to FP32 first would shift the
+    // abscissa by up to 6e-8 k, to which the phase of the kernel is sensitive
+    // near k = w/V).  k^2 - Re (w/V)^2 cancels near k = Re w/V, but there
+    // |Im (w/V)^2| = 2 |Re w/V| |Im w/V| bounds |z| from below: with Q
+    // attenuation and the damping w_I, |Im / Re (w/V)| >= about w_I / w, so
+    // the rounding of k^2 costs at most 1e-16 w / w_I (4e-13 at the highest
+    // frequency of a 4096-sample trace) of |z|
     using W = typename WideOf<T>::type;
-    const cx<W> zr = (C<W>(kd, W(0)) - L.sad) * (C<W>(kd, W(0)) + L.sad);
-    const cx<W> zv = (C<W>(kd, W(0)) - L.sbd) * (C<W>(kd, W(0)) + L.sbd);
+    const W k2 = kd * kd;
+    const cx<W> zr = C<W>(k2 - L.sa2.x, -L.sa2.y);
+    const cx<W> zv = C<W>(k2 - L.sb2.x, -L.sb2.y);
     if constexpr (sizeof(T) == 4) {
         // FP32 cells: FP32 estimate + FP64 Newton step (about 1e-13 relative)
         q.rd = csqrt_nr(zr);
@@ -105,7 +109,7 @@ GRTM_HD LQ<T> layer_q(const LayerW<T> &L, typename WideOf<T>::type kd)
     q.v = narrow<T>(q.vd);
     // k^2 + v^2 = 2 k^2 - (w/beta)^2, formed in FP64: it cancels near
     // k = w / (beta sqrt 2)
-    q.x = narrow<T>(C<W>(W(2) * kd * kd, W(0)) - L.sbd * L.sbd);
+    q.x = narrow<T>(C<W>(W(2) * k2 - L.sb2.x, -L.sb2.y));
     q.mg = L.mg;
     q.mu = L.mu;
     q.sa = L.sa;
@@ -215,16 +219,16 @@ GRTM_HD void grt_step_sh(cx<T> wa, cx<T> wb, cx<T> q, cx<T> &r, cx<T> &t)
 // [x, 2kv]], E22 = diag(-1, 1) E21.  det F = 4 k^2 r v - x^2 (the Rayleigh
 // function of the top layer) cancels near the layer's Rayleigh slowness,
 // where a shallow source excites the surface waves strongly: it is formed in
-// FP64 from the FP64 wavenumber, vertical wavenumbers and w / beta (sbd).
+// FP64 from the FP64 wavenumber, vertical wavenumbers and (w / beta)^2 (sb2).
 template <typename T>
-GRTM_HD m2<T> free_surface(const LQ<T> &q, typename WideOf<T>::type kd, cx<typename WideOf<T>::type> sbd)
+GRTM_HD m2<T> free_surface(const LQ<T> &q, typename WideOf<T>::type kd, cx<typename WideOf<T>::type> sb2)
 {
     using W = typename WideOf<T>::type;
     const T k = T(kd);
     const T k2 = k * k;
     const cx<T> rv4 = (T(4) * k2) * (q.r * q.v);
     const cx<T> x2 = q.x * q.x;
-    const cx<W> xd = C<W>(W(2) * kd * kd, W(0)) - sbd * sbd;
+    const cx<W> xd = C<W>(W(2) * kd * kd - sb2.x, -sb2.y);
     const cx<T> nr = narrow<T>(recip(C<W>(W(4) * kd * kd, W(0)) * (q.rd * q.vd) - xd * xd));  // 1/det F
     const cx<T> dd = (x2 + rv4) * nr;
     const cx<T> k4x = (T(4) * k) * q.x * nr;
@@ -291,7 +295,7 @@ GRTM_HD int sweep_bottom(const LayerW<T> *lw, int L, int s, typename WideOf<T>::
     const T k = T(kd), cut = T(kDecayCut);
     T x = T(0);
     for (int j = s; j < L - 1; j++) {
-        const cx<T> sb = narrow<T>(lw[j].sbd);
+        const cx<T> sb = lw[j].sb;
         const cx<T> z = (C<T>(k, T(0)) - sb) * (C<T>(k, T(0)) + sb);
         const T re = sqrt(T(0.5) * (sqrt(z.x * z.x + z.y * z.y) + z.x));  // Re sqrt(z)
         x = x + re * T(j == s ? lw[j].h - z0 : lw[j].h);
@@ -404,7 +408,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
     LQ<T> qr;  // receiver layer quantities
     {
         LQ<T> qu = layer_q(lw[0], kd);
-        m2<T> Rg = free_surface(qu, kd, lw[0].sbd);
+        m2<T> Rg = free_surface(qu, kd, lw[0].sb2);
         cx<T> rg0 = C<T>(1, 0);
         if (rl == 0) qr = qu;
         Rud_r = Rg;

@@ -207,8 +207,8 @@ void layer_w(const Stack &st, int compat, cd w, LayerW<double> *out)
         const cd sa = w / a, sb = w / b;
         auto C = [](cd z) { return grtm::cx<double>{z.real(), z.imag()}; };
         LayerW<double> &q = out[j];
-        q.sad = C(sa);
-        q.sbd = C(sb);
+        q.sa2 = C(sa * sa);
+        q.sb2 = C(sb * sb);
         q.sa = C(sa);
         q.sb = C(sb);
         q.mg = C(mu * sb * sb);
@@ -565,7 +565,7 @@ std::vector<LayerW<float>> narrow_tables(const std::vector<LayerW<double>> &lw)
     auto F = [](grtm::cx<double> z) { return grtm::cx<float>{float(z.x), float(z.y)}; };
     for (size_t i = 0; i < lw.size(); i++) {
         const LayerW<double> &d = lw[i];
-        lf[i] = LayerW<float>{d.sad, d.sbd, F(d.sa), F(d.sb), F(d.mg), F(d.ca), F(d.cb), F(d.mu),
+        lf[i] = LayerW<float>{d.sa2, d.sb2, F(d.sa), F(d.sb), F(d.mg), F(d.ca), F(d.cb), F(d.mu),
                               F(d.a), F(d.b), d.h, 0.0};
     }
     return lf;
@@ -1610,8 +1610,8 @@ int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi,
         const double *l = lay + 7 * j;
         const cd a(l[1], l[2]), b(l[3], l[4]), mu(l[5], l[6]);
         auto C = [](cd z) { return grtm::cx<double>{z.real(), z.imag()}; };
-        lw[j].sad = C(w / a);
-        lw[j].sbd = C(w / b);
+        lw[j].sa2 = C((w / a) * (w / a));
+        lw[j].sb2 = C((w / b) * (w / b));
         lw[j].sa = C(w / a);
         lw[j].sb = C(w / b);
         lw[j].mg = C(mu * (w / b) * (w / b));

@@ -75,8 +75,8 @@ static int count_cell(int kind, int L, int s, int rl, int branch, int up, int bo
     for (int j = 0; j < L; j++) {
         const double a = 5.0 + 0.3 * j, b = 3.0 + 0.15 * j;
         LayerW<Cnt> &q = lw[j];
-        q.sad = cx<Cnt>{Cnt(w_re / a), Cnt(w_im / a)};
-        q.sbd = cx<Cnt>{Cnt(w_re / b), Cnt(w_im / b)};
+        q.sa2 = cx<Cnt>{Cnt((w_re * w_re - w_im * w_im) / (a * a)), Cnt(2 * w_re * w_im / (a * a))};
+        q.sb2 = cx<Cnt>{Cnt((w_re * w_re - w_im * w_im) / (b * b)), Cnt(2 * w_re * w_im / (b * b))};
         q.sa = cx<Cnt>{Cnt(w_re / a), Cnt(w_im / a)};
         q.sb = cx<Cnt>{Cnt(w_re / b), Cnt(w_im / b)};
         q.mg = cx<Cnt>{Cnt(2.7 * (w_re * w_re - w_im * w_im)), Cnt(2.7 * 2 * w_re * w_im)};
