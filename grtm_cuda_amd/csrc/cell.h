@@ -229,7 +229,8 @@ GRTM_HD m2<T> free_surface(const LQ<T> &q, typename WideOf<T>::type kd, cx<typen
     const cx<T> rv4 = (T(4) * k2) * (q.r * q.v);
     const cx<T> x2 = q.x * q.x;
     const cx<W> xd = C<W>(W(2) * kd * kd - sb2.x, -sb2.y);
-    const cx<T> nr = narrow<T>(recip(C<W>(W(4) * kd * kd, W(0)) * (q.rd * q.vd) - xd * xd));  // 1/det F
+    // det F in FP64, its reciprocal in the cell's precision
+    const cx<T> nr = recip(narrow<T>(C<W>(W(4) * kd * kd, W(0)) * (q.rd * q.vd) - xd * xd));  // 1/det F
     const cx<T> dd = (x2 + rv4) * nr;
     const cx<T> k4x = (T(4) * k) * q.x * nr;
     return m2<T>{dd, k4x * q.v, -(k4x * q.r), -dd};
@@ -297,7 +298,7 @@ GRTM_HD int sweep_bottom(const LayerW<T> *lw, int L, int s, typename WideOf<T>::
     for (int j = s; j < L - 1; j++) {
         const cx<T> sb = lw[j].sb;
         const cx<T> z = (C<T>(k, T(0)) - sb) * (C<T>(k, T(0)) + sb);
-        const T re = sqrt(T(0.5) * (sqrt(z.x * z.x + z.y * z.y) + z.x));  // Re sqrt(z)
+        const T re = sqrt_est(T(0.5) * (sqrt_est(z.x * z.x + z.y * z.y) + z.x));  // Re sqrt(z)
         x = x + re * T(j == s ? lw[j].h - z0 : lw[j].h);
         if (x >= cut) return j;
     }

@@ -63,6 +63,11 @@ GRTM_HD float fsqrt_est(float x)
 #endif
 }
 
+// Square root for estimates: the hardware FP32 instruction (about 1 ulp,
+// no correctly rounded sequence); FP64 keeps the library root.
+GRTM_HD float sqrt_est(float x) { return fsqrt_est(x); }
+GRTM_HD double sqrt_est(double x) { return ::sqrt(x); }
+
 // Reciprocal with Smith-style scaling (robust against over/underflow of |z|^2).
 template <typename T> GRTM_HD cx<T> recip(cx<T> b)
 {

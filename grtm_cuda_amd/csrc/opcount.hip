@@ -32,6 +32,7 @@ inline Cnt fabs(Cnt a) { return Cnt(std::fabs(a.v)); }
 inline Cnt copysign(Cnt a, Cnt b) { return Cnt(std::copysign(a.v, b.v)); }
 inline Cnt sqrt(Cnt a) { g_t.flops += 1; return Cnt(std::sqrt(a.v)); }
 inline Cnt fma(Cnt a, Cnt b, Cnt c) { g_t.flops += 2; return Cnt(std::fma(a.v, b.v, c.v)); }
+inline Cnt sqrt_est(Cnt a) { g_t.flops += 1; return Cnt(std::sqrt(a.v)); }
 inline Cnt hypot(Cnt a, Cnt b) { g_t.trans += 1; return Cnt(std::hypot(a.v, b.v)); }
 inline Cnt exp(Cnt a) { g_t.trans += 1; return Cnt(std::exp(a.v)); }
 inline Cnt log(Cnt a) { g_t.trans += 1; return Cnt(std::log(a.v)); }
@@ -52,6 +53,7 @@ using grtm_count::hypot;
 using grtm_count::sincos_t;
 using grtm_count::sqrt;
 using grtm_count::fma;
+using grtm_count::sqrt_est;
 }  // namespace grtm
 
 #include "cell.h"
