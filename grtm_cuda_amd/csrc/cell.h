@@ -593,7 +593,7 @@ template <typename T, bool NEED_T, bool COMPAT>
 GRTM_HD void coef_from_cell(const CellOut<T> &c, int src_type, int up, T k, cx<T> w, Coef<T> &o)
 {
     const LQ<T> &q = c.sq;
-    const cx<T> b = w * recip(q.sb);  // beta
+    const cx<T> b = w * q.cb;  // beta
     const m2<T> I2 = ident<T>();
     // wr?0 and wr?1 for the selected direction
     const m2<T> W0 = up ? (c.ERd - I2) : (I2 - c.ERu);
@@ -602,8 +602,8 @@ GRTM_HD void coef_from_cell(const CellOut<T> &c, int src_type, int up, T k, cx<T
     if (src_type == 0) {
         // f0 = b/(2 mu a w r v) * (b v r, -a k r); f1 = (k b v, -a r v) * same
         const cx<T> tmp = b * recip(T(2) * q.mu * w * w * q.r * q.v) * q.sa;  // b/(2 mu a w r v)
-        cx<T> f00 = b * q.v * q.r * tmp, f01 = -(k * (w * recip(q.sa)) * q.r * tmp);
-        cx<T> f10 = k * b * q.v * tmp, f11 = -((w * recip(q.sa)) * q.r * q.v * tmp);
+        cx<T> f00 = b * q.v * q.r * tmp, f01 = -(k * (w * q.ca) * q.r * tmp);
+        cx<T> f10 = k * b * q.v * tmp, f11 = -((w * q.ca) * q.r * q.v * tmp);
         src_scaled(q, f00, f01);
         src_scaled(q, f10, f11);
         cx<T> g00, g01, g10, g11;
@@ -621,7 +621,7 @@ GRTM_HD void coef_from_cell(const CellOut<T> &c, int src_type, int up, T k, cx<T
             o.a2t[0] = o.a2t[1] = o.b2t = C<T>(0, 0);
         }
     } else {
-        const cx<T> av = w * recip(q.sa);
+        const cx<T> av = w * q.ca;  // alpha
         const cx<T> tmp = b * recip(T(2) * q.mu * av * w);  // b/(2 mu a w)
         cx<T> s20 = tmp * k * recip(q.r) * k * b, s21 = -(k * tmp * av);
         cx<T> s10 = T(-2) * k * b * tmp, s11 = tmp * recip(q.v) * q.x * av;
@@ -666,8 +666,8 @@ GRTM_HD void zr_from_cell(const CellOut<T> &c, int up, T k, cx<T> w, cx<T> &kz_e
     const m2<T> I2 = ident<T>();
     const m2<T> W0 = up ? (c.ERd - I2) : (I2 - c.ERu);
     const m2<T> W1 = up ? (c.ERd + I2) : (I2 + c.ERu);
-    const cx<T> b = w * recip(q.sb);
-    const cx<T> av = w * recip(q.sa);
+    const cx<T> b = w * q.cb;  // beta
+    const cx<T> av = w * q.ca;  // alpha
     const cx<T> tmp = b * recip(T(2) * q.mu * av * w);
     // explosion; the source vector carries the a/w normalisation of the
     // source layer (q.sa q.ca = 1)

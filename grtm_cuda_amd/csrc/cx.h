@@ -54,6 +54,14 @@ GRTM_HD double drcp_est(double x)
     return 1.0 / x;
 #endif
 }
+GRTM_HD float frsq(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
 GRTM_HD float fsqrt_est(float x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -137,7 +145,7 @@ GRTM_HD float fcos_rev(float x)
 
 // Principal square root of an FP64 value to about 1e-13 relative without FP64
 // divides or square roots: an FP32 estimate (hardware square root and
-// reciprocal, about 2^-21) refined by one Newton step s = (s0 + z / s0) / 2 in
+// reciprocal square root, about 2^-21) refined by one Newton step s = (s0 + z / s0) / 2 in
 // FP64, z / s0 = z conj(s0) / |s0|^2 with a Newton-refined reciprocal.  The
 // step keeps the estimate's branch (Re >= 0).  |z| must lie well inside the
 // FP32 range (the vertical wavenumbers: 1e-12 < |z| < 1e18).
@@ -145,8 +153,10 @@ GRTM_HD cx<double> csqrt_nr(cx<double> z)
 {
     const float a = float(z.x), b = float(z.y);
     const float m = fsqrt_est(a * a + b * b);
-    const float t = fsqrt_est(0.5f * (m + fabsf(a)));
-    const float u = 0.5f * b * frcp(t);
+    const float h = 0.5f * (m + fabsf(a));
+    const float rh = frsq(h);  // t = sqrt(h) = h / sqrt(h), u = b / (2 t)
+    const float t = h * rh;
+    const float u = 0.5f * b * rh;
     const double s0x = a >= 0.f ? double(t) : double(fabsf(u));
     const double s0y = a >= 0.f ? double(u) : double(copysignf(t, b));
     const double n = s0x * s0x + s0y * s0y;
