@@ -365,7 +365,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
                         Md = TdJ;
                         md0 = td0J;
                     } else if (jj < rl && jj >= s + 1) {
-                        Md = Md * dmd(lj0, lj1, TdJ, C<T>(1, 0), C<T>(1, 0));
+                        Md = Md * dm(lj0, lj1, TdJ);
                         md0 = md0 * lj1 * td0J;
                     }
                 } else {
@@ -377,7 +377,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
                         LQ<T> q2 = layer_q(lw[jj - 2], kd);
                         cx<T> e0, e1;
                         lam(q2, lw[jj - 2].h, e0, e1);
-                        Md = Md * dmd(e0, e1, TdJ, C<T>(1, 0), C<T>(1, 0));
+                        Md = Md * dm(e0, e1, TdJ);
                         md0 = md0 * e1 * td0J;
                     }
                 }
@@ -437,7 +437,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
                     Mu = Tu;
                     mu0 = tu0;
                 } else if (jn > rl + 1) {
-                    Mu = Mu * dmd(e0, e1, Tu, C<T>(1, 0), C<T>(1, 0));
+                    Mu = Mu * dm(e0, e1, Tu);
                     mu0 = mu0 * e1 * tu0;
                 }
             }
@@ -504,12 +504,12 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
     } else if (branch == 1) {  // green.cu:775-857
         cx<T> rd0 = cexp_nz<T>(qr.rd, zr), rd1 = cexp_nz<T>(qr.vd, zr);
         cx<T> ru0 = cexp_nz<T>(qr.rd, hr_ - zr), ru1 = cexp_nz<T>(qr.vd, hr_ - zr);
-        cx<T> ex0, ex1;
-        lam(qr, hr_, ex0, ex1);
+        // Lambda of the receiver layer: e^{-nu h} = e^{-nu zr} e^{-nu (h - zr)}
+        const cx<T> ex0 = rd0 * ru0, ex1 = rd1 * ru1;
         const m2<T> RS = dmd(rd0, rd1, Rud_r, ex0, ex1);
         const m2<T> u = mmadd(Ed, RS, md(Eu, ru0, ru1));
         const cx<T> u0 = rd1 * rud0_r * ex1 + ru1;
-        const m2<T> chain = Mu * dmd(sexu0, sexu1, yu_psv, C<T>(1, 0), C<T>(1, 0));
+        const m2<T> chain = Mu * dm(sexu0, sexu1, yu_psv);
         const cx<T> chain0 = mu0 * sexu1 * y_sh;
         out.Y = u * chain;
         out.y0 = u0 * chain0;
@@ -533,7 +533,10 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             LQ<T> qp = layer_q(lw[rl - 1], kd);
             lam(qp, lw[rl - 1].h, ex0, ex1);
         } else {
-            lam(qr, hr_, ex0, ex1);
+            // e^{-nu h} = e^{-nu zr} e^{-nu (h - zr)}; ex only enters times
+            // ru, which is zero in the half-space
+            ex0 = rd0 * ru0;
+            ex1 = rd1 * ru1;
         }
         const m2<T> RS = dmd(ru0, ru1, Rdu_r1, ex0, ex1);
         const m2<T> d = mmadd(Eu, RS, md(Ed, rd0, rd1));
@@ -547,7 +550,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             const LayerW<T> &lr = lw[rl], &lr1 = lw[rl + 1], &ls1 = lw[s + 1], &ls = lw[s];
             Mc = dmd(lr.ca * recip(lr1.ca), lr.cb * recip(lr1.cb), Md, ls1.ca * recip(ls.ca), ls1.cb * recip(ls.cb));
         }
-        const m2<T> chain = Mc * dmd(sexd0, sexd1, ypsv, C<T>(1, 0), C<T>(1, 0));
+        const m2<T> chain = Mc * dm(sexd0, sexd1, ypsv);
         const cx<T> chain0 = md0 * sexd1 * y_sh;
         out.Y = d * chain;
         out.y0 = d0 * chain0;

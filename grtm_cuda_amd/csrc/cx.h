@@ -219,6 +219,11 @@ template <typename T> GRTM_HD m2<T> dmd(cx<T> l0, cx<T> l1, const m2<T> &p, cx<T
 {
     return {l0 * p.a * r0, l0 * p.b * r1, l1 * p.c * r0, l1 * p.d * r1};
 }
+// diag(l) * p
+template <typename T> GRTM_HD m2<T> dm(cx<T> l0, cx<T> l1, const m2<T> &p)
+{
+    return {l0 * p.a, l0 * p.b, l1 * p.c, l1 * p.d};
+}
 // p * diag(r)
 template <typename T> GRTM_HD m2<T> md(const m2<T> &p, cx<T> r0, cx<T> r1)
 {
