@@ -33,7 +33,7 @@ struct LayerW {
     cx<T> mu;      // shear modulus
     cx<T> a, b;    // alpha, beta
     W h;           // thickness
-    W pad;
+    W hrev, hlog;  // -h / (2 pi), -h log2(e): exponent factors of Lambda
 };
 
 // Geometry of one (model, source) pair.
@@ -245,6 +245,28 @@ GRTM_HD void src_scaled(const LQ<T> &q, cx<T> &v0, cx<T> &v1)
     v1 = v1 * q.cb;
 }
 
+// e^{-nu h} for a layer's thickness from its precomputed factors: the phase
+// in revolutions nu_y (-h / (2 pi)) and the modulus exponent nu_x (-h log2 e)
+// each in one FP64 product (as cexp_nz for FP32 cells).
+template <typename T, typename W>
+GRTM_HD cx<T> cexp_layer(cx<W> nu, const LayerW<T> &L)
+{
+    if constexpr (sizeof(T) == sizeof(W)) {
+        return cexp_nz<T>(nu, L.h);
+    } else {
+        const double rev = nu.y * L.hrev;
+        const float f = float(rev - rint(rev));
+        const float e = fexp2(float(nu.x * L.hlog));
+        return C<T>(e * fcos_rev(f), e * fsin_rev(f));
+    }
+}
+template <typename T>
+GRTM_HD void lam_layer(const LQ<T> &q, const LayerW<T> &L, cx<T> &e0, cx<T> &e1)
+{
+    e0 = cexp_layer<T>(q.rd, L);
+    e1 = cexp_layer<T>(q.vd, L);
+}
+
 template <typename T, typename W>
 GRTM_HD void lam(const LQ<T> &q, W h, cx<T> &e0, cx<T> &e1)
 {
@@ -387,7 +409,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
         for (int lay = bot - 2; lay >= s; lay--) {
             // interface lay: between layer lay (above) and lay+1 (below, = qu)
             cx<T> e0, e1;
-            lam(qu, lw[lay + 1].h, e0, e1);
+            lam_layer(qu, lw[lay + 1], e0, e1);
             qd = qu;
             qu = layer_q(lw[lay], kd);
             // green.cu:532-594: generalized R_du, T_d across interface lay with
@@ -416,7 +438,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
         rud0_r = rg0;
         for (int lay = 0; lay < s; lay++) {
             cx<T> e0, e1;
-            lam(qu, lw[lay].h, e0, e1);  // Lambda_lay
+            lam_layer(qu, lw[lay], e0, e1);  // Lambda_lay
             LQ<T> qd = layer_q(lw[lay + 1], kd);
             // green.cu:471-520: generalized R_ud, T_u across interface lay
             // (incidence from below) with the stack above reflecting

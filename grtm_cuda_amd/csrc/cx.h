@@ -146,7 +146,8 @@ GRTM_HD float fcos_rev(float x)
 // Principal square root of an FP64 value to about 1e-13 relative without FP64
 // divides or square roots: an FP32 estimate (hardware square root and
 // reciprocal square root, about 2^-21) refined by one Newton step s = (s0 + z / s0) / 2 in
-// FP64, z / s0 = z conj(s0) / |s0|^2 with a Newton-refined reciprocal.  The
+// FP64, z / s0 = z conj(s0) / |s0|^2 with a reciprocal refined by one Newton
+// step (the square root's step needs about 2^-45).  The
 // step keeps the estimate's branch (Re >= 0).  |z| must lie well inside the
 // FP32 range (the vertical wavenumbers: 1e-12 < |z| < 1e18).
 GRTM_HD cx<double> csqrt_nr(cx<double> z)
@@ -161,8 +162,7 @@ GRTM_HD cx<double> csqrt_nr(cx<double> z)
     const double s0y = a >= 0.f ? double(u) : double(copysignf(t, b));
     const double n = s0x * s0x + s0y * s0y;
     double inv = drcp_est(n);
-    inv = inv * (2.0 - n * inv);
-    inv = inv * (2.0 - n * inv);
+    inv = inv * (2.0 - n * inv);  // about 2^-52 from the 2^-26 estimate
     const double qx = (z.x * s0x + z.y * s0y) * inv, qy = (z.y * s0x - z.x * s0y) * inv;
     return {0.5 * (s0x + qx), 0.5 * (s0y + qy)};
 }

@@ -218,7 +218,8 @@ void layer_w(const Stack &st, int compat, cd w, LayerW<double> *out)
         q.a = C(a);
         q.b = C(b);
         q.h = st.h[j];
-        q.pad = 0;
+        q.hrev = -st.h[j] / (2 * M_PI);
+        q.hlog = -st.h[j] * M_LOG2E;
     }
 }
 
@@ -566,7 +567,7 @@ std::vector<LayerW<float>> narrow_tables(const std::vector<LayerW<double>> &lw)
     for (size_t i = 0; i < lw.size(); i++) {
         const LayerW<double> &d = lw[i];
         lf[i] = LayerW<float>{d.sa2, d.sb2, F(d.sa), F(d.sb), F(d.mg), F(d.ca), F(d.cb), F(d.mu),
-                              F(d.a), F(d.b), d.h, 0.0};
+                              F(d.a), F(d.b), d.h, d.hrev, d.hlog};
     }
     return lf;
 }
@@ -1621,7 +1622,8 @@ int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi,
         lw[j].a = C(a);
         lw[j].b = C(b);
         lw[j].h = l[0];
-        lw[j].pad = 0;
+        lw[j].hrev = -l[0] / (2 * M_PI);
+        lw[j].hlog = -l[0] * M_LOG2E;
     }
     Stack st;
     st.L = n_layer;

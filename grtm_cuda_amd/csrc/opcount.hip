@@ -88,7 +88,8 @@ static int count_cell(int kind, int L, int s, int rl, int branch, int up, int bo
         q.a = cx<Cnt>{Cnt(a), Cnt(0)};
         q.b = cx<Cnt>{Cnt(b), Cnt(0)};
         q.h = Cnt(2.0 + j);
-        q.pad = Cnt(0);
+        q.hrev = Cnt(-(2.0 + j) / (2 * M_PI));
+        q.hlog = Cnt(-(2.0 + j) * M_LOG2E);
     }
     grtm::Geo g;
     g.L = L;
