@@ -437,35 +437,43 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f32sk(int n
 // C of every tile split between blocks: the sum, in FP64 and block order, of
 // its partials; C of a tile without slabs: zero.  Thread e covers element e
 // of the concatenated outputs; tiles written whole are left alone.
+// row slices per tile of the reduce (enough workgroups for a few hundred tiles)
+constexpr int kReduceSlices = 8;
 template <int BM, int BN>
-__global__ void k_streamk_reduce(int n, int N, GemmSet gs, StreamK sk, const int *__restrict__ off,
-                                 const float *__restrict__ part, long total)
+__global__ __launch_bounds__(256) void k_streamk_reduce(int n, int N, GemmSet gs, StreamK sk,
+                                                        const int *__restrict__ off, const float *__restrict__ part)
 {
-    const long e0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e0 >= total) return;
+    // workgroups (tile, slice of BM / gridDim.y rows): blockIdx.x runs over
+    // all contractions' tiles; the tile's bounds and blocks are found once
+    // per workgroup, not per element
+    const int t = blockIdx.x;
+    const int i0 = blockIdx.y * (BM / gridDim.y) * BN, i1 = i0 + (BM / gridDim.y) * BN;
     int p = 0;
-    long e = e0;
-    while (p + 1 < n && e >= (long)gs.M[p] * N) {
-        e -= (long)gs.M[p] * N;
-        p++;
-    }
-    const int row = int(e / N), col = int(e % N);
-    const int t = sk.cum[p] + row / BM * sk.ncol + col / BN;
+    while (p + 1 < n && t >= sk.cum[p + 1]) p++;
+    const int tl = t - sk.cum[p];
+    const int m0 = tl / sk.ncol * BM, n0 = tl % sk.ncol * BN;
+    const int M = gs.M[p];
+    float *__restrict__ C = gs.C[p];
     const long G = off[sk.cum[n]], t0 = off[t], t1 = off[t + 1];
-    if (t1 == t0) {
-        gs.C[p][e] = 0.f;
+    if (t1 == t0) {  // empty K extent: the contraction kernel wrote nothing
+        for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            const int r = m0 + i / BN, c = n0 + i % BN;
+            if (r < M && c < N) C[(size_t)r * N + c] = 0.f;
+        }
         return;
     }
     const int nbe = int(min((long)sk.nb, G));
     const int b0 = sk_block_of(t0, G, nbe), b1 = sk_block_of(t1 - 1, G, nbe);
     if (b0 == b1) return;  // written whole by block b0
-    const size_t loc = (size_t)(row % BM) * BN + col % BN;
-    double s = 0;
-    for (int b = b0; b <= b1; b++) {
-        const int slot = (b == b0 && G * b / nbe < t0) ? 1 : 0;
-        s += double(part[((size_t)b * 2 + slot) * BM * BN + loc]);
+    // block b0 left its part in slot 1 when the tile is not its first
+    const int slot0 = (G * b0 / nbe < t0) ? 1 : 0;
+    for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const int r = m0 + i / BN, c = n0 + i % BN;
+        if (r >= M || c >= N) continue;
+        double s = double(part[((size_t)b0 * 2 + slot0) * BM * BN + i]);
+        for (int b = b0 + 1; b <= b1; b++) s += double(part[(size_t)b * 2 * BM * BN + i]);
+        C[(size_t)r * N + c] = float(s);
     }
-    gs.C[p][e] = float(s);
 }
 
 // FP32 block tile for N columns: 128 rows by 128 columns (waves of 32 x 64),
@@ -792,10 +800,9 @@ static hipError_t launch_streamk(int n, const GemmSet &gs, int N, int K, long ld
     const int bn = f32_bn(N);
     StreamK sk{};
     int t = 0;
-    long total = 0;
     for (int q = 0; q < kGemmSetMax; q++) {
         sk.cum[q] = t;
-        if (q < n) t += (gs.M[q] + kTBM - 1) / kTBM * ((N + bn - 1) / bn), total += (long)gs.M[q] * N;
+        if (q < n) t += (gs.M[q] + kTBM - 1) / kTBM * ((N + bn - 1) / bn);
     }
     sk.cum[kGemmSetMax] = t;
     sk.ncol = (N + bn - 1) / bn;
@@ -806,13 +813,13 @@ static hipError_t launch_streamk(int n, const GemmSet &gs, int N, int K, long ld
     if (bn == 128) {
         hipLaunchKernelGGL((k_gemm_f32sk<kTBM, 128, 32, 64>), dim3(sk.nb), dim3(512), 0, st, n, N, K, gs, sk, lda,
                            ldb, nk_w, off, part);
-        hipLaunchKernelGGL((k_streamk_reduce<kTBM, 128>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, n,
-                           N, gs, sk, off, part, total);
+        hipLaunchKernelGGL((k_streamk_reduce<kTBM, 128>), dim3((unsigned)t, kReduceSlices), dim3(256), 0, st, n, N, gs,
+                           sk, off, part);
     } else {
         hipLaunchKernelGGL((k_gemm_f32sk<kTBM, 64, 32, 32>), dim3(sk.nb), dim3(512), 0, st, n, N, K, gs, sk, lda, ldb,
                            nk_w, off, part);
-        hipLaunchKernelGGL((k_streamk_reduce<kTBM, 64>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, n, N,
-                           gs, sk, off, part, total);
+        hipLaunchKernelGGL((k_streamk_reduce<kTBM, 64>), dim3((unsigned)t, kReduceSlices), dim3(256), 0, st, n, N, gs,
+                           sk, off, part);
     }
     return hipGetLastError();
 }
