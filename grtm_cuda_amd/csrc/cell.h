@@ -693,10 +693,13 @@ GRTM_HD void zr_from_cell(const CellOut<T> &c, int up, T k, cx<T> w, cx<T> &kz_e
     const m2<T> W1 = up ? (c.ERd + I2) : (I2 + c.ERu);
     const cx<T> b = w * q.cb;  // beta
     const cx<T> av = w * q.ca;  // alpha
-    const cx<T> tmp = b * recip(T(2) * q.mu * av * w);
+    // one reciprocal for the three denominators: R = 1 / (2 mu w alpha r v)
+    const cx<T> rv = q.r * q.v;
+    const cx<T> bR = b * recip((T(2) * q.mu * w) * av * rv);
+    const cx<T> tmp = bR * rv;  // b / (2 mu alpha w)
     // explosion; the source vector carries the a/w normalisation of the
-    // source layer (q.sa q.ca = 1)
-    const cx<T> sx = -(b * q.sa * tmp * recip(q.r));
+    // source layer (q.sa q.ca = 1): -b (w/alpha) tmp / r
+    const cx<T> sx = -(b * q.sa * (bR * q.v));
     const cx<T> e0 = W1.a * sx, e1 = W1.c * sx;
     cx<T> a0, a1;
     mv(c.Y, e0, e1, a0, a1);
@@ -704,7 +707,7 @@ GRTM_HD void zr_from_cell(const CellOut<T> &c, int up, T k, cx<T> w, cx<T> &kz_e
     kr_ex = COMPAT ? -(a1 * k) : -(a0 * k);
     // vertical force: f0 = (b^2/(2 mu a w), -b k/(2 mu w v))
     cx<T> f00 = b * tmp;
-    cx<T> f01 = -(b * k * recip(T(2) * q.mu * w * q.v));
+    cx<T> f01 = -(k * (bR * av * q.r));  // b k / (2 mu w v)
     src_scaled(q, f00, f01);
     cx<T> g0, g1, v0, v1;
     mv(W0, f00, f01, g0, g1);
