@@ -1218,6 +1218,23 @@ int grtm_plan_set_rows64(grtm_plan *p, int rows64)
     return p->adaptive ? build_items(p) : GRTM_OK;
 }
 
+// Cut n and residual-check cells nc of one frequency row.  Check cell j
+// sits at wavenumber n + stride j + stride / 2 and must lie below nk; a row
+// whose cut leaves no room for one is kept dense, since its tail would
+// otherwise be neither checked nor solved.
+static void row_cut(int nk, double kcut, double dk, int stride, double window, int &n, int &nc)
+{
+    n = (int)std::min<double>(nk, std::max(1.0, std::ceil(kcut / dk)));
+    const int room = nk - n;
+    const int fit = room > stride / 2 ? (room - stride / 2 - 1) / stride + 1 : 0;
+    if (room > 0 && fit == 0) {
+        n = nk;
+        nc = 0;
+        return;
+    }
+    nc = std::max(0, std::min(fit, (int)std::ceil(window / (dk * stride))));
+}
+
 int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int stride, double tol)
 {
     if (!p) return GRTM_E_INPUT;
@@ -1244,12 +1261,10 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
     int mx = 0;
     for (int iw = 0; iw < p->nw; iw++) {
         if (!(kcut[iw] >= 0)) return GRTM_E_INPUT;
-        const double nkd = std::ceil(kcut[iw] / p->dk);
-        const int n = (int)std::min<double>(p->nk, std::max(1.0, nkd));
+        int n, nc;
+        row_cut(p->nk, kcut[iw], p->dk, stride, window, n, nc);
         p->nk_pred_h[iw] = n;
-        const int room = p->nk - n;
-        const int nc = std::min((room + stride - 1) / stride, (int)std::ceil(window / (p->dk * stride)));
-        p->nchk_h[iw] = std::max(0, nc);
+        p->nchk_h[iw] = nc;
         mx = std::max(mx, p->nchk_h[iw]);
     }
     int st;
@@ -2078,11 +2093,10 @@ int grtm_batch_set_adaptive(grtm_batch *b, const double *kcut, const double *win
             for (int iw = 0; iw < b->nw; iw++) {
                 const size_t i = (size_t)m * b->nw + iw;
                 if (!(kcut[i] >= 0)) return GRTM_E_INPUT;
-                const int nkp = (int)std::min<double>(b->nk, std::max(1.0, std::ceil(kcut[i] / b->dk)));
-                const int room = b->nk - nkp;
-                const int nc = std::min((room + stride - 1) / stride, (int)std::ceil(window[m] / (b->dk * stride)));
+                int nkp, nc;
+                row_cut(b->nk, kcut[i], b->dk, stride, window[m], nkp, nc);
                 b->nk_pred_h[i] = nkp;
-                b->nchk_h[i] = std::max(0, nc);
+                b->nchk_h[i] = nc;
                 b->max_chk[m] = std::max(b->max_chk[m], b->nchk_h[i]);
             }
         }

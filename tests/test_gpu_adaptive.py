@@ -80,3 +80,26 @@ def test_analytic_cut(c1_dense):
     frac = st["cells_final"] / (cfg["grid"]["nw"] * cfg["grid"]["nk"])
     print(f"analytic cut: {frac * 100:.1f}% cells, {st['rows_fallback']} fallback rows, rel-L2 {worst:.2e}")
     assert worst < 1e-4
+
+
+@pytest.mark.parametrize("prec,tol", [(g.PREC_FP64, 1e-9), (g.PREC_FP32, 1e-4)])
+def test_cut_next_to_nk_keeps_the_row_dense(prec, tol):
+    """A cut so close to the end of the grid that no residual-check cell fits
+    below it (check cells sit at cut + stride j + stride / 2) keeps the row
+    dense: for a source 30 m below the receivers the last wavenumbers still
+    carry the trace, and an unchecked tail would be dropped."""
+    layers = models.C1_LAYERS
+    grid = models.dense_grid(128, 0.1, 1024, float(layers[:, 2].min()))
+    dists = [5.0, 20.0]
+    ref = g.TracePlan(layers, 0.03, 0.0, dists, grid, precision=g.PREC_FP64).traces()
+    plan = g.TracePlan(layers, 0.03, 0.0, dists, grid, precision=prec)
+    dk = grid["kmax"] / grid["nk"]
+    print(f"gpu: cut next to nk, precision {prec}", flush=True)
+    for room in (1, 4):
+        plan.set_adaptive(np.full(grid["nw"], (grid["nk"] - room) * dk), window=1.0, stride=8, tol=3e-6)
+        tr = plan.traces()
+        st = plan.adaptive_stats()
+        err = (np.linalg.norm(tr - ref, axis=-1) / np.linalg.norm(ref, axis=-1)).max()
+        print(f"  room {room}: cells {st['cells_final']:.0f}, rel-L2 {err:.2e}")
+        assert st["cells_final"] == grid["nw"] * grid["nk"], room
+        assert err < tol, (room, err)
