@@ -157,4 +157,12 @@ class Predictor:
         w = 2 * math.pi * np.arange(nw) / (2 * nw * dt)
         y = self.mlp.predict(features(layers, src_depth, w))
         kc = decode_kcut(y, layers, src_depth, rcv_depth, w, self.lmargin)
+        # Rows below the plans' FP64 split (|w| < 2.5 beta_min / dz) keep at
+        # least the analytic cut: they are few and their cuts short, while a
+        # failed check sends such a row to a dense FP64 completion, a
+        # latency-bound launch on the critical path
+        from grtm_cuda_amd import models
+        dz = max(abs(src_depth - rcv_depth), 1e-3)
+        low = w < 2.5 * float(np.min(layers[:, 2])) / dz
+        kc = np.where(low, np.maximum(kc, models.analytic_kcut(layers, src_depth, rcv_depth, grid)), kc)
         return np.minimum(kc, grid["kmax"])
