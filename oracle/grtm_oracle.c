@@ -5,6 +5,17 @@
  * restates.  Complex semantics follow cusrc/scilib/cucomplex.hpp:
  * sqrt = polar(sqrt|z|, arg/2) (:945-958), log = (log|z|, atan2) (:925-930),
  * exp = e^re (cos, sin) (:963-985).
+ *
+ * Arithmetic type.  The default build computes in FP64, as the reference does
+ * (float32 is double, cusrc/scilib/mathematics.h:26).  Built with -DOR_QUAD
+ * (libgrtm_oracle_q.so) every cell, k-sum, post-processing and inverse-FFT
+ * value is computed in IEEE binary128 (__float128, 113-bit significand)
+ * instead: the same restatement with 2^-60 smaller rounding, which decides
+ * what the FP64 answer should be where the formulation is ill-conditioned
+ * (the lowest frequency rows).  The grid (k spacing, per-row k counts,
+ * frequencies) and the input parsing stay in FP64 in both builds, so both
+ * evaluate the same (w, k) points as the reference.  The API takes and
+ * returns double in both builds.
  */
 #include "grtm_oracle.h"
 
@@ -17,25 +28,69 @@
 #include <omp.h>
 #endif
 
+#ifdef OR_QUAD
+#include <quadmath.h>
+typedef __float128 or_real;
+typedef __complex128 cd;
+#define CRE crealq
+#define CIM cimagq
+#define CABS cabsq
+#define CARG cargq
+#define CONJ conjq
+#define RSQRT sqrtq
+#define RCOS cosq
+#define RSIN sinq
+#define REXP expq
+#define RLOG logq
+#define RFABS fabsq
+#define RJN jnq
+#define R_PI M_PIq
+static inline cd MKC(or_real re, or_real im)
+{
+    cd z;
+    __real__ z = re;
+    __imag__ z = im;
+    return z;
+}
+#else
+typedef double or_real;
 typedef double complex cd;
+#define CRE creal
+#define CIM cimag
+#define CABS cabs
+#define CARG carg
+#define CONJ conj
+#define RSQRT sqrt
+#define RCOS cos
+#define RSIN sin
+#define REXP exp
+#define RLOG log
+#define RFABS fabs
+#define RJN jn
+#define R_PI 3.1415926535897932384626433
+#define MKC CMPLX
+#endif
 
+/* OR_PI: the FP64 constant of the grid (green.cu:2065-2080, 2146);
+ * R_PI: pi in the arithmetic type, for cell values and the transforms. */
 #define OR_PI 3.1415926535897932384626433
 #define OR_PI2 (OR_PI * 2.0)
+#define R_PI2 (R_PI * 2)
 
 /* ------------------------------------------------------------------ */
 /* complex helpers (cucomplex.hpp semantics)                           */
 /* ------------------------------------------------------------------ */
 static cd c_sqrt(cd z)
 {
-    double m = sqrt(cabs(z));
-    double t = carg(z) * 0.5;
-    return CMPLX(m * cos(t), m * sin(t));
+    or_real m = RSQRT(CABS(z));
+    or_real t = CARG(z) * 0.5;
+    return MKC(m * RCOS(t), m * RSIN(t));
 }
-static cd c_log(cd z) { return CMPLX(log(cabs(z)), carg(z)); }
+static cd c_log(cd z) { return MKC(RLOG(CABS(z)), CARG(z)); }
 static cd c_exp(cd z)
 {
-    double e = exp(creal(z));
-    return CMPLX(e * cos(cimag(z)), e * sin(cimag(z)));
+    or_real e = REXP(CRE(z));
+    return MKC(e * RCOS(CIM(z)), e * RSIN(CIM(z)));
 }
 
 /* 2x2 helpers (green.cu:388-423) */
@@ -57,7 +112,7 @@ static void m2_mul(cd a[2][2], cd b[2][2], cd c[2][2])
 /* ------------------------------------------------------------------ */
 /* per-cell state (field meanings of cusrc/popm/popm.h:16-136)        */
 /* ------------------------------------------------------------------ */
-typedef struct { double h; cd a, b, u, r, v, x; } OPar;
+typedef struct { or_real h; cd a, b, u, r, v, x; } OPar;
 typedef struct { cd tdd1[2][2], tuu1[2][2], rdu1[2][2], rud1[2][2], tdd0, tuu0, rdu0, rud0; } OMrt;
 typedef struct { cd rud1[2][2], rdu1[2][2], tu1[2][2], td1[2][2], rud0, rdu0, tu0, td0; } OGrt;
 typedef struct {
@@ -68,32 +123,40 @@ typedef struct {
 typedef struct { cd a0[2], a1[2], a2[2], b1, b2, a0t[2], a1t[2], a2t[2], b1t, b2t; } OPpar;
 
 /* Dispersed velocity v0*(1 + ln(w/2pi)/(pi Q) + i/(2Q))  (green.cu:177-180) */
-static cd disp_vel(cd w, double v0, double q)
+static cd disp_vel(cd w, or_real v0, or_real q)
 {
-    cd f = 1.0 + c_log(w / OR_PI2) / (OR_PI * q) + I / (2.0 * q);
+    cd f = 1.0 + c_log(w / R_PI2) / (R_PI * q) + I / (2.0 * q);
+    return v0 * f;
+}
+
+/* The same dispersion in FP64 for the grid (cal_kmax), as the reference
+ * forms it; identical to disp_vel in the FP64 build. */
+static double complex disp_vel_d(double complex w, double v0, double q)
+{
+    double complex f = 1.0 + CMPLX(log(cabs(w / OR_PI2)), carg(w / OR_PI2)) / (OR_PI * q) + I / (2.0 * q);
     return v0 * f;
 }
 
 void or_dispersion(double wr, double wi, double v0, double q, double *o)
 {
-    cd z = disp_vel(CMPLX(wr, wi), v0, q);
-    o[0] = creal(z);
-    o[1] = cimag(z);
+    cd z = disp_vel(MKC(wr, wi), v0, q);
+    o[0] = CRE(z);
+    o[1] = CIM(z);
 }
 
 /* set_pars vertical wavenumbers (green.cu:184-194) */
-static void set_par(OPar *p, double k, cd w)
+static void set_par(OPar *p, or_real k, cd w)
 {
     cd wa = w / p->a, wb = w / p->b;
     p->r = c_sqrt(k * k - wa * wa);
     p->v = c_sqrt(k * k - wb * wb);
-    if (creal(p->r) < 0) p->r = -p->r;
-    if (creal(p->v) < 0) p->v = -p->v;
+    if (CRE(p->r) < 0) p->r = -p->r;
+    if (CRE(p->v) < 0) p->v = -p->v;
     p->x = k * k + p->v * p->v;
 }
 
 /* P-SV eigenvector matrix E (green.cu:196-214) */
-static void psv_e(const OPar *p, double k, cd w, cd E[4][4])
+static void psv_e(const OPar *p, or_real k, cd w, cd E[4][4])
 {
     E[0][0] = p->a * k / w;
     E[0][1] = p->b * p->v / w;
@@ -122,9 +185,9 @@ static void solve4(cd A[4][4], cd B[4][4], cd X[4][4])
     memcpy(b, B, sizeof(b));
     for (int c = 0; c < 4; c++) {
         int p = c;
-        double best = cabs(a[c][c]);
+        or_real best = CABS(a[c][c]);
         for (int r = c + 1; r < 4; r++)
-            if (cabs(a[r][c]) > best) { best = cabs(a[r][c]); p = r; }
+            if (CABS(a[r][c]) > best) { best = CABS(a[r][c]); p = r; }
         if (p != c) {
             for (int j = 0; j < 4; j++) {
                 cd t = a[c][j]; a[c][j] = a[p][j]; a[p][j] = t;
@@ -158,7 +221,7 @@ void or_set_tensor_sh_reference(int on) { g_tensor_sh_reference = on; }
 
 /* Interface between layer j (above) and j+1 (below):
  * SH (green.cu:339-353) and P-SV (green.cu:235-262 + 1813-1817 + 359-387). */
-static void interface_rt(const OPar *pu, const OPar *pd, double k, cd w, OMrt *m)
+static void interface_rt(const OPar *pu, const OPar *pd, or_real k, cd w, OMrt *m)
 {
     cd wa = pu->v * pu->u, wb = pd->v * pd->u;
     m->tdd0 = 2.0 * wa / (wa + wb);
@@ -202,9 +265,9 @@ static void interface_rt(const OPar *pu, const OPar *pd, double k, cd w, OMrt *m
 static void load_lay(const double *l, OPar *p)
 {
     p->h = l[0];
-    p->a = CMPLX(l[1], l[2]);
-    p->b = CMPLX(l[3], l[4]);
-    p->u = CMPLX(l[5], l[6]);
+    p->a = MKC(l[1], l[2]);
+    p->b = MKC(l[3], l[4]);
+    p->u = MKC(l[5], l[6]);
 }
 
 void or_interface_rt(double k, double wr, double wi, const double *lu,
@@ -215,7 +278,7 @@ void or_interface_rt(double k, double wr, double wi, const double *lu,
     double td[7] = {0, ld[0], ld[1], ld[2], ld[3], ld[4], ld[5]};
     load_lay(tu, &pu);
     load_lay(td, &pd);
-    cd w = CMPLX(wr, wi);
+    cd w = MKC(wr, wi);
     set_par(&pu, k, w);
     set_par(&pd, k, w);
     OMrt m;
@@ -224,19 +287,19 @@ void or_interface_rt(double k, double wr, double wi, const double *lu,
     int o = 0;
     for (int b = 0; b < 4; b++)
         for (int e = 0; e < 4; e++) {
-            out[o++] = creal(blk[b][e]);
-            out[o++] = cimag(blk[b][e]);
+            out[o++] = CRE(blk[b][e]);
+            out[o++] = CIM(blk[b][e]);
         }
     cd sh[4] = {m.tdd0, m.rud0, m.rdu0, m.tuu0};
     for (int e = 0; e < 4; e++) {
-        out[o++] = creal(sh[e]);
-        out[o++] = cimag(sh[e]);
+        out[o++] = CRE(sh[e]);
+        out[o++] = CIM(sh[e]);
     }
 }
 
 
 /* Generalized R/T recursion over layers (green.cu:424-613). */
-static void grt_sweeps(const OPar *par, const OMrt *mrt, int L, int s, double k, cd w,
+static void grt_sweeps(const OPar *par, const OMrt *mrt, int L, int s, or_real k, cd w,
                        OGrt *grt)
 {
     cd E[4][4], a[2][2], b[2][2], exu[2];
@@ -322,8 +385,8 @@ static void grt_sweeps(const OPar *par, const OMrt *mrt, int L, int s, double k,
 }
 
 /* Source terms.  Single force: green.cu:1271-1332; tensor: green.cu:1334-1413. */
-static void source_terms(int compat, int src_type, const OPar *sp, const OGrt *grt, int s, double zs,
-                         double k, cd w, OSrc *src)
+static void source_terms(int compat, int src_type, const OPar *sp, const OGrt *grt, int s, or_real zs,
+                         or_real k, cd w, OSrc *src)
 {
     cd sexd[2], sexu[2], wru0[2][2], wrd0[2][2], wru1[2][2], wrd1[2][2];
     sexd[0] = c_exp(-sp->r * (sp->h - zs));
@@ -399,7 +462,7 @@ static void diag_mul(cd m[2][2], const cd *dg, cd y[2][2], cd out[2][2])
  * u0t, :720), the r<s/r>s chains with yu0t/yd0t formed from the already
  * multiplied yu0/yd0 (:842, :902), and the r>s index shifts (Q6, :860-902). */
 static int ypars(int compat, int src_type, const OPar *par, const OGrt *grt, const OSrc *src,
-                 int L, int s, double zs, int rl, double zr, double k, cd w, cd par0_r,
+                 int L, int s, or_real zs, int rl, or_real zr, or_real k, cd w, cd par0_r,
                  cd par0_v, OPpar *pp)
 {
     cd sexd[2], sexu[2], rexd[2], rexu[2], rexdt[2], rexut[2];
@@ -617,8 +680,8 @@ static int ypars(int compat, int src_type, const OPar *par, const OGrt *grt, con
 }
 
 /* Full per-cell chain: set_pars .. get_ypars (green.cu:1799-1855). */
-static int cell_ppar(int compat, int src_type, double k, cd w, int L, const OPar *lay0,
-                     int s, double zs, int rl, double zr, cd par0_r, cd par0_v, OPpar *pp)
+static int cell_ppar(int compat, int src_type, or_real k, cd w, int L, const OPar *lay0,
+                     int s, or_real zs, int rl, or_real zr, cd par0_r, cd par0_v, OPpar *pp)
 {
     OPar par[OR_MAX_LAYERS];
     OMrt mrt[OR_MAX_LAYERS];
@@ -642,18 +705,18 @@ int or_cell(int compat, int src_type, double k, double wr, double wi, int n_laye
     OPar l0[OR_MAX_LAYERS];
     if (n_layer < 2 || n_layer > OR_MAX_LAYERS) return OR_E_INPUT;
     for (int j = 0; j < n_layer; j++) load_lay(lay + 7 * j, &l0[j]);
-    cd p0r = par0_rv ? CMPLX(par0_rv[0], par0_rv[1]) : 0.0;
-    cd p0v = par0_rv ? CMPLX(par0_rv[2], par0_rv[3]) : 0.0;
+    cd p0r = par0_rv ? MKC(par0_rv[0], par0_rv[1]) : 0.0;
+    cd p0v = par0_rv ? MKC(par0_rv[2], par0_rv[3]) : 0.0;
     OPpar pp;
-    int st = cell_ppar(compat, src_type, k, CMPLX(wr, wi), n_layer, l0, s_layer, zs, r_layer, zr,
+    int st = cell_ppar(compat, src_type, k, MKC(wr, wi), n_layer, l0, s_layer, zs, r_layer, zr,
                        p0r, p0v, &pp);
     if (st) return st;
     cd v[18] = {pp.a0[0], pp.a0[1], pp.a1[0], pp.a1[1], pp.a2[0], pp.a2[1], pp.b1, pp.b2,
                 pp.a0t[0], pp.a0t[1], pp.a1t[0], pp.a1t[1], pp.a2t[0], pp.a2t[1], pp.b1t, pp.b2t,
                 0, 0};
     for (int i = 0; i < 18; i++) {
-        out[2 * i] = creal(v[i]);
-        out[2 * i + 1] = cimag(v[i]);
+        out[2 * i] = CRE(v[i]);
+        out[2 * i + 1] = CIM(v[i]);
     }
     return OR_OK;
 }
@@ -663,73 +726,73 @@ int or_cell(int compat, int src_type, double k, double wr, double wi, int n_laye
 /* ------------------------------------------------------------------ */
 /* Abramowitz & Stegun 9.4.1-9.4.6 polynomial forms, as used by the reference
  * (green.cu:1418-1465).  The coefficients are the published table values. */
-static double as_j0(double x)
+static or_real as_j0(or_real x)
 {
-    static const double a[7] = {1.00000000, -2.24999970, 1.26562080, -0.31638660,
+    static const or_real a[7] = {1.00000000, -2.24999970, 1.26562080, -0.31638660,
                                 0.04444790, -0.00394440, 0.00021000};
-    static const double b[7] = {0.79788456, -0.00000077, -0.00552740, -0.00009512,
+    static const or_real b[7] = {0.79788456, -0.00000077, -0.00552740, -0.00009512,
                                 0.00137237, -0.00072805, 0.00014476};
-    static const double c[7] = {-0.78539816, -0.04166397, -0.00003954, 0.00262573,
+    static const or_real c[7] = {-0.78539816, -0.04166397, -0.00003954, 0.00262573,
                                 -0.00054125, -0.00029333, 0.00013558};
-    double ax = fabs(x);
+    or_real ax = RFABS(x);
     if (ax <= 3) {
-        double y = (ax / 3.0) * (ax / 3.0);
+        or_real y = (ax / 3.0) * (ax / 3.0);
         return a[0] + y * (a[1] + y * (a[2] + y * (a[3] + y * (a[4] + y * (a[5] + y * a[6])))));
     }
-    double t = 3.0 / ax;
-    double f = b[0] + t * (b[1] + t * (b[2] + t * (b[3] + t * (b[4] + t * (b[5] + t * b[6])))));
-    double th = ax + c[0] + t * (c[1] + t * (c[2] + t * (c[3] + t * (c[4] + t * (c[5] + t * c[6])))));
-    return f * cos(th) / sqrt(ax);
+    or_real t = 3.0 / ax;
+    or_real f = b[0] + t * (b[1] + t * (b[2] + t * (b[3] + t * (b[4] + t * (b[5] + t * b[6])))));
+    or_real th = ax + c[0] + t * (c[1] + t * (c[2] + t * (c[3] + t * (c[4] + t * (c[5] + t * c[6])))));
+    return f * RCOS(th) / RSQRT(ax);
 }
-static double as_j1(double x)
+static or_real as_j1(or_real x)
 {
-    static const double a[7] = {0.50000000, -0.56249985, 0.21093573, -0.03954289,
+    static const or_real a[7] = {0.50000000, -0.56249985, 0.21093573, -0.03954289,
                                 0.00443319, -0.00031761, 0.00001109};
-    static const double b[7] = {0.79788456, 0.00000156, 0.01659667, 0.00017105,
+    static const or_real b[7] = {0.79788456, 0.00000156, 0.01659667, 0.00017105,
                                 -0.00249511, 0.00113653, -0.00020033};
-    static const double c[7] = {-2.35619449, 0.12499612, 0.00005650, -0.00637879,
+    static const or_real c[7] = {-2.35619449, 0.12499612, 0.00005650, -0.00637879,
                                 0.00074348, 0.00079824, -0.00029166};
-    double ax = fabs(x);
+    or_real ax = RFABS(x);
     if (ax <= 3.0) {
-        double y = (ax / 3.0) * (ax / 3.0);
+        or_real y = (ax / 3.0) * (ax / 3.0);
         return x * (a[0] + y * (a[1] + y * (a[2] + y * (a[3] + y * (a[4] + y * (a[5] + y * a[6]))))));
     }
-    double t = 3.0 / ax;
-    double f = b[0] + t * (b[1] + t * (b[2] + t * (b[3] + t * (b[4] + t * (b[5] + t * b[6])))));
-    double th = ax + c[0] + t * (c[1] + t * (c[2] + t * (c[3] + t * (c[4] + t * (c[5] + t * c[6])))));
-    return (x >= 0 ? 1.0 : -1.0) * f * cos(th) / sqrt(ax);
+    or_real t = 3.0 / ax;
+    or_real f = b[0] + t * (b[1] + t * (b[2] + t * (b[3] + t * (b[4] + t * (b[5] + t * b[6])))));
+    or_real th = ax + c[0] + t * (c[1] + t * (c[2] + t * (c[3] + t * (c[4] + t * (c[5] + t * c[6])))));
+    return (x >= 0 ? 1.0 : -1.0) * f * RCOS(th) / RSQRT(ax);
 }
 /* J_n for integer n: upward recurrence above n, Miller's downward recurrence
  * with renormalisation below (green.cu:1467-1529; iacc 40, bigno 1e10). */
-static double ref_jn(double x, int n)
+static or_real ref_jn(or_real x, int n)
 {
     const int iacc = 40;
-    const double bigno = 1e10, bigni = 1e-10;
+    const or_real bigno = 1e10, bigni = 1e-10;
     int nn = n;
-    double res;
+    or_real res;
     if (n == 0) res = as_j0(x);
     else if (n == 1) res = as_j1(x);
     else {
         if (n < 0) n = -n;
-        double ax = fabs(x);
+        or_real ax = RFABS(x);
         if (ax == 0) res = 0.0;
         else if (ax > n) {
-            double tox = 2.0 / ax, bjm = as_j0(ax), bj = as_j1(ax), bjp;
+            or_real tox = 2.0 / ax, bjm = as_j0(ax), bj = as_j1(ax), bjp;
             for (int j = 0; j < n - 1; j++) {
-                bjp = (double)(j + 1) * tox * bj - bjm;
+                bjp = (or_real)(j + 1) * tox * bj - bjm;
                 bjm = bj;
                 bj = bjp;
             }
             res = bj;
         } else {
-            double tox = 2.0 / ax, bjm, bjp = 0.0, bj = 1.0, sum = 0.0;
-            int m = 2 * ((n + (int)(sqrt((double)(iacc * n)))) / 2), jsum = 0;
+            or_real tox = 2.0 / ax, bjm, bjp = 0.0, bj = 1.0, sum = 0.0;
+            int m = 2 * ((n + (int)(RSQRT((or_real)(iacc * n)))) / 2), jsum = 0;
             res = 0.0;
             for (int j = m; j >= 1; j--) {
-                bjm = (double)j * tox * bj - bjp;
+                bjm = (or_real)j * tox * bj - bjp;
                 bjp = bj;
                 bj = bjm;
-                if (fabs(bj) > bigno) {
+                if (RFABS(bj) > bigno) {
                     bj *= bigni;
                     bjp *= bigni;
                     res *= bigni;
@@ -747,41 +810,42 @@ static double ref_jn(double x, int n)
     if (nn < 0 && (n % 2) == 1) res = -res;
     return res;
 }
-static double ref_djn(double x, int n) /* green.cu:1530-1541 */
+static or_real ref_djn(or_real x, int n) /* green.cu:1530-1541 */
 {
-    const double bigni = 1e-10;
-    if (fabs(x) < bigni) return (ref_jn(bigni, n) - ref_jn(0, n)) / bigni;
-    return (double)n * ref_jn(x, n) / x - ref_jn(x, n + 1);
+    const or_real bigni = 1e-10;
+    if (RFABS(x) < bigni) return (ref_jn(bigni, n) - ref_jn(0, n)) / bigni;
+    return (or_real)n * ref_jn(x, n) / x - ref_jn(x, n + 1);
 }
-static double ref_ddjn(double x, int n) /* green.cu:1543-1554 (Q7: not J'') */
+static or_real ref_ddjn(or_real x, int n) /* green.cu:1543-1554 (Q7: not J'') */
 {
-    const double bigni = 1e-10;
-    if (fabs(x) < bigni) return (ref_djn(bigni, n) - ref_djn(0, n)) / bigni;
-    return (double)n * ref_jn(x, n - 1) / x - ref_jn(x, n) * (n + n * n + x * x) / x / x;
+    const or_real bigni = 1e-10;
+    if (RFABS(x) < bigni) return (ref_djn(bigni, n) - ref_djn(0, n)) / bigni;
+    return (or_real)n * ref_jn(x, n - 1) / x - ref_jn(x, n) * (n + n * n + x * x) / x / x;
 }
 /* physics mode: libm jn and exact derivative identities */
-static double phy_jn(double x, int n) { return jn(n, x); }
-static double phy_djn(double x, int n) { return 0.5 * (jn(n - 1, x) - jn(n + 1, x)); }
-static double phy_ddjn(double x, int n)
+static or_real phy_jn(or_real x, int n) { return RJN(n, x); }
+static or_real phy_djn(or_real x, int n) { return 0.5 * (RJN(n - 1, x) - RJN(n + 1, x)); }
+static or_real phy_ddjn(or_real x, int n)
 {
-    return 0.25 * (jn(n - 2, x) - 2.0 * jn(n, x) + jn(n + 2, x));
+    return 0.25 * (RJN(n - 2, x) - 2.0 * RJN(n, x) + RJN(n + 2, x));
 }
 
-double or_bessel(int mode, int which, int n, double x)
+static or_real bes(int mode, int which, int n, or_real x)
 {
     if (mode == 0) return which == 0 ? ref_jn(x, n) : which == 1 ? ref_djn(x, n) : ref_ddjn(x, n);
     return which == 0 ? phy_jn(x, n) : which == 1 ? phy_djn(x, n) : phy_ddjn(x, n);
 }
+double or_bessel(int mode, int which, int n, double x) { return (double)bes(mode, which, n, x); }
 
 /* Integrand columns (green.cu:1558-1621 single, 1626-1716 tensor). */
-static void integrand(int compat, int src_type, const OPpar *p, double k, double dist, cd *c)
+static void integrand(int compat, int src_type, const OPpar *p, or_real k, or_real dist, cd *c)
 {
-    double x = dist * k;
+    or_real x = dist * k;
     int m = compat ? 0 : 1;
-    double j0 = or_bessel(m, 0, 0, x), j1 = or_bessel(m, 0, 1, x), j2 = or_bessel(m, 0, 2, x);
-    double j0d = or_bessel(m, 1, 0, x), j1d = or_bessel(m, 1, 1, x), j2d = or_bessel(m, 1, 2, x);
-    double j0dd = or_bessel(m, 2, 0, x), j1dd = or_bessel(m, 2, 1, x), j2dd = or_bessel(m, 2, 2, x);
-    double r = dist;
+    or_real j0 = bes(m, 0, 0, x), j1 = bes(m, 0, 1, x), j2 = bes(m, 0, 2, x);
+    or_real j0d = bes(m, 1, 0, x), j1d = bes(m, 1, 1, x), j2d = bes(m, 1, 2, x);
+    or_real j0dd = bes(m, 2, 0, x), j1dd = bes(m, 2, 1, x), j2dd = bes(m, 2, 2, x);
+    or_real r = dist;
     for (int i = 0; i < 30; i++) c[i] = 0.0;
     if (src_type == 0) {
         c[0] = p->b1 * j1 / r + p->a1[0] * k * j1d;
@@ -986,7 +1050,7 @@ done:
 
 /* cal_kmax (green.cu:108-162).  b = dispersed S velocities at the grid's
  * current frequency; w is the frequency the estimate is evaluated at. */
-static double cal_kmax(cd w, const cd *b, const or_run *run, int nf)
+static double cal_kmax(double complex w, const double complex *b, const or_run *run, int nf)
 {
     int s = run->s_layer, rl = run->r_layer;
     const double *h = run->h;
@@ -1073,14 +1137,13 @@ static void layer_disp(const or_run *run, int compat, cd w, OPar *l0)
 
 static int nk_for(const or_run *run, int compat, const grid_consts *g, int nw)
 {
-    OPar l0[OR_MAX_LAYERS];
-    cd b[OR_MAX_LAYERS];
-    cd w = CMPLX(2 * OR_PI * g->df * nw, -g->omega_i);
-    layer_disp(run, compat, w, l0);
-    for (int i = 0; i < run->n_layer; i++) b[i] = l0[i].b;
+    double complex b[OR_MAX_LAYERS];
+    double complex w = CMPLX(2 * OR_PI * g->df * nw, -g->omega_i);
+    (void)compat;
+    for (int i = 0; i < run->n_layer; i++) b[i] = disp_vel_d(w, run->b0[i], run->qs[i]);
     double km;
     if (nw < 3) {
-        cd wt = OR_PI2 * g->df * 3.0 - g->omega_i * I;
+        double complex wt = OR_PI2 * g->df * 3.0 - g->omega_i * I;
         km = cal_kmax(wt, b, run, nw);
     } else {
         km = cal_kmax(w, b, run, nw);
@@ -1111,35 +1174,35 @@ static int green_row_raw(const or_run *run, int compat, const grid_consts *g, in
                          cd par0_v, cd *sums)
 {
     OPar l0[OR_MAX_LAYERS];
-    cd w = CMPLX(2 * OR_PI * g->df * nw, -g->omega_i);
+    cd w = MKC(2 * OR_PI * g->df * nw, -g->omega_i);
     layer_disp(run, compat, w, l0);
     int nk = nk_for(run, compat, g, nw);
     for (int c = 0; c < 30; c++) sums[c] = 0;
     for (int ik = 0; ik < nk; ik++) {
-        double k = g->dk * ik;
+        or_real k = g->dk * ik;
         OPpar pp;
         int st = cell_ppar(compat, run->src_type ? 1 : 0, k, w, run->n_layer, l0, run->s_layer,
                            run->zs, run->r_layer, run->zr, par0_r, par0_v, &pp);
         if (st) return st;
         cd c[30];
         integrand(compat, run->src_type ? 1 : 0, &pp, k, run->dist, c);
-        double wt = (ik == 0) ? g->dk / 2.0 : g->dk;
+        or_real wt = (ik == 0) ? g->dk / 2.0 : g->dk;
         for (int q = 0; q < 30; q++) sums[q] += c[q] * wt;
     }
     return OR_OK;
 }
 
 /* Phase, cosine taper and scale of one row (green.cu:1905-1921). */
-static void post_row(int nw_tot, int iw, double wr, double t0, double taper, double df, cd *v,
+static void post_row(int nw_tot, int iw, or_real wr, or_real t0, or_real taper, or_real df, cd *v,
                      int ncol)
 {
-    int wc = (int)((double)nw_tot * (1 - taper));
-    double phi = wr * t0;
-    double st0 = 1;
-    if (iw > wc) st0 = 0.5 * (1. + cos((iw - wc + 1) * OR_PI / (nw_tot - wc + 1)));
-    double coef = df * nw_tot / OR_PI;
+    int wc = (int)((or_real)nw_tot * (1 - taper));
+    or_real phi = wr * t0;
+    or_real st0 = 1;
+    if (iw > wc) st0 = 0.5 * (1. + RCOS((iw - wc + 1) * R_PI / (nw_tot - wc + 1)));
+    or_real coef = df * nw_tot / R_PI;
     for (int j = 0; j < ncol; j++) {
-        v[j] = v[j] * CMPLX(cos(phi), sin(phi));
+        v[j] = v[j] * MKC(RCOS(phi), RSIN(phi));
         v[j] = st0 * v[j] * coef;
     }
 }
@@ -1147,7 +1210,7 @@ static void post_row(int nw_tot, int iw, double wr, double t0, double taper, dou
 static void par0_rv(const or_run *run, int compat, const grid_consts *g, cd *r0, cd *v0)
 {
     OPar l0[OR_MAX_LAYERS];
-    cd w0 = CMPLX(0.0, -g->omega_i);
+    cd w0 = MKC(0.0, -g->omega_i);
     layer_disp(run, compat, w0, l0);
     OPar p = l0[run->r_layer];
     set_par(&p, 0.0, w0);
@@ -1183,8 +1246,8 @@ int or_green_rows(const char *iname, int compat, const int *rows, int nrows, dou
         }
         post_row(g.nw, nw, 2 * OR_PI * g.df * nw, run.t0, run.taper, g.df, s, 30);
         for (int c = 0; c < 30; c++) {
-            out[i * 60 + 2 * c] = creal(s[c]);
-            out[i * 60 + 2 * c + 1] = cimag(s[c]);
+            out[i * 60 + 2 * c] = CRE(s[c]);
+            out[i * 60 + 2 * c + 1] = CIM(s[c]);
         }
     }
     return err ? OR_E_INPUT : OR_OK;
@@ -1204,10 +1267,10 @@ static void ifft_inplace(cd *x, int n)
         }
     }
     for (int len = 2; len <= n; len <<= 1) {
-        double ang = 2 * OR_PI / len;
+        or_real ang = 2 * R_PI / len;
         for (int i = 0; i < n; i += len)
             for (int j = 0; j < len / 2; j++) {
-                cd wv = CMPLX(cos(ang * j), sin(ang * j));
+                cd wv = MKC(RCOS(ang * j), RSIN(ang * j));
                 cd u = x[i + j], v = x[i + j + len / 2] * wv;
                 x[i + j] = u + v;
                 x[i + j + len / 2] = u - v;
@@ -1217,40 +1280,59 @@ static void ifft_inplace(cd *x, int n)
 
 /* Hermitian completion + inverse FFT + time scaling (green.cu:1925-1938,
  * 2222-2237) on an already post-processed half spectrum. */
-static void hermitian_ifft(int nw, const cd *half, double dt, double omega_i, double t0, cd *full,
-                           double *trace)
+static void hermitian_ifft(int nw, const cd *half, or_real dt, or_real omega_i, or_real t0, cd *full,
+                           or_real *trace)
 {
     int n = 2 * nw;
     for (int i = 0; i < nw; i++) full[i] = half[i];
     for (int iw = 0; iw < nw; iw++) {
         cd invs = (nw - iw >= nw) ? 0.0 : half[nw - iw];
-        full[nw + iw] = conj(invs);
+        full[nw + iw] = CONJ(invs);
     }
     cd *x = (cd *)malloc(sizeof(cd) * n);
     memcpy(x, full, sizeof(cd) * n);
     ifft_inplace(x, n);
     for (int j = 0; j < n; j++) {
-        double v = creal(x[j]) / (double)nw / 2.0;
-        trace[j] = v * exp((double)j * dt * omega_i) * exp(t0 * omega_i);
+        or_real v = CRE(x[j]) / (or_real)nw / 2.0;
+        trace[j] = v * REXP((or_real)j * dt * omega_i) * REXP(t0 * omega_i);
     }
     free(x);
+}
+
+/* post_row + hermitian_ifft of one raw half spectrum in the arithmetic type;
+ * spec_post (optional) gets the post-processed half spectrum. */
+static void post_ifft(int nw, const cd *raw, double df, double omega_i, double dt, double t0,
+                      double taper, cd *spec_post, or_real *trace)
+{
+    cd *half = (cd *)malloc(sizeof(cd) * nw), *full = (cd *)malloc(sizeof(cd) * 2 * nw);
+    for (int i = 0; i < nw; i++) {
+        half[i] = raw[i];
+        post_row(nw, i, 2 * OR_PI * df * i, t0, taper, df, &half[i], 1);
+        if (spec_post) spec_post[i] = half[i];
+    }
+    hermitian_ifft(nw, half, dt, omega_i, t0, full, trace);
+    free(half);
+    free(full);
 }
 
 void or_post_ifft(int nw, const double *spec_raw, double df, double omega_i, double dt, double t0,
                   double taper, double *spec_post, double *trace)
 {
-    cd *half = (cd *)malloc(sizeof(cd) * nw), *full = (cd *)malloc(sizeof(cd) * 2 * nw);
+    if (nw < 1) return;
+    cd *raw = (cd *)malloc(sizeof(cd) * nw), *sp = (cd *)malloc(sizeof(cd) * nw);
+    or_real *tr = (or_real *)malloc(sizeof(or_real) * 2 * nw);
+    for (int i = 0; i < nw; i++) raw[i] = MKC(spec_raw[2 * i], spec_raw[2 * i + 1]);
+    post_ifft(nw, raw, df, omega_i, dt, t0, taper, sp, tr);
     for (int i = 0; i < nw; i++) {
-        half[i] = CMPLX(spec_raw[2 * i], spec_raw[2 * i + 1]);
-        post_row(nw, i, 2 * OR_PI * df * i, t0, taper, df, &half[i], 1);
         if (spec_post) {
-            spec_post[2 * i] = creal(half[i]);
-            spec_post[2 * i + 1] = cimag(half[i]);
+            spec_post[2 * i] = (double)CRE(sp[i]);
+            spec_post[2 * i + 1] = (double)CIM(sp[i]);
         }
     }
-    hermitian_ifft(nw, half, dt, omega_i, t0, full, trace);
-    free(half);
-    free(full);
+    for (int j = 0; j < 2 * nw; j++) trace[j] = (double)tr[j];
+    free(raw);
+    free(sp);
+    free(tr);
 }
 
 int or_green_files(const char *iname, const char *oname_w, const char *oname_t, int compat,
@@ -1284,7 +1366,7 @@ int or_green_files(const char *iname, const char *oname_w, const char *oname_t, 
         return OR_E_INPUT;
     }
     cd *full = (cd *)malloc(sizeof(cd) * (size_t)n * 30);
-    double *tr = (double *)malloc(sizeof(double) * (size_t)n * 30);
+    or_real *tr = (or_real *)malloc(sizeof(or_real) * (size_t)n * 30);
     for (int c = 0; c < 30; c++)
         hermitian_ifft(nw, half + (size_t)c * nw, run.dt, g.omega_i, run.t0, full + (size_t)c * n,
                        tr + (size_t)c * n);
@@ -1299,12 +1381,13 @@ int or_green_files(const char *iname, const char *oname_w, const char *oname_t, 
         return OR_E_IO;
     }
     for (int j = 0; j < n; j++) {
-        for (int c = 0; c < 30; c++) fprintf(ft, "%.18e,", tr[(size_t)c * n + j]);
+        for (int c = 0; c < 30; c++) fprintf(ft, "%.18e,", (double)tr[(size_t)c * n + j]);
         fprintf(ft, "\n");
     }
     for (int j = 0; j < n; j++) {
         for (int c = 0; c < 30; c++)
-            fprintf(fw, "%.18e,%.18e,", creal(full[(size_t)c * n + j]), cimag(full[(size_t)c * n + j]));
+            fprintf(fw, "%.18e,%.18e,", (double)CRE(full[(size_t)c * n + j]),
+                    (double)CIM(full[(size_t)c * n + j]));
         fprintf(fw, "\n");
     }
     fclose(ft);
@@ -1341,7 +1424,7 @@ static int trace_run(int compat, const double *lay, int n_in, double src_depth, 
 /* Z and R k-kernels for a source kind from the cell coefficients:
  * explosion = s0 - s2 tensor terms (columns 8-9 / 3-2), vertical force =
  * single-force n=0 terms (columns 4 / 1).  R uses J0' = -J1. */
-static void zr_kernels(int compat, int kind, const OPpar *p, double k, cd *kz, cd *kr)
+static void zr_kernels(int compat, int kind, const OPpar *p, or_real k, cd *kz, cd *kr)
 {
     if (kind == 0) {
         *kz = (p->a0[1] - p->a2[1]) * k;
@@ -1352,9 +1435,10 @@ static void zr_kernels(int compat, int kind, const OPpar *p, double k, cd *kz, c
     }
 }
 
-int or_kernels(int compat, const double *lay, int n_in, double src_depth, double rcv_depth,
-               int src_kind, int nw, double dt, int nk, double kmax, double omega_i, int w_lo,
-               int w_hi, double *kern, int nthreads)
+/* kern[((iw - w_lo) * nk + ik) * 2 + {0: Kz, 1: Kr}] in the arithmetic type */
+static int kernels_cd(int compat, const double *lay, int n_in, double src_depth, double rcv_depth,
+                      int src_kind, int nw, double dt, int nk, double kmax, double omega_i, int w_lo,
+                      int w_hi, cd *kern, int nthreads)
 {
     or_run run;
     int st = trace_run(compat, lay, n_in, src_depth, rcv_depth, &run);
@@ -1370,7 +1454,7 @@ int or_kernels(int compat, const double *lay, int n_in, double src_depth, double
 #endif
     for (int iw = w_lo; iw < w_hi; iw++) {
         OPar l0[OR_MAX_LAYERS];
-        cd w = CMPLX(2 * OR_PI * df * iw, -omega_i);
+        cd w = MKC(2 * OR_PI * df * iw, -omega_i);
         layer_disp(&run, compat, w, l0);
         for (int ik = 0; ik < nk; ik++) {
             double k = dk * ik;
@@ -1380,16 +1464,28 @@ int or_kernels(int compat, const double *lay, int n_in, double src_depth, double
                 err |= 1;
                 break;
             }
-            cd kz, kr;
-            zr_kernels(compat, src_kind, &pp, k, &kz, &kr);
-            double *o = kern + ((size_t)(iw - w_lo) * nk + ik) * 4;
-            o[0] = creal(kz);
-            o[1] = cimag(kz);
-            o[2] = creal(kr);
-            o[3] = cimag(kr);
+            cd *o = kern + ((size_t)(iw - w_lo) * nk + ik) * 2;
+            zr_kernels(compat, src_kind, &pp, k, &o[0], &o[1]);
         }
     }
     return err ? OR_E_UNSUPPORTED : OR_OK;
+}
+
+int or_kernels(int compat, const double *lay, int n_in, double src_depth, double rcv_depth,
+               int src_kind, int nw, double dt, int nk, double kmax, double omega_i, int w_lo,
+               int w_hi, double *kern, int nthreads)
+{
+    size_t n = (size_t)(w_hi > w_lo ? w_hi - w_lo : 0) * nk * 2;
+    cd *kc = (cd *)malloc(sizeof(cd) * n + 1);
+    int st = kernels_cd(compat, lay, n_in, src_depth, rcv_depth, src_kind, nw, dt, nk, kmax, omega_i,
+                        w_lo, w_hi, kc, nthreads);
+    if (!st)
+        for (size_t i = 0; i < n; i++) {
+            kern[2 * i] = (double)CRE(kc[i]);
+            kern[2 * i + 1] = (double)CIM(kc[i]);
+        }
+    free(kc);
+    return st;
 }
 
 int or_traces(int compat, const double *lay, int n_in, double src_depth, double rcv_depth,
@@ -1399,9 +1495,9 @@ int or_traces(int compat, const double *lay, int n_in, double src_depth, double 
 {
     if (w_lo < 0 || w_hi > nw || w_lo > w_hi || nr < 1) return OR_E_INPUT;
     int nwr = w_hi - w_lo;
-    double *kern = (double *)malloc(sizeof(double) * 4 * (size_t)nwr * nk + 1);
-    int st = or_kernels(compat, lay, n_in, src_depth, rcv_depth, src_kind, nw, dt, nk, kmax,
-                        omega_i, w_lo, w_hi, kern, nthreads);
+    cd *kern = (cd *)malloc(sizeof(cd) * 2 * (size_t)nwr * nk + 1);
+    int st = kernels_cd(compat, lay, n_in, src_depth, rcv_depth, src_kind, nw, dt, nk, kmax, omega_i,
+                        w_lo, w_hi, kern, nthreads);
     if (st) {
         free(kern);
         return st;
@@ -1409,14 +1505,14 @@ int or_traces(int compat, const double *lay, int n_in, double src_depth, double 
     double df = 1.0 / (2 * nw * dt), dk = kmax / nk;
     int m = compat ? 0 : 1;
     /* Bessel tables J0(k r), J1(k r) */
-    double *bt = (double *)malloc(sizeof(double) * 2 * (size_t)nk * nr);
+    or_real *bt = (or_real *)malloc(sizeof(or_real) * 2 * (size_t)nk * nr);
     for (int ir = 0; ir < nr; ir++)
         for (int ik = 0; ik < nk; ik++) {
-            double x = dk * ik * dists[ir];
-            bt[((size_t)ir * nk + ik) * 2] = or_bessel(m, 0, 0, x);
-            bt[((size_t)ir * nk + ik) * 2 + 1] = or_bessel(m, 0, 1, x);
+            or_real x = dk * ik * dists[ir];
+            bt[((size_t)ir * nk + ik) * 2] = bes(m, 0, 0, x);
+            bt[((size_t)ir * nk + ik) * 2 + 1] = bes(m, 0, 1, x);
         }
-    double *raw = (double *)calloc((size_t)nr * 2 * nw * 2, sizeof(double));
+    cd *raw = (cd *)calloc((size_t)nr * 2 * nw, sizeof(cd));
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1)
@@ -1424,28 +1520,31 @@ int or_traces(int compat, const double *lay, int n_in, double src_depth, double 
     for (int ir = 0; ir < nr; ir++) {
         for (int iw = w_lo; iw < w_hi; iw++) {
             cd sz = 0, sr = 0;
-            const double *kw = kern + (size_t)(iw - w_lo) * nk * 4;
-            const double *b = bt + (size_t)ir * nk * 2;
+            const cd *kw = kern + (size_t)(iw - w_lo) * nk * 2;
+            const or_real *b = bt + (size_t)ir * nk * 2;
             for (int ik = 0; ik < nk; ik++) {
-                double wgt = (ik == 0) ? dk / 2.0 : dk;
-                sz += CMPLX(kw[4 * ik], kw[4 * ik + 1]) * (b[2 * ik] * wgt);
-                sr += CMPLX(kw[4 * ik + 2], kw[4 * ik + 3]) * (b[2 * ik + 1] * wgt);
+                or_real wgt = (ik == 0) ? dk / 2.0 : dk;
+                sz += kw[2 * ik] * (b[2 * ik] * wgt);
+                sr += kw[2 * ik + 1] * (b[2 * ik + 1] * wgt);
             }
-            double *oz = raw + ((size_t)(ir * 2 + 0) * nw + iw) * 2;
-            double *orr = raw + ((size_t)(ir * 2 + 1) * nw + iw) * 2;
-            oz[0] = creal(sz);
-            oz[1] = cimag(sz);
-            orr[0] = creal(sr);
-            orr[1] = cimag(sr);
+            raw[(size_t)(ir * 2 + 0) * nw + iw] = sz;
+            raw[(size_t)(ir * 2 + 1) * nw + iw] = sr;
         }
     }
+    cd *sp = (cd *)malloc(sizeof(cd) * nw);
+    or_real *tr = (or_real *)malloc(sizeof(or_real) * 2 * nw);
     for (int t = 0; t < nr * 2; t++) {
-        double *sp = spec ? spec + (size_t)t * nw * 2 : NULL;
-        double *tr = traces ? traces + (size_t)t * 2 * nw : NULL;
-        double *tmp_tr = tr ? tr : (double *)malloc(sizeof(double) * 2 * nw);
-        or_post_ifft(nw, raw + (size_t)t * nw * 2, df, omega_i, dt, t0, taper, sp, tmp_tr);
-        if (!tr) free(tmp_tr);
+        post_ifft(nw, raw + (size_t)t * nw, df, omega_i, dt, t0, taper, sp, tr);
+        if (spec)
+            for (int i = 0; i < nw; i++) {
+                spec[((size_t)t * nw + i) * 2] = (double)CRE(sp[i]);
+                spec[((size_t)t * nw + i) * 2 + 1] = (double)CIM(sp[i]);
+            }
+        if (traces)
+            for (int j = 0; j < 2 * nw; j++) traces[(size_t)t * 2 * nw + j] = (double)tr[j];
     }
+    free(sp);
+    free(tr);
     free(raw);
     free(bt);
     free(kern);
@@ -1491,10 +1590,11 @@ int or_columns_all(int compat, int all30, const double *lay, int n_in, double sr
 #endif
     for (int iw = 0; iw < nw; iw++) {
         OPar l0[OR_MAX_LAYERS];
-        cd w = CMPLX(2 * OR_PI * df * iw, -omega_i);
+        cd w = MKC(2 * OR_PI * df * iw, -omega_i);
         layer_disp(&run, compat, w, l0);
         for (int ik = 0; ik < nk && !err; ik++) {
-            double k = dk * ik, wgt = ik == 0 ? dk / 2.0 : dk;
+            double k = dk * ik;
+            or_real wgt = ik == 0 ? dk / 2.0 : dk;
             OPpar pt, pf;
             if (cell_ppar(compat, 1, k, w, run.n_layer, l0, run.s_layer, run.zs, run.r_layer, run.zr, r0, v0,
                           &pt) ||
@@ -1517,15 +1617,12 @@ int or_columns_all(int compat, int all30, const double *lay, int n_in, double sr
         free(raw);
         return OR_E_UNSUPPORTED;
     }
-    double *rw = (double *)malloc(sizeof(double) * 2 * nw);
+    or_real *tr = (or_real *)malloc(sizeof(or_real) * 2 * nw);
     for (size_t t = 0; t < (size_t)ncol * nr; t++) {
-        for (int iw = 0; iw < nw; iw++) {
-            rw[2 * iw] = creal(raw[t * nw + iw]);
-            rw[2 * iw + 1] = cimag(raw[t * nw + iw]);
-        }
-        or_post_ifft(nw, rw, df, omega_i, dt, t0, taper, NULL, traces + t * 2 * nw);
+        post_ifft(nw, raw + t * nw, df, omega_i, dt, t0, taper, NULL, tr);
+        for (int j = 0; j < 2 * nw; j++) traces[t * 2 * nw + j] = (double)tr[j];
     }
-    free(rw);
+    free(tr);
     free(raw);
     return OR_OK;
 }

@@ -11,24 +11,28 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgrtm_oracle.so")
+# the same restatement computed in binary128 (__float128); same API, in double
+LIB_PATH_QUAD = os.path.join(_HERE, "libgrtm_oracle_q.so")
 P = ctypes.c_void_p
 D = ctypes.c_double
-_LIB = None
+_LIBS = {}
 
 
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
-    global _LIB
-    if _LIB is None:
-        if not os.path.exists(LIB_PATH):
+def lib(quad=False):
+    """The FP64 oracle, or with quad=True its binary128 build."""
+    if quad not in _LIBS:
+        path = LIB_PATH_QUAD if quad else LIB_PATH
+        if not os.path.exists(path):
             build()
-        _LIB = ctypes.CDLL(LIB_PATH)
-        _LIB.or_bessel.restype = D
-        _LIB.or_bessel.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, D]
-    return _LIB
+        h = ctypes.CDLL(path)
+        h.or_bessel.restype = D
+        h.or_bessel.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, D]
+        _LIBS[quad] = h
+    return _LIBS[quad]
 
 
 class Run(ctypes.Structure):
@@ -45,20 +49,20 @@ def parse_input(path, compat=True):
     return st, r
 
 
-def ref_grid(run, compat=True):
+def ref_grid(run, compat=True, quad=False):
     dk, oi, nkw = D(), D(), ctypes.c_longlong()
     nw = 1 << (run.log2n - 1)
     nk = np.zeros(nw, dtype=np.int32)
-    lib().or_ref_grid(ctypes.byref(run), int(compat), ctypes.byref(dk), ctypes.byref(oi), nk.ctypes.data_as(P),
+    lib(quad).or_ref_grid(ctypes.byref(run), int(compat), ctypes.byref(dk), ctypes.byref(oi), nk.ctypes.data_as(P),
                       ctypes.byref(nkw))
     return dk.value, oi.value, nk, nkw.value
 
 
-def green_rows(iname, rows, compat=True, nthreads=0):
+def green_rows(iname, rows, compat=True, nthreads=0, quad=False):
     """Post-processed spectrum rows: array [len(rows), 30] complex."""
     rows = np.ascontiguousarray(rows, dtype=np.int32)
     out = np.zeros((len(rows), 60))
-    st = lib().or_green_rows(iname.encode(), int(compat), rows.ctypes.data_as(P), len(rows),
+    st = lib(quad).or_green_rows(iname.encode(), int(compat), rows.ctypes.data_as(P), len(rows),
                              out.ctypes.data_as(P), int(nthreads))
     if st:
         raise RuntimeError(f"oracle green_rows status {st}")
@@ -69,8 +73,8 @@ def green_files(iname, oname_w, oname_t, compat=True, nthreads=0):
     return lib().or_green_files(iname.encode(), oname_w.encode(), oname_t.encode(), int(compat), int(nthreads))
 
 
-def set_inverse_mode(m):
-    lib().or_set_inverse_mode(int(m))
+def set_inverse_mode(m, quad=False):
+    lib(quad).or_set_inverse_mode(int(m))
 
 
 def set_tensor_sh_reference(on):
@@ -84,11 +88,11 @@ def dispersion(w, v0, q):
     return o[0] + 1j * o[1]
 
 
-def cell(compat, src_type, k, w, lay, s_layer, zs, r_layer, zr, par0_rv=None):
+def cell(compat, src_type, k, w, lay, s_layer, zs, r_layer, zr, par0_rv=None, quad=False):
     lay = np.ascontiguousarray(lay, dtype=np.float64)
     out = np.zeros(36)
     p0 = None if par0_rv is None else np.ascontiguousarray(par0_rv, dtype=np.float64)
-    st = lib().or_cell(int(compat), int(src_type), D(k), D(w.real), D(w.imag), lay.shape[0], lay.ctypes.data_as(P),
+    st = lib(quad).or_cell(int(compat), int(src_type), D(k), D(w.real), D(w.imag), lay.shape[0], lay.ctypes.data_as(P),
                        int(s_layer), D(zs), int(r_layer), D(zr), None if p0 is None else p0.ctypes.data_as(P),
                        out.ctypes.data_as(P))
     if st:
@@ -112,7 +116,8 @@ def bessel(mode, which, n, x):
     return lib().or_bessel(int(mode), int(which), int(n), D(x))
 
 
-def traces(layers, src_depth, rcv_depth, dists, src_kind, grid, compat=False, w_range=None, nthreads=0):
+def traces(layers, src_depth, rcv_depth, dists, src_kind, grid, compat=False, w_range=None, nthreads=0,
+           quad=False):
     """Z/R traces [nr, 2, N] and post-processed spectra [nr, 2, nw] of one source kind
     (0 explosion, 1 vertical force) on the dense grid."""
     layers = np.ascontiguousarray(layers, dtype=np.float64)
@@ -121,7 +126,7 @@ def traces(layers, src_depth, rcv_depth, dists, src_kind, grid, compat=False, w_
     lo, hi = (0, nw) if w_range is None else w_range
     tr = np.zeros((len(dists), 2, 2 * nw))
     sp = np.zeros((len(dists), 2, nw, 2))
-    st = lib().or_traces(int(compat), layers.ctypes.data_as(P), layers.shape[0], D(src_depth), D(rcv_depth),
+    st = lib(quad).or_traces(int(compat), layers.ctypes.data_as(P), layers.shape[0], D(src_depth), D(rcv_depth),
                          dists.ctypes.data_as(P), len(dists), int(src_kind), nw, D(grid["dt"]), int(grid["nk"]),
                          D(grid["kmax"]), D(grid["omega_i"]), D(grid.get("t0", 0.0)), D(grid.get("taper", 0.0)),
                          int(lo), int(hi), tr.ctypes.data_as(P), sp.ctypes.data_as(P), int(nthreads))
@@ -133,7 +138,7 @@ def traces(layers, src_depth, rcv_depth, dists, src_kind, grid, compat=False, w_
 N_COLUMNS = 15
 
 
-def columns(layers, src_depth, rcv_depth, dists, grid, compat=False, nthreads=0, all30=False):
+def columns(layers, src_depth, rcv_depth, dists, grid, compat=False, nthreads=0, all30=False, quad=False):
     """Displacement-column traces [15, nr, N]: tensor columns 0-9, then
     single-force columns 0-4, on the dense grid.  all30: every integrand
     column of both sources instead, [2 (tensor, single force), 30, nr, N]."""
@@ -141,7 +146,7 @@ def columns(layers, src_depth, rcv_depth, dists, grid, compat=False, nthreads=0,
     dists = np.ascontiguousarray(dists, dtype=np.float64)
     nw = int(grid["nw"])
     tr = np.zeros((60 if all30 else N_COLUMNS, len(dists), 2 * nw))
-    st = lib().or_columns_all(int(compat), int(all30), layers.ctypes.data_as(P), layers.shape[0], D(src_depth),
+    st = lib(quad).or_columns_all(int(compat), int(all30), layers.ctypes.data_as(P), layers.shape[0], D(src_depth),
                               D(rcv_depth), dists.ctypes.data_as(P), len(dists), nw, D(grid["dt"]), int(grid["nk"]),
                               D(grid["kmax"]), D(grid["omega_i"]), D(grid.get("t0", 0.0)), D(grid.get("taper", 0.0)),
                               tr.ctypes.data_as(P), int(nthreads))
@@ -150,12 +155,12 @@ def columns(layers, src_depth, rcv_depth, dists, grid, compat=False, nthreads=0,
     return tr.reshape(2, 30, len(dists), 2 * nw) if all30 else tr
 
 
-def kernels(layers, src_depth, rcv_depth, src_kind, grid, w_lo, w_hi, compat=False, nthreads=0):
+def kernels(layers, src_depth, rcv_depth, src_kind, grid, w_lo, w_hi, compat=False, nthreads=0, quad=False):
     """Unweighted Z/R k-kernels [w_hi-w_lo, nk, 2] complex."""
     layers = np.ascontiguousarray(layers, dtype=np.float64)
     nk = int(grid["nk"])
     out = np.zeros((w_hi - w_lo, nk, 4))
-    st = lib().or_kernels(int(compat), layers.ctypes.data_as(P), layers.shape[0], D(src_depth), D(rcv_depth),
+    st = lib(quad).or_kernels(int(compat), layers.ctypes.data_as(P), layers.shape[0], D(src_depth), D(rcv_depth),
                           int(src_kind), int(grid["nw"]), D(grid["dt"]), nk, D(grid["kmax"]), D(grid["omega_i"]),
                           int(w_lo), int(w_hi), out.ctypes.data_as(P), int(nthreads))
     if st:
