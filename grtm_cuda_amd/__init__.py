@@ -170,12 +170,14 @@ class TracePlan:
             self.ptr = ctypes.c_void_p()
 
 
-def selftest_cell(compat, src_type, k, w, lay, s_layer, zs, r_layer, zr, par0_rv=None):
-    """Host-compiled cell solve (test hook): returns 18 complex coefficients."""
+def selftest_cell(compat, src_type, k, w, lay, s_layer, zs, r_layer, zr, par0_rv=None, dd=False):
+    """Host-compiled cell solve (test hook): returns 18 complex coefficients.
+    dd: solve in double-double arithmetic."""
     lay = np.ascontiguousarray(lay, dtype=np.float64)
     out = np.zeros(36)
     p0 = None if par0_rv is None else np.ascontiguousarray(par0_rv, dtype=np.float64)
-    check(_lib().grtm_selftest_cell(int(compat), int(src_type), float(k), float(w.real), float(w.imag),
+    fn = _lib().grtm_selftest_cell_dd if dd else _lib().grtm_selftest_cell
+    check(fn(int(compat), int(src_type), float(k), float(w.real), float(w.imag),
                                     lay.shape[0], lay.ctypes.data_as(P), int(s_layer), float(zs), int(r_layer),
                                     float(zr), None if p0 is None else p0.ctypes.data_as(P),
                                     out.ctypes.data_as(P)), "grtm_selftest_cell")

@@ -12,14 +12,19 @@
 // closed-form 2x2 block factorisation (see interface_xy below).
 #pragma once
 #include "cx.h"
+#include "dd.h"
 
 namespace grtm {
 
 // Precision of the vertical wavenumbers and exponent arguments: FP64 for
-// both FP32 and FP64 cells.
+// both FP32 and FP64 cells, double-double for double-double cells.
 template <typename T>
 struct WideOf {
     using type = double;
+};
+template <>
+struct WideOf<dd> {
+    using type = dd;
 };
 
 // Constants of one layer at one complex frequency w.
@@ -35,6 +40,17 @@ struct LayerW {
     W h;           // thickness
     W hrev, hlog;  // -h / (2 pi), -h log2(e): exponent factors of Lambda
 };
+
+// The FP64 layer constants of one frequency as double-double values (exact).
+GRTM_HD LayerW<dd> widen_dd(const LayerW<double> &l)
+{
+    auto W = [](cx<double> z) { return cx<dd>{dd(z.x), dd(z.y)}; };
+    LayerW<dd> o;
+    o.sa2 = W(l.sa2), o.sb2 = W(l.sb2), o.sa = W(l.sa), o.sb = W(l.sb), o.mg = W(l.mg);
+    o.ca = W(l.ca), o.cb = W(l.cb), o.mu = W(l.mu), o.a = W(l.a), o.b = W(l.b);
+    o.h = dd(l.h), o.hrev = dd(l.hrev), o.hlog = dd(l.hlog);
+    return o;
+}
 
 // Geometry of one (model, source) pair.
 struct Geo {

@@ -334,6 +334,19 @@ int post_tables(grtm::PostParams &pp, int prec, DBuf &buf)
 }
 
 // ------------------------------------------------------------------ green()
+// Relative accuracy an FP64 solve of a frequency row loses where the P and SV
+// columns of the layer matrices approach each other (|w| -> w_I): about
+// (k V / |w|)^4 rounding units at the row's largest wavenumber kmax and the
+// fastest velocity.  On the 2048-sample, 2 s demo model the estimate is 0.7
+// at row 0 (the FP64 solve is 40% off per cell there), 1e-5 at row 5, and it
+// overstates the measured loss of rows 1-8 about 100-fold.
+double dd_row_loss(double kmax, double vmax, double wabs)
+{
+    const double q = kmax * vmax / wabs;
+    return q * q * q * q * 0x1p-53;
+}
+constexpr double kDDRowLoss = 1e-5;
+
 // The whole drop-in run; spec/traces in the layout of grtm_green_mem.
 int green_core(const char *iname, int compat, std::vector<double> *spec, std::vector<double> *traces, int *nw_out)
 {
@@ -404,8 +417,15 @@ int green_core(const char *iname, int compat, std::vector<double> *spec, std::ve
     } sg{s};
     HIPCHK(hipMemcpyAsync(d_lw.p, lw.data(), lw.size() * sizeof(LayerW<double>), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(d_nk.p, nk_w.data(), nk_w.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    HIPCHK(grtm::launch_cells_full(compat, rp.src_type ? 1 : 0, d_lw.p, d_nk.as<int>(), st.L, g, nw, nblk, dk,
-                                   rp.dist, df, omega_i, p0, d_part.as<double>(), s));
+    // the lowest rows, where an FP64 solve loses more than kDDRowLoss, in
+    // double-double; the rest in FP64
+    int n_dd = 0;
+    while (n_dd < nw && dd_row_loss(nk_w[n_dd] * dk, vmax, std::abs(cd(2 * M_PI * df * n_dd, -omega_i))) > kDDRowLoss)
+        n_dd++;
+    HIPCHK(grtm::launch_cells_full(compat, rp.src_type ? 1 : 0, 1, d_lw.p, d_nk.as<int>(), st.L, g, 0, n_dd, nblk,
+                                   dk, rp.dist, df, omega_i, p0, d_part.as<double>(), s));
+    HIPCHK(grtm::launch_cells_full(compat, rp.src_type ? 1 : 0, 0, d_lw.p, d_nk.as<int>(), st.L, g, n_dd, nw - n_dd,
+                                   nblk, dk, rp.dist, df, omega_i, p0, d_part.as<double>(), s));
     HIPCHK(grtm::launch_green_finish(nw, nblk, d_part.as<double>(), d_raw.as<double>(), s));
     grtm::PostParams pp{};
     pp.nw = nw;
@@ -1616,8 +1636,14 @@ int grtm_selftest_copy(int n_int32)
     return GRTM_OK;
 }
 
-int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi, int n_layer, const double *lay,
-                       int s_layer, double zs, int r_layer, double zr, const double *par0_rv, double *out)
+}  // extern "C"
+
+namespace {
+// The device cell solve compiled for the host in arithmetic T (double or
+// double-double; the layer constants are formed in FP64 either way).
+template <typename T>
+int selftest_cell_t(int compat, int src_type, double k, double wr, double wi, int n_layer, const double *lay,
+                    int s_layer, double zs, int r_layer, double zr, const double *par0_rv, double *out)
 {
     if (n_layer < 2 || n_layer > grtm::kMaxLayers || !lay || !out) return GRTM_E_INPUT;
     const cd w(wr, wi);
@@ -1640,6 +1666,11 @@ int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi,
         lw[j].hrev = -l[0] / (2 * M_PI);
         lw[j].hlog = -l[0] * M_LOG2E;
     }
+    std::vector<LayerW<T>> lt(n_layer);
+    for (int j = 0; j < n_layer; j++) {
+        if constexpr (std::is_same<T, double>::value) lt[j] = lw[j];
+        else lt[j] = grtm::widen_dd(lw[j]);
+    }
     Stack st;
     st.L = n_layer;
     st.s = s_layer;
@@ -1649,18 +1680,36 @@ int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi,
     Geo g;
     int status = make_geo(st, compat, g);
     if (status) return status;
-    grtm::cx<double> p0r{0, 0}, p0v{0, 0};
-    if (par0_rv) p0r = {par0_rv[0], par0_rv[1]}, p0v = {par0_rv[2], par0_rv[3]};
-    grtm::CellOut<double> c;
-    grtm::Coef<double> p;
-    if (compat) grtm::cell_solve<double, true, true>(lw.data(), g, k, p0r, p0v, c);
-    else grtm::cell_solve<double, false, true>(lw.data(), g, k, p0r, p0v, c);
-    if (compat) grtm::coef_from_cell<double, true, true>(c, src_type, g.up, k, grtm::cx<double>{wr, wi}, p);
-    else grtm::coef_from_cell<double, true, false>(c, src_type, g.up, k, grtm::cx<double>{wr, wi}, p);
-    const grtm::cx<double> v[18] = {p.a0[0], p.a0[1], p.a1[0], p.a1[1], p.a2[0], p.a2[1], p.b1, p.b2, p.a0t[0],
-                                    p.a0t[1], p.a1t[0], p.a1t[1], p.a2t[0], p.a2t[1], p.b1t, p.b2t, {0, 0}, {0, 0}};
-    for (int i = 0; i < 18; i++) out[2 * i] = v[i].x, out[2 * i + 1] = v[i].y;
+    grtm::cx<T> p0r{T(0), T(0)}, p0v{T(0), T(0)};
+    if (par0_rv) p0r = {T(par0_rv[0]), T(par0_rv[1])}, p0v = {T(par0_rv[2]), T(par0_rv[3])};
+    grtm::CellOut<T> c;
+    grtm::Coef<T> p;
+    const T kt = T(k);
+    if (compat) grtm::cell_solve<T, true, true>(lt.data(), g, kt, p0r, p0v, c);
+    else grtm::cell_solve<T, false, true>(lt.data(), g, kt, p0r, p0v, c);
+    if (compat) grtm::coef_from_cell<T, true, true>(c, src_type, g.up, kt, grtm::cx<T>{T(wr), T(wi)}, p);
+    else grtm::coef_from_cell<T, true, false>(c, src_type, g.up, kt, grtm::cx<T>{T(wr), T(wi)}, p);
+    const grtm::cx<T> z{T(0), T(0)};
+    const grtm::cx<T> v[18] = {p.a0[0], p.a0[1], p.a1[0], p.a1[1], p.a2[0], p.a2[1], p.b1, p.b2, p.a0t[0],
+                               p.a0t[1], p.a1t[0], p.a1t[1], p.a2t[0], p.a2t[1], p.b1t, p.b2t, z, z};
+    for (int i = 0; i < 18; i++) out[2 * i] = double(v[i].x), out[2 * i + 1] = double(v[i].y);
     return GRTM_OK;
+}
+}  // namespace
+
+extern "C" {
+int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi, int n_layer, const double *lay,
+                       int s_layer, double zs, int r_layer, double zr, const double *par0_rv, double *out)
+{
+    return selftest_cell_t<double>(compat, src_type, k, wr, wi, n_layer, lay, s_layer, zs, r_layer, zr, par0_rv,
+                                   out);
+}
+
+int grtm_selftest_cell_dd(int compat, int src_type, double k, double wr, double wi, int n_layer, const double *lay,
+                          int s_layer, double zs, int r_layer, double zr, const double *par0_rv, double *out)
+{
+    return selftest_cell_t<grtm::dd>(compat, src_type, k, wr, wi, n_layer, lay, s_layer, zs, r_layer, zr, par0_rv,
+                                     out);
 }
 
 }  // extern "C"

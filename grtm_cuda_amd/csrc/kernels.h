@@ -90,9 +90,11 @@ hipError_t launch_cells_fix(int compat, const void *lw64, int L, const Geo &g, i
                             double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,
                             const unsigned *flag_list, hipStream_t st);
 hipError_t launch_zero_u32(unsigned *p, int n, hipStream_t st);
-hipError_t launch_cells_full(int compat, int src, const void *lw, const int *nk_w, int L, const Geo &g,
-                             int nw, int nblk, double dk, double dist, double df, double omega_i,
-                             const double p0[4], double *partial, hipStream_t st);
+// drop-in cell kernel (green.hip) over rows [w_base, w_base + nrows); dd_rows:
+// solve them in double-double arithmetic
+hipError_t launch_cells_full(int compat, int src, int dd_rows, const void *lw, const int *nk_w, int L,
+                             const Geo &g, int w_base, int nrows, int nblk, double dk, double dist, double df,
+                             double omega_i, const double p0[4], double *partial, hipStream_t st);
 hipError_t launch_green_finish(int nw, int nblk, const double *partial, double *raw, hipStream_t st);
 // B0 = J_n0(k r), B1 = J_n0+1(k r)
 hipError_t launch_bessel_tab(int prec, int mode, int nk, double dk, const double *dists, int nr, void *B0,

@@ -3,9 +3,8 @@
 //   k_cells_zr     per-(omega,k) recursion -> Z/R k-kernels of the explosion
 //                  and vertical-force sources, weighted by the k-quadrature
 //                  and written as the A operands of the k-sum contraction
-//   k_cells_full   per-(omega,k) recursion -> all 30 integrand columns of the
-//                  drop-in output at one distance, reduced over k per block
-//   k_green_finish ordered sum of the per-block partials
+//   k_green_finish ordered sum of the per-block partials of the drop-in
+//                  path's cell kernel (green.hip)
 //   k_fallback_select, k_fallback_cells  row selection and dense completion
 //                  of the learned adaptive grid (the residual check runs as
 //                  flagged items of the recursion launches)
@@ -327,58 +326,6 @@ __global__ void k_zero_u32(unsigned *p, int n)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = 0u;
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-template <typename T, bool COMPAT, int SRC>
-__global__ __launch_bounds__(kCellThreads) void k_cells_full(const LayerW<T> *__restrict__ lw_all,
-                                                             const int *__restrict__ nk_w, int L, Geo g,
-                                                             double dk, double dist, double df,
-                                                             double omega_i, double p0r_x, double p0r_y,
-                                                             double p0v_x, double p0v_y, int nblk,
-                                                             double *__restrict__ partial)
-{
-    __shared__ LayerW<T> lw[kMaxLayers];
-    __shared__ double red[kCellThreads / 64][60];
-    const int iw = blockIdx.y;
-    for (int j = threadIdx.x; j < L; j += blockDim.x) lw[j] = lw_all[(size_t)iw * L + j];
-    __syncthreads();
-    const int ik = blockIdx.x * blockDim.x + threadIdx.x;
-    cx<double> col[30];
-    for (int i = 0; i < 30; i++) col[i] = cx<double>{0, 0};
-    if (ik < nk_w[iw]) {
-        const T k = T(dk * ik);
-        const cx<T> w = C<T>(T(2.0 * M_PI * df * iw), T(-omega_i));
-        CellOut<T> c;
-        cell_solve<T, COMPAT, true>(lw, g, k, C<T>(T(p0r_x), T(p0r_y)), C<T>(T(p0v_x), T(p0v_y)), c);
-        Coef<T> p;
-        coef_from_cell<T, true, COMPAT>(c, SRC, g.up, k, w, p);
-        BesselSet bs;
-        bessel_set(COMPAT ? 0 : 1, dist * double(k), bs);
-        integrand_cols<T>(COMPAT ? 1 : 0, SRC, p, double(k), dist, bs, col);
-        const double wt = (ik == 0) ? dk / 2.0 : dk;
-        for (int i = 0; i < 30; i++) col[i] = col[i] * wt;
-    }
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int i = 0; i < 30; i++) {
-        double sx = wave_sum(col[i].x), sy = wave_sum(col[i].y);
-        if (lane == 0) {
-            red[wv][2 * i] = sx;
-            red[wv][2 * i + 1] = sy;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < 60) {
-        double s = 0;
-        for (int q = 0; q < kCellThreads / 64; q++) s += red[q][threadIdx.x];
-        partial[((size_t)iw * nblk + blockIdx.x) * 60 + threadIdx.x] = s;
-    }
 }
 
 __global__ void k_green_finish(int nw, int nblk, const double *__restrict__ partial, double *__restrict__ raw)
@@ -792,26 +739,6 @@ hipError_t launch_cells_fix(int compat, const void *lw64, int L, const Geo &g, i
 hipError_t launch_zero_u32(unsigned *p, int n, hipStream_t st)
 {
     hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
-    return hipGetLastError();
-}
-
-hipError_t launch_cells_full(int compat, int src, const void *lw, const int *nk_w, int L, const Geo &g,
-                             int nw, int nblk, double dk, double dist, double df, double omega_i,
-                             const double p0[4], double *partial, hipStream_t st)
-{
-    dim3 grid(nblk, nw);
-#define GRTM_FULL(CP, SR)                                                                              \
-    hipLaunchKernelGGL((k_cells_full<double, CP, SR>), grid, dim3(kCellThreads), 0, st,              \
-                       (const LayerW<double> *)lw, nk_w, L, g, dk, dist, df, omega_i, p0[0], p0[1],  \
-                       p0[2], p0[3], nblk, partial)
-    if (compat) {
-        if (src == 0) GRTM_FULL(true, 0);
-        else GRTM_FULL(true, 1);
-    } else {
-        if (src == 0) GRTM_FULL(false, 0);
-        else GRTM_FULL(false, 1);
-    }
-#undef GRTM_FULL
     return hipGetLastError();
 }
 

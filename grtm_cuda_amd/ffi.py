@@ -75,6 +75,7 @@ def _lib():
         lib.grtm_plan_destroy.argtypes = [P]
         lib.grtm_plan_destroy.restype = None
         lib.grtm_selftest_cell.argtypes = [I, I, D, D, D, I, P, I, D, I, D, P, P]
+        lib.grtm_selftest_cell_dd.argtypes = [I, I, D, D, D, I, P, I, D, I, D, P, P]
         lib.grtm_opcount_cell.argtypes = [I, I, I, I, I, P]
         lib.grtm_opcount_columns_cell.argtypes = [I, I, I, I, I, I, P]
         lib.grtm_selftest_gemm.argtypes = [I, I, I, I, P, P, P]
@@ -118,7 +119,7 @@ EXPORTS = [
     "green", "grtm_green", "grtm_green_mem", "grtm_last_status", "grtm_status_str", "grtm_version",
     "grtm_create", "grtm_destroy", "grtm_plan_traces", "grtm_plan_execute", "grtm_plan_sync",
     "grtm_plan_traces_out", "grtm_plan_kernels_out", "grtm_plan_timing", "grtm_plan_work",
-    "grtm_plan_destroy", "grtm_selftest_cell", "grtm_opcount_cell", "grtm_selftest_gemm", "grtm_selftest_gemm_cut",
+    "grtm_plan_destroy", "grtm_selftest_cell", "grtm_selftest_cell_dd", "grtm_opcount_cell", "grtm_selftest_gemm", "grtm_selftest_gemm_cut",
     "grtm_plan_columns", "grtm_opcount_columns_cell",
     "grtm_selftest_mfma_probe", "grtm_selftest_copy", "grtm_plan_set_adaptive", "grtm_plan_set_rows64",
     "grtm_plan_adaptive_stats",

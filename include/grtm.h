@@ -246,6 +246,12 @@ int grtm_selftest_copy(int n_int32);
 int grtm_selftest_cell(int compat, int src_type, double k, double wr, double wi, int n_layer,
                        const double *lay, int s_layer, double zs, int r_layer, double zr,
                        const double *par0_rv, double *out);
+/* The same cell solved in double-double arithmetic (the precision of the
+ * drop-in path's lowest frequency rows).  No reference counterpart (test
+ * hook). */
+int grtm_selftest_cell_dd(int compat, int src_type, double k, double wr, double wi, int n_layer,
+                          const double *lay, int s_layer, double zs, int r_layer, double zr,
+                          const double *par0_rv, double *out);
 
 /* Exact operation count of one recursion cell of the trace path with the
  * full upward sweep (the device template run on a counting scalar type on

@@ -142,3 +142,48 @@ def test_cell_solver_free_surface_receiver(orc):
             a = orc.cell(0, src, k, w, lay, s, zs, 0, 0.0, p0)
             b = g.selftest_cell(0, src, k, w, lay, s, zs, 0, 0.0, p0)
             assert np.abs(a - b).max() <= 1e-10 * np.abs(a).max()
+
+
+def _demo_cells(orc, row, compat):
+    """Layer rows (h, alpha, beta, mu) of the demo model at its frequency row
+    `row`, its geometry and wavenumbers (the reference's ragged grid)."""
+    st, run = orc.parse_input(os.path.join(GOLDEN, "demo_input.txt"), compat=compat)
+    assert st == 0
+    dk, oi, nk, _ = orc.ref_grid(run, compat=compat)
+    df = 1.0 / ((1 << run.log2n) * run.dt)
+    w = complex(2 * np.pi * df * row, -oi)
+    L = run.n_layer
+    lay = np.zeros((L, 7))
+    for j in range(L):
+        a = orc.dispersion(w, run.a0[j], run.qp[j])
+        b = orc.dispersion(w, run.b0[j], run.qs[j])
+        mu = run.b0[j] ** 2 * run.rho[j] if compat else run.rho[j] * b * b
+        lay[j] = [run.h[j], a.real, a.imag, b.real, b.imag, np.real(mu), np.imag(mu)]
+    w0 = complex(0, -oi)
+    a0 = orc.dispersion(w0, run.a0[run.r_layer], run.qp[run.r_layer])
+    b0 = orc.dispersion(w0, run.b0[run.r_layer], run.qs[run.r_layer])
+    r0, v0 = np.sqrt(-(w0 / a0) ** 2), np.sqrt(-(w0 / b0) ** 2)
+    r0, v0 = (r0 if r0.real >= 0 else -r0), (v0 if v0.real >= 0 else -v0)
+    p0 = np.array([r0.real, r0.imag, v0.real, v0.imag])
+    return run, w, lay, p0, dk * np.arange(nk[row])
+
+
+@pytest.mark.parametrize("compat", [1, 0])
+def test_cell_solver_double_double_at_the_lowest_rows(orc, compat):
+    """Demo rows 0-2 (|w| = 1, 3.3, 6.4 w_I): the device cell solve compiled in
+    FP64 is rounding noise at row 0 (tens of % per cell, like the oracle's FP64
+    4x4 solve); compiled in double-double (dd.h) it agrees with the binary128
+    oracle to 1e-13, from the same FP64 layer constants."""
+    worst64 = 0.0
+    for row in (0, 1, 2):
+        run, w, lay, p0, ks = _demo_cells(orc, row, compat)
+        for src in (0, 1):
+            for k in ks[::211]:
+                q = orc.cell(compat, src, k, w, lay, run.s_layer, run.zs, run.r_layer, run.zr, p0, quad=True)
+                d = g.selftest_cell(compat, src, k, w, lay, run.s_layer, run.zs, run.r_layer, run.zr, p0, dd=True)
+                f = g.selftest_cell(compat, src, k, w, lay, run.s_layer, run.zs, run.r_layer, run.zr, p0)
+                m = np.abs(q).max()
+                assert np.abs(d - q).max() <= 1e-13 * m, (row, src, k)
+                if row == 0:
+                    worst64 = max(worst64, np.abs(f - q).max() / m)
+    assert worst64 > 1e-2  # the FP64 solve at row 0: noise
