@@ -33,17 +33,23 @@ def green(iname, oname_w, oname_t, compat=True):
     check(_lib().grtm_green(iname.encode(), oname_w.encode(), oname_t.encode(), int(bool(compat))), "green")
 
 
-def green_mem(iname, compat=True):
+def green_mem(iname, compat=True, noalloc=False):
     """Returns (spec, traces): spec[30, nw] complex post-processed spectrum
-    (rows 0..nw-1 of oname_w), traces[30, N] (columns of oname_t)."""
+    (rows 0..nw-1 of oname_w), traces[30, N] (columns of oname_t).
+    noalloc: run through grtm_green_mem_noalloc (no device allocation; the
+    library's fixed per-GPU scratch)."""
     lib = _lib()
     nw = ctypes.c_int(0)
     check(lib.grtm_green_mem(iname.encode(), int(bool(compat)), None, None, ctypes.byref(nw)), "green_mem")
     n = nw.value
     spec = np.zeros((30, n, 2))
     tr = np.zeros((30, 2 * n))
-    check(lib.grtm_green_mem(iname.encode(), int(bool(compat)), spec.ctypes.data_as(P), tr.ctypes.data_as(P),
-                             ctypes.byref(nw)), "green_mem")
+    if noalloc:
+        check(lib.grtm_green_mem_noalloc(iname.encode(), int(bool(compat)), spec.ctypes.data_as(P),
+                                         tr.ctypes.data_as(P), ctypes.byref(nw), None), "green_mem_noalloc")
+    else:
+        check(lib.grtm_green_mem(iname.encode(), int(bool(compat)), spec.ctypes.data_as(P), tr.ctypes.data_as(P),
+                                 ctypes.byref(nw)), "green_mem")
     return spec[..., 0] + 1j * spec[..., 1], tr
 
 

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/grtm.h"
@@ -162,14 +163,14 @@ int parse_input(const char *path, int compat, Stack &st, RunParams &rp)
     if (!f) return GRTM_E_IO;
     double hs, hr;
     int n;
-    std::vector<double> top, vp, vs, rho, qp, qs;
+    double top[grtm::kMaxLayers], vp[grtm::kMaxLayers], vs[grtm::kMaxLayers], rho[grtm::kMaxLayers];
+    double qp[grtm::kMaxLayers], qs[grtm::kMaxLayers];
     int status = GRTM_OK;
     if (fscanf(f, "%lf", &hs) != 1 || fscanf(f, "%lf", &hr) != 1 || fscanf(f, "%d", &n) != 1) {
         status = GRTM_E_PARSE;
     } else if (n < 2 || n + 1 > grtm::kMaxLayers) {
         status = GRTM_E_INPUT;
     } else {
-        top.resize(n), vp.resize(n), vs.resize(n), rho.resize(n), qp.resize(n), qs.resize(n);
         for (int i = 0; i < n && status == GRTM_OK; i++)
             if (fscanf(f, " %lf , %lf , %lf , %lf , %lf , %lf", &top[i], &vp[i], &vs[i], &rho[i], &qp[i], &qs[i]) !=
                 6)
@@ -186,7 +187,7 @@ int parse_input(const char *path, int compat, Stack &st, RunParams &rp)
     if (rp.log2n < 3 || rp.log2n > 14 || !(rp.dt > 0) || !(rp.dist > 0) || !std::isfinite(rp.t0) ||
         !(rp.taper >= 0 && rp.taper <= 1))
         return GRTM_E_INPUT;
-    return prepare_stack(compat, n, top.data(), vp.data(), vs.data(), rho.data(), qp.data(), qs.data(), hs, hr, st);
+    return prepare_stack(compat, n, top, vp, vs, rho, qp, qs, hs, hr, st);
 }
 
 // Dispersed velocity v0 (1 + ln(w/2pi)/(pi Q) + i/(2Q)) (green.cu:177-180).
@@ -300,23 +301,34 @@ int device_ready()
     return GRTM_OK;
 }
 
+// Element i of the post-processing tables of pp, in FP64: pv[2 nw] (phase,
+// taper and scale of each frequency, green.cu:1905-1921), then tw[2 nw]
+// (twiddles e^{i pi m / nw}), then ts[N] (e^{w_I t} / N, green.cu:2236-2237).
+double post_table_value(const grtm::PostParams &pp, size_t i)
+{
+    const int M = pp.nw, N = 2 * M;
+    if (i < (size_t)2 * M) {
+        const int iw = int(i / 2);
+        const double phi = 2.0 * M_PI * pp.df * iw * pp.t0;
+        const double st0 = iw > pp.wc ? 0.5 * (1. + std::cos((iw - pp.wc + 1) * M_PI / (pp.nw - pp.wc + 1))) : 1.0;
+        const double coef = pp.df * pp.nw / M_PI * st0;
+        return (i & 1) ? std::sin(phi) * coef : std::cos(phi) * coef;
+    }
+    if (i < (size_t)4 * M) {
+        const int m = int((i - 2 * M) / 2);
+        return (i & 1) ? std::sin(M_PI * m / M) : std::cos(M_PI * m / M);
+    }
+    const int j = int(i - 4 * M);
+    return std::exp(pp.omega_i * (j * pp.dt + pp.t0)) / N;
+}
+
 // The post-processing tables of pp (the PostParams fields pv, tw, ts), computed
 // in FP64 and stored in buf in the trace precision (prec 0 FP64, else FP32).
 int post_tables(grtm::PostParams &pp, int prec, DBuf &buf)
 {
     const int M = pp.nw, N = 2 * M;
     std::vector<double> v((size_t)4 * M + N);
-    double *pv = v.data(), *tw = pv + 2 * M, *ts = tw + 2 * M;
-    for (int iw = 0; iw < M; iw++) {
-        const double phi = 2.0 * M_PI * pp.df * iw * pp.t0;
-        const double st0 = iw > pp.wc ? 0.5 * (1. + std::cos((iw - pp.wc + 1) * M_PI / (pp.nw - pp.wc + 1))) : 1.0;
-        const double coef = pp.df * pp.nw / M_PI * st0;
-        pv[2 * iw] = std::cos(phi) * coef;
-        pv[2 * iw + 1] = std::sin(phi) * coef;
-        tw[2 * iw] = std::cos(M_PI * iw / M);
-        tw[2 * iw + 1] = std::sin(M_PI * iw / M);
-    }
-    for (int j = 0; j < N; j++) ts[j] = std::exp(pp.omega_i * (j * pp.dt + pp.t0)) / N;
+    for (size_t i = 0; i < v.size(); i++) v[i] = post_table_value(pp, i);
     const size_t es = prec == 0 ? sizeof(double) : sizeof(float);
     int st;
     if ((st = buf.alloc(v.size() * es))) return st;
@@ -347,8 +359,84 @@ double dd_row_loss(double kmax, double vmax, double wabs)
 }
 constexpr double kDDRowLoss = 1e-5;
 
-// The whole drop-in run; spec/traces in the layout of grtm_green_mem.
-int green_core(const char *iname, int compat, std::vector<double> *spec, std::vector<double> *traces, int *nw_out)
+// ------------------------------------------------ no-allocation scratch
+// Device memory of grtm_green_mem_noalloc: one fixed array per GPU, held
+// until the process ends.  Host code never uses the array's name as an
+// address: each device's copy is looked up once with hipGetSymbolAddress
+// (with that device current) and that pointer is what kernels receive.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kScratchDevices];
+
+// The current device's scratch address (looked up on the first call).
+int scratch_ptr(void **out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchDevices) return GRTM_E_DEVICE;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return GRTM_E_DEVICE;
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return GRTM_OK;
+}
+
+// Device memory of one drop-in run, carved from one block in 256-byte
+// aligned pieces: layer tables [nw][L], per-row k counts, the cell kernel's
+// per-block partial sums [nw][nblk][60], the summed spectra, the traces and
+// post-processed spectra before their copy to the host, the post tables.
+struct GreenLayout {
+    size_t lw, nk, part, raw, tr, sp, pt, total;
+};
+GreenLayout green_layout(int nw, int L, int nblk)
+{
+    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t N = 2 * (size_t)nw;
+    GreenLayout o;
+    size_t off = 0;
+    o.lw = off, off += up((size_t)nw * L * sizeof(LayerW<double>));
+    o.nk = off, off += up((size_t)nw * sizeof(int));
+    o.part = off, off += up((size_t)nw * nblk * 60 * sizeof(double));
+    o.raw = off, off += up((size_t)30 * nw * 2 * sizeof(double));
+    o.tr = off, off += up((size_t)30 * N * sizeof(double));
+    o.sp = off, off += up((size_t)30 * nw * 2 * sizeof(double));
+    o.pt = off, off += up(((size_t)4 * nw + N) * sizeof(double));
+    o.total = off;
+    return o;
+}
+
+// Largest frequency count of a drop-in run (log2 N <= 14).
+constexpr int kMaxRows = 8192;
+
+// Host-to-device copy through a bounded buffer on the caller's stack, chunk by
+// chunk; fill(i0, i1, buf) writes elements [i0, i1).  No heap memory.
+template <typename E, int CHUNK, typename Fill>
+int stage_h2d(E *dst, size_t n, hipStream_t s, Fill fill)
+{
+    E buf[CHUNK];
+    for (size_t i0 = 0; i0 < n; i0 += CHUNK) {
+        const size_t i1 = std::min(n, i0 + CHUNK);
+        fill(i0, i1, buf);
+        HIPCHK(hipMemcpyAsync(dst + i0, buf, (i1 - i0) * sizeof(E), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));  // buf is reused
+    }
+    return GRTM_OK;
+}
+
+// The whole drop-in run (green.cu:1975-2239) into host arrays in the layout
+// of grtm_green_mem (spec: 30 x nw complex, traces: 30 x N; either NULL).
+// get_dev: NULL allocates the run's device memory; otherwise it supplies
+// `cap` bytes of device memory (the no-allocation entry point's scratch of the
+// current device), and a run needing more is rejected with GRTM_E_TOO_LARGE
+// before any device work.  stream: NULL uses a stream of the run's own
+// (allocating form) or the null stream.  Host side: fixed-size arrays only
+// (the layer tables and post tables are staged through bounded stack buffers).
+int green_core(const char *iname, int compat, double *spec, double *traces, int *nw_out, int (*get_dev)(void **),
+               size_t cap, hipStream_t user_stream)
 {
     Stack st;
     RunParams rp;
@@ -357,7 +445,6 @@ int green_core(const char *iname, int compat, std::vector<double> *spec, std::ve
     Geo g;
     status = make_geo(st, compat, g);
     if (status) return status;
-    if ((status = device_ready())) return status;
 
     // grid constants (green.cu:2063-2080)
     const int nw = 1 << (rp.log2n - 1), N = 2 * nw;
@@ -373,23 +460,25 @@ int green_core(const char *iname, int compat, std::vector<double> *spec, std::ve
     if (nw_out) *nw_out = nw;
     if (!spec && !traces) return GRTM_OK;
 
-    // per-frequency layer tables and ragged k counts (green.cu:2145-2194)
-    std::vector<LayerW<double>> lw((size_t)nw * st.L);
-    std::vector<int> nk_w(nw);
+    // ragged k counts per frequency (green.cu:2145-2194)
+    int nk_w[kMaxRows];
     int nk_max = 0;
-    std::vector<cd> bvel(st.L);
+    cd bvel[grtm::kMaxLayers];
     for (int iw = 0; iw < nw; iw++) {
         const cd w(2 * M_PI * df * iw, -omega_i);
-        layer_w(st, compat, w, &lw[(size_t)iw * st.L]);
         for (int j = 0; j < st.L; j++) bvel[j] = disp_vel(w, st.b0[j], st.qs[j]);
-        double km = (iw < 3) ? cal_kmax(cd(2 * M_PI * df * 3.0, -omega_i), bvel.data(), st, iw)
-                             : cal_kmax(w, bvel.data(), st, iw);
+        double km = (iw < 3) ? cal_kmax(cd(2 * M_PI * df * 3.0, -omega_i), bvel, st, iw) : cal_kmax(w, bvel, st, iw);
         km = 1.1 * km;
         const double nkd = km / dk + 2.0;
         if (!(nkd < 2.0e9)) return GRTM_E_INPUT;
         nk_w[iw] = (int)(km / dk) + 2;
         if (nk_w[iw] > nk_max) nk_max = nk_w[iw];
     }
+    const int nblk = (nk_max + 255) / 256;
+    const GreenLayout lay = green_layout(nw, st.L, nblk);
+    if (get_dev && lay.total > cap) return GRTM_E_TOO_LARGE;
+    if ((status = device_ready())) return status;
+
     // vertical wavenumbers of the receiver layer in the first cell (k = 0, w_0),
     // used by the reference's d/dz columns (green.cu:674-675)
     double p0[4];
@@ -401,32 +490,65 @@ int green_core(const char *iname, int compat, std::vector<double> *spec, std::ve
         if (v.real() < 0) v = -v;
         p0[0] = r.real(), p0[1] = r.imag(), p0[2] = v.real(), p0[3] = v.imag();
     }
-    const int nblk = (nk_max + 255) / 256;
-    DBuf d_lw, d_nk, d_part, d_raw, d_tr, d_sp;
-    if ((status = d_lw.alloc(lw.size() * sizeof(LayerW<double>)))) return status;
-    if ((status = d_nk.alloc(nk_w.size() * sizeof(int)))) return status;
-    if ((status = d_part.alloc((size_t)nw * nblk * 60 * sizeof(double)))) return status;
-    if ((status = d_raw.alloc((size_t)30 * nw * 2 * sizeof(double)))) return status;
-    if ((status = d_tr.alloc((size_t)30 * N * sizeof(double)))) return status;
-    if ((status = d_sp.alloc((size_t)30 * nw * 2 * sizeof(double)))) return status;
-    hipStream_t s;
-    HIPCHK(hipStreamCreate(&s));
+    DBuf owned;
+    void *dev = nullptr;
+    if (get_dev) {
+        if ((status = get_dev(&dev))) return status;
+    } else {
+        if ((status = owned.alloc(lay.total))) return status;
+        dev = owned.p;
+    }
+    char *base = static_cast<char *>(dev);
+    auto *d_lw = reinterpret_cast<LayerW<double> *>(base + lay.lw);
+    auto *d_nk = reinterpret_cast<int *>(base + lay.nk);
+    auto *d_part = reinterpret_cast<double *>(base + lay.part);
+    auto *d_raw = reinterpret_cast<double *>(base + lay.raw);
+    auto *d_tr = reinterpret_cast<double *>(base + lay.tr);
+    auto *d_sp = reinterpret_cast<double *>(base + lay.sp);
+    auto *d_pt = reinterpret_cast<double *>(base + lay.pt);
+
+    // the caller's stream; else a stream of the run's own (allocating form) or
+    // the null stream (no-allocation form)
+    hipStream_t s = user_stream;
     struct SG {
-        hipStream_t s;
-        ~SG() { (void)hipStreamDestroy(s); }
-    } sg{s};
-    HIPCHK(hipMemcpyAsync(d_lw.p, lw.data(), lw.size() * sizeof(LayerW<double>), hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(d_nk.p, nk_w.data(), nk_w.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        hipStream_t s = nullptr;
+        ~SG()
+        {
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } sg;
+    if (!s && owned.p) {
+        HIPCHK(hipStreamCreate(&s));
+        sg.s = s;
+    }
+
+    // per-frequency layer tables, staged whole rows at a time
+    const int L = st.L;
+    {
+        constexpr int kRowsPerChunk = 8;
+        LayerW<double> buf[kRowsPerChunk * grtm::kMaxLayers];
+        for (int r0 = 0; r0 < nw; r0 += kRowsPerChunk) {
+            const int r1 = std::min(nw, r0 + kRowsPerChunk);
+            for (int iw = r0; iw < r1; iw++)
+                layer_w(st, compat, cd(2 * M_PI * df * iw, -omega_i), buf + (size_t)(iw - r0) * L);
+            HIPCHK(hipMemcpyAsync(d_lw + (size_t)r0 * L, buf, (size_t)(r1 - r0) * L * sizeof(LayerW<double>),
+                                  hipMemcpyHostToDevice, s));
+            HIPCHK(hipStreamSynchronize(s));  // buf is reused
+        }
+    }
+    HIPCHK(hipMemcpyAsync(d_nk, nk_w, (size_t)nw * sizeof(int), hipMemcpyHostToDevice, s));
+
     // the lowest rows, where an FP64 solve loses more than kDDRowLoss, in
     // double-double; the rest in FP64
     int n_dd = 0;
     while (n_dd < nw && dd_row_loss(nk_w[n_dd] * dk, vmax, std::abs(cd(2 * M_PI * df * n_dd, -omega_i))) > kDDRowLoss)
         n_dd++;
-    HIPCHK(grtm::launch_cells_full(compat, rp.src_type ? 1 : 0, 1, d_lw.p, d_nk.as<int>(), st.L, g, 0, n_dd, nblk,
-                                   dk, rp.dist, df, omega_i, p0, d_part.as<double>(), s));
-    HIPCHK(grtm::launch_cells_full(compat, rp.src_type ? 1 : 0, 0, d_lw.p, d_nk.as<int>(), st.L, g, n_dd, nw - n_dd,
-                                   nblk, dk, rp.dist, df, omega_i, p0, d_part.as<double>(), s));
-    HIPCHK(grtm::launch_green_finish(nw, nblk, d_part.as<double>(), d_raw.as<double>(), s));
+    HIPCHK(grtm::launch_cells_full(compat, rp.src_type ? 1 : 0, 1, d_lw, d_nk, L, g, 0, n_dd, nblk, dk, rp.dist, df,
+                                   omega_i, p0, d_part, s));
+    HIPCHK(grtm::launch_cells_full(compat, rp.src_type ? 1 : 0, 0, d_lw, d_nk, L, g, n_dd, nw - n_dd, nblk, dk,
+                                   rp.dist, df, omega_i, p0, d_part, s));
+    HIPCHK(grtm::launch_green_finish(nw, nblk, d_part, d_raw, s));
+
     grtm::PostParams pp{};
     pp.nw = nw;
     pp.wc = (int)((double)nw * (1 - rp.taper));
@@ -434,17 +556,17 @@ int green_core(const char *iname, int compat, std::vector<double> *spec, std::ve
     pp.dt = rp.dt;
     pp.t0 = rp.t0;
     pp.omega_i = omega_i;
-    DBuf d_pt;
-    if ((status = post_tables(pp, 0, d_pt))) return status;
-    HIPCHK(grtm::launch_post_ifft(0, pp, 0, 30, d_raw.p, nullptr, nullptr, 0, 0, d_tr.p, d_sp.p, s));
-    if (spec) {
-        spec->resize((size_t)30 * nw * 2);
-        HIPCHK(hipMemcpyAsync(spec->data(), d_sp.p, spec->size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    }
-    if (traces) {
-        traces->resize((size_t)30 * N);
-        HIPCHK(hipMemcpyAsync(traces->data(), d_tr.p, traces->size() * sizeof(double), hipMemcpyDeviceToHost, s));
-    }
+    // post tables (as post_tables, FP64): pv [2 nw], tw [2 nw], ts [N]
+    status = stage_h2d<double, 4096>(d_pt, (size_t)4 * nw + N, s, [&](size_t i0, size_t i1, double *buf) {
+        for (size_t i = i0; i < i1; i++) buf[i - i0] = post_table_value(pp, i);
+    });
+    if (status) return status;
+    pp.pv = d_pt;
+    pp.tw = d_pt + 2 * nw;
+    pp.ts = d_pt + 4 * nw;
+    HIPCHK(grtm::launch_post_ifft(0, pp, 0, 30, d_raw, nullptr, nullptr, 0, 0, d_tr, d_sp, s));
+    if (spec) HIPCHK(hipMemcpyAsync(spec, d_sp, (size_t)30 * nw * 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (traces) HIPCHK(hipMemcpyAsync(traces, d_tr, (size_t)30 * N * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return GRTM_OK;
 }
@@ -499,6 +621,7 @@ const char *grtm_status_str(int s)
     case GRTM_E_UNSUPPORTED: return "configuration not supported in this mode";
     case GRTM_E_DEVICE: return "HIP device error";
     case GRTM_E_NOMEM: return "device allocation failed";
+    case GRTM_E_TOO_LARGE: return "run exceeds the fixed scratch of the no-allocation entry point";
     default: return "unknown status";
     }
 }
@@ -508,22 +631,24 @@ int grtm_last_status(void) { return g_last_status; }
 int grtm_green_mem(const char *iname, int compat, double *spec, double *traces, int *nw_out)
 {
     if (!iname) return GRTM_E_INPUT;
-    std::vector<double> sp, tr;
-    int nw = 0;
-    int st = green_core(iname, compat, spec ? &sp : nullptr, traces ? &tr : nullptr, &nw);
-    if (nw_out) *nw_out = nw;
-    if (st) return st;
-    if (spec) memcpy(spec, sp.data(), sp.size() * sizeof(double));
-    if (traces) memcpy(traces, tr.data(), tr.size() * sizeof(double));
-    return GRTM_OK;
+    return green_core(iname, compat, spec, traces, nw_out, nullptr, 0, nullptr);
+}
+
+int grtm_green_mem_noalloc(const char *iname, int compat, double *spec, double *traces, int *nw_out, void *stream)
+{
+    if (!iname) return GRTM_E_INPUT;
+    return green_core(iname, compat, spec, traces, nw_out, scratch_ptr, kScratchBytes,
+                      static_cast<hipStream_t>(stream));
 }
 
 int grtm_green(const char *iname, const char *oname_w, const char *oname_t, int compat)
 {
     if (!iname || !oname_w || !oname_t) return GRTM_E_INPUT;
-    std::vector<double> sp, tr;
     int nw = 0;
-    int st = green_core(iname, compat, &sp, &tr, &nw);
+    int st = green_core(iname, compat, nullptr, nullptr, &nw, nullptr, 0, nullptr);
+    if (st) return st;
+    std::vector<double> sp((size_t)30 * nw * 2), tr((size_t)30 * 2 * nw);
+    st = green_core(iname, compat, sp.data(), tr.data(), &nw, nullptr, 0, nullptr);
     if (st) return st;
     return write_outputs(oname_w, oname_t, nw, sp, tr);
 }

@@ -17,6 +17,7 @@ GRTM_E_INPUT = -3
 GRTM_E_UNSUPPORTED = -4
 GRTM_E_DEVICE = -5
 GRTM_E_NOMEM = -6
+GRTM_E_TOO_LARGE = -7
 
 PREC_FP64 = 0
 PREC_FP32 = 1
@@ -61,6 +62,7 @@ def _lib():
         lib.green.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
         lib.grtm_green.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, I]
         lib.grtm_green_mem.argtypes = [ctypes.c_char_p, I, P, P, P]
+        lib.grtm_green_mem_noalloc.argtypes = [ctypes.c_char_p, I, P, P, P, P]
         lib.grtm_create.argtypes = [P, I]
         lib.grtm_destroy.argtypes = [P]
         lib.grtm_destroy.restype = None
@@ -116,7 +118,7 @@ def _lib():
 
 # every symbol include/grtm.h declares
 EXPORTS = [
-    "green", "grtm_green", "grtm_green_mem", "grtm_last_status", "grtm_status_str", "grtm_version",
+    "green", "grtm_green", "grtm_green_mem", "grtm_green_mem_noalloc", "grtm_last_status", "grtm_status_str", "grtm_version",
     "grtm_create", "grtm_destroy", "grtm_plan_traces", "grtm_plan_execute", "grtm_plan_sync",
     "grtm_plan_traces_out", "grtm_plan_kernels_out", "grtm_plan_timing", "grtm_plan_work",
     "grtm_plan_destroy", "grtm_selftest_cell", "grtm_selftest_cell_dd", "grtm_opcount_cell", "grtm_selftest_gemm", "grtm_selftest_gemm_cut",

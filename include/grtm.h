@@ -20,6 +20,7 @@ extern "C" {
 #define GRTM_E_UNSUPPORTED -4 /* a configuration the selected mode cannot run */
 #define GRTM_E_DEVICE -5      /* HIP runtime error or no GPU */
 #define GRTM_E_NOMEM -6       /* device allocation failed */
+#define GRTM_E_TOO_LARGE -7   /* run exceeds the fixed scratch of grtm_green_mem_noalloc */
 
 #define GRTM_PREC_FP64 0 /* FP64 recursion, FP64 contraction */
 #define GRTM_PREC_FP32 1 /* mixed: FP32 recursion with FP64 vertical wavenumbers,
@@ -49,6 +50,24 @@ int grtm_green(const char *iname, const char *oname_w, const char *oname_t, int 
  * traces: 30 x N reals (column c of oname_t at [c*N + j]).  Either may be
  * NULL.  *nw_out receives nw (call first with both NULL to size buffers). */
 int grtm_green_mem(const char *iname, int compat, double *spec, double *traces, int *nw_out);
+
+/* grtm_green_mem without any device allocation, for callers that cannot
+ * allocate (serving loops, harnesses that forbid hipMalloc).  No counterpart
+ * in the reference.  All device memory of the run -- layer tables, per-row
+ * k counts, the k-sum partials, the spectra and traces before their copy to
+ * the host, the post-processing tables -- lives in one fixed 32 MiB array per
+ * GPU (held until the process ends); a run needing more returns
+ * GRTM_E_TOO_LARGE before any device work.  The partials dominate:
+ * 480 nw ceil(nk_max / 256) bytes for nw frequencies and at most nk_max
+ * wavenumbers per frequency (the 2048-sample demo, nw = 1024 and
+ * nk_max = 10929, needs 23.8 MB in all).  Host staging uses bounded
+ * stack buffers.  Runs on `stream` (NULL: the null stream) and returns when the
+ * outputs are on the host.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU race.  Each call writes every scratch value
+ * it reads, so nothing carries over between calls.  Outputs as
+ * grtm_green_mem, bit for bit. */
+int grtm_green_mem_noalloc(const char *iname, int compat, double *spec, double *traces, int *nw_out,
+                           void *stream);
 
 /* Status of the last green() call on this thread. */
 int grtm_last_status(void);

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
 def test_status_strings():
     lib = _lib._lib()
     assert b"gfx950" in lib.grtm_version()
-    for s in range(-6, 1):
+    for s in range(-7, 1):
         assert lib.grtm_status_str(s)
 
 
@@ -187,3 +187,21 @@ def test_cell_solver_double_double_at_the_lowest_rows(orc, compat):
                 if row == 0:
                     worst64 = max(worst64, np.abs(f - q).max() / m)
     assert worst64 > 1e-2  # the FP64 solve at row 0: noise
+
+
+def test_noalloc_rejects_a_run_beyond_its_scratch(tmp_path):
+    """A run whose device memory exceeds the no-allocation entry point's fixed
+    32 MiB scratch is refused with GRTM_E_TOO_LARGE before any device work
+    (so also without a GPU): 8192 frequencies of the demo model need about
+    190 MB of k-sum partials."""
+    from grtm_cuda_amd import inputfile
+    rows = np.array([[0, 6.1, 3.5, 2.7, 600, 300], [10, 6.6, 3.8, 2.9, 800, 400], [30, 8.0, 4.5, 3.3, 1200, 600]])
+    p = tmp_path / "big.txt"
+    inputfile.write_input(str(p), rows, 5.0, 0.0, 2.0, 50.0, 14, 0.0)
+    lib = _lib._lib()
+    nw = ctypes.c_int(0)
+    spec = np.zeros((30, 8192, 2))
+    st = lib.grtm_green_mem_noalloc(str(p).encode(), 1, spec.ctypes.data_as(ctypes.c_void_p), None,
+                                    ctypes.byref(nw), None)
+    assert st == _lib.GRTM_E_TOO_LARGE and nw.value == 8192
+    assert b"scratch" in lib.grtm_status_str(st)
