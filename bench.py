@@ -35,22 +35,51 @@ FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 
 
-def cpu_baseline(cfg, rows, threads):
-    """FP64 oracle on frequency rows [lo, lo+rows) of the workload, both sources."""
+def host_cpus():
+    """The host cores this process may use (its CPU affinity, capped by
+    OMP_NUM_THREADS where the machine sets a share), and what the host reports."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(aff, int(omp)) if omp.isdigit() and int(omp) > 0 else aff
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
+    return threads, dict(nproc=os.cpu_count(), affinity=aff, omp_num_threads=omp or None, cpu_model=model)
+
+
+def cpu_baseline(cfg, rows, rows_1core):
+    """FP64 oracle (-O3, OpenMP over frequency rows) on frequency rows
+    [nw/2, nw/2 + rows) of the workload, both sources, on every host core this
+    process may use, and on one core over `rows_1core` rows; each scaled to
+    the full job."""
     from oracle import oracle
     nw = cfg["grid"]["nw"]
     lo = nw // 2
-    t0 = time.perf_counter()
-    for kind in (0, 1):
-        oracle.traces(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], kind, cfg["grid"],
-                      w_range=(lo, lo + rows), nthreads=threads)
-    dt = time.perf_counter() - t0
-    full = dt * nw / rows
     traces = 2 * len(cfg["dists"])
-    return dict(value=traces / full, unit="traces/s", cores=threads, kind="port",
+    threads, host = host_cpus()
+
+    def run(nrows, nthreads):
+        t0 = time.perf_counter()
+        for kind in (0, 1):
+            oracle.traces(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], kind, cfg["grid"],
+                          w_range=(lo, lo + nrows), nthreads=nthreads)
+        return time.perf_counter() - t0
+
+    dt = run(rows, threads)
+    dt1 = run(rows_1core, 1)
+    return dict(value=traces / (dt * nw / rows), unit="traces/s", cores=threads, kind="port",
                 sample=f"FP64 oracle, {rows} of {nw} frequency rows x {cfg['grid']['nk']} k x "
                        f"{len(cfg['dists'])} receivers, both sources, {dt:.1f} s on {threads} threads, "
-                       f"scaled x{nw / rows:.0f}")
+                       f"scaled x{nw / rows:.0f}",
+                value_1core=traces / (dt1 * nw / rows_1core),
+                sample_1core=f"{rows_1core} rows, {dt1:.1f} s on 1 thread, scaled x{nw / rows_1core:.0f}",
+                host=host)
 
 
 def time_plan(plan, steps, warmup):
@@ -339,7 +368,8 @@ def main():
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16x3"],
                     help="arithmetic of the measured path (default fp32 adaptive / fp64 dense)")
     ap.add_argument("--derivatives", action="store_true", help="mt: the d/dz and d/dr columns as well")
-    ap.add_argument("--cpu-rows", type=int, default=64)
+    ap.add_argument("--cpu-rows", type=int, default=64, help="CPU baseline sample: frequency rows on all cores")
+    ap.add_argument("--cpu-rows-1core", type=int, default=8, help="CPU baseline sample: frequency rows on one core")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain 256 MiB device copy (calibrates hardware byte counters)")
@@ -409,8 +439,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(cfg, args.cpu_rows, threads)
+        cpu = cpu_baseline(cfg, args.cpu_rows, args.cpu_rows_1core)
 
     line = {
         "metric": "Green's traces/s at <=1e-4 rel-L2 vs FP64; learned-adaptive vs plain dense-k",

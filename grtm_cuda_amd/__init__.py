@@ -82,6 +82,7 @@ class TracePlan:
 
     def __init__(self, layers, src_depth, rcv_depth, dists, grid, precision=PREC_FP64, compat=False, device=0):
         self._ctx = _ctx(device)
+        self.src_depth, self.rcv_depth = float(src_depth), float(rcv_depth)
         self.layers = np.ascontiguousarray(layers, dtype=np.float64)
         self.dists = np.ascontiguousarray(dists, dtype=np.float64)
         self.grid = dict(grid)
@@ -144,8 +145,9 @@ class TracePlan:
     def set_adaptive(self, kcut, window=None, stride=8, tol=1e-5):
         """Use a truncated wavenumber grid: kcut[iw] (km^-1) per frequency row,
         residual check every `stride`-th wavenumber over `window` km^-1 beyond
-        the cut (default: 8 / |z_s - z_r| bounded below by 0.5), fallback to the
-        dense row when the checked tail exceeds tol x the row peak.
+        the cut (default: models.check_window of the plan's source and
+        receiver depths, 4 / max(|z_s - z_r|, 0.5)), fallback to the dense row
+        when the checked tail exceeds tol x the row peak.
         kcut=None restores the dense grid."""
         if kcut is None:
             check(_lib().grtm_plan_set_adaptive(self.ptr, None, 0.0, 1, 1.0), "grtm_plan_set_adaptive")
@@ -154,7 +156,7 @@ class TracePlan:
         if kc.shape != (self.nw,):
             raise ValueError("kcut must have one entry per frequency row")
         if window is None:
-            window = 0.5
+            window = models.check_window(self.src_depth, self.rcv_depth)
         check(_lib().grtm_plan_set_adaptive(self.ptr, kc.ctypes.data_as(P), float(window), int(stride), float(tol)),
               "grtm_plan_set_adaptive")
 

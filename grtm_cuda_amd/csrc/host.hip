@@ -763,6 +763,7 @@ struct grtm_plan {
     bool adaptive = false;
     int chk_stride = 8, max_chk = 0;
     double chk_tol = 1e-5;
+    hipStream_t last_stream = nullptr;  // stream of the last grtm_plan_execute
     std::vector<int> nk_pred_h, nchk_h;
     DBuf nk_pred, nk_w, nchk, counters, fb_list, kend;  // counters: [row peak nw][tail max nw][fb count]
     DBuf tiles;             // recursion work items of the predicted grid, ordered by row
@@ -1228,6 +1229,7 @@ int grtm_plan_execute(grtm_plan *p, void *stream)
 {
     if (!p) return GRTM_E_INPUT;
     hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    p->last_stream = s;
     int st;
     if ((st = run_cells(p, s))) return st;
     if ((st = run_gemm(p, s))) return st;
@@ -1238,7 +1240,7 @@ int grtm_plan_sync(grtm_plan *p)
 {
     if (!p) return GRTM_E_INPUT;
     HIPCHK(hipStreamSynchronize(p->stream));
-    HIPCHK(hipDeviceSynchronize());
+    if (p->last_stream && p->last_stream != p->stream) HIPCHK(hipStreamSynchronize(p->last_stream));
     return GRTM_OK;
 }
 
@@ -1365,19 +1367,15 @@ int grtm_plan_set_rows64(grtm_plan *p, int rows64)
 
 // Cut n and residual-check cells nc of one frequency row.  Check cell j
 // sits at wavenumber n + stride j + stride / 2 and must lie below nk; a row
-// whose cut leaves no room for one is kept dense, since its tail would
-// otherwise be neither checked nor solved.
+// cut below nk with no check cell (no room for one, or a zero window) is
+// kept dense, since its tail would otherwise be neither checked nor solved.
 static void row_cut(int nk, double kcut, double dk, int stride, double window, int &n, int &nc)
 {
     n = (int)std::min<double>(nk, std::max(1.0, std::ceil(kcut / dk)));
     const int room = nk - n;
     const int fit = room > stride / 2 ? (room - stride / 2 - 1) / stride + 1 : 0;
-    if (room > 0 && fit == 0) {
-        n = nk;
-        nc = 0;
-        return;
-    }
     nc = std::max(0, std::min(fit, (int)std::ceil(window / (dk * stride))));
+    if (room > 0 && nc == 0) n = nk;
 }
 
 int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int stride, double tol)
@@ -1401,11 +1399,12 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
         return GRTM_OK;
     }
     if (!(window >= 0) || stride < 1 || !(tol > 0)) return GRTM_E_INPUT;
+    for (int iw = 0; iw < p->nw; iw++)
+        if (!(kcut[iw] >= 0)) return GRTM_E_INPUT;  // validated before any plan state changes
     p->nk_pred_h.assign(p->nw, 0);
     p->nchk_h.assign(p->nw, 0);
     int mx = 0;
     for (int iw = 0; iw < p->nw; iw++) {
-        if (!(kcut[iw] >= 0)) return GRTM_E_INPUT;
         int n, nc;
         row_cut(p->nk, kcut[iw], p->dk, stride, window, n, nc);
         p->nk_pred_h[iw] = n;
@@ -2259,14 +2258,16 @@ int grtm_batch_set_adaptive(grtm_batch *b, const double *kcut, const double *win
     } else {
         if (!window || stride < 1 || !(tol > 0)) return GRTM_E_INPUT;
         const size_t n = (size_t)b->nm * b->nw;
+        for (int m = 0; m < b->nm; m++)  // validated before any batch state changes
+            if (!(window[m] >= 0)) return GRTM_E_INPUT;
+        for (size_t i = 0; i < n; i++)
+            if (!(kcut[i] >= 0)) return GRTM_E_INPUT;
         b->nk_pred_h.assign(n, 0);
         b->nchk_h.assign(n, 0);
         b->max_chk.assign(b->nm, 0);
         for (int m = 0; m < b->nm; m++) {
-            if (!(window[m] >= 0)) return GRTM_E_INPUT;
             for (int iw = 0; iw < b->nw; iw++) {
                 const size_t i = (size_t)m * b->nw + iw;
-                if (!(kcut[i] >= 0)) return GRTM_E_INPUT;
                 int nkp, nc;
                 row_cut(b->nk, kcut[i], b->dk, stride, window[m], nkp, nc);
                 b->nk_pred_h[i] = nkp;

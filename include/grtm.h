@@ -133,6 +133,8 @@ int grtm_plan_columns(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid,
                       int derivatives, grtm_plan **plan);
 /* Run recursion + contraction + IFFT on resident data (asynchronous). */
 int grtm_plan_execute(grtm_plan *plan, void *stream);
+/* Wait for the plan's work: its own stream and the stream its last
+ * grtm_plan_execute ran on (other streams and plans are not waited for). */
 int grtm_plan_sync(grtm_plan *plan);
 /* Copy the traces out: traces[((q*2 + c)*nr + ir)*N + j], q: 0 explosion,
  * 1 vertical force; c: 0 Z, 1 R (radial).  Column plans: the layout above

@@ -163,3 +163,23 @@ def test_branches_match_oracle(tmp_path, orc, compat, src_type, geom):
     # (the reference's receiver search keys on the source depth): e^{+20 nu}
     # growth makes that output rounding noise in any FP64 implementation
     assert rel < max(1e-9, 5 * noise), (rel, noise)
+
+
+def test_c_caller_writes_the_reference_files(tmp_path, demo_gpu):
+    """The C caller (tests/c_caller/green_main, built by build() against
+    include/grtm.h) run as the reference's ./test is: green("input.txt",
+    "out4.txt", "signal.txt") with paths relative to its working directory."""
+    import shutil
+    import subprocess
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "tests", "c_caller", "green_main")
+    shutil.copy(DEMO, tmp_path / "input.txt")
+    print("gpu: C caller green()", flush=True)
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    w = inputfile.read_output(str(tmp_path / "out4.txt"))
+    t = inputfile.read_output(str(tmp_path / "signal.txt"))
+    spec, tr = demo_gpu
+    nw = spec.shape[1]
+    assert np.allclose(w[:nw, 0::2] + 1j * w[:nw, 1::2], spec.T, rtol=1e-15, atol=0)
+    assert np.allclose(t, tr.T, rtol=1e-15, atol=0)
