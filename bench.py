@@ -412,13 +412,11 @@ def main():
     else:
         plan = mk(prec)
         if args.path == "adaptive":
+            # the learned grid is formed on the GPU inside every execution:
+            # network, per-row cuts, residual-check cells and item list
             from grtm_cuda_amd import learned
-            t0 = time.perf_counter()
-            pred = learned.Predictor()
-            kcut = pred.kcut(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["grid"])
-            extra["predictor_ms"] = 1e3 * (time.perf_counter() - t0)
-            plan.set_adaptive(kcut, window=models.check_window(cfg["src_depth"], cfg["rcv_depth"]), stride=8,
-                              tol=3e-6)
+            learned.Predictor().attach(plan, window=models.check_window(cfg["src_depth"], cfg["rcv_depth"]),
+                                       stride=8, tol=3e-6)
         print(f"bench: {args.path} {pname}", file=sys.stderr, flush=True)
         t_step = time_plan(plan, args.steps, args.warmup)
         out = plan.traces()
@@ -432,6 +430,9 @@ def main():
     value = traces_per_step / t_step
     plan.timing(reps=1)  # warm: the stage split is taken on the steady state
     tm = plan.timing(reps=5)
+    if args.path == "adaptive":
+        extra.update(predictor_ms=round(tm["predictor"], 4), predictor_share=round(tm["predictor"] / tm["total"], 4),
+                     predictor_in_step=True)
     # the committed counter summary was taken on the default run (c2, learned adaptive, fp32)
     counted = args.workload == "c2" and args.path == "adaptive" and prec == g.PREC_FP32
     roof, kernels = roofline(g, plan, prec, tm, plan.work(), counted)

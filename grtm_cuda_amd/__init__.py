@@ -126,10 +126,12 @@ class TracePlan:
 
     def timing(self, reps=5):
         """ms per execution: total, recursion stage, contraction, IFFT, Bessel
-        tables, and the main recursion launch alone (``main``)."""
-        ms = np.zeros(6)
+        tables, the main recursion launch alone (``main``), and a learned
+        grid's formation on the device (``predictor``, part of the recursion
+        stage)."""
+        ms = np.zeros(7)
         check(_lib().grtm_plan_timing(self.ptr, int(reps), ms.ctypes.data_as(P)), "grtm_plan_timing")
-        return dict(total=ms[0], cells=ms[1], ksum=ms[2], ifft=ms[3], bessel=ms[4], main=ms[5])
+        return dict(total=ms[0], cells=ms[1], ksum=ms[2], ifft=ms[3], bessel=ms[4], main=ms[5], predictor=ms[6])
 
     def work(self):
         w = np.zeros(11)
@@ -159,6 +161,30 @@ class TracePlan:
             window = models.check_window(self.src_depth, self.rcv_depth)
         check(_lib().grtm_plan_set_adaptive(self.ptr, kc.ctypes.data_as(P), float(window), int(stride), float(tol)),
               "grtm_plan_set_adaptive")
+
+    def set_learned(self, predictor, window=None, stride=8, tol=3e-6):
+        """Learned adaptive grid formed on the GPU at every execution: the
+        network of `predictor` (learned.Predictor) evaluates this model's rows
+        inside execute(), followed by the residual check and dense fallback of
+        set_adaptive.  predictor=None returns to the dense grid."""
+        if predictor is None:
+            check(_lib().grtm_plan_set_learned(self.ptr, None, 0.0, 0.0, 1, 1.0), "grtm_plan_set_learned")
+            self._predictor = None
+            return
+        if window is None:
+            window = models.check_window(self.src_depth, self.rcv_depth)
+        check(_lib().grtm_plan_set_learned(self.ptr, predictor.mlp.ptr, float(predictor.lmargin), float(window),
+                                           int(stride), float(tol)), "grtm_plan_set_learned")
+        self._predictor = predictor  # the plan uses its network: keep it alive
+
+    def adaptive_rows(self):
+        """(cut, checks) per frequency row of the current grid (a learned grid:
+        as of the last execution)."""
+        cut = np.zeros(self.nw, dtype=np.int32)
+        chk = np.zeros(self.nw, dtype=np.int32)
+        check(_lib().grtm_plan_adaptive_rows(self.ptr, cut.ctypes.data_as(P), chk.ctypes.data_as(P)),
+              "grtm_plan_adaptive_rows")
+        return cut, chk
 
     def row_labels(self, tau=1e-7):
         """Per-row training labels after a dense execute(): array [nw, 26]

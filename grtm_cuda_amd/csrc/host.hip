@@ -768,6 +768,19 @@ struct grtm_plan {
     DBuf nk_pred, nk_w, nchk, counters, fb_list, kend;  // counters: [row peak nw][tail max nw][fb count]
     DBuf tiles;             // recursion work items of the predicted grid, ordered by row
     int ntiles = 0, ntiles64 = 0;  // all items; items of rows [0, rows64)
+    // learned grid built on the device at every execution (grtm_plan_set_learned):
+    // the network, the model's predictor inputs, the device item counts
+    // [rows < rows64, all] and the persistent launches' item counters
+    grtm_mlp *net = nullptr;
+    grtm::LearnGrid lg{};
+    grtm::LearnModel lm{};
+    DBuf lmodel, icounts, ictr, ioff;
+    int tiles_cap = 0;       // capacity of tiles (items)
+    bool pred_stale = false;  // nk_pred_h / nchk_h behind the device's last execution
+    // the job's model, kept for the predictor's inputs
+    std::vector<double> layers_h;
+    double src_depth = 0, rcv_depth = 0;
+    int rows64_rule = 0;  // the default FP64-row split (also for FP64 plans: the learned floor)
     double bessel_ms = 0;
     // kind 0: explosion and vertical-force Z / R traces; kind 1: the
     // displacement columns, whose operands, tables and outputs are the four
@@ -783,6 +796,7 @@ struct grtm_plan {
     // timing: events around the main recursion launch (the FP32 rows of a
     // mixed plan, all rows of an FP64 plan), recorded when set
     hipEvent_t ev_main0 = nullptr, ev_main1 = nullptr;
+    hipEvent_t ev_pred0 = nullptr, ev_pred1 = nullptr;  // around a learned grid's formation
     int mt_M(int q) const { return 2 * mto.rows[q] * nw; }
     ~grtm_plan()
     {
@@ -856,19 +870,38 @@ int run_cells(grtm_plan *p, hipStream_t s)
 {
     if (p->kind == 1) return run_cells_mt(p, s);
     const bool ad = p->adaptive;
+    const bool learned = ad && p->net;
     int *nkw = ad ? p->nk_w.as<int>() : nullptr;
     unsigned *rowpk = ad ? p->counters.as<unsigned>() : nullptr;
     unsigned *tailmx = ad ? rowpk + p->nw : nullptr;
     unsigned *fbcnt = ad ? rowpk + 2 * p->nw : nullptr;
+    const int r64 = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;
+    if (learned) {
+        // the learned grid of this execution: network -> per-row cut and
+        // check count -> recursion items, all on the stream
+        int st;
+        if (p->ev_pred0) HIPCHK(hipEventRecord(p->ev_pred0, s));
+        if ((st = grtm::learned_grid(p->net, p->lmodel.as<grtm::LearnModel>(), 1, p->lg, p->nk_pred.as<int>(),
+                                     p->nchk.as<int>(), s)))
+            return st;
+        HIPCHK(grtm::launch_grid_items(p->nw, r64, p->nk_pred.as<int>(), p->nchk.as<int>(), p->tiles.as<unsigned>(),
+                                       p->ioff.as<unsigned>(), p->icounts.as<unsigned>(), s));
+        if (p->ev_pred1) HIPCHK(hipEventRecord(p->ev_pred1, s));
+        p->pred_stale = true;
+    }
     if (ad) {
         HIPCHK(grtm::launch_copy_i32(p->nk_pred.as<int>(), nkw, p->nw, s));
         HIPCHK(grtm::launch_zero_u32(rowpk, 2 * p->nw + 1, s));
     }
-    const int r64 = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;
-    // work items: the predicted grid's list (rows [0, r64) first), or every tile
+    // work items: the predicted grid's list (rows [0, r64) first), or every
+    // tile; a learned grid's list and counts are on the device (persistent
+    // launches, the item count's upper bound is the list's capacity)
     const unsigned *t_lo = ad ? p->tiles.as<unsigned>() : nullptr;
-    const unsigned *t_hi = ad ? t_lo + p->ntiles64 : nullptr;
-    const int n_lo = ad ? p->ntiles64 : 0, n_hi = ad ? p->ntiles - p->ntiles64 : 0;
+    const unsigned *t_hi = ad ? (learned ? t_lo : t_lo + p->ntiles64) : nullptr;
+    const int n_lo = learned ? p->tiles_cap : ad ? p->ntiles64 : 0;
+    const int n_hi = learned ? p->tiles_cap : ad ? p->ntiles - p->ntiles64 : 0;
+    const unsigned *dcnt = learned ? p->icounts.as<unsigned>() : nullptr;
+    unsigned *ctr = learned ? p->ictr.as<unsigned>() : nullptr;
     // adaptive: the item lists carry the residual check beyond the cut (FP64 on
     // the FP64 rows) after each launch's tiles
     const int *nchk = ad ? p->nchk.as<int>() : nullptr;
@@ -876,18 +909,18 @@ int run_cells(grtm_plan *p, hipStream_t s)
         if (p->ev_main0) HIPCHK(hipEventRecord(p->ev_main0, s));
         HIPCHK(grtm::launch_cells_zr(0, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->nw,
                                      t_lo, n_lo, p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk, s, nchk,
-                                     p->chk_stride, tailmx));
+                                     p->chk_stride, tailmx, dcnt, 0, ctr));
         if (p->ev_main1) HIPCHK(hipEventRecord(p->ev_main1, s));
     } else {
         unsigned *cnt = p->flags.as<unsigned>(), *list = cnt + 1;
         HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
         HIPCHK(grtm::launch_cells_zr(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0,
                                      p->rows64, t_lo, n_lo, p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk, s,
-                                     nchk, p->chk_stride, tailmx));
+                                     nchk, p->chk_stride, tailmx, dcnt, 0, ctr));
         if (p->ev_main0) HIPCHK(hipEventRecord(p->ev_main0, s));
         HIPCHK(grtm::launch_cells_zr(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->rows64,
                                      p->nw - p->rows64, t_hi, n_hi, p->A0.p, p->A1.p, p->nk, cnt, list, nkw, rowpk, s,
-                                     nchk, p->chk_stride, tailmx));
+                                     nchk, p->chk_stride, tailmx, dcnt, 1, ctr ? ctr + 1 : nullptr));
         if (p->ev_main1) HIPCHK(hipEventRecord(p->ev_main1, s));
     }
     const bool mixed = p->prec != GRTM_PREC_FP64;
@@ -1085,13 +1118,17 @@ static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid
     const double df = 1.0 / T, omega_i = grid->omega_i > 0 ? grid->omega_i : M_PI / T;
     ModelTables mt;
     int status = model_tables(job->layers, job->n_layers, job->src_depth, job->rcv_depth, compat, nw, df, omega_i,
-                              precision != GRTM_PREC_FP64, mt);
+                              true, mt);
     if (status) return status;
     grtm_plan *p = new grtm_plan;
     struct Guard {
         grtm_plan *&p;
         ~Guard() { delete p; }
     } guard{p};
+    p->layers_h.assign(job->layers, job->layers + 6 * (size_t)job->n_layers);
+    p->src_depth = job->src_depth;
+    p->rcv_depth = job->rcv_depth;
+    p->rows64_rule = mt.rows64;
     p->g = mt.g;
     p->ctx = ctx;
     p->prec = precision;
@@ -1288,12 +1325,14 @@ int grtm_plan_kernels_out(grtm_plan *p, int q, int w_lo, int w_hi, double *kern)
 int grtm_plan_timing(grtm_plan *p, int reps, double *ms)
 {
     if (!p || !ms || reps < 1) return GRTM_E_INPUT;
-    hipEvent_t ev[6];
-    for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&ev[i]));
-    double acc[5] = {0, 0, 0, 0, 0};
+    hipEvent_t ev[8];
+    for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&ev[i]));
+    double acc[6] = {0, 0, 0, 0, 0, 0};
     int st = GRTM_OK;
     p->ev_main0 = ev[4];
     p->ev_main1 = ev[5];
+    const bool learned = p->adaptive && p->net && p->kind == 0;
+    if (learned) p->ev_pred0 = ev[6], p->ev_pred1 = ev[7];
     for (int r = 0; r < reps && !st; r++) {
         if (hipEventRecord(ev[0], p->stream) != hipSuccess) { st = GRTM_E_DEVICE; break; }
         if ((st = run_cells(p, p->stream))) break;
@@ -1312,14 +1351,20 @@ int grtm_plan_timing(grtm_plan *p, int reps, double *ms)
             st = GRTM_E_DEVICE;
             break;
         }
-        acc[0] += a, acc[1] += b, acc[2] += c, acc[3] += d, acc[4] += e;
+        float f = 0;
+        if (learned && hipEventElapsedTime(&f, ev[6], ev[7]) != hipSuccess) {
+            st = GRTM_E_DEVICE;
+            break;
+        }
+        acc[0] += a, acc[1] += b, acc[2] += c, acc[3] += d, acc[4] += e, acc[5] += f;
     }
-    p->ev_main0 = p->ev_main1 = nullptr;
-    for (int i = 0; i < 6; i++) (void)hipEventDestroy(ev[i]);
+    p->ev_main0 = p->ev_main1 = p->ev_pred0 = p->ev_pred1 = nullptr;
+    for (int i = 0; i < 8; i++) (void)hipEventDestroy(ev[i]);
     if (st) return st;
     for (int i = 0; i < 4; i++) ms[i] = acc[i] / reps;
     ms[4] = p->bessel_ms;
     ms[5] = acc[4] / reps;
+    ms[6] = acc[5] / reps;
     return GRTM_OK;
 }
 
@@ -1358,10 +1403,20 @@ static int build_items(grtm_plan *p)
     return GRTM_OK;
 }
 
+// The learned grid's predictor inputs to the device (the floor rows follow the
+// plan's FP64 split).
+static int upload_learn_model(grtm_plan *p)
+{
+    p->lm.floor_rows = p->prec == GRTM_PREC_FP64 ? p->rows64_rule : p->rows64;
+    HIPCHK(hipMemcpy(p->lmodel.p, &p->lm, sizeof(grtm::LearnModel), hipMemcpyHostToDevice));
+    return GRTM_OK;
+}
+
 int grtm_plan_set_rows64(grtm_plan *p, int rows64)
 {
     if (!p || p->prec == GRTM_PREC_FP64 || rows64 < 0 || rows64 > p->nw) return GRTM_E_INPUT;
     p->rows64 = rows64;
+    if (p->adaptive && p->net) return upload_learn_model(p);
     return p->adaptive ? build_items(p) : GRTM_OK;
 }
 
@@ -1391,6 +1446,8 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
             ntile += (Mq[q] + 127) / 128;
         }
     }
+    p->net = nullptr;  // a host cut replaces a learned grid
+    p->pred_stale = false;
     if (!kcut) {
         p->adaptive = false;
         if (p->pair)
@@ -1456,6 +1513,95 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
     return GRTM_OK;
 }
 
+// The device-built grid of the last execution into nk_pred_h / nchk_h.
+static int pred_to_host(grtm_plan *p)
+{
+    if (!p->net || !p->pred_stale) return GRTM_OK;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (p->last_stream && p->last_stream != p->stream) HIPCHK(hipStreamSynchronize(p->last_stream));
+    HIPCHK(hipMemcpy(p->nk_pred_h.data(), p->nk_pred.p, sizeof(int) * p->nw, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(p->nchk_h.data(), p->nchk.p, sizeof(int) * p->nw, hipMemcpyDeviceToHost));
+    p->pred_stale = false;
+    return GRTM_OK;
+}
+
+int grtm_plan_set_learned(grtm_plan *p, grtm_mlp *net, double lmargin, double window, int stride, double tol)
+{
+    if (!p) return GRTM_E_INPUT;
+    if (!net) return grtm_plan_set_adaptive(p, nullptr, 0.0, 1, 1.0);
+    if (p->kind != 0 || !p->pair || p->gemm_mode == 2) return GRTM_E_UNSUPPORTED;
+    if (!(window >= 0) || stride < 1 || !(tol > 0) || !std::isfinite(lmargin)) return GRTM_E_INPUT;
+    const int n = (int)p->layers_h.size() / 6;
+    if (n > grtm::kLearnSlots) return GRTM_E_UNSUPPORTED;  // the network's 30 layer slots
+    int st;
+    // per-row buffers as grtm_plan_set_adaptive, and the device item list
+    const int tpr = grtm::cells_tiles_per_row(p->nk);
+    const int cap = p->nw * (2 * tpr + 1);
+    if (!p->nk_pred.p) {
+        if ((st = p->nk_pred.alloc(sizeof(int) * p->nw)) || (st = p->nk_w.alloc(sizeof(int) * p->nw)) ||
+            (st = p->nchk.alloc(sizeof(int) * p->nw)) || (st = p->counters.alloc(sizeof(unsigned) * (2 * p->nw + 1))) ||
+            (st = p->fb_list.alloc(sizeof(unsigned) * 2 * p->nw)) || (st = p->kend.alloc(sizeof(int) * ((p->nw + 15) / 16))))
+            return st;
+    }
+    if (p->tiles.n < sizeof(unsigned) * (size_t)cap && (st = p->tiles.alloc(sizeof(unsigned) * (size_t)cap)))
+        return st;
+    if (!p->lmodel.p && ((st = p->lmodel.alloc(sizeof(grtm::LearnModel))) ||
+                         (st = p->icounts.alloc(2 * sizeof(unsigned))) || (st = p->ictr.alloc(2 * sizeof(unsigned))) ||
+                         (st = p->ioff.alloc(2 * sizeof(unsigned) * p->nw))))
+        return st;
+    if (p->gemm_mode == 1) {
+        const int M2[2] = {4 * p->nw, 4 * p->nw};
+        const size_t need = grtm::gemm_streamk_workspace_bytes(2, M2, p->nr, p->nk);
+        if (need > p->ws.n && (st = p->ws.alloc(need))) return st;
+    } else {
+        p->kc = grtm::gemm_pair_slice(p->gemm_mode, 4 * p->nw, p->nr, p->nk, p->nk);
+    }
+    grtm::LearnModel &m = p->lm;
+    m = grtm::LearnModel{};
+    m.n = n;
+    double vs_min = 1e30;
+    for (int j = 0; j < n; j++) {
+        for (int c = 0; c < 6; c++) m.rows[j][c] = p->layers_h[(size_t)6 * j + c];
+        vs_min = std::min(vs_min, m.rows[j][2]);
+    }
+    m.src = p->src_depth;
+    m.rcv = p->rcv_depth;
+    m.vs_min = vs_min;
+    m.window = window;
+    p->lg = grtm::LearnGrid{p->nw, p->nk, stride, 0, p->pp.dt, p->dk, p->dk * p->nk, lmargin};
+    p->tiles_cap = cap;
+    p->chk_stride = stride;
+    p->chk_tol = tol;
+    p->net = net;
+    p->adaptive = true;
+    if ((st = upload_learn_model(p))) return st;
+    // one evaluation now, so the host's view of the grid (work, stats) is set
+    p->nk_pred_h.assign(p->nw, 0);
+    p->nchk_h.assign(p->nw, 0);
+    if ((st = grtm::learned_grid(net, p->lmodel.as<grtm::LearnModel>(), 1, p->lg, p->nk_pred.as<int>(),
+                                 p->nchk.as<int>(), p->stream))) {
+        p->net = nullptr;
+        p->adaptive = false;
+        return st;
+    }
+    p->pred_stale = true;
+    return pred_to_host(p);
+}
+
+int grtm_plan_adaptive_rows(grtm_plan *p, int *cut, int *checks)
+{
+    if (!p || !cut || !checks) return GRTM_E_INPUT;
+    if (!p->adaptive) {
+        for (int iw = 0; iw < p->nw; iw++) cut[iw] = p->nk, checks[iw] = 0;
+        return GRTM_OK;
+    }
+    int st;
+    if ((st = pred_to_host(p))) return st;
+    memcpy(cut, p->nk_pred_h.data(), sizeof(int) * p->nw);
+    memcpy(checks, p->nchk_h.data(), sizeof(int) * p->nw);
+    return GRTM_OK;
+}
+
 int grtm_plan_row_labels(grtm_plan *p, double tau, float *out)
 {
     if (!p || !out || !(tau > 0) || p->adaptive) return GRTM_E_INPUT;
@@ -1476,6 +1622,8 @@ int grtm_plan_adaptive_stats(grtm_plan *p, double *out)
     if (!p || !out) return GRTM_E_INPUT;
     out[0] = out[1] = out[2] = out[3] = 0;
     if (!p->adaptive) return GRTM_OK;
+    int st;
+    if ((st = pred_to_host(p))) return st;
     HIPCHK(hipStreamSynchronize(p->stream));
     unsigned fb = 0;
     HIPCHK(hipMemcpy(&fb, p->counters.as<unsigned>() + 2 * p->nw, sizeof(unsigned), hipMemcpyDeviceToHost));
@@ -1497,6 +1645,10 @@ int grtm_plan_adaptive_stats(grtm_plan *p, double *out)
 int grtm_plan_work(grtm_plan *p, double *w)
 {
     if (!p || !w) return GRTM_E_INPUT;
+    {
+        const int st = pred_to_host(p);
+        if (st) return st;
+    }
     const double cells = (double)p->nw * p->nk;
     w[0] = cells;
     w[1] = cells * p->L;

@@ -4,6 +4,8 @@
 
 #include "cell.h"
 
+struct grtm_mlp;  // the predictor network (mlp.hip; C ABI in grtm.h)
+
 namespace grtm {
 
 // Operation count of one cell whose upward sweep ends at layer bot (the depth
@@ -62,11 +64,43 @@ int cells_tiles_per_row(int nk);
 unsigned cells_tile(int iw, int kb);
 constexpr unsigned kCheckItem = 1u << 31;
 int cells_tile_limit(int nw, int nk);  // 1 if (nw, nk) fit the item encoding
+// dev_count (optional): the item list was built on the device and its length
+// is only known there -- items [which ? dev_count[0] : 0, dev_count[which])
+// of `tiles`, taken by a persistent grid from the counter *ctr (zeroed by the
+// launcher); ntiles is then an upper bound on the item count.
 hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
                            double df, double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles, void *A0,
                            void *A1, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
                            unsigned *row_peak, hipStream_t st, const int *nchk = nullptr, int stride = 0,
-                           unsigned *tail_max = nullptr);
+                           unsigned *tail_max = nullptr, const unsigned *dev_count = nullptr, int which = 0,
+                           unsigned *ctr = nullptr);
+
+// ---- learned wavenumber grid built on the device
+// The predictor's inputs for one model and the decoding of its cut output
+// (the rows of learned.py: features, decode_kcut, the low-frequency floor).
+constexpr int kLearnSlots = 30;
+struct LearnModel {
+    double rows[kLearnSlots][6];  // thickness km, vp, vs, rho, qp, qs (first n rows)
+    int n;                        // layer rows, <= kLearnSlots
+    int floor_rows;               // rows [0, floor_rows) keep at least the analytic cut
+    double src, rcv;              // source / receiver depth km
+    double vs_min;                // slowest shear velocity of the model
+    double window;                // residual-check window km^-1
+};
+struct LearnGrid {
+    int nw, nk, stride, pad;
+    double dt, dk, kmax, lmargin;
+};
+// Per-row cut nk_pred[m nw + iw] and residual-check count nchk[...] of
+// n_models models from one execution of the network (learned.hip / mlp.hip).
+int learned_grid(grtm_mlp *net, const LearnModel *models, int n_models, const LearnGrid &lg, int *nk_pred,
+                 int *nchk, hipStream_t st);
+// Recursion items of one model's learned grid from nk_pred / nchk: tiles of
+// rows [0, r64), their residual-check items, tiles of rows [r64, nw), theirs;
+// counts[0] = items of rows [0, r64), counts[1] = all items.  off: 2 nw
+// scratch words (each row's first tile and first check item).
+hipError_t launch_grid_items(int nw, int r64, const int *nk_pred, const int *nchk, unsigned *items, unsigned *off,
+                             unsigned *counts, hipStream_t st);
 // adaptive grid: dense fallback, tile extents
 // Rows whose checked tail exceeds tol x row peak are completed on the dense
 // grid: FP64 plans in FP64; mixed plans in FP64 below row r64 and in FP32

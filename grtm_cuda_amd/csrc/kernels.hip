@@ -35,6 +35,10 @@ namespace grtm {
 // nk_w (optional): per-row number of wavenumbers solved (adaptive grid);
 // row_peak (optional): per-row max |A| as float bits.
 // BR: the plan's receiver branch, fixed at compile time.
+// dev_count (persistent form): the item list was built on the device; the
+// grid is sized to the resident blocks, and each block takes items
+// [lo, hi) = [which ? dev_count[0] : 0, dev_count[which]) one at a time from
+// the counter *ctr until the list is drained.
 template <typename T, typename TO, bool COMPAT, int BR>
 __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__restrict__ lw_all, int L,
                                                            Geo g, int nk, double dk, double df,
@@ -46,50 +50,138 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__re
                                                            const int *__restrict__ nk_w,
                                                            unsigned *__restrict__ row_peak,
                                                            const int *__restrict__ nchk, int stride,
-                                                           unsigned *__restrict__ tail_max)
+                                                           unsigned *__restrict__ tail_max,
+                                                           const unsigned *__restrict__ dev_count, int which,
+                                                           unsigned *__restrict__ ctr)
 {
     __shared__ LayerW<T> lw[kMaxLayers];
-    int iw, kb;
-    bool chk = false;
-    if (tiles) {
-        const unsigned e = tiles[blockIdx.x];
-        chk = (e & kCheckItem) != 0;
-        iw = int((e & ~kCheckItem) >> kTileBits);
-        kb = int(e & ((1u << kTileBits) - 1));
-    } else {
-        const int q = int(blockIdx.x) / tpr;
-        iw = w0 + q;
-        kb = int(blockIdx.x) - q * tpr;
-    }
-    const int nkr = nk_w ? nk_w[iw] : nk;
-    const int lim = chk ? nchk[iw] : nkr;   // cells (check cells) of the row
-    if (kb * kCellThreads >= lim) return;  // uniform over the block
-    for (int j = threadIdx.x; j < L; j += blockDim.x) lw[j] = lw_all[(size_t)iw * L + j];
-    __syncthreads();
-    const int j = kb * kCellThreads + threadIdx.x;
-    const int ik = chk ? nkr + stride * j + stride / 2 : j;
-    unsigned mag = 0;
-    if (j < lim && ik < nk) {
-        TO v[8];
-        const bool ok = cell_zr<T, TO, COMPAT, BR>(lw, g, iw, ik, dk, df, omega_i, v);
-        if (chk) {
-            float m = 0.f;
-            for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(float(v[i])));
-            mag = __float_as_uint(isfinite(m) ? m : 3.0e38f);  // a non-finite check value fails the row
+    __shared__ unsigned item_sh;
+    const unsigned lo = dev_count && which ? dev_count[0] : 0u, hi = dev_count ? dev_count[which] : 0u;
+    for (int pass = 0;; pass++) {
+        unsigned idx;
+        if (dev_count) {
+            __syncthreads();  // the previous item's reads of lw and item_sh are done
+            if (threadIdx.x == 0) item_sh = lo + atomicAdd(ctr, 1u);
+            __syncthreads();
+            idx = item_sh;
+            if (idx >= hi) return;  // uniform over the block: the list is drained
         } else {
-            store_zr(v, iw, ik, A0, A1, lda);
-            mag = cell_mag_bits(v);
-            if (!ok && flag_count) {
-                const unsigned slot = atomicAdd(flag_count, 1u);
-                flag_list[slot] = unsigned(iw) * unsigned(nk) + unsigned(ik);
+            if (pass) return;
+            idx = blockIdx.x;
+        }
+        int iw, kb;
+        bool chk = false;
+        if (tiles) {
+            const unsigned e = tiles[idx];
+            chk = (e & kCheckItem) != 0;
+            iw = int((e & ~kCheckItem) >> kTileBits);
+            kb = int(e & ((1u << kTileBits) - 1));
+        } else {
+            const int q = int(idx) / tpr;
+            iw = w0 + q;
+            kb = int(idx) - q * tpr;
+        }
+        const int nkr = nk_w ? nk_w[iw] : nk;
+        const int lim = chk ? nchk[iw] : nkr;      // cells (check cells) of the row
+        if (kb * kCellThreads >= lim) continue;  // uniform over the block
+        for (int j = threadIdx.x; j < L; j += blockDim.x) lw[j] = lw_all[(size_t)iw * L + j];
+        __syncthreads();
+        const int j = kb * kCellThreads + threadIdx.x;
+        const int ik = chk ? nkr + stride * j + stride / 2 : j;
+        unsigned mag = 0;
+        if (j < lim && ik < nk) {
+            TO v[8];
+            const bool ok = cell_zr<T, TO, COMPAT, BR>(lw, g, iw, ik, dk, df, omega_i, v);
+            if (chk) {
+                float m = 0.f;
+                for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(float(v[i])));
+                mag = __float_as_uint(isfinite(m) ? m : 3.0e38f);  // a non-finite check value fails the row
+            } else {
+                store_zr(v, iw, ik, A0, A1, lda);
+                mag = cell_mag_bits(v);
+                if (!ok && flag_count) {
+                    const unsigned slot = atomicAdd(flag_count, 1u);
+                    flag_list[slot] = unsigned(iw) * unsigned(nk) + unsigned(ik);
+                }
             }
         }
+        unsigned *dst = chk ? tail_max : row_peak;
+        if (dst) {
+            mag = wave_max_u32(mag);
+            if ((threadIdx.x & 63) == 0 && mag) atomicMax(dst + iw, mag);
+        }
     }
-    unsigned *dst = chk ? tail_max : row_peak;
-    if (dst) {
-        mag = wave_max_u32(mag);
-        if ((threadIdx.x & 63) == 0 && mag) atomicMax(dst + iw, mag);
+}
+
+// Recursion items of one model's learned grid, in two steps.  k_grid_scan
+// (one block of 1024 threads, rows [8 t, 8 t + 8) per thread, nw <= 8192):
+// each row's first tile item and first check item in the list order of the
+// host (tiles of rows [0, r64) in row order, then their residual-check items,
+// then the tiles of rows [r64, nw), then theirs): off[iw], off[nw + iw]; and
+// counts[0] = items of rows [0, r64), counts[1] = all items.  k_grid_fill
+// (one wave per row) writes the items.
+__global__ __launch_bounds__(1024) void k_grid_scan(int nw, int r64, const int *__restrict__ nk_pred,
+                                                    const int *__restrict__ nchk, unsigned *__restrict__ off,
+                                                    unsigned *__restrict__ counts)
+{
+    constexpr int R = 8;
+    __shared__ unsigned st[1024], sc[1024];
+    __shared__ unsigned t64_sh, c64_sh;
+    const int t = threadIdx.x;
+    unsigned nt[R], nc[R], ti = 0, ci = 0;
+    for (int q = 0; q < R; q++) {
+        const int iw = t * R + q;
+        nt[q] = iw < nw ? unsigned((nk_pred[iw] + kCellThreads - 1) / kCellThreads) : 0u;
+        nc[q] = iw < nw ? unsigned((nchk[iw] + kCellThreads - 1) / kCellThreads) : 0u;
+        ti += nt[q];
+        ci += nc[q];
     }
+    st[t] = ti;
+    sc[t] = ci;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {  // inclusive scans over the threads
+        const unsigned a = t >= o ? st[t - o] : 0u, b = t >= o ? sc[t - o] : 0u;
+        __syncthreads();
+        st[t] += a;
+        sc[t] += b;
+        __syncthreads();
+    }
+    const unsigned T = st[1023], Cn = sc[1023];
+    unsigned pt = st[t] - ti, pc = sc[t] - ci;  // exclusive prefixes of this thread's first row
+    for (int q = 0; q < R; q++) {  // the prefixes at row r64
+        if (t * R + q == r64) t64_sh = pt, c64_sh = pc;
+        pt += nt[q];
+        pc += nc[q];
+    }
+    if (t == 0 && r64 >= nw) t64_sh = T, c64_sh = Cn;
+    __syncthreads();
+    const unsigned T64 = t64_sh, C64 = c64_sh;
+    pt = st[t] - ti, pc = sc[t] - ci;
+    for (int q = 0; q < R; q++) {
+        const int iw = t * R + q;
+        if (iw >= nw) break;
+        off[iw] = iw < r64 ? pt : C64 + pt;
+        off[nw + iw] = iw < r64 ? T64 + pc : T + pc;
+        pt += nt[q];
+        pc += nc[q];
+    }
+    if (t == 0) {
+        counts[0] = T64 + C64;
+        counts[1] = T + Cn;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_grid_fill(int nw, const int *__restrict__ nk_pred,
+                                                   const int *__restrict__ nchk, const unsigned *__restrict__ off,
+                                                   unsigned *__restrict__ items)
+{
+    const int iw = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (iw >= nw) return;
+    const unsigned nt = unsigned((nk_pred[iw] + kCellThreads - 1) / kCellThreads);
+    const unsigned nc = unsigned((nchk[iw] + kCellThreads - 1) / kCellThreads);
+    const unsigned tb = off[iw], cb = off[nw + iw];
+    for (unsigned k = lane; k < nt; k += 64) items[tb + k] = (unsigned(iw) << kTileBits) | k;
+    for (unsigned k = lane; k < nc; k += 64) items[cb + k] = ((unsigned(iw) << kTileBits) | k) | kCheckItem;
 }
 
 // Rows whose checked tail exceeds tol * row peak fall back to the dense grid.
@@ -612,18 +704,46 @@ int cells_tile_limit(int nw, int nk)
     return (nw < (1 << (32 - kTileBits)) && cells_tiles_per_row(nk) <= (1 << kTileBits)) ? 1 : 0;
 }
 
+// Blocks of `kernel` (kCellThreads threads) resident on the whole device at
+// once, capped at ub: the grid of a persistent launch.
+static int resident_blocks(const void *kernel, int ub)
+{
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kCellThreads, 0) != hipSuccess || per < 1)
+        per = 1, cus = cus > 0 ? cus : 256;
+    return std::max(1, std::min(ub, per * cus));
+}
+
+hipError_t launch_grid_items(int nw, int r64, const int *nk_pred, const int *nchk, unsigned *items, unsigned *off,
+                             unsigned *counts, hipStream_t st)
+{
+    if (nw < 1 || nw > 8192) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_grid_scan, dim3(1), dim3(1024), 0, st, nw, r64, nk_pred, nchk, off, counts);
+    hipLaunchKernelGGL(k_grid_fill, dim3((nw + 3) / 4), dim3(256), 0, st, nw, nk_pred, nchk, off, items);
+    return hipGetLastError();
+}
+
 hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
                            double df, double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles, void *A0,
                            void *A1, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
-                           unsigned *row_peak, hipStream_t st, const int *nchk, int stride, unsigned *tail_max)
+                           unsigned *row_peak, hipStream_t st, const int *nchk, int stride, unsigned *tail_max,
+                           const unsigned *dev_count, int which, unsigned *ctr)
 {
     const int tpr = cells_tiles_per_row(nk);
     if (!tiles) ntiles = nwr * tpr;
     if (ntiles <= 0) return hipSuccess;
+    if (dev_count) {
+        if (!tiles || !ctr || which < 0 || which > 1) return hipErrorInvalidValue;
+        hipError_t e = launch_zero_u32(ctr, 1, st);
+        if (e != hipSuccess) return e;
+    }
 #define GRTM_ZR_BR(T, TO, CP, BR, FC, FL)                                                                       \
-    hipLaunchKernelGGL((k_cells_zr<T, TO, CP, BR>), dim3(ntiles), dim3(kCellThreads), 0, st,                     \
-                       (const LayerW<T> *)lw, L, g, nk, dk, df, omega_i, w0, tpr, tiles, (TO *)A0, (TO *)A1, lda, FC, \
-                       FL, nk_w, row_peak, nchk, stride, tail_max)
+    hipLaunchKernelGGL((k_cells_zr<T, TO, CP, BR>),                                                              \
+                       dim3(dev_count ? resident_blocks((const void *)k_cells_zr<T, TO, CP, BR>, ntiles) : ntiles), \
+                       dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, L, g, nk, dk, df, omega_i, w0, tpr, tiles, \
+                       (TO *)A0, (TO *)A1, lda, FC, FL, nk_w, row_peak, nchk, stride, tail_max, dev_count, which, ctr)
 #define GRTM_ZR(T, TO, CP, FC, FL)                                                 \
     do {                                                                           \
         switch (g.branch) {                                                        \

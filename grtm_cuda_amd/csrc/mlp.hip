@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/grtm.h"
+#include "kernels.h"
 
 namespace {
 
@@ -151,6 +152,173 @@ __global__ void k_gather(int B, int n_in, int n_out, const int *__restrict__ idx
     }
 }
 
+// ---------------------------------------------------------------- inference
+// The fused forward: one block of 4 waves takes 32 rows through every layer
+// on v_mfma_f32_32x32x16_bf16 (BF16 operands, FP32 accumulation: the same
+// arithmetic as the training forward), activations in LDS as BF16, weights
+// read as 16-byte fragments of W^T from L2.  Hidden layers are 256 wide (each
+// wave computes 64 output columns); the last layer's 32 padded columns are
+// wave 0's.  MODE 0: rows of X in, outputs Y out.  MODE 1: the rows are
+// (model, frequency) pairs; features are formed from the model (learned.py
+// features()), and output 0 is decoded into the row's cut and residual-check
+// count (learned.py decode_kcut, the low-frequency floor, row_cut).
+constexpr int kMlpW = 256, kMlpKin = 192, kMlpNout = 32, kMlpRows = 32, kMlpMaxLayers = 8;
+constexpr int kMlpLd = kMlpW + 8;  // LDS row stride (bf16): 16-byte row skew
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+struct MlpDev {
+    const bf16 *wT[kMlpMaxLayers];
+    const float *b[kMlpMaxLayers];
+    int kp[kMlpMaxLayers], np[kMlpMaxLayers];
+    int nl, n_in, n_out;
+};
+
+__device__ __forceinline__ void learned_decode(const grtm::LearnModel &m, const grtm::LearnGrid &lg, int iw,
+                                               float y0, int &n, int &nc)
+{
+    const double w = 2.0 * M_PI * iw / (2.0 * lg.nw * lg.dt);
+    const double dz = fmax(fabs(m.src - m.rcv), 0.5);
+    const double s = fabs(w) / m.vs_min + 1.0 / dz;
+    double kc = exp(double(y0) + lg.lmargin) * s;
+    // rows solved in FP64 keep at least the analytic cut (models.analytic_kcut)
+    if (iw < m.floor_rows) kc = fmax(kc, fmin(1.15 * w / m.vs_min + log(1e7) / dz, lg.kmax));
+    kc = fmin(kc, lg.kmax);
+    // row_cut (host.hip): check cell j at n + stride j + stride / 2 < nk; a
+    // cut row without a check cell stays dense
+    n = (int)fmin((double)lg.nk, fmax(1.0, ceil(kc / lg.dk)));
+    const int room = lg.nk - n;
+    const int fit = room > lg.stride / 2 ? (room - lg.stride / 2 - 1) / lg.stride + 1 : 0;
+    nc = max(0, min(fit, (int)ceil(m.window / (lg.dk * lg.stride))));
+    if (room > 0 && nc == 0) n = lg.nk;
+}
+
+// One layer of the fused forward for this wave's NT output tiles: every
+// weight fragment of the layer is requested before the first product, so the
+// wave waits on memory once per layer, not once per k step.
+template <int KP, int NT>
+__device__ __forceinline__ void mlp_layer(const bf16 (*in)[kMlpLd], const bf16 *__restrict__ W, int n0, int lane,
+                                          v16f acc[2])
+{
+    const int li = lane & 31, kq = (lane >> 5) * 8;
+    v8bf b[KP / 16][NT];
+#pragma unroll
+    for (int s = 0; s < KP / 16; s++)
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+            b[s][t] = *reinterpret_cast<const v8bf *>(W + (size_t)(n0 + 32 * t + li) * KP + 16 * s + kq);
+#pragma unroll
+    for (int s = 0; s < KP / 16; s++) {
+        const v8bf a = *reinterpret_cast<const v8bf *>(&in[li][16 * s + kq]);
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[s][t], acc[t], 0, 0, 0);
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_mlp_infer(MlpDev net, int rows, const bf16 *__restrict__ X,
+                                                   float *__restrict__ Y, const grtm::LearnModel *__restrict__ models,
+                                                   grtm::LearnGrid lg, int *__restrict__ nk_pred,
+                                                   int *__restrict__ nchk)
+{
+    constexpr int NF = 6 * grtm::kLearnSlots;  // layer features of a model
+    __shared__ __attribute__((aligned(16))) bf16 act[2][kMlpRows][kMlpLd];
+    __shared__ float mf[MODE ? kMlpRows + 1 : 1][MODE ? NF : 1];  // the block's models' layer features
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int r0 = blockIdx.x * kMlpRows;
+    int m_first = 0;
+    if constexpr (MODE == 1) {
+        // learned.py features(): (h / 10, vp / 8, vs / 4.5, rho / 3.3,
+        // log10 qp / 3, log10 qs / 3) per layer slot, zero beyond the model's
+        // layers, formed in FP64 and rounded to FP32 like the host's; once
+        // per model of the block's rows
+        m_first = r0 / lg.nw;
+        const int m_last = min(rows - 1, r0 + kMlpRows - 1) / lg.nw;
+        for (int e = tid; e < (m_last - m_first + 1) * NF; e += 256) {
+            const int mm = e / NF, k = e % NF, j = k / 6, c = k % 6;
+            const grtm::LearnModel &md = models[m_first + mm];
+            float v = 0.f;
+            if (j < md.n) {
+                const double x = md.rows[j][c];
+                const double sc[6] = {10.0, 8.0, 4.5, 3.3, 3.0, 3.0};
+                v = float(c >= 4 ? log10(x) / sc[c] : x / sc[c]);
+            }
+            mf[mm][k] = v;
+        }
+        __syncthreads();
+    }
+    // input rows, zero padded to kMlpKin
+    for (int e = tid; e < kMlpRows * kMlpKin; e += 256) {
+        const int i = e / kMlpKin, k = e % kMlpKin, row = r0 + i;
+        float v = 0.f;
+        if (row < rows && k < net.n_in) {
+            if constexpr (MODE == 0) {
+                v = float(X[(size_t)row * net.n_in + k]);
+            } else {
+                const int m = row / lg.nw, iw = row - m * lg.nw;
+                if (k < NF) v = mf[m - m_first][k];
+                else if (k == NF) v = float(models[m].src / 40.0);
+                else if (k == NF + 1) v = float(2.0 * M_PI * iw / (2.0 * lg.nw * lg.dt) / (2.0 * M_PI * 10.0));
+            }
+        }
+        act[0][i][k] = bf16(v);
+    }
+    __syncthreads();
+    int cur = 0;
+    const int li = lane & 31;
+    for (int l = 0; l < net.nl; l++) {
+        const bool last = l == net.nl - 1;
+        const int n0 = last ? 0 : wv * 64;
+        v16f acc[2];
+        for (int t = 0; t < 2; t++)
+            for (int r = 0; r < 16; r++) acc[t][r] = 0.f;
+        // layer shapes of the fused kernel: kMlpKin in, kMlpW hidden, kMlpNout out
+        if (last) {
+            if (wv == 0) mlp_layer<kMlpW, 1>(act[cur], net.wT[l], n0, lane, acc);
+        } else if (l == 0) {
+            mlp_layer<kMlpKin, 2>(act[cur], net.wT[l], n0, lane, acc);
+        } else {
+            mlp_layer<kMlpW, 2>(act[cur], net.wT[l], n0, lane, acc);
+        }
+        const int ntile = last ? (wv == 0 ? 1 : 0) : 2;
+        // accumulator r of lane l: row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
+        for (int t = 0; t < ntile; t++)
+            for (int r = 0; r < 16; r++) {
+                const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int col = n0 + 32 * t + li;
+                const float z = acc[t][r] + net.b[l][col];
+                if (!last) {
+                    act[cur ^ 1][row][col] = bf16(fmaxf(z, 0.f));
+                } else if (r0 + row < rows) {
+                    if constexpr (MODE == 0) {
+                        if (col < net.n_out) Y[(size_t)(r0 + row) * net.n_out + col] = z;
+                    } else if (col == 0) {
+                        const int g = r0 + row, m = g / lg.nw, iw = g - m * lg.nw;
+                        int n, nc;
+                        learned_decode(models[m], lg, iw, z, n, nc);
+                        nk_pred[g] = n;
+                        nchk[g] = nc;
+                    }
+                }
+            }
+        __syncthreads();
+        cur ^= 1;
+    }
+}
+
+// W^T (bf16, padded) and padded biases of every layer from the FP32 master
+// parameters.
+__global__ void k_mlp_prep(int I, int O, int Kp, int Np, const float *__restrict__ w, const float *__restrict__ b,
+                           bf16 *__restrict__ wT, float *__restrict__ bT)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < Kp * Np) {
+        const int n = t / Kp, k = t % Kp;
+        wT[t] = bf16(n < O && k < I ? w[(size_t)k * O + n] : 0.f);
+    }
+    if (t < Np) bT[t] = t < O ? b[t] : 0.f;
+}
+
 struct DB {
     void *p = nullptr;
     ~DB()
@@ -182,6 +350,10 @@ struct grtm_mlp {
     int ndata = 0;
     DB Xb, Tb, Mb, idx, lossp;      // batch buffers
     std::vector<DB> Z, H, dH;       // per layer
+    // inference copy for the fused matrix-core forward: per layer W^T as bf16
+    // [Np][Kp] (K padded to 16, N to 32, zeros), biases [Np] in FP32
+    DB wT, bT;
+    bool fused = false;             // the shape fits the fused kernel
     ~grtm_mlp()
     {
         if (st) (void)hipStreamDestroy(st);
@@ -263,6 +435,40 @@ int refresh_bf16(grtm_mlp *n)
     return GRTM_OK;
 }
 
+// Layout of the fused kernel's padded copy: K of layer 0 padded to kMlpKin,
+// hidden layers kMlpW wide, the output padded to kMlpNout.
+MlpDev mlp_dev(const grtm_mlp *n)
+{
+    MlpDev d{};
+    const int L = n->depth + 1;
+    size_t wo = 0, bo = 0;
+    for (int l = 0; l < L; l++) {
+        d.kp[l] = l == 0 ? kMlpKin : kMlpW;  // the fused kernel's compile-time shapes (zero padded)
+        d.np[l] = l == L - 1 ? kMlpNout : kMlpW;
+        d.wT[l] = n->wT.as<bf16>() + wo;
+        d.b[l] = n->bT.as<float>() + bo;
+        wo += (size_t)d.kp[l] * d.np[l];
+        bo += d.np[l];
+    }
+    d.nl = L, d.n_in = n->fin[0], d.n_out = n->n_out;
+    return d;
+}
+
+// The fused inference copy of the current parameters (after every change).
+int refresh_infer(grtm_mlp *n)
+{
+    if (!n->fused) return GRTM_OK;
+    const MlpDev d = mlp_dev(n);
+    for (int l = 0; l < d.nl; l++) {
+        const int cnt = d.kp[l] * d.np[l];
+        hipLaunchKernelGGL(k_mlp_prep, dim3((cnt + 255) / 256), dim3(256), 0, n->st, n->fin[l], n->fout[l], d.kp[l],
+                           d.np[l], n->w32.as<float>() + n->woff[l], n->w32.as<float>() + n->boff[l],
+                           const_cast<bf16 *>(d.wT[l]), const_cast<float *>(d.b[l]));
+    }
+    HIPCHK(hipGetLastError());
+    return GRTM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -308,6 +514,16 @@ int grtm_mlp_create(int n_in, int width, int depth, int n_out, int max_batch, gr
     if (!st && (hipMemset(n->m.p, 0, sizeof(float) * off) != hipSuccess ||
                 hipMemset(n->v.p, 0, sizeof(float) * off) != hipSuccess))
         st = GRTM_E_DEVICE;
+    // the fused matrix-core forward: inputs up to kMlpKin, hidden width kMlpW,
+    // outputs up to kMlpNout, at most kMlpMaxLayers layers
+    n->fused = n_in <= kMlpKin && width == kMlpW && n_out <= kMlpNout && L <= kMlpMaxLayers;
+    if (!st && n->fused) {
+        const MlpDev d = mlp_dev(n);
+        size_t wn = 0, bn = 0;
+        for (int l = 0; l < L; l++) wn += (size_t)d.kp[l] * d.np[l], bn += d.np[l];
+        st = n->wT.alloc(sizeof(bf16) * wn);
+        if (!st) st = n->bT.alloc(sizeof(float) * bn);
+    }
     if (st) {
         delete n;
         return st;
@@ -336,6 +552,7 @@ int grtm_mlp_set_params(grtm_mlp *n, const float *p)
     HIPCHK(hipMemset(n->v.p, 0, sizeof(float) * n->nparam));
     n->step = 0;
     int st = refresh_bf16(n);
+    if (!st) st = refresh_infer(n);
     if (st) return st;
     HIPCHK(hipStreamSynchronize(n->st));
     return GRTM_OK;
@@ -401,6 +618,8 @@ int grtm_mlp_train(grtm_mlp *n, int steps, int batch, float lr, unsigned long lo
                            n->v.as<float>(), lr, b1, b2, eps, c1, c2, 1);
         HIPCHK(hipGetLastError());
     }
+    int st = refresh_infer(n);
+    if (st) return st;
     HIPCHK(hipStreamSynchronize(n->st));
     return GRTM_OK;
 }
@@ -421,21 +640,54 @@ int grtm_mlp_grads(grtm_mlp *n, const float *X, const float *T, const float *Mk,
     return GRTM_OK;
 }
 
-int grtm_mlp_predict(grtm_mlp *n, const float *X, int cnt, float *Y)
+// fused: the matrix-core forward (k_mlp_infer); otherwise the training
+// forward (k_mm).  Both compute BF16 products with FP32 accumulation.
+static int mlp_predict(grtm_mlp *n, const float *X, int cnt, float *Y, bool fused)
 {
     if (!n || !X || !Y || cnt < 1) return GRTM_E_INPUT;
+    if (fused && !n->fused) return GRTM_E_UNSUPPORTED;
     for (int b0 = 0; b0 < cnt; b0 += n->bmax) {
         const int B = std::min(n->bmax, cnt - b0);
         std::vector<bf16> xb((size_t)B * n->n_in);
         for (size_t i = 0; i < xb.size(); i++) xb[i] = bf16(X[(size_t)b0 * n->n_in + i]);
         HIPCHK(hipMemcpy(n->Xb.p, xb.data(), sizeof(bf16) * xb.size(), hipMemcpyHostToDevice));
-        int st = mlp_forward(n, B, n->Xb.as<bf16>());
-        if (st) return st;
+        const float *src = n->Z[n->depth].as<float>();
+        if (fused) {
+            hipLaunchKernelGGL((k_mlp_infer<0>), dim3((B + kMlpRows - 1) / kMlpRows), dim3(256), 0, n->st, mlp_dev(n),
+                               B, n->Xb.as<bf16>(), n->Z[n->depth].as<float>(), nullptr, grtm::LearnGrid{}, nullptr,
+                               nullptr);
+            HIPCHK(hipGetLastError());
+        } else {
+            int st = mlp_forward(n, B, n->Xb.as<bf16>());
+            if (st) return st;
+        }
         HIPCHK(hipStreamSynchronize(n->st));
-        HIPCHK(hipMemcpy(Y + (size_t)b0 * n->n_out, n->Z[n->depth].p, sizeof(float) * (size_t)B * n->n_out,
-                         hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(Y + (size_t)b0 * n->n_out, src, sizeof(float) * (size_t)B * n->n_out, hipMemcpyDeviceToHost));
     }
     return GRTM_OK;
 }
 
+int grtm_mlp_predict(grtm_mlp *n, const float *X, int cnt, float *Y)
+{
+    return mlp_predict(n, X, cnt, Y, n && n->fused);
+}
+
+int grtm_mlp_predict_train_forward(grtm_mlp *n, const float *X, int cnt, float *Y)
+{
+    return mlp_predict(n, X, cnt, Y, false);
+}
+
 }  // extern "C"
+
+namespace grtm {
+int learned_grid(grtm_mlp *net, const LearnModel *models, int n_models, const LearnGrid &lg, int *nk_pred,
+                 int *nchk, hipStream_t st)
+{
+    if (!net || !net->fused || net->fin[0] != 6 * kLearnSlots + 2 || n_models < 1) return GRTM_E_UNSUPPORTED;
+    const int rows = n_models * lg.nw;
+    hipLaunchKernelGGL((k_mlp_infer<1>), dim3((rows + kMlpRows - 1) / kMlpRows), dim3(256), 0, st, mlp_dev(net), rows,
+                       nullptr, nullptr, models, lg, nk_pred, nchk);
+    HIPCHK(hipGetLastError());
+    return GRTM_OK;
+}
+}  // namespace grtm

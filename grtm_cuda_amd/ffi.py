@@ -85,8 +85,10 @@ def _lib():
         lib.grtm_selftest_mfma_probe.argtypes = [I, P]
         lib.grtm_selftest_copy.argtypes = [I]
         lib.grtm_plan_set_adaptive.argtypes = [P, P, D, I, D]
+        lib.grtm_plan_set_learned.argtypes = [P, P, D, D, I, D]
         lib.grtm_plan_set_rows64.argtypes = [P, I]
         lib.grtm_plan_adaptive_stats.argtypes = [P, P]
+        lib.grtm_plan_adaptive_rows.argtypes = [P, P, P]
         lib.grtm_plan_row_labels.argtypes = [P, D, P]
         lib.grtm_mlp_create.argtypes = [I, I, I, I, I, P]
         lib.grtm_mlp_destroy.argtypes = [P]
@@ -99,6 +101,7 @@ def _lib():
         lib.grtm_mlp_train.argtypes = [P, I, I, ctypes.c_float, ctypes.c_ulonglong, P]
         lib.grtm_mlp_grads.argtypes = [P, P, P, P, I, P, P]
         lib.grtm_mlp_predict.argtypes = [P, P, I, P]
+        lib.grtm_mlp_predict_train_forward.argtypes = [P, P, I, P]
         lib.grtm_mlp_set_loss.argtypes = [P, I, ctypes.c_float]
         lib.grtm_batch_create.argtypes = [P, P, I, P, I, P, I, I, I, P]
         lib.grtm_batch_execute.argtypes = [P, P]
@@ -123,10 +126,10 @@ EXPORTS = [
     "grtm_plan_traces_out", "grtm_plan_kernels_out", "grtm_plan_timing", "grtm_plan_work",
     "grtm_plan_destroy", "grtm_selftest_cell", "grtm_selftest_cell_dd", "grtm_opcount_cell", "grtm_selftest_gemm", "grtm_selftest_gemm_cut",
     "grtm_plan_columns", "grtm_opcount_columns_cell",
-    "grtm_selftest_mfma_probe", "grtm_selftest_copy", "grtm_plan_set_adaptive", "grtm_plan_set_rows64",
-    "grtm_plan_adaptive_stats",
+    "grtm_selftest_mfma_probe", "grtm_selftest_copy", "grtm_plan_set_adaptive", "grtm_plan_set_learned", "grtm_plan_set_rows64",
+    "grtm_plan_adaptive_stats", "grtm_plan_adaptive_rows",
     "grtm_plan_row_labels", "grtm_mlp_create", "grtm_mlp_destroy", "grtm_mlp_nparams", "grtm_mlp_set_params",
-    "grtm_mlp_get_params", "grtm_mlp_set_data", "grtm_mlp_train", "grtm_mlp_grads", "grtm_mlp_predict",
+    "grtm_mlp_get_params", "grtm_mlp_set_data", "grtm_mlp_train", "grtm_mlp_grads", "grtm_mlp_predict", "grtm_mlp_predict_train_forward",
     "grtm_mlp_set_loss", "grtm_batch_create", "grtm_batch_execute", "grtm_batch_execute_eager", "grtm_batch_sync",
     "grtm_batch_traces_out", "grtm_batch_timing", "grtm_batch_config", "grtm_batch_work", "grtm_batch_destroy",
     "grtm_batch_set_adaptive", "grtm_batch_adaptive_stats",

@@ -152,17 +152,40 @@ class Predictor:
         self.mlp.set_params(data["params"])
         self.lmargin = float(data["lmargin"]) if "lmargin" in data else 0.0
 
-    def kcut(self, layers, src_depth, rcv_depth, grid):
+    def kcut(self, layers, src_depth, rcv_depth, grid, floor_rows=None):
+        """Per-row truncation wavenumbers of the grid.  Rows [0, floor_rows)
+        keep at least the analytic cut; default: the plans' FP64 rows
+        (fp64_rows)."""
         nw, dt = grid["nw"], grid["dt"]
         w = 2 * math.pi * np.arange(nw) / (2 * nw * dt)
         y = self.mlp.predict(features(layers, src_depth, w))
         kc = decode_kcut(y, layers, src_depth, rcv_depth, w, self.lmargin)
-        # Rows below the plans' FP64 split (|w| < 2.5 beta_min / dz) keep at
-        # least the analytic cut: they are few and their cuts short, while a
-        # failed check sends such a row to a dense FP64 completion, a
-        # latency-bound launch on the critical path
+        # Rows solved in FP64 keep at least the analytic cut: they are few and
+        # their cuts short, while a failed check sends such a row to a dense
+        # FP64 completion, a latency-bound launch on the critical path
         from grtm_cuda_amd import models
-        dz = max(abs(src_depth - rcv_depth), 1e-3)
-        low = w < 2.5 * float(np.min(layers[:, 2])) / dz
+        if floor_rows is None:
+            floor_rows = fp64_rows(layers, src_depth, rcv_depth, grid)
+        low = np.arange(nw) < floor_rows
         kc = np.where(low, np.maximum(kc, models.analytic_kcut(layers, src_depth, rcv_depth, grid)), kc)
         return np.minimum(kc, grid["kmax"])
+
+    def attach(self, plan, window=None, stride=8, tol=3e-6):
+        """Give `plan` (TracePlan) the learned grid formed on the GPU at every
+        execution by this predictor's network."""
+        plan.set_learned(self, window=window, stride=stride, tol=tol)
+
+
+def fp64_rows(layers, src_depth, rcv_depth, grid):
+    """The mixed-precision plans' default FP64 rows: |w + i w_I| below
+    2.5 Vs_min / max(|z_s - z_r|, 1e-3), at least 4 (host.hip model_tables)."""
+    nw, dt = grid["nw"], grid["dt"]
+    T = 2 * nw * dt
+    oi = grid.get("omega_i", 0.0)
+    oi = oi if oi > 0 else math.pi / T
+    dz = max(abs(src_depth - rcv_depth), 1e-3)
+    wc = 2.5 * float(np.min(np.asarray(layers)[:, 2])) / dz
+    r = 0
+    while r < nw and math.hypot(2 * math.pi * r / T, oi) < wc:
+        r += 1
+    return min(nw, max(r, 4))

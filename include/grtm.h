@@ -147,7 +147,9 @@ int grtm_plan_kernels_out(grtm_plan *plan, int q, int w_lo, int w_hi, double *ke
  * ms[0] total per execution, ms[1] recursion, ms[2] contraction,
  * ms[3] IFFT, ms[4] Bessel tables (one-off, part of planning), ms[5] the
  * main recursion launch alone (FP32 rows of a mixed plan, all rows of an
- * FP64 plan; its cells are work[9]).  6 doubles. */
+ * FP64 plan; its cells are work[9]), ms[6] the formation of a learned grid
+ * (grtm_plan_set_learned: network, cuts, item list; part of ms[1]; 0
+ * otherwise).  7 doubles. */
 int grtm_plan_timing(grtm_plan *plan, int reps, double *ms);
 /* Work of one execution: [0] cells, [1] cell-layers, [2] contraction flops,
  * [3] IFFT algorithmic bytes, [4] traces (column plans: column traces),
@@ -183,6 +185,10 @@ int grtm_plan_row_labels(grtm_plan *plan, double tau, float *out);
 /* After an execution: out[0] cells of the predicted grid, out[1] residual-check
  * cells, out[2] cells after fallback, out[3] rows that fell back. */
 int grtm_plan_adaptive_stats(grtm_plan *plan, double *out);
+/* The current grid per frequency row (a learned grid: as of the last
+ * execution): cut[iw] wavenumbers solved before any fallback, checks[iw]
+ * residual-check cells; the dense grid gives nk and 0. */
+int grtm_plan_adaptive_rows(grtm_plan *plan, int *cut, int *checks);
 
 /* ------------------------------------------------------------ batches
  * Many layered models (uneven layer counts and source depths) on one grid
@@ -311,6 +317,23 @@ int grtm_mlp_train(grtm_mlp *mlp, int steps, int batch, float lr, unsigned long 
 /* Gradient of the loss on n <= max_batch samples (params layout). */
 int grtm_mlp_grads(grtm_mlp *mlp, const float *X, const float *T, const float *M, int n, float *grads, float *loss);
 int grtm_mlp_predict(grtm_mlp *mlp, const float *X, int n, float *Y);
+/* The same outputs from the training forward (VALU tiles) instead of the
+ * fused matrix-core forward grtm_mlp_predict uses: test hook. */
+int grtm_mlp_predict_train_forward(grtm_mlp *mlp, const float *X, int n, float *Y);
+
+/* Learned adaptive grid formed on the GPU at every grtm_plan_execute, inside
+ * the execution (no counterpart in the reference, whose ragged grid uses the
+ * cal_kmax heuristic, green.cu:108-162): the network's features of the plan's
+ * model at every frequency row, one fused matrix-core forward, the cut
+ * k = exp(y0 + lmargin) (|w| / Vs_min + 1 / max(|z_s - z_r|, 0.5)) (at least
+ * the analytic cut 1.15 w / Vs_min + ln(1e7) / dz on the rows solved in
+ * FP64, at most kmax), the residual check of grtm_plan_set_adaptive (window,
+ * stride, tol) and the recursion's item list, taken by persistent launches.
+ * Z / R plans with the streamed contraction (FP64, FP32; nk and nr multiples
+ * of 4), models of at most 30 layers.  The plan keeps `net`, which must
+ * outlive it or be detached with net = NULL (which returns the plan to the
+ * dense grid). */
+int grtm_plan_set_learned(grtm_plan *plan, grtm_mlp *net, double lmargin, double window, int stride, double tol);
 
 #ifdef __cplusplus
 }

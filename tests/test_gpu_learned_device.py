@@ -1,0 +1,90 @@
+"""The learned grid formed on the GPU inside each execution
+(grtm_plan_set_learned): the fused matrix-core forward against the training
+forward and the FP64 restatement, the device-built grid against the host's
+decode of the same network, and the traces against the plain FP64 dense path."""
+import numpy as np
+import pytest
+
+import grtm_cuda_amd as g
+from grtm_cuda_amd import learned, models
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pred():
+    return learned.Predictor()
+
+
+def test_fused_forward_matches_training_forward_and_restatement(pred):
+    from oracle import mlp_ref
+    rng = np.random.default_rng(11)
+    layers = models.layered_model(5, 12)
+    w = 2 * np.pi * rng.uniform(0, 10, 300)
+    X = learned.features(layers, 17.0, w)
+    print("gpu: fused mlp forward", flush=True)
+    a = pred.mlp.predict(X)  # fused matrix-core forward
+    b = np.zeros_like(a)
+    from grtm_cuda_amd.ffi import P, _lib, check
+    Xc = np.ascontiguousarray(X, dtype=np.float32)
+    check(_lib().grtm_mlp_predict_train_forward(pred.mlp.ptr, Xc.ctypes.data_as(P), Xc.shape[0], b.ctypes.data_as(P)),
+          "train forward")
+    # the same BF16 products, FP32 sums in another order
+    assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max()
+    _, _, y = mlp_ref.loss_and_grads(pred.mlp.get_params(), pred.mlp.shapes(), X, np.zeros_like(a),
+                                     np.zeros_like(a))
+    assert np.linalg.norm(a - y) <= 2e-3 * np.linalg.norm(y)
+    # a planted error in the restatement (a dropped bias of the last layer) is seen
+    p = pred.mlp.get_params().copy()
+    p[-learned.N_OUT:] += 0.05
+    _, _, y_bad = mlp_ref.loss_and_grads(p, pred.mlp.shapes(), X, np.zeros_like(a), np.zeros_like(a))
+    assert np.linalg.norm(a - y_bad) > 10 * np.linalg.norm(a - y)
+
+
+@pytest.fixture(scope="module")
+def c2_plans(pred):
+    cfg = models.config("c2")
+    mk = lambda p: g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
+                               precision=p)
+    print("gpu: c2 fp64 dense reference", flush=True)
+    ref = mk(g.PREC_FP64).traces()
+    print("gpu: c2 fp32 learned grid on the device", flush=True)
+    dev = mk(g.PREC_FP32)
+    pred.attach(dev, stride=8, tol=3e-6)
+    tr_dev = dev.traces()
+    print("gpu: c2 fp32 learned grid from the host", flush=True)
+    host = mk(g.PREC_FP32)
+    host.set_adaptive(pred.kcut(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["grid"]),
+                      window=models.check_window(cfg["src_depth"], cfg["rcv_depth"]), stride=8, tol=3e-6)
+    tr_host = host.traces()
+    return cfg, ref, dev, tr_dev, host, tr_host
+
+
+def test_device_grid_matches_host_decode(c2_plans):
+    cfg, _, dev, _, host, _ = c2_plans
+    cd, kd = dev.adaptive_rows()
+    ch, kh = host.adaptive_rows()
+    same = cd == ch
+    print(f"rows with equal cuts: {same.mean() * 100:.2f}%, max |diff| {np.abs(cd - ch).max()}")
+    # the two forwards sum in different orders: a cut may move by one cell
+    assert np.abs(cd - ch).max() <= 1 and same.mean() > 0.95
+    assert np.array_equal(kd[same], kh[same])
+    # the floor follows the plan's FP64 rows, also in the host decode
+    assert dev.work()["rows_fp64"] == learned.fp64_rows(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"],
+                                                        cfg["grid"])
+
+
+def test_device_learned_traces(c2_plans):
+    cfg, ref, dev, tr_dev, host, tr_host = c2_plans
+    rl = np.linalg.norm(tr_dev - ref, axis=-1) / np.linalg.norm(ref, axis=-1)
+    pk = np.abs(tr_dev - ref).max(axis=-1) / np.abs(ref).max(axis=-1)
+    d = np.linalg.norm(tr_dev - tr_host, axis=-1) / np.linalg.norm(tr_host, axis=-1)
+    st = dev.adaptive_stats()
+    print(f"device learned grid: {st['cells_final'] / (cfg['grid']['nw'] * cfg['grid']['nk']) * 100:.1f}% cells, "
+          f"{st['rows_fallback']} fallback rows; vs FP64 dense rel-L2 {rl.max():.2e} peak {pk.max():.2e}; "
+          f"vs host-built grid {d.max():.2e}")
+    assert rl.max() < 1e-4 and pk.max() < 1e-4
+    assert d.max() < 1e-5
+    # every execution rebuilds the grid: a second run is identical
+    again = dev.traces()
+    assert np.array_equal(again, tr_dev)
