@@ -123,13 +123,9 @@ def bench_batch(args, g, models):
     build = time.perf_counter() - t_build
     extra = {}
     if args.path == "adaptive":
+        # every model's learned grid is formed on the GPU inside the graph
         from grtm_cuda_amd import learned
-        t0 = time.perf_counter()
-        pred = learned.Predictor()
-        kc = np.stack([pred.kcut(l, zs, cfg["rcv_depth"], cfg["grid"]) for l, zs in cfg["models"]])
-        win = np.array([models.check_window(zs, cfg["rcv_depth"]) for _, zs in cfg["models"]])
-        extra["predictor_s"] = round(time.perf_counter() - t0, 2)
-        plan.set_adaptive(kc, win, stride=8, tol=3e-6)
+        plan.set_learned(learned.Predictor(), stride=8, tol=3e-6)
     t_step = time_batch(plan, args.steps, args.warmup)
     if args.path == "adaptive":
         st = plan.adaptive_stats()
@@ -144,6 +140,9 @@ def bench_batch(args, g, models):
         worst_pk = max(worst_pk, float((np.abs(a - b).max(axis=-1) / np.abs(b).max(axis=-1)).max()))
     work = plan.work()
     tm = plan.timing(reps=1)
+    if args.path == "adaptive":
+        extra.update(predictor_ms=round(tm["predictor"], 3), predictor_share=round(tm["predictor"] / tm["total"], 4),
+                     predictor_in_step=True)
     peak = FP64_PEAK_TFLOPS if prec == g.PREC_FP64 else FP32_PEAK_TFLOPS
     # counted recursion flops of the cells actually solved, each with its upward
     # sweep as cut by depth (adaptive: the predicted cells, plus the check cells)

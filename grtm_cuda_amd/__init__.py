@@ -269,6 +269,8 @@ class BatchPlan:
         for i, (l, (_, zs)) in enumerate(zip(self._layers, models)):
             ms[i] = Model(l.shape[0], l.ctypes.data_as(P), float(zs), float(rcv_depth))
         self.dists = np.ascontiguousarray(dists, dtype=np.float64)
+        self.src_depths = [float(zs) for _, zs in models]
+        self.rcv_depths = [float(rcv_depth)] * len(models)
         self.nm, self.nr = len(models), len(self.dists)
         self.nw = int(grid["nw"])
         self.N = 2 * self.nw
@@ -302,6 +304,24 @@ class BatchPlan:
         check(_lib().grtm_batch_set_adaptive(self.ptr, kc.ctypes.data_as(P), wn.ctypes.data_as(P), int(stride),
                                              float(tol)), "grtm_batch_set_adaptive")
 
+    def set_learned(self, predictor, windows=None, stride=8, tol=3e-6):
+        """Learned adaptive grids of every model formed on the GPU inside the
+        captured graph (the network of `predictor` runs in every execution);
+        windows[m] km^-1 (default models.check_window of each model's depths).
+        predictor=None returns to the dense grids."""
+        if predictor is None:
+            check(_lib().grtm_batch_set_learned(self.ptr, None, 0.0, None, 1, 1.0), "grtm_batch_set_learned")
+            self._predictor = None
+            return
+        if windows is None:
+            windows = [models.check_window(zs, zr) for zs, zr in zip(self.src_depths, self.rcv_depths)]
+        wn = np.ascontiguousarray(windows, dtype=np.float64)
+        if wn.shape != (self.nm,):
+            raise ValueError("windows must have one entry per model")
+        check(_lib().grtm_batch_set_learned(self.ptr, predictor.mlp.ptr, float(predictor.lmargin),
+                                            wn.ctypes.data_as(P), int(stride), float(tol)), "grtm_batch_set_learned")
+        self._predictor = predictor  # the batch uses its network: keep it alive
+
     def adaptive_stats(self):
         o = np.zeros(4)
         check(_lib().grtm_batch_adaptive_stats(self.ptr, o.ctypes.data_as(P)), "grtm_batch_adaptive_stats")
@@ -317,16 +337,18 @@ class BatchPlan:
 
     def timing(self, reps=3):
         """ms: graph replay (mean of reps) and the recursion / contraction /
-        IFFT split of one eager pass."""
-        ms = np.zeros(4)
+        IFFT split of one eager pass, with the learned grids' formation
+        (``predictor``, part of the recursion stage)."""
+        ms = np.zeros(5)
         check(_lib().grtm_batch_timing(self.ptr, int(reps), ms.ctypes.data_as(P)), "grtm_batch_timing")
-        return dict(total=float(ms[0]), cells=float(ms[1]), ksum=float(ms[2]), ifft=float(ms[3]))
+        return dict(total=float(ms[0]), cells=float(ms[1]), ksum=float(ms[2]), ifft=float(ms[3]),
+                    predictor=float(ms[4]))
 
     def config(self):
-        o = np.zeros(7)
+        o = np.zeros(8)
         check(_lib().grtm_batch_config(self.ptr, o.ctypes.data_as(P)), "grtm_batch_config")
         return dict(models_per_group=int(o[0]), tile=int(o[1]), k_slice=int(o[2]), groups=int(o[3]),
-                    graph_nodes=int(o[4]), tuned=int(o[5]), tune_ms=float(o[6]))
+                    graph_nodes=int(o[4]), tuned=int(o[5]), tune_ms=float(o[6]), item_chunk=int(o[7]))
 
     def work(self):
         o = np.zeros(6)

@@ -2,6 +2,8 @@
 // grid, groups of models per launch).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "cellops.h"
 
 namespace grtm {
@@ -18,6 +20,10 @@ namespace grtm {
 // cells BS c .. BS c + BS - 1 (wavenumbers nk_w + stride j + stride / 2, j <
 // nchk[group row]) and takes their largest |value| to tail_max[group row].
 // BR: the receiver branch shared by every model of the launch's items.
+// dev_count (persistent form, learned grids formed on the device): the grid
+// is the resident blocks, each taking items [dev_count[which],
+// dev_count[which + 1]) `chunk` at a time from the counter *ctr (one counter
+// serialises its atomics: small tiles take several per visit).
 template <typename T, typename TO, bool COMPAT, int BS, int BR>
 __global__ __launch_bounds__(BS, sizeof(T) == 4 ? 4 : 1) void k_cells_zr_batch(const ModelDesc *__restrict__ md, int m0,
                                                        const LayerW<T> *__restrict__ lw_all, int nw, int nk,
@@ -29,46 +35,133 @@ __global__ __launch_bounds__(BS, sizeof(T) == 4 ? 4 : 1) void k_cells_zr_batch(c
                                                        const int *__restrict__ nk_w,
                                                        unsigned *__restrict__ row_peak,
                                                        const int *__restrict__ nchk, int stride,
-                                                       unsigned *__restrict__ tail_max)
+                                                       unsigned *__restrict__ tail_max,
+                                                       const unsigned *__restrict__ dev_count, int which,
+                                                       unsigned *__restrict__ ctr, int chunk)
 {
     __shared__ LayerW<T> lw[kMaxLayers];
-    const unsigned long long e = items[blockIdx.x];
-    const int j = int(e >> 32);
-    const unsigned lo = unsigned(e & 0xffffffffull);
-    const bool chk = (lo & kCheckItem) != 0;
-    const int iw = int((lo & ~kCheckItem) >> kTileBits), kb = int(lo & ((1u << kTileBits) - 1));
-    const int grow = j * nw + iw;  // row of the group
-    const int nkr = nk_w ? nk_w[grow] : nk;
-    const int lim = chk ? nchk[grow] : nkr;  // cells (check cells) of the row
-    if (kb * BS >= lim) return;  // uniform over the block
-    const Geo g = md[m0 + j].g;
-    const size_t off = md[m0 + j].lw_off + (size_t)iw * g.L;
-    for (int q = threadIdx.x; q < g.L; q += BS) lw[q] = lw_all[off + q];
-    __syncthreads();
-    const int jj = kb * BS + threadIdx.x;
-    const int ik = chk ? nkr + stride * jj + stride / 2 : jj;
-    unsigned mag = 0;
-    if (jj < lim && ik < nk) {
-        TO v[8];
-        const bool ok = cell_zr<T, TO, COMPAT, BR>(lw, g, iw, ik, dk, df, omega_i, v);
-        if (chk) {
-            float m = 0.f;
-            for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(float(v[i])));
-            mag = __float_as_uint(isfinite(m) ? m : 3.0e38f);  // a non-finite check value fails the row
+    __shared__ unsigned item_sh;
+    const unsigned lo_i = dev_count ? dev_count[which] : 0u, hi_i = dev_count ? dev_count[which + 1] : 0u;
+    unsigned base = 0;
+    for (int pass = 0;; pass++) {
+        unsigned idx;
+        if (dev_count) {
+            const int c = pass % chunk;
+            if (c == 0) {
+                __syncthreads();  // the previous item's reads of lw and item_sh are done
+                if (threadIdx.x == 0) item_sh = lo_i + atomicAdd(ctr, unsigned(chunk));
+                __syncthreads();
+                base = item_sh;
+            }
+            idx = base + unsigned(c);
+            if (idx >= hi_i) return;  // uniform over the block: the list is drained
+            if (c) __syncthreads();   // the previous item's reads of lw are done
         } else {
-            store_zr(v, grow, ik, A0, A1, lda);
-            mag = cell_mag_bits(v);
-            if (!ok && flag_count) {
-                const unsigned slot = atomicAdd(flag_count, 1u);
-                flag_list[slot] = unsigned(grow) * unsigned(nk) + unsigned(ik);
+            if (pass) return;
+            idx = blockIdx.x;
+        }
+        const unsigned long long e = items[idx];
+        const int j = int(e >> 32);
+        const unsigned lo = unsigned(e & 0xffffffffull);
+        const bool chk = (lo & kCheckItem) != 0;
+        const int iw = int((lo & ~kCheckItem) >> kTileBits), kb = int(lo & ((1u << kTileBits) - 1));
+        const int grow = j * nw + iw;  // row of the group
+        const int nkr = nk_w ? nk_w[grow] : nk;
+        const int lim = chk ? nchk[grow] : nkr;  // cells (check cells) of the row
+        if (kb * BS >= lim) continue;  // uniform over the block
+        const Geo g = md[m0 + j].g;
+        const size_t off = md[m0 + j].lw_off + (size_t)iw * g.L;
+        for (int q = threadIdx.x; q < g.L; q += BS) lw[q] = lw_all[off + q];
+        __syncthreads();
+        const int jj = kb * BS + threadIdx.x;
+        const int ik = chk ? nkr + stride * jj + stride / 2 : jj;
+        unsigned mag = 0;
+        if (jj < lim && ik < nk) {
+            TO v[8];
+            const bool ok = cell_zr<T, TO, COMPAT, BR>(lw, g, iw, ik, dk, df, omega_i, v);
+            if (chk) {
+                float m = 0.f;
+                for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(float(v[i])));
+                mag = __float_as_uint(isfinite(m) ? m : 3.0e38f);  // a non-finite check value fails the row
+            } else {
+                store_zr(v, grow, ik, A0, A1, lda);
+                mag = cell_mag_bits(v);
+                if (!ok && flag_count) {
+                    const unsigned slot = atomicAdd(flag_count, 1u);
+                    flag_list[slot] = unsigned(grow) * unsigned(nk) + unsigned(ik);
+                }
             }
         }
+        unsigned *dst = chk ? tail_max : row_peak;
+        if (dst) {
+            mag = wave_max_u32(mag);
+            if ((threadIdx.x & 63) == 0 && mag) atomicMax(dst + grow, mag);
+        }
     }
-    unsigned *dst = chk ? tail_max : row_peak;
-    if (dst) {
-        mag = wave_max_u32(mag);
-        if ((threadIdx.x & 63) == 0 && mag) atomicMax(dst + grow, mag);
+}
+
+// Items of a group's learned grids formed on the device.  The group's item
+// order is fixed by the host as a list of slots, each a group row with a
+// phase (kCheckItem: its residual-check items, else its tiles of BS
+// wavenumbers): k_batch_scan (one block of 1024 threads) gives every slot its
+// first item (an exclusive scan of the slots' item counts) and seg_items[s]
+// the first item of segment s, whose slots start at seg_slots[s] (nseg + 1
+// bounds); k_batch_fill (one thread per slot) writes the items (j << 32 |
+// row << kTileBits | tile, with kCheckItem on check items).
+__global__ __launch_bounds__(1024) void k_batch_scan(int nslots, const unsigned *__restrict__ slots, int nw, int bs,
+                                                     const int *__restrict__ nk_pred, const int *__restrict__ nchk,
+                                                     unsigned *__restrict__ slot_off, int nseg,
+                                                     const unsigned *__restrict__ seg_slots,
+                                                     unsigned *__restrict__ seg_items)
+{
+    __shared__ unsigned sh[1024];
+    const int t = threadIdx.x;
+    const int per = (nslots + 1023) / 1024, s0 = min(nslots, t * per), s1 = min(nslots, s0 + per);
+    auto count = [&](int q) {
+        const unsigned sl = slots[q];
+        const int grow = int(sl & ~kCheckItem);
+        const int n = (sl & kCheckItem) ? nchk[grow] : nk_pred[grow];
+        return unsigned((n + bs - 1) / bs);
+    };
+    unsigned sum = 0;
+    for (int q = s0; q < s1; q++) sum += count(q);
+    sh[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const unsigned a = t >= o ? sh[t - o] : 0u;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
     }
+    unsigned run = sh[t] - sum;
+    for (int q = s0; q < s1; q++) {
+        slot_off[q] = run;
+        run += count(q);
+    }
+    __syncthreads();  // slot_off written by the block is visible to it
+    // segment bounds: the first item of the segment's first slot (the total
+    // for a bound at nslots)
+    const unsigned total = sh[1023];
+    for (int q = t; q <= nseg; q += 1024) {
+        const unsigned b = seg_slots[q];
+        seg_items[q] = b < unsigned(nslots) ? slot_off[b] : total;
+    }
+}
+
+__global__ void k_batch_fill(int nslots, const unsigned *__restrict__ slots, int nw, int bs,
+                             const int *__restrict__ nk_pred, const int *__restrict__ nchk,
+                             const unsigned *__restrict__ slot_off, unsigned long long *__restrict__ items)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nslots) return;
+    const unsigned sl = slots[q];
+    const bool chk = (sl & kCheckItem) != 0;
+    const int grow = int(sl & ~kCheckItem), j = grow / nw, iw = grow - j * nw;
+    const int n = chk ? nchk[grow] : nk_pred[grow];
+    const unsigned nt = unsigned((n + bs - 1) / bs), base = slot_off[q];
+    const unsigned long long hi = (unsigned long long)j << 32;
+    for (unsigned k = 0; k < nt; k++)
+        items[base + k] = hi | ((unsigned(iw) << kTileBits) | k) | (chk ? kCheckItem : 0u);
 }
 
 // Rows of a group whose checked tail exceeds tol x row peak fall back to the
@@ -152,17 +245,49 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_fix_batch(const ModelDes
 }
 
 // ---------------------------------------------------------------- launchers
+// Blocks of a BS-thread kernel resident on the whole device, capped at ub.
+static int resident_blocks_bs(const void *kernel, int bs, int ub)
+{
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, bs, 0) != hipSuccess || per < 1)
+        per = 1, cus = cus > 0 ? cus : 256;
+    return std::max(1, std::min(ub, per * cus));
+}
+
+hipError_t launch_batch_items(int nslots, const unsigned *slots, int nw, int bs, const int *nk_pred, const int *nchk,
+                              unsigned *slot_off, int nseg, const unsigned *seg_slots, unsigned *seg_items,
+                              unsigned long long *items, hipStream_t st)
+{
+    if (nslots < 1 || nseg < 1 || nseg > 1023) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_batch_scan, dim3(1), dim3(1024), 0, st, nslots, slots, nw, bs, nk_pred, nchk, slot_off, nseg,
+                       seg_slots, seg_items);
+    hipLaunchKernelGGL(k_batch_fill, dim3((nslots + 255) / 256), dim3(256), 0, st, nslots, slots, nw, bs, nk_pred, nchk,
+                       slot_off, items);
+    return hipGetLastError();
+}
+
 hipError_t launch_cells_zr_batch(int prec, int compat, int bs, int br, const ModelDesc *md, int m0, const void *lw_all,
                                  int nw, int nk, double dk, double df, double omega_i,
                                  const unsigned long long *items, int nitems, void *A0, void *A1, long lda,
                                  unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
-                                 hipStream_t st, const int *nchk, int stride, unsigned *tail_max)
+                                 hipStream_t st, const int *nchk, int stride, unsigned *tail_max,
+                                 const unsigned *dev_count, int which, unsigned *ctr, int chunk)
 {
     if (nitems <= 0) return hipSuccess;
+    if (dev_count) {
+        if (!ctr || which < 0 || chunk < 1) return hipErrorInvalidValue;
+        hipError_t e = launch_zero_u32(ctr, 1, st);
+        if (e != hipSuccess) return e;
+    }
 #define GRTM_ZB(T, TO, CP, BSZ, BR, FC, FL)                                                                      \
-    hipLaunchKernelGGL((k_cells_zr_batch<T, TO, CP, BSZ, BR>), dim3(nitems), dim3(BSZ), 0, st, md, m0,          \
-                       (const LayerW<T> *)lw_all, nw, nk, dk, df, omega_i, items, (TO *)A0, (TO *)A1, lda, FC, FL, \
-                       nk_w, row_peak, nchk, stride, tail_max)
+    hipLaunchKernelGGL((k_cells_zr_batch<T, TO, CP, BSZ, BR>),                                                    \
+                       dim3(dev_count ? resident_blocks_bs((const void *)k_cells_zr_batch<T, TO, CP, BSZ, BR>, BSZ, \
+                                                           nitems)                                                 \
+                                      : nitems),                                                                 \
+                       dim3(BSZ), 0, st, md, m0, (const LayerW<T> *)lw_all, nw, nk, dk, df, omega_i, items, (TO *)A0, \
+                       (TO *)A1, lda, FC, FL, nk_w, row_peak, nchk, stride, tail_max, dev_count, which, ctr, chunk)
 #define GRTM_ZB_BR(T, TO, CP, BSZ, FC, FL)                           \
     do {                                                             \
         if (br == 1) GRTM_ZB(T, TO, CP, BSZ, 1, FC, FL);             \

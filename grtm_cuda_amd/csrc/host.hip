@@ -769,8 +769,8 @@ struct grtm_plan {
     DBuf tiles;             // recursion work items of the predicted grid, ordered by row
     int ntiles = 0, ntiles64 = 0;  // all items; items of rows [0, rows64)
     // learned grid built on the device at every execution (grtm_plan_set_learned):
-    // the network, the model's predictor inputs, the device item counts
-    // [rows < rows64, all] and the persistent launches' item counters
+    // the network, the model's predictor inputs, the device item bounds
+    // {0, end of rows < rows64, end} and the persistent launches' item counters
     grtm_mlp *net = nullptr;
     grtm::LearnGrid lg{};
     grtm::LearnModel lm{};
@@ -1546,7 +1546,7 @@ int grtm_plan_set_learned(grtm_plan *p, grtm_mlp *net, double lmargin, double wi
     if (p->tiles.n < sizeof(unsigned) * (size_t)cap && (st = p->tiles.alloc(sizeof(unsigned) * (size_t)cap)))
         return st;
     if (!p->lmodel.p && ((st = p->lmodel.alloc(sizeof(grtm::LearnModel))) ||
-                         (st = p->icounts.alloc(2 * sizeof(unsigned))) || (st = p->ictr.alloc(2 * sizeof(unsigned))) ||
+                         (st = p->icounts.alloc(3 * sizeof(unsigned))) || (st = p->ictr.alloc(2 * sizeof(unsigned))) ||
                          (st = p->ioff.alloc(2 * sizeof(unsigned) * p->nw))))
         return st;
     if (p->gemm_mode == 1) {
@@ -2026,6 +2026,24 @@ struct grtm_batch {
     std::vector<int> nk_pred_h, nchk_h, max_chk;
     DBuf nk_pred, nk_w, nchk, row_peak, tail_max, fb_count, fb_list, kend;
     DBuf fb_gcount;  // fallback rows listed per group (one counter per group)
+    // learned grids formed on the device inside the graph (grtm_batch_set_learned):
+    // the network, every model's predictor inputs, and per group its item
+    // slots (group row | kCheckItem), the slots' first items and the
+    // segments' item bounds (segment = pass * 4 + branch - 1)
+    grtm_mlp *net = nullptr;
+    grtm::LearnGrid lg{};
+    DBuf lmodels, lslots, lslot_off, lseg_slots, lseg_items, lctr, litems;
+    hipEvent_t ev_pred0 = nullptr, ev_pred1 = nullptr;  // around a group's grid formation (timing)
+    std::vector<size_t> lslot0;     // per group: first slot in lslots
+    std::vector<int> lnslots;       // per group: slots
+    std::vector<unsigned> lseg_h;   // per group: 9 segment slot bounds
+    int litems_cap = 0;             // item capacity of one group
+    int lchunk = 1;                 // items a persistent block takes per visit of its counter (autotuned)
+    bool pred_stale = false;
+    // the models, kept for the predictor's inputs
+    std::vector<std::vector<double>> layers_h;
+    std::vector<double> src_h, rcv_h;
+    std::vector<int> rows64_rule;
     ~grtm_batch()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -2100,9 +2118,38 @@ int batch_items(grtm_batch *b, int bs, int g0, int g1)
 
 int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
 {
-    const unsigned long long *it = b->items.as<unsigned long long>();
     const grtm::ModelDesc *md = b->md.as<grtm::ModelDesc>();
     const size_t r0 = (size_t)g.m0 * b->nw, nrow = (size_t)g.n * b->nw;
+    const int gi = int(&g - b->groups.data());
+    const bool learned = b->adaptive && b->net;
+    // a learned grid: the group's network rows, cuts and items, formed here
+    const unsigned long long *it = learned ? b->litems.as<unsigned long long>() : b->items.as<unsigned long long>();
+    const unsigned *seg = learned ? b->lseg_items.as<unsigned>() + 9 * gi : nullptr;
+    if (learned) {
+        int st;
+        if (b->ev_pred0) HIPCHK(hipEventRecord(b->ev_pred0, s));
+        if ((st = grtm::learned_grid(b->net, b->lmodels.as<grtm::LearnModel>() + g.m0, g.n, b->lg,
+                                     b->nk_pred.as<int>() + r0, b->nchk.as<int>() + r0, s)))
+            return st;
+        HIPCHK(grtm::launch_batch_items(b->lnslots[gi], b->lslots.as<unsigned>() + b->lslot0[gi], b->nw, b->bs,
+                                        b->nk_pred.as<int>() + r0, b->nchk.as<int>() + r0, b->lslot_off.as<unsigned>(),
+                                        8, b->lseg_slots.as<unsigned>() + 9 * gi, b->lseg_items.as<unsigned>() + 9 * gi,
+                                        b->litems.as<unsigned long long>(), s));
+        if (b->ev_pred1) HIPCHK(hipEventRecord(b->ev_pred1, s));
+        b->pred_stale = true;
+    }
+    // item range of (pass, branch): a host list's range, or a device-built segment
+    auto items_of = [&](int pass, int br, const unsigned long long *&p, int &n, const unsigned *&dc, int &which,
+                        unsigned *&ctr) {
+        const int sg = pass * 4 + br - 1;
+        if (learned) {
+            const unsigned *bounds = b->lseg_h.data() + 9 * gi;
+            p = it, n = bounds[sg + 1] > bounds[sg] ? b->litems_cap : 0, dc = seg, which = sg;
+            ctr = b->lctr.as<unsigned>() + sg;
+        } else {
+            p = it + g.off[pass][br - 1], n = (int)g.cnt[pass][br - 1], dc = nullptr, which = 0, ctr = nullptr;
+        }
+    };
     int *nkw = b->adaptive ? b->nk_w.as<int>() + r0 : nullptr;
     unsigned *rowpk = b->adaptive ? b->row_peak.as<unsigned>() + r0 : nullptr;
     if (b->adaptive) {
@@ -2114,28 +2161,38 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
     // learned grids: the item lists carry the residual-check cells (FP64 on the FP64 rows)
     const int *nchk = b->adaptive ? b->nchk.as<int>() + r0 : nullptr;
     unsigned *tailmx = b->adaptive ? b->tail_max.as<unsigned>() + r0 : nullptr;
+    const unsigned long long *ip;
+    const unsigned *dc;
+    unsigned *ctr;
+    int ni, which;
     if (!batch_mixed(b)) {
-        for (int br = 1; br <= 4; br++)
-            if (g.cnt[0][br - 1])
+        for (int br = 1; br <= 4; br++) {
+            items_of(0, br, ip, ni, dc, which, ctr);
+            if (ni)
                 HIPCHK(grtm::launch_cells_zr_batch(0, b->compat, b->bs, br, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk,
-                                                   b->df, b->omega_i, it + g.off[0][br - 1], (int)g.cnt[0][br - 1],
-                                                   b->A0.p, b->A1.p, b->nk, nullptr, nullptr, nkw, rowpk, s, nchk,
-                                                   b->chk_stride, tailmx));
+                                                   b->df, b->omega_i, ip, ni, b->A0.p, b->A1.p, b->nk, nullptr,
+                                                   nullptr, nkw, rowpk, s, nchk, b->chk_stride, tailmx, dc, which,
+                                                   ctr, b->lchunk));
+        }
     } else {
         unsigned *cnt = b->flags.as<unsigned>(), *list = cnt + 1;
         HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
-        for (int br = 1; br <= 4; br++)
-            if (g.cnt[0][br - 1])
+        for (int br = 1; br <= 4; br++) {
+            items_of(0, br, ip, ni, dc, which, ctr);
+            if (ni)
                 HIPCHK(grtm::launch_cells_zr_batch(2, b->compat, b->bs, br, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk,
-                                                   b->df, b->omega_i, it + g.off[0][br - 1], (int)g.cnt[0][br - 1],
-                                                   b->A0.p, b->A1.p, b->nk, nullptr, nullptr, nkw, rowpk, s, nchk,
-                                                   b->chk_stride, tailmx));
-        for (int br = 1; br <= 4; br++)
-            if (g.cnt[1][br - 1])
+                                                   b->df, b->omega_i, ip, ni, b->A0.p, b->A1.p, b->nk, nullptr,
+                                                   nullptr, nkw, rowpk, s, nchk, b->chk_stride, tailmx, dc, which,
+                                                   ctr, b->lchunk));
+        }
+        for (int br = 1; br <= 4; br++) {
+            items_of(1, br, ip, ni, dc, which, ctr);
+            if (ni)
                 HIPCHK(grtm::launch_cells_zr_batch(1, b->compat, b->bs, br, md, g.m0, b->lw32.p, b->nw, b->nk, b->dk,
-                                                   b->df, b->omega_i, it + g.off[1][br - 1], (int)g.cnt[1][br - 1],
-                                                   b->A0.p, b->A1.p, b->nk, cnt, list, nkw, rowpk, s, nchk,
-                                                   b->chk_stride, tailmx));
+                                                   b->df, b->omega_i, ip, ni, b->A0.p, b->A1.p, b->nk, cnt, list, nkw,
+                                                   rowpk, s, nchk, b->chk_stride, tailmx, dc, which, ctr,
+                                                   b->lchunk));
+        }
     }
     unsigned *cnt = batch_mixed(b) ? b->flags.as<unsigned>() : nullptr, *list = cnt ? cnt + 1 : nullptr;
     if (!b->adaptive) {
@@ -2146,7 +2203,6 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
     }
     // dense fallback of the group's rows whose check failed (each row in its
     // own precision), then the group's tile extents
-    const int gi = int(&g - b->groups.data());
     unsigned *gcount = b->fb_gcount.as<unsigned>() + gi;
     HIPCHK(grtm::launch_zero_u32(gcount, 1, s));
     HIPCHK(grtm::launch_fallback_batch(batch_mixed(b) ? 1 : 0, b->compat, md, g.m0, b->lw64.p, b->lw32.p, (int)nrow,
@@ -2328,10 +2384,14 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
         ModelTables mt;
         const grtm_model &gm = models[m];
         if ((st = model_tables(gm.layers, gm.n_layers, gm.src_depth, gm.rcv_depth, compat, b->nw, b->df, b->omega_i,
-                               precision != GRTM_PREC_FP64, mt)))
+                               true, mt)))
             return st;
         b->L[m] = mt.st.L;
         b->rows64[m] = mt.rows64;
+        b->layers_h.emplace_back(gm.layers, gm.layers + 6 * (size_t)gm.n_layers);
+        b->src_h.push_back(gm.src_depth);
+        b->rcv_h.push_back(gm.rcv_depth);
+        b->rows64_rule.push_back(mt.rows64);
         b->md_h[m].g = mt.g;
         b->md_h[m].lw_off = (long)all64.size();
         b->md_h[m].r64 = precision != GRTM_PREC_FP64 ? mt.rows64 : b->nw;
@@ -2404,6 +2464,8 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
 int grtm_batch_set_adaptive(grtm_batch *b, const double *kcut, const double *window, int stride, double tol)
 {
     if (!b) return GRTM_E_INPUT;
+    b->net = nullptr;  // host cuts replace learned grids
+    b->pred_stale = false;
     int st;
     if (!kcut) {
         b->adaptive = false;
@@ -2460,11 +2522,170 @@ int grtm_batch_set_adaptive(grtm_batch *b, const double *kcut, const double *win
     return batch_capture(b);
 }
 
+// The device-built grids of the last execution into nk_pred_h / nchk_h.
+static int batch_pred_to_host(grtm_batch *b)
+{
+    if (!b->net || !b->pred_stale) return GRTM_OK;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const size_t n = (size_t)b->nm * b->nw;
+    HIPCHK(hipMemcpy(b->nk_pred_h.data(), b->nk_pred.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(b->nchk_h.data(), b->nchk.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    b->pred_stale = false;
+    return GRTM_OK;
+}
+
+int grtm_batch_set_learned(grtm_batch *b, grtm_mlp *net, double lmargin, const double *window, int stride, double tol)
+{
+    if (!b) return GRTM_E_INPUT;
+    if (!net) return grtm_batch_set_adaptive(b, nullptr, nullptr, 1, 1.0);
+    if (!b->pair || b->gemm_mode == 2) return GRTM_E_UNSUPPORTED;
+    if (!window || stride < 1 || !(tol > 0) || !std::isfinite(lmargin)) return GRTM_E_INPUT;
+    for (int m = 0; m < b->nm; m++) {
+        if (!(window[m] >= 0)) return GRTM_E_INPUT;
+        if ((int)b->layers_h[m].size() / 6 > grtm::kLearnSlots) return GRTM_E_UNSUPPORTED;
+    }
+    int st;
+    const size_t n = (size_t)b->nm * b->nw;
+    if (!b->nk_pred.p) {
+        if ((st = b->nk_pred.alloc(sizeof(int) * n)) || (st = b->nk_w.alloc(sizeof(int) * n)) ||
+            (st = b->nchk.alloc(sizeof(int) * n)) || (st = b->row_peak.alloc(sizeof(unsigned) * n)) ||
+            (st = b->tail_max.alloc(sizeof(unsigned) * n)) || (st = b->fb_count.alloc(sizeof(unsigned) * b->nm)) ||
+            (st = b->fb_list.alloc(sizeof(unsigned) * 2 * n)) ||
+            (st = b->fb_gcount.alloc(sizeof(unsigned) * b->groups.size())) ||
+            (st = b->kend.alloc(sizeof(int) * b->groups.size() * batch_kend_per_group(b))))
+            return st;
+    }
+    if (b->gemm_mode == 1) {
+        int gmax = 0;
+        for (const auto &gr : b->groups) gmax = std::max(gmax, gr.n);
+        const int M2[2] = {4 * b->nw * gmax, 4 * b->nw * gmax};
+        const size_t need = grtm::gemm_streamk_workspace_bytes(2, M2, b->nr, b->nk);
+        if (need > b->ws.n && (st = b->ws.alloc(need))) return st;
+    }
+    // predictor inputs of every model (the floor follows each model's FP64 rows)
+    std::vector<grtm::LearnModel> lm(b->nm);
+    for (int m = 0; m < b->nm; m++) {
+        grtm::LearnModel &d = lm[m];
+        d = grtm::LearnModel{};
+        d.n = (int)b->layers_h[m].size() / 6;
+        d.vs_min = 1e30;
+        for (int j = 0; j < d.n; j++) {
+            for (int c = 0; c < 6; c++) d.rows[j][c] = b->layers_h[m][(size_t)6 * j + c];
+            d.vs_min = std::min(d.vs_min, d.rows[j][2]);
+        }
+        d.floor_rows = batch_mixed(b) ? b->rows64[m] : b->rows64_rule[m];
+        d.src = b->src_h[m];
+        d.rcv = b->rcv_h[m];
+        d.window = window[m];
+    }
+    // each group's slot order: per (pass, branch) its models heaviest first,
+    // their rows' tiles, then their rows' check cells (the host lists' order)
+    const int ng = (int)b->groups.size();
+    std::vector<unsigned> slots, segs(9 * (size_t)ng);
+    b->lslot0.assign(ng, 0);
+    b->lnslots.assign(ng, 0);
+    int maxslots = 0;
+    for (int gi = 0; gi < ng; gi++) {
+        const grtm_batch::Group &g = b->groups[gi];
+        b->lslot0[gi] = slots.size();
+        std::vector<int> order(g.n);
+        for (int j = 0; j < g.n; j++) order[j] = j;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->L[g.m0 + x] > b->L[g.m0 + y]; });
+        for (int pass = 0; pass < 2; pass++)
+            for (int br = 1; br <= 4; br++) {
+                segs[9 * (size_t)gi + pass * 4 + br - 1] = unsigned(slots.size() - b->lslot0[gi]);
+                for (int phase = 0; phase < 2; phase++)
+                    for (int j : order) {
+                        if (b->md_h[g.m0 + j].g.branch != br) continue;
+                        const int r64 = batch_mixed(b) ? b->rows64[g.m0 + j] : b->nw;
+                        const int lo = pass == 0 ? 0 : r64, hi = pass == 0 ? r64 : b->nw;
+                        for (int iw = lo; iw < hi; iw++)
+                            slots.push_back(unsigned(j * b->nw + iw) | (phase ? grtm::kCheckItem : 0u));
+                    }
+            }
+        b->lnslots[gi] = int(slots.size() - b->lslot0[gi]);
+        segs[9 * (size_t)gi + 8] = unsigned(b->lnslots[gi]);
+        maxslots = std::max(maxslots, b->lnslots[gi]);
+    }
+    const int tpb = (b->nk + b->bs - 1) / b->bs;  // items of one slot at most
+    const size_t cap = (size_t)maxslots * tpb;
+    if (cap > 0x7fffffffu) return GRTM_E_UNSUPPORTED;
+    if ((st = b->lmodels.alloc(sizeof(grtm::LearnModel) * b->nm)) ||
+        (st = b->lslots.alloc(sizeof(unsigned) * std::max<size_t>(1, slots.size()))) ||
+        (st = b->lslot_off.alloc(sizeof(unsigned) * std::max(1, maxslots))) ||
+        (st = b->lseg_slots.alloc(sizeof(unsigned) * segs.size())) ||
+        (st = b->lseg_items.alloc(sizeof(unsigned) * segs.size())) || (st = b->lctr.alloc(sizeof(unsigned) * 8)) ||
+        (st = b->litems.alloc(sizeof(unsigned long long) * std::max<size_t>(1, cap))))
+        return st;
+    HIPCHK(hipMemcpy(b->lmodels.p, lm.data(), sizeof(grtm::LearnModel) * b->nm, hipMemcpyHostToDevice));
+    if (!slots.empty()) HIPCHK(hipMemcpy(b->lslots.p, slots.data(), sizeof(unsigned) * slots.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->lseg_slots.p, segs.data(), sizeof(unsigned) * segs.size(), hipMemcpyHostToDevice));
+    b->lseg_h = segs;
+    b->litems_cap = (int)cap;
+    b->lg = grtm::LearnGrid{b->nw, b->nk, stride, 0, b->pp.dt, b->dk, b->dk * b->nk, lmargin};
+    b->chk_stride = stride;
+    b->chk_tol = tol;
+    b->net = net;
+    b->adaptive = true;
+    b->nk_pred_h.assign(n, 0);
+    b->nchk_h.assign(n, 0);
+    b->max_chk.assign(b->nm, 0);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    // autotune: the items a persistent block takes per counter visit, timed
+    // on the first group's recursion stage
+    {
+        const int cand[5] = {1, 2, 4, 8, 16};
+        double best = 1e30;
+        int bc = 1;
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        for (int c : cand) {
+            b->lchunk = c;
+            for (int rep = 0; rep < 2 && !st; rep++) {  // the first run warms the kernels up
+                HIPCHK(hipEventRecord(e0, b->stream));
+                st = batch_cells(b, b->groups[0], b->stream);
+                HIPCHK(hipEventRecord(e1, b->stream));
+                HIPCHK(hipEventSynchronize(e1));
+            }
+            if (st) break;
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            b->tuned++;
+            b->tune_ms += ms;
+            if (ms < best) best = ms, bc = c;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        b->lchunk = bc;
+        if (st) {
+            b->net = nullptr;
+            b->adaptive = false;
+            return st;
+        }
+    }
+    // one eager pass of every group (the host's view of the grids; kernel
+    // attributes set outside the capture), then the capture
+    for (const grtm_batch::Group &g : b->groups)
+        if ((st = batch_group(b, g, b->stream))) {
+            b->net = nullptr;
+            b->adaptive = false;
+            return st;
+        }
+    if ((st = batch_pred_to_host(b))) return st;
+    return batch_capture(b);
+}
+
 int grtm_batch_adaptive_stats(grtm_batch *b, double *out)
 {
     if (!b || !out) return GRTM_E_INPUT;
     out[0] = out[1] = out[2] = out[3] = 0;
     if (!b->adaptive) return GRTM_OK;
+    int st;
+    if ((st = batch_pred_to_host(b))) return st;
     HIPCHK(hipStreamSynchronize(b->stream));
     const size_t n = (size_t)b->nm * b->nw;
     std::vector<int> nkw(n);
@@ -2518,12 +2739,16 @@ int grtm_batch_timing(grtm_batch *b, int reps, double *ms)
     if (st) return st;
     // stage split of one eager pass: events around every group's stages
     const size_t ng = b->groups.size();
-    std::vector<hipEvent_t> ev(4 * ng, nullptr);
+    const bool learned = b->adaptive && b->net;
+    std::vector<hipEvent_t> ev(6 * ng, nullptr);
     for (auto &e : ev) HIPCHK(hipEventCreate(&e));
     for (size_t i = 0; i < ng && !st; i++) {
         const grtm_batch::Group &g = b->groups[i];
         HIPCHK(hipEventRecord(ev[4 * i], b->stream));
-        if ((st = batch_cells(b, g, b->stream))) break;
+        if (learned) b->ev_pred0 = ev[4 * ng + 2 * i], b->ev_pred1 = ev[4 * ng + 2 * i + 1];
+        st = batch_cells(b, g, b->stream);
+        b->ev_pred0 = b->ev_pred1 = nullptr;
+        if (st) break;
         HIPCHK(hipEventRecord(ev[4 * i + 1], b->stream));
         if ((st = batch_gemm(b, g, b->stream))) break;
         HIPCHK(hipEventRecord(ev[4 * i + 2], b->stream));
@@ -2535,13 +2760,14 @@ int grtm_batch_timing(grtm_batch *b, int reps, double *ms)
     }
     if (!st) {
         HIPCHK(hipStreamSynchronize(b->stream));
-        ms[1] = ms[2] = ms[3] = 0;
+        ms[1] = ms[2] = ms[3] = ms[4] = 0;
         for (size_t i = 0; i < ng; i++) {
-            float a = 0, c = 0, d = 0;
+            float a = 0, c = 0, d = 0, f = 0;
             HIPCHK(hipEventElapsedTime(&a, ev[4 * i], ev[4 * i + 1]));
             HIPCHK(hipEventElapsedTime(&c, ev[4 * i + 1], ev[4 * i + 2]));
             HIPCHK(hipEventElapsedTime(&d, ev[4 * i + 2], ev[4 * i + 3]));
-            ms[1] += a, ms[2] += c, ms[3] += d;
+            if (learned) HIPCHK(hipEventElapsedTime(&f, ev[4 * ng + 2 * i], ev[4 * ng + 2 * i + 1]));
+            ms[1] += a, ms[2] += c, ms[3] += d, ms[4] += f;
         }
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
@@ -2558,12 +2784,17 @@ int grtm_batch_config(grtm_batch *b, double *out)
     out[4] = (double)b->nodes;
     out[5] = b->tuned;
     out[6] = b->tune_ms;
+    out[7] = b->lchunk;
     return GRTM_OK;
 }
 
 int grtm_batch_work(grtm_batch *b, double *out)
 {
     if (!b || !out) return GRTM_E_INPUT;
+    {
+        const int st = batch_pred_to_host(b);
+        if (st) return st;
+    }
     double cl = 0;
     for (int m = 0; m < b->nm; m++) cl += b->L[m];
     const double cells = (double)b->nw * b->nk;

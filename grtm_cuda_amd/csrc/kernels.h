@@ -65,8 +65,8 @@ unsigned cells_tile(int iw, int kb);
 constexpr unsigned kCheckItem = 1u << 31;
 int cells_tile_limit(int nw, int nk);  // 1 if (nw, nk) fit the item encoding
 // dev_count (optional): the item list was built on the device and its length
-// is only known there -- items [which ? dev_count[0] : 0, dev_count[which])
-// of `tiles`, taken by a persistent grid from the counter *ctr (zeroed by the
+// is only known there -- items [dev_count[which], dev_count[which + 1]) of
+// `tiles`, taken by a persistent grid from the counter *ctr (zeroed by the
 // launcher); ntiles is then an upper bound on the item count.
 hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
                            double df, double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles, void *A0,
@@ -97,8 +97,8 @@ int learned_grid(grtm_mlp *net, const LearnModel *models, int n_models, const Le
                  int *nchk, hipStream_t st);
 // Recursion items of one model's learned grid from nk_pred / nchk: tiles of
 // rows [0, r64), their residual-check items, tiles of rows [r64, nw), theirs;
-// counts[0] = items of rows [0, r64), counts[1] = all items.  off: 2 nw
-// scratch words (each row's first tile and first check item).
+// counts = {0, items of rows [0, r64), all items}.  off: 2 nw scratch words
+// (each row's first tile and first check item).
 hipError_t launch_grid_items(int nw, int r64, const int *nk_pred, const int *nchk, unsigned *items, unsigned *off,
                              unsigned *counts, hipStream_t st);
 // adaptive grid: dense fallback, tile extents
@@ -199,12 +199,23 @@ struct ModelDesc {
 // Recursion of a group of models; items (j << 32) | cells_tile(row, tile)
 // with tiles of bs in {64, 128, 256} wavenumbers; prec as launch_cells_zr;
 // every item's model has receiver branch br (1-4).
+// dev_count: persistent form over items [dev_count[which], dev_count[which + 1])
+// of a list built on the device (nitems: an upper bound), taken `chunk` at a
+// time from counter *ctr.
 hipError_t launch_cells_zr_batch(int prec, int compat, int bs, int br, const ModelDesc *md, int m0, const void *lw_all,
                                  int nw, int nk, double dk, double df, double omega_i,
                                  const unsigned long long *items, int nitems, void *A0, void *A1, long lda,
                                  unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
                                  hipStream_t st, const int *nchk = nullptr, int stride = 0,
-                                 unsigned *tail_max = nullptr);
+                                 unsigned *tail_max = nullptr, const unsigned *dev_count = nullptr, int which = 0,
+                                 unsigned *ctr = nullptr, int chunk = 1);
+// A group's learned-grid items from its slot order (batch.hip k_batch_scan,
+// k_batch_fill): slots[q] = group row | kCheckItem for its check items;
+// segment s covers slots [seg_slots[s], seg_slots[s + 1]) and gets items
+// [seg_items[s], seg_items[s + 1]); slot_off: nslots scratch words.
+hipError_t launch_batch_items(int nslots, const unsigned *slots, int nw, int bs, const int *nk_pred, const int *nchk,
+                              unsigned *slot_off, int nseg, const unsigned *seg_slots, unsigned *seg_items,
+                              unsigned long long *items, hipStream_t st);
 // Dense fallback of a group's rows whose residual check failed (learned
 // grids): one selection over the group's nrow rows (list: group rows, then
 // their first wavenumbers; count: the group's counter; model_count: per

@@ -37,8 +37,8 @@ namespace grtm {
 // BR: the plan's receiver branch, fixed at compile time.
 // dev_count (persistent form): the item list was built on the device; the
 // grid is sized to the resident blocks, and each block takes items
-// [lo, hi) = [which ? dev_count[0] : 0, dev_count[which]) one at a time from
-// the counter *ctr until the list is drained.
+// [lo, hi) = [dev_count[which], dev_count[which + 1]) one at a time from the
+// counter *ctr until the list is drained.
 template <typename T, typename TO, bool COMPAT, int BR>
 __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__restrict__ lw_all, int L,
                                                            Geo g, int nk, double dk, double df,
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__re
 {
     __shared__ LayerW<T> lw[kMaxLayers];
     __shared__ unsigned item_sh;
-    const unsigned lo = dev_count && which ? dev_count[0] : 0u, hi = dev_count ? dev_count[which] : 0u;
+    const unsigned lo = dev_count ? dev_count[which] : 0u, hi = dev_count ? dev_count[which + 1] : 0u;
     for (int pass = 0;; pass++) {
         unsigned idx;
         if (dev_count) {
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__re
 // each row's first tile item and first check item in the list order of the
 // host (tiles of rows [0, r64) in row order, then their residual-check items,
 // then the tiles of rows [r64, nw), then theirs): off[iw], off[nw + iw]; and
-// counts[0] = items of rows [0, r64), counts[1] = all items.  k_grid_fill
+// counts = {0, items of rows [0, r64), all items}.  k_grid_fill
 // (one wave per row) writes the items.
 __global__ __launch_bounds__(1024) void k_grid_scan(int nw, int r64, const int *__restrict__ nk_pred,
                                                     const int *__restrict__ nchk, unsigned *__restrict__ off,
@@ -166,8 +166,9 @@ __global__ __launch_bounds__(1024) void k_grid_scan(int nw, int r64, const int *
         pc += nc[q];
     }
     if (t == 0) {
-        counts[0] = T64 + C64;
-        counts[1] = T + Cn;
+        counts[0] = 0u;
+        counts[1] = T64 + C64;
+        counts[2] = T + Cn;
     }
 }
 
@@ -735,7 +736,7 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
     if (!tiles) ntiles = nwr * tpr;
     if (ntiles <= 0) return hipSuccess;
     if (dev_count) {
-        if (!tiles || !ctr || which < 0 || which > 1) return hipErrorInvalidValue;
+        if (!tiles || !ctr || which < 0) return hipErrorInvalidValue;
         hipError_t e = launch_zero_u32(ctr, 1, st);
         if (e != hipSuccess) return e;
     }

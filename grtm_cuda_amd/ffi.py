@@ -112,6 +112,7 @@ def _lib():
         lib.grtm_batch_config.argtypes = [P, P]
         lib.grtm_batch_work.argtypes = [P, P]
         lib.grtm_batch_set_adaptive.argtypes = [P, P, P, I, D]
+        lib.grtm_batch_set_learned.argtypes = [P, P, D, P, I, D]
         lib.grtm_batch_adaptive_stats.argtypes = [P, P]
         lib.grtm_batch_destroy.argtypes = [P]
         lib.grtm_batch_destroy.restype = None
@@ -132,7 +133,7 @@ EXPORTS = [
     "grtm_mlp_get_params", "grtm_mlp_set_data", "grtm_mlp_train", "grtm_mlp_grads", "grtm_mlp_predict", "grtm_mlp_predict_train_forward",
     "grtm_mlp_set_loss", "grtm_batch_create", "grtm_batch_execute", "grtm_batch_execute_eager", "grtm_batch_sync",
     "grtm_batch_traces_out", "grtm_batch_timing", "grtm_batch_config", "grtm_batch_work", "grtm_batch_destroy",
-    "grtm_batch_set_adaptive", "grtm_batch_adaptive_stats",
+    "grtm_batch_set_adaptive", "grtm_batch_set_learned", "grtm_batch_adaptive_stats",
 ]
 
 

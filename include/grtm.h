@@ -229,11 +229,14 @@ int grtm_batch_sync(grtm_batch *batch);
 int grtm_batch_traces_out(grtm_batch *batch, int m0, int m1, double *traces);
 /* ms[0]: mean of reps graph replays (HIP events on the batch's stream);
  * ms[1..3]: recursion, contraction and inverse FFT of one eager pass (sums
- * over the groups). */
+ * over the groups); ms[4]: the learned grids' formation in that pass
+ * (grtm_batch_set_learned; part of ms[1], 0 otherwise).  5 doubles. */
 int grtm_batch_timing(grtm_batch *batch, int reps, double *ms);
 /* out[0] models per group, [1] recursion tile (wavenumbers per block),
  * [2] contraction K slice (0: split-K plan of the FP64 / BF16x3 kernels),
- * [3] groups, [4] graph nodes, [5] autotune candidates timed, [6] autotune ms. */
+ * [3] groups, [4] graph nodes, [5] autotune candidates timed, [6] autotune ms,
+ * [7] items a persistent recursion block takes at a time (learned grids
+ * formed on the device).  8 doubles. */
 int grtm_batch_config(grtm_batch *batch, double *out);
 /* out[0] cells, [1] cell-layers, [2] contraction flops, [3] traces,
  * [4] recursion flops of the dense grid with full upward sweeps (operation
@@ -334,6 +337,13 @@ int grtm_mlp_predict_train_forward(grtm_mlp *mlp, const float *X, int n, float *
  * outlive it or be detached with net = NULL (which returns the plan to the
  * dense grid). */
 int grtm_plan_set_learned(grtm_plan *plan, grtm_mlp *net, double lmargin, double window, int stride, double tol);
+/* The same for every model of a batch, inside its captured graph: per group,
+ * one forward of the network over the group's (model, frequency) rows, the
+ * cuts and check counts, the group's item list, persistent recursion
+ * launches.  window[m] per model; batches with the streamed contraction
+ * (FP64, FP32).  The graph is captured again. */
+int grtm_batch_set_learned(grtm_batch *batch, grtm_mlp *net, double lmargin, const double *window, int stride,
+                           double tol);
 
 #ifdef __cplusplus
 }

@@ -88,3 +88,58 @@ def test_device_learned_traces(c2_plans):
     # every execution rebuilds the grid: a second run is identical
     again = dev.traces()
     assert np.array_equal(again, tr_dev)
+
+
+def _batch_host_cuts(pred, ms, grid):
+    kc = np.stack([pred.kcut(l, zs, 0.0, grid) for l, zs in ms])
+    win = np.array([models.check_window(zs, 0.0) for _, zs in ms])
+    return kc, win
+
+
+@pytest.mark.parametrize("prec", [g.PREC_FP64, g.PREC_FP32])
+def test_batch_device_grids_match_host_grids(pred, prec):
+    """A batch's learned grids formed inside its graph against the same
+    network's cuts decoded on the host: the same cells, fallbacks and traces."""
+    ms = models.batch_models(83, 6, n_min=4, n_max=14, src_lo=2.0, src_hi=30.0)
+    vs_min = min(float(l[:, 2].min()) for l, _ in ms)
+    grid = models.dense_grid(256, 0.05, 2048, vs_min)
+    dists = np.linspace(5.0, 150.0, 8)
+    print(f"gpu: batch of {len(ms)} models, device learned grids, precision {prec}", flush=True)
+    dev = g.BatchPlan(ms, 0.0, dists, grid, precision=prec)
+    dev.set_learned(pred, stride=8, tol=3e-6)
+    dev.execute()
+    tr_dev = dev.traces()
+    host = g.BatchPlan(ms, 0.0, dists, grid, precision=prec)
+    kc, win = _batch_host_cuts(pred, ms, grid)
+    host.set_adaptive(kc, win, stride=8, tol=3e-6)
+    host.execute()
+    tr_host = host.traces()
+    sd, sh = dev.adaptive_stats(), host.adaptive_stats()
+    print("device", sd, "host", sh)
+    assert sd == sh
+    d = np.linalg.norm(tr_dev - tr_host, axis=-1) / np.linalg.norm(tr_host, axis=-1)
+    assert d.max() < 1e-6
+
+
+def test_batch_device_grids_graph_replays(pred):
+    """Two groups of device-formed learned grids: graph replays equal the
+    eager pass bit for bit and stay within 1e-4 of the dense FP64 batch."""
+    ms = models.batch_models(92, 70, n_min=4, n_max=9, src_lo=3.0, src_hi=20.0)
+    vs_min = min(float(l[:, 2].min()) for l, _ in ms)
+    grid = models.dense_grid(64, 0.1, 512, vs_min)
+    dists = np.linspace(5.0, 60.0, 4)
+    print("gpu: batch of 70 models, device learned grids, graph replays", flush=True)
+    bp = g.BatchPlan(ms, 0.0, dists, grid, precision=g.PREC_FP32)
+    bp.set_learned(pred, stride=8, tol=3e-6)
+    bp.execute_eager()
+    eager = bp.traces()
+    for rep in range(3):
+        bp.execute()
+        bp.sync()
+        assert np.array_equal(bp.traces(), eager), rep
+        bp.execute_eager()
+        bp.sync()
+    ref = g.BatchPlan(ms, 0.0, dists, grid, precision=g.PREC_FP64)
+    ref.execute()
+    rl = np.linalg.norm(eager - ref.traces(), axis=-1) / np.linalg.norm(ref.traces(), axis=-1)
+    assert rl.max() < 1e-4
