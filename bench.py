@@ -370,6 +370,7 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=64, help="CPU baseline sample: frequency rows on all cores")
     ap.add_argument("--cpu-rows-1core", type=int, default=8, help="CPU baseline sample: frequency rows on one core")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="c2: leave out the 1024-model batch figure")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain 256 MiB device copy (calibrates hardware byte counters)")
     args = ap.parse_args()
@@ -436,6 +437,19 @@ def main():
     counted = args.workload == "c2" and args.path == "adaptive" and prec == g.PREC_FP32
     roof, kernels = roofline(g, plan, prec, tm, plan.work(), counted)
 
+    # the 1024-model batch end to end (learned grids formed inside its graph)
+    c4 = None
+    if args.workload == "c2" and args.path == "adaptive" and not args.no_c4:
+        import copy
+        a4 = copy.copy(args)
+        a4.steps, a4.warmup, a4.precision = 3, 1, None
+        print("bench: c4 batch", file=sys.stderr, flush=True)
+        b = bench_batch(a4, g, models)
+        c4 = {k: b[k] for k in ("value", "unit", "ms_per_step", "speedup_vs_plain_fp64", "max_rel_l2_vs_fp64",
+                                "max_peak_err_vs_fp64")}
+        c4.update(workload=b["config"]["workload"], path=b["config"]["path"], steps=a4.steps,
+                  plain_fp64_batch=b["plain_fp64_batch"], adaptive=b["adaptive"], kernel_ms=b["kernel_ms"])
+
     cpu = None
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
@@ -466,6 +480,7 @@ def main():
         "kernel_ms": {k: round(v, 4) for k, v in kernels.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "c4_batch": c4,
     }
     print(json.dumps(line))
 

@@ -559,46 +559,32 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             out.Yt = ut * chain;
             out.y0t = COMPAT ? u0t * out.y0 : u0t * chain0;
         }
-    } else {  // branch 4, rl > s: green.cu:858-915
+    } else {  // branch 4, rl > s (physics mode only): green.cu:858-915
+        // The reference's receiver search stops at a layer <= s (it compares
+        // the source's depth inside its layer with each thickness), so its
+        // r > s code is never reached; the host refuses compat geometries here.
         cx<T> rd0 = cexp_nz<T>(qr.rd, zr), rd1 = cexp_nz<T>(qr.vd, zr);
         cx<T> ru0 = C<T>(0, 0), ru1 = C<T>(0, 0);
         if (!rl_half) {
             ru0 = cexp_nz<T>(qr.rd, hr_ - zr);
             ru1 = cexp_nz<T>(qr.vd, hr_ - zr);
         }
-        cx<T> ex0, ex1;
-        if (COMPAT) {
-            LQ<T> qp = layer_q(lw[rl - 1], kd);
-            lam(qp, lw[rl - 1].h, ex0, ex1);
-        } else {
-            // e^{-nu h} = e^{-nu zr} e^{-nu (h - zr)}; ex only enters times
-            // ru, which is zero in the half-space
-            ex0 = rd0 * ru0;
-            ex1 = rd1 * ru1;
-        }
+        // e^{-nu h} = e^{-nu zr} e^{-nu (h - zr)}; ex only enters times ru,
+        // which is zero in the half-space
+        const cx<T> ex0 = rd0 * ru0, ex1 = rd1 * ru1;
         const m2<T> RS = dmd(ru0, ru1, Rdu_r1, ex0, ex1);
         const m2<T> d = mmadd(Eu, RS, md(Ed, rd0, rd1));
         const cx<T> d0 = ru1 * rdu0_r1 * ex1 + rd1;
-        const m2<T> &ypsv = COMPAT ? yu_psv : yd_psv;
-        m2<T> Mc = Md;
-        if (COMPAT) {
-            // the reference's chain T_d[rl+1] .. T_d[s+2] (Q6): without the E
-            // column normalisation its ends leave D_rl D_{rl+1}^-1 on the left
-            // and D_{s+1} D_s^-1 on the right (D = diag(a/w, b/w) per layer)
-            const LayerW<T> &lr = lw[rl], &lr1 = lw[rl + 1], &ls1 = lw[s + 1], &ls = lw[s];
-            Mc = dmd(lr.ca * recip(lr1.ca), lr.cb * recip(lr1.cb), Md, ls1.ca * recip(ls.ca), ls1.cb * recip(ls.cb));
-        }
-        const m2<T> chain = Mc * dm(sexd0, sexd1, ypsv);
+        const m2<T> chain = Md * dm(sexd0, sexd1, yd_psv);
         const cx<T> chain0 = md0 * sexd1 * y_sh;
         out.Y = d * chain;
         out.y0 = d0 * chain0;
         if (NEED_T) {
-            const cx<T> pr = COMPAT ? par0_r : qr.r, pv = COMPAT ? par0_v : qr.v;
-            const m2<T> RSt = dmd(ru0 * pr, ru1 * pv, Rdu_r1, ex0, ex1);
+            const m2<T> RSt = dmd(ru0 * qr.r, ru1 * qr.v, Rdu_r1, ex0, ex1);
             const m2<T> dt = mmadd(Eu, RSt, md(Ed, -(qr.r * rd0), -(qr.v * rd1)));
-            const cx<T> d0t = ru1 * pv * rdu0_r1 * ex1 - qr.v * rd1;
+            const cx<T> d0t = ru1 * qr.v * rdu0_r1 * ex1 - qr.v * rd1;
             out.Yt = dt * chain;
-            out.y0t = COMPAT ? d0 * out.y0 : d0t * chain0;
+            out.y0t = d0t * chain0;
         }
     }
     if (!COMPAT && NEED_T && rl == 0 && zr == W(0)) {

@@ -153,7 +153,8 @@ int make_geo(const Stack &st, int compat, Geo &g)
         if (compat) return GRTM_E_UNSUPPORTED;  // the reference writes no coefficients
         g.up = 1;
     }
-    if (compat && g.branch == 4 && st.rl >= st.L - 1) return GRTM_E_UNSUPPORTED;
+    // the reference's search never places the receiver below the source layer
+    if (compat && g.branch == 4) return GRTM_E_UNSUPPORTED;
     return GRTM_OK;
 }
 

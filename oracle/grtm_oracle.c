@@ -459,8 +459,8 @@ static void diag_mul(cd m[2][2], const cd *dg, cd y[2][2], cd out[2][2])
 /* Receiver propagation + source coupling.  green.cu:615-939 (single force) and
  * 942-1267 (tensor) share everything but the final coupling.  compat keeps the
  * reference's choices: Q4 (rexut from the first cell, :674-675), Q5 (extra v in
- * u0t, :720), the r<s/r>s chains with yu0t/yd0t formed from the already
- * multiplied yu0/yd0 (:842, :902), and the r>s index shifts (Q6, :860-902). */
+ * u0t, :720) and the r<s chain with yu0t formed from the already multiplied
+ * yu0 (:842).  The r>s branch is physics mode only (see the check below). */
 static int ypars(int compat, int src_type, const OPar *par, const OGrt *grt, const OSrc *src,
                  int L, int s, or_real zs, int rl, or_real zr, or_real k, cd w, cd par0_r,
                  cd par0_v, OPpar *pp)
@@ -484,7 +484,11 @@ static int ypars(int compat, int src_type, const OPar *par, const OGrt *grt, con
         if (compat) return OR_E_UNSUPPORTED;
         up = 1;
     }
-    if (compat && rl > s && rl >= L - 1) return OR_E_UNSUPPORTED; /* reads grt[L] */
+    /* The reference's receiver search (green.cu:2092-2101, Q1) compares the
+     * source's depth inside its layer, which is at most h[s], with each
+     * thickness, so it stops at a layer <= s: its r > s code (green.cu:858-915)
+     * is never reached from an input, and compat mode has no such geometry. */
+    if (compat && rl > s) return OR_E_UNSUPPORTED;
 
     sexd[0] = c_exp(-sp->r * (sp->h - zs));
     sexd[1] = c_exp(-sp->v * (sp->h - zs));
@@ -595,10 +599,9 @@ static int ypars(int compat, int src_type, const OPar *par, const OGrt *grt, con
         memcpy(ch, yu1, sizeof(ch));
         m2_mul(u, ch, yu1);
         m2_mul(ut, ch, yu1t);
-    } else { /* rl > s, green.cu:858-915 */
-        const OPar *pe = compat ? &par[rl - 1] : rp;
-        ex[0] = c_exp(-pe->r * pe->h);
-        ex[1] = c_exp(-pe->v * pe->h);
+    } else { /* rl > s (physics mode only), green.cu:858-915 */
+        ex[0] = c_exp(-rp->r * rp->h);
+        ex[1] = c_exp(-rp->v * rp->h);
         int has_below = (rl + 1 <= L - 1);
         cd rd0 = has_below ? grt[rl + 1].rdu0 : 0.0;
         d0 = rexu[1] * rd0 * ex[1] + rexd[1];
@@ -612,33 +615,18 @@ static int ypars(int compat, int src_type, const OPar *par, const OGrt *grt, con
                 dt[i][j] = E[i][j] * rexdt[j] + E[i][2] * rexut[0] * r0j * ex[j] +
                            E[i][3] * rexut[1] * r1j * ex[j];
             }
-        if (compat) {
-            for (int lay = s + 1; lay <= rl; lay++) {
-                if (lay == s + 1) {
-                    yd0 = grt[lay + 1].td0 * sexd[1] * y_sh;
-                    diag_mul((cd(*)[2])grt[lay + 1].td1, sexd, yu_psv, yd1);
-                } else {
-                    ex[0] = c_exp(-par[lay - 2].r * par[lay - 2].h);
-                    ex[1] = c_exp(-par[lay - 2].v * par[lay - 2].h);
-                    yd0 = grt[lay + 1].td0 * ex[1] * yd0;
-                    diag_mul((cd(*)[2])grt[lay + 1].td1, ex, yd1, ch);
-                    memcpy(yd1, ch, sizeof(ch));
-                }
-            }
-        } else {
-            yd0 = grt[s + 1].td0 * sexd[1] * y_sh;
-            diag_mul((cd(*)[2])grt[s + 1].td1, sexd, yd_psv, yd1);
-            for (int lay = s + 2; lay <= rl; lay++) {
-                ex[0] = c_exp(-par[lay - 1].r * par[lay - 1].h);
-                ex[1] = c_exp(-par[lay - 1].v * par[lay - 1].h);
-                yd0 = grt[lay].td0 * ex[1] * yd0;
-                diag_mul((cd(*)[2])grt[lay].td1, ex, yd1, ch);
-                memcpy(yd1, ch, sizeof(ch));
-            }
+        yd0 = grt[s + 1].td0 * sexd[1] * y_sh;
+        diag_mul((cd(*)[2])grt[s + 1].td1, sexd, yd_psv, yd1);
+        for (int lay = s + 2; lay <= rl; lay++) {
+            ex[0] = c_exp(-par[lay - 1].r * par[lay - 1].h);
+            ex[1] = c_exp(-par[lay - 1].v * par[lay - 1].h);
+            yd0 = grt[lay].td0 * ex[1] * yd0;
+            diag_mul((cd(*)[2])grt[lay].td1, ex, yd1, ch);
+            memcpy(yd1, ch, sizeof(ch));
         }
         cd chain0 = yd0;
         yd0 = d0 * chain0;
-        yd0t = compat ? d0 * yd0 : d0t * chain0;
+        yd0t = d0t * chain0;
         memcpy(ch, yd1, sizeof(ch));
         m2_mul(d, ch, yd1);
         m2_mul(dt, ch, yd1t);

@@ -70,7 +70,8 @@ def _random_case(rng, orc, compat):
         mu = vs[j] ** 2 * rho[j] if compat else rho[j] * b * b
         lay[j] = [rng.uniform(1, 8), a.real, a.imag, b.real, b.imag, np.real(mu), np.imag(mu)]
     s = int(rng.integers(0, L - 1))
-    rl = int(rng.integers(0, L - 1 if compat else L))
+    # compat: the reference's receiver search ends at a layer <= s
+    rl = int(rng.integers(0, s + 1 if compat else L))
     zs = rng.uniform(0.05, lay[s, 0] - 0.05)
     zr = rng.uniform(0.0, lay[rl, 0]) if rl < L - 1 else rng.uniform(0, 5)
     if rl == s and abs(zr - zs) < 1e-3:
@@ -94,7 +95,25 @@ def test_cell_solver_matches_oracle(orc, compat, src):
         b = g.selftest_cell(compat, src, k, w, lay, s, zs, rl, zr, p0)
         assert np.abs(a - b).max() <= 1e-10 * np.abs(a).max()
         seen.add(1 if rl < s else (2 if rl == s and zr <= zs else (3 if rl == s else 4)))
-    assert seen == {1, 2, 3, 4}
+    assert seen == ({1, 2, 3} if compat else {1, 2, 3, 4})
+
+
+def test_compat_refuses_a_receiver_below_the_source_layer(orc):
+    """A receiver layer below the source layer cannot come out of the
+    reference's search (green.cu:2092-2101), so compat mode refuses it on the
+    device path and in the oracle alike; physics mode solves it."""
+    rng = np.random.default_rng(5)
+    L, w, lay, s, zs, _, _, k = _random_case(rng, orc, 0)
+    while s >= L - 2:
+        L, w, lay, s, zs, _, _, k = _random_case(rng, orc, 0)
+    rl, zr = s + 1, 0.5 * lay[s + 1, 0]
+    with pytest.raises(RuntimeError, match="status -4"):
+        orc.cell(1, 0, k, w, lay, s, zs, rl, zr, np.zeros(4))
+    with pytest.raises(g.GrtmError):
+        g.selftest_cell(1, 0, k, w, lay, s, zs, rl, zr, np.zeros(4))
+    a = orc.cell(0, 0, k, w, lay, s, zs, rl, zr, np.zeros(4))
+    b = g.selftest_cell(0, 0, k, w, lay, s, zs, rl, zr, np.zeros(4))
+    assert np.abs(a - b).max() <= 1e-10 * np.abs(a).max()
 
 
 def test_cell_solver_depth_truncation(orc):
