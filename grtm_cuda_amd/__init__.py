@@ -144,6 +144,13 @@ class TracePlan:
         """Mixed-precision plans: solve frequency rows [0, rows64) in FP64."""
         check(_lib().grtm_plan_set_rows64(self.ptr, int(rows64)), "grtm_plan_set_rows64")
 
+    def plant_nonfinite(self, cells):
+        """Test hook: give the FP32 cells [(row, k), ...] non-finite operands at
+        every execution, so that the FP64 re-solve of overflowed cells takes
+        them (an empty list removes them)."""
+        c = np.ascontiguousarray(np.asarray(cells, dtype=np.int32).reshape(-1, 2))
+        check(_lib().grtm_plan_plant_nonfinite(self.ptr, c.ctypes.data_as(P), len(c)), "grtm_plan_plant_nonfinite")
+
     def set_adaptive(self, kcut, window=None, stride=8, tol=1e-5):
         """Use a truncated wavenumber grid: kcut[iw] (km^-1) per frequency row,
         residual check every `stride`-th wavenumber over `window` km^-1 beyond
@@ -321,6 +328,12 @@ class BatchPlan:
         check(_lib().grtm_batch_set_learned(self.ptr, predictor.mlp.ptr, float(predictor.lmargin),
                                             wn.ctypes.data_as(P), int(stride), float(tol)), "grtm_batch_set_learned")
         self._predictor = predictor  # the batch uses its network: keep it alive
+
+    def plant_nonfinite(self, cells):
+        """Test hook: non-finite FP32 operands in the cells [(model, row, k), ...]
+        at every execution (TracePlan.plant_nonfinite); recaptures the graph."""
+        c = np.ascontiguousarray(np.asarray(cells, dtype=np.int32).reshape(-1, 3))
+        check(_lib().grtm_batch_plant_nonfinite(self.ptr, c.ctypes.data_as(P), len(c)), "grtm_batch_plant_nonfinite")
 
     def adaptive_stats(self):
         o = np.zeros(4)

@@ -759,6 +759,8 @@ struct grtm_plan {
     bool pair = false;  // FP32: both contractions in one streamed launch
     int kc = 0;         // its split-K slice length
     DBuf lw, lw64, flags, dists, A0, A1, B0, B1, C0, C1, tr, ws;
+    DBuf plant;      // test hook: cells given non-finite FP32 operands at every execution
+    int nplant = 0;
     DBuf ptab;  // post-processing tables of pp
     // adaptive wavenumber grid
     bool adaptive = false;
@@ -923,6 +925,9 @@ int run_cells(grtm_plan *p, hipStream_t s)
                                      p->nw - p->rows64, t_hi, n_hi, p->A0.p, p->A1.p, p->nk, cnt, list, nkw, rowpk, s,
                                      nchk, p->chk_stride, tailmx, dcnt, 1, ctr ? ctr + 1 : nullptr));
         if (p->ev_main1) HIPCHK(hipEventRecord(p->ev_main1, s));
+        if (p->nplant)
+            HIPCHK(grtm::launch_plant_nonfinite(p->plant.as<unsigned>(), p->nplant, p->nk, p->A0.p, p->A1.p, p->nk,
+                                                cnt, list, s));
     }
     const bool mixed = p->prec != GRTM_PREC_FP64;
     unsigned *cnt = mixed ? p->flags.as<unsigned>() : nullptr, *list = mixed ? cnt + 1 : nullptr;
@@ -1410,6 +1415,27 @@ static int upload_learn_model(grtm_plan *p)
 {
     p->lm.floor_rows = p->prec == GRTM_PREC_FP64 ? p->rows64_rule : p->rows64;
     HIPCHK(hipMemcpy(p->lmodel.p, &p->lm, sizeof(grtm::LearnModel), hipMemcpyHostToDevice));
+    return GRTM_OK;
+}
+
+int grtm_plan_plant_nonfinite(grtm_plan *p, const int *cells, int n)
+{
+    if (!p || n < 0 || n > GRTM_MAX_PLANT || (n && !cells)) return GRTM_E_INPUT;
+    if (p->kind != 0 || p->prec == GRTM_PREC_FP64) return GRTM_E_UNSUPPORTED;
+    std::vector<unsigned> c(std::max(n, 1));
+    for (int i = 0; i < n; i++) {
+        const int iw = cells[2 * i], ik = cells[2 * i + 1];
+        if (iw < p->rows64 || iw >= p->nw || ik < 0 || ik >= p->nk) return GRTM_E_INPUT;
+        c[i] = unsigned(iw) * unsigned(p->nk) + unsigned(ik);
+    }
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (p->last_stream && p->last_stream != p->stream) HIPCHK(hipStreamSynchronize(p->last_stream));
+    int st;
+    // room for every cell of the grid once plus the planted ones
+    if ((st = p->flags.alloc(sizeof(unsigned) * ((size_t)p->nw * p->nk + 1 + n)))) return st;
+    if ((st = p->plant.alloc(sizeof(unsigned) * c.size()))) return st;
+    HIPCHK(hipMemcpy(p->plant.p, c.data(), sizeof(unsigned) * c.size(), hipMemcpyHostToDevice));
+    p->nplant = n;
     return GRTM_OK;
 }
 
@@ -2003,6 +2029,9 @@ struct grtm_batch {
     double flops = 0;            // recursion flops of one execution (operation count)
     std::vector<grtm::ModelDesc> md_h;
     DBuf md, lw64, lw32, dists, B0, B1, A0, A1, C0, C1, ws, flags, tr, items;
+    // test hook: cells (group row * nk + k) given non-finite FP32 operands, per group
+    DBuf plant;
+    std::vector<size_t> plant_off;  // per group: first entry in plant, then the end
     DBuf ptab;  // post-processing tables of pp
     int nb = 0;       // models per group
     int bs = 256;     // recursion tile: wavenumbers per block
@@ -2194,6 +2223,10 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
                                                    rowpk, s, nchk, b->chk_stride, tailmx, dc, which, ctr,
                                                    b->lchunk));
         }
+        if (!b->plant_off.empty() && b->plant_off[gi + 1] > b->plant_off[gi])
+            HIPCHK(grtm::launch_plant_nonfinite(b->plant.as<unsigned>() + b->plant_off[gi],
+                                                int(b->plant_off[gi + 1] - b->plant_off[gi]), b->nk, b->A0.p,
+                                                b->A1.p, b->nk, cnt, list, s));
     }
     unsigned *cnt = batch_mixed(b) ? b->flags.as<unsigned>() : nullptr, *list = cnt ? cnt + 1 : nullptr;
     if (!b->adaptive) {
@@ -2696,6 +2729,39 @@ int grtm_batch_adaptive_stats(grtm_batch *b, double *out)
     for (size_t i = 0; i < n; i++) out[0] += b->nk_pred_h[i], out[1] += b->nchk_h[i], out[2] += nkw[i];
     for (int m = 0; m < b->nm; m++) out[3] += fb[m];
     return GRTM_OK;
+}
+
+int grtm_batch_plant_nonfinite(grtm_batch *b, const int *cells, int n)
+{
+    if (!b || n < 0 || n > GRTM_MAX_PLANT || (n && !cells)) return GRTM_E_INPUT;
+    if (!batch_mixed(b)) return GRTM_E_UNSUPPORTED;
+    const int nm = (int)b->L.size();
+    std::vector<std::vector<unsigned>> per(b->groups.size());
+    for (int i = 0; i < n; i++) {
+        const int m = cells[3 * i], iw = cells[3 * i + 1], ik = cells[3 * i + 2];
+        if (m < 0 || m >= nm || iw < b->rows64[m] || iw >= b->nw || ik < 0 || ik >= b->nk) return GRTM_E_INPUT;
+        const size_t gi = (size_t)(m / b->nb);
+        const int j = m - b->groups[gi].m0;
+        per[gi].push_back(unsigned(j * b->nw + iw) * unsigned(b->nk) + unsigned(ik));
+    }
+    std::vector<unsigned> all;
+    std::vector<size_t> off(1, 0);
+    for (const auto &v : per) {
+        all.insert(all.end(), v.begin(), v.end());
+        off.push_back(all.size());
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    int st;
+    if ((st = b->flags.alloc(sizeof(unsigned) * ((size_t)b->nb * b->nw * b->nk + 1 + n)))) return st;
+    if ((st = b->plant.alloc(sizeof(unsigned) * std::max<size_t>(1, all.size())))) return st;
+    if (!all.empty())
+        HIPCHK(hipMemcpy(b->plant.p, all.data(), sizeof(unsigned) * all.size(), hipMemcpyHostToDevice));
+    b->plant_off = n ? off : std::vector<size_t>();
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    return batch_capture(b);
 }
 
 int grtm_batch_execute(grtm_batch *b, void *stream)

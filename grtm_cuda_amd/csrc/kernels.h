@@ -120,6 +120,10 @@ hipError_t launch_copy_i32(const int *a, int *b, int n, hipStream_t st);
 // per-row training labels (peak, truncation index, 8 local maxima); out: nw x 26 floats
 hipError_t launch_row_labels(int prec, int nw, int nk, long lda, const void *A0, const void *A1, double tau,
                              float *out, hipStream_t st);
+// Test hook: non-finite FP32 operands in the listed cells (row * nk + k), each
+// appended to the flag list.
+hipError_t launch_plant_nonfinite(const unsigned *cells, int n, int nk, void *A0, void *A1, long lda,
+                                  unsigned *flag_count, unsigned *flag_list, hipStream_t st);
 hipError_t launch_cells_fix(int compat, const void *lw64, int L, const Geo &g, int nk, double dk, double df,
                             double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,
                             const unsigned *flag_list, hipStream_t st);

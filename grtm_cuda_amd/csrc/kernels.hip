@@ -415,6 +415,21 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_fix(const LayerW<double>
     }
 }
 
+// Test hook: the listed cells (row * nk + k) get non-finite operands and join
+// the flag list, as a FP32 cell that overflowed does in the recursion kernels.
+__global__ void k_plant_nonfinite(const unsigned *__restrict__ cells, int n, int nk, float *__restrict__ A0,
+                                  float *__restrict__ A1, long lda, unsigned *__restrict__ flag_count,
+                                  unsigned *__restrict__ flag_list)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const unsigned cell = cells[t];
+    float v[8];
+    for (int i = 0; i < 8; i++) v[i] = __builtin_nanf("");
+    store_zr(v, int(cell / unsigned(nk)), int(cell % unsigned(nk)), A0, A1, lda);
+    flag_list[atomicAdd(flag_count, 1u)] = cell;
+}
+
 __global__ void k_zero_u32(unsigned *p, int n)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -854,6 +869,15 @@ hipError_t launch_cells_fix(int compat, const void *lw64, int L, const Geo &g, i
         hipLaunchKernelGGL((k_cells_fix<float, false>), grid, dim3(kCellThreads), 0, st,
                            (const LayerW<double> *)lw64, L, g, nk, dk, df, omega_i, (float *)A0, (float *)A1, lda,
                            flag_count, flag_list);
+    return hipGetLastError();
+}
+
+hipError_t launch_plant_nonfinite(const unsigned *cells, int n, int nk, void *A0, void *A1, long lda,
+                                  unsigned *flag_count, unsigned *flag_list, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plant_nonfinite, dim3((n + 255) / 256), dim3(256), 0, st, cells, n, nk, (float *)A0,
+                       (float *)A1, lda, flag_count, flag_list);
     return hipGetLastError();
 }
 

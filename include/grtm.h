@@ -168,6 +168,15 @@ void grtm_plan_destroy(grtm_plan *plan);
  * counterpart (precision control of the new path). */
 int grtm_plan_set_rows64(grtm_plan *plan, int rows64);
 
+/* Test hook (no reference counterpart): at every execution of a
+ * mixed-precision Z/R plan, give the FP32 cells (cells[2i] = frequency row in
+ * [rows64, nw), cells[2i+1] = wavenumber index) non-finite operands and put
+ * them on the list of cells re-solved in FP64, as an FP32 cell that overflowed
+ * is.  n <= GRTM_MAX_PLANT; n = 0 removes them.  For the batch: cells[3i] =
+ * model, then row and wavenumber (recaptures the batch graph). */
+#define GRTM_MAX_PLANT 4096
+int grtm_plan_plant_nonfinite(grtm_plan *plan, const int *cells, int n);
+
 /* Adaptive wavenumber grid (no counterpart in the reference, whose ragged
  * grid uses the cal_kmax heuristic, green.cu:108-162).  kcut[iw] (km^-1) is
  * the truncation wavenumber of each frequency row (from the learned
@@ -222,6 +231,8 @@ int grtm_batch_set_adaptive(grtm_batch *batch, const double *kcut, const double 
 int grtm_batch_adaptive_stats(grtm_batch *batch, double *out);
 /* Replay the captured graph on `stream` (NULL: the batch's stream). */
 int grtm_batch_execute(grtm_batch *batch, void *stream);
+/* The test hook of grtm_plan_plant_nonfinite for a mixed-precision batch. */
+int grtm_batch_plant_nonfinite(grtm_batch *batch, const int *cells, int n);
 /* The same launches issued one by one on the batch's stream, no graph. */
 int grtm_batch_execute_eager(grtm_batch *batch);
 int grtm_batch_sync(grtm_batch *batch);
