@@ -144,6 +144,17 @@ class TracePlan:
         """Mixed-precision plans: solve frequency rows [0, rows64) in FP64."""
         check(_lib().grtm_plan_set_rows64(self.ptr, int(rows64)), "grtm_plan_set_rows64")
 
+    def set_ptam(self, n_ptam=10, tail=0):
+        """Continue every k-sum beyond the dense grid by peak-trough averaging
+        of its partial sums at the first n_ptam extrema (tail: wavenumbers
+        solved beyond the grid, 0 = from the nearest receiver); n_ptam=0 removes it."""
+        check(_lib().grtm_plan_set_ptam(self.ptr, int(n_ptam), int(tail)), "grtm_plan_set_ptam")
+
+    def ptam_stats(self):
+        o = np.zeros(4)
+        check(_lib().grtm_plan_ptam_stats(self.ptr, o.ctypes.data_as(P)), "grtm_plan_ptam_stats")
+        return dict(n_ptam=int(o[0]), tail=int(o[1]), extended=int(o[2]), sums=int(o[3]))
+
     def plant_nonfinite(self, cells):
         """Test hook: give the FP32 cells [(row, k), ...] non-finite operands at
         every execution, so that the FP64 re-solve of overflowed cells takes

@@ -761,6 +761,11 @@ struct grtm_plan {
     DBuf lw, lw64, flags, dists, A0, A1, B0, B1, C0, C1, tr, ws;
     DBuf plant;      // test hook: cells given non-finite FP32 operands at every execution
     int nplant = 0;
+    // peak-trough averaging of the k-sums' tails (grtm_plan_set_ptam): ptam_nt
+    // wavenumbers beyond the grid per row, their weighted kernels At and
+    // Bessel values Bt, and the count of sums that took the extended sum
+    int ptam_n = 0, ptam_nt = 0;
+    DBuf At0, At1, Bt0, Bt1, ptam_miss;
     DBuf ptab;  // post-processing tables of pp
     // adaptive wavenumber grid
     bool adaptive = false;
@@ -869,7 +874,23 @@ int run_cells_mt(grtm_plan *p, hipStream_t s)
     return GRTM_OK;
 }
 
+int run_cells_core(grtm_plan *p, hipStream_t s);
+
+// The plan's recursion, then the PTAM tail's cells (FP64 recursion, operands
+// in the plan's type) when set.
 int run_cells(grtm_plan *p, hipStream_t s)
+{
+    int st;
+    if ((st = run_cells_core(p, s))) return st;
+    if (!p->ptam_n) return GRTM_OK;
+    const bool f64 = p->prec == GRTM_PREC_FP64;
+    HIPCHK(grtm::launch_cells_zr(f64 ? 0 : 2, p->compat, f64 ? p->lw.p : p->lw64.p, p->L, p->g, p->ptam_nt, p->dk,
+                                 p->df, p->omega_i, 0, p->nw, nullptr, 0, p->At0.p, p->At1.p, p->ptam_nt, nullptr,
+                                 nullptr, nullptr, nullptr, s, nullptr, 0, nullptr, nullptr, 0, nullptr, p->nk));
+    return GRTM_OK;
+}
+
+int run_cells_core(grtm_plan *p, hipStream_t s)
 {
     if (p->kind == 1) return run_cells_mt(p, s);
     const bool ad = p->adaptive;
@@ -981,7 +1002,22 @@ int run_gemm_mt(grtm_plan *p, hipStream_t s)
     return GRTM_OK;
 }
 
+int run_gemm_core(grtm_plan *p, hipStream_t s);
+
+// The contraction, then the PTAM continuation of every sum when set.
 int run_gemm(grtm_plan *p, hipStream_t s)
+{
+    int st;
+    if ((st = run_gemm_core(p, s))) return st;
+    if (!p->ptam_n) return GRTM_OK;
+    HIPCHK(grtm::launch_zero_u32(p->ptam_miss.as<unsigned>(), 1, s));
+    HIPCHK(grtm::launch_ptam(p->prec == GRTM_PREC_FP64 ? 0 : 1, p->nw, p->nr, p->ptam_nt, p->ptam_n, p->dk,
+                             p->dists.as<double>(), p->At0.p, p->At1.p, p->Bt0.p, p->Bt1.p, p->C0.p, p->C1.p,
+                             p->ptam_miss.as<unsigned>(), s));
+    return GRTM_OK;
+}
+
+int run_gemm_core(grtm_plan *p, hipStream_t s)
 {
     if (p->kind == 1) return run_gemm_mt(p, s);
     const int M = 4 * p->nw;
@@ -1418,6 +1454,56 @@ static int upload_learn_model(grtm_plan *p)
     return GRTM_OK;
 }
 
+int grtm_plan_set_ptam(grtm_plan *p, int n_ptam, int tail)
+{
+    if (!p || n_ptam < 0 || n_ptam == 1 || n_ptam > GRTM_PTAM_MAX || tail < 0) return GRTM_E_INPUT;
+    if (p->kind != 0 || p->adaptive) return GRTM_E_UNSUPPORTED;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (p->last_stream && p->last_stream != p->stream) HIPCHK(hipStreamSynchronize(p->last_stream));
+    if (n_ptam == 0) {
+        p->ptam_n = p->ptam_nt = 0;
+        return GRTM_OK;
+    }
+    std::vector<double> d(p->nr);
+    HIPCHK(hipMemcpy(d.data(), p->dists.p, sizeof(double) * p->nr, hipMemcpyDeviceToHost));
+    const double rmin = *std::min_element(d.begin(), d.end());
+    if (!(rmin > 0)) return GRTM_E_INPUT;
+    // (n/2 + 2) periods of J(k r) at the nearest receiver, at most 4 grids long
+    const double want = std::ceil((0.5 * n_ptam + 2.0) * 2.0 * M_PI / (rmin * p->dk));
+    const int nt = tail ? tail : (int)std::min<double>(want, 4.0 * p->nk);
+    if (nt < 2 || nt > 4 * p->nk || !grtm::cells_tile_limit(p->nw, nt)) return GRTM_E_UNSUPPORTED;
+    const size_t es = esize(p->prec);
+    int st;
+    if ((st = p->At0.alloc((size_t)4 * p->nw * nt * es)) || (st = p->At1.alloc((size_t)4 * p->nw * nt * es)) ||
+        (st = p->Bt0.alloc((size_t)nt * p->nr * es)) || (st = p->Bt1.alloc((size_t)nt * p->nr * es)) ||
+        (st = p->ptam_miss.alloc(sizeof(unsigned))))
+        return st;
+    HIPCHK(grtm::launch_bessel_tab(p->prec == GRTM_PREC_FP64 ? 0 : 1, p->compat ? 0 : 1, nt, p->dk,
+                                   p->dists.as<double>(), p->nr, p->Bt0.p, p->Bt1.p, p->nr, p->stream, 0, p->nk));
+    HIPCHK(grtm::launch_zero_u32(p->ptam_miss.as<unsigned>(), 1, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    p->ptam_n = n_ptam;
+    p->ptam_nt = nt;
+    return GRTM_OK;
+}
+
+int grtm_plan_ptam_stats(grtm_plan *p, double *out)
+{
+    if (!p || !out) return GRTM_E_INPUT;
+    out[0] = p->ptam_n;
+    out[1] = p->ptam_nt;
+    out[2] = 0;
+    if (p->ptam_n) {
+        unsigned m = 0;
+        HIPCHK(hipStreamSynchronize(p->stream));
+        if (p->last_stream && p->last_stream != p->stream) HIPCHK(hipStreamSynchronize(p->last_stream));
+        HIPCHK(hipMemcpy(&m, p->ptam_miss.p, sizeof(unsigned), hipMemcpyDeviceToHost));
+        out[2] = m;
+    }
+    out[3] = (double)p->nw * 4 * p->nr;
+    return GRTM_OK;
+}
+
 int grtm_plan_plant_nonfinite(grtm_plan *p, const int *cells, int n)
 {
     if (!p || n < 0 || n > GRTM_MAX_PLANT || (n && !cells)) return GRTM_E_INPUT;
@@ -1463,6 +1549,7 @@ static void row_cut(int nk, double kcut, double dk, int stride, double window, i
 int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int stride, double tol)
 {
     if (!p) return GRTM_E_INPUT;
+    if (p->ptam_n && kcut) return GRTM_E_UNSUPPORTED;  // the PTAM tail continues the dense grid
     // column plans: only the streamed contraction set skips the rows' tails
     if (p->kind == 1 && !p->pair) return GRTM_E_UNSUPPORTED;
     int Mq[grtm::kMtSetsMax], ntile = (p->nw + 15) / 16;
@@ -1556,7 +1643,7 @@ int grtm_plan_set_learned(grtm_plan *p, grtm_mlp *net, double lmargin, double wi
 {
     if (!p) return GRTM_E_INPUT;
     if (!net) return grtm_plan_set_adaptive(p, nullptr, 0.0, 1, 1.0);
-    if (p->kind != 0 || !p->pair || p->gemm_mode == 2) return GRTM_E_UNSUPPORTED;
+    if (p->kind != 0 || !p->pair || p->gemm_mode == 2 || p->ptam_n) return GRTM_E_UNSUPPORTED;
     if (!(window >= 0) || stride < 1 || !(tol > 0) || !std::isfinite(lmargin)) return GRTM_E_INPUT;
     const int n = (int)p->layers_h.size() / 6;
     if (n > grtm::kLearnSlots) return GRTM_E_UNSUPPORTED;  // the network's 30 layer slots

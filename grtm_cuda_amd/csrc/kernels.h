@@ -68,12 +68,15 @@ int cells_tile_limit(int nw, int nk);  // 1 if (nw, nk) fit the item encoding
 // is only known there -- items [dev_count[which], dev_count[which + 1]) of
 // `tiles`, taken by a persistent grid from the counter *ctr (zeroed by the
 // launcher); ntiles is then an upper bound on the item count.
+// k_base: the rows' cells are the wavenumbers k_base .. k_base + nk - 1, stored
+// from column 0 of A (a tail beyond a grid of k_base wavenumbers; dense
+// launches without flags only).
 hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Geo &g, int nk, double dk,
                            double df, double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles, void *A0,
                            void *A1, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
                            unsigned *row_peak, hipStream_t st, const int *nchk = nullptr, int stride = 0,
                            unsigned *tail_max = nullptr, const unsigned *dev_count = nullptr, int which = 0,
-                           unsigned *ctr = nullptr);
+                           unsigned *ctr = nullptr, int k_base = 0);
 
 // ---- learned wavenumber grid built on the device
 // The predictor's inputs for one model and the decoding of its cut output
@@ -136,7 +139,18 @@ hipError_t launch_cells_full(int compat, int src, int dd_rows, const void *lw, c
 hipError_t launch_green_finish(int nw, int nblk, const double *partial, double *raw, hipStream_t st);
 // B0 = J_n0(k r), B1 = J_n0+1(k r)
 hipError_t launch_bessel_tab(int prec, int mode, int nk, double dk, const double *dists, int nr, void *B0,
-                             void *B1, long ldb, hipStream_t st, int n0 = 0);
+                             void *B1, long ldb, hipStream_t st, int n0 = 0, int k0 = 0);
+// Peak-trough averaging of the k-sums' tails (Z/R plans): for every frequency
+// row, source, component and receiver, the contraction's sum C (FP64 or FP32
+// per prec, [4 nw][nr]) is continued with the tail terms At[row][j] Bt[j][r]
+// (At: [4 nw][nt] weighted kernels of the wavenumbers beyond the grid, Bt:
+// [nt][nr] J0 / J1 there) over min(nt, ceil((n/2 + 2) 2 pi / (r dk))) terms,
+// and replaced by the binomially weighted mean of its first n extrema
+// (parabola vertices), or by the extended sum if fewer appear; misses counts
+// the sums of the latter kind.
+hipError_t launch_ptam(int prec, int nw, int nr, int nt, int n, double dk, const double *dists, const void *At0,
+                       const void *At1, const void *Bt0, const void *Bt1, void *C0, void *C1, unsigned *misses,
+                       hipStream_t st);
 hipError_t launch_gemm(int prec, int M, int N, int K, const void *A, long lda, const void *B, long ldb, void *Cm,
                        long ldc, hipStream_t st);
 size_t post_ifft_lds_bytes(int prec, int nw, int R);

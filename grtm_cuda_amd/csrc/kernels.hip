@@ -42,7 +42,7 @@ namespace grtm {
 template <typename T, typename TO, bool COMPAT, int BR>
 __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__restrict__ lw_all, int L,
                                                            Geo g, int nk, double dk, double df,
-                                                           double omega_i, int w0, int tpr,
+                                                           double omega_i, int w0, int tpr, int k_base,
                                                            const unsigned *__restrict__ tiles,
                                                            TO *__restrict__ A0, TO *__restrict__ A1, long lda,
                                                            unsigned *__restrict__ flag_count,
@@ -87,9 +87,9 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__re
         for (int j = threadIdx.x; j < L; j += blockDim.x) lw[j] = lw_all[(size_t)iw * L + j];
         __syncthreads();
         const int j = kb * kCellThreads + threadIdx.x;
-        const int ik = chk ? nkr + stride * j + stride / 2 : j;
+        const int ik = chk ? nkr + stride * j + stride / 2 : k_base + j;
         unsigned mag = 0;
-        if (j < lim && ik < nk) {
+        if (j < lim && (!chk || ik < nk)) {
             TO v[8];
             const bool ok = cell_zr<T, TO, COMPAT, BR>(lw, g, iw, ik, dk, df, omega_i, v);
             if (chk) {
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__re
                 for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(float(v[i])));
                 mag = __float_as_uint(isfinite(m) ? m : 3.0e38f);  // a non-finite check value fails the row
             } else {
-                store_zr(v, iw, ik, A0, A1, lda);
+                store_zr(v, iw, j, A0, A1, lda);
                 mag = cell_mag_bits(v);
                 if (!ok && flag_count) {
                     const unsigned slot = atomicAdd(flag_count, 1u);
@@ -453,13 +453,13 @@ __global__ void k_green_finish(int nw, int nblk, const double *__restrict__ part
 // reference's forms (A&S polynomials, Miller / upward recurrence), mode 1:
 // the math library.
 template <typename T>
-__global__ void k_bessel_tab(int mode, int n0, int nk, double dk, const double *__restrict__ dists, int nr,
+__global__ void k_bessel_tab(int mode, int n0, int nk, int k0, double dk, const double *__restrict__ dists, int nr,
                              T *__restrict__ B0, T *__restrict__ B1, long ldb)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)nk * nr) return;
     const int ik = int(t / nr), ir = int(t % nr);
-    const double x = dk * ik * dists[ir];  // phase formed in FP64
+    const double x = dk * (k0 + ik) * dists[ir];  // phase formed in FP64
     double j0v, j1v;
     if (mode == 0) {
         j0v = n0 == 0 ? as_j0(x) : as_jn(x, n0);
@@ -745,7 +745,7 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
                            double df, double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles, void *A0,
                            void *A1, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
                            unsigned *row_peak, hipStream_t st, const int *nchk, int stride, unsigned *tail_max,
-                           const unsigned *dev_count, int which, unsigned *ctr)
+                           const unsigned *dev_count, int which, unsigned *ctr, int k_base)
 {
     const int tpr = cells_tiles_per_row(nk);
     if (!tiles) ntiles = nwr * tpr;
@@ -758,7 +758,8 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
 #define GRTM_ZR_BR(T, TO, CP, BR, FC, FL)                                                                       \
     hipLaunchKernelGGL((k_cells_zr<T, TO, CP, BR>),                                                              \
                        dim3(dev_count ? resident_blocks((const void *)k_cells_zr<T, TO, CP, BR>, ntiles) : ntiles), \
-                       dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, L, g, nk, dk, df, omega_i, w0, tpr, tiles, \
+                       dim3(kCellThreads), 0, st, (const LayerW<T> *)lw, L, g, nk, dk, df, omega_i, w0, tpr, k_base, \
+                       tiles,                                                                                        \
                        (TO *)A0, (TO *)A1, lda, FC, FL, nk_w, row_peak, nchk, stride, tail_max, dev_count, which, ctr)
 #define GRTM_ZR(T, TO, CP, FC, FL)                                                 \
     do {                                                                           \
@@ -895,16 +896,86 @@ hipError_t launch_green_finish(int nw, int nblk, const double *partial, double *
 }
 
 hipError_t launch_bessel_tab(int prec, int mode, int nk, double dk, const double *dists, int nr, void *B0,
-                             void *B1, long ldb, hipStream_t st, int n0)
+                             void *B1, long ldb, hipStream_t st, int n0, int k0)
 {
     const size_t n = (size_t)nk * nr;
     dim3 grid((unsigned)((n + 255) / 256));
     if (prec == 0)
-        hipLaunchKernelGGL(k_bessel_tab<double>, grid, dim3(256), 0, st, mode, n0, nk, dk, dists, nr, (double *)B0,
-                           (double *)B1, ldb);
+        hipLaunchKernelGGL(k_bessel_tab<double>, grid, dim3(256), 0, st, mode, n0, nk, k0, dk, dists, nr,
+                           (double *)B0, (double *)B1, ldb);
     else
-        hipLaunchKernelGGL(k_bessel_tab<float>, grid, dim3(256), 0, st, mode, n0, nk, dk, dists, nr, (float *)B0,
-                           (float *)B1, ldb);
+        hipLaunchKernelGGL(k_bessel_tab<float>, grid, dim3(256), 0, st, mode, n0, nk, k0, dk, dists, nr,
+                           (float *)B0, (float *)B1, ldb);
+    return hipGetLastError();
+}
+
+// One thread per (row, source q, component, receiver); receivers fastest, so
+// a wave reads one tail row (broadcast) and consecutive Bt entries.  The tail
+// sums are formed in FP64 relative to the contraction's sum (their extrema
+// are those of the partial sums); the records' binomial weights C(n-1, i) /
+// 2^(n-1) are the n - 1 rounds of pairwise averaging.
+template <typename TO>
+__global__ __launch_bounds__(256) void k_ptam(int nw, int nr, int nt, int n, double dk,
+                                              const double *__restrict__ dists, const TO *__restrict__ At0,
+                                              const TO *__restrict__ At1, const TO *__restrict__ Bt0,
+                                              const TO *__restrict__ Bt1, TO *__restrict__ C0, TO *__restrict__ C1,
+                                              unsigned *__restrict__ misses)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nw * 4 * nr) return;
+    const int ir = int(t % nr);
+    const int sel = int((t / nr) % 4), iw = int(t / ((size_t)nr * 4));
+    const int comp = sel >> 1, q = sel & 1;  // comp 0: Z (J0), 1: R (J1)
+    const TO *At = comp ? At1 : At0, *Bt = comp ? Bt1 : Bt0;
+    TO *C = comp ? C1 : C0;
+    const size_t row = (size_t)4 * iw + 2 * q;  // real part; the imaginary part follows
+    const TO *are = At + row * nt, *aim = are + nt;
+    const double want = ceil((0.5 * n + 2.0) * 2.0 * M_PI / (dists[ir] * dk));
+    const int ntr = want < nt ? int(want) : nt;
+    // T: the tail sum before term j; an extremum of the partial sums sits at
+    // j - 1 when terms j - 1 and j have opposite signs
+    double T[2] = {0.0, 0.0}, est[2] = {0.0, 0.0}, cw[2] = {1.0, 1.0}, prev[2] = {0.0, 0.0};
+    int m[2] = {0, 0};
+    for (int j = 0; j < ntr; j++) {
+        const double b = double(Bt[(size_t)j * nr + ir]);
+        const double cur[2] = {double(are[j]) * b, double(aim[j]) * b};
+        for (int p = 0; p < 2; p++) {
+            const double a = prev[p], c = cur[p];
+            if (j > 0 && m[p] < n && ((a < 0.0 && c > 0.0) || (a > 0.0 && c < 0.0))) {
+                est[p] += cw[p] * (T[p] - (a + c) * (a + c) / (8.0 * (c - a)));
+                cw[p] = cw[p] * double(n - 1 - m[p]) / double(m[p] + 1);
+                m[p]++;
+            }
+            T[p] += c;
+            prev[p] = c;
+        }
+    }
+    const double norm = ldexp(1.0, -(n - 1));
+    const size_t ci = row * nr + ir;
+    const double s0[2] = {double(C[ci]), double(C[ci + nr])};
+    bool miss = false;
+    for (int p = 0; p < 2; p++) {
+        const double v = m[p] == n ? s0[p] + est[p] * norm : s0[p] + T[p];
+        miss = miss || m[p] < n;
+        C[ci + (size_t)p * nr] = TO(v);
+    }
+    if (miss && misses) atomicAdd(misses, 1u);
+}
+
+hipError_t launch_ptam(int prec, int nw, int nr, int nt, int n, double dk, const double *dists, const void *At0,
+                       const void *At1, const void *Bt0, const void *Bt1, void *C0, void *C1, unsigned *misses,
+                       hipStream_t st)
+{
+    const size_t nth = (size_t)nw * 4 * nr;
+    dim3 grid((unsigned)((nth + 255) / 256));
+    if (prec == 0)
+        hipLaunchKernelGGL(k_ptam<double>, grid, dim3(256), 0, st, nw, nr, nt, n, dk, dists, (const double *)At0,
+                           (const double *)At1, (const double *)Bt0, (const double *)Bt1, (double *)C0,
+                           (double *)C1, misses);
+    else
+        hipLaunchKernelGGL(k_ptam<float>, grid, dim3(256), 0, st, nw, nr, nt, n, dk, dists, (const float *)At0,
+                           (const float *)At1, (const float *)Bt0, (const float *)Bt1, (float *)C0, (float *)C1,
+                           misses);
     return hipGetLastError();
 }
 

@@ -88,6 +88,8 @@ def _lib():
         lib.grtm_plan_set_learned.argtypes = [P, P, D, D, I, D]
         lib.grtm_plan_set_rows64.argtypes = [P, I]
         lib.grtm_plan_plant_nonfinite.argtypes = [P, P, I]
+        lib.grtm_plan_set_ptam.argtypes = [P, I, I]
+        lib.grtm_plan_ptam_stats.argtypes = [P, P]
         lib.grtm_batch_plant_nonfinite.argtypes = [P, P, I]
         lib.grtm_plan_adaptive_stats.argtypes = [P, P]
         lib.grtm_plan_adaptive_rows.argtypes = [P, P, P]
@@ -136,7 +138,7 @@ EXPORTS = [
     "grtm_mlp_set_loss", "grtm_batch_create", "grtm_batch_execute", "grtm_batch_execute_eager", "grtm_batch_sync",
     "grtm_batch_traces_out", "grtm_batch_timing", "grtm_batch_config", "grtm_batch_work", "grtm_batch_destroy",
     "grtm_batch_set_adaptive", "grtm_batch_set_learned", "grtm_batch_adaptive_stats",
-    "grtm_plan_plant_nonfinite", "grtm_batch_plant_nonfinite",
+    "grtm_plan_plant_nonfinite", "grtm_batch_plant_nonfinite", "grtm_plan_set_ptam", "grtm_plan_ptam_stats",
 ]
 
 

@@ -168,6 +168,22 @@ void grtm_plan_destroy(grtm_plan *plan);
  * counterpart (precision control of the new path). */
 int grtm_plan_set_rows64(grtm_plan *plan, int rows64);
 
+/* Peak-trough averaging (PTAM) of every k-sum's tail on a dense Z/R plan.
+ * The reference reads n_ptam from input.txt (green.cu:2059) and leaves the
+ * method as an unused sketch (integral.cu:38-88).  At every execution the
+ * recursion also solves `tail` wavenumbers beyond the grid (same dk; 0: enough
+ * for n_ptam / 2 + 2 periods of J(k r) at the nearest receiver, at most 4 nk),
+ * and each sum over the grid is continued over min(tail, ceil((n_ptam/2 + 2)
+ * 2 pi / (r dk))) of them and replaced by the binomially weighted mean of the
+ * partial sums at its first n_ptam extrema (parabola vertices), or by the
+ * extended sum where fewer appear.  2 <= n_ptam <= GRTM_PTAM_MAX; n_ptam = 0
+ * removes it.  Not with adaptive grids (GRTM_E_UNSUPPORTED). */
+#define GRTM_PTAM_MAX 32
+int grtm_plan_set_ptam(grtm_plan *plan, int n_ptam, int tail);
+/* out[0] n_ptam, out[1] tail wavenumbers, out[2] sums of the last execution
+ * that took the extended sum, out[3] sums per execution (4 doubles). */
+int grtm_plan_ptam_stats(grtm_plan *plan, double *out);
+
 /* Test hook (no reference counterpart): at every execution of a
  * mixed-precision Z/R plan, give the FP32 cells (cells[2i] = frequency row in
  * [rows64, nw), cells[2i+1] = wavenumber index) non-finite operands and put

@@ -44,6 +44,7 @@ typedef __complex128 cd;
 #define RLOG logq
 #define RFABS fabsq
 #define RJN jnq
+#define RPOW2(e) ldexpq((__float128)1, (e))
 #define R_PI M_PIq
 static inline cd MKC(or_real re, or_real im)
 {
@@ -67,6 +68,7 @@ typedef double complex cd;
 #define RLOG log
 #define RFABS fabs
 #define RJN jn
+#define RPOW2(e) ldexp(1.0, (e))
 #define R_PI 3.1415926535897932384626433
 #define MKC CMPLX
 #endif
@@ -1476,29 +1478,91 @@ int or_kernels(int compat, const double *lay, int n_in, double src_depth, double
     return st;
 }
 
+/* Peak-trough averaging (PTAM) of the tail of a discrete k-sum.  The
+ * reference reads n_ptam (green.cu:2059) and leaves the method as an unused
+ * sketch (integral.cu:38-88): keep adding grid terms beyond k_max, record the
+ * partial sums at the peaks and troughs of their oscillation, and average
+ * neighbouring records repeatedly (Zhang, Chen & Chang 2003).  Here, for one
+ * real part: partial sums S_j = s0 + t_0 + ... + t_j; an extremum at j where
+ * t_j and t_{j+1} have opposite signs, recorded as the vertex of the parabola
+ * through S_{j-1}, S_j, S_{j+1}; the first n records averaged pairwise n - 1
+ * times, i.e. weighted by C(n-1, i) / 2^(n-1).  Fewer than n records in the
+ * nt terms: the plain extended sum.  *found: records seen (at most n). */
+static or_real ptam_part(or_real s0, const or_real *t, int stride, int nt, int n, int *found)
+{
+    or_real T = 0, est = 0, c = 1, all = 0;
+    const or_real norm = RPOW2(-(n - 1));
+    int m = 0;
+    for (int j = 0; j < nt; j++) all += t[(size_t)j * stride];
+    for (int j = 0; j + 1 < nt && m < n; j++) {
+        const or_real a = t[(size_t)j * stride], b = t[(size_t)(j + 1) * stride];
+        T += a;
+        if ((a < 0 && b > 0) || (a > 0 && b < 0)) {
+            est += c * (T - (a + b) * (a + b) / (8 * (b - a)));
+            c = c * (n - 1 - m) / (m + 1);
+            m++;
+        }
+    }
+    *found = m;
+    return m == n ? s0 + est * norm : s0 + all;
+}
+
+/* Terms of the PTAM tail of receiver r: (n/2 + 2) periods of J(k r) on the
+ * grid's dk, at most nt. */
+static int ptam_terms(int n, double r, double dk, int nt)
+{
+    const double want = ceil((0.5 * n + 2.0) * 2.0 * OR_PI / (r * dk));
+    return want < nt ? (int)want : nt;
+}
+
+int or_ptam(double s0_re, double s0_im, const double *terms, int nt, int n, double *out, int *found)
+{
+    if (nt < 0 || n < 2 || n > OR_PTAM_MAX) return OR_E_INPUT;
+    or_real *t = (or_real *)malloc(sizeof(or_real) * 2 * (size_t)nt + 1);
+    for (int i = 0; i < 2 * nt; i++) t[i] = terms[i];
+    int f0, f1;
+    out[0] = (double)ptam_part(s0_re, t, 2, nt, n, &f0);
+    out[1] = (double)ptam_part(s0_im, t + 1, 2, nt, n, &f1);
+    if (found) *found = f0 < f1 ? f0 : f1;
+    free(t);
+    return OR_OK;
+}
+
 int or_traces(int compat, const double *lay, int n_in, double src_depth, double rcv_depth,
               const double *dists, int nr, int src_kind, int nw, double dt, int nk, double kmax,
               double omega_i, double t0, double taper, int w_lo, int w_hi, double *traces,
               double *spec, int nthreads)
 {
-    if (w_lo < 0 || w_hi > nw || w_lo > w_hi || nr < 1) return OR_E_INPUT;
+    return or_traces_ptam(compat, lay, n_in, src_depth, rcv_depth, dists, nr, src_kind, nw, dt, nk, kmax,
+                          omega_i, t0, taper, w_lo, w_hi, 0, 0, traces, spec, nthreads);
+}
+
+int or_traces_ptam(int compat, const double *lay, int n_in, double src_depth, double rcv_depth,
+                   const double *dists, int nr, int src_kind, int nw, double dt, int nk, double kmax,
+                   double omega_i, double t0, double taper, int w_lo, int w_hi, int n_ptam, int nt,
+                   double *traces, double *spec, int nthreads)
+{
+    if (w_lo < 0 || w_hi > nw || w_lo > w_hi || nr < 1 || nt < 0) return OR_E_INPUT;
+    if (nt > 0 && (n_ptam < 2 || n_ptam > OR_PTAM_MAX)) return OR_E_INPUT;
     int nwr = w_hi - w_lo;
-    cd *kern = (cd *)malloc(sizeof(cd) * 2 * (size_t)nwr * nk + 1);
-    int st = kernels_cd(compat, lay, n_in, src_depth, rcv_depth, src_kind, nw, dt, nk, kmax, omega_i,
+    const double dk = kmax / nk;
+    const int nkt = nk + nt; /* the grid, then the PTAM tail on the same dk */
+    cd *kern = (cd *)malloc(sizeof(cd) * 2 * (size_t)nwr * nkt + 1);
+    int st = kernels_cd(compat, lay, n_in, src_depth, rcv_depth, src_kind, nw, dt, nkt, dk * nkt, omega_i,
                         w_lo, w_hi, kern, nthreads);
     if (st) {
         free(kern);
         return st;
     }
-    double df = 1.0 / (2 * nw * dt), dk = kmax / nk;
+    double df = 1.0 / (2 * nw * dt);
     int m = compat ? 0 : 1;
     /* Bessel tables J0(k r), J1(k r) */
-    or_real *bt = (or_real *)malloc(sizeof(or_real) * 2 * (size_t)nk * nr);
+    or_real *bt = (or_real *)malloc(sizeof(or_real) * 2 * (size_t)nkt * nr);
     for (int ir = 0; ir < nr; ir++)
-        for (int ik = 0; ik < nk; ik++) {
+        for (int ik = 0; ik < nkt; ik++) {
             or_real x = dk * ik * dists[ir];
-            bt[((size_t)ir * nk + ik) * 2] = bes(m, 0, 0, x);
-            bt[((size_t)ir * nk + ik) * 2 + 1] = bes(m, 0, 1, x);
+            bt[((size_t)ir * nkt + ik) * 2] = bes(m, 0, 0, x);
+            bt[((size_t)ir * nkt + ik) * 2 + 1] = bes(m, 0, 1, x);
         }
     cd *raw = (cd *)calloc((size_t)nr * 2 * nw, sizeof(cd));
 #ifdef _OPENMP
@@ -1506,18 +1570,36 @@ int or_traces(int compat, const double *lay, int n_in, double src_depth, double 
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
     for (int ir = 0; ir < nr; ir++) {
+        or_real *tail = (or_real *)malloc(sizeof(or_real) * 4 * (size_t)nt + 1);
+        const int ntr = nt ? ptam_terms(n_ptam, dists[ir], dk, nt) : 0;
         for (int iw = w_lo; iw < w_hi; iw++) {
             cd sz = 0, sr = 0;
-            const cd *kw = kern + (size_t)(iw - w_lo) * nk * 2;
-            const or_real *b = bt + (size_t)ir * nk * 2;
+            const cd *kw = kern + (size_t)(iw - w_lo) * nkt * 2;
+            const or_real *b = bt + (size_t)ir * nkt * 2;
             for (int ik = 0; ik < nk; ik++) {
                 or_real wgt = (ik == 0) ? dk / 2.0 : dk;
                 sz += kw[2 * ik] * (b[2 * ik] * wgt);
                 sr += kw[2 * ik + 1] * (b[2 * ik + 1] * wgt);
             }
+            if (ntr) {
+                /* tail terms: Z re, Z im, R re, R im */
+                for (int j = 0; j < ntr; j++) {
+                    const int ik = nk + j;
+                    const cd tz = kw[2 * ik] * (b[2 * ik] * dk), trr = kw[2 * ik + 1] * (b[2 * ik + 1] * dk);
+                    tail[4 * j] = CRE(tz);
+                    tail[4 * j + 1] = CIM(tz);
+                    tail[4 * j + 2] = CRE(trr);
+                    tail[4 * j + 3] = CIM(trr);
+                }
+                int f;
+                sz = MKC(ptam_part(CRE(sz), tail, 4, ntr, n_ptam, &f), ptam_part(CIM(sz), tail + 1, 4, ntr, n_ptam, &f));
+                sr = MKC(ptam_part(CRE(sr), tail + 2, 4, ntr, n_ptam, &f),
+                         ptam_part(CIM(sr), tail + 3, 4, ntr, n_ptam, &f));
+            }
             raw[(size_t)(ir * 2 + 0) * nw + iw] = sz;
             raw[(size_t)(ir * 2 + 1) * nw + iw] = sr;
         }
+        free(tail);
     }
     cd *sp = (cd *)malloc(sizeof(cd) * nw);
     or_real *tr = (or_real *)malloc(sizeof(or_real) * 2 * nw);

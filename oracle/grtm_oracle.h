@@ -103,6 +103,23 @@ int or_traces(int compat, const double *lay, int n_in, double src_depth,
               double t0, double taper, int w_lo, int w_hi, double *traces,
               double *spec, int nthreads);
 
+/* The same with the tail of every k-sum beyond the grid taken by peak-trough
+ * averaging (PTAM; the reference's n_ptam, green.cu:2059, integral.cu:38-88):
+ * nt more wavenumbers on the same dk, of which receiver r uses
+ * min(nt, ceil((n_ptam/2 + 2) 2 pi / (r dk))); n_ptam extrema per real
+ * sequence, 2 <= n_ptam <= OR_PTAM_MAX.  nt = 0: or_traces. */
+#define OR_PTAM_MAX 32
+int or_traces_ptam(int compat, const double *lay, int n_in, double src_depth,
+                   double rcv_depth, const double *dists, int nr, int src_kind,
+                   int nw, double dt, int nk, double kmax, double omega_i,
+                   double t0, double taper, int w_lo, int w_hi, int n_ptam,
+                   int nt, double *traces, double *spec, int nthreads);
+/* PTAM of one complex k-sum: s0 = the sum so far, terms[2 nt] the further
+ * terms (re, im interleaved); out[2] the estimate, *found the extrema seen
+ * (the smaller count of the two parts, at most n). */
+int or_ptam(double s0_re, double s0_im, const double *terms, int nt, int n,
+            double *out, int *found);
+
 /* Displacement columns of both sources on the dense grid (all frequency
  * rows): tensor columns 0-9, then single-force columns 0-4, of the
  * reference's integrand, each summed over k and post-processed like

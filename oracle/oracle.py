@@ -117,22 +117,38 @@ def bessel(mode, which, n, x):
 
 
 def traces(layers, src_depth, rcv_depth, dists, src_kind, grid, compat=False, w_range=None, nthreads=0,
-           quad=False):
+           quad=False, ptam=None):
     """Z/R traces [nr, 2, N] and post-processed spectra [nr, 2, nw] of one source kind
-    (0 explosion, 1 vertical force) on the dense grid."""
+    (0 explosion, 1 vertical force) on the dense grid.  ptam=(n_ptam, nt): the
+    tail of every k-sum from nt further wavenumbers by peak-trough averaging."""
     layers = np.ascontiguousarray(layers, dtype=np.float64)
     dists = np.ascontiguousarray(dists, dtype=np.float64)
     nw = int(grid["nw"])
     lo, hi = (0, nw) if w_range is None else w_range
     tr = np.zeros((len(dists), 2, 2 * nw))
     sp = np.zeros((len(dists), 2, nw, 2))
-    st = lib(quad).or_traces(int(compat), layers.ctypes.data_as(P), layers.shape[0], D(src_depth), D(rcv_depth),
-                         dists.ctypes.data_as(P), len(dists), int(src_kind), nw, D(grid["dt"]), int(grid["nk"]),
-                         D(grid["kmax"]), D(grid["omega_i"]), D(grid.get("t0", 0.0)), D(grid.get("taper", 0.0)),
-                         int(lo), int(hi), tr.ctypes.data_as(P), sp.ctypes.data_as(P), int(nthreads))
+    n_ptam, nt = ptam if ptam else (0, 0)
+    st = lib(quad).or_traces_ptam(int(compat), layers.ctypes.data_as(P), layers.shape[0], D(src_depth),
+                                  D(rcv_depth), dists.ctypes.data_as(P), len(dists), int(src_kind), nw,
+                                  D(grid["dt"]), int(grid["nk"]), D(grid["kmax"]), D(grid["omega_i"]),
+                                  D(grid.get("t0", 0.0)), D(grid.get("taper", 0.0)), int(lo), int(hi), int(n_ptam),
+                                  int(nt), tr.ctypes.data_as(P), sp.ctypes.data_as(P), int(nthreads))
     if st:
         raise RuntimeError(f"oracle traces status {st}")
     return tr, sp[..., 0] + 1j * sp[..., 1]
+
+
+def ptam(s0, terms, n):
+    """PTAM estimate of one complex k-sum: s0 the sum so far, terms the further
+    complex terms; returns (estimate, extrema seen)."""
+    t = np.ascontiguousarray(np.stack([np.real(terms), np.imag(terms)], axis=-1).ravel(), dtype=np.float64)
+    out = np.zeros(2)
+    found = ctypes.c_int()
+    st = lib().or_ptam(D(np.real(s0)), D(np.imag(s0)), t.ctypes.data_as(P), len(terms), int(n), out.ctypes.data_as(P),
+                       ctypes.byref(found))
+    if st:
+        raise RuntimeError(f"oracle ptam status {st}")
+    return out[0] + 1j * out[1], found.value
 
 
 N_COLUMNS = 15
