@@ -71,6 +71,29 @@ def test_c1_mixed_precision_traces(c1_oracle, prec):
     assert worst < 1e-4
 
 
+def test_c1_learned_adaptive_traces_match_oracle(c1_oracle):
+    """The learned adaptive FP32 path (the network's cuts formed on the GPU,
+    residual check, dense fallback) against the FP64 oracle's dense traces:
+    every trace within 1e-4 rel-L2 and peak error, on a fraction of the cells."""
+    from grtm_cuda_amd import learned
+    cfg, ref = c1_oracle
+    print("gpu: c1 learned adaptive FP32", flush=True)
+    plan = g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
+                       precision=g.PREC_FP32)
+    learned.Predictor().attach(plan)
+    tr = plan.traces()
+    st = plan.adaptive_stats()
+    frac = st["cells_final"] / (cfg["grid"]["nw"] * cfg["grid"]["nk"])
+    worst = worst_pk = 0.0
+    for q in (0, 1):
+        a, o = tr[q].transpose(1, 0, 2), ref[(False, q)]
+        worst = max(worst, rel_l2(a, o).max())
+        worst_pk = max(worst_pk, (np.abs(a - o).max(axis=-1) / np.abs(o).max(axis=-1)).max())
+    print(f"  learned adaptive vs oracle: rel-L2 {worst:.2e}, peak {worst_pk:.2e}, cells {frac:.3f} of dense, "
+          f"fallback rows {st['rows_fallback']}")
+    assert worst < 1e-4 and worst_pk < 1e-4 and frac < 0.6
+
+
 def test_c1_rows64_all_fp64_recursion(c1_oracle):
     """set_rows64(nw): every row's recursion in FP64, FP32 contraction; the
     error is then that of the FP32 contraction and FFT alone, and smaller than
