@@ -174,22 +174,33 @@ struct MlpDev {
     int nl, n_in, n_out;
 };
 
+// Outputs of the network used by the grid: y[0] = log(kcut / s), then per
+// pole cluster i < 8 y[1 + 2i] = centre / s, y[2 + 2i] = log(half width * 100).
+constexpr int kLearnOut = 17;
+
 __device__ __forceinline__ void learned_decode(const grtm::LearnModel &m, const grtm::LearnGrid &lg, int iw,
-                                               float y0, int &n, int &nc)
+                                               const float *y, int &n, int &nc)
 {
     const double w = 2.0 * M_PI * iw / (2.0 * lg.nw * lg.dt);
     const double dz = fmax(fabs(m.src - m.rcv), 0.5);
     const double s = fabs(w) / m.vs_min + 1.0 / dz;
-    double kc = exp(double(y0) + lg.lmargin) * s;
+    double kc = exp(double(y[0]) + lg.lmargin) * s;
     // rows solved in FP64 keep at least the analytic cut (models.analytic_kcut)
     if (iw < m.floor_rows) kc = fmax(kc, fmin(1.15 * w / m.vs_min + log(1e7) / dz, lg.kmax));
     kc = fmin(kc, lg.kmax);
+    // the residual check covers the window past the cut and reaches every
+    // predicted pole cluster that starts beyond it
+    double reach = kc + m.window;
+    for (int i = 0; i < 8; i++) {
+        const double c = double(y[1 + 2 * i]) * s, hw = exp(double(y[2 + 2 * i])) / 100.0;
+        if (c - hw > kc + m.window && c - hw < lg.kmax) reach = fmax(reach, c + hw);
+    }
     // row_cut (host.hip): check cell j at n + stride j + stride / 2 < nk; a
     // cut row without a check cell stays dense
     n = (int)fmin((double)lg.nk, fmax(1.0, ceil(kc / lg.dk)));
     const int room = lg.nk - n;
     const int fit = room > lg.stride / 2 ? (room - lg.stride / 2 - 1) / lg.stride + 1 : 0;
-    nc = max(0, min(fit, (int)ceil(m.window / (lg.dk * lg.stride))));
+    nc = max(0, min(fit, (int)ceil((reach - kc) / (lg.dk * lg.stride))));
     if (room > 0 && nc == 0) n = lg.nk;
 }
 
@@ -224,6 +235,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev net, int rows, const b
     constexpr int NF = 6 * grtm::kLearnSlots;  // layer features of a model
     __shared__ __attribute__((aligned(16))) bf16 act[2][kMlpRows][kMlpLd];
     __shared__ float mf[MODE ? kMlpRows + 1 : 1][MODE ? NF : 1];  // the block's models' layer features
+    __shared__ float ysh[MODE ? kMlpRows : 1][MODE ? kLearnOut : 1];  // the rows' grid outputs
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int r0 = blockIdx.x * kMlpRows;
     int m_first = 0;
@@ -292,17 +304,23 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev net, int rows, const b
                 } else if (r0 + row < rows) {
                     if constexpr (MODE == 0) {
                         if (col < net.n_out) Y[(size_t)(r0 + row) * net.n_out + col] = z;
-                    } else if (col == 0) {
-                        const int g = r0 + row, m = g / lg.nw, iw = g - m * lg.nw;
-                        int n, nc;
-                        learned_decode(models[m], lg, iw, z, n, nc);
-                        nk_pred[g] = n;
-                        nchk[g] = nc;
+                    } else if (col < kLearnOut) {
+                        ysh[row][col] = z;
                     }
                 }
             }
         __syncthreads();
         cur ^= 1;
+    }
+    if constexpr (MODE == 1) {
+        // one thread per row: its cut and check cells from all 17 outputs
+        if (tid < kMlpRows && r0 + tid < rows) {
+            const int g = r0 + tid, m = g / lg.nw, iw = g - m * lg.nw;
+            int n, nc;
+            learned_decode(models[m], lg, iw, ysh[tid], n, nc);
+            nk_pred[g] = n;
+            nchk[g] = nc;
+        }
     }
 }
 
@@ -683,7 +701,8 @@ namespace grtm {
 int learned_grid(grtm_mlp *net, const LearnModel *models, int n_models, const LearnGrid &lg, int *nk_pred,
                  int *nchk, hipStream_t st)
 {
-    if (!net || !net->fused || net->fin[0] != 6 * kLearnSlots + 2 || n_models < 1) return GRTM_E_UNSUPPORTED;
+    if (!net || !net->fused || net->fin[0] != 6 * kLearnSlots + 2 || net->n_out != kLearnOut || n_models < 1)
+        return GRTM_E_UNSUPPORTED;
     const int rows = n_models * lg.nw;
     hipLaunchKernelGGL((k_mlp_infer<1>), dim3((rows + kMlpRows - 1) / kMlpRows), dim3(256), 0, st, mlp_dev(net), rows,
                        nullptr, nullptr, models, lg, nk_pred, nchk);

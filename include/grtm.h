@@ -358,9 +358,11 @@ int grtm_mlp_predict_train_forward(grtm_mlp *mlp, const float *X, int n, float *
  * k = exp(y0 + lmargin) (|w| / Vs_min + 1 / max(|z_s - z_r|, 0.5)) (at least
  * the analytic cut 1.15 w / Vs_min + ln(1e7) / dz on the rows solved in
  * FP64, at most kmax), the residual check of grtm_plan_set_adaptive (window,
- * stride, tol) and the recursion's item list, taken by persistent launches.
- * Z / R plans with the streamed contraction (FP64, FP32; nk and nr multiples
- * of 4), models of at most 30 layers.  The plan keeps `net`, which must
+ * stride, tol) extended to reach every predicted pole cluster (outputs 1-16:
+ * centre / s, log(half width * 100)) that starts beyond the cut plus window,
+ * and the recursion's item list, taken by persistent launches.  A network of
+ * 17 outputs; Z / R plans with the streamed contraction (FP64, FP32; nk and nr
+ * multiples of 4), models of at most 30 layers.  The plan keeps `net`, which must
  * outlive it or be detached with net = NULL (which returns the plan to the
  * dense grid). */
 int grtm_plan_set_learned(grtm_plan *plan, grtm_mlp *net, double lmargin, double window, int stride, double tol);

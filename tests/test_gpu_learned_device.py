@@ -143,3 +143,53 @@ def test_batch_device_grids_graph_replays(pred):
     ref.execute()
     rl = np.linalg.norm(eager - ref.traces(), axis=-1) / np.linalg.norm(ref.traces(), axis=-1)
     assert rl.max() < 1e-4
+
+
+def test_predicted_far_cluster_extends_the_check(pred):
+    """The pole-cluster outputs steer the residual check: a network whose first
+    cluster sits at 3 s (s = |w| / Vs_min + 1 / dz) with half width 0.05 km^-1,
+    beyond most rows' cut plus window, must make the device grid check every
+    such row up to that cluster (and no further elsewhere), and the traces stay
+    within 1e-4 of FP64."""
+    cfg = models.config("c1")
+    grid = cfg["grid"]
+    lay, zs, zr = cfg["layers"], cfg["src_depth"], cfg["rcv_depth"]
+    far = learned.Predictor()
+    p = far.mlp.get_params()
+    fo = learned.N_OUT
+    nW = learned.WIDTH * fo
+    W = p[-(nW + fo):-fo].reshape(learned.WIDTH, fo)
+    b = p[-fo:]
+    W[:, 1:3] = 0.0
+    b[1], b[2] = 3.0, np.log(0.05 * 100)
+    p[-(nW + fo):-fo] = W.ravel()
+    p[-fo:] = b
+    far.mlp.set_params(p)
+    mk = lambda: g.TracePlan(lay, zs, zr, cfg["dists"], grid, precision=g.PREC_FP32)
+    print("gpu: c1 learned grid with a far predicted cluster", flush=True)
+    base, ext = mk(), mk()
+    pred.attach(base, stride=8, tol=3e-6)
+    far.attach(ext, stride=8, tol=3e-6)
+    tr = ext.traces()
+    base.traces()
+    cut_b, chk_b = base.adaptive_rows()
+    cut_e, chk_e = ext.adaptive_rows()
+    assert np.array_equal(cut_b, cut_e)  # the cut output is untouched
+    nw, nk, dk, stride = grid["nw"], grid["nk"], grid["kmax"] / grid["nk"], 8
+    w = 2 * np.pi * np.arange(nw) / (2 * nw * grid["dt"])
+    s = np.abs(w) / lay[:, 2].min() + 1.0 / max(abs(zs - zr), 0.5)
+    win = models.check_window(zs, zr)
+    c, hw = 3.0 * s, 0.05
+    # expected: the check reaches c + hw where the cluster starts beyond cut + window
+    kc = cut_e * dk  # the cut in cells (the continuous cut lies within one cell below)
+    hit = (c - hw > kc + win + dk) & (c - hw < grid["kmax"]) & (cut_e < nk)
+    room = nk - cut_e
+    fit = np.where(room > stride // 2, (room - stride // 2 - 1) // stride + 1, 0)
+    want = np.minimum(fit, np.ceil((c + hw - kc) / (dk * stride)))
+    print(f"  rows reaching a far cluster: {hit.sum()} of {nw}; check cells {chk_b.sum()} -> {chk_e.sum()}")
+    assert hit.sum() > nw // 8
+    assert (np.abs(chk_e[hit] - want[hit]) <= 1).all()
+    assert (chk_e[hit] > chk_b[hit]).all() and np.array_equal(chk_e[~hit & (c + hw < kc)], chk_b[~hit & (c + hw < kc)])
+    ref = g.TracePlan(lay, zs, zr, cfg["dists"], grid, precision=g.PREC_FP64).traces()
+    rl = np.linalg.norm(tr - ref, axis=-1) / np.linalg.norm(ref, axis=-1)
+    assert rl.max() < 1e-4
