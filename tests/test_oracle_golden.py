@@ -178,3 +178,19 @@ def test_example_input_parses(orc):
     assert run.n_layer == 5 and run.s_layer == 0 and abs(run.zs - 8.0) < 1e-12
     dk, oi, nk, nkw = orc.ref_grid(run)
     assert nkw == nk.sum() and nk.min() >= 2
+
+
+def test_c1_fp64_oracle_against_binary128(orc):
+    """The FP64 oracle on the 3-layer workload against its binary128 build
+    (the same restatement in __float128): the lowest rows, where the P and S
+    columns of the basis grow parallel, within 2e-8 of each receiver's
+    largest spectrum value; a middle band within 1e-12."""
+    from grtm_cuda_amd import models
+    cfg = models.config("c1")
+    for (lo, hi), tol in (((0, 8), 2e-8), ((200, 204), 1e-12)):
+        for kind in (0, 1):
+            args = (cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], kind, cfg["grid"])
+            _, s64 = orc.traces(*args, w_range=(lo, hi))
+            _, sq = orc.traces(*args, w_range=(lo, hi), quad=True)
+            a, b = s64[..., lo:hi], sq[..., lo:hi]
+            assert (np.abs(a - b).max(axis=-1) / np.abs(b).max(axis=-1)).max() < tol, (lo, kind)
