@@ -369,10 +369,11 @@ class BatchPlan:
                     predictor=float(ms[4]))
 
     def config(self):
-        o = np.zeros(8)
+        o = np.zeros(10)
         check(_lib().grtm_batch_config(self.ptr, o.ctypes.data_as(P)), "grtm_batch_config")
         return dict(models_per_group=int(o[0]), tile=int(o[1]), k_slice=int(o[2]), groups=int(o[3]),
-                    graph_nodes=int(o[4]), tuned=int(o[5]), tune_ms=float(o[6]), item_chunk=int(o[7]))
+                    graph_nodes=int(o[4]), tuned=int(o[5]), tune_ms=float(o[6]), item_chunk=int(o[7]),
+                    resident=bool(o[8]), rows_cut=int(o[9]))
 
     def work(self):
         o = np.zeros(6)

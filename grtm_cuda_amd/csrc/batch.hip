@@ -8,18 +8,64 @@
 
 namespace grtm {
 
+// The cell's eight operand values into the group row's four component rows
+// of A0 / A1 (RowLayout); a resident row stores only its first fext wavenumbers.
+template <typename TO>
+__device__ __forceinline__ void store_zr_rl(const TO v[8], int grow, int ik, TO *A0, TO *A1, const RowLayout &rl)
+{
+    size_t r0, st;
+    if (rl.foff) {
+        const int ext = rl.fext[grow];
+        if (ik >= ext) return;
+        r0 = size_t(rl.foff[grow]) + size_t(ik), st = size_t(ext);
+    } else {
+        r0 = size_t(4 * grow) * size_t(rl.lda) + size_t(ik), st = size_t(rl.lda);
+    }
+    for (int i = 0; i < 4; i++) {
+        A0[r0 + i * st] = v[i];
+        A1[r0 + i * st] = v[4 + i];
+    }
+}
+
+__device__ __forceinline__ void flag_cell(const FlagList &fl, int grow, int nk, int ik)
+{
+    const unsigned slot = atomicAdd(fl.count, 1u);
+    if (slot < fl.cap) fl.list[slot] = (unsigned long long)grow * (unsigned long long)nk + (unsigned long long)ik;
+}
+
+// The recursion of one cell with the receiver branch fixed at compile time
+// (BR 1-4), or for BR = 0 the model's own branch, each branch's solve
+// compiled separately (so the kernel's registers are the largest branch's,
+// not the union a run-time branch would carry).
+template <typename T, typename TO, bool COMPAT, int BR>
+__device__ __forceinline__ bool cell_zr_br(const LayerW<T> *lw, const Geo &g, int iw, int ik, double dk, double df,
+                                           double omega_i, TO v[8])
+{
+    if constexpr (BR != 0) {
+        return cell_zr<T, TO, COMPAT, BR>(lw, g, iw, ik, dk, df, omega_i, v);
+    } else {
+        switch (g.branch) {
+        case 1: return cell_zr<T, TO, COMPAT, 1>(lw, g, iw, ik, dk, df, omega_i, v);
+        case 2: return cell_zr<T, TO, COMPAT, 2>(lw, g, iw, ik, dk, df, omega_i, v);
+        case 3: return cell_zr<T, TO, COMPAT, 3>(lw, g, iw, ik, dk, df, omega_i, v);
+        default: return cell_zr<T, TO, COMPAT, 4>(lw, g, iw, ik, dk, df, omega_i, v);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- batches
 // Recursion of a group of models in one launch: one block of BS threads per
 // work item (model j of the group, row, k tile of BS wavenumbers), items
 // ordered heaviest model first so the launch ends on light items.  Model
 // m0 + j has its geometry and the offset of its layer table in md; its A
-// operand rows are group rows j nw + iw.  Non-finite FP32 cells go to the
-// flag list as (j nw + iw) nk + ik.  Adaptive grids: nk_w[group row] limits
+// operand rows are group rows j nw + iw (RowLayout).  Non-finite FP32 cells
+// go to the flag list as (j nw + iw) nk + ik.  Adaptive grids: nk_w[group row] limits
 // the wavenumbers solved and row_peak[group row] collects the row maxima; an
 // item whose row / tile word carries kCheckItem solves the row's residual-check
 // cells BS c .. BS c + BS - 1 (wavenumbers nk_w + stride j + stride / 2, j <
 // nchk[group row]) and takes their largest |value| to tail_max[group row].
-// BR: the receiver branch shared by every model of the launch's items.
+// BR: the receiver branch shared by every model of the launch's items, or 0
+// (each item's model's own).
 // dev_count (persistent form, learned grids formed on the device): the grid
 // is the resident blocks, each taking items [dev_count[which],
 // dev_count[which + 1]) `chunk` at a time from the counter *ctr (one counter
@@ -29,10 +75,8 @@ __global__ __launch_bounds__(BS, sizeof(T) == 4 ? 4 : 1) void k_cells_zr_batch(c
                                                        const LayerW<T> *__restrict__ lw_all, int nw, int nk,
                                                        double dk, double df, double omega_i,
                                                        const unsigned long long *__restrict__ items,
-                                                       TO *__restrict__ A0, TO *__restrict__ A1, long lda,
-                                                       unsigned *__restrict__ flag_count,
-                                                       unsigned *__restrict__ flag_list,
-                                                       const int *__restrict__ nk_w,
+                                                       TO *__restrict__ A0, TO *__restrict__ A1, RowLayout rl,
+                                                       FlagList fl, const int *__restrict__ nk_w,
                                                        unsigned *__restrict__ row_peak,
                                                        const int *__restrict__ nchk, int stride,
                                                        unsigned *__restrict__ tail_max,
@@ -78,18 +122,15 @@ __global__ __launch_bounds__(BS, sizeof(T) == 4 ? 4 : 1) void k_cells_zr_batch(c
         unsigned mag = 0;
         if (jj < lim && ik < nk) {
             TO v[8];
-            const bool ok = cell_zr<T, TO, COMPAT, BR>(lw, g, iw, ik, dk, df, omega_i, v);
+            const bool ok = cell_zr_br<T, TO, COMPAT, BR>(lw, g, iw, ik, dk, df, omega_i, v);
             if (chk) {
                 float m = 0.f;
                 for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(float(v[i])));
                 mag = __float_as_uint(isfinite(m) ? m : 3.0e38f);  // a non-finite check value fails the row
             } else {
-                store_zr(v, grow, ik, A0, A1, lda);
+                store_zr_rl(v, grow, ik, A0, A1, rl);
                 mag = cell_mag_bits(v);
-                if (!ok && flag_count) {
-                    const unsigned slot = atomicAdd(flag_count, 1u);
-                    flag_list[slot] = unsigned(grow) * unsigned(nk) + unsigned(ik);
-                }
+                if (!ok && fl.count) flag_cell(fl, grow, nk, ik);
             }
         }
         unsigned *dst = chk ? tail_max : row_peak;
@@ -193,11 +234,10 @@ template <typename T, typename TO, bool COMPAT, bool F64ROWS>
 __global__ __launch_bounds__(kCellThreads) void k_fallback_batch(const ModelDesc *__restrict__ md, int m0,
                                                                  const LayerW<T> *__restrict__ lw_all, int nw, int nk,
                                                                  double dk, double df, double omega_i,
-                                                                 TO *__restrict__ A0, TO *__restrict__ A1, long lda,
-                                                                 const unsigned *__restrict__ count,
+                                                                 TO *__restrict__ A0, TO *__restrict__ A1,
+                                                                 RowLayout rl, const unsigned *__restrict__ count,
                                                                  const unsigned *__restrict__ list, int nrow,
-                                                                 unsigned *__restrict__ flag_count,
-                                                                 unsigned *__restrict__ flag_list)
+                                                                 FlagList fl)
 {
     __shared__ LayerW<T> lw[kMaxLayers];
     const unsigned n = *count;
@@ -213,11 +253,8 @@ __global__ __launch_bounds__(kCellThreads) void k_fallback_batch(const ModelDesc
         for (int ik = start + blockIdx.x * blockDim.x + threadIdx.x; ik < nk; ik += gridDim.x * blockDim.x) {
             TO v[8];
             const bool ok = cell_zr<T, TO, COMPAT>(lw, d.g, iw, ik, dk, df, omega_i, v);
-            store_zr(v, grow, ik, A0, A1, lda);
-            if (!ok && flag_count) {
-                const unsigned s2 = atomicAdd(flag_count, 1u);
-                flag_list[s2] = unsigned(grow) * unsigned(nk) + unsigned(ik);
-            }
+            store_zr_rl(v, grow, ik, A0, A1, rl);
+            if (!ok && fl.count) flag_cell(fl, grow, nk, ik);
         }
     }
 }
@@ -228,19 +265,44 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_fix_batch(const ModelDes
                                                                   const LayerW<double> *__restrict__ lw_all,
                                                                   int nw, int nk, double dk, double df,
                                                                   double omega_i, TO *__restrict__ A0,
-                                                                  TO *__restrict__ A1, long lda,
-                                                                  const unsigned *__restrict__ flag_count,
-                                                                  const unsigned *__restrict__ flag_list)
+                                                                  TO *__restrict__ A1, RowLayout rl, FlagList fl)
 {
-    const unsigned n = *flag_count;
+    const unsigned n = min(*fl.count, fl.cap);
     for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-        const unsigned cell = flag_list[t];
-        const int grow = int(cell / unsigned(nk)), ik = int(cell % unsigned(nk));
+        const unsigned long long cell = fl.list[t];
+        const int grow = int(cell / (unsigned long long)nk), ik = int(cell % (unsigned long long)nk);
         const int j = grow / nw, iw = grow - j * nw;
         const Geo g = md[m0 + j].g;
         TO v[8];
         cell_zr<double, TO, COMPAT>(lw_all + md[m0 + j].lw_off + (size_t)iw * g.L, g, iw, ik, dk, df, omega_i, v);
-        store_zr(v, grow, ik, A0, A1, lda);
+        store_zr_rl(v, grow, ik, A0, A1, rl);
+    }
+}
+
+// Test hook: the listed cells (group row nk + k) get non-finite operands and
+// join the flag list.
+__global__ void k_plant_nonfinite_batch(const unsigned long long *__restrict__ cells, int n, int nk,
+                                        float *__restrict__ A0, float *__restrict__ A1, RowLayout rl, FlagList fl)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const unsigned long long cell = cells[t];
+    const int grow = int(cell / (unsigned long long)nk), ik = int(cell % (unsigned long long)nk);
+    float v[8];
+    for (int i = 0; i < 8; i++) v[i] = __builtin_nanf("");
+    store_zr_rl(v, grow, ik, A0, A1, rl);
+    flag_cell(fl, grow, nk, ik);
+}
+
+// A resident row's final extent cut to its stored length (counted in *over).
+__global__ void k_clamp_extent(int nrow, int *__restrict__ nk_w, const int *__restrict__ fext,
+                               unsigned *__restrict__ over)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    if (nk_w[r] > fext[r]) {
+        nk_w[r] = fext[r];
+        atomicAdd(over, 1u);
     }
 }
 
@@ -270,10 +332,10 @@ hipError_t launch_batch_items(int nslots, const unsigned *slots, int nw, int bs,
 
 hipError_t launch_cells_zr_batch(int prec, int compat, int bs, int br, const ModelDesc *md, int m0, const void *lw_all,
                                  int nw, int nk, double dk, double df, double omega_i,
-                                 const unsigned long long *items, int nitems, void *A0, void *A1, long lda,
-                                 unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
-                                 hipStream_t st, const int *nchk, int stride, unsigned *tail_max,
-                                 const unsigned *dev_count, int which, unsigned *ctr, int chunk)
+                                 const unsigned long long *items, int nitems, void *A0, void *A1, RowLayout rl,
+                                 FlagList fl, const int *nk_w, unsigned *row_peak, hipStream_t st, const int *nchk,
+                                 int stride, unsigned *tail_max, const unsigned *dev_count, int which, unsigned *ctr,
+                                 int chunk)
 {
     if (nitems <= 0) return hipSuccess;
     if (dev_count) {
@@ -287,7 +349,7 @@ hipError_t launch_cells_zr_batch(int prec, int compat, int bs, int br, const Mod
                                                            nitems)                                                 \
                                       : nitems),                                                                 \
                        dim3(BSZ), 0, st, md, m0, (const LayerW<T> *)lw_all, nw, nk, dk, df, omega_i, items, (TO *)A0, \
-                       (TO *)A1, lda, FC, FL, nk_w, row_peak, nchk, stride, tail_max, dev_count, which, ctr, chunk)
+                       (TO *)A1, rl, FC, nk_w, row_peak, nchk, stride, tail_max, dev_count, which, ctr, chunk)
 #define GRTM_ZB_BR(T, TO, CP, BSZ, FC, FL)                           \
     do {                                                             \
         if (br == 1) GRTM_ZB(T, TO, CP, BSZ, 1, FC, FL);             \
@@ -295,25 +357,33 @@ hipError_t launch_cells_zr_batch(int prec, int compat, int bs, int br, const Mod
         else if (br == 3) GRTM_ZB(T, TO, CP, BSZ, 3, FC, FL);        \
         else GRTM_ZB(T, TO, CP, BSZ, 4, FC, FL);                     \
     } while (0)
-#define GRTM_ZB_BS(T, TO, CP, FC, FL)                                \
+// br = 0 (every branch in one kernel) for the FP32-operand launches only
+#define GRTM_ZB_BR0(T, TO, CP, BSZ, FC, FL)                          \
     do {                                                             \
-        if (bs == 64) GRTM_ZB_BR(T, TO, CP, 64, FC, FL);             \
-        else if (bs == 128) GRTM_ZB_BR(T, TO, CP, 128, FC, FL);      \
-        else GRTM_ZB_BR(T, TO, CP, 256, FC, FL);                     \
+        if (br == 0) GRTM_ZB(T, TO, CP, BSZ, 0, FC, FL);             \
+        else GRTM_ZB_BR(T, TO, CP, BSZ, FC, FL);                     \
+    } while (0)
+#define GRTM_ZB_BS(T, TO, CP, FC, FL, BRM)                           \
+    do {                                                             \
+        if (bs == 64) BRM(T, TO, CP, 64, FC, FL);                    \
+        else if (bs == 128) BRM(T, TO, CP, 128, FC, FL);             \
+        else BRM(T, TO, CP, 256, FC, FL);                            \
     } while (0)
     if (bs != 64 && bs != 128 && bs != 256) return hipErrorInvalidValue;
-    if (br < 1 || br > 4) return hipErrorInvalidValue;
+    if (br < 0 || br > 4 || (br == 0 && prec == 0)) return hipErrorInvalidValue;
+    const FlagList none{nullptr, nullptr, 0u};
     if (prec == 0) {
-        if (compat) GRTM_ZB_BS(double, double, true, nullptr, nullptr);
-        else GRTM_ZB_BS(double, double, false, nullptr, nullptr);
+        if (compat) GRTM_ZB_BS(double, double, true, none, none, GRTM_ZB_BR);
+        else GRTM_ZB_BS(double, double, false, none, none, GRTM_ZB_BR);
     } else if (prec == 1) {
-        if (compat) GRTM_ZB_BS(float, float, true, flag_count, flag_list);
-        else GRTM_ZB_BS(float, float, false, flag_count, flag_list);
+        if (compat) GRTM_ZB_BS(float, float, true, fl, fl, GRTM_ZB_BR0);
+        else GRTM_ZB_BS(float, float, false, fl, fl, GRTM_ZB_BR0);
     } else {
-        if (compat) GRTM_ZB_BS(double, float, true, nullptr, nullptr);
-        else GRTM_ZB_BS(double, float, false, nullptr, nullptr);
+        if (compat) GRTM_ZB_BS(double, float, true, none, none, GRTM_ZB_BR0);
+        else GRTM_ZB_BS(double, float, false, none, none, GRTM_ZB_BR0);
     }
 #undef GRTM_ZB_BS
+#undef GRTM_ZB_BR0
 #undef GRTM_ZB_BR
 #undef GRTM_ZB
     return hipGetLastError();
@@ -322,43 +392,57 @@ hipError_t launch_cells_zr_batch(int prec, int compat, int bs, int br, const Mod
 hipError_t launch_fallback_batch(int prec, int compat, const ModelDesc *md, int m0, const void *lw64, const void *lw32,
                                  int nrow, int nw, int nk, double dk, double df, double omega_i, double tol,
                                  const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *count,
-                                 unsigned *list, unsigned *model_count, void *A0, void *A1, long lda,
-                                 unsigned *flag_count, unsigned *flag_list, hipStream_t st)
+                                 unsigned *list, unsigned *model_count, void *A0, void *A1, RowLayout rl,
+                                 FlagList fl, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fallback_select_batch, dim3((nrow + 255) / 256), dim3(256), 0, st, nrow, nw, nk, tol, row_peak,
                        tail_max, nk_w, count, list, model_count);
     const dim3 grid((nk + kCellThreads - 1) / kCellThreads, 16);
-#define GRTM_FBB(T, TO, CP, F64, LWP, FC, FL)                                                                    \
+#define GRTM_FBB(T, TO, CP, F64, LWP, FLL)                                                                       \
     hipLaunchKernelGGL((k_fallback_batch<T, TO, CP, F64>), grid, dim3(kCellThreads), 0, st, md, m0,              \
-                       (const LayerW<T> *)LWP, nw, nk, dk, df, omega_i, (TO *)A0, (TO *)A1, lda, count, list, nrow, \
-                       FC, FL)
+                       (const LayerW<T> *)LWP, nw, nk, dk, df, omega_i, (TO *)A0, (TO *)A1, rl, count, list, nrow,  \
+                       FLL)
+    const FlagList none{nullptr, nullptr, 0u};
     if (prec == 0) {
-        if (compat) GRTM_FBB(double, double, true, true, lw64, nullptr, nullptr);
-        else GRTM_FBB(double, double, false, true, lw64, nullptr, nullptr);
+        if (compat) GRTM_FBB(double, double, true, true, lw64, none);
+        else GRTM_FBB(double, double, false, true, lw64, none);
     } else if (compat) {
-        GRTM_FBB(double, float, true, true, lw64, nullptr, nullptr);
-        GRTM_FBB(float, float, true, false, lw32, flag_count, flag_list);
+        GRTM_FBB(double, float, true, true, lw64, none);
+        GRTM_FBB(float, float, true, false, lw32, fl);
     } else {
-        GRTM_FBB(double, float, false, true, lw64, nullptr, nullptr);
-        GRTM_FBB(float, float, false, false, lw32, flag_count, flag_list);
+        GRTM_FBB(double, float, false, true, lw64, none);
+        GRTM_FBB(float, float, false, false, lw32, fl);
     }
 #undef GRTM_FBB
     return hipGetLastError();
 }
 
 hipError_t launch_cells_fix_batch(int compat, const ModelDesc *md, int m0, const void *lw64, int nw, int nk, double dk,
-                                  double df, double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,
-                                  const unsigned *flag_list, hipStream_t st)
+                                  double df, double omega_i, void *A0, void *A1, RowLayout rl, FlagList fl,
+                                  hipStream_t st)
 {
     dim3 grid(1024);
     if (compat)
         hipLaunchKernelGGL((k_cells_fix_batch<float, true>), grid, dim3(kCellThreads), 0, st, md, m0,
-                           (const LayerW<double> *)lw64, nw, nk, dk, df, omega_i, (float *)A0, (float *)A1, lda,
-                           flag_count, flag_list);
+                           (const LayerW<double> *)lw64, nw, nk, dk, df, omega_i, (float *)A0, (float *)A1, rl, fl);
     else
         hipLaunchKernelGGL((k_cells_fix_batch<float, false>), grid, dim3(kCellThreads), 0, st, md, m0,
-                           (const LayerW<double> *)lw64, nw, nk, dk, df, omega_i, (float *)A0, (float *)A1, lda,
-                           flag_count, flag_list);
+                           (const LayerW<double> *)lw64, nw, nk, dk, df, omega_i, (float *)A0, (float *)A1, rl, fl);
+    return hipGetLastError();
+}
+
+hipError_t launch_plant_nonfinite_batch(const unsigned long long *cells, int n, int nk, void *A0, void *A1,
+                                        RowLayout rl, FlagList fl, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plant_nonfinite_batch, dim3((n + 255) / 256), dim3(256), 0, st, cells, n, nk, (float *)A0,
+                       (float *)A1, rl, fl);
+    return hipGetLastError();
+}
+
+hipError_t launch_clamp_extent(int nrow, int *nk_w, const int *fext, unsigned *over, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_clamp_extent, dim3((nrow + 255) / 256), dim3(256), 0, st, nrow, nk_w, fext, over);
     return hipGetLastError();
 }
 

@@ -148,6 +148,10 @@ struct GemmSetT {
     const int *kend[kGemmSetMax];  // K extent per kKendRows operand rows (null: K)
     int M[kGemmSetMax];            // rows of contraction p
     int rpw[kGemmSetMax];          // rows per frequency row of nk_w
+    // resident batch layout of every contraction's A (null: rows of lda):
+    // row r at foff[r / rpw] + (r % rpw) fext[r / rpw]
+    const long long *foff;
+    const int *fext;
 };
 using GemmSet = GemmSetT<float>;
 
@@ -194,17 +198,23 @@ struct F32Tile {
     // Ends with a barrier, so the shared buffers may be refilled at once.
     static __device__ __forceinline__ void mma(const float *__restrict__ A, const float *__restrict__ B, long lda,
                                                long ldb, int M, int N, int m0, int n0, int kb, int ke, int rpw,
-                                               const int *__restrict__ nk_w, Smem &sm, v16f (&acc)[TM][TN])
+                                               const int *__restrict__ nk_w, Smem &sm, v16f (&acc)[TM][TN],
+                                               const long long *__restrict__ foff = nullptr,
+                                               const int *__restrict__ fext = nullptr)
     {
         const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
         const int wm = wv / (BN / WN), wn = wv % (BN / WN);
         // staging: A item idx = tid + NT e -> row idx >> 3, k offset (idx & 7) * 4;
         // B item -> k row idx / BNV, column offset (idx % BNV) * 4
         int a_lim[AE];
+        size_t a_row[AE];
 #pragma unroll
         for (int e = 0; e < AE; e++) {
             const int gm = m0 + ((tid + NT * e) >> 3);
             a_lim[e] = gm < M ? (nk_w ? min(ke, nk_w[gm / rpw]) : ke) : 0;
+            // resident layout: a row's stored length bounds nk_w (and is a multiple of 4)
+            a_row[e] = foff ? (gm < M ? size_t(foff[gm / rpw]) + size_t(gm % rpw) * size_t(fext[gm / rpw]) : 0)
+                            : size_t(gm) * size_t(lda);
         }
         float4 ra[AE], rb[BE];
         auto load = [&](int k0) {
@@ -213,8 +223,9 @@ struct F32Tile {
                 const int idx = tid + NT * e, gk = k0 + (idx & 7) * 4;
                 float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (gk < a_lim[e]) {
-                    // gk + 3 < lda: gk and lda are multiples of 4 and gk < K <= lda
-                    v = *reinterpret_cast<const float4 *>(A + (size_t)(m0 + (idx >> 3)) * lda + gk);
+                    // gk + 3 < lda (or the row's stored length): both are
+                    // multiples of 4 and gk is below them
+                    v = *reinterpret_cast<const float4 *>(A + a_row[e] + gk);
                     const int lim = a_lim[e] - gk;
                     if (lim < 4) {
                         v.w = 0.f;
@@ -425,7 +436,7 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_f32sk(int n
         const int kt = tile_extent(gp.kend[p], M, K, m0, BM);
         v16f acc[Tl::TM][Tl::TN];
         Tl::mma(gp.A[p], gp.B[p], lda, ldb, M, N, m0, n0, int(lo - t0) * 32, min(kt, int(seg - t0) * 32), gp.rpw[p],
-                nk_w, sm, acc);
+                nk_w, sm, acc, gp.foff, gp.fext);
         const bool whole = lo == t0 && seg == t1;
         float *dst = whole ? gp.C[p] + (size_t)m0 * N + n0 : part + ((size_t)b * 2 + (lo == lo_b ? 0 : 1)) * BM * BN;
         Tl::put(acc, dst, whole ? N : BN, whole ? M - m0 : BM, whole ? N - n0 : BN);
@@ -910,10 +921,16 @@ bool gemm_pair_ok(long lda, long ldb, const void *A0, const void *A1, const void
 
 hipError_t launch_gemm_pair_f32(int M, int N, int K, int kc, const float *A0, const float *A1, long lda,
                                 const float *B0, const float *B1, long ldb, float *C0, float *C1, float *P,
-                                const int *kend, const int *nk_w, hipStream_t st)
+                                const int *kend, const int *nk_w, hipStream_t st, const long long *foff,
+                                const int *fext)
 {
     if (kc < kTBK || kc % kTBK) return hipErrorInvalidValue;
-    return launch_gemm_set_t<float>(2, pair_set(M, A0, A1, B0, B1, C0, C1, kend), N, K, kc, lda, ldb, P, nk_w, st);
+    // the resident layout is read by the stream-K kernel only (per-row extents)
+    if (foff && (!fext || !nk_w || !kend)) return hipErrorInvalidValue;
+    GemmSet gs = pair_set(M, A0, A1, B0, B1, C0, C1, kend);
+    gs.foff = foff;
+    gs.fext = fext;
+    return launch_gemm_set_t<float>(2, gs, N, K, kc, lda, ldb, P, nk_w, st);
 }
 
 hipError_t launch_gemm_set(int prec, int n, const int *M, const int *rpw, int N, int K, int kc, const void *const *A,

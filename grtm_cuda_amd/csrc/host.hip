@@ -2116,9 +2116,19 @@ struct grtm_batch {
     double flops = 0;            // recursion flops of one execution (operation count)
     std::vector<grtm::ModelDesc> md_h;
     DBuf md, lw64, lw32, dists, B0, B1, A0, A1, C0, C1, ws, flags, tr, items;
-    // test hook: cells (group row * nk + k) given non-finite FP32 operands, per group
+    unsigned flag_cap = 0;  // flag list entries (flags: counter, then 64-bit cells)
+    // test hook: cells (group row * nk + k) given non-finite FP32 operands, per
+    // group, from the (model, row, k) triples in plant_h
     DBuf plant;
     std::vector<size_t> plant_off;  // per group: first entry in plant, then the end
+    std::vector<int> plant_h;
+    // resident layout (learned FP32 batches): one group of every model, each
+    // frequency row's operands stored to its final extent (foff, fext), which
+    // an eager pass fixes; rows that would exceed it are cut and counted
+    bool resident = false;
+    int nb_grouped = 0;  // models per group of the grouped layout
+    DBuf foff, fext, ext_over;
+    size_t resident_floats = 0;  // entries of A0 (and of A1)
     DBuf ptab;  // post-processing tables of pp
     int nb = 0;       // models per group
     int bs = 256;     // recursion tile: wavenumbers per block
@@ -2172,6 +2182,27 @@ struct grtm_batch {
 namespace {
 
 bool batch_mixed(const grtm_batch *b) { return b->prec != GRTM_PREC_FP64; }
+
+// Flag list capacity of a batch: every cell of a group, at most 16M entries.
+constexpr size_t kBatchFlagCap = size_t(1) << 24;
+int batch_alloc_flags(grtm_batch *b, size_t extra)
+{
+    if (!batch_mixed(b)) return GRTM_OK;
+    const size_t cap = std::min<size_t>((size_t)b->nb * b->nw * b->nk, kBatchFlagCap) + extra;
+    b->flag_cap = (unsigned)cap;
+    return b->flags.alloc(8 + sizeof(unsigned long long) * cap);
+}
+grtm::FlagList batch_fl(const grtm_batch *b)
+{
+    if (!batch_mixed(b)) return grtm::FlagList{nullptr, nullptr, 0u};
+    return grtm::FlagList{b->flags.as<unsigned>(),
+                          reinterpret_cast<unsigned long long *>(static_cast<char *>(b->flags.p) + 8), b->flag_cap};
+}
+grtm::RowLayout batch_rl(const grtm_batch *b)
+{
+    if (b->resident) return grtm::RowLayout{b->foff.as<long long>(), b->fext.as<int>(), 0};
+    return grtm::RowLayout{nullptr, nullptr, (long)b->nk};
+}
 
 // Frequency rows per K-extent tile of the contraction, and the offset of a
 // group's tile extents (each group's tiles start at its own first row).
@@ -2282,44 +2313,45 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
     const unsigned *dc;
     unsigned *ctr;
     int ni, which;
+    const grtm::RowLayout rl = batch_rl(b);
+    const grtm::FlagList fl = batch_fl(b);
+    // resident layout: one launch per pass for every receiver branch (the
+    // pass's slots are ordered by branch, then by decreasing layer count)
+    auto launch_pass = [&](int pass, int prec, const void *lw, grtm::FlagList f) -> int {
+        if (b->resident) {
+            const unsigned *bounds = b->lseg_h.data() + 9 * gi;
+            if (bounds[pass + 1] > bounds[pass])
+                HIPCHK(grtm::launch_cells_zr_batch(prec, b->compat, b->bs, 0, md, g.m0, lw, b->nw, b->nk, b->dk,
+                                                   b->df, b->omega_i, it, b->litems_cap, b->A0.p, b->A1.p, rl, f, nkw,
+                                                   rowpk, s, nchk, b->chk_stride, tailmx, seg, pass,
+                                                   b->lctr.as<unsigned>() + pass, b->lchunk));
+            return GRTM_OK;
+        }
+        for (int br = 1; br <= 4; br++) {
+            items_of(pass, br, ip, ni, dc, which, ctr);
+            if (ni)
+                HIPCHK(grtm::launch_cells_zr_batch(prec, b->compat, b->bs, br, md, g.m0, lw, b->nw, b->nk, b->dk,
+                                                   b->df, b->omega_i, ip, ni, b->A0.p, b->A1.p, rl, f, nkw, rowpk, s,
+                                                   nchk, b->chk_stride, tailmx, dc, which, ctr, b->lchunk));
+        }
+        return GRTM_OK;
+    };
+    const grtm::FlagList none{nullptr, nullptr, 0u};
+    int pst;
     if (!batch_mixed(b)) {
-        for (int br = 1; br <= 4; br++) {
-            items_of(0, br, ip, ni, dc, which, ctr);
-            if (ni)
-                HIPCHK(grtm::launch_cells_zr_batch(0, b->compat, b->bs, br, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk,
-                                                   b->df, b->omega_i, ip, ni, b->A0.p, b->A1.p, b->nk, nullptr,
-                                                   nullptr, nkw, rowpk, s, nchk, b->chk_stride, tailmx, dc, which,
-                                                   ctr, b->lchunk));
-        }
+        if ((pst = launch_pass(0, 0, b->lw64.p, none))) return pst;
     } else {
-        unsigned *cnt = b->flags.as<unsigned>(), *list = cnt + 1;
-        HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
-        for (int br = 1; br <= 4; br++) {
-            items_of(0, br, ip, ni, dc, which, ctr);
-            if (ni)
-                HIPCHK(grtm::launch_cells_zr_batch(2, b->compat, b->bs, br, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk,
-                                                   b->df, b->omega_i, ip, ni, b->A0.p, b->A1.p, b->nk, nullptr,
-                                                   nullptr, nkw, rowpk, s, nchk, b->chk_stride, tailmx, dc, which,
-                                                   ctr, b->lchunk));
-        }
-        for (int br = 1; br <= 4; br++) {
-            items_of(1, br, ip, ni, dc, which, ctr);
-            if (ni)
-                HIPCHK(grtm::launch_cells_zr_batch(1, b->compat, b->bs, br, md, g.m0, b->lw32.p, b->nw, b->nk, b->dk,
-                                                   b->df, b->omega_i, ip, ni, b->A0.p, b->A1.p, b->nk, cnt, list, nkw,
-                                                   rowpk, s, nchk, b->chk_stride, tailmx, dc, which, ctr,
-                                                   b->lchunk));
-        }
+        HIPCHK(grtm::launch_zero_u32(fl.count, 1, s));
+        if ((pst = launch_pass(0, 2, b->lw64.p, none)) || (pst = launch_pass(1, 1, b->lw32.p, fl))) return pst;
         if (!b->plant_off.empty() && b->plant_off[gi + 1] > b->plant_off[gi])
-            HIPCHK(grtm::launch_plant_nonfinite(b->plant.as<unsigned>() + b->plant_off[gi],
-                                                int(b->plant_off[gi + 1] - b->plant_off[gi]), b->nk, b->A0.p,
-                                                b->A1.p, b->nk, cnt, list, s));
+            HIPCHK(grtm::launch_plant_nonfinite_batch(b->plant.as<unsigned long long>() + b->plant_off[gi],
+                                                      int(b->plant_off[gi + 1] - b->plant_off[gi]), b->nk, b->A0.p,
+                                                      b->A1.p, rl, fl, s));
     }
-    unsigned *cnt = batch_mixed(b) ? b->flags.as<unsigned>() : nullptr, *list = cnt ? cnt + 1 : nullptr;
     if (!b->adaptive) {
-        if (cnt)
+        if (fl.count)
             HIPCHK(grtm::launch_cells_fix_batch(b->compat, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df,
-                                                b->omega_i, b->A0.p, b->A1.p, b->nk, cnt, list, s));
+                                                b->omega_i, b->A0.p, b->A1.p, rl, fl, s));
         return GRTM_OK;
     }
     // dense fallback of the group's rows whose check failed (each row in its
@@ -2329,10 +2361,11 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
     HIPCHK(grtm::launch_fallback_batch(batch_mixed(b) ? 1 : 0, b->compat, md, g.m0, b->lw64.p, b->lw32.p, (int)nrow,
                                        b->nw, b->nk, b->dk, b->df, b->omega_i, b->chk_tol, rowpk, tailmx, nkw, gcount,
                                        b->fb_list.as<unsigned>() + 2 * r0, b->fb_count.as<unsigned>() + g.m0, b->A0.p,
-                                       b->A1.p, b->nk, cnt, list, s));
-    if (cnt)
+                                       b->A1.p, rl, fl, s));
+    if (fl.count)
         HIPCHK(grtm::launch_cells_fix_batch(b->compat, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df, b->omega_i,
-                                            b->A0.p, b->A1.p, b->nk, cnt, list, s));
+                                            b->A0.p, b->A1.p, rl, fl, s));
+    if (b->resident) HIPCHK(grtm::launch_clamp_extent((int)nrow, nkw, b->fext.as<int>(), b->ext_over.as<unsigned>(), s));
     HIPCHK(grtm::launch_kend_tiles((int)nrow, batch_rpt(b), b->nk, nkw, b->kend.as<int>() + batch_kend_off(b, g), s));
     return GRTM_OK;
 }
@@ -2346,7 +2379,9 @@ int batch_gemm(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
     if (b->pair && b->gemm_mode == 1) {
         HIPCHK(grtm::launch_gemm_pair_f32(M, b->nr, b->nk, b->kc, b->A0.as<float>(), b->A1.as<float>(), b->nk,
                                           b->B0.as<float>(), b->B1.as<float>(), b->nr, b->C0.as<float>(),
-                                          b->C1.as<float>(), b->ws.as<float>(), kend, nkw, s));
+                                          b->C1.as<float>(), b->ws.as<float>(), kend, nkw, s,
+                                          b->resident ? b->foff.as<long long>() : nullptr,
+                                          b->resident ? b->fext.as<int>() : nullptr));
         return GRTM_OK;
     }
     if (b->pair && b->gemm_mode == 0) {
@@ -2435,6 +2470,210 @@ int batch_autotune(grtm_batch *b)
     }
     b->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return GRTM_OK;
+}
+
+// The planted cells (test hook) as per-group lists of group row nk + k, and
+// room for them in the flag list.
+int batch_upload_plant(grtm_batch *b)
+{
+    const size_t n = b->plant_h.size() / 3;
+    std::vector<std::vector<unsigned long long>> per(b->groups.size());
+    for (size_t i = 0; i < n; i++) {
+        const int m = b->plant_h[3 * i], iw = b->plant_h[3 * i + 1], ik = b->plant_h[3 * i + 2];
+        size_t gi = 0;
+        while (gi + 1 < b->groups.size() && m >= b->groups[gi + 1].m0) gi++;
+        const int j = m - b->groups[gi].m0;
+        per[gi].push_back((unsigned long long)(j * b->nw + iw) * (unsigned long long)b->nk + (unsigned long long)ik);
+    }
+    std::vector<unsigned long long> all;
+    std::vector<size_t> off(1, 0);
+    for (const auto &v : per) {
+        all.insert(all.end(), v.begin(), v.end());
+        off.push_back(all.size());
+    }
+    int st;
+    if ((st = batch_alloc_flags(b, n))) return st;
+    if ((st = b->plant.alloc(sizeof(unsigned long long) * std::max<size_t>(1, all.size())))) return st;
+    if (!all.empty())
+        HIPCHK(hipMemcpy(b->plant.p, all.data(), sizeof(unsigned long long) * all.size(), hipMemcpyHostToDevice));
+    b->plant_off = n ? off : std::vector<size_t>();
+    return GRTM_OK;
+}
+
+// The learned grids' slot order of every group: per pass and receiver branch
+// the group's models, heaviest first, with their rows' tiles, then their rows'
+// check cells.  Segment s = pass * 4 + branch - 1 of a group spans slots
+// [lseg_h[9 g + s], lseg_h[9 g + s + 1]).  A resident batch has one segment
+// per pass (one launch runs every branch): bounds [0, end of pass 0, total...].
+int batch_learned_slots(grtm_batch *b)
+{
+    int st;
+    const int ng = (int)b->groups.size();
+    std::vector<unsigned> slots, segs(9 * (size_t)ng);
+    b->lslot0.assign(ng, 0);
+    b->lnslots.assign(ng, 0);
+    int maxslots = 0;
+    for (int gi = 0; gi < ng; gi++) {
+        const grtm_batch::Group &g = b->groups[gi];
+        b->lslot0[gi] = slots.size();
+        std::vector<int> order(g.n);
+        for (int j = 0; j < g.n; j++) order[j] = j;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->L[g.m0 + x] > b->L[g.m0 + y]; });
+        for (int pass = 0; pass < 2; pass++) {
+            if (b->resident) segs[9 * (size_t)gi + pass] = unsigned(slots.size() - b->lslot0[gi]);
+            for (int br = 1; br <= 4; br++) {
+                if (!b->resident) segs[9 * (size_t)gi + pass * 4 + br - 1] = unsigned(slots.size() - b->lslot0[gi]);
+                for (int phase = 0; phase < 2; phase++)
+                    for (int j : order) {
+                        if (b->md_h[g.m0 + j].g.branch != br) continue;
+                        const int r64 = batch_mixed(b) ? b->rows64[g.m0 + j] : b->nw;
+                        const int lo = pass == 0 ? 0 : r64, hi = pass == 0 ? r64 : b->nw;
+                        for (int iw = lo; iw < hi; iw++)
+                            slots.push_back(unsigned(j * b->nw + iw) | (phase ? grtm::kCheckItem : 0u));
+                    }
+            }
+        }
+        b->lnslots[gi] = int(slots.size() - b->lslot0[gi]);
+        for (int q = b->resident ? 2 : 8; q <= 8; q++) segs[9 * (size_t)gi + q] = unsigned(b->lnslots[gi]);
+        maxslots = std::max(maxslots, b->lnslots[gi]);
+    }
+    const int tpb = (b->nk + b->bs - 1) / b->bs;  // items of one slot at most
+    const size_t cap = (size_t)maxslots * tpb;
+    if (cap > 0x7fffffffu || (b->resident && (size_t)b->nm * b->nw >= (size_t(1) << 31) / 4))
+        return GRTM_E_UNSUPPORTED;
+    if ((st = b->lslots.alloc(sizeof(unsigned) * std::max<size_t>(1, slots.size()))) ||
+        (st = b->lslot_off.alloc(sizeof(unsigned) * std::max(1, maxslots))) ||
+        (st = b->lseg_slots.alloc(sizeof(unsigned) * segs.size())) ||
+        (st = b->lseg_items.alloc(sizeof(unsigned) * segs.size())) || (st = b->lctr.alloc(sizeof(unsigned) * 8)) ||
+        (st = b->litems.alloc(sizeof(unsigned long long) * std::max<size_t>(1, cap))))
+        return st;
+    if (!slots.empty()) HIPCHK(hipMemcpy(b->lslots.p, slots.data(), sizeof(unsigned) * slots.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->lseg_slots.p, segs.data(), sizeof(unsigned) * segs.size(), hipMemcpyHostToDevice));
+    b->lseg_h = segs;
+    b->litems_cap = (int)cap;
+    return GRTM_OK;
+}
+
+// The device-built grids of the last execution into nk_pred_h / nchk_h.
+int batch_pred_to_host(grtm_batch *b)
+{
+    if (!b->net || !b->pred_stale) return GRTM_OK;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const size_t n = (size_t)b->nm * b->nw;
+    HIPCHK(hipMemcpy(b->nk_pred_h.data(), b->nk_pred.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(b->nchk_h.data(), b->nchk.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    b->pred_stale = false;
+    return GRTM_OK;
+}
+
+// Back to the grouped layout (dense operand rows for nb_grouped models per
+// group); the caller rebuilds items or slots and captures the graph again.
+int batch_make_grouped(grtm_batch *b)
+{
+    if (!b->resident) return GRTM_OK;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->resident = false;
+    b->resident_floats = 0;
+    b->foff.release();
+    b->fext.release();
+    b->ext_over.release();
+    b->A0.release();
+    b->A1.release();
+    b->C0.release();
+    b->C1.release();
+    b->nb = b->nb_grouped;
+    b->groups.clear();
+    for (int m0 = 0; m0 < b->nm; m0 += b->nb) {
+        grtm_batch::Group g{};
+        g.m0 = m0;
+        g.n = std::min(b->nb, b->nm - m0);
+        b->groups.push_back(g);
+    }
+    const size_t arows = (size_t)4 * b->nw * b->nb, es = esize(b->prec);
+    int gmax = 0;
+    for (const auto &gr : b->groups) gmax = std::max(gmax, gr.n);
+    const int M2[2] = {4 * b->nw * gmax, 4 * b->nw * gmax};
+    size_t wsb = std::max(grtm::gemm_workspace_bytes(b->gemm_mode, (int)arows, b->nr, b->nk),
+                          grtm::gemm_pair_workspace_bytes((int)arows, b->nr, b->nk, std::max(256, b->kc / 2)) * es / 4);
+    if (b->gemm_mode == 1) wsb = std::max(wsb, grtm::gemm_streamk_workspace_bytes(2, M2, b->nr, b->nk));
+    int st;
+    if ((st = b->A0.alloc(arows * b->nk * es)) || (st = b->A1.alloc(arows * b->nk * es)) ||
+        (st = b->C0.alloc(arows * b->nr * es)) || (st = b->C1.alloc(arows * b->nr * es)) ||
+        (st = b->ws.alloc(wsb)) || (st = b->fb_gcount.alloc(sizeof(unsigned) * b->groups.size())) ||
+        (st = b->kend.alloc(sizeof(int) * b->groups.size() * batch_kend_per_group(b))))
+        return st;
+    return batch_upload_plant(b);
+}
+
+// The resident layout for a learned FP32 batch whose grids the last pass
+// formed: row r keeps fext[r] = its final extent rounded up to 4 wavenumbers
+// (nk_w after the fallback), one group of every model, the branches of a
+// pass in one recursion launch.  It stays grouped when the layout does not
+// fit the device with 4 GiB to spare.
+int batch_make_resident(grtm_batch *b)
+{
+    if (b->resident || !b->adaptive || !b->net || !batch_mixed(b) || b->gemm_mode != 1 || !b->pair) return GRTM_OK;
+    const size_t n = (size_t)b->nm * b->nw, nrows = 4 * n;
+    if (nrows >= (size_t(1) << 31)) return GRTM_OK;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    std::vector<int> nkw(n), ext(n);
+    std::vector<long long> off(n);
+    HIPCHK(hipMemcpy(nkw.data(), b->nk_w.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    size_t tot = 0;
+    for (size_t r = 0; r < n; r++) {
+        ext[r] = std::max(4, (std::min(nkw[r], b->nk) + 3) / 4 * 4);
+        off[r] = (long long)tot;
+        tot += 4 * (size_t)ext[r];
+    }
+    const int M2[2] = {(int)nrows, (int)nrows};
+    const size_t ws_need = grtm::gemm_streamk_workspace_bytes(2, M2, b->nr, b->nk);
+    const size_t a_bytes = (tot + 64) * sizeof(float);  // 64 entries past the last row (16-byte loads)
+    const size_t c_bytes = nrows * b->nr * sizeof(float);
+    const int tpb = (b->nk + b->bs - 1) / b->bs;
+    const size_t need = 2 * a_bytes + 2 * c_bytes + ws_need + n * (sizeof(long long) + sizeof(int)) +
+                        2 * n * tpb * sizeof(unsigned long long) + (kBatchFlagCap + 1) * sizeof(unsigned long long);
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    const size_t held = b->A0.n + b->A1.n + b->C0.n + b->C1.n + b->ws.n + b->litems.n + b->flags.n;
+    if (need + (size_t(4) << 30) > free_b + held) return GRTM_OK;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->A0.release();
+    b->A1.release();
+    b->C0.release();
+    b->C1.release();
+    b->ws.release();
+    b->litems.release();
+    int st;
+    if ((st = b->A0.alloc(a_bytes)) || (st = b->A1.alloc(a_bytes)) || (st = b->C0.alloc(c_bytes)) ||
+        (st = b->C1.alloc(c_bytes)) || (st = b->ws.alloc(ws_need)) ||
+        (st = b->foff.alloc(sizeof(long long) * n)) || (st = b->fext.alloc(sizeof(int) * n)) ||
+        (st = b->ext_over.alloc(sizeof(unsigned))))
+        return st;
+    HIPCHK(hipMemcpy(b->foff.p, off.data(), sizeof(long long) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->fext.p, ext.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(b->ext_over.p, 0, sizeof(unsigned)));  // set-up, outside any graph
+    b->resident = true;
+    b->resident_floats = tot;
+    b->nb = b->nm;
+    b->groups.clear();
+    grtm_batch::Group g{};
+    g.m0 = 0;
+    g.n = b->nm;
+    b->groups.push_back(g);
+    if ((st = b->fb_gcount.alloc(sizeof(unsigned))) ||
+        (st = b->kend.alloc(sizeof(int) * batch_kend_per_group(b))) || (st = batch_upload_plant(b)) ||
+        (st = batch_learned_slots(b)))
+        return st;
+    // the resident layout's own eager pass (the host's view of the grids)
+    if ((st = batch_group(b, b->groups[0], b->stream))) return st;
+    return batch_pred_to_host(b);
 }
 
 int batch_capture(grtm_batch *b)
@@ -2543,9 +2782,7 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
     size_t nb = std::max<size_t>(1, budget / per_model);
     nb = std::min<size_t>(nb, 64);
     nb = std::min<size_t>(nb, (size_t)n_models);
-    // flag entries (j nw + iw) nk + ik must fit 32 bits
-    while (nb > 1 && (double)nb * b->nw * b->nk >= 4294967295.0) nb--;
-    b->nb = (int)nb;
+    b->nb = b->nb_grouped = (int)nb;
     for (int m0 = 0; m0 < n_models; m0 += b->nb)
         b->groups.push_back({m0, std::min(b->nb, n_models - m0), 0, 0, 0, 0});
 
@@ -2555,8 +2792,7 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
         (st = b->A1.alloc(arows * b->nk * es)) || (st = b->C0.alloc(arows * nr * es)) ||
         (st = b->C1.alloc(arows * nr * es)) || (st = b->tr.alloc((size_t)n_models * 4 * nr * b->N * es)))
         return st;
-    if (precision != GRTM_PREC_FP64 && (st = b->flags.alloc(sizeof(unsigned) * ((size_t)b->nb * b->nw * b->nk + 1))))
-        return st;
+    if ((st = batch_alloc_flags(b, 0))) return st;
     size_t wsb = grtm::gemm_workspace_bytes(b->gemm_mode, (int)arows, nr, b->nk);
     if (b->gemm_mode != 2 && grtm::gemm_pair_ok(b->nk, nr, b->A0.p, b->A1.p, b->B0.p, b->B1.p)) {
         b->pair = true;
@@ -2585,18 +2821,22 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
 int grtm_batch_set_adaptive(grtm_batch *b, const double *kcut, const double *window, int stride, double tol)
 {
     if (!b) return GRTM_E_INPUT;
-    b->net = nullptr;  // host cuts replace learned grids
-    b->pred_stale = false;
     int st;
-    if (!kcut) {
-        b->adaptive = false;
-    } else {
+    if (kcut) {
         if (!window || stride < 1 || !(tol > 0)) return GRTM_E_INPUT;
         const size_t n = (size_t)b->nm * b->nw;
         for (int m = 0; m < b->nm; m++)  // validated before any batch state changes
             if (!(window[m] >= 0)) return GRTM_E_INPUT;
         for (size_t i = 0; i < n; i++)
             if (!(kcut[i] >= 0)) return GRTM_E_INPUT;
+    }
+    b->net = nullptr;  // host cuts replace learned grids
+    b->pred_stale = false;
+    if (b->resident && (st = batch_make_grouped(b))) return st;  // host-built items run grouped
+    if (!kcut) {
+        b->adaptive = false;
+    } else {
+        const size_t n = (size_t)b->nm * b->nw;
         b->nk_pred_h.assign(n, 0);
         b->nchk_h.assign(n, 0);
         b->max_chk.assign(b->nm, 0);
@@ -2643,17 +2883,6 @@ int grtm_batch_set_adaptive(grtm_batch *b, const double *kcut, const double *win
     return batch_capture(b);
 }
 
-// The device-built grids of the last execution into nk_pred_h / nchk_h.
-static int batch_pred_to_host(grtm_batch *b)
-{
-    if (!b->net || !b->pred_stale) return GRTM_OK;
-    HIPCHK(hipStreamSynchronize(b->stream));
-    const size_t n = (size_t)b->nm * b->nw;
-    HIPCHK(hipMemcpy(b->nk_pred_h.data(), b->nk_pred.p, sizeof(int) * n, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(b->nchk_h.data(), b->nchk.p, sizeof(int) * n, hipMemcpyDeviceToHost));
-    b->pred_stale = false;
-    return GRTM_OK;
-}
 
 int grtm_batch_set_learned(grtm_batch *b, grtm_mlp *net, double lmargin, const double *window, int stride, double tol)
 {
@@ -2666,6 +2895,9 @@ int grtm_batch_set_learned(grtm_batch *b, grtm_mlp *net, double lmargin, const d
         if ((int)b->layers_h[m].size() / 6 > grtm::kLearnSlots) return GRTM_E_UNSUPPORTED;
     }
     int st;
+    // the grids are formed on the grouped layout first (its eager pass sizes
+    // the resident one)
+    if (b->resident && (st = batch_make_grouped(b))) return st;
     const size_t n = (size_t)b->nm * b->nw;
     if (!b->nk_pred.p) {
         if ((st = b->nk_pred.alloc(sizeof(int) * n)) || (st = b->nk_w.alloc(sizeof(int) * n)) ||
@@ -2699,50 +2931,9 @@ int grtm_batch_set_learned(grtm_batch *b, grtm_mlp *net, double lmargin, const d
         d.rcv = b->rcv_h[m];
         d.window = window[m];
     }
-    // each group's slot order: per (pass, branch) its models heaviest first,
-    // their rows' tiles, then their rows' check cells (the host lists' order)
-    const int ng = (int)b->groups.size();
-    std::vector<unsigned> slots, segs(9 * (size_t)ng);
-    b->lslot0.assign(ng, 0);
-    b->lnslots.assign(ng, 0);
-    int maxslots = 0;
-    for (int gi = 0; gi < ng; gi++) {
-        const grtm_batch::Group &g = b->groups[gi];
-        b->lslot0[gi] = slots.size();
-        std::vector<int> order(g.n);
-        for (int j = 0; j < g.n; j++) order[j] = j;
-        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->L[g.m0 + x] > b->L[g.m0 + y]; });
-        for (int pass = 0; pass < 2; pass++)
-            for (int br = 1; br <= 4; br++) {
-                segs[9 * (size_t)gi + pass * 4 + br - 1] = unsigned(slots.size() - b->lslot0[gi]);
-                for (int phase = 0; phase < 2; phase++)
-                    for (int j : order) {
-                        if (b->md_h[g.m0 + j].g.branch != br) continue;
-                        const int r64 = batch_mixed(b) ? b->rows64[g.m0 + j] : b->nw;
-                        const int lo = pass == 0 ? 0 : r64, hi = pass == 0 ? r64 : b->nw;
-                        for (int iw = lo; iw < hi; iw++)
-                            slots.push_back(unsigned(j * b->nw + iw) | (phase ? grtm::kCheckItem : 0u));
-                    }
-            }
-        b->lnslots[gi] = int(slots.size() - b->lslot0[gi]);
-        segs[9 * (size_t)gi + 8] = unsigned(b->lnslots[gi]);
-        maxslots = std::max(maxslots, b->lnslots[gi]);
-    }
-    const int tpb = (b->nk + b->bs - 1) / b->bs;  // items of one slot at most
-    const size_t cap = (size_t)maxslots * tpb;
-    if (cap > 0x7fffffffu) return GRTM_E_UNSUPPORTED;
-    if ((st = b->lmodels.alloc(sizeof(grtm::LearnModel) * b->nm)) ||
-        (st = b->lslots.alloc(sizeof(unsigned) * std::max<size_t>(1, slots.size()))) ||
-        (st = b->lslot_off.alloc(sizeof(unsigned) * std::max(1, maxslots))) ||
-        (st = b->lseg_slots.alloc(sizeof(unsigned) * segs.size())) ||
-        (st = b->lseg_items.alloc(sizeof(unsigned) * segs.size())) || (st = b->lctr.alloc(sizeof(unsigned) * 8)) ||
-        (st = b->litems.alloc(sizeof(unsigned long long) * std::max<size_t>(1, cap))))
-        return st;
+    if ((st = b->lmodels.alloc(sizeof(grtm::LearnModel) * b->nm))) return st;
     HIPCHK(hipMemcpy(b->lmodels.p, lm.data(), sizeof(grtm::LearnModel) * b->nm, hipMemcpyHostToDevice));
-    if (!slots.empty()) HIPCHK(hipMemcpy(b->lslots.p, slots.data(), sizeof(unsigned) * slots.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(b->lseg_slots.p, segs.data(), sizeof(unsigned) * segs.size(), hipMemcpyHostToDevice));
-    b->lseg_h = segs;
-    b->litems_cap = (int)cap;
+    if ((st = batch_learned_slots(b))) return st;
     b->lg = grtm::LearnGrid{b->nw, b->nk, stride, 0, b->pp.dt, b->dk, b->dk * b->nk, lmargin};
     b->chk_stride = stride;
     b->chk_tol = tol;
@@ -2797,6 +2988,13 @@ int grtm_batch_set_learned(grtm_batch *b, grtm_mlp *net, double lmargin, const d
             return st;
         }
     if ((st = batch_pred_to_host(b))) return st;
+    // FP32 batches: one group of every model with each row stored to the
+    // extent this pass reached, when it fits the device (else the groups stay)
+    if ((st = batch_make_resident(b))) {
+        b->net = nullptr;
+        b->adaptive = false;
+        return st;
+    }
     return batch_capture(b);
 }
 
@@ -2823,27 +3021,14 @@ int grtm_batch_plant_nonfinite(grtm_batch *b, const int *cells, int n)
     if (!b || n < 0 || n > GRTM_MAX_PLANT || (n && !cells)) return GRTM_E_INPUT;
     if (!batch_mixed(b)) return GRTM_E_UNSUPPORTED;
     const int nm = (int)b->L.size();
-    std::vector<std::vector<unsigned>> per(b->groups.size());
     for (int i = 0; i < n; i++) {
         const int m = cells[3 * i], iw = cells[3 * i + 1], ik = cells[3 * i + 2];
         if (m < 0 || m >= nm || iw < b->rows64[m] || iw >= b->nw || ik < 0 || ik >= b->nk) return GRTM_E_INPUT;
-        const size_t gi = (size_t)(m / b->nb);
-        const int j = m - b->groups[gi].m0;
-        per[gi].push_back(unsigned(j * b->nw + iw) * unsigned(b->nk) + unsigned(ik));
-    }
-    std::vector<unsigned> all;
-    std::vector<size_t> off(1, 0);
-    for (const auto &v : per) {
-        all.insert(all.end(), v.begin(), v.end());
-        off.push_back(all.size());
     }
     HIPCHK(hipStreamSynchronize(b->stream));
+    b->plant_h.assign(cells, cells + 3 * (size_t)n);
     int st;
-    if ((st = b->flags.alloc(sizeof(unsigned) * ((size_t)b->nb * b->nw * b->nk + 1 + n)))) return st;
-    if ((st = b->plant.alloc(sizeof(unsigned) * std::max<size_t>(1, all.size())))) return st;
-    if (!all.empty())
-        HIPCHK(hipMemcpy(b->plant.p, all.data(), sizeof(unsigned) * all.size(), hipMemcpyHostToDevice));
-    b->plant_off = n ? off : std::vector<size_t>();
+    if ((st = batch_upload_plant(b))) return st;
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
@@ -2939,6 +3124,14 @@ int grtm_batch_config(grtm_batch *b, double *out)
     out[5] = b->tuned;
     out[6] = b->tune_ms;
     out[7] = b->lchunk;
+    out[8] = b->resident ? 1 : 0;
+    out[9] = 0;
+    if (b->resident) {
+        unsigned over = 0;
+        HIPCHK(hipStreamSynchronize(b->stream));
+        HIPCHK(hipMemcpy(&over, b->ext_over.p, sizeof(unsigned), hipMemcpyDeviceToHost));
+        out[9] = over;
+    }
     return GRTM_OK;
 }
 

@@ -167,9 +167,12 @@ hipError_t launch_gemm_mfma(int mode, int M, int N, int K, const void *A, long l
 int gemm_pair_slice(int mode, int M, int N, int K, double k_avg);
 size_t gemm_pair_workspace_bytes(int M, int N, int K, int kc);
 bool gemm_pair_ok(long lda, long ldb, const void *A0, const void *A1, const void *B0, const void *B1);
+// foff / fext (optional, with nk_w): A rows in the resident batch layout
+// (RowLayout: row r at foff[r / 4] + (r % 4) fext[r / 4]) instead of r lda.
 hipError_t launch_gemm_pair_f32(int M, int N, int K, int kc, const float *A0, const float *A1, long lda,
                                 const float *B0, const float *B1, long ldb, float *C0, float *C1, float *P,
-                                const int *kend, const int *nk_w, hipStream_t st);
+                                const int *kend, const int *nk_w, hipStream_t st, const long long *foff = nullptr,
+                                const int *fext = nullptr);
 // The same in FP64 (128 x 128 tiles, kend per 128-row tile; workspace
 // 2 x gemm_pair_workspace_bytes(M, N, K, kc)).
 hipError_t launch_gemm_pair_f64(int M, int N, int K, int kc, const double *A0, const double *A1, long lda,
@@ -214,19 +217,36 @@ struct ModelDesc {
     long lw_off;
     long r64;  // frequency rows [0, r64) solved in FP64 (all rows of an FP64 batch)
 };
+// Where the A operand rows of a batch group live.  Dense: group frequency row
+// g has its four component rows at (4 g + i) lda.  Resident (foff set): row g
+// has them at foff[g] + i fext[g], fext[g] a multiple of 4 wavenumbers; cells
+// at or past fext[g] are not stored.
+struct RowLayout {
+    const long long *foff;
+    const int *fext;
+    long lda;
+};
+// Flag list of a batch's non-finite FP32 cells: *count entries (those below
+// cap stored) of (group row) nk + k.
+struct FlagList {
+    unsigned *count;
+    unsigned long long *list;
+    unsigned cap;
+};
 // Recursion of a group of models; items (j << 32) | cells_tile(row, tile)
 // with tiles of bs in {64, 128, 256} wavenumbers; prec as launch_cells_zr;
-// every item's model has receiver branch br (1-4).
+// every item's model has receiver branch br (1-4), or br = 0: each item's own
+// (the kernel holds every branch's solve).
 // dev_count: persistent form over items [dev_count[which], dev_count[which + 1])
 // of a list built on the device (nitems: an upper bound), taken `chunk` at a
 // time from counter *ctr.
 hipError_t launch_cells_zr_batch(int prec, int compat, int bs, int br, const ModelDesc *md, int m0, const void *lw_all,
                                  int nw, int nk, double dk, double df, double omega_i,
-                                 const unsigned long long *items, int nitems, void *A0, void *A1, long lda,
-                                 unsigned *flag_count, unsigned *flag_list, const int *nk_w, unsigned *row_peak,
-                                 hipStream_t st, const int *nchk = nullptr, int stride = 0,
-                                 unsigned *tail_max = nullptr, const unsigned *dev_count = nullptr, int which = 0,
-                                 unsigned *ctr = nullptr, int chunk = 1);
+                                 const unsigned long long *items, int nitems, void *A0, void *A1, RowLayout rl,
+                                 FlagList fl, const int *nk_w, unsigned *row_peak, hipStream_t st,
+                                 const int *nchk = nullptr, int stride = 0, unsigned *tail_max = nullptr,
+                                 const unsigned *dev_count = nullptr, int which = 0, unsigned *ctr = nullptr,
+                                 int chunk = 1);
 // A group's learned-grid items from its slot order (batch.hip k_batch_scan,
 // k_batch_fill): slots[q] = group row | kCheckItem for its check items;
 // segment s covers slots [seg_slots[s], seg_slots[s + 1]) and gets items
@@ -242,10 +262,18 @@ hipError_t launch_batch_items(int nslots, const unsigned *slots, int nw, int bs,
 hipError_t launch_fallback_batch(int prec, int compat, const ModelDesc *md, int m0, const void *lw64, const void *lw32,
                                  int nrow, int nw, int nk, double dk, double df, double omega_i, double tol,
                                  const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *count,
-                                 unsigned *list, unsigned *model_count, void *A0, void *A1, long lda,
-                                 unsigned *flag_count, unsigned *flag_list, hipStream_t st);
+                                 unsigned *list, unsigned *model_count, void *A0, void *A1, RowLayout rl,
+                                 FlagList fl, hipStream_t st);
 hipError_t launch_cells_fix_batch(int compat, const ModelDesc *md, int m0, const void *lw64, int nw, int nk, double dk,
-                                  double df, double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,
-                                  const unsigned *flag_list, hipStream_t st);
+                                  double df, double omega_i, void *A0, void *A1, RowLayout rl, FlagList fl,
+                                  hipStream_t st);
+// Test hook: non-finite FP32 operands in the listed cells (group row nk + k),
+// each appended to the flag list.
+hipError_t launch_plant_nonfinite_batch(const unsigned long long *cells, int n, int nk, void *A0, void *A1,
+                                        RowLayout rl, FlagList fl, hipStream_t st);
+// Resident layout: a row's final extent beyond its stored length (a fallback
+// the layout was not sized for) is cut to the stored length and counted in
+// *over.
+hipError_t launch_clamp_extent(int nrow, int *nk_w, const int *fext, unsigned *over, hipStream_t st);
 
 }  // namespace grtm

@@ -263,7 +263,9 @@ int grtm_batch_timing(grtm_batch *batch, int reps, double *ms);
  * [2] contraction K slice (0: split-K plan of the FP64 / BF16x3 kernels),
  * [3] groups, [4] graph nodes, [5] autotune candidates timed, [6] autotune ms,
  * [7] items a persistent recursion block takes at a time (learned grids
- * formed on the device).  8 doubles. */
+ * formed on the device), [8] 1 for the resident layout, [9] rows whose extent
+ * it had to cut since it was set up (0 unless a later pass fell back on rows
+ * the set-up pass did not).  10 doubles. */
 int grtm_batch_config(grtm_batch *batch, double *out);
 /* out[0] cells, [1] cell-layers, [2] contraction flops, [3] traces,
  * [4] recursion flops of the dense grid with full upward sweeps (operation
@@ -370,7 +372,11 @@ int grtm_plan_set_learned(grtm_plan *plan, grtm_mlp *net, double lmargin, double
  * one forward of the network over the group's (model, frequency) rows, the
  * cuts and check counts, the group's item list, persistent recursion
  * launches.  window[m] per model; batches with the streamed contraction
- * (FP64, FP32).  The graph is captured again. */
+ * (FP64, FP32).  The graph is captured again.  An FP32 batch then takes the
+ * resident layout when it fits the device: one group of every model, each
+ * frequency row's operands stored up to the extent the set-up pass reached
+ * (its cut, or nk where it fell back), one recursion launch per precision pass
+ * for every receiver branch, one contraction, one inverse FFT. */
 int grtm_batch_set_learned(grtm_batch *batch, grtm_mlp *net, double lmargin, const double *window, int stride,
                            double tol);
 

@@ -81,3 +81,37 @@ def test_planted_cells_in_a_batch():
         e = rel_l2(tr, ref64).max()
         print(f"  replay {rep}: {len(cells)} planted, vs unplanted {d:.2e}, vs FP64 {e:.2e}")
         assert d < 1e-5 and e < 1e-4
+
+
+def test_planted_cells_in_a_resident_learned_batch():
+    """The learned FP32 batch in its resident layout (one group, rows stored to
+    their extents, 64-bit flag list): planted cells below the cuts come out
+    re-solved, the traces within 1e-5 of the unplanted batch."""
+    from grtm_cuda_amd import learned
+    ms = models.batch_models(77, 7, n_min=4, n_max=12, src_lo=2.0, src_hi=30.0)
+    vs_min = min(float(l[:, 2].min()) for l, _ in ms)
+    grid = models.dense_grid(128, 0.1, 1024, vs_min)
+    dists = np.linspace(5.0, 120.0, 8)
+    print(f"gpu: planted non-finite cells, resident learned batch of {len(ms)} models", flush=True)
+    pred = learned.Predictor()
+    bp = g.BatchPlan(ms, 0.0, dists, grid, precision=g.PREC_FP32)
+    bp.set_learned(pred)
+    cfg = bp.config()
+    assert cfg["resident"] and cfg["groups"] == 1 and cfg["rows_cut"] == 0
+    bp.execute()
+    base = bp.traces()
+    rng = np.random.default_rng(10)
+    cells = []
+    for m in (1, 4):
+        rows64 = g.TracePlan(ms[m][0], ms[m][1], 0.0, dists, grid, precision=g.PREC_FP32).work()["rows_fp64"]
+        cells += [(m, int(r), int(rng.integers(0, 8))) for r in rng.choice(np.arange(rows64, grid["nw"]), 30,
+                                                                             replace=False)]
+    bp.plant_nonfinite(cells)
+    assert bp.config()["resident"]
+    for rep in range(2):
+        bp.execute()
+        tr = bp.traces()
+        assert np.isfinite(tr).all()
+        d = rel_l2(tr, base).max()
+        print(f"  replay {rep}: {len(cells)} planted, vs unplanted {d:.2e}")
+        assert d < 1e-5
