@@ -367,8 +367,8 @@ def main():
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16x3"],
                     help="arithmetic of the measured path (default fp32 adaptive / fp64 dense)")
     ap.add_argument("--derivatives", action="store_true", help="mt: the d/dz and d/dr columns as well")
-    ap.add_argument("--cpu-rows", type=int, default=64, help="CPU baseline sample: frequency rows on all cores")
-    ap.add_argument("--cpu-rows-1core", type=int, default=8, help="CPU baseline sample: frequency rows on one core")
+    ap.add_argument("--cpu-rows", type=int, default=512, help="CPU baseline sample: frequency rows on all cores")
+    ap.add_argument("--cpu-rows-1core", type=int, default=48, help="CPU baseline sample: frequency rows on one core")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="c2: leave out the 1024-model batch figure")
     ap.add_argument("--copy-probe", action="store_true",

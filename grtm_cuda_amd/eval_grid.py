@@ -31,7 +31,34 @@ HELDOUT_SEED = 900000
 C5_TOL = 1e-4
 
 
-def evaluate_model(layers, src, dists, grid, pred, torch, dev, Jt):
+def numpy_contract(K, w, J):
+    """sum_k K[w, k] w[w, k] J[k, r] on the host: [nw, nr]."""
+    return (K * w) @ J
+
+
+def torch_contract(torch, dev):
+    """The same contraction on the GPU through torch (the evaluation's own
+    plumbing, not the library's path); J tables go in as numpy arrays."""
+    cache = {}
+
+    def f(K, w, J):
+        key = id(J)
+        if key not in cache:
+            cache.clear()
+            cache[key] = (J, torch.as_tensor(J, dtype=torch.complex128, device=dev))
+        Jt = cache[key][1]
+        Kt = torch.as_tensor(np.ascontiguousarray(K), dtype=torch.complex128, device=dev)
+        wt = torch.as_tensor(np.ascontiguousarray(w), dtype=torch.float64, device=dev)
+        return ((Kt * wt) @ Jt).cpu().numpy()
+    return f
+
+
+def bessel_tables(grid, dists):
+    k = (grid["kmax"] / grid["nk"]) * np.arange(grid["nk"])
+    return [f(np.outer(k, dists)).astype(np.complex128) for f in (sp.j0, sp.j1)]
+
+
+def evaluate_model(layers, src, dists, grid, pred, contract, Jt):
     nw, nk = grid["nw"], grid["nk"]
     dk = grid["kmax"] / nk
     p64 = TracePlan(layers, src, 0.0, dists, grid, precision=PREC_FP64)
@@ -49,15 +76,12 @@ def evaluate_model(layers, src, dists, grid, pred, torch, dev, Jt):
     full[0, 0] = 0.5 * dk
     full = np.broadcast_to(full, (nw, nk))
     res = np.zeros_like(err)
-    check_t = torch.as_tensor(check, dtype=torch.float64, device=dev)
-    full_t = torch.as_tensor(np.ascontiguousarray(full), dtype=torch.float64, device=dev)
     for q in (0, 1):
         K = p64.kernels(q, 0, nw)  # [nw, nk, (Z, R)]
         for c in (0, 1):
-            # sum_k K[w, k] weight[w, k] J_c(k r) on the GPU: [nw, nr]
-            Kt = torch.as_tensor(np.ascontiguousarray(K[..., c]), dtype=torch.complex128, device=dev)
-            G = ((Kt * full_t) @ Jt[c]).cpu().numpy()
-            D = ((Kt * check_t) @ Jt[c]).cpu().numpy()
+            # sum_k K[w, k] weight[w, k] J_c(k r): [nw, nr]
+            G = contract(K[..., c], full, Jt[c])
+            D = contract(K[..., c], check, Jt[c])
             res[q, c] = np.linalg.norm(D, axis=0) / np.linalg.norm(G, axis=0)
     # labelled local maxima beyond the checked region
     reach = cut + 8 * chk
@@ -76,7 +100,7 @@ def main(argv=None):
     ap.add_argument("--report", default=None)
     args = ap.parse_args(argv)
     import torch
-    dev = torch.device("cuda")
+    contract = torch_contract(torch, torch.device("cuda"))
     pred = learned.Predictor()
     dists = np.linspace(5.0, 300.0, 64)
     tot = dict(traces=0, c5_fail=0, err_fail=0, c5_fail_err_fail=0, rows_beyond=0, rows=0, fallback=0)
@@ -88,9 +112,7 @@ def main(argv=None):
     for i in range(args.start, args.start + args.models):
         layers, src = model_sample(HELDOUT_SEED + i)
         g = models.dense_grid(1024, 0.05, 8192, float(layers[:, 2].min()))
-        k = (g["kmax"] / g["nk"]) * np.arange(g["nk"])
-        Jt = [torch.as_tensor(f(np.outer(k, dists)), dtype=torch.complex128, device=dev) for f in (sp.j0, sp.j1)]
-        r = evaluate_model(layers, src, dists, g, pred, torch, dev, Jt)
+        r = evaluate_model(layers, src, dists, g, pred, contract, bessel_tables(g, dists))
         c5 = r["res"] > C5_TOL
         bad = r["err"] > C5_TOL
         tot["traces"] += r["err"].size

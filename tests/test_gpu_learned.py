@@ -60,23 +60,21 @@ def test_learned_cut_beats_analytic(predictor, seed):
 
 
 def test_row_check_catches_every_trace_a_per_trace_check_would():
-    """eval_grid on 4 held-out models: every learned trace within 1e-4 of the
+    """eval_grid on 3 held-out models: every learned trace within 1e-4 of the
     plain method, none of the traces whose 1/8-density tail check exceeds 1e-4
     left wrong, and no labelled pole cluster beyond the checked region (the
-    1024-model run of the same evaluation is in the design notes)."""
-    import scipy.special as sp
-    import torch
+    1024-model run of the same evaluation is in the design notes; the
+    evaluation's contraction runs on the host here)."""
     from grtm_cuda_amd import eval_grid
-    print("gpu: eval_grid on 4 held-out models", flush=True)
-    dev = torch.device("cuda")
+    print("gpu: eval_grid on 3 held-out models", flush=True)
     pred = learned.Predictor()
     dists = np.linspace(5.0, 300.0, 64)
-    for i in range(4):
+    for i in range(3):
         layers, src = model_sample(eval_grid.HELDOUT_SEED + i)
         grid = models.dense_grid(1024, 0.05, 8192, float(layers[:, 2].min()))
-        k = (grid["kmax"] / grid["nk"]) * np.arange(grid["nk"])
-        Jt = [torch.as_tensor(f(np.outer(k, dists)), dtype=torch.complex128, device=dev) for f in (sp.j0, sp.j1)]
-        r = eval_grid.evaluate_model(layers, src, dists, grid, pred, torch, dev, Jt)
+        r = eval_grid.evaluate_model(layers, src, dists, grid, pred, eval_grid.numpy_contract,
+                                     eval_grid.bessel_tables(grid, dists))
+        bad = (r["res"] > 1e-4) & (r["err"] > 1e-4)
         print(f"  model {i}: worst error {r['err'].max():.2e}, traces the tail check fails "
               f"{int((r['res'] > 1e-4).sum())}, clusters beyond {r['beyond']}, cells {r['cells']:.3f}")
-        assert r["err"].max() < 1e-4 and r["beyond"] == 0
+        assert r["err"].max() < 1e-4 and r["beyond"] == 0 and not bad.any()
