@@ -35,6 +35,21 @@ FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 
 
+def prec_code(g, name):
+    return {"fp64": g.PREC_FP64, "fp32": g.PREC_FP32, "bf16x3": g.PREC_BF16X3, "bf16x6": g.PREC_BF16X6}[name]
+
+
+def dtype_name(g, prec):
+    return {g.PREC_FP64: "f64", g.PREC_FP32: "f32", g.PREC_BF16X3: "f32+bf16x3", g.PREC_BF16X6: "f32+bf16x6"}[prec]
+
+
+def ksum_peak(g, prec):
+    """Dense matrix-core peak of the contraction's arithmetic (split-BF16: the
+    bf16 rate over the products per term)."""
+    return {g.PREC_FP64: FP64_PEAK_TFLOPS, g.PREC_FP32: FP32_PEAK_TFLOPS, g.PREC_BF16X3: BF16_PEAK_TFLOPS / 3,
+            g.PREC_BF16X6: BF16_PEAK_TFLOPS / 6}[prec]
+
+
 def host_cpus():
     """The host cores this process may use (its CPU affinity, capped by
     OMP_NUM_THREADS where the machine sets a share), and what the host reports."""
@@ -109,7 +124,7 @@ def bench_batch(args, g, models):
     precision against the plain FP64 batch."""
     cfg = models.config("c4", n_models=args.models)
     pname = args.precision or "fp32"
-    prec = {"fp64": g.PREC_FP64, "fp32": g.PREC_FP32, "bf16x3": g.PREC_BF16X3}[pname]
+    prec = prec_code(g, pname)
     mk = lambda p: g.BatchPlan(cfg["models"], cfg["rcv_depth"], cfg["dists"], cfg["grid"], precision=p)
     nm = len(cfg["models"])
     print(f"bench: c4 plain FP64 batch of {nm} models", file=sys.stderr, flush=True)
@@ -160,7 +175,7 @@ def bench_batch(args, g, models):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {g.PREC_FP64: "f64", g.PREC_FP32: "f32", g.PREC_BF16X3: "f32+bf16x3"}[prec],
+        "dtype": dtype_name(g, prec),
         "data": "synthetic (seeded layered models)",
         "config": {"workload": f"c4: {nm} models of {min(layers)}-{max(layers)} layers (mean {np.mean(layers):.1f}), "
                                f"sources 0-40 km, 64 receivers 5-300 km, Nw={cfg['grid']['nw']}, "
@@ -192,7 +207,7 @@ def bench_precision(args, g, models):
     ref = g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
                       precision=g.PREC_FP64).traces()
     errors = {}
-    for pname, prec in (("fp32", g.PREC_FP32), ("bf16x3", g.PREC_BF16X3)):
+    for pname, prec in (("fp32", g.PREC_FP32), ("bf16x3", g.PREC_BF16X3), ("bf16x6", g.PREC_BF16X6)):
         tr = g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
                          precision=prec).traces()
         d = tr - ref
@@ -202,8 +217,9 @@ def bench_precision(args, g, models):
                              max_peak=float(pk.max()), traces=int(rl.size))
     c2 = models.config("c2")
     gemm = {}
-    for pname, prec, peak in (("fp64", g.PREC_FP64, FP64_PEAK_TFLOPS), ("fp32", g.PREC_FP32, FP32_PEAK_TFLOPS),
-                              ("bf16x3", g.PREC_BF16X3, BF16_PEAK_TFLOPS / 3)):
+    for pname in ("fp64", "fp32", "bf16x3", "bf16x6"):
+        prec = prec_code(g, pname)
+        peak = ksum_peak(g, prec)
         plan = g.TracePlan(c2["layers"], c2["src_depth"], c2["rcv_depth"], c2["dists"], c2["grid"], precision=prec)
         plan.execute()
         tm = plan.timing(reps=max(2, args.steps // 3))
@@ -215,7 +231,7 @@ def bench_precision(args, g, models):
         "metric": "Green's traces/s at <=1e-4 rel-L2 vs FP64; learned-adaptive vs plain dense-k",
         "value": None, "unit": "traces/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 / f32+bf16x3", "data": "synthetic (3-layer model; seeded 8-layer model for the contraction)",
+        "dtype": "f32 / f32+bf16x3 / f32+bf16x6", "data": "synthetic (3-layer model; seeded 8-layer model for the contraction)",
         "config": {"workload": "c3: 3 layers over a half-space, explosion at 10 km, 8 receivers 10-80 km, "
                                "Nw=512, Nk=4096, dt=0.1 (errors); contraction at Nw=2048, Nk=16384, 256 receivers",
                    "path": "precision study"},
@@ -245,7 +261,7 @@ def bench_columns(args, g, models):
     tm_ref = ref_plan.timing(reps=3)
     del ref_plan
     pname = args.precision or "fp32"
-    prec = {"fp64": g.PREC_FP64, "fp32": g.PREC_FP32, "bf16x3": g.PREC_BF16X3}[pname]
+    prec = prec_code(g, pname)
     print(f"bench: columns {args.path} {pname}", file=sys.stderr, flush=True)
     plan = mk(prec)
     extra = None
@@ -286,7 +302,7 @@ def bench_columns(args, g, models):
                   "-column traces/s at <=1e-4 rel-L2 vs FP64 (plain dense-k FP64 alongside)",
         "value": ntr / t_step, "unit": "column traces/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * t_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {g.PREC_FP64: "f64", g.PREC_FP32: "f32", g.PREC_BF16X3: "f32+bf16x3"}[prec],
+        "dtype": dtype_name(g, prec),
         "data": "synthetic (seeded layered model)",
         "config": {"workload": f"mt: {len(cfg['layers']) - 1} layers + half-space, tensor and single-force sources "
                                f"at {cfg['src_depth']} km, {len(plan.columns)} columns"
@@ -345,7 +361,7 @@ def roofline(g, plan, prec, tm, work, counted=False):
                     transcendentals_per_cell=work["trans_per_cell"], cells=cells), kernels
     if dom == "ksum":
         achieved = work["ksum_flop"] / (tm["ksum"] * 1e-3) / 1e12
-        kpeak = {g.PREC_FP64: FP64_PEAK_TFLOPS, g.PREC_FP32: FP32_PEAK_TFLOPS, g.PREC_BF16X3: BF16_PEAK_TFLOPS / 3}[prec]
+        kpeak = ksum_peak(g, prec)
         return dict(bound="mfma", achieved=achieved, peak=kpeak, unit="TFLOP/s", frac=achieved / kpeak,
                     traffic=None, kernel="k_gemm_mfma"), kernels
     achieved = work["ifft_bytes"] / (tm["ifft"] * 1e-3) / 1e9
@@ -364,7 +380,7 @@ def main():
     ap.add_argument("--models", type=int, default=1024, help="c4: models in the batch")
     ap.add_argument("--path", default="adaptive", choices=["adaptive", "dense"],
                     help="adaptive: learned truncated grid, mixed precision (the headline); dense: plain grid")
-    ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16x3"],
+    ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16x3", "bf16x6"],
                     help="arithmetic of the measured path (default fp32 adaptive / fp64 dense)")
     ap.add_argument("--derivatives", action="store_true", help="mt: the d/dz and d/dr columns as well")
     ap.add_argument("--cpu-rows", type=int, default=512, help="CPU baseline sample: frequency rows on all cores")
@@ -394,7 +410,7 @@ def main():
         check(_lib().grtm_selftest_copy(64 << 20), "grtm_selftest_copy")
     cfg = models.config(args.workload)
     pname = args.precision or ("fp32" if args.path == "adaptive" else "fp64")
-    prec = {"fp64": g.PREC_FP64, "fp32": g.PREC_FP32, "bf16x3": g.PREC_BF16X3}[pname]
+    prec = prec_code(g, pname)
     mk = lambda p: g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
                                precision=p)
 
@@ -466,7 +482,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {g.PREC_FP64: "f64", g.PREC_FP32: "f32", g.PREC_BF16X3: "f32+bf16x3"}[prec],
+        "dtype": dtype_name(g, prec),
         "data": "synthetic (seeded layered model)",
         "config": {"workload": f"{args.workload}: {len(cfg['layers']) - 1} layers + half-space, explosion + "
                                f"vertical force at {cfg['src_depth']} km, {len(cfg['dists'])} receivers, "

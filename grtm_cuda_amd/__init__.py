@@ -18,10 +18,10 @@ import ctypes
 import numpy as np
 
 from . import inputfile, models  # noqa: F401
-from .ffi import (GRTM_OK, PREC_BF16X3, PREC_FP32, PREC_FP64, EXPORTS, GrtmError, Grid, Job, Model, D, P,  # noqa: F401
+from .ffi import (GRTM_OK, PREC_BF16X3, PREC_BF16X6, PREC_FP32, PREC_FP64, EXPORTS, GrtmError, Grid, Job, Model, D, P,  # noqa: F401
                    _lib, check)
 
-__all__ = ["green", "green_mem", "TracePlan", "BatchPlan", "GrtmError", "PREC_FP64", "PREC_FP32", "PREC_BF16X3", "models",
+__all__ = ["green", "green_mem", "TracePlan", "BatchPlan", "GrtmError", "PREC_FP64", "PREC_FP32", "PREC_BF16X3", "PREC_BF16X6", "models",
            "inputfile"]
 
 
@@ -388,7 +388,8 @@ class BatchPlan:
 
 def selftest_gemm_cut(prec, A, B, nk_w):
     """The learned-grid contraction (test hook): C = A B with row r of A read up
-    to column nk_w[r // 4]; prec 1 FP32 stream-K, 0 FP64 split-K."""
+    to column nk_w[r // 4]; prec 1 FP32 stream-K, 0 FP64 split-K, 3 / 6 the
+    split-BF16 stream-K kernel with that many products."""
     A = np.ascontiguousarray(A, dtype=np.float64)
     B = np.ascontiguousarray(B, dtype=np.float64)
     nk_w = np.ascontiguousarray(nk_w, dtype=np.int32)

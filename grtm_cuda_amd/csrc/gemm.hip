@@ -621,6 +621,235 @@ __global__ __launch_bounds__(256) void k_gemm_f64s(int N, int K, int kc, int S, 
             }
 }
 
+// ------------------------------------------------------------ split BF16
+// FP32 operands split into NPL bfloat16 terms, x = x1 + x2 (+ x3), each the
+// bf16 rounding of what the previous terms leave; the products of the pair
+// orders that matter are summed by v_mfma_f32_32x32x16_bf16 in FP32:
+// NP = 3: x1 y1 + x1 y2 + x2 y1 (NPL = 2, relative error about 2^-16 per
+// product), NP = 6: also x1 y3 + x2 y2 + x3 y1 (NPL = 3, about 2^-24: the FP32
+// contraction's accuracy at 6 / 16 of its matrix-core time).  The B operand
+// (a Bessel table, fixed per plan) is split once into planes [NPL][N][ldk]
+// (k contiguous, zero past K); A is split while it is staged to LDS.
+template <int NPL>
+__device__ __forceinline__ void split_terms(float x, __bf16 (&t)[NPL])
+{
+    float r = x;
+#pragma unroll
+    for (int p = 0; p < NPL; p++) {
+        t[p] = __bf16(r);
+        r = r - float(t[p]);
+    }
+}
+
+template <int BM, int BN, int WM, int WN, int NPL, int NP>
+struct BSTile {
+    static constexpr int NT = (BM / WM) * (BN / WN) * 64, BK = 32, KP = BK + 8;
+    static constexpr int TM = WM / 32, TN = WN / 32, AE = BM * BK / 4 / NT;
+    static constexpr int BC = BN * BK / 8;  // 16-byte B chunks per plane
+    static_assert(AE * NT * 4 == BM * BK, "A staging must cover the slab");
+    static_assert((NPL == 2 && NP == 3) || (NPL == 3 && NP == 6), "split");
+    struct Smem {
+        __bf16 As[2][NPL][BM][KP];
+        __bf16 Bs[2][NPL][BN][KP];
+    };
+
+    // acc = A[m0 : m0 + BM, kb : ke] B[kb : ke, n0 : n0 + BN] as F32Tile::mma;
+    // S: the split B planes, plane stride N ldk.
+    static __device__ __forceinline__ void mma(const float *__restrict__ A, const __bf16 *__restrict__ S, long lda,
+                                               long ldk, int M, int N, int m0, int n0, int kb, int ke, int rpw,
+                                               const int *__restrict__ nk_w, Smem &sm, v16f (&acc)[TM][TN],
+                                               const long long *__restrict__ foff, const int *__restrict__ fext)
+    {
+        const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+        const int wm = wv / (BN / WN), wn = wv % (BN / WN);
+        int a_lim[AE];
+        size_t a_row[AE];
+#pragma unroll
+        for (int e = 0; e < AE; e++) {
+            const int gm = m0 + ((tid + NT * e) >> 3);
+            a_lim[e] = gm < M ? (nk_w ? min(ke, nk_w[gm / rpw]) : ke) : 0;
+            a_row[e] = foff ? (gm < M ? size_t(foff[gm / rpw]) + size_t(gm % rpw) * size_t(fext[gm / rpw]) : 0)
+                            : size_t(gm) * size_t(lda);
+        }
+        const size_t pstride = size_t(N) * size_t(ldk);
+        float4 ra[AE];
+        uint4 rb[NPL][(BC + NT - 1) / NT];
+        auto load = [&](int k0) {
+#pragma unroll
+            for (int e = 0; e < AE; e++) {
+                const int idx = tid + NT * e, gk = k0 + (idx & 7) * 4;
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (gk < a_lim[e]) {
+                    // 16-byte aligned and inside the row (gk and the row's length are multiples of 4)
+                    v = *reinterpret_cast<const float4 *>(A + a_row[e] + gk);
+                    const int lim = a_lim[e] - gk;
+                    if (lim < 4) {
+                        v.w = 0.f;
+                        if (lim < 3) v.z = 0.f;
+                        if (lim < 2) v.y = 0.f;
+                    }
+                }
+                ra[e] = v;
+            }
+#pragma unroll
+            for (int c = 0; c < (BC + NT - 1) / NT; c++) {
+                const int q = tid + NT * c, n = q >> 2, kq = (q & 3) * 8;
+#pragma unroll
+                for (int p = 0; p < NPL; p++) {
+                    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                    // k0 + kq + 7 < ldk (a multiple of 32 at least ke); columns past N read zeros
+                    if (q < BC && n0 + n < N)
+                        v = *reinterpret_cast<const uint4 *>(S + p * pstride + size_t(n0 + n) * ldk + k0 + kq);
+                    rb[p][c] = v;
+                }
+            }
+        };
+        auto store = [&](int buf) {
+#pragma unroll
+            for (int e = 0; e < AE; e++) {
+                const int idx = tid + NT * e, r = idx >> 3, k = (idx & 7) * 4;
+                const float x[4] = {ra[e].x, ra[e].y, ra[e].z, ra[e].w};
+                __bf16 t[4][NPL];
+#pragma unroll
+                for (int j = 0; j < 4; j++) split_terms<NPL>(x[j], t[j]);
+#pragma unroll
+                for (int p = 0; p < NPL; p++) {
+                    typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+                    v4bf w;
+                    w[0] = t[0][p], w[1] = t[1][p], w[2] = t[2][p], w[3] = t[3][p];
+                    *reinterpret_cast<v4bf *>(&sm.As[buf][p][r][k]) = w;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < (BC + NT - 1) / NT; c++) {
+                const int q = tid + NT * c, n = q >> 2, kq = (q & 3) * 8;
+                if (q < BC)
+#pragma unroll
+                    for (int p = 0; p < NPL; p++) *reinterpret_cast<uint4 *>(&sm.Bs[buf][p][n][kq]) = rb[p][c];
+            }
+        };
+#pragma unroll
+        for (int a = 0; a < TM; a++)
+#pragma unroll
+            for (int b = 0; b < TN; b++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[a][b][r] = 0.f;
+        load(kb);
+        store(0);
+        __syncthreads();
+        const int i = lane & 31, kq = (lane >> 5) * 8;
+        int buf = 0;
+        for (int k0 = kb; k0 < ke; k0 += BK) {
+            const bool more = k0 + BK < ke;
+            if (more) load(k0 + BK);
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 16) {
+                v8bf fa[NPL][TM], fb[NPL][TN];
+#pragma unroll
+                for (int p = 0; p < NPL; p++) {
+#pragma unroll
+                    for (int t = 0; t < TM; t++)
+                        fa[p][t] = *reinterpret_cast<const v8bf *>(&sm.As[buf][p][wm * WM + t * 32 + i][kk + kq]);
+#pragma unroll
+                    for (int t = 0; t < TN; t++)
+                        fb[p][t] = *reinterpret_cast<const v8bf *>(&sm.Bs[buf][p][wn * WN + t * 32 + i][kk + kq]);
+                }
+#pragma unroll
+                for (int tm = 0; tm < TM; tm++)
+#pragma unroll
+                    for (int tn = 0; tn < TN; tn++) {
+                        // small products first
+                        if constexpr (NP == 6) {
+                            acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[2][tm], fb[0][tn], acc[tm][tn], 0, 0, 0);
+                            acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][tm], fb[1][tn], acc[tm][tn], 0, 0, 0);
+                            acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][tm], fb[2][tn], acc[tm][tn], 0, 0, 0);
+                        }
+                        acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][tm], fb[0][tn], acc[tm][tn], 0, 0, 0);
+                        acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][tm], fb[1][tn], acc[tm][tn], 0, 0, 0);
+                        acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][tm], fb[0][tn], acc[tm][tn], 0, 0, 0);
+                    }
+            }
+            if (more) store(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+
+    // dst[r][c] = acc for tile-local rows r < nrow, columns c < ncol, row stride ld
+    static __device__ __forceinline__ void put(const v16f (&acc)[TM][TN], float *__restrict__ dst, long ld, int nrow,
+                                               int ncol)
+    {
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int wm = wv / (BN / WN), wn = wv % (BN / WN);
+#pragma unroll
+        for (int tm = 0; tm < TM; tm++)
+#pragma unroll
+            for (int tn = 0; tn < TN; tn++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int row = wm * WM + tm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    const int col = wn * WN + tn * 32 + (lane & 31);
+                    if (row < nrow && col < ncol) dst[(size_t)row * ld + col] = acc[tm][tn][r];
+                }
+    }
+};
+
+// Stream-K over the tiles' K slabs as k_gemm_f32sk, with the split-BF16 tile;
+// S[p]: the split B planes of contraction p.
+struct SplitB {
+    const __bf16 *S[kGemmSetMax];
+};
+template <int BM, int BN, int WM, int WN, int NPL, int NP>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bssk(int n, int N, int K, GemmSet gp, SplitB sb,
+                                                                       StreamK sk, long lda, long ldk,
+                                                                       const int *__restrict__ nk_w,
+                                                                       const int *__restrict__ off,
+                                                                       float *__restrict__ part)
+{
+    using Tl = BSTile<BM, BN, WM, WN, NPL, NP>;
+    __shared__ __attribute__((aligned(16))) typename Tl::Smem sm;
+    const int T = sk.cum[n];
+    const long G = off[T];
+    const int b = blockIdx.x, nbe = int(min((long)sk.nb, G));
+    if (b >= nbe) return;  // uniform
+    long lo = G * b / nbe;
+    const long hi = G * (b + 1) / nbe, lo_b = lo;
+    int t = 0;
+    for (int step = 1 << 30; step; step >>= 1)
+        if (t + step <= T && off[t + step] <= lo) t += step;
+    while (lo < hi) {
+        while (off[t + 1] <= lo) t++;
+        const long t0 = off[t], t1 = off[t + 1];
+        const long seg = min(hi, t1);
+        int p = 0;
+        while (p + 1 < n && t >= sk.cum[p + 1]) p++;
+        const int M = gp.M[p];
+        const int m0 = (t - sk.cum[p]) / sk.ncol * BM, n0 = (t - sk.cum[p]) % sk.ncol * BN;
+        const int kt = tile_extent(gp.kend[p], M, K, m0, BM);
+        v16f acc[Tl::TM][Tl::TN];
+        Tl::mma(gp.A[p], sb.S[p], lda, ldk, M, N, m0, n0, int(lo - t0) * 32, min(kt, int(seg - t0) * 32), gp.rpw[p],
+                nk_w, sm, acc, gp.foff, gp.fext);
+        const bool whole = lo == t0 && seg == t1;
+        float *dst = whole ? gp.C[p] + (size_t)m0 * N + n0 : part + ((size_t)b * 2 + (lo == lo_b ? 0 : 1)) * BM * BN;
+        Tl::put(acc, dst, whole ? N : BN, whole ? M - m0 : BM, whole ? N - n0 : BN);
+        lo = seg;
+        t++;
+    }
+}
+
+// B [K][ldb] (FP32, N columns) into NPL bf16 planes [NPL][N][ldk], zero for K <= k < ldk.
+template <int NPL>
+__global__ void k_split_table(int K, int N, const float *__restrict__ B, long ldb, __bf16 *__restrict__ S, long ldk)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)N * ldk) return;
+    const int n = int(t / ldk), k = int(t % ldk);
+    __bf16 v[NPL];
+    split_terms<NPL>(k < K ? B[(size_t)k * ldb + n] : 0.f, v);
+#pragma unroll
+    for (int p = 0; p < NPL; p++) S[(size_t)p * N * ldk + t] = v[p];
+}
+
 // ------------------------------------------------------------------ BF16x3
 __device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo)
 {
@@ -832,6 +1061,77 @@ static hipError_t launch_streamk(int n, const GemmSet &gs, int N, int K, long ld
         hipLaunchKernelGGL((k_streamk_reduce<kTBM, 64>), dim3((unsigned)t, kReduceSlices), dim3(256), 0, st, n, N, gs,
                            sk, off, part);
     }
+    return hipGetLastError();
+}
+
+// Split-BF16 pair (Z and R contractions) by stream-K, one resident block of
+// 8 waves per compute unit (the double-buffered split tiles take 82-123 KB of
+// LDS).  Workspace: gemm_bsplit_workspace_bytes.
+constexpr int kSplitBlocks = 256;
+static int bs_blocks(long tiles, int K) { return (int)std::min<long>(kSplitBlocks, tiles * ((K + kTBK - 1) / kTBK)); }
+
+size_t gemm_bsplit_workspace_bytes(int M, int N, int K)
+{
+    const int Ms[2] = {M, M};
+    const long tiles = set_tiles(1, 2, Ms, N);
+    return (size_t)bs_blocks(tiles, K) * 2 * kTBM * f32_bn(N) * sizeof(float) + (size_t)(tiles + 1) * sizeof(int);
+}
+
+hipError_t launch_split_table(int nprod, int K, int N, const float *B, long ldb, void *S, long ldk, hipStream_t st)
+{
+    if ((nprod != 3 && nprod != 6) || ldk < K || ldk % 32) return hipErrorInvalidValue;
+    const size_t n = (size_t)N * ldk;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (nprod == 3) hipLaunchKernelGGL(k_split_table<2>, grid, dim3(256), 0, st, K, N, B, ldb, (__bf16 *)S, ldk);
+    else hipLaunchKernelGGL(k_split_table<3>, grid, dim3(256), 0, st, K, N, B, ldb, (__bf16 *)S, ldk);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_pair_bsplit(int nprod, int M, int N, int K, const float *A0, const float *A1, long lda,
+                                   const void *S0, const void *S1, long ldk, float *C0, float *C1, float *P,
+                                   const int *kend, const int *nk_w, hipStream_t st, const long long *foff,
+                                   const int *fext)
+{
+    if ((nprod != 3 && nprod != 6) || ldk < K || ldk % 32 || lda % 4 || (foff && (!fext || !nk_w || !kend)))
+        return hipErrorInvalidValue;
+    GemmSet gs{};
+    gs.A[0] = A0, gs.A[1] = A1, gs.C[0] = C0, gs.C[1] = C1;
+    gs.kend[0] = gs.kend[1] = kend;
+    gs.M[0] = gs.M[1] = M;
+    gs.rpw[0] = gs.rpw[1] = 4;
+    gs.foff = foff;
+    gs.fext = fext;
+    SplitB sb{};
+    sb.S[0] = (const __bf16 *)S0;
+    sb.S[1] = (const __bf16 *)S1;
+    const int bn = f32_bn(N);
+    StreamK sk{};
+    int t = 0;
+    for (int q = 0; q < kGemmSetMax; q++) {
+        sk.cum[q] = t;
+        if (q < 2) t += (M + kTBM - 1) / kTBM * ((N + bn - 1) / bn);
+    }
+    sk.cum[kGemmSetMax] = t;
+    sk.ncol = (N + bn - 1) / bn;
+    sk.nb = bs_blocks(t, K);
+    float *part = P;
+    int *off = reinterpret_cast<int *>(P + (size_t)sk.nb * 2 * kTBM * bn);
+    hipLaunchKernelGGL(k_streamk_plan<kTBM>, dim3(1), dim3(1024), 0, st, 2, K, gs, sk, off);
+#define GRTM_BS(BNV, WNV, NPL, NP)                                                                                   \
+    hipLaunchKernelGGL((k_gemm_bssk<kTBM, BNV, 32, WNV, NPL, NP>), dim3(sk.nb), dim3(512), 0, st, 2, N, K, gs, sb, sk, \
+                       lda, ldk, nk_w, off, part)
+    if (bn == 128) {
+        if (nprod == 6) GRTM_BS(128, 64, 3, 6);
+        else GRTM_BS(128, 64, 2, 3);
+        hipLaunchKernelGGL((k_streamk_reduce<kTBM, 128>), dim3((unsigned)t, kReduceSlices), dim3(256), 0, st, 2, N, gs,
+                           sk, off, part);
+    } else {
+        if (nprod == 6) GRTM_BS(64, 32, 3, 6);
+        else GRTM_BS(64, 32, 2, 3);
+        hipLaunchKernelGGL((k_streamk_reduce<kTBM, 64>), dim3((unsigned)t, kReduceSlices), dim3(256), 0, st, 2, N, gs,
+                           sk, off, part);
+    }
+#undef GRTM_BS
     return hipGetLastError();
 }
 

@@ -758,7 +758,11 @@ struct grtm_plan {
     int gemm_mode = 0;  // 0 FP64 MFMA, 1 FP32 MFMA, 2 BF16x3 MFMA
     bool pair = false;  // FP32: both contractions in one streamed launch
     int kc = 0;         // its split-K slice length
-    DBuf lw, lw64, flags, dists, A0, A1, B0, B1, C0, C1, tr, ws;
+    // split-BF16 pair contraction (3 or 6 products, gemm_mode 1): the Bessel
+    // tables as bf16 planes S0 / S1 of row length ldk
+    int bsplit = 0;
+    long ldk = 0;
+    DBuf lw, lw64, flags, dists, A0, A1, B0, B1, C0, C1, tr, ws, S0, S1;
     DBuf plant;      // test hook: cells given non-finite FP32 operands at every execution
     int nplant = 0;
     // peak-trough averaging of the k-sums' tails (grtm_plan_set_ptam): ptam_nt
@@ -815,6 +819,28 @@ struct grtm_plan {
 namespace {
 
 size_t esize(int prec) { return prec == GRTM_PREC_FP64 ? sizeof(double) : sizeof(float); }
+
+// Stream-K workspace of an FP32 pair contraction of M rows a side, enough for
+// the split-BF16 kernel too when bsplit is set.
+size_t pair_sk_bytes(int bsplit, int M, int N, int K)
+{
+    const int M2[2] = {M, M};
+    size_t b = grtm::gemm_streamk_workspace_bytes(2, M2, N, K);
+    if (bsplit) b = std::max(b, grtm::gemm_bsplit_workspace_bytes(M, N, K));
+    return b;
+}
+
+// Split-BF16 contraction (GRTM_PREC_BF16X3 / GRTM_PREC_BF16X6 with a pair
+// layout): the two Bessel tables' bf16 planes, [planes][nr][ldk] each, ldk
+// the wavenumber count rounded up to 32.
+int split_alloc(int bsplit, int nk, int nr, DBuf &S0, DBuf &S1, long &ldk)
+{
+    ldk = (nk + 31) / 32 * 32;
+    const size_t bytes = (size_t)(bsplit == 6 ? 3 : 2) * nr * ldk * 2;
+    int st;
+    if ((st = S0.alloc(bytes)) || (st = S1.alloc(bytes))) return st;
+    return GRTM_OK;
+}
 
 // Displacement-column plans: the same sequence as the Z / R plans (FP64 rows,
 // FP32 rows, residual check, dense fallback, FP64 re-solve of non-finite
@@ -1023,6 +1049,12 @@ int run_gemm_core(grtm_plan *p, hipStream_t s)
     const int M = 4 * p->nw;
     const int *kend = p->adaptive ? p->kend.as<int>() : nullptr;
     const int *nkw = p->adaptive ? p->nk_w.as<int>() : nullptr;
+    if (p->bsplit) {
+        HIPCHK(grtm::launch_gemm_pair_bsplit(p->bsplit, M, p->nr, p->nk, p->A0.as<float>(), p->A1.as<float>(), p->nk,
+                                             p->S0.p, p->S1.p, p->ldk, p->C0.as<float>(), p->C1.as<float>(),
+                                             p->ws.as<float>(), kend, nkw, s, nullptr, nullptr));
+        return GRTM_OK;
+    }
     if (p->pair && p->gemm_mode == 1) {
         HIPCHK(grtm::launch_gemm_pair_f32(M, p->nr, p->nk, p->kc, p->A0.as<float>(), p->A1.as<float>(), p->nk,
                                           p->B0.as<float>(), p->B1.as<float>(), p->nr, p->C0.as<float>(),
@@ -1143,7 +1175,8 @@ static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid
 {
     if (!ctx || !job || !grid || !plan_out) return GRTM_E_INPUT;
     *plan_out = nullptr;
-    if (precision != GRTM_PREC_FP64 && precision != GRTM_PREC_FP32 && precision != GRTM_PREC_BF16X3)
+    if (precision != GRTM_PREC_FP64 && precision != GRTM_PREC_FP32 && precision != GRTM_PREC_BF16X3 &&
+        precision != GRTM_PREC_BF16X6)
         return GRTM_E_INPUT;
     if (job->n_layers < 2 || job->n_layers + 1 > grtm::kMaxLayers || !job->layers || job->nr < 1 || !job->dists)
         return GRTM_E_INPUT;
@@ -1174,7 +1207,7 @@ static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid
     p->g = mt.g;
     p->ctx = ctx;
     p->prec = precision;
-    p->gemm_mode = precision == GRTM_PREC_FP64 ? 0 : precision == GRTM_PREC_FP32 ? 1 : 2;
+    p->gemm_mode = precision == GRTM_PREC_FP64 ? 0 : precision == GRTM_PREC_BF16X3 ? 2 : 1;
     p->compat = compat;
     p->nw = nw;
     p->N = 2 * nw;
@@ -1252,11 +1285,20 @@ static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid
         if ((status = p->B1.alloc((size_t)p->nk * p->nr * es))) return status;
         if ((status = p->C0.alloc(arows * p->nr * es))) return status;
         if ((status = p->C1.alloc(arows * p->nr * es))) return status;
+        const bool pair_ok = grtm::gemm_pair_ok(p->nk, p->nr, p->A0.p, p->A1.p, p->B0.p, p->B1.p);
+        if (pair_ok && (precision == GRTM_PREC_BF16X3 || precision == GRTM_PREC_BF16X6)) {
+            p->gemm_mode = 1;
+            p->bsplit = precision == GRTM_PREC_BF16X3 ? 3 : 6;
+        }
         size_t wsb = grtm::gemm_workspace_bytes(p->gemm_mode, (int)arows, p->nr, p->nk);
-        if (p->gemm_mode != 2 && grtm::gemm_pair_ok(p->nk, p->nr, p->A0.p, p->A1.p, p->B0.p, p->B1.p)) {
+        if (p->gemm_mode != 2 && pair_ok) {
             p->pair = true;
             p->kc = grtm::gemm_pair_slice(p->gemm_mode, (int)arows, p->nr, p->nk, p->nk);
             wsb = std::max(wsb, grtm::gemm_pair_workspace_bytes((int)arows, p->nr, p->nk, p->kc) * es / 4);
+        }
+        if (p->bsplit) {
+            wsb = std::max(wsb, pair_sk_bytes(p->bsplit, (int)arows, p->nr, p->nk));
+            if ((status = split_alloc(p->bsplit, p->nk, p->nr, p->S0, p->S1, p->ldk))) return status;
         }
         if ((status = p->ws.alloc(wsb))) return status;
     }
@@ -1277,6 +1319,12 @@ static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid
     } else {
         HIPCHK(grtm::launch_bessel_tab(precision == GRTM_PREC_FP64 ? 0 : 1, compat ? 0 : 1, p->nk, p->dk,
                                        p->dists.as<double>(), p->nr, p->B0.p, p->B1.p, p->nr, p->stream));
+        if (p->bsplit) {
+            HIPCHK(grtm::launch_split_table(p->bsplit, p->nk, p->nr, p->B0.as<float>(), p->nr, p->S0.p, p->ldk,
+                                            p->stream));
+            HIPCHK(grtm::launch_split_table(p->bsplit, p->nk, p->nr, p->B1.as<float>(), p->nr, p->S1.p, p->ldk,
+                                            p->stream));
+        }
     }
     HIPCHK(hipEventRecord(e1, p->stream));
     HIPCHK(hipEventSynchronize(e1));
@@ -1613,10 +1661,7 @@ int grtm_plan_set_adaptive(grtm_plan *p, const double *kcut, double window, int 
         }
         const int kc = grtm::gemm_pair_slice(p->gemm_mode, 4 * p->nw, p->nr, p->nk, ksum / std::max(1, nt));
         size_t need = grtm::gemm_pair_workspace_bytes(4 * p->nw, p->nr, p->nk, kc) * esize(p->prec) / 4;
-        if (p->gemm_mode == 1) {
-            const int M2[2] = {4 * p->nw, 4 * p->nw};
-            need = std::max(need, grtm::gemm_streamk_workspace_bytes(2, M2, p->nr, p->nk));
-        }
+        if (p->gemm_mode == 1) need = std::max(need, pair_sk_bytes(p->bsplit, 4 * p->nw, p->nr, p->nk));
         if (need > p->ws.n && (st = p->ws.alloc(need))) return st;
         p->kc = kc;
     }
@@ -1664,8 +1709,7 @@ int grtm_plan_set_learned(grtm_plan *p, grtm_mlp *net, double lmargin, double wi
                          (st = p->ioff.alloc(2 * sizeof(unsigned) * p->nw))))
         return st;
     if (p->gemm_mode == 1) {
-        const int M2[2] = {4 * p->nw, 4 * p->nw};
-        const size_t need = grtm::gemm_streamk_workspace_bytes(2, M2, p->nr, p->nk);
+        const size_t need = pair_sk_bytes(p->bsplit, 4 * p->nw, p->nr, p->nk);
         if (need > p->ws.n && (st = p->ws.alloc(need))) return st;
     } else {
         p->kc = grtm::gemm_pair_slice(p->gemm_mode, 4 * p->nw, p->nr, p->nk, p->nk);
@@ -1851,7 +1895,9 @@ void grtm_plan_destroy(grtm_plan *p) { delete p; }
 // ------------------------------------------------------------ self-tests
 int grtm_selftest_gemm_cut(int prec, int M, int N, int K, const double *A, const double *B, const int *nk_w, double *C)
 {
-    if ((prec != 0 && prec != 1) || M < 4 || M % 4 || N < 1 || K < 1 || !A || !B || !nk_w || !C) return GRTM_E_INPUT;
+    if ((prec != 0 && prec != 1 && prec != 3 && prec != 6) || M < 4 || M % 4 || N < 1 || K < 1 || !A || !B || !nk_w ||
+        !C)
+        return GRTM_E_INPUT;
     const int nw = M / 4;
     double ksum = 0;
     for (int i = 0; i < nw; i++) {
@@ -1867,14 +1913,19 @@ int grtm_selftest_gemm_cut(int prec, int M, int N, int K, const double *A, const
         for (int k = 0; k < K; k++) a[(size_t)i * lda + k] = A[(size_t)i * K + k];
     for (int k = 0; k < K; k++)
         for (int j = 0; j < N; j++) b[(size_t)k * ldb + j] = B[(size_t)k * N + j];
-    const int kc = grtm::gemm_pair_slice(prec, M, N, K, std::max(1.0, ksum / nw));
-    const int M2[2] = {M, M};
+    const int bsplit = prec > 1 ? prec : 0;
+    const int kc = grtm::gemm_pair_slice(prec ? 1 : 0, M, N, K, std::max(1.0, ksum / nw));
     size_t wsb = grtm::gemm_pair_workspace_bytes(M, N, K, kc) * es / 4;
-    if (prec == 1) wsb = std::max(wsb, grtm::gemm_streamk_workspace_bytes(2, M2, N, K));
-    DBuf dA, dB, dC, dP, dn, dk;
+    if (prec) wsb = std::max(wsb, pair_sk_bytes(bsplit, M, N, K));
+    DBuf dA, dB, dC, dP, dn, dk, dS;
+    long ldk = 0;
     if ((st = dA.alloc(a.size() * es)) || (st = dB.alloc(b.size() * es)) || (st = dC.alloc((size_t)2 * M * N * es)) ||
         (st = dP.alloc(wsb)) || (st = dn.alloc(sizeof(int) * nw)) || (st = dk.alloc(sizeof(int) * ((nw + 31) / 32))))
         return st;
+    if (bsplit) {
+        ldk = (K + 31) / 32 * 32;
+        if ((st = dS.alloc((size_t)(bsplit == 6 ? 3 : 2) * N * ldk * 2))) return st;
+    }
     if (prec == 0) {
         HIPCHK(hipMemcpy(dA.p, a.data(), a.size() * es, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(dB.p, b.data(), b.size() * es, hipMemcpyHostToDevice));
@@ -1891,6 +1942,13 @@ int grtm_selftest_gemm_cut(int prec, int M, int N, int K, const double *A, const
         HIPCHK(grtm::launch_gemm_pair_f64(M, N, K, kc, dA.as<double>(), dA.as<double>(), lda, dB.as<double>(),
                                           dB.as<double>(), ldb, c0, c1, dP.as<double>(), dk.as<int>(), dn.as<int>(),
                                           nullptr));
+    } else if (bsplit) {
+        // the split planes of B, one table for both halves
+        float *c0 = dC.as<float>(), *c1 = c0 + (size_t)M * N;
+        HIPCHK(grtm::launch_split_table(bsplit, K, N, dB.as<float>(), ldb, dS.p, ldk, nullptr));
+        HIPCHK(grtm::launch_gemm_pair_bsplit(bsplit, M, N, K, dA.as<float>(), dA.as<float>(), lda, dS.p, dS.p, ldk, c0,
+                                             c1, dP.as<float>(), dk.as<int>(), dn.as<int>(), nullptr, nullptr,
+                                             nullptr));
     } else {
         float *c0 = dC.as<float>(), *c1 = c0 + (size_t)M * N;
         HIPCHK(grtm::launch_gemm_pair_f32(M, N, K, kc, dA.as<float>(), dA.as<float>(), lda, dB.as<float>(),
@@ -2116,6 +2174,9 @@ struct grtm_batch {
     double flops = 0;            // recursion flops of one execution (operation count)
     std::vector<grtm::ModelDesc> md_h;
     DBuf md, lw64, lw32, dists, B0, B1, A0, A1, C0, C1, ws, flags, tr, items;
+    int bsplit = 0;  // split-BF16 pair contraction (3 or 6 products) on the planes S0 / S1
+    long ldk = 0;
+    DBuf S0, S1;
     unsigned flag_cap = 0;  // flag list entries (flags: counter, then 64-bit cells)
     // test hook: cells (group row * nk + k) given non-finite FP32 operands, per
     // group, from the (model, row, k) triples in plant_h
@@ -2376,6 +2437,14 @@ int batch_gemm(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
     const size_t r0 = (size_t)g.m0 * b->nw;
     const int *kend = b->adaptive ? b->kend.as<int>() + batch_kend_off(b, g) : nullptr;
     const int *nkw = b->adaptive ? b->nk_w.as<int>() + r0 : nullptr;
+    if (b->bsplit) {
+        HIPCHK(grtm::launch_gemm_pair_bsplit(b->bsplit, M, b->nr, b->nk, b->A0.as<float>(), b->A1.as<float>(), b->nk,
+                                             b->S0.p, b->S1.p, b->ldk, b->C0.as<float>(), b->C1.as<float>(),
+                                             b->ws.as<float>(), kend, nkw, s,
+                                             b->resident ? b->foff.as<long long>() : nullptr,
+                                             b->resident ? b->fext.as<int>() : nullptr));
+        return GRTM_OK;
+    }
     if (b->pair && b->gemm_mode == 1) {
         HIPCHK(grtm::launch_gemm_pair_f32(M, b->nr, b->nk, b->kc, b->A0.as<float>(), b->A1.as<float>(), b->nk,
                                           b->B0.as<float>(), b->B1.as<float>(), b->nr, b->C0.as<float>(),
@@ -2599,7 +2668,7 @@ int batch_make_grouped(grtm_batch *b)
     const int M2[2] = {4 * b->nw * gmax, 4 * b->nw * gmax};
     size_t wsb = std::max(grtm::gemm_workspace_bytes(b->gemm_mode, (int)arows, b->nr, b->nk),
                           grtm::gemm_pair_workspace_bytes((int)arows, b->nr, b->nk, std::max(256, b->kc / 2)) * es / 4);
-    if (b->gemm_mode == 1) wsb = std::max(wsb, grtm::gemm_streamk_workspace_bytes(2, M2, b->nr, b->nk));
+    if (b->gemm_mode == 1) wsb = std::max(wsb, pair_sk_bytes(b->bsplit, M2[0], b->nr, b->nk));
     int st;
     if ((st = b->A0.alloc(arows * b->nk * es)) || (st = b->A1.alloc(arows * b->nk * es)) ||
         (st = b->C0.alloc(arows * b->nr * es)) || (st = b->C1.alloc(arows * b->nr * es)) ||
@@ -2629,8 +2698,7 @@ int batch_make_resident(grtm_batch *b)
         off[r] = (long long)tot;
         tot += 4 * (size_t)ext[r];
     }
-    const int M2[2] = {(int)nrows, (int)nrows};
-    const size_t ws_need = grtm::gemm_streamk_workspace_bytes(2, M2, b->nr, b->nk);
+    const size_t ws_need = pair_sk_bytes(b->bsplit, (int)nrows, b->nr, b->nk);
     const size_t a_bytes = (tot + 64) * sizeof(float);  // 64 entries past the last row (16-byte loads)
     const size_t c_bytes = nrows * b->nr * sizeof(float);
     const int tpb = (b->nk + b->bs - 1) / b->bs;
@@ -2704,7 +2772,8 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
 {
     if (!ctx || !models || n_models < 1 || !dists || nr < 1 || !grid || !out) return GRTM_E_INPUT;
     *out = nullptr;
-    if (precision != GRTM_PREC_FP64 && precision != GRTM_PREC_FP32 && precision != GRTM_PREC_BF16X3)
+    if (precision != GRTM_PREC_FP64 && precision != GRTM_PREC_FP32 && precision != GRTM_PREC_BF16X3 &&
+        precision != GRTM_PREC_BF16X6)
         return GRTM_E_INPUT;
     if (!grid_ok(grid, precision)) return GRTM_E_INPUT;
     if (precision == GRTM_PREC_FP64 && grid->nw > 4096) return GRTM_E_UNSUPPORTED;
@@ -2720,7 +2789,7 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
     b->ctx = ctx;
     b->prec = precision;
     b->compat = compat;
-    b->gemm_mode = precision == GRTM_PREC_FP64 ? 0 : precision == GRTM_PREC_FP32 ? 1 : 2;
+    b->gemm_mode = precision == GRTM_PREC_FP64 ? 0 : precision == GRTM_PREC_BF16X3 ? 2 : 1;
     b->nm = n_models;
     b->nw = grid->nw;
     b->N = 2 * grid->nw;
@@ -2793,17 +2862,30 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
         (st = b->C1.alloc(arows * nr * es)) || (st = b->tr.alloc((size_t)n_models * 4 * nr * b->N * es)))
         return st;
     if ((st = batch_alloc_flags(b, 0))) return st;
+    const bool pair_ok = grtm::gemm_pair_ok(b->nk, nr, b->A0.p, b->A1.p, b->B0.p, b->B1.p);
+    if (pair_ok && (precision == GRTM_PREC_BF16X3 || precision == GRTM_PREC_BF16X6)) {
+        b->gemm_mode = 1;
+        b->bsplit = precision == GRTM_PREC_BF16X3 ? 3 : 6;
+    }
     size_t wsb = grtm::gemm_workspace_bytes(b->gemm_mode, (int)arows, nr, b->nk);
-    if (b->gemm_mode != 2 && grtm::gemm_pair_ok(b->nk, nr, b->A0.p, b->A1.p, b->B0.p, b->B1.p)) {
+    if (b->gemm_mode != 2 && pair_ok) {
         b->pair = true;
         b->kc = grtm::gemm_pair_slice(b->gemm_mode, (int)arows, nr, b->nk, b->nk);
         const int kmin = std::max(256, b->kc / 2);
         wsb = std::max(wsb, grtm::gemm_pair_workspace_bytes((int)arows, nr, b->nk, kmin) * es / 4);
     }
+    if (b->bsplit) {
+        wsb = std::max(wsb, pair_sk_bytes(b->bsplit, (int)arows, nr, b->nk));
+        if ((st = split_alloc(b->bsplit, b->nk, nr, b->S0, b->S1, b->ldk))) return st;
+    }
     if ((st = b->ws.alloc(wsb))) return st;
     HIPCHK(hipMemcpy(b->dists.p, dists, sizeof(double) * nr, hipMemcpyHostToDevice));
     HIPCHK(grtm::launch_bessel_tab(precision == GRTM_PREC_FP64 ? 0 : 1, compat ? 0 : 1, b->nk, b->dk,
                                    b->dists.as<double>(), nr, b->B0.p, b->B1.p, nr, b->stream));
+    if (b->bsplit) {
+        HIPCHK(grtm::launch_split_table(b->bsplit, b->nk, nr, b->B0.as<float>(), nr, b->S0.p, b->ldk, b->stream));
+        HIPCHK(grtm::launch_split_table(b->bsplit, b->nk, nr, b->B1.as<float>(), nr, b->S1.p, b->ldk, b->stream));
+    }
 
     if (flags & GRTM_BATCH_AUTOTUNE) {
         if ((st = batch_autotune(b))) return st;
@@ -2865,8 +2947,7 @@ int grtm_batch_set_adaptive(grtm_batch *b, const double *kcut, const double *win
             // the FP32 contraction of a learned grid runs stream-K
             int gmax = 0;
             for (const auto &gr : b->groups) gmax = std::max(gmax, gr.n);
-            const int M2[2] = {4 * b->nw * gmax, 4 * b->nw * gmax};
-            const size_t need = grtm::gemm_streamk_workspace_bytes(2, M2, b->nr, b->nk);
+            const size_t need = pair_sk_bytes(b->bsplit, 4 * b->nw * gmax, b->nr, b->nk);
             if (need > b->ws.n && (st = b->ws.alloc(need))) return st;
         }
         b->chk_stride = stride;
@@ -2911,8 +2992,7 @@ int grtm_batch_set_learned(grtm_batch *b, grtm_mlp *net, double lmargin, const d
     if (b->gemm_mode == 1) {
         int gmax = 0;
         for (const auto &gr : b->groups) gmax = std::max(gmax, gr.n);
-        const int M2[2] = {4 * b->nw * gmax, 4 * b->nw * gmax};
-        const size_t need = grtm::gemm_streamk_workspace_bytes(2, M2, b->nr, b->nk);
+        const size_t need = pair_sk_bytes(b->bsplit, 4 * b->nw * gmax, b->nr, b->nk);
         if (need > b->ws.n && (st = b->ws.alloc(need))) return st;
     }
     // predictor inputs of every model (the floor follows each model's FP64 rows)

@@ -173,6 +173,18 @@ hipError_t launch_gemm_pair_f32(int M, int N, int K, int kc, const float *A0, co
                                 const float *B0, const float *B1, long ldb, float *C0, float *C1, float *P,
                                 const int *kend, const int *nk_w, hipStream_t st, const long long *foff = nullptr,
                                 const int *fext = nullptr);
+// Split-BF16 pair: the contractions of launch_gemm_pair_f32 on bf16 matrix
+// cores, A split into 2 (nprod 3) or 3 (nprod 6) bf16 terms while staged, B
+// given as split planes S0 / S1 (launch_split_table: [planes][N][ldk] bf16,
+// ldk a multiple of 32 >= K); stream-K for any kend / nk_w (null: dense).
+// nprod 6 keeps about 2^-24 per product, nprod 3 about 2^-16.  Workspace:
+// gemm_bsplit_workspace_bytes(M, N, K).
+size_t gemm_bsplit_workspace_bytes(int M, int N, int K);
+hipError_t launch_split_table(int nprod, int K, int N, const float *B, long ldb, void *S, long ldk, hipStream_t st);
+hipError_t launch_gemm_pair_bsplit(int nprod, int M, int N, int K, const float *A0, const float *A1, long lda,
+                                   const void *S0, const void *S1, long ldk, float *C0, float *C1, float *P,
+                                   const int *kend, const int *nk_w, hipStream_t st, const long long *foff = nullptr,
+                                   const int *fext = nullptr);
 // The same in FP64 (128 x 128 tiles, kend per 128-row tile; workspace
 // 2 x gemm_pair_workspace_bytes(M, N, K, kc)).
 hipError_t launch_gemm_pair_f64(int M, int N, int K, int kc, const double *A0, const double *A1, long lda,

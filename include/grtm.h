@@ -27,7 +27,17 @@ extern "C" {
                             FP64 low-frequency rows and FP64 re-solve of
                             non-finite cells; FP32 contraction */
 #define GRTM_PREC_BF16X3 2 /* mixed recursion as GRTM_PREC_FP32; contraction on
-                              split-BF16x3 matrix-core products, FP32 accumulation */
+                              split-BF16 matrix-core products, FP32 accumulation:
+                              each FP32 operand as two bf16 terms, the three
+                              products above 2^-16 of the leading one */
+#define GRTM_PREC_BF16X6 3 /* as GRTM_PREC_BF16X3 with three bf16 terms per
+                              operand and the six products above 2^-24 (FP32
+                              rounding level).  Both split forms apply to trace
+                              plans and batches whose wavenumber and receiver
+                              counts are multiples of 4 (stream-K over the
+                              pair of contractions); other shapes, and column
+                              plans, contract as GRTM_PREC_FP32 (BF16X6) or on
+                              the streamed BF16x3 kernel (BF16X3) */
 
 /* ---------------------------------------------------------- drop-in entry */
 /* Replaces the reference's only exported symbol:
@@ -287,7 +297,8 @@ void grtm_batch_destroy(grtm_batch *batch);
  * required to agree bit for bit); operands rounded to FP32 first for modes
  * 1-3. */
 int grtm_selftest_gemm(int mode, int M, int N, int K, const double *A, const double *B, double *C);
-/* The learned-grid contraction pair (prec 1: FP32 stream-K, 0: FP64 split-K)
+/* The learned-grid contraction pair (prec 1: FP32 stream-K, 0: FP64 split-K,
+ * 3 / 6: split-BF16 stream-K with that many products, on FP32 operands)
  * on (A, B) twice: row r of A is read up to column nk_w[r / 4] (M / 4
  * entries, four operand rows per frequency row), the tile extents formed on
  * the GPU as the plans form them; the halves must agree bit for bit. */

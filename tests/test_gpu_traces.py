@@ -51,7 +51,7 @@ def test_c1_kernels_match_oracle(c1_oracle, orc):
         assert np.abs(kg - ko).max() <= 1e-11 * np.abs(ko).max()
 
 
-@pytest.mark.parametrize("prec", [g.PREC_FP32, g.PREC_BF16X3])
+@pytest.mark.parametrize("prec", [g.PREC_FP32, g.PREC_BF16X3, g.PREC_BF16X6])
 def test_c1_mixed_precision_traces(c1_oracle, prec):
     """FP32 recursion (FP64 vertical wavenumbers, FP64 low-frequency rows,
     FP64 re-solve of non-finite cells) + FP32 contraction: every trace within
