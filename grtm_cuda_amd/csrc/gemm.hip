@@ -170,6 +170,20 @@ __device__ __forceinline__ int tile_extent(const int *__restrict__ kend, int M, 
     return min(K, kt);
 }
 
+// The first `lim` values of a 4-vector of A, the rest zero (a row's values
+// at or past its cut are zeros of the product).
+__device__ __forceinline__ float4 cut_tail(float4 v, int lim)
+{
+    return make_float4(lim > 0 ? v.x : 0.f, lim > 1 ? v.y : 0.f, lim > 2 ? v.z : 0.f, lim > 3 ? v.w : 0.f);
+}
+// Operand loads of the contraction tiles go to registers unconditionally, from
+// the operand's own address where it is valid and from the buffer's first
+// element elsewhere; the masking happens when the registers are stored to LDS.
+// A load under a branch would join the zero it replaces at the branch's end,
+// and the compiler waits for the load right there: each slab's global loads
+// would then stall the wave instead of staying in flight through the current
+// slab's products.
+
 // Block tile BM x BN of 8 waves, wave tile WM x WN (WM/32 x WN/32 MFMA
 // 32x32x2 tiles), K slab 32.  The next slab's 16-byte global loads are in
 // flight while the matrix cores work on the current one (two LDS buffers),
@@ -217,44 +231,37 @@ struct F32Tile {
                             : size_t(gm) * size_t(lda);
         }
         float4 ra[AE], rb[BE];
+        int rlim[AE];   // values of ra[e] at or past the row's cut are zeroed as they are stored
+        bool bval[BE];  // rb[e] is a value of B (else it is stored as zeros)
         auto load = [&](int k0) {
 #pragma unroll
             for (int e = 0; e < AE; e++) {
                 const int idx = tid + NT * e, gk = k0 + (idx & 7) * 4;
-                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (gk < a_lim[e]) {
-                    // gk + 3 < lda (or the row's stored length): both are
-                    // multiples of 4 and gk is below them
-                    v = *reinterpret_cast<const float4 *>(A + a_row[e] + gk);
-                    const int lim = a_lim[e] - gk;
-                    if (lim < 4) {
-                        v.w = 0.f;
-                        if (lim < 3) v.z = 0.f;
-                        if (lim < 2) v.y = 0.f;
-                    }
-                }
-                ra[e] = v;
+                rlim[e] = a_lim[e] - gk;
+                // gk + 3 < lda (or the row's stored length) where rlim > 0: both
+                // are multiples of 4 and gk is below them
+                ra[e] = *reinterpret_cast<const float4 *>(A + (rlim[e] > 0 ? a_row[e] + gk : 0));
             }
 #pragma unroll
             for (int e = 0; e < BE; e++) {
                 const int idx = tid + NT * e, gk = k0 + idx / BNV, gn = n0 + (idx % BNV) * 4;
-                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
                 // gn + 3 < ldb: gn and ldb are multiples of 4 and gn < N <= ldb;
                 // columns at or past N only reach accumulators that are not written
-                if (gk < ke && gn < N) v = *reinterpret_cast<const float4 *>(B + (size_t)gk * ldb + gn);
-                rb[e] = v;
+                bval[e] = gk < ke && gn < N;
+                rb[e] = *reinterpret_cast<const float4 *>(B + (bval[e] ? (size_t)gk * ldb + gn : 0));
             }
         };
         auto store = [&](int buf) {
 #pragma unroll
             for (int e = 0; e < AE; e++) {
                 const int idx = tid + NT * e;
-                *reinterpret_cast<float4 *>(&sm.As[buf][idx >> 3][(idx & 7) * 4]) = ra[e];
+                *reinterpret_cast<float4 *>(&sm.As[buf][idx >> 3][(idx & 7) * 4]) = cut_tail(ra[e], rlim[e]);
             }
 #pragma unroll
             for (int e = 0; e < BE; e++) {
                 const int idx = tid + NT * e;
-                *reinterpret_cast<float4 *>(&sm.Bs[buf][idx / BNV][(idx % BNV) * 4]) = rb[e];
+                *reinterpret_cast<float4 *>(&sm.Bs[buf][idx / BNV][(idx % BNV) * 4]) =
+                    bval[e] ? rb[e] : make_float4(0.f, 0.f, 0.f, 0.f);
             }
         };
 #pragma unroll
@@ -628,8 +635,15 @@ __global__ __launch_bounds__(256) void k_gemm_f64s(int N, int K, int kc, int S, 
 // NP = 3: x1 y1 + x1 y2 + x2 y1 (NPL = 2, relative error about 2^-16 per
 // product), NP = 6: also x1 y3 + x2 y2 + x3 y1 (NPL = 3, about 2^-24: the FP32
 // contraction's accuracy at 6 / 16 of its matrix-core time).  The B operand
-// (a Bessel table, fixed per plan) is split once into planes [NPL][N][ldk]
-// (k contiguous, zero past K); A is split while it is staged to LDS.
+// (a Bessel table, fixed per plan) is split once into NPL planes of N ldk
+// values, zero past K, laid out slab-major: [ldk / 32][N][32], so the B of
+// one 32-wavenumber slab of a tile's columns is one contiguous run (with the
+// k rows of each column 32 KB apart, every slab touched a separate cache
+// line per column and plane); A is split while it is staged to LDS.
+__host__ __device__ __forceinline__ size_t split_off(int N, int n, int k)
+{
+    return size_t(k >> 5) * (size_t(N) << 5) + (size_t(n) << 5) + size_t(k & 31);
+}
 template <int NPL>
 __device__ __forceinline__ void split_terms(float x, __bf16 (&t)[NPL])
 {
@@ -641,20 +655,22 @@ __device__ __forceinline__ void split_terms(float x, __bf16 (&t)[NPL])
     }
 }
 
-template <int BM, int BN, int WM, int WN, int NPL, int NP>
+template <int BM, int BN, int WM, int WN, int NPL, int NP, int BK = 32>
 struct BSTile {
-    static constexpr int NT = (BM / WM) * (BN / WN) * 64, BK = 32, KP = BK + 8;
+    static constexpr int NT = (BM / WM) * (BN / WN) * 64, KP = BK + 8;
+    static constexpr int AR = BK / 4, BR = BK / 8;  // 16-byte A (FP32) and B (bf16) chunks per slab row
     static constexpr int TM = WM / 32, TN = WN / 32, AE = BM * BK / 4 / NT;
     static constexpr int BC = BN * BK / 8;  // 16-byte B chunks per plane
     static_assert(AE * NT * 4 == BM * BK, "A staging must cover the slab");
     static_assert((NPL == 2 && NP == 3) || (NPL == 3 && NP == 6), "split");
+    static_assert(BK % 16 == 0, "one MFMA k step is 16");
     struct Smem {
         __bf16 As[2][NPL][BM][KP];
         __bf16 Bs[2][NPL][BN][KP];
     };
 
     // acc = A[m0 : m0 + BM, kb : ke] B[kb : ke, n0 : n0 + BN] as F32Tile::mma;
-    // S: the split B planes, plane stride N ldk.
+    // S: the split B planes (split_off layout), plane stride N ldk.
     static __device__ __forceinline__ void mma(const float *__restrict__ A, const __bf16 *__restrict__ S, long lda,
                                                long ldk, int M, int N, int m0, int n0, int kb, int ke, int rpw,
                                                const int *__restrict__ nk_w, Smem &sm, v16f (&acc)[TM][TN],
@@ -666,49 +682,46 @@ struct BSTile {
         size_t a_row[AE];
 #pragma unroll
         for (int e = 0; e < AE; e++) {
-            const int gm = m0 + ((tid + NT * e) >> 3);
+            const int gm = m0 + (tid + NT * e) / AR;
             a_lim[e] = gm < M ? (nk_w ? min(ke, nk_w[gm / rpw]) : ke) : 0;
             a_row[e] = foff ? (gm < M ? size_t(foff[gm / rpw]) + size_t(gm % rpw) * size_t(fext[gm / rpw]) : 0)
                             : size_t(gm) * size_t(lda);
         }
         const size_t pstride = size_t(N) * size_t(ldk);
-        float4 ra[AE];
-        uint4 rb[NPL][(BC + NT - 1) / NT];
-        auto load = [&](int k0) {
+        // one K slab of the block's A (FP32) and split B (bf16 planes) in registers
+        struct Regs {
+            float4 ra[AE];
+            int rlim[AE];  // values at or past the row's cut are zeroed as they are stored
+            uint4 rb[NPL][(BC + NT - 1) / NT];
+            bool bval[(BC + NT - 1) / NT];  // rb[.][c] holds B values (else stored as zeros)
+        };
+        auto load = [&](int k0, Regs &R) {
 #pragma unroll
             for (int e = 0; e < AE; e++) {
-                const int idx = tid + NT * e, gk = k0 + (idx & 7) * 4;
-                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (gk < a_lim[e]) {
-                    // 16-byte aligned and inside the row (gk and the row's length are multiples of 4)
-                    v = *reinterpret_cast<const float4 *>(A + a_row[e] + gk);
-                    const int lim = a_lim[e] - gk;
-                    if (lim < 4) {
-                        v.w = 0.f;
-                        if (lim < 3) v.z = 0.f;
-                        if (lim < 2) v.y = 0.f;
-                    }
-                }
-                ra[e] = v;
+                const int idx = tid + NT * e, gk = k0 + (idx % AR) * 4;
+                R.rlim[e] = a_lim[e] - gk;
+                // 16-byte aligned and inside the row where rlim > 0 (gk and the
+                // row's length are multiples of 4)
+                R.ra[e] = *reinterpret_cast<const float4 *>(A + (R.rlim[e] > 0 ? a_row[e] + gk : 0));
             }
 #pragma unroll
             for (int c = 0; c < (BC + NT - 1) / NT; c++) {
-                const int q = tid + NT * c, n = q >> 2, kq = (q & 3) * 8;
+                const int q = tid + NT * c, n = q / BR, kq = (q % BR) * 8;
+                // k0 + kq + 7 < ldk (a multiple of 32 at least ke) for k0 < ke,
+                // within one 32-value run of the slab-major planes; columns past
+                // N and slabs past ke are stored as zeros
+                R.bval[c] = q < BC && n0 + n < N && k0 < ke;
+                const size_t o = R.bval[c] ? split_off(N, n0 + n, k0 + kq) : 0;
 #pragma unroll
-                for (int p = 0; p < NPL; p++) {
-                    uint4 v = make_uint4(0u, 0u, 0u, 0u);
-                    // k0 + kq + 7 < ldk (a multiple of 32 at least ke); columns past N read zeros
-                    if (q < BC && n0 + n < N)
-                        v = *reinterpret_cast<const uint4 *>(S + p * pstride + size_t(n0 + n) * ldk + k0 + kq);
-                    rb[p][c] = v;
-                }
+                for (int p = 0; p < NPL; p++) R.rb[p][c] = *reinterpret_cast<const uint4 *>(S + p * pstride + o);
             }
         };
-        auto store = [&](int buf) {
+        auto store = [&](int buf, const Regs &R) {
 #pragma unroll
             for (int e = 0; e < AE; e++) {
-                const int idx = tid + NT * e, r = idx >> 3, k = (idx & 7) * 4;
-                const float x[4] = {ra[e].x, ra[e].y, ra[e].z, ra[e].w};
+                const int idx = tid + NT * e, r = idx / AR, k = (idx % AR) * 4;
+                const float4 a = cut_tail(R.ra[e], R.rlim[e]);
+                const float x[4] = {a.x, a.y, a.z, a.w};
                 __bf16 t[4][NPL];
 #pragma unroll
                 for (int j = 0; j < 4; j++) split_terms<NPL>(x[j], t[j]);
@@ -722,26 +735,16 @@ struct BSTile {
             }
 #pragma unroll
             for (int c = 0; c < (BC + NT - 1) / NT; c++) {
-                const int q = tid + NT * c, n = q >> 2, kq = (q & 3) * 8;
+                const int q = tid + NT * c, n = q / BR, kq = (q % BR) * 8;
                 if (q < BC)
 #pragma unroll
-                    for (int p = 0; p < NPL; p++) *reinterpret_cast<uint4 *>(&sm.Bs[buf][p][n][kq]) = rb[p][c];
+                    for (int p = 0; p < NPL; p++)
+                        *reinterpret_cast<uint4 *>(&sm.Bs[buf][p][n][kq]) =
+                            R.bval[c] ? R.rb[p][c] : make_uint4(0u, 0u, 0u, 0u);
             }
         };
-#pragma unroll
-        for (int a = 0; a < TM; a++)
-#pragma unroll
-            for (int b = 0; b < TN; b++)
-#pragma unroll
-                for (int r = 0; r < 16; r++) acc[a][b][r] = 0.f;
-        load(kb);
-        store(0);
-        __syncthreads();
         const int i = lane & 31, kq = (lane >> 5) * 8;
-        int buf = 0;
-        for (int k0 = kb; k0 < ke; k0 += BK) {
-            const bool more = k0 + BK < ke;
-            if (more) load(k0 + BK);
+        auto compute = [&](int buf) {
 #pragma unroll
             for (int kk = 0; kk < BK; kk += 16) {
                 v8bf fa[NPL][TM], fb[NPL][TN];
@@ -754,22 +757,39 @@ struct BSTile {
                     for (int t = 0; t < TN; t++)
                         fb[p][t] = *reinterpret_cast<const v8bf *>(&sm.Bs[buf][p][wn * WN + t * 32 + i][kk + kq]);
                 }
+                // products (small ones first) outer, tiles inner: consecutive
+                // MFMAs of a wave with several tiles go to different accumulators
+                constexpr int PA[6] = {2, 1, 0, 1, 0, 0}, PB[6] = {0, 1, 2, 0, 1, 0};
 #pragma unroll
-                for (int tm = 0; tm < TM; tm++)
+                for (int q = 6 - NP; q < 6; q++)
 #pragma unroll
-                    for (int tn = 0; tn < TN; tn++) {
-                        // small products first
-                        if constexpr (NP == 6) {
-                            acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[2][tm], fb[0][tn], acc[tm][tn], 0, 0, 0);
-                            acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][tm], fb[1][tn], acc[tm][tn], 0, 0, 0);
-                            acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][tm], fb[2][tn], acc[tm][tn], 0, 0, 0);
-                        }
-                        acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[1][tm], fb[0][tn], acc[tm][tn], 0, 0, 0);
-                        acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][tm], fb[1][tn], acc[tm][tn], 0, 0, 0);
-                        acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[0][tm], fb[0][tn], acc[tm][tn], 0, 0, 0);
-                    }
+                    for (int tm = 0; tm < TM; tm++)
+#pragma unroll
+                        for (int tn = 0; tn < TN; tn++)
+                            acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[PA[q]][tm], fb[PB[q]][tn],
+                                                                                  acc[tm][tn], 0, 0, 0);
             }
-            if (more) store(buf ^ 1);
+        };
+#pragma unroll
+        for (int a = 0; a < TM; a++)
+#pragma unroll
+            for (int b = 0; b < TN; b++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[a][b][r] = 0.f;
+        // The next slab's global loads are issued before the current slab's
+        // products and stored to the other LDS buffer after them.  They are
+        // issued unconditionally (a slab at or past ke loads clamped
+        // addresses and is never stored): a load on a conditional path would
+        // make the compiler wait for it where the paths join.
+        Regs R;
+        load(kb, R);
+        store(0, R);
+        __syncthreads();
+        int buf = 0;
+        for (int k0 = kb; k0 < ke; k0 += BK) {
+            load(k0 + BK, R);
+            compute(buf);
+            if (k0 + BK < ke) store(buf ^ 1, R);
             __syncthreads();
             buf ^= 1;
         }
@@ -799,14 +819,15 @@ struct BSTile {
 struct SplitB {
     const __bf16 *S[kGemmSetMax];
 };
-template <int BM, int BN, int WM, int WN, int NPL, int NP>
-__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bssk(int n, int N, int K, GemmSet gp, SplitB sb,
+// WPE: waves per SIMD the registers must leave room for (the launch bound)
+template <int BM, int BN, int WM, int WN, int NPL, int NP, int BK, int WPE>
+__global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, WPE) void k_gemm_bssk(int n, int N, int K, GemmSet gp, SplitB sb,
                                                                        StreamK sk, long lda, long ldk,
                                                                        const int *__restrict__ nk_w,
                                                                        const int *__restrict__ off,
                                                                        float *__restrict__ part)
 {
-    using Tl = BSTile<BM, BN, WM, WN, NPL, NP>;
+    using Tl = BSTile<BM, BN, WM, WN, NPL, NP, BK>;
     __shared__ __attribute__((aligned(16))) typename Tl::Smem sm;
     const int T = sk.cum[n];
     const long G = off[T];
@@ -837,13 +858,16 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64) void k_gemm_bssk(int n,
     }
 }
 
-// B [K][ldb] (FP32, N columns) into NPL bf16 planes [NPL][N][ldk], zero for K <= k < ldk.
+// B [K][ldb] (FP32, N columns) into NPL bf16 planes of N ldk values (split_off
+// layout), zero for K <= k < ldk.
 template <int NPL>
 __global__ void k_split_table(int K, int N, const float *__restrict__ B, long ldb, __bf16 *__restrict__ S, long ldk)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)N * ldk) return;
-    const int n = int(t / ldk), k = int(t % ldk);
+    // t = split_off(N, n, k)
+    const size_t slab = t / (size_t(N) << 5), r = t - slab * (size_t(N) << 5);
+    const int n = int(r >> 5), k = int(slab * 32 + (r & 31));
     __bf16 v[NPL];
     split_terms<NPL>(k < K ? B[(size_t)k * ldb + n] : 0.f, v);
 #pragma unroll
@@ -1067,14 +1091,22 @@ static hipError_t launch_streamk(int n, const GemmSet &gs, int N, int K, long ld
 // Split-BF16 pair (Z and R contractions) by stream-K, one resident block of
 // 8 waves per compute unit (the double-buffered split tiles take 82-123 KB of
 // LDS).  Workspace: gemm_bsplit_workspace_bytes.
+// More than 128 receivers: 128 x 256 block tiles of 64 x 64 wave tiles and
+// K slabs of 16 (111 KB of LDS).  A wave then reads NPL (TM + TN) = 12
+// fragments from LDS for NP TM TN = 24 products where the 32 x 64 wave tile
+// read 9 for 12: at 32 x 64 the LDS reads alone (0.75 KB per 32-cycle
+// product per SIMD, against 1 KB of LDS bandwidth) left the matrix cores
+// under half busy.  A 256-wide tile also stages and splits each A slab once
+// for 256 receivers instead of once per 128.
 constexpr int kSplitBlocks = 256;
 static int bs_blocks(long tiles, int K) { return (int)std::min<long>(kSplitBlocks, tiles * ((K + kTBK - 1) / kTBK)); }
+static int bs_bn(int N) { return N > 128 ? 256 : f32_bn(N); }
 
 size_t gemm_bsplit_workspace_bytes(int M, int N, int K)
 {
-    const int Ms[2] = {M, M};
-    const long tiles = set_tiles(1, 2, Ms, N);
-    return (size_t)bs_blocks(tiles, K) * 2 * kTBM * f32_bn(N) * sizeof(float) + (size_t)(tiles + 1) * sizeof(int);
+    const int bn = bs_bn(N);
+    const long tiles = 2L * ((M + kTBM - 1) / kTBM) * ((N + bn - 1) / bn);
+    return (size_t)bs_blocks(tiles, K) * 2 * kTBM * bn * sizeof(float) + (size_t)(tiles + 1) * sizeof(int);
 }
 
 hipError_t launch_split_table(int nprod, int K, int N, const float *B, long ldb, void *S, long ldk, hipStream_t st)
@@ -1104,7 +1136,7 @@ hipError_t launch_gemm_pair_bsplit(int nprod, int M, int N, int K, const float *
     SplitB sb{};
     sb.S[0] = (const __bf16 *)S0;
     sb.S[1] = (const __bf16 *)S1;
-    const int bn = f32_bn(N);
+    const int bn = bs_bn(N);
     StreamK sk{};
     int t = 0;
     for (int q = 0; q < kGemmSetMax; q++) {
@@ -1117,17 +1149,22 @@ hipError_t launch_gemm_pair_bsplit(int nprod, int M, int N, int K, const float *
     float *part = P;
     int *off = reinterpret_cast<int *>(P + (size_t)sk.nb * 2 * kTBM * bn);
     hipLaunchKernelGGL(k_streamk_plan<kTBM>, dim3(1), dim3(1024), 0, st, 2, K, gs, sk, off);
-#define GRTM_BS(BNV, WNV, NPL, NP)                                                                                   \
-    hipLaunchKernelGGL((k_gemm_bssk<kTBM, BNV, 32, WNV, NPL, NP>), dim3(sk.nb), dim3(512), 0, st, 2, N, K, gs, sb, sk, \
-                       lda, ldk, nk_w, off, part)
-    if (bn == 128) {
-        if (nprod == 6) GRTM_BS(128, 64, 3, 6);
-        else GRTM_BS(128, 64, 2, 3);
+#define GRTM_BS(BNV, WMV, WNV, NPL, NP, BKV, WPEV)                                                             \
+    hipLaunchKernelGGL((k_gemm_bssk<kTBM, BNV, WMV, WNV, NPL, NP, BKV, WPEV>), dim3(sk.nb), dim3(512), 0, st, 2, N, K, \
+                       gs, sb, sk, lda, ldk, nk_w, off, part)
+    if (bn == 256) {
+        if (nprod == 6) GRTM_BS(256, 64, 64, 3, 6, 16, 2);
+        else GRTM_BS(256, 64, 64, 2, 3, 16, 2);
+        hipLaunchKernelGGL((k_streamk_reduce<kTBM, 256>), dim3((unsigned)t, kReduceSlices), dim3(256), 0, st, 2, N, gs,
+                           sk, off, part);
+    } else if (bn == 128) {
+        if (nprod == 6) GRTM_BS(128, 32, 64, 3, 6, 32, 2);
+        else GRTM_BS(128, 32, 64, 2, 3, 32, 2);
         hipLaunchKernelGGL((k_streamk_reduce<kTBM, 128>), dim3((unsigned)t, kReduceSlices), dim3(256), 0, st, 2, N, gs,
                            sk, off, part);
     } else {
-        if (nprod == 6) GRTM_BS(64, 32, 3, 6);
-        else GRTM_BS(64, 32, 2, 3);
+        if (nprod == 6) GRTM_BS(64, 32, 32, 3, 6, 32, 2);
+        else GRTM_BS(64, 32, 32, 2, 3, 32, 2);
         hipLaunchKernelGGL((k_streamk_reduce<kTBM, 64>), dim3((unsigned)t, kReduceSlices), dim3(256), 0, st, 2, N, gs,
                            sk, off, part);
     }

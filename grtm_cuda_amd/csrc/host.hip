@@ -831,8 +831,8 @@ size_t pair_sk_bytes(int bsplit, int M, int N, int K)
 }
 
 // Split-BF16 contraction (GRTM_PREC_BF16X3 / GRTM_PREC_BF16X6 with a pair
-// layout): the two Bessel tables' bf16 planes, [planes][nr][ldk] each, ldk
-// the wavenumber count rounded up to 32.
+// layout): the two Bessel tables' bf16 planes, planes x nr x ldk values each
+// (slab-major by 32 wavenumbers), ldk the wavenumber count rounded up to 32.
 int split_alloc(int bsplit, int nk, int nr, DBuf &S0, DBuf &S1, long &ldk)
 {
     ldk = (nk + 31) / 32 * 32;
