@@ -373,8 +373,8 @@ def roofline(g, plan, prec, tm, work, counted=False):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "mt"],
                     help="c2: one model, 256 receivers (the headline); c3: precision study; "
                          "c4: batch of --models models; mt: all displacement columns of the c2 model")

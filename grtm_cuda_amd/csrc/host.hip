@@ -934,12 +934,21 @@ int run_cells_core(grtm_plan *p, hipStream_t s)
         if ((st = grtm::learned_grid(p->net, p->lmodel.as<grtm::LearnModel>(), 1, p->lg, p->nk_pred.as<int>(),
                                      p->nchk.as<int>(), s)))
             return st;
+        // the item kernels also start the execution's state: nk_w = the cut,
+        // zero row peaks, tail maxima, fallback and flag counts, item counters
+        grtm::GridInit gi{};
+        gi.zero[0] = fbcnt;
+        gi.zero[1] = p->prec != GRTM_PREC_FP64 ? p->flags.as<unsigned>() : nullptr;
+        gi.zero[2] = p->ictr.as<unsigned>();
+        gi.zero[3] = p->ictr.as<unsigned>() + 1;
+        gi.nk_w = nkw;
+        gi.row_peak = rowpk;
+        gi.tail_max = tailmx;
         HIPCHK(grtm::launch_grid_items(p->nw, r64, p->nk_pred.as<int>(), p->nchk.as<int>(), p->tiles.as<unsigned>(),
-                                       p->ioff.as<unsigned>(), p->icounts.as<unsigned>(), s));
+                                       p->ioff.as<unsigned>(), p->icounts.as<unsigned>(), s, gi));
         if (p->ev_pred1) HIPCHK(hipEventRecord(p->ev_pred1, s));
         p->pred_stale = true;
-    }
-    if (ad) {
+    } else if (ad) {
         HIPCHK(grtm::launch_copy_i32(p->nk_pred.as<int>(), nkw, p->nw, s));
         HIPCHK(grtm::launch_zero_u32(rowpk, 2 * p->nw + 1, s));
     }
@@ -959,18 +968,18 @@ int run_cells_core(grtm_plan *p, hipStream_t s)
         if (p->ev_main0) HIPCHK(hipEventRecord(p->ev_main0, s));
         HIPCHK(grtm::launch_cells_zr(0, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0, p->nw,
                                      t_lo, n_lo, p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk, s, nchk,
-                                     p->chk_stride, tailmx, dcnt, 0, ctr));
+                                     p->chk_stride, tailmx, dcnt, 0, ctr, 0, learned));
         if (p->ev_main1) HIPCHK(hipEventRecord(p->ev_main1, s));
     } else {
         unsigned *cnt = p->flags.as<unsigned>(), *list = cnt + 1;
-        HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
+        if (!learned) HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
         HIPCHK(grtm::launch_cells_zr(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0,
                                      p->rows64, t_lo, n_lo, p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk, s,
-                                     nchk, p->chk_stride, tailmx, dcnt, 0, ctr));
+                                     nchk, p->chk_stride, tailmx, dcnt, 0, ctr, 0, learned));
         if (p->ev_main0) HIPCHK(hipEventRecord(p->ev_main0, s));
         HIPCHK(grtm::launch_cells_zr(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->rows64,
                                      p->nw - p->rows64, t_hi, n_hi, p->A0.p, p->A1.p, p->nk, cnt, list, nkw, rowpk, s,
-                                     nchk, p->chk_stride, tailmx, dcnt, 1, ctr ? ctr + 1 : nullptr));
+                                     nchk, p->chk_stride, tailmx, dcnt, 1, ctr ? ctr + 1 : nullptr, 0, learned));
         if (p->ev_main1) HIPCHK(hipEventRecord(p->ev_main1, s));
         if (p->nplant)
             HIPCHK(grtm::launch_plant_nonfinite(p->plant.as<unsigned>(), p->nplant, p->nk, p->A0.p, p->A1.p, p->nk,
@@ -979,19 +988,18 @@ int run_cells_core(grtm_plan *p, hipStream_t s)
     const bool mixed = p->prec != GRTM_PREC_FP64;
     unsigned *cnt = mixed ? p->flags.as<unsigned>() : nullptr, *list = mixed ? cnt + 1 : nullptr;
     if (ad) {
-        // dense fallback of the rows whose check failed (each row in its own precision)
+        // dense fallback of the rows whose check failed (each row in its own
+        // precision); the selection also forms the contraction tiles' K extents
         const void *lwf64 = mixed ? p->lw64.p : p->lw.p;
         HIPCHK(grtm::launch_fallback(!mixed, p->compat, lwf64, mixed ? p->lw.p : nullptr, r64, p->L, p->g, p->nk,
                                      p->nw, p->dk, p->df, p->omega_i, p->chk_tol, rowpk, tailmx, nkw, fbcnt,
-                                     p->fb_list.as<unsigned>(), p->A0.p, p->A1.p, p->nk, cnt, list, 0, s));
+                                     p->fb_list.as<unsigned>(), p->A0.p, p->A1.p, p->nk, cnt, list, 0, s, nullptr, 0,
+                                     p->kend.as<int>(), (p->gemm_mode == 0 && !p->pair) ? 16 : 32));
     }
     // FP64 re-solve of the non-finite FP32 cells (main grid and fallback rows)
     if (mixed)
         HIPCHK(grtm::launch_cells_fix(p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->A0.p,
                                       p->A1.p, p->nk, cnt, list, s));
-    if (ad)
-        HIPCHK(grtm::launch_kend_tiles(p->nw, (p->gemm_mode == 0 && !p->pair) ? 16 : 32, p->nk, nkw,
-                                       p->kend.as<int>(), s));
     return GRTM_OK;
 }
 int run_gemm_mt(grtm_plan *p, hipStream_t s)

@@ -76,7 +76,7 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
                            void *A1, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
                            unsigned *row_peak, hipStream_t st, const int *nchk = nullptr, int stride = 0,
                            unsigned *tail_max = nullptr, const unsigned *dev_count = nullptr, int which = 0,
-                           unsigned *ctr = nullptr, int k_base = 0);
+                           unsigned *ctr = nullptr, int k_base = 0, bool ctr_zeroed = false);
 
 // ---- learned wavenumber grid built on the device
 // The predictor's inputs for one model and the decoding of its cut output
@@ -98,23 +98,35 @@ struct LearnGrid {
 // n_models models from one execution of the network (learned.hip / mlp.hip).
 int learned_grid(grtm_mlp *net, const LearnModel *models, int n_models, const LearnGrid &lg, int *nk_pred,
                  int *nchk, hipStream_t st);
+// The state an execution of a learned grid starts from, set by the item
+// kernels instead of launches of their own: up to kGridZero scalar counters
+// cleared, and per row nk_w = nk_pred, row peak and checked-tail maximum zero
+// (null pointers are skipped).
+constexpr int kGridZero = 4;
+struct GridInit {
+    unsigned *zero[kGridZero];
+    int *nk_w;
+    unsigned *row_peak, *tail_max;
+};
 // Recursion items of one model's learned grid from nk_pred / nchk: tiles of
 // rows [0, r64), their residual-check items, tiles of rows [r64, nw), theirs;
 // counts = {0, items of rows [0, r64), all items}.  off: 2 nw scratch words
 // (each row's first tile and first check item).
 hipError_t launch_grid_items(int nw, int r64, const int *nk_pred, const int *nchk, unsigned *items, unsigned *off,
-                             unsigned *counts, hipStream_t st);
+                             unsigned *counts, hipStream_t st, const GridInit &gi);
 // adaptive grid: dense fallback, tile extents
 // Rows whose checked tail exceeds tol x row peak are completed on the dense
 // grid: FP64 plans in FP64; mixed plans in FP64 below row r64 and in FP32
 // (lw32) above it, non-finite FP32 cells appended to the flag list as
-// (row_base + row) nk + k.
+// (row_base + row) nk + k.  With kend, the same launch that selects the rows
+// also forms the K extents of contraction tiles of rows_per_tile (<= 64) rows
+// from the final counts, as launch_kend_tiles.
 hipError_t launch_fallback(int out_f64, int compat, const void *lw64, const void *lw32, int r64, int L, const Geo &g,
                            int nk, int nw, double dk, double df, double omega_i, double tol,
                            const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *fb_count,
                            unsigned *fb_list, void *A0, void *A1, long lda, unsigned *flag_count,
                            unsigned *flag_list, int row_base, hipStream_t st, const MtOps *mto = nullptr,
-                           int deriv = 0);
+                           int deriv = 0, int *kend = nullptr, int rows_per_tile = 0);
 hipError_t launch_kend_tiles(int nw, int rows_per_tile, int nk, const int *nk_w, int *kend, hipStream_t st);
 // extents of the bm-row tiles of n contractions with rpw[q] rows per frequency
 // row, laid end to end in kend

@@ -119,11 +119,13 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__re
 // host (tiles of rows [0, r64) in row order, then their residual-check items,
 // then the tiles of rows [r64, nw), then theirs): off[iw], off[nw + iw]; and
 // counts = {0, items of rows [0, r64), all items}.  k_grid_fill
-// (one wave per row) writes the items.
+// (one wave per row) writes the items.  The two also start the execution's
+// per-row and scalar state (GridInit), which took five launches of their own.
 __global__ __launch_bounds__(1024) void k_grid_scan(int nw, int r64, const int *__restrict__ nk_pred,
                                                     const int *__restrict__ nchk, unsigned *__restrict__ off,
-                                                    unsigned *__restrict__ counts)
+                                                    unsigned *__restrict__ counts, GridInit gi)
 {
+    if (threadIdx.x < kGridZero && gi.zero[threadIdx.x]) *gi.zero[threadIdx.x] = 0u;
     constexpr int R = 8;
     __shared__ unsigned st[1024], sc[1024];
     __shared__ unsigned t64_sh, c64_sh;
@@ -174,10 +176,15 @@ __global__ __launch_bounds__(1024) void k_grid_scan(int nw, int r64, const int *
 
 __global__ __launch_bounds__(256) void k_grid_fill(int nw, const int *__restrict__ nk_pred,
                                                    const int *__restrict__ nchk, const unsigned *__restrict__ off,
-                                                   unsigned *__restrict__ items)
+                                                   unsigned *__restrict__ items, GridInit gi)
 {
     const int iw = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (iw >= nw) return;
+    if (lane == 0) {
+        if (gi.nk_w) gi.nk_w[iw] = nk_pred[iw];
+        if (gi.row_peak) gi.row_peak[iw] = 0u;
+        if (gi.tail_max) gi.tail_max[iw] = 0u;
+    }
     const unsigned nt = unsigned((nk_pred[iw] + kCellThreads - 1) / kCellThreads);
     const unsigned nc = unsigned((nchk[iw] + kCellThreads - 1) / kCellThreads);
     const unsigned tb = off[iw], cb = off[nw + iw];
@@ -185,20 +192,45 @@ __global__ __launch_bounds__(256) void k_grid_fill(int nw, const int *__restrict
     for (unsigned k = lane; k < nc; k += 64) items[cb + k] = ((unsigned(iw) << kTileBits) | k) | kCheckItem;
 }
 
-// Rows whose checked tail exceeds tol * row peak fall back to the dense grid.
+// Rows whose checked tail exceeds tol * row peak fall back to the dense grid;
+// returns the row's final count of wavenumbers.
+__device__ __forceinline__ int fallback_pick(int iw, int nw, int nk, double tol, const unsigned *__restrict__ row_peak,
+                                             const unsigned *__restrict__ tail_max, int *__restrict__ nk_w,
+                                             unsigned *__restrict__ fb_count, unsigned *__restrict__ fb_list)
+{
+    const float pk = __uint_as_float(row_peak[iw]), tm = __uint_as_float(tail_max[iw]);
+    const int n = nk_w[iw];
+    if (n < nk && !(tm <= float(tol) * pk)) {
+        const unsigned slot = atomicAdd(fb_count, 1u);
+        fb_list[slot] = unsigned(iw);
+        fb_list[nw + slot] = unsigned(n);  // first wavenumber still to solve
+        nk_w[iw] = nk;
+        return nk;
+    }
+    return n;
+}
 __global__ void k_fallback_select(int nw, int nk, double tol, const unsigned *__restrict__ row_peak,
                                   const unsigned *__restrict__ tail_max, int *__restrict__ nk_w,
                                   unsigned *__restrict__ fb_count, unsigned *__restrict__ fb_list)
 {
     const int iw = blockIdx.x * blockDim.x + threadIdx.x;
     if (iw >= nw) return;
-    const float pk = __uint_as_float(row_peak[iw]), tm = __uint_as_float(tail_max[iw]);
-    if (nk_w[iw] < nk && !(tm <= float(tol) * pk)) {
-        const unsigned slot = atomicAdd(fb_count, 1u);
-        fb_list[slot] = unsigned(iw);
-        fb_list[nw + slot] = unsigned(nk_w[iw]);  // first wavenumber still to solve
-        nk_w[iw] = nk;
-    }
+    (void)fallback_pick(iw, nw, nk, tol, row_peak, tail_max, nk_w, fb_count, fb_list);
+}
+// The selection and the contraction tiles' K extents (k_kend_tiles) in one
+// launch: wave t takes the rpt <= 64 rows of tile t, selects its fallback
+// rows, and forms the tile's extent from their final counts.
+__global__ __launch_bounds__(64) void k_fallback_select_kend(int nw, int nk, double tol, int rpt,
+                                                             const unsigned *__restrict__ row_peak,
+                                                             const unsigned *__restrict__ tail_max,
+                                                             int *__restrict__ nk_w, unsigned *__restrict__ fb_count,
+                                                             unsigned *__restrict__ fb_list, int *__restrict__ kend)
+{
+    const int t = blockIdx.x, iw = t * rpt + int(threadIdx.x);
+    int n = 0;
+    if (int(threadIdx.x) < rpt && iw < nw) n = fallback_pick(iw, nw, nk, tol, row_peak, tail_max, nk_w, fb_count, fb_list);
+    const int m = int(wave_max_u32(unsigned(n)));
+    if (threadIdx.x == 0) kend[t] = min(nk, (m + 31) / 32 * 32);
 }
 
 // Dense completion of fallback rows: wavenumbers [start, nk) of each listed
@@ -733,11 +765,11 @@ static int resident_blocks(const void *kernel, int ub)
 }
 
 hipError_t launch_grid_items(int nw, int r64, const int *nk_pred, const int *nchk, unsigned *items, unsigned *off,
-                             unsigned *counts, hipStream_t st)
+                             unsigned *counts, hipStream_t st, const GridInit &gi)
 {
     if (nw < 1 || nw > 8192) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_grid_scan, dim3(1), dim3(1024), 0, st, nw, r64, nk_pred, nchk, off, counts);
-    hipLaunchKernelGGL(k_grid_fill, dim3((nw + 3) / 4), dim3(256), 0, st, nw, nk_pred, nchk, off, items);
+    hipLaunchKernelGGL(k_grid_scan, dim3(1), dim3(1024), 0, st, nw, r64, nk_pred, nchk, off, counts, gi);
+    hipLaunchKernelGGL(k_grid_fill, dim3((nw + 3) / 4), dim3(256), 0, st, nw, nk_pred, nchk, off, items, gi);
     return hipGetLastError();
 }
 
@@ -745,15 +777,17 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
                            double df, double omega_i, int w0, int nwr, const unsigned *tiles, int ntiles, void *A0,
                            void *A1, long lda, unsigned *flag_count, unsigned *flag_list, const int *nk_w,
                            unsigned *row_peak, hipStream_t st, const int *nchk, int stride, unsigned *tail_max,
-                           const unsigned *dev_count, int which, unsigned *ctr, int k_base)
+                           const unsigned *dev_count, int which, unsigned *ctr, int k_base, bool ctr_zeroed)
 {
     const int tpr = cells_tiles_per_row(nk);
     if (!tiles) ntiles = nwr * tpr;
     if (ntiles <= 0) return hipSuccess;
     if (dev_count) {
         if (!tiles || !ctr || which < 0) return hipErrorInvalidValue;
-        hipError_t e = launch_zero_u32(ctr, 1, st);
-        if (e != hipSuccess) return e;
+        if (!ctr_zeroed) {
+            hipError_t e = launch_zero_u32(ctr, 1, st);
+            if (e != hipSuccess) return e;
+        }
     }
 #define GRTM_ZR_BR(T, TO, CP, BR, FC, FL)                                                                       \
     hipLaunchKernelGGL((k_cells_zr<T, TO, CP, BR>),                                                              \
@@ -789,11 +823,18 @@ hipError_t launch_fallback(int out_f64, int compat, const void *lw64, const void
                            int nk, int nw, double dk, double df, double omega_i, double tol,
                            const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *fb_count,
                            unsigned *fb_list, void *A0, void *A1, long lda, unsigned *flag_count,
-                           unsigned *flag_list, int row_base, hipStream_t st, const MtOps *mto, int deriv)
+                           unsigned *flag_list, int row_base, hipStream_t st, const MtOps *mto, int deriv,
+                           int *kend, int rows_per_tile)
 {
     if (deriv && (compat || !mto)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_fallback_select, dim3((nw + 255) / 256), dim3(256), 0, st, nw, nk, tol, row_peak, tail_max,
-                       nk_w, fb_count, fb_list);
+    if (kend) {
+        if (rows_per_tile < 1 || rows_per_tile > 64) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_fallback_select_kend, dim3((nw + rows_per_tile - 1) / rows_per_tile), dim3(64), 0, st, nw,
+                           nk, tol, rows_per_tile, row_peak, tail_max, nk_w, fb_count, fb_list, kend);
+    } else {
+        hipLaunchKernelGGL(k_fallback_select, dim3((nw + 255) / 256), dim3(256), 0, st, nw, nk, tol, row_peak,
+                           tail_max, nk_w, fb_count, fb_list);
+    }
     dim3 grid((nk + kCellThreads - 1) / kCellThreads, 16);
     const MtOps o = mto ? *mto : MtOps{};
 #define GRTM_FB(T, TO, CP, MT, LWP, LO, HI, FC, FL)                                                             \
