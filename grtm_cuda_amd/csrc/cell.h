@@ -540,13 +540,26 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             out.y0t = d0t * y_sh;
         }
     } else if (branch == 1) {  // green.cu:775-857
-        cx<T> rd0 = cexp_nz<T>(qr.rd, zr), rd1 = cexp_nz<T>(qr.vd, zr);
         cx<T> ru0 = cexp_nz<T>(qr.rd, hr_ - zr), ru1 = cexp_nz<T>(qr.vd, hr_ - zr);
         // Lambda of the receiver layer: e^{-nu h} = e^{-nu zr} e^{-nu (h - zr)}
-        const cx<T> ex0 = rd0 * ru0, ex1 = rd1 * ru1;
-        const m2<T> RS = dmd(rd0, rd1, Rud_r, ex0, ex1);
+        cx<T> rd0, rd1, ex0, ex1, u0;
+        m2<T> RS;
+        if (!NEED_T && zr == W(0)) {
+            // receiver on its layer's top (the free surface for rl = 0):
+            // e^{-nu zr} = 1 and Lambda = ru, the general products below with
+            // their unit factors dropped (two exponentials and six complex
+            // products fewer per cell)
+            RS = md(Rud_r, ru0, ru1);
+            u0 = rud0_r * ru1 + ru1;
+        } else {
+            rd0 = cexp_nz<T>(qr.rd, zr);
+            rd1 = cexp_nz<T>(qr.vd, zr);
+            ex0 = rd0 * ru0;
+            ex1 = rd1 * ru1;
+            RS = dmd(rd0, rd1, Rud_r, ex0, ex1);
+            u0 = rd1 * rud0_r * ex1 + ru1;
+        }
         const m2<T> u = mmadd(Ed, RS, md(Eu, ru0, ru1));
-        const cx<T> u0 = rd1 * rud0_r * ex1 + ru1;
         const m2<T> chain = Mu * dm(sexu0, sexu1, yu_psv);
         const cx<T> chain0 = mu0 * sexu1 * y_sh;
         out.Y = u * chain;
