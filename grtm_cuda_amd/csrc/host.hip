@@ -1823,7 +1823,8 @@ int grtm_plan_work(grtm_plan *p, double *w)
         w[2] = 2.0 * 2.0 * (4.0 * p->nw) * p->nk * p->nr;  // two contractions, 2 flop per MAC
         w[3] = 4.0 * p->nr * (8.0 * p->nw + 4.0 * p->N);   // complex64 in + fp32 out per trace component
         w[4] = 2.0 * p->nr;                                 // (model, source, distance) traces, Z+R each
-        if (grtm_opcount_cell(p->L, p->g.s, p->g.rl, p->g.branch, p->g.up, oc)) return GRTM_E_INPUT;
+        if (grtm::opcount_cell_bot(0, p->L, p->g.s, p->g.rl, p->g.branch, p->g.up, -1, oc, p->g.zr == 0.0))
+            return GRTM_E_INPUT;
     } else {
         double rows = 0, terms = 0;
         for (int q = 0; q < p->nsets; q++) rows += p->mt_M(q);
@@ -1831,7 +1832,8 @@ int grtm_plan_work(grtm_plan *p, double *w)
         w[2] = 2.0 * rows * p->nk * p->nr;                         // the contractions of the set
         w[3] = p->nr * (terms * 8.0 * p->nw + p->ncols * 4.0 * p->N);  // complex64 terms in, fp32 out
         w[4] = (double)p->ncols * p->nr;                           // column traces
-        if (grtm_opcount_columns_cell(p->L, p->g.s, p->g.rl, p->g.branch, p->g.up, p->deriv, oc))
+        if (grtm::opcount_cell_bot(p->deriv ? 2 : 1, p->L, p->g.s, p->g.rl, p->g.branch, p->g.up, -1, oc,
+                                   p->g.zr == 0.0))
             return GRTM_E_INPUT;
     }
     w[5] = oc[0];  // flops per cell
@@ -1882,7 +1884,7 @@ int grtm_plan_work(grtm_plan *p, double *w)
             if (nbot[b] == 0) continue;
             double ob[2];
             if (grtm::opcount_cell_bot(p->kind == 0 ? 0 : (p->deriv ? 2 : 1), p->L, p->g.s, p->g.rl, p->g.branch,
-                                       p->g.up, b, ob))
+                                       p->g.up, b, ob, p->g.zr == 0.0))
                 return GRTM_E_INPUT;
             fl += nbot[b] * ob[0];
         }
@@ -2834,7 +2836,8 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
         b->md_h[m].r64 = precision != GRTM_PREC_FP64 ? mt.rows64 : b->nw;
         all64.insert(all64.end(), mt.lw.begin(), mt.lw.end());
         double oc[2] = {0, 0};
-        if (grtm_opcount_cell(mt.st.L, mt.g.s, mt.g.rl, mt.g.branch, mt.g.up, oc)) return GRTM_E_INPUT;
+        if (grtm::opcount_cell_bot(0, mt.st.L, mt.g.s, mt.g.rl, mt.g.branch, mt.g.up, -1, oc, mt.g.zr == 0.0))
+            return GRTM_E_INPUT;
         b->flops += oc[0] * b->nw * b->nk;
     }
     if ((st = b->md.alloc(sizeof(grtm::ModelDesc) * n_models)) ||
@@ -3259,7 +3262,7 @@ int grtm_batch_work(grtm_batch *b, double *out)
         for (int j = 0; j < g.L; j++) {
             if (nbot[j] == 0) continue;
             double oc[2];
-            if (grtm::opcount_cell_bot(0, g.L, g.s, g.rl, g.branch, g.up, j, oc)) return GRTM_E_INPUT;
+            if (grtm::opcount_cell_bot(0, g.L, g.s, g.rl, g.branch, g.up, j, oc, g.zr == 0.0)) return GRTM_E_INPUT;
             fl += nbot[j] * oc[0];
         }
     }

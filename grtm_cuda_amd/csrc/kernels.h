@@ -9,9 +9,11 @@ struct grtm_mlp;  // the predictor network (mlp.hip; C ABI in grtm.h)
 namespace grtm {
 
 // Operation count of one cell whose upward sweep ends at layer bot (the depth
-// truncation's choice; L - 1: none), as grtm_opcount_cell (kind 0) and
-// grtm_opcount_columns_cell (kind 1; 2 with the strain rows).
-int opcount_cell_bot(int kind, int L, int s, int rl, int branch, int up, int bot, double *out);
+// truncation's choice; L - 1: none; -1: the criterion's choice), as
+// grtm_opcount_cell (kind 0) and grtm_opcount_columns_cell (kind 1; 2 with
+// the strain rows); top: the receiver on its layer's top, as a receiver on
+// the free surface (the public counters take a buried receiver).
+int opcount_cell_bot(int kind, int L, int s, int rl, int branch, int up, int bot, double *out, bool top = false);
 
 constexpr int kMaxLayers = 64;
 

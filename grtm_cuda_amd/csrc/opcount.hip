@@ -64,7 +64,7 @@ struct grtm::WideOf<grtm_count::Cnt> {
     using type = grtm_count::Cnt;
 };
 
-static int count_cell(int kind, int L, int s, int rl, int branch, int up, int bot, double *out)
+static int count_cell(int kind, int L, int s, int rl, int branch, int up, int bot, double *out, bool top = false)
 {
     using grtm::cx;
     using grtm::LayerW;
@@ -99,7 +99,8 @@ static int count_cell(int kind, int L, int s, int rl, int branch, int up, int bo
     g.up = up;
     g.pad = 0;
     g.zs = 0.5 * (2.0 + s);
-    g.zr = (rl == s) ? (branch == 2 ? 0.25 * (2.0 + s) : 0.75 * (2.0 + s)) : 0.3;
+    // top: the receiver on its layer's top (the free surface for rl = 0)
+    g.zr = top ? 0.0 : (rl == s) ? (branch == 2 ? 0.25 * (2.0 + s) : 0.75 * (2.0 + s)) : 0.3;
     // the depth-truncation criterion: one iteration per layer from the source
     // down to bot (all L - 1 - s when nothing is cut)
     double crit = 0;
@@ -129,9 +130,9 @@ static int count_cell(int kind, int L, int s, int rl, int branch, int up, int bo
 }
 
 namespace grtm {
-int opcount_cell_bot(int kind, int L, int s, int rl, int branch, int up, int bot, double *out)
+int opcount_cell_bot(int kind, int L, int s, int rl, int branch, int up, int bot, double *out, bool top)
 {
-    return count_cell(kind, L, s, rl, branch, up, bot, out);
+    return count_cell(kind, L, s, rl, branch, up, bot, out, top);
 }
 }  // namespace grtm
 
