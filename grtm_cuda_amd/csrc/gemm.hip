@@ -744,7 +744,17 @@ struct BSTile {
             }
         };
         const int i = lane & 31, kq = (lane >> 5) * 8;
-        auto compute = [&](int buf) {
+        // one slab: its fragments read from LDS buffer buf, the next slab (in
+        // R, loaded an iteration earlier) split and written to buf ^ 1, the
+        // loads of the slab after it into R, then the products.  The split's
+        // VALU work and the LDS writes sit between
+        // the fragment reads and the MFMAs of the same basic block, so they
+        // issue while the matrix cores run (with the split and the writes
+        // after the products, every wave of the block spent each slab first
+        // on the products and then on the split, all at once).  The write to
+        // buf ^ 1 is unconditional: past the last slab it fills a buffer that
+        // is not read again before the next barrier.
+        auto step = [&](int buf, Regs &R, int knext) {
 #pragma unroll
             for (int kk = 0; kk < BK; kk += 16) {
                 v8bf fa[NPL][TM], fb[NPL][TN];
@@ -756,6 +766,10 @@ struct BSTile {
 #pragma unroll
                     for (int t = 0; t < TN; t++)
                         fb[p][t] = *reinterpret_cast<const v8bf *>(&sm.Bs[buf][p][wn * WN + t * 32 + i][kk + kq]);
+                }
+                if (kk == 0) {
+                    store(buf ^ 1, R);
+                    load(knext, R);
                 }
                 // products (small ones first) outer, tiles inner: consecutive
                 // MFMAs of a wave with several tiles go to different accumulators
@@ -776,20 +790,20 @@ struct BSTile {
             for (int b = 0; b < TN; b++)
 #pragma unroll
                 for (int r = 0; r < 16; r++) acc[a][b][r] = 0.f;
-        // The next slab's global loads are issued before the current slab's
-        // products and stored to the other LDS buffer after them.  They are
-        // issued unconditionally (a slab at or past ke loads clamped
-        // addresses and is never stored): a load on a conditional path would
-        // make the compiler wait for it where the paths join.
+        // Slab k + 1 waits in R (loaded during slab k - 1's products) while
+        // slab k is read from LDS; it is split and stored, then slab k + 2's
+        // loads go out, then slab k's products.  The loads are issued
+        // unconditionally (a slab at or past ke loads clamped addresses and is
+        // never read back): a load on a conditional path would make the
+        // compiler wait for it where the paths join.
         Regs R;
         load(kb, R);
         store(0, R);
+        load(kb + BK, R);
         __syncthreads();
         int buf = 0;
         for (int k0 = kb; k0 < ke; k0 += BK) {
-            load(k0 + BK, R);
-            compute(buf);
-            if (k0 + BK < ke) store(buf ^ 1, R);
+            step(buf, R, k0 + 2 * BK);
             __syncthreads();
             buf ^= 1;
         }
