@@ -9,8 +9,10 @@ one pass of recursion + k-sum contraction + inverse FFT producing
 
 The default measures the learned adaptive path (MLP-predicted per-frequency
 wavenumber cut, residual check and dense fallback, FP32 recursion with FP64
-vertical wavenumbers and low-frequency rows, split-BF16 MFMA contraction with
-three bf16 terms per FP32 operand and six products, FP32 accumulation); the
+vertical wavenumbers and low-frequency rows, split-FP16 MFMA contraction: two
+fp16 terms of each row-scaled FP32 operand, three products, FP32
+accumulation; the 1024-model batch line uses the split-BF16x6 contraction,
+three bf16 terms and six products); the
 plain FP64 dense path runs in the same process as the denominator and as the
 accuracy reference (max trace rel-L2 reported).
 
@@ -37,18 +39,20 @@ BF16_PEAK_TFLOPS = 2500.0
 
 
 def prec_code(g, name):
-    return {"fp64": g.PREC_FP64, "fp32": g.PREC_FP32, "bf16x3": g.PREC_BF16X3, "bf16x6": g.PREC_BF16X6}[name]
+    return {"fp64": g.PREC_FP64, "fp32": g.PREC_FP32, "bf16x3": g.PREC_BF16X3, "bf16x6": g.PREC_BF16X6,
+            "fp16x3": g.PREC_FP16X3}[name]
 
 
 def dtype_name(g, prec):
-    return {g.PREC_FP64: "f64", g.PREC_FP32: "f32", g.PREC_BF16X3: "f32+bf16x3", g.PREC_BF16X6: "f32+bf16x6"}[prec]
+    return {g.PREC_FP64: "f64", g.PREC_FP32: "f32", g.PREC_BF16X3: "f32+bf16x3", g.PREC_BF16X6: "f32+bf16x6",
+            g.PREC_FP16X3: "f32+fp16x3"}[prec]
 
 
 def ksum_peak(g, prec):
     """Dense matrix-core peak of the contraction's arithmetic (split-BF16: the
     bf16 rate over the products per term)."""
     return {g.PREC_FP64: FP64_PEAK_TFLOPS, g.PREC_FP32: FP32_PEAK_TFLOPS, g.PREC_BF16X3: BF16_PEAK_TFLOPS / 3,
-            g.PREC_BF16X6: BF16_PEAK_TFLOPS / 6}[prec]
+            g.PREC_BF16X6: BF16_PEAK_TFLOPS / 6, g.PREC_FP16X3: BF16_PEAK_TFLOPS / 3}[prec]
 
 
 def host_cpus():
@@ -381,9 +385,9 @@ def main():
     ap.add_argument("--models", type=int, default=1024, help="c4: models in the batch")
     ap.add_argument("--path", default="adaptive", choices=["adaptive", "dense"],
                     help="adaptive: learned truncated grid, mixed precision (the headline); dense: plain grid")
-    ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16x3", "bf16x6"],
-                    help="arithmetic of the measured path (default bf16x6 adaptive: FP32 recursion, split-BF16 "
-                         "contraction; fp64 dense)")
+    ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16x3", "bf16x6", "fp16x3"],
+                    help="arithmetic of the measured path (default adaptive: FP32 recursion with the split-FP16 "
+                         "contraction, fp16x3, on plans and the split-BF16x6 one on batches; dense: fp64)")
     ap.add_argument("--derivatives", action="store_true", help="mt: the d/dz and d/dr columns as well")
     ap.add_argument("--cpu-rows", type=int, default=512, help="CPU baseline sample: frequency rows on all cores")
     ap.add_argument("--cpu-rows-1core", type=int, default=48, help="CPU baseline sample: frequency rows on one core")
@@ -411,7 +415,7 @@ def main():
         from grtm_cuda_amd.ffi import _lib, check
         check(_lib().grtm_selftest_copy(64 << 20), "grtm_selftest_copy")
     cfg = models.config(args.workload)
-    pname = args.precision or ("bf16x6" if args.path == "adaptive" else "fp64")
+    pname = args.precision or ("fp16x3" if args.path == "adaptive" else "fp64")
     prec = prec_code(g, pname)
     mk = lambda p: g.TracePlan(cfg["layers"], cfg["src_depth"], cfg["rcv_depth"], cfg["dists"], cfg["grid"],
                                precision=p)

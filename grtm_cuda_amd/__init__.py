@@ -18,10 +18,10 @@ import ctypes
 import numpy as np
 
 from . import inputfile, models  # noqa: F401
-from .ffi import (GRTM_OK, PREC_BF16X3, PREC_BF16X6, PREC_FP32, PREC_FP64, EXPORTS, GrtmError, Grid, Job, Model, D, P,  # noqa: F401
+from .ffi import (GRTM_OK, PREC_BF16X3, PREC_BF16X6, PREC_FP16X3, PREC_FP32, PREC_FP64, EXPORTS, GrtmError, Grid, Job, Model, D, P,  # noqa: F401
                    _lib, check)
 
-__all__ = ["green", "green_mem", "TracePlan", "BatchPlan", "GrtmError", "PREC_FP64", "PREC_FP32", "PREC_BF16X3", "PREC_BF16X6", "models",
+__all__ = ["green", "green_mem", "TracePlan", "BatchPlan", "GrtmError", "PREC_FP64", "PREC_FP32", "PREC_BF16X3", "PREC_BF16X6", "PREC_FP16X3", "models",
            "inputfile"]
 
 

@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 #include "kernels.h"
 
@@ -152,6 +153,9 @@ struct GemmSetT {
     // row r at foff[r / rpw] + (r % rpw) fext[r / rpw]
     const long long *foff;
     const int *fext;
+    // FP16 split contraction: per frequency row (operand row r / rpw) the
+    // row's largest |A| as float bits, which sets the row's scale
+    const unsigned *rpk;
 };
 using GemmSet = GemmSetT<float>;
 
@@ -644,19 +648,42 @@ __host__ __device__ __forceinline__ size_t split_off(int N, int n, int k)
 {
     return size_t(k >> 5) * (size_t(N) << 5) + (size_t(n) << 5) + size_t(k & 31);
 }
-template <int NPL>
-__device__ __forceinline__ void split_terms(float x, __bf16 (&t)[NPL])
+template <int NPL, typename E>
+__device__ __forceinline__ void split_terms(float x, E (&t)[NPL])
 {
     float r = x;
 #pragma unroll
     for (int p = 0; p < NPL; p++) {
-        t[p] = __bf16(r);
+        t[p] = E(r);
         r = r - float(t[p]);
     }
 }
 
-template <int BM, int BN, int WM, int WN, int NPL, int NP, int BK = 32>
+// FP16 split (E = _Float16, two terms, three products): fp16 has 11 mantissa
+// bits against bf16's 8, so two terms carry 22 bits and the dropped product
+// x2 y2 is about 2^-22 of x y (two bf16 terms: 2^-16, which misses the 1e-4
+// bar on the 256-receiver workload; three: 2^-24 at six products).  Its
+// exponent range is what costs: each frequency row of A is scaled by a power
+// of two that takes the row's peak |A| to [2^9, 2^10) (values below 2^-14 of
+// the scaled peak go subnormal, an absolute error under 2^-34 of the peak),
+// and B by 2^13.  Powers of two are exact, so the products come
+// back unscaled exactly as they are written.  B must stay below 8 in
+// magnitude (the Bessel tables hold |J_n| <= 1).
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+constexpr float kF16BScale = 8192.f;
+__device__ __forceinline__ float f16_row_scale(const unsigned *__restrict__ rpk, int f)
+{
+    const float pk = __uint_as_float(rpk[f]);
+    if (!(pk > 0.f) || !(pk < 3.0e38f)) return 1.f;
+    int e;
+    (void)frexpf(pk, &e);  // pk in [2^(e-1), 2^e)
+    return ldexpf(1.f, max(-100, min(100, 10 - e)));
+}
+
+template <int BM, int BN, int WM, int WN, int NPL, int NP, int BK = 32, typename E = __bf16>
 struct BSTile {
+    static constexpr bool F16 = std::is_same<E, _Float16>::value;
+    using V8 = typename std::conditional<F16, v8h, v8bf>::type;
     static constexpr int NT = (BM / WM) * (BN / WN) * 64, KP = BK + 8;
     static constexpr int AR = BK / 4, BR = BK / 8;  // 16-byte A (FP32) and B (bf16) chunks per slab row
     static constexpr int TM = WM / 32, TN = WN / 32, AE = BM * BK / 4 / NT;
@@ -665,27 +692,30 @@ struct BSTile {
     static_assert((NPL == 2 && NP == 3) || (NPL == 3 && NP == 6), "split");
     static_assert(BK % 16 == 0, "one MFMA k step is 16");
     struct Smem {
-        __bf16 As[2][NPL][BM][KP];
-        __bf16 Bs[2][NPL][BN][KP];
+        E As[2][NPL][BM][KP];
+        E Bs[2][NPL][BN][KP];
     };
 
     // acc = A[m0 : m0 + BM, kb : ke] B[kb : ke, n0 : n0 + BN] as F32Tile::mma;
     // S: the split B planes (split_off layout), plane stride N ldk.
-    static __device__ __forceinline__ void mma(const float *__restrict__ A, const __bf16 *__restrict__ S, long lda,
+    static __device__ __forceinline__ void mma(const float *__restrict__ A, const E *__restrict__ S, long lda,
                                                long ldk, int M, int N, int m0, int n0, int kb, int ke, int rpw,
                                                const int *__restrict__ nk_w, Smem &sm, v16f (&acc)[TM][TN],
-                                               const long long *__restrict__ foff, const int *__restrict__ fext)
+                                               const long long *__restrict__ foff, const int *__restrict__ fext,
+                                               const unsigned *__restrict__ rpk)
     {
         const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
         const int wm = wv / (BN / WN), wn = wv % (BN / WN);
         int a_lim[AE];
         size_t a_row[AE];
+        float a_sc[AE];  // FP16 split: the rows' scales
 #pragma unroll
         for (int e = 0; e < AE; e++) {
             const int gm = m0 + (tid + NT * e) / AR;
             a_lim[e] = gm < M ? (nk_w ? min(ke, nk_w[gm / rpw]) : ke) : 0;
             a_row[e] = foff ? (gm < M ? size_t(foff[gm / rpw]) + size_t(gm % rpw) * size_t(fext[gm / rpw]) : 0)
                             : size_t(gm) * size_t(lda);
+            if constexpr (F16) a_sc[e] = gm < M ? f16_row_scale(rpk, gm / rpw) : 1.f;
         }
         const size_t pstride = size_t(N) * size_t(ldk);
         // one K slab of the block's A (FP32) and split B (bf16 planes) in registers
@@ -721,16 +751,19 @@ struct BSTile {
             for (int e = 0; e < AE; e++) {
                 const int idx = tid + NT * e, r = idx / AR, k = (idx % AR) * 4;
                 const float4 a = cut_tail(R.ra[e], R.rlim[e]);
-                const float x[4] = {a.x, a.y, a.z, a.w};
-                __bf16 t[4][NPL];
+                float x[4] = {a.x, a.y, a.z, a.w};
+                if constexpr (F16)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) x[j] *= a_sc[e];
+                E t[4][NPL];
 #pragma unroll
                 for (int j = 0; j < 4; j++) split_terms<NPL>(x[j], t[j]);
 #pragma unroll
                 for (int p = 0; p < NPL; p++) {
-                    typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
-                    v4bf w;
+                    typedef E v4e __attribute__((ext_vector_type(4)));
+                    v4e w;
                     w[0] = t[0][p], w[1] = t[1][p], w[2] = t[2][p], w[3] = t[3][p];
-                    *reinterpret_cast<v4bf *>(&sm.As[buf][p][r][k]) = w;
+                    *reinterpret_cast<v4e *>(&sm.As[buf][p][r][k]) = w;
                 }
             }
 #pragma unroll
@@ -757,15 +790,15 @@ struct BSTile {
         auto step = [&](int buf, Regs &R, int knext) {
 #pragma unroll
             for (int kk = 0; kk < BK; kk += 16) {
-                v8bf fa[NPL][TM], fb[NPL][TN];
+                V8 fa[NPL][TM], fb[NPL][TN];
 #pragma unroll
                 for (int p = 0; p < NPL; p++) {
 #pragma unroll
                     for (int t = 0; t < TM; t++)
-                        fa[p][t] = *reinterpret_cast<const v8bf *>(&sm.As[buf][p][wm * WM + t * 32 + i][kk + kq]);
+                        fa[p][t] = *reinterpret_cast<const V8 *>(&sm.As[buf][p][wm * WM + t * 32 + i][kk + kq]);
 #pragma unroll
                     for (int t = 0; t < TN; t++)
-                        fb[p][t] = *reinterpret_cast<const v8bf *>(&sm.Bs[buf][p][wn * WN + t * 32 + i][kk + kq]);
+                        fb[p][t] = *reinterpret_cast<const V8 *>(&sm.Bs[buf][p][wn * WN + t * 32 + i][kk + kq]);
                 }
                 if (kk == 0) {
                     store(buf ^ 1, R);
@@ -779,9 +812,14 @@ struct BSTile {
 #pragma unroll
                     for (int tm = 0; tm < TM; tm++)
 #pragma unroll
-                        for (int tn = 0; tn < TN; tn++)
-                            acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[PA[q]][tm], fb[PB[q]][tn],
-                                                                                  acc[tm][tn], 0, 0, 0);
+                        for (int tn = 0; tn < TN; tn++) {
+                            if constexpr (F16)
+                                acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_f16(fa[PA[q]][tm], fb[PB[q]][tn],
+                                                                                     acc[tm][tn], 0, 0, 0);
+                            else
+                                acc[tm][tn] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[PA[q]][tm], fb[PB[q]][tn],
+                                                                                      acc[tm][tn], 0, 0, 0);
+                        }
             }
         };
 #pragma unroll
@@ -809,9 +847,11 @@ struct BSTile {
         }
     }
 
-    // dst[r][c] = acc for tile-local rows r < nrow, columns c < ncol, row stride ld
+    // dst[r][c] = acc for tile-local rows r < nrow, columns c < ncol, row stride
+    // ld; the FP16 split divides out the scales of row m0 + r (rows per
+    // frequency rpw) and of B
     static __device__ __forceinline__ void put(const v16f (&acc)[TM][TN], float *__restrict__ dst, long ld, int nrow,
-                                               int ncol)
+                                               int ncol, const unsigned *__restrict__ rpk, int m0, int rpw)
     {
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
         const int wm = wv / (BN / WN), wn = wv % (BN / WN);
@@ -823,7 +863,11 @@ struct BSTile {
                 for (int r = 0; r < 16; r++) {
                     const int row = wm * WM + tm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                     const int col = wn * WN + tn * 32 + (lane & 31);
-                    if (row < nrow && col < ncol) dst[(size_t)row * ld + col] = acc[tm][tn][r];
+                    if (row < nrow && col < ncol) {
+                        float v = acc[tm][tn][r];
+                        if constexpr (F16) v *= 1.f / (f16_row_scale(rpk, (m0 + row) / rpw) * kF16BScale);
+                        dst[(size_t)row * ld + col] = v;
+                    }
                 }
     }
 };
@@ -831,17 +875,18 @@ struct BSTile {
 // Stream-K over the tiles' K slabs as k_gemm_f32sk, with the split-BF16 tile;
 // S[p]: the split B planes of contraction p.
 struct SplitB {
-    const __bf16 *S[kGemmSetMax];
+    const void *S[kGemmSetMax];
 };
-// WPE: waves per SIMD the registers must leave room for (the launch bound)
-template <int BM, int BN, int WM, int WN, int NPL, int NP, int BK, int WPE>
+// WPE: waves per SIMD the registers must leave room for (the launch bound);
+// E: __bf16, or _Float16 for the scaled FP16 split
+template <int BM, int BN, int WM, int WN, int NPL, int NP, int BK, int WPE, typename E>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, WPE) void k_gemm_bssk(int n, int N, int K, GemmSet gp, SplitB sb,
                                                                        StreamK sk, long lda, long ldk,
                                                                        const int *__restrict__ nk_w,
                                                                        const int *__restrict__ off,
                                                                        float *__restrict__ part)
 {
-    using Tl = BSTile<BM, BN, WM, WN, NPL, NP, BK>;
+    using Tl = BSTile<BM, BN, WM, WN, NPL, NP, BK, E>;
     __shared__ __attribute__((aligned(16))) typename Tl::Smem sm;
     const int T = sk.cum[n];
     const long G = off[T];
@@ -862,11 +907,11 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, WPE) void k_gemm_bssk(i
         const int m0 = (t - sk.cum[p]) / sk.ncol * BM, n0 = (t - sk.cum[p]) % sk.ncol * BN;
         const int kt = tile_extent(gp.kend[p], M, K, m0, BM);
         v16f acc[Tl::TM][Tl::TN];
-        Tl::mma(gp.A[p], sb.S[p], lda, ldk, M, N, m0, n0, int(lo - t0) * 32, min(kt, int(seg - t0) * 32), gp.rpw[p],
-                nk_w, sm, acc, gp.foff, gp.fext);
+        Tl::mma(gp.A[p], reinterpret_cast<const E *>(sb.S[p]), lda, ldk, M, N, m0, n0, int(lo - t0) * 32,
+                min(kt, int(seg - t0) * 32), gp.rpw[p], nk_w, sm, acc, gp.foff, gp.fext, gp.rpk);
         const bool whole = lo == t0 && seg == t1;
         float *dst = whole ? gp.C[p] + (size_t)m0 * N + n0 : part + ((size_t)b * 2 + (lo == lo_b ? 0 : 1)) * BM * BN;
-        Tl::put(acc, dst, whole ? N : BN, whole ? M - m0 : BM, whole ? N - n0 : BN);
+        Tl::put(acc, dst, whole ? N : BN, whole ? M - m0 : BM, whole ? N - n0 : BN, gp.rpk, m0, gp.rpw[p]);
         lo = seg;
         t++;
     }
@@ -874,16 +919,17 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, WPE) void k_gemm_bssk(i
 
 // B [K][ldb] (FP32, N columns) into NPL bf16 planes of N ldk values (split_off
 // layout), zero for K <= k < ldk.
-template <int NPL>
-__global__ void k_split_table(int K, int N, const float *__restrict__ B, long ldb, __bf16 *__restrict__ S, long ldk)
+template <int NPL, typename E>
+__global__ void k_split_table(int K, int N, const float *__restrict__ B, long ldb, E *__restrict__ S, long ldk)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)N * ldk) return;
     // t = split_off(N, n, k)
     const size_t slab = t / (size_t(N) << 5), r = t - slab * (size_t(N) << 5);
     const int n = int(r >> 5), k = int(slab * 32 + (r & 31));
-    __bf16 v[NPL];
-    split_terms<NPL>(k < K ? B[(size_t)k * ldb + n] : 0.f, v);
+    E v[NPL];
+    const float x = k < K ? B[(size_t)k * ldb + n] : 0.f;
+    split_terms<NPL>(std::is_same<E, _Float16>::value ? x * kF16BScale : x, v);
 #pragma unroll
     for (int p = 0; p < NPL; p++) S[(size_t)p * N * ldk + t] = v[p];
 }
@@ -1125,20 +1171,26 @@ size_t gemm_bsplit_workspace_bytes(int M, int N, int K)
 
 hipError_t launch_split_table(int nprod, int K, int N, const float *B, long ldb, void *S, long ldk, hipStream_t st)
 {
-    if ((nprod != 3 && nprod != 6) || ldk < K || ldk % 32) return hipErrorInvalidValue;
+    if ((nprod != kSplitBf16x3 && nprod != kSplitBf16x6 && nprod != kSplitF16x3) || ldk < K || ldk % 32)
+        return hipErrorInvalidValue;
     const size_t n = (size_t)N * ldk;
     const dim3 grid((unsigned)((n + 255) / 256));
-    if (nprod == 3) hipLaunchKernelGGL(k_split_table<2>, grid, dim3(256), 0, st, K, N, B, ldb, (__bf16 *)S, ldk);
-    else hipLaunchKernelGGL(k_split_table<3>, grid, dim3(256), 0, st, K, N, B, ldb, (__bf16 *)S, ldk);
+    if (nprod == kSplitBf16x3)
+        hipLaunchKernelGGL((k_split_table<2, __bf16>), grid, dim3(256), 0, st, K, N, B, ldb, (__bf16 *)S, ldk);
+    else if (nprod == kSplitBf16x6)
+        hipLaunchKernelGGL((k_split_table<3, __bf16>), grid, dim3(256), 0, st, K, N, B, ldb, (__bf16 *)S, ldk);
+    else
+        hipLaunchKernelGGL((k_split_table<2, _Float16>), grid, dim3(256), 0, st, K, N, B, ldb, (_Float16 *)S, ldk);
     return hipGetLastError();
 }
 
 hipError_t launch_gemm_pair_bsplit(int nprod, int M, int N, int K, const float *A0, const float *A1, long lda,
                                    const void *S0, const void *S1, long ldk, float *C0, float *C1, float *P,
                                    const int *kend, const int *nk_w, hipStream_t st, const long long *foff,
-                                   const int *fext)
+                                   const int *fext, const unsigned *rpk)
 {
-    if ((nprod != 3 && nprod != 6) || ldk < K || ldk % 32 || lda % 4 || (foff && (!fext || !nk_w || !kend)))
+    if ((nprod != kSplitBf16x3 && nprod != kSplitBf16x6 && nprod != kSplitF16x3) || ldk < K || ldk % 32 || lda % 4 ||
+        (foff && (!fext || !nk_w || !kend)) || (nprod == kSplitF16x3 && !rpk))
         return hipErrorInvalidValue;
     GemmSet gs{};
     gs.A[0] = A0, gs.A[1] = A1, gs.C[0] = C0, gs.C[1] = C1;
@@ -1147,9 +1199,10 @@ hipError_t launch_gemm_pair_bsplit(int nprod, int M, int N, int K, const float *
     gs.rpw[0] = gs.rpw[1] = 4;
     gs.foff = foff;
     gs.fext = fext;
+    gs.rpk = rpk;
     SplitB sb{};
-    sb.S[0] = (const __bf16 *)S0;
-    sb.S[1] = (const __bf16 *)S1;
+    sb.S[0] = S0;
+    sb.S[1] = S1;
     const int bn = bs_bn(N);
     StreamK sk{};
     int t = 0;
@@ -1163,22 +1216,25 @@ hipError_t launch_gemm_pair_bsplit(int nprod, int M, int N, int K, const float *
     float *part = P;
     int *off = reinterpret_cast<int *>(P + (size_t)sk.nb * 2 * kTBM * bn);
     hipLaunchKernelGGL(k_streamk_plan<kTBM>, dim3(1), dim3(1024), 0, st, 2, K, gs, sk, off);
-#define GRTM_BS(BNV, WMV, WNV, NPL, NP, BKV, WPEV)                                                             \
-    hipLaunchKernelGGL((k_gemm_bssk<kTBM, BNV, WMV, WNV, NPL, NP, BKV, WPEV>), dim3(sk.nb), dim3(512), 0, st, 2, N, K, \
-                       gs, sb, sk, lda, ldk, nk_w, off, part)
+#define GRTM_BS(BNV, WMV, WNV, NPL, NP, BKV, WPEV, E)                                                          \
+    hipLaunchKernelGGL((k_gemm_bssk<kTBM, BNV, WMV, WNV, NPL, NP, BKV, WPEV, E>), dim3(sk.nb), dim3(512), 0, st, 2, N, \
+                       K, gs, sb, sk, lda, ldk, nk_w, off, part)
     if (bn == 256) {
-        if (nprod == 6) GRTM_BS(256, 64, 64, 3, 6, 16, 2);
-        else GRTM_BS(256, 64, 64, 2, 3, 16, 2);
+        if (nprod == kSplitBf16x6) GRTM_BS(256, 64, 64, 3, 6, 16, 2, __bf16);
+        else if (nprod == kSplitF16x3) GRTM_BS(256, 64, 64, 2, 3, 16, 2, _Float16);
+        else GRTM_BS(256, 64, 64, 2, 3, 16, 2, __bf16);
         hipLaunchKernelGGL((k_streamk_reduce<kTBM, 256>), dim3((unsigned)t, kReduceSlices), dim3(256), 0, st, 2, N, gs,
                            sk, off, part);
     } else if (bn == 128) {
-        if (nprod == 6) GRTM_BS(128, 32, 64, 3, 6, 32, 2);
-        else GRTM_BS(128, 32, 64, 2, 3, 32, 2);
+        if (nprod == kSplitBf16x6) GRTM_BS(128, 32, 64, 3, 6, 32, 2, __bf16);
+        else if (nprod == kSplitF16x3) GRTM_BS(128, 32, 64, 2, 3, 32, 2, _Float16);
+        else GRTM_BS(128, 32, 64, 2, 3, 32, 2, __bf16);
         hipLaunchKernelGGL((k_streamk_reduce<kTBM, 128>), dim3((unsigned)t, kReduceSlices), dim3(256), 0, st, 2, N, gs,
                            sk, off, part);
     } else {
-        if (nprod == 6) GRTM_BS(64, 32, 32, 3, 6, 32, 2);
-        else GRTM_BS(64, 32, 32, 2, 3, 32, 2);
+        if (nprod == kSplitBf16x6) GRTM_BS(64, 32, 32, 3, 6, 32, 2, __bf16);
+        else if (nprod == kSplitF16x3) GRTM_BS(64, 32, 32, 2, 3, 32, 2, _Float16);
+        else GRTM_BS(64, 32, 32, 2, 3, 32, 2, __bf16);
         hipLaunchKernelGGL((k_streamk_reduce<kTBM, 64>), dim3((unsigned)t, kReduceSlices), dim3(256), 0, st, 2, N, gs,
                            sk, off, part);
     }

@@ -830,13 +830,13 @@ size_t pair_sk_bytes(int bsplit, int M, int N, int K)
     return b;
 }
 
-// Split-BF16 contraction (GRTM_PREC_BF16X3 / GRTM_PREC_BF16X6 with a pair
-// layout): the two Bessel tables' bf16 planes, planes x nr x ldk values each
+// Split-precision contraction (GRTM_PREC_BF16X3 / BF16X6 / FP16X3 with a
+// pair layout): the two Bessel tables' 16-bit planes, planes x nr x ldk values each
 // (slab-major by 32 wavenumbers), ldk the wavenumber count rounded up to 32.
 int split_alloc(int bsplit, int nk, int nr, DBuf &S0, DBuf &S1, long &ldk)
 {
     ldk = (nk + 31) / 32 * 32;
-    const size_t bytes = (size_t)(bsplit == 6 ? 3 : 2) * nr * ldk * 2;
+    const size_t bytes = (size_t)grtm::split_planes(bsplit) * nr * ldk * 2;
     int st;
     if ((st = S0.alloc(bytes)) || (st = S1.alloc(bytes))) return st;
     return GRTM_OK;
@@ -921,10 +921,13 @@ int run_cells_core(grtm_plan *p, hipStream_t s)
     if (p->kind == 1) return run_cells_mt(p, s);
     const bool ad = p->adaptive;
     const bool learned = ad && p->net;
+    const bool f16 = p->bsplit == grtm::kSplitF16x3;
     int *nkw = ad ? p->nk_w.as<int>() : nullptr;
-    unsigned *rowpk = ad ? p->counters.as<unsigned>() : nullptr;
+    // row peaks: the adaptive grid's check, and the FP16 split's row scales
+    unsigned *rowpk = (ad || f16) ? p->counters.as<unsigned>() : nullptr;
     unsigned *tailmx = ad ? rowpk + p->nw : nullptr;
     unsigned *fbcnt = ad ? rowpk + 2 * p->nw : nullptr;
+    if (f16 && !ad) HIPCHK(grtm::launch_zero_u32(rowpk, p->nw, s));
     const int r64 = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;
     if (learned) {
         // the learned grid of this execution: network -> per-row cut and
@@ -999,7 +1002,7 @@ int run_cells_core(grtm_plan *p, hipStream_t s)
     // FP64 re-solve of the non-finite FP32 cells (main grid and fallback rows)
     if (mixed)
         HIPCHK(grtm::launch_cells_fix(p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->A0.p,
-                                      p->A1.p, p->nk, cnt, list, s));
+                                      p->A1.p, p->nk, cnt, list, s, f16 ? rowpk : nullptr));
     return GRTM_OK;
 }
 int run_gemm_mt(grtm_plan *p, hipStream_t s)
@@ -1060,7 +1063,8 @@ int run_gemm_core(grtm_plan *p, hipStream_t s)
     if (p->bsplit) {
         HIPCHK(grtm::launch_gemm_pair_bsplit(p->bsplit, M, p->nr, p->nk, p->A0.as<float>(), p->A1.as<float>(), p->nk,
                                              p->S0.p, p->S1.p, p->ldk, p->C0.as<float>(), p->C1.as<float>(),
-                                             p->ws.as<float>(), kend, nkw, s, nullptr, nullptr));
+                                             p->ws.as<float>(), kend, nkw, s, nullptr, nullptr,
+                                             p->bsplit == grtm::kSplitF16x3 ? p->counters.as<unsigned>() : nullptr));
         return GRTM_OK;
     }
     if (p->pair && p->gemm_mode == 1) {
@@ -1184,8 +1188,9 @@ static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid
     if (!ctx || !job || !grid || !plan_out) return GRTM_E_INPUT;
     *plan_out = nullptr;
     if (precision != GRTM_PREC_FP64 && precision != GRTM_PREC_FP32 && precision != GRTM_PREC_BF16X3 &&
-        precision != GRTM_PREC_BF16X6)
+        precision != GRTM_PREC_BF16X6 && precision != GRTM_PREC_FP16X3)
         return GRTM_E_INPUT;
+    if (precision == GRTM_PREC_FP16X3 && kind != 0) return GRTM_E_UNSUPPORTED;  // Z / R plans only
     if (job->n_layers < 2 || job->n_layers + 1 > grtm::kMaxLayers || !job->layers || job->nr < 1 || !job->dists)
         return GRTM_E_INPUT;
     if (grid->nw < 4 || (grid->nw & (grid->nw - 1)) || grid->nw > 8192 || grid->nk < 1 || !(grid->kmax > 0) ||
@@ -1296,7 +1301,14 @@ static int plan_create(grtm_ctx *ctx, const grtm_job *job, const grtm_grid *grid
         const bool pair_ok = grtm::gemm_pair_ok(p->nk, p->nr, p->A0.p, p->A1.p, p->B0.p, p->B1.p);
         if (pair_ok && (precision == GRTM_PREC_BF16X3 || precision == GRTM_PREC_BF16X6)) {
             p->gemm_mode = 1;
-            p->bsplit = precision == GRTM_PREC_BF16X3 ? 3 : 6;
+            p->bsplit = precision == GRTM_PREC_BF16X3 ? grtm::kSplitBf16x3 : grtm::kSplitBf16x6;
+        }
+        if (precision == GRTM_PREC_FP16X3) {
+            // the FP16 split runs on the pair layout only; its row scales come
+            // from the recursion's per-row peaks, kept on dense grids too
+            if (!pair_ok) return GRTM_E_UNSUPPORTED;
+            p->bsplit = grtm::kSplitF16x3;
+            if ((status = p->counters.alloc(sizeof(unsigned) * (2 * p->nw + 1)))) return status;
         }
         size_t wsb = grtm::gemm_workspace_bytes(p->gemm_mode, (int)arows, p->nr, p->nk);
         if (p->gemm_mode != 2 && pair_ok) {
@@ -1905,8 +1917,9 @@ void grtm_plan_destroy(grtm_plan *p) { delete p; }
 // ------------------------------------------------------------ self-tests
 int grtm_selftest_gemm_cut(int prec, int M, int N, int K, const double *A, const double *B, const int *nk_w, double *C)
 {
-    if ((prec != 0 && prec != 1 && prec != 3 && prec != 6) || M < 4 || M % 4 || N < 1 || K < 1 || !A || !B || !nk_w ||
-        !C)
+    if ((prec != 0 && prec != 1 && prec != grtm::kSplitBf16x3 && prec != grtm::kSplitBf16x6 &&
+         prec != grtm::kSplitF16x3) ||
+        M < 4 || M % 4 || N < 1 || K < 1 || !A || !B || !nk_w || !C)
         return GRTM_E_INPUT;
     const int nw = M / 4;
     double ksum = 0;
@@ -1927,14 +1940,27 @@ int grtm_selftest_gemm_cut(int prec, int M, int N, int K, const double *A, const
     const int kc = grtm::gemm_pair_slice(prec ? 1 : 0, M, N, K, std::max(1.0, ksum / nw));
     size_t wsb = grtm::gemm_pair_workspace_bytes(M, N, K, kc) * es / 4;
     if (prec) wsb = std::max(wsb, pair_sk_bytes(bsplit, M, N, K));
-    DBuf dA, dB, dC, dP, dn, dk, dS;
+    DBuf dA, dB, dC, dP, dn, dk, dS, dpk;
     long ldk = 0;
     if ((st = dA.alloc(a.size() * es)) || (st = dB.alloc(b.size() * es)) || (st = dC.alloc((size_t)2 * M * N * es)) ||
         (st = dP.alloc(wsb)) || (st = dn.alloc(sizeof(int) * nw)) || (st = dk.alloc(sizeof(int) * ((nw + 31) / 32))))
         return st;
     if (bsplit) {
         ldk = (K + 31) / 32 * 32;
-        if ((st = dS.alloc((size_t)(bsplit == 6 ? 3 : 2) * N * ldk * 2))) return st;
+        if ((st = dS.alloc((size_t)grtm::split_planes(bsplit) * N * ldk * 2))) return st;
+    }
+    if (bsplit == grtm::kSplitF16x3) {
+        // the FP16 split's row scales: the peak |A| of each frequency row's
+        // four operand rows below its cut, as float bits (as the recursion keeps them)
+        std::vector<unsigned> pk(nw, 0u);
+        for (int i = 0; i < nw; i++) {
+            float m = 0.f;
+            for (int q = 0; q < 4; q++)
+                for (int k = 0; k < nk_w[i]; k++) m = std::max(m, std::fabs(float(A[(size_t)(4 * i + q) * K + k])));
+            std::memcpy(&pk[i], &m, sizeof(float));
+        }
+        if ((st = dpk.alloc(sizeof(unsigned) * nw))) return st;
+        HIPCHK(hipMemcpy(dpk.p, pk.data(), sizeof(unsigned) * nw, hipMemcpyHostToDevice));
     }
     if (prec == 0) {
         HIPCHK(hipMemcpy(dA.p, a.data(), a.size() * es, hipMemcpyHostToDevice));
@@ -1958,7 +1984,7 @@ int grtm_selftest_gemm_cut(int prec, int M, int N, int K, const double *A, const
         HIPCHK(grtm::launch_split_table(bsplit, K, N, dB.as<float>(), ldb, dS.p, ldk, nullptr));
         HIPCHK(grtm::launch_gemm_pair_bsplit(bsplit, M, N, K, dA.as<float>(), dA.as<float>(), lda, dS.p, dS.p, ldk, c0,
                                              c1, dP.as<float>(), dk.as<int>(), dn.as<int>(), nullptr, nullptr,
-                                             nullptr));
+                                             nullptr, dpk.as<unsigned>()));
     } else {
         float *c0 = dC.as<float>(), *c1 = c0 + (size_t)M * N;
         HIPCHK(grtm::launch_gemm_pair_f32(M, N, K, kc, dA.as<float>(), dA.as<float>(), lda, dB.as<float>(),
@@ -2782,6 +2808,7 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
 {
     if (!ctx || !models || n_models < 1 || !dists || nr < 1 || !grid || !out) return GRTM_E_INPUT;
     *out = nullptr;
+    if (precision == GRTM_PREC_FP16X3) return GRTM_E_UNSUPPORTED;  // plans only
     if (precision != GRTM_PREC_FP64 && precision != GRTM_PREC_FP32 && precision != GRTM_PREC_BF16X3 &&
         precision != GRTM_PREC_BF16X6)
         return GRTM_E_INPUT;

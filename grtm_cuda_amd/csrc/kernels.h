@@ -125,7 +125,7 @@ hipError_t launch_grid_items(int nw, int r64, const int *nk_pred, const int *nch
 // from the final counts, as launch_kend_tiles.
 hipError_t launch_fallback(int out_f64, int compat, const void *lw64, const void *lw32, int r64, int L, const Geo &g,
                            int nk, int nw, double dk, double df, double omega_i, double tol,
-                           const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *fb_count,
+                           unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *fb_count,
                            unsigned *fb_list, void *A0, void *A1, long lda, unsigned *flag_count,
                            unsigned *flag_list, int row_base, hipStream_t st, const MtOps *mto = nullptr,
                            int deriv = 0, int *kend = nullptr, int rows_per_tile = 0);
@@ -141,9 +141,10 @@ hipError_t launch_row_labels(int prec, int nw, int nk, long lda, const void *A0,
 // appended to the flag list.
 hipError_t launch_plant_nonfinite(const unsigned *cells, int n, int nk, void *A0, void *A1, long lda,
                                   unsigned *flag_count, unsigned *flag_list, hipStream_t st);
+// row_peak (optional): per-row peaks raised to cover the re-solved values
 hipError_t launch_cells_fix(int compat, const void *lw64, int L, const Geo &g, int nk, double dk, double df,
                             double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,
-                            const unsigned *flag_list, hipStream_t st);
+                            const unsigned *flag_list, hipStream_t st, unsigned *row_peak = nullptr);
 hipError_t launch_zero_u32(unsigned *p, int n, hipStream_t st);
 // drop-in cell kernel (green.hip) over rows [w_base, w_base + nrows); dd_rows:
 // solve them in double-double arithmetic
@@ -187,18 +188,24 @@ hipError_t launch_gemm_pair_f32(int M, int N, int K, int kc, const float *A0, co
                                 const float *B0, const float *B1, long ldb, float *C0, float *C1, float *P,
                                 const int *kend, const int *nk_w, hipStream_t st, const long long *foff = nullptr,
                                 const int *fext = nullptr);
-// Split-BF16 pair: the contractions of launch_gemm_pair_f32 on bf16 matrix
-// cores, A split into 2 (nprod 3) or 3 (nprod 6) bf16 terms while staged, B
-// given as split planes S0 / S1 (launch_split_table: [planes][N][ldk] bf16,
-// ldk a multiple of 32 >= K); stream-K for any kend / nk_w (null: dense).
-// nprod 6 keeps about 2^-24 per product, nprod 3 about 2^-16.  Workspace:
+// Split-precision pair: the contractions of launch_gemm_pair_f32 on 16-bit
+// matrix cores, A split while it is staged, B given as split planes S0 / S1
+// (launch_split_table: planes of N ldk values, ldk a multiple of 32 >= K);
+// stream-K for any kend / nk_w (null: dense).  nprod (the split code):
+// kSplitBf16x3 two bf16 terms, three products (about 2^-16 per product);
+// kSplitBf16x6 three bf16 terms, six products (about 2^-24); kSplitF16x3 two
+// fp16 terms, three products (about 2^-22), each frequency row of A scaled by
+// a power of two set by its peak |A| (rpk: per frequency row, float bits; the
+// rows' largest |A| over every value they hold) and B (|B| < 8) by 2^13.  Workspace:
 // gemm_bsplit_workspace_bytes(M, N, K).
+constexpr int kSplitBf16x3 = 3, kSplitBf16x6 = 6, kSplitF16x3 = 16;
+inline int split_planes(int nprod) { return nprod == kSplitBf16x6 ? 3 : 2; }
 size_t gemm_bsplit_workspace_bytes(int M, int N, int K);
 hipError_t launch_split_table(int nprod, int K, int N, const float *B, long ldb, void *S, long ldk, hipStream_t st);
 hipError_t launch_gemm_pair_bsplit(int nprod, int M, int N, int K, const float *A0, const float *A1, long lda,
                                    const void *S0, const void *S1, long ldk, float *C0, float *C1, float *P,
                                    const int *kend, const int *nk_w, hipStream_t st, const long long *foff = nullptr,
-                                   const int *fext = nullptr);
+                                   const int *fext = nullptr, const unsigned *rpk = nullptr);
 // The same in FP64 (128 x 128 tiles, kend per 128-row tile; workspace
 // 2 x gemm_pair_workspace_bytes(M, N, K, kc)).
 hipError_t launch_gemm_pair_f64(int M, int N, int K, int kc, const double *A0, const double *A1, long lda,

@@ -244,7 +244,7 @@ __global__ __launch_bounds__(kCellThreads) void k_fallback_cells(
     const LayerW<T> *__restrict__ lw_all, int L, Geo g, int nk, int nw, double dk, double df, double omega_i,
     int r_lo, int r_hi, TO *__restrict__ A0, TO *__restrict__ A1, long lda, const unsigned *__restrict__ fb_count,
     const unsigned *__restrict__ fb_list, unsigned *__restrict__ flag_count, unsigned *__restrict__ flag_list,
-    int row_base, MtOps o)
+    int row_base, MtOps o, unsigned *__restrict__ row_peak)
 {
     __shared__ LayerW<T> lw[kMaxLayers];
     const unsigned n = *fb_count;
@@ -264,6 +264,11 @@ __global__ __launch_bounds__(kCellThreads) void k_fallback_cells(
                 TO v[8];
                 ok = cell_zr<T, TO, COMPAT>(lw, g, iw, ik, dk, df, omega_i, v);
                 store_zr(v, iw, ik, A0, A1, lda);
+                // the row peak covers the completed tail (it sets the FP16 split's row scale)
+                if (row_peak) {
+                    const unsigned m = cell_mag_bits(v);
+                    if (m) atomicMax(row_peak + iw, m);
+                }
             }
             if (!ok && flag_count) {
                 const unsigned slot2 = atomicAdd(flag_count, 1u);
@@ -435,7 +440,8 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_fix(const LayerW<double>
                                                             Geo g, int nk, double dk, double df, double omega_i,
                                                             TO *__restrict__ A0, TO *__restrict__ A1, long lda,
                                                             const unsigned *__restrict__ flag_count,
-                                                            const unsigned *__restrict__ flag_list)
+                                                            const unsigned *__restrict__ flag_list,
+                                                            unsigned *__restrict__ row_peak)
 {
     const unsigned n = *flag_count;
     for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
@@ -444,6 +450,11 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_fix(const LayerW<double>
         TO v[8];
         cell_zr<double, TO, COMPAT>(lw_all + (size_t)iw * L, g, iw, ik, dk, df, omega_i, v);
         store_zr(v, iw, ik, A0, A1, lda);
+        // the row peak covers the re-solved values (it sets the FP16 split's row scale)
+        if (row_peak) {
+            const unsigned m = cell_mag_bits(v);
+            if (m) atomicMax(row_peak + iw, m);
+        }
     }
 }
 
@@ -821,7 +832,7 @@ hipError_t launch_cells_zr(int prec, int compat, const void *lw, int L, const Ge
 
 hipError_t launch_fallback(int out_f64, int compat, const void *lw64, const void *lw32, int r64, int L, const Geo &g,
                            int nk, int nw, double dk, double df, double omega_i, double tol,
-                           const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *fb_count,
+                           unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *fb_count,
                            unsigned *fb_list, void *A0, void *A1, long lda, unsigned *flag_count,
                            unsigned *flag_list, int row_base, hipStream_t st, const MtOps *mto, int deriv,
                            int *kend, int rows_per_tile)
@@ -840,7 +851,7 @@ hipError_t launch_fallback(int out_f64, int compat, const void *lw64, const void
 #define GRTM_FB(T, TO, CP, MT, LWP, LO, HI, FC, FL)                                                             \
     hipLaunchKernelGGL((k_fallback_cells<T, TO, CP, MT>), grid, dim3(kCellThreads), 0, st, (const LayerW<T> *)LWP, \
                        L, g, nk, nw, dk, df, omega_i, LO, HI, (TO *)A0, (TO *)A1, lda, fb_count, fb_list, FC, FL,  \
-                       row_base, o)
+                       row_base, o, MT ? nullptr : row_peak)
 #define GRTM_FB_ALL(MT)                                                                              \
     if (out_f64) {                                                                                   \
         if (compat) GRTM_FB(double, double, true, MT, lw64, 0, nw, nullptr, nullptr);                \
@@ -900,17 +911,17 @@ hipError_t launch_copy_i32(const int *a, int *b, int n, hipStream_t st)
 
 hipError_t launch_cells_fix(int compat, const void *lw64, int L, const Geo &g, int nk, double dk, double df,
                             double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,
-                            const unsigned *flag_list, hipStream_t st)
+                            const unsigned *flag_list, hipStream_t st, unsigned *row_peak)
 {
     dim3 grid(1024);
     if (compat)
         hipLaunchKernelGGL((k_cells_fix<float, true>), grid, dim3(kCellThreads), 0, st,
                            (const LayerW<double> *)lw64, L, g, nk, dk, df, omega_i, (float *)A0, (float *)A1, lda,
-                           flag_count, flag_list);
+                           flag_count, flag_list, row_peak);
     else
         hipLaunchKernelGGL((k_cells_fix<float, false>), grid, dim3(kCellThreads), 0, st,
                            (const LayerW<double> *)lw64, L, g, nk, dk, df, omega_i, (float *)A0, (float *)A1, lda,
-                           flag_count, flag_list);
+                           flag_count, flag_list, row_peak);
     return hipGetLastError();
 }
 

@@ -23,6 +23,7 @@ PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_BF16X3 = 2
 PREC_BF16X6 = 3
+PREC_FP16X3 = 4
 
 P = ctypes.c_void_p
 D = ctypes.c_double

@@ -38,6 +38,13 @@ extern "C" {
                               pair of contractions); other shapes, and column
                               plans, contract as GRTM_PREC_FP32 (BF16X6) or on
                               the streamed BF16x3 kernel (BF16X3) */
+#define GRTM_PREC_FP16X3 4 /* mixed recursion as GRTM_PREC_FP32; contraction on
+                              split-FP16 matrix-core products: each frequency
+                              row of the operands scaled by a power of two set
+                              by its peak, two fp16 terms per value, the three
+                              products above 2^-22.  Trace plans of the pair
+                              shape only (GRTM_E_UNSUPPORTED otherwise, and for
+                              column plans and batches) */
 
 /* ---------------------------------------------------------- drop-in entry */
 /* Replaces the reference's only exported symbol:

@@ -1,7 +1,9 @@
-"""The split-BF16 contraction in plans and batches (GRTM_PREC_BF16X6: three
-bf16 terms per FP32 operand, six products; GRTM_PREC_BF16X3: two terms,
-three products), on the same FP32 recursion as GRTM_PREC_FP32: traces
-against the plain FP64 plans, and against the FP32 plans they must track.
+"""The split-precision contractions in plans and batches (GRTM_PREC_BF16X6:
+three bf16 terms per FP32 operand, six products; GRTM_PREC_BF16X3: two
+terms, three products; GRTM_PREC_FP16X3, plans only: two fp16 terms of each
+row-scaled operand, three products), on the same FP32 recursion as
+GRTM_PREC_FP32: traces against the plain FP64 plans, and against the FP32
+plans they must track.
 (The 3-layer model's traces of both forms against the oracle sit with the
 other mixed-precision cases in test_gpu_traces.)"""
 import numpy as np
@@ -71,3 +73,35 @@ def test_c4_sample_bf16x6_learned_batch():
         worst, worst_pk = max(worst, float(rl.max())), max(worst_pk, float(pk.max()))
     print(f"  vs FP64 plans: rel-L2 {worst:.2e}, peak {worst_pk:.2e}")
     assert worst < 1e-4 and worst_pk < 1e-4
+
+
+@pytest.mark.parametrize("learn", [True, False])
+def test_c2_fp16x3_traces(c2_ref, learn):
+    """The scaled FP16 split (about 2^-22 per product) on the 8-layer /
+    256-receiver model, learned adaptive and dense: within 1e-4 rel-L2 and
+    peak of FP64 and as close to it as the FP32 plan, from which it differs
+    (the split kernel ran)."""
+    print(f"gpu: c2 fp16x3 {'learned' if learn else 'dense'}", flush=True)
+    t16 = c2_plan(g.PREC_FP16X3, learn).traces()
+    t32 = c2_plan(g.PREC_FP32, learn).traces()
+    rl, pk = errs(t16, c2_ref)
+    rl_32, _ = errs(t32, c2_ref)
+    d32, _ = errs(t16, t32)
+    print(f"  vs FP64: rel-L2 {rl.max():.2e} (FP32 plan {rl_32.max():.2e}), peak {pk.max():.2e}; "
+          f"vs FP32 {d32.max():.2e}")
+    assert rl.max() < 1e-4 and pk.max() < 1e-4
+    assert rl.max() < 1.5 * rl_32.max() and np.median(rl) < 1.5 * np.median(rl_32)
+    assert d32.max() > 0
+
+
+def test_fp16x3_plans_only():
+    """The FP16 split runs on Z / R trace plans only: batches and column plans
+    refuse it."""
+    cfg = models.config("c4")
+    ms = cfg["models"][:2]
+    with pytest.raises(g.GrtmError):
+        g.BatchPlan(ms, 0.0, cfg["dists"][:8], dict(cfg["grid"], nw=64, nk=512), precision=g.PREC_FP16X3)
+    c1 = models.config("c1")
+    grid = dict(c1["grid"], nw=64, nk=512)
+    with pytest.raises(g.GrtmError):
+        g.ColumnPlan(c1["layers"], c1["src_depth"], 0.0, c1["dists"][:4], grid, precision=g.PREC_FP16X3)

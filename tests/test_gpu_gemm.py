@@ -29,19 +29,25 @@ def test_gemm_matches_numpy(mode, shape):
 
 
 @pytest.mark.parametrize("shape", [(8192, 256, 2048), (1024, 64, 1000), (400, 72, 333), (4, 8, 5), (2048, 130, 4096)])
-@pytest.mark.parametrize("prec", [1, 0, 6, 3])
+@pytest.mark.parametrize("prec", [1, 0, 6, 3, 16])
 def test_learned_grid_contraction(prec, shape):
     """The learned-grid contraction (FP32: stream-K over the tiles' K slabs;
     FP64: split-K limited to each tile's extent): every operand row read up to
     its frequency row's cut, tiles without K (cut 0 throughout), rows cut at
-    full K, unaligned cuts, column tiles of 64 and 128, shapes with ragged
-    tails.  Against numpy on the masked operands; the split-BF16 forms (prec
-    6: three bf16 terms per operand, six products; 3: two terms, three
-    products) to their truncation, 2^-24 and 2^-16 of each product."""
+    full K, unaligned cuts, column tiles of 64, 128 and 256, shapes with
+    ragged tails.  Against numpy on the masked operands; the split-BF16 forms
+    (prec 6: three bf16 terms per operand, six products; 3: two terms, three
+    products) to their truncation, 2^-24 and 2^-16 of each product, and the
+    scaled FP16 split (prec 16: two fp16 terms, three products, about 2^-22;
+    B in [-1, 1] as the Bessel tables, rows of A over eight decades so the
+    per-row scales differ)."""
     M, N, K = shape
     rng = np.random.default_rng(3 * M + N + K)
     A = rng.standard_normal((M, K))
     B = rng.standard_normal((K, N))
+    if prec == 16:
+        B = rng.uniform(-1.0, 1.0, (K, N))
+        A *= np.repeat(10.0 ** rng.uniform(-4, 4, M // 4), 4)[:, None]
     nw = M // 4
     nk_w = rng.integers(0, K + 1, nw)
     nk_w[: min(nw, 40)] = 0                    # leading tiles without K at all
@@ -58,7 +64,7 @@ def test_learned_grid_contraction(prec, shape):
     bound = np.abs(Am) @ np.abs(B)  # per-element scale of the rounding
     # FP32 / FP64 accumulation over up to 4096 terms; two-term splits drop
     # products near 3 2^-16 of each term
-    tol = {0: 1e-13, 1: 2e-5, 6: 2e-5, 3: 1.5e-4}[prec]
+    tol = {0: 1e-13, 1: 2e-5, 6: 2e-5, 3: 1.5e-4, 16: 2e-5}[prec]
     assert np.all(np.abs(C - ref) <= tol * bound + 1e-300), np.max(np.abs(C - ref) / (bound + 1e-300))
     assert np.all(C[np.repeat(nk_w, 4) == 0] == 0.0)
 
