@@ -11,8 +11,7 @@ The default measures the learned adaptive path (MLP-predicted per-frequency
 wavenumber cut, residual check and dense fallback, FP32 recursion with FP64
 vertical wavenumbers and low-frequency rows, split-FP16 MFMA contraction: two
 fp16 terms of each row-scaled FP32 operand, three products, FP32
-accumulation; the 1024-model batch line uses the split-BF16x6 contraction,
-three bf16 terms and six products); the
+accumulation); the
 plain FP64 dense path runs in the same process as the denominator and as the
 accuracy reference (max trace rel-L2 reported).
 
@@ -128,7 +127,7 @@ def bench_batch(args, g, models):
     """Batched workload: many layered models in one captured graph, mixed
     precision against the plain FP64 batch."""
     cfg = models.config("c4", n_models=args.models)
-    pname = args.precision or ("bf16x6" if args.path == "adaptive" else "fp32")
+    pname = args.precision or ("fp16x3" if args.path == "adaptive" else "fp32")
     prec = prec_code(g, pname)
     mk = lambda p: g.BatchPlan(cfg["models"], cfg["rcv_depth"], cfg["dists"], cfg["grid"], precision=p)
     nm = len(cfg["models"])
@@ -387,7 +386,7 @@ def main():
                     help="adaptive: learned truncated grid, mixed precision (the headline); dense: plain grid")
     ap.add_argument("--precision", default=None, choices=["fp64", "fp32", "bf16x3", "bf16x6", "fp16x3"],
                     help="arithmetic of the measured path (default adaptive: FP32 recursion with the split-FP16 "
-                         "contraction, fp16x3, on plans and the split-BF16x6 one on batches; dense: fp64)")
+                         "contraction, fp16x3; dense: fp64)")
     ap.add_argument("--derivatives", action="store_true", help="mt: the d/dz and d/dr columns as well")
     ap.add_argument("--cpu-rows", type=int, default=512, help="CPU baseline sample: frequency rows on all cores")
     ap.add_argument("--cpu-rows-1core", type=int, default=48, help="CPU baseline sample: frequency rows on one core")

@@ -237,7 +237,7 @@ __global__ __launch_bounds__(kCellThreads) void k_fallback_batch(const ModelDesc
                                                                  TO *__restrict__ A0, TO *__restrict__ A1,
                                                                  RowLayout rl, const unsigned *__restrict__ count,
                                                                  const unsigned *__restrict__ list, int nrow,
-                                                                 FlagList fl)
+                                                                 FlagList fl, unsigned *__restrict__ row_peak)
 {
     __shared__ LayerW<T> lw[kMaxLayers];
     const unsigned n = *count;
@@ -255,6 +255,11 @@ __global__ __launch_bounds__(kCellThreads) void k_fallback_batch(const ModelDesc
             const bool ok = cell_zr<T, TO, COMPAT>(lw, d.g, iw, ik, dk, df, omega_i, v);
             store_zr_rl(v, grow, ik, A0, A1, rl);
             if (!ok && fl.count) flag_cell(fl, grow, nk, ik);
+            // the row peak covers the completed tail (it sets the FP16 split's row scale)
+            if (row_peak) {
+                const unsigned m = cell_mag_bits(v);
+                if (m) atomicMax(row_peak + grow, m);
+            }
         }
     }
 }
@@ -265,7 +270,8 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_fix_batch(const ModelDes
                                                                   const LayerW<double> *__restrict__ lw_all,
                                                                   int nw, int nk, double dk, double df,
                                                                   double omega_i, TO *__restrict__ A0,
-                                                                  TO *__restrict__ A1, RowLayout rl, FlagList fl)
+                                                                  TO *__restrict__ A1, RowLayout rl, FlagList fl,
+                                                                  unsigned *__restrict__ row_peak)
 {
     const unsigned n = min(*fl.count, fl.cap);
     for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
@@ -276,6 +282,10 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_fix_batch(const ModelDes
         TO v[8];
         cell_zr<double, TO, COMPAT>(lw_all + md[m0 + j].lw_off + (size_t)iw * g.L, g, iw, ik, dk, df, omega_i, v);
         store_zr_rl(v, grow, ik, A0, A1, rl);
+        if (row_peak) {
+            const unsigned m = cell_mag_bits(v);
+            if (m) atomicMax(row_peak + grow, m);
+        }
     }
 }
 
@@ -391,9 +401,9 @@ hipError_t launch_cells_zr_batch(int prec, int compat, int bs, int br, const Mod
 
 hipError_t launch_fallback_batch(int prec, int compat, const ModelDesc *md, int m0, const void *lw64, const void *lw32,
                                  int nrow, int nw, int nk, double dk, double df, double omega_i, double tol,
-                                 const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *count,
+                                 unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *count,
                                  unsigned *list, unsigned *model_count, void *A0, void *A1, RowLayout rl,
-                                 FlagList fl, hipStream_t st)
+                                 FlagList fl, hipStream_t st, bool raise_peaks)
 {
     hipLaunchKernelGGL(k_fallback_select_batch, dim3((nrow + 255) / 256), dim3(256), 0, st, nrow, nw, nk, tol, row_peak,
                        tail_max, nk_w, count, list, model_count);
@@ -401,7 +411,7 @@ hipError_t launch_fallback_batch(int prec, int compat, const ModelDesc *md, int 
 #define GRTM_FBB(T, TO, CP, F64, LWP, FLL)                                                                       \
     hipLaunchKernelGGL((k_fallback_batch<T, TO, CP, F64>), grid, dim3(kCellThreads), 0, st, md, m0,              \
                        (const LayerW<T> *)LWP, nw, nk, dk, df, omega_i, (TO *)A0, (TO *)A1, rl, count, list, nrow,  \
-                       FLL)
+                       FLL, raise_peaks ? row_peak : nullptr)
     const FlagList none{nullptr, nullptr, 0u};
     if (prec == 0) {
         if (compat) GRTM_FBB(double, double, true, true, lw64, none);
@@ -419,15 +429,17 @@ hipError_t launch_fallback_batch(int prec, int compat, const ModelDesc *md, int 
 
 hipError_t launch_cells_fix_batch(int compat, const ModelDesc *md, int m0, const void *lw64, int nw, int nk, double dk,
                                   double df, double omega_i, void *A0, void *A1, RowLayout rl, FlagList fl,
-                                  hipStream_t st)
+                                  hipStream_t st, unsigned *row_peak)
 {
     dim3 grid(1024);
     if (compat)
         hipLaunchKernelGGL((k_cells_fix_batch<float, true>), grid, dim3(kCellThreads), 0, st, md, m0,
-                           (const LayerW<double> *)lw64, nw, nk, dk, df, omega_i, (float *)A0, (float *)A1, rl, fl);
+                           (const LayerW<double> *)lw64, nw, nk, dk, df, omega_i, (float *)A0, (float *)A1, rl, fl,
+                           row_peak);
     else
         hipLaunchKernelGGL((k_cells_fix_batch<float, false>), grid, dim3(kCellThreads), 0, st, md, m0,
-                           (const LayerW<double> *)lw64, nw, nk, dk, df, omega_i, (float *)A0, (float *)A1, rl, fl);
+                           (const LayerW<double> *)lw64, nw, nk, dk, df, omega_i, (float *)A0, (float *)A1, rl, fl,
+                           row_peak);
     return hipGetLastError();
 }
 

@@ -2396,7 +2396,10 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
         }
     };
     int *nkw = b->adaptive ? b->nk_w.as<int>() + r0 : nullptr;
-    unsigned *rowpk = b->adaptive ? b->row_peak.as<unsigned>() + r0 : nullptr;
+    // row peaks: the adaptive grid's check, and the FP16 split's row scales
+    const bool f16 = b->bsplit == grtm::kSplitF16x3;
+    unsigned *rowpk = (b->adaptive || f16) ? b->row_peak.as<unsigned>() + r0 : nullptr;
+    if (f16 && !b->adaptive) HIPCHK(grtm::launch_zero_u32(rowpk, (int)nrow, s));
     if (b->adaptive) {
         HIPCHK(grtm::launch_copy_i32(b->nk_pred.as<int>() + r0, nkw, (int)nrow, s));
         HIPCHK(grtm::launch_zero_u32(rowpk, (int)nrow, s));
@@ -2448,7 +2451,7 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
     if (!b->adaptive) {
         if (fl.count)
             HIPCHK(grtm::launch_cells_fix_batch(b->compat, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df,
-                                                b->omega_i, b->A0.p, b->A1.p, rl, fl, s));
+                                                b->omega_i, b->A0.p, b->A1.p, rl, fl, s, f16 ? rowpk : nullptr));
         return GRTM_OK;
     }
     // dense fallback of the group's rows whose check failed (each row in its
@@ -2458,10 +2461,10 @@ int batch_cells(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
     HIPCHK(grtm::launch_fallback_batch(batch_mixed(b) ? 1 : 0, b->compat, md, g.m0, b->lw64.p, b->lw32.p, (int)nrow,
                                        b->nw, b->nk, b->dk, b->df, b->omega_i, b->chk_tol, rowpk, tailmx, nkw, gcount,
                                        b->fb_list.as<unsigned>() + 2 * r0, b->fb_count.as<unsigned>() + g.m0, b->A0.p,
-                                       b->A1.p, rl, fl, s));
+                                       b->A1.p, rl, fl, s, f16));
     if (fl.count)
         HIPCHK(grtm::launch_cells_fix_batch(b->compat, md, g.m0, b->lw64.p, b->nw, b->nk, b->dk, b->df, b->omega_i,
-                                            b->A0.p, b->A1.p, rl, fl, s));
+                                            b->A0.p, b->A1.p, rl, fl, s, f16 ? rowpk : nullptr));
     if (b->resident) HIPCHK(grtm::launch_clamp_extent((int)nrow, nkw, b->fext.as<int>(), b->ext_over.as<unsigned>(), s));
     HIPCHK(grtm::launch_kend_tiles((int)nrow, batch_rpt(b), b->nk, nkw, b->kend.as<int>() + batch_kend_off(b, g), s));
     return GRTM_OK;
@@ -2478,7 +2481,9 @@ int batch_gemm(grtm_batch *b, const grtm_batch::Group &g, hipStream_t s)
                                              b->S0.p, b->S1.p, b->ldk, b->C0.as<float>(), b->C1.as<float>(),
                                              b->ws.as<float>(), kend, nkw, s,
                                              b->resident ? b->foff.as<long long>() : nullptr,
-                                             b->resident ? b->fext.as<int>() : nullptr));
+                                             b->resident ? b->fext.as<int>() : nullptr,
+                                             b->bsplit == grtm::kSplitF16x3 ? b->row_peak.as<unsigned>() + r0
+                                                                             : nullptr));
         return GRTM_OK;
     }
     if (b->pair && b->gemm_mode == 1) {
@@ -2808,9 +2813,8 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
 {
     if (!ctx || !models || n_models < 1 || !dists || nr < 1 || !grid || !out) return GRTM_E_INPUT;
     *out = nullptr;
-    if (precision == GRTM_PREC_FP16X3) return GRTM_E_UNSUPPORTED;  // plans only
     if (precision != GRTM_PREC_FP64 && precision != GRTM_PREC_FP32 && precision != GRTM_PREC_BF16X3 &&
-        precision != GRTM_PREC_BF16X6)
+        precision != GRTM_PREC_BF16X6 && precision != GRTM_PREC_FP16X3)
         return GRTM_E_INPUT;
     if (!grid_ok(grid, precision)) return GRTM_E_INPUT;
     if (precision == GRTM_PREC_FP64 && grid->nw > 4096) return GRTM_E_UNSUPPORTED;
@@ -2903,7 +2907,14 @@ int grtm_batch_create(grtm_ctx *ctx, const grtm_model *models, int n_models, con
     const bool pair_ok = grtm::gemm_pair_ok(b->nk, nr, b->A0.p, b->A1.p, b->B0.p, b->B1.p);
     if (pair_ok && (precision == GRTM_PREC_BF16X3 || precision == GRTM_PREC_BF16X6)) {
         b->gemm_mode = 1;
-        b->bsplit = precision == GRTM_PREC_BF16X3 ? 3 : 6;
+        b->bsplit = precision == GRTM_PREC_BF16X3 ? grtm::kSplitBf16x3 : grtm::kSplitBf16x6;
+    }
+    if (precision == GRTM_PREC_FP16X3) {
+        // the FP16 split on the pair layout only, its row scales from the
+        // recursion's per-row peaks (kept on dense grids too)
+        if (!pair_ok) return GRTM_E_UNSUPPORTED;
+        b->bsplit = grtm::kSplitF16x3;
+        if ((st = b->row_peak.alloc(sizeof(unsigned) * (size_t)n_models * b->nw))) return st;
     }
     size_t wsb = grtm::gemm_workspace_bytes(b->gemm_mode, (int)arows, nr, b->nk);
     if (b->gemm_mode != 2 && pair_ok) {

@@ -292,14 +292,17 @@ hipError_t launch_batch_items(int nslots, const unsigned *slots, int nw, int bs,
 // their first wavenumbers; count: the group's counter; model_count: per
 // model), then the completion of the listed rows, FP64 below each model's r64
 // and in prec (0 FP64, else FP32) above.
+// raise_peaks: the completed cells raise their rows' peaks (the FP16 split's
+// row scales)
 hipError_t launch_fallback_batch(int prec, int compat, const ModelDesc *md, int m0, const void *lw64, const void *lw32,
                                  int nrow, int nw, int nk, double dk, double df, double omega_i, double tol,
-                                 const unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *count,
+                                 unsigned *row_peak, const unsigned *tail_max, int *nk_w, unsigned *count,
                                  unsigned *list, unsigned *model_count, void *A0, void *A1, RowLayout rl,
-                                 FlagList fl, hipStream_t st);
+                                 FlagList fl, hipStream_t st, bool raise_peaks = false);
+// row_peak (optional, group rows): raised to cover the re-solved values
 hipError_t launch_cells_fix_batch(int compat, const ModelDesc *md, int m0, const void *lw64, int nw, int nk, double dk,
                                   double df, double omega_i, void *A0, void *A1, RowLayout rl, FlagList fl,
-                                  hipStream_t st);
+                                  hipStream_t st, unsigned *row_peak = nullptr);
 // Test hook: non-finite FP32 operands in the listed cells (group row nk + k),
 // each appended to the flag list.
 hipError_t launch_plant_nonfinite_batch(const unsigned long long *cells, int n, int nk, void *A0, void *A1,

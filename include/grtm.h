@@ -42,9 +42,9 @@ extern "C" {
                               split-FP16 matrix-core products: each frequency
                               row of the operands scaled by a power of two set
                               by its peak, two fp16 terms per value, the three
-                              products above 2^-22.  Trace plans of the pair
-                              shape only (GRTM_E_UNSUPPORTED otherwise, and for
-                              column plans and batches) */
+                              products above 2^-22.  Trace plans and batches
+                              of the pair shape only (GRTM_E_UNSUPPORTED
+                              otherwise, and for column plans) */
 
 /* ---------------------------------------------------------- drop-in entry */
 /* Replaces the reference's only exported symbol:

@@ -1,7 +1,7 @@
 """The split-precision contractions in plans and batches (GRTM_PREC_BF16X6:
 three bf16 terms per FP32 operand, six products; GRTM_PREC_BF16X3: two
-terms, three products; GRTM_PREC_FP16X3, plans only: two fp16 terms of each
-row-scaled operand, three products), on the same FP32 recursion as
+terms, three products; GRTM_PREC_FP16X3: two fp16 terms of each row-scaled
+operand, three products), on the same FP32 recursion as
 GRTM_PREC_FP32: traces against the plain FP64 plans, and against the FP32
 plans they must track.
 (The 3-layer model's traces of both forms against the oracle sit with the
@@ -94,13 +94,46 @@ def test_c2_fp16x3_traces(c2_ref, learn):
     assert d32.max() > 0
 
 
-def test_fp16x3_plans_only():
-    """The FP16 split runs on Z / R trace plans only: batches and column plans
-    refuse it."""
+def test_c4_sample_fp16x3_learned_batch():
+    """The same twelve models of the 1024-model batch with the scaled FP16
+    split, learned grids on the resident layout, against per-model FP64 plans."""
     cfg = models.config("c4")
-    ms = cfg["models"][:2]
-    with pytest.raises(g.GrtmError):
-        g.BatchPlan(ms, 0.0, cfg["dists"][:8], dict(cfg["grid"], nw=64, nk=512), precision=g.PREC_FP16X3)
+    ms = [cfg["models"][i] for i in range(0, 1024, 86)][:12]
+    grid, dists = cfg["grid"], cfg["dists"]
+    print(f"gpu: c4 sample of {len(ms)} models, fp16x3 learned batch", flush=True)
+    bp = g.BatchPlan(ms, 0.0, dists, grid, precision=g.PREC_FP16X3)
+    bp.set_learned(learned.Predictor(), stride=8, tol=3e-6)
+    bp.execute()
+    tr = bp.traces()
+    assert bp.config()["resident"] == 1
+    worst = worst_pk = 0.0
+    for m, (lay, zs) in enumerate(ms):
+        ref = g.TracePlan(lay, zs, 0.0, dists, grid, precision=g.PREC_FP64).traces()
+        rl, pk = errs(tr[m], ref)
+        worst, worst_pk = max(worst, float(rl.max())), max(worst_pk, float(pk.max()))
+    print(f"  vs FP64 plans: rel-L2 {worst:.2e}, peak {worst_pk:.2e}")
+    assert worst < 1e-4 and worst_pk < 1e-4
+
+
+def test_fp16x3_dense_batch_matches_plans():
+    """A dense FP16X3 batch (row peaks kept by its recursion for the scales)
+    against per-model FP16X3 plans on the same grid: the same arithmetic up to
+    the order of the stream-K sums."""
+    cfg = models.config("c4", n_models=6)
+    grid = dict(cfg["grid"], nw=128, nk=1024)
+    dists = cfg["dists"][:16]
+    print("gpu: dense fp16x3 batch of 6 models vs plans", flush=True)
+    bp = g.BatchPlan(cfg["models"], 0.0, dists, grid, precision=g.PREC_FP16X3)
+    tr = bp.traces()
+    for m, (lay, zs) in enumerate(cfg["models"]):
+        ref = g.TracePlan(lay, zs, 0.0, dists, grid, precision=g.PREC_FP16X3).traces()
+        rl, _ = errs(tr[m], ref)
+        assert rl.max() < 2e-5, (m, rl.max())
+
+
+def test_fp16x3_not_for_columns():
+    """Column plans refuse the FP16 split (their operands are the column rows of
+    four contractions, without per-row peaks)."""
     c1 = models.config("c1")
     grid = dict(c1["grid"], nw=64, nk=512)
     with pytest.raises(g.GrtmError):
