@@ -329,7 +329,7 @@ def bench_columns(args, g, models):
     }
 
 
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "c2_adaptive_fp32_pmc_bytes.md")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "c2_adaptive_pmc_bytes.md")
 
 
 def pmc_traffic(kernel):
