@@ -489,6 +489,43 @@ __global__ __launch_bounds__(256) void k_streamk_reduce(int n, int N, GemmSet gs
     if (b0 == b1) return;  // written whole by block b0
     // block b0 left its part in slot 1 when the tile is not its first
     const int slot0 = (G * b0 / nbe < t0) ? 1 : 0;
+    // 16-byte path: each thread's NV groups of 4 columns, all loads of one
+    // block's partials issued before they are added (the sums keep the
+    // scalar path's order, so both give the same bits)
+    constexpr int NV = BM * BN / 4 / 256 / kReduceSlices;
+    static_assert(NV >= 1 && NV * 4 * 256 * kReduceSlices == BM * BN, "reduce slice");
+    if (gridDim.y == kReduceSlices && blockDim.x == 256 && (N & 3) == 0 &&
+        ((reinterpret_cast<uintptr_t>(C) | reinterpret_cast<uintptr_t>(part)) & 15) == 0) {
+        int e[NV];
+        bool ok[NV];
+        double s[NV][4];
+#pragma unroll
+        for (int j = 0; j < NV; j++) {
+            e[j] = i0 + (int(threadIdx.x) + 256 * j) * 4;
+            ok[j] = m0 + e[j] / BN < M && n0 + e[j] % BN < N;  // N % 4 == 0: all four columns in range
+        }
+        float4 v[NV];
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            v[j] = ok[j] ? *reinterpret_cast<const float4 *>(part + ((size_t)b0 * 2 + slot0) * BM * BN + e[j])
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int j = 0; j < NV; j++) s[j][0] = v[j].x, s[j][1] = v[j].y, s[j][2] = v[j].z, s[j][3] = v[j].w;
+        for (int b = b0 + 1; b <= b1; b++) {
+#pragma unroll
+            for (int j = 0; j < NV; j++)
+                v[j] = ok[j] ? *reinterpret_cast<const float4 *>(part + (size_t)b * 2 * BM * BN + e[j])
+                             : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int j = 0; j < NV; j++) s[j][0] += v[j].x, s[j][1] += v[j].y, s[j][2] += v[j].z, s[j][3] += v[j].w;
+        }
+#pragma unroll
+        for (int j = 0; j < NV; j++)
+            if (ok[j])
+                *reinterpret_cast<float4 *>(C + (size_t)(m0 + e[j] / BN) * N + n0 + e[j] % BN) =
+                    make_float4(float(s[j][0]), float(s[j][1]), float(s[j][2]), float(s[j][3]));
+        return;
+    }
     for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
         const int r = m0 + i / BN, c = n0 + i % BN;
         if (r >= M || c >= N) continue;

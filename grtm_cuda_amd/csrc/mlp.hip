@@ -153,16 +153,18 @@ __global__ void k_gather(int B, int n_in, int n_out, const int *__restrict__ idx
 }
 
 // ---------------------------------------------------------------- inference
-// The fused forward: one block of 4 waves takes 32 rows through every layer
+// The fused forward: one block of 8 waves takes 32 rows through every layer
 // on v_mfma_f32_32x32x16_bf16 (BF16 operands, FP32 accumulation: the same
 // arithmetic as the training forward), activations in LDS as BF16, weights
 // read as 16-byte fragments of W^T from L2.  Hidden layers are 256 wide (each
-// wave computes 64 output columns); the last layer's 32 padded columns are
-// wave 0's.  MODE 0: rows of X in, outputs Y out.  MODE 1: the rows are
+// wave computes one 32-column tile, so a layer's serial chain per wave is 16
+// products after one round of weight loads); the last layer's 32 padded
+// columns are wave 0's.  MODE 0: rows of X in, outputs Y out.  MODE 1: the rows are
 // (model, frequency) pairs; features are formed from the model (learned.py
 // features()), and output 0 is decoded into the row's cut and residual-check
 // count (learned.py decode_kcut, the low-frequency floor, row_cut).
 constexpr int kMlpW = 256, kMlpKin = 192, kMlpNout = 32, kMlpRows = 32, kMlpMaxLayers = 8;
+constexpr int kMlpThreads = 64 * kMlpW / 32;  // one wave per 32-column tile of a hidden layer
 constexpr int kMlpLd = kMlpW + 8;  // LDS row stride (bf16): 16-byte row skew
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
@@ -209,7 +211,7 @@ __device__ __forceinline__ void learned_decode(const grtm::LearnModel &m, const 
 // wave waits on memory once per layer, not once per k step.
 template <int KP, int NT>
 __device__ __forceinline__ void mlp_layer(const bf16 (*in)[kMlpLd], const bf16 *__restrict__ W, int n0, int lane,
-                                          v16f acc[2])
+                                          v16f *acc)
 {
     const int li = lane & 31, kq = (lane >> 5) * 8;
     v8bf b[KP / 16][NT];
@@ -227,7 +229,7 @@ __device__ __forceinline__ void mlp_layer(const bf16 (*in)[kMlpLd], const bf16 *
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_mlp_infer(MlpDev net, int rows, const bf16 *__restrict__ X,
+__global__ __launch_bounds__(kMlpThreads) void k_mlp_infer(MlpDev net, int rows, const bf16 *__restrict__ X,
                                                    float *__restrict__ Y, const grtm::LearnModel *__restrict__ models,
                                                    grtm::LearnGrid lg, int *__restrict__ nk_pred,
                                                    int *__restrict__ nchk)
@@ -246,7 +248,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev net, int rows, const b
         // per model of the block's rows
         m_first = r0 / lg.nw;
         const int m_last = min(rows - 1, r0 + kMlpRows - 1) / lg.nw;
-        for (int e = tid; e < (m_last - m_first + 1) * NF; e += 256) {
+        for (int e = tid; e < (m_last - m_first + 1) * NF; e += kMlpThreads) {
             const int mm = e / NF, k = e % NF, j = k / 6, c = k % 6;
             const grtm::LearnModel &md = models[m_first + mm];
             float v = 0.f;
@@ -260,7 +262,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev net, int rows, const b
         __syncthreads();
     }
     // input rows, zero padded to kMlpKin
-    for (int e = tid; e < kMlpRows * kMlpKin; e += 256) {
+    for (int e = tid; e < kMlpRows * kMlpKin; e += kMlpThreads) {
         const int i = e / kMlpKin, k = e % kMlpKin, row = r0 + i;
         float v = 0.f;
         if (row < rows && k < net.n_in) {
@@ -280,19 +282,18 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpDev net, int rows, const b
     const int li = lane & 31;
     for (int l = 0; l < net.nl; l++) {
         const bool last = l == net.nl - 1;
-        const int n0 = last ? 0 : wv * 64;
-        v16f acc[2];
-        for (int t = 0; t < 2; t++)
-            for (int r = 0; r < 16; r++) acc[t][r] = 0.f;
+        const int n0 = last ? 0 : wv * 32;
+        v16f acc[1];
+        for (int r = 0; r < 16; r++) acc[0][r] = 0.f;
         // layer shapes of the fused kernel: kMlpKin in, kMlpW hidden, kMlpNout out
         if (last) {
             if (wv == 0) mlp_layer<kMlpW, 1>(act[cur], net.wT[l], n0, lane, acc);
         } else if (l == 0) {
-            mlp_layer<kMlpKin, 2>(act[cur], net.wT[l], n0, lane, acc);
+            mlp_layer<kMlpKin, 1>(act[cur], net.wT[l], n0, lane, acc);
         } else {
-            mlp_layer<kMlpW, 2>(act[cur], net.wT[l], n0, lane, acc);
+            mlp_layer<kMlpW, 1>(act[cur], net.wT[l], n0, lane, acc);
         }
-        const int ntile = last ? (wv == 0 ? 1 : 0) : 2;
+        const int ntile = last ? (wv == 0 ? 1 : 0) : 1;
         // accumulator r of lane l: row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
         for (int t = 0; t < ntile; t++)
             for (int r = 0; r < 16; r++) {
@@ -671,7 +672,7 @@ static int mlp_predict(grtm_mlp *n, const float *X, int cnt, float *Y, bool fuse
         HIPCHK(hipMemcpy(n->Xb.p, xb.data(), sizeof(bf16) * xb.size(), hipMemcpyHostToDevice));
         const float *src = n->Z[n->depth].as<float>();
         if (fused) {
-            hipLaunchKernelGGL((k_mlp_infer<0>), dim3((B + kMlpRows - 1) / kMlpRows), dim3(256), 0, n->st, mlp_dev(n),
+            hipLaunchKernelGGL((k_mlp_infer<0>), dim3((B + kMlpRows - 1) / kMlpRows), dim3(kMlpThreads), 0, n->st, mlp_dev(n),
                                B, n->Xb.as<bf16>(), n->Z[n->depth].as<float>(), nullptr, grtm::LearnGrid{}, nullptr,
                                nullptr);
             HIPCHK(hipGetLastError());
@@ -704,7 +705,8 @@ int learned_grid(grtm_mlp *net, const LearnModel *models, int n_models, const Le
     if (!net || !net->fused || net->fin[0] != 6 * kLearnSlots + 2 || net->n_out != kLearnOut || n_models < 1)
         return GRTM_E_UNSUPPORTED;
     const int rows = n_models * lg.nw;
-    hipLaunchKernelGGL((k_mlp_infer<1>), dim3((rows + kMlpRows - 1) / kMlpRows), dim3(256), 0, st, mlp_dev(net), rows,
+    hipLaunchKernelGGL((k_mlp_infer<1>), dim3((rows + kMlpRows - 1) / kMlpRows), dim3(kMlpThreads), 0, st, mlp_dev(net),
+                       rows,
                        nullptr, nullptr, models, lg, nk_pred, nchk);
     HIPCHK(hipGetLastError());
     return GRTM_OK;
