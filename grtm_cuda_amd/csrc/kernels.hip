@@ -127,7 +127,7 @@ __global__ __launch_bounds__(1024) void k_grid_scan(int nw, int r64, const int *
 {
     if (threadIdx.x < kGridZero && gi.zero[threadIdx.x]) *gi.zero[threadIdx.x] = 0u;
     constexpr int R = 8;
-    __shared__ unsigned st[1024], sc[1024];
+    __shared__ unsigned wt[16], wc[16];
     __shared__ unsigned t64_sh, c64_sh;
     const int t = threadIdx.x;
     unsigned nt[R], nc[R], ti = 0, ci = 0;
@@ -138,18 +138,23 @@ __global__ __launch_bounds__(1024) void k_grid_scan(int nw, int r64, const int *
         ti += nt[q];
         ci += nc[q];
     }
-    st[t] = ti;
-    sc[t] = ci;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {  // inclusive scans over the threads
-        const unsigned a = t >= o ? st[t - o] : 0u, b = t >= o ? sc[t - o] : 0u;
-        __syncthreads();
-        st[t] += a;
-        sc[t] += b;
-        __syncthreads();
+    // inclusive scans over the threads: within each wave by shuffles, then
+    // the 16 wave totals
+    const int lane = t & 63, wv = t >> 6;
+    unsigned si = ti, sci = ci;
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned a = unsigned(__shfl_up(int(si), o, 64)), b = unsigned(__shfl_up(int(sci), o, 64));
+        if (lane >= o) si += a, sci += b;
     }
-    const unsigned T = st[1023], Cn = sc[1023];
-    unsigned pt = st[t] - ti, pc = sc[t] - ci;  // exclusive prefixes of this thread's first row
+    if (lane == 63) wt[wv] = si, wc[wv] = sci;
+    __syncthreads();
+    unsigned T = 0, Cn = 0;
+    for (int w = 0; w < 16; w++) {
+        if (w == wv) si += T, sci += Cn;  // the earlier waves' totals
+        T += wt[w];
+        Cn += wc[w];
+    }
+    unsigned pt = si - ti, pc = sci - ci;  // exclusive prefixes of this thread's first row
     for (int q = 0; q < R; q++) {  // the prefixes at row r64
         if (t * R + q == r64) t64_sh = pt, c64_sh = pc;
         pt += nt[q];
@@ -158,7 +163,7 @@ __global__ __launch_bounds__(1024) void k_grid_scan(int nw, int r64, const int *
     if (t == 0 && r64 >= nw) t64_sh = T, c64_sh = Cn;
     __syncthreads();
     const unsigned T64 = t64_sh, C64 = c64_sh;
-    pt = st[t] - ti, pc = sc[t] - ci;
+    pt = si - ti, pc = sci - ci;
     for (int q = 0; q < R; q++) {
         const int iw = t * R + q;
         if (iw >= nw) break;
@@ -576,8 +581,11 @@ __device__ __forceinline__ cx<T> post_val(cx<T> v, int iw, const PostParams &pp)
 // 128-byte line of the row-major contraction outputs: R consecutive lanes
 // load R consecutive values, and all R spectra are staged in LDS before they
 // are transformed one after the other.
-// 256 threads; LDS post_ifft_lds_bytes.
-constexpr int kIfftThreads = 256;
+// kIfftThreads threads (16 waves per compute unit at two workgroups: 30.0
+// us for the 1024 traces of the 256-receiver workload, 31.6 at 256 threads;
+// a copy of the twiddle table in LDS measured slower, 33.6); LDS
+// post_ifft_lds_bytes.
+constexpr int kIfftThreads = 512;
 template <typename T, int R, typename Load>
 __device__ __forceinline__ void post_ifft_group(const PostParams &pp, Load &&load, int t0, T *__restrict__ traces,
                                                 T *__restrict__ spec_post)
@@ -685,7 +693,7 @@ static unsigned xcd_grid(int ng) { return (unsigned)((ng + kXcds - 1) / kXcds * 
 // belongs to model t / (4 nr), whose contraction outputs start cms elements
 // further per model.
 template <typename T, int R>
-__global__ __launch_bounds__(256) void k_post_ifft(PostParams pp, int layout, const T *__restrict__ raw,
+__global__ __launch_bounds__(kIfftThreads) void k_post_ifft(PostParams pp, int layout, const T *__restrict__ raw,
                                                    const T *__restrict__ C0, const T *__restrict__ C1,
                                                    long ldc, int nr, long cms, int ntr, T *__restrict__ traces,
                                                    T *__restrict__ spec_post)
@@ -715,7 +723,7 @@ __global__ __launch_bounds__(256) void k_post_ifft(PostParams pp, int layout, co
 // the contraction outputs (rows 2 (rows_set iw + row) + {0 re, 1 im}).
 // Groups of R receivers of one column (R divides nr).
 template <typename T, int R>
-__global__ __launch_bounds__(256) void k_post_ifft_cols(PostParams pp, const MtCols *__restrict__ mcp, long ldc,
+__global__ __launch_bounds__(kIfftThreads) void k_post_ifft_cols(PostParams pp, const MtCols *__restrict__ mcp, long ldc,
                                                         int nr, int ntr, const double *__restrict__ dists,
                                                         T *__restrict__ traces)
 {
@@ -1050,6 +1058,7 @@ size_t post_ifft_lds_bytes(int prec, int nw, int R)
 {
     return (size_t)(R + 1) * nw * (prec == 0 ? 16 : 8);
 }
+
 
 // Traces per IFFT workgroup: 2 adjacent receivers when nr is even and their
 // LDS (with `extra` bytes of static LDS) fits a workgroup (160 KiB), else 1.
