@@ -144,48 +144,100 @@ __global__ __launch_bounds__(BS, sizeof(T) == 4 ? 4 : 1) void k_cells_zr_batch(c
 // Items of a group's learned grids formed on the device.  The group's item
 // order is fixed by the host as a list of slots, each a group row with a
 // phase (kCheckItem: its residual-check items, else its tiles of BS
-// wavenumbers): k_batch_scan (one block of 1024 threads) gives every slot its
-// first item (an exclusive scan of the slots' item counts) and seg_items[s]
-// the first item of segment s, whose slots start at seg_slots[s] (nseg + 1
-// bounds); k_batch_fill (one thread per slot) writes the items (j << 32 |
-// row << kTileBits | tile, with kCheckItem on check items).
-__global__ __launch_bounds__(1024) void k_batch_scan(int nslots, const unsigned *__restrict__ slots, int nw, int bs,
-                                                     const int *__restrict__ nk_pred, const int *__restrict__ nchk,
-                                                     unsigned *__restrict__ slot_off, int nseg,
-                                                     const unsigned *__restrict__ seg_slots,
-                                                     unsigned *__restrict__ seg_items)
+// wavenumbers).  Every slot's first item is an exclusive scan of the slots'
+// item counts, over blocks of 1024 slots in three launches: k_batch_count
+// (each block's total), k_batch_scan_sums (one block: the totals' exclusive
+// scan and the grand total), k_batch_offsets (each block's own scan plus its
+// offset; seg_items[s], the first item of segment s whose slots start at
+// seg_slots[s], nseg + 1 bounds).  k_batch_fill (one thread per slot) then
+// writes the items (j << 32 | row << kTileBits | tile, with kCheckItem on
+// check items).  A single block walking all the slots took 8.7 ms for the
+// 2.1 M slots of a 1024-model batch.
+__device__ __forceinline__ unsigned slot_items(int q, const unsigned *__restrict__ slots, int bs,
+                                               const int *__restrict__ nk_pred, const int *__restrict__ nchk)
 {
-    __shared__ unsigned sh[1024];
-    const int t = threadIdx.x;
-    const int per = (nslots + 1023) / 1024, s0 = min(nslots, t * per), s1 = min(nslots, s0 + per);
-    auto count = [&](int q) {
-        const unsigned sl = slots[q];
-        const int grow = int(sl & ~kCheckItem);
-        const int n = (sl & kCheckItem) ? nchk[grow] : nk_pred[grow];
-        return unsigned((n + bs - 1) / bs);
-    };
-    unsigned sum = 0;
-    for (int q = s0; q < s1; q++) sum += count(q);
-    sh[t] = sum;
+    const unsigned sl = slots[q];
+    const int grow = int(sl & ~kCheckItem);
+    const int n = (sl & kCheckItem) ? nchk[grow] : nk_pred[grow];
+    return unsigned((n + bs - 1) / bs);
+}
+
+// Inclusive scan over a block of 1024 threads (16 waves): shuffles within
+// each wave, then the wave totals (wsum: 16 words of LDS).  Returns the
+// thread's inclusive prefix; *total gets the block's sum.
+__device__ __forceinline__ unsigned block_scan_1024(unsigned v, unsigned *wsum, unsigned *total)
+{
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned a = unsigned(__shfl_up(int(v), o, 64));
+        if (lane >= o) v += a;
+    }
+    if (lane == 63) wsum[wv] = v;
     __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const unsigned a = t >= o ? sh[t - o] : 0u;
-        __syncthreads();
-        sh[t] += a;
-        __syncthreads();
+    unsigned run = 0;
+    for (int w = 0; w < 16; w++) {
+        if (w == wv) v += run;  // the earlier waves' totals
+        run += wsum[w];
     }
-    unsigned run = sh[t] - sum;
+    *total = run;
+    return v;
+}
+
+__global__ __launch_bounds__(1024) void k_batch_count(int nslots, const unsigned *__restrict__ slots, int bs,
+                                                      const int *__restrict__ nk_pred, const int *__restrict__ nchk,
+                                                      unsigned *__restrict__ bsum)
+{
+    __shared__ unsigned wsum[16];
+    const int q = blockIdx.x * 1024 + threadIdx.x;
+    const unsigned c = q < nslots ? slot_items(q, slots, bs, nk_pred, nchk) : 0u;
+    unsigned total;
+    block_scan_1024(c, wsum, &total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// bsum[0 .. nb): block totals -> their exclusive prefixes; bsum[nb] = the sum.
+__global__ __launch_bounds__(1024) void k_batch_scan_sums(int nb, unsigned *__restrict__ bsum)
+{
+    __shared__ unsigned wsum[16];
+    const int t = threadIdx.x;
+    const int per = (nb + 1023) / 1024, s0 = min(nb, t * per), s1 = min(nb, s0 + per);
+    unsigned sum = 0;
+    for (int q = s0; q < s1; q++) sum += bsum[q];
+    unsigned total;
+    unsigned run = block_scan_1024(sum, wsum, &total) - sum;
+    __syncthreads();  // every thread has read its totals before any is overwritten
     for (int q = s0; q < s1; q++) {
-        slot_off[q] = run;
-        run += count(q);
+        const unsigned v = bsum[q];
+        bsum[q] = run;
+        run += v;
     }
-    __syncthreads();  // slot_off written by the block is visible to it
-    // segment bounds: the first item of the segment's first slot (the total
-    // for a bound at nslots)
-    const unsigned total = sh[1023];
-    for (int q = t; q <= nseg; q += 1024) {
-        const unsigned b = seg_slots[q];
-        seg_items[q] = b < unsigned(nslots) ? slot_off[b] : total;
+    if (t == 0) bsum[nb] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_batch_offsets(int nslots, const unsigned *__restrict__ slots, int bs,
+                                                        const int *__restrict__ nk_pred, const int *__restrict__ nchk,
+                                                        const unsigned *__restrict__ bsum, int nb,
+                                                        unsigned *__restrict__ slot_off, int nseg,
+                                                        const unsigned *__restrict__ seg_slots,
+                                                        unsigned *__restrict__ seg_items)
+{
+    __shared__ unsigned wsum[16], off[1024];
+    const int q0 = blockIdx.x * 1024, q = q0 + threadIdx.x;
+    const unsigned c = q < nslots ? slot_items(q, slots, bs, nk_pred, nchk) : 0u;
+    unsigned total;
+    const unsigned o = bsum[blockIdx.x] + block_scan_1024(c, wsum, &total) - c;
+    off[threadIdx.x] = o;
+    if (q < nslots) slot_off[q] = o;
+    __syncthreads();
+    // segment bounds: the first item of the bound's slot, or the grand total
+    // for a bound at nslots (taken by block 0)
+    for (int sg = threadIdx.x; sg <= nseg; sg += 1024) {
+        const unsigned b = seg_slots[sg];
+        if (b >= unsigned(nslots)) {
+            if (blockIdx.x == 0) seg_items[sg] = bsum[nb];
+        } else if (b >= unsigned(q0) && b < unsigned(q0) + 1024u) {
+            seg_items[sg] = off[b - unsigned(q0)];
+        }
     }
 }
 
@@ -328,13 +380,19 @@ static int resident_blocks_bs(const void *kernel, int bs, int ub)
     return std::max(1, std::min(ub, per * cus));
 }
 
+size_t batch_items_words(int nslots) { return (size_t)nslots + (size_t)(nslots + 1023) / 1024 + 1; }
+
 hipError_t launch_batch_items(int nslots, const unsigned *slots, int nw, int bs, const int *nk_pred, const int *nchk,
                               unsigned *slot_off, int nseg, const unsigned *seg_slots, unsigned *seg_items,
                               unsigned long long *items, hipStream_t st)
 {
     if (nslots < 1 || nseg < 1 || nseg > 1023) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_batch_scan, dim3(1), dim3(1024), 0, st, nslots, slots, nw, bs, nk_pred, nchk, slot_off, nseg,
-                       seg_slots, seg_items);
+    const int nb = (nslots + 1023) / 1024;
+    unsigned *bsum = slot_off + nslots;  // nb + 1 words after the slots' offsets
+    hipLaunchKernelGGL(k_batch_count, dim3(nb), dim3(1024), 0, st, nslots, slots, bs, nk_pred, nchk, bsum);
+    hipLaunchKernelGGL(k_batch_scan_sums, dim3(1), dim3(1024), 0, st, nb, bsum);
+    hipLaunchKernelGGL(k_batch_offsets, dim3(nb), dim3(1024), 0, st, nslots, slots, bs, nk_pred, nchk, bsum, nb,
+                       slot_off, nseg, seg_slots, seg_items);
     hipLaunchKernelGGL(k_batch_fill, dim3((nslots + 255) / 256), dim3(256), 0, st, nslots, slots, nw, bs, nk_pred, nchk,
                        slot_off, items);
     return hipGetLastError();

@@ -2652,7 +2652,7 @@ int batch_learned_slots(grtm_batch *b)
     if (cap > 0x7fffffffu || (b->resident && (size_t)b->nm * b->nw >= (size_t(1) << 31) / 4))
         return GRTM_E_UNSUPPORTED;
     if ((st = b->lslots.alloc(sizeof(unsigned) * std::max<size_t>(1, slots.size()))) ||
-        (st = b->lslot_off.alloc(sizeof(unsigned) * std::max(1, maxslots))) ||
+        (st = b->lslot_off.alloc(sizeof(unsigned) * grtm::batch_items_words(std::max(1, maxslots)))) ||
         (st = b->lseg_slots.alloc(sizeof(unsigned) * segs.size())) ||
         (st = b->lseg_items.alloc(sizeof(unsigned) * segs.size())) || (st = b->lctr.alloc(sizeof(unsigned) * 8)) ||
         (st = b->litems.alloc(sizeof(unsigned long long) * std::max<size_t>(1, cap))))

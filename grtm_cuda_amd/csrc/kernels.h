@@ -280,10 +280,12 @@ hipError_t launch_cells_zr_batch(int prec, int compat, int bs, int br, const Mod
                                  const int *nchk = nullptr, int stride = 0, unsigned *tail_max = nullptr,
                                  const unsigned *dev_count = nullptr, int which = 0, unsigned *ctr = nullptr,
                                  int chunk = 1);
-// A group's learned-grid items from its slot order (batch.hip k_batch_scan,
-// k_batch_fill): slots[q] = group row | kCheckItem for its check items;
-// segment s covers slots [seg_slots[s], seg_slots[s + 1]) and gets items
-// [seg_items[s], seg_items[s + 1]); slot_off: nslots scratch words.
+// A group's learned-grid items from its slot order (batch.hip k_batch_count,
+// k_batch_scan_sums, k_batch_offsets, k_batch_fill): slots[q] = group row |
+// kCheckItem for its check items; segment s covers slots [seg_slots[s],
+// seg_slots[s + 1]) and gets items [seg_items[s], seg_items[s + 1]);
+// slot_off: batch_items_words(nslots) scratch words.
+size_t batch_items_words(int nslots);
 hipError_t launch_batch_items(int nslots, const unsigned *slots, int nw, int bs, const int *nk_pred, const int *nchk,
                               unsigned *slot_off, int nseg, const unsigned *seg_slots, unsigned *seg_items,
                               unsigned long long *items, hipStream_t st);
