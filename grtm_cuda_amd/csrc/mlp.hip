@@ -206,28 +206,6 @@ __device__ __forceinline__ void learned_decode(const grtm::LearnModel &m, const 
     if (room > 0 && nc == 0) n = lg.nk;
 }
 
-// One layer of the fused forward for this wave's NT output tiles: every
-// weight fragment of the layer is requested before the first product, so the
-// wave waits on memory once per layer, not once per k step.
-template <int KP, int NT>
-__device__ __forceinline__ void mlp_layer(const bf16 (*in)[kMlpLd], const bf16 *__restrict__ W, int n0, int lane,
-                                          v16f *acc)
-{
-    const int li = lane & 31, kq = (lane >> 5) * 8;
-    v8bf b[KP / 16][NT];
-#pragma unroll
-    for (int s = 0; s < KP / 16; s++)
-#pragma unroll
-        for (int t = 0; t < NT; t++)
-            b[s][t] = *reinterpret_cast<const v8bf *>(W + (size_t)(n0 + 32 * t + li) * KP + 16 * s + kq);
-#pragma unroll
-    for (int s = 0; s < KP / 16; s++) {
-        const v8bf a = *reinterpret_cast<const v8bf *>(&in[li][16 * s + kq]);
-#pragma unroll
-        for (int t = 0; t < NT; t++) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[s][t], acc[t], 0, 0, 0);
-    }
-}
-
 template <int MODE>
 __global__ __launch_bounds__(kMlpThreads) void k_mlp_infer(MlpDev net, int rows, const bf16 *__restrict__ X,
                                                    float *__restrict__ Y, const grtm::LearnModel *__restrict__ models,
@@ -240,6 +218,25 @@ __global__ __launch_bounds__(kMlpThreads) void k_mlp_infer(MlpDev net, int rows,
     __shared__ float ysh[MODE ? kMlpRows : 1][MODE ? kLearnOut : 1];  // the rows' grid outputs
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int r0 = blockIdx.x * kMlpRows;
+    const int li = lane & 31, kq = (lane >> 5) * 8;
+    // layer l's weight fragments of this wave (16-byte pieces of W^T: its 32
+    // output columns of a hidden layer, wave 0 the last layer's 32 padded
+    // ones) and the lane's bias.  The next layer's are requested before this
+    // layer's products, so only layer 0's load latency is exposed, and it
+    // overlaps the inputs.
+    auto wload = [&](int l, v8bf(&b)[kMlpW / 16], float &bias) {
+        const bool last = l == net.nl - 1;
+        if (last && wv != 0) return;
+        const int kp = l == 0 ? kMlpKin : kMlpW, n0 = last ? 0 : wv * 32;
+        const bf16 *__restrict__ W = net.wT[l];
+#pragma unroll
+        for (int s = 0; s < kMlpW / 16; s++)
+            if (s < kp / 16) b[s] = *reinterpret_cast<const v8bf *>(W + (size_t)(n0 + li) * kp + 16 * s + kq);
+        bias = net.b[l][n0 + li];
+    };
+    v8bf wa[kMlpW / 16], wb[kMlpW / 16];
+    float ba = 0.f, bb = 0.f;
+    wload(0, wa, ba);
     int m_first = 0;
     if constexpr (MODE == 1) {
         // learned.py features(): (h / 10, vp / 8, vs / 4.5, rho / 3.3,
@@ -279,27 +276,26 @@ __global__ __launch_bounds__(kMlpThreads) void k_mlp_infer(MlpDev net, int rows,
     }
     __syncthreads();
     int cur = 0;
-    const int li = lane & 31;
-    for (int l = 0; l < net.nl; l++) {
+    // one layer: this wave's products over the layer's K (kMlpKin in, kMlpW
+    // hidden, kMlpNout out), then bias, ReLU and the BF16 activations of the
+    // next layer (the last layer's outputs to Y or the grid decode)
+    auto layer = [&](int l, const v8bf(&b)[kMlpW / 16], float bias) {
         const bool last = l == net.nl - 1;
-        const int n0 = last ? 0 : wv * 32;
-        v16f acc[1];
-        for (int r = 0; r < 16; r++) acc[0][r] = 0.f;
-        // layer shapes of the fused kernel: kMlpKin in, kMlpW hidden, kMlpNout out
-        if (last) {
-            if (wv == 0) mlp_layer<kMlpW, 1>(act[cur], net.wT[l], n0, lane, acc);
-        } else if (l == 0) {
-            mlp_layer<kMlpKin, 1>(act[cur], net.wT[l], n0, lane, acc);
-        } else {
-            mlp_layer<kMlpW, 1>(act[cur], net.wT[l], n0, lane, acc);
-        }
-        const int ntile = last ? (wv == 0 ? 1 : 0) : 1;
-        // accumulator r of lane l: row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
-        for (int t = 0; t < ntile; t++)
+        const int kp = l == 0 ? kMlpKin : kMlpW, n0 = last ? 0 : wv * 32;
+        v16f acc;
+        for (int r = 0; r < 16; r++) acc[r] = 0.f;
+        if (!last || wv == 0) {
+#pragma unroll
+            for (int s = 0; s < kMlpW / 16; s++)
+                if (s < kp / 16) {
+                    const v8bf a = *reinterpret_cast<const v8bf *>(&act[cur][li][16 * s + kq]);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[s], acc, 0, 0, 0);
+                }
+            // accumulator r of lane l: row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
             for (int r = 0; r < 16; r++) {
                 const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                const int col = n0 + 32 * t + li;
-                const float z = acc[t][r] + net.b[l][col];
+                const int col = n0 + li;
+                const float z = acc[r] + bias;
                 if (!last) {
                     act[cur ^ 1][row][col] = bf16(fmaxf(z, 0.f));
                 } else if (r0 + row < rows) {
@@ -310,8 +306,17 @@ __global__ __launch_bounds__(kMlpThreads) void k_mlp_infer(MlpDev net, int rows,
                     }
                 }
             }
+        }
         __syncthreads();
         cur ^= 1;
+    };
+    for (int l = 0; l < net.nl; l += 2) {
+        if (l + 1 < net.nl) wload(l + 1, wb, bb);
+        layer(l, wa, ba);
+        if (l + 1 < net.nl) {
+            if (l + 2 < net.nl) wload(l + 2, wa, ba);
+            layer(l + 1, wb, bb);
+        }
     }
     if constexpr (MODE == 1) {
         // one thread per row: its cut and check cells from all 17 outputs
