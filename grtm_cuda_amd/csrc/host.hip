@@ -788,6 +788,15 @@ struct grtm_plan {
     grtm::LearnModel lm{};
     DBuf lmodel, icounts, ictr, ioff;
     int tiles_cap = 0;       // capacity of tiles (items)
+    // mixed learned plans: rows [0, rows64) take the analytic cut (network
+    // free, set up once: nk_pred / nchk and the item list low_tiles), so their
+    // FP64 launch runs on the side stream beside the network and the item
+    // kernels; ev_fork / ev_join bracket it
+    DBuf low_tiles;
+    int n_low_tiles = 0;
+    bool low_apart = false;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool pred_stale = false;  // nk_pred_h / nchk_h behind the device's last execution
     // the job's model, kept for the predictor's inputs
     std::vector<double> layers_h;
@@ -812,6 +821,9 @@ struct grtm_plan {
     int mt_M(int q) const { return 2 * mto.rows[q] * nw; }
     ~grtm_plan()
     {
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (side) (void)hipStreamDestroy(side);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -929,6 +941,19 @@ int run_cells_core(grtm_plan *p, hipStream_t s)
     unsigned *fbcnt = ad ? rowpk + 2 * p->nw : nullptr;
     if (f16 && !ad) HIPCHK(grtm::launch_zero_u32(rowpk, p->nw, s));
     const int r64 = p->prec == GRTM_PREC_FP64 ? p->nw : p->rows64;
+    const bool apart = learned && p->low_apart;
+    if (apart) {
+        // the FP64 rows on the side stream: their state, then their launch
+        // over the set-up list, while the network and the item kernels run
+        HIPCHK(hipEventRecord(p->ev_fork, s));
+        HIPCHK(hipStreamWaitEvent(p->side, p->ev_fork, 0));
+        HIPCHK(grtm::launch_rows_init(p->rows64, p->nk_pred.as<int>(), nkw, rowpk, tailmx, p->side));
+        HIPCHK(grtm::launch_cells_zr(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0,
+                                     p->rows64, p->low_tiles.as<unsigned>(), p->n_low_tiles, p->A0.p, p->A1.p, p->nk,
+                                     nullptr, nullptr, nkw, rowpk, p->side, p->nchk.as<int>(), p->chk_stride, tailmx,
+                                     nullptr, 0, nullptr, 0, false));
+        HIPCHK(hipEventRecord(p->ev_join, p->side));
+    }
     if (learned) {
         // the learned grid of this execution: network -> per-row cut and
         // check count -> recursion items, all on the stream
@@ -948,7 +973,7 @@ int run_cells_core(grtm_plan *p, hipStream_t s)
         gi.row_peak = rowpk;
         gi.tail_max = tailmx;
         HIPCHK(grtm::launch_grid_items(p->nw, r64, p->nk_pred.as<int>(), p->nchk.as<int>(), p->tiles.as<unsigned>(),
-                                       p->ioff.as<unsigned>(), p->icounts.as<unsigned>(), s, gi));
+                                       p->ioff.as<unsigned>(), p->icounts.as<unsigned>(), s, gi, apart));
         if (p->ev_pred1) HIPCHK(hipEventRecord(p->ev_pred1, s));
         p->pred_stale = true;
     } else if (ad) {
@@ -976,14 +1001,16 @@ int run_cells_core(grtm_plan *p, hipStream_t s)
     } else {
         unsigned *cnt = p->flags.as<unsigned>(), *list = cnt + 1;
         if (!learned) HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
-        HIPCHK(grtm::launch_cells_zr(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0,
-                                     p->rows64, t_lo, n_lo, p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk, s,
-                                     nchk, p->chk_stride, tailmx, dcnt, 0, ctr, 0, learned));
+        if (!apart)
+            HIPCHK(grtm::launch_cells_zr(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0,
+                                         p->rows64, t_lo, n_lo, p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk,
+                                         s, nchk, p->chk_stride, tailmx, dcnt, 0, ctr, 0, learned));
         if (p->ev_main0) HIPCHK(hipEventRecord(p->ev_main0, s));
         HIPCHK(grtm::launch_cells_zr(1, p->compat, p->lw.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, p->rows64,
                                      p->nw - p->rows64, t_hi, n_hi, p->A0.p, p->A1.p, p->nk, cnt, list, nkw, rowpk, s,
                                      nchk, p->chk_stride, tailmx, dcnt, 1, ctr ? ctr + 1 : nullptr, 0, learned));
         if (p->ev_main1) HIPCHK(hipEventRecord(p->ev_main1, s));
+        if (apart) HIPCHK(hipStreamWaitEvent(s, p->ev_join, 0));  // the FP64 rows are in
         if (p->nplant)
             HIPCHK(grtm::launch_plant_nonfinite(p->plant.as<unsigned>(), p->nplant, p->nk, p->A0.p, p->A1.p, p->nk,
                                                 cnt, list, s));
@@ -1513,13 +1540,64 @@ static int build_items(grtm_plan *p)
     return GRTM_OK;
 }
 
+static void row_cut(int nk, double kcut, double dk, int stride, double window, int &n, int &nc);
+
+// Mixed learned plans: rows [0, rows64), solved in FP64, take the analytic
+// cut kc = min(1.15 w / Vs_min + ln(1e7) / dz, kmax) (the floor the network's
+// cut keeps on those rows: on the 256-receiver model the network's cut is
+// below it on every such row) with the residual check over the window beyond
+// it.  Their nk_pred / nchk and item list (tiles, then check items) are set
+// here once, so their launch needs nothing from the network and runs on the
+// side stream beside it.
+static int set_low_rows(grtm_plan *p)
+{
+    p->low_apart = false;
+    p->lg.low_fixed = 0;
+    if (p->prec == GRTM_PREC_FP64 || !p->net || p->rows64 < 1) return GRTM_OK;
+    // no execution may still read the buffers rewritten below
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (p->last_stream && p->last_stream != p->stream) HIPCHK(hipStreamSynchronize(p->last_stream));
+    if (p->side) HIPCHK(hipStreamSynchronize(p->side));
+    if (!p->side) {
+        HIPCHK(hipStreamCreate(&p->side));
+        HIPCHK(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
+    }
+    const int r64 = p->rows64;
+    std::vector<int> n(r64), nc(r64);
+    std::vector<unsigned> items;
+    const double dz = std::max(std::fabs(p->lm.src - p->lm.rcv), 0.5);
+    for (int iw = 0; iw < r64; iw++) {
+        const double w = 2.0 * M_PI * iw / (2.0 * p->lg.nw * p->lg.dt);
+        const double kc = std::min(1.15 * w / p->lm.vs_min + std::log(1e7) / dz, p->lg.kmax);
+        row_cut(p->nk, kc, p->dk, p->chk_stride, p->lm.window, n[iw], nc[iw]);
+        for (int kb = 0; kb < grtm::cells_tiles_per_row(n[iw]); kb++) items.push_back(grtm::cells_tile(iw, kb));
+    }
+    for (int iw = 0; iw < r64; iw++)
+        for (int c = 0; c < grtm::cells_tiles_per_row(nc[iw]); c++)
+            items.push_back(grtm::cells_tile(iw, c) | grtm::kCheckItem);
+    int st;
+    const size_t bytes = sizeof(unsigned) * std::max<size_t>(1, items.size());
+    if (p->low_tiles.n < bytes && (st = p->low_tiles.alloc(bytes))) return st;
+    if (!items.empty())
+        HIPCHK(hipMemcpy(p->low_tiles.p, items.data(), sizeof(unsigned) * items.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p->nk_pred.p, n.data(), sizeof(int) * r64, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p->nchk.p, nc.data(), sizeof(int) * r64, hipMemcpyHostToDevice));
+    if ((int)p->nk_pred_h.size() == p->nw && (int)p->nchk_h.size() == p->nw)
+        for (int iw = 0; iw < r64; iw++) p->nk_pred_h[iw] = n[iw], p->nchk_h[iw] = nc[iw];
+    p->n_low_tiles = (int)items.size();
+    p->lg.low_fixed = 1;
+    p->low_apart = true;
+    return GRTM_OK;
+}
+
 // The learned grid's predictor inputs to the device (the floor rows follow the
 // plan's FP64 split).
 static int upload_learn_model(grtm_plan *p)
 {
     p->lm.floor_rows = p->prec == GRTM_PREC_FP64 ? p->rows64_rule : p->rows64;
     HIPCHK(hipMemcpy(p->lmodel.p, &p->lm, sizeof(grtm::LearnModel), hipMemcpyHostToDevice));
-    return GRTM_OK;
+    return set_low_rows(p);
 }
 
 int grtm_plan_set_ptam(grtm_plan *p, int n_ptam, int tail)

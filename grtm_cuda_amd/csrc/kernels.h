@@ -93,7 +93,10 @@ struct LearnModel {
     double window;                // residual-check window km^-1
 };
 struct LearnGrid {
-    int nw, nk, stride, pad;
+    int nw, nk, stride;
+    // 1: rows [0, floor_rows) of every model keep the cut and check count the
+    // host stored in nk_pred / nchk (the network's outputs for them unused)
+    int low_fixed;
     double dt, dk, kmax, lmargin;
 };
 // Per-row cut nk_pred[m nw + iw] and residual-check count nchk[...] of
@@ -113,9 +116,15 @@ struct GridInit {
 // Recursion items of one model's learned grid from nk_pred / nchk: tiles of
 // rows [0, r64), their residual-check items, tiles of rows [r64, nw), theirs;
 // counts = {0, items of rows [0, r64), all items}.  off: 2 nw scratch words
-// (each row's first tile and first check item).
+// (each row's first tile and first check item).  low_apart: rows [0, r64)
+// get no items and gi leaves their state alone (their launch has a list and
+// state of its own), counts = {0, 0, items of rows [r64, nw)}.
 hipError_t launch_grid_items(int nw, int r64, const int *nk_pred, const int *nchk, unsigned *items, unsigned *off,
-                             unsigned *counts, hipStream_t st, const GridInit &gi);
+                             unsigned *counts, hipStream_t st, const GridInit &gi, bool low_apart = false);
+// Rows [0, n) start an execution: nk_w = nk_src, row peak and checked-tail
+// maximum zero.
+hipError_t launch_rows_init(int n, const int *nk_src, int *nk_w, unsigned *row_peak, unsigned *tail_max,
+                            hipStream_t st);
 // adaptive grid: dense fallback, tile extents
 // Rows whose checked tail exceeds tol x row peak are completed on the dense
 // grid: FP64 plans in FP64; mixed plans in FP64 below row r64 and in FP32

@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_zr(const LayerW<T> *__re
 // per-row and scalar state (GridInit), which took five launches of their own.
 __global__ __launch_bounds__(1024) void k_grid_scan(int nw, int r64, const int *__restrict__ nk_pred,
                                                     const int *__restrict__ nchk, unsigned *__restrict__ off,
-                                                    unsigned *__restrict__ counts, GridInit gi)
+                                                    unsigned *__restrict__ counts, GridInit gi, int w_lo)
 {
     if (threadIdx.x < kGridZero && gi.zero[threadIdx.x]) *gi.zero[threadIdx.x] = 0u;
     constexpr int R = 8;
@@ -132,9 +132,9 @@ __global__ __launch_bounds__(1024) void k_grid_scan(int nw, int r64, const int *
     const int t = threadIdx.x;
     unsigned nt[R], nc[R], ti = 0, ci = 0;
     for (int q = 0; q < R; q++) {
-        const int iw = t * R + q;
-        nt[q] = iw < nw ? unsigned((nk_pred[iw] + kCellThreads - 1) / kCellThreads) : 0u;
-        nc[q] = iw < nw ? unsigned((nchk[iw] + kCellThreads - 1) / kCellThreads) : 0u;
+        const int iw = t * R + q;  // rows below w_lo have no items here
+        nt[q] = iw < nw && iw >= w_lo ? unsigned((nk_pred[iw] + kCellThreads - 1) / kCellThreads) : 0u;
+        nc[q] = iw < nw && iw >= w_lo ? unsigned((nchk[iw] + kCellThreads - 1) / kCellThreads) : 0u;
         ti += nt[q];
         ci += nc[q];
     }
@@ -181,10 +181,10 @@ __global__ __launch_bounds__(1024) void k_grid_scan(int nw, int r64, const int *
 
 __global__ __launch_bounds__(256) void k_grid_fill(int nw, const int *__restrict__ nk_pred,
                                                    const int *__restrict__ nchk, const unsigned *__restrict__ off,
-                                                   unsigned *__restrict__ items, GridInit gi)
+                                                   unsigned *__restrict__ items, GridInit gi, int w_lo)
 {
     const int iw = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (iw >= nw) return;
+    if (iw >= nw || iw < w_lo) return;
     if (lane == 0) {
         if (gi.nk_w) gi.nk_w[iw] = nk_pred[iw];
         if (gi.row_peak) gi.row_peak[iw] = 0u;
@@ -784,11 +784,30 @@ static int resident_blocks(const void *kernel, int ub)
 }
 
 hipError_t launch_grid_items(int nw, int r64, const int *nk_pred, const int *nchk, unsigned *items, unsigned *off,
-                             unsigned *counts, hipStream_t st, const GridInit &gi)
+                             unsigned *counts, hipStream_t st, const GridInit &gi, bool low_apart)
 {
-    if (nw < 1 || nw > 8192) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_grid_scan, dim3(1), dim3(1024), 0, st, nw, r64, nk_pred, nchk, off, counts, gi);
-    hipLaunchKernelGGL(k_grid_fill, dim3((nw + 3) / 4), dim3(256), 0, st, nw, nk_pred, nchk, off, items, gi);
+    if (nw < 1 || nw > 8192 || r64 < 0) return hipErrorInvalidValue;
+    const int w_lo = low_apart ? std::min(r64, nw) : 0;
+    hipLaunchKernelGGL(k_grid_scan, dim3(1), dim3(1024), 0, st, nw, r64, nk_pred, nchk, off, counts, gi, w_lo);
+    hipLaunchKernelGGL(k_grid_fill, dim3((nw + 3) / 4), dim3(256), 0, st, nw, nk_pred, nchk, off, items, gi, w_lo);
+    return hipGetLastError();
+}
+
+__global__ void k_rows_init(int n, const int *__restrict__ nk_src, int *__restrict__ nk_w,
+                            unsigned *__restrict__ row_peak, unsigned *__restrict__ tail_max)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    nk_w[r] = nk_src[r];
+    row_peak[r] = 0u;
+    tail_max[r] = 0u;
+}
+
+hipError_t launch_rows_init(int n, const int *nk_src, int *nk_w, unsigned *row_peak, unsigned *tail_max,
+                            hipStream_t st)
+{
+    if (n < 1) return hipSuccess;
+    hipLaunchKernelGGL(k_rows_init, dim3((n + 255) / 256), dim3(256), 0, st, n, nk_src, nk_w, row_peak, tail_max);
     return hipGetLastError();
 }
 

@@ -322,10 +322,12 @@ __global__ __launch_bounds__(kMlpThreads) void k_mlp_infer(MlpDev net, int rows,
         // one thread per row: its cut and check cells from all 17 outputs
         if (tid < kMlpRows && r0 + tid < rows) {
             const int g = r0 + tid, m = g / lg.nw, iw = g - m * lg.nw;
-            int n, nc;
-            learned_decode(models[m], lg, iw, ysh[tid], n, nc);
-            nk_pred[g] = n;
-            nchk[g] = nc;
+            if (!lg.low_fixed || iw >= models[m].floor_rows) {
+                int n, nc;
+                learned_decode(models[m], lg, iw, ysh[tid], n, nc);
+                nk_pred[g] = n;
+                nchk[g] = nc;
+            }
         }
     }
 }

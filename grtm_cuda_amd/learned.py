@@ -152,22 +152,25 @@ class Predictor:
         self.mlp.set_params(data["params"])
         self.lmargin = float(data["lmargin"]) if "lmargin" in data else 0.0
 
-    def kcut(self, layers, src_depth, rcv_depth, grid, floor_rows=None):
+    def kcut(self, layers, src_depth, rcv_depth, grid, floor_rows=None, low_fixed=True):
         """Per-row truncation wavenumbers of the grid.  Rows [0, floor_rows)
-        keep at least the analytic cut; default: the plans' FP64 rows
-        (fp64_rows)."""
+        (default: the plans' FP64 rows, fp64_rows) take the analytic cut, as
+        the mixed-precision TracePlan's device grid does (low_fixed), or keep
+        at least it, as batches and FP64 plans do (low_fixed=False)."""
         nw, dt = grid["nw"], grid["dt"]
         w = 2 * math.pi * np.arange(nw) / (2 * nw * dt)
         y = self.mlp.predict(features(layers, src_depth, w))
         kc = decode_kcut(y, layers, src_depth, rcv_depth, w, self.lmargin)
-        # Rows solved in FP64 keep at least the analytic cut: they are few and
-        # their cuts short, while a failed check sends such a row to a dense
-        # FP64 completion, a latency-bound launch on the critical path
+        # Rows solved in FP64 take (or keep at least) the analytic cut: they
+        # are few and their cuts short, while a failed check sends such a row
+        # to a dense FP64 completion, a latency-bound launch; taken without
+        # the network, their grid lets their launch run beside it
         from grtm_cuda_amd import models
         if floor_rows is None:
             floor_rows = fp64_rows(layers, src_depth, rcv_depth, grid)
         low = np.arange(nw) < floor_rows
-        kc = np.where(low, np.maximum(kc, models.analytic_kcut(layers, src_depth, rcv_depth, grid)), kc)
+        ak = models.analytic_kcut(layers, src_depth, rcv_depth, grid)
+        kc = np.where(low, ak if low_fixed else np.maximum(kc, ak), kc)
         return np.minimum(kc, grid["kmax"])
 
     def attach(self, plan, window=None, stride=8, tol=3e-6):

@@ -90,8 +90,54 @@ def test_device_learned_traces(c2_plans):
     assert np.array_equal(again, tr_dev)
 
 
+def _row_cut(nk, kc, dk, stride, window):
+    """The plans' cut and check count of one row (host.hip row_cut)."""
+    n = int(min(nk, max(1.0, np.ceil(kc / dk))))
+    room = nk - n
+    fit = (room - stride // 2 - 1) // stride + 1 if room > stride // 2 else 0
+    nc = max(0, min(fit, int(np.ceil(window / (dk * stride)))))
+    if room > 0 and nc == 0:
+        n = nk
+    return n, nc
+
+
+def test_fp64_rows_take_the_analytic_cut(pred):
+    """Mixed-precision plans give the FP64 rows the analytic cut, without the
+    network, so their launch runs beside it.  Model 152 of the 1024-model set
+    (29 layers, source at 3.96 km) is one where the network's cut exceeds the
+    analytic one on every FP64 row: there the rows' cuts and check counts are
+    the host's row_cut of the analytic cut, the host decode with low_fixed
+    agrees, and the traces stay within 1e-4 of the plain FP64 path (the
+    residual check completes any row the shorter cut leaves short)."""
+    cfg = models.config("c4")
+    lay, zs = cfg["models"][152]
+    grid, dists = cfg["grid"], cfg["dists"][:16]
+    print("gpu: fp64 rows of model 152 on the analytic cut", flush=True)
+    plan = g.TracePlan(lay, zs, 0.0, dists, grid, precision=g.PREC_FP32)
+    window = models.check_window(zs, 0.0)
+    pred.attach(plan, window=window, stride=8, tol=3e-6)
+    tr = plan.traces()
+    cut, chk = plan.adaptive_rows()
+    r64 = plan.work()["rows_fp64"]
+    assert r64 == learned.fp64_rows(lay, zs, 0.0, grid)
+    ak = models.analytic_kcut(lay, zs, 0.0, grid)
+    dk = grid["kmax"] / grid["nk"]
+    want = np.array([_row_cut(grid["nk"], ak[iw], dk, 8, window) for iw in range(r64)])
+    assert np.array_equal(cut[:r64], want[:, 0]) and np.array_equal(chk[:r64], want[:, 1])
+    fixed = pred.kcut(lay, zs, 0.0, grid)
+    floored = pred.kcut(lay, zs, 0.0, grid, low_fixed=False)
+    assert np.array_equal(fixed[:r64], ak[:r64])
+    # the network's cut is the larger on (nearly) every FP64 row here
+    assert np.sum(floored[:r64] > ak[:r64]) >= r64 // 2
+    ref = g.TracePlan(lay, zs, 0.0, dists, grid, precision=g.PREC_FP64).traces()
+    rl = np.linalg.norm(tr - ref, axis=-1) / np.linalg.norm(ref, axis=-1)
+    st = plan.adaptive_stats()
+    print(f"  {r64} FP64 rows, {st['rows_fallback']} fallback rows, rel-L2 {rl.max():.2e}")
+    assert rl.max() < 1e-4
+
+
 def _batch_host_cuts(pred, ms, grid):
-    kc = np.stack([pred.kcut(l, zs, 0.0, grid) for l, zs in ms])
+    kc = np.stack([pred.kcut(l, zs, 0.0, grid, low_fixed=False) for l, zs in ms])
     win = np.array([models.check_window(zs, 0.0) for _, zs in ms])
     return kc, win
 
