@@ -390,6 +390,9 @@ def main():
     ap.add_argument("--derivatives", action="store_true", help="mt: the d/dz and d/dr columns as well")
     ap.add_argument("--cpu-rows", type=int, default=512, help="CPU baseline sample: frequency rows on all cores")
     ap.add_argument("--cpu-rows-1core", type=int, default=48, help="CPU baseline sample: frequency rows on one core")
+    ap.add_argument("--flag64", default=None, metavar="B,P",
+                    help="c1/c2 mixed plans: also re-solve in FP64 the cells within B cells of every branch point "
+                         "and (learned grid) within P half widths of each predicted pole cluster")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="c2: leave out the 1024-model batch figure")
     ap.add_argument("--copy-probe", action="store_true",
@@ -438,6 +441,9 @@ def main():
             from grtm_cuda_amd import learned
             learned.Predictor().attach(plan, window=models.check_window(cfg["src_depth"], cfg["rcv_depth"]),
                                        stride=8, tol=3e-6)
+        if args.flag64:
+            hb, pw = args.flag64.split(",")
+            plan.set_flag64(int(hb), float(pw))
         print(f"bench: {args.path} {pname}", file=sys.stderr, flush=True)
         t_step = time_plan(plan, args.steps, args.warmup)
         out = plan.traces()
@@ -446,6 +452,8 @@ def main():
             dense_cells = cfg["grid"]["nw"] * cfg["grid"]["nk"]
             extra.update(cells_frac=st["cells_final"] / dense_cells, check_frac=st["cells_check"] / dense_cells,
                          rows_fallback=st["rows_fallback"])
+        if args.flag64:
+            extra.update(flag64=args.flag64, cells_fp64_flagged=int(plan.work()["cells_refixed"]))
     err = np.linalg.norm(out - ref, axis=-1) / np.linalg.norm(ref, axis=-1)
     peak = np.abs(out - ref).max(axis=-1) / np.abs(ref).max(axis=-1)
     value = traces_per_step / t_step

@@ -162,6 +162,26 @@ class TracePlan:
         c = np.ascontiguousarray(np.asarray(cells, dtype=np.int32).reshape(-1, 2))
         check(_lib().grtm_plan_plant_nonfinite(self.ptr, c.ctypes.data_as(P), len(c)), "grtm_plan_plant_nonfinite")
 
+    def set_flag64(self, branch_cells=-1, pole_widths=0.0):
+        """Re-solve in FP64, at every execution, the FP32 cells within
+        `branch_cells` wavenumber cells of every layer's branch points
+        Re(w/alpha), Re(w/beta) and (learned plans) within `pole_widths` half
+        widths of each pole cluster the network predicts; branch_cells < 0 and
+        pole_widths = 0 turn the two parts off."""
+        check(_lib().grtm_plan_set_flag64(self.ptr, int(branch_cells), float(pole_widths)), "grtm_plan_set_flag64")
+
+    def flag_list(self):
+        """(rows, ks) of the cells re-solved in FP64 in the last execution."""
+        n = _lib().grtm_plan_flag_list(self.ptr, None, 0)
+        if n < 0:
+            check(n, "grtm_plan_flag_list")
+        buf = np.zeros(max(n, 1), dtype=np.uint32)
+        m = _lib().grtm_plan_flag_list(self.ptr, buf.ctypes.data_as(P), n)
+        if m < 0:
+            check(m, "grtm_plan_flag_list")
+        c = buf[:min(n, m)].astype(np.int64)
+        return c // self.nk, c % self.nk
+
     def set_adaptive(self, kcut, window=None, stride=8, tol=1e-5):
         """Use a truncated wavenumber grid: kcut[iw] (km^-1) per frequency row,
         residual check every `stride`-th wavenumber over `window` km^-1 beyond

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -765,6 +766,13 @@ struct grtm_plan {
     DBuf lw, lw64, flags, dists, A0, A1, B0, B1, C0, C1, tr, ws, S0, S1;
     DBuf plant;      // test hook: cells given non-finite FP32 operands at every execution
     int nplant = 0;
+    // FP64 cells next to branch points (within flag_hb cells of Re(w/V) of
+    // every layer; < 0: none) and predicted pole clusters (within flag_pw
+    // half widths; 0: none) of the FP32 rows (grtm_plan_set_flag64); poles:
+    // the network's clusters of the last execution, [nw][8][2] km^-1
+    int flag_hb = -1;
+    float flag_pw = 0.f;
+    DBuf poles;
     // peak-trough averaging of the k-sums' tails (grtm_plan_set_ptam): ptam_nt
     // wavenumbers beyond the grid per row, their weighted kernels At and
     // Bessel values Bt, and the count of sums that took the extended sum
@@ -960,7 +968,7 @@ int run_cells_core(grtm_plan *p, hipStream_t s)
         int st;
         if (p->ev_pred0) HIPCHK(hipEventRecord(p->ev_pred0, s));
         if ((st = grtm::learned_grid(p->net, p->lmodel.as<grtm::LearnModel>(), 1, p->lg, p->nk_pred.as<int>(),
-                                     p->nchk.as<int>(), s)))
+                                     p->nchk.as<int>(), s, p->flag_pw > 0.f ? p->poles.as<float>() : nullptr)))
             return st;
         // the item kernels also start the execution's state: nk_w = the cut,
         // zero row peaks, tail maxima, fallback and flag counts, item counters
@@ -1001,6 +1009,10 @@ int run_cells_core(grtm_plan *p, hipStream_t s)
     } else {
         unsigned *cnt = p->flags.as<unsigned>(), *list = cnt + 1;
         if (!learned) HIPCHK(grtm::launch_zero_u32(cnt, 1, s));
+        if (p->flag_hb >= 0 || p->flag_pw > 0.f)
+            HIPCHK(grtm::launch_flag_cells(p->lw64.p, p->L, p->rows64, p->nw, p->nk, p->dk, p->flag_hb,
+                                           learned && p->flag_pw > 0.f ? p->poles.as<float>() : nullptr,
+                                           p->flag_pw, nkw, cnt, list, s));
         if (!apart)
             HIPCHK(grtm::launch_cells_zr(2, p->compat, p->lw64.p, p->L, p->g, p->nk, p->dk, p->df, p->omega_i, 0,
                                          p->rows64, t_lo, n_lo, p->A0.p, p->A1.p, p->nk, nullptr, nullptr, nkw, rowpk,
@@ -1650,6 +1662,15 @@ int grtm_plan_ptam_stats(grtm_plan *p, double *out)
     return GRTM_OK;
 }
 
+// Room in the flag list for every cell of the grid once (a non-finite FP32
+// result), the planted cells and the cells of grtm_plan_set_flag64.
+static int alloc_flags(grtm_plan *p, int nplant, int hb, float pw)
+{
+    const size_t rows = (size_t)p->nw;  // any rows64 set later
+    const size_t per_row = (hb >= 0 ? (size_t)2 * p->L * (2 * hb + 1) : 0) + (pw > 0.f ? 8 * (2 * 64 + 2) : 0);
+    return p->flags.alloc(sizeof(unsigned) * ((size_t)p->nw * p->nk + 1 + nplant + rows * per_row));
+}
+
 int grtm_plan_plant_nonfinite(grtm_plan *p, const int *cells, int n)
 {
     if (!p || n < 0 || n > GRTM_MAX_PLANT || (n && !cells)) return GRTM_E_INPUT;
@@ -1663,12 +1684,40 @@ int grtm_plan_plant_nonfinite(grtm_plan *p, const int *cells, int n)
     HIPCHK(hipStreamSynchronize(p->stream));
     if (p->last_stream && p->last_stream != p->stream) HIPCHK(hipStreamSynchronize(p->last_stream));
     int st;
-    // room for every cell of the grid once plus the planted ones
-    if ((st = p->flags.alloc(sizeof(unsigned) * ((size_t)p->nw * p->nk + 1 + n)))) return st;
+    if ((st = alloc_flags(p, n, p->flag_hb, p->flag_pw))) return st;
     if ((st = p->plant.alloc(sizeof(unsigned) * c.size()))) return st;
     HIPCHK(hipMemcpy(p->plant.p, c.data(), sizeof(unsigned) * c.size(), hipMemcpyHostToDevice));
     p->nplant = n;
     return GRTM_OK;
+}
+
+int grtm_plan_set_flag64(grtm_plan *p, int branch_cells, double pole_widths)
+{
+    if (!p || branch_cells > GRTM_MAX_FLAG_CELLS || !(pole_widths >= 0 && pole_widths <= 8)) return GRTM_E_INPUT;
+    if (p->kind != 0 || p->prec == GRTM_PREC_FP64 || p->ptam_n) return GRTM_E_UNSUPPORTED;
+    if (pole_widths > 0 && !p->net) return GRTM_E_UNSUPPORTED;  // clusters come from the network
+    const int hb = branch_cells < 0 ? -1 : branch_cells;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (p->last_stream && p->last_stream != p->stream) HIPCHK(hipStreamSynchronize(p->last_stream));
+    int st;
+    if (pole_widths > 0 && !p->poles.p && (st = p->poles.alloc(sizeof(float) * 16 * (size_t)p->nw))) return st;
+    if ((st = alloc_flags(p, p->nplant, hb, float(pole_widths)))) return st;
+    p->flag_hb = hb;
+    p->flag_pw = float(pole_widths);
+    return GRTM_OK;
+}
+
+int grtm_plan_flag_list(grtm_plan *p, unsigned *cells, int cap)
+{
+    if (!p || cap < 0 || (cap && !cells)) return GRTM_E_INPUT;
+    if (p->kind != 0 || p->prec == GRTM_PREC_FP64) return GRTM_E_UNSUPPORTED;
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (p->last_stream && p->last_stream != p->stream) HIPCHK(hipStreamSynchronize(p->last_stream));
+    unsigned c = 0;
+    HIPCHK(hipMemcpy(&c, p->flags.p, sizeof(unsigned), hipMemcpyDeviceToHost));
+    const size_t n = std::min<size_t>(c, (size_t)cap);
+    if (n) HIPCHK(hipMemcpy(cells, p->flags.as<unsigned>() + 1, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
+    return (int)std::min<unsigned>(c, (unsigned)INT_MAX);
 }
 
 int grtm_plan_set_rows64(grtm_plan *p, int rows64)

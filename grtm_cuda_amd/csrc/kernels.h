@@ -101,8 +101,10 @@ struct LearnGrid {
 };
 // Per-row cut nk_pred[m nw + iw] and residual-check count nchk[...] of
 // n_models models from one execution of the network (learned.hip / mlp.hip).
+// poles (optional): each row's 8 predicted pole clusters as (centre, half
+// width) in km^-1, [n_models nw][16] floats.
 int learned_grid(grtm_mlp *net, const LearnModel *models, int n_models, const LearnGrid &lg, int *nk_pred,
-                 int *nchk, hipStream_t st);
+                 int *nchk, hipStream_t st, float *poles = nullptr);
 // The state an execution of a learned grid starts from, set by the item
 // kernels instead of launches of their own: up to kGridZero scalar counters
 // cleared, and per row nk_w = nk_pred, row peak and checked-tail maximum zero
@@ -150,6 +152,14 @@ hipError_t launch_row_labels(int prec, int nw, int nk, long lda, const void *A0,
 // appended to the flag list.
 hipError_t launch_plant_nonfinite(const unsigned *cells, int n, int nk, void *A0, void *A1, long lda,
                                   unsigned *flag_count, unsigned *flag_list, hipStream_t st);
+// FP64 cells next to branch points and predicted surface-wave poles
+// (grtm_plan_set_flag64): per row of [r64, nw), the cells within hb of
+// Re(w/alpha), Re(w/beta) of every layer (hb < 0: none) and within
+// pole_widths half widths of each cluster in poles ([nw][8][centre, half
+// width] km^-1; null: none), below nk_w[iw], join the flag list once each.
+hipError_t launch_flag_cells(const void *lw64, int L, int r64, int nw, int nk, double dk, int hb, const float *poles,
+                             float pole_widths, const int *nk_w, unsigned *flag_count, unsigned *flag_list,
+                             hipStream_t st);
 // row_peak (optional): per-row peaks raised to cover the re-solved values
 hipError_t launch_cells_fix(int compat, const void *lw64, int L, const Geo &g, int nk, double dk, double df,
                             double omega_i, void *A0, void *A1, long lda, const unsigned *flag_count,

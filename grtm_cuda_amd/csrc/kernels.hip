@@ -439,8 +439,9 @@ __global__ void k_copy_i32(const int *__restrict__ a, int *__restrict__ b, int n
     if (t < n) b[t] = a[t];
 }
 
-// FP64 re-solve of the flagged cells (grid-stride over the list).
-template <typename TO, bool COMPAT>
+// FP64 re-solve of the flagged cells (grid-stride over the list); BR: the
+// plan's receiver branch, fixed at compile time as in k_cells_zr.
+template <typename TO, bool COMPAT, int BR>
 __global__ __launch_bounds__(kCellThreads) void k_cells_fix(const LayerW<double> *__restrict__ lw_all, int L,
                                                             Geo g, int nk, double dk, double df, double omega_i,
                                                             TO *__restrict__ A0, TO *__restrict__ A1, long lda,
@@ -453,7 +454,7 @@ __global__ __launch_bounds__(kCellThreads) void k_cells_fix(const LayerW<double>
         const unsigned cell = flag_list[t];
         const int iw = int(cell / unsigned(nk)), ik = int(cell % unsigned(nk));
         TO v[8];
-        cell_zr<double, TO, COMPAT>(lw_all + (size_t)iw * L, g, iw, ik, dk, df, omega_i, v);
+        cell_zr<double, TO, COMPAT, BR>(lw_all + (size_t)iw * L, g, iw, ik, dk, df, omega_i, v);
         store_zr(v, iw, ik, A0, A1, lda);
         // the row peak covers the re-solved values (it sets the FP16 split's row scale)
         if (row_peak) {
@@ -476,6 +477,71 @@ __global__ void k_plant_nonfinite(const unsigned *__restrict__ cells, int n, int
     for (int i = 0; i < 8; i++) v[i] = __builtin_nanf("");
     store_zr(v, int(cell / unsigned(nk)), int(cell % unsigned(nk)), A0, A1, lda);
     flag_list[atomicAdd(flag_count, 1u)] = cell;
+}
+
+// Cells of the FP32 rows re-solved in FP64 because they lie next to a branch
+// point or a surface-wave pole (grtm_plan_set_flag64).  One block per row
+// iw = r64 + blockIdx.x.  Intervals [lo, hi) of wavenumber cells, all below
+// the row's cut nk_w[iw]:
+//  - every layer's P and S branch points: the vertical wavenumbers
+//    sqrt(k^2 - (w/V)^2) (green.cu:184-194) branch where k^2 = (w/V)^2, i.e.
+//    nearest the grid at k = Re(w/V); cells within hb of it (hb >= 0);
+//  - each pole cluster the network predicted for the row (centre c and half
+//    width hw in km^-1, poles[iw][2i], [2i + 1]): cells within pole_widths
+//    half widths of c, at most kFlagPoleCells on either side.
+// Each cell joins the flag list once (a cell also covered by an earlier
+// interval is skipped): a count pass, one atomic reservation per block, a
+// write pass.  The order of the row's entries is arbitrary.
+constexpr int kFlagPoleCells = 64;
+__global__ __launch_bounds__(256) void k_flag_cells(const LayerW<double> *__restrict__ lw_all, int L, int r64, int nk,
+                                                    double dk, int hb, const float *__restrict__ poles,
+                                                    float pole_widths, const int *__restrict__ nk_w,
+                                                    unsigned *__restrict__ flag_count, unsigned *__restrict__ flag_list)
+{
+    constexpr int kMaxIv = 2 * kMaxLayers + 8;
+    __shared__ int lo[kMaxIv], hi[kMaxIv];
+    __shared__ unsigned cnt, pos, base;
+    const int iw = r64 + blockIdx.x, tid = threadIdx.x;
+    const int nkr = nk_w ? nk_w[iw] : nk;
+    const int nb = hb >= 0 ? 2 * L : 0, np = poles ? 8 : 0, ni = nb + np;
+    if (tid < nb) {
+        const LayerW<double> &l = lw_all[(size_t)iw * L + tid / 2];
+        const double kb = fabs((tid & 1) ? l.sb.x : l.sa.x);
+        const long c = lrint(kb / dk);
+        lo[tid] = int(max(0L, c - hb));
+        hi[tid] = int(min((long)nkr, c + hb + 1));
+    } else if (tid < ni) {
+        const int i = tid - nb;
+        const float c = poles[(size_t)iw * 16 + 2 * i], w = pole_widths * poles[(size_t)iw * 16 + 2 * i + 1];
+        int a = 0, b = 0;
+        if (c > 0.f && w > 0.f) {
+            const double cc = double(c) / dk, ww = fmin(double(w) / dk, double(kFlagPoleCells));
+            a = int(fmin(fmax(0.0, floor(cc - ww)), double(nkr)));
+            b = int(fmin(double(nkr), fmax(0.0, ceil(cc + ww) + 1.0)));
+        }
+        lo[tid] = a;
+        hi[tid] = max(a, b);
+    }
+    if (tid == 0) cnt = 0u, pos = 0u;
+    __syncthreads();
+    for (int pass = 0; pass < 2; pass++) {
+        for (int i = 0; i < ni; i++) {
+            for (int k = lo[i] + tid; k < hi[i]; k += blockDim.x) {
+                bool fresh = true;
+                for (int j = 0; j < i && fresh; j++) fresh = k < lo[j] || k >= hi[j];
+                if (!fresh) continue;
+                if (pass == 0)
+                    atomicAdd(&cnt, 1u);
+                else
+                    flag_list[base + atomicAdd(&pos, 1u)] = unsigned(iw) * unsigned(nk) + unsigned(k);
+            }
+        }
+        __syncthreads();
+        if (pass == 0) {
+            if (tid == 0) base = cnt ? atomicAdd(flag_count, cnt) : 0u;
+            __syncthreads();
+        }
+    }
 }
 
 __global__ void k_zero_u32(unsigned *p, int n)
@@ -941,14 +1007,23 @@ hipError_t launch_cells_fix(int compat, const void *lw64, int L, const Geo &g, i
                             const unsigned *flag_list, hipStream_t st, unsigned *row_peak)
 {
     dim3 grid(1024);
-    if (compat)
-        hipLaunchKernelGGL((k_cells_fix<float, true>), grid, dim3(kCellThreads), 0, st,
-                           (const LayerW<double> *)lw64, L, g, nk, dk, df, omega_i, (float *)A0, (float *)A1, lda,
-                           flag_count, flag_list, row_peak);
-    else
-        hipLaunchKernelGGL((k_cells_fix<float, false>), grid, dim3(kCellThreads), 0, st,
-                           (const LayerW<double> *)lw64, L, g, nk, dk, df, omega_i, (float *)A0, (float *)A1, lda,
-                           flag_count, flag_list, row_peak);
+#define GRTM_FIX(CP, BR)                                                                                           \
+    hipLaunchKernelGGL((k_cells_fix<float, CP, BR>), grid, dim3(kCellThreads), 0, st, (const LayerW<double> *)lw64, \
+                       L, g, nk, dk, df, omega_i, (float *)A0, (float *)A1, lda, flag_count, flag_list, row_peak)
+    const int br = g.branch >= 1 && g.branch <= 4 ? g.branch : 0;
+    if (compat) {
+        if (br == 1) GRTM_FIX(true, 1);
+        else if (br == 2) GRTM_FIX(true, 2);
+        else if (br == 3) GRTM_FIX(true, 3);
+        else GRTM_FIX(true, 0);  // compat refuses branch 4
+    } else {
+        if (br == 1) GRTM_FIX(false, 1);
+        else if (br == 2) GRTM_FIX(false, 2);
+        else if (br == 3) GRTM_FIX(false, 3);
+        else if (br == 4) GRTM_FIX(false, 4);
+        else GRTM_FIX(false, 0);
+    }
+#undef GRTM_FIX
     return hipGetLastError();
 }
 
@@ -958,6 +1033,16 @@ hipError_t launch_plant_nonfinite(const unsigned *cells, int n, int nk, void *A0
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_plant_nonfinite, dim3((n + 255) / 256), dim3(256), 0, st, cells, n, nk, (float *)A0,
                        (float *)A1, lda, flag_count, flag_list);
+    return hipGetLastError();
+}
+
+hipError_t launch_flag_cells(const void *lw64, int L, int r64, int nw, int nk, double dk, int hb, const float *poles,
+                             float pole_widths, const int *nk_w, unsigned *flag_count, unsigned *flag_list,
+                             hipStream_t st)
+{
+    if (nw <= r64 || L > kMaxLayers) return nw <= r64 ? hipSuccess : hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_flag_cells, dim3(nw - r64), dim3(256), 0, st, (const LayerW<double> *)lw64, L, r64, nk, dk,
+                       hb, poles, pole_widths, nk_w, flag_count, flag_list);
     return hipGetLastError();
 }
 

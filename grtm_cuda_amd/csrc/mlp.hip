@@ -328,6 +328,15 @@ __global__ __launch_bounds__(kMlpThreads) void k_mlp_infer(MlpDev net, int rows,
                 nk_pred[g] = n;
                 nchk[g] = nc;
             }
+            if (Y) {
+                // the pole clusters in km^-1 (centre, half width), as the decode reads them
+                const double w = 2.0 * M_PI * iw / (2.0 * lg.nw * lg.dt);
+                const double s = fabs(w) / models[m].vs_min + 1.0 / fmax(fabs(models[m].src - models[m].rcv), 0.5);
+                for (int i = 0; i < 8; i++) {
+                    Y[(size_t)g * 16 + 2 * i] = float(double(ysh[tid][1 + 2 * i]) * s);
+                    Y[(size_t)g * 16 + 2 * i + 1] = float(exp(double(ysh[tid][2 + 2 * i])) / 100.0);
+                }
+            }
         }
     }
 }
@@ -707,14 +716,14 @@ int grtm_mlp_predict_train_forward(grtm_mlp *n, const float *X, int cnt, float *
 
 namespace grtm {
 int learned_grid(grtm_mlp *net, const LearnModel *models, int n_models, const LearnGrid &lg, int *nk_pred,
-                 int *nchk, hipStream_t st)
+                 int *nchk, hipStream_t st, float *poles)
 {
     if (!net || !net->fused || net->fin[0] != 6 * kLearnSlots + 2 || net->n_out != kLearnOut || n_models < 1)
         return GRTM_E_UNSUPPORTED;
     const int rows = n_models * lg.nw;
     hipLaunchKernelGGL((k_mlp_infer<1>), dim3((rows + kMlpRows - 1) / kMlpRows), dim3(kMlpThreads), 0, st, mlp_dev(net),
                        rows,
-                       nullptr, nullptr, models, lg, nk_pred, nchk);
+                       nullptr, poles, models, lg, nk_pred, nchk);
     HIPCHK(hipGetLastError());
     return GRTM_OK;
 }

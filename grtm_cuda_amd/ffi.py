@@ -91,6 +91,8 @@ def _lib():
         lib.grtm_plan_set_rows64.argtypes = [P, I]
         lib.grtm_plan_plant_nonfinite.argtypes = [P, P, I]
         lib.grtm_plan_set_ptam.argtypes = [P, I, I]
+        lib.grtm_plan_set_flag64.argtypes = [P, I, D]
+        lib.grtm_plan_flag_list.argtypes = [P, P, I]
         lib.grtm_plan_ptam_stats.argtypes = [P, P]
         lib.grtm_batch_plant_nonfinite.argtypes = [P, P, I]
         lib.grtm_plan_adaptive_stats.argtypes = [P, P]
@@ -141,6 +143,7 @@ EXPORTS = [
     "grtm_batch_traces_out", "grtm_batch_timing", "grtm_batch_config", "grtm_batch_work", "grtm_batch_destroy",
     "grtm_batch_set_adaptive", "grtm_batch_set_learned", "grtm_batch_adaptive_stats",
     "grtm_plan_plant_nonfinite", "grtm_batch_plant_nonfinite", "grtm_plan_set_ptam", "grtm_plan_ptam_stats",
+    "grtm_plan_set_flag64", "grtm_plan_flag_list",
 ]
 
 

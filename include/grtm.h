@@ -210,6 +210,23 @@ int grtm_plan_ptam_stats(grtm_plan *plan, double *out);
 #define GRTM_MAX_PLANT 4096
 int grtm_plan_plant_nonfinite(grtm_plan *plan, const int *cells, int n);
 
+/* FP64 cells next to branch points and surface-wave poles (mixed-precision
+ * Z/R plans).  At every execution, cells of the FP32 rows within
+ * branch_cells wavenumber cells of Re(w/alpha) and Re(w/beta) of every layer
+ * (the branch points of the vertical wavenumbers, green.cu:184-194), and
+ * within pole_widths half widths of each pole cluster the plan's network
+ * predicts for the row (learned plans only; at most 64 cells either side),
+ * below the row's cut, are re-solved in FP64 after the FP32 launch, as
+ * non-finite FP32 cells are.  branch_cells < 0 and pole_widths = 0 turn the
+ * two parts off (the default).  branch_cells <= GRTM_MAX_FLAG_CELLS,
+ * 0 <= pole_widths <= 8.  The reference solves every cell in FP64; it has no
+ * counterpart. */
+#define GRTM_MAX_FLAG_CELLS 64
+int grtm_plan_set_flag64(grtm_plan *plan, int branch_cells, double pole_widths);
+/* Cells (row * nk + k) re-solved in FP64 in the last execution, up to cap
+ * of them into cells; returns their number (a status < 0 on error). */
+int grtm_plan_flag_list(grtm_plan *plan, unsigned *cells, int cap);
+
 /* Adaptive wavenumber grid (no counterpart in the reference, whose ragged
  * grid uses the cal_kmax heuristic, green.cu:108-162).  kcut[iw] (km^-1) is
  * the truncation wavenumber of each frequency row (from the learned
