@@ -181,43 +181,48 @@ GRTM_HD void interface_xy(const LQ<T> &a, const LQ<T> &b, T k, m2<T> &X, m2<T> &
     d2 = T(2) * (a.mg * rv);
 }
 
-// R and T = d2 G^-1 from F = X (I + Q), H = Y (I - Q), G = F + H.
+// One step of the recursion across an interface (X, Y, d2 of interface_xy)
+// with b's side reflecting Q: F = X (I + Q), H = Y (I - Q) (entry by entry in
+// fused multiply-adds, X + X Q and Y - Y Q), G = F + H.  R is kept as
+// R = I - c P: F - H = G - 2 H, so R = (F - H) G^-1 = I - 2 H G^-1 with
+// P = H adj(G) and c = 2 / det G.  P needs one 2x2 product where R needs G^-1
+// and a product; c and the identity enter where R is used next (lrl_p,
+// r_from_p).  T = d2 G^-1.  grt_p takes F and H (F = X, H = Y when nothing
+// reflects on b's side, b the half-space).
 template <typename T>
-GRTM_HD void grt_rt(const m2<T> &F, const m2<T> &H, cx<T> d2, m2<T> &R, m2<T> &Tr)
+GRTM_HD void grt_p(const m2<T> &F, const m2<T> &H, cx<T> d2, m2<T> &P, cx<T> &c, m2<T> &Tr)
 {
     const m2<T> G = F + H;
     const cx<T> di = recip(cfms(G.b, G.c, G.a * G.d));
-    const m2<T> Gi = {G.d * di, -(G.b * di), -(G.c * di), G.a * di};
-    R = (F - H) * Gi;
+    P = {cfms(H.b, G.c, H.a * G.d), cfms(H.a, G.b, H.b * G.a), cfms(H.d, G.c, H.c * G.d), cfms(H.c, G.b, H.d * G.a)};
+    c = T(2) * di;
     const cx<T> dt = d2 * di;
     Tr = {G.d * dt, -(G.b * dt), -(G.c * dt), G.a * dt};
 }
-// One step of the recursion: R, T across an interface (X, Y, d2 of
-// interface_xy) with b's side reflecting Q.
-// X (I + Q) = X + X Q and Y (I - Q) = Y - Y Q entry by entry in fused
-// multiply-adds.
 template <typename T>
-GRTM_HD void grt_step(const m2<T> &X, const m2<T> &Y, cx<T> d2, const m2<T> &Q, m2<T> &R, m2<T> &Tr)
+GRTM_HD void grt_step_p(const m2<T> &X, const m2<T> &Y, cx<T> d2, const m2<T> &Q, m2<T> &P, cx<T> &c, m2<T> &Tr)
 {
     const m2<T> F = {cfma(X.b, Q.c, cfma(X.a, Q.a, X.a)), cfma(X.a, Q.b, cfma(X.b, Q.d, X.b)),
                      cfma(X.d, Q.c, cfma(X.c, Q.a, X.c)), cfma(X.c, Q.b, cfma(X.d, Q.d, X.d))};
     const m2<T> H = {cfms(Y.b, Q.c, cfms(Y.a, Q.a, Y.a)), cfms(Y.a, Q.b, cfms(Y.b, Q.d, Y.b)),
                      cfms(Y.d, Q.c, cfms(Y.c, Q.a, Y.c)), cfms(Y.c, Q.b, cfms(Y.d, Q.d, Y.d))};
-    grt_rt(F, H, d2, R, Tr);
+    grt_p(F, H, d2, P, c, Tr);
 }
-// The same with nothing reflecting on b's side (b the half-space).
+// R = I - c P.
 template <typename T>
-GRTM_HD void grt_step0(const m2<T> &X, const m2<T> &Y, cx<T> d2, m2<T> &R, m2<T> &Tr)
+GRTM_HD m2<T> r_from_p(const m2<T> &P, cx<T> c)
 {
-    grt_rt(X, Y, d2, R, Tr);
+    return {cfms(c, P.a, C<T>(1, 0)), -(c * P.b), -(c * P.c), cfms(c, P.d, C<T>(1, 0))};
 }
-// Lambda R Lambda with Lambda = diag(e0, e1); e11 = e1^2 for the SH term.
+// Lambda (I - c P) Lambda with Lambda = diag(e0, e1), R never formed; e11 =
+// e1^2 for the SH term.
 template <typename T>
-GRTM_HD m2<T> lrl(cx<T> e0, cx<T> e1, const m2<T> &R, cx<T> &e11)
+GRTM_HD m2<T> lrl_p(cx<T> e0, cx<T> e1, const m2<T> &P, cx<T> c, cx<T> &e11)
 {
     const cx<T> e00 = e0 * e0, e01 = e0 * e1;
     e11 = e1 * e1;
-    return {e00 * R.a, e01 * R.b, e01 * R.c, e11 * R.d};
+    const cx<T> c00 = c * e00, c01 = c * e01, c11 = c * e11;
+    return {cfms(c00, P.a, e00), -(c01 * P.b), -(c01 * P.c), cfms(c11, P.d, e11)};
 }
 // SH (E_SH = [[1, 1], [-mu v, mu v]]): impedances wa = mu_a v_a, wb = mu_b v_b,
 // q the generalized SH reflection of b's side.
@@ -364,37 +369,40 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
     out.status = 0;
 
     // ---------------- upward sweep: half-space -> bottom of source layer
-    m2<T> Rdu_s1, Rdu_r1, Md;  // R_du at bottom of s, at bottom of rl, T_d chain
-    cx<T> rdu0_s1, rdu0_r1, md0;
+    // R_du at the bottom of s as I - cs1 Ps1 (grt_p), at the bottom of rl, T_d chain
+    m2<T> Ps1, Rdu_r1, Md;
+    cx<T> cs1, rdu0_s1, rdu0_r1, md0;
     Rdu_r1 = ident<T>();
     rdu0_r1 = C<T>(0, 0);
     Md = ident<T>();
     md0 = C<T>(1, 0);
     const int bot = bot_in >= 0 ? bot_in : branch != 4 ? sweep_bottom(lw, L, s, branch == 3 ? zr : zs, kd) : L - 1;
     if (bot == s) {
-        Rdu_s1 = m2<T>{C<T>(0, 0), C<T>(0, 0), C<T>(0, 0), C<T>(0, 0)};
+        Ps1 = ident<T>();  // R_du = 0
+        cs1 = C<T>(1, 0);
         rdu0_s1 = C<T>(0, 0);
     } else {
         LQ<T> qd = layer_q(lw[bot], kd);  // layer below the interface
         LQ<T> qu = layer_q(lw[bot - 1], kd);
-        m2<T> X, Y, Rg, Td;
-        cx<T> rg0, td0, d2;
+        m2<T> X, Y, P, Td;
+        cx<T> c, rg0, td0, d2;
         interface_xy(qu, qd, k, X, Y, d2);
-        grt_step0(X, Y, d2, Rg, Td);
+        grt_p(X, Y, d2, P, c, Td);  // nothing reflects below: F = X, H = Y
         grt_step_sh(qu.v * qu.mu, qd.v * qd.mu, C<T>(0, 0), rg0, td0);
         // grt[bot] = (Rg, Td)
         int j = bot;  // index of the grt entry just formed
         // lj0, lj1 = Lambda_jj (layer jj, just below the interface formed)
-        auto visit = [&](int jj, const m2<T> &RgJ, cx<T> rg0J, const m2<T> &TdJ, cx<T> td0J,
+        auto visit = [&](int jj, const m2<T> &PJ, cx<T> cJ, cx<T> rg0J, const m2<T> &TdJ, cx<T> td0J,
                          cx<T> lj0, cx<T> lj1) {
-            // jj: grt index formed (R_du at interface (jj-1, jj), T_d into jj)
+            // jj: grt index formed (R_du = I - cJ PJ at interface (jj-1, jj), T_d into jj)
             if (jj == s + 1) {
-                Rdu_s1 = RgJ;
+                Ps1 = PJ;
+                cs1 = cJ;
                 rdu0_s1 = rg0J;
             }
             if (branch == 4) {
                 if (jj == rl + 1) {
-                    Rdu_r1 = RgJ;
+                    Rdu_r1 = r_from_p(PJ, cJ);
                     rdu0_r1 = rg0J;
                 }
                 if (!COMPAT) {
@@ -421,7 +429,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
                 }
             }
         };
-        visit(j, Rg, rg0, Td, td0, C<T>(0, 0), C<T>(0, 0));  // Lambda of the half-space is never used
+        visit(j, P, c, rg0, Td, td0, C<T>(0, 0), C<T>(0, 0));  // Lambda of the half-space is never used
         for (int lay = bot - 2; lay >= s; lay--) {
             // interface lay: between layer lay (above) and lay+1 (below, = qu)
             cx<T> e0, e1;
@@ -431,17 +439,17 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             // green.cu:532-594: generalized R_du, T_d across interface lay with
             // the stack below reflecting Lambda R Lambda
             cx<T> e11;
-            const m2<T> Q = lrl(e0, e1, Rg, e11);
+            const m2<T> Q = lrl_p(e0, e1, P, c, e11);
             interface_xy(qu, qd, k, X, Y, d2);
-            grt_step(X, Y, d2, Q, Rg, Td);
+            grt_step_p(X, Y, d2, Q, P, c, Td);
             grt_step_sh(qu.v * qu.mu, qd.v * qd.mu, e11 * rg0, rg0, td0);
-            visit(lay + 1, Rg, rg0, Td, td0, e0, e1);
+            visit(lay + 1, P, c, rg0, Td, td0, e0, e1);
         }
     }
 
     // ---------------- downward sweep: free surface -> top of source layer
-    m2<T> Rud_s, Rud_r, Mu;
-    cx<T> rud0_s, rud0_r, mu0;
+    m2<T> Ps, Rud_r, Mu;  // R_ud at the top of s as I - cs Ps
+    cx<T> cs, rud0_s, rud0_r, mu0;
     Mu = ident<T>();
     mu0 = C<T>(1, 0);
     LQ<T> qr;  // receiver layer quantities
@@ -452,6 +460,9 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
         if (rl == 0) qr = qu;
         Rud_r = Rg;
         rud0_r = rg0;
+        // the free surface's R as I - c P with c = 1
+        m2<T> P = {C<T>(1, 0) - Rg.a, -Rg.b, -Rg.c, C<T>(1, 0) - Rg.d};
+        cx<T> c = C<T>(1, 0);
         for (int lay = 0; lay < s; lay++) {
             cx<T> e0, e1;
             lam_layer(qu, lw[lay], e0, e1);  // Lambda_lay
@@ -461,14 +472,14 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             // Lambda R Lambda
             m2<T> X, Y, Tu;
             cx<T> tu0, d2, e11;
-            const m2<T> Q = lrl(e0, e1, Rg, e11);
+            const m2<T> Q = lrl_p(e0, e1, P, c, e11);
             interface_xy(qd, qu, k, X, Y, d2);
-            grt_step(X, Y, d2, Q, Rg, Tu);
+            grt_step_p(X, Y, d2, Q, P, c, Tu);
             grt_step_sh(qd.v * qd.mu, qu.v * qu.mu, e11 * rg0, rg0, tu0);
             const int jn = lay + 1;  // grt index formed
             if (branch == 1) {
                 if (jn == rl) {
-                    Rud_r = Rg;
+                    Rud_r = r_from_p(P, c);
                     rud0_r = rg0;
                 }
                 if (jn == rl + 1) {
@@ -482,7 +493,8 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
             if (jn == rl) qr = qd;
             qu = qd;
         }
-        Rud_s = Rg;
+        Ps = P;
+        cs = c;
         rud0_s = rg0;
         out.sq = qu;  // layer s
     }
@@ -493,8 +505,11 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
     const W hs_ = lw[s].h;
     cx<T> sexd0 = cexp_nz<T>(sq.rd, hs_ - zs), sexd1 = cexp_nz<T>(sq.vd, hs_ - zs);
     cx<T> sexu0 = cexp_nz<T>(sq.rd, zs), sexu1 = cexp_nz<T>(sq.vd, zs);
-    out.ERd = dmd(sexd0, sexd1, Rdu_s1, sexd0, sexd1);
-    out.ERu = dmd(sexu0, sexu1, Rud_s, sexu0, sexu1);
+    {
+        cx<T> t;
+        out.ERd = lrl_p(sexd0, sexd1, Ps1, cs1, t);
+        out.ERu = lrl_p(sexu0, sexu1, Ps, cs, t);
+    }
     out.erd0 = sexd1 * rdu0_s1 * sexd1;
     out.eru0 = sexu1 * rud0_s * sexu1;
     const m2<T> yu_psv = minv(mmsub(out.ERd, out.ERu, ident<T>()));
@@ -511,6 +526,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
     if (branch == 2) {  // green.cu:709-743
         cx<T> rd0 = cexp_nz<T>(qr.rd, zr), rd1 = cexp_nz<T>(qr.vd, zr);
         cx<T> ru0 = cexp_nz<T>(qr.rd, zs - zr), ru1 = cexp_nz<T>(qr.vd, zs - zr);
+        const m2<T> Rud_s = r_from_p(Ps, cs);
         const m2<T> RS = dmd(rd0, rd1, Rud_s, sexu0, sexu1);
         const m2<T> u = mmadd(Ed, RS, md(Eu, ru0, ru1));
         const cx<T> u0 = rd1 * rud0_s * sexu1 + ru1;
@@ -527,6 +543,7 @@ GRTM_HD void cell_solve(const LayerW<T> *lw, const Geo &g, typename WideOf<T>::t
     } else if (branch == 3) {  // green.cu:744-774
         cx<T> rd0 = cexp_nz<T>(qr.rd, zr - zs), rd1 = cexp_nz<T>(qr.vd, zr - zs);
         cx<T> ru0 = cexp_nz<T>(qr.rd, hr_ - zr), ru1 = cexp_nz<T>(qr.vd, hr_ - zr);
+        const m2<T> Rdu_s1 = r_from_p(Ps1, cs1);
         const m2<T> RS = dmd(ru0, ru1, Rdu_s1, sexd0, sexd1);
         const m2<T> d = mmadd(Eu, RS, md(Ed, rd0, rd1));
         const cx<T> d0 = rd1 + ru1 * rdu0_s1 * sexd1;
